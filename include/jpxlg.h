@@ -1,0 +1,189 @@
+/*
+ * jpxlg.h -- C ABI of the MI355X (gfx950) VarDCT pixel path and SSIMULACRA2
+ * quality controller.
+ *
+ * The library keeps each image's XYB planes, forward-DCT coefficients and
+ * SSIMULACRA2 reference pyramid resident in HBM, and runs the
+ * quantize -> reconstruct -> score chain of the quality search on the GPU.
+ * Entropy coding and bitstream writing stay with the caller (CPU); the plan
+ * handed back (quantized coefficients, LF planes, block map, CfL factors)
+ * is the input of the reference's `attach_entropy`.
+ *
+ * Conventions: plain C types, caller-owned host buffers, int32 status
+ * (0 = OK, negative = JPXLG_E_*).  One context = one device + one HIP stream;
+ * a context is not shared across threads, contexts are independent.
+ *
+ * Reference interfaces replaced (paths relative to JPXL/crates/):
+ *   jpxlg_image_upload_srgb   PreparedFrame::from_srgb8_with / from_srgb16_with
+ *                             (jpxl-encode-policy/src/source.rs:226, :355) and
+ *                             PlanRenderEvaluator::from_srgb8 / from_srgb16
+ *                             (jpxl-perceptual/src/evaluator.rs:87-122)
+ *   jpxlg_plan_structure      the cover + CfL half of build_pixel_plan
+ *                             (jpxl-encode-policy/src/lib.rs:555-845)
+ *   jpxlg_quantize            quantize_group / quantize_square_varblock
+ *                             (jpxl-encode-policy/src/lib.rs:4469, :3237)
+ *   jpxlg_render_score        PerceptualEvaluator::evaluate =
+ *                             PlanRenderer::render_linear_at_depth_with +
+ *                             Ssimulacra2::score
+ *                             (jpxl-encode-policy/src/quality.rs:87-157,
+ *                              jpxl-plan-render/src/lib.rs:680,
+ *                              jpxl-perceptual/src/ssimulacra2.rs:79)
+ *   jpxlg_plan_download       the QuantizedFrameIr hand-off to attach_entropy
+ *                             (jpxl-encode/src/vardct/plan.rs:614-636)
+ *   jpxlg_encode_quality      search_frame_perceptual
+ *                             (jpxl-encode-policy/src/quality.rs:2405)
+ *   jpxlg_score_pair          jpxl_perceptual::score_pair
+ *                             (jpxl-perceptual/src/lib.rs:195)
+ *   jpxlg_dct_blocks          dct_2d_in_place / idct_2d_in_place
+ *                             (jpxl-core/src/dct.rs:956, :982)
+ *   jpxlg_epf_step            epf_step_rows (jpxl-core/src/restoration.rs:583)
+ *   jpxlg_gaborish            gaborish_into (jpxl-core/src/restoration.rs:148)
+ */
+#ifndef JPXLG_H
+#define JPXLG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JPXLG_OK 0
+#define JPXLG_E_INVALID (-1)
+#define JPXLG_E_UNSUPPORTED (-2)
+#define JPXLG_E_OOM (-3)
+#define JPXLG_E_DEVICE (-4)
+#define JPXLG_E_TOO_SMALL (-5)
+#define JPXLG_E_RANGE (-6)
+
+typedef struct jpxlg_ctx jpxlg_ctx;
+typedef struct jpxlg_img jpxlg_img;
+
+/* I.5.3 / I.2.1 quantizer scalars of one probe plus the quantizer rule.
+ * quantize.rs:301-366, lib.rs:2033-2172, request.rs:855-972. */
+typedef struct {
+  uint32_t global_scale;   /* 1..73728 */
+  uint32_t hf_mul;         /* >= 1 */
+  uint32_t quant_lf;       /* >= 1 */
+  uint32_t x_qm_scale;     /* u(3), 2 = neutral */
+  uint32_t b_qm_scale;     /* u(3), 2 = neutral */
+  int32_t truncate;        /* 1 = QuantizerChoiceMode::TrailingTruncation */
+  float lambda_scale;      /* request.lambda_scale (4.0 on the quality path) */
+  float zero_token_bits;   /* request.zero_token_bits */
+} jpxlg_quantizer;
+
+/* J.1 restoration decisions of a plan (plan.rs:221-228, request.rs:905). */
+typedef struct {
+  int32_t epf_iters;   /* 0..3 */
+  int32_t gaborish;    /* 0/1 */
+  int32_t sharpness;   /* uniform EPF sharpness 0..7 */
+} jpxlg_restoration;
+
+/* Cover modes (request.rs CoverMode). */
+#define JPXLG_COVER_FIXED_DCT8 0
+#define JPXLG_COVER_HIERARCHICAL 1
+
+/* ---- context ------------------------------------------------------------ */
+int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out);
+void jpxlg_ctx_destroy(jpxlg_ctx* ctx);
+const char* jpxlg_last_error(const jpxlg_ctx* ctx);
+/* Bytes of device memory an image of this size keeps resident. */
+uint64_t jpxlg_image_footprint(uint32_t width, uint32_t height);
+
+/* ---- image: XYB + metric reference resident in HBM ------------------------ */
+/* rgb: interleaved RGB, row-major; bits 8 (uint8) or 9..16 (uint16). */
+int32_t jpxlg_image_upload_srgb(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_t bits,
+                                const void* rgb, jpxlg_img** out);
+/* Same, reading the interleaved samples from device memory (already in HBM). */
+int32_t jpxlg_image_from_device(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_t bits,
+                                const void* device_rgb, jpxlg_img** out);
+void jpxlg_image_destroy(jpxlg_img* img);
+int32_t jpxlg_image_info(const jpxlg_img* img, uint32_t* blocks_x, uint32_t* blocks_y,
+                         int32_t* grayscale, uint32_t* metric_scales);
+
+/* ---- plan structure: block map + CfL, then the forward DCT --------------- */
+/* block_map: blocks_x * blocks_y bytes; 8/16/32 at a varblock's top-left 8x8
+ * block, 0 on the blocks it covers.  cfl_x/cfl_b: one int8 per 64x64 tile
+ * (tiles_x = ceil(blocks_x/8)); lf_factors: {x_factor_lf-128, b_factor_lf-128}.
+ * NULL cfl pointers mean the neutral (all-zero) factors. */
+int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* block_map, const int8_t* cfl_x,
+                            const int8_t* cfl_b, const int32_t* lf_factors);
+/* Builds the structure on the device: the cover (fixed DCT8 or hierarchical
+ * R-D quadtree under `q`) and, when enable_cfl, the CfL estimate. */
+int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t cover_mode,
+                             int32_t enable_cfl);
+
+/* ---- probe ---------------------------------------------------------------- */
+int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q);
+/* Reconstructs the last quantized plan at the source depth and scores it.
+ * terms (optional) receives 3 * scales * 6 pooled terms in the metric's
+ * (component, scale, norm, map) order; *nterms their count. */
+int32_t jpxlg_render_score(jpxlg_img* img, const jpxlg_restoration* r, double* score,
+                           double* raw_error, double* terms, uint32_t* nterms);
+
+/* ---- plan hand-off and debugging downloads ------------------------------- */
+/* quant: 3 planes of (8*blocks_x) x (8*blocks_y) int32 in the footprint layout
+ * (cell k of a side-s varblock at block (bx,by) sits at row 8*by + k/s,
+ * column 8*bx + k%s); lf: 3 planes of blocks_x * blocks_y int32. */
+int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* lf, uint8_t* block_map,
+                            int8_t* cfl_x, int8_t* cfl_b, int32_t* lf_factors);
+
+#define JPXLG_BUF_XYB 0          /* 3 x W x H f32: source XYB */
+#define JPXLG_BUF_COEFF 1        /* 3 x 8bw x 8bh f32: forward DCT (footprint) */
+#define JPXLG_BUF_RENDER_XYB 2   /* 3 x W x H f32: reconstructed XYB after restoration */
+#define JPXLG_BUF_LINEAR 3       /* 3 x W x H f32: reconstructed linear RGB at depth */
+#define JPXLG_BUF_SIGMA 4        /* bw x bh f32: EPF sigma */
+#define JPXLG_BUF_REF_XYB0 5     /* 3 x W x H f32: metric reference positive XYB, scale 0 */
+int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* host, size_t bytes);
+
+/* ---- quality controller ---------------------------------------------------- */
+#define JPXLG_EFFORT_FAST 0
+#define JPXLG_EFFORT_BALANCED 1
+typedef struct {
+  double target;        /* minimum SSIMULACRA2 score */
+  int32_t effort;       /* JPXLG_EFFORT_* */
+  int32_t max_probes;   /* 0 = the effort's budget */
+  int32_t mode;         /* 0 = JPXL navigator, 1 = bisection over the ladder */
+} jpxlg_quality_request;
+
+typedef struct {
+  double score;              /* achieved score of the chosen plan */
+  uint32_t global_scale, hf_mul, quant_lf;
+  uint32_t rung;
+  int32_t met;               /* target met */
+  int32_t probes;
+  double probe_scores[16];
+  uint32_t probe_rungs[16];
+  double probe_ms[16];
+} jpxlg_quality_result;
+
+int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
+                             jpxlg_quality_result* out);
+
+/* ---- per-kernel timing ----------------------------------------------------- */
+/* When enabled, every launch the context issues is bracketed by HIP events on
+ * its stream.  jpxlg_profile_read synchronises, folds the events into per-
+ * kernel totals (name, milliseconds, launches) and returns the entry count;
+ * names holds cap entries of 48 chars.  Reading also clears the totals. */
+int32_t jpxlg_profile_enable(jpxlg_ctx* ctx, int32_t on);
+int32_t jpxlg_profile_read(jpxlg_ctx* ctx, char* names, double* total_ms, uint64_t* launches,
+                           int32_t cap);
+
+/* ---- stand-alone kernels (metric alone, transforms, filters) -------------- */
+/* Planes are planar f32 linear sRGB, 3 x W x H, host memory. */
+int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t height, const float* reference,
+                         const float* candidate, double* score, double* terms, uint32_t* nterms);
+/* count N x N blocks (N = 8, 16, 32), row-major; inverse = 0 forward DCT_2D. */
+int32_t jpxlg_dct_blocks(jpxlg_ctx* ctx, int32_t side, int32_t inverse, uint32_t count,
+                         const float* in, float* out);
+/* One J.4 step over three W x H planes with a (ceil(W/8) x ceil(H/8)) sigma grid. */
+int32_t jpxlg_epf_step(jpxlg_ctx* ctx, int32_t step, uint32_t width, uint32_t height,
+                       const float* in, const float* sigma, float* out);
+int32_t jpxlg_gaborish(jpxlg_ctx* ctx, uint32_t width, uint32_t height, const float* in,
+                       float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

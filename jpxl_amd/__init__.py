@@ -1,0 +1,353 @@
+"""jpxl_amd: the MI355X (gfx950) VarDCT pixel path of JPXL and its SSIMULACRA2
+quality controller, behind the C ABI of ``include/jpxlg.h``.
+
+The Python layer mirrors the reference's call surface for this path:
+
+* ``Encoder().with_quality(85).with_effort("balanced").encode_rgb8_reported(w, h, rgb)``
+  (jpxl/src/lib.rs:507-794) -> the quantized plan the CPU entropy coder
+  consumes, plus a report (the reference's ``EncodeReport::Perceptual``);
+* ``score_pair(reference, candidate)`` (jpxl-perceptual/src/lib.rs:195);
+* ``Image`` exposes the resident per-image state: ``plan_structure``,
+  ``quantize``, ``render_score`` (the ``PerceptualEvaluator`` seam of
+  jpxl-encode-policy/src/quality.rs:87-157), ``plan_download``.
+
+Every call runs on the GPU through ``libjpxlg.so``; there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import JpxlgError, Quantizer, Restoration
+
+__all__ = [
+    "Context",
+    "Image",
+    "Encoder",
+    "EncodeReport",
+    "Plan",
+    "Quantizer",
+    "Restoration",
+    "JpxlgError",
+    "score_pair",
+    "quality_quantizer",
+]
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One device + one HIP stream (jpxlg_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _lib.load()
+        h = ctypes.c_void_p()
+        rc = self.lib.jpxlg_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise JpxlgError(rc, "context creation failed (no usable GPU?)")
+        self.handle = h
+
+    def check(self, rc: int) -> None:
+        if rc != 0:
+            msg = self.lib.jpxlg_last_error(self.handle)
+            raise JpxlgError(rc, msg.decode() if msg else "")
+
+    def close(self) -> None:
+        if self.handle:
+            self.lib.jpxlg_ctx_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- per-kernel timing ---------------------------------------------
+    def profile(self, on: bool = True) -> None:
+        self.check(self.lib.jpxlg_profile_enable(self.handle, int(on)))
+
+    def profile_read(self) -> dict:
+        """{kernel: (total_ms, launches)} since the last read (HIP events)."""
+        cap = 64
+        names = ctypes.create_string_buffer(48 * cap)
+        ms = np.zeros(cap, np.float64)
+        n_l = np.zeros(cap, np.uint64)
+        n = self.lib.jpxlg_profile_read(self.handle, names, ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        n_l.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), cap)
+        if n < 0:
+            self.check(n)
+        out = {}
+        raw = names.raw
+        for i in range(n):
+            nm = raw[48 * i : 48 * i + 48].split(b"\0", 1)[0].decode()
+            out[nm] = (float(ms[i]), int(n_l[i]))
+        return out
+
+    # --- stand-alone kernels -------------------------------------------
+    def dct_blocks(self, blocks: np.ndarray, inverse: bool = False) -> np.ndarray:
+        blocks = np.ascontiguousarray(blocks, dtype=np.float32)
+        n, s, s2 = blocks.shape
+        assert s == s2
+        out = np.empty_like(blocks)
+        self.check(self.lib.jpxlg_dct_blocks(self.handle, s, int(inverse), n, _ptr(blocks), _ptr(out)))
+        return out
+
+    def epf_step(self, step: int, planes: np.ndarray, sigma: np.ndarray) -> np.ndarray:
+        planes = np.ascontiguousarray(planes, dtype=np.float32)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float32)
+        _, h, w = planes.shape
+        assert sigma.shape == ((h + 7) // 8, (w + 7) // 8)
+        out = np.empty_like(planes)
+        self.check(self.lib.jpxlg_epf_step(self.handle, step, w, h, _ptr(planes), _ptr(sigma), _ptr(out)))
+        return out
+
+    def gaborish(self, planes: np.ndarray) -> np.ndarray:
+        planes = np.ascontiguousarray(planes, dtype=np.float32)
+        _, h, w = planes.shape
+        out = np.empty_like(planes)
+        self.check(self.lib.jpxlg_gaborish(self.handle, w, h, _ptr(planes), _ptr(out)))
+        return out
+
+    def score_pair(self, reference: np.ndarray, candidate: np.ndarray):
+        """SSIMULACRA2 of two planar linear-sRGB images (3, H, W)."""
+        reference = np.ascontiguousarray(reference, dtype=np.float32)
+        candidate = np.ascontiguousarray(candidate, dtype=np.float32)
+        _, h, w = reference.shape
+        score = ctypes.c_double()
+        terms = np.zeros(108, np.float64)
+        nterms = ctypes.c_uint32()
+        self.check(self.lib.jpxlg_score_pair(self.handle, w, h, _ptr(reference), _ptr(candidate),
+                                             ctypes.byref(score), terms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                             ctypes.byref(nterms)))
+        return score.value, terms[: nterms.value]
+
+
+_DEFAULT_CTX: Context | None = None
+
+
+def default_context() -> Context:
+    global _DEFAULT_CTX
+    if _DEFAULT_CTX is None:
+        _DEFAULT_CTX = Context(0)
+    return _DEFAULT_CTX
+
+
+def score_pair(reference: np.ndarray, candidate: np.ndarray) -> float:
+    """jpxl_perceptual::score_pair on the GPU; planar linear sRGB (3, H, W)."""
+    return default_context().score_pair(reference, candidate)[0]
+
+
+def quality_quantizer(rung_fields: tuple[int, int], effort: str = "balanced", quant_lf: int = 4) -> Quantizer:
+    """The quantizer a quality probe uses at (global_scale, hf_mul)."""
+    gs, mul = rung_fields
+    fast = effort == "fast"
+    q = Quantizer()
+    q.global_scale, q.hf_mul, q.quant_lf = gs, mul, quant_lf
+    q.x_qm_scale = q.b_qm_scale = 2 if fast else 3
+    q.truncate = 0 if fast else 1
+    q.lambda_scale = 4.0
+    q.zero_token_bits = 1.0
+    return q
+
+
+@dataclass
+class Plan:
+    """The quantized plan handed to the CPU entropy coder (QuantizedFrameIr)."""
+
+    quant: np.ndarray        # (3, 8*bh, 8*bw) int32, footprint layout
+    lf: np.ndarray           # (3, bh, bw) int32
+    block_map: np.ndarray    # (bh, bw) uint8: 8/16/32 at varblock origins, 0 covered
+    cfl_x: np.ndarray        # (th, tw) int8
+    cfl_b: np.ndarray        # (th, tw) int8
+    lf_factors: tuple        # (x_factor_lf - 128, b_factor_lf - 128)
+
+
+class Image:
+    """One image resident in HBM: XYB planes, coefficients, metric reference."""
+
+    def __init__(self, ctx: Context, rgb: np.ndarray, bits: int | None = None, device_ptr: int | None = None,
+                 shape: tuple | None = None):
+        self.ctx = ctx
+        lib = ctx.lib
+        h = ctypes.c_void_p()
+        if device_ptr is not None:
+            height, width = shape
+            self.bits = bits
+            rc = lib.jpxlg_image_from_device(ctx.handle, width, height, bits, ctypes.c_void_p(device_ptr),
+                                             ctypes.byref(h))
+        else:
+            rgb = np.ascontiguousarray(rgb)
+            height, width, three = rgb.shape
+            assert three == 3
+            if bits is None:
+                bits = 8 if rgb.dtype == np.uint8 else 16
+            self.bits = bits
+            want = np.uint8 if bits == 8 else np.uint16
+            rgb = rgb.astype(want, copy=False)
+            rc = lib.jpxlg_image_upload_srgb(ctx.handle, width, height, bits, _ptr(rgb), ctypes.byref(h))
+        ctx.check(rc)
+        self.handle = h
+        self.width, self.height = width, height
+        bx, by, gray, scales = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_uint32()
+        ctx.check(lib.jpxlg_image_info(h, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(gray),
+                                       ctypes.byref(scales)))
+        self.bw, self.bh, self.grayscale, self.scales = bx.value, by.value, bool(gray.value), scales.value
+        self.tw, self.th = (self.bw + 7) // 8, (self.bh + 7) // 8
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.jpxlg_image_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_structure(self, block_map: np.ndarray, cfl_x=None, cfl_b=None, lf_factors=(0, 0)):
+        bm = np.ascontiguousarray(block_map, dtype=np.uint8)
+        assert bm.shape == (self.bh, self.bw)
+        cx = None if cfl_x is None else np.ascontiguousarray(cfl_x, dtype=np.int8)
+        cb = None if cfl_b is None else np.ascontiguousarray(cfl_b, dtype=np.int8)
+        lf = np.asarray(lf_factors, dtype=np.int32)
+        self.ctx.check(self.ctx.lib.jpxlg_set_structure(
+            self.handle, _ptr(bm), None if cx is None else _ptr(cx), None if cb is None else _ptr(cb), _ptr(lf)))
+
+    def plan_structure(self, q: Quantizer, cover_mode: int = _lib.COVER_FIXED_DCT8, enable_cfl: bool = True):
+        self.ctx.check(self.ctx.lib.jpxlg_plan_structure(self.handle, ctypes.byref(q), cover_mode, int(enable_cfl)))
+
+    def quantize(self, q: Quantizer):
+        self.ctx.check(self.ctx.lib.jpxlg_quantize(self.handle, ctypes.byref(q)))
+
+    def render_score(self, restoration: Restoration | None = None):
+        r = restoration or Restoration(1, 0, 7)
+        score, raw = ctypes.c_double(), ctypes.c_double()
+        terms = np.zeros(108, np.float64)
+        n = ctypes.c_uint32()
+        self.ctx.check(self.ctx.lib.jpxlg_render_score(
+            self.handle, ctypes.byref(r), ctypes.byref(score), ctypes.byref(raw),
+            terms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+        return score.value, raw.value, terms[: n.value]
+
+    def plan_download(self) -> Plan:
+        q = np.empty((3, 8 * self.bh, 8 * self.bw), np.int32)
+        lf = np.empty((3, self.bh, self.bw), np.int32)
+        bm = np.empty((self.bh, self.bw), np.uint8)
+        cx = np.empty((self.th, self.tw), np.int8)
+        cb = np.empty((self.th, self.tw), np.int8)
+        lff = np.empty(2, np.int32)
+        self.ctx.check(self.ctx.lib.jpxlg_plan_download(self.handle, _ptr(q), _ptr(lf), _ptr(bm), _ptr(cx),
+                                                        _ptr(cb), _ptr(lff)))
+        return Plan(q, lf, bm, cx, cb, (int(lff[0]), int(lff[1])))
+
+    def download(self, which: int) -> np.ndarray:
+        if which in (_lib.BUF_XYB, _lib.BUF_RENDER_XYB, _lib.BUF_LINEAR, _lib.BUF_REF_XYB0):
+            out = np.empty((3, self.height, self.width), np.float32)
+        elif which == _lib.BUF_COEFF:
+            out = np.empty((3, 8 * self.bh, 8 * self.bw), np.float32)
+        elif which == _lib.BUF_SIGMA:
+            out = np.empty((self.bh, self.bw), np.float32)
+        else:
+            raise ValueError(which)
+        self.ctx.check(self.ctx.lib.jpxlg_image_download(self.handle, which, _ptr(out), out.nbytes))
+        return out
+
+    def encode_quality(self, target: float, effort: str = "balanced", max_probes: int = 0, mode: int = 0):
+        req = _lib.QualityRequest(target, _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED,
+                                  max_probes, mode)
+        res = _lib.QualityResult()
+        self.ctx.check(self.ctx.lib.jpxlg_encode_quality(self.handle, ctypes.byref(req), ctypes.byref(res)))
+        return res
+
+
+@dataclass
+class EncodeReport:
+    """The perceptual half of the reference's EncodeReport (jpxl/src/lib.rs:353)."""
+
+    target: float
+    achieved: float
+    met: bool
+    global_scale: int
+    hf_mul: int
+    quant_lf: int
+    probes: list = field(default_factory=list)
+
+
+class Encoder:
+    """Builder mirroring jpxl::Encoder for the quality / fixed-quantizer paths.
+
+    ``encode_rgb8_reported`` returns ``(Plan, EncodeReport)``: the quantized
+    plan is what the reference hands to its CPU entropy coder; the entropy
+    coder itself is not part of this library.
+    """
+
+    def __init__(self, ctx: Context | None = None):
+        self.ctx = ctx or default_context()
+        self.quality: float | None = None
+        self.global_scale: int | None = None
+        self.effort = "balanced"
+
+    def with_quality(self, quality: float) -> "Encoder":
+        if not (0.0 <= quality <= 100.0):
+            raise ValueError("quality must be within 0..=100")
+        self.quality, self.global_scale = float(quality), None
+        return self
+
+    with_ssimulacra2_score = with_quality
+
+    def with_global_scale(self, global_scale: int) -> "Encoder":
+        if not (1 <= global_scale <= 73728):
+            raise ValueError("global_scale outside 1..=73728")
+        self.global_scale, self.quality = int(global_scale), None
+        return self
+
+    def with_effort(self, effort: str) -> "Encoder":
+        if effort not in ("fast", "balanced"):
+            raise ValueError(f"unsupported effort {effort!r}")
+        self.effort = effort
+        return self
+
+    def with_threads(self, _threads: int) -> "Encoder":
+        return self  # one GPU stream; kept for call-surface parity
+
+    def _encode(self, rgb: np.ndarray, bits: int):
+        img = Image(self.ctx, rgb, bits)
+        try:
+            if self.quality is not None:
+                res = img.encode_quality(self.quality, self.effort)
+                report = EncodeReport(self.quality, res.score, bool(res.met), res.global_scale, res.hf_mul,
+                                      res.quant_lf, [(res.probe_rungs[i], res.probe_scores[i]) for i in range(res.probes)])
+            else:
+                q = Quantizer(self.global_scale or 32768, 1, 16, 2, 2, 0, 1.0, 1.0)
+                img.plan_structure(q, _lib.COVER_FIXED_DCT8, False)
+                img.quantize(q)
+                report = EncodeReport(float("nan"), float("nan"), True, q.global_scale, 1, 16)
+            return img.plan_download(), report
+        finally:
+            img.close()
+
+    def encode_rgb8_reported(self, width: int, height: int, rgb: bytes | np.ndarray):
+        a = np.frombuffer(bytes(rgb), np.uint8) if not isinstance(rgb, np.ndarray) else rgb
+        return self._encode(a.reshape(height, width, 3).astype(np.uint8), 8)
+
+    def encode_rgb16_reported(self, width: int, height: int, rgb: np.ndarray, bits: int = 16):
+        return self._encode(np.asarray(rgb, np.uint16).reshape(height, width, 3), bits)
+
+    def encode_rgb8(self, width: int, height: int, rgb):
+        return self.encode_rgb8_reported(width, height, rgb)[0]
+
+    def encode_rgb16(self, width: int, height: int, rgb, bits: int = 16):
+        return self.encode_rgb16_reported(width, height, rgb, bits)[0]

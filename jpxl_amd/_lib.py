@@ -1,0 +1,149 @@
+"""ctypes binding of libjpxlg.so (the C ABI declared in include/jpxlg.h).
+
+The shared library is built in-tree (``jpxl_amd/libjpxlg.so``) by
+``__graft_entry__.build()``.  Loading fails loudly when it is missing: there
+is no CPU fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjpxlg.so")
+
+OK = 0
+E_INVALID = -1
+E_UNSUPPORTED = -2
+E_OOM = -3
+E_DEVICE = -4
+E_TOO_SMALL = -5
+E_RANGE = -6
+
+BUF_XYB = 0
+BUF_COEFF = 1
+BUF_RENDER_XYB = 2
+BUF_LINEAR = 3
+BUF_SIGMA = 4
+BUF_REF_XYB0 = 5
+
+COVER_FIXED_DCT8 = 0
+COVER_HIERARCHICAL = 1
+
+EFFORT_FAST = 0
+EFFORT_BALANCED = 1
+
+
+class Quantizer(ctypes.Structure):
+    """jpxlg_quantizer."""
+
+    _fields_ = [
+        ("global_scale", ctypes.c_uint32),
+        ("hf_mul", ctypes.c_uint32),
+        ("quant_lf", ctypes.c_uint32),
+        ("x_qm_scale", ctypes.c_uint32),
+        ("b_qm_scale", ctypes.c_uint32),
+        ("truncate", ctypes.c_int32),
+        ("lambda_scale", ctypes.c_float),
+        ("zero_token_bits", ctypes.c_float),
+    ]
+
+
+class Restoration(ctypes.Structure):
+    """jpxlg_restoration."""
+
+    _fields_ = [
+        ("epf_iters", ctypes.c_int32),
+        ("gaborish", ctypes.c_int32),
+        ("sharpness", ctypes.c_int32),
+    ]
+
+
+class QualityRequest(ctypes.Structure):
+    """jpxlg_quality_request."""
+
+    _fields_ = [
+        ("target", ctypes.c_double),
+        ("effort", ctypes.c_int32),
+        ("max_probes", ctypes.c_int32),
+        ("mode", ctypes.c_int32),
+    ]
+
+
+class QualityResult(ctypes.Structure):
+    """jpxlg_quality_result."""
+
+    _fields_ = [
+        ("score", ctypes.c_double),
+        ("global_scale", ctypes.c_uint32),
+        ("hf_mul", ctypes.c_uint32),
+        ("quant_lf", ctypes.c_uint32),
+        ("rung", ctypes.c_uint32),
+        ("met", ctypes.c_int32),
+        ("probes", ctypes.c_int32),
+        ("probe_scores", ctypes.c_double * 16),
+        ("probe_rungs", ctypes.c_uint32 * 16),
+        ("probe_ms", ctypes.c_double * 16),
+    ]
+
+
+_vp = ctypes.c_void_p
+_i32 = ctypes.c_int32
+_u32 = ctypes.c_uint32
+_dp = ctypes.POINTER(ctypes.c_double)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_i32p = ctypes.POINTER(ctypes.c_int32)
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "jpxlg_ctx_create": (_i32, [_i32, ctypes.POINTER(_vp)]),
+    "jpxlg_ctx_destroy": (None, [_vp]),
+    "jpxlg_last_error": (ctypes.c_char_p, [_vp]),
+    "jpxlg_image_footprint": (ctypes.c_uint64, [_u32, _u32]),
+    "jpxlg_image_upload_srgb": (_i32, [_vp, _u32, _u32, _u32, _vp, ctypes.POINTER(_vp)]),
+    "jpxlg_image_from_device": (_i32, [_vp, _u32, _u32, _u32, _vp, ctypes.POINTER(_vp)]),
+    "jpxlg_image_destroy": (None, [_vp]),
+    "jpxlg_image_info": (_i32, [_vp, _u32p, _u32p, _i32p, _u32p]),
+    "jpxlg_set_structure": (_i32, [_vp, _vp, _vp, _vp, _vp]),
+    "jpxlg_plan_structure": (_i32, [_vp, ctypes.POINTER(Quantizer), _i32, _i32]),
+    "jpxlg_quantize": (_i32, [_vp, ctypes.POINTER(Quantizer)]),
+    "jpxlg_render_score": (_i32, [_vp, ctypes.POINTER(Restoration), _dp, _dp, _dp, _u32p]),
+    "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),
+    "jpxlg_encode_quality": (_i32, [_vp, ctypes.POINTER(QualityRequest), ctypes.POINTER(QualityResult)]),
+    "jpxlg_score_pair": (_i32, [_vp, _u32, _u32, _vp, _vp, _dp, _dp, _u32p]),
+    "jpxlg_dct_blocks": (_i32, [_vp, _i32, _i32, _u32, _vp, _vp]),
+    "jpxlg_epf_step": (_i32, [_vp, _i32, _u32, _u32, _vp, _vp, _vp]),
+    "jpxlg_gaborish": (_i32, [_vp, _u32, _u32, _vp, _vp]),
+    "jpxlg_profile_enable": (_i32, [_vp, _i32]),
+    "jpxlg_profile_read": (_i32, [_vp, _vp, _dp, ctypes.POINTER(ctypes.c_uint64), _i32]),
+}
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Loads libjpxlg.so once; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')"
+        )
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class JpxlgError(RuntimeError):
+    """A non-zero status from the C ABI, with the context's last error."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"jpxlg status {code}: {message}")
+        self.code = code

@@ -1,0 +1,948 @@
+// Host runtime behind include/jpxlg.h: contexts, resident images, the probe
+// chain (quantize -> reconstruct -> score) and the quality controller.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/jpxlg.h"
+#include "controller.h"
+#include "internal.h"
+#include "metric_host.h"
+#include "tables.h"
+
+using namespace jpxlg;
+
+struct DepthSet {
+  float* thr = nullptr;
+  float* lut = nullptr;
+  int levels = 0;
+};
+
+struct ProfRec {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+struct jpxlg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool prof = false;
+  std::vector<ProfRec> recs;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<double, uint64_t>> totals;
+  ConstTables* d_tab = nullptr;
+  float* d_lut8 = nullptr;
+  std::map<int, float*> d_lut16;
+  std::map<int, DepthSet> depth;
+  std::string last_error;
+};
+
+namespace {
+
+constexpr int kMaxScales = 6;
+
+hipEvent_t take_event(jpxlg_ctx* ctx) {
+  if (!ctx->pool.empty()) {
+    hipEvent_t e = ctx->pool.back();
+    ctx->pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// Brackets one launch with events on the context stream when profiling.
+struct Prof {
+  jpxlg_ctx* ctx;
+  const char* name;
+  hipEvent_t a = nullptr;
+  Prof(jpxlg_ctx* c, const char* n) : ctx(c), name(n) {
+    if (ctx->prof) {
+      a = take_event(ctx);
+      (void)hipEventRecord(a, ctx->stream);
+    }
+  }
+  ~Prof() {
+    if (a) {
+      hipEvent_t b = take_event(ctx);
+      (void)hipEventRecord(b, ctx->stream);
+      ctx->recs.push_back({name, a, b});
+    }
+  }
+};
+
+int fail(jpxlg_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ctx, JPXLG_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// One device allocation carved into the image's buffers.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0, used = 0;
+  void* take(size_t bytes) {
+    size_t off = (used + 255) & ~(size_t)255;
+    if (off + bytes > size) return nullptr;
+    used = off + bytes;
+    return base + off;
+  }
+};
+
+void scale_dims(int w, int h, int* n, int sw[kMaxScales], int sh[kMaxScales]) {
+  int s = 0;
+  for (; s < kMaxScales; s++) {
+    if (s > 0) {
+      if (w < 8 || h < 8) break;
+      w = (w + 1) / 2;
+      h = (h + 1) / 2;
+    }
+    sw[s] = w;
+    sh[s] = h;
+  }
+  *n = s;
+}
+
+size_t image_bytes(int w, int h) {
+  const size_t px = (size_t)w * h;
+  const size_t bw = (w + 7) / 8, bh = (h + 7) / 8;
+  const size_t cpx = 64 * bw * bh;
+  int n, sw[kMaxScales], sh[kMaxScales];
+  scale_dims(w, h, &n, sw, sh);
+  size_t pyr = 0, pyr1 = 0;
+  for (int s = 0; s < n; s++) {
+    pyr += (size_t)sw[s] * sh[s];
+    if (s > 0) pyr1 += (size_t)sw[s] * sh[s];
+  }
+  size_t b = 0;
+  b += 6 * px;                       // rgb (up to 16-bit)
+  b += 3 * px * 4;                   // source XYB
+  b += 9 * pyr * 4;                  // reference pyramid: XYB', mu1, s11
+  b += 3 * cpx * 4 * 2;              // coefficients f32 + quantized i32
+  b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
+  b += 3 * bw * bh * 4;              // work lists
+  b += 6 * px * 4;                   // render planes + ping-pong
+  b += 3 * pyr * 4;                  // candidate XYB' pyramid
+  b += 3 * pyr1 * 4;                 // linear RGB pyramid (scales >= 1)
+  b += 3 * px * 4;                   // linear RGB at scale 0 (reference build)
+  size_t tiles = ((w + 63) / 64 + 1) * ((h + 15) / 16 + 1);
+  b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
+  b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
+  b += 256 * 256;  // alignment of every carved buffer
+  return b;
+}
+
+}  // namespace
+
+struct jpxlg_img {
+  jpxlg_ctx* ctx = nullptr;
+  Geom g{};
+  int bits = 8;
+  int gray = 0;
+  int nscales = 0;
+  int sw[kMaxScales], sh[kMaxScales];
+  Arena arena;
+  void* d_rgb = nullptr;
+  float* xyb[3];
+  float* ref_x[kMaxScales][3];
+  float* ref_mu[kMaxScales][3];
+  float* ref_s11[kMaxScales][3];
+  float* coeff[3];
+  int32_t* quant[3];
+  int32_t* lf[3];
+  float* sigma = nullptr;
+  uint8_t* d_bmap = nullptr;
+  uint32_t* d_list[3];
+  int list_count[3] = {0, 0, 0};
+  int8_t* d_cfl_x = nullptr;
+  int8_t* d_cfl_b = nullptr;
+  CflParams* d_cflp = nullptr;
+  ProbeParams* d_pp = nullptr;
+  uint32_t* d_err = nullptr;
+  int* d_gray = nullptr;
+  float* rx[3];
+  float* aux[3];
+  float* cand_x[kMaxScales][3];
+  float* cand_rgb[kMaxScales][3];  // [0] is the reference's linear RGB scratch
+  double* d_partials = nullptr;
+  double* d_sums = nullptr;
+  // host mirrors of the structure
+  std::vector<uint8_t> bmap;
+  std::vector<int8_t> cfl_x, cfl_b;
+  int32_t lf_factors[2] = {0, 0};
+  bool has_structure = false;
+  bool has_quant = false;
+  bool has_render = false;
+  jpxlg_quantizer last_q{};
+  int last_sharpness = 7;
+  ~jpxlg_img() {
+    if (arena.base) (void)hipFree(arena.base);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// Context
+// ----------------------------------------------------------------------------
+
+extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
+  if (!out) return JPXLG_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return JPXLG_E_DEVICE;
+  std::unique_ptr<jpxlg_ctx> ctx(new jpxlg_ctx());
+  ctx->device = device;
+  HIP_TRY(ctx.get(), hipSetDevice(device));
+  HIP_TRY(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  ConstTables host;
+  build_const_tables(&host);
+  HIP_TRY(ctx.get(), hipMalloc(&ctx->d_tab, sizeof(ConstTables)));
+  HIP_TRY(ctx.get(), hipMemcpy(ctx->d_tab, &host, sizeof(ConstTables), hipMemcpyHostToDevice));
+  float lut8[256];
+  srgb8_lut(lut8);
+  HIP_TRY(ctx.get(), hipMalloc(&ctx->d_lut8, sizeof(lut8)));
+  HIP_TRY(ctx.get(), hipMemcpy(ctx->d_lut8, lut8, sizeof(lut8), hipMemcpyHostToDevice));
+  *out = ctx.release();
+  return JPXLG_OK;
+}
+
+extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  (void)hipFree(ctx->d_tab);
+  (void)hipFree(ctx->d_lut8);
+  for (auto& kv : ctx->d_lut16) (void)hipFree(kv.second);
+  for (auto& kv : ctx->depth) {
+    (void)hipFree(kv.second.thr);
+    (void)hipFree(kv.second.lut);
+  }
+  for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+  for (auto e : ctx->pool) (void)hipEventDestroy(e);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+extern "C" const char* jpxlg_last_error(const jpxlg_ctx* ctx) {
+  return ctx ? ctx->last_error.c_str() : "no context";
+}
+
+extern "C" uint64_t jpxlg_image_footprint(uint32_t width, uint32_t height) {
+  return (uint64_t)image_bytes((int)width, (int)height);
+}
+
+static int get_lut16(jpxlg_ctx* ctx, int bits, float** out) {
+  auto it = ctx->d_lut16.find(bits);
+  if (it != ctx->d_lut16.end()) { *out = it->second; return JPXLG_OK; }
+  std::vector<float> lut(65536);
+  srgb16_lut(bits, lut.data());
+  float* d = nullptr;
+  HIP_TRY(ctx, hipMalloc(&d, 65536 * sizeof(float)));
+  HIP_TRY(ctx, hipMemcpy(d, lut.data(), 65536 * sizeof(float), hipMemcpyHostToDevice));
+  ctx->d_lut16[bits] = d;
+  *out = d;
+  return JPXLG_OK;
+}
+
+static int get_depth(jpxlg_ctx* ctx, int bits, DepthSet* out) {
+  auto it = ctx->depth.find(bits);
+  if (it != ctx->depth.end()) { *out = it->second; return JPXLG_OK; }
+  std::vector<float> thr, lut;
+  DepthSet d;
+  d.levels = depth_tables(bits, &thr, &lut);
+  HIP_TRY(ctx, hipMalloc(&d.thr, thr.size() * sizeof(float) + 4));
+  HIP_TRY(ctx, hipMalloc(&d.lut, lut.size() * sizeof(float)));
+  HIP_TRY(ctx, hipMemcpy(d.thr, thr.data(), thr.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(d.lut, lut.data(), lut.size() * sizeof(float), hipMemcpyHostToDevice));
+  ctx->depth[bits] = d;
+  *out = d;
+  return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Images
+// ----------------------------------------------------------------------------
+
+static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_t bits,
+                        const void* rgb, bool device_src, jpxlg_img** out) {
+  if (!ctx || !out || !rgb) return fail(ctx, JPXLG_E_INVALID, "null argument");
+  *out = nullptr;
+  if (width == 0 || height == 0) return fail(ctx, JPXLG_E_INVALID, "a zero frame dimension");
+  if (width < 8 || height < 8)
+    return fail(ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
+  if (width > 65535 * 8 || height > 65535 * 8) return fail(ctx, JPXLG_E_UNSUPPORTED, "frame too large");
+  if (bits < 8 || bits > 16) return fail(ctx, JPXLG_E_UNSUPPORTED, "bit depth outside 8..16");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::unique_ptr<jpxlg_img> img(new jpxlg_img());
+  img->ctx = ctx;
+  img->bits = (int)bits;
+  Geom& g = img->g;
+  g.w = (int)width;
+  g.h = (int)height;
+  g.bw = (g.w + 7) / 8;
+  g.bh = (g.h + 7) / 8;
+  g.cstride = 8 * g.bw;
+  g.crows = 8 * g.bh;
+  g.tw = (g.bw + 7) / 8;
+  g.th = (g.bh + 7) / 8;
+  scale_dims(g.w, g.h, &img->nscales, img->sw, img->sh);
+  const size_t bytes = image_bytes(g.w, g.h);
+  void* base = nullptr;
+  if (hipMalloc(&base, bytes) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
+  img->arena.base = (char*)base;
+  img->arena.size = bytes;
+  Arena& A = img->arena;
+  const size_t px = (size_t)g.w * g.h, bpx = (size_t)g.bw * g.bh, cpx = (size_t)g.cstride * g.crows;
+  const int tiles_cfl = g.tw * g.th;
+  const int sample_bytes = bits == 8 ? 1 : 2;
+  img->d_rgb = A.take(px * 3 * sample_bytes);
+  for (int c = 0; c < 3; c++) {
+    img->xyb[c] = (float*)A.take(px * 4);
+    img->coeff[c] = (float*)A.take(cpx * 4);
+    img->quant[c] = (int32_t*)A.take(cpx * 4);
+    img->lf[c] = (int32_t*)A.take(bpx * 4);
+    img->d_list[c] = (uint32_t*)A.take(bpx * 4);
+    img->rx[c] = (float*)A.take(px * 4);
+    img->aux[c] = (float*)A.take(px * 4);
+  }
+  for (int s = 0; s < img->nscales; s++) {
+    size_t sp = (size_t)img->sw[s] * img->sh[s];
+    for (int c = 0; c < 3; c++) {
+      img->ref_x[s][c] = (float*)A.take(sp * 4);
+      img->ref_mu[s][c] = (float*)A.take(sp * 4);
+      img->ref_s11[s][c] = (float*)A.take(sp * 4);
+      img->cand_x[s][c] = (float*)A.take(sp * 4);
+      img->cand_rgb[s][c] = (float*)A.take(sp * 4);
+    }
+  }
+  img->sigma = (float*)A.take(bpx * 4);
+  img->d_bmap = (uint8_t*)A.take(bpx);
+  img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
+  img->d_cfl_b = (int8_t*)A.take(tiles_cfl);
+  img->d_cflp = (CflParams*)A.take(sizeof(CflParams));
+  img->d_pp = (ProbeParams*)A.take(sizeof(ProbeParams));
+  img->d_err = (uint32_t*)A.take(4);
+  img->d_gray = (int*)A.take(4);
+  size_t ptiles = ((g.w + 63) / 64 + 1) * ((g.h + 15) / 16 + 1);
+  img->d_partials = (double*)A.take(ptiles * 18 * 8);
+  img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
+  bool carved = img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
+                img->d_cflp && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums;
+  for (int c = 0; c < 3; c++) {
+    carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
+             img->d_list[c] && img->rx[c] && img->aux[c];
+    for (int s = 0; s < img->nscales; s++)
+      carved = carved && img->ref_x[s][c] && img->ref_mu[s][c] && img->ref_s11[s][c] &&
+               img->cand_x[s][c] && img->cand_rgb[s][c];
+  }
+  if (!carved) return fail(ctx, JPXLG_E_OOM, "arena sizing");
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_rgb, rgb, px * 3 * sample_bytes,
+                              device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+  const int one = 1;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_gray, &one, 4, hipMemcpyHostToDevice, st));
+  const float* lut = ctx->d_lut8;
+  if (bits != 8) {
+    float* l16 = nullptr;
+    int rc = get_lut16(ctx, (int)bits, &l16);
+    if (rc) return rc;
+    lut = l16;
+  }
+  // Source XYB + linear RGB (scale 0 of the metric reference).
+  float** lin0 = img->cand_rgb[0];
+  { Prof p(ctx, "xyb_fwd"); launch_xyb_fwd(st, g, (int)bits, img->d_rgb, lut, img->xyb, lin0, img->d_gray); }
+  // The reference pyramid (reference.rs:176-283): positive XYB, mu1, s11.
+  for (int s = 0; s < img->nscales; s++) {
+    float** down = s + 1 < img->nscales ? img->cand_rgb[s + 1] : nullptr;
+    { Prof p(ctx, "ref_colour"); launch_colour_metric(st, 1, img->sw[s], img->sh[s], img->cand_rgb[s], nullptr, nullptr, 0,
+                         img->ref_x[s], down); }
+    { Prof p(ctx, "ref_blur_moments"); launch_blur_moments(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s], img->ref_mu[s],
+                        img->ref_s11[s]); }
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(&img->gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  *out = img.release();
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_image_upload_srgb(jpxlg_ctx* ctx, uint32_t width, uint32_t height,
+                                           uint32_t bits, const void* rgb, jpxlg_img** out) {
+  return image_create(ctx, width, height, bits, rgb, false, out);
+}
+
+extern "C" int32_t jpxlg_image_from_device(jpxlg_ctx* ctx, uint32_t width, uint32_t height,
+                                           uint32_t bits, const void* device_rgb, jpxlg_img** out) {
+  return image_create(ctx, width, height, bits, device_rgb, true, out);
+}
+
+extern "C" void jpxlg_image_destroy(jpxlg_img* img) {
+  if (!img) return;
+  (void)hipSetDevice(img->ctx->device);
+  (void)hipStreamSynchronize(img->ctx->stream);
+  delete img;
+}
+
+extern "C" int32_t jpxlg_image_info(const jpxlg_img* img, uint32_t* bx, uint32_t* by,
+                                    int32_t* gray, uint32_t* scales) {
+  if (!img) return JPXLG_E_INVALID;
+  if (bx) *bx = (uint32_t)img->g.bw;
+  if (by) *by = (uint32_t)img->g.bh;
+  if (gray) *gray = img->gray;
+  if (scales) *scales = (uint32_t)img->nscales;
+  return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Structure
+// ----------------------------------------------------------------------------
+
+static int validate_map(const Geom& g, const uint8_t* bmap, std::string* why) {
+  std::vector<uint8_t> covered((size_t)g.bw * g.bh, 0);
+  for (int by = 0; by < g.bh; by++)
+    for (int bx = 0; bx < g.bw; bx++) {
+      int side = bmap[by * g.bw + bx];
+      if (side == 0) continue;
+      if (side != 8 && side != 16 && side != 32) { *why = "block map entry not 0/8/16/32"; return 0; }
+      int n = side / 8;
+      if (bx % n || by % n) { *why = "varblock not aligned to its size"; return 0; }
+      if (bx + n > g.bw || by + n > g.bh) { *why = "varblock leaves the block grid"; return 0; }
+      if ((bx / 8) != (bx + n - 1) / 8 || (by / 8) != (by + n - 1) / 8) { *why = "varblock crosses a 64x64 tile"; return 0; }
+      for (int dy = 0; dy < n; dy++)
+        for (int dx = 0; dx < n; dx++) {
+          uint8_t& c = covered[(by + dy) * g.bw + bx + dx];
+          if (c) { *why = "overlapping varblocks"; return 0; }
+          c = 1;
+          if ((dx || dy) && bmap[(by + dy) * g.bw + bx + dx] != 0) { *why = "covered block carries a size"; return 0; }
+        }
+    }
+  for (uint8_t c : covered)
+    if (!c) { *why = "block map leaves blocks uncovered"; return 0; }
+  return 1;
+}
+
+static int upload_structure(jpxlg_img* img) {
+  jpxlg_ctx* ctx = img->ctx;
+  const Geom& g = img->g;
+  std::vector<uint32_t> lists[3];
+  for (int by = 0; by < g.bh; by++)
+    for (int bx = 0; bx < g.bw; bx++) {
+      int side = img->bmap[by * g.bw + bx];
+      if (side) lists[size_index(side)].push_back(((uint32_t)by << 16) | (uint32_t)bx);
+    }
+  hipStream_t st = ctx->stream;
+  for (int i = 0; i < 3; i++) {
+    img->list_count[i] = (int)lists[i].size();
+    if (!lists[i].empty())
+      HIP_TRY(ctx, hipMemcpyAsync(img->d_list[i], lists[i].data(), lists[i].size() * 4,
+                                  hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_bmap, img->bmap.data(), img->bmap.size(), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_x, img->cfl_x.data(), img->cfl_x.size(), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_b, img->cfl_b.data(), img->cfl_b.size(), hipMemcpyHostToDevice, st));
+  CflParams cp{img->lf_factors[0], img->lf_factors[1]};
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_cflp, &cp, sizeof(cp), hipMemcpyHostToDevice, st));
+  static const char* kDctNames[3] = {"dct_fwd_8", "dct_fwd_16", "dct_fwd_32"};
+  for (int i = 0; i < 3; i++) {
+    if (!img->list_count[i]) continue;
+    Prof p(ctx, kDctNames[i]);
+    launch_dct_fwd(st, kSides[i], g, img->d_list[i], img->list_count[i], ctx->d_tab, img->xyb, img->coeff);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  img->has_structure = true;
+  img->has_quant = false;
+  img->has_render = false;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, const int8_t* cfl_x,
+                                       const int8_t* cfl_b, const int32_t* lf_factors) {
+  if (!img || !bmap) return JPXLG_E_INVALID;
+  const Geom& g = img->g;
+  std::string why;
+  if (!validate_map(g, bmap, &why)) return fail(img->ctx, JPXLG_E_INVALID, why);
+  img->bmap.assign(bmap, bmap + (size_t)g.bw * g.bh);
+  const size_t nt = (size_t)g.tw * g.th;
+  img->cfl_x.assign(nt, 0);
+  img->cfl_b.assign(nt, 0);
+  if (cfl_x) img->cfl_x.assign(cfl_x, cfl_x + nt);
+  if (cfl_b) img->cfl_b.assign(cfl_b, cfl_b + nt);
+  img->lf_factors[0] = lf_factors ? lf_factors[0] : 0;
+  img->lf_factors[1] = lf_factors ? lf_factors[1] : 0;
+  if (img->lf_factors[0] < -128 || img->lf_factors[0] > 127 || img->lf_factors[1] < -128 ||
+      img->lf_factors[1] > 127)
+    return fail(img->ctx, JPXLG_E_RANGE, "LF CfL factor outside u8 after bias");
+  return upload_structure(img);
+}
+
+extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t cover_mode,
+                                        int32_t enable_cfl) {
+  if (!img || !q) return JPXLG_E_INVALID;
+  const Geom& g = img->g;
+  if (cover_mode != JPXLG_COVER_FIXED_DCT8)
+    return fail(img->ctx, JPXLG_E_UNSUPPORTED, "hierarchical cover is not available in this build");
+  (void)enable_cfl;
+  std::vector<uint8_t> bmap((size_t)g.bw * g.bh, 8);
+  return jpxlg_set_structure(img, bmap.data(), nullptr, nullptr, nullptr);
+}
+
+// ----------------------------------------------------------------------------
+// Probe chain
+// ----------------------------------------------------------------------------
+
+static int check_quantizer(jpxlg_ctx* ctx, const jpxlg_quantizer* q) {
+  if (q->global_scale < 1 || q->global_scale > 73728) return fail(ctx, JPXLG_E_RANGE, "global_scale outside 1..73728");
+  if (q->hf_mul < 1) return fail(ctx, JPXLG_E_RANGE, "hf_mul below 1");
+  if (q->quant_lf < 1) return fail(ctx, JPXLG_E_RANGE, "quant_lf below 1");
+  if (q->x_qm_scale > 7 || q->b_qm_scale > 7) return fail(ctx, JPXLG_E_RANGE, "qm_scale outside u(3)");
+  return JPXLG_OK;
+}
+
+static int enqueue_quantize(jpxlg_img* img, const jpxlg_quantizer* q, int sharpness) {
+  jpxlg_ctx* ctx = img->ctx;
+  hipStream_t st = ctx->stream;
+  ProbeParams pp;
+  build_probe_params(*q, sharpness, &pp);
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+  const uint32_t zero = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
+  static const char* kQNames[3] = {"quantize_8", "quantize_16", "quantize_32"};
+  for (int i = 0; i < 3; i++) {
+    if (!img->list_count[i]) continue;
+    Prof p(ctx, kQNames[i]);
+    launch_quantize(st, kSides[i], img->g, img->d_list[i], img->list_count[i], ctx->d_tab, img->d_pp,
+                    img->d_cflp, img->d_cfl_x, img->d_cfl_b, img->coeff, img->quant, img->lf, img->d_err);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  img->last_q = *q;
+  img->last_sharpness = sharpness;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q) {
+  if (!img || !q) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_structure) return fail(ctx, JPXLG_E_INVALID, "no plan structure yet");
+  int rc = check_quantizer(ctx, q);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  rc = enqueue_quantize(img, q, img->last_sharpness);
+  if (rc) return rc;
+  uint32_t err = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
+  img->has_quant = true;
+  img->has_render = false;
+  return JPXLG_OK;
+}
+
+// Enqueues reconstruction + metric for the last quantized plan; the 6 x 18
+// sums land in img->d_sums.
+static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
+  jpxlg_ctx* ctx = img->ctx;
+  hipStream_t st = ctx->stream;
+  const Geom& g = img->g;
+  if (r->sharpness != img->last_sharpness) {
+    // sigma depends on the sharpness: refresh the probe parameters.
+    ProbeParams pp;
+    build_probe_params(img->last_q, r->sharpness, &pp);
+    HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+    img->last_sharpness = r->sharpness;
+  }
+  static const char* kRNames[3] = {"render_8", "render_16", "render_32"};
+  for (int i = 0; i < 3; i++) {
+    if (!img->list_count[i]) continue;
+    Prof p(ctx, kRNames[i]);
+    launch_render(st, kSides[i], g, img->d_list[i], img->list_count[i], ctx->d_tab, img->d_pp,
+                  img->d_cflp, img->d_cfl_x, img->d_cfl_b, img->quant, img->lf, img->rx, img->sigma);
+  }
+  float* cur[3] = {img->rx[0], img->rx[1], img->rx[2]};
+  float* oth[3] = {img->aux[0], img->aux[1], img->aux[2]};
+  if (r->gaborish) {
+    { Prof p(ctx, "gaborish"); launch_gaborish(st, g.w, g.h, cur, oth); }
+    std::swap(cur, oth);
+  }
+  int steps[3], ns = 0;
+  if (r->epf_iters == 1) { steps[0] = 1; ns = 1; }
+  else if (r->epf_iters == 2) { steps[0] = 1; steps[1] = 2; ns = 2; }
+  else if (r->epf_iters >= 3) { steps[0] = 0; steps[1] = 1; steps[2] = 2; ns = 3; }
+  for (int i = 0; i < ns; i++) {
+    static const char* kEpfNames[3] = {"epf_step0", "epf_step1", "epf_step2"};
+    { Prof p(ctx, kEpfNames[steps[i]]); launch_epf(st, steps[i], g.w, g.h, cur, img->sigma, oth); }
+    std::swap(cur, oth);
+  }
+  // Keep the restored planes in rx (downloads read them there).
+  if (cur[0] != img->rx[0])
+    for (int c = 0; c < 3; c++) std::swap(img->rx[c], img->aux[c]);
+  DepthSet d;
+  int rc = get_depth(ctx, img->bits, &d);
+  if (rc) return rc;
+  for (int s = 0; s < img->nscales; s++) {
+    float** down = s + 1 < img->nscales ? img->cand_rgb[s + 1] : nullptr;
+    if (s == 0) {
+      Prof p(ctx, "xyb_inv_depth_metric0");
+      launch_colour_metric(st, 0, img->sw[0], img->sh[0], img->rx, d.thr, d.lut, d.levels,
+                           img->cand_x[0], down);
+    } else {
+      Prof p(ctx, "metric_colour");
+      launch_colour_metric(st, 1, img->sw[s], img->sh[s], img->cand_rgb[s], nullptr, nullptr, 0,
+                           img->cand_x[s], down);
+    }
+    int tiles;
+    {
+      Prof p(ctx, s == 0 ? "blur_pool_s0" : "blur_pool");
+      tiles = launch_blur_pool(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s],
+                               img->ref_mu[s], img->ref_s11[s], img->cand_x[s], img->d_partials);
+    }
+    { Prof p(ctx, "pool_reduce"); launch_reduce_partials(st, tiles, img->d_partials, img->d_sums + s * 18); }
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return JPXLG_OK;
+}
+
+static int finish_score(jpxlg_img* img, double* score, double* raw, double* terms, uint32_t* nterms) {
+  jpxlg_ctx* ctx = img->ctx;
+  double sums[kMaxScales * 18];
+  HIP_TRY(ctx, hipMemcpyAsync(sums, img->d_sums, sizeof(double) * 18 * img->nscales,
+                              hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  MetricResult m = pool_terms(sums, img->nscales, img->sw, img->sh);
+  if (score) *score = m.score;
+  if (raw) *raw = m.raw_error;
+  if (terms) memcpy(terms, m.terms, sizeof(double) * 18 * img->nscales);
+  if (nterms) *nterms = (uint32_t)(18 * img->nscales);
+  img->has_render = true;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_render_score(jpxlg_img* img, const jpxlg_restoration* r, double* score,
+                                      double* raw_error, double* terms, uint32_t* nterms) {
+  if (!img || !r) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  if (r->epf_iters < 0 || r->epf_iters > 3) return fail(ctx, JPXLG_E_UNSUPPORTED, "epf_iters above 3");
+  if (r->sharpness < 0 || r->sharpness > 7) return fail(ctx, JPXLG_E_RANGE, "sharpness outside 0..7");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  int rc = enqueue_render_score(img, r);
+  if (rc) return rc;
+  return finish_score(img, score, raw_error, terms, nterms);
+}
+
+// One probe: quantize + reconstruct + score, one host synchronisation.
+int jpxlg_probe(jpxlg_img* img, const jpxlg_quantizer* q, const jpxlg_restoration* r, double* score) {
+  jpxlg_ctx* ctx = img->ctx;
+  int rc = check_quantizer(ctx, q);
+  if (rc) return rc;
+  rc = enqueue_quantize(img, q, r->sharpness);
+  if (rc) return rc;
+  img->has_quant = true;
+  rc = enqueue_render_score(img, r);
+  if (rc) return rc;
+  uint32_t err = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  rc = finish_score(img, score, nullptr, nullptr, nullptr);
+  if (rc) return rc;
+  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
+  return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Downloads
+// ----------------------------------------------------------------------------
+
+extern "C" int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* lf, uint8_t* bmap,
+                                       int8_t* cfl_x, int8_t* cfl_b, int32_t* lf_factors) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  const Geom& g = img->g;
+  const size_t cpx = (size_t)g.cstride * g.crows, bpx = (size_t)g.bw * g.bh;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (int c = 0; c < 3; c++) {
+    if (quant) HIP_TRY(ctx, hipMemcpyAsync(quant + c * cpx, img->quant[c], cpx * 4, hipMemcpyDeviceToHost, ctx->stream));
+    if (lf) HIP_TRY(ctx, hipMemcpyAsync(lf + c * bpx, img->lf[c], bpx * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  if (bmap) memcpy(bmap, img->bmap.data(), img->bmap.size());
+  if (cfl_x) memcpy(cfl_x, img->cfl_x.data(), img->cfl_x.size());
+  if (cfl_b) memcpy(cfl_b, img->cfl_b.data(), img->cfl_b.size());
+  if (lf_factors) { lf_factors[0] = img->lf_factors[0]; lf_factors[1] = img->lf_factors[1]; }
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* host, size_t bytes) {
+  if (!img || !host) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  const Geom& g = img->g;
+  const size_t px = (size_t)g.w * g.h, cpx = (size_t)g.cstride * g.crows, bpx = (size_t)g.bw * g.bh;
+  float* const* planes = nullptr;
+  size_t per = 0;
+  int count = 3;
+  float* single = nullptr;
+  switch (which) {
+    case JPXLG_BUF_XYB: planes = img->xyb; per = px; break;
+    case JPXLG_BUF_COEFF:
+      if (!img->has_structure) return fail(ctx, JPXLG_E_INVALID, "no plan structure yet");
+      planes = img->coeff; per = cpx; break;
+    case JPXLG_BUF_RENDER_XYB:
+      if (!img->has_render) return fail(ctx, JPXLG_E_INVALID, "nothing rendered yet");
+      planes = img->rx; per = px; break;
+    case JPXLG_BUF_REF_XYB0: planes = img->ref_x[0]; per = px; break;
+    case JPXLG_BUF_SIGMA:
+      if (!img->has_render) return fail(ctx, JPXLG_E_INVALID, "nothing rendered yet");
+      single = img->sigma; per = bpx; count = 1; break;
+    case JPXLG_BUF_LINEAR: {
+      if (!img->has_render) return fail(ctx, JPXLG_E_INVALID, "nothing rendered yet");
+      // Linear RGB at depth is recomputed from the restored planes.
+      if (bytes < 3 * px * 4) return fail(ctx, JPXLG_E_INVALID, "buffer too small");
+      DepthSet d;
+      int rc = get_depth(ctx, img->bits, &d);
+      if (rc) return rc;
+      HIP_TRY(ctx, hipSetDevice(ctx->device));
+      // aux holds scratch after a render; it receives the linear planes here.
+      launch_linear_at_depth(ctx->stream, g.w, g.h, img->rx, d.thr, d.lut, d.levels, img->aux);
+      for (int c = 0; c < 3; c++)
+        HIP_TRY(ctx, hipMemcpyAsync((char*)host + c * px * 4, img->aux[c], px * 4, hipMemcpyDeviceToHost, ctx->stream));
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      return JPXLG_OK;
+    }
+    default: return fail(ctx, JPXLG_E_INVALID, "unknown buffer");
+  }
+  if (bytes < count * per * 4) return fail(ctx, JPXLG_E_INVALID, "buffer too small");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (int c = 0; c < count; c++) {
+    const void* src = single ? (const void*)single : (const void*)planes[c];
+    HIP_TRY(ctx, hipMemcpyAsync((char*)host + c * per * 4, src, per * 4, hipMemcpyDeviceToHost, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Quality controller
+// ----------------------------------------------------------------------------
+
+extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
+                                        jpxlg_quality_result* out) {
+  if (!img || !req || !out) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  memset(out, 0, sizeof(*out));
+  QualityPolicy pol = quality_policy(req->effort);
+  if (!img->has_structure) {
+    jpxlg_quantizer q0 = pol.quantizer_at(Rung::for_global_scale(32768));
+    int rc = jpxlg_plan_structure(img, &q0, pol.cover_mode, 1);
+    if (rc) return rc;
+  }
+  ProbeFn probe = [&](uint32_t rung, double* score, double* ms) -> int {
+    auto t0 = std::chrono::steady_clock::now();
+    jpxlg_quantizer q = pol.quantizer_at(rung);
+    jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
+    int rc = jpxlg_probe(img, &q, &r, score);
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return rc;
+  };
+  ControllerOutcome oc;
+  int rc = req->mode == 1 ? bisect_controller(req->target, req->max_probes, probe, &oc)
+                          : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc);
+  if (rc) return rc;
+  // Leave the chosen plan quantized on the device for the hand-off.
+  jpxlg_quantizer qc = pol.quantizer_at(oc.rung);
+  rc = jpxlg_quantize(img, &qc);
+  if (rc) return rc;
+  out->score = oc.score;
+  out->rung = oc.rung;
+  out->global_scale = qc.global_scale;
+  out->hf_mul = qc.hf_mul;
+  out->quant_lf = qc.quant_lf;
+  out->met = oc.met;
+  out->probes = (int32_t)oc.probes.size();
+  for (size_t i = 0; i < oc.probes.size() && i < 16; i++) {
+    out->probe_scores[i] = oc.probes[i].score;
+    out->probe_rungs[i] = oc.probes[i].rung;
+    out->probe_ms[i] = oc.probes[i].ms;
+  }
+  return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Stand-alone kernels
+// ----------------------------------------------------------------------------
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+extern "C" int32_t jpxlg_dct_blocks(jpxlg_ctx* ctx, int32_t side, int32_t inverse, uint32_t count,
+                                    const float* in, float* out) {
+  if (!ctx || !in || !out) return JPXLG_E_INVALID;
+  if (side != 8 && side != 16 && side != 32) return fail(ctx, JPXLG_E_UNSUPPORTED, "side not 8/16/32");
+  if (count == 0) return JPXLG_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  size_t bytes = (size_t)count * side * side * 4;
+  DevBuf<float> di, dout;
+  HIP_TRY(ctx, hipMalloc(&di.p, bytes));
+  HIP_TRY(ctx, hipMalloc(&dout.p, bytes));
+  HIP_TRY(ctx, hipMemcpyAsync(di.p, in, bytes, hipMemcpyHostToDevice, ctx->stream));
+  launch_dct_blocks(ctx->stream, side, inverse, (int)count, ctx->d_tab, di.p, dout.p);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(out, dout.p, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_epf_step(jpxlg_ctx* ctx, int32_t step, uint32_t width, uint32_t height,
+                                  const float* in, const float* sigma, float* out) {
+  if (!ctx || !in || !sigma || !out) return JPXLG_E_INVALID;
+  if (step < 0 || step > 2) return fail(ctx, JPXLG_E_INVALID, "step outside 0..2");
+  if (width == 0 || height == 0) return fail(ctx, JPXLG_E_INVALID, "a zero frame dimension");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t px = (size_t)width * height, bpx = (size_t)((width + 7) / 8) * ((height + 7) / 8);
+  DevBuf<float> di, ds, dout;
+  HIP_TRY(ctx, hipMalloc(&di.p, 3 * px * 4));
+  HIP_TRY(ctx, hipMalloc(&ds.p, bpx * 4));
+  HIP_TRY(ctx, hipMalloc(&dout.p, 3 * px * 4));
+  HIP_TRY(ctx, hipMemcpyAsync(di.p, in, 3 * px * 4, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(ds.p, sigma, bpx * 4, hipMemcpyHostToDevice, ctx->stream));
+  float* ip[3] = {di.p, di.p + px, di.p + 2 * px};
+  float* op[3] = {dout.p, dout.p + px, dout.p + 2 * px};
+  launch_epf(ctx->stream, step, (int)width, (int)height, ip, ds.p, op);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(out, dout.p, 3 * px * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_gaborish(jpxlg_ctx* ctx, uint32_t width, uint32_t height, const float* in,
+                                  float* out) {
+  if (!ctx || !in || !out) return JPXLG_E_INVALID;
+  if (width == 0 || height == 0) return fail(ctx, JPXLG_E_INVALID, "a zero frame dimension");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t px = (size_t)width * height;
+  DevBuf<float> di, dout;
+  HIP_TRY(ctx, hipMalloc(&di.p, 3 * px * 4));
+  HIP_TRY(ctx, hipMalloc(&dout.p, 3 * px * 4));
+  HIP_TRY(ctx, hipMemcpyAsync(di.p, in, 3 * px * 4, hipMemcpyHostToDevice, ctx->stream));
+  float* ip[3] = {di.p, di.p + px, di.p + 2 * px};
+  float* op[3] = {dout.p, dout.p + px, dout.p + 2 * px};
+  launch_gaborish(ctx->stream, (int)width, (int)height, ip, op);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(out, dout.p, 3 * px * 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t height,
+                                    const float* reference, const float* candidate, double* score,
+                                    double* terms, uint32_t* nterms) {
+  if (!ctx || !reference || !candidate) return JPXLG_E_INVALID;
+  if (width < 8 || height < 8) return fail(ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int w = (int)width, h = (int)height;
+  int n, sw[kMaxScales], sh[kMaxScales];
+  scale_dims(w, h, &n, sw, sh);
+  size_t pyr = 0;
+  for (int s = 0; s < n; s++) pyr += (size_t)sw[s] * sh[s];
+  size_t ptiles = ((w + 63) / 64 + 1) * ((h + 15) / 16 + 1);
+  DevBuf<char> mem;
+  size_t bytes = 18 * pyr * 4 + ptiles * 18 * 8 + kMaxScales * 18 * 8 + 256 * 256;
+  HIP_TRY(ctx, hipMalloc(&mem.p, bytes));
+  Arena A{mem.p, bytes, 0};
+  float *rrgb[kMaxScales][3], *crgb[kMaxScales][3], *rx[kMaxScales][3], *mu[kMaxScales][3],
+      *s11[kMaxScales][3], *cx[kMaxScales][3];
+  for (int s = 0; s < n; s++)
+    for (int c = 0; c < 3; c++) {
+      size_t sp = (size_t)sw[s] * sh[s] * 4;
+      rrgb[s][c] = (float*)A.take(sp); crgb[s][c] = (float*)A.take(sp);
+      rx[s][c] = (float*)A.take(sp); mu[s][c] = (float*)A.take(sp);
+      s11[s][c] = (float*)A.take(sp); cx[s][c] = (float*)A.take(sp);
+    }
+  double* partials = (double*)A.take(ptiles * 18 * 8);
+  double* sums_d = (double*)A.take(kMaxScales * 18 * 8);
+  bool carved = partials && sums_d;
+  for (int s = 0; s < n; s++)
+    for (int c = 0; c < 3; c++)
+      carved = carved && rrgb[s][c] && crgb[s][c] && rx[s][c] && mu[s][c] && s11[s][c] && cx[s][c];
+  if (!carved) return fail(ctx, JPXLG_E_OOM, "arena sizing");
+  const size_t px = (size_t)w * h;
+  hipStream_t st = ctx->stream;
+  for (int c = 0; c < 3; c++) {
+    HIP_TRY(ctx, hipMemcpyAsync(rrgb[0][c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(crgb[0][c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
+  }
+  for (int s = 0; s < n; s++) {
+    float** rd = s + 1 < n ? rrgb[s + 1] : nullptr;
+    float** cd = s + 1 < n ? crgb[s + 1] : nullptr;
+    launch_colour_metric(st, 1, sw[s], sh[s], rrgb[s], nullptr, nullptr, 0, rx[s], rd);
+    launch_colour_metric(st, 1, sw[s], sh[s], crgb[s], nullptr, nullptr, 0, cx[s], cd);
+    launch_blur_moments(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s]);
+    int tiles = launch_blur_pool(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s], cx[s], partials);
+    launch_reduce_partials(st, tiles, partials, sums_d + s * 18);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  double sums[kMaxScales * 18];
+  HIP_TRY(ctx, hipMemcpyAsync(sums, sums_d, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  MetricResult m = pool_terms(sums, n, sw, sh);
+  if (score) *score = m.score;
+  if (terms) memcpy(terms, m.terms, sizeof(double) * 18 * n);
+  if (nterms) *nterms = (uint32_t)(18 * n);
+  return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Per-kernel timing
+// ----------------------------------------------------------------------------
+
+extern "C" int32_t jpxlg_profile_enable(jpxlg_ctx* ctx, int32_t on) {
+  if (!ctx) return JPXLG_E_INVALID;
+  ctx->prof = on != 0;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_profile_read(jpxlg_ctx* ctx, char* names, double* total_ms,
+                                      uint64_t* launches, int32_t cap) {
+  if (!ctx) return JPXLG_E_INVALID;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& r : ctx->recs) {
+    float ms = 0.0f;
+    HIP_TRY(ctx, hipEventElapsedTime(&ms, r.a, r.b));
+    auto& t = ctx->totals[r.name];
+    t.first += ms;
+    t.second += 1;
+    ctx->pool.push_back(r.a);
+    ctx->pool.push_back(r.b);
+  }
+  ctx->recs.clear();
+  int32_t n = 0;
+  for (auto& kv : ctx->totals) {
+    if (n >= cap) break;
+    if (names) {
+      strncpy(names + 48 * n, kv.first.c_str(), 47);
+      names[48 * n + 47] = 0;
+    }
+    if (total_ms) total_ms[n] = kv.second.first;
+    if (launches) launches[n] = kv.second.second;
+    n++;
+  }
+  ctx->totals.clear();
+  return n;
+}

@@ -1,0 +1,101 @@
+// Internal types shared by the host runtime and the kernel translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "device_math.h"
+
+namespace jpxlg {
+
+// Square transform sizes on the path: DCT8x8, DCT16x16, DCT32x32.  Index 0,
+// 1, 2; `cell_base` is each size's offset in the concatenated per-cell tables.
+constexpr int kSides[3] = {8, 16, 32};
+constexpr int kCellBase[3] = {0, 64, 64 + 256};
+constexpr int kCellTotal = 64 + 256 + 1024;
+
+__host__ __device__ constexpr int size_index(int side) { return side == 8 ? 0 : (side == 16 ? 1 : 2); }
+
+// Frame geometry on the 8x8 block grid.
+struct Geom {
+  int w, h;        // pixels
+  int bw, bh;      // 8x8 blocks
+  int cstride;     // coefficient plane stride (8*bw)
+  int crows;       // coefficient plane rows (8*bh)
+  int tw, th;      // 64x64 chroma-from-luma tiles
+};
+
+// Constant tables uploaded once per context (device memory).
+struct ConstTables {
+  DctTables dct;
+  float matrix[3][kCellTotal];    // dequantization matrices per channel, sizes 8/16/32 concatenated
+  float donor[kCellTotal];        // quantizer-donor frequency weights
+  uint16_t order[kCellTotal];     // natural coefficient order (position -> cell)
+  float scale_f[3][4];            // ScaleF(c, n) for n = 1, 2, 4 (index by size), c < n
+  float fir[5];                   // metric blur taps h[0..4] (symmetric)
+};
+
+// Per-probe quantizer parameters (device memory, written before each probe).
+struct ProbeParams {
+  float sc[3];          // Mul * qm[c]: the I.5.3 step scale
+  double lambda[3];     // Lagrange multiplier per channel (bits per squared sample error)
+  float mdc[3];         // I.2.1 LF multipliers
+  float mul;            // I.5.3 Mul (hf_multiplier)
+  float sigma;          // J.4.3 sigma for the uniform sharpness
+  float zero_token_bits;
+  int32_t truncate;     // trailing-truncation pass on/off
+};
+
+// Chroma-from-luma structure of a plan (device memory).
+struct CflParams {
+  int32_t x_factor_lf, b_factor_lf;  // unbiased (-128..127)
+};
+
+// Error word bits raised by kernels (device memory).
+enum : uint32_t { kErrQuantRange = 1u };
+
+// ----------------------------------------------------------------------------
+// Kernel launchers (defined in kernels_pixel.hip / kernels_metric.hip).
+// ----------------------------------------------------------------------------
+
+void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, const float* lut,
+                    float* xyb[3], float* lin[3], int* gray);
+void launch_dct_fwd(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
+                    const ConstTables* tab, float* const xyb[3], float* coeff[3]);
+void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const ConstTables* tab,
+                       const float* in, float* out);
+void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
+                     const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
+                     const int8_t* cfl_x, const int8_t* cfl_b, float* const coeff[3],
+                     int32_t* quant[3], int32_t* lf[3], uint32_t* err);
+void launch_render(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
+                   const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
+                   const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3],
+                   int32_t* const lf[3], float* xyb[3], float* sigma);
+void launch_fill(hipStream_t s, float* p, float v, size_t n);
+void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]);
+void launch_epf(hipStream_t s, int step, int w, int h, float* const in[3], const float* sigma,
+                float* out[3]);
+
+// Colour + metric conversion at one scale.  mode 0: input is XYB (render
+// output) -> opsin inverse -> depth quantizer -> linear; mode 1: input is
+// linear RGB.  Writes positive XYB at this scale and, when `down` is non-null,
+// the 2:1 box-filtered linear RGB of the next scale.
+void launch_colour_metric(hipStream_t s, int mode, int w, int h, float* const in[3],
+                          const float* thresholds, const float* depth_lut, int levels,
+                          float* pxyb[3], float* down[3]);
+// XYB -> linear RGB at the source depth, one plane set (test download).
+void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], const float* thresholds,
+                            const float* depth_lut, int levels, float* out[3]);
+void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
+                         float* mu1[3], float* s11[3]);
+// Blurs the candidate's moments and pools all three maps; writes 6 partial
+// f64 sums per (tile, channel) into `partials` and returns the tile count.
+int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
+                     float* const mu1[3], float* const s11[3], float* const x2[3],
+                     double* partials);
+void launch_reduce_partials(hipStream_t s, int tiles, const double* partials, double* sums18);
+
+}  // namespace jpxlg

@@ -1,0 +1,352 @@
+// gfx950 kernels for the colour stage and SSIMULACRA2: XYB^-1 + depth
+// quantizer, positive XYB, the 2:1 pyramid, the sigma-1.5 Gaussian blur and
+// the fused SSIM / artifact / detail-lost pooling with f64 partial sums.
+//
+// The blur is the 9-tap FIR that the reference's recursive truncated-cosine
+// filter equals in exact arithmetic (jpxl-perceptual/src/blur.rs:1-51): the
+// three poles sit on the unit circle at 2cos(w_k) with w_k = k*pi/10, k odd,
+// so the response to an impulse vanishes beyond |t| = 4.  The taps are the
+// recursion's own impulse response, computed on the host in f64.  Zero padding
+// at the plane border is kept, exactly as the metric defines it.
+#include "internal.h"
+
+namespace jpxlg {
+
+namespace {
+
+inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// jpxl-plan-render/src/lib.rs:341-347 level_of and :421-454 linear_at_depth:
+// the level's linear value for one sample, `lut[#thresholds <= x]`.
+__device__ __forceinline__ float depth_value(float x, const float* thr, const float* lut,
+                                             int levels) {
+  if (!isfinite(x)) return lut[0];
+  int lo = 0, hi = levels - 1;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (thr[mid] <= x) lo = mid + 1;
+    else hi = mid;
+  }
+  return lut[lo];
+}
+
+// jpxl-perceptual/src/color.rs:66-81 linear_srgb_to_positive_xyb.
+__device__ __forceinline__ void positive_xyb(float r, float g, float b, float* ox, float* oy,
+                                             float* ob) {
+  const float bias = 0.0037930734f, bias_cbrt = 0.1559542f;
+  float lm = 0.3f * r + 0.622f * g + 0.078f * b + bias;
+  float mm = 0.23f * r + 0.692f * g + 0.078f * b + bias;
+  float sm = 0.2434227f * r + 0.2047674f * g + 0.5518099f * b + bias;
+  float lg = cbrt_metric(lm) - bias_cbrt;
+  float mg = cbrt_metric(mm) - bias_cbrt;
+  float sg = cbrt_metric(sm) - bias_cbrt;
+  float x = 0.5f * (lg - mg);
+  float y = 0.5f * (lg + mg);
+  *ox = x * 14.0f + 0.42f;
+  *oy = y + 0.01f;
+  *ob = (sg - y) + 0.55f;
+}
+
+// jpxl-core/src/color.rs:673-700 OpsinInverse::convert with the Table L.1
+// defaults of jpxl-plan-render/src/lib.rs:55-68 (itscale = 1).
+__device__ __forceinline__ void opsin_inverse(float x, float y, float b, float root, float* o) {
+  const float bias = -0.0037930733f;
+  const float m[9] = {11.031567f, -9.866944f, -0.16462299f, -3.2541474f, 4.4187705f,
+                      -0.16462299f, -3.6588513f, 2.712923f, 1.9459282f};
+  float gam[3] = {y + x, y - x, b};
+  float mix[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    float v = gam[c] - root;
+    mix[c] = (v * v * v + bias) * 1.0f;
+  }
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc = fmaf(m[r * 3 + c], mix[c], acc);
+    o[r] = acc;
+  }
+}
+
+// One thread per 2x2 quad: converts up to four samples of this scale and
+// writes the next scale's box-filtered linear RGB (pyramid.rs:19-44).
+template <int MODE, bool kLdsThresholds>
+__global__ __launch_bounds__(256) void k_colour_metric(
+    int w, int h, const float* __restrict__ in0, const float* __restrict__ in1,
+    const float* __restrict__ in2, const float* __restrict__ thr_g, const float* __restrict__ lut_g,
+    int levels, float* __restrict__ p0, float* __restrict__ p1, float* __restrict__ p2,
+    float* __restrict__ d0, float* __restrict__ d1, float* __restrict__ d2) {
+  __shared__ float s_thr[kLdsThresholds ? 256 : 1];
+  __shared__ float s_lut[kLdsThresholds ? 256 : 1];
+  const float* thr = thr_g;
+  const float* lut = lut_g;
+  if constexpr (MODE == 0 && kLdsThresholds) {
+    for (int i = threadIdx.x; i < levels; i += blockDim.x) {
+      s_lut[i] = lut_g[i];
+      if (i < levels - 1) s_thr[i] = thr_g[i];
+    }
+    __syncthreads();
+    thr = s_thr;
+    lut = s_lut;
+  }
+  const int w2 = (w + 1) / 2, h2 = (h + 1) / 2;
+  const int ox = blockIdx.x * blockDim.x + threadIdx.x, oy = blockIdx.y;
+  if (ox >= w2 || oy >= h2) return;
+  const float root = MODE == 0 ? cbrt_core(-0.0037930733f) : 0.0f;
+  float lin[2][2][3];
+#pragma unroll
+  for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+    for (int dx = 0; dx < 2; dx++) {
+      const int x = min(2 * ox + dx, w - 1), y = min(2 * oy + dy, h - 1);
+      const size_t i = (size_t)y * w + x;
+      float v[3];
+      if constexpr (MODE == 0) {
+        float o[3];
+        opsin_inverse(in0[i], in1[i], in2[i], root, o);
+#pragma unroll
+        for (int c = 0; c < 3; c++) v[c] = depth_value(o[c], thr, lut, levels);
+      } else {
+        v[0] = in0[i]; v[1] = in1[i]; v[2] = in2[i];
+      }
+      lin[dy][dx][0] = v[0]; lin[dy][dx][1] = v[1]; lin[dy][dx][2] = v[2];
+      if (2 * ox + dx < w && 2 * oy + dy < h) {
+        float a, b, c;
+        positive_xyb(v[0], v[1], v[2], &a, &b, &c);
+        p0[i] = a; p1[i] = b; p2[i] = c;
+      }
+    }
+  if (d0) {
+    const size_t o = (size_t)oy * w2 + ox;
+    float* d[3] = {d0, d1, d2};
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+      d[c][o] = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __restrict__ x,
+                                                         const float* __restrict__ y,
+                                                         const float* __restrict__ b,
+                                                         const float* __restrict__ thr,
+                                                         const float* __restrict__ lut, int levels,
+                                                         float* __restrict__ r, float* __restrict__ g,
+                                                         float* __restrict__ bl) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float root = cbrt_core(-0.0037930733f);
+  float o[3];
+  opsin_inverse(x[i], y[i], b[i], root, o);
+  r[i] = depth_value(o[0], thr, lut, levels);
+  g[i] = depth_value(o[1], thr, lut, levels);
+  bl[i] = depth_value(o[2], thr, lut, levels);
+}
+
+// ----------------------------------------------------------------------------
+// Blur (FIR form of blur.rs) and pooling (pool.rs:97-278)
+// ----------------------------------------------------------------------------
+
+constexpr int kTW = 64, kTH = 16, kHalo = 4;
+constexpr int kLW = kTW + 2 * kHalo, kLH = kTH + 2 * kHalo;
+
+__device__ __forceinline__ float fir9(const float* v, int stride, const float* hf) {
+  // v points at the centre sample.
+  float acc = hf[0] * v[0];
+#pragma unroll
+  for (int t = 1; t <= 4; t++) acc = fmaf(hf[t], v[-t * stride] + v[t * stride], acc);
+  return acc;
+}
+
+__global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstTables* __restrict__ tab,
+                                                      const float* __restrict__ x,
+                                                      float* __restrict__ mu,
+                                                      float* __restrict__ s11) {
+  __shared__ float tin[kLH][kLW];
+  __shared__ float hb[2][kLH][kTW];
+  const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
+  float hf[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
+  for (int e = threadIdx.x; e < kLH * kLW; e += 256) {
+    int ly = e / kLW, lx = e % kLW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
+    tin[ly][lx] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? x[(size_t)gy * w + gx] : 0.0f;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kLH * kTW; e += 256) {
+    int ly = e / kTW, lx = e % kTW;
+    const float* c = &tin[ly][lx + kHalo];
+    float a = hf[0] * c[0], b = hf[0] * (c[0] * c[0]);
+#pragma unroll
+    for (int t = 1; t <= 4; t++) {
+      a = fmaf(hf[t], c[-t] + c[t], a);
+      b = fmaf(hf[t], c[-t] * c[-t] + c[t] * c[t], b);
+    }
+    hb[0][ly][lx] = a;
+    hb[1][ly][lx] = b;
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < kTH * kTW; e += 256) {
+    int ly = e / kTW, lx = e % kTW, gx = x0 + lx, gy = y0 + ly;
+    if (gx >= w || gy >= h) continue;
+    mu[(size_t)gy * w + gx] = fir9(&hb[0][ly + kHalo][lx], kTW, hf);
+    s11[(size_t)gy * w + gx] = fir9(&hb[1][ly + kHalo][lx], kTW, hf);
+  }
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+__global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTables* __restrict__ tab,
+                                                   const float* __restrict__ a0, const float* __restrict__ a1,
+                                                   const float* __restrict__ a2, const float* __restrict__ m0,
+                                                   const float* __restrict__ m1, const float* __restrict__ m2,
+                                                   const float* __restrict__ v0, const float* __restrict__ v1,
+                                                   const float* __restrict__ v2, const float* __restrict__ b0,
+                                                   const float* __restrict__ b1, const float* __restrict__ b2,
+                                                   double* __restrict__ partials) {
+  __shared__ float t1[kLH][kLW];
+  __shared__ float t2[kLH][kLW];
+  __shared__ float hb[3][kLH][kTW];
+  __shared__ double red[4][6];
+  const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
+  const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+  float hf[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
+  const float* X1[3] = {a0, a1, a2};
+  const float* MU1[3] = {m0, m1, m2};
+  const float* S11[3] = {v0, v1, v2};
+  const float* X2[3] = {b0, b1, b2};
+  const float C2 = 0.0009f;
+  for (int c = 0; c < 3; c++) {
+    for (int e = threadIdx.x; e < kLH * kLW; e += 256) {
+      int ly = e / kLW, lx = e % kLW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
+      bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+      size_t o = (size_t)gy * w + gx;
+      t1[ly][lx] = in ? X1[c][o] : 0.0f;
+      t2[ly][lx] = in ? X2[c][o] : 0.0f;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kLH * kTW; e += 256) {
+      int ly = e / kTW, lx = e % kTW;
+      const float* p1 = &t1[ly][lx + kHalo];
+      const float* p2 = &t2[ly][lx + kHalo];
+      float m = hf[0] * p2[0], s22 = hf[0] * (p2[0] * p2[0]), s12 = hf[0] * (p1[0] * p2[0]);
+#pragma unroll
+      for (int t = 1; t <= 4; t++) {
+        m = fmaf(hf[t], p2[-t] + p2[t], m);
+        s22 = fmaf(hf[t], p2[-t] * p2[-t] + p2[t] * p2[t], s22);
+        s12 = fmaf(hf[t], p1[-t] * p2[-t] + p1[t] * p2[t], s12);
+      }
+      hb[0][ly][lx] = m;
+      hb[1][ly][lx] = s22;
+      hb[2][ly][lx] = s12;
+    }
+    __syncthreads();
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    for (int e = threadIdx.x; e < kTH * kTW; e += 256) {
+      int ly = e / kTW, lx = e % kTW, gx = x0 + lx, gy = y0 + ly;
+      if (gx >= w || gy >= h) continue;
+      const size_t o = (size_t)gy * w + gx;
+      const float mu2 = fir9(&hb[0][ly + kHalo][lx], kTW, hf);
+      const float s22 = fir9(&hb[1][ly + kHalo][lx], kTW, hf);
+      const float s12 = fir9(&hb[2][ly + kHalo][lx], kTW, hf);
+      const float i1 = t1[ly + kHalo][lx + kHalo], i2 = t2[ly + kHalo][lx + kHalo];
+      const float mu1 = MU1[c][o], s11 = S11[c][o];
+      const float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
+      const float md = mu1 - mu2;
+      const float num_m = 1.0f - md * md;
+      const float num_s = 2.0f * (s12 - mu12) + C2;
+      const float den_s = (s11 - mu11) + (s22 - mu22) + C2;
+      double d = 1.0 - (double)((num_m * num_s) / den_s);
+      d = d > 0.0 ? d : 0.0;
+      acc[0] += d;
+      acc[1] += d * d * d * d;
+      double d1 = (1.0 + (double)fabsf(i2 - mu2)) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
+      double art = d1 > 0.0 ? d1 : 0.0;
+      double lost = -d1 > 0.0 ? -d1 : 0.0;
+      acc[2] += art;
+      acc[3] += art * art * art * art;
+      acc[4] += lost;
+      acc[5] += lost * lost * lost * lost;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      double v = wave_sum(acc[k]);
+      if (lane == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) {
+      double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+      partials[(size_t)tile * 18 + c * 6 + threadIdx.x] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Fixed-order reduction of the per-tile partial sums: one workgroup per term.
+__global__ __launch_bounds__(256) void k_reduce(int tiles, const double* __restrict__ partials,
+                                                double* __restrict__ sums18) {
+  __shared__ double red[4];
+  const int term = blockIdx.x;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += 256) acc += partials[(size_t)t * 18 + term];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) sums18[term] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+}  // namespace
+
+void launch_colour_metric(hipStream_t s, int mode, int w, int h, float* const in[3],
+                          const float* thresholds, const float* depth_lut, int levels,
+                          float* pxyb[3], float* down[3]) {
+  const int w2 = (w + 1) / 2, h2 = (h + 1) / 2;
+  dim3 grid(div_up(w2, 256), h2);
+  float* d0 = down ? down[0] : nullptr;
+  float* d1 = down ? down[1] : nullptr;
+  float* d2 = down ? down[2] : nullptr;
+  if (mode == 1)
+    k_colour_metric<1, false><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut,
+                                                   levels, pxyb[0], pxyb[1], pxyb[2], d0, d1, d2);
+  else if (levels <= 256)
+    k_colour_metric<0, true><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut,
+                                                  levels, pxyb[0], pxyb[1], pxyb[2], d0, d1, d2);
+  else
+    k_colour_metric<0, false><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut,
+                                                   levels, pxyb[0], pxyb[1], pxyb[2], d0, d1, d2);
+}
+
+void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], const float* thresholds,
+                            const float* depth_lut, int levels, float* out[3]) {
+  int n = w * h;
+  k_linear_at_depth<<<div_up(n, 256), 256, 0, s>>>(n, in[0], in[1], in[2], thresholds, depth_lut,
+                                                   levels, out[0], out[1], out[2]);
+}
+
+void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
+                         float* mu1[3], float* s11[3]) {
+  dim3 grid(div_up(w, kTW), div_up(h, kTH));
+  for (int c = 0; c < 3; c++) k_blur_moments<<<grid, 256, 0, s>>>(w, h, tab, x1[c], mu1[c], s11[c]);
+}
+
+int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
+                     float* const mu1[3], float* const s11[3], float* const x2[3],
+                     double* partials) {
+  dim3 grid(div_up(w, kTW), div_up(h, kTH));
+  k_blur_pool<<<grid, 256, 0, s>>>(w, h, tab, x1[0], x1[1], x1[2], mu1[0], mu1[1], mu1[2], s11[0],
+                                   s11[1], s11[2], x2[0], x2[1], x2[2], partials);
+  return (int)(grid.x * grid.y);
+}
+
+void launch_reduce_partials(hipStream_t s, int tiles, const double* partials, double* sums18) {
+  k_reduce<<<18, 256, 0, s>>>(tiles, partials, sums18);
+}
+
+}  // namespace jpxlg

@@ -1,0 +1,776 @@
+// gfx950 kernels for the VarDCT pixel path: sRGB -> XYB, forward DCT over the
+// block map, quantization (LF, HF nearest rule, trailing truncation, CfL),
+// reconstruction (dequant, CfL, LLF, inverse DCT), Gaborish and the EPF.
+//
+// Layouts in HBM:
+//  * XYB / linear planes: f32, W x H, row-major, stride W.
+//  * Coefficient planes (f32 and i32): the "footprint" layout.  A varblock of
+//    side s at block (bx, by) stores cell k (row k/s, column k%s of its
+//    coefficient matrix) at plane[(8*by + k/s) * 8*bw + 8*bx + k%s].  Every
+//    coefficient of the frame therefore has exactly one slot and the planes
+//    are 8*bw x 8*bh, so each stage reads and writes dense, coalesced rows.
+//  * LF planes: i32, bw x bh per channel.
+//  * Varblock work lists: one list per transform size, entries (by << 16) | bx.
+//
+// One workgroup of 192 threads (three wave64s) processes 64/N varblocks of
+// size N, so every 1-D transform pass keeps all 192 lanes busy (3 channels x
+// N lines x 64/N varblocks).
+#include "internal.h"
+
+namespace jpxlg {
+
+namespace {
+
+constexpr int kThreads = 192;
+
+// ----------------------------------------------------------------------------
+// sRGB -> linear -> XYB  (jpxl-encode-policy/src/source.rs:226-331, :355;
+// jpxl-core/src/color.rs:84-97)
+// ----------------------------------------------------------------------------
+
+template <bool kEight>
+__global__ __launch_bounds__(256) void k_xyb_fwd(int n, const void* __restrict__ rgb,
+                                                 const float* __restrict__ lut,
+                                                 float* __restrict__ X, float* __restrict__ Y,
+                                                 float* __restrict__ B, float* __restrict__ R,
+                                                 float* __restrict__ G, float* __restrict__ Bl,
+                                                 int* __restrict__ gray) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  bool g = true;
+  if (i < n) {
+    uint32_t cr, cg, cb;
+    if constexpr (kEight) {
+      const uint8_t* p = (const uint8_t*)rgb + 3 * (size_t)i;
+      cr = p[0]; cg = p[1]; cb = p[2];
+    } else {
+      const uint16_t* p = (const uint16_t*)rgb + 3 * (size_t)i;
+      cr = p[0]; cg = p[1]; cb = p[2];
+    }
+    g = cr == cg && cg == cb;
+    float r = lut[cr], gg = lut[cg], b = lut[cb];
+    R[i] = r; G[i] = gg; Bl[i] = b;
+    const float bias = 0.0037930734f, bias_cbrt = 0.1559542f;
+    float lm = 0.3f * r + 0.622f * gg + 0.078f * b + bias;
+    float mm = 0.23f * r + 0.692f * gg + 0.078f * b + bias;
+    float sm = 0.2434227f * r + 0.2047674f * gg + 0.5518099f * b + bias;
+    float lg = cbrt_core(lm) - bias_cbrt;
+    float mg = cbrt_core(mm) - bias_cbrt;
+    float sg = cbrt_core(sm) - bias_cbrt;
+    X[i] = 0.5f * (lg - mg);
+    Y[i] = 0.5f * (lg + mg);
+    B[i] = sg;
+  }
+  if (__any(!g) && (threadIdx.x & 63) == 0) atomicAnd(gray, 0);
+}
+
+// ----------------------------------------------------------------------------
+// Square 2-D transforms (jpxl-core/src/dct.rs:956-997) on an LDS tile of
+// N rows x (N+1) padded columns.  Each call is one thread's 1-D line.
+// ----------------------------------------------------------------------------
+
+// DCT_2D pass 1: column `col` of the samples, transformed in place.
+template <int N>
+__device__ __forceinline__ void fwd_pass1(float* tile, int col, const DctTables& t) {
+  constexpr int P = N + 1;
+  float v[N];
+#pragma unroll
+  for (int r = 0; r < N; r++) v[r] = tile[r * P + col];
+  dct_ii<N>(v, t);
+  const float f = fwd_factor<N>();
+#pragma unroll
+  for (int r = 0; r < N; r++) tile[r * P + col] = v[r] * f;
+}
+
+// DCT_2D pass 2: row `row` of the pass-1 result; out[k] is coefficient cell
+// k*N + row (the transposed orientation dct_2d_in_place keeps for squares).
+template <int N>
+__device__ __forceinline__ void fwd_pass2(const float* tile, int row, float* out,
+                                          const DctTables& t) {
+  constexpr int P = N + 1;
+#pragma unroll
+  for (int k = 0; k < N; k++) out[k] = tile[row * P + k];
+  dct_ii<N>(out, t);
+  const float f = fwd_factor<N>();
+#pragma unroll
+  for (int k = 0; k < N; k++) out[k] = out[k] * f;
+}
+
+// IDCT_2D pass 1: coefficient column r (cells k*N + r), inverse along k,
+// written back in place (tile[c][r] then holds the row-r, column-c partial).
+template <int N>
+__device__ __forceinline__ void inv_pass1(float* tile, int r, const DctTables& t) {
+  constexpr int P = N + 1;
+  float v[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) v[k] = tile[k * P + r];
+  dct_iii<N>(v, t);
+  const float f = inv_factor<N>();
+#pragma unroll
+  for (int k = 0; k < N; k++) tile[k * P + r] = v[k] * f;
+}
+
+// IDCT_2D pass 2: sample column c; out[r] is sample (row r, column c).
+template <int N>
+__device__ __forceinline__ void inv_pass2(const float* tile, int c, float* out,
+                                          const DctTables& t) {
+  constexpr int P = N + 1;
+#pragma unroll
+  for (int r = 0; r < N; r++) out[r] = tile[c * P + r];
+  dct_iii<N>(out, t);
+  const float f = inv_factor<N>();
+#pragma unroll
+  for (int r = 0; r < N; r++) out[r] = out[r] * f;
+}
+
+// Small n x n DCT_2D / IDCT_2D held in registers (the LF <-> LLF maps of
+// jpxl-core/src/varblock.rs:817-855 and forward.rs:779-830), same operation
+// order as dct_2d_in_place / idct_2d_in_place.
+template <int n>
+__device__ __forceinline__ void small_dct2_fwd(float* w, const DctTables& t) {
+  if constexpr (n == 1) return;
+  const float f = fwd_factor<n>();
+  float v[n], s[n * n];
+#pragma unroll
+  for (int c = 0; c < n; c++) {
+#pragma unroll
+    for (int r = 0; r < n; r++) v[r] = w[r * n + c];
+    dct_ii<n>(v, t);
+#pragma unroll
+    for (int r = 0; r < n; r++) w[r * n + c] = v[r] * f;
+  }
+#pragma unroll
+  for (int r = 0; r < n; r++)
+#pragma unroll
+    for (int c = 0; c < n; c++) s[c * n + r] = w[r * n + c];
+#pragma unroll
+  for (int r = 0; r < n; r++) {
+#pragma unroll
+    for (int k = 0; k < n; k++) v[k] = s[k * n + r];
+    dct_ii<n>(v, t);
+#pragma unroll
+    for (int k = 0; k < n; k++) s[k * n + r] = v[k] * f;
+  }
+#pragma unroll
+  for (int i = 0; i < n * n; i++) w[i] = s[i];
+}
+
+template <int n>
+__device__ __forceinline__ void small_dct2_inv(float* w, const DctTables& t) {
+  if constexpr (n == 1) return;
+  const float f = inv_factor<n>();
+  float v[n], s[n * n];
+#pragma unroll
+  for (int i = 0; i < n * n; i++) s[i] = w[i];
+#pragma unroll
+  for (int r = 0; r < n; r++) {
+#pragma unroll
+    for (int k = 0; k < n; k++) v[k] = s[k * n + r];
+    dct_iii<n>(v, t);
+#pragma unroll
+    for (int k = 0; k < n; k++) s[k * n + r] = v[k] * f;
+  }
+#pragma unroll
+  for (int r = 0; r < n; r++)
+#pragma unroll
+    for (int c = 0; c < n; c++) w[r * n + c] = s[c * n + r];
+#pragma unroll
+  for (int c = 0; c < n; c++) {
+#pragma unroll
+    for (int r = 0; r < n; r++) v[r] = w[r * n + c];
+    dct_iii<n>(v, t);
+#pragma unroll
+    for (int r = 0; r < n; r++) w[r * n + c] = v[r] * f;
+  }
+}
+
+__device__ __forceinline__ void unpack(uint32_t o, int* bx, int* by) {
+  *bx = (int)(o & 0xffffu);
+  *by = (int)(o >> 16);
+}
+
+// ----------------------------------------------------------------------------
+// Forward DCT over the block map (jpxl-encode-policy/src/lib.rs:3009-3098).
+// ----------------------------------------------------------------------------
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __restrict__ list,
+                                                      int count, const ConstTables* __restrict__ tab,
+                                                      const float* __restrict__ X,
+                                                      const float* __restrict__ Y,
+                                                      const float* __restrict__ B,
+                                                      float* __restrict__ CX,
+                                                      float* __restrict__ CY,
+                                                      float* __restrict__ CB) {
+  constexpr int VB = 64 / N, P = N + 1, NN = N * N;
+  __shared__ float lds[VB * 3 * N * P];
+  const int base = blockIdx.x * VB;
+  const float* planes[3] = {X, Y, B};
+  for (int e = threadIdx.x; e < VB * 3 * NN; e += kThreads) {
+    int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
+    int dy = k / N, dx = k % N;
+    float v = 0.0f;
+    if (base + j < count) {
+      int bx, by;
+      unpack(list[base + j], &bx, &by);
+      int x = min(8 * bx + dx, g.w - 1), y = min(8 * by + dy, g.h - 1);
+      v = planes[c][(size_t)y * g.w + x];
+    }
+    lds[((j * 3 + c) * N + dy) * P + dx] = v;
+  }
+  __syncthreads();
+  const DctTables& t = tab->dct;
+  const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
+  float* tile = lds + (j * 3 + c) * N * P;
+  fwd_pass1<N>(tile, line, t);
+  __syncthreads();
+  float out[N];
+  fwd_pass2<N>(tile, line, out, t);
+  if (base + j < count) {
+    int bx, by;
+    unpack(list[base + j], &bx, &by);
+    float* dst = c == 0 ? CX : (c == 1 ? CY : CB);
+#pragma unroll
+    for (int k = 0; k < N; k++) dst[(size_t)(8 * by + k) * g.cstride + 8 * bx + line] = out[k];
+  }
+}
+
+// Forward or inverse DCT of `count` contiguous N x N blocks, through the same
+// device passes the pipeline uses (the C-ABI's transform entry point).
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_dct_blocks(int inverse, int count,
+                                                         const ConstTables* __restrict__ tab,
+                                                         const float* __restrict__ in,
+                                                         float* __restrict__ out) {
+  constexpr int VB = 64 / N, P = N + 1, NN = N * N;
+  __shared__ float lds[VB * 3 * N * P];
+  const int base = blockIdx.x * VB * 3;  // 3 consecutive blocks per "varblock"
+  for (int e = threadIdx.x; e < VB * 3 * NN; e += kThreads) {
+    int blk = e / NN, k = e % NN;
+    lds[(blk * N + k / N) * P + k % N] = base + blk < count ? in[(size_t)(base + blk) * NN + k] : 0.0f;
+  }
+  __syncthreads();
+  const DctTables& t = tab->dct;
+  const int blk = threadIdx.x / N, line = threadIdx.x % N;
+  float* tile = lds + blk * N * P;
+  float v[N];
+  if (!inverse) {
+    fwd_pass1<N>(tile, line, t);
+    __syncthreads();
+    fwd_pass2<N>(tile, line, v, t);
+    if (base + blk < count)
+#pragma unroll
+      for (int k = 0; k < N; k++) out[(size_t)(base + blk) * NN + k * N + line] = v[k];
+  } else {
+    inv_pass1<N>(tile, line, t);
+    __syncthreads();
+    inv_pass2<N>(tile, line, v, t);
+    if (base + blk < count)
+#pragma unroll
+      for (int r = 0; r < N; r++) out[(size_t)(base + blk) * NN + r * N + line] = v[r];
+  }
+}
+
+// ----------------------------------------------------------------------------
+// Quantization (jpxl-encode-policy/src/lib.rs:3237-3384 quantize_square_varblock;
+// quantize.rs:150-213 LfQuantizer, :1400-1499 choose, :468-545 truncate_trailing)
+// ----------------------------------------------------------------------------
+
+__device__ __constant__ float kQuantBias[3] = {1.0f - 0.054650073f, 1.0f - 0.0700545f,
+                                               1.0f - 0.049935103f};
+
+// I.5.3 nearest-reconstruction choice, ties to the smaller magnitude.
+__device__ __forceinline__ int32_t hf_choose(float target, float step, float bias, int* bad) {
+  float thr = 0.5f * bias * step;
+  if (fabsf(target) <= thr) return 0;
+  int32_t e = clamp_round(target / step, bad);
+  int32_t cand[4] = {0, e - 1, e, e + 1};
+  int32_t best = 0;
+  float best_err = INFINITY;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    int32_t v = cand[i];
+    if (abs(v) > kMaxQuant) continue;
+    float err = fabsf(bias_adjust(v, bias) * step - target);
+    if (err < best_err || (err == best_err && abs(v) < abs(best))) {
+      best = v;
+      best_err = err;
+    }
+  }
+  return best;
+}
+
+__device__ __forceinline__ int prev_nonzero(const int32_t* q, const uint16_t* order, int first,
+                                            int before) {
+  for (int pos = before - 1; pos >= first; pos--)
+    if (q[order[pos]] != 0) return pos;
+  return -1;
+}
+
+// Trailing-truncation backward pass over one varblock channel (one thread).
+template <int N>
+__device__ void truncate_walk(int32_t* q, const float* targets, float* recon_y,
+                              const uint16_t* order, const float* mat, const float* donor,
+                              float sc, float bias, double lambda, float ztb) {
+  constexpr int NN = N * N, nb = (N / 8) * (N / 8);
+  int last = prev_nonzero(q, order, nb, NN);
+  while (last >= 0) {
+    int cell = order[last];
+    int32_t v = q[cell];
+    float weight = (float)(lambda * (double)NN * (double)donor[cell]);
+    int previous = prev_nonzero(q, order, nb, last);
+    int exposed = previous >= 0 ? last - previous - 1 : last - nb;
+    float zeros = (float)exposed;
+    float own = (float)(33 - __clz((unsigned)abs(v)));
+    float saved = own + zeros * ztb;
+    float target = targets[cell];
+    float step = sc * mat[cell];
+    float recon = bias_adjust(v, bias) * step;
+    float kept = recon - target;
+    float added = fmaf(target, target, -(kept * kept));
+    float cost = weight * added;
+    if (saved <= cost) break;
+    q[cell] = 0;
+    if (recon_y) recon_y[cell] = bias_adjust(0, bias) * step;
+    last = previous;
+  }
+}
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_quantize(
+    Geom g, const uint32_t* __restrict__ list, int count, const ConstTables* __restrict__ tab,
+    const ProbeParams* __restrict__ pp, const CflParams* __restrict__ cflp,
+    const int8_t* __restrict__ cfl_x, const int8_t* __restrict__ cfl_b, const float* __restrict__ CX,
+    const float* __restrict__ CY, const float* __restrict__ CB, int32_t* __restrict__ QX,
+    int32_t* __restrict__ QY, int32_t* __restrict__ QB, int32_t* __restrict__ LX,
+    int32_t* __restrict__ LY, int32_t* __restrict__ LB, uint32_t* __restrict__ err) {
+  constexpr int VB = 64 / N, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  __shared__ float co[3][VB][NN];
+  __shared__ int32_t qs[3][VB][NN];
+  __shared__ float dyh[VB][NN];
+  __shared__ float dylf[VB][16];
+  __shared__ float khf[2][VB];
+  __shared__ int vbx[VB], vby[VB];
+  const int base = blockIdx.x * VB;
+  const int nvb = min(VB, count - base);
+  int bad = 0;
+  if (threadIdx.x < VB) {
+    int bx = 0, by = 0;
+    if (threadIdx.x < nvb) unpack(list[base + threadIdx.x], &bx, &by);
+    vbx[threadIdx.x] = bx;
+    vby[threadIdx.x] = by;
+    int tile = (by / 8) * g.tw + bx / 8;
+    khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
+    khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
+  }
+  __syncthreads();
+  const float* cin[3] = {CX, CY, CB};
+  for (int e = threadIdx.x; e < 3 * VB * NN; e += kThreads) {
+    int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
+    float v = 0.0f;
+    if (j < nvb) v = cin[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N];
+    co[c][j][k] = v;
+  }
+  __syncthreads();
+  const DctTables& t = tab->dct;
+  const float* sf = tab->scale_f[si];
+  const float mdc0 = pp->mdc[0], mdc1 = pp->mdc[1], mdc2 = pp->mdc[2];
+  // --- Y LF (one thread per varblock) ---
+  if (threadIdx.x < nvb) {
+    const int j = threadIdx.x;
+    float lf[n * n];
+#pragma unroll
+    for (int y = 0; y < n; y++)
+#pragma unroll
+      for (int x = 0; x < n; x++)
+        lf[y * n + x] = n == 1 ? co[1][j][0] : co[1][j][y * N + x] / (sf[y] * sf[x]);
+    small_dct2_inv<n>(lf, t);
+#pragma unroll
+    for (int i = 0; i < n * n; i++) {
+      int32_t v = clamp_round(lf[i] * 1.0f / mdc1, &bad);
+      LY[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = v;
+      dylf[j][i] = mdc1 * (float)v / 1.0f;
+    }
+  }
+  // --- Y HF ---
+  {
+    const float sc = pp->sc[1], bias = kQuantBias[1];
+    for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
+      int j = e / NN, k = e % NN;
+      float step = sc * tab->matrix[1][cb + k];
+      int32_t v = 0;
+      if ((k % N) >= n || (k / N) >= n) v = hf_choose(co[1][j][k], step, bias, &bad);
+      qs[1][j][k] = v;
+      dyh[j][k] = bias_adjust(v, bias) * step;
+    }
+  }
+  __syncthreads();
+  if (pp->truncate && threadIdx.x < nvb) {
+    const int j = threadIdx.x;
+    truncate_walk<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
+                     tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
+  }
+  __syncthreads();
+  // --- X and B LF (one thread per varblock channel) ---
+  if (threadIdx.x < 2 * VB && (threadIdx.x % VB) < nvb) {
+    const int ci = threadIdx.x / VB, j = threadIdx.x % VB, c = ci == 0 ? 0 : 2;
+    const float klf = ci == 0 ? 0.0f + (float)cflp->x_factor_lf / 84.0f
+                              : 1.0f + (float)cflp->b_factor_lf / 84.0f;
+    const float mdc = ci == 0 ? mdc0 : mdc2;
+    float lf[n * n];
+#pragma unroll
+    for (int y = 0; y < n; y++)
+#pragma unroll
+      for (int x = 0; x < n; x++)
+        lf[y * n + x] = n == 1 ? co[c][j][0] : co[c][j][y * N + x] / (sf[y] * sf[x]);
+    small_dct2_inv<n>(lf, t);
+    int32_t* L = ci == 0 ? LX : LB;
+#pragma unroll
+    for (int i = 0; i < n * n; i++) {
+      float target = lf[i] - klf * dylf[j][i];
+      L[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = clamp_round(target * 1.0f / mdc, &bad);
+    }
+  }
+  __syncthreads();
+  // --- X and B HF: targets against the reconstructed dY, then choose ---
+  for (int e = threadIdx.x; e < 2 * VB * NN; e += kThreads) {
+    int ci = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN, c = ci == 0 ? 0 : 2;
+    int32_t v = 0;
+    if ((k % N) >= n || (k / N) >= n) {
+      float target = co[c][j][k] - khf[ci][j] * dyh[j][k];
+      co[c][j][k] = target;
+      float step = pp->sc[c] * tab->matrix[c][cb + k];
+      v = hf_choose(target, step, kQuantBias[c], &bad);
+    }
+    qs[c][j][k] = v;
+  }
+  __syncthreads();
+  if (pp->truncate && threadIdx.x < 2 * VB && (threadIdx.x % VB) < nvb) {
+    const int ci = threadIdx.x / VB, j = threadIdx.x % VB, c = ci == 0 ? 0 : 2;
+    truncate_walk<N>(qs[c][j], co[c][j], nullptr, tab->order + cb, tab->matrix[c] + cb,
+                     tab->donor + cb, pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
+  }
+  __syncthreads();
+  int32_t* qout[3] = {QX, QY, QB};
+  for (int e = threadIdx.x; e < 3 * VB * NN; e += kThreads) {
+    int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
+    if (j < nvb) qout[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N] = qs[c][j][k];
+  }
+  if (bad) atomicOr(err, kErrQuantRange);
+}
+
+// ----------------------------------------------------------------------------
+// Reconstruction (jpxl-plan-render/src/lib.rs:853-1032, :1294-1359)
+// ----------------------------------------------------------------------------
+
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_render(
+    Geom g, const uint32_t* __restrict__ list, int count, const ConstTables* __restrict__ tab,
+    const ProbeParams* __restrict__ pp, const CflParams* __restrict__ cflp,
+    const int8_t* __restrict__ cfl_x, const int8_t* __restrict__ cfl_b,
+    const int32_t* __restrict__ QX, const int32_t* __restrict__ QY, const int32_t* __restrict__ QB,
+    const int32_t* __restrict__ LX, const int32_t* __restrict__ LY, const int32_t* __restrict__ LB,
+    float* __restrict__ X, float* __restrict__ Y, float* __restrict__ B, float* __restrict__ sigma) {
+  constexpr int VB = 64 / N, P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  __shared__ float lds[VB * 3 * N * P];
+  __shared__ int vbx[VB], vby[VB];
+  __shared__ float khf[2][VB];
+  const int base = blockIdx.x * VB;
+  const int nvb = min(VB, count - base);
+  if (threadIdx.x < VB) {
+    int bx = 0, by = 0;
+    if (threadIdx.x < nvb) unpack(list[base + threadIdx.x], &bx, &by);
+    vbx[threadIdx.x] = bx;
+    vby[threadIdx.x] = by;
+    int tile = (by / 8) * g.tw + bx / 8;
+    khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
+    khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
+  }
+  __syncthreads();
+  // I.5.3 dequantization of all three channels, then I.6 HF CfL.
+  const float s0 = pp->sc[0], s1 = pp->sc[1], s2 = pp->sc[2];
+  for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
+    int j = e / NN, k = e % NN;
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+    if (j < nvb) {
+      size_t o = (size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N;
+      d0 = bias_adjust(QX[o], kQuantBias[0]) * s0 * tab->matrix[0][cb + k];
+      d1 = bias_adjust(QY[o], kQuantBias[1]) * s1 * tab->matrix[1][cb + k];
+      d2 = bias_adjust(QB[o], kQuantBias[2]) * s2 * tab->matrix[2][cb + k];
+    }
+    int r = k / N, col = k % N;
+    lds[((j * 3 + 0) * N + r) * P + col] = d0 + khf[0][j] * d1;
+    lds[((j * 3 + 1) * N + r) * P + col] = d1;
+    lds[((j * 3 + 2) * N + r) * P + col] = d2 + khf[1][j] * d1;
+  }
+  __syncthreads();
+  const DctTables& t = tab->dct;
+  // I.8: the LLF cells from the dequantized, LF-CfL-corrected LF planes.
+  if (threadIdx.x < 3 * VB) {
+    const int j = threadIdx.x / 3, c = threadIdx.x % 3;
+    if (j < nvb) {
+      const float kxl = 0.0f + (float)cflp->x_factor_lf / 84.0f;
+      const float kbl = 1.0f + (float)cflp->b_factor_lf / 84.0f;
+      float lf[n * n];
+#pragma unroll
+      for (int dy = 0; dy < n; dy++)
+#pragma unroll
+        for (int dx = 0; dx < n; dx++) {
+          size_t o = (size_t)(vby[j] + dy) * g.bw + vbx[j] + dx;
+          float vy = pp->mdc[1] * (float)LY[o] / 1.0f;
+          float v;
+          if (c == 1) v = vy;
+          else if (c == 0) v = pp->mdc[0] * (float)LX[o] / 1.0f + kxl * vy;
+          else v = pp->mdc[2] * (float)LB[o] / 1.0f + kbl * vy;
+          lf[dy * n + dx] = v;
+        }
+      small_dct2_fwd<n>(lf, t);
+      const float* sf = tab->scale_f[si];
+      float* tile = lds + (j * 3 + c) * N * P;
+#pragma unroll
+      for (int y = 0; y < n; y++)
+#pragma unroll
+        for (int x = 0; x < n; x++)
+          tile[y * P + x] = n == 1 ? lf[0] : lf[y * n + x] * sf[y] * sf[x];
+    }
+  }
+  __syncthreads();
+  // I.9: IDCT_2D.
+  const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
+  float* tile = lds + (j * 3 + c) * N * P;
+  inv_pass1<N>(tile, line, t);
+  __syncthreads();
+  float out[N];
+  inv_pass2<N>(tile, line, out, t);
+  if (j < nvb) {
+    float* dst = c == 0 ? X : (c == 1 ? Y : B);
+    const int fx = 8 * vbx[j] + line;
+    if (fx < g.w) {
+#pragma unroll
+      for (int r = 0; r < N; r++) {
+        int fy = 8 * vby[j] + r;
+        if (fy < g.h) dst[(size_t)fy * g.w + fx] = out[r];
+      }
+    }
+  }
+  // J.4.3 sigma of every 8x8 block the varblock covers.
+  if (threadIdx.x < VB * n * n) {
+    const int jj = threadIdx.x / (n * n), b = threadIdx.x % (n * n);
+    if (jj < nvb) {
+      int sx = vbx[jj] + b % n, sy = vby[jj] + b / n;
+      if (sx < g.bw && sy < g.bh) sigma[(size_t)sy * g.bw + sx] = pp->sigma;
+    }
+  }
+}
+
+__global__ void k_fill(float* p, float v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ----------------------------------------------------------------------------
+// J.3 Gaborish smoothing (jpxl-core/src/restoration.rs:148-193)
+// ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_gaborish(int w, int h, const float* __restrict__ in0,
+                                                  const float* __restrict__ in1,
+                                                  const float* __restrict__ in2,
+                                                  float* __restrict__ o0, float* __restrict__ o1,
+                                                  float* __restrict__ o2) {
+  const float w1 = 0.115169525f, w2 = 0.061248592f;
+  const float sum = fmaf(4.0f, w2, fmaf(4.0f, w1, 1.0f));
+  const float scale = 1.0f / sum;
+  const float ce = scale, ed = w1 * scale, co = w2 * scale;
+  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+  int c = blockIdx.z;
+  if (x >= w) return;
+  const float* in = c == 0 ? in0 : (c == 1 ? in1 : in2);
+  float* out = c == 0 ? o0 : (c == 1 ? o1 : o2);
+  float acc = 0.0f;
+  for (int dy = -1; dy <= 1; dy++) {
+    const float* row = in + (size_t)mirror1d(y + dy, h) * w;
+    for (int dx = -1; dx <= 1; dx++) {
+      float wgt = (dx == 0 && dy == 0) ? ce : ((dx == 0 || dy == 0) ? ed : co);
+      acc = fmaf(wgt, row[mirror1d(x + dx, w)], acc);
+    }
+  }
+  out[(size_t)y * w + x] = acc;
+}
+
+// ----------------------------------------------------------------------------
+// J.4 edge-preserving filter step (jpxl-core/src/restoration.rs:583-731)
+// ----------------------------------------------------------------------------
+
+constexpr int kEpfTW = 64, kEpfTH = 4, kEpfR = 3;
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restrict__ in0,
+                                             const float* __restrict__ in1,
+                                             const float* __restrict__ in2,
+                                             const float* __restrict__ sigma,
+                                             float* __restrict__ o0, float* __restrict__ o1,
+                                             float* __restrict__ o2) {
+  constexpr int LW = kEpfTW + 2 * kEpfR, LH = kEpfTH + 2 * kEpfR;
+  __shared__ float tile[3][LH][LW];
+  const int x0 = blockIdx.x * kEpfTW, y0 = blockIdx.y * kEpfTH;
+  const float* in[3] = {in0, in1, in2};
+  for (int e = threadIdx.x; e < 3 * LH * LW; e += 256) {
+    int c = e / (LH * LW), rem = e % (LH * LW), ly = rem / LW, lx = rem % LW;
+    int sy = mirror1d(y0 + ly - kEpfR, h), sx = mirror1d(x0 + lx - kEpfR, w);
+    tile[c][ly][lx] = in[c][(size_t)sy * w + sx];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % kEpfTW, ty = threadIdx.x / kEpfTW;
+  const int x = x0 + tx, y = y0 + ty;
+  if (x >= w || y >= h) return;
+  const int bw = (w + 7) / 8;
+  const size_t idx = (size_t)y * w + x;
+  const int cx = tx + kEpfR, cy = ty + kEpfR;
+  float* out[3] = {o0, o1, o2};
+  const float s = sigma[(y / 8) * bw + x / 8];
+  if (s < 0.3f) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) out[c][idx] = tile[c][cy][cx];
+    return;
+  }
+  constexpr float kScale = STEP == 0 ? 0.9f : (STEP == 1 ? 1.0f : 6.5f);
+  const float step_mul = 1.65f * kScale;
+  const float inv_sigma = step_mul * 4.0f * (1.0f - __fsqrt_rn(0.5f)) / s;
+  const int rx = x % 8, ry = y % 8;
+  const bool at_border = ry == 0 || ry == 7 || rx == 0 || rx == 7;
+  const float pm = at_border ? 2.0f / 3.0f : 1.0f;
+  constexpr int kTaps = STEP == 0 ? 13 : 5;
+  const int tapx[13] = {0, -1, 1, 0, 0, 1, 1, -1, -1, -2, 2, 0, 0};
+  const int tapy[13] = {0, 0, 0, -1, 1, -1, 1, 1, -1, 0, 0, 2, -2};
+  const float cscale[3] = {40.0f, 5.0f, 3.5f};
+  float sum_w = 0.0f, acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
+#pragma unroll
+  for (int tp = 0; tp < kTaps; tp++) {
+    const int kx = tapx[tp], ky = tapy[tp];
+    float d = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      if constexpr (STEP == 2) {
+        d = fmaf(fabsf(tile[c][cy][cx] - tile[c][cy + ky][cx + kx]), cscale[c], d);
+      } else {
+#pragma unroll
+        for (int jx = 0; jx < 5; jx++) {
+          const int ix = tapx[jx], iy = tapy[jx];
+          d = fmaf(fabsf(tile[c][cy + iy][cx + ix] - tile[c][cy + ky + iy][cx + kx + ix]), cscale[c], d);
+        }
+      }
+    }
+    const float sd = pm * d;
+    float wgt;
+    if (sd == 0.0f) {
+      wgt = 1.0f;
+    } else {
+      const float v = fmaf(sd, -inv_sigma, 1.0f);
+      wgt = v > 0.0f ? v : 0.0f;
+    }
+    sum_w += wgt;
+    acc0 = fmaf(tile[0][cy + ky][cx + kx], wgt, acc0);
+    acc1 = fmaf(tile[1][cy + ky][cx + kx], wgt, acc1);
+    acc2 = fmaf(tile[2][cy + ky][cx + kx], wgt, acc2);
+  }
+  out[0][idx] = acc0 / sum_w;
+  out[1][idx] = acc1 / sum_w;
+  out[2][idx] = acc2 / sum_w;
+}
+
+inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+}  // namespace
+
+// ----------------------------------------------------------------------------
+// Launchers
+// ----------------------------------------------------------------------------
+
+void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, const float* lut,
+                    float* xyb[3], float* lin[3], int* gray) {
+  int n = g.w * g.h;
+  if (bits == 8)
+    k_xyb_fwd<true><<<div_up(n, 256), 256, 0, s>>>(n, rgb, lut, xyb[0], xyb[1], xyb[2], lin[0],
+                                                   lin[1], lin[2], gray);
+  else
+    k_xyb_fwd<false><<<div_up(n, 256), 256, 0, s>>>(n, rgb, lut, xyb[0], xyb[1], xyb[2], lin[0],
+                                                    lin[1], lin[2], gray);
+}
+
+void launch_dct_fwd(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
+                    const ConstTables* tab, float* const xyb[3], float* coeff[3]) {
+  if (count <= 0) return;
+  switch (side) {
+    case 8: k_dct_fwd<8><<<div_up(count, 8), kThreads, 0, s>>>(g, list, count, tab, xyb[0], xyb[1], xyb[2], coeff[0], coeff[1], coeff[2]); break;
+    case 16: k_dct_fwd<16><<<div_up(count, 4), kThreads, 0, s>>>(g, list, count, tab, xyb[0], xyb[1], xyb[2], coeff[0], coeff[1], coeff[2]); break;
+    default: k_dct_fwd<32><<<div_up(count, 2), kThreads, 0, s>>>(g, list, count, tab, xyb[0], xyb[1], xyb[2], coeff[0], coeff[1], coeff[2]); break;
+  }
+}
+
+void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const ConstTables* tab,
+                       const float* in, float* out) {
+  if (count <= 0) return;
+  switch (side) {
+    case 8: k_dct_blocks<8><<<div_up(count, 24), kThreads, 0, s>>>(inverse, count, tab, in, out); break;
+    case 16: k_dct_blocks<16><<<div_up(count, 12), kThreads, 0, s>>>(inverse, count, tab, in, out); break;
+    default: k_dct_blocks<32><<<div_up(count, 6), kThreads, 0, s>>>(inverse, count, tab, in, out); break;
+  }
+}
+
+void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
+                     const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
+                     const int8_t* cfl_x, const int8_t* cfl_b, float* const coeff[3],
+                     int32_t* quant[3], int32_t* lf[3], uint32_t* err) {
+  if (count <= 0) return;
+#define JPXLG_Q(NS, VBN)                                                                          \
+  k_quantize<NS><<<div_up(count, VBN), kThreads, 0, s>>>(g, list, count, tab, pp, cfl, cfl_x,   \
+                                                         cfl_b, coeff[0], coeff[1], coeff[2],   \
+                                                         quant[0], quant[1], quant[2], lf[0],   \
+                                                         lf[1], lf[2], err)
+  switch (side) {
+    case 8: JPXLG_Q(8, 8); break;
+    case 16: JPXLG_Q(16, 4); break;
+    default: JPXLG_Q(32, 2); break;
+  }
+#undef JPXLG_Q
+}
+
+void launch_render(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
+                   const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
+                   const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3],
+                   int32_t* const lf[3], float* xyb[3], float* sigma) {
+  if (count <= 0) return;
+#define JPXLG_R(NS, VBN)                                                                         \
+  k_render<NS><<<div_up(count, VBN), kThreads, 0, s>>>(g, list, count, tab, pp, cfl, cfl_x,    \
+                                                       cfl_b, quant[0], quant[1], quant[2],    \
+                                                       lf[0], lf[1], lf[2], xyb[0], xyb[1],    \
+                                                       xyb[2], sigma)
+  switch (side) {
+    case 8: JPXLG_R(8, 8); break;
+    case 16: JPXLG_R(16, 4); break;
+    default: JPXLG_R(32, 2); break;
+  }
+#undef JPXLG_R
+}
+
+void launch_fill(hipStream_t s, float* p, float v, size_t n) {
+  if (n == 0) return;
+  k_fill<<<div_up(n, 256), 256, 0, s>>>(p, v, n);
+}
+
+void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]) {
+  dim3 grid(div_up(w, 256), h, 3);
+  k_gaborish<<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], out[0], out[1], out[2]);
+}
+
+void launch_epf(hipStream_t s, int step, int w, int h, float* const in[3], const float* sigma,
+                float* out[3]) {
+  dim3 grid(div_up(w, kEpfTW), div_up(h, kEpfTH));
+  if (step == 0)
+    k_epf<0><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], sigma, out[0], out[1], out[2]);
+  else if (step == 1)
+    k_epf<1><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], sigma, out[0], out[1], out[2]);
+  else
+    k_epf<2><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], sigma, out[0], out[1], out[2]);
+}
+
+}  // namespace jpxlg

