@@ -72,10 +72,10 @@ def cpu_baseline(rgb: np.ndarray, probes: int, budget_s: float = 25.0) -> dict:
     h, w, _ = rgb.shape
     t0 = time.perf_counter()
     xyb, lin, _ = orc.prepare_frame(rgb, BITS)
-    bmap = orc.fixed_dct8_map(w, h)
+    q = orc.QuantParams(900, 1, 4, 3, 3, 1, 4.0, 1.0)
+    bmap = orc.cover(xyb, q)
     coeff = orc.forward_frame(xyb, bmap)
     t_prep = time.perf_counter() - t0
-    q = orc.QuantParams(900, 1, 4, 3, 3, 1, 4.0, 1.0)
     t1 = time.perf_counter()
     quant, lf = orc.quantize_frame(w, h, bmap, coeff, q)
     rp = orc.RenderParams(900, 1, 4, 3, 3, 1, 0, 7, BITS)
@@ -89,7 +89,7 @@ def cpu_baseline(rgb: np.ndarray, probes: int, budget_s: float = 25.0) -> dict:
         "unit": "MP/s",
         "cores": 1,
         "kind": "port",
-        "sample": f"oracle C restatement, {w}x{h} RGB{BITS}: prepare+DCT8 ({t_prep:.2f}s) + one probe "
+        "sample": f"oracle C restatement, {w}x{h} RGB{BITS}: prepare+cover+DCT ({t_prep:.2f}s) + one probe "
                   f"quantize+render+SSIMULACRA2 ({t_probe:.2f}s), scaled to {probes} probes",
     }
 

@@ -253,6 +253,13 @@ class Image:
                                                         _ptr(cb), _ptr(lff)))
         return Plan(q, lf, bm, cx, cb, (int(lff[0]), int(lff[1])))
 
+    def plan_download_map(self) -> np.ndarray:
+        """The current block map (valid right after plan_structure)."""
+        bm = np.empty((self.bh, self.bw), np.uint8)
+        q = self.ctx.lib
+        self.ctx.check(q.jpxlg_plan_download(self.handle, None, None, _ptr(bm), None, None, None))
+        return bm
+
     def download(self, which: int) -> np.ndarray:
         if which in (_lib.BUF_XYB, _lib.BUF_RENDER_XYB, _lib.BUF_LINEAR, _lib.BUF_REF_XYB0):
             out = np.empty((3, self.height, self.width), np.float32)

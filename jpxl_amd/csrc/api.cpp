@@ -135,6 +135,7 @@ size_t image_bytes(int w, int h) {
   b += 3 * cpx * 4 * 2;              // coefficients f32 + quantized i32
   b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
   b += 3 * bw * bh * 4;              // work lists
+  b += 3 * bw * bh * 8;              // cover candidate costs (f64)
   b += 6 * px * 4;                   // render planes + ping-pong
   b += 3 * pyr * 4;                  // candidate XYB' pyramid
   b += 3 * pyr1 * 4;                 // linear RGB pyramid (scales >= 1)
@@ -180,6 +181,7 @@ struct jpxlg_img {
   float* cand_rgb[kMaxScales][3];  // [0] is the reference's linear RGB scratch
   double* d_partials = nullptr;
   double* d_sums = nullptr;
+  double* d_cost[3] = {nullptr, nullptr, nullptr};
   // host mirrors of the structure
   std::vector<uint8_t> bmap;
   std::vector<int8_t> cfl_x, cfl_b;
@@ -329,6 +331,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     }
   }
   img->sigma = (float*)A.take(bpx * 4);
+  for (int i = 0; i < 3; i++) img->d_cost[i] = (double*)A.take(bpx * 8);
   img->d_bmap = (uint8_t*)A.take(bpx);
   img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
   img->d_cfl_b = (int8_t*)A.take(tiles_cfl);
@@ -339,7 +342,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   size_t ptiles = ((g.w + 63) / 64 + 1) * ((g.h + 15) / 16 + 1);
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
-  bool carved = img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
+  bool carved = img->d_cost[2] && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums;
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
@@ -409,6 +412,8 @@ extern "C" int32_t jpxlg_image_info(const jpxlg_img* img, uint32_t* bx, uint32_t
 // ----------------------------------------------------------------------------
 // Structure
 // ----------------------------------------------------------------------------
+
+static int check_quantizer(jpxlg_ctx* ctx, const jpxlg_quantizer* q);
 
 static int validate_map(const Geom& g, const uint8_t* bmap, std::string* why) {
   std::vector<uint8_t> covered((size_t)g.bw * g.bh, 0);
@@ -492,11 +497,34 @@ extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, cons
 extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t cover_mode,
                                         int32_t enable_cfl) {
   if (!img || !q) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
-  if (cover_mode != JPXLG_COVER_FIXED_DCT8)
-    return fail(img->ctx, JPXLG_E_UNSUPPORTED, "hierarchical cover is not available in this build");
+  if (cover_mode != JPXLG_COVER_FIXED_DCT8 && cover_mode != JPXLG_COVER_HIERARCHICAL)
+    return fail(ctx, JPXLG_E_UNSUPPORTED, "unknown cover mode");
+  int rc = check_quantizer(ctx, q);
+  if (rc) return rc;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
   (void)enable_cfl;
   std::vector<uint8_t> bmap((size_t)g.bw * g.bh, 8);
+  if (cover_mode == JPXLG_COVER_HIERARCHICAL) {
+    hipStream_t st = ctx->stream;
+    ProbeParams pp;
+    build_probe_params(*q, img->last_sharpness, &pp);
+    HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+    const uint32_t zero = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
+    {
+      Prof p(ctx, "cover");
+      launch_cover(st, g, ctx->d_tab, img->d_pp, img->xyb, img->d_cost[0], img->d_cost[1],
+                   img->d_cost[2], img->d_bmap, img->d_err);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    uint32_t err = 0;
+    HIP_TRY(ctx, hipMemcpyAsync(bmap.data(), img->d_bmap, bmap.size(), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
+  }
   return jpxlg_set_structure(img, bmap.data(), nullptr, nullptr, nullptr);
 }
 
@@ -669,7 +697,8 @@ extern "C" int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* 
                                        int8_t* cfl_x, int8_t* cfl_b, int32_t* lf_factors) {
   if (!img) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = img->ctx;
-  if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  if (!img->has_structure) return fail(ctx, JPXLG_E_INVALID, "no plan structure yet");
+  if ((quant || lf) && !img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
   const Geom& g = img->g;
   const size_t cpx = (size_t)g.cstride * g.crows, bpx = (size_t)g.bw * g.bh;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -744,14 +773,17 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   memset(out, 0, sizeof(*out));
   QualityPolicy pol = quality_policy(req->effort);
-  if (!img->has_structure) {
-    jpxlg_quantizer q0 = pol.quantizer_at(Rung::for_global_scale(32768));
-    int rc = jpxlg_plan_structure(img, &q0, pol.cover_mode, 1);
-    if (rc) return rc;
-  }
+  bool anchored = false;
   ProbeFn probe = [&](uint32_t rung, double* score, double* ms) -> int {
     auto t0 = std::chrono::steady_clock::now();
     jpxlg_quantizer q = pol.quantizer_at(rung);
+    // The first probe builds the structure (cover + CfL) at its own
+    // quantizer; later probes reuse it (quality.rs:1153-1190).
+    if (!anchored) {
+      int rc = jpxlg_plan_structure(img, &q, pol.cover_mode, 1);
+      if (rc) return rc;
+      anchored = true;
+    }
     jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
     int rc = jpxlg_probe(img, &q, &r, score);
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

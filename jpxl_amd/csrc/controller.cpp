@@ -169,7 +169,7 @@ QualityPolicy quality_policy(int32_t effort) {
   } else {
     p.x_qm = 3; p.b_qm = 3;   // request.rs:968-971
     p.truncate = 1;           // TrailingTruncation
-    p.cover_mode = JPXLG_COVER_FIXED_DCT8;  // hierarchical cover: see plan_structure
+    p.cover_mode = JPXLG_COVER_HIERARCHICAL;
     p.pixel_probes = 5;
     p.reserve = 0.03;
   }

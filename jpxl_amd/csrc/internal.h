@@ -34,7 +34,7 @@ struct ConstTables {
   float donor[kCellTotal];        // quantizer-donor frequency weights
   uint16_t order[kCellTotal];     // natural coefficient order (position -> cell)
   float scale_f[3][4];            // ScaleF(c, n) for n = 1, 2, 4 (index by size), c < n
-  float fir[5];                   // metric blur taps h[0..4] (symmetric)
+  double fir[5];                  // metric blur taps h[0..4] (symmetric), f64
 };
 
 // Per-probe quantizer parameters (device memory, written before each probe).
@@ -96,6 +96,12 @@ void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, fl
 int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
                      float* const mu1[3], float* const s11[3], float* const x2[3],
                      double* partials);
+// Hierarchical cover (kernels_plan.hip): candidate costs for every square
+// (f64, indexed by the candidate's top-left block) and the per-region
+// quadtree decision written into the block map.
+void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                  float* const xyb[3], double* cost8, double* cost16, double* cost32, uint8_t* bmap,
+                  uint32_t* err);
 void launch_reduce_partials(hipStream_t s, int tiles, const double* partials, double* sums18);
 
 }  // namespace jpxlg

@@ -150,12 +150,14 @@ __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __r
 constexpr int kTW = 64, kTH = 16, kHalo = 4;
 constexpr int kLW = kTW + 2 * kHalo, kLH = kTH + 2 * kHalo;
 
-__device__ __forceinline__ float fir9(const float* v, int stride, const float* hf) {
+// Accumulated in f64 with one rounding to f32 per pass, as the reference's
+// f64-state recursion rounds each pass's output once.
+__device__ __forceinline__ float fir9(const float* v, int stride, const double* hf) {
   // v points at the centre sample.
-  float acc = hf[0] * v[0];
+  double acc = hf[0] * (double)v[0];
 #pragma unroll
-  for (int t = 1; t <= 4; t++) acc = fmaf(hf[t], v[-t * stride] + v[t * stride], acc);
-  return acc;
+  for (int t = 1; t <= 4; t++) acc = fma(hf[t], (double)v[-t * stride] + (double)v[t * stride], acc);
+  return (float)acc;
 }
 
 __global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstTables* __restrict__ tab,
@@ -165,7 +167,7 @@ __global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstT
   __shared__ float tin[kLH][kLW];
   __shared__ float hb[2][kLH][kTW];
   const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
-  float hf[5];
+  double hf[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
   for (int e = threadIdx.x; e < kLH * kLW; e += 256) {
@@ -176,14 +178,15 @@ __global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstT
   for (int e = threadIdx.x; e < kLH * kTW; e += 256) {
     int ly = e / kTW, lx = e % kTW;
     const float* c = &tin[ly][lx + kHalo];
-    float a = hf[0] * c[0], b = hf[0] * (c[0] * c[0]);
+    // the squared input is rounded to f32 first (blur.rs:97-113)
+    double a = hf[0] * (double)c[0], b = hf[0] * (double)(c[0] * c[0]);
 #pragma unroll
     for (int t = 1; t <= 4; t++) {
-      a = fmaf(hf[t], c[-t] + c[t], a);
-      b = fmaf(hf[t], c[-t] * c[-t] + c[t] * c[t], b);
+      a = fma(hf[t], (double)c[-t] + (double)c[t], a);
+      b = fma(hf[t], (double)(c[-t] * c[-t]) + (double)(c[t] * c[t]), b);
     }
-    hb[0][ly][lx] = a;
-    hb[1][ly][lx] = b;
+    hb[0][ly][lx] = (float)a;
+    hb[1][ly][lx] = (float)b;
   }
   __syncthreads();
   for (int e = threadIdx.x; e < kTH * kTW; e += 256) {
@@ -214,7 +217,7 @@ __global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTabl
   __shared__ double red[4][6];
   const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
   const int tile = blockIdx.y * gridDim.x + blockIdx.x;
-  float hf[5];
+  double hf[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
   const float* X1[3] = {a0, a1, a2};
@@ -235,16 +238,17 @@ __global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTabl
       int ly = e / kTW, lx = e % kTW;
       const float* p1 = &t1[ly][lx + kHalo];
       const float* p2 = &t2[ly][lx + kHalo];
-      float m = hf[0] * p2[0], s22 = hf[0] * (p2[0] * p2[0]), s12 = hf[0] * (p1[0] * p2[0]);
+      double m = hf[0] * (double)p2[0], s22 = hf[0] * (double)(p2[0] * p2[0]),
+             s12 = hf[0] * (double)(p1[0] * p2[0]);
 #pragma unroll
       for (int t = 1; t <= 4; t++) {
-        m = fmaf(hf[t], p2[-t] + p2[t], m);
-        s22 = fmaf(hf[t], p2[-t] * p2[-t] + p2[t] * p2[t], s22);
-        s12 = fmaf(hf[t], p1[-t] * p2[-t] + p1[t] * p2[t], s12);
+        m = fma(hf[t], (double)p2[-t] + (double)p2[t], m);
+        s22 = fma(hf[t], (double)(p2[-t] * p2[-t]) + (double)(p2[t] * p2[t]), s22);
+        s12 = fma(hf[t], (double)(p1[-t] * p2[-t]) + (double)(p1[t] * p2[t]), s12);
       }
-      hb[0][ly][lx] = m;
-      hb[1][ly][lx] = s22;
-      hb[2][ly][lx] = s12;
+      hb[0][ly][lx] = (float)m;
+      hb[1][ly][lx] = (float)s22;
+      hb[2][ly][lx] = (float)s12;
     }
     __syncthreads();
     double acc[6] = {0, 0, 0, 0, 0, 0};

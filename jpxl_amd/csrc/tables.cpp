@@ -218,7 +218,7 @@ void build_const_tables(ConstTables* t) {
     for (int k = 0; k < 3; k++) { prev2[k] = prev[k]; prev[k] = o[k]; }
     if (nn >= 0) resp[nn] = o[0] + o[1] + o[2];
   }
-  for (int k = 0; k <= 4; k++) t->fir[k] = (float)resp[centre + k];
+  for (int k = 0; k <= 4; k++) t->fir[k] = resp[centre + k];
 }
 
 void srgb8_lut(float* lut) {  // source.rs:253-254, evaluator.rs:93-94

@@ -1348,3 +1348,117 @@ EXPORT int32_t orc_hf_choose(float target, float step, int channel) {
     return ok ? v : INT32_MIN;
 }
 EXPORT float orc_hf_recon(int32_t v, float step, int channel) { return bias_adjust(v, QUANT_BIAS[channel]) * step; }
+
+/* ------------------------------------------------------------------------ */
+/* Hierarchical cover (jpxl-encode-policy/src/lib.rs:4685-5302)              */
+/* ------------------------------------------------------------------------ */
+
+/* lib.rs:3794 residual_bits. */
+static uint64_t residual_bits(int32_t q) {
+    if (q == 0) return 0;
+    return (uint64_t)(32 - __builtin_clz((uint32_t)abs(q))) + 1;
+}
+
+/* request.rs:585-636 CoverRateModel::Calibrated. */
+static double cover_rate_scale(int side) { return side == 16 ? 1.63 : (side == 32 ? 1.88 : 1.56); }
+static double cover_fixed_bits(int side) { return side == 16 ? 8.4 + 4.0 : (side == 32 ? 6.5 + 4.0 : 5.5); }
+
+typedef struct {
+    int w, h, bw, bh;
+    const float *xyb[3];
+    hf_quantizer *hq[3];
+    double lambda[3];
+    float donor[3][1024];
+} cover_ctx;
+
+/* lib.rs:4909-5062 block_cost_bounded without the cutoff (the cutoff only
+ * prunes candidates whose full cost could not win), with lib.rs:4685-4890
+ * score_channel_lanes for Y, then X and B against the reconstructed dY. */
+static double block_cost(cover_ctx *cc, int side, int bx, int by) {
+    int si = side == 8 ? 0 : (side == 16 ? 1 : 2), n = side / 8, cells = side * side;
+    const hf_quantizer *q = cc->hq[si];
+    float smp[1024], co[3][1024], dyh[1024];
+    for (int c = 0; c < 3; c++) {
+        gather_square(cc->xyb[c], cc->w, cc->h, 8 * bx, 8 * by, side, smp);
+        orc_dct2d_fwd(side, side, smp, co[c]);
+    }
+    uint64_t bits = 0;
+    double sse = 0.0;
+    double tsd = (double)(side * side) * 1.0;
+    int order[3] = {1, 0, 2};
+    int ok = 1;
+    for (int oi = 0; oi < 3; oi++) {
+        int c = order[oi];
+        for (int row = 0; row < side; row++) {
+            int col0 = row < n ? n : 0;
+            for (int col = col0; col < side; col++) {
+                int cell = row * side + col;
+                float target;
+                if (c == 1) target = co[1][cell];
+                else if (c == 0) target = co[0][cell] - 0.0f * dyh[cell];
+                else target = co[2][cell] - 1.0f * dyh[cell];
+                int32_t v = hf_choose(q, target, c, cell, &ok);
+                float recon = hf_recon(q, v, c, cell);
+                if (c == 1) dyh[cell] = recon;
+                bits += residual_bits(v);
+                double d = (double)(recon - target);
+                sse += cc->lambda[c] * tsd * (double)cc->donor[si][cell] * (d * d);
+            }
+        }
+    }
+    return (double)bits * cover_rate_scale(side) + sse;
+}
+
+/* lib.rs:5115-5223 tile_region_with. */
+static double tile_region(cover_ctx *cc, int bx, int by, int size, uint8_t *bmap, int write) {
+    if (bx >= cc->bw || by >= cc->bh) return 0.0;
+    if (size == 1) {
+        double cost = block_cost(cc, 8, bx, by) + cover_fixed_bits(8) + 0.0;
+        if (write) bmap[by * cc->bw + bx] = 8;
+        return cost;
+    }
+    int half = size / 2;
+    double split = 0.0;
+    int qs[4][2] = {{bx, by}, {bx + half, by}, {bx, by + half}, {bx + half, by + half}};
+    for (int i = 0; i < 4; i++) split += tile_region(cc, qs[i][0], qs[i][1], half, bmap, 0);
+    int fits = bx + size <= cc->bw && by + size <= cc->bh;
+    int side = size * 8;
+    int single = 0;
+    double best = split;
+    if (fits) {
+        double fixed = cover_fixed_bits(side) + 0.0;
+        if (!(fixed >= split)) {
+            double c = block_cost(cc, side, bx, by) + fixed;
+            if (c < split) { single = 1; best = c; }
+        }
+    }
+    if (write) {
+        if (single) {
+            for (int y = 0; y < size; y++)
+                for (int x = 0; x < size; x++)
+                    if (by + y < cc->bh && bx + x < cc->bw) bmap[(by + y) * cc->bw + bx + x] = 0;
+            bmap[by * cc->bw + bx] = (uint8_t)side;
+        } else {
+            for (int i = 0; i < 4; i++) tile_region(cc, qs[i][0], qs[i][1], half, bmap, 1);
+        }
+    }
+    return best;
+}
+
+/* lib.rs:5260-5302 select_blocks over the frame (one LF group per 2048 px;
+ * regions never cross a group, so the frame-level walk is the same). */
+EXPORT void orc_cover(int w, int h, const float *xyb[3], const orc_quant_params *p, uint8_t *bmap) {
+    cover_ctx cc;
+    cc.w = w; cc.h = h; cc.bw = (w + 7) / 8; cc.bh = (h + 7) / 8;
+    for (int c = 0; c < 3; c++) cc.xyb[c] = xyb[c];
+    for (int i = 0; i < 3; i++) {
+        cc.hq[i] = malloc(sizeof(hf_quantizer));
+        build_hf_quantizer(cc.hq[i], 8 << i, p);
+        orc_donor_weights(8 << i, cc.donor[i]);
+    }
+    for (int c = 0; c < 3; c++) cc.lambda[c] = lambda_for(p, c);
+    memset(bmap, 0, (size_t)cc.bw * cc.bh);
+    for (int sby = 0; sby < cc.bh; sby += 4)
+        for (int sbx = 0; sbx < cc.bw; sbx += 4) tile_region(&cc, sbx, sby, 4, bmap, 1);
+    for (int i = 0; i < 3; i++) free(cc.hq[i]);
+}

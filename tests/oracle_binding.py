@@ -187,6 +187,15 @@ def render(w, h, bmap, quant, lf, r: RenderParams, cfl_x=None, cfl_b=None, lf_fa
     return lin, xyb
 
 
+def cover(xyb: np.ndarray, q: QuantParams) -> np.ndarray:
+    """The hierarchical cover's block map (8/16/32 at varblock origins)."""
+    _, h, w = xyb.shape
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    bm = np.zeros(((h + 7) // 8, (w + 7) // 8), np.uint8)
+    lib().orc_cover(w, h, planes_ptr([xyb[0], xyb[1], xyb[2]]), ctypes.byref(q), P(bm))
+    return bm
+
+
 def epf_step(step: int, planes: np.ndarray, sigma: np.ndarray) -> np.ndarray:
     planes = np.ascontiguousarray(planes, np.float32)
     sigma = np.ascontiguousarray(sigma, np.float32)

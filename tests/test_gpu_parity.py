@@ -16,6 +16,14 @@ pytestmark = pytest.mark.gpu
 SCORE_TOL = 0.02
 
 
+def _terms_close(terms, oterms):
+    # The FIR blur differs from the f64 recursion by ~1e-7 per sample; the
+    # 4-norms of near-zero artifact/detail-lost maps amplify that, so terms
+    # are compared with an absolute bound (the score bound is the contract).
+    assert len(terms) == len(oterms)
+    assert np.abs(terms - oterms).max() < 2e-5
+
+
 @pytest.fixture(scope="module")
 def ctx():
     import jpxl_amd
@@ -135,6 +143,23 @@ def test_quantize_bit_exact(ctx, case, mixed):
     assert np.array_equal(plan.block_map, bmap)
 
 
+@pytest.mark.parametrize("wh,gs", [((256, 192), 1200), ((131, 77), 400), ((200, 136), 5000)])
+def test_cover_block_map_matches_the_oracle(ctx, wh, gs):
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    w, h = wh
+    rgb = _rgb(w, h, 17)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    q = _quantizer(gs=gs)
+    img.plan_structure(q, _lib.COVER_HIERARCHICAL, False)
+    plan_map = img.plan_download_map()
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    want = orc.cover(xyb, _qp(q))
+    assert np.array_equal(plan_map, want)
+    assert set(np.unique(want)) >= {0, 8}  # the synthetic image merges some blocks
+
+
 @pytest.mark.parametrize("epf,gab", [(0, 0), (1, 0), (2, 0), (3, 0), (1, 1), (3, 1)])
 @pytest.mark.parametrize("mixed", [False, True])
 def test_render_bit_exact(ctx, epf, gab, mixed):
@@ -158,7 +183,7 @@ def test_render_bit_exact(ctx, epf, gab, mixed):
     assert np.array_equal(got_lin.view(np.uint32), olin.view(np.uint32))
     oscore, oterms, _ = orc.ssimulacra2(lin, olin)
     assert abs(score - oscore) < SCORE_TOL
-    assert np.allclose(terms, oterms, rtol=2e-3, atol=2e-6)
+    _terms_close(terms, oterms)
 
 
 @pytest.mark.parametrize("step", [0, 1, 2])
@@ -188,7 +213,7 @@ def test_score_pair_identity_and_noise(ctx):
         s, terms = ctx.score_pair(ref, cand)
         os_, oterms, _ = orc.ssimulacra2(ref, cand)
         assert abs(s - os_) < SCORE_TOL
-        assert np.allclose(terms, oterms, rtol=2e-3, atol=2e-6)
+        _terms_close(terms, oterms)
 
 
 @pytest.mark.parametrize("wh", [(8, 8), (9, 17), (256, 192)])
