@@ -94,6 +94,22 @@ def cpu_baseline(rgb: np.ndarray, probes: int, budget_s: float = 25.0) -> dict:
     }
 
 
+def copy_calibration(mib: int = 512, reps: int = 3) -> int:
+    """A plain device-to-device copy of a known size (torch's elementwise copy
+    kernel), run under the same HBM counters as the bench so the counters'
+    bytes-per-KiB factor can be read off it."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.arange(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(reps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4, "reps": reps}))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,7 +117,11 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--effort", default="balanced")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain copy of known size (for HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        return copy_calibration()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

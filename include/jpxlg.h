@@ -86,6 +86,8 @@ typedef struct {
 
 /* ---- context ------------------------------------------------------------ */
 int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out);
+/* Destroy every image created on a context before the context itself: an
+ * image keeps a pointer to its context's stream and tables. */
 void jpxlg_ctx_destroy(jpxlg_ctx* ctx);
 const char* jpxlg_last_error(const jpxlg_ctx* ctx);
 /* Bytes of device memory an image of this size keeps resident. */

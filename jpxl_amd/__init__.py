@@ -16,6 +16,7 @@ Every call runs on the GPU through ``libjpxlg.so``; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import weakref
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -51,6 +52,9 @@ class Context:
         if rc != 0:
             raise JpxlgError(rc, "context creation failed (no usable GPU?)")
         self.handle = h
+        # Images hold device memory and the context's stream; the library
+        # requires every image to be destroyed before its context.
+        self._images = weakref.WeakSet()
 
     def check(self, rc: int) -> None:
         if rc != 0:
@@ -59,6 +63,8 @@ class Context:
 
     def close(self) -> None:
         if self.handle:
+            for img in list(self._images):
+                img.close()
             self.lib.jpxlg_ctx_destroy(self.handle)
             self.handle = None
 
@@ -179,6 +185,7 @@ class Image:
 
     def __init__(self, ctx: Context, rgb: np.ndarray, bits: int | None = None, device_ptr: int | None = None,
                  shape: tuple | None = None):
+        self.handle = None
         self.ctx = ctx
         lib = ctx.lib
         h = ctypes.c_void_p()
@@ -199,6 +206,7 @@ class Image:
             rc = lib.jpxlg_image_upload_srgb(ctx.handle, width, height, bits, _ptr(rgb), ctypes.byref(h))
         ctx.check(rc)
         self.handle = h
+        ctx._images.add(self)
         self.width, self.height = width, height
         bx, by, gray, scales = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_uint32()
         ctx.check(lib.jpxlg_image_info(h, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(gray),
@@ -208,7 +216,8 @@ class Image:
 
     def close(self):
         if self.handle:
-            self.ctx.lib.jpxlg_image_destroy(self.handle)
+            if self.ctx.handle:
+                self.ctx.lib.jpxlg_image_destroy(self.handle)
             self.handle = None
 
     def __del__(self):

@@ -136,6 +136,7 @@ size_t image_bytes(int w, int h) {
   b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
   b += 3 * bw * bh * 4;              // work lists
   b += 3 * bw * bh * 8;              // cover candidate costs (f64)
+  b += cpx * 18 + bw * bh * (16 + 3 * 16) + 2 * ((bw + 7) / 8 + 1) * ((bh + 7) / 8 + 1) * 4;  // CfL samples
   b += 6 * px * 4;                   // render planes + ping-pong
   b += 3 * pyr * 4;                  // candidate XYB' pyramid
   b += 3 * pyr1 * 4;                 // linear RGB pyramid (scales >= 1)
@@ -182,6 +183,11 @@ struct jpxlg_img {
   double* d_partials = nullptr;
   double* d_sums = nullptr;
   double* d_cost[3] = {nullptr, nullptr, nullptr};
+  float4* d_lf_s = nullptr;
+  float4* d_hf_s = nullptr;
+  uint16_t* d_hf_cell = nullptr;
+  uint4* d_cfl_items[3] = {nullptr, nullptr, nullptr};
+  uint32_t* d_tile_base = nullptr;
   // host mirrors of the structure
   std::vector<uint8_t> bmap;
   std::vector<int8_t> cfl_x, cfl_b;
@@ -332,6 +338,11 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   }
   img->sigma = (float*)A.take(bpx * 4);
   for (int i = 0; i < 3; i++) img->d_cost[i] = (double*)A.take(bpx * 8);
+  img->d_lf_s = (float4*)A.take(bpx * 16);
+  img->d_hf_s = (float4*)A.take(cpx * 16);
+  img->d_hf_cell = (uint16_t*)A.take(cpx * 2);
+  for (int i = 0; i < 3; i++) img->d_cfl_items[i] = (uint4*)A.take(bpx * 16);
+  img->d_tile_base = (uint32_t*)A.take((size_t)(tiles_cfl + 1) * 4);
   img->d_bmap = (uint8_t*)A.take(bpx);
   img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
   img->d_cfl_b = (int8_t*)A.take(tiles_cfl);
@@ -342,7 +353,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   size_t ptiles = ((g.w + 63) / 64 + 1) * ((g.h + 15) / 16 + 1);
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
-  bool carved = img->d_cost[2] && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
+  bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
+                img->d_cfl_items[2] && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums;
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
@@ -494,6 +506,83 @@ extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, cons
   return upload_structure(img);
 }
 
+// The chroma-from-luma estimate of build_pixel_plan (lib.rs:3437-3763) on
+// the structure just uploaded: the host lays out the sample slots in the
+// reference's order, the device gathers, regresses and prices.
+static int estimate_cfl(jpxlg_img* img, const jpxlg_quantizer* q) {
+  jpxlg_ctx* ctx = img->ctx;
+  const Geom& g = img->g;
+  hipStream_t st = ctx->stream;
+  std::vector<uint4> items[3];
+  const int tiles = g.tw * g.th;
+  std::vector<uint32_t> tile_count(tiles, 0), tile_base(tiles + 1, 0);
+  // Pass 1: per-tile HF sample counts.
+  for (int by = 0; by < g.bh; by++)
+    for (int bx = 0; bx < g.bw; bx++) {
+      int side = img->bmap[by * g.bw + bx];
+      if (!side) continue;
+      int n = side / 8;
+      tile_count[(by / 8) * g.tw + bx / 8] += (uint32_t)(side * side - n * n);
+    }
+  for (int t = 0; t < tiles; t++) tile_base[t + 1] = tile_base[t] + tile_count[t];
+  std::vector<uint32_t> tile_fill(tiles, 0);
+  uint32_t lf_off = 0;
+  // Pass 2: LF groups of 256 blocks, tile rows, varblocks in raster order.
+  for (int gy = 0; gy < (g.bh + 255) / 256; gy++)
+    for (int gx = 0; gx < (g.bw + 255) / 256; gx++) {
+      const int gbx0 = gx * 256, gby0 = gy * 256;
+      const int gbx1 = std::min(gbx0 + 256, g.bw), gby1 = std::min(gby0 + 256, g.bh);
+      for (int by = gby0; by < gby1; by++)
+        for (int bx = gbx0; bx < gbx1; bx++) {
+          int side = img->bmap[by * g.bw + bx];
+          if (!side) continue;
+          int n = side / 8, t = (by / 8) * g.tw + bx / 8;
+          uint4 it;
+          it.x = ((uint32_t)by << 16) | (uint32_t)bx;
+          it.y = lf_off;
+          it.z = tile_base[t] + tile_fill[t];
+          it.w = 0;
+          items[size_index(side)].push_back(it);
+          lf_off += (uint32_t)(n * n);
+          tile_fill[t] += (uint32_t)(side * side - n * n);
+        }
+    }
+  ProbeParams pp;
+  build_probe_params(*q, img->last_sharpness, &pp);
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+  const uint32_t zero = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
+  int counts[3];
+  const uint4* ditems[3];
+  for (int i = 0; i < 3; i++) {
+    counts[i] = (int)items[i].size();
+    ditems[i] = img->d_cfl_items[i];
+    if (counts[i])
+      HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_items[i], items[i].data(), items[i].size() * sizeof(uint4),
+                                  hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_tile_base, tile_base.data(), tile_base.size() * 4, hipMemcpyHostToDevice, st));
+  {
+    Prof p(ctx, "cfl");
+    launch_cfl(st, g, ctx->d_tab, img->d_pp, ditems, counts, img->d_tile_base, lf_off, img->coeff,
+               img->d_lf_s, img->d_hf_s, img->d_hf_cell, img->d_cfl_x, img->d_cfl_b, img->d_cflp, img->d_err);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  CflParams cp;
+  uint32_t err = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_x.data(), img->d_cfl_x, tiles, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_b.data(), img->d_cfl_b, tiles, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(&cp, img->d_cflp, sizeof(cp), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
+  img->lf_factors[0] = cp.x_factor_lf;
+  img->lf_factors[1] = cp.b_factor_lf;
+  img->has_quant = false;
+  img->has_render = false;
+  return JPXLG_OK;
+}
+
 extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t cover_mode,
                                         int32_t enable_cfl) {
   if (!img || !q) return JPXLG_E_INVALID;
@@ -525,7 +614,9 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
     HIP_TRY(ctx, hipStreamSynchronize(st));
     if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
   }
-  return jpxlg_set_structure(img, bmap.data(), nullptr, nullptr, nullptr);
+  rc = jpxlg_set_structure(img, bmap.data(), nullptr, nullptr, nullptr);
+  if (rc || !enable_cfl || img->gray) return rc;
+  return estimate_cfl(img, q);
 }
 
 // ----------------------------------------------------------------------------

@@ -102,6 +102,13 @@ int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float*
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
                   float* const xyb[3], double* cost8, double* cost16, double* cost32, uint8_t* bmap,
                   uint32_t* err);
+// Chroma-from-luma estimate: per-size varblock items (x = (by << 16) | bx,
+// y = LF sample slot, z = HF sample slot), per-tile HF sample ranges, and the
+// outputs (tile factors, LF factors) on the device.
+void launch_cfl(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                const uint4* const items[3], const int counts[3], const uint32_t* tile_base,
+                uint32_t lf_count, float* const coeff[3], float4* lf_s, float4* hf_s, uint16_t* hf_cell,
+                int8_t* cfl_x, int8_t* cfl_b, CflParams* cflp, uint32_t* err);
 void launch_reduce_partials(hipStream_t s, int tiles, const double* partials, double* sums18);
 
 }  // namespace jpxlg

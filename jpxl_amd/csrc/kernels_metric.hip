@@ -21,12 +21,17 @@ inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b);
 __device__ __forceinline__ float depth_value(float x, const float* thr, const float* lut,
                                              int levels) {
   if (!isfinite(x)) return lut[0];
-  int lo = 0, hi = levels - 1;
-  while (lo < hi) {
-    int mid = (lo + hi) >> 1;
-    if (thr[mid] <= x) lo = mid + 1;
-    else hi = mid;
-  }
+  // A first guess from the device's own sRGB encode (within a level of the
+  // truth), then corrected against the exact thresholds, so the result is
+  // the count of thresholds <= x whatever the guess.
+  const float max = (float)(levels - 1);
+  float e;
+  if (x <= 0.0031308f) e = 12.92f * x;
+  else e = 1.055f * __powf(x, 1.0f / 2.4f) - 0.055f;
+  float gl = rintf(e * max);
+  int lo = gl <= 0.0f ? 0 : (gl >= max ? levels - 1 : (int)gl);
+  while (lo > 0 && thr[lo - 1] > x) lo--;
+  while (lo < levels - 1 && thr[lo] <= x) lo++;
   return lut[lo];
 }
 

@@ -60,7 +60,9 @@ __global__ __launch_bounds__(256) void k_xyb_fwd(int n, const void* __restrict__
     Y[i] = 0.5f * (lg + mg);
     B[i] = sg;
   }
-  if (__any(!g) && (threadIdx.x & 63) == 0) atomicAnd(gray, 0);
+  // One atomic per wave at most, and none once another wave has cleared it.
+  if (__any(!g) && (threadIdx.x & 63) == 0 && __atomic_load_n(gray, __ATOMIC_RELAXED) != 0)
+    atomicAnd(gray, 0);
 }
 
 // ----------------------------------------------------------------------------

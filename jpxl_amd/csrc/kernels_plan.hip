@@ -10,6 +10,8 @@
 // row-major over the non-LLF cells) so the f64 value is the reference's own.
 // One thread per 32x32 region then walks the quadtree: a merge wins only when
 // strictly cheaper than its split.
+#include <climits>
+
 #include "internal.h"
 
 namespace jpxlg {
@@ -227,6 +229,288 @@ __global__ __launch_bounds__(256) void k_cover_decide(Geom g, const double* __re
       if (bx0 + x < bw && by0 + y < bh) bmap[(size_t)(by0 + y) * bw + bx0 + x] = dec[y][x];
 }
 
+// ----------------------------------------------------------------------------
+// Chroma-from-luma estimate (jpxl-encode-policy/src/lib.rs:3437-4014,
+// quantize.rs:1599-1717).  Samples are gathered per varblock into slots the
+// host laid out in the reference's order (LF group, tile row, varblock raster,
+// cell); the f64 regressions then run sequentially in that order (one thread
+// per tile, one for the frame's LF), so every sum is the reference's own, and
+// the integer residual-bit prices of the few candidate factors are
+// order-free reductions.
+// ----------------------------------------------------------------------------
+
+__device__ __forceinline__ float cfl_k(float base, int factor) { return base + (float)factor / 84.0f; }
+
+__device__ int best_factor_dev(double s_yy, double s_yc, double s_cc, float base) {
+  if (s_yy <= 0.0) return 0;
+  double k_ls = s_yc / s_yy;
+  double star = (k_ls - (double)base) * 84.0;
+  double r = round(star);
+  long long centre = r >= 9.2e18 ? LLONG_MAX : (r <= -9.2e18 ? LLONG_MIN : (long long)r);
+  int best = 0;
+  double best_e = 0.0;
+  bool have = false;
+  for (int i = 0; i < 6; i++) {
+    int cand;
+    if (i < 5) {
+      long long c = centre + (i - 2);
+      c = c < -128 ? -128 : (c > 127 ? 127 : c);
+      cand = (int)c;
+    } else {
+      cand = 0;
+    }
+    double k = (double)cfl_k(base, cand);
+    double e = s_cc - 2.0 * k * s_yc + k * k * s_yy;
+    if (!have || e < best_e || (e == best_e && abs(cand) < abs(best))) {
+      have = true;
+      best = cand;
+      best_e = e;
+    }
+  }
+  return best;
+}
+
+__device__ __forceinline__ uint32_t factor_bits(int f) {
+  return f == 0 ? 0u : (uint32_t)(32 - __clz((unsigned)abs(f))) + 2u;
+}
+
+// seed-1..seed+1 clamped to [-128, 127], deduplicated, plus 0 (lib.rs:3771).
+__device__ int factor_candidates(int seed, int* out) {
+  int n = 0;
+  for (int d = -1; d <= 1; d++) {
+    int c = seed + d;
+    c = c < -128 ? -128 : (c > 127 ? 127 : c);
+    bool dup = false;
+    for (int i = 0; i < n; i++) dup |= out[i] == c;
+    if (!dup) out[n++] = c;
+  }
+  bool has0 = false;
+  for (int i = 0; i < n; i++) has0 |= out[i] == 0;
+  if (!has0) out[n++] = 0;
+  return n;
+}
+
+template <int N>
+__global__ __launch_bounds__(64) void k_cfl_collect(Geom g, const uint4* __restrict__ items, int count,
+                                                    const ConstTables* __restrict__ tab,
+                                                    const ProbeParams* __restrict__ pp,
+                                                    const float* __restrict__ CX, const float* __restrict__ CY,
+                                                    const float* __restrict__ CB, float4* __restrict__ lf_s,
+                                                    float4* __restrict__ hf_s, uint16_t* __restrict__ hf_cell,
+                                                    uint32_t* __restrict__ err) {
+  constexpr int NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  const int i = blockIdx.x;
+  if (i >= count) return;
+  const uint4 it = items[i];
+  const int bx = (int)(it.x & 0xffffu), by = (int)(it.x >> 16);
+  const uint32_t lf_off = it.y, hf_off = it.z;
+  int bad = 0;
+  const DctTables& t = tab->dct;
+  if (threadIdx.x == 0) {
+    const float* sf = tab->scale_f[si];
+    float lf[3][n * n];
+    const float* cp[3] = {CX, CY, CB};
+    for (int c = 0; c < 3; c++) {
+#pragma unroll
+      for (int y = 0; y < n; y++)
+#pragma unroll
+        for (int x = 0; x < n; x++) {
+          float v = cp[c][(size_t)(8 * by + y) * g.cstride + 8 * bx + x];
+          lf[c][y * n + x] = n == 1 ? v : v / (sf[y] * sf[x]);
+        }
+      if constexpr (n == 2) {
+        float w2[4] = {lf[c][0], lf[c][1], lf[c][2], lf[c][3]};
+        (void)w2;
+      }
+    }
+    // small inverse DCTs (forward.rs:779-830)
+    for (int c = 0; c < 3; c++) {
+      if constexpr (n > 1) {
+        const float f = inv_factor<n>();
+        float v[n], s[n * n];
+        for (int k = 0; k < n * n; k++) s[k] = lf[c][k];
+        for (int r = 0; r < n; r++) {
+          for (int k = 0; k < n; k++) v[k] = s[k * n + r];
+          dct_iii<n>(v, t);
+          for (int k = 0; k < n; k++) s[k * n + r] = v[k] * f;
+        }
+        for (int r = 0; r < n; r++)
+          for (int c2 = 0; c2 < n; c2++) lf[c][r * n + c2] = s[c2 * n + r];
+        for (int c2 = 0; c2 < n; c2++) {
+          for (int r = 0; r < n; r++) v[r] = lf[c][r * n + c2];
+          dct_iii<n>(v, t);
+          for (int r = 0; r < n; r++) lf[c][r * n + c2] = v[r] * f;
+        }
+      }
+    }
+    const float mdc1 = pp->mdc[1];
+    for (int k = 0; k < n * n; k++) {
+      float y = lf[1][k];
+      int32_t q = clamp_round(y * 1.0f / mdc1, &bad);
+      float dy = mdc1 * (float)q / 1.0f;
+      lf_s[lf_off + k] = make_float4(lf[0][k], y, lf[2][k], dy);
+    }
+  }
+  const float sc = pp->sc[1], bias = kBias[1];
+  for (int k = threadIdx.x; k < NN; k += 64) {
+    const int row = k / N, col = k % N;
+    if (row < n && col < n) continue;
+    const int slot = row < n ? row * (N - n) + (col - n) : n * (N - n) + (row - n) * N + col;
+    const size_t o = (size_t)(8 * by + row) * g.cstride + 8 * bx + col;
+    const float y = CY[o];
+    const float step = sc * tab->matrix[1][cb + k];
+    const int32_t q = choose_nearest(y, step, bias, &bad);
+    const float dy = bias_adjust(q, bias) * step;
+    constexpr int fold = N / 8;
+    const int cell8 = (k / N / fold) * 8 + (k % N / fold);
+    hf_s[hf_off + slot] = make_float4(CX[o], y, CB[o], dy);
+    hf_cell[hf_off + slot] = (uint16_t)cell8;
+  }
+  if (bad) atomicOr(err, kErrQuantRange);
+}
+
+// One workgroup per 64x64 tile: sequential regression, then the residual
+// prices of the candidate factors against the DCT8 baseline quantizer.
+__global__ __launch_bounds__(256) void k_cfl_tiles(const uint32_t* __restrict__ tile_base, int tiles,
+                                                   const ConstTables* __restrict__ tab,
+                                                   const ProbeParams* __restrict__ pp,
+                                                   const float4* __restrict__ hf_s,
+                                                   const uint16_t* __restrict__ hf_cell,
+                                                   int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b) {
+  __shared__ int seeds[2];
+  __shared__ int cands[2][4];
+  __shared__ int ncand[2];
+  __shared__ unsigned long long cost[2][4];
+  const int tile = blockIdx.x;
+  if (tile >= tiles) return;
+  const uint32_t b0 = tile_base[tile], b1 = tile_base[tile + 1];
+  if (threadIdx.x == 0) {
+    double yy = 0.0, ycx = 0.0, ccx = 0.0, ycb = 0.0, ccb = 0.0;
+    for (uint32_t i = b0; i < b1; i++) {
+      const float4 s = hf_s[i];
+      const double y = (double)s.y, x = (double)s.x, b = (double)s.z;
+      yy += y * y;
+      ycx += y * x;
+      ccx += x * x;
+      ycb += y * b;
+      ccb += b * b;
+    }
+    // both channels share s_yy (same sample sequence)
+    seeds[0] = b1 > b0 ? best_factor_dev(yy, ycx, ccx, 0.0f) : 0;
+    seeds[1] = b1 > b0 ? best_factor_dev(yy, ycb, ccb, 1.0f) : 0;
+    for (int ch = 0; ch < 2; ch++) {
+      ncand[ch] = seeds[ch] == 0 ? 0 : factor_candidates(seeds[ch], cands[ch]);
+      for (int i = 0; i < 4; i++) cost[ch][i] = 0;
+    }
+  }
+  __syncthreads();
+  const float sx = pp->sc[0], sb = pp->sc[2];
+  unsigned long long local[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  int dummy = 0;
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) {
+    const float4 s = hf_s[i];
+    const int cell = hf_cell[i];
+    for (int ch = 0; ch < 2; ch++) {
+      const int c = ch == 0 ? 0 : 2;
+      const float base = ch == 0 ? 0.0f : 1.0f;
+      const float src = ch == 0 ? s.x : s.z;
+      const float step = (ch == 0 ? sx : sb) * tab->matrix[c][cell];
+      for (int j = 0; j < ncand[ch]; j++) {
+        const float t = src - cfl_k(base, cands[ch][j]) * s.w;
+        local[ch][j] += residual_bits(choose_nearest(t, step, kBias[c], &dummy));
+      }
+    }
+  }
+  for (int ch = 0; ch < 2; ch++)
+    for (int j = 0; j < 4; j++)
+      if (local[ch][j]) atomicAdd(&cost[ch][j], local[ch][j]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int ch = 0; ch < 2; ch++) {
+      int best = 0;
+      if (ncand[ch] > 0) {
+        unsigned long long best_cost = 0;
+        for (int j = 0; j < ncand[ch]; j++)
+          if (cands[ch][j] == 0) best_cost = cost[ch][j];
+        for (int j = 0; j < ncand[ch]; j++) {
+          const int f = cands[ch][j];
+          if (f == 0) continue;
+          unsigned long long cst = cost[ch][j] + factor_bits(f);
+          if (cst < best_cost || (cst == best_cost && abs(f) < abs(best))) {
+            best_cost = cst;
+            best = f;
+          }
+        }
+      }
+      (ch == 0 ? cfl_x : cfl_b)[tile] = (int8_t)best;
+    }
+  }
+}
+
+// The frame-wide LF factors (lib.rs:3831-3894 refine_lf_factors).
+__global__ __launch_bounds__(256) void k_cfl_lf(uint32_t count, const ProbeParams* __restrict__ pp,
+                                                const float4* __restrict__ lf_s, CflParams* __restrict__ out) {
+  __shared__ int cands[2][4];
+  __shared__ int ncand[2];
+  __shared__ unsigned long long cost[2][4];
+  if (threadIdx.x == 0) {
+    double yy = 0.0, ycx = 0.0, ccx = 0.0, ycb = 0.0, ccb = 0.0;
+    for (uint32_t i = 0; i < count; i++) {
+      const float4 s = lf_s[i];
+      const double y = (double)s.y, x = (double)s.x, b = (double)s.z;
+      yy += y * y;
+      ycx += y * x;
+      ccx += x * x;
+      ycb += y * b;
+      ccb += b * b;
+    }
+    ncand[0] = factor_candidates(best_factor_dev(yy, ycx, ccx, 0.0f), cands[0]);
+    ncand[1] = factor_candidates(best_factor_dev(yy, ycb, ccb, 1.0f), cands[1]);
+    for (int ch = 0; ch < 2; ch++)
+      for (int j = 0; j < 4; j++) cost[ch][j] = 0;
+  }
+  __syncthreads();
+  const float mdc[2] = {pp->mdc[0], pp->mdc[2]};
+  unsigned long long local[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+  int dummy = 0;
+  for (uint32_t i = threadIdx.x; i < count; i += 256) {
+    const float4 s = lf_s[i];
+    for (int ch = 0; ch < 2; ch++) {
+      const float base = ch == 0 ? 0.0f : 1.0f, src = ch == 0 ? s.x : s.z;
+      for (int j = 0; j < ncand[ch]; j++) {
+        const float t = src - cfl_k(base, cands[ch][j]) * s.w;
+        local[ch][j] += residual_bits(clamp_round(t * 1.0f / mdc[ch], &dummy));
+      }
+    }
+  }
+  for (int ch = 0; ch < 2; ch++)
+    for (int j = 0; j < 4; j++)
+      if (local[ch][j]) atomicAdd(&cost[ch][j], local[ch][j]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long nx = 0, nb = 0;
+    for (int j = 0; j < ncand[0]; j++) if (cands[0][j] == 0) nx = cost[0][j];
+    for (int j = 0; j < ncand[1]; j++) if (cands[1][j] == 0) nb = cost[1][j];
+    int bx = 0, bb = 0;
+    unsigned long long best = nx + nb;
+    for (int i = 0; i < ncand[0]; i++)
+      for (int j = 0; j < ncand[1]; j++) {
+        const int fx = cands[0][i], fb = cands[1][j];
+        unsigned long long c = cost[0][i] + cost[1][j];
+        if (fx != 0 || fb != 0) c += 50ull + factor_bits(fx) + factor_bits(fb);
+        const unsigned mag = (unsigned)abs(fx) + (unsigned)abs(fb);
+        const unsigned bmag = (unsigned)abs(bx) + (unsigned)abs(bb);
+        if (c < best || (c == best && mag < bmag)) {
+          bx = fx;
+          bb = fb;
+          best = c;
+        }
+      }
+    out->x_factor_lf = bx;
+    out->b_factor_lf = bb;
+  }
+}
+
 }  // namespace
 
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
@@ -240,6 +524,22 @@ void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const Pr
     k_cover_cost<32><<<div_up(c32, 2), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost32, err);
   dim3 grid(div_up((g.bw + 3) / 4, 256), (g.bh + 3) / 4);
   k_cover_decide<<<grid, 256, 0, s>>>(g, cost8, cost16, cost32, bmap);
+}
+
+}  // namespace jpxlg
+
+namespace jpxlg {
+
+void launch_cfl(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                const uint4* const items[3], const int counts[3], const uint32_t* tile_base,
+                uint32_t lf_count, float* const coeff[3], float4* lf_s, float4* hf_s, uint16_t* hf_cell,
+                int8_t* cfl_x, int8_t* cfl_b, CflParams* cflp, uint32_t* err) {
+  if (counts[0]) k_cfl_collect<8><<<counts[0], 64, 0, s>>>(g, items[0], counts[0], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+  if (counts[1]) k_cfl_collect<16><<<counts[1], 64, 0, s>>>(g, items[1], counts[1], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+  if (counts[2]) k_cfl_collect<32><<<counts[2], 64, 0, s>>>(g, items[2], counts[2], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+  const int tiles = g.tw * g.th;
+  k_cfl_tiles<<<tiles, 256, 0, s>>>(tile_base, tiles, tab, pp, hf_s, hf_cell, cfl_x, cfl_b);
+  k_cfl_lf<<<1, 256, 0, s>>>(lf_count, pp, lf_s, cflp);
 }
 
 }  // namespace jpxlg

@@ -1462,3 +1462,194 @@ EXPORT void orc_cover(int w, int h, const float *xyb[3], const orc_quant_params 
         for (int sbx = 0; sbx < cc.bw; sbx += 4) tile_region(&cc, sbx, sby, 4, bmap, 1);
     for (int i = 0; i < 3; i++) free(cc.hq[i]);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Chroma-from-luma estimate (jpxl-encode-policy/src/lib.rs:3437-4014,       */
+/* quantize.rs:1599-1717)                                                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { double s_yy, s_yc, s_cc; } cfl_acc;
+
+static void acc_add(cfl_acc *a, float y, float c) { /* quantize.rs:1628-1634 */
+    double yy = (double)y, cc = (double)c;
+    a->s_yy += yy * yy;
+    a->s_yc += yy * cc;
+    a->s_cc += cc * cc;
+}
+
+static float cfl_multiplier(float base, int factor) { return base + (float)factor / (float)84; }
+
+/* quantize.rs:1652-1717 best_factor + consider. */
+static int best_factor(const cfl_acc *a, float base, int lo, int hi) {
+    if (a->s_yy <= 0.0) return 0;
+    double k_ls = a->s_yc / a->s_yy;
+    double star = (k_ls - (double)base) * 84.0;
+    double r = round(star);
+    int64_t centre = r >= 9.2e18 ? INT64_MAX : (r <= -9.2e18 ? INT64_MIN : (int64_t)r);
+    int have = 0, best = 0;
+    double best_e = 0.0;
+    for (int i = 0; i < 6; i++) {
+        int cand;
+        if (i < 5) {
+            int64_t c = centre + (i - 2);
+            if (c < lo) c = lo;
+            if (c > hi) c = hi;
+            cand = (int)c;
+        } else cand = 0;
+        double k = (double)cfl_multiplier(base, cand);
+        double e = a->s_cc - 2.0 * k * a->s_yc + k * k * a->s_yy;
+        if (!have || e < best_e || (e == best_e && abs(cand) < abs(best))) { have = 1; best = cand; best_e = e; }
+    }
+    return best;
+}
+
+static uint64_t factor_bits(int f) { return f == 0 ? 0 : (uint64_t)(32 - __builtin_clz((uint32_t)abs(f))) + 2; }
+
+/* lib.rs:3771-3783 factor_candidates: seed-1..seed+1 clamped, deduplicated, plus 0. */
+static int factor_candidates(int seed, int lo, int hi, int *out) {
+    int n = 0;
+    for (int d = -1; d <= 1; d++) {
+        int c = seed + d;
+        if (c < lo) c = lo;
+        if (c > hi) c = hi;
+        int dup = 0;
+        for (int i = 0; i < n; i++) dup |= out[i] == c;
+        if (!dup) out[n++] = c;
+    }
+    int has0 = 0;
+    for (int i = 0; i < n; i++) has0 |= out[i] == 0;
+    if (!has0) out[n++] = 0;
+    return n;
+}
+
+typedef struct { float source, ry, dy; int cell; } cfl_sample;
+typedef struct { cfl_acc acc; cfl_sample *s; int n, cap; } cfl_set;
+
+static void set_push(cfl_set *s, float source, float y, float dy, int cell) {
+    if (s->n == s->cap) { s->cap = s->cap ? 2 * s->cap : 1024; s->s = realloc(s->s, sizeof(cfl_sample) * s->cap); }
+    acc_add(&s->acc, y, source);
+    s->s[s->n].source = source; s->s[s->n].ry = y; s->s[s->n].dy = dy; s->s[s->n].cell = cell;
+    s->n++;
+}
+
+/* lib.rs:3899-3955 hf_residual_cost_bounded (no cutoff: it cannot change the winner). */
+static uint64_t hf_residual_cost(const cfl_set *s, float base, int factor, int channel, const hf_quantizer *q) {
+    float k = cfl_multiplier(base, factor);
+    uint64_t bits = 0;
+    int ok = 1;
+    for (int i = 0; i < s->n; i++) {
+        float t = s->s[i].source - k * s->s[i].dy;
+        bits += residual_bits(hf_choose(q, t, channel, s->s[i].cell, &ok));
+    }
+    return bits;
+}
+
+/* lib.rs:3966-4014 refine_hf_factor. */
+static int refine_hf_factor(const cfl_set *s, float base, int channel, const hf_quantizer *q) {
+    if (s->n == 0) return 0;
+    int seed = best_factor(&s->acc, base, -128, 127);
+    if (seed == 0) return 0;
+    int best = 0;
+    uint64_t best_cost = hf_residual_cost(s, base, 0, channel, q);
+    int cands[4];
+    int nc = factor_candidates(seed, -128, 127, cands);
+    for (int i = 0; i < nc; i++) {
+        int f = cands[i];
+        if (f == 0) continue;
+        uint64_t cost = hf_residual_cost(s, base, f, channel, q) + factor_bits(f);
+        if (cost < best_cost || (cost == best_cost && abs(f) < abs(best))) { best_cost = cost; best = f; }
+    }
+    return best;
+}
+
+static uint64_t lf_residual_cost(const cfl_set *s, float base, int factor, int channel, const float *mdc) {
+    float k = cfl_multiplier(base, factor);
+    uint64_t bits = 0;
+    int ok = 1;
+    for (int i = 0; i < s->n; i++)
+        bits += residual_bits(lf_quantize(mdc, s->s[i].source - k * s->s[i].dy, channel, &ok));
+    return bits;
+}
+
+/* lib.rs:3831-3894 refine_lf_factors. */
+static void refine_lf_factors(const cfl_set *x, const cfl_set *b, const float *mdc, int *ox, int *ob) {
+    int xs = best_factor(&x->acc, 0.0f, -128, 127), bs = best_factor(&b->acc, 1.0f, -128, 127);
+    int xc[4], bc[4];
+    int nx = factor_candidates(xs, -128, 127, xc), nb = factor_candidates(bs, -128, 127, bc);
+    uint64_t xcost[4], bcost[4], neutral_x = 0, neutral_b = 0;
+    for (int i = 0; i < nx; i++) { xcost[i] = lf_residual_cost(x, 0.0f, xc[i], 0, mdc); if (xc[i] == 0) neutral_x = xcost[i]; }
+    for (int i = 0; i < nb; i++) { bcost[i] = lf_residual_cost(b, 1.0f, bc[i], 2, mdc); if (bc[i] == 0) neutral_b = bcost[i]; }
+    int best_x = 0, best_b = 0;
+    uint64_t best_cost = neutral_x + neutral_b;
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < nb; j++) {
+            uint64_t cost = xcost[i] + bcost[j];
+            if (xc[i] != 0 || bc[j] != 0) cost += 50 + factor_bits(xc[i]) + factor_bits(bc[j]);
+            unsigned mag = (unsigned)abs(xc[i]) + (unsigned)abs(bc[j]);
+            unsigned bmag = (unsigned)abs(best_x) + (unsigned)abs(best_b);
+            if (cost < best_cost || (cost == best_cost && mag < bmag)) { best_x = xc[i]; best_b = bc[j]; best_cost = cost; }
+        }
+    *ox = best_x; *ob = best_b;
+}
+
+/* lib.rs:3437-3763 estimate_cfl over the whole frame.  LF groups are 256
+ * blocks square; within a group, tile rows (bands) in order, varblocks in
+ * raster order inside each band. */
+EXPORT void orc_estimate_cfl(int w, int h, const uint8_t *bmap, const float *coeff[3],
+                             const orc_quant_params *p, int8_t *cfl_x, int8_t *cfl_b, int *lf_x, int *lf_b) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8, stride = 8 * bw, tw = (bw + 7) / 8, th = (bh + 7) / 8;
+    static hf_quantizer hq[3];
+    build_hf_quantizer(&hq[0], 8, p);
+    build_hf_quantizer(&hq[1], 16, p);
+    build_hf_quantizer(&hq[2], 32, p);
+    float mdc[3];
+    lf_multipliers(p->global_scale, p->quant_lf, mdc);
+    cfl_set lfx = {0}, lfb = {0};
+    cfl_set *tx = calloc((size_t)tw * th, sizeof(cfl_set)), *tb = calloc((size_t)tw * th, sizeof(cfl_set));
+    int ok = 1;
+    int gw = (bw + 255) / 256, gh = (bh + 255) / 256;
+    float vb[3][1024], lfs[3][16];
+    for (int gy = 0; gy < gh; gy++)
+        for (int gx = 0; gx < gw; gx++) {
+            int gbx0 = gx * 256, gby0 = gy * 256;
+            int gbx1 = gbx0 + 256 < bw ? gbx0 + 256 : bw, gby1 = gby0 + 256 < bh ? gby0 + 256 : bh;
+            for (int band = gby0; band < gby1; band += 8)
+                for (int by = band; by < band + 8 && by < gby1; by++)
+                    for (int bx = gbx0; bx < gbx1; bx++) {
+                        int side = bmap[by * bw + bx];
+                        if (!side) continue;
+                        int si = side == 8 ? 0 : (side == 16 ? 1 : 2), n = side / 8, cells = side * side;
+                        for (int c = 0; c < 3; c++)
+                            for (int k = 0; k < cells; k++)
+                                vb[c][k] = coeff[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side];
+                        lf_from_llf_square(vb[1], side, n, lfs[1]);
+                        lf_from_llf_square(vb[0], side, n, lfs[0]);
+                        lf_from_llf_square(vb[2], side, n, lfs[2]);
+                        for (int i = 0; i < n * n; i++) {
+                            float y = lfs[1][i];
+                            float dy = lf_recon(mdc, lf_quantize(mdc, y, 1, &ok), 1);
+                            set_push(&lfx, lfs[0][i], y, dy, 0);
+                            set_push(&lfb, lfs[2][i], y, dy, 0);
+                        }
+                        int tile = (by / 8) * tw + bx / 8, fold = side / 8;
+                        for (int row = 0; row < side; row++) {
+                            int c0 = row < n ? n : 0;
+                            for (int col = c0; col < side; col++) {
+                                int cell = row * side + col;
+                                int cell8 = (cell / side / fold) * 8 + (cell % side / fold);
+                                float y = vb[1][cell];
+                                float dy = hf_recon(&hq[si], hf_choose(&hq[si], y, 1, cell, &ok), 1, cell);
+                                set_push(&tx[tile], vb[0][cell], y, dy, cell8);
+                                set_push(&tb[tile], vb[2][cell], y, dy, cell8);
+                            }
+                        }
+                    }
+        }
+    refine_lf_factors(&lfx, &lfb, mdc, lf_x, lf_b);
+    for (int t = 0; t < tw * th; t++) {
+        cfl_x[t] = (int8_t)refine_hf_factor(&tx[t], 0.0f, 0, &hq[0]);
+        cfl_b[t] = (int8_t)refine_hf_factor(&tb[t], 1.0f, 2, &hq[0]);
+        free(tx[t].s); free(tb[t].s);
+    }
+    free(tx); free(tb); free(lfx.s); free(lfb.s);
+}

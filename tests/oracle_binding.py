@@ -196,6 +196,20 @@ def cover(xyb: np.ndarray, q: QuantParams) -> np.ndarray:
     return bm
 
 
+def estimate_cfl(w: int, h: int, bmap: np.ndarray, coeff: np.ndarray, q: QuantParams):
+    """-> (cfl_x tiles, cfl_b tiles, (x_factor_lf-128, b_factor_lf-128))."""
+    bh, bw = bmap.shape
+    th, tw = (bh + 7) // 8, (bw + 7) // 8
+    cx = np.zeros((th, tw), np.int8)
+    cb = np.zeros((th, tw), np.int8)
+    lx, lb = ctypes.c_int(), ctypes.c_int()
+    coeff = np.ascontiguousarray(coeff, np.float32)
+    bm = np.ascontiguousarray(bmap, np.uint8)
+    lib().orc_estimate_cfl(w, h, P(bm), planes_ptr([coeff[0], coeff[1], coeff[2]]), ctypes.byref(q), P(cx), P(cb),
+                           ctypes.byref(lx), ctypes.byref(lb))
+    return cx, cb, (lx.value, lb.value)
+
+
 def epf_step(step: int, planes: np.ndarray, sigma: np.ndarray) -> np.ndarray:
     planes = np.ascontiguousarray(planes, np.float32)
     sigma = np.ascontiguousarray(sigma, np.float32)
