@@ -42,7 +42,45 @@ struct jpxlg_ctx {
   std::map<int, float*> d_lut16;
   std::map<int, DepthSet> depth;
   std::string last_error;
+  // Image arenas released by jpxlg_image_destroy, kept for the next image
+  // of a similar size (size -> device pointer); freed with the context.
+  std::multimap<size_t, void*> spare;
+  size_t spare_bytes = 0;
 };
+
+namespace {
+
+// At most this much device memory is kept in released arenas per context.
+constexpr size_t kSpareCap = (size_t)8 << 30;
+
+// A released arena of at least `bytes` (and at most twice that), or null.
+void* take_spare(jpxlg_ctx* ctx, size_t bytes, size_t* got) {
+  auto it = ctx->spare.lower_bound(bytes);
+  if (it == ctx->spare.end() || it->first > 2 * bytes) return nullptr;
+  void* p = it->second;
+  *got = it->first;
+  ctx->spare_bytes -= it->first;
+  ctx->spare.erase(it);
+  return p;
+}
+
+void release_arena(jpxlg_ctx* ctx, void* base, size_t bytes) {
+  if (!base) return;
+  while (ctx->spare_bytes + bytes > kSpareCap && !ctx->spare.empty()) {
+    auto it = ctx->spare.begin();
+    (void)hipFree(it->second);
+    ctx->spare_bytes -= it->first;
+    ctx->spare.erase(it);
+  }
+  if (bytes > kSpareCap) {
+    (void)hipFree(base);
+    return;
+  }
+  ctx->spare.emplace(bytes, base);
+  ctx->spare_bytes += bytes;
+}
+
+}  // namespace
 
 namespace {
 
@@ -141,7 +179,8 @@ size_t image_bytes(int w, int h) {
   b += 3 * pyr * 4;                  // candidate XYB' pyramid
   b += 3 * pyr1 * 4;                 // linear RGB pyramid (scales >= 1)
   b += 3 * px * 4;                   // linear RGB at scale 0 (reference build)
-  size_t tiles = ((w + 63) / 64 + 1) * ((h + 15) / 16 + 1);
+  size_t tiles = 0;
+  for (int s = 0; s < n; s++) tiles += (size_t)pool_tiles(sw[s], sh[s]);
   b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
   b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
   b += 256 * 256;  // alignment of every carved buffer
@@ -197,8 +236,10 @@ struct jpxlg_img {
   bool has_render = false;
   jpxlg_quantizer last_q{};
   int last_sharpness = 7;
+  // The arena goes back to the context (or is freed) when the image dies.
   ~jpxlg_img() {
-    if (arena.base) (void)hipFree(arena.base);
+    if (arena.base) (void)hipStreamSynchronize(ctx->stream);
+    release_arena(ctx, arena.base, arena.size);
   }
 };
 
@@ -238,6 +279,7 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
     (void)hipFree(kv.second.thr);
     (void)hipFree(kv.second.lut);
   }
+  for (auto& kv : ctx->spare) (void)hipFree(kv.second);
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -307,9 +349,9 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   g.tw = (g.bw + 7) / 8;
   g.th = (g.bh + 7) / 8;
   scale_dims(g.w, g.h, &img->nscales, img->sw, img->sh);
-  const size_t bytes = image_bytes(g.w, g.h);
-  void* base = nullptr;
-  if (hipMalloc(&base, bytes) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
+  size_t bytes = image_bytes(g.w, g.h);
+  void* base = take_spare(ctx, bytes, &bytes);
+  if (!base && hipMalloc(&base, bytes) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
   img->arena.base = (char*)base;
   img->arena.size = bytes;
   Arena& A = img->arena;
@@ -350,7 +392,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_pp = (ProbeParams*)A.take(sizeof(ProbeParams));
   img->d_err = (uint32_t*)A.take(4);
   img->d_gray = (int*)A.take(4);
-  size_t ptiles = ((g.w + 63) / 64 + 1) * ((g.h + 15) / 16 + 1);
+  size_t ptiles = 0;
+  for (int s = 0; s < img->nscales; s++) ptiles += (size_t)pool_tiles(img->sw[s], img->sh[s]);
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
@@ -711,6 +754,9 @@ static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
   DepthSet d;
   int rc = get_depth(ctx, img->bits, &d);
   if (rc) return rc;
+  PoolLayout L{};
+  L.nscales = img->nscales;
+  int ptiles = 0;
   for (int s = 0; s < img->nscales; s++) {
     float** down = s + 1 < img->nscales ? img->cand_rgb[s + 1] : nullptr;
     if (s == 0) {
@@ -722,14 +768,15 @@ static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
       launch_colour_metric(st, 1, img->sw[s], img->sh[s], img->cand_rgb[s], nullptr, nullptr, 0,
                            img->cand_x[s], down);
     }
-    int tiles;
     {
       Prof p(ctx, s == 0 ? "blur_pool_s0" : "blur_pool");
-      tiles = launch_blur_pool(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s],
-                               img->ref_mu[s], img->ref_s11[s], img->cand_x[s], img->d_partials);
+      L.off[s] = ptiles;
+      L.tiles[s] = launch_blur_pool(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s], img->ref_mu[s],
+                                    img->ref_s11[s], img->cand_x[s], img->d_partials + (size_t)ptiles * 18);
+      ptiles += L.tiles[s];
     }
-    { Prof p(ctx, "pool_reduce"); launch_reduce_partials(st, tiles, img->d_partials, img->d_sums + s * 18); }
   }
+  { Prof p(ctx, "pool_reduce"); launch_reduce_scales(st, L, img->d_partials, img->d_sums); }
   HIP_TRY(ctx, hipGetLastError());
   return JPXLG_OK;
 }
@@ -983,7 +1030,8 @@ extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t hei
   scale_dims(w, h, &n, sw, sh);
   size_t pyr = 0;
   for (int s = 0; s < n; s++) pyr += (size_t)sw[s] * sh[s];
-  size_t ptiles = ((w + 63) / 64 + 1) * ((h + 15) / 16 + 1);
+  size_t ptiles = 0;
+  for (int s = 0; s < n; s++) ptiles += (size_t)pool_tiles(sw[s], sh[s]);
   DevBuf<char> mem;
   size_t bytes = 18 * pyr * 4 + ptiles * 18 * 8 + kMaxScales * 18 * 8 + 256 * 256;
   HIP_TRY(ctx, hipMalloc(&mem.p, bytes));
@@ -1010,15 +1058,21 @@ extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t hei
     HIP_TRY(ctx, hipMemcpyAsync(rrgb[0][c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(crgb[0][c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
   }
+  PoolLayout L{};
+  L.nscales = n;
+  int used = 0;
   for (int s = 0; s < n; s++) {
     float** rd = s + 1 < n ? rrgb[s + 1] : nullptr;
     float** cd = s + 1 < n ? crgb[s + 1] : nullptr;
     launch_colour_metric(st, 1, sw[s], sh[s], rrgb[s], nullptr, nullptr, 0, rx[s], rd);
     launch_colour_metric(st, 1, sw[s], sh[s], crgb[s], nullptr, nullptr, 0, cx[s], cd);
     launch_blur_moments(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s]);
-    int tiles = launch_blur_pool(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s], cx[s], partials);
-    launch_reduce_partials(st, tiles, partials, sums_d + s * 18);
+    L.off[s] = used;
+    L.tiles[s] = launch_blur_pool(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s], cx[s],
+                                  partials + (size_t)used * 18);
+    used += L.tiles[s];
   }
+  launch_reduce_scales(st, L, partials, sums_d);
   HIP_TRY(ctx, hipGetLastError());
   double sums[kMaxScales * 18];
   HIP_TRY(ctx, hipMemcpyAsync(sums, sums_d, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));

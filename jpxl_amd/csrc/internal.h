@@ -109,6 +109,15 @@ void launch_cfl(hipStream_t s, const Geom& g, const ConstTables* tab, const Prob
                 const uint4* const items[3], const int counts[3], const uint32_t* tile_base,
                 uint32_t lf_count, float* const coeff[3], float4* lf_s, float4* hf_s, uint16_t* hf_cell,
                 int8_t* cfl_x, int8_t* cfl_b, CflParams* cflp, uint32_t* err);
-void launch_reduce_partials(hipStream_t s, int tiles, const double* partials, double* sums18);
+// Pooling tiles of one scale (one partial row of 18 sums per tile).
+int pool_tiles(int w, int h);
+// Where each scale's tiles start in the partials buffer (in tiles).
+struct PoolLayout {
+  int nscales;
+  int off[6];
+  int tiles[6];
+};
+// Sums every scale's partials into 18 f64 sums per scale, in a fixed order.
+void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums);
 
 }  // namespace jpxlg

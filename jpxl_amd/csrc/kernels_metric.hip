@@ -208,6 +208,28 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// The candidate pass of one scale and channel over a 64x24 tile: the three
+// candidate blurs (mu2 = G*x2, s22 = G*x2^2, s12 = G*x1x2), the SSIM and
+// edge maps, and the tile's f64 partial sums (pool.rs:97-278).  Each lane
+// slides its 9-tap window over a run of outputs held in registers, so every
+// input sample is widened to f64 once per run instead of once per tap; the
+// taps, their pairing and the one f32 rounding per pass are those of fir9.
+constexpr int kPW = 64, kPH = 24;
+constexpr int kPIW = kPW + 2 * kHalo, kPIH = kPH + 2 * kHalo;  // 72 x 32
+constexpr int kHRun = 8;                                      // horizontal outputs per lane
+constexpr int kVRun = 6;                                      // vertical outputs per lane
+
+template <int R>
+__device__ __forceinline__ void fir_run(const double (&d)[R + 8], const double* hf, double (&o)[R]) {
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    double acc = hf[0] * d[i + 4];
+#pragma unroll
+    for (int t = 1; t <= 4; t++) acc = fma(hf[t], d[i + 4 - t] + d[i + 4 + t], acc);
+    o[i] = acc;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTables* __restrict__ tab,
                                                    const float* __restrict__ a0, const float* __restrict__ a1,
                                                    const float* __restrict__ a2, const float* __restrict__ m0,
@@ -216,99 +238,130 @@ __global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTabl
                                                    const float* __restrict__ v2, const float* __restrict__ b0,
                                                    const float* __restrict__ b1, const float* __restrict__ b2,
                                                    double* __restrict__ partials) {
-  __shared__ float t1[kLH][kLW];
-  __shared__ float t2[kLH][kLW];
-  __shared__ float hb[3][kLH][kTW];
+  __shared__ float t1[kPIH][kPIW];
+  __shared__ float t2[kPIH][kPIW];
+  __shared__ float hb[3][kPIH][kPW];
   __shared__ double red[4][6];
-  const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
+  const int c = blockIdx.z;
+  const int x0 = blockIdx.x * kPW, y0 = blockIdx.y * kPH;
   const int tile = blockIdx.y * gridDim.x + blockIdx.x;
   double hf[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
-  const float* X1[3] = {a0, a1, a2};
-  const float* MU1[3] = {m0, m1, m2};
-  const float* S11[3] = {v0, v1, v2};
-  const float* X2[3] = {b0, b1, b2};
+  const float* X1 = c == 0 ? a0 : (c == 1 ? a1 : a2);
+  const float* MU1 = c == 0 ? m0 : (c == 1 ? m1 : m2);
+  const float* S11 = c == 0 ? v0 : (c == 1 ? v1 : v2);
+  const float* X2 = c == 0 ? b0 : (c == 1 ? b1 : b2);
+  for (int e = threadIdx.x; e < kPIH * kPIW; e += 256) {
+    const int ly = e / kPIW, lx = e % kPIW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
+    const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
+    const size_t o = (size_t)gy * w + gx;
+    t1[ly][lx] = in ? X1[o] : 0.0f;
+    t2[ly][lx] = in ? X2[o] : 0.0f;
+  }
+  __syncthreads();
+  // Horizontal: 32 rows x 8 runs of 8 outputs = one run per lane.
+  {
+    const int ly = threadIdx.x >> 3, lx0 = (threadIdx.x & 7) * kHRun;
+    float p1[kHRun + 8], p2[kHRun + 8];
+#pragma unroll
+    for (int i = 0; i < kHRun + 8; i++) {
+      p1[i] = t1[ly][lx0 + i];
+      p2[i] = t2[ly][lx0 + i];
+    }
+    double d[kHRun + 8], o[kHRun];
+#pragma unroll
+    for (int i = 0; i < kHRun + 8; i++) d[i] = (double)p2[i];
+    fir_run<kHRun>(d, hf, o);
+#pragma unroll
+    for (int i = 0; i < kHRun; i++) hb[0][ly][lx0 + i] = (float)o[i];
+    // the squared and product inputs are rounded to f32 first (blur.rs:97-113)
+#pragma unroll
+    for (int i = 0; i < kHRun + 8; i++) d[i] = (double)(p2[i] * p2[i]);
+    fir_run<kHRun>(d, hf, o);
+#pragma unroll
+    for (int i = 0; i < kHRun; i++) hb[1][ly][lx0 + i] = (float)o[i];
+#pragma unroll
+    for (int i = 0; i < kHRun + 8; i++) d[i] = (double)(p1[i] * p2[i]);
+    fir_run<kHRun>(d, hf, o);
+#pragma unroll
+    for (int i = 0; i < kHRun; i++) hb[2][ly][lx0 + i] = (float)o[i];
+  }
+  __syncthreads();
+  // Vertical + maps: 64 columns x 4 runs of 6 rows; wave k takes rows 6k..6k+5.
+  const int lx = threadIdx.x & 63, ly0 = (threadIdx.x >> 6) * kVRun;
+  const int gx = x0 + lx;
+  float mu2[kVRun], s22[kVRun], s12[kVRun];
+  {
+    double d[kVRun + 8], o[kVRun];
+#pragma unroll
+    for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[0][ly0 + i][lx];
+    fir_run<kVRun>(d, hf, o);
+#pragma unroll
+    for (int i = 0; i < kVRun; i++) mu2[i] = (float)o[i];
+#pragma unroll
+    for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[1][ly0 + i][lx];
+    fir_run<kVRun>(d, hf, o);
+#pragma unroll
+    for (int i = 0; i < kVRun; i++) s22[i] = (float)o[i];
+#pragma unroll
+    for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[2][ly0 + i][lx];
+    fir_run<kVRun>(d, hf, o);
+#pragma unroll
+    for (int i = 0; i < kVRun; i++) s12[i] = (float)o[i];
+  }
   const float C2 = 0.0009f;
-  for (int c = 0; c < 3; c++) {
-    for (int e = threadIdx.x; e < kLH * kLW; e += 256) {
-      int ly = e / kLW, lx = e % kLW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
-      bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-      size_t o = (size_t)gy * w + gx;
-      t1[ly][lx] = in ? X1[c][o] : 0.0f;
-      t2[ly][lx] = in ? X2[c][o] : 0.0f;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < kLH * kTW; e += 256) {
-      int ly = e / kTW, lx = e % kTW;
-      const float* p1 = &t1[ly][lx + kHalo];
-      const float* p2 = &t2[ly][lx + kHalo];
-      double m = hf[0] * (double)p2[0], s22 = hf[0] * (double)(p2[0] * p2[0]),
-             s12 = hf[0] * (double)(p1[0] * p2[0]);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll
-      for (int t = 1; t <= 4; t++) {
-        m = fma(hf[t], (double)p2[-t] + (double)p2[t], m);
-        s22 = fma(hf[t], (double)(p2[-t] * p2[-t]) + (double)(p2[t] * p2[t]), s22);
-        s12 = fma(hf[t], (double)(p1[-t] * p2[-t]) + (double)(p1[t] * p2[t]), s12);
-      }
-      hb[0][ly][lx] = (float)m;
-      hb[1][ly][lx] = (float)s22;
-      hb[2][ly][lx] = (float)s12;
-    }
-    __syncthreads();
-    double acc[6] = {0, 0, 0, 0, 0, 0};
-    for (int e = threadIdx.x; e < kTH * kTW; e += 256) {
-      int ly = e / kTW, lx = e % kTW, gx = x0 + lx, gy = y0 + ly;
-      if (gx >= w || gy >= h) continue;
-      const size_t o = (size_t)gy * w + gx;
-      const float mu2 = fir9(&hb[0][ly + kHalo][lx], kTW, hf);
-      const float s22 = fir9(&hb[1][ly + kHalo][lx], kTW, hf);
-      const float s12 = fir9(&hb[2][ly + kHalo][lx], kTW, hf);
-      const float i1 = t1[ly + kHalo][lx + kHalo], i2 = t2[ly + kHalo][lx + kHalo];
-      const float mu1 = MU1[c][o], s11 = S11[c][o];
-      const float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
-      const float md = mu1 - mu2;
-      const float num_m = 1.0f - md * md;
-      const float num_s = 2.0f * (s12 - mu12) + C2;
-      const float den_s = (s11 - mu11) + (s22 - mu22) + C2;
-      double d = 1.0 - (double)((num_m * num_s) / den_s);
-      d = d > 0.0 ? d : 0.0;
-      acc[0] += d;
-      acc[1] += d * d * d * d;
-      double d1 = (1.0 + (double)fabsf(i2 - mu2)) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
-      double art = d1 > 0.0 ? d1 : 0.0;
-      double lost = -d1 > 0.0 ? -d1 : 0.0;
-      acc[2] += art;
-      acc[3] += art * art * art * art;
-      acc[4] += lost;
-      acc[5] += lost * lost * lost * lost;
-    }
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int i = 0; i < kVRun; i++) {
+    const int gy = y0 + ly0 + i;
+    if (gx >= w || gy >= h) continue;
+    const size_t o = (size_t)gy * w + gx;
+    const float i1 = t1[ly0 + i + kHalo][lx + kHalo], i2 = t2[ly0 + i + kHalo][lx + kHalo];
+    const float mu1 = MU1[o], s11 = S11[o];
+    const float mu11 = mu1 * mu1, mu22 = mu2[i] * mu2[i], mu12 = mu1 * mu2[i];
+    const float md = mu1 - mu2[i];
+    const float num_m = 1.0f - md * md;
+    const float num_s = 2.0f * (s12[i] - mu12) + C2;
+    const float den_s = (s11 - mu11) + (s22[i] - mu22) + C2;
+    double d = 1.0 - (double)((num_m * num_s) / den_s);
+    d = d > 0.0 ? d : 0.0;
+    acc[0] += d;
+    acc[1] += d * d * d * d;
+    double d1 = (1.0 + (double)fabsf(i2 - mu2[i])) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
+    double art = d1 > 0.0 ? d1 : 0.0;
+    double lost = -d1 > 0.0 ? -d1 : 0.0;
+    acc[2] += art;
+    acc[3] += art * art * art * art;
+    acc[4] += lost;
+    acc[5] += lost * lost * lost * lost;
+  }
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
 #pragma unroll
-    for (int k = 0; k < 6; k++) {
-      double v = wave_sum(acc[k]);
-      if (lane == 0) red[wave][k] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < 6) {
-      double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-      partials[(size_t)tile * 18 + c * 6 + threadIdx.x] = v;
-    }
-    __syncthreads();
+  for (int k = 0; k < 6; k++) {
+    double v = wave_sum(acc[k]);
+    if (lane == 0) red[wave][k] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+    partials[(size_t)tile * 18 + c * 6 + threadIdx.x] = v;
   }
 }
 
-// Fixed-order reduction of the per-tile partial sums: one workgroup per term.
-__global__ __launch_bounds__(256) void k_reduce(int tiles, const double* __restrict__ partials,
-                                                double* __restrict__ sums18) {
+// Fixed-order reduction of every scale's per-tile partial sums in one launch:
+// workgroup (scale, term) sums that term over the scale's tiles.
+__global__ __launch_bounds__(256) void k_reduce_scales(PoolLayout L, const double* __restrict__ partials,
+                                                       double* __restrict__ sums) {
   __shared__ double red[4];
-  const int term = blockIdx.x;
+  const int s = blockIdx.x / 18, term = blockIdx.x % 18;
+  const double* p = partials + (size_t)L.off[s] * 18;
   double acc = 0.0;
-  for (int t = threadIdx.x; t < tiles; t += 256) acc += partials[(size_t)t * 18 + term];
+  for (int t = threadIdx.x; t < L.tiles[s]; t += 256) acc += p[(size_t)t * 18 + term];
   acc = wave_sum(acc);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) sums18[term] = ((red[0] + red[1]) + red[2]) + red[3];
+  if (threadIdx.x == 0) sums[s * 18 + term] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
 }  // namespace
@@ -345,17 +398,19 @@ void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, fl
   for (int c = 0; c < 3; c++) k_blur_moments<<<grid, 256, 0, s>>>(w, h, tab, x1[c], mu1[c], s11[c]);
 }
 
+int pool_tiles(int w, int h) { return (int)(div_up(w, kPW) * div_up(h, kPH)); }
+
 int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
                      float* const mu1[3], float* const s11[3], float* const x2[3],
                      double* partials) {
-  dim3 grid(div_up(w, kTW), div_up(h, kTH));
+  dim3 grid(div_up(w, kPW), div_up(h, kPH), 3);
   k_blur_pool<<<grid, 256, 0, s>>>(w, h, tab, x1[0], x1[1], x1[2], mu1[0], mu1[1], mu1[2], s11[0],
                                    s11[1], s11[2], x2[0], x2[1], x2[2], partials);
   return (int)(grid.x * grid.y);
 }
 
-void launch_reduce_partials(hipStream_t s, int tiles, const double* partials, double* sums18) {
-  k_reduce<<<18, 256, 0, s>>>(tiles, partials, sums18);
+void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums) {
+  k_reduce_scales<<<18 * L.nscales, 256, 0, s>>>(L, partials, sums);
 }
 
 }  // namespace jpxlg

@@ -301,51 +301,88 @@ __device__ __forceinline__ int32_t hf_choose(float target, float step, float bia
   return best;
 }
 
-__device__ __forceinline__ int prev_nonzero(const int32_t* q, const uint16_t* order, int first,
-                                            int before) {
-  for (int pos = before - 1; pos >= first; pos--)
-    if (q[order[pos]] != 0) return pos;
-  return -1;
-}
-
-// Trailing-truncation backward pass over one varblock channel (one thread).
+// Trailing truncation of one varblock channel by one wave64
+// (jpxl-encode-policy/src/quantize.rs:468-545).  The reference walks the
+// natural order backwards from the last nonzero and zeroes it while the bits
+// it saves (its own magnitude class plus the zero tokens it exposes back to
+// the previous nonzero) exceed its weighted distortion gain; it stops at the
+// first coefficient that fails.  The walk only ever removes a suffix, so each
+// nonzero's "previous nonzero" is that of the unmodified block and every
+// coefficient's test is independent: the lanes evaluate all of them at once
+// and zero every nonzero after the last one that fails.
 template <int N>
-__device__ void truncate_walk(int32_t* q, const float* targets, float* recon_y,
+__device__ void truncate_wave(int32_t* q, const float* targets, float* recon_y,
                               const uint16_t* order, const float* mat, const float* donor,
                               float sc, float bias, double lambda, float ztb) {
-  constexpr int NN = N * N, nb = (N / 8) * (N / 8);
-  int last = prev_nonzero(q, order, nb, NN);
-  while (last >= 0) {
-    int cell = order[last];
-    int32_t v = q[cell];
-    float weight = (float)(lambda * (double)NN * (double)donor[cell]);
-    int previous = prev_nonzero(q, order, nb, last);
-    int exposed = previous >= 0 ? last - previous - 1 : last - nb;
-    float zeros = (float)exposed;
-    float own = (float)(33 - __clz((unsigned)abs(v)));
-    float saved = own + zeros * ztb;
-    float target = targets[cell];
-    float step = sc * mat[cell];
-    float recon = bias_adjust(v, bias) * step;
-    float kept = recon - target;
-    float added = fmaf(target, target, -(kept * kept));
-    float cost = weight * added;
-    if (saved <= cost) break;
-    q[cell] = 0;
-    if (recon_y) recon_y[cell] = bias_adjust(0, bias) * step;
-    last = previous;
+  constexpr int NN = N * N, nb = (N / 8) * (N / 8), R = NN / 64;
+  const int lane = threadIdx.x & 63;
+  const int pos0 = lane * R;
+  int cell[R];
+  int32_t v[R];
+  int lastnz = -1;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int p = pos0 + r;
+    cell[r] = order[p];
+    v[r] = p >= nb ? q[cell[r]] : 0;
+    if (v[r] != 0) lastnz = p;
+  }
+  // Inclusive max-scan of the lanes' last nonzero positions.
+  int incl = lastnz;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = max(incl, o);
+  }
+  int prev = __shfl_up(incl, 1, 64);
+  if (lane == 0) prev = -1;
+  int fail = -1;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int p = pos0 + r;
+    if (v[r] == 0) continue;
+    const int c = cell[r];
+    const float weight = (float)(lambda * (double)NN * (double)donor[c]);
+    const int exposed = prev >= 0 ? p - prev - 1 : p - nb;
+    const float own = (float)(33 - __clz((unsigned)abs(v[r])));
+    const float saved = own + (float)exposed * ztb;
+    const float target = targets[c];
+    const float step = sc * mat[c];
+    const float recon = bias_adjust(v[r], bias) * step;
+    const float kept = recon - target;
+    const float added = fmaf(target, target, -(kept * kept));
+    if (saved <= weight * added) fail = p;
+    prev = p;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) fail = max(fail, __shfl_xor(fail, d, 64));
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int p = pos0 + r;
+    if (v[r] != 0 && p > fail) {
+      const int c = cell[r];
+      q[c] = 0;
+      if (recon_y) recon_y[c] = bias_adjust(0, bias) * (sc * mat[c]);
+    }
   }
 }
 
+// Varblocks per quantize workgroup and its width: DCT32 takes one varblock
+// per 256-thread workgroup so its LDS footprint (~29 KB) leaves room for
+// five workgroups per CU; the smaller sizes keep 64/N varblocks on 192.
+constexpr int quant_vb(int n) { return n == 32 ? 1 : 64 / n; }
+constexpr int quant_threads(int n) { return n == 32 ? 256 : kThreads; }
+
 template <int N>
-__global__ __launch_bounds__(kThreads) void k_quantize(
+__global__ __launch_bounds__(quant_threads(N)) void k_quantize(
     Geom g, const uint32_t* __restrict__ list, int count, const ConstTables* __restrict__ tab,
     const ProbeParams* __restrict__ pp, const CflParams* __restrict__ cflp,
     const int8_t* __restrict__ cfl_x, const int8_t* __restrict__ cfl_b, const float* __restrict__ CX,
     const float* __restrict__ CY, const float* __restrict__ CB, int32_t* __restrict__ QX,
     int32_t* __restrict__ QY, int32_t* __restrict__ QB, int32_t* __restrict__ LX,
     int32_t* __restrict__ LY, int32_t* __restrict__ LB, uint32_t* __restrict__ err) {
-  constexpr int VB = 64 / N, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  constexpr int VB = quant_vb(N), T = quant_threads(N), NN = N * N, n = N / 8, si = size_index(N),
+                cb = kCellBase[si];
   __shared__ float co[3][VB][NN];
   __shared__ int32_t qs[3][VB][NN];
   __shared__ float dyh[VB][NN];
@@ -366,7 +403,7 @@ __global__ __launch_bounds__(kThreads) void k_quantize(
   }
   __syncthreads();
   const float* cin[3] = {CX, CY, CB};
-  for (int e = threadIdx.x; e < 3 * VB * NN; e += kThreads) {
+  for (int e = threadIdx.x; e < 3 * VB * NN; e += T) {
     int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
     float v = 0.0f;
     if (j < nvb) v = cin[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N];
@@ -396,7 +433,7 @@ __global__ __launch_bounds__(kThreads) void k_quantize(
   // --- Y HF ---
   {
     const float sc = pp->sc[1], bias = kQuantBias[1];
-    for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
+    for (int e = threadIdx.x; e < VB * NN; e += T) {
       int j = e / NN, k = e % NN;
       float step = sc * tab->matrix[1][cb + k];
       int32_t v = 0;
@@ -406,11 +443,11 @@ __global__ __launch_bounds__(kThreads) void k_quantize(
     }
   }
   __syncthreads();
-  if (pp->truncate && threadIdx.x < nvb) {
-    const int j = threadIdx.x;
-    truncate_walk<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
-                     tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
-  }
+  const int wave = threadIdx.x >> 6;
+  if (pp->truncate)
+    for (int j = wave; j < nvb; j += T / 64)
+      truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
+                       tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
   __syncthreads();
   // --- X and B LF (one thread per varblock channel) ---
   if (threadIdx.x < 2 * VB && (threadIdx.x % VB) < nvb) {
@@ -434,7 +471,7 @@ __global__ __launch_bounds__(kThreads) void k_quantize(
   }
   __syncthreads();
   // --- X and B HF: targets against the reconstructed dY, then choose ---
-  for (int e = threadIdx.x; e < 2 * VB * NN; e += kThreads) {
+  for (int e = threadIdx.x; e < 2 * VB * NN; e += T) {
     int ci = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN, c = ci == 0 ? 0 : 2;
     int32_t v = 0;
     if ((k % N) >= n || (k / N) >= n) {
@@ -446,14 +483,16 @@ __global__ __launch_bounds__(kThreads) void k_quantize(
     qs[c][j][k] = v;
   }
   __syncthreads();
-  if (pp->truncate && threadIdx.x < 2 * VB && (threadIdx.x % VB) < nvb) {
-    const int ci = threadIdx.x / VB, j = threadIdx.x % VB, c = ci == 0 ? 0 : 2;
-    truncate_walk<N>(qs[c][j], co[c][j], nullptr, tab->order + cb, tab->matrix[c] + cb,
-                     tab->donor + cb, pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
-  }
+  if (pp->truncate)
+    for (int w = wave; w < 2 * VB; w += T / 64) {
+      const int ci = w / VB, j = w % VB, c = ci == 0 ? 0 : 2;
+      if (j < nvb)
+        truncate_wave<N>(qs[c][j], co[c][j], nullptr, tab->order + cb, tab->matrix[c] + cb,
+                         tab->donor + cb, pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
+    }
   __syncthreads();
   int32_t* qout[3] = {QX, QY, QB};
-  for (int e = threadIdx.x; e < 3 * VB * NN; e += kThreads) {
+  for (int e = threadIdx.x; e < 3 * VB * NN; e += T) {
     int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
     if (j < nvb) qout[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N] = qs[c][j][k];
   }
@@ -724,7 +763,7 @@ void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* lis
                      int32_t* quant[3], int32_t* lf[3], uint32_t* err) {
   if (count <= 0) return;
 #define JPXLG_Q(NS, VBN)                                                                          \
-  k_quantize<NS><<<div_up(count, VBN), kThreads, 0, s>>>(g, list, count, tab, pp, cfl, cfl_x,   \
+  k_quantize<NS><<<div_up(count, quant_vb(NS)), quant_threads(NS), 0, s>>>(g, list, count, tab, pp, cfl, cfl_x,   \
                                                          cfl_b, coeff[0], coeff[1], coeff[2],   \
                                                          quant[0], quant[1], quant[2], lf[0],   \
                                                          lf[1], lf[2], err)

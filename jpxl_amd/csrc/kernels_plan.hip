@@ -369,45 +369,114 @@ __global__ __launch_bounds__(64) void k_cfl_collect(Geom g, const uint4* __restr
   if (bad) atomicOr(err, kErrQuantRange);
 }
 
-// One workgroup per 64x64 tile: sequential regression, then the residual
-// prices of the candidate factors against the DCT8 baseline quantizer.
-__global__ __launch_bounds__(256) void k_cfl_tiles(const uint32_t* __restrict__ tile_base, int tiles,
-                                                   const ConstTables* __restrict__ tab,
-                                                   const ProbeParams* __restrict__ pp,
-                                                   const float4* __restrict__ hf_s,
-                                                   const uint16_t* __restrict__ hf_cell,
-                                                   int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b) {
-  __shared__ int seeds[2];
+// Sums of y*y, y*x, x*x, y*b and b*b over samples [b0, b1) in their order,
+// each accumulated sequentially in f64 as the reference's CflAccumulator
+// does (quantize.rs:1620-1640).  A product of two f32 values is exact in
+// f64, so the workgroup first forms a chunk's products in LDS in parallel;
+// then five waves each run one of the five sums, one lane per sum, in sample
+// order.  The sums are valid in thread 0 after the call; every thread of the
+// (kCflThreads-wide) workgroup must call it.
+constexpr int kCflThreads = 320;
+constexpr int kCflStage = 1024;
+
+struct CflSums {
+  double yy, ycx, ccx, ycb, ccb;
+};
+
+__device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0, uint32_t b1,
+                                       double (*prod)[kCflStage]) {
+  __shared__ double total[5];
+  constexpr int kPer = (kCflStage + kCflThreads - 1) / kCflThreads;  // samples per thread per chunk
+  constexpr int kU = 16;                                             // chain unroll
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc = 0.0;
+  // Each thread prefetches its share of the next chunk into registers while
+  // the five summing lanes work through the current one.
+  float4 pre[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const uint32_t i = b0 + threadIdx.x + k * kCflThreads;
+    pre[k] = i < b1 && threadIdx.x + k * kCflThreads < kCflStage ? s[i] : make_float4(0, 0, 0, 0);
+  }
+  for (uint32_t c0 = b0; c0 < b1; c0 += kCflStage) {
+    const uint32_t m = min((uint32_t)kCflStage, b1 - c0);
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = threadIdx.x + k * kCflThreads;
+      if (i < m) {
+        const double y = (double)pre[k].y, x = (double)pre[k].x, b = (double)pre[k].z;
+        prod[0][i] = y * y;
+        prod[1][i] = y * x;
+        prod[2][i] = x * x;
+        prod[3][i] = y * b;
+        prod[4][i] = b * b;
+      }
+    }
+    __syncthreads();
+    const uint32_t n0 = c0 + kCflStage;
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+      const uint32_t i = n0 + threadIdx.x + k * kCflThreads;
+      if (n0 < b1 && i < b1 && threadIdx.x + k * kCflThreads < kCflStage) pre[k] = s[i];
+    }
+    if (lane == 0 && wave < 5) {
+      const double* p = prod[wave];
+      uint32_t i = 0;
+      if (m >= kU) {
+        double cur[kU];
+#pragma unroll
+        for (int k = 0; k < kU; k++) cur[k] = p[k];
+        for (; i + kU <= m; i += kU) {
+          double nxt[kU];
+          const bool more = i + 2 * kU <= m;
+          if (more) {
+#pragma unroll
+            for (int k = 0; k < kU; k++) nxt[k] = p[i + kU + k];
+          }
+#pragma unroll
+          for (int k = 0; k < kU; k++) acc += cur[k];
+          if (more) {
+#pragma unroll
+            for (int k = 0; k < kU; k++) cur[k] = nxt[k];
+          }
+        }
+      }
+      for (; i < m; i++) acc += p[i];
+    }
+    __syncthreads();
+  }
+  if (lane == 0 && wave < 5) total[wave] = acc;
+  __syncthreads();
+  return CflSums{total[0], total[1], total[2], total[3], total[4]};
+}
+
+// One 64x64 tile's HF factors: sequential regression seeds, then the
+// residual prices of the candidate factors against the DCT8 baseline
+// quantizer (lib.rs:3899-4014).
+__device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
+                         const ConstTables* __restrict__ tab, const ProbeParams* __restrict__ pp,
+                         const float4* __restrict__ hf_s, const uint16_t* __restrict__ hf_cell,
+                         int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
+                         double (*stage)[kCflStage]) {
   __shared__ int cands[2][4];
   __shared__ int ncand[2];
   __shared__ unsigned long long cost[2][4];
-  const int tile = blockIdx.x;
-  if (tile >= tiles) return;
   const uint32_t b0 = tile_base[tile], b1 = tile_base[tile + 1];
+  const CflSums a = cfl_sequential_sums(hf_s, b0, b1, stage);
   if (threadIdx.x == 0) {
-    double yy = 0.0, ycx = 0.0, ccx = 0.0, ycb = 0.0, ccb = 0.0;
-    for (uint32_t i = b0; i < b1; i++) {
-      const float4 s = hf_s[i];
-      const double y = (double)s.y, x = (double)s.x, b = (double)s.z;
-      yy += y * y;
-      ycx += y * x;
-      ccx += x * x;
-      ycb += y * b;
-      ccb += b * b;
-    }
     // both channels share s_yy (same sample sequence)
-    seeds[0] = b1 > b0 ? best_factor_dev(yy, ycx, ccx, 0.0f) : 0;
-    seeds[1] = b1 > b0 ? best_factor_dev(yy, ycb, ccb, 1.0f) : 0;
-    for (int ch = 0; ch < 2; ch++) {
-      ncand[ch] = seeds[ch] == 0 ? 0 : factor_candidates(seeds[ch], cands[ch]);
+    const int seed_x = b1 > b0 ? best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f) : 0;
+    const int seed_b = b1 > b0 ? best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f) : 0;
+    ncand[0] = seed_x == 0 ? 0 : factor_candidates(seed_x, cands[0]);
+    ncand[1] = seed_b == 0 ? 0 : factor_candidates(seed_b, cands[1]);
+    for (int ch = 0; ch < 2; ch++)
       for (int i = 0; i < 4; i++) cost[ch][i] = 0;
-    }
   }
   __syncthreads();
   const float sx = pp->sc[0], sb = pp->sc[2];
   unsigned long long local[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
   int dummy = 0;
-  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) {
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
     const float4 s = hf_s[i];
     const int cell = hf_cell[i];
     for (int ch = 0; ch < 2; ch++) {
@@ -448,24 +517,16 @@ __global__ __launch_bounds__(256) void k_cfl_tiles(const uint32_t* __restrict__ 
 }
 
 // The frame-wide LF factors (lib.rs:3831-3894 refine_lf_factors).
-__global__ __launch_bounds__(256) void k_cfl_lf(uint32_t count, const ProbeParams* __restrict__ pp,
-                                                const float4* __restrict__ lf_s, CflParams* __restrict__ out) {
+__device__ void cfl_frame_lf(uint32_t count, const ProbeParams* __restrict__ pp,
+                             const float4* __restrict__ lf_s, CflParams* __restrict__ out,
+                             double (*stage)[kCflStage]) {
   __shared__ int cands[2][4];
   __shared__ int ncand[2];
   __shared__ unsigned long long cost[2][4];
+  const CflSums a = cfl_sequential_sums(lf_s, 0, count, stage);
   if (threadIdx.x == 0) {
-    double yy = 0.0, ycx = 0.0, ccx = 0.0, ycb = 0.0, ccb = 0.0;
-    for (uint32_t i = 0; i < count; i++) {
-      const float4 s = lf_s[i];
-      const double y = (double)s.y, x = (double)s.x, b = (double)s.z;
-      yy += y * y;
-      ycx += y * x;
-      ccx += x * x;
-      ycb += y * b;
-      ccb += b * b;
-    }
-    ncand[0] = factor_candidates(best_factor_dev(yy, ycx, ccx, 0.0f), cands[0]);
-    ncand[1] = factor_candidates(best_factor_dev(yy, ycb, ccb, 1.0f), cands[1]);
+    ncand[0] = factor_candidates(best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f), cands[0]);
+    ncand[1] = factor_candidates(best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f), cands[1]);
     for (int ch = 0; ch < 2; ch++)
       for (int j = 0; j < 4; j++) cost[ch][j] = 0;
   }
@@ -473,7 +534,7 @@ __global__ __launch_bounds__(256) void k_cfl_lf(uint32_t count, const ProbeParam
   const float mdc[2] = {pp->mdc[0], pp->mdc[2]};
   unsigned long long local[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
   int dummy = 0;
-  for (uint32_t i = threadIdx.x; i < count; i += 256) {
+  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
     const float4 s = lf_s[i];
     for (int ch = 0; ch < 2; ch++) {
       const float base = ch == 0 ? 0.0f : 1.0f, src = ch == 0 ? s.x : s.z;
@@ -511,6 +572,24 @@ __global__ __launch_bounds__(256) void k_cfl_lf(uint32_t count, const ProbeParam
   }
 }
 
+// Workgroup 0 takes the frame's LF regression (the longest sequential sum),
+// so it runs beside the per-tile workgroups 1..tiles instead of after them.
+__global__ __launch_bounds__(kCflThreads) void k_cfl_estimate(const uint32_t* __restrict__ tile_base, int tiles,
+                                                      uint32_t lf_count, const ConstTables* __restrict__ tab,
+                                                      const ProbeParams* __restrict__ pp,
+                                                      const float4* __restrict__ lf_s,
+                                                      const float4* __restrict__ hf_s,
+                                                      const uint16_t* __restrict__ hf_cell,
+                                                      int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
+                                                      CflParams* __restrict__ cflp) {
+  __shared__ double stage[5][kCflStage];
+  if (blockIdx.x == 0) {
+    cfl_frame_lf(lf_count, pp, lf_s, cflp, stage);
+  } else if ((int)blockIdx.x <= tiles) {
+    cfl_tile((int)blockIdx.x - 1, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage);
+  }
+}
+
 }  // namespace
 
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
@@ -538,8 +617,8 @@ void launch_cfl(hipStream_t s, const Geom& g, const ConstTables* tab, const Prob
   if (counts[1]) k_cfl_collect<16><<<counts[1], 64, 0, s>>>(g, items[1], counts[1], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
   if (counts[2]) k_cfl_collect<32><<<counts[2], 64, 0, s>>>(g, items[2], counts[2], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
   const int tiles = g.tw * g.th;
-  k_cfl_tiles<<<tiles, 256, 0, s>>>(tile_base, tiles, tab, pp, hf_s, hf_cell, cfl_x, cfl_b);
-  k_cfl_lf<<<1, 256, 0, s>>>(lf_count, pp, lf_s, cflp);
+  k_cfl_estimate<<<tiles + 1, kCflThreads, 0, s>>>(tile_base, tiles, lf_count, tab, pp, lf_s, hf_s, hf_cell,
+                                          cfl_x, cfl_b, cflp);
 }
 
 }  // namespace jpxlg
