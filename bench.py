@@ -36,29 +36,45 @@ def hip():
     return lib
 
 
-def algorithmic_bytes(name: str, w: int, h: int, nvb: dict) -> float | None:
-    """Bytes each launch of a kernel family must move (inputs read once,
-    outputs written once), for the 2048x2048 single-image workload."""
+def pyramid_pixels(w: int, h: int) -> int:
+    """Pixels over the metric's scales (halving while both sides are >= 8)."""
+    total, s = 0, 0
+    while s < 6:
+        if s > 0:
+            if w < 8 or h < 8:
+                break
+            w, h = (w + 1) // 2, (h + 1) // 2
+        total += w * h
+        s += 1
+    return total
+
+
+def algorithmic_bytes(name: str, w: int, h: int, sizes: dict) -> float | None:
+    """Bytes one launch of a kernel must move (each declared input read once,
+    each output written once) for one w x h image whose block map has
+    `sizes[s]` varblocks of side s."""
     px = w * h
     blocks = ((w + 7) // 8) * ((h + 7) // 8)
+    for side in (8, 16, 32):
+        cov = sizes.get(side, 0) * side * side          # pixels covered by this size
+        nb = sizes.get(side, 0) * (side // 8) ** 2      # 8x8 blocks covered
+        if name == f"quantize_{side}":                  # f32 coefficients in, i32 out, i32 LF out
+            return 24.0 * cov + 12.0 * nb
+        if name == f"render_{side}":                    # i32 in + LF in, f32 XYB out, sigma out
+            return 24.0 * cov + 16.0 * nb
+        if name == f"dct_fwd_{side}":                   # XYB in, f32 coefficients out
+            return 24.0 * cov
     per_px = {
-        "quantize_8": 24.0,       # f32 coefficients in, i32 coefficients out (3 channels)
-        "render_8": 24.0,         # i32 coefficients in, f32 XYB out
-        "epf_step0": 24.0,
-        "epf_step1": 24.0,
-        "epf_step2": 24.0,
+        "xyb_fwd": 6.0 + 24.0,                # RGB16 in; XYB + linear RGB out
+        "epf_step0": 24.0, "epf_step1": 24.0, "epf_step2": 24.0,
         "gaborish": 24.0,
-        "dct_fwd_8": 24.0,        # XYB in, f32 coefficients out
-        "xyb_inv_depth_metric0": 27.0,  # XYB in, positive XYB out, 1/4-size RGB out
-        "blur_pool_s0": 48.0,     # candidate XYB' + reference XYB', mu1, s11 in
+        "xyb_inv_depth_metric0": 27.0,        # XYB in, positive XYB out, 1/4-size RGB out
     }
     if name in per_px:
-        extra = 0.0
-        if name in ("quantize_8", "render_8"):
-            extra = blocks * 12.0 + (blocks * 4.0 if name == "render_8" else 0.0)
-        if name.startswith("epf"):
-            extra = blocks * 4.0
+        extra = blocks * 4.0 if name.startswith("epf") else 0.0
         return per_px[name] * px + extra
+    if name == "blur_pool":                   # every scale: candidate XYB' + reference XYB', mu1, s11
+        return 48.0 * pyramid_pixels(w, h)
     return None
 
 
@@ -138,9 +154,15 @@ def main() -> int:
 
     ctx = jpxl_amd.Context(0)
 
+    sizes = {}
+
     def step():
         img = jpxl_amd.Image(ctx, None, BITS, device_ptr=dptr.value, shape=(HEIGHT, WIDTH))
         res = img.encode_quality(TARGET, args.effort)
+        if not sizes:
+            bm = img.plan_download_map()
+            for side in (8, 16, 32):
+                sizes[side] = int((bm == side).sum())
         img.close()
         return res
 
@@ -161,11 +183,10 @@ def main() -> int:
     value = mp / (elapsed / args.steps)
 
     # Dominant kernel by total time over the timed region.
-    nvb = {}
     rows = sorted(prof.items(), key=lambda kv: -kv[1][0])
     roofline = None
     for name, (ms, launches) in rows:
-        b = algorithmic_bytes(name, WIDTH, HEIGHT, nvb)
+        b = algorithmic_bytes(name, WIDTH, HEIGHT, sizes)
         if b is None:
             continue
         avg_s = ms / 1000.0 / launches
@@ -203,6 +224,7 @@ def main() -> int:
             "achieved_score": round(r0.score, 4),
             "chosen_global_scale": int(r0.global_scale),
             "chosen_hf_mul": int(r0.hf_mul),
+            "varblocks": {str(k): v for k, v in sizes.items()},
         },
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in rows},

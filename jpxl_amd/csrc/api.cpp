@@ -33,6 +33,10 @@ struct ProfRec {
 struct jpxlg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // A second stream for work that can overlap the main chain (forked and
+  // joined with events; both at the default priority).
+  hipStream_t side = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   bool prof = false;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
@@ -101,17 +105,18 @@ hipEvent_t take_event(jpxlg_ctx* ctx) {
 struct Prof {
   jpxlg_ctx* ctx;
   const char* name;
+  hipStream_t st;
   hipEvent_t a = nullptr;
-  Prof(jpxlg_ctx* c, const char* n) : ctx(c), name(n) {
+  Prof(jpxlg_ctx* c, const char* n, hipStream_t s = nullptr) : ctx(c), name(n), st(s ? s : c->stream) {
     if (ctx->prof) {
       a = take_event(ctx);
-      (void)hipEventRecord(a, ctx->stream);
+      (void)hipEventRecord(a, st);
     }
   }
   ~Prof() {
     if (a) {
       hipEvent_t b = take_event(ctx);
-      (void)hipEventRecord(b, ctx->stream);
+      (void)hipEventRecord(b, st);
       ctx->recs.push_back({name, a, b});
     }
   }
@@ -220,6 +225,7 @@ struct jpxlg_img {
   float* cand_x[kMaxScales][3];
   float* cand_rgb[kMaxScales][3];  // [0] is the reference's linear RGB scratch
   double* d_partials = nullptr;
+  CflLfScratch* d_lfc = nullptr;
   double* d_sums = nullptr;
   double* d_cost[3] = {nullptr, nullptr, nullptr};
   float4* d_lf_s = nullptr;
@@ -256,6 +262,9 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   ctx->device = device;
   HIP_TRY(ctx.get(), hipSetDevice(device));
   HIP_TRY(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  HIP_TRY(ctx.get(), hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
+  HIP_TRY(ctx.get(), hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+  HIP_TRY(ctx.get(), hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
   ConstTables host;
   build_const_tables(&host);
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_tab, sizeof(ConstTables)));
@@ -282,6 +291,10 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   for (auto& kv : ctx->spare) (void)hipFree(kv.second);
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
+  if (ctx->side) (void)hipStreamSynchronize(ctx->side);
+  if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+  if (ctx->join) (void)hipEventDestroy(ctx->join);
+  if (ctx->side) (void)hipStreamDestroy(ctx->side);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
@@ -389,6 +402,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
   img->d_cfl_b = (int8_t*)A.take(tiles_cfl);
   img->d_cflp = (CflParams*)A.take(sizeof(CflParams));
+  img->d_lfc = (CflLfScratch*)A.take(sizeof(CflLfScratch));
   img->d_pp = (ProbeParams*)A.take(sizeof(ProbeParams));
   img->d_err = (uint32_t*)A.take(4);
   img->d_gray = (int*)A.take(4);
@@ -398,7 +412,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
                 img->d_cfl_items[2] && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
-                img->d_cflp && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums;
+                img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums;
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
              img->d_list[c] && img->rx[c] && img->aux[c];
@@ -606,10 +620,25 @@ static int estimate_cfl(jpxlg_img* img, const jpxlg_quantizer* q) {
   }
   HIP_TRY(ctx, hipMemcpyAsync(img->d_tile_base, tile_base.data(), tile_base.size() * 4, hipMemcpyHostToDevice, st));
   {
-    Prof p(ctx, "cfl");
-    launch_cfl(st, g, ctx->d_tab, img->d_pp, ditems, counts, img->d_tile_base, lf_off, img->coeff,
-               img->d_lf_s, img->d_hf_s, img->d_hf_cell, img->d_cfl_x, img->d_cfl_b, img->d_cflp, img->d_err);
+    Prof p(ctx, "cfl_collect");
+    launch_cfl_collect(st, g, ctx->d_tab, img->d_pp, ditems, counts, img->coeff, img->d_lf_s, img->d_hf_s,
+                       img->d_hf_cell, img->d_err);
   }
+  // The frame-LF regression (one long sequential sum) runs on the side
+  // stream beside the per-tile regressions.
+  HIP_TRY(ctx, hipEventRecord(ctx->fork, st));
+  HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+  {
+    Prof p(ctx, "cfl_lf", ctx->side);
+    launch_cfl_lf(ctx->side, img->d_pp, lf_off, img->d_lf_s, img->d_lfc, img->d_cflp);
+  }
+  HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->side));
+  {
+    Prof p(ctx, "cfl_tiles");
+    launch_cfl_tiles(st, g, ctx->d_tab, img->d_pp, img->d_tile_base, img->d_hf_s, img->d_hf_cell, img->d_cfl_x,
+                     img->d_cfl_b);
+  }
+  HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join, 0));
   HIP_TRY(ctx, hipGetLastError());
   CflParams cp;
   uint32_t err = 0;
@@ -754,9 +783,8 @@ static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
   DepthSet d;
   int rc = get_depth(ctx, img->bits, &d);
   if (rc) return rc;
-  PoolLayout L{};
-  L.nscales = img->nscales;
-  int ptiles = 0;
+  PoolScales P{};
+  P.L.nscales = img->nscales;
   for (int s = 0; s < img->nscales; s++) {
     float** down = s + 1 < img->nscales ? img->cand_rgb[s + 1] : nullptr;
     if (s == 0) {
@@ -768,15 +796,17 @@ static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
       launch_colour_metric(st, 1, img->sw[s], img->sh[s], img->cand_rgb[s], nullptr, nullptr, 0,
                            img->cand_x[s], down);
     }
-    {
-      Prof p(ctx, s == 0 ? "blur_pool_s0" : "blur_pool");
-      L.off[s] = ptiles;
-      L.tiles[s] = launch_blur_pool(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s], img->ref_mu[s],
-                                    img->ref_s11[s], img->cand_x[s], img->d_partials + (size_t)ptiles * 18);
-      ptiles += L.tiles[s];
+    P.w[s] = img->sw[s];
+    P.h[s] = img->sh[s];
+    for (int c = 0; c < 3; c++) {
+      P.x1[s][c] = img->ref_x[s][c];
+      P.mu1[s][c] = img->ref_mu[s][c];
+      P.s11[s][c] = img->ref_s11[s][c];
+      P.x2[s][c] = img->cand_x[s][c];
     }
   }
-  { Prof p(ctx, "pool_reduce"); launch_reduce_scales(st, L, img->d_partials, img->d_sums); }
+  { Prof p(ctx, "blur_pool"); launch_blur_pool(st, ctx->d_tab, P, img->d_partials); }
+  { Prof p(ctx, "pool_reduce"); launch_reduce_scales(st, P.L, img->d_partials, img->d_sums); }
   HIP_TRY(ctx, hipGetLastError());
   return JPXLG_OK;
 }
@@ -1058,21 +1088,25 @@ extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t hei
     HIP_TRY(ctx, hipMemcpyAsync(rrgb[0][c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
     HIP_TRY(ctx, hipMemcpyAsync(crgb[0][c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
   }
-  PoolLayout L{};
-  L.nscales = n;
-  int used = 0;
+  PoolScales P{};
+  P.L.nscales = n;
   for (int s = 0; s < n; s++) {
     float** rd = s + 1 < n ? rrgb[s + 1] : nullptr;
     float** cd = s + 1 < n ? crgb[s + 1] : nullptr;
     launch_colour_metric(st, 1, sw[s], sh[s], rrgb[s], nullptr, nullptr, 0, rx[s], rd);
     launch_colour_metric(st, 1, sw[s], sh[s], crgb[s], nullptr, nullptr, 0, cx[s], cd);
     launch_blur_moments(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s]);
-    L.off[s] = used;
-    L.tiles[s] = launch_blur_pool(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s], cx[s],
-                                  partials + (size_t)used * 18);
-    used += L.tiles[s];
+    P.w[s] = sw[s];
+    P.h[s] = sh[s];
+    for (int c = 0; c < 3; c++) {
+      P.x1[s][c] = rx[s][c];
+      P.mu1[s][c] = mu[s][c];
+      P.s11[s][c] = s11[s][c];
+      P.x2[s][c] = cx[s][c];
+    }
   }
-  launch_reduce_scales(st, L, partials, sums_d);
+  launch_blur_pool(st, ctx->d_tab, P, partials);
+  launch_reduce_scales(st, P.L, partials, sums_d);
   HIP_TRY(ctx, hipGetLastError());
   double sums[kMaxScales * 18];
   HIP_TRY(ctx, hipMemcpyAsync(sums, sums_d, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));
@@ -1099,6 +1133,7 @@ extern "C" int32_t jpxlg_profile_read(jpxlg_ctx* ctx, char* names, double* total
   if (!ctx) return JPXLG_E_INVALID;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
   for (auto& r : ctx->recs) {
     float ms = 0.0f;
     HIP_TRY(ctx, hipEventElapsedTime(&ms, r.a, r.b));

@@ -47,19 +47,23 @@ __device__ __forceinline__ float cbrt_core(float value) {
   return (float)t;
 }
 
-// jpxl-perceptual/src/color.rs:46-61 reproducible_cbrt (metric copy: double
-// bit-hack estimate, four f64 Newton steps).
+// The metric's cube root (jpxl-perceptual/src/color.rs:46-61): the reference
+// rounds to f32 the f64 root its bit-hack estimate reaches after four Newton
+// steps, i.e. the f64 cube root to within an ulp or two.  Here one Halley
+// step in f64 from the f32 library root (relative error < 2^-22, cubed by the
+// step to far below f64 precision) reaches the same f64 root to within a few
+// ulps, so the two f32 results can differ only when the true root lies within
+// a few f64 ulps of an f32 rounding boundary (about one sample in 2^27),
+// and then by one f32 ulp.  It spends one f64 division where the
+// reference's recursion spends four.
 __device__ __forceinline__ float cbrt_metric(float value) {
-  double x = (double)value;
-  double a = fabs(x);
-  unsigned long long ab = (unsigned long long)__double_as_longlong(a);
-  double y = __longlong_as_double((long long)(ab / 3ull + (0x2A9F7893ull << 32)));
-#pragma unroll
-  for (int i = 0; i < 4; i++) y -= (y * y * y - a) / (3.0 * y * y);
-  float root = (float)y;
-  float sgn = x < 0.0 ? -root : root;
   if (value == 0.0f || !isfinite(value)) return value;
-  return sgn;
+  const double a = fabs((double)value);
+  const double y = (double)cbrtf(fabsf(value));
+  const double y3 = y * y * y;
+  const double r = y * ((y3 + 2.0 * a) / (2.0 * y3 + a));
+  const float root = (float)r;
+  return value < 0.0f ? -root : root;
 }
 
 // ---------------------------------------------------------------------------
@@ -191,6 +195,27 @@ __device__ __forceinline__ int mirror1d(int c, int size) {
     else if (c >= size) c = 2 * size - 1 - c;
     else return c;
   }
+}
+
+// acc + p[0] + p[1] + ... + p[m-1], each addition rounded in that order (a
+// sequential f64 sum), by one lane.  Loads are issued 32 at a time ahead of
+// the additions that consume them, and the wave raises its issue priority
+// for the chain so that the other waves sharing its SIMD do not stretch the
+// dependent additions.
+__device__ __forceinline__ double sequential_sum(const double* p, int m, double acc) {
+  constexpr int kU = 32;
+  __builtin_amdgcn_s_setprio(3);
+  int i = 0;
+  for (; i + kU <= m; i += kU) {
+    double t[kU];
+#pragma unroll
+    for (int k = 0; k < kU; k++) t[k] = p[i + k];
+#pragma unroll
+    for (int k = 0; k < kU; k++) acc += t[k];
+  }
+  for (; i < m; i++) acc += p[i];
+  __builtin_amdgcn_s_setprio(0);
+  return acc;
 }
 
 }  // namespace jpxlg

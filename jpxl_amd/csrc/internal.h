@@ -53,6 +53,14 @@ struct CflParams {
   int32_t x_factor_lf, b_factor_lf;  // unbiased (-128..127)
 };
 
+// Device scratch of the frame-LF CfL refinement: the candidate factors of
+// each channel and their residual-bit prices.
+struct CflLfScratch {
+  int32_t cands[2][4];
+  int32_t ncand[2];
+  unsigned long long cost[2][4];
+};
+
 // Error word bits raised by kernels (device memory).
 enum : uint32_t { kErrQuantRange = 1u };
 
@@ -91,32 +99,49 @@ void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], con
                             const float* depth_lut, int levels, float* out[3]);
 void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
                          float* mu1[3], float* s11[3]);
-// Blurs the candidate's moments and pools all three maps; writes 6 partial
-// f64 sums per (tile, channel) into `partials` and returns the tile count.
-int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
-                     float* const mu1[3], float* const s11[3], float* const x2[3],
-                     double* partials);
 // Hierarchical cover (kernels_plan.hip): candidate costs for every square
 // (f64, indexed by the candidate's top-left block) and the per-region
 // quadtree decision written into the block map.
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
                   float* const xyb[3], double* cost8, double* cost16, double* cost32, uint8_t* bmap,
                   uint32_t* err);
-// Chroma-from-luma estimate: per-size varblock items (x = (by << 16) | bx,
-// y = LF sample slot, z = HF sample slot), per-tile HF sample ranges, and the
-// outputs (tile factors, LF factors) on the device.
-void launch_cfl(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
-                const uint4* const items[3], const int counts[3], const uint32_t* tile_base,
-                uint32_t lf_count, float* const coeff[3], float4* lf_s, float4* hf_s, uint16_t* hf_cell,
-                int8_t* cfl_x, int8_t* cfl_b, CflParams* cflp, uint32_t* err);
+// Chroma-from-luma samples: per-size varblock items (x = (by << 16) | bx,
+// y = LF sample slot, z = HF sample slot) gather the LF and HF samples in
+// the reference's accumulation order.
+void launch_cfl_collect(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                        const uint4* const items[3], const int counts[3], float* const coeff[3],
+                        float4* lf_s, float4* hf_s, uint16_t* hf_cell, uint32_t* err);
+// The per-64x64-tile HF factors from each tile's sample range.
+void launch_cfl_tiles(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                      const uint32_t* tile_base, const float4* hf_s, const uint16_t* hf_cell,
+                      int8_t* cfl_x, int8_t* cfl_b);
+// The frame-LF factors, on their own stream: sequential regression, then a
+// grid-wide pricing of the candidates, then the selection.
+void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, const float4* lf_s,
+                   CflLfScratch* scratch, CflParams* cflp);
 // Pooling tiles of one scale (one partial row of 18 sums per tile).
 int pool_tiles(int w, int h);
+constexpr int kMaxPoolScales = 6;
 // Where each scale's tiles start in the partials buffer (in tiles).
 struct PoolLayout {
   int nscales;
-  int off[6];
-  int tiles[6];
+  int off[kMaxPoolScales];
+  int tiles[kMaxPoolScales];
 };
+// Every scale of one candidate: sizes and the reference (x1, mu1, s11) and
+// candidate (x2) positive-XYB planes per scale and channel.
+struct PoolScales {
+  PoolLayout L;
+  int w[kMaxPoolScales], h[kMaxPoolScales], tx[kMaxPoolScales];
+  const float* x1[kMaxPoolScales][3];
+  const float* mu1[kMaxPoolScales][3];
+  const float* s11[kMaxPoolScales][3];
+  const float* x2[kMaxPoolScales][3];
+};
+// The candidate blurs, SSIM/edge maps and per-tile f64 partial sums of every
+// scale in one launch (18 partial sums per tile; tile offsets and counts are
+// filled into P.L).
+void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, double* partials);
 // Sums every scale's partials into 18 f64 sums per scale, in a fixed order.
 void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums);
 

@@ -230,28 +230,37 @@ __device__ __forceinline__ void fir_run(const double (&d)[R + 8], const double* 
   }
 }
 
-__global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTables* __restrict__ tab,
-                                                   const float* __restrict__ a0, const float* __restrict__ a1,
-                                                   const float* __restrict__ a2, const float* __restrict__ m0,
-                                                   const float* __restrict__ m1, const float* __restrict__ m2,
-                                                   const float* __restrict__ v0, const float* __restrict__ v1,
-                                                   const float* __restrict__ v2, const float* __restrict__ b0,
-                                                   const float* __restrict__ b1, const float* __restrict__ b2,
+__global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict__ tab, PoolScales P,
                                                    double* __restrict__ partials) {
   __shared__ float t1[kPIH][kPIW];
   __shared__ float t2[kPIH][kPIW];
   __shared__ float hb[3][kPIH][kPW];
   __shared__ double red[4][6];
   const int c = blockIdx.z;
-  const int x0 = blockIdx.x * kPW, y0 = blockIdx.y * kPH;
-  const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+  const int tile = blockIdx.x;
+  int sc = 0;
+#pragma unroll
+  for (int i = 1; i < kMaxPoolScales; i++)
+    if (i < P.L.nscales && tile >= P.L.off[i]) sc = i;
+  const int lt = tile - P.L.off[sc], w = P.w[sc], h = P.h[sc];
+  const int x0 = (lt % P.tx[sc]) * kPW, y0 = (lt / P.tx[sc]) * kPH;
   double hf[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
-  const float* X1 = c == 0 ? a0 : (c == 1 ? a1 : a2);
-  const float* MU1 = c == 0 ? m0 : (c == 1 ? m1 : m2);
-  const float* S11 = c == 0 ? v0 : (c == 1 ? v1 : v2);
-  const float* X2 = c == 0 ? b0 : (c == 1 ? b1 : b2);
+  const float* X1 = P.x1[sc][c];
+  const float* MU1 = P.mu1[sc][c];
+  const float* S11 = P.s11[sc][c];
+  const float* X2 = P.x2[sc][c];
+  // The pooling inputs do not depend on the blurs: issue their loads first.
+  const int plx = threadIdx.x & 63, ply0 = (threadIdx.x >> 6) * kVRun;
+  float pmu[kVRun], ps11[kVRun];
+#pragma unroll
+  for (int i = 0; i < kVRun; i++) {
+    const int gx = x0 + plx, gy = y0 + ply0 + i;
+    const bool in = gx < w && gy < h;
+    pmu[i] = in ? MU1[(size_t)gy * w + gx] : 0.0f;
+    ps11[i] = in ? S11[(size_t)gy * w + gx] : 0.0f;
+  }
   for (int e = threadIdx.x; e < kPIH * kPIW; e += 256) {
     const int ly = e / kPIW, lx = e % kPIW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
     const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
@@ -316,9 +325,8 @@ __global__ __launch_bounds__(256) void k_blur_pool(int w, int h, const ConstTabl
   for (int i = 0; i < kVRun; i++) {
     const int gy = y0 + ly0 + i;
     if (gx >= w || gy >= h) continue;
-    const size_t o = (size_t)gy * w + gx;
     const float i1 = t1[ly0 + i + kHalo][lx + kHalo], i2 = t2[ly0 + i + kHalo][lx + kHalo];
-    const float mu1 = MU1[o], s11 = S11[o];
+    const float mu1 = pmu[i], s11 = ps11[i];
     const float mu11 = mu1 * mu1, mu22 = mu2[i] * mu2[i], mu12 = mu1 * mu2[i];
     const float md = mu1 - mu2[i];
     const float num_m = 1.0f - md * md;
@@ -400,13 +408,16 @@ void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, fl
 
 int pool_tiles(int w, int h) { return (int)(div_up(w, kPW) * div_up(h, kPH)); }
 
-int launch_blur_pool(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
-                     float* const mu1[3], float* const s11[3], float* const x2[3],
-                     double* partials) {
-  dim3 grid(div_up(w, kPW), div_up(h, kPH), 3);
-  k_blur_pool<<<grid, 256, 0, s>>>(w, h, tab, x1[0], x1[1], x1[2], mu1[0], mu1[1], mu1[2], s11[0],
-                                   s11[1], s11[2], x2[0], x2[1], x2[2], partials);
-  return (int)(grid.x * grid.y);
+void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, double* partials) {
+  int total = 0;
+  for (int i = 0; i < P.L.nscales; i++) {
+    P.tx[i] = (int)div_up(P.w[i], kPW);
+    P.L.off[i] = total;
+    P.L.tiles[i] = pool_tiles(P.w[i], P.h[i]);
+    total += P.L.tiles[i];
+  }
+  dim3 grid(total, 1, 3);
+  k_blur_pool<<<grid, 256, 0, s>>>(tab, P, partials);
 }
 
 void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums) {

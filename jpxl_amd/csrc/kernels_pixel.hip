@@ -387,6 +387,7 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(
   __shared__ int32_t qs[3][VB][NN];
   __shared__ float dyh[VB][NN];
   __shared__ float dylf[VB][16];
+  __shared__ float lfs[3][VB][16], lfo[3][VB][16];
   __shared__ float khf[2][VB];
   __shared__ int vbx[VB], vby[VB];
   const int base = blockIdx.x * VB;
@@ -412,25 +413,47 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(
   __syncthreads();
   const DctTables& t = tab->dct;
   const float* sf = tab->scale_f[si];
-  const float mdc0 = pp->mdc[0], mdc1 = pp->mdc[1], mdc2 = pp->mdc[2];
-  // --- Y LF (one thread per varblock) ---
-  if (threadIdx.x < nvb) {
-    const int j = threadIdx.x;
-    float lf[n * n];
+  const float mdc[3] = {pp->mdc[0], pp->mdc[1], pp->mdc[2]};
+  // --- LF of all three channels: the n x n inverse DCT of the LLF cells
+  // (forward.rs:779-830), one lane per line in each of its two passes, with
+  // the operation order of small_dct2_inv ---
+  if constexpr (n == 1) {
+    if (threadIdx.x < 3 * VB) {
+      const int c = threadIdx.x / VB, j = threadIdx.x % VB;
+      lfo[c][j][0] = co[c][j][0];
+    }
+  } else {
+    const float f = inv_factor<n>();
+    if (threadIdx.x < 3 * VB * n) {
+      const int c = threadIdx.x / (VB * n), j = (threadIdx.x / n) % VB, r = threadIdx.x % n;
+      float v[n];
 #pragma unroll
-    for (int y = 0; y < n; y++)
+      for (int k = 0; k < n; k++) v[k] = co[c][j][k * N + r] / (sf[k] * sf[r]);
+      dct_iii<n>(v, t);
 #pragma unroll
-      for (int x = 0; x < n; x++)
-        lf[y * n + x] = n == 1 ? co[1][j][0] : co[1][j][y * N + x] / (sf[y] * sf[x]);
-    small_dct2_inv<n>(lf, t);
+      for (int k = 0; k < n; k++) lfs[c][j][k * n + r] = v[k] * f;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * VB * n) {
+      const int c = threadIdx.x / (VB * n), j = (threadIdx.x / n) % VB, col = threadIdx.x % n;
+      float v[n];
 #pragma unroll
-    for (int i = 0; i < n * n; i++) {
-      int32_t v = clamp_round(lf[i] * 1.0f / mdc1, &bad);
-      LY[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = v;
-      dylf[j][i] = mdc1 * (float)v / 1.0f;
+      for (int r = 0; r < n; r++) v[r] = lfs[c][j][col * n + r];
+      dct_iii<n>(v, t);
+#pragma unroll
+      for (int r = 0; r < n; r++) lfo[c][j][r * n + col] = v[r] * f;
     }
   }
-  // --- Y HF ---
+  __syncthreads();
+  // --- Y LF quantize, and Y HF ---
+  for (int e = threadIdx.x; e < VB * n * n; e += T) {
+    const int j = e / (n * n), i = e % (n * n);
+    if (j < nvb) {
+      int32_t v = clamp_round(lfo[1][j][i] * 1.0f / mdc[1], &bad);
+      LY[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = v;
+      dylf[j][i] = mdc[1] * (float)v / 1.0f;
+    }
+  }
   {
     const float sc = pp->sc[1], bias = kQuantBias[1];
     for (int e = threadIdx.x; e < VB * NN; e += T) {
@@ -448,25 +471,16 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(
     for (int j = wave; j < nvb; j += T / 64)
       truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
                        tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
-  __syncthreads();
-  // --- X and B LF (one thread per varblock channel) ---
-  if (threadIdx.x < 2 * VB && (threadIdx.x % VB) < nvb) {
-    const int ci = threadIdx.x / VB, j = threadIdx.x % VB, c = ci == 0 ? 0 : 2;
-    const float klf = ci == 0 ? 0.0f + (float)cflp->x_factor_lf / 84.0f
-                              : 1.0f + (float)cflp->b_factor_lf / 84.0f;
-    const float mdc = ci == 0 ? mdc0 : mdc2;
-    float lf[n * n];
-#pragma unroll
-    for (int y = 0; y < n; y++)
-#pragma unroll
-      for (int x = 0; x < n; x++)
-        lf[y * n + x] = n == 1 ? co[c][j][0] : co[c][j][y * N + x] / (sf[y] * sf[x]);
-    small_dct2_inv<n>(lf, t);
-    int32_t* L = ci == 0 ? LX : LB;
-#pragma unroll
-    for (int i = 0; i < n * n; i++) {
-      float target = lf[i] - klf * dylf[j][i];
-      L[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = clamp_round(target * 1.0f / mdc, &bad);
+  // --- X and B LF against the reconstructed Y LF ---
+  {
+    const float klf[2] = {0.0f + (float)cflp->x_factor_lf / 84.0f, 1.0f + (float)cflp->b_factor_lf / 84.0f};
+    for (int e = threadIdx.x; e < 2 * VB * n * n; e += T) {
+      const int ci = e / (VB * n * n), j = (e / (n * n)) % VB, i = e % (n * n), c = ci == 0 ? 0 : 2;
+      if (j < nvb) {
+        const float target = lfo[c][j][i] - klf[ci] * dylf[j][i];
+        int32_t* L = ci == 0 ? LX : LB;
+        L[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = clamp_round(target * 1.0f / mdc[c], &bad);
+      }
     }
   }
   __syncthreads();
@@ -513,6 +527,7 @@ __global__ __launch_bounds__(kThreads) void k_render(
     float* __restrict__ X, float* __restrict__ Y, float* __restrict__ B, float* __restrict__ sigma) {
   constexpr int VB = 64 / N, P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
   __shared__ float lds[VB * 3 * N * P];
+  __shared__ float llf[VB][3][16];
   __shared__ int vbx[VB], vby[VB];
   __shared__ float khf[2][VB];
   const int base = blockIdx.x * VB;
@@ -545,33 +560,56 @@ __global__ __launch_bounds__(kThreads) void k_render(
   }
   __syncthreads();
   const DctTables& t = tab->dct;
-  // I.8: the LLF cells from the dequantized, LF-CfL-corrected LF planes.
-  if (threadIdx.x < 3 * VB) {
-    const int j = threadIdx.x / 3, c = threadIdx.x % 3;
-    if (j < nvb) {
-      const float kxl = 0.0f + (float)cflp->x_factor_lf / 84.0f;
-      const float kbl = 1.0f + (float)cflp->b_factor_lf / 84.0f;
-      float lf[n * n];
-#pragma unroll
-      for (int dy = 0; dy < n; dy++)
-#pragma unroll
-        for (int dx = 0; dx < n; dx++) {
-          size_t o = (size_t)(vby[j] + dy) * g.bw + vbx[j] + dx;
-          float vy = pp->mdc[1] * (float)LY[o] / 1.0f;
+  // I.8: the LLF cells from the dequantized, LF-CfL-corrected LF planes
+  // (varblock.rs:817-855): the n x n DCT_2D of small_dct2_fwd, one lane per
+  // line in each pass, then ScaleF.
+  {
+    const float kxl = 0.0f + (float)cflp->x_factor_lf / 84.0f;
+    const float kbl = 1.0f + (float)cflp->b_factor_lf / 84.0f;
+    const float* sf = tab->scale_f[si];
+    if constexpr (n == 1) {
+      if (threadIdx.x < 3 * VB) {
+        const int j = threadIdx.x / 3, c = threadIdx.x % 3;
+        if (j < nvb) {
+          const size_t o = (size_t)vby[j] * g.bw + vbx[j];
+          const float vy = pp->mdc[1] * (float)LY[o] / 1.0f;
           float v;
           if (c == 1) v = vy;
           else if (c == 0) v = pp->mdc[0] * (float)LX[o] / 1.0f + kxl * vy;
           else v = pp->mdc[2] * (float)LB[o] / 1.0f + kbl * vy;
-          lf[dy * n + dx] = v;
+          lds[(j * 3 + c) * N * P] = v;
         }
-      small_dct2_fwd<n>(lf, t);
-      const float* sf = tab->scale_f[si];
-      float* tile = lds + (j * 3 + c) * N * P;
+      }
+    } else {
+      const float f = fwd_factor<n>();
+      const int j = threadIdx.x / (3 * n), c = (threadIdx.x / n) % 3, line = threadIdx.x % n;
+      const bool act = threadIdx.x < 3 * VB * n && j < nvb;
+      if (act) {
+        // pass 1: column `line` of the LF samples
+        float v[n];
 #pragma unroll
-      for (int y = 0; y < n; y++)
+        for (int r = 0; r < n; r++) {
+          const size_t o = (size_t)(vby[j] + r) * g.bw + vbx[j] + line;
+          const float vy = pp->mdc[1] * (float)LY[o] / 1.0f;
+          if (c == 1) v[r] = vy;
+          else if (c == 0) v[r] = pp->mdc[0] * (float)LX[o] / 1.0f + kxl * vy;
+          else v[r] = pp->mdc[2] * (float)LB[o] / 1.0f + kbl * vy;
+        }
+        dct_ii<n>(v, t);
 #pragma unroll
-        for (int x = 0; x < n; x++)
-          tile[y * P + x] = n == 1 ? lf[0] : lf[y * n + x] * sf[y] * sf[x];
+        for (int r = 0; r < n; r++) llf[j][c][r * n + line] = v[r] * f;
+      }
+      __syncthreads();
+      if (act) {
+        // pass 2: row `line` of pass 1 -> cells (k, line)
+        float v[n];
+#pragma unroll
+        for (int k = 0; k < n; k++) v[k] = llf[j][c][line * n + k];
+        dct_ii<n>(v, t);
+        float* tile = lds + (j * 3 + c) * N * P;
+#pragma unroll
+        for (int k = 0; k < n; k++) tile[k * P + line] = v[k] * f * sf[k] * sf[line];
+      }
     }
   }
   __syncthreads();
@@ -641,27 +679,14 @@ __global__ __launch_bounds__(256) void k_gaborish(int w, int h, const float* __r
 // J.4 edge-preserving filter step (jpxl-core/src/restoration.rs:583-731)
 // ----------------------------------------------------------------------------
 
-constexpr int kEpfTW = 64, kEpfTH = 4, kEpfR = 3;
+// 64x16 output tile per 256-thread workgroup (each lane four rows), halo 3.
+constexpr int kEpfTW = 64, kEpfTH = 16, kEpfR = 3, kEpfRows = kEpfTH / 4;
 
-template <int STEP>
-__global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restrict__ in0,
-                                             const float* __restrict__ in1,
-                                             const float* __restrict__ in2,
-                                             const float* __restrict__ sigma,
-                                             float* __restrict__ o0, float* __restrict__ o1,
-                                             float* __restrict__ o2) {
-  constexpr int LW = kEpfTW + 2 * kEpfR, LH = kEpfTH + 2 * kEpfR;
-  __shared__ float tile[3][LH][LW];
-  const int x0 = blockIdx.x * kEpfTW, y0 = blockIdx.y * kEpfTH;
-  const float* in[3] = {in0, in1, in2};
-  for (int e = threadIdx.x; e < 3 * LH * LW; e += 256) {
-    int c = e / (LH * LW), rem = e % (LH * LW), ly = rem / LW, lx = rem % LW;
-    int sy = mirror1d(y0 + ly - kEpfR, h), sx = mirror1d(x0 + lx - kEpfR, w);
-    tile[c][ly][lx] = in[c][(size_t)sy * w + sx];
-  }
-  __syncthreads();
-  const int tx = threadIdx.x % kEpfTW, ty = threadIdx.x / kEpfTW;
-  const int x = x0 + tx, y = y0 + ty;
+template <int STEP, int LH, int LW>
+__device__ __forceinline__ void epf_pixel(int w, int h, int x, int y, int tx, int ty,
+                                          const float (&tile)[3][LH][LW], const float* __restrict__ sigma,
+                                          float* __restrict__ o0, float* __restrict__ o1,
+                                          float* __restrict__ o2) {
   if (x >= w || y >= h) return;
   const int bw = (w + 7) / 8;
   const size_t idx = (size_t)y * w + x;
@@ -716,6 +741,28 @@ __global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restri
   out[0][idx] = acc0 / sum_w;
   out[1][idx] = acc1 / sum_w;
   out[2][idx] = acc2 / sum_w;
+}
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restrict__ in0,
+                                             const float* __restrict__ in1,
+                                             const float* __restrict__ in2,
+                                             const float* __restrict__ sigma,
+                                             float* __restrict__ o0, float* __restrict__ o1,
+                                             float* __restrict__ o2) {
+  constexpr int LW = kEpfTW + 2 * kEpfR, LH = kEpfTH + 2 * kEpfR;
+  __shared__ float tile[3][LH][LW];
+  const int x0 = blockIdx.x * kEpfTW, y0 = blockIdx.y * kEpfTH;
+  const float* in[3] = {in0, in1, in2};
+  for (int e = threadIdx.x; e < 3 * LH * LW; e += 256) {
+    int c = e / (LH * LW), rem = e % (LH * LW), ly = rem / LW, lx = rem % LW;
+    int sy = mirror1d(y0 + ly - kEpfR, h), sx = mirror1d(x0 + lx - kEpfR, w);
+    tile[c][ly][lx] = in[c][(size_t)sy * w + sx];
+  }
+  __syncthreads();
+  const int tx = threadIdx.x % kEpfTW;
+  for (int ty = (threadIdx.x / kEpfTW) * kEpfRows; ty < (int)(threadIdx.x / kEpfTW + 1) * kEpfRows; ty++)
+    epf_pixel<STEP>(w, h, x0 + tx, y0 + ty, tx, ty, tile, sigma, o0, o1, o2);
 }
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }

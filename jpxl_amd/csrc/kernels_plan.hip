@@ -10,6 +10,7 @@
 // row-major over the non-LLF cells) so the f64 value is the reference's own.
 // One thread per 32x32 region then walks the quadtree: a merge wins only when
 // strictly cheaper than its split.
+#include <algorithm>
 #include <climits>
 
 #include "internal.h"
@@ -125,7 +126,10 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
     const double lw = pp->lambda[c] * tsd;
     for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
       int j = e / NN, k = e % NN;
-      if ((k % N) < n && (k / N) < n) continue;
+      if ((k % N) < n && (k / N) < n) {
+        terms[j][k] = 0.0;
+        continue;
+      }
       float coef = tile[((j * 3 + c) * N + k / N) * P + k % N];
       float target;
       if (c == 1) target = coef;
@@ -142,15 +146,8 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
     }
     __syncthreads();
     // Sequential f64 sum in cell order, one thread per candidate.
-    if (threadIdx.x < VB) {
-      const int j = threadIdx.x;
-      double s = sse[j];
-      for (int k = 0; k < NN; k++) {
-        if ((k % N) < n && (k / N) < n) continue;
-        s += terms[j][k];
-      }
-      sse[j] = s;
-    }
+    // The LLF cells hold +0.0, which leaves the (non-negative) sum unchanged.
+    if (threadIdx.x < VB) sse[threadIdx.x] = sequential_sum(terms[threadIdx.x], NN, sse[threadIdx.x]);
     __syncthreads();
   }
   if (threadIdx.x < VB && valid[threadIdx.x]) {
@@ -387,7 +384,6 @@ __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0
                                        double (*prod)[kCflStage]) {
   __shared__ double total[5];
   constexpr int kPer = (kCflStage + kCflThreads - 1) / kCflThreads;  // samples per thread per chunk
-  constexpr int kU = 16;                                             // chain unroll
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc = 0.0;
   // Each thread prefetches its share of the next chunk into registers while
@@ -419,30 +415,7 @@ __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0
       const uint32_t i = n0 + threadIdx.x + k * kCflThreads;
       if (n0 < b1 && i < b1 && threadIdx.x + k * kCflThreads < kCflStage) pre[k] = s[i];
     }
-    if (lane == 0 && wave < 5) {
-      const double* p = prod[wave];
-      uint32_t i = 0;
-      if (m >= kU) {
-        double cur[kU];
-#pragma unroll
-        for (int k = 0; k < kU; k++) cur[k] = p[k];
-        for (; i + kU <= m; i += kU) {
-          double nxt[kU];
-          const bool more = i + 2 * kU <= m;
-          if (more) {
-#pragma unroll
-            for (int k = 0; k < kU; k++) nxt[k] = p[i + kU + k];
-          }
-#pragma unroll
-          for (int k = 0; k < kU; k++) acc += cur[k];
-          if (more) {
-#pragma unroll
-            for (int k = 0; k < kU; k++) cur[k] = nxt[k];
-          }
-        }
-      }
-      for (; i < m; i++) acc += p[i];
-    }
+    if (lane == 0 && wave < 5) acc = sequential_sum(prod[wave], (int)m, acc);
     __syncthreads();
   }
   if (lane == 0 && wave < 5) total[wave] = acc;
@@ -516,25 +489,37 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
   }
 }
 
-// The frame-wide LF factors (lib.rs:3831-3894 refine_lf_factors).
-__device__ void cfl_frame_lf(uint32_t count, const ProbeParams* __restrict__ pp,
-                             const float4* __restrict__ lf_s, CflParams* __restrict__ out,
-                             double (*stage)[kCflStage]) {
-  __shared__ int cands[2][4];
-  __shared__ int ncand[2];
-  __shared__ unsigned long long cost[2][4];
+// The frame-wide LF factors (lib.rs:3831-3894 refine_lf_factors), in three
+// launches: the regression seeds (one workgroup, sequential sums), the
+// residual prices of every candidate (an order-free integer reduction over
+// the whole grid), and the joint selection.
+__global__ __launch_bounds__(kCflThreads) void k_cfl_lf_seed(uint32_t count, const float4* __restrict__ lf_s,
+                                                             CflLfScratch* __restrict__ sc) {
+  __shared__ double stage[5][kCflStage];
   const CflSums a = cfl_sequential_sums(lf_s, 0, count, stage);
   if (threadIdx.x == 0) {
-    ncand[0] = factor_candidates(best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f), cands[0]);
-    ncand[1] = factor_candidates(best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f), cands[1]);
+    sc->ncand[0] = factor_candidates(best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f), sc->cands[0]);
+    sc->ncand[1] = factor_candidates(best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f), sc->cands[1]);
     for (int ch = 0; ch < 2; ch++)
-      for (int j = 0; j < 4; j++) cost[ch][j] = 0;
+      for (int j = 0; j < 4; j++) sc->cost[ch][j] = 0;
   }
+}
+
+__global__ __launch_bounds__(256) void k_cfl_lf_price(uint32_t count, const ProbeParams* __restrict__ pp,
+                                                      const float4* __restrict__ lf_s,
+                                                      CflLfScratch* __restrict__ sc) {
+  __shared__ unsigned long long cost[2][4];
+  __shared__ int cands[2][4], ncand[2];
+  if (threadIdx.x < 8) {
+    cost[threadIdx.x / 4][threadIdx.x % 4] = 0;
+    cands[threadIdx.x / 4][threadIdx.x % 4] = sc->cands[threadIdx.x / 4][threadIdx.x % 4];
+  }
+  if (threadIdx.x < 2) ncand[threadIdx.x] = sc->ncand[threadIdx.x];
   __syncthreads();
   const float mdc[2] = {pp->mdc[0], pp->mdc[2]};
   unsigned long long local[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
   int dummy = 0;
-  for (uint32_t i = threadIdx.x; i < count; i += blockDim.x) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
     const float4 s = lf_s[i];
     for (int ch = 0; ch < 2; ch++) {
       const float base = ch == 0 ? 0.0f : 1.0f, src = ch == 0 ? s.x : s.z;
@@ -548,46 +533,42 @@ __device__ void cfl_frame_lf(uint32_t count, const ProbeParams* __restrict__ pp,
     for (int j = 0; j < 4; j++)
       if (local[ch][j]) atomicAdd(&cost[ch][j], local[ch][j]);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned long long nx = 0, nb = 0;
-    for (int j = 0; j < ncand[0]; j++) if (cands[0][j] == 0) nx = cost[0][j];
-    for (int j = 0; j < ncand[1]; j++) if (cands[1][j] == 0) nb = cost[1][j];
-    int bx = 0, bb = 0;
-    unsigned long long best = nx + nb;
-    for (int i = 0; i < ncand[0]; i++)
-      for (int j = 0; j < ncand[1]; j++) {
-        const int fx = cands[0][i], fb = cands[1][j];
-        unsigned long long c = cost[0][i] + cost[1][j];
-        if (fx != 0 || fb != 0) c += 50ull + factor_bits(fx) + factor_bits(fb);
-        const unsigned mag = (unsigned)abs(fx) + (unsigned)abs(fb);
-        const unsigned bmag = (unsigned)abs(bx) + (unsigned)abs(bb);
-        if (c < best || (c == best && mag < bmag)) {
-          bx = fx;
-          bb = fb;
-          best = c;
-        }
-      }
-    out->x_factor_lf = bx;
-    out->b_factor_lf = bb;
-  }
+  if (threadIdx.x < 8 && cost[threadIdx.x / 4][threadIdx.x % 4])
+    atomicAdd(&sc->cost[threadIdx.x / 4][threadIdx.x % 4], cost[threadIdx.x / 4][threadIdx.x % 4]);
 }
 
-// Workgroup 0 takes the frame's LF regression (the longest sequential sum),
-// so it runs beside the per-tile workgroups 1..tiles instead of after them.
-__global__ __launch_bounds__(kCflThreads) void k_cfl_estimate(const uint32_t* __restrict__ tile_base, int tiles,
-                                                      uint32_t lf_count, const ConstTables* __restrict__ tab,
-                                                      const ProbeParams* __restrict__ pp,
-                                                      const float4* __restrict__ lf_s,
-                                                      const float4* __restrict__ hf_s,
-                                                      const uint16_t* __restrict__ hf_cell,
-                                                      int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
-                                                      CflParams* __restrict__ cflp) {
+__global__ void k_cfl_lf_select(const CflLfScratch* __restrict__ sc, CflParams* __restrict__ out) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long nx = 0, nb = 0;
+  for (int j = 0; j < sc->ncand[0]; j++) if (sc->cands[0][j] == 0) nx = sc->cost[0][j];
+  for (int j = 0; j < sc->ncand[1]; j++) if (sc->cands[1][j] == 0) nb = sc->cost[1][j];
+  int bx = 0, bb = 0;
+  unsigned long long best = nx + nb;
+  for (int i = 0; i < sc->ncand[0]; i++)
+    for (int j = 0; j < sc->ncand[1]; j++) {
+      const int fx = sc->cands[0][i], fb = sc->cands[1][j];
+      unsigned long long c = sc->cost[0][i] + sc->cost[1][j];
+      if (fx != 0 || fb != 0) c += 50ull + factor_bits(fx) + factor_bits(fb);
+      const unsigned mag = (unsigned)abs(fx) + (unsigned)abs(fb);
+      const unsigned bmag = (unsigned)abs(bx) + (unsigned)abs(bb);
+      if (c < best || (c == best && mag < bmag)) {
+        bx = fx;
+        bb = fb;
+        best = c;
+      }
+    }
+  out->x_factor_lf = bx;
+  out->b_factor_lf = bb;
+}
+
+__global__ __launch_bounds__(kCflThreads) void k_cfl_tiles(const uint32_t* __restrict__ tile_base, int tiles,
+                                                           const ConstTables* __restrict__ tab,
+                                                           const ProbeParams* __restrict__ pp,
+                                                           const float4* __restrict__ hf_s,
+                                                           const uint16_t* __restrict__ hf_cell,
+                                                           int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b) {
   __shared__ double stage[5][kCflStage];
-  if (blockIdx.x == 0) {
-    cfl_frame_lf(lf_count, pp, lf_s, cflp, stage);
-  } else if ((int)blockIdx.x <= tiles) {
-    cfl_tile((int)blockIdx.x - 1, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage);
-  }
+  if ((int)blockIdx.x < tiles) cfl_tile((int)blockIdx.x, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage);
 }
 
 }  // namespace
@@ -609,16 +590,27 @@ void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const Pr
 
 namespace jpxlg {
 
-void launch_cfl(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
-                const uint4* const items[3], const int counts[3], const uint32_t* tile_base,
-                uint32_t lf_count, float* const coeff[3], float4* lf_s, float4* hf_s, uint16_t* hf_cell,
-                int8_t* cfl_x, int8_t* cfl_b, CflParams* cflp, uint32_t* err) {
+void launch_cfl_collect(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                        const uint4* const items[3], const int counts[3], float* const coeff[3],
+                        float4* lf_s, float4* hf_s, uint16_t* hf_cell, uint32_t* err) {
   if (counts[0]) k_cfl_collect<8><<<counts[0], 64, 0, s>>>(g, items[0], counts[0], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
   if (counts[1]) k_cfl_collect<16><<<counts[1], 64, 0, s>>>(g, items[1], counts[1], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
   if (counts[2]) k_cfl_collect<32><<<counts[2], 64, 0, s>>>(g, items[2], counts[2], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+}
+
+void launch_cfl_tiles(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
+                      const uint32_t* tile_base, const float4* hf_s, const uint16_t* hf_cell,
+                      int8_t* cfl_x, int8_t* cfl_b) {
   const int tiles = g.tw * g.th;
-  k_cfl_estimate<<<tiles + 1, kCflThreads, 0, s>>>(tile_base, tiles, lf_count, tab, pp, lf_s, hf_s, hf_cell,
-                                          cfl_x, cfl_b, cflp);
+  k_cfl_tiles<<<tiles, kCflThreads, 0, s>>>(tile_base, tiles, tab, pp, hf_s, hf_cell, cfl_x, cfl_b);
+}
+
+void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, const float4* lf_s,
+                   CflLfScratch* scratch, CflParams* cflp) {
+  k_cfl_lf_seed<<<1, kCflThreads, 0, s>>>(lf_count, lf_s, scratch);
+  const unsigned blocks = std::max(1u, std::min(1024u, (lf_count + 2047) / 2048));
+  k_cfl_lf_price<<<blocks, 256, 0, s>>>(lf_count, pp, lf_s, scratch);
+  k_cfl_lf_select<<<1, 64, 0, s>>>(scratch, cflp);
 }
 
 }  // namespace jpxlg

@@ -32,6 +32,18 @@ def test_metric_cbrt_kats():
     assert orc.cbrt_metric(27.0) == 3.0
 
 
+def test_metric_cbrt_is_the_f32_rounding_of_the_f64_root():
+    # Four Newton steps from the bit-hack estimate converge to the f64 cube
+    # root, so the metric's f32 result is that root rounded once to f32 (the
+    # device computes it by one Halley step from the f32 library root).
+    rng = np.random.default_rng(5)
+    xs = np.concatenate([rng.random(20000, dtype=np.float32) * 3.0 + 0.0037,
+                         np.geomspace(1e-6, 1e6, 2001).astype(np.float32)])
+    for x in xs:
+        want = np.float32(np.cbrt(np.float64(x)))
+        assert orc.cbrt_metric(float(x)) == float(want), x
+
+
 def test_core_cbrt_is_correctly_rounded_on_a_sweep():
     # The musl algorithm (estimate + two Halley steps in f64) is correctly
     # rounded for normal inputs; compare with an exact f64 cube root rounded
