@@ -126,6 +126,71 @@ def copy_calibration(mib: int = 512, reps: int = 3) -> int:
     return 0
 
 
+def precision_form(args) -> int:
+    """The precision study on 4096x4096 RGB8 (seed 3): the chosen q85 plan's
+    structure is transformed by the reference's butterflies (exact), the FP32
+    matrix-core contraction and the bf16 hi/lo split contraction; each is
+    compared with the float64 closed form per transform size, its quantized
+    coefficients with the exact path's, and its SSIMULACRA2 at the same
+    quantizer with the exact path's.  One JSON line."""
+    import jpxl_amd
+    from jpxl_amd import _lib
+    from jpxl_amd.precision import error_report, forward_frame_f64
+    from jpxl_amd.synth import synthetic_rgb
+
+    w = h = 4096
+    rgb = synthetic_rgb(w, h, 3, 8)
+    ctx = jpxl_amd.Context(0)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    t0 = time.perf_counter()
+    res = img.encode_quality(TARGET, args.effort)
+    t_exact = time.perf_counter() - t0
+    q = jpxl_amd.quality_quantizer((int(res.global_scale), int(res.hf_mul)), args.effort, int(res.quant_lf))
+    plan0 = img.plan_download()
+    bmap = plan0.block_map
+    xyb = img.download(_lib.BUF_XYB)
+    want64 = forward_frame_f64(xyb, bmap)
+    del xyb
+    out = {"metric": "DCT precision forms at q85", "config": {"workload": f"{w}x{h} RGB8 synthetic (seed 3), "
+           f"quality {TARGET:g}, {args.effort} effort"}, "score_exact": round(res.score, 4),
+           "global_scale": int(res.global_scale), "varblocks": {str(sd): int((bmap == sd).sum()) for sd in (8, 16, 32)},
+           "forms": {}}
+    for name, form in (("exact", _lib.DCT_EXACT), ("fp32", _lib.DCT_FP32), ("bf16x3", _lib.DCT_BF16X3)):
+        img.set_dct_form(form)
+        coeff = img.download(_lib.BUF_COEFF)
+        rep = error_report(coeff, want64, bmap)
+        del coeff
+        img.quantize(q)
+        plan = img.plan_download()
+        diff = int((plan.quant != plan0.quant).sum()) + int((plan.lf != plan0.lf).sum())
+        total = plan.quant.size + plan.lf.size
+        score, _, _ = img.render_score()
+        # time one probe (quantize + render + score) in this form
+        ctx.profile(True)
+        ctx.profile_read()
+        t1 = time.perf_counter()
+        for _ in range(3):
+            img.quantize(q)
+            img.render_score()
+        probe_ms = 1000 * (time.perf_counter() - t1) / 3
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        out["forms"][name] = {
+            "coeff_error_vs_f64": rep,
+            "quantized_differ_fraction": diff / total,
+            "ssimulacra2": round(score, 4),
+            "ssimulacra2_delta": round(score - res.score, 5),
+            "probe_ms": round(probe_ms, 3),
+            "render_ms_per_probe": round(sum(v[0] for k, v in prof.items() if k.startswith("render")) / 3, 4),
+        }
+    img.set_dct_form(_lib.DCT_EXACT)
+    out["encode_ms_exact"] = round(1000 * t_exact, 2)
+    print(json.dumps(out))
+    img.close()
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -135,9 +200,14 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain copy of known size (for HBM counter calibration)")
+    ap.add_argument("--form", default="single", choices=["single", "precision"],
+                    help="single: the q85 encode+verify throughput line (default); "
+                         "precision: the DCT precision-form report on 4096x4096 RGB8")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
+    if args.form == "precision":
+        return precision_form(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

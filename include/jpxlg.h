@@ -118,6 +118,18 @@ int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t c
 
 /* ---- probe ---------------------------------------------------------------- */
 int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q);
+
+/* Arithmetic of the square DCTs (forward DCT of the structure, IDCT of the
+ * render).  EXACT is the reference's butterfly network, bit for bit
+ * (jpxl-core/src/dct.rs:956-997; the default).  FP32 and BF16X3 are the
+ * matrix-core contractions Out = L In^T L^T of the precision study: FP32
+ * products, or bf16 hi/lo split operands (hi*hi + hi*lo + lo*hi) accumulated
+ * in FP32.  They have no counterpart in the reference.  Changing the form
+ * recomputes the forward coefficients of an existing structure. */
+#define JPXLG_DCT_EXACT 0
+#define JPXLG_DCT_FP32 1
+#define JPXLG_DCT_BF16X3 2
+int32_t jpxlg_set_dct_form(jpxlg_img* img, int32_t form);
 /* Reconstructs the last quantized plan at the source depth and scores it.
  * terms (optional) receives 3 * scales * 6 pooled terms in the metric's
  * (component, scale, norm, map) order; *nterms their count. */

@@ -241,6 +241,11 @@ class Image:
     def quantize(self, q: Quantizer):
         self.ctx.check(self.ctx.lib.jpxlg_quantize(self.handle, ctypes.byref(q)))
 
+    def set_dct_form(self, form: int):
+        """_lib.DCT_EXACT (the reference's butterflies), DCT_FP32 or DCT_BF16X3
+        (matrix-core contractions of the precision study)."""
+        self.ctx.check(self.ctx.lib.jpxlg_set_dct_form(self.handle, int(form)))
+
     def render_score(self, restoration: Restoration | None = None):
         r = restoration or Restoration(1, 0, 7)
         score, raw = ctypes.c_double(), ctypes.c_double()

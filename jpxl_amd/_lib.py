@@ -25,6 +25,9 @@ BUF_COEFF = 1
 BUF_RENDER_XYB = 2
 BUF_LINEAR = 3
 BUF_SIGMA = 4
+DCT_EXACT = 0
+DCT_FP32 = 1
+DCT_BF16X3 = 2
 BUF_REF_XYB0 = 5
 
 COVER_FIXED_DCT8 = 0
@@ -107,6 +110,7 @@ SIGNATURES = {
     "jpxlg_set_structure": (_i32, [_vp, _vp, _vp, _vp, _vp]),
     "jpxlg_plan_structure": (_i32, [_vp, ctypes.POINTER(Quantizer), _i32, _i32]),
     "jpxlg_quantize": (_i32, [_vp, ctypes.POINTER(Quantizer)]),
+    "jpxlg_set_dct_form": (_i32, [_vp, _i32]),
     "jpxlg_render_score": (_i32, [_vp, ctypes.POINTER(Restoration), _dp, _dp, _dp, _u32p]),
     "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),

@@ -42,6 +42,7 @@ struct jpxlg_ctx {
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, uint64_t>> totals;
   ConstTables* d_tab = nullptr;
+  DctMats* d_mats = nullptr;
   float* d_lut8 = nullptr;
   std::map<int, float*> d_lut16;
   std::map<int, DepthSet> depth;
@@ -133,6 +134,8 @@ int fail(jpxlg_ctx* ctx, int code, const std::string& msg) {
     if (e_ != hipSuccess)                                                               \
       return fail(ctx, JPXLG_E_DEVICE, std::string(#expr ": ") + hipGetErrorString(e_)); \
   } while (0)
+
+const char* const kDctNames[3] = {"dct_fwd_8", "dct_fwd_16", "dct_fwd_32"};
 
 // One device allocation carved into the image's buffers.
 struct Arena {
@@ -242,6 +245,7 @@ struct jpxlg_img {
   bool has_render = false;
   jpxlg_quantizer last_q{};
   int last_sharpness = 7;
+  int dct_form = kDctExact;  // DctForm of the structure's forward DCT and the render's IDCT
   // The arena goes back to the context (or is freed) when the image dies.
   ~jpxlg_img() {
     if (arena.base) (void)hipStreamSynchronize(ctx->stream);
@@ -269,6 +273,12 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   build_const_tables(&host);
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_tab, sizeof(ConstTables)));
   HIP_TRY(ctx.get(), hipMemcpy(ctx->d_tab, &host, sizeof(ConstTables), hipMemcpyHostToDevice));
+  {
+    DctMats mats;
+    build_dct_mats(&mats);
+    HIP_TRY(ctx.get(), hipMalloc(&ctx->d_mats, sizeof(DctMats)));
+    HIP_TRY(ctx.get(), hipMemcpy(ctx->d_mats, &mats, sizeof(DctMats), hipMemcpyHostToDevice));
+  }
   float lut8[256];
   srgb8_lut(lut8);
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_lut8, sizeof(lut8)));
@@ -282,6 +292,7 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   (void)hipFree(ctx->d_tab);
+  (void)hipFree(ctx->d_mats);
   (void)hipFree(ctx->d_lut8);
   for (auto& kv : ctx->d_lut16) (void)hipFree(kv.second);
   for (auto& kv : ctx->depth) {
@@ -529,11 +540,11 @@ static int upload_structure(jpxlg_img* img) {
   HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_b, img->cfl_b.data(), img->cfl_b.size(), hipMemcpyHostToDevice, st));
   CflParams cp{img->lf_factors[0], img->lf_factors[1]};
   HIP_TRY(ctx, hipMemcpyAsync(img->d_cflp, &cp, sizeof(cp), hipMemcpyHostToDevice, st));
-  static const char* kDctNames[3] = {"dct_fwd_8", "dct_fwd_16", "dct_fwd_32"};
   for (int i = 0; i < 3; i++) {
     if (!img->list_count[i]) continue;
     Prof p(ctx, kDctNames[i]);
-    launch_dct_fwd(st, kSides[i], g, img->d_list[i], img->list_count[i], ctx->d_tab, img->xyb, img->coeff);
+    launch_dct_fwd(st, kSides[i], img->dct_form, g, img->d_list[i], img->list_count[i], ctx->d_tab, ctx->d_mats,
+                   img->xyb, img->coeff);
   }
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -561,6 +572,28 @@ extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, cons
       img->lf_factors[1] > 127)
     return fail(img->ctx, JPXLG_E_RANGE, "LF CfL factor outside u8 after bias");
   return upload_structure(img);
+}
+
+extern "C" int32_t jpxlg_set_dct_form(jpxlg_img* img, int32_t form) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (form != JPXLG_DCT_EXACT && form != JPXLG_DCT_FP32 && form != JPXLG_DCT_BF16X3)
+    return fail(ctx, JPXLG_E_UNSUPPORTED, "unknown DCT form");
+  if (form == img->dct_form) return JPXLG_OK;
+  img->dct_form = form;
+  img->has_quant = false;
+  img->has_render = false;
+  if (!img->has_structure) return JPXLG_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  for (int i = 0; i < 3; i++) {
+    if (!img->list_count[i]) continue;
+    Prof p(ctx, kDctNames[i]);
+    launch_dct_fwd(st, kSides[i], img->dct_form, img->g, img->d_list[i], img->list_count[i], ctx->d_tab,
+                   ctx->d_mats, img->xyb, img->coeff);
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  return JPXLG_OK;
 }
 
 // The chroma-from-luma estimate of build_pixel_plan (lib.rs:3437-3763) on
@@ -759,8 +792,8 @@ static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
   for (int i = 0; i < 3; i++) {
     if (!img->list_count[i]) continue;
     Prof p(ctx, kRNames[i]);
-    launch_render(st, kSides[i], g, img->d_list[i], img->list_count[i], ctx->d_tab, img->d_pp,
-                  img->d_cflp, img->d_cfl_x, img->d_cfl_b, img->quant, img->lf, img->rx, img->sigma);
+    launch_render(st, kSides[i], img->dct_form, g, img->d_list[i], img->list_count[i], ctx->d_tab, ctx->d_mats,
+                  img->d_pp, img->d_cflp, img->d_cfl_x, img->d_cfl_b, img->quant, img->lf, img->rx, img->sigma);
   }
   float* cur[3] = {img->rx[0], img->rx[1], img->rx[2]};
   float* oth[3] = {img->aux[0], img->aux[1], img->aux[2]};

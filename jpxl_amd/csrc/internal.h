@@ -6,6 +6,7 @@
 #include <string>
 #include <vector>
 
+#include "dct_matrix.h"
 #include "device_math.h"
 
 namespace jpxlg {
@@ -70,16 +71,18 @@ enum : uint32_t { kErrQuantRange = 1u };
 
 void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, const float* lut,
                     float* xyb[3], float* lin[3], int* gray);
-void launch_dct_fwd(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
-                    const ConstTables* tab, float* const xyb[3], float* coeff[3]);
+// `form` is a DctForm: the reference's butterflies (bit-exact) or one of the
+// matrix-core contractions of dct_matrix.h.
+void launch_dct_fwd(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
+                    const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]);
 void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const ConstTables* tab,
                        const float* in, float* out);
 void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
                      const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
                      const int8_t* cfl_x, const int8_t* cfl_b, float* const coeff[3],
                      int32_t* quant[3], int32_t* lf[3], uint32_t* err);
-void launch_render(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
-                   const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
+void launch_render(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
+                   const ConstTables* tab, const DctMats* mats, const ProbeParams* pp, const CflParams* cfl,
                    const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3],
                    int32_t* const lf[3], float* xyb[3], float* sigma);
 void launch_fill(hipStream_t s, float* p, float v, size_t n);

@@ -15,6 +15,7 @@
 // One workgroup of 192 threads (three wave64s) processes 64/N varblocks of
 // size N, so every 1-D transform pass keeps all 192 lanes busy (3 channels x
 // N lines x 64/N varblocks).
+#include "dct_matrix.h"
 #include "internal.h"
 
 namespace jpxlg {
@@ -194,9 +195,10 @@ __device__ __forceinline__ void unpack(uint32_t o, int* bx, int* by) {
 // Forward DCT over the block map (jpxl-encode-policy/src/lib.rs:3009-3098).
 // ----------------------------------------------------------------------------
 
-template <int N>
+template <int N, int FORM>
 __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __restrict__ list,
                                                       int count, const ConstTables* __restrict__ tab,
+                                                      const DctMats* __restrict__ mats,
                                                       const float* __restrict__ X,
                                                       const float* __restrict__ Y,
                                                       const float* __restrict__ B,
@@ -205,21 +207,36 @@ __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __
                                                       float* __restrict__ CB) {
   constexpr int VB = 64 / N, P = N + 1, NN = N * N;
   __shared__ float lds[VB * 3 * N * P];
+  __shared__ int vbx[VB], vby[VB];
   const int base = blockIdx.x * VB;
+  if (threadIdx.x < VB) {
+    int bx = 0, by = 0;
+    if (base + (int)threadIdx.x < count) unpack(list[base + threadIdx.x], &bx, &by);
+    vbx[threadIdx.x] = bx;
+    vby[threadIdx.x] = by;
+  }
+  __syncthreads();
   const float* planes[3] = {X, Y, B};
   for (int e = threadIdx.x; e < VB * 3 * NN; e += kThreads) {
     int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
     int dy = k / N, dx = k % N;
     float v = 0.0f;
     if (base + j < count) {
-      int bx, by;
-      unpack(list[base + j], &bx, &by);
-      int x = min(8 * bx + dx, g.w - 1), y = min(8 * by + dy, g.h - 1);
+      int x = min(8 * vbx[j] + dx, g.w - 1), y = min(8 * vby[j] + dy, g.h - 1);
       v = planes[c][(size_t)y * g.w + x];
     }
     lds[((j * 3 + c) * N + dy) * P + dx] = v;
   }
   __syncthreads();
+  if constexpr (FORM != kDctExact) {
+    const float* L = N == 8 ? mats->fwd8 : (N == 16 ? mats->fwd16 : mats->fwd32);
+    float* dsts[3] = {CX, CY, CB};
+    matrix_transform_tiles<N, FORM>(lds, VB * 3, L, [&](int t, int k, int r, float v) {
+      const int j = t / 3;
+      if (base + j < count) dsts[t % 3][(size_t)(8 * vby[j] + k) * g.cstride + 8 * vbx[j] + r] = v;
+    });
+    return;
+  }
   const DctTables& t = tab->dct;
   const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
   float* tile = lds + (j * 3 + c) * N * P;
@@ -228,11 +245,9 @@ __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __
   float out[N];
   fwd_pass2<N>(tile, line, out, t);
   if (base + j < count) {
-    int bx, by;
-    unpack(list[base + j], &bx, &by);
     float* dst = c == 0 ? CX : (c == 1 ? CY : CB);
 #pragma unroll
-    for (int k = 0; k < N; k++) dst[(size_t)(8 * by + k) * g.cstride + 8 * bx + line] = out[k];
+    for (int k = 0; k < N; k++) dst[(size_t)(8 * vby[j] + k) * g.cstride + 8 * vbx[j] + line] = out[k];
   }
 }
 
@@ -517,9 +532,10 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(
 // Reconstruction (jpxl-plan-render/src/lib.rs:853-1032, :1294-1359)
 // ----------------------------------------------------------------------------
 
-template <int N>
+template <int N, int FORM>
 __global__ __launch_bounds__(kThreads) void k_render(
     Geom g, const uint32_t* __restrict__ list, int count, const ConstTables* __restrict__ tab,
+    const DctMats* __restrict__ mats,
     const ProbeParams* __restrict__ pp, const CflParams* __restrict__ cflp,
     const int8_t* __restrict__ cfl_x, const int8_t* __restrict__ cfl_b,
     const int32_t* __restrict__ QX, const int32_t* __restrict__ QY, const int32_t* __restrict__ QB,
@@ -614,6 +630,15 @@ __global__ __launch_bounds__(kThreads) void k_render(
   }
   __syncthreads();
   // I.9: IDCT_2D.
+  if constexpr (FORM != kDctExact) {
+    const float* L = N == 8 ? mats->inv8 : (N == 16 ? mats->inv16 : mats->inv32);
+    float* dsts[3] = {X, Y, B};
+    matrix_transform_tiles<N, FORM>(lds, VB * 3, L, [&](int tt, int y, int x, float v) {
+      const int jj = tt / 3;
+      const int fx = 8 * vbx[jj] + x, fy = 8 * vby[jj] + y;
+      if (jj < nvb && fx < g.w && fy < g.h) dsts[tt % 3][(size_t)fy * g.w + fx] = v;
+    });
+  } else {
   const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
   float* tile = lds + (j * 3 + c) * N * P;
   inv_pass1<N>(tile, line, t);
@@ -630,6 +655,7 @@ __global__ __launch_bounds__(kThreads) void k_render(
         if (fy < g.h) dst[(size_t)fy * g.w + fx] = out[r];
       }
     }
+  }
   }
   // J.4.3 sigma of every 8x8 block the varblock covers.
   if (threadIdx.x < VB * n * n) {
@@ -784,13 +810,28 @@ void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, con
                                                     lin[1], lin[2], gray);
 }
 
-void launch_dct_fwd(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
-                    const ConstTables* tab, float* const xyb[3], float* coeff[3]) {
+template <int N, int FORM>
+static void dct_fwd_form(hipStream_t s, const Geom& g, const uint32_t* list, int count, const ConstTables* tab,
+                         const DctMats* mats, float* const xyb[3], float* coeff[3]) {
+  k_dct_fwd<N, FORM><<<div_up(count, 64 / N), kThreads, 0, s>>>(g, list, count, tab, mats, xyb[0], xyb[1], xyb[2],
+                                                                 coeff[0], coeff[1], coeff[2]);
+}
+
+template <int N>
+static void dct_fwd_size(hipStream_t s, int form, const Geom& g, const uint32_t* list, int count,
+                         const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]) {
+  if (form == kDctFp32) dct_fwd_form<N, kDctFp32>(s, g, list, count, tab, mats, xyb, coeff);
+  else if (form == kDctBf16x3) dct_fwd_form<N, kDctBf16x3>(s, g, list, count, tab, mats, xyb, coeff);
+  else dct_fwd_form<N, kDctExact>(s, g, list, count, tab, mats, xyb, coeff);
+}
+
+void launch_dct_fwd(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
+                    const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]) {
   if (count <= 0) return;
   switch (side) {
-    case 8: k_dct_fwd<8><<<div_up(count, 8), kThreads, 0, s>>>(g, list, count, tab, xyb[0], xyb[1], xyb[2], coeff[0], coeff[1], coeff[2]); break;
-    case 16: k_dct_fwd<16><<<div_up(count, 4), kThreads, 0, s>>>(g, list, count, tab, xyb[0], xyb[1], xyb[2], coeff[0], coeff[1], coeff[2]); break;
-    default: k_dct_fwd<32><<<div_up(count, 2), kThreads, 0, s>>>(g, list, count, tab, xyb[0], xyb[1], xyb[2], coeff[0], coeff[1], coeff[2]); break;
+    case 8: dct_fwd_size<8>(s, form, g, list, count, tab, mats, xyb, coeff); break;
+    case 16: dct_fwd_size<16>(s, form, g, list, count, tab, mats, xyb, coeff); break;
+    default: dct_fwd_size<32>(s, form, g, list, count, tab, mats, xyb, coeff); break;
   }
 }
 
@@ -822,22 +863,39 @@ void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* lis
 #undef JPXLG_Q
 }
 
-void launch_render(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
-                   const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
-                   const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3],
-                   int32_t* const lf[3], float* xyb[3], float* sigma) {
+template <int N, int FORM>
+static void render_form(hipStream_t s, const Geom& g, const uint32_t* list, int count, const ConstTables* tab,
+                        const DctMats* mats, const ProbeParams* pp, const CflParams* cfl, const int8_t* cfl_x,
+                        const int8_t* cfl_b, int32_t* const quant[3], int32_t* const lf[3], float* xyb[3],
+                        float* sigma) {
+  k_render<N, FORM><<<div_up(count, 64 / N), kThreads, 0, s>>>(g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b,
+                                                                quant[0], quant[1], quant[2], lf[0], lf[1], lf[2],
+                                                                xyb[0], xyb[1], xyb[2], sigma);
+}
+
+template <int N>
+static void render_size(hipStream_t s, int form, const Geom& g, const uint32_t* list, int count,
+                        const ConstTables* tab, const DctMats* mats, const ProbeParams* pp, const CflParams* cfl,
+                        const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3], int32_t* const lf[3],
+                        float* xyb[3], float* sigma) {
+  if (form == kDctFp32)
+    render_form<N, kDctFp32>(s, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma);
+  else if (form == kDctBf16x3)
+    render_form<N, kDctBf16x3>(s, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma);
+  else
+    render_form<N, kDctExact>(s, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma);
+}
+
+void launch_render(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
+                   const ConstTables* tab, const DctMats* mats, const ProbeParams* pp, const CflParams* cfl,
+                   const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3], int32_t* const lf[3],
+                   float* xyb[3], float* sigma) {
   if (count <= 0) return;
-#define JPXLG_R(NS, VBN)                                                                         \
-  k_render<NS><<<div_up(count, VBN), kThreads, 0, s>>>(g, list, count, tab, pp, cfl, cfl_x,    \
-                                                       cfl_b, quant[0], quant[1], quant[2],    \
-                                                       lf[0], lf[1], lf[2], xyb[0], xyb[1],    \
-                                                       xyb[2], sigma)
   switch (side) {
-    case 8: JPXLG_R(8, 8); break;
-    case 16: JPXLG_R(16, 4); break;
-    default: JPXLG_R(32, 2); break;
+    case 8: render_size<8>(s, form, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma); break;
+    case 16: render_size<16>(s, form, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma); break;
+    default: render_size<32>(s, form, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma); break;
   }
-#undef JPXLG_R
 }
 
 void launch_fill(hipStream_t s, float* p, float v, size_t n) {

@@ -185,6 +185,21 @@ float hf_multiplier(uint32_t gs, uint32_t hf_mul) {  // reconstruct.rs:27-36
   return denom <= 0.0 ? 0.0f : (float)(65536.0 / denom);
 }
 
+void build_dct_mats(DctMats* m) {
+  float* fwd[3] = {m->fwd8, m->fwd16, m->fwd32};
+  float* inv[3] = {m->inv8, m->inv16, m->inv32};
+  for (int si = 0; si < 3; si++) {
+    const int n = kSides[si];
+    for (int k = 0; k < n; k++)
+      for (int x = 0; x < n; x++) {
+        // orthonormal DCT-II basis D[k][x]
+        const double d = (k == 0 ? sqrt(1.0 / n) : sqrt(2.0 / n)) * cos(M_PI * (2 * x + 1) * k / (2.0 * n));
+        fwd[si][k * n + x] = (float)(d / sqrt((double)n));  // L = D / sqrt(N)
+        inv[si][x * n + k] = (float)(d * sqrt((double)n));  // L = sqrt(N) D^T
+      }
+  }
+}
+
 void build_const_tables(ConstTables* t) {
   memset(t, 0, sizeof(*t));
   t->dct.cos_pi_8 = 0.9238795f;

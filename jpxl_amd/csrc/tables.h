@@ -7,6 +7,8 @@
 namespace jpxlg {
 
 void build_const_tables(ConstTables* t);
+// The L matrices of the matrix-core DCT forms (f64 closed form, rounded once).
+void build_dct_mats(DctMats* m);
 void srgb8_lut(float* lut256);
 void srgb16_lut(int bits, float* lut65536);
 int depth_tables(int bits, std::vector<float>* thresholds, std::vector<float>* lut);

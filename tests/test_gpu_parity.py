@@ -109,6 +109,46 @@ def test_forward_coefficients_bit_exact(ctx, wh):
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
+@pytest.mark.parametrize("form,tol", [(1, 2e-6), (2, 1e-4)])
+def test_matrix_dct_forms_against_f64(ctx, form, tol):
+    """The matrix-core DCT forms (FP32 products, bf16 hi/lo split) against the
+    float64 closed form; switching back to the butterflies is bit-exact."""
+    from jpxl_amd import _lib
+    from jpxl_amd.precision import error_report, forward_frame_f64
+
+    w, h = 257, 131
+    bmap = orc.mixed_map(w, h, seed=7)
+    img, xyb, _ = _image_and_oracle(ctx, w, h, 5, 8, bmap)
+    want64 = forward_frame_f64(xyb, bmap)
+    exact = img.download(_lib.BUF_COEFF)
+    img.set_dct_form(form)
+    got = img.download(_lib.BUF_COEFF)
+    rep = error_report(got, want64, bmap)
+    assert set(rep) == {"8", "16", "32"}
+    assert all(r["max_abs"] < tol for r in rep.values()), rep
+    assert not np.array_equal(got, exact) or form == 1
+    img.set_dct_form(_lib.DCT_EXACT)
+    assert np.array_equal(img.download(_lib.BUF_COEFF).view(np.uint32), exact.view(np.uint32))
+
+
+@pytest.mark.parametrize("form", [1, 2])
+def test_matrix_idct_render_close_to_exact(ctx, form):
+    """Render through the matrix-core inverse: close to the bit-exact render."""
+    from jpxl_amd import Restoration, _lib
+
+    w, h = 200, 120
+    bmap = orc.mixed_map(w, h, seed=3)
+    img, xyb, _ = _image_and_oracle(ctx, w, h, 9, 8, bmap)
+    img.quantize(_quantizer(gs=8000))
+    img.render_score(Restoration(0, 0, 7))
+    exact = img.download(_lib.BUF_RENDER_XYB)
+    img.set_dct_form(form)
+    img.quantize(_quantizer(gs=8000))
+    img.render_score(Restoration(0, 0, 7))
+    got = img.download(_lib.BUF_RENDER_XYB)
+    assert np.abs(got - exact).max() < (1e-4 if form == 2 else 1e-5)
+
+
 def _cfl(bmap, seed):
     th, tw = (bmap.shape[0] + 7) // 8, (bmap.shape[1] + 7) // 8
     rng = np.random.default_rng(seed)

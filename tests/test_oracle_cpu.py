@@ -426,3 +426,18 @@ def test_library_exports_every_header_symbol():
         assert hasattr(lib, n), n
     from jpxl_amd import _lib
     assert names == set(_lib.SIGNATURES), names ^ set(_lib.SIGNATURES)
+
+
+def test_f64_frame_dct_matches_the_butterfly_oracle():
+    # The precision study's float64 closed form (jpxl_amd.precision) and the
+    # oracle's f32 butterflies agree to f32 rounding on mixed block maps, so
+    # the closed form has the reference's scale, transposition and footprint.
+    from jpxl_amd.precision import error_report, forward_frame_f64
+    from jpxl_amd.synth import synthetic_rgb
+
+    for w, h in [(64, 64), (61, 37), (130, 70)]:
+        rgb = synthetic_rgb(w, h, 5, 8)
+        xyb, _, _ = orc.prepare_frame(rgb, 8)
+        bmap = orc.mixed_map(w, h, seed=w)
+        rep = error_report(orc.forward_frame(xyb, bmap), forward_frame_f64(xyb, bmap), bmap)
+        assert rep and all(r["max_abs"] < 5e-7 for r in rep.values()), rep
