@@ -191,6 +191,94 @@ def precision_form(args) -> int:
     return 0
 
 
+def _gen(args):
+    from jpxl_amd.synth import synthetic_rgb
+
+    w, h, seed = args
+    return synthetic_rgb(w, h, seed, 8)
+
+
+def batched_form(args) -> int:
+    """The batched form: uneven RGB8 images, W and H uniform in [256, 3072]
+    (seeded), each with a quality target drawn from {70, 85, 90}, all
+    resident in HBM.  Times the plain loop (one image at a time: create,
+    quality-encode, close) against the batched encode (all images created,
+    then every round of probes one launch per stage), in the exact DCT form
+    and the bf16 hi/lo split form.  One JSON line."""
+    import multiprocessing as mp
+
+    import jpxl_amd
+    from jpxl_amd import _lib
+    from jpxl_amd.synth import uneven_batch
+
+    n = args.batch_images
+    sizes, targets = uneven_batch(n, 4)
+    jobs = [(w, h, 1000 + k) for k, (w, h) in enumerate(sizes)]
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        rgbs = pool.map(_gen, jobs, chunksize=1)
+    t_gen = time.perf_counter() - t0
+    total_mp = sum(w * h for w, h in sizes) / 1e6
+    hl = hip()
+    ptrs = []
+    for x in rgbs:
+        d = ctypes.c_void_p()
+        assert hl.hipMalloc(ctypes.byref(d), x.nbytes) == 0
+        assert hl.hipMemcpy(d, x.ctypes.data, x.nbytes, 1) == 0
+        ptrs.append(d)
+    del rgbs
+    ctx = jpxl_amd.Context(0)
+
+    def plain(form):
+        scores = []
+        for k, (w, h) in enumerate(sizes):
+            img = jpxl_amd.Image(ctx, None, 8, device_ptr=ptrs[k].value, shape=(h, w))
+            img.set_dct_form(form)
+            scores.append(img.encode_quality(targets[k], args.effort).score)
+            img.close()
+        return scores
+
+    def batched(form):
+        imgs = [jpxl_amd.Image(ctx, None, 8, device_ptr=ptrs[k].value, shape=(h, w))
+                for k, (w, h) in enumerate(sizes)]
+        for im in imgs:
+            im.set_dct_form(form)
+        res = jpxl_amd.batch_encode_quality(imgs, targets, args.effort)
+        for im in imgs:
+            im.close()
+        return [r.score for r in res], [r.probes for r in res]
+
+    out = {"metric": "batched VarDCT encode+verify MP/s (uneven images)", "unit": "MP/s",
+           "config": {"workload": f"{n} RGB8 synthetic images, W,H ~ U[256,3072] (seed 4), "
+                                  f"targets from {{70,85,90}}, {args.effort} effort, all resident in HBM",
+                      "megapixels": round(total_mp, 2)},
+           "generate_s": round(t_gen, 1)}
+    plain(_lib.DCT_EXACT)  # warm-up (also sizes the arena pool)
+    for label, form in (("exact", _lib.DCT_EXACT), ("bf16x3", _lib.DCT_BF16X3)):
+        hl.hipDeviceSynchronize()
+        t = time.perf_counter()
+        ps = plain(form)
+        hl.hipDeviceSynchronize()
+        t_plain = time.perf_counter() - t
+        t = time.perf_counter()
+        bs, probes = batched(form)
+        hl.hipDeviceSynchronize()
+        t_batch = time.perf_counter() - t
+        out[label] = {"plain_mps": round(total_mp / t_plain, 2), "batched_mps": round(total_mp / t_batch, 2),
+                      "plain_s": round(t_plain, 3), "batched_s": round(t_batch, 3),
+                      "batched_over_plain": round(t_plain / t_batch, 3),
+                      "scores_identical": bool(np.array_equal(np.array(ps), np.array(bs))),
+                      "probes_total": int(sum(probes)),
+                      "met": int(sum(1 for k, sc in enumerate(bs) if sc >= targets[k]))}
+    out["value"] = out["bf16x3"]["batched_mps"]
+    out["vs_plain_exact_one_at_a_time"] = round(out["bf16x3"]["batched_mps"] / out["exact"]["plain_mps"], 3)
+    print(json.dumps(out))
+    ctx.close()
+    for d in ptrs:
+        hl.hipFree(d)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -200,14 +288,18 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain copy of known size (for HBM counter calibration)")
-    ap.add_argument("--form", default="single", choices=["single", "precision"],
+    ap.add_argument("--form", default="single", choices=["single", "precision", "batched"],
                     help="single: the q85 encode+verify throughput line (default); "
-                         "precision: the DCT precision-form report on 4096x4096 RGB8")
+                         "precision: the DCT precision-form report on 4096x4096 RGB8; "
+                         "batched: uneven images, plain loop against the batched encode")
+    ap.add_argument("--batch-images", type=int, default=256)
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
     if args.form == "precision":
         return precision_form(args)
+    if args.form == "batched":
+        return batched_form(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

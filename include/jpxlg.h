@@ -175,6 +175,44 @@ typedef struct {
 int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
                              jpxlg_quality_result* out);
 
+/* ---- batches: many images of one context, one launch per stage ---------- */
+/* One probe of n images: each quantized with its own quantizer, all
+ * reconstructed with restoration r and scored, every stage one launch over
+ * the batch's block-tile work list; scores[k] per image.  No counterpart in
+ * the reference (which probes one image at a time through
+ * PerceptualEvaluator::evaluate, quality.rs:87-157). */
+int32_t jpxlg_batch_probe(uint32_t n, jpxlg_img* const* imgs, const jpxlg_quantizer* qs,
+                          const jpxlg_restoration* r, double* scores);
+/* search_frame_perceptual (quality.rs:2405) for n images at once: each
+ * image's navigator runs as in jpxlg_encode_quality, and every round of
+ * probes of the images still searching is one batched chain.  Each chosen
+ * plan is left quantized on the device.  No counterpart in the reference. */
+int32_t jpxlg_batch_encode_quality(uint32_t n, jpxlg_img* const* imgs, const jpxlg_quality_request* reqs,
+                                   jpxlg_quality_result* out);
+
+/* ---- serving: continuous batching of quality requests ------------------- */
+/* A server owns the navigators of the requests submitted to it (images of
+ * its context, resident until released).  Each step() runs one round: the
+ * pending probe of every active request, new requests' structures included,
+ * as one batched chain, and the final quantize of requests whose search
+ * ended.  Requests may be submitted between steps and join the next round.
+ * One thread drives a server.  No counterpart in the reference (whose
+ * encoder serves one request per call, jpxl/src/lib.rs:507-794). */
+typedef struct jpxlg_server jpxlg_server;
+int32_t jpxlg_server_create(jpxlg_ctx* ctx, jpxlg_server** out);
+int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quality_request* req,
+                            uint64_t* ticket);
+/* >= 0: requests completed by this round; < 0: JPXLG_E_*. */
+int32_t jpxlg_server_step(jpxlg_server* srv);
+/* 1: done (result written), 0: running, < 0: the request failed / bad ticket. */
+int32_t jpxlg_server_poll(jpxlg_server* srv, uint64_t ticket, jpxlg_quality_result* out);
+/* Forgets a completed request (its image stays the caller's). */
+int32_t jpxlg_server_release(jpxlg_server* srv, uint64_t ticket);
+/* Requests not yet completed. */
+int32_t jpxlg_server_pending(jpxlg_server* srv);
+/* Runs outstanding requests to completion, then frees the server. */
+void jpxlg_server_destroy(jpxlg_server* srv);
+
 /* ---- per-kernel timing ----------------------------------------------------- */
 /* When enabled, every launch the context issues is bracketed by HIP events on
  * its stream.  jpxlg_profile_read synchronises, folds the events into per-

@@ -294,6 +294,94 @@ class Image:
         return res
 
 
+def _image_array(images):
+    if not images:
+        raise ValueError("empty batch")
+    ctx = images[0].ctx
+    if any(im.ctx is not ctx for im in images):
+        raise ValueError("a batch's images must share one Context")
+    arr = (ctypes.c_void_p * len(images))(*[im.handle.value for im in images])
+    return ctx, arr
+
+
+def batch_probe(images, quantizers, restoration: Restoration | None = None) -> np.ndarray:
+    """One probe of every image (each with its own Quantizer), one launch per
+    stage over the batch; returns the SSIMULACRA2 scores."""
+    ctx, arr = _image_array(images)
+    if len(quantizers) != len(images):
+        raise ValueError("one quantizer per image")
+    qs = (Quantizer * len(images))(*quantizers)
+    r = restoration or Restoration(1, 0, 7)
+    scores = np.zeros(len(images), np.float64)
+    ctx.check(ctx.lib.jpxlg_batch_probe(len(images), arr, qs, ctypes.byref(r),
+                                        scores.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+    return scores
+
+
+def batch_encode_quality(images, targets, effort: str = "balanced", max_probes: int = 0, mode: int = 0):
+    """The quality search of every image at once (each its own target): each
+    round of probes of the images still searching is one batched chain.
+    Returns one QualityResult per image; each chosen plan stays quantized."""
+    ctx, arr = _image_array(images)
+    eff = _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED
+    reqs = (_lib.QualityRequest * len(images))(*[_lib.QualityRequest(float(t), eff, max_probes, mode)
+                                                  for t in targets])
+    res = (_lib.QualityResult * len(images))()
+    ctx.check(ctx.lib.jpxlg_batch_encode_quality(len(images), arr, reqs, res))
+    return list(res)
+
+
+class Server:
+    """Continuous batching of quality requests (jpxlg_server): submit images
+    with their targets at any time; each step() runs one round of probes of
+    every active request as one batched chain."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.jpxlg_server_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def submit(self, image: "Image", target: float, effort: str = "balanced", max_probes: int = 0,
+               mode: int = 0) -> int:
+        req = _lib.QualityRequest(float(target), _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED,
+                                  max_probes, mode)
+        t = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.jpxlg_server_submit(self.handle, image.handle, ctypes.byref(req), ctypes.byref(t)))
+        return t.value
+
+    def step(self) -> int:
+        rc = self.ctx.lib.jpxlg_server_step(self.handle)
+        if rc < 0:
+            self.ctx.check(rc)
+        return rc
+
+    def poll(self, ticket: int):
+        """The QualityResult once the request is done, else None."""
+        res = _lib.QualityResult()
+        rc = self.ctx.lib.jpxlg_server_poll(self.handle, ticket, ctypes.byref(res))
+        if rc < 0:
+            self.ctx.check(rc)
+        return res if rc == 1 else None
+
+    def release(self, ticket: int) -> None:
+        self.ctx.check(self.ctx.lib.jpxlg_server_release(self.handle, ticket))
+
+    def pending(self) -> int:
+        return self.ctx.lib.jpxlg_server_pending(self.handle)
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx.lib.jpxlg_server_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 @dataclass
 class EncodeReport:
     """The perceptual half of the reference's EncodeReport (jpxl/src/lib.rs:353)."""

@@ -115,6 +115,17 @@ SIGNATURES = {
     "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),
     "jpxlg_encode_quality": (_i32, [_vp, ctypes.POINTER(QualityRequest), ctypes.POINTER(QualityResult)]),
+    "jpxlg_batch_probe": (_i32, [ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(Quantizer),
+                                 ctypes.POINTER(Restoration), ctypes.POINTER(ctypes.c_double)]),
+    "jpxlg_batch_encode_quality": (_i32, [ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(QualityRequest),
+                                          ctypes.POINTER(QualityResult)]),
+    "jpxlg_server_create": (_i32, [_vp, ctypes.POINTER(_vp)]),
+    "jpxlg_server_submit": (_i32, [_vp, _vp, ctypes.POINTER(QualityRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    "jpxlg_server_step": (_i32, [_vp]),
+    "jpxlg_server_poll": (_i32, [_vp, ctypes.c_uint64, ctypes.POINTER(QualityResult)]),
+    "jpxlg_server_release": (_i32, [_vp, ctypes.c_uint64]),
+    "jpxlg_server_pending": (_i32, [_vp]),
+    "jpxlg_server_destroy": (None, [_vp]),
     "jpxlg_score_pair": (_i32, [_vp, _u32, _u32, _vp, _vp, _dp, _dp, _u32p]),
     "jpxlg_dct_blocks": (_i32, [_vp, _i32, _i32, _u32, _vp, _vp]),
     "jpxlg_epf_step": (_i32, [_vp, _i32, _u32, _u32, _vp, _vp, _vp]),

@@ -6,9 +6,12 @@
 
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/jpxlg.h"
@@ -51,6 +54,17 @@ struct jpxlg_ctx {
   // of a similar size (size -> device pointer); freed with the context.
   std::multimap<size_t, void*> spare;
   size_t spare_bytes = 0;
+  // Ring of upload slots for the probe chain's descriptors (device buffer,
+  // host staging, completion event); a slot is reused only after its event.
+  struct Slot {
+    char* dev = nullptr;
+    char* host = nullptr;  // pinned staging of the same size
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+  };
+  Slot slots[8];
+  int next_slot = 0;
 };
 
 namespace {
@@ -300,6 +314,11 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
     (void)hipFree(kv.second.lut);
   }
   for (auto& kv : ctx->spare) (void)hipFree(kv.second);
+  for (auto& sl : ctx->slots) {
+    if (sl.dev) (void)hipFree(sl.dev);
+    if (sl.host) (void)hipHostFree(sl.host);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+  }
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
@@ -736,24 +755,259 @@ static int check_quantizer(jpxlg_ctx* ctx, const jpxlg_quantizer* q) {
   return JPXLG_OK;
 }
 
-static int enqueue_quantize(jpxlg_img* img, const jpxlg_quantizer* q, int sharpness) {
-  jpxlg_ctx* ctx = img->ctx;
+// ----------------------------------------------------------------------------
+// The probe chain over a batch of images (one image is a batch of one).
+// ----------------------------------------------------------------------------
+
+namespace {
+
+// Per-slot layout: descs, probe params, error words, workgroup offsets, and
+// the 18-per-scale metric sums of every image.
+struct ChainLayout {
+  size_t desc = 0, pp = 0, err = 0, off = 0, sums = 0, total = 0;
+  int n = 0;
+  // offset arrays (each n+1 ints, except pool: 6n+1)
+  int q_off[3], r_off[3], g_off, e_off, c_off[kDescScales], p_off;
+  int noffs = 0;
+};
+
+ChainLayout chain_layout(int n) {
+  ChainLayout L;
+  L.n = n;
+  auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+  size_t o = 0;
+  L.desc = o; o = al(o + sizeof(ProbeDesc) * n);
+  L.pp = o; o = al(o + sizeof(ProbeParams) * n);
+  L.err = o; o = al(o + 4 * n);
+  int k = 0;
+  for (int i = 0; i < 3; i++) { L.q_off[i] = k; k += n + 1; }
+  for (int i = 0; i < 3; i++) { L.r_off[i] = k; k += n + 1; }
+  L.g_off = k; k += n + 1;
+  L.e_off = k; k += n + 1;
+  for (int i = 0; i < kDescScales; i++) { L.c_off[i] = k; k += n + 1; }
+  L.p_off = k; k += kDescScales * n + 1;
+  L.noffs = k;
+  L.off = o; o = al(o + 4 * (size_t)k);
+  L.sums = o; o = al(o + sizeof(double) * 18 * kDescScales * n);
+  L.total = o;
+  return L;
+}
+
+}  // namespace
+
+// Takes the next upload slot with room for `bytes` (waiting for its last use).
+static int take_slot(jpxlg_ctx* ctx, size_t bytes, jpxlg_ctx::Slot** out) {
+  jpxlg_ctx::Slot& sl = ctx->slots[ctx->next_slot];
+  ctx->next_slot = (ctx->next_slot + 1) % 8;
+  if (sl.pending) HIP_TRY(ctx, hipEventSynchronize(sl.done));
+  sl.pending = false;
+  if (!sl.done) HIP_TRY(ctx, hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
+  if (sl.cap < bytes) {
+    if (sl.dev) (void)hipFree(sl.dev);
+    if (sl.host) (void)hipHostFree(sl.host);
+    sl.dev = sl.host = nullptr;
+    sl.cap = 0;
+    const size_t cap = std::max(bytes, (size_t)64 << 10);
+    if (hipMalloc(&sl.dev, cap) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "chain slot allocation failed");
+    if (hipHostMalloc(&sl.host, cap, hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, JPXLG_E_OOM, "chain slot staging allocation failed");
+    sl.cap = cap;
+  }
+  *out = &sl;
+  return JPXLG_OK;
+}
+
+struct ChainRun {
+  jpxlg_ctx::Slot* slot = nullptr;
+  ChainLayout L;
+};
+
+// Enqueues quantize (with each image's quantizer in qs, when qs is non-null)
+// and/or reconstruction + restoration + SSIMULACRA2 (when r is non-null) for
+// n images of one context, with one descriptor upload; the sums and error
+// words come back with read_chain.
+static int enqueue_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
+                         const jpxlg_restoration* r, ChainRun* run) {
   hipStream_t st = ctx->stream;
-  ProbeParams pp;
-  build_probe_params(*q, sharpness, &pp);
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
-  const uint32_t zero = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
-  static const char* kQNames[3] = {"quantize_8", "quantize_16", "quantize_32"};
+  const ChainLayout L = chain_layout(n);
+  jpxlg_ctx::Slot* sl = nullptr;
+  int rc = take_slot(ctx, L.total, &sl);
+  if (rc) return rc;
+  char* H = sl->host;
+  memset(H, 0, L.total);
+  ProbeDesc* hd = (ProbeDesc*)(H + L.desc);
+  ProbeParams* hp = (ProbeParams*)(H + L.pp);
+  int* ho = (int*)(H + L.off);
+  char* dev = sl->dev;
+  int nsteps = 0, steps[3] = {0, 0, 0};
+  if (r) {
+    if (r->epf_iters == 1) { steps[0] = 1; nsteps = 1; }
+    else if (r->epf_iters == 2) { steps[0] = 1; steps[1] = 2; nsteps = 2; }
+    else if (r->epf_iters >= 3) { steps[0] = 0; steps[1] = 1; steps[2] = 2; nsteps = 3; }
+  }
+  const int gab = r && r->gaborish ? 1 : 0;
+  int tot_q[3] = {0, 0, 0}, tot_r[3] = {0, 0, 0}, tot_g = 0, tot_e = 0, tot_c[kDescScales] = {0}, tot_p = 0;
+  for (int k = 0; k < n; k++) {
+    jpxlg_img* img = imgs[k];
+    const Geom& g = img->g;
+    const jpxlg_quantizer& q = qs ? qs[k] : img->last_q;
+    const int sharp = r ? r->sharpness : img->last_sharpness;
+    build_probe_params(q, sharp, &hp[k]);
+    ProbeDesc& P = hd[k];
+    P.g = g;
+    P.nscales = img->nscales;
+    for (int s = 0; s < kDescScales; s++) {
+      P.sw[s] = s < img->nscales ? img->sw[s] : 0;
+      P.sh[s] = s < img->nscales ? img->sh[s] : 0;
+    }
+    for (int i = 0; i < 3; i++) {
+      P.list[i] = img->d_list[i];
+      P.count[i] = img->list_count[i];
+      P.coeff[i] = img->coeff[i];
+      P.quant[i] = img->quant[i];
+      P.lf[i] = img->lf[i];
+      P.rx[i] = img->rx[i];
+      P.aux[i] = img->aux[i];
+    }
+    P.sigma = img->sigma;
+    P.cfl_x = img->d_cfl_x;
+    P.cfl_b = img->d_cfl_b;
+    P.cflp = img->d_cflp;
+    P.pp = (const ProbeParams*)(dev + L.pp) + k;
+    P.err = (uint32_t*)(dev + L.err) + k;
+    int pt = 0;
+    for (int s = 0; s < img->nscales; s++)
+      for (int c = 0; c < 3; c++) {
+        P.ref_x[s][c] = img->ref_x[s][c];
+        P.ref_mu[s][c] = img->ref_mu[s][c];
+        P.ref_s11[s][c] = img->ref_s11[s][c];
+        P.cand_x[s][c] = img->cand_x[s][c];
+        P.cand_rgb[s][c] = img->cand_rgb[s][c];
+      }
+    for (int s = 0; s < img->nscales; s++) {
+      P.ptile_off[s] = pt;
+      pt += pool_tiles(img->sw[s], img->sh[s]);
+    }
+    P.partials = img->d_partials;
+    P.sums = (double*)(dev + L.sums) + (size_t)k * 18 * kDescScales;
+    DepthSet dd;
+    rc = get_depth(ctx, img->bits, &dd);
+    if (rc) return rc;
+    P.thr = dd.thr;
+    P.lut = dd.lut;
+    P.levels = dd.levels;
+    // workgroup prefixes
+    for (int i = 0; i < 3; i++) {
+      ho[L.q_off[i] + k] = tot_q[i];
+      tot_q[i] += img->list_count[i] ? quant_groups(kSides[i], img->list_count[i]) : 0;
+      ho[L.r_off[i] + k] = tot_r[i];
+      tot_r[i] += img->list_count[i] ? render_groups(kSides[i], img->list_count[i]) : 0;
+    }
+    ho[L.g_off + k] = tot_g;
+    tot_g += gaborish_blocks(g.w, g.h);
+    ho[L.e_off + k] = tot_e;
+    tot_e += epf_tiles(g.w, g.h);
+    for (int s = 0; s < kDescScales; s++) {
+      ho[L.c_off[s] + k] = tot_c[s];
+      if (s < img->nscales) tot_c[s] += colour_blocks(img->sw[s], img->sh[s]);
+    }
+    for (int s = 0; s < kDescScales; s++) {
+      ho[L.p_off + k * kDescScales + s] = tot_p;
+      if (s < img->nscales) tot_p += pool_tiles(img->sw[s], img->sh[s]);
+    }
+  }
   for (int i = 0; i < 3; i++) {
-    if (!img->list_count[i]) continue;
-    Prof p(ctx, kQNames[i]);
-    launch_quantize(st, kSides[i], img->g, img->d_list[i], img->list_count[i], ctx->d_tab, img->d_pp,
-                    img->d_cflp, img->d_cfl_x, img->d_cfl_b, img->coeff, img->quant, img->lf, img->d_err);
+    ho[L.q_off[i] + n] = tot_q[i];
+    ho[L.r_off[i] + n] = tot_r[i];
+  }
+  ho[L.g_off + n] = tot_g;
+  ho[L.e_off + n] = tot_e;
+  for (int s = 0; s < kDescScales; s++) ho[L.c_off[s] + n] = tot_c[s];
+  ho[L.p_off + kDescScales * n] = tot_p;
+  HIP_TRY(ctx, hipMemcpyAsync(dev, H, L.off + 4 * (size_t)L.noffs, hipMemcpyHostToDevice, st));
+  const ProbeDesc* D = (const ProbeDesc*)(dev + L.desc);
+  const int* O = (const int*)(dev + L.off);
+  if (qs) {
+    static const char* kQNames[3] = {"quantize_8", "quantize_16", "quantize_32"};
+    for (int i = 0; i < 3; i++) {
+      if (!tot_q[i]) continue;
+      Prof p(ctx, kQNames[i]);
+      launch_quantize(st, kSides[i], D, O + L.q_off[i], n, tot_q[i], ctx->d_tab);
+    }
+  }
+  if (r) {
+    static const char* kRNames[3] = {"render_8", "render_16", "render_32"};
+    const int form = imgs[0]->dct_form;
+    for (int i = 0; i < 3; i++) {
+      if (!tot_r[i]) continue;
+      Prof p(ctx, kRNames[i]);
+      launch_render(st, kSides[i], form, D, O + L.r_off[i], n, tot_r[i], ctx->d_tab, ctx->d_mats);
+    }
+    int src = 0;
+    if (gab) {
+      { Prof p(ctx, "gaborish"); launch_gaborish_b(st, D, O + L.g_off, n, tot_g, src); }
+      src ^= 1;
+    }
+    static const char* kEpfNames[3] = {"epf_step0", "epf_step1", "epf_step2"};
+    for (int i = 0; i < nsteps; i++) {
+      { Prof p(ctx, kEpfNames[steps[i]]); launch_epf_b(st, steps[i], D, O + L.e_off, n, tot_e, src); }
+      src ^= 1;
+    }
+    for (int s = 0; s < kDescScales; s++) {
+      if (!tot_c[s]) continue;
+      Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour");
+      launch_colour_b(st, s, D, O + L.c_off[s], n, tot_c[s], src);
+    }
+    { Prof p(ctx, "blur_pool"); launch_blur_pool_b(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, tot_p); }
+    { Prof p(ctx, "pool_reduce"); launch_reduce_b(st, D, n); }
+    // Keep the restored planes in rx (downloads read them there).
+    if (src)
+      for (int k = 0; k < n; k++)
+        for (int c = 0; c < 3; c++) std::swap(imgs[k]->rx[c], imgs[k]->aux[c]);
   }
   HIP_TRY(ctx, hipGetLastError());
-  img->last_q = *q;
-  img->last_sharpness = sharpness;
+  // error words and sums come back to the slot's host staging
+  HIP_TRY(ctx, hipMemcpyAsync(H + L.err, dev + L.err, L.off - L.err, hipMemcpyDeviceToHost, st));
+  if (r) HIP_TRY(ctx, hipMemcpyAsync(H + L.sums, dev + L.sums, L.total - L.sums, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipEventRecord(sl->done, st));
+  sl->pending = true;
+  for (int k = 0; k < n; k++) {
+    if (qs) {
+      imgs[k]->last_q = qs[k];
+      imgs[k]->has_quant = true;
+    }
+    if (r) imgs[k]->last_sharpness = r->sharpness;
+  }
+  run->slot = sl;
+  run->L = L;
+  return JPXLG_OK;
+}
+
+// Waits for a chain and reads back each image's error word and, when it
+// scored, its metric (scores[k], optional raw/terms of image 0).
+static int read_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const ChainRun& run, bool scored,
+                      double* scores, double* raw0, double* terms0, uint32_t* nterms0) {
+  HIP_TRY(ctx, hipEventSynchronize(run.slot->done));
+  run.slot->pending = false;
+  const char* H = run.slot->host;
+  const uint32_t* err = (const uint32_t*)(H + run.L.err);
+  int bad = 0;
+  for (int k = 0; k < n; k++) bad |= err[k] != 0;
+  if (scored) {
+    for (int k = 0; k < n; k++) {
+      jpxlg_img* img = imgs[k];
+      const double* sums = (const double*)(H + run.L.sums) + (size_t)k * 18 * kDescScales;
+      MetricResult m = pool_terms(sums, img->nscales, img->sw, img->sh);
+      if (scores) scores[k] = m.score;
+      if (k == 0) {
+        if (raw0) *raw0 = m.raw_error;
+        if (terms0) memcpy(terms0, m.terms, sizeof(double) * 18 * img->nscales);
+        if (nterms0) *nterms0 = (uint32_t)(18 * img->nscales);
+      }
+      img->has_render = true;
+    }
+  }
+  if (bad) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
   return JPXLG_OK;
 }
 
@@ -764,99 +1018,11 @@ extern "C" int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q) {
   int rc = check_quantizer(ctx, q);
   if (rc) return rc;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  rc = enqueue_quantize(img, q, img->last_sharpness);
+  ChainRun run;
+  rc = enqueue_chain(ctx, &img, 1, q, nullptr, &run);
   if (rc) return rc;
-  uint32_t err = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
-  img->has_quant = true;
   img->has_render = false;
-  return JPXLG_OK;
-}
-
-// Enqueues reconstruction + metric for the last quantized plan; the 6 x 18
-// sums land in img->d_sums.
-static int enqueue_render_score(jpxlg_img* img, const jpxlg_restoration* r) {
-  jpxlg_ctx* ctx = img->ctx;
-  hipStream_t st = ctx->stream;
-  const Geom& g = img->g;
-  if (r->sharpness != img->last_sharpness) {
-    // sigma depends on the sharpness: refresh the probe parameters.
-    ProbeParams pp;
-    build_probe_params(img->last_q, r->sharpness, &pp);
-    HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
-    img->last_sharpness = r->sharpness;
-  }
-  static const char* kRNames[3] = {"render_8", "render_16", "render_32"};
-  for (int i = 0; i < 3; i++) {
-    if (!img->list_count[i]) continue;
-    Prof p(ctx, kRNames[i]);
-    launch_render(st, kSides[i], img->dct_form, g, img->d_list[i], img->list_count[i], ctx->d_tab, ctx->d_mats,
-                  img->d_pp, img->d_cflp, img->d_cfl_x, img->d_cfl_b, img->quant, img->lf, img->rx, img->sigma);
-  }
-  float* cur[3] = {img->rx[0], img->rx[1], img->rx[2]};
-  float* oth[3] = {img->aux[0], img->aux[1], img->aux[2]};
-  if (r->gaborish) {
-    { Prof p(ctx, "gaborish"); launch_gaborish(st, g.w, g.h, cur, oth); }
-    std::swap(cur, oth);
-  }
-  int steps[3], ns = 0;
-  if (r->epf_iters == 1) { steps[0] = 1; ns = 1; }
-  else if (r->epf_iters == 2) { steps[0] = 1; steps[1] = 2; ns = 2; }
-  else if (r->epf_iters >= 3) { steps[0] = 0; steps[1] = 1; steps[2] = 2; ns = 3; }
-  for (int i = 0; i < ns; i++) {
-    static const char* kEpfNames[3] = {"epf_step0", "epf_step1", "epf_step2"};
-    { Prof p(ctx, kEpfNames[steps[i]]); launch_epf(st, steps[i], g.w, g.h, cur, img->sigma, oth); }
-    std::swap(cur, oth);
-  }
-  // Keep the restored planes in rx (downloads read them there).
-  if (cur[0] != img->rx[0])
-    for (int c = 0; c < 3; c++) std::swap(img->rx[c], img->aux[c]);
-  DepthSet d;
-  int rc = get_depth(ctx, img->bits, &d);
-  if (rc) return rc;
-  PoolScales P{};
-  P.L.nscales = img->nscales;
-  for (int s = 0; s < img->nscales; s++) {
-    float** down = s + 1 < img->nscales ? img->cand_rgb[s + 1] : nullptr;
-    if (s == 0) {
-      Prof p(ctx, "xyb_inv_depth_metric0");
-      launch_colour_metric(st, 0, img->sw[0], img->sh[0], img->rx, d.thr, d.lut, d.levels,
-                           img->cand_x[0], down);
-    } else {
-      Prof p(ctx, "metric_colour");
-      launch_colour_metric(st, 1, img->sw[s], img->sh[s], img->cand_rgb[s], nullptr, nullptr, 0,
-                           img->cand_x[s], down);
-    }
-    P.w[s] = img->sw[s];
-    P.h[s] = img->sh[s];
-    for (int c = 0; c < 3; c++) {
-      P.x1[s][c] = img->ref_x[s][c];
-      P.mu1[s][c] = img->ref_mu[s][c];
-      P.s11[s][c] = img->ref_s11[s][c];
-      P.x2[s][c] = img->cand_x[s][c];
-    }
-  }
-  { Prof p(ctx, "blur_pool"); launch_blur_pool(st, ctx->d_tab, P, img->d_partials); }
-  { Prof p(ctx, "pool_reduce"); launch_reduce_scales(st, P.L, img->d_partials, img->d_sums); }
-  HIP_TRY(ctx, hipGetLastError());
-  return JPXLG_OK;
-}
-
-static int finish_score(jpxlg_img* img, double* score, double* raw, double* terms, uint32_t* nterms) {
-  jpxlg_ctx* ctx = img->ctx;
-  double sums[kMaxScales * 18];
-  HIP_TRY(ctx, hipMemcpyAsync(sums, img->d_sums, sizeof(double) * 18 * img->nscales,
-                              hipMemcpyDeviceToHost, ctx->stream));
-  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
-  MetricResult m = pool_terms(sums, img->nscales, img->sw, img->sh);
-  if (score) *score = m.score;
-  if (raw) *raw = m.raw_error;
-  if (terms) memcpy(terms, m.terms, sizeof(double) * 18 * img->nscales);
-  if (nterms) *nterms = (uint32_t)(18 * img->nscales);
-  img->has_render = true;
-  return JPXLG_OK;
+  return read_chain(ctx, &img, 1, run, false, nullptr, nullptr, nullptr, nullptr);
 }
 
 extern "C" int32_t jpxlg_render_score(jpxlg_img* img, const jpxlg_restoration* r, double* score,
@@ -867,9 +1033,10 @@ extern "C" int32_t jpxlg_render_score(jpxlg_img* img, const jpxlg_restoration* r
   if (r->epf_iters < 0 || r->epf_iters > 3) return fail(ctx, JPXLG_E_UNSUPPORTED, "epf_iters above 3");
   if (r->sharpness < 0 || r->sharpness > 7) return fail(ctx, JPXLG_E_RANGE, "sharpness outside 0..7");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  int rc = enqueue_render_score(img, r);
+  ChainRun run;
+  int rc = enqueue_chain(ctx, &img, 1, nullptr, r, &run);
   if (rc) return rc;
-  return finish_score(img, score, raw_error, terms, nterms);
+  return read_chain(ctx, &img, 1, run, true, score, raw_error, terms, nterms);
 }
 
 // One probe: quantize + reconstruct + score, one host synchronisation.
@@ -877,17 +1044,10 @@ int jpxlg_probe(jpxlg_img* img, const jpxlg_quantizer* q, const jpxlg_restoratio
   jpxlg_ctx* ctx = img->ctx;
   int rc = check_quantizer(ctx, q);
   if (rc) return rc;
-  rc = enqueue_quantize(img, q, r->sharpness);
+  ChainRun run;
+  rc = enqueue_chain(ctx, &img, 1, q, r, &run);
   if (rc) return rc;
-  img->has_quant = true;
-  rc = enqueue_render_score(img, r);
-  if (rc) return rc;
-  uint32_t err = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
-  rc = finish_score(img, score, nullptr, nullptr, nullptr);
-  if (rc) return rc;
-  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
-  return JPXLG_OK;
+  return read_chain(ctx, &img, 1, run, true, score, nullptr, nullptr, nullptr);
 }
 
 // ----------------------------------------------------------------------------
@@ -1011,6 +1171,292 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
     out->probe_ms[i] = oc.probes[i].ms;
   }
   return JPXLG_OK;
+}
+
+// ----------------------------------------------------------------------------
+// Batches: many images of one context advance their probes together, one
+// launch per stage per iteration.
+// ----------------------------------------------------------------------------
+
+static int batch_check(uint32_t n, jpxlg_img* const* imgs) {
+  if (!imgs || n == 0) return JPXLG_E_INVALID;
+  for (uint32_t k = 0; k < n; k++)
+    if (!imgs[k] || imgs[k]->ctx != imgs[0]->ctx) return JPXLG_E_INVALID;
+  for (uint32_t k = 0; k < n; k++)
+    if (imgs[k]->dct_form != imgs[0]->dct_form)
+      return fail(imgs[0]->ctx, JPXLG_E_UNSUPPORTED, "a batch mixes DCT forms");
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_batch_probe(uint32_t n, jpxlg_img* const* imgs, const jpxlg_quantizer* qs,
+                                     const jpxlg_restoration* r, double* scores) {
+  int rc = batch_check(n, imgs);
+  if (rc) return rc;
+  jpxlg_ctx* ctx = imgs[0]->ctx;
+  if (!qs || !r || !scores) return fail(ctx, JPXLG_E_INVALID, "null argument");
+  for (uint32_t k = 0; k < n; k++) {
+    if (!imgs[k]->has_structure) return fail(ctx, JPXLG_E_INVALID, "an image has no plan structure yet");
+    rc = check_quantizer(ctx, &qs[k]);
+    if (rc) return rc;
+  }
+  if (r->epf_iters < 0 || r->epf_iters > 3) return fail(ctx, JPXLG_E_UNSUPPORTED, "epf_iters above 3");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  ChainRun run;
+  rc = enqueue_chain(ctx, imgs, (int)n, qs, r, &run);
+  if (rc) return rc;
+  return read_chain(ctx, imgs, (int)n, run, true, scores, nullptr, nullptr, nullptr);
+}
+
+// ----------------------------------------------------------------------------
+// Continuous batching: a server of quality requests.  Each request runs its
+// own navigator thread; step() gathers the pending probe of every active
+// request (building the structure of new ones), runs them as one batched
+// chain, and completes the requests whose navigators finished.  Requests
+// can be submitted between steps and join the next round.
+// ----------------------------------------------------------------------------
+
+struct ServerReq {
+  jpxlg_img* img = nullptr;
+  jpxlg_quality_request q{};
+  QualityPolicy pol{};
+  std::thread th;
+  bool anchored = false;
+  bool pending = false;
+  uint32_t rung = 0;
+  double score = 0.0;
+  int rc = 0;
+  bool finished = false;  // navigator returned
+  bool done = false;      // result available
+  int nav_rc = 0;
+  ControllerOutcome oc;
+  jpxlg_quality_result res{};
+};
+
+struct jpxlg_server {
+  jpxlg_ctx* ctx = nullptr;
+  std::mutex m;
+  std::condition_variable cv_req, cv_done;
+  int active = 0, waiting = 0;
+  uint64_t next_id = 1;
+  std::map<uint64_t, std::unique_ptr<ServerReq>> reqs;
+};
+
+extern "C" int32_t jpxlg_server_create(jpxlg_ctx* ctx, jpxlg_server** out) {
+  if (!ctx || !out) return JPXLG_E_INVALID;
+  *out = new jpxlg_server();
+  (*out)->ctx = ctx;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quality_request* q,
+                                       uint64_t* ticket) {
+  if (!srv || !img || !q || !ticket) return JPXLG_E_INVALID;
+  if (img->ctx != srv->ctx) return fail(srv->ctx, JPXLG_E_INVALID, "the image belongs to another context");
+  auto r = std::make_unique<ServerReq>();
+  ServerReq* R = r.get();
+  R->img = img;
+  R->q = *q;
+  R->pol = quality_policy(q->effort);
+  std::lock_guard<std::mutex> lk(srv->m);
+  const uint64_t id = srv->next_id++;
+  srv->reqs.emplace(id, std::move(r));
+  srv->active++;
+  R->th = std::thread([srv, R]() {
+    ProbeFn probe = [srv, R](uint32_t rung, double* score, double* ms) -> int {
+      auto t0 = std::chrono::steady_clock::now();
+      std::unique_lock<std::mutex> lk(srv->m);
+      R->rung = rung;
+      R->pending = true;
+      srv->waiting++;
+      srv->cv_req.notify_all();
+      srv->cv_done.wait(lk, [R] { return !R->pending; });
+      *score = R->score;
+      *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+      return R->rc;
+    };
+    ControllerOutcome oc;
+    const int rc = R->q.mode == 1 ? bisect_controller(R->q.target, R->q.max_probes, probe, &oc)
+                                  : navigate_controller(R->q.target, R->q.effort, R->q.max_probes, probe, &oc);
+    std::lock_guard<std::mutex> lk(srv->m);
+    R->oc = oc;
+    R->nav_rc = rc;
+    R->finished = true;
+    srv->active--;
+    srv->cv_req.notify_all();
+  });
+  *ticket = id;
+  return JPXLG_OK;
+}
+
+// One round: returns the number of requests completed by it (>= 0), or a
+// negative status.
+extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
+  if (!srv) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = srv->ctx;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  std::vector<ServerReq*> probing, finishing;
+  {
+    std::unique_lock<std::mutex> lk(srv->m);
+    srv->cv_req.wait(lk, [srv] { return srv->waiting == srv->active; });
+    for (auto& kv : srv->reqs) {
+      ServerReq* R = kv.second.get();
+      if (R->pending) probing.push_back(R);
+      else if (R->finished && !R->done) finishing.push_back(R);
+    }
+  }
+  int status = JPXLG_OK;
+  // Completed searches: the chosen plans quantized in one batched launch.
+  std::vector<jpxlg_img*> fimgs;
+  std::vector<jpxlg_quantizer> fqs;
+  for (ServerReq* R : finishing) {
+    jpxlg_quality_result& o = R->res;
+    memset(&o, 0, sizeof(o));
+    if (R->nav_rc) continue;
+    const jpxlg_quantizer qc = R->pol.quantizer_at(R->oc.rung);
+    fimgs.push_back(R->img);
+    fqs.push_back(qc);
+    o.score = R->oc.score;
+    o.rung = R->oc.rung;
+    o.global_scale = qc.global_scale;
+    o.hf_mul = qc.hf_mul;
+    o.quant_lf = qc.quant_lf;
+    o.met = R->oc.met;
+    o.probes = (int32_t)R->oc.probes.size();
+    for (size_t i = 0; i < R->oc.probes.size() && i < 16; i++) {
+      o.probe_scores[i] = R->oc.probes[i].score;
+      o.probe_rungs[i] = R->oc.probes[i].rung;
+      o.probe_ms[i] = R->oc.probes[i].ms;
+    }
+  }
+  if (!fimgs.empty()) {
+    ChainRun run;
+    int qrc = enqueue_chain(ctx, fimgs.data(), (int)fimgs.size(), fqs.data(), nullptr, &run);
+    if (!qrc) qrc = read_chain(ctx, fimgs.data(), (int)fimgs.size(), run, false, nullptr, nullptr, nullptr, nullptr);
+    if (qrc) status = qrc;
+  }
+  // The round of probes: structures of new requests, then one chain per DCT
+  // form present (a batch shares its form).
+  std::vector<int> prc(probing.size(), 0);
+  std::vector<double> pscore(probing.size(), 0.0);
+  for (int form = kDctExact; form <= kDctBf16x3; form++) {
+    for (int effort = 0; effort <= 1; effort++) {
+      std::vector<jpxlg_img*> bimgs;
+      std::vector<jpxlg_quantizer> bqs;
+      std::vector<size_t> bt;
+      for (size_t t = 0; t < probing.size(); t++) {
+        ServerReq* R = probing[t];
+        if (R->img->dct_form != form || (R->q.effort == JPXLG_EFFORT_FAST ? 0 : 1) != effort) continue;
+        const jpxlg_quantizer q = R->pol.quantizer_at(R->rung);
+        if (!R->anchored) {
+          prc[t] = jpxlg_plan_structure(R->img, &q, R->pol.cover_mode, 1);
+          if (prc[t]) continue;
+          R->anchored = true;
+        }
+        bimgs.push_back(R->img);
+        bqs.push_back(q);
+        bt.push_back(t);
+      }
+      if (bimgs.empty()) continue;
+      const QualityPolicy& pol = probing[bt[0]]->pol;
+      jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
+      std::vector<double> sc(bimgs.size(), 0.0);
+      ChainRun run;
+      int brc = enqueue_chain(ctx, bimgs.data(), (int)bimgs.size(), bqs.data(), &r, &run);
+      if (!brc) brc = read_chain(ctx, bimgs.data(), (int)bimgs.size(), run, true, sc.data(), nullptr, nullptr, nullptr);
+      for (size_t b = 0; b < bt.size(); b++) {
+        pscore[bt[b]] = sc[b];
+        prc[bt[b]] = brc;
+      }
+    }
+  }
+  {
+    std::lock_guard<std::mutex> lk(srv->m);
+    for (size_t t = 0; t < probing.size(); t++) {
+      ServerReq* R = probing[t];
+      R->score = pscore[t];
+      R->rc = prc[t];
+      R->pending = false;
+      srv->waiting--;
+    }
+    for (ServerReq* R : finishing) R->done = true;
+  }
+  srv->cv_done.notify_all();
+  if (status) return status;
+  return (int32_t)finishing.size();
+}
+
+extern "C" int32_t jpxlg_server_poll(jpxlg_server* srv, uint64_t ticket, jpxlg_quality_result* out) {
+  if (!srv) return JPXLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->m);
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end()) return fail(srv->ctx, JPXLG_E_INVALID, "unknown ticket");
+  ServerReq* R = it->second.get();
+  if (!R->done) return 0;
+  if (out) *out = R->res;
+  return R->nav_rc ? R->nav_rc : 1;
+}
+
+extern "C" int32_t jpxlg_server_release(jpxlg_server* srv, uint64_t ticket) {
+  if (!srv) return JPXLG_E_INVALID;
+  std::unique_ptr<ServerReq> r;
+  {
+    std::lock_guard<std::mutex> lk(srv->m);
+    auto it = srv->reqs.find(ticket);
+    if (it == srv->reqs.end()) return fail(srv->ctx, JPXLG_E_INVALID, "unknown ticket");
+    if (!it->second->done) return fail(srv->ctx, JPXLG_E_INVALID, "the request is still running");
+    r = std::move(it->second);
+    srv->reqs.erase(it);
+  }
+  if (r->th.joinable()) r->th.join();
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_pending(jpxlg_server* srv) {
+  if (!srv) return JPXLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->m);
+  int n = 0;
+  for (auto& kv : srv->reqs) n += kv.second->done ? 0 : 1;
+  return n;
+}
+
+extern "C" void jpxlg_server_destroy(jpxlg_server* srv) {
+  if (!srv) return;
+  // Drive every outstanding request to completion before the threads go.
+  while (jpxlg_server_pending(srv) > 0)
+    if (jpxlg_server_step(srv) < 0) break;
+  for (auto& kv : srv->reqs)
+    if (kv.second->th.joinable()) kv.second->th.join();
+  delete srv;
+}
+
+extern "C" int32_t jpxlg_batch_encode_quality(uint32_t n, jpxlg_img* const* imgs,
+                                              const jpxlg_quality_request* reqs, jpxlg_quality_result* out) {
+  int rc = batch_check(n, imgs);
+  if (rc) return rc;
+  jpxlg_ctx* ctx = imgs[0]->ctx;
+  if (!reqs || !out) return fail(ctx, JPXLG_E_INVALID, "null argument");
+  jpxlg_server srv;
+  srv.ctx = ctx;
+  std::vector<uint64_t> tickets(n);
+  for (uint32_t k = 0; k < n; k++) {
+    rc = jpxlg_server_submit(&srv, imgs[k], &reqs[k], &tickets[k]);
+    if (rc) break;
+  }
+  int status = rc;
+  while (jpxlg_server_pending(&srv) > 0) {
+    const int st = jpxlg_server_step(&srv);
+    if (st < 0) {
+      if (!status) status = st;
+      // keep stepping so every navigator thread ends
+    }
+  }
+  for (uint32_t k = 0; k < n && !rc; k++) {
+    const int p = jpxlg_server_poll(&srv, tickets[k], &out[k]);
+    if (p < 0 && !status) status = p;
+  }
+  for (auto& kv : srv.reqs)
+    if (kv.second->th.joinable()) kv.second->th.join();
+  return status;
 }
 
 // ----------------------------------------------------------------------------

@@ -62,6 +62,52 @@ struct CflLfScratch {
   unsigned long long cost[2][4];
 };
 
+// One image of a probe batch: everything the per-probe chain (quantize,
+// render, restoration, colour, blur + pooling, reduction) reads or writes.
+// A probe of one image is a batch of one.
+constexpr int kDescScales = 6;
+struct ProbeDesc {
+  Geom g;
+  int nscales;
+  int sw[kDescScales], sh[kDescScales];
+  const uint32_t* list[3];  // varblock lists per size, entries (by << 16) | bx
+  int count[3];
+  const float* coeff[3];
+  int32_t* quant[3];
+  int32_t* lf[3];
+  float* rx[3];             // render planes and their restoration ping-pong
+  float* aux[3];
+  float* sigma;
+  const int8_t* cfl_x;
+  const int8_t* cfl_b;
+  const CflParams* cflp;
+  const ProbeParams* pp;
+  uint32_t* err;
+  const float* ref_x[kDescScales][3];
+  const float* ref_mu[kDescScales][3];
+  const float* ref_s11[kDescScales][3];
+  float* cand_x[kDescScales][3];
+  float* cand_rgb[kDescScales][3];  // [0] unused here
+  double* partials;                 // 18 f64 per pooling tile, scale after scale
+  int ptile_off[kDescScales];       // first tile of each scale in `partials`
+  double* sums;                     // 18 f64 per scale
+  const float* thr;                 // depth quantizer thresholds and levels
+  const float* lut;
+  int levels;
+};
+
+// Workgroup prefix sums of one batched launch: the workgroups of desc d are
+// [off[d], off[d+1]).  The desc of a workgroup is found by binary search.
+__device__ __forceinline__ int find_desc(const int* __restrict__ off, int nd, int b) {
+  int lo = 0, hi = nd;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // Error word bits raised by kernels (device memory).
 enum : uint32_t { kErrQuantRange = 1u };
 
@@ -77,14 +123,26 @@ void launch_dct_fwd(hipStream_t s, int side, int form, const Geom& g, const uint
                     const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]);
 void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const ConstTables* tab,
                        const float* in, float* out);
-void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
-                     const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
-                     const int8_t* cfl_x, const int8_t* cfl_b, float* const coeff[3],
-                     int32_t* quant[3], int32_t* lf[3], uint32_t* err);
-void launch_render(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
-                   const ConstTables* tab, const DctMats* mats, const ProbeParams* pp, const CflParams* cfl,
-                   const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3],
-                   int32_t* const lf[3], float* xyb[3], float* sigma);
+// The per-probe chain over a batch of ProbeDescs (one image = a batch of
+// one).  `off` (device) holds the workgroup prefix of each desc, `nwg` the
+// total; src selects the restoration ping-pong input (0: rx, 1: aux).
+void launch_quantize(hipStream_t s, int side, const ProbeDesc* D, const int* off, int nd, int nwg,
+                     const ConstTables* tab);
+void launch_render(hipStream_t s, int side, int form, const ProbeDesc* D, const int* off, int nd, int nwg,
+                   const ConstTables* tab, const DctMats* mats);
+void launch_gaborish_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
+void launch_epf_b(hipStream_t s, int step, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
+void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
+// off over the flattened (desc, scale) index d * kDescScales + scale.
+void launch_blur_pool_b(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
+                        int nwg);
+void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd);
+// Workgroup counts of the batched stages for one image.
+int quant_groups(int side, int count);
+int render_groups(int side, int count);
+int epf_tiles(int w, int h);
+int gaborish_blocks(int w, int h);
+int colour_blocks(int w, int h);
 void launch_fill(hipStream_t s, float* p, float v, size_t n);
 void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]);
 void launch_epf(hipStream_t s, int step, int w, int h, float* const in[3], const float* sigma,

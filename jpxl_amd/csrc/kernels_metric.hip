@@ -76,27 +76,13 @@ __device__ __forceinline__ void opsin_inverse(float x, float y, float b, float r
 
 // One thread per 2x2 quad: converts up to four samples of this scale and
 // writes the next scale's box-filtered linear RGB (pyramid.rs:19-44).
-template <int MODE, bool kLdsThresholds>
-__global__ __launch_bounds__(256) void k_colour_metric(
-    int w, int h, const float* __restrict__ in0, const float* __restrict__ in1,
-    const float* __restrict__ in2, const float* __restrict__ thr_g, const float* __restrict__ lut_g,
-    int levels, float* __restrict__ p0, float* __restrict__ p1, float* __restrict__ p2,
-    float* __restrict__ d0, float* __restrict__ d1, float* __restrict__ d2) {
-  __shared__ float s_thr[kLdsThresholds ? 256 : 1];
-  __shared__ float s_lut[kLdsThresholds ? 256 : 1];
-  const float* thr = thr_g;
-  const float* lut = lut_g;
-  if constexpr (MODE == 0 && kLdsThresholds) {
-    for (int i = threadIdx.x; i < levels; i += blockDim.x) {
-      s_lut[i] = lut_g[i];
-      if (i < levels - 1) s_thr[i] = thr_g[i];
-    }
-    __syncthreads();
-    thr = s_thr;
-    lut = s_lut;
-  }
+template <int MODE>
+__device__ __forceinline__ void colour_quad(int w, int h, const float* __restrict__ in0,
+                                            const float* __restrict__ in1, const float* __restrict__ in2,
+                                            const float* thr, const float* lut, int levels, float* __restrict__ p0,
+                                            float* __restrict__ p1, float* __restrict__ p2, float* __restrict__ d0,
+                                            float* __restrict__ d1, float* __restrict__ d2, int ox, int oy) {
   const int w2 = (w + 1) / 2, h2 = (h + 1) / 2;
-  const int ox = blockIdx.x * blockDim.x + threadIdx.x, oy = blockIdx.y;
   if (ox >= w2 || oy >= h2) return;
   const float root = MODE == 0 ? cbrt_core(-0.0037930733f) : 0.0f;
   float lin[2][2][3];
@@ -129,6 +115,59 @@ __global__ __launch_bounds__(256) void k_colour_metric(
     for (int c = 0; c < 3; c++)
       d[c][o] = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
   }
+}
+
+// Stages depth thresholds of up to 256 levels in LDS (the workgroup's image
+// is uniform); larger tables are read from global memory.
+__device__ __forceinline__ void stage_depth(const float* thr_g, const float* lut_g, int levels, float* s_thr,
+                                            float* s_lut, const float** thr, const float** lut) {
+  *thr = thr_g;
+  *lut = lut_g;
+  if (levels <= 256) {
+    for (int i = threadIdx.x; i < levels; i += blockDim.x) {
+      s_lut[i] = lut_g[i];
+      if (i < levels - 1) s_thr[i] = thr_g[i];
+    }
+    *thr = s_thr;
+    *lut = s_lut;
+  }
+  __syncthreads();
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_colour_metric(
+    int w, int h, const float* __restrict__ in0, const float* __restrict__ in1,
+    const float* __restrict__ in2, const float* __restrict__ thr_g, const float* __restrict__ lut_g,
+    int levels, float* __restrict__ p0, float* __restrict__ p1, float* __restrict__ p2,
+    float* __restrict__ d0, float* __restrict__ d1, float* __restrict__ d2) {
+  __shared__ float s_thr[256], s_lut[256];
+  const float* thr = thr_g;
+  const float* lut = lut_g;
+  if constexpr (MODE == 0) stage_depth(thr_g, lut_g, levels, s_thr, s_lut, &thr, &lut);
+  colour_quad<MODE>(w, h, in0, in1, in2, thr, lut, levels, p0, p1, p2, d0, d1, d2,
+                    blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// Batched colour pass of scale s: workgroup (quad row, 256-quad block) of
+// each desc.  Scale 0 reads the reconstructed XYB (render planes, src 0, or
+// their ping-pong, src 1) through the depth quantizer; later scales read the
+// previous scale's downsampled linear RGB.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_colour_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
+                                                  int nd, int s, int src) {
+  __shared__ float s_thr[256], s_lut[256];
+  const int d = find_desc(off, nd, blockIdx.x);
+  const ProbeDesc& P = D[d];
+  const int w = P.sw[s], h = P.sh[s];
+  const float* thr = P.thr;
+  const float* lut = P.lut;
+  if constexpr (MODE == 0) stage_depth(P.thr, P.lut, P.levels, s_thr, s_lut, &thr, &lut);
+  const int lb = blockIdx.x - off[d], xb = ((w + 1) / 2 + 255) / 256;
+  const float* const* in = MODE == 0 ? (src ? P.aux : P.rx) : P.cand_rgb[s];
+  float* const* dn = s + 1 < P.nscales ? P.cand_rgb[s + 1] : nullptr;
+  colour_quad<MODE>(w, h, in[0], in[1], in[2], thr, lut, P.levels, P.cand_x[s][0], P.cand_x[s][1], P.cand_x[s][2],
+                    dn ? dn[0] : nullptr, dn ? dn[1] : nullptr, dn ? dn[2] : nullptr,
+                    (lb % xb) * 256 + threadIdx.x, lb / xb);
 }
 
 __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __restrict__ x,
@@ -217,6 +256,7 @@ __device__ __forceinline__ double wave_sum(double v) {
 constexpr int kPW = 64, kPH = 24;
 constexpr int kPIW = kPW + 2 * kHalo, kPIH = kPH + 2 * kHalo;  // 72 x 32
 constexpr int kHRun = 8;                                      // horizontal outputs per lane
+__host__ __device__ constexpr int pool_tiles_dev(int w, int h) { return ((w + kPW - 1) / kPW) * ((h + kPH - 1) / kPH); }
 constexpr int kVRun = 6;                                      // vertical outputs per lane
 
 template <int R>
@@ -230,27 +270,19 @@ __device__ __forceinline__ void fir_run(const double (&d)[R + 8], const double* 
   }
 }
 
-__global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict__ tab, PoolScales P,
-                                                   double* __restrict__ partials) {
+// One 64x24 tile of channel c at one scale; writes the tile's 6 partial
+// sums of that channel to out6[0..5].
+__device__ __forceinline__ void blur_pool_tile(const ConstTables* __restrict__ tab, int w, int h, int x0, int y0,
+                                               const float* __restrict__ X1, const float* __restrict__ MU1,
+                                               const float* __restrict__ S11, const float* __restrict__ X2,
+                                               double* __restrict__ out6) {
   __shared__ float t1[kPIH][kPIW];
   __shared__ float t2[kPIH][kPIW];
   __shared__ float hb[3][kPIH][kPW];
   __shared__ double red[4][6];
-  const int c = blockIdx.z;
-  const int tile = blockIdx.x;
-  int sc = 0;
-#pragma unroll
-  for (int i = 1; i < kMaxPoolScales; i++)
-    if (i < P.L.nscales && tile >= P.L.off[i]) sc = i;
-  const int lt = tile - P.L.off[sc], w = P.w[sc], h = P.h[sc];
-  const int x0 = (lt % P.tx[sc]) * kPW, y0 = (lt / P.tx[sc]) * kPH;
   double hf[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
-  const float* X1 = P.x1[sc][c];
-  const float* MU1 = P.mu1[sc][c];
-  const float* S11 = P.s11[sc][c];
-  const float* X2 = P.x2[sc][c];
   // The pooling inputs do not depend on the blurs: issue their loads first.
   const int plx = threadIdx.x & 63, ply0 = (threadIdx.x >> 6) * kVRun;
   float pmu[kVRun], ps11[kVRun];
@@ -353,8 +385,34 @@ __global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict
   __syncthreads();
   if (threadIdx.x < 6) {
     double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-    partials[(size_t)tile * 18 + c * 6 + threadIdx.x] = v;
+    out6[threadIdx.x] = v;
   }
+}
+
+__global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict__ tab, PoolScales P,
+                                                   double* __restrict__ partials) {
+  const int c = blockIdx.z;
+  const int tile = blockIdx.x;
+  int sc = 0;
+#pragma unroll
+  for (int i = 1; i < kMaxPoolScales; i++)
+    if (i < P.L.nscales && tile >= P.L.off[i]) sc = i;
+  const int lt = tile - P.L.off[sc];
+  blur_pool_tile(tab, P.w[sc], P.h[sc], (lt % P.tx[sc]) * kPW, (lt / P.tx[sc]) * kPH, P.x1[sc][c], P.mu1[sc][c],
+                 P.s11[sc][c], P.x2[sc][c], partials + (size_t)tile * 18 + c * 6);
+}
+
+// Batched: workgroups enumerate (desc, scale, tile) through off[] over the
+// flattened index d * kDescScales + scale.
+__global__ __launch_bounds__(256) void k_blur_pool_b(const ConstTables* __restrict__ tab,
+                                                     const ProbeDesc* __restrict__ D, const int* __restrict__ off,
+                                                     int nds) {
+  const int ds = find_desc(off, nds, blockIdx.x);
+  const int d = ds / kDescScales, sc = ds % kDescScales, c = blockIdx.z;
+  const ProbeDesc& P = D[d];
+  const int lt = blockIdx.x - off[ds], w = P.sw[sc], h = P.sh[sc], tx = (w + kPW - 1) / kPW;
+  blur_pool_tile(tab, w, h, (lt % tx) * kPW, (lt / tx) * kPH, P.ref_x[sc][c], P.ref_mu[sc][c], P.ref_s11[sc][c],
+                 P.cand_x[sc][c], P.partials + (size_t)(P.ptile_off[sc] + lt) * 18 + c * 6);
 }
 
 // Fixed-order reduction of every scale's per-tile partial sums in one launch:
@@ -372,6 +430,24 @@ __global__ __launch_bounds__(256) void k_reduce_scales(PoolLayout L, const doubl
   if (threadIdx.x == 0) sums[s * 18 + term] = ((red[0] + red[1]) + red[2]) + red[3];
 }
 
+// Batched reduction: workgroup (desc, scale, term); scales past a desc's
+// count do nothing.
+__global__ __launch_bounds__(256) void k_reduce_b(const ProbeDesc* __restrict__ D) {
+  __shared__ double red[4];
+  const int d = blockIdx.x / (kDescScales * 18), rem = blockIdx.x % (kDescScales * 18);
+  const int s = rem / 18, term = rem % 18;
+  const ProbeDesc& P = D[d];
+  if (s >= P.nscales) return;
+  const int tiles = pool_tiles_dev(P.sw[s], P.sh[s]);
+  const double* p = P.partials + (size_t)P.ptile_off[s] * 18;
+  double acc = 0.0;
+  for (int t = threadIdx.x; t < tiles; t += 256) acc += p[(size_t)t * 18 + term];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) P.sums[s * 18 + term] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
 }  // namespace
 
 void launch_colour_metric(hipStream_t s, int mode, int w, int h, float* const in[3],
@@ -383,14 +459,19 @@ void launch_colour_metric(hipStream_t s, int mode, int w, int h, float* const in
   float* d1 = down ? down[1] : nullptr;
   float* d2 = down ? down[2] : nullptr;
   if (mode == 1)
-    k_colour_metric<1, false><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut,
-                                                   levels, pxyb[0], pxyb[1], pxyb[2], d0, d1, d2);
-  else if (levels <= 256)
-    k_colour_metric<0, true><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut,
-                                                  levels, pxyb[0], pxyb[1], pxyb[2], d0, d1, d2);
+    k_colour_metric<1><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut, levels, pxyb[0],
+                                            pxyb[1], pxyb[2], d0, d1, d2);
   else
-    k_colour_metric<0, false><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut,
-                                                   levels, pxyb[0], pxyb[1], pxyb[2], d0, d1, d2);
+    k_colour_metric<0><<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], thresholds, depth_lut, levels, pxyb[0],
+                                            pxyb[1], pxyb[2], d0, d1, d2);
+}
+
+int colour_blocks(int w, int h) { return (int)(div_up((w + 1) / 2, 256) * ((h + 1) / 2)); }
+
+void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
+  if (nwg <= 0) return;
+  if (scale == 0) k_colour_b<0><<<nwg, 256, 0, s>>>(D, off, nd, 0, src);
+  else k_colour_b<1><<<nwg, 256, 0, s>>>(D, off, nd, scale, src);
 }
 
 void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], const float* thresholds,
@@ -406,7 +487,7 @@ void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, fl
   for (int c = 0; c < 3; c++) k_blur_moments<<<grid, 256, 0, s>>>(w, h, tab, x1[c], mu1[c], s11[c]);
 }
 
-int pool_tiles(int w, int h) { return (int)(div_up(w, kPW) * div_up(h, kPH)); }
+int pool_tiles(int w, int h) { return pool_tiles_dev(w, h); }
 
 void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, double* partials) {
   int total = 0;
@@ -422,6 +503,17 @@ void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, doub
 
 void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums) {
   k_reduce_scales<<<18 * L.nscales, 256, 0, s>>>(L, partials, sums);
+}
+
+void launch_blur_pool_b(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
+                        int nwg) {
+  if (nwg <= 0) return;
+  k_blur_pool_b<<<dim3(nwg, 1, 3), 256, 0, s>>>(tab, D, off, nds);
+}
+
+void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd) {
+  if (nd <= 0) return;
+  k_reduce_b<<<nd * kDescScales * 18, 256, 0, s>>>(D);
 }
 
 }  // namespace jpxlg

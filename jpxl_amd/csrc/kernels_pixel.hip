@@ -389,13 +389,28 @@ constexpr int quant_vb(int n) { return n == 32 ? 1 : 64 / n; }
 constexpr int quant_threads(int n) { return n == 32 ? 256 : kThreads; }
 
 template <int N>
-__global__ __launch_bounds__(quant_threads(N)) void k_quantize(
-    Geom g, const uint32_t* __restrict__ list, int count, const ConstTables* __restrict__ tab,
-    const ProbeParams* __restrict__ pp, const CflParams* __restrict__ cflp,
-    const int8_t* __restrict__ cfl_x, const int8_t* __restrict__ cfl_b, const float* __restrict__ CX,
-    const float* __restrict__ CY, const float* __restrict__ CB, int32_t* __restrict__ QX,
-    int32_t* __restrict__ QY, int32_t* __restrict__ QB, int32_t* __restrict__ LX,
-    int32_t* __restrict__ LY, int32_t* __restrict__ LB, uint32_t* __restrict__ err) {
+__global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* __restrict__ D,
+                                                               const int* __restrict__ off, int nd,
+                                                               const ConstTables* __restrict__ tab) {
+  const int d = find_desc(off, nd, blockIdx.x);
+  const ProbeDesc& P = D[d];
+  const Geom g = P.g;
+  const uint32_t* __restrict__ list = P.list[size_index(N)];
+  const int count = P.count[size_index(N)];
+  const ProbeParams* __restrict__ pp = P.pp;
+  const CflParams* __restrict__ cflp = P.cflp;
+  const int8_t* __restrict__ cfl_x = P.cfl_x;
+  const int8_t* __restrict__ cfl_b = P.cfl_b;
+  const float* __restrict__ CX = P.coeff[0];
+  const float* __restrict__ CY = P.coeff[1];
+  const float* __restrict__ CB = P.coeff[2];
+  int32_t* __restrict__ QX = P.quant[0];
+  int32_t* __restrict__ QY = P.quant[1];
+  int32_t* __restrict__ QB = P.quant[2];
+  int32_t* __restrict__ LX = P.lf[0];
+  int32_t* __restrict__ LY = P.lf[1];
+  int32_t* __restrict__ LB = P.lf[2];
+  uint32_t* __restrict__ err = P.err;
   constexpr int VB = quant_vb(N), T = quant_threads(N), NN = N * N, n = N / 8, si = size_index(N),
                 cb = kCellBase[si];
   __shared__ float co[3][VB][NN];
@@ -405,7 +420,7 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(
   __shared__ float lfs[3][VB][16], lfo[3][VB][16];
   __shared__ float khf[2][VB];
   __shared__ int vbx[VB], vby[VB];
-  const int base = blockIdx.x * VB;
+  const int base = (blockIdx.x - off[d]) * VB;
   const int nvb = min(VB, count - base);
   int bad = 0;
   if (threadIdx.x < VB) {
@@ -533,20 +548,34 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(
 // ----------------------------------------------------------------------------
 
 template <int N, int FORM>
-__global__ __launch_bounds__(kThreads) void k_render(
-    Geom g, const uint32_t* __restrict__ list, int count, const ConstTables* __restrict__ tab,
-    const DctMats* __restrict__ mats,
-    const ProbeParams* __restrict__ pp, const CflParams* __restrict__ cflp,
-    const int8_t* __restrict__ cfl_x, const int8_t* __restrict__ cfl_b,
-    const int32_t* __restrict__ QX, const int32_t* __restrict__ QY, const int32_t* __restrict__ QB,
-    const int32_t* __restrict__ LX, const int32_t* __restrict__ LY, const int32_t* __restrict__ LB,
-    float* __restrict__ X, float* __restrict__ Y, float* __restrict__ B, float* __restrict__ sigma) {
+__global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
+                                                     int nd, const ConstTables* __restrict__ tab,
+                                                     const DctMats* __restrict__ mats) {
+  const int d = find_desc(off, nd, blockIdx.x);
+  const ProbeDesc& PD = D[d];
+  const Geom g = PD.g;
+  const uint32_t* __restrict__ list = PD.list[size_index(N)];
+  const int count = PD.count[size_index(N)];
+  const ProbeParams* __restrict__ pp = PD.pp;
+  const CflParams* __restrict__ cflp = PD.cflp;
+  const int8_t* __restrict__ cfl_x = PD.cfl_x;
+  const int8_t* __restrict__ cfl_b = PD.cfl_b;
+  const int32_t* __restrict__ QX = PD.quant[0];
+  const int32_t* __restrict__ QY = PD.quant[1];
+  const int32_t* __restrict__ QB = PD.quant[2];
+  const int32_t* __restrict__ LX = PD.lf[0];
+  const int32_t* __restrict__ LY = PD.lf[1];
+  const int32_t* __restrict__ LB = PD.lf[2];
+  float* __restrict__ X = PD.rx[0];
+  float* __restrict__ Y = PD.rx[1];
+  float* __restrict__ B = PD.rx[2];
+  float* __restrict__ sigma = PD.sigma;
   constexpr int VB = 64 / N, P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
   __shared__ float lds[VB * 3 * N * P];
   __shared__ float llf[VB][3][16];
   __shared__ int vbx[VB], vby[VB];
   __shared__ float khf[2][VB];
-  const int base = blockIdx.x * VB;
+  const int base = (blockIdx.x - off[d]) * VB;
   const int nvb = min(VB, count - base);
   if (threadIdx.x < VB) {
     int bx = 0, by = 0;
@@ -676,20 +705,13 @@ __global__ void k_fill(float* p, float v, size_t n) {
 // J.3 Gaborish smoothing (jpxl-core/src/restoration.rs:148-193)
 // ----------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_gaborish(int w, int h, const float* __restrict__ in0,
-                                                  const float* __restrict__ in1,
-                                                  const float* __restrict__ in2,
-                                                  float* __restrict__ o0, float* __restrict__ o1,
-                                                  float* __restrict__ o2) {
+__device__ __forceinline__ void gaborish_px(int w, int h, const float* __restrict__ in, float* __restrict__ out,
+                                            int x, int y) {
   const float w1 = 0.115169525f, w2 = 0.061248592f;
   const float sum = fmaf(4.0f, w2, fmaf(4.0f, w1, 1.0f));
   const float scale = 1.0f / sum;
   const float ce = scale, ed = w1 * scale, co = w2 * scale;
-  int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-  int c = blockIdx.z;
   if (x >= w) return;
-  const float* in = c == 0 ? in0 : (c == 1 ? in1 : in2);
-  float* out = c == 0 ? o0 : (c == 1 ? o1 : o2);
   float acc = 0.0f;
   for (int dy = -1; dy <= 1; dy++) {
     const float* row = in + (size_t)mirror1d(y + dy, h) * w;
@@ -699,6 +721,28 @@ __global__ __launch_bounds__(256) void k_gaborish(int w, int h, const float* __r
     }
   }
   out[(size_t)y * w + x] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_gaborish(int w, int h, const float* __restrict__ in0,
+                                                  const float* __restrict__ in1,
+                                                  const float* __restrict__ in2,
+                                                  float* __restrict__ o0, float* __restrict__ o1,
+                                                  float* __restrict__ o2) {
+  const int c = blockIdx.z;
+  gaborish_px(w, h, c == 0 ? in0 : (c == 1 ? in1 : in2), c == 0 ? o0 : (c == 1 ? o1 : o2),
+              blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+}
+
+// Batched: workgroup (row y, 256-column block) of channel blockIdx.z per desc;
+// reads the render planes (src 0) or their ping-pong (src 1), writes the other.
+__global__ __launch_bounds__(256) void k_gaborish_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
+                                                    int nd, int src) {
+  const int d = find_desc(off, nd, blockIdx.x);
+  const ProbeDesc& P = D[d];
+  const int lb = blockIdx.x - off[d], xb = (P.g.w + 255) / 256;
+  const int c = blockIdx.z;
+  gaborish_px(P.g.w, P.g.h, src ? P.aux[c] : P.rx[c], src ? P.rx[c] : P.aux[c], (lb % xb) * 256 + threadIdx.x,
+              lb / xb);
 }
 
 // ----------------------------------------------------------------------------
@@ -770,15 +814,12 @@ __device__ __forceinline__ void epf_pixel(int w, int h, int x, int y, int tx, in
 }
 
 template <int STEP>
-__global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restrict__ in0,
-                                             const float* __restrict__ in1,
-                                             const float* __restrict__ in2,
-                                             const float* __restrict__ sigma,
-                                             float* __restrict__ o0, float* __restrict__ o1,
-                                             float* __restrict__ o2) {
+__device__ __forceinline__ void epf_tile(int w, int h, const float* __restrict__ in0, const float* __restrict__ in1,
+                                         const float* __restrict__ in2, const float* __restrict__ sigma,
+                                         float* __restrict__ o0, float* __restrict__ o1, float* __restrict__ o2,
+                                         int x0, int y0) {
   constexpr int LW = kEpfTW + 2 * kEpfR, LH = kEpfTH + 2 * kEpfR;
   __shared__ float tile[3][LH][LW];
-  const int x0 = blockIdx.x * kEpfTW, y0 = blockIdx.y * kEpfTH;
   const float* in[3] = {in0, in1, in2};
   for (int e = threadIdx.x; e < 3 * LH * LW; e += 256) {
     int c = e / (LH * LW), rem = e % (LH * LW), ly = rem / LW, lx = rem % LW;
@@ -789,6 +830,29 @@ __global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restri
   const int tx = threadIdx.x % kEpfTW;
   for (int ty = (threadIdx.x / kEpfTW) * kEpfRows; ty < (int)(threadIdx.x / kEpfTW + 1) * kEpfRows; ty++)
     epf_pixel<STEP>(w, h, x0 + tx, y0 + ty, tx, ty, tile, sigma, o0, o1, o2);
+}
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_epf(int w, int h, const float* __restrict__ in0,
+                                             const float* __restrict__ in1,
+                                             const float* __restrict__ in2,
+                                             const float* __restrict__ sigma,
+                                             float* __restrict__ o0, float* __restrict__ o1,
+                                             float* __restrict__ o2) {
+  epf_tile<STEP>(w, h, in0, in1, in2, sigma, o0, o1, o2, blockIdx.x * kEpfTW, blockIdx.y * kEpfTH);
+}
+
+// Batched: one 64x16 tile of one desc per workgroup; src as for Gaborish.
+template <int STEP>
+__global__ __launch_bounds__(256) void k_epf_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
+                                               int nd, int src) {
+  const int d = find_desc(off, nd, blockIdx.x);
+  const ProbeDesc& P = D[d];
+  const int lb = blockIdx.x - off[d], tx = (P.g.w + kEpfTW - 1) / kEpfTW;
+  float* const* in = src ? P.aux : P.rx;
+  float* const* out = src ? P.rx : P.aux;
+  epf_tile<STEP>(P.g.w, P.g.h, in[0], in[1], in[2], P.sigma, out[0], out[1], out[2], (lb % tx) * kEpfTW,
+                 (lb / tx) * kEpfTH);
 }
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
@@ -845,56 +909,37 @@ void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const Co
   }
 }
 
-void launch_quantize(hipStream_t s, int side, const Geom& g, const uint32_t* list, int count,
-                     const ConstTables* tab, const ProbeParams* pp, const CflParams* cfl,
-                     const int8_t* cfl_x, const int8_t* cfl_b, float* const coeff[3],
-                     int32_t* quant[3], int32_t* lf[3], uint32_t* err) {
-  if (count <= 0) return;
-#define JPXLG_Q(NS, VBN)                                                                          \
-  k_quantize<NS><<<div_up(count, quant_vb(NS)), quant_threads(NS), 0, s>>>(g, list, count, tab, pp, cfl, cfl_x,   \
-                                                         cfl_b, coeff[0], coeff[1], coeff[2],   \
-                                                         quant[0], quant[1], quant[2], lf[0],   \
-                                                         lf[1], lf[2], err)
-  switch (side) {
-    case 8: JPXLG_Q(8, 8); break;
-    case 16: JPXLG_Q(16, 4); break;
-    default: JPXLG_Q(32, 2); break;
-  }
-#undef JPXLG_Q
+int quant_groups(int side, int count) {
+  return side == 8 ? (int)div_up(count, quant_vb(8)) : (side == 16 ? (int)div_up(count, quant_vb(16))
+                                                                    : (int)div_up(count, quant_vb(32)));
 }
+int render_groups(int side, int count) { return (int)div_up(count, 64 / side); }
 
-template <int N, int FORM>
-static void render_form(hipStream_t s, const Geom& g, const uint32_t* list, int count, const ConstTables* tab,
-                        const DctMats* mats, const ProbeParams* pp, const CflParams* cfl, const int8_t* cfl_x,
-                        const int8_t* cfl_b, int32_t* const quant[3], int32_t* const lf[3], float* xyb[3],
-                        float* sigma) {
-  k_render<N, FORM><<<div_up(count, 64 / N), kThreads, 0, s>>>(g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b,
-                                                                quant[0], quant[1], quant[2], lf[0], lf[1], lf[2],
-                                                                xyb[0], xyb[1], xyb[2], sigma);
+void launch_quantize(hipStream_t s, int side, const ProbeDesc* D, const int* off, int nd, int nwg,
+                     const ConstTables* tab) {
+  if (nwg <= 0) return;
+  switch (side) {
+    case 8: k_quantize<8><<<nwg, quant_threads(8), 0, s>>>(D, off, nd, tab); break;
+    case 16: k_quantize<16><<<nwg, quant_threads(16), 0, s>>>(D, off, nd, tab); break;
+    default: k_quantize<32><<<nwg, quant_threads(32), 0, s>>>(D, off, nd, tab); break;
+  }
 }
 
 template <int N>
-static void render_size(hipStream_t s, int form, const Geom& g, const uint32_t* list, int count,
-                        const ConstTables* tab, const DctMats* mats, const ProbeParams* pp, const CflParams* cfl,
-                        const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3], int32_t* const lf[3],
-                        float* xyb[3], float* sigma) {
-  if (form == kDctFp32)
-    render_form<N, kDctFp32>(s, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma);
-  else if (form == kDctBf16x3)
-    render_form<N, kDctBf16x3>(s, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma);
-  else
-    render_form<N, kDctExact>(s, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma);
+static void render_size(hipStream_t s, int form, const ProbeDesc* D, const int* off, int nd, int nwg,
+                        const ConstTables* tab, const DctMats* mats) {
+  if (form == kDctFp32) k_render<N, kDctFp32><<<nwg, kThreads, 0, s>>>(D, off, nd, tab, mats);
+  else if (form == kDctBf16x3) k_render<N, kDctBf16x3><<<nwg, kThreads, 0, s>>>(D, off, nd, tab, mats);
+  else k_render<N, kDctExact><<<nwg, kThreads, 0, s>>>(D, off, nd, tab, mats);
 }
 
-void launch_render(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
-                   const ConstTables* tab, const DctMats* mats, const ProbeParams* pp, const CflParams* cfl,
-                   const int8_t* cfl_x, const int8_t* cfl_b, int32_t* const quant[3], int32_t* const lf[3],
-                   float* xyb[3], float* sigma) {
-  if (count <= 0) return;
+void launch_render(hipStream_t s, int side, int form, const ProbeDesc* D, const int* off, int nd, int nwg,
+                   const ConstTables* tab, const DctMats* mats) {
+  if (nwg <= 0) return;
   switch (side) {
-    case 8: render_size<8>(s, form, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma); break;
-    case 16: render_size<16>(s, form, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma); break;
-    default: render_size<32>(s, form, g, list, count, tab, mats, pp, cfl, cfl_x, cfl_b, quant, lf, xyb, sigma); break;
+    case 8: render_size<8>(s, form, D, off, nd, nwg, tab, mats); break;
+    case 16: render_size<16>(s, form, D, off, nd, nwg, tab, mats); break;
+    default: render_size<32>(s, form, D, off, nd, nwg, tab, mats); break;
   }
 }
 
@@ -906,6 +951,21 @@ void launch_fill(hipStream_t s, float* p, float v, size_t n) {
 void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]) {
   dim3 grid(div_up(w, 256), h, 3);
   k_gaborish<<<grid, 256, 0, s>>>(w, h, in[0], in[1], in[2], out[0], out[1], out[2]);
+}
+
+int epf_tiles(int w, int h) { return (int)(div_up(w, kEpfTW) * div_up(h, kEpfTH)); }
+int gaborish_blocks(int w, int h) { return (int)(div_up(w, 256) * h); }
+
+void launch_gaborish_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
+  if (nwg <= 0) return;
+  k_gaborish_b<<<dim3(nwg, 1, 3), 256, 0, s>>>(D, off, nd, src);
+}
+
+void launch_epf_b(hipStream_t s, int step, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
+  if (nwg <= 0) return;
+  if (step == 0) k_epf_b<0><<<nwg, 256, 0, s>>>(D, off, nd, src);
+  else if (step == 1) k_epf_b<1><<<nwg, 256, 0, s>>>(D, off, nd, src);
+  else k_epf_b<2><<<nwg, 256, 0, s>>>(D, off, nd, src);
 }
 
 void launch_epf(hipStream_t s, int step, int w, int h, float* const in[3], const float* sigma,

@@ -37,7 +37,6 @@ def synthetic_rgb(width: int, height: int, seed: int, bits: int = 8) -> np.ndarr
         zc = 0.8 * z0 + 0.6 * _field(rng, height, width)
         lo, hi = zc.min(), zc.max()
         img[..., c] = _srgb_encode((zc - lo) / max(hi - lo, 1e-12))
-    yy, xx = np.mgrid[0:height, 0:width]
     for _ in range(40):
         colour = rng.uniform(0.0, 1.0, 3)
         if rng.uniform() < 0.5:
@@ -47,15 +46,23 @@ def synthetic_rgb(width: int, height: int, seed: int, bits: int = 8) -> np.ndarr
         else:
             cx, cy = rng.uniform(0, width), rng.uniform(0, height)
             r = rng.uniform(0.02, 0.15) * min(width, height)
-            img[(xx - cx) ** 2 + (yy - cy) ** 2 <= r * r] = colour
+            # the disk's mask, evaluated over its bounding box only
+            bx0, bx1 = max(0, int(np.floor(cx - r)) - 1), min(width, int(np.ceil(cx + r)) + 2)
+            by0, by1 = max(0, int(np.floor(cy - r)) - 1), min(height, int(np.ceil(cy + r)) + 2)
+            yy, xx = np.mgrid[by0:by1, bx0:bx1]
+            m = (xx - cx) ** 2 + (yy - cy) ** 2 <= r * r
+            img[by0:by1, bx0:bx1][m] = colour
     covered = np.zeros((height, width), bool)
     target = 0.10 * width * height
-    while covered.sum() < target:
+    ncov = 0
+    while ncov < target:
         pw = int(rng.integers(max(1, width // 32), max(2, width // 6)))
         ph = int(rng.integers(max(1, height // 32), max(2, height // 6)))
         x0 = int(rng.integers(0, max(1, width - pw)))
         y0 = int(rng.integers(0, max(1, height - ph)))
-        covered[y0 : y0 + ph, x0 : x0 + pw] = True
+        patch = covered[y0 : y0 + ph, x0 : x0 + pw]
+        ncov += int(patch.size - np.count_nonzero(patch))
+        patch[...] = True
     img[covered] += rng.uniform(-0.1, 0.1, (int(covered.sum()), 3))
     img = np.clip(img, 0.0, 1.0)
     maxv = (1 << bits) - 1

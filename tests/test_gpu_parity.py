@@ -352,3 +352,50 @@ def test_full_size_config_properties(ctx):
     s3 = img.render_score(Restoration(1, 0, 7))[0]
     assert s1 < s3 < 100.0
     img.close()
+
+
+def test_batch_probe_matches_one_image_at_a_time(ctx):
+    """A batched probe (one launch per stage over uneven images of different
+    depths, maps, CfL and quantizers) reproduces each image's own probe."""
+    import jpxl_amd
+    from jpxl_amd import Restoration
+
+    specs = [((203, 141), 8, 7), ((64, 64), 16, 3), ((130, 97), 8, 11), ((300, 77), 12, 5)]
+    imgs, qs = [], []
+    for k, ((w, h), bits, seed) in enumerate(specs):
+        bmap = orc.mixed_map(w, h, seed=seed)
+        img, _, _ = _image_and_oracle(ctx, w, h, seed, bits, bmap, _cfl(bmap, seed), (k, -k))
+        imgs.append(img)
+        qs.append(_quantizer(gs=1500 + 700 * k, mul=1 + k % 2))
+    r = Restoration(1, 0, 7)
+    scores = jpxl_amd.batch_probe(imgs, qs, r)
+    plans = [im.plan_download() for im in imgs]
+    for k, im in enumerate(imgs):
+        im.quantize(qs[k])
+        s1 = im.render_score(r)[0]
+        p1 = im.plan_download()
+        assert np.array_equal(p1.quant, plans[k].quant) and np.array_equal(p1.lf, plans[k].lf)
+        assert s1 == scores[k]
+    for im in imgs:
+        im.close()
+
+
+def test_batch_encode_quality_matches_one_image_at_a_time(ctx):
+    import jpxl_amd
+
+    sizes = [(256, 192), (120, 200), (333, 111)]
+    targets = [85.0, 70.0, 90.0]
+    rgbs = [_rgb(w, h, 40 + k) for k, (w, h) in enumerate(sizes)]
+    batch = [jpxl_amd.Image(ctx, x, 8) for x in rgbs]
+    res = jpxl_amd.batch_encode_quality(batch, targets, "balanced")
+    for k, x in enumerate(rgbs):
+        one = jpxl_amd.Image(ctx, x, 8)
+        r1 = one.encode_quality(targets[k], "balanced")
+        assert (res[k].rung, res[k].probes, res[k].met) == (r1.rung, r1.probes, r1.met)
+        assert res[k].score == r1.score
+        assert [res[k].probe_rungs[i] for i in range(r1.probes)] == [r1.probe_rungs[i] for i in range(r1.probes)]
+        assert np.array_equal(batch[k].plan_download().quant, one.plan_download().quant)
+        one.close()
+    for im in batch:
+        im.close()
+
