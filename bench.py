@@ -279,6 +279,115 @@ def batched_form(args) -> int:
     return 0
 
 
+def serving_sizes(count: int, seed: int):
+    """Request resolutions: megapixels lognormal (median 2, sigma 0.8) clipped
+    to [0.06, 24], aspect ratio uniform in [0.5, 2]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        mp = float(np.clip(np.exp(rng.normal(np.log(2.0), 0.8)), 0.06, 24.0))
+        aspect = float(rng.uniform(0.5, 2.0))
+        w = max(8, int(round(np.sqrt(mp * 1e6 * aspect))))
+        h = max(8, int(round(mp * 1e6 / w)))
+        out.append((w, h))
+    return out
+
+
+def served_form(args) -> int:
+    """The serving form: Poisson arrivals of quality requests (targets uniform
+    over {70, 85, 90}, effort fast/balanced 50/50) at each swept rate, served
+    with continuous batching: arrived requests are admitted while their
+    projected HBM footprint fits, join the running rounds at the next step,
+    and leave with their plan downloaded.  Latency = completion - arrival,
+    including the upload of the request's pixels.  One JSON line."""
+    import multiprocessing as mp
+
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    pool_n = args.pool_images
+    sizes = serving_sizes(pool_n, 5)
+    with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
+        rgbs = pool.map(_gen, [(w, h, 2000 + k) for k, (w, h) in enumerate(sizes)], chunksize=1)
+    ctx = jpxl_amd.Context(0)
+    lib = ctx.lib
+    budget = 288e9 * 0.9 - sum(x.nbytes for x in rgbs)
+    rates = [float(r) for r in args.rates.split(",")]
+    out = {"metric": "served VarDCT encode latency and MP/s", "unit": "ms",
+           "config": {"workload": f"Poisson arrivals, {args.requests} requests per rate, resolutions lognormal "
+                                  f"(median 2 MP, sigma 0.8, clipped [0.06, 24] MP, aspect U[0.5, 2]) drawn from "
+                                  f"{pool_n} generated RGB8 images; targets U{{70,85,90}}; effort fast/balanced 50/50; "
+                                  f"admission by projected HBM footprint; continuous batching"},
+           "rates": {}}
+    # warm-up
+    im = jpxl_amd.Image(ctx, rgbs[0], 8)
+    im.encode_quality(85.0, "fast")
+    im.close()
+    for rate in rates:
+        rng = np.random.default_rng(int(rate) + 17)
+        n = args.requests
+        arrive = np.cumsum(rng.exponential(1.0 / rate, n))
+        pick = rng.integers(0, pool_n, n)
+        targets = rng.choice([70.0, 85.0, 90.0], n)
+        efforts = np.where(rng.uniform(size=n) < 0.5, "fast", "balanced")
+        srv = jpxl_amd.Server(ctx)
+        live = {}            # ticket -> (request index, image, footprint)
+        lat = np.zeros(n)
+        mps = np.zeros(n)
+        met = 0
+        used = 0.0
+        nxt = 0
+        t0 = time.perf_counter()
+        done = 0
+        while done < n:
+            now = time.perf_counter() - t0
+            # admit arrived requests in order while their footprint fits
+            while nxt < n and arrive[nxt] <= now:
+                w, h = sizes[pick[nxt]]
+                fp = float(lib.jpxlg_image_footprint(w, h))
+                if used + fp > budget and live:
+                    break
+                img = jpxl_amd.Image(ctx, rgbs[pick[nxt]], 8)
+                t = srv.submit(img, float(targets[nxt]), str(efforts[nxt]))
+                live[t] = (nxt, img, fp)
+                used += fp
+                nxt += 1
+            if not live:
+                if nxt < n:
+                    time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - t0)))
+                continue
+            srv.step()
+            for t in list(live):
+                res = srv.poll(t)
+                if res is None:
+                    continue
+                k, img, fp = live.pop(t)
+                img.plan_download()  # the hand-off to the CPU entropy coder
+                lat[k] = (time.perf_counter() - t0) - arrive[k]
+                w, h = sizes[pick[k]]
+                mps[k] = w * h / 1e6
+                met += int(res.met)
+                srv.release(t)
+                img.close()
+                used -= fp
+                done += 1
+        t_end = time.perf_counter() - t0
+        srv.close()
+        two = np.array([abs(mps[k] - 2.0) <= 0.5 for k in range(n)])
+        out["rates"][str(int(rate))] = {
+            "p50_ms": round(1000 * float(np.percentile(lat, 50)), 2),
+            "p99_ms": round(1000 * float(np.percentile(lat, 99)), 2),
+            "p99_ms_2mp": round(1000 * float(np.percentile(lat[two], 99)), 2) if two.any() else None,
+            "mps": round(float(mps.sum()) / t_end, 2),
+            "served_rate": round(n / t_end, 1),
+            "met": int(met),
+        }
+    out["value"] = out["rates"][str(int(rates[0]))]["p99_ms"]
+    print(json.dumps(out))
+    ctx.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,11 +397,15 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain copy of known size (for HBM counter calibration)")
-    ap.add_argument("--form", default="single", choices=["single", "precision", "batched"],
+    ap.add_argument("--form", default="single", choices=["single", "precision", "batched", "served"],
                     help="single: the q85 encode+verify throughput line (default); "
                          "precision: the DCT precision-form report on 4096x4096 RGB8; "
-                         "batched: uneven images, plain loop against the batched encode")
+                         "batched: uneven images, plain loop against the batched encode; "
+                         "served: Poisson request stream with continuous batching")
     ap.add_argument("--batch-images", type=int, default=256)
+    ap.add_argument("--requests", type=int, default=200, help="served: requests per rate")
+    ap.add_argument("--pool-images", type=int, default=48, help="served: distinct generated images")
+    ap.add_argument("--rates", default="50,100,200,400,800", help="served: arrival rates (req/s)")
     args = ap.parse_args()
     if args.copy_calibration:
         return copy_calibration()
@@ -300,6 +413,8 @@ def main() -> int:
         return precision_form(args)
     if args.form == "batched":
         return batched_form(args)
+    if args.form == "served":
+        return served_form(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus})",
               file=sys.stderr)

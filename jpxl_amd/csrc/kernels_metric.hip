@@ -278,7 +278,7 @@ __device__ __forceinline__ void blur_pool_tile(const ConstTables* __restrict__ t
                                                double* __restrict__ out6) {
   __shared__ float t1[kPIH][kPIW];
   __shared__ float t2[kPIH][kPIW];
-  __shared__ float hb[3][kPIH][kPW];
+  __shared__ float hb[kPIH][kPW];
   __shared__ double red[4][6];
   double hf[5];
 #pragma unroll
@@ -301,56 +301,42 @@ __device__ __forceinline__ void blur_pool_tile(const ConstTables* __restrict__ t
     t2[ly][lx] = in ? X2[o] : 0.0f;
   }
   __syncthreads();
-  // Horizontal: 32 rows x 8 runs of 8 outputs = one run per lane.
-  {
-    const int ly = threadIdx.x >> 3, lx0 = (threadIdx.x & 7) * kHRun;
-    float p1[kHRun + 8], p2[kHRun + 8];
-#pragma unroll
-    for (int i = 0; i < kHRun + 8; i++) {
-      p1[i] = t1[ly][lx0 + i];
-      p2[i] = t2[ly][lx0 + i];
-    }
-    double d[kHRun + 8], o[kHRun];
-#pragma unroll
-    for (int i = 0; i < kHRun + 8; i++) d[i] = (double)p2[i];
-    fir_run<kHRun>(d, hf, o);
-#pragma unroll
-    for (int i = 0; i < kHRun; i++) hb[0][ly][lx0 + i] = (float)o[i];
-    // the squared and product inputs are rounded to f32 first (blur.rs:97-113)
-#pragma unroll
-    for (int i = 0; i < kHRun + 8; i++) d[i] = (double)(p2[i] * p2[i]);
-    fir_run<kHRun>(d, hf, o);
-#pragma unroll
-    for (int i = 0; i < kHRun; i++) hb[1][ly][lx0 + i] = (float)o[i];
-#pragma unroll
-    for (int i = 0; i < kHRun + 8; i++) d[i] = (double)(p1[i] * p2[i]);
-    fir_run<kHRun>(d, hf, o);
-#pragma unroll
-    for (int i = 0; i < kHRun; i++) hb[2][ly][lx0 + i] = (float)o[i];
-  }
-  __syncthreads();
-  // Vertical + maps: 64 columns x 4 runs of 6 rows; wave k takes rows 6k..6k+5.
+  // The three moments one after another through one LDS plane (a smaller
+  // footprint keeps more workgroups resident): horizontal 32 rows x 8 runs
+  // of 8 outputs, then vertical 64 columns x 4 runs of 6 rows (wave k takes
+  // rows 6k..6k+5).
+  const int hly = threadIdx.x >> 3, hlx0 = (threadIdx.x & 7) * kHRun;
   const int lx = threadIdx.x & 63, ly0 = (threadIdx.x >> 6) * kVRun;
   const int gx = x0 + lx;
-  float mu2[kVRun], s22[kVRun], s12[kVRun];
-  {
-    double d[kVRun + 8], o[kVRun];
+  float vm[3][kVRun];
 #pragma unroll
-    for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[0][ly0 + i][lx];
-    fir_run<kVRun>(d, hf, o);
+  for (int m = 0; m < 3; m++) {
+    {
+      double d[kHRun + 8], o[kHRun];
 #pragma unroll
-    for (int i = 0; i < kVRun; i++) mu2[i] = (float)o[i];
+      for (int i = 0; i < kHRun + 8; i++) {
+        const float a = t1[hly][hlx0 + i], b = t2[hly][hlx0 + i];
+        // the squared and product inputs are rounded to f32 first (blur.rs:97-113)
+        d[i] = m == 0 ? (double)b : (m == 1 ? (double)(b * b) : (double)(a * b));
+      }
+      fir_run<kHRun>(d, hf, o);
 #pragma unroll
-    for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[1][ly0 + i][lx];
-    fir_run<kVRun>(d, hf, o);
+      for (int i = 0; i < kHRun; i++) hb[hly][hlx0 + i] = (float)o[i];
+    }
+    __syncthreads();
+    {
+      double d[kVRun + 8], o[kVRun];
 #pragma unroll
-    for (int i = 0; i < kVRun; i++) s22[i] = (float)o[i];
+      for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[ly0 + i][lx];
+      fir_run<kVRun>(d, hf, o);
 #pragma unroll
-    for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[2][ly0 + i][lx];
-    fir_run<kVRun>(d, hf, o);
-#pragma unroll
-    for (int i = 0; i < kVRun; i++) s12[i] = (float)o[i];
+      for (int i = 0; i < kVRun; i++) vm[m][i] = (float)o[i];
+    }
+    __syncthreads();
   }
+  float* mu2 = vm[0];
+  float* s22 = vm[1];
+  float* s12 = vm[2];
   const float C2 = 0.0009f;
   double acc[6] = {0, 0, 0, 0, 0, 0};
 #pragma unroll

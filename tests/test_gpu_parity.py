@@ -399,3 +399,46 @@ def test_batch_encode_quality_matches_one_image_at_a_time(ctx):
     for im in batch:
         im.close()
 
+
+def test_server_results_are_the_same_alone_and_in_the_mix(ctx):
+    """Continuous batching: requests submitted while others are mid-search
+    join the running rounds; every result equals the request served alone."""
+    import jpxl_amd
+
+    sizes = [(256, 192), (150, 222), (300, 120), (96, 96), (200, 200)]
+    targets = [85.0, 90.0, 70.0, 85.0, 90.0]
+    efforts = ["balanced", "fast", "balanced", "fast", "balanced"]
+    rgbs = [_rgb(w, h, 60 + k) for k, (w, h) in enumerate(sizes)]
+    alone = []
+    for k, x in enumerate(rgbs):
+        im = jpxl_amd.Image(ctx, x, 8)
+        r = im.encode_quality(targets[k], efforts[k])
+        alone.append((r.rung, r.probes, r.score, im.plan_download().quant))
+        im.close()
+    srv = jpxl_amd.Server(ctx)
+    imgs = [jpxl_amd.Image(ctx, x, 8) for x in rgbs]
+    tickets = {}
+    # two requests first, the rest arrive one per round
+    for k in (0, 1):
+        tickets[k] = srv.submit(imgs[k], targets[k], efforts[k])
+    nxt, done = 2, {}
+    while len(done) < len(rgbs):
+        srv.step()
+        for k, t in tickets.items():
+            if k not in done:
+                res = srv.poll(t)
+                if res is not None:
+                    done[k] = res
+        if nxt < len(rgbs):
+            tickets[nxt] = srv.submit(imgs[nxt], targets[nxt], efforts[nxt])
+            nxt += 1
+    for k in range(len(rgbs)):
+        r = done[k]
+        assert (r.rung, r.probes, r.score) == alone[k][:3]
+        assert np.array_equal(imgs[k].plan_download().quant, alone[k][3])
+        srv.release(tickets[k])
+    assert srv.pending() == 0
+    srv.close()
+    for im in imgs:
+        im.close()
+
