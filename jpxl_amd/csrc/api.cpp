@@ -70,7 +70,19 @@ struct jpxlg_ctx {
 namespace {
 
 // At most this much device memory is kept in released arenas per context.
-constexpr size_t kSpareCap = (size_t)8 << 30;
+constexpr size_t kSpareCap = (size_t)16 << 30;
+
+// Arena sizes are rounded up to classes of 2^k x {1, 1.25, 1.5, 1.75} so a
+// released arena fits the next image of a similar size (at most 25% slack).
+size_t arena_class(size_t bytes) {
+  size_t p = 1;
+  while (p * 2 <= bytes) p *= 2;
+  for (int q = 4; q <= 8; q++) {
+    const size_t c = p / 4 * q;
+    if (c >= bytes) return c;
+  }
+  return p * 2;
+}
 
 // A released arena of at least `bytes` (and at most twice that), or null.
 void* take_spare(jpxlg_ctx* ctx, size_t bytes, size_t* got) {
@@ -334,7 +346,7 @@ extern "C" const char* jpxlg_last_error(const jpxlg_ctx* ctx) {
 }
 
 extern "C" uint64_t jpxlg_image_footprint(uint32_t width, uint32_t height) {
-  return (uint64_t)image_bytes((int)width, (int)height);
+  return (uint64_t)arena_class(image_bytes((int)width, (int)height));
 }
 
 static int get_lut16(jpxlg_ctx* ctx, int bits, float** out) {
@@ -392,7 +404,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   g.tw = (g.bw + 7) / 8;
   g.th = (g.bh + 7) / 8;
   scale_dims(g.w, g.h, &img->nscales, img->sw, img->sh);
-  size_t bytes = image_bytes(g.w, g.h);
+  size_t bytes = arena_class(image_bytes(g.w, g.h));
   void* base = take_spare(ctx, bytes, &bytes);
   if (!base && hipMalloc(&base, bytes) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
   img->arena.base = (char*)base;
@@ -826,9 +838,8 @@ struct ChainRun {
 // and/or reconstruction + restoration + SSIMULACRA2 (when r is non-null) for
 // n images of one context, with one descriptor upload; the sums and error
 // words come back with read_chain.
-static int enqueue_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
+static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
                          const jpxlg_restoration* r, ChainRun* run) {
-  hipStream_t st = ctx->stream;
   const ChainLayout L = chain_layout(n);
   jpxlg_ctx::Slot* sl = nullptr;
   int rc = take_slot(ctx, L.total, &sl);
@@ -931,7 +942,7 @@ static int enqueue_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jp
     static const char* kQNames[3] = {"quantize_8", "quantize_16", "quantize_32"};
     for (int i = 0; i < 3; i++) {
       if (!tot_q[i]) continue;
-      Prof p(ctx, kQNames[i]);
+      Prof p(ctx, kQNames[i], st);
       launch_quantize(st, kSides[i], D, O + L.q_off[i], n, tot_q[i], ctx->d_tab);
     }
   }
@@ -940,26 +951,26 @@ static int enqueue_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jp
     const int form = imgs[0]->dct_form;
     for (int i = 0; i < 3; i++) {
       if (!tot_r[i]) continue;
-      Prof p(ctx, kRNames[i]);
+      Prof p(ctx, kRNames[i], st);
       launch_render(st, kSides[i], form, D, O + L.r_off[i], n, tot_r[i], ctx->d_tab, ctx->d_mats);
     }
     int src = 0;
     if (gab) {
-      { Prof p(ctx, "gaborish"); launch_gaborish_b(st, D, O + L.g_off, n, tot_g, src); }
+      { Prof p(ctx, "gaborish", st); launch_gaborish_b(st, D, O + L.g_off, n, tot_g, src); }
       src ^= 1;
     }
     static const char* kEpfNames[3] = {"epf_step0", "epf_step1", "epf_step2"};
     for (int i = 0; i < nsteps; i++) {
-      { Prof p(ctx, kEpfNames[steps[i]]); launch_epf_b(st, steps[i], D, O + L.e_off, n, tot_e, src); }
+      { Prof p(ctx, kEpfNames[steps[i]], st); launch_epf_b(st, steps[i], D, O + L.e_off, n, tot_e, src); }
       src ^= 1;
     }
     for (int s = 0; s < kDescScales; s++) {
       if (!tot_c[s]) continue;
-      Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour");
+      Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour", st);
       launch_colour_b(st, s, D, O + L.c_off[s], n, tot_c[s], src);
     }
-    { Prof p(ctx, "blur_pool"); launch_blur_pool_b(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, tot_p); }
-    { Prof p(ctx, "pool_reduce"); launch_reduce_b(st, D, n); }
+    { Prof p(ctx, "blur_pool", st); launch_blur_pool_b(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, tot_p); }
+    { Prof p(ctx, "pool_reduce", st); launch_reduce_b(st, D, n); }
     // Keep the restored planes in rx (downloads read them there).
     if (src)
       for (int k = 0; k < n; k++)
@@ -1019,7 +1030,7 @@ extern "C" int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q) {
   if (rc) return rc;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   ChainRun run;
-  rc = enqueue_chain(ctx, &img, 1, q, nullptr, &run);
+  rc = enqueue_chain(ctx, ctx->stream, &img, 1, q, nullptr, &run);
   if (rc) return rc;
   img->has_render = false;
   return read_chain(ctx, &img, 1, run, false, nullptr, nullptr, nullptr, nullptr);
@@ -1034,7 +1045,7 @@ extern "C" int32_t jpxlg_render_score(jpxlg_img* img, const jpxlg_restoration* r
   if (r->sharpness < 0 || r->sharpness > 7) return fail(ctx, JPXLG_E_RANGE, "sharpness outside 0..7");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   ChainRun run;
-  int rc = enqueue_chain(ctx, &img, 1, nullptr, r, &run);
+  int rc = enqueue_chain(ctx, ctx->stream, &img, 1, nullptr, r, &run);
   if (rc) return rc;
   return read_chain(ctx, &img, 1, run, true, score, raw_error, terms, nterms);
 }
@@ -1045,7 +1056,7 @@ int jpxlg_probe(jpxlg_img* img, const jpxlg_quantizer* q, const jpxlg_restoratio
   int rc = check_quantizer(ctx, q);
   if (rc) return rc;
   ChainRun run;
-  rc = enqueue_chain(ctx, &img, 1, q, r, &run);
+  rc = enqueue_chain(ctx, ctx->stream, &img, 1, q, r, &run);
   if (rc) return rc;
   return read_chain(ctx, &img, 1, run, true, score, nullptr, nullptr, nullptr);
 }
@@ -1202,7 +1213,7 @@ extern "C" int32_t jpxlg_batch_probe(uint32_t n, jpxlg_img* const* imgs, const j
   if (r->epf_iters < 0 || r->epf_iters > 3) return fail(ctx, JPXLG_E_UNSUPPORTED, "epf_iters above 3");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   ChainRun run;
-  rc = enqueue_chain(ctx, imgs, (int)n, qs, r, &run);
+  rc = enqueue_chain(ctx, ctx->stream, imgs, (int)n, qs, r, &run);
   if (rc) return rc;
   return read_chain(ctx, imgs, (int)n, run, true, scores, nullptr, nullptr, nullptr);
 }
@@ -1225,11 +1236,22 @@ struct ServerReq {
   uint32_t rung = 0;
   double score = 0.0;
   int rc = 0;
+  bool in_lane = false;   // its probe is in a lane's running round
   bool finished = false;  // navigator returned
   bool done = false;      // result available
   int nav_rc = 0;
   ControllerOutcome oc;
   jpxlg_quality_result res{};
+};
+
+// A lane runs the rounds of one class of requests (small or large images)
+// on its own stream, so a large image's round does not hold back the
+// rounds of small ones; the GPU overlaps the two.
+struct ServerLane {
+  hipStream_t stream = nullptr;
+  bool busy = false;
+  ChainRun run;
+  std::vector<ServerReq*> reqs;
 };
 
 struct jpxlg_server {
@@ -1239,12 +1261,32 @@ struct jpxlg_server {
   int active = 0, waiting = 0;
   uint64_t next_id = 1;
   std::map<uint64_t, std::unique_ptr<ServerReq>> reqs;
+  ServerLane lane[2];
+  ~jpxlg_server() {
+    for (auto& l : lane)
+      if (l.stream) {
+        (void)hipStreamSynchronize(l.stream);
+        (void)hipStreamDestroy(l.stream);
+      }
+  }
 };
+
+// Images above this many pixels run in the large-image lane.
+constexpr uint64_t kLargeLanePixels = 4u << 20;
+
+static int server_init(jpxlg_server* srv, jpxlg_ctx* ctx) {
+  srv->ctx = ctx;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (auto& l : srv->lane) HIP_TRY(ctx, hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+  return JPXLG_OK;
+}
 
 extern "C" int32_t jpxlg_server_create(jpxlg_ctx* ctx, jpxlg_server** out) {
   if (!ctx || !out) return JPXLG_E_INVALID;
-  *out = new jpxlg_server();
-  (*out)->ctx = ctx;
+  auto srv = std::make_unique<jpxlg_server>();
+  int rc = server_init(srv.get(), ctx);
+  if (rc) return rc;
+  *out = srv.release();
   return JPXLG_OK;
 }
 
@@ -1288,20 +1330,49 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
   return JPXLG_OK;
 }
 
-// One round: returns the number of requests completed by it (>= 0), or a
-// negative status.
+// Hands a lane's finished round back to its requests' navigators.
+static int server_collect(jpxlg_server* srv, ServerLane& l) {
+  std::vector<jpxlg_img*> imgs;
+  for (ServerReq* R : l.reqs) imgs.push_back(R->img);
+  std::vector<double> sc(imgs.size(), 0.0);
+  const int rc = read_chain(srv->ctx, imgs.data(), (int)imgs.size(), l.run, true, sc.data(), nullptr, nullptr, nullptr);
+  {
+    std::lock_guard<std::mutex> lk(srv->m);
+    for (size_t t = 0; t < l.reqs.size(); t++) {
+      ServerReq* R = l.reqs[t];
+      R->score = sc[t];
+      R->rc = rc;
+      R->pending = false;
+      R->in_lane = false;
+      srv->waiting--;
+    }
+  }
+  l.reqs.clear();
+  l.busy = false;
+  srv->cv_done.notify_all();
+  return rc;
+}
+
+// One scheduling pass: completes finished searches, starts a round in every
+// idle lane with pending probes (building new requests' structures first),
+// and waits until at least one lane's round has finished, handing its scores
+// back.  Returns the number of requests completed (>= 0) or a status.
 extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
   if (!srv) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = srv->ctx;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  std::vector<ServerReq*> probing, finishing;
+  std::vector<ServerReq*> pending[2], finishing;
   {
     std::unique_lock<std::mutex> lk(srv->m);
     srv->cv_req.wait(lk, [srv] { return srv->waiting == srv->active; });
     for (auto& kv : srv->reqs) {
       ServerReq* R = kv.second.get();
-      if (R->pending) probing.push_back(R);
-      else if (R->finished && !R->done) finishing.push_back(R);
+      if (R->pending && !R->in_lane) {
+        const uint64_t px = (uint64_t)R->img->g.w * R->img->g.h;
+        pending[px > kLargeLanePixels ? 1 : 0].push_back(R);
+      } else if (R->finished && !R->done) {
+        finishing.push_back(R);
+      }
     }
   }
   int status = JPXLG_OK;
@@ -1330,57 +1401,87 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
   }
   if (!fimgs.empty()) {
     ChainRun run;
-    int qrc = enqueue_chain(ctx, fimgs.data(), (int)fimgs.size(), fqs.data(), nullptr, &run);
+    int qrc = enqueue_chain(ctx, ctx->stream, fimgs.data(), (int)fimgs.size(), fqs.data(), nullptr, &run);
     if (!qrc) qrc = read_chain(ctx, fimgs.data(), (int)fimgs.size(), run, false, nullptr, nullptr, nullptr, nullptr);
     if (qrc) status = qrc;
   }
-  // The round of probes: structures of new requests, then one chain per DCT
-  // form present (a batch shares its form).
-  std::vector<int> prc(probing.size(), 0);
-  std::vector<double> pscore(probing.size(), 0.0);
-  for (int form = kDctExact; form <= kDctBf16x3; form++) {
-    for (int effort = 0; effort <= 1; effort++) {
-      std::vector<jpxlg_img*> bimgs;
-      std::vector<jpxlg_quantizer> bqs;
-      std::vector<size_t> bt;
-      for (size_t t = 0; t < probing.size(); t++) {
-        ServerReq* R = probing[t];
-        if (R->img->dct_form != form || (R->q.effort == JPXLG_EFFORT_FAST ? 0 : 1) != effort) continue;
-        const jpxlg_quantizer q = R->pol.quantizer_at(R->rung);
-        if (!R->anchored) {
-          prc[t] = jpxlg_plan_structure(R->img, &q, R->pol.cover_mode, 1);
-          if (prc[t]) continue;
-          R->anchored = true;
-        }
-        bimgs.push_back(R->img);
-        bqs.push_back(q);
-        bt.push_back(t);
-      }
-      if (bimgs.empty()) continue;
-      const QualityPolicy& pol = probing[bt[0]]->pol;
-      jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
-      std::vector<double> sc(bimgs.size(), 0.0);
-      ChainRun run;
-      int brc = enqueue_chain(ctx, bimgs.data(), (int)bimgs.size(), bqs.data(), &r, &run);
-      if (!brc) brc = read_chain(ctx, bimgs.data(), (int)bimgs.size(), run, true, sc.data(), nullptr, nullptr, nullptr);
-      for (size_t b = 0; b < bt.size(); b++) {
-        pscore[bt[b]] = sc[b];
-        prc[bt[b]] = brc;
-      }
-    }
-  }
   {
     std::lock_guard<std::mutex> lk(srv->m);
-    for (size_t t = 0; t < probing.size(); t++) {
-      ServerReq* R = probing[t];
-      R->score = pscore[t];
-      R->rc = prc[t];
-      R->pending = false;
-      srv->waiting--;
-    }
     for (ServerReq* R : finishing) R->done = true;
   }
-  srv->cv_done.notify_all();
+  // Start a round in each idle lane: structures of new requests first (on the
+  // context's stream), then one chain per (DCT form, effort) group.
+  for (int li = 0; li < 2; li++) {
+    ServerLane& l = srv->lane[li];
+    if (l.busy || pending[li].empty()) continue;
+    std::vector<ServerReq*> ready;
+    for (ServerReq* R : pending[li]) {
+      const jpxlg_quantizer q = R->pol.quantizer_at(R->rung);
+      if (!R->anchored) {
+        const int prc = jpxlg_plan_structure(R->img, &q, R->pol.cover_mode, 1);
+        if (prc) {
+          std::lock_guard<std::mutex> lk(srv->m);
+          R->rc = prc;
+          R->pending = false;
+          srv->waiting--;
+          continue;
+        }
+        R->anchored = true;
+      }
+      ready.push_back(R);
+    }
+    srv->cv_done.notify_all();
+    if (ready.empty()) continue;
+    // one chain per lane round: group by (form, effort) and take the largest
+    // group now; the rest wait for the lane's next round
+    std::map<std::pair<int, int>, std::vector<ServerReq*>> groups;
+    for (ServerReq* R : ready) groups[{R->img->dct_form, R->q.effort}].push_back(R);
+    auto best = groups.begin();
+    for (auto it = groups.begin(); it != groups.end(); ++it)
+      if (it->second.size() > best->second.size()) best = it;
+    std::vector<jpxlg_img*> bimgs;
+    std::vector<jpxlg_quantizer> bqs;
+    for (ServerReq* R : best->second) {
+      bimgs.push_back(R->img);
+      bqs.push_back(R->pol.quantizer_at(R->rung));
+    }
+    const QualityPolicy& pol = best->second[0]->pol;
+    jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
+    const int brc = enqueue_chain(ctx, l.stream, bimgs.data(), (int)bimgs.size(), bqs.data(), &r, &l.run);
+    if (brc) {
+      std::lock_guard<std::mutex> lk(srv->m);
+      for (ServerReq* R : best->second) {
+        R->rc = brc;
+        R->pending = false;
+        srv->waiting--;
+      }
+      srv->cv_done.notify_all();
+      status = brc;
+      continue;
+    }
+    {
+      std::lock_guard<std::mutex> lk(srv->m);
+      for (ServerReq* R : best->second) R->in_lane = true;
+    }
+    l.reqs = best->second;
+    l.busy = true;
+  }
+  // Wait for the first lane round to finish (short host polls), then hand
+  // back every finished round.
+  if (srv->lane[0].busy || srv->lane[1].busy) {
+    for (;;) {
+      bool any = false;
+      for (auto& l : srv->lane)
+        if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) any = true;
+      if (any) break;
+      std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    for (auto& l : srv->lane)
+      if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) {
+        const int rc = server_collect(srv, l);
+        if (rc && !status) status = rc;
+      }
+  }
   if (status) return status;
   return (int32_t)finishing.size();
 }
@@ -1436,7 +1537,8 @@ extern "C" int32_t jpxlg_batch_encode_quality(uint32_t n, jpxlg_img* const* imgs
   jpxlg_ctx* ctx = imgs[0]->ctx;
   if (!reqs || !out) return fail(ctx, JPXLG_E_INVALID, "null argument");
   jpxlg_server srv;
-  srv.ctx = ctx;
+  rc = server_init(&srv, ctx);
+  if (rc) return rc;
   std::vector<uint64_t> tickets(n);
   for (uint32_t k = 0; k < n; k++) {
     rc = jpxlg_server_submit(&srv, imgs[k], &reqs[k], &tickets[k]);
