@@ -116,6 +116,12 @@ int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* block_map, const int8
 int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t cover_mode,
                              int32_t enable_cfl);
 
+/* prepare_gaborish_frame (jpxl-encode-policy/src/lib.rs:2256-2268,
+ * jpxl-encode/src/vardct/gaborish.rs:218-315): replaces the image's source XYB
+ * planes by their 12-step Jacobi Gaborish inverse (for plans rendered with
+ * Gaborish on).  Drops any structure: plan or set it again afterwards. */
+int32_t jpxlg_precondition_gaborish(jpxlg_img* img);
+
 /* ---- probe ---------------------------------------------------------------- */
 int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q);
 

@@ -241,6 +241,10 @@ class Image:
     def quantize(self, q: Quantizer):
         self.ctx.check(self.ctx.lib.jpxlg_quantize(self.handle, ctypes.byref(q)))
 
+    def precondition_gaborish(self):
+        """Replace the source XYB by its Gaborish inverse (drops the structure)."""
+        self.ctx.check(self.ctx.lib.jpxlg_precondition_gaborish(self.handle))
+
     def set_dct_form(self, form: int):
         """_lib.DCT_EXACT (the reference's butterflies), DCT_FP32 or DCT_BF16X3
         (matrix-core contractions of the precision study)."""

@@ -111,6 +111,7 @@ SIGNATURES = {
     "jpxlg_plan_structure": (_i32, [_vp, ctypes.POINTER(Quantizer), _i32, _i32]),
     "jpxlg_quantize": (_i32, [_vp, ctypes.POINTER(Quantizer)]),
     "jpxlg_set_dct_form": (_i32, [_vp, _i32]),
+    "jpxlg_precondition_gaborish": (_i32, [_vp]),
     "jpxlg_render_score": (_i32, [_vp, ctypes.POINTER(Restoration), _dp, _dp, _dp, _u32p]),
     "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),

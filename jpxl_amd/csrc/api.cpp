@@ -605,6 +605,39 @@ extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, cons
   return upload_structure(img);
 }
 
+// prepare_gaborish_frame (jpxl-encode-policy/src/lib.rs:2256-2268): the
+// source XYB planes replaced by their 12-step Jacobi Gaborish inverse, so a
+// plan rendered with Gaborish restores them.  Any structure is dropped.
+extern "C" int32_t jpxlg_precondition_gaborish(jpxlg_img* img) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t bytes = (size_t)img->g.w * img->g.h * 4;
+  float* cur[3] = {img->rx[0], img->rx[1], img->rx[2]};
+  float* nxt[3] = {img->aux[0], img->aux[1], img->aux[2]};
+  for (int c = 0; c < 3; c++) HIP_TRY(ctx, hipMemcpyAsync(cur[c], img->xyb[c], bytes, hipMemcpyDeviceToDevice, st));
+  {
+    Prof p(ctx, "gaborish_sharpen");
+    for (int it = 0; it < 12; it++) {
+      launch_gaborish_jacobi(st, img->g.w, img->g.h, img->xyb, cur, nxt);
+      std::swap(cur, nxt);
+    }
+  }
+  HIP_TRY(ctx, hipGetLastError());
+  // the result (in cur) becomes the source planes; the old planes become scratch
+  for (int c = 0; c < 3; c++) {
+    float* old = img->xyb[c];
+    img->xyb[c] = cur[c];
+    if (img->rx[c] == cur[c]) img->rx[c] = old;
+    else img->aux[c] = old;
+  }
+  img->has_structure = false;
+  img->has_quant = false;
+  img->has_render = false;
+  return JPXLG_OK;
+}
+
 extern "C" int32_t jpxlg_set_dct_form(jpxlg_img* img, int32_t form) {
   if (!img) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = img->ctx;

@@ -145,6 +145,8 @@ int gaborish_blocks(int w, int h);
 int colour_blocks(int w, int h);
 void launch_fill(hipStream_t s, float* p, float v, size_t n);
 void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]);
+// One Jacobi step of the encoder's Gaborish inverse: out = x + (f - Kx)/centre.
+void launch_gaborish_jacobi(hipStream_t s, int w, int h, float* const f[3], float* const x[3], float* out[3]);
 void launch_epf(hipStream_t s, int step, int w, int h, float* const in[3], const float* sigma,
                 float* out[3]);
 

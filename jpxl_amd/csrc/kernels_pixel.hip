@@ -733,6 +733,55 @@ __global__ __launch_bounds__(256) void k_gaborish(int w, int h, const float* __r
               blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
+// Encoder-side Gaborish sharpen (jpxl-encode/src/vardct/gaborish.rs:66-115,
+// :218-315 as the default "simd" build evaluates it): one Jacobi step
+// x' = x + (f - Kx)/centre per launch.  Kx: border pixels by the scalar 3x3
+// loop over mirrored samples (products added in dy, dx order), interior
+// columns in groups of four from x = 1 as mid*c + (n+s+w+e)*e +
+// (nw+ne+sw+se)*k, the rest of a row's interior by the scalar loop; the update
+// multiplies by 1/centre except on the last n % 4 samples of the plane,
+// which divide.  No fused multiply-add (the reference does not contract).
+__global__ __launch_bounds__(256) void k_gaborish_jacobi(int w, int h, const float* __restrict__ f0,
+                                                         const float* __restrict__ f1,
+                                                         const float* __restrict__ f2,
+                                                         const float* __restrict__ x0,
+                                                         const float* __restrict__ x1,
+                                                         const float* __restrict__ x2, float* __restrict__ o0,
+                                                         float* __restrict__ o1, float* __restrict__ o2) {
+  const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, c = blockIdx.z;
+  if (x >= w) return;
+  const float* F = c == 0 ? f0 : (c == 1 ? f1 : f2);
+  const float* X = c == 0 ? x0 : (c == 1 ? x1 : x2);
+  float* O = c == 0 ? o0 : (c == 1 ? o1 : o2);
+  const float w1 = 0.115169525f, w2 = 0.061248592f;
+  const float a4 = 4.0f * w1, b4 = 4.0f * w2;
+  const float unnorm = (1.0f + a4) + b4;
+  const float ce = 1.0f / unnorm, ed = w1 * ce, co = w2 * ce;
+  const bool edge = x == 0 || y == 0 || x + 1 == w || y + 1 == h;
+  const bool grouped = !edge && (x - 1) < 4 * ((w - 2) / 4);
+  float acc;
+  if (grouped) {
+    const float* r0 = X + (size_t)(y - 1) * w;
+    const float* r1 = X + (size_t)y * w;
+    const float* r2 = X + (size_t)(y + 1) * w;
+    const float mid = r1[x] * ce;
+    const float nsew = ((r0[x] + r2[x]) + r1[x - 1]) + r1[x + 1];
+    const float diag = ((r0[x - 1] + r0[x + 1]) + r2[x - 1]) + r2[x + 1];
+    acc = (mid + nsew * ed) + diag * co;
+  } else {
+    acc = 0.0f;
+    for (int dy = -1; dy <= 1; dy++)
+      for (int dx = -1; dx <= 1; dx++) {
+        const float wgt = (dx == 0 && dy == 0) ? ce : ((dx == 0 || dy == 0) ? ed : co);
+        const float prod = wgt * X[(size_t)mirror1d(y + dy, h) * w + mirror1d(x + dx, w)];
+        acc = acc + prod;
+      }
+  }
+  const size_t i = (size_t)y * w + x, n = (size_t)w * h;
+  const float d = F[i] - acc;
+  O[i] = i < n - n % 4 ? X[i] + d * (1.0f / ce) : X[i] + d / ce;
+}
+
 // Batched: workgroup (row y, 256-column block) of channel blockIdx.z per desc;
 // reads the render planes (src 0) or their ping-pong (src 1), writes the other.
 __global__ __launch_bounds__(256) void k_gaborish_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
@@ -946,6 +995,11 @@ void launch_render(hipStream_t s, int side, int form, const ProbeDesc* D, const 
 void launch_fill(hipStream_t s, float* p, float v, size_t n) {
   if (n == 0) return;
   k_fill<<<div_up(n, 256), 256, 0, s>>>(p, v, n);
+}
+
+void launch_gaborish_jacobi(hipStream_t s, int w, int h, float* const f[3], float* const x[3], float* out[3]) {
+  dim3 grid(div_up(w, 256), h, 3);
+  k_gaborish_jacobi<<<grid, 256, 0, s>>>(w, h, f[0], f[1], f[2], x[0], x[1], x[2], out[0], out[1], out[2]);
 }
 
 void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]) {

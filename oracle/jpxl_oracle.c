@@ -963,6 +963,80 @@ EXPORT void orc_gaborish(int w, int h, const float *in, float *out) {
         }
 }
 
+/* jpxl-encode/src/vardct/gaborish.rs:66-115 gaborish_forward, as the default
+ * build ("simd" feature) evaluates it: border pixels by the scalar 3x3 loop
+ * over mirrored samples (w * s added in dy, dx order); interior columns in
+ * groups of four from x = 1 as mid*c + (n + s + w + e)*e + (nw + ne + sw + se)*k;
+ * the remaining interior columns of a row by the scalar loop.  No fused
+ * multiply-add anywhere (Rust does not contract). */
+/* gaborish.rs:29-47 GaborKernel::new(w1, w2): unnorm = 1 + 4 w1 + 4 w2
+ * evaluated left to right without fused multiply-add. */
+static void gabor_kernel_enc(float *centre, float *edge, float *corner) {
+    const float w1 = 0.115169525f, w2 = 0.061248592f;
+    float a = 4.0f * w1, b = 4.0f * w2;
+    float unnorm = (1.0f + a) + b;
+    float scale = 1.0f / unnorm;
+    *centre = scale;
+    *edge = w1 * scale;
+    *corner = w2 * scale;
+}
+
+static void gabor_forward_enc(int w, int h, const float *in, float *out) {
+    float ce, ed, co;
+    gabor_kernel_enc(&ce, &ed, &co);
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            int edge = x == 0 || y == 0 || x + 1 == w || y + 1 == h;
+            int simd = 0;
+            if (!edge) {
+                /* groups x = 1 + 4j while x + 4 <= w - 1 */
+                int full = (w - 2) / 4;  /* number of complete groups */
+                simd = (x - 1) < 4 * full;
+            }
+            float acc;
+            if (simd) {
+                const float *r0 = in + (size_t)(y - 1) * w, *r1 = in + (size_t)y * w, *r2 = in + (size_t)(y + 1) * w;
+                float mid = r1[x] * ce;
+                float nsew = ((r0[x] + r2[x]) + r1[x - 1]) + r1[x + 1];
+                float diag = ((r0[x - 1] + r0[x + 1]) + r2[x - 1]) + r2[x + 1];
+                acc = (mid + nsew * ed) + diag * co;
+            } else {
+                acc = 0.0f;
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dx = -1; dx <= 1; dx++) {
+                        float wgt = (dx == 0 && dy == 0) ? ce : ((dx == 0 || dy == 0) ? ed : co);
+                        float sm = in[(size_t)mirror1d(y + dy, h) * w + mirror1d(x + dx, w)];
+                        float prod = wgt * sm;
+                        acc = acc + prod;
+                    }
+            }
+            out[(size_t)y * w + x] = acc;
+        }
+}
+
+EXPORT void orc_gaborish_forward_enc(int w, int h, const float *in, float *out) { gabor_forward_enc(w, h, in, out); }
+
+/* gaborish.rs:218-315 precondition_xyb_planes / gaborish_inverse: 12 Jacobi
+ * steps x <- x + (f - Kx) * (1/centre) over the flattened plane in groups of
+ * four, the last n % 4 samples as x + (f - Kx) / centre. */
+EXPORT void orc_gaborish_sharpen(int w, int h, int iters, const float *f, float *x) {
+    float ce, ed, co;
+    gabor_kernel_enc(&ce, &ed, &co);
+    size_t n = (size_t)w * h;
+    float *kx = (float *)malloc(n * sizeof(float));
+    memcpy(x, f, n * sizeof(float));
+    const float inv_c = 1.0f / ce;
+    const size_t vec_end = n - n % 4;
+    for (int it = 0; it < iters; it++) {
+        gabor_forward_enc(w, h, x, kx);
+        for (size_t i = 0; i < n; i++) {
+            float d = f[i] - kx[i];
+            x[i] = i < vec_end ? x[i] + d * inv_c : x[i] + d / ce;
+        }
+    }
+    free(kx);
+}
+
 /* restoration.rs:201-256 EPF parameters (Table J.1 defaults). */
 static const float EPF_SHARP_LUT[8] = {0.0f, 1.0f / 7.0f, 2.0f / 7.0f, 3.0f / 7.0f,
                                        4.0f / 7.0f, 5.0f / 7.0f, 6.0f / 7.0f, 1.0f};

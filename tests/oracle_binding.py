@@ -220,6 +220,26 @@ def epf_step(step: int, planes: np.ndarray, sigma: np.ndarray) -> np.ndarray:
     return out
 
 
+def gaborish_sharpen(planes: np.ndarray, iters: int = 12) -> np.ndarray:
+    """The encoder's Jacobi Gaborish inverse of each plane."""
+    planes = np.ascontiguousarray(planes, np.float32)
+    out = np.empty_like(planes)
+    h, w = planes.shape[1:]
+    for c in range(planes.shape[0]):
+        lib().orc_gaborish_sharpen(w, h, iters, P(planes[c]), P(out[c]))
+    return out
+
+
+def gaborish_forward_enc(planes: np.ndarray) -> np.ndarray:
+    """The encoder's Gaborish forward convolution of each plane."""
+    planes = np.ascontiguousarray(planes, np.float32)
+    out = np.empty_like(planes)
+    h, w = planes.shape[1:]
+    for c in range(planes.shape[0]):
+        lib().orc_gaborish_forward_enc(w, h, P(planes[c]), P(out[c]))
+    return out
+
+
 def gaborish(planes: np.ndarray) -> np.ndarray:
     planes = np.ascontiguousarray(planes, np.float32)
     _, h, w = planes.shape

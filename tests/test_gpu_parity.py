@@ -442,3 +442,28 @@ def test_server_results_are_the_same_alone_and_in_the_mix(ctx):
     for im in imgs:
         im.close()
 
+
+@pytest.mark.parametrize("wh", [(61, 37), (256, 130), (7, 9)])
+def test_gaborish_precondition_bit_exact(ctx, wh):
+    """prepare_gaborish_frame: the source XYB replaced by its 12-step Jacobi
+    Gaborish inverse, bit for bit; the structure must be planned again."""
+    import jpxl_amd
+    from jpxl_amd import JpxlgError, _lib
+
+    w, h = wh
+    rgb = _rgb(w, h, 31)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    bmap = orc.fixed_dct8_map(w, h)
+    img.set_structure(bmap)
+    img.precondition_gaborish()
+    got = img.download(_lib.BUF_XYB)
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    want = orc.gaborish_sharpen(xyb, 12)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    with pytest.raises(JpxlgError):
+        img.quantize(_quantizer())
+    img.set_structure(bmap)
+    assert np.array_equal(img.download(_lib.BUF_COEFF).view(np.uint32),
+                          orc.forward_frame(want, bmap).view(np.uint32))
+    img.close()
+

@@ -441,3 +441,50 @@ def test_f64_frame_dct_matches_the_butterfly_oracle():
         bmap = orc.mixed_map(w, h, seed=w)
         rep = error_report(orc.forward_frame(xyb, bmap), forward_frame_f64(xyb, bmap), bmap)
         assert rep and all(r["max_abs"] < 5e-7 for r in rep.values()), rep
+
+
+# --- encoder-side Gaborish sharpen (jpxl-encode/src/vardct/gaborish.rs:66-315)
+
+def _plain_conv3(plane):
+    # the normalised J.1 kernel applied with mirrored edges, in float64
+    w1, w2 = 0.115169525, 0.061248592
+    k = np.array([[w2, w1, w2], [w1, 1.0, w1], [w2, w1, w2]]) / (1 + 4 * w1 + 4 * w2)
+    h, w = plane.shape
+    ys = np.array([[abs(y + d) if y + d >= 0 else -(y + d) - 1 for d in (-1, 0, 1)] for y in range(h)])
+    ys = np.where(ys >= h, 2 * h - 1 - ys, ys)
+    xs = np.array([[abs(x + d) if x + d >= 0 else -(x + d) - 1 for d in (-1, 0, 1)] for x in range(w)])
+    xs = np.where(xs >= w, 2 * w - 1 - xs, xs)
+    p = plane.astype(np.float64)
+    out = np.zeros_like(p)
+    for a in range(3):
+        for b in range(3):
+            out += k[a, b] * p[ys[:, a]][:, xs[:, b]]
+    return out
+
+
+@pytest.mark.parametrize("wh", [(13, 9), (64, 40), (5, 7)])
+def test_encoder_gaborish_forward_is_the_normalised_kernel(wh):
+    # Every branch (mirrored border, four-wide interior groups, scalar tail)
+    # evaluates the same normalised 3x3 kernel up to f32 rounding.
+    w, h = wh
+    rng = np.random.default_rng(w * 7 + h)
+    planes = rng.uniform(-1, 1, (1, h, w)).astype(np.float32)
+    got = orc.gaborish_forward_enc(planes)[0]
+    assert np.abs(got - _plain_conv3(planes[0])).max() < 2e-6
+
+
+def test_gaborish_sharpen_inverts_the_forward_filter():
+    # Jacobi on K x = f: the residual |K x - f| shrinks with the step count
+    # and the 12-step result is close to an exact inverse; a constant plane
+    # (K preserves constants) is a fixed point.
+    rng = np.random.default_rng(3)
+    base = rng.uniform(0, 1, (1, 48, 61)).astype(np.float32)
+    f = orc.gaborish_forward_enc(base)
+    res = []
+    for it in (1, 4, 12):
+        x = orc.gaborish_sharpen(f, it)
+        res.append(np.abs(orc.gaborish_forward_enc(x) - f).max())
+    assert res[0] > res[1] > res[2] and res[2] < 1e-3
+    assert np.abs(orc.gaborish_sharpen(f, 12) - base).max() < 5e-3
+    const = np.full((1, 17, 23), 0.375, np.float32)
+    assert np.abs(orc.gaborish_sharpen(const, 12) - 0.375).max() < 1e-6
