@@ -56,10 +56,16 @@ struct CflParams {
 
 // Device scratch of the frame-LF CfL refinement: the candidate factors of
 // each channel and their residual-bit prices.
+constexpr int kCflLfFastGroups = 256;
 struct CflLfScratch {
   int32_t cands[2][4];
   int32_t ncand[2];
   unsigned long long cost[2][4];
+  // Certified fast path: per-workgroup double-double partial sums (yy, yx,
+  // xx, yb, bb, |yx|, |yb|), and whether their decision stands for the
+  // sequential sums'.
+  double2 part[kCflLfFastGroups][7];
+  int32_t certified;
 };
 
 // One image of a probe batch: everything the per-probe chain (quantize,

@@ -29,9 +29,19 @@ __device__ __forceinline__ float depth_value(float x, const float* thr, const fl
   if (x <= 0.0031308f) e = 12.92f * x;
   else e = 1.055f * __powf(x, 1.0f / 2.4f) - 0.055f;
   float gl = rintf(e * max);
-  int lo = gl <= 0.0f ? 0 : (gl >= max ? levels - 1 : (int)gl);
-  while (lo > 0 && thr[lo - 1] > x) lo--;
-  while (lo < levels - 1 && thr[lo] <= x) lo++;
+  const int g = gl <= 0.0f ? 0 : (gl >= max ? levels - 1 : (int)gl);
+  // The two thresholds either side of the guess, loaded together; the walks
+  // run only when the guess is off.
+  const float tl = g > 0 ? thr[g - 1] : -INFINITY;
+  const float th = g < levels - 1 ? thr[g] : INFINITY;
+  int lo = g;
+  if (tl > x) {
+    lo = g - 1;
+    while (lo > 0 && thr[lo - 1] > x) lo--;
+  } else if (th <= x) {
+    lo = g + 1;
+    while (lo < levels - 1 && thr[lo] <= x) lo++;
+  }
   return lut[lo];
 }
 

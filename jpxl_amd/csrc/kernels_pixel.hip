@@ -388,6 +388,80 @@ __device__ void truncate_wave(int32_t* q, const float* targets, float* recon_y,
 constexpr int quant_vb(int n) { return n == 32 ? 1 : 64 / n; }
 constexpr int quant_threads(int n) { return n == 32 ? 256 : kThreads; }
 
+// The same truncation spread over a group of `nw` waves (DCT32, where one
+// varblock per workgroup would otherwise leave waves idle): lane L of the
+// group holds positions [L*R, L*R + R); the scans and the reduction cross the
+// waves through `xs` (LDS, 2 * nw ints per group).  Every wave of the
+// workgroup must call it (it synchronises the workgroup).
+template <int N>
+__device__ void truncate_group(int32_t* q, const float* targets, float* recon_y, const uint16_t* order,
+                               const float* mat, const float* donor, float sc, float bias, double lambda,
+                               float ztb, int nw, int wi, int* xs) {
+  constexpr int NN = N * N, nb = (N / 8) * (N / 8);
+  constexpr int RMAX = NN / 64;
+  const int R = NN / (64 * nw);
+  const int lane = threadIdx.x & 63;
+  const int pos0 = (wi * 64 + lane) * R;
+  int cell[RMAX];
+  int32_t v[RMAX];
+  int lastnz = -1;
+#pragma unroll
+  for (int r = 0; r < RMAX; r++) {
+    if (r >= R) break;
+    const int p = pos0 + r;
+    cell[r] = order[p];
+    v[r] = p >= nb ? q[cell[r]] : 0;
+    if (v[r] != 0) lastnz = p;
+  }
+  int incl = lastnz;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = max(incl, o);
+  }
+  if (lane == 63) xs[wi] = incl;
+  __syncthreads();
+  int before = -1;  // last nonzero in the group's earlier waves
+  for (int k = 0; k < wi; k++) before = max(before, xs[k]);
+  int prev = __shfl_up(incl, 1, 64);
+  if (lane == 0) prev = -1;
+  prev = max(prev, before);
+  int fail = -1;
+#pragma unroll
+  for (int r = 0; r < RMAX; r++) {
+    if (r >= R) break;
+    const int p = pos0 + r;
+    if (v[r] == 0) continue;
+    const int c = cell[r];
+    const float weight = (float)(lambda * (double)NN * (double)donor[c]);
+    const int exposed = prev >= 0 ? p - prev - 1 : p - nb;
+    const float own = (float)(33 - __clz((unsigned)abs(v[r])));
+    const float saved = own + (float)exposed * ztb;
+    const float target = targets[c];
+    const float step = sc * mat[c];
+    const float recon = bias_adjust(v[r], bias) * step;
+    const float kept = recon - target;
+    const float added = fmaf(target, target, -(kept * kept));
+    if (saved <= weight * added) fail = p;
+    prev = p;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) fail = max(fail, __shfl_xor(fail, d, 64));
+  if (lane == 0) xs[nw + wi] = fail;
+  __syncthreads();
+  for (int k = 0; k < nw; k++) fail = max(fail, xs[nw + k]);
+#pragma unroll
+  for (int r = 0; r < RMAX; r++) {
+    if (r >= R) break;
+    const int p = pos0 + r;
+    if (v[r] != 0 && p > fail) {
+      const int c = cell[r];
+      q[c] = 0;
+      if (recon_y) recon_y[c] = bias_adjust(0, bias) * (sc * mat[c]);
+    }
+  }
+}
+
 template <int N>
 __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* __restrict__ D,
                                                                const int* __restrict__ off, int nd,
@@ -497,10 +571,18 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6;
-  if (pp->truncate)
-    for (int j = wave; j < nvb; j += T / 64)
-      truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
-                       tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
+  __shared__ int xs[2][8];
+  if (pp->truncate) {
+    if constexpr (N == 32) {
+      // one varblock: all T/64 waves walk its luma block together
+      truncate_group<N>(qs[1][0], co[1][0], dyh[0], tab->order + cb, tab->matrix[1] + cb, tab->donor + cb,
+                        pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits, T / 64, wave, xs[0]);
+    } else {
+      for (int j = wave; j < nvb; j += T / 64)
+        truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
+                         tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
+    }
+  }
   // --- X and B LF against the reconstructed Y LF ---
   {
     const float klf[2] = {0.0f + (float)cflp->x_factor_lf / 84.0f, 1.0f + (float)cflp->b_factor_lf / 84.0f};
@@ -527,13 +609,21 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
     qs[c][j][k] = v;
   }
   __syncthreads();
-  if (pp->truncate)
-    for (int w = wave; w < 2 * VB; w += T / 64) {
-      const int ci = w / VB, j = w % VB, c = ci == 0 ? 0 : 2;
-      if (j < nvb)
-        truncate_wave<N>(qs[c][j], co[c][j], nullptr, tab->order + cb, tab->matrix[c] + cb,
-                         tab->donor + cb, pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
+  if (pp->truncate) {
+    if constexpr (N == 32) {
+      // X on the first half of the waves, B on the second half
+      const int half = (T / 64) / 2, ci = wave / half, c = ci == 0 ? 0 : 2;
+      truncate_group<N>(qs[c][0], co[c][0], nullptr, tab->order + cb, tab->matrix[c] + cb, tab->donor + cb,
+                        pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits, half, wave % half, xs[ci]);
+    } else {
+      for (int w = wave; w < 2 * VB; w += T / 64) {
+        const int ci = w / VB, j = w % VB, c = ci == 0 ? 0 : 2;
+        if (j < nvb)
+          truncate_wave<N>(qs[c][j], co[c][j], nullptr, tab->order + cb, tab->matrix[c] + cb,
+                           tab->donor + cb, pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
+      }
     }
+  }
   __syncthreads();
   int32_t* qout[3] = {QX, QY, QB};
   for (int e = threadIdx.x; e < 3 * VB * NN; e += T) {

@@ -493,9 +493,121 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
 // launches: the regression seeds (one workgroup, sequential sums), the
 // residual prices of every candidate (an order-free integer reduction over
 // the whole grid), and the joint selection.
+// --- certified fast path of the frame-LF regression ------------------------
+// The reference sums ~(blocks) products sequentially in f64.  The exact sums
+// (double-double, any order) differ from the sequential ones by at most
+// gamma_(n-1) * sum|t| (the recursive-summation bound, gamma_k = k u/(1-k u));
+// when the seeds best_factor derives are provably the same for every sum
+// within that bound of the exact ones, they are the sequential path's seeds
+// and the long chain is skipped.  Otherwise the chain runs.
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double err = (a.hi - (s - bb)) + (b.hi - bb);
+  const double e = err + (a.lo + b.lo);
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
+}
+
+__device__ __forceinline__ DD dd_of(double v) { return DD{v, 0.0}; }
+
+__global__ __launch_bounds__(256) void k_cfl_lf_fast_partial(uint32_t count, const float4* __restrict__ lf_s,
+                                                             CflLfScratch* __restrict__ sc) {
+  __shared__ DD red[256][7];
+  DD acc[7];
+#pragma unroll
+  for (int k = 0; k < 7; k++) acc[k] = DD{0.0, 0.0};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x) {
+    const float4 v = lf_s[i];
+    const double y = (double)v.y, x = (double)v.x, b = (double)v.z;
+    const double t[7] = {y * y, y * x, x * x, y * b, b * b, fabs(y * x), fabs(y * b)};  // exact products
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] = dd_add(acc[k], dd_of(t[k]));
+  }
+#pragma unroll
+  for (int k = 0; k < 7; k++) red[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride)
+#pragma unroll
+      for (int k = 0; k < 7; k++) red[threadIdx.x][k] = dd_add(red[threadIdx.x][k], red[threadIdx.x + stride][k]);
+    __syncthreads();
+  }
+  if (threadIdx.x < 7) sc->part[blockIdx.x][threadIdx.x] = make_double2(red[0][threadIdx.x].hi, red[0][threadIdx.x].lo);
+}
+
+// best_factor_dev's seed from the exact sums, or INT_MIN when some sum within
+// `gam` (relative to its absolute sum) of the exact one could change it.
+__device__ int certified_seed(double syy, double syc, double scc, double ayc, float base, double gam) {
+  if (syy <= 0.0) return 0;  // non-negative terms: zero exactly when every term is zero
+  const double u = 1.1102230246251565e-16;
+  const double dyy = gam * syy, dyc = gam * ayc, dcc = gam * scc;
+  if (dyy >= 0.5 * syy) return INT_MIN;
+  const double k = syc / syy;
+  const double dk = (dyc + fabs(k) * dyy) / (syy - dyy) + 8.0 * u * fabs(k) + 1e-300;
+  const double star = (k - (double)base) * 84.0;
+  const double dstar = 84.0 * dk + 8.0 * u * (fabs(star) + 84.0 * fabs((double)base)) + 1e-300;
+  const double fr = fabs(star) - floor(fabs(star));
+  if (fabs(fr - 0.5) <= 2.0 * dstar) return INT_MIN;
+  const int seed = best_factor_dev(syy, syc, scc, base);
+  // the argmin among the candidates must hold under the perturbation
+  const double r = round(star);
+  const long long centre = r >= 9.2e18 ? LLONG_MAX : (r <= -9.2e18 ? LLONG_MIN : (long long)r);
+  int cand[6];
+  for (int i = 0; i < 6; i++) {
+    if (i < 5) {
+      long long c = centre + (i - 2);
+      cand[i] = (int)(c < -128 ? -128 : (c > 127 ? 127 : c));
+    } else {
+      cand[i] = 0;
+    }
+  }
+  const double kb = (double)cfl_k(base, seed);
+  const double eb = scc - 2.0 * kb * syc + kb * kb * syy;
+  const double bb = dcc + 2.0 * fabs(kb) * dyc + kb * kb * dyy + 12.0 * u * (scc + 2.0 * fabs(kb) * fabs(syc) + kb * kb * syy);
+  for (int i = 0; i < 6; i++) {
+    if (cand[i] == seed) continue;
+    const double kj = (double)cfl_k(base, cand[i]);
+    const double ej = scc - 2.0 * kj * syc + kj * kj * syy;
+    const double bj = dcc + 2.0 * fabs(kj) * dyc + kj * kj * dyy + 12.0 * u * (scc + 2.0 * fabs(kj) * fabs(syc) + kj * kj * syy);
+    if (!(ej - eb > 2.0 * (bj + bb))) return INT_MIN;
+  }
+  return seed;
+}
+
+__global__ void k_cfl_lf_fast_final(uint32_t count, int groups, CflLfScratch* __restrict__ sc) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DD tot[7];
+  for (int k = 0; k < 7; k++) tot[k] = DD{0.0, 0.0};
+  for (int g = 0; g < groups; g++)
+    for (int k = 0; k < 7; k++) tot[k] = dd_add(tot[k], DD{sc->part[g][k].x, sc->part[g][k].y});
+  double S[7];
+  for (int k = 0; k < 7; k++) S[k] = tot[k].hi + tot[k].lo;
+  // recursive-summation bound for n terms, with room for the double-double
+  // and final rounding
+  const double n = (double)count + 4.0;
+  const double u = 1.1102230246251565e-16;
+  const double gam = 2.0 * (n * u / (1.0 - n * u)) + 1e-28;
+  const int sx = certified_seed(S[0], S[1], S[2], S[5], 0.0f, gam);
+  const int sb = certified_seed(S[0], S[3], S[4], S[6], 1.0f, gam);
+  const int ok = sx != INT_MIN && sb != INT_MIN;
+  if (ok) {
+    sc->ncand[0] = factor_candidates(sx, sc->cands[0]);
+    sc->ncand[1] = factor_candidates(sb, sc->cands[1]);
+    for (int ch = 0; ch < 2; ch++)
+      for (int j = 0; j < 4; j++) sc->cost[ch][j] = 0;
+  }
+  sc->certified = ok;
+}
+
 __global__ __launch_bounds__(kCflThreads) void k_cfl_lf_seed(uint32_t count, const float4* __restrict__ lf_s,
                                                              CflLfScratch* __restrict__ sc) {
   __shared__ double stage[5][kCflStage];
+  if (sc->certified) return;  // the fast path's seeds stand
   const CflSums a = cfl_sequential_sums(lf_s, 0, count, stage);
   if (threadIdx.x == 0) {
     sc->ncand[0] = factor_candidates(best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f), sc->cands[0]);
@@ -607,6 +719,9 @@ void launch_cfl_tiles(hipStream_t s, const Geom& g, const ConstTables* tab, cons
 
 void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, const float4* lf_s,
                    CflLfScratch* scratch, CflParams* cflp) {
+  const int groups = (int)std::max(1u, std::min((unsigned)kCflLfFastGroups, (lf_count + 1023) / 1024));
+  k_cfl_lf_fast_partial<<<groups, 256, 0, s>>>(lf_count, lf_s, scratch);
+  k_cfl_lf_fast_final<<<1, 64, 0, s>>>(lf_count, groups, scratch);
   k_cfl_lf_seed<<<1, kCflThreads, 0, s>>>(lf_count, lf_s, scratch);
   const unsigned blocks = std::max(1u, std::min(1024u, (lf_count + 2047) / 2048));
   k_cfl_lf_price<<<blocks, 256, 0, s>>>(lf_count, pp, lf_s, scratch);

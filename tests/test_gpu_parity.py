@@ -443,7 +443,7 @@ def test_server_results_are_the_same_alone_and_in_the_mix(ctx):
         im.close()
 
 
-@pytest.mark.parametrize("wh", [(61, 37), (256, 130), (7, 9)])
+@pytest.mark.parametrize("wh", [(61, 37), (256, 130), (9, 11)])
 def test_gaborish_precondition_bit_exact(ctx, wh):
     """prepare_gaborish_frame: the source XYB replaced by its 12-step Jacobi
     Gaborish inverse, bit for bit; the structure must be planned again."""
