@@ -214,7 +214,7 @@ size_t image_bytes(int w, int h) {
   b += 3 * pyr1 * 4;                 // linear RGB pyramid (scales >= 1)
   b += 3 * px * 4;                   // linear RGB at scale 0 (reference build)
   size_t tiles = 0;
-  for (int s = 0; s < n; s++) tiles += (size_t)pool_tiles(sw[s], sh[s]);
+  for (int s = 0; s < n; s++) tiles += (size_t)std::max(pool_tiles(sw[s], sh[s]), pool_units(sw[s], sh[s]));
   b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
   b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
   b += 256 * 256;  // alignment of every carved buffer
@@ -449,7 +449,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_err = (uint32_t*)A.take(4);
   img->d_gray = (int*)A.take(4);
   size_t ptiles = 0;
-  for (int s = 0; s < img->nscales; s++) ptiles += (size_t)pool_tiles(img->sw[s], img->sh[s]);
+  for (int s = 0; s < img->nscales; s++)
+    ptiles += (size_t)std::max(pool_tiles(img->sw[s], img->sh[s]), pool_units(img->sw[s], img->sh[s]));
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
@@ -930,7 +931,7 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
       }
     for (int s = 0; s < img->nscales; s++) {
       P.ptile_off[s] = pt;
-      pt += pool_tiles(img->sw[s], img->sh[s]);
+      pt += pool_units(img->sw[s], img->sh[s]);
     }
     P.partials = img->d_partials;
     P.sums = (double*)(dev + L.sums) + (size_t)k * 18 * kDescScales;
@@ -957,7 +958,7 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
     }
     for (int s = 0; s < kDescScales; s++) {
       ho[L.p_off + k * kDescScales + s] = tot_p;
-      if (s < img->nscales) tot_p += pool_tiles(img->sw[s], img->sh[s]);
+      if (s < img->nscales) tot_p += 3 * pool_units(img->sw[s], img->sh[s]);
     }
   }
   for (int i = 0; i < 3; i++) {
@@ -1002,7 +1003,7 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
       Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour", st);
       launch_colour_b(st, s, D, O + L.c_off[s], n, tot_c[s], src);
     }
-    { Prof p(ctx, "blur_pool", st); launch_blur_pool_b(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, tot_p); }
+    { Prof p(ctx, "blur_pool", st); launch_blur_pool_stream(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, tot_p); }
     { Prof p(ctx, "pool_reduce", st); launch_reduce_b(st, D, n); }
     // Keep the restored planes in rx (downloads read them there).
     if (src)

@@ -190,6 +190,13 @@ void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, cons
                    CflLfScratch* scratch, CflParams* cflp);
 // Pooling tiles of one scale (one partial row of 18 sums per tile).
 int pool_tiles(int w, int h);
+// Tiles of the streaming candidate pass (64-column strips x row segments);
+// the batched chain's partial rows and reduction use these.
+int pool_units(int w, int h);
+// The streaming candidate pass: `units` (tile, channel) pairs over the
+// flattened (desc, scale) prefix `off` (in units), four per workgroup.
+void launch_blur_pool_stream(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
+                             int units);
 constexpr int kMaxPoolScales = 6;
 // Where each scale's tiles start in the partials buffer (in tiles).
 struct PoolLayout {

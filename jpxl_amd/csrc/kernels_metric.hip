@@ -267,6 +267,9 @@ constexpr int kPW = 64, kPH = 24;
 constexpr int kPIW = kPW + 2 * kHalo, kPIH = kPH + 2 * kHalo;  // 72 x 32
 constexpr int kHRun = 8;                                      // horizontal outputs per lane
 __host__ __device__ constexpr int pool_tiles_dev(int w, int h) { return ((w + kPW - 1) / kPW) * ((h + kPH - 1) / kPH); }
+// Streaming pass: 64-column strips cut into segments of kSH output rows.
+constexpr int kSH = 64;
+__host__ __device__ constexpr int pool_units_dev(int w, int h) { return ((w + kPW - 1) / kPW) * ((h + kSH - 1) / kSH); }
 constexpr int kVRun = 6;                                      // vertical outputs per lane
 
 template <int R>
@@ -398,6 +401,183 @@ __global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict
                  P.s11[sc][c], P.x2[sc][c], partials + (size_t)tile * 18 + c * 6);
 }
 
+// ---------------------------------------------------------------------------
+// Streaming form of the candidate pass: one wave walks a 64-column strip of
+// one channel down a segment of kPH output rows.  Each input row is loaded
+// once (plus a 4-column halo either side), widened to f64 once per sample
+// into the wave's LDS row, filtered horizontally (one output column per
+// lane) and pushed into a 9-row ring held in the lane's registers; the
+// vertical filter of the row 4 behind runs from that ring, so the vertical
+// halo costs 8 rows per segment instead of a recomputed band per tile.  The
+// taps, pairing and roundings are those of blur_pool_tile, so the sums are
+// the same per pixel.  The row loop is unrolled by the ring size so every
+// ring index is a register.
+// ---------------------------------------------------------------------------
+struct StreamRing {
+  float m[9][3];       // horizontally filtered moments (mu2, s22, s12) per ring row
+  float c1[9], c2[9];  // centre samples x1, x2 of each ring row
+  // loads issued two rows ahead: input samples (a: column gx-4, b: the right
+  // halo on lanes 0..7) and the pooling inputs of the output row
+  float a1[3], a2[3], b1[3], b2[3], mu1[3], s11[3];
+};
+
+// Issues the loads of step t (input row y0-4+t, output row y0+t-8) into slot Q.
+template <int Q>
+__device__ __forceinline__ void stream_load(int t, int y0, int y1, int w, int h, int gx, int lane,
+                                            const float* __restrict__ X1, const float* __restrict__ X2,
+                                            const float* __restrict__ MU1, const float* __restrict__ S11,
+                                            StreamRing& R) {
+  const int yi = y0 - 4 + t, yo = yi - 4;
+  const bool rin = yi >= 0 && yi < h;
+  const int xa = gx - 4, xb = gx + 60;
+  const bool ina = rin && xa >= 0 && xa < w;
+  R.a1[Q] = ina ? X1[(size_t)yi * w + xa] : 0.0f;
+  R.a2[Q] = ina ? X2[(size_t)yi * w + xa] : 0.0f;
+  const bool inb = lane < 8 && rin && xb < w;
+  R.b1[Q] = inb ? X1[(size_t)yi * w + xb] : 0.0f;
+  R.b2[Q] = inb ? X2[(size_t)yi * w + xb] : 0.0f;
+  const bool out = t >= 8 && yo < y1 && gx < w;
+  R.mu1[Q] = out ? MU1[(size_t)yo * w + gx] : 0.0f;
+  R.s11[Q] = out ? S11[(size_t)yo * w + gx] : 0.0f;
+}
+
+template <int K>
+__device__ __forceinline__ void stream_row(int t, int rows, int y0, int y1, int w, int h, int gx, int lane,
+                                           const float* __restrict__ X1, const float* __restrict__ X2,
+                                           const float* __restrict__ MU1, const float* __restrict__ S11,
+                                           const double* hf, double (*row)[72], StreamRing& R,
+                                           double* acc) {
+  constexpr int Q = K % 3;
+  // the loads of two steps ahead go out before this step's work
+  if (t + 2 < rows) stream_load<(K + 2) % 3>(t + 2, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, R);
+  const float a1 = R.a1[Q], a2 = R.a2[Q], b1 = R.b1[Q], b2 = R.b2[Q];
+  const int yo = y0 - 8 + t;
+  const bool out = t >= 8 && yo < y1 && gx < w;
+  const float mu1 = R.mu1[Q], s11 = R.s11[Q];
+  // widened samples into the wave's LDS row: x2, x2^2, x1*x2 (the squared and
+  // product inputs rounded to f32 first, blur.rs:97-113)
+  row[0][lane] = (double)a2;
+  row[1][lane] = (double)(a2 * a2);
+  row[2][lane] = (double)(a1 * a2);
+  if (lane < 8) {
+    row[0][64 + lane] = (double)b2;
+    row[1][64 + lane] = (double)(b2 * b2);
+    row[2][64 + lane] = (double)(b1 * b2);
+  }
+  // the row's writes by every lane before any lane reads its neighbours
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // centre samples of this input row at column gx: lane+4's a*, or for the
+  // last four lanes the right halo's b* (all lanes take part in both shuffles)
+  {
+    const float sa1 = __shfl(a1, (lane + 4) & 63, 64), sa2 = __shfl(a2, (lane + 4) & 63, 64);
+    const float sb1 = __shfl(b1, (lane - 60) & 63, 64), sb2 = __shfl(b2, (lane - 60) & 63, 64);
+    R.c1[K] = lane >= 60 ? sb1 : sa1;
+    R.c2[K] = lane >= 60 ? sb2 : sa2;
+  }
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    const double* v = row[m] + lane + 4;  // centre of this lane's window
+    double acc_h = hf[0] * v[0];
+#pragma unroll
+    for (int k = 1; k <= 4; k++) acc_h = fma(hf[k], v[-k] + v[k], acc_h);
+    R.m[K][m] = (float)acc_h;
+  }
+  // every lane's reads of this row before the next row overwrites it
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (!out) return;
+  // vertical filter over ring rows yo-4 .. yo+4: slots (K+1) .. K (mod 9)
+  float vm[3];
+#pragma unroll
+  for (int m = 0; m < 3; m++) {
+    double d[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) d[k] = (double)R.m[(K + 1 + k) % 9][m];
+    double a = hf[0] * d[4];
+#pragma unroll
+    for (int k = 1; k <= 4; k++) a = fma(hf[k], d[4 - k] + d[4 + k], a);
+    vm[m] = (float)a;
+  }
+  const float i1 = R.c1[(K + 5) % 9], i2 = R.c2[(K + 5) % 9];
+  const float mu2 = vm[0], s22 = vm[1], s12 = vm[2];
+  const float C2 = 0.0009f;
+  const float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
+  const float md = mu1 - mu2;
+  const float num_m = 1.0f - md * md;
+  const float num_s = 2.0f * (s12 - mu12) + C2;
+  const float den_s = (s11 - mu11) + (s22 - mu22) + C2;
+  double dd = 1.0 - (double)((num_m * num_s) / den_s);
+  dd = dd > 0.0 ? dd : 0.0;
+  acc[0] += dd;
+  acc[1] += dd * dd * dd * dd;
+  double d1 = (1.0 + (double)fabsf(i2 - mu2)) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
+  double art = d1 > 0.0 ? d1 : 0.0;
+  double lost = -d1 > 0.0 ? -d1 : 0.0;
+  acc[2] += art;
+  acc[3] += art * art * art * art;
+  acc[4] += lost;
+  acc[5] += lost * lost * lost * lost;
+}
+
+// One wave: strip x0 .. x0+63 of channel planes, output rows y0 .. y1-1; the
+// 6 sums of this (strip, segment, channel) go to out6 (lane 0).
+__device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, int x0, int y0,
+                            const float* __restrict__ X1, const float* __restrict__ MU1,
+                            const float* __restrict__ S11, const float* __restrict__ X2,
+                            double (*row)[72], double* __restrict__ out6) {
+  const int lane = threadIdx.x & 63;
+  const int gx = x0 + lane;
+  const int y1 = min(y0 + kSH, h);
+  double hf[5];
+#pragma unroll
+  for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
+  StreamRing R;
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    R.m[k][0] = R.m[k][1] = R.m[k][2] = 0.0f;
+    R.c1[k] = R.c2[k] = 0.0f;
+  }
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  const int rows = (y1 - y0) + 8;  // input rows y0-4 .. y1+3 (rows >= 9)
+  stream_load<0>(0, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, R);
+  stream_load<1>(1, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, R);
+  for (int t = 0; t < rows; t += 9) {
+#define JPXLG_STEP(K)                                                                                  \
+    if (t + K < rows)                                                                                  \
+      stream_row<K>(t + K, rows, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, hf, row, R, acc);
+    JPXLG_STEP(0) JPXLG_STEP(1) JPXLG_STEP(2) JPXLG_STEP(3) JPXLG_STEP(4)
+    JPXLG_STEP(5) JPXLG_STEP(6) JPXLG_STEP(7) JPXLG_STEP(8)
+#undef JPXLG_STEP
+  }
+#pragma unroll
+  for (int k = 0; k < 6; k++) {
+    const double v = wave_sum(acc[k]);
+    if (lane == 0) out6[k] = v;
+  }
+}
+
+// Batched streaming launch: four (tile, channel) units per workgroup, one
+// per wave; units enumerate (desc, scale) through off[] in units (tiles * 3).
+__global__ __launch_bounds__(256) void k_blur_pool_stream(const ConstTables* __restrict__ tab,
+                                                          const ProbeDesc* __restrict__ D,
+                                                          const int* __restrict__ off, int nds, int total) {
+  __shared__ double rows[4][3][72];
+  const int wave = threadIdx.x >> 6;
+  const int unit = blockIdx.x * 4 + wave;
+  if (unit >= total) return;
+  const int ds = find_desc(off, nds, unit);
+  const int d = ds / kDescScales, sc = ds % kDescScales;
+  const ProbeDesc& P = D[d];
+  const int w = P.sw[sc], h = P.sh[sc], tx = (w + kPW - 1) / kPW;
+  const int tiles = pool_units_dev(w, h);
+  const int lu = unit - off[ds], c = lu / tiles, lt = lu % tiles;
+  stream_unit(tab, w, h, (lt % tx) * kPW, (lt / tx) * kSH, P.ref_x[sc][c], P.ref_mu[sc][c], P.ref_s11[sc][c],
+              P.cand_x[sc][c], rows[wave], P.partials + (size_t)(P.ptile_off[sc] + lt) * 18 + c * 6);
+}
+
 // Batched: workgroups enumerate (desc, scale, tile) through off[] over the
 // flattened index d * kDescScales + scale.
 __global__ __launch_bounds__(256) void k_blur_pool_b(const ConstTables* __restrict__ tab,
@@ -434,7 +614,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const ProbeDesc* __restrict__ 
   const int s = rem / 18, term = rem % 18;
   const ProbeDesc& P = D[d];
   if (s >= P.nscales) return;
-  const int tiles = pool_tiles_dev(P.sw[s], P.sh[s]);
+  const int tiles = pool_units_dev(P.sw[s], P.sh[s]);
   const double* p = P.partials + (size_t)P.ptile_off[s] * 18;
   double acc = 0.0;
   for (int t = threadIdx.x; t < tiles; t += 256) acc += p[(size_t)t * 18 + term];
@@ -484,6 +664,7 @@ void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, fl
 }
 
 int pool_tiles(int w, int h) { return pool_tiles_dev(w, h); }
+int pool_units(int w, int h) { return pool_units_dev(w, h); }
 
 void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, double* partials) {
   int total = 0;
@@ -505,6 +686,12 @@ void launch_blur_pool_b(hipStream_t s, const ConstTables* tab, const ProbeDesc* 
                         int nwg) {
   if (nwg <= 0) return;
   k_blur_pool_b<<<dim3(nwg, 1, 3), 256, 0, s>>>(tab, D, off, nds);
+}
+
+void launch_blur_pool_stream(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
+                             int units) {
+  if (units <= 0) return;
+  k_blur_pool_stream<<<div_up(units, 4), 256, 0, s>>>(tab, D, off, nds, units);
 }
 
 void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd) {
