@@ -403,7 +403,7 @@ __global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict
 
 // ---------------------------------------------------------------------------
 // Streaming form of the candidate pass: one wave walks a 64-column strip of
-// one channel down a segment of kPH output rows.  Each input row is loaded
+// one channel down a segment of kSH output rows.  Each input row is loaded
 // once (plus a 4-column halo either side), widened to f64 once per sample
 // into the wave's LDS row, filtered horizontally (one output column per
 // lane) and pushed into a 9-row ring held in the lane's registers; the
@@ -416,44 +416,28 @@ __global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict
 struct StreamRing {
   float m[9][3];       // horizontally filtered moments (mu2, s22, s12) per ring row
   float c1[9], c2[9];  // centre samples x1, x2 of each ring row
-  // loads issued two rows ahead: input samples (a: column gx-4, b: the right
-  // halo on lanes 0..7) and the pooling inputs of the output row
-  float a1[3], a2[3], b1[3], b2[3], mu1[3], s11[3];
 };
 
-// Issues the loads of step t (input row y0-4+t, output row y0+t-8) into slot Q.
-template <int Q>
-__device__ __forceinline__ void stream_load(int t, int y0, int y1, int w, int h, int gx, int lane,
-                                            const float* __restrict__ X1, const float* __restrict__ X2,
-                                            const float* __restrict__ MU1, const float* __restrict__ S11,
-                                            StreamRing& R) {
-  const int yi = y0 - 4 + t, yo = yi - 4;
-  const bool rin = yi >= 0 && yi < h;
-  const int xa = gx - 4, xb = gx + 60;
-  const bool ina = rin && xa >= 0 && xa < w;
-  R.a1[Q] = ina ? X1[(size_t)yi * w + xa] : 0.0f;
-  R.a2[Q] = ina ? X2[(size_t)yi * w + xa] : 0.0f;
-  const bool inb = lane < 8 && rin && xb < w;
-  R.b1[Q] = inb ? X1[(size_t)yi * w + xb] : 0.0f;
-  R.b2[Q] = inb ? X2[(size_t)yi * w + xb] : 0.0f;
-  const bool out = t >= 8 && yo < y1 && gx < w;
-  R.mu1[Q] = out ? MU1[(size_t)yo * w + gx] : 0.0f;
-  R.s11[Q] = out ? S11[(size_t)yo * w + gx] : 0.0f;
-}
-
 template <int K>
-__device__ __forceinline__ void stream_row(int t, int rows, int y0, int y1, int w, int h, int gx, int lane,
+__device__ __forceinline__ void stream_row(int t, int y0, int y1, int w, int h, int gx, int lane,
                                            const float* __restrict__ X1, const float* __restrict__ X2,
                                            const float* __restrict__ MU1, const float* __restrict__ S11,
                                            const double* hf, double (*row)[72], StreamRing& R,
                                            double* acc) {
-  constexpr int Q = K % 3;
-  // the loads of two steps ahead go out before this step's work
-  if (t + 2 < rows) stream_load<(K + 2) % 3>(t + 2, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, R);
-  const float a1 = R.a1[Q], a2 = R.a2[Q], b1 = R.b1[Q], b2 = R.b2[Q];
-  const int yo = y0 - 8 + t;
+  const int yi = y0 - 4 + t, yo = yi - 4;
   const bool out = t >= 8 && yo < y1 && gx < w;
-  const float mu1 = R.mu1[Q], s11 = R.s11[Q];
+  // input samples of row yi (a: column gx-4, b: the right halo on lanes
+  // 0..7) and the pooling inputs of output row yo
+  const bool rin = yi >= 0 && yi < h;
+  const int xa = gx - 4, xb = gx + 60;
+  const bool ina = rin && xa >= 0 && xa < w;
+  const bool inb = lane < 8 && rin && xb < w;
+  const float a1 = ina ? X1[(size_t)yi * w + xa] : 0.0f;
+  const float a2 = ina ? X2[(size_t)yi * w + xa] : 0.0f;
+  const float b1 = inb ? X1[(size_t)yi * w + xb] : 0.0f;
+  const float b2 = inb ? X2[(size_t)yi * w + xb] : 0.0f;
+  const float mu1 = out ? MU1[(size_t)yo * w + gx] : 0.0f;
+  const float s11 = out ? S11[(size_t)yo * w + gx] : 0.0f;
   // widened samples into the wave's LDS row: x2, x2^2, x1*x2 (the squared and
   // product inputs rounded to f32 first, blur.rs:97-113)
   row[0][lane] = (double)a2;
@@ -542,12 +526,10 @@ __device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, i
   }
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const int rows = (y1 - y0) + 8;  // input rows y0-4 .. y1+3 (rows >= 9)
-  stream_load<0>(0, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, R);
-  stream_load<1>(1, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, R);
   for (int t = 0; t < rows; t += 9) {
-#define JPXLG_STEP(K)                                                                                  \
-    if (t + K < rows)                                                                                  \
-      stream_row<K>(t + K, rows, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, hf, row, R, acc);
+// step t + K writes ring slot K (compile-time over the unroll of 9)
+#define JPXLG_STEP(K) \
+    if (t + K < rows) stream_row<K>(t + K, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, hf, row, R, acc);
     JPXLG_STEP(0) JPXLG_STEP(1) JPXLG_STEP(2) JPXLG_STEP(3) JPXLG_STEP(4)
     JPXLG_STEP(5) JPXLG_STEP(6) JPXLG_STEP(7) JPXLG_STEP(8)
 #undef JPXLG_STEP
