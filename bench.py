@@ -445,7 +445,21 @@ def main() -> int:
 
     for _ in range(args.warmup):
         step()
+    # Per-kernel breakdown: one untimed pass with every launch bracketed by
+    # events (the events themselves cost ~15% of the step, so this pass is
+    # kept out of the timed region).
     ctx.profile(True)
+    ctx.profile_read()  # clear
+    for _ in range(args.steps):
+        step()
+    breakdown = ctx.profile_read()
+    rows = sorted(breakdown.items(), key=lambda kv: -kv[1][0])
+    # Dominant kernel by total time: the one whose roofline is reported.
+    dominant = next((name for name, _ in rows if algorithmic_bytes(name, WIDTH, HEIGHT, sizes) is not None),
+                    None)
+
+    # Timed region: only the dominant kernel's launches carry events.
+    ctx.profile_only(dominant)
     ctx.profile_read()  # clear
     hl.hipDeviceSynchronize()
     t0 = time.perf_counter()
@@ -454,22 +468,20 @@ def main() -> int:
     elapsed = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
+    ctx.profile_only(None)
 
     ms_per_step = 1000.0 * elapsed / args.steps
     mp = WIDTH * HEIGHT / 1e6
     value = mp / (elapsed / args.steps)
 
-    # Dominant kernel by total time over the timed region.
-    rows = sorted(prof.items(), key=lambda kv: -kv[1][0])
     roofline = None
-    for name, (ms, launches) in rows:
-        b = algorithmic_bytes(name, WIDTH, HEIGHT, sizes)
-        if b is None:
-            continue
+    if dominant in prof:
+        ms, launches = prof[dominant]
+        b = algorithmic_bytes(dominant, WIDTH, HEIGHT, sizes)
         avg_s = ms / 1000.0 / launches
         achieved = b / avg_s / 1e9
         roofline = {
-            "kernel": name,
+            "kernel": dominant,
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -479,7 +491,6 @@ def main() -> int:
             "avg_launch_us": round(1e6 * avg_s, 2),
             "bytes_per_launch": int(b),
         }
-        break
     r0 = results[-1]
     out = {
         "metric": "VarDCT encode+verify MP/s at q85",
@@ -504,7 +515,7 @@ def main() -> int:
             "varblocks": {str(k): v for k, v in sizes.items()},
         },
         "roofline": roofline,
-        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in rows},
+        "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in rows},  # untimed breakdown pass
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(rgb, int(r0.probes))

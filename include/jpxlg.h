@@ -225,6 +225,9 @@ void jpxlg_server_destroy(jpxlg_server* srv);
  * kernel totals (name, milliseconds, launches) and returns the entry count;
  * names holds cap entries of 48 chars.  Reading also clears the totals. */
 int32_t jpxlg_profile_enable(jpxlg_ctx* ctx, int32_t on);
+/* Restricts the timing to launches of one kernel name (NULL or "" = all), so a
+ * timed run carries two events per launch of that kernel only. */
+int32_t jpxlg_profile_only(jpxlg_ctx* ctx, const char* name);
 int32_t jpxlg_profile_read(jpxlg_ctx* ctx, char* names, double* total_ms, uint64_t* launches,
                            int32_t cap);
 

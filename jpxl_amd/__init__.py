@@ -84,6 +84,10 @@ class Context:
     def profile(self, on: bool = True) -> None:
         self.check(self.lib.jpxlg_profile_enable(self.handle, int(on)))
 
+    def profile_only(self, name: str | None) -> None:
+        """Time only launches of kernel `name` (None: every launch)."""
+        self.check(self.lib.jpxlg_profile_only(self.handle, name.encode() if name else None))
+
     def profile_read(self) -> dict:
         """{kernel: (total_ms, launches)} since the last read (HIP events)."""
         cap = 64

@@ -132,6 +132,7 @@ SIGNATURES = {
     "jpxlg_epf_step": (_i32, [_vp, _i32, _u32, _u32, _vp, _vp, _vp]),
     "jpxlg_gaborish": (_i32, [_vp, _u32, _u32, _vp, _vp]),
     "jpxlg_profile_enable": (_i32, [_vp, _i32]),
+    "jpxlg_profile_only": (_i32, [_vp, ctypes.c_char_p]),
     "jpxlg_profile_read": (_i32, [_vp, _vp, _dp, ctypes.POINTER(ctypes.c_uint64), _i32]),
 }
 

@@ -41,6 +41,7 @@ struct jpxlg_ctx {
   hipStream_t side = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   bool prof = false;
+  std::string prof_only;  // when non-empty, only launches of this name are timed
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, uint64_t>> totals;
@@ -135,7 +136,7 @@ struct Prof {
   hipStream_t st;
   hipEvent_t a = nullptr;
   Prof(jpxlg_ctx* c, const char* n, hipStream_t s = nullptr) : ctx(c), name(n), st(s ? s : c->stream) {
-    if (ctx->prof) {
+    if (ctx->prof && (ctx->prof_only.empty() || ctx->prof_only == name)) {
       a = take_event(ctx);
       (void)hipEventRecord(a, st);
     }
@@ -1740,6 +1741,12 @@ extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t hei
 extern "C" int32_t jpxlg_profile_enable(jpxlg_ctx* ctx, int32_t on) {
   if (!ctx) return JPXLG_E_INVALID;
   ctx->prof = on != 0;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_profile_only(jpxlg_ctx* ctx, const char* name) {
+  if (!ctx) return JPXLG_E_INVALID;
+  ctx->prof_only = name ? name : "";
   return JPXLG_OK;
 }
 

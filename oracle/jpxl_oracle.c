@@ -12,9 +12,9 @@
  * -ffp-contract=off so that no multiply-add is fused unless the reference
  * itself calls mul_add (written here as fmaf).
  *
- * Parity status per function (how each is pinned) is listed in the header of
- * oracle/README-free form in DESIGN.md; functions without a golden vector or
- * closed-form check are marked "parity unpinned" there.
+ * Parity: tests/test_oracle_cpu.py pins each function against the
+ * reference's own known-answer tests, closed forms or brute force.  The
+ * functions no such check reaches say "parity unpinned" beside them.
  */
 #include <math.h>
 #include <stdint.h>
@@ -681,11 +681,13 @@ EXPORT void orc_donor_weights(int side, float *weights) {
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
-    uint32_t global_scale, hf_mul, quant_lf;
+    uint32_t global_scale, hf_mul, quant_lf;  /* the wire values (after the AQ factorization) */
     int x_qm, b_qm;
     int truncate;       /* QuantizerChoiceMode::TrailingTruncation */
     float lambda_scale; /* request.lambda_scale */
     float zero_token_bits;
+    const float *aq_adj; /* per-atom AQ adjustment (bw x bh), NULL = no field */
+    int aq_refine_only, aq_fine;
 } orc_quant_params;
 
 typedef struct {
@@ -726,10 +728,12 @@ static double lambda_for(const orc_quant_params *p, int channel) {
     return mean > 0.0 ? scale * 16.0 / mean : 0.0;
 }
 
-/* quantize.rs:301-366 HfQuantizer::new_with_matrices + lib.rs:2149-2170. */
-static void build_hf_quantizer(hf_quantizer *q, int side, const orc_quant_params *p) {
+/* quantize.rs:301-366 HfQuantizer::new_with_matrices + lib.rs:2149-2170, for
+ * the (transform, HfMul) key `hf_mul`; the Lagrange weights come from the
+ * frame's baseline DCT8 quantizer (p->hf_mul) whatever the key. */
+static void build_hf_quantizer_mul(hf_quantizer *q, int side, const orc_quant_params *p, uint32_t hf_mul) {
     q->side = side;
-    float mul = hf_multiplier(p->global_scale, p->hf_mul);
+    float mul = hf_multiplier(p->global_scale, hf_mul);
     float scale[3] = {mul * qm_multiplier(p->x_qm), mul, mul * qm_multiplier(p->b_qm)};
     float freq[1024];
     orc_donor_weights(side, freq);
@@ -747,6 +751,10 @@ static void build_hf_quantizer(hf_quantizer *q, int side, const orc_quant_params
         }
     }
     orc_natural_order(side, q->order);
+}
+
+static void build_hf_quantizer(hf_quantizer *q, int side, const orc_quant_params *p) {
+    build_hf_quantizer_mul(q, side, p, p->hf_mul);
 }
 
 /* quantize.rs:1400-1499 choose (nearest reconstruction, ties to smaller |q|). */
@@ -816,6 +824,204 @@ static int32_t lf_quantize(const float *mdc, float target, int c, int *ok) {
 }
 
 /* ------------------------------------------------------------------------ */
+/* Adaptive quantization (jpxl-encode-policy/src/analysis.rs:111-158,        */
+/* field.rs:224-420, lib.rs:1847-1944)                                       */
+/* ------------------------------------------------------------------------ */
+
+/* field.rs:52-91 AqMode, in wire order of the C ABI. */
+enum { AQ_OFF = 0, AQ_MASKING = 1, AQ_UNIFORM = 2, AQ_FINE_MASKING = 3, AQ_FINE_UNIFORM = 4, AQ_EDGE_REFINE = 5 };
+#define AQ_FINE_BASELINE 16u                 /* field.rs:93 */
+#define AQ_STRENGTH 0.25f                    /* field.rs:99 */
+#define AQ_CLAMP 1.0f                        /* field.rs:104 */
+#define AQ_EDGE_CONTRAST 6.0f                /* field.rs:108 */
+#define AQ_CHROMA_WEIGHT 0.35f               /* field.rs:112 */
+#define AQ_VARIANCE_TO_8BIT (255.0f * 255.0f) /* field.rs:117 */
+#define MAX_GLOBAL_SCALE (8193u + 65535u)     /* jpxl-encode/src/vardct/ids.rs:20 */
+#define MAX_QUANT_LF 65536u                   /* ids.rs:24 */
+#define MAX_HF_MUL 2147483647u                /* ids.rs:28 */
+
+/* The activity's f32 log2.  The reference calls the platform's log2f; this
+ * is the correctly rounded value (log2 in f64, one rounding), which a
+ * faithful libm log2f matches except within an ulp of a rounding boundary.
+ * The field's decisions (a half-octave snap of footprint means) move only
+ * at exact ties, so that ulp is parity unpinned. */
+static float aq_log2f(float x) { return (float)log2((double)x); }
+
+/* analysis.rs:111-158 AnalysisAtlas::analyze: per 8x8 atom and channel the
+ * sample variance over the samples that exist, sums in f64 in raster
+ * order.  var: 3 planes of bw x bh. */
+EXPORT void orc_atlas_variance(int w, int h, const float *xyb[3], float *var) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8;
+    for (int ay = 0; ay < bh; ay++)
+        for (int ax = 0; ax < bw; ax++) {
+            int x0 = ax * 8, y0 = ay * 8, x1 = x0 + 8 < w ? x0 + 8 : w, y1 = y0 + 8 < h ? y0 + 8 : h;
+            double count = (double)(x1 - x0) * (double)(y1 - y0);
+            for (int c = 0; c < 3; c++) {
+                double sum = 0.0, sum_sq = 0.0;
+                for (int y = y0; y < y1; y++)
+                    for (int x = x0; x < x1; x++) {
+                        double v = (double)xyb[c][(size_t)y * w + x];
+                        sum += v;
+                        sum_sq += v * v;
+                    }
+                double mean = count > 0.0 ? sum / count : 0.0;
+                double variance = count > 0.0 ? sum_sq / count - mean * mean : 0.0;
+                if (!(variance > 0.0)) variance = 0.0; /* f64::max(0.0) */
+                var[(size_t)c * bw * bh + (size_t)ay * bw + ax] = (float)variance;
+            }
+        }
+}
+
+/* field.rs:200-219 erode_min_3x3 (clamped at the frame edge). */
+static void erode_min_3x3(const float *a, int w, int h, float *out) {
+    for (int y = 0; y < h; y++)
+        for (int x = 0; x < w; x++) {
+            float m = INFINITY;
+            for (int ny = y > 0 ? y - 1 : 0; ny < (y + 2 < h ? y + 2 : h); ny++)
+                for (int nx = x > 0 ? x - 1 : 0; nx < (x + 2 < w ? x + 2 : w); nx++) {
+                    float v = a[ny * w + nx];
+                    m = fminf(m, v);
+                }
+            out[y * w + x] = isfinite(m) ? m : 0.0f;
+        }
+}
+
+/* field.rs:383-395 mul_of: baseline * 2^-adj rounded to the nearest integer,
+ * clamped to [1, 2 * baseline]. */
+EXPORT uint32_t orc_aq_mul_of(uint32_t baseline, float adj) {
+    double v = round((double)baseline * exp2((double)-adj));
+    uint32_t mul = !(v > 0.0) ? 0u : (v >= 4294967295.0 ? 4294967295u : (uint32_t)v);
+    uint32_t hi = baseline > 0x7fffffffu ? 0xffffffffu : baseline * 2u;
+    if (mul < 1u) mul = 1u;
+    if (mul > hi) mul = hi;
+    return (mul >= 1u && mul <= MAX_HF_MUL) ? mul : baseline;
+}
+
+/* field.rs:224-300 DesiredQuantField::from_atlas_tuned at the default tuning
+ * (field.rs:129-158), and field.rs:312-320 is_neutral.  adj: bw x bh.
+ * Returns 0 when there is no field (AQ_OFF) or it is neutral (AqSetup::build
+ * then treats it as off, lib.rs:1868-1875), else 1; *fine = the lattice. */
+EXPORT int orc_aq_field(int w, int h, const float *xyb[3], int mode, float *adj, int *fine) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8, count = bw * bh;
+    float sign;
+    int is_fine, edge = mode == AQ_EDGE_REFINE;
+    switch (mode) {
+        case AQ_MASKING: sign = 1.0f; is_fine = 0; break;
+        case AQ_UNIFORM: sign = -1.0f; is_fine = 0; break;
+        case AQ_FINE_MASKING: sign = 1.0f; is_fine = 1; break;
+        case AQ_FINE_UNIFORM: sign = -1.0f; is_fine = 1; break;
+        case AQ_EDGE_REFINE: sign = -1.0f; is_fine = 1; break;
+        default: *fine = 0; return 0;
+    }
+    *fine = is_fine;
+    float *var = malloc(sizeof(float) * 3 * count), *act = malloc(sizeof(float) * count);
+    float *ero = malloc(sizeof(float) * count);
+    orc_atlas_variance(w, h, xyb, var);
+    for (int i = 0; i < count; i++) {
+        float vx = var[i], vy = var[count + i], vb = var[2 * count + i];
+        float luma = aq_log2f(1.0f + vy * AQ_VARIANCE_TO_8BIT);
+        float chroma = aq_log2f(1.0f + (vx + vb) * AQ_VARIANCE_TO_8BIT);
+        act[i] = luma + AQ_CHROMA_WEIGHT * chroma;
+    }
+    float k = sign * AQ_STRENGTH;
+    if (edge) {
+        erode_min_3x3(act, bw, bh, ero);
+        for (int i = 0; i < count; i++) {
+            float d = act[i] - ero[i] - AQ_EDGE_CONTRAST;
+            adj[i] = k * (d > 0.0f ? d : 0.0f);
+        }
+    } else {
+        const float *a = act;
+        if (is_fine) { erode_min_3x3(act, bw, bh, ero); a = ero; }
+        double sum = 0.0;
+        for (int i = 0; i < count; i++) sum += (double)a[i];
+        float mean = count > 0 ? (float)(sum / (double)count) : 0.0f;
+        for (int i = 0; i < count; i++) {
+            float v = k * (a[i] - mean);
+            adj[i] = v < -AQ_CLAMP ? -AQ_CLAMP : (v > AQ_CLAMP ? AQ_CLAMP : v);
+        }
+    }
+    free(var); free(act); free(ero);
+    int neutral = 1;
+    for (int i = 0; i < count && neutral; i++) {
+        if (is_fine) neutral = orc_aq_mul_of(AQ_FINE_BASELINE, adj[i]) == AQ_FINE_BASELINE;
+        else neutral = roundf(adj[i] * 2.0f) == 0.0f;
+    }
+    return neutral ? 0 : 1;
+}
+
+/* field.rs:338-378 mul_for_footprint: the log-mean of the footprint's
+ * adjustments (f32, rows then columns, zero outside the grid), snapped to
+ * the half-octave lattice (legacy) or clamped to one octave (fine), capped
+ * at zero under refine-only. */
+EXPORT uint32_t orc_aq_footprint_mul(const float *adj, int bw, int bh, int bx, int by, int rows, int cols,
+                                     uint32_t baseline, int refine_only, int fine) {
+    if (!adj) return baseline;
+    float sum = 0.0f;
+    uint32_t count = 0;
+    for (int dy = 0; dy < rows; dy++)
+        for (int dx = 0; dx < cols; dx++) {
+            int x = bx + dx, y = by + dy;
+            sum += (x < bw && y < bh) ? adj[(size_t)y * bw + x] : 0.0f;
+            count++;
+        }
+    float a = count > 0 ? sum / (float)count : 0.0f;
+    if (fine) a = a < -1.0f ? -1.0f : (a > 1.0f ? 1.0f : a);
+    else a = roundf(a * 2.0f) / 2.0f;
+    if (refine_only) a = fminf(a, 0.0f);
+    return orc_aq_mul_of(baseline, a);
+}
+
+/* lib.rs:1859-1908 AqSetup::build's factorization: global_scale / factor
+ * (snapped down to a multiple), quant_lf and the baseline HfMul times the
+ * factor (2 legacy, 16 fine); refine-only with the request's values when a
+ * factored value is not representable.  Returns refine_only. */
+EXPORT int orc_aq_setup(uint32_t gs, uint32_t quant_lf, uint32_t hf_mul, int fine, uint32_t *gs_out,
+                        uint32_t *qlf_out, uint32_t *base_out) {
+    uint32_t factor = fine ? AQ_FINE_BASELINE : 2u;
+    uint32_t snapped = gs - gs % factor;
+    uint64_t g2 = snapped / factor;
+    if (g2 < 1) g2 = 1;
+    uint64_t q2 = (uint64_t)quant_lf * factor, m2 = (uint64_t)hf_mul * factor;
+    if (q2 > 0xffffffffu) q2 = 0xffffffffu;
+    if (m2 > 0xffffffffu) m2 = 0xffffffffu;
+    int ok = snapped >= factor && g2 >= 1 && g2 <= MAX_GLOBAL_SCALE && q2 >= 1 && q2 <= MAX_QUANT_LF && m2 >= 1 &&
+             m2 <= MAX_HF_MUL;
+    if (ok) {
+        *gs_out = (uint32_t)g2; *qlf_out = (uint32_t)q2; *base_out = (uint32_t)m2;
+        return 0;
+    }
+    *gs_out = gs; *qlf_out = quant_lf; *base_out = hf_mul;
+    return 1;
+}
+
+/* lib.rs:4569-4577 mul_signal_bits and lib.rs:1933-1938: 8 bits for a
+ * non-baseline multiplier on the legacy lattice, a flat 2 on the fine one. */
+static double aq_signal_bits(const orc_quant_params *p, uint32_t mul) {
+    if (!p->aq_adj) return 0.0;
+    if (p->aq_fine) return 2.0;
+    return mul == p->hf_mul ? 0.0 : 8.0;
+}
+
+static uint32_t aq_vb_mul(const orc_quant_params *p, int bw, int bh, int bx, int by, int n) {
+    return orc_aq_footprint_mul(p->aq_adj, bw, bh, bx, by, n, n, p->hf_mul, p->aq_refine_only, p->aq_fine);
+}
+
+/* The (transform, HfMul) quantizers of one frame, built on first use. */
+typedef struct { int side; uint32_t mul; hf_quantizer q; } hq_entry;
+typedef struct { hq_entry *e; int n, cap; const orc_quant_params *p; } hq_cache;
+static const hf_quantizer *hq_get(hq_cache *c, int side, uint32_t mul) {
+    for (int i = 0; i < c->n; i++)
+        if (c->e[i].side == side && c->e[i].mul == mul) return &c->e[i].q;
+    if (c->n == c->cap) { c->cap = c->cap ? 2 * c->cap : 8; c->e = realloc(c->e, sizeof(hq_entry) * c->cap); }
+    hq_entry *e = &c->e[c->n++];
+    e->side = side; e->mul = mul;
+    build_hf_quantizer_mul(&e->q, side, c->p, mul);
+    return &e->q;
+}
+static void hq_free(hq_cache *c) { free(c->e); c->e = NULL; c->n = c->cap = 0; }
+
+/* ------------------------------------------------------------------------ */
 /* Frame planning: forward DCT + quantize (lib.rs:3009-3384, :4469)          */
 /* ------------------------------------------------------------------------ */
 
@@ -865,10 +1071,7 @@ EXPORT int orc_quantize_frame(int w, int h, const uint8_t *bmap, const float *co
                               const int8_t *cfl_b, int lf_x, int lf_b, int32_t *quant[3],
                               int32_t *lf[3]) {
     int bw = (w + 7) / 8, bh = (h + 7) / 8, stride = 8 * bw, tw = (bw + 7) / 8;
-    static hf_quantizer hq[3];
-    build_hf_quantizer(&hq[0], 8, p);
-    build_hf_quantizer(&hq[1], 16, p);
-    build_hf_quantizer(&hq[2], 32, p);
+    hq_cache hc = {NULL, 0, 0, p};
     float mdc[3];
     lf_multipliers(p->global_scale, p->quant_lf, mdc);
     float k_x_lf = cfl_k(0.0f, lf_x), k_b_lf = cfl_k(1.0f, lf_b);
@@ -879,8 +1082,8 @@ EXPORT int orc_quantize_frame(int w, int h, const uint8_t *bmap, const float *co
         for (int bx = 0; bx < bw; bx++) {
             int side = bmap[by * bw + bx];
             if (!side) continue;
-            const hf_quantizer *q = &hq[side == 8 ? 0 : (side == 16 ? 1 : 2)];
             int n = side / 8, cells = side * side;
+            const hf_quantizer *q = hq_get(&hc, side, aq_vb_mul(p, bw, bh, bx, by, n));
             for (int c = 0; c < 3; c++)
                 for (int k = 0; k < cells; k++)
                     vb[c][k] = coeff[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side];
@@ -920,7 +1123,19 @@ EXPORT int orc_quantize_frame(int w, int h, const uint8_t *bmap, const float *co
                 for (int k = 0; k < cells; k++)
                     quant[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side] = qv[c][k];
         }
+    hq_free(&hc);
     return ok ? 0 : -1;
+}
+
+/* The HfMul of every varblock of `bmap` (at its top-left block, 0 on covered
+ * blocks): lib.rs:736-750 (fixed DCT8 cover), :1618-1650 (a given map). */
+EXPORT void orc_aq_block_muls(int w, int h, const uint8_t *bmap, const orc_quant_params *p, uint32_t *muls) {
+    int bw = (w + 7) / 8, bh = (h + 7) / 8;
+    for (int by = 0; by < bh; by++)
+        for (int bx = 0; bx < bw; bx++) {
+            int side = bmap[by * bw + bx];
+            muls[by * bw + bx] = side ? aq_vb_mul(p, bw, bh, bx, by, side / 8) : 0u;
+        }
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1101,6 +1316,8 @@ typedef struct {
     int gaborish;    /* 0/1 */
     int sharpness;   /* uniform EPF sharpness 0..7 */
     int bits;        /* output depth for linear_at_depth */
+    const float *aq_adj; /* per-atom AQ adjustment, NULL = hf_mul everywhere */
+    int aq_refine_only, aq_fine;
 } orc_render_params;
 
 /* lib.rs:749-1309 render_inner with linear_at_depth: dequant, CfL, LLF, IDCT,
@@ -1133,7 +1350,6 @@ EXPORT void orc_render(int w, int h, const uint8_t *bmap, const int32_t *quant[3
         lfp[1][i] = dy;
         lfp[2][i] = db + kb_lf * dy;
     }
-    float mul = hf_multiplier(p->global_scale, p->hf_mul);
     float mats[3][3][1024];
     for (int s = 0; s < 3; s++)
         for (int c = 0; c < 3; c++) orc_dequant_matrix(8 << s, c, mats[s][c]);
@@ -1143,6 +1359,10 @@ EXPORT void orc_render(int w, int h, const uint8_t *bmap, const int32_t *quant[3
             int side = bmap[by * bw + bx];
             if (!side) continue;
             int si = side == 8 ? 0 : (side == 16 ? 1 : 2), nb = side / 8, cells = side * side;
+            /* lib.rs:876-877: the varblock's own HfMul */
+            uint32_t vmul = orc_aq_footprint_mul(p->aq_adj, bw, bh, bx, by, nb, nb, p->hf_mul, p->aq_refine_only,
+                                                 p->aq_fine);
+            float mul = hf_multiplier(p->global_scale, vmul);
             for (int c = 0; c < 3; c++) {
                 float scale = mul * qm[c];
                 for (int k = 0; k < cells; k++) {
@@ -1440,7 +1660,8 @@ static double cover_fixed_bits(int side) { return side == 16 ? 8.4 + 4.0 : (side
 typedef struct {
     int w, h, bw, bh;
     const float *xyb[3];
-    hf_quantizer *hq[3];
+    hq_cache hc;
+    const orc_quant_params *p;
     double lambda[3];
     float donor[3][1024];
 } cover_ctx;
@@ -1448,9 +1669,9 @@ typedef struct {
 /* lib.rs:4909-5062 block_cost_bounded without the cutoff (the cutoff only
  * prunes candidates whose full cost could not win), with lib.rs:4685-4890
  * score_channel_lanes for Y, then X and B against the reconstructed dY. */
-static double block_cost(cover_ctx *cc, int side, int bx, int by) {
-    int si = side == 8 ? 0 : (side == 16 ? 1 : 2), n = side / 8, cells = side * side;
-    const hf_quantizer *q = cc->hq[si];
+static double block_cost(cover_ctx *cc, int side, int bx, int by, uint32_t mul) {
+    int si = side == 8 ? 0 : (side == 16 ? 1 : 2), n = side / 8;
+    const hf_quantizer *q = hq_get(&cc->hc, side, mul);
     float smp[1024], co[3][1024], dyh[1024];
     for (int c = 0; c < 3; c++) {
         gather_square(cc->xyb[c], cc->w, cc->h, 8 * bx, 8 * by, side, smp);
@@ -1487,7 +1708,8 @@ static double block_cost(cover_ctx *cc, int side, int bx, int by) {
 static double tile_region(cover_ctx *cc, int bx, int by, int size, uint8_t *bmap, int write) {
     if (bx >= cc->bw || by >= cc->bh) return 0.0;
     if (size == 1) {
-        double cost = block_cost(cc, 8, bx, by) + cover_fixed_bits(8) + 0.0;
+        uint32_t mul = aq_vb_mul(cc->p, cc->bw, cc->bh, bx, by, 1);
+        double cost = block_cost(cc, 8, bx, by, mul) + cover_fixed_bits(8) + aq_signal_bits(cc->p, mul);
         if (write) bmap[by * cc->bw + bx] = 8;
         return cost;
     }
@@ -1500,9 +1722,10 @@ static double tile_region(cover_ctx *cc, int bx, int by, int size, uint8_t *bmap
     int single = 0;
     double best = split;
     if (fits) {
-        double fixed = cover_fixed_bits(side) + 0.0;
+        uint32_t mul = aq_vb_mul(cc->p, cc->bw, cc->bh, bx, by, size);
+        double fixed = cover_fixed_bits(side) + aq_signal_bits(cc->p, mul);
         if (!(fixed >= split)) {
-            double c = block_cost(cc, side, bx, by) + fixed;
+            double c = block_cost(cc, side, bx, by, mul) + fixed;
             if (c < split) { single = 1; best = c; }
         }
     }
@@ -1525,16 +1748,14 @@ EXPORT void orc_cover(int w, int h, const float *xyb[3], const orc_quant_params 
     cover_ctx cc;
     cc.w = w; cc.h = h; cc.bw = (w + 7) / 8; cc.bh = (h + 7) / 8;
     for (int c = 0; c < 3; c++) cc.xyb[c] = xyb[c];
-    for (int i = 0; i < 3; i++) {
-        cc.hq[i] = malloc(sizeof(hf_quantizer));
-        build_hf_quantizer(cc.hq[i], 8 << i, p);
-        orc_donor_weights(8 << i, cc.donor[i]);
-    }
+    cc.hc = (hq_cache){NULL, 0, 0, p};
+    cc.p = p;
+    for (int i = 0; i < 3; i++) orc_donor_weights(8 << i, cc.donor[i]);
     for (int c = 0; c < 3; c++) cc.lambda[c] = lambda_for(p, c);
     memset(bmap, 0, (size_t)cc.bw * cc.bh);
     for (int sby = 0; sby < cc.bh; sby += 4)
         for (int sbx = 0; sbx < cc.bw; sbx += 4) tile_region(&cc, sbx, sby, 4, bmap, 1);
-    for (int i = 0; i < 3; i++) free(cc.hq[i]);
+    hq_free(&cc.hc);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1672,10 +1893,7 @@ static void refine_lf_factors(const cfl_set *x, const cfl_set *b, const float *m
 EXPORT void orc_estimate_cfl(int w, int h, const uint8_t *bmap, const float *coeff[3],
                              const orc_quant_params *p, int8_t *cfl_x, int8_t *cfl_b, int *lf_x, int *lf_b) {
     int bw = (w + 7) / 8, bh = (h + 7) / 8, stride = 8 * bw, tw = (bw + 7) / 8, th = (bh + 7) / 8;
-    static hf_quantizer hq[3];
-    build_hf_quantizer(&hq[0], 8, p);
-    build_hf_quantizer(&hq[1], 16, p);
-    build_hf_quantizer(&hq[2], 32, p);
+    hq_cache hc = {NULL, 0, 0, p};
     float mdc[3];
     lf_multipliers(p->global_scale, p->quant_lf, mdc);
     cfl_set lfx = {0}, lfb = {0};
@@ -1692,7 +1910,8 @@ EXPORT void orc_estimate_cfl(int w, int h, const uint8_t *bmap, const float *coe
                     for (int bx = gbx0; bx < gbx1; bx++) {
                         int side = bmap[by * bw + bx];
                         if (!side) continue;
-                        int si = side == 8 ? 0 : (side == 16 ? 1 : 2), n = side / 8, cells = side * side;
+                        int n = side / 8, cells = side * side;
+                        const hf_quantizer *q = hq_get(&hc, side, aq_vb_mul(p, bw, bh, bx, by, n));
                         for (int c = 0; c < 3; c++)
                             for (int k = 0; k < cells; k++)
                                 vb[c][k] = coeff[c][(size_t)(8 * by + k / side) * stride + 8 * bx + k % side];
@@ -1712,7 +1931,7 @@ EXPORT void orc_estimate_cfl(int w, int h, const uint8_t *bmap, const float *coe
                                 int cell = row * side + col;
                                 int cell8 = (cell / side / fold) * 8 + (cell % side / fold);
                                 float y = vb[1][cell];
-                                float dy = hf_recon(&hq[si], hf_choose(&hq[si], y, 1, cell, &ok), 1, cell);
+                                float dy = hf_recon(q, hf_choose(q, y, 1, cell, &ok), 1, cell);
                                 set_push(&tx[tile], vb[0][cell], y, dy, cell8);
                                 set_push(&tb[tile], vb[2][cell], y, dy, cell8);
                             }
@@ -1720,10 +1939,12 @@ EXPORT void orc_estimate_cfl(int w, int h, const uint8_t *bmap, const float *coe
                     }
         }
     refine_lf_factors(&lfx, &lfb, mdc, lf_x, lf_b);
+    const hf_quantizer *base8 = hq_get(&hc, 8, p->hf_mul);  /* lib.rs:3737 the baseline DCT8 quantizer */
     for (int t = 0; t < tw * th; t++) {
-        cfl_x[t] = (int8_t)refine_hf_factor(&tx[t], 0.0f, 0, &hq[0]);
-        cfl_b[t] = (int8_t)refine_hf_factor(&tb[t], 1.0f, 2, &hq[0]);
+        cfl_x[t] = (int8_t)refine_hf_factor(&tx[t], 0.0f, 0, base8);
+        cfl_b[t] = (int8_t)refine_hf_factor(&tb[t], 1.0f, 2, base8);
         free(tx[t].s); free(tb[t].s);
     }
     free(tx); free(tb); free(lfx.s); free(lfb.s);
+    hq_free(&hc);
 }

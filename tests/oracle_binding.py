@@ -25,6 +25,9 @@ class QuantParams(ctypes.Structure):
         ("truncate", ctypes.c_int),
         ("lambda_scale", ctypes.c_float),
         ("zero_token_bits", ctypes.c_float),
+        ("aq_adj", ctypes.c_void_p),
+        ("aq_refine_only", ctypes.c_int),
+        ("aq_fine", ctypes.c_int),
     ]
 
 
@@ -39,6 +42,9 @@ class RenderParams(ctypes.Structure):
         ("gaborish", ctypes.c_int),
         ("sharpness", ctypes.c_int),
         ("bits", ctypes.c_int),
+        ("aq_adj", ctypes.c_void_p),
+        ("aq_refine_only", ctypes.c_int),
+        ("aq_fine", ctypes.c_int),
     ]
 
 
@@ -62,6 +68,13 @@ def lib():
         o.orc_quantize_frame.restype = ctypes.c_int
         o.orc_prepare_frame.restype = ctypes.c_int
         o.orc_depth_tables.restype = ctypes.c_int
+        o.orc_aq_field.restype = ctypes.c_int
+        o.orc_aq_mul_of.restype = ctypes.c_uint32
+        o.orc_aq_mul_of.argtypes = [ctypes.c_uint32, ctypes.c_float]
+        o.orc_aq_footprint_mul.restype = ctypes.c_uint32
+        o.orc_aq_footprint_mul.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_uint32, ctypes.c_int,
+                                                                                   ctypes.c_int]
+        o.orc_aq_setup.restype = ctypes.c_int
         _o = o
     return _o
 
@@ -318,3 +331,67 @@ def mixed_map(w: int, h: int, seed: int) -> np.ndarray:
                 taken[by : by + n, bx : bx + n] = True
                 break
     return bm
+
+
+# --- adaptive quantization ------------------------------------------------
+AQ_OFF, AQ_MASKING, AQ_UNIFORM, AQ_FINE_MASKING, AQ_FINE_UNIFORM, AQ_EDGE_REFINE = range(6)
+
+
+def atlas_variance(xyb: np.ndarray) -> np.ndarray:
+    """Per-atom sample variance, (3, bh, bw) f32."""
+    _, h, w = xyb.shape
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    out = np.zeros((3, (h + 7) // 8, (w + 7) // 8), np.float32)
+    lib().orc_atlas_variance(w, h, planes_ptr([xyb[0], xyb[1], xyb[2]]), P(out))
+    return out
+
+
+def aq_field(xyb: np.ndarray, mode: int):
+    """-> (adj (bh, bw) f32, active, fine): active is False for no field or
+    a neutral one."""
+    _, h, w = xyb.shape
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    adj = np.zeros(((h + 7) // 8, (w + 7) // 8), np.float32)
+    fine = ctypes.c_int()
+    active = lib().orc_aq_field(w, h, planes_ptr([xyb[0], xyb[1], xyb[2]]), int(mode), P(adj), ctypes.byref(fine))
+    return adj, bool(active), bool(fine.value)
+
+
+def aq_setup(gs: int, quant_lf: int, hf_mul: int, fine: bool):
+    """-> (global_scale, quant_lf, baseline hf_mul, refine_only) on the wire."""
+    g, q, m = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+    ro = lib().orc_aq_setup(gs, quant_lf, hf_mul, int(fine), ctypes.byref(g), ctypes.byref(q), ctypes.byref(m))
+    return g.value, q.value, m.value, bool(ro)
+
+
+def aq_mul_of(baseline: int, adj: float) -> int:
+    return int(lib().orc_aq_mul_of(baseline, adj))
+
+
+def aq_footprint_mul(adj: np.ndarray, bx: int, by: int, rows: int, cols: int, baseline: int, refine_only: bool,
+                     fine: bool) -> int:
+    adj = np.ascontiguousarray(adj, np.float32)
+    bh, bw = adj.shape
+    return int(lib().orc_aq_footprint_mul(P(adj), bw, bh, bx, by, rows, cols, baseline, int(refine_only),
+                                          int(fine)))
+
+
+def with_aq(params, adj: np.ndarray | None, refine_only: bool = False, fine: bool = False):
+    """Attach an AQ field to QuantParams/RenderParams (kept alive on the object)."""
+    if adj is None:
+        params.aq_adj = None
+        params._aq = None
+    else:
+        adj = np.ascontiguousarray(adj, np.float32)
+        params._aq = adj
+        params.aq_adj = adj.ctypes.data
+    params.aq_refine_only = int(refine_only)
+    params.aq_fine = int(fine)
+    return params
+
+
+def aq_block_muls(w: int, h: int, bmap: np.ndarray, q: QuantParams) -> np.ndarray:
+    bm = np.ascontiguousarray(bmap, np.uint8)
+    out = np.zeros(bm.shape, np.uint32)
+    lib().orc_aq_block_muls(w, h, P(bm), ctypes.byref(q), P(out))
+    return out

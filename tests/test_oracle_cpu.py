@@ -488,3 +488,200 @@ def test_gaborish_sharpen_inverts_the_forward_filter():
     assert np.abs(orc.gaborish_sharpen(f, 12) - base).max() < 5e-3
     const = np.full((1, 17, 23), 0.375, np.float32)
     assert np.abs(orc.gaborish_sharpen(const, 12) - 0.375).max() < 1e-6
+
+
+# --- adaptive quantization (jpxl-encode-policy/src/analysis.rs, field.rs,
+# lib.rs:1847-1944)
+
+def _half_flat_half_noise(w: int, h: int) -> np.ndarray:
+    """field.rs:441-456: left half gray 128, right half hashed noise."""
+    x = np.arange(w, dtype=np.uint64)[None, :]
+    y = np.arange(h, dtype=np.uint64)[:, None]
+    m32 = np.uint64(0xFFFFFFFF)
+    hsh = (((x * np.uint64(0x9E37)) & m32) + ((y * np.uint64(0x79B9)) & m32)) & m32
+    hsh = (hsh * np.uint64(0x85EBCA6B)) & m32
+    noise = (96 + ((hsh >> np.uint64(24)) & np.uint64(0x3F))).astype(np.uint8)
+    luma = np.where(x < w // 2, np.uint8(128), noise).astype(np.uint8)
+    return np.repeat(luma[:, :, None], 3, axis=2)
+
+
+def _xyb_of_rgb8(rgb: np.ndarray) -> np.ndarray:
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    return xyb
+
+
+def _xyb_of_linear(planes: np.ndarray) -> np.ndarray:
+    _, h, w = planes.shape
+    out = np.zeros((3, h, w), np.float32)
+    for y in range(h):
+        for x in range(w):
+            out[:, y, x] = orc.xyb_pixel(planes[:, y, x])
+    return out
+
+
+def test_atlas_flat_frame_has_zero_variance():
+    # analysis.rs:493-518: 20x12 flat linear 0.25 -> a 3x2 grid, all zero
+    xyb = _xyb_of_linear(np.full((3, 12, 20), 0.25, np.float32))
+    var = orc.atlas_variance(xyb)
+    assert var.shape == (3, 2, 3)
+    assert np.all(np.abs(var) < 1e-9)
+
+
+def test_atlas_edge_variance_stays_in_its_atom():
+    # analysis.rs:521-542: a step at x = 4 lies in atom column 0 only
+    plane = np.full((8, 16), 0.1, np.float32)
+    plane[:, 4:8] = 0.9
+    var = orc.atlas_variance(_xyb_of_linear(np.stack([plane] * 3)))
+    assert var[1, 0, 0] > var[1, 0, 1]
+    assert abs(var[1, 0, 1]) < 1e-9
+
+
+def test_atlas_variance_matches_the_population_variance():
+    # closed form: E[x^2] - E[x]^2 over the samples that exist (partial edge
+    # atoms included), f64, narrowed to f32
+    rng = np.random.default_rng(11)
+    xyb = rng.random((3, 21, 27), dtype=np.float32)
+    var = orc.atlas_variance(xyb)
+    for c in range(3):
+        for ay in range(3):
+            for ax in range(4):
+                blk = xyb[c, 8 * ay:8 * ay + 8, 8 * ax:8 * ax + 8].astype(np.float64)
+                want = max(float(np.mean(blk * blk) - np.mean(blk) ** 2), 0.0)
+                assert abs(float(var[c, ay, ax]) - want) <= 1e-7 * max(want, 1e-3)
+
+
+def test_masking_points_coarse_at_noise_and_fine_at_flat():
+    # field.rs:459-464
+    adj, active, fine = orc.aq_field(_xyb_of_rgb8(_half_flat_half_noise(128, 64)), orc.AQ_MASKING)
+    assert active and not fine
+    assert adj[1, 1] < 0.0
+    assert adj[1, 12] > 0.0
+
+
+def test_uniform_is_the_exact_mirror_of_masking():
+    # field.rs:467-478
+    xyb = _xyb_of_rgb8(_half_flat_half_noise(128, 64))
+    m, _, _ = orc.aq_field(xyb, orc.AQ_MASKING)
+    u, _, _ = orc.aq_field(xyb, orc.AQ_UNIFORM)
+    for x, y in [(1, 1), (12, 1), (6, 4)]:
+        assert abs(m[y, x] + u[y, x]) < 1e-6
+
+
+def test_off_and_flat_frames_give_no_field():
+    # field.rs:481-493: Off has no field; a flat frame's field is neutral
+    # (every multiplier the baseline), which AqSetup treats as off
+    xyb = _xyb_of_rgb8(np.full((64, 64, 3), 100, np.uint8))
+    _, active, _ = orc.aq_field(xyb, orc.AQ_OFF)
+    assert not active
+    adj, active, _ = orc.aq_field(xyb, orc.AQ_MASKING)
+    assert not active
+    for ax, ay in [(0, 0), (3, 3), (7, 7)]:
+        assert orc.aq_footprint_mul(adj, ax, ay, 1, 1, 2, False, False) == 2
+
+
+def test_factorized_multipliers_and_refine_only_cap():
+    # field.rs:496-515
+    adj, _, _ = orc.aq_field(_xyb_of_rgb8(_half_flat_half_noise(128, 64)), orc.AQ_MASKING)
+    fine = orc.aq_footprint_mul(adj, 1, 1, 1, 1, 2, False, False)
+    coarse = orc.aq_footprint_mul(adj, 12, 1, 1, 1, 2, False, False)
+    assert 2 < fine <= 4
+    assert coarse < 2
+    assert orc.aq_footprint_mul(adj, 12, 1, 1, 1, 2, True, False) == 2
+    assert orc.aq_footprint_mul(adj, 1, 1, 1, 1, 2, True, False) == fine
+
+
+def test_fine_lattice_and_edge_refine_never_coarsens():
+    # field.rs:609-657 (the edge arm at the default tuning)
+    xyb = _xyb_of_rgb8(_half_flat_half_noise(128, 64))
+    adj, active, fine = orc.aq_field(xyb, orc.AQ_FINE_MASKING)
+    assert active and fine
+    muls = {orc.aq_footprint_mul(adj, x, y, 1, 1, 16, False, True) for y in range(8) for x in range(16)}
+    assert min(muls) >= 8 and max(muls) <= 32
+    assert orc.aq_footprint_mul(adj, 1, 1, 1, 1, 16, False, True) > orc.aq_footprint_mul(adj, 12, 1, 1, 1, 16,
+                                                                                           False, True)
+    edge, _, efine = orc.aq_field(xyb, orc.AQ_EDGE_REFINE)
+    assert efine
+    for y in range(8):
+        for x in range(16):
+            assert orc.aq_footprint_mul(edge, x, y, 1, 1, 16, False, True) >= 16
+    assert orc.aq_footprint_mul(edge, 1, 3, 1, 1, 16, False, True) == 16
+
+
+def test_mul_of_closed_forms():
+    # field.rs:383-395: round(baseline * 2^-adj), clamped to [1, 2 baseline]
+    assert [orc.aq_mul_of(2, a) for a in (-1.0, -0.5, 0.0, 0.5, 1.0)] == [4, 3, 2, 1, 1]
+    for a in np.linspace(-1.0, 1.0, 33, dtype=np.float32):
+        want = min(max(int(math.floor(16 * 2.0 ** (-float(a)) + 0.5)), 1), 32)
+        assert orc.aq_mul_of(16, float(a)) == want
+    assert orc.aq_mul_of(1, 3.0) == 1  # never below 1
+    assert orc.aq_mul_of(5, -4.0) == 10  # capped one octave finer
+
+
+def test_footprint_mean_snaps_to_half_octaves():
+    # field.rs:338-378: f32 mean in row-major order, half-octave snap (ties
+    # away from zero), zero outside the grid
+    rng = np.random.default_rng(3)
+    adj = (rng.random((9, 13), dtype=np.float32) * 2 - 1).astype(np.float32)
+    for (bx, by, n) in [(0, 0, 1), (3, 2, 2), (8, 4, 4), (12, 8, 2), (11, 7, 4)]:
+        s = np.float32(0.0)
+        for dy in range(n):
+            for dx in range(n):
+                x, y = bx + dx, by + dy
+                s = np.float32(s + (adj[y, x] if x < 13 and y < 9 else np.float32(0.0)))
+        a = np.float32(s / np.float32(n * n))
+        snapped = math.copysign(math.floor(abs(float(a * np.float32(2.0))) + 0.5), float(a)) / 2.0
+        assert orc.aq_footprint_mul(adj, bx, by, n, n, 2, False, False) == orc.aq_mul_of(2, snapped)
+        assert orc.aq_footprint_mul(adj, bx, by, n, n, 2, True, False) == orc.aq_mul_of(2, min(snapped, 0.0))
+
+
+def test_aq_setup_factorization():
+    # lib.rs:1859-1908
+    assert orc.aq_setup(32768, 16, 1, False) == (16384, 32, 2, False)
+    assert orc.aq_setup(32769, 16, 1, False) == (16384, 32, 2, False)  # snapped down
+    assert orc.aq_setup(1, 16, 1, False) == (1, 16, 1, True)  # snapped < factor
+    assert orc.aq_setup(4096, 40000, 1, False) == (4096, 40000, 1, True)  # quant_lf x2 not representable
+    assert orc.aq_setup(32768, 4, 3, True) == (2048, 64, 48, False)
+    assert orc.aq_setup(17, 4, 3, True) == (1, 64, 48, False)
+    assert orc.aq_setup(15, 4, 3, True) == (15, 4, 3, True)
+
+
+def test_constant_aq_field_equals_a_global_multiplier():
+    # With every varblock at one multiplier m (a constant field) and the
+    # nearest rule (no Lagrange weights), quantize and render equal the
+    # plain path at hf_mul = m: the per-varblock multiplier is the only
+    # thing AQ changes.
+    w, h = 72, 56
+    rng = np.random.default_rng(9)
+    rgb = (rng.random((h, w, 3)) * 255).astype(np.uint8)
+    xyb = _xyb_of_rgb8(rgb)
+    bmap = orc.mixed_map(w, h, 4)
+    coeff = orc.forward_frame(xyb, bmap)
+    bh, bw = bmap.shape
+    for c, m in [(-1.0, 4), (-0.5, 3), (1.0, 1)]:
+        adj = np.full((bh, bw), c, np.float32)
+        q_aq = orc.with_aq(orc.QuantParams(4096, 2, 32, 2, 2, 0, 1.0, 0.0), adj)
+        q_pl = orc.QuantParams(4096, m, 32, 2, 2, 0, 1.0, 0.0)
+        assert np.all(orc.aq_block_muls(w, h, bmap, q_aq)[bmap > 0] == m)
+        qa, la = orc.quantize_frame(w, h, bmap, coeff, q_aq)
+        qp, lp = orc.quantize_frame(w, h, bmap, coeff, q_pl)
+        assert np.array_equal(qa, qp) and np.array_equal(la, lp)
+        r_aq = orc.with_aq(orc.RenderParams(4096, 2, 32, 2, 2, 1, 0, 7, 8), adj)
+        r_pl = orc.RenderParams(4096, m, 32, 2, 2, 1, 0, 7, 8)
+        la_, xa = orc.render(w, h, bmap, qa, la, r_aq)
+        lp_, xp = orc.render(w, h, bmap, qp, lp, r_pl)
+        assert np.array_equal(la_, lp_) and np.array_equal(xa, xp)
+
+
+def test_chroma_texture_alone_drives_the_masking_field():
+    # field.rs:275-283: activity = luma + 0.35 * chroma.  With flat Y and
+    # textured X/B only the chroma term sees texture, so a dropped chroma
+    # term (a plausible slip) would leave this field neutral.
+    rng = np.random.default_rng(2)
+    xyb = np.zeros((3, 64, 64), np.float32)
+    xyb[1] = 0.5
+    xyb[0, :, 32:] = rng.random((64, 32), dtype=np.float32) * 0.02
+    xyb[2, :, 32:] = 0.4 + rng.random((64, 32), dtype=np.float32) * 0.1
+    xyb[2, :, :32] = 0.45
+    adj, active, _ = orc.aq_field(xyb, orc.AQ_MASKING)
+    assert active
+    assert np.all(adj[:, 5:] > 0) and np.all(adj[:, :3] < 0)
