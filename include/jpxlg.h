@@ -122,6 +122,33 @@ int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t c
  * Gaborish on).  Drops any structure: plan or set it again afterwards. */
 int32_t jpxlg_precondition_gaborish(jpxlg_img* img);
 
+/* ---- adaptive quantization ------------------------------------------------ */
+/* The AQ field of the image (AqMode, jpxl-encode-policy/src/field.rs:52-91;
+ * DesiredQuantField::from_atlas_tuned at the default tuning, field.rs:231,
+ * over AnalysisAtlas::analyze of the source planes, analysis.rs:111).  With
+ * an active field (a mode other than OFF whose field is not neutral) every
+ * later cover, CfL estimate, quantize and render under a quantizer q uses
+ * AqSetup::build's wire quantizer (lib.rs:1859-1908: global_scale / 2,
+ * quant_lf * 2, baseline HfMul hf_mul * 2; factor 16 on the fine lattice)
+ * and each varblock's HfMul from the field over its footprint
+ * (mul_for_footprint, field.rs:338).  OFF is the quality path's setting
+ * (request.rs:903), MASKING the fixed-quantizer default (request.rs:1097).
+ * Call before jpxlg_precondition_gaborish (the field reads the source
+ * planes); drops any structure.  *active (optional) = the field is active. */
+#define JPXLG_AQ_OFF 0
+#define JPXLG_AQ_MASKING 1
+#define JPXLG_AQ_UNIFORM 2
+#define JPXLG_AQ_FINE_MASKING 3
+#define JPXLG_AQ_FINE_UNIFORM 4
+#define JPXLG_AQ_EDGE_REFINE 5
+int32_t jpxlg_set_aq(jpxlg_img* img, int32_t mode, int32_t* active);
+/* The AQ side of the last quantized plan, for the entropy coder: hf_mul
+ * (blocks_x * blocks_y, optional) holds each varblock's HfMul at its
+ * top-left block and 0 on covered blocks (VarblockDecision::hf_mul,
+ * lib.rs:736-750); *wire the quantizer the frame header signals;
+ * *refine_only AqSetup's fallback flag. */
+int32_t jpxlg_plan_aq(jpxlg_img* img, uint32_t* hf_mul, jpxlg_quantizer* wire, int32_t* refine_only);
+
 /* ---- probe ---------------------------------------------------------------- */
 int32_t jpxlg_quantize(jpxlg_img* img, const jpxlg_quantizer* q);
 
@@ -155,6 +182,7 @@ int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* lf, uint8_t
 #define JPXLG_BUF_LINEAR 3       /* 3 x W x H f32: reconstructed linear RGB at depth */
 #define JPXLG_BUF_SIGMA 4        /* bw x bh f32: EPF sigma */
 #define JPXLG_BUF_REF_XYB0 5     /* 3 x W x H f32: metric reference positive XYB, scale 0 */
+#define JPXLG_BUF_AQ_FIELD 6     /* bw x bh f32: the active AQ field's per-atom adjustment */
 int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* host, size_t bytes);
 
 /* ---- quality controller ---------------------------------------------------- */

@@ -245,6 +245,23 @@ class Image:
     def quantize(self, q: Quantizer):
         self.ctx.check(self.ctx.lib.jpxlg_quantize(self.handle, ctypes.byref(q)))
 
+    def set_aq(self, mode: int) -> bool:
+        """Build the AQ field of `mode` (_lib.AQ_*) from the source planes;
+        returns whether it is active (not off, not neutral).  Drops the
+        structure; call before precondition_gaborish."""
+        active = ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.jpxlg_set_aq(self.handle, int(mode), ctypes.byref(active)))
+        return bool(active.value)
+
+    def plan_aq(self):
+        """-> (per-block HfMul (bh, bw) uint32, wire Quantizer, refine_only) of
+        the last quantized plan."""
+        muls = np.zeros((self.bh, self.bw), np.uint32)
+        wire = Quantizer()
+        ro = ctypes.c_int32()
+        self.ctx.check(self.ctx.lib.jpxlg_plan_aq(self.handle, _ptr(muls), ctypes.byref(wire), ctypes.byref(ro)))
+        return muls, wire, bool(ro.value)
+
     def precondition_gaborish(self):
         """Replace the source XYB by its Gaborish inverse (drops the structure)."""
         self.ctx.check(self.ctx.lib.jpxlg_precondition_gaborish(self.handle))
@@ -287,7 +304,7 @@ class Image:
             out = np.empty((3, self.height, self.width), np.float32)
         elif which == _lib.BUF_COEFF:
             out = np.empty((3, 8 * self.bh, 8 * self.bw), np.float32)
-        elif which == _lib.BUF_SIGMA:
+        elif which in (_lib.BUF_SIGMA, _lib.BUF_AQ_FIELD):
             out = np.empty((self.bh, self.bw), np.float32)
         else:
             raise ValueError(which)
@@ -448,10 +465,16 @@ class Encoder:
                 report = EncodeReport(self.quality, res.score, bool(res.met), res.global_scale, res.hf_mul,
                                       res.quant_lf, [(res.probe_rungs[i], res.probe_scores[i]) for i in range(res.probes)])
             else:
+                # EncodeRequest::for_fixed_quantizer over the defaults
+                # (request.rs:855-880, :936-943): quant_lf 16, hf_mul 1, AQ
+                # masking, hierarchical cover, CfL on, the nearest rule.
                 q = Quantizer(self.global_scale or 32768, 1, 16, 2, 2, 0, 1.0, 1.0)
-                img.plan_structure(q, _lib.COVER_FIXED_DCT8, False)
+                img.set_aq(_lib.AQ_MASKING)
+                img.plan_structure(q, _lib.COVER_HIERARCHICAL, True)
                 img.quantize(q)
-                report = EncodeReport(float("nan"), float("nan"), True, q.global_scale, 1, 16)
+                _, wire, _ = img.plan_aq()
+                report = EncodeReport(float("nan"), float("nan"), True, wire.global_scale, wire.hf_mul,
+                                      wire.quant_lf)
             return img.plan_download(), report
         finally:
             img.close()

@@ -29,9 +29,11 @@ DCT_EXACT = 0
 DCT_FP32 = 1
 DCT_BF16X3 = 2
 BUF_REF_XYB0 = 5
+BUF_AQ_FIELD = 6
 
 COVER_FIXED_DCT8 = 0
 COVER_HIERARCHICAL = 1
+AQ_OFF, AQ_MASKING, AQ_UNIFORM, AQ_FINE_MASKING, AQ_FINE_UNIFORM, AQ_EDGE_REFINE = range(6)
 
 EFFORT_FAST = 0
 EFFORT_BALANCED = 1
@@ -131,6 +133,8 @@ SIGNATURES = {
     "jpxlg_dct_blocks": (_i32, [_vp, _i32, _i32, _u32, _vp, _vp]),
     "jpxlg_epf_step": (_i32, [_vp, _i32, _u32, _u32, _vp, _vp, _vp]),
     "jpxlg_gaborish": (_i32, [_vp, _u32, _u32, _vp, _vp]),
+    "jpxlg_set_aq": (_i32, [_vp, _i32, ctypes.POINTER(ctypes.c_int32)]),
+    "jpxlg_plan_aq": (_i32, [_vp, _vp, ctypes.POINTER(Quantizer), ctypes.POINTER(ctypes.c_int32)]),
     "jpxlg_profile_enable": (_i32, [_vp, _i32]),
     "jpxlg_profile_only": (_i32, [_vp, ctypes.c_char_p]),
     "jpxlg_profile_read": (_i32, [_vp, _vp, _dp, ctypes.POINTER(ctypes.c_uint64), _i32]),

@@ -218,6 +218,7 @@ size_t image_bytes(int w, int h) {
   for (int s = 0; s < n; s++) tiles += (size_t)std::max(pool_tiles(sw[s], sh[s]), pool_units(sw[s], sh[s]));
   b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
   b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
+  b += bw * bh * 4 + 256;             // AQ field + scratch
   b += 256 * 256;  // alignment of every carved buffer
   return b;
 }
@@ -263,6 +264,13 @@ struct jpxlg_img {
   uint16_t* d_hf_cell = nullptr;
   uint4* d_cfl_items[3] = {nullptr, nullptr, nullptr};
   uint32_t* d_tile_base = nullptr;
+  // Adaptive quantization: the per-atom field (bw x bh), its scratch, and
+  // whether it is active (a mode other than off whose field is not neutral).
+  float* aq_adj = nullptr;
+  AqScratch* d_aq = nullptr;
+  int aq_mode = kAqOff;
+  bool aq_active = false;
+  bool preconditioned = false;
   // host mirrors of the structure
   std::vector<uint8_t> bmap;
   std::vector<int8_t> cfl_x, cfl_b;
@@ -449,6 +457,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_pp = (ProbeParams*)A.take(sizeof(ProbeParams));
   img->d_err = (uint32_t*)A.take(4);
   img->d_gray = (int*)A.take(4);
+  img->aq_adj = (float*)A.take(bpx * 4);
+  img->d_aq = (AqScratch*)A.take(sizeof(AqScratch));
   size_t ptiles = 0;
   for (int s = 0; s < img->nscales; s++)
     ptiles += (size_t)std::max(pool_tiles(img->sw[s], img->sh[s]), pool_units(img->sw[s], img->sh[s]));
@@ -456,7 +466,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
                 img->d_cfl_items[2] && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
-                img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums;
+                img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
+                img->aq_adj && img->d_aq;
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
              img->d_list[c] && img->rx[c] && img->aux[c];
@@ -637,6 +648,98 @@ extern "C" int32_t jpxlg_precondition_gaborish(jpxlg_img* img) {
   img->has_structure = false;
   img->has_quant = false;
   img->has_render = false;
+  img->preconditioned = true;
+  return JPXLG_OK;
+}
+
+// AqSetup::build's factorization (jpxl-encode-policy/src/lib.rs:1859-1908):
+// global_scale / factor (snapped down to a multiple), quant_lf and the
+// baseline HfMul times the factor (2, or 16 on the fine lattice); the
+// request's values and refine-only when a factored value is not on the wire.
+// Returns refine_only.
+static int aq_wire(const jpxlg_quantizer& q, bool fine, jpxlg_quantizer* w) {
+  const uint32_t factor = fine ? 16u : 2u;
+  const uint32_t snapped = q.global_scale - q.global_scale % factor;
+  const uint64_t g2 = std::max<uint64_t>(snapped / factor, 1);
+  const uint64_t q2 = std::min<uint64_t>((uint64_t)q.quant_lf * factor, 0xffffffffu);
+  const uint64_t m2 = std::min<uint64_t>((uint64_t)q.hf_mul * factor, 0xffffffffu);
+  *w = q;
+  const bool ok = snapped >= factor && g2 <= 8193u + 65535u && q2 >= 1 && q2 <= 65536u && m2 >= 1 &&
+                  m2 <= 0x7fffffffu;
+  if (!ok) return 1;
+  w->global_scale = (uint32_t)g2;
+  w->quant_lf = (uint32_t)q2;
+  w->hf_mul = (uint32_t)m2;
+  return 0;
+}
+
+// The probe parameters of quantizer q on this image: the wire quantizer and
+// the per-varblock HfMul field when the image's AQ field is active.
+static void probe_params_for(const jpxlg_img* img, const jpxlg_quantizer& q, int sharpness, ProbeParams* p) {
+  const bool fine = img->aq_mode >= kAqFineMasking;
+  jpxlg_quantizer w = q;
+  const int refine_only = img->aq_active ? aq_wire(q, fine, &w) : 0;
+  build_probe_params(w, sharpness, p);
+  if (img->aq_active) {
+    p->aq_adj = img->aq_adj;
+    p->aq_refine_only = refine_only;
+    p->aq_fine = fine ? 1 : 0;
+  }
+}
+
+extern "C" int32_t jpxlg_set_aq(jpxlg_img* img, int32_t mode, int32_t* active) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (mode < kAqOff || mode > kAqEdgeRefine) return fail(ctx, JPXLG_E_UNSUPPORTED, "unknown AQ mode");
+  if (img->preconditioned)
+    return fail(ctx, JPXLG_E_INVALID, "the AQ field reads the source planes: set it before the Gaborish precondition");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  img->aq_mode = mode;
+  img->aq_active = false;
+  if (mode != kAqOff) {
+    hipStream_t st = ctx->stream;
+    HIP_TRY(ctx, hipMemsetAsync(img->d_aq, 0, sizeof(AqScratch), st));
+    {
+      Prof p(ctx, "aq_field");
+      launch_aq_field(st, img->g, mode, img->xyb, (float*)img->d_cost[0], (float*)img->d_cost[1], img->d_aq,
+                      img->aq_adj);
+    }
+    HIP_TRY(ctx, hipGetLastError());
+    AqScratch sc;
+    HIP_TRY(ctx, hipMemcpyAsync(&sc, img->d_aq, sizeof(sc), hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    img->aq_active = sc.active != 0;
+  }
+  if (active) *active = img->aq_active ? 1 : 0;
+  // the cover and the CfL estimate depend on the field
+  img->has_structure = false;
+  img->has_quant = false;
+  img->has_render = false;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_plan_aq(jpxlg_img* img, uint32_t* hf_mul, jpxlg_quantizer* wire, int32_t* refine_only) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  const bool fine = img->aq_mode >= kAqFineMasking;
+  jpxlg_quantizer w = img->last_q;
+  const int ro = img->aq_active ? aq_wire(img->last_q, fine, &w) : 0;
+  if (wire) *wire = w;
+  if (refine_only) *refine_only = ro;
+  if (!hf_mul) return JPXLG_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  const size_t bpx = (size_t)img->g.bw * img->g.bh;
+  ProbeParams pp;
+  probe_params_for(img, img->last_q, img->last_sharpness, &pp);
+  uint32_t* d_muls = (uint32_t*)img->d_cost[2];
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemsetAsync(d_muls, 0, bpx * 4, st));
+  launch_aq_block_muls(st, img->g, img->d_pp, img->d_list, img->list_count, d_muls);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipMemcpyAsync(hf_mul, d_muls, bpx * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
   return JPXLG_OK;
 }
 
@@ -704,7 +807,7 @@ static int estimate_cfl(jpxlg_img* img, const jpxlg_quantizer* q) {
         }
     }
   ProbeParams pp;
-  build_probe_params(*q, img->last_sharpness, &pp);
+  probe_params_for(img, *q, img->last_sharpness, &pp);
   HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
   const uint32_t zero = 0;
   HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
@@ -769,7 +872,7 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
   if (cover_mode == JPXLG_COVER_HIERARCHICAL) {
     hipStream_t st = ctx->stream;
     ProbeParams pp;
-    build_probe_params(*q, img->last_sharpness, &pp);
+    probe_params_for(img, *q, img->last_sharpness, &pp);
     HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
     const uint32_t zero = 0;
     HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
@@ -898,7 +1001,7 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
     const Geom& g = img->g;
     const jpxlg_quantizer& q = qs ? qs[k] : img->last_q;
     const int sharp = r ? r->sharpness : img->last_sharpness;
-    build_probe_params(q, sharp, &hp[k]);
+    probe_params_for(img, q, sharp, &hp[k]);
     ProbeDesc& P = hd[k];
     P.g = g;
     P.nscales = img->nscales;
@@ -1142,6 +1245,9 @@ extern "C" int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* hos
     case JPXLG_BUF_SIGMA:
       if (!img->has_render) return fail(ctx, JPXLG_E_INVALID, "nothing rendered yet");
       single = img->sigma; per = bpx; count = 1; break;
+    case JPXLG_BUF_AQ_FIELD:
+      if (!img->aq_active) return fail(ctx, JPXLG_E_INVALID, "no active AQ field");
+      single = img->aq_adj; per = bpx; count = 1; break;
     case JPXLG_BUF_LINEAR: {
       if (!img->has_render) return fail(ctx, JPXLG_E_INVALID, "nothing rendered yet");
       // Linear RGB at depth is recomputed from the restored planes.

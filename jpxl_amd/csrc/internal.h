@@ -47,6 +47,83 @@ struct ProbeParams {
   float sigma;          // J.4.3 sigma for the uniform sharpness
   float zero_token_bits;
   int32_t truncate;     // trailing-truncation pass on/off
+  // Adaptive quantization: with an AQ field (aq_adj non-null, one f32
+  // adjustment per 8x8 atom) every varblock carries its own HfMul from the
+  // field over its footprint, around the wire baseline; sc/sigma above are
+  // the baseline's.  Null: every varblock at the baseline.
+  const float* aq_adj;
+  uint32_t gs;           // the wire global_scale
+  uint32_t aq_baseline;  // the wire baseline HfMul
+  int32_t aq_refine_only, aq_fine;
+  float qm[3];           // per-channel qm multipliers (Y: 1)
+  float sig_lut;         // the EPF sharpness LUT entry of sigma
+};
+
+// field.rs:383-395 mul_of: baseline * 2^-adj rounded to the nearest integer,
+// clamped to [1, 2 * baseline] (an unrepresentable result keeps the baseline).
+__host__ __device__ inline uint32_t aq_mul_of(uint32_t base, float adj) {
+  const double v = round((double)base * exp2((double)-adj));
+  uint32_t mul = !(v > 0.0) ? 0u : (v >= 4294967295.0 ? 0xffffffffu : (uint32_t)v);
+  const uint32_t hi = base > 0x7fffffffu ? 0xffffffffu : base * 2u;
+  mul = mul < 1u ? 1u : (mul > hi ? hi : mul);
+  return mul <= 0x7fffffffu ? mul : base;
+}
+
+// field.rs:338-378 mul_for_footprint over the n x n atoms at (bx, by): the
+// f32 mean of the adjustments (rows, then columns; zero outside the grid),
+// snapped to half octaves (legacy) or clamped to one octave (fine), capped
+// at zero under refine-only.
+__host__ __device__ inline uint32_t aq_footprint_mul(const ProbeParams* pp, int bw, int bh, int bx, int by, int n) {
+  const float* adj = pp->aq_adj;
+  float sum = 0.0f;
+  for (int dy = 0; dy < n; dy++)
+    for (int dx = 0; dx < n; dx++) {
+      const int x = bx + dx, y = by + dy;
+      sum += (x < bw && y < bh) ? adj[(size_t)y * bw + x] : 0.0f;
+    }
+  float a = sum / (float)(n * n);
+  if (pp->aq_fine) a = a < -1.0f ? -1.0f : (a > 1.0f ? 1.0f : a);
+  else a = roundf(a * 2.0f) / 2.0f;
+  if (pp->aq_refine_only) a = fminf(a, 0.0f);
+  return aq_mul_of(pp->aq_baseline, a);
+}
+
+// The I.5.3 step scales (Mul * qm[c]) and the J.4.3 sigma of the varblock of
+// side 8n at block (bx, by) (reconstruct.rs:27-36, restoration.rs:338-349).
+__device__ inline void varblock_scales(const ProbeParams* pp, int bw, int bh, int bx, int by, int n, float* sc,
+                                       float* sigma) {
+  if (!pp->aq_adj) {
+    sc[0] = pp->sc[0];
+    sc[1] = pp->sc[1];
+    sc[2] = pp->sc[2];
+    if (sigma) *sigma = pp->sigma;
+    return;
+  }
+  const uint32_t m = aq_footprint_mul(pp, bw, bh, bx, by, n);
+  const double den = (double)pp->gs * (double)m;
+  const float mul = den <= 0.0 ? 0.0f : (float)(65536.0 / den);
+  sc[0] = mul * pp->qm[0];
+  sc[1] = mul;
+  sc[2] = mul * pp->qm[2];
+  if (sigma) *sigma = mul * 0.46f * pp->sig_lut;
+}
+
+// lib.rs:4569-4577 / :1933-1938: the cover's metadata bits for a varblock's
+// HfMul (8 for a non-baseline multiplier on the legacy lattice, a flat 2 on
+// the fine one, none without a field).
+__device__ inline double aq_signal_bits(const ProbeParams* pp, int bw, int bh, int bx, int by, int n) {
+  if (!pp->aq_adj) return 0.0;
+  if (pp->aq_fine) return 2.0;
+  return aq_footprint_mul(pp, bw, bh, bx, by, n) == pp->aq_baseline ? 0.0 : 8.0;
+}
+
+// Adaptive-quantization modes (field.rs:52-91 AqMode) and the device
+// scratch of the field's construction.
+enum AqMode { kAqOff = 0, kAqMasking = 1, kAqUniform = 2, kAqFineMasking = 3, kAqFineUniform = 4, kAqEdgeRefine = 5 };
+struct AqScratch {
+  float mean;       // the frame mean of the (eroded) activities
+  int32_t certified;
+  int32_t active;   // some atom leaves the baseline (the field is not neutral)
 };
 
 // Chroma-from-luma structure of a plan (device memory).
@@ -188,6 +265,13 @@ void launch_cfl_tiles(hipStream_t s, const Geom& g, const ConstTables* tab, cons
 // grid-wide pricing of the candidates, then the selection.
 void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, const float4* lf_s,
                    CflLfScratch* scratch, CflParams* cflp);
+// The AQ field of `mode` from the source XYB planes into adj (bw x bh);
+// act/ero are bw x bh scratch planes; scratch->active must be zero before.
+void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3], float* act, float* ero,
+                     AqScratch* scratch, float* adj);
+// Each listed varblock's HfMul at its top-left block.
+void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
+                          const int counts[3], uint32_t* muls);
 // Pooling tiles of one scale (one partial row of 18 sums per tile).
 int pool_tiles(int w, int h);
 // Tiles of the streaming candidate pass (64-column strips x row segments);

@@ -493,6 +493,7 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
   __shared__ float dylf[VB][16];
   __shared__ float lfs[3][VB][16], lfo[3][VB][16];
   __shared__ float khf[2][VB];
+  __shared__ float vsc[VB][3];  // each varblock's step scales (its own HfMul under AQ)
   __shared__ int vbx[VB], vby[VB];
   const int base = (blockIdx.x - off[d]) * VB;
   const int nvb = min(VB, count - base);
@@ -505,6 +506,7 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
     int tile = (by / 8) * g.tw + bx / 8;
     khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
     khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
+    varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], nullptr);
   }
   __syncthreads();
   const float* cin[3] = {CX, CY, CB};
@@ -559,10 +561,10 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
     }
   }
   {
-    const float sc = pp->sc[1], bias = kQuantBias[1];
+    const float bias = kQuantBias[1];
     for (int e = threadIdx.x; e < VB * NN; e += T) {
       int j = e / NN, k = e % NN;
-      float step = sc * tab->matrix[1][cb + k];
+      float step = vsc[j][1] * tab->matrix[1][cb + k];
       int32_t v = 0;
       if ((k % N) >= n || (k / N) >= n) v = hf_choose(co[1][j][k], step, bias, &bad);
       qs[1][j][k] = v;
@@ -576,11 +578,11 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
     if constexpr (N == 32) {
       // one varblock: all T/64 waves walk its luma block together
       truncate_group<N>(qs[1][0], co[1][0], dyh[0], tab->order + cb, tab->matrix[1] + cb, tab->donor + cb,
-                        pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits, T / 64, wave, xs[0]);
+                        vsc[0][1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits, T / 64, wave, xs[0]);
     } else {
       for (int j = wave; j < nvb; j += T / 64)
         truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
-                         tab->donor + cb, pp->sc[1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
+                         tab->donor + cb, vsc[j][1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
     }
   }
   // --- X and B LF against the reconstructed Y LF ---
@@ -603,7 +605,7 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
     if ((k % N) >= n || (k / N) >= n) {
       float target = co[c][j][k] - khf[ci][j] * dyh[j][k];
       co[c][j][k] = target;
-      float step = pp->sc[c] * tab->matrix[c][cb + k];
+      float step = vsc[j][c] * tab->matrix[c][cb + k];
       v = hf_choose(target, step, kQuantBias[c], &bad);
     }
     qs[c][j][k] = v;
@@ -614,13 +616,13 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
       // X on the first half of the waves, B on the second half
       const int half = (T / 64) / 2, ci = wave / half, c = ci == 0 ? 0 : 2;
       truncate_group<N>(qs[c][0], co[c][0], nullptr, tab->order + cb, tab->matrix[c] + cb, tab->donor + cb,
-                        pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits, half, wave % half, xs[ci]);
+                        vsc[0][c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits, half, wave % half, xs[ci]);
     } else {
       for (int w = wave; w < 2 * VB; w += T / 64) {
         const int ci = w / VB, j = w % VB, c = ci == 0 ? 0 : 2;
         if (j < nvb)
           truncate_wave<N>(qs[c][j], co[c][j], nullptr, tab->order + cb, tab->matrix[c] + cb,
-                           tab->donor + cb, pp->sc[c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
+                           tab->donor + cb, vsc[j][c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits);
       }
     }
   }
@@ -665,6 +667,7 @@ __global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict
   __shared__ float llf[VB][3][16];
   __shared__ int vbx[VB], vby[VB];
   __shared__ float khf[2][VB];
+  __shared__ float vsc[VB][4];  // step scales and sigma (the varblock's own HfMul under AQ)
   const int base = (blockIdx.x - off[d]) * VB;
   const int nvb = min(VB, count - base);
   if (threadIdx.x < VB) {
@@ -675,18 +678,18 @@ __global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict
     int tile = (by / 8) * g.tw + bx / 8;
     khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
     khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
+    varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], &vsc[threadIdx.x][3]);
   }
   __syncthreads();
   // I.5.3 dequantization of all three channels, then I.6 HF CfL.
-  const float s0 = pp->sc[0], s1 = pp->sc[1], s2 = pp->sc[2];
   for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
     int j = e / NN, k = e % NN;
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
     if (j < nvb) {
       size_t o = (size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N;
-      d0 = bias_adjust(QX[o], kQuantBias[0]) * s0 * tab->matrix[0][cb + k];
-      d1 = bias_adjust(QY[o], kQuantBias[1]) * s1 * tab->matrix[1][cb + k];
-      d2 = bias_adjust(QB[o], kQuantBias[2]) * s2 * tab->matrix[2][cb + k];
+      d0 = bias_adjust(QX[o], kQuantBias[0]) * vsc[j][0] * tab->matrix[0][cb + k];
+      d1 = bias_adjust(QY[o], kQuantBias[1]) * vsc[j][1] * tab->matrix[1][cb + k];
+      d2 = bias_adjust(QB[o], kQuantBias[2]) * vsc[j][2] * tab->matrix[2][cb + k];
     }
     int r = k / N, col = k % N;
     lds[((j * 3 + 0) * N + r) * P + col] = d0 + khf[0][j] * d1;
@@ -781,7 +784,7 @@ __global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict
     const int jj = threadIdx.x / (n * n), b = threadIdx.x % (n * n);
     if (jj < nvb) {
       int sx = vbx[jj] + b % n, sy = vby[jj] + b / n;
-      if (sx < g.bw && sy < g.bh) sigma[(size_t)sy * g.bw + sx] = pp->sigma;
+      if (sx < g.bw && sy < g.bh) sigma[(size_t)sy * g.bw + sx] = vsc[jj][3];
     }
   }
 }

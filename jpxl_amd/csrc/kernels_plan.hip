@@ -82,6 +82,7 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
   __shared__ uint32_t bits[VB];
   __shared__ double sse[VB];
   __shared__ int vbx[VB], vby[VB], valid[VB];
+  __shared__ float vsc[VB][3];  // the candidate's step scales at its footprint's HfMul
   const int ncx = g.bw / n, ncy = g.bh / n, count = ncx * ncy;
   const int base = blockIdx.x * VB;
   if (threadIdx.x < VB) {
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
     vby[threadIdx.x] = i < count ? (i / ncx) * n : 0;
     bits[threadIdx.x] = 0;
     sse[threadIdx.x] = 0.0;
+    varblock_scales(pp, g.bw, g.bh, vbx[threadIdx.x], vby[threadIdx.x], n, vsc[threadIdx.x], nullptr);
   }
   __syncthreads();
   const float* planes[3] = {X, Y, B};
@@ -122,7 +124,7 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
   const double tsd = (double)(N * N) * 1.0;
   for (int ci = 0; ci < 3; ci++) {
     const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);  // Y, then X, then B
-    const float sc = pp->sc[c], bias = kBias[c];
+    const float bias = kBias[c];
     const double lw = pp->lambda[c] * tsd;
     for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
       int j = e / NN, k = e % NN;
@@ -135,7 +137,7 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
       if (c == 1) target = coef;
       else if (c == 0) target = coef - 0.0f * dyh[j][k];
       else target = coef - 1.0f * dyh[j][k];
-      float step = sc * tab->matrix[c][cb + k];
+      float step = vsc[j][c] * tab->matrix[c][cb + k];
       int32_t q = choose_nearest(target, step, bias, &bad);
       float recon = bias_adjust(q, bias) * step;
       if (c == 1) dyh[j][k] = recon;
@@ -158,7 +160,8 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
 }
 
 // The quadtree of tile_region_with over one 32x32 region (4x4 blocks).
-__global__ __launch_bounds__(256) void k_cover_decide(Geom g, const double* __restrict__ c8,
+__global__ __launch_bounds__(256) void k_cover_decide(Geom g, const ProbeParams* __restrict__ pp,
+                                                      const double* __restrict__ c8,
                                                       const double* __restrict__ c16,
                                                       const double* __restrict__ c32,
                                                       uint8_t* __restrict__ bmap) {
@@ -183,11 +186,11 @@ __global__ __launch_bounds__(256) void k_cover_decide(Geom g, const double* __re
     for (int si = 0; si < 4; si++) {
       const int sx = qx + so[si][0], sy = qy + so[si][1];
       if (sx >= bw || sy >= bh) continue;
-      split2 += (c8[(size_t)sy * bw + sx] + fixed_bits(8)) + 0.0;
+      split2 += (c8[(size_t)sy * bw + sx] + fixed_bits(8)) + aq_signal_bits(pp, bw, bh, sx, sy, 1);
     }
     double best2 = split2;
     if (qx + 2 <= bw && qy + 2 <= bh) {
-      const double fixed = fixed_bits(16) + 0.0;
+      const double fixed = fixed_bits(16) + aq_signal_bits(pp, bw, bh, qx, qy, 2);
       if (!(fixed >= split2)) {
         double single = c16[(size_t)qy * bw + qx] + fixed;
         if (single < split2) {
@@ -200,7 +203,7 @@ __global__ __launch_bounds__(256) void k_cover_decide(Geom g, const double* __re
   }
   bool use32 = false;
   if (bx0 + 4 <= bw && by0 + 4 <= bh) {
-    const double fixed = fixed_bits(32) + 0.0;
+    const double fixed = fixed_bits(32) + aq_signal_bits(pp, bw, bh, bx0, by0, 4);
     if (!(fixed >= split4)) {
       double single = c32[(size_t)by0 * bw + bx0] + fixed;
       if (single < split4) use32 = true;
@@ -348,7 +351,9 @@ __global__ __launch_bounds__(64) void k_cfl_collect(Geom g, const uint4* __restr
       lf_s[lf_off + k] = make_float4(lf[0][k], y, lf[2][k], dy);
     }
   }
-  const float sc = pp->sc[1], bias = kBias[1];
+  float vsc[3];
+  varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc, nullptr);  // lib.rs:3582 the varblock's (transform, HfMul)
+  const float sc = vsc[1], bias = kBias[1];
   for (int k = threadIdx.x; k < NN; k += 64) {
     const int row = k / N, col = k % N;
     if (row < n && col < n) continue;
@@ -683,6 +688,123 @@ __global__ __launch_bounds__(kCflThreads) void k_cfl_tiles(const uint32_t* __res
   if ((int)blockIdx.x < tiles) cfl_tile((int)blockIdx.x, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage);
 }
 
+// ----------------------------------------------------------------------------
+// Adaptive quantization field (jpxl-encode-policy/src/analysis.rs:111-158,
+// field.rs:200-320).  One thread per 8x8 atom for the atlas and the field;
+// the frame mean of the activities is the reference's sequential f64 sum,
+// reproduced by a certified double-double sum (the narrowed f32 mean is the
+// same for every sum within the recursive-summation bound of the exact
+// one) with the sequential chain as the fallback.
+// ----------------------------------------------------------------------------
+
+// The activity's f32 log2: log2 in f64, one rounding to f32.
+__device__ __forceinline__ float aq_log2f(float x) { return (float)log2((double)x); }
+
+__global__ __launch_bounds__(256) void k_aq_activity(Geom g, const float* __restrict__ X,
+                                                     const float* __restrict__ Y, const float* __restrict__ B,
+                                                     float* __restrict__ act) {
+  const int atom = blockIdx.x * blockDim.x + threadIdx.x;
+  if (atom >= g.bw * g.bh) return;
+  const int ax = atom % g.bw, ay = atom / g.bw;
+  const int x0 = ax * 8, y0 = ay * 8, x1 = min(x0 + 8, g.w), y1 = min(y0 + 8, g.h);
+  const double count = (double)(x1 - x0) * (double)(y1 - y0);
+  const float* planes[3] = {X, Y, B};
+  float var[3];
+  for (int c = 0; c < 3; c++) {
+    double sum = 0.0, sum_sq = 0.0;
+    for (int y = y0; y < y1; y++)
+      for (int x = x0; x < x1; x++) {
+        const double v = (double)planes[c][(size_t)y * g.w + x];
+        sum += v;
+        sum_sq += v * v;
+      }
+    const double mean = sum / count;
+    double variance = sum_sq / count - mean * mean;
+    if (!(variance > 0.0)) variance = 0.0;
+    var[c] = (float)variance;
+  }
+  const float k8 = 255.0f * 255.0f;
+  const float luma = aq_log2f(1.0f + var[1] * k8);
+  const float chroma = aq_log2f(1.0f + (var[0] + var[2]) * k8);
+  act[atom] = luma + 0.35f * chroma;
+}
+
+// field.rs:200-219: the minimum over the clamped 3x3 neighbourhood.
+__global__ __launch_bounds__(256) void k_aq_erode(int bw, int bh, const float* __restrict__ a,
+                                                  float* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= bw * bh) return;
+  const int x = i % bw, y = i / bw;
+  float m = INFINITY;
+  for (int ny = max(y - 1, 0); ny < min(y + 2, bh); ny++)
+    for (int nx = max(x - 1, 0); nx < min(x + 2, bw); nx++) m = fminf(m, a[ny * bw + nx]);
+  out[i] = isfinite(m) ? m : 0.0f;
+}
+
+// field.rs:286-295: mean = (sum_i a_i in f64, raster order) / count, as f32.
+// The activities are non-negative, so the sequential sum lies within
+// gamma_(n-1) * S of the exact sum S.
+__global__ __launch_bounds__(1024) void k_aq_mean(const float* __restrict__ a, int n, AqScratch* __restrict__ sc) {
+  __shared__ DD red[1024];
+  DD acc{0.0, 0.0};
+  for (int i = threadIdx.x; i < n; i += 1024) acc = dd_add(acc, dd_of((double)a[i]));
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int stride = 512; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride) red[threadIdx.x] = dd_add(red[threadIdx.x], red[threadIdx.x + stride]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  const double S = red[0].hi + red[0].lo;
+  const double u = 1.1102230246251565e-16, k = (double)n + 4.0;
+  const double E = 2.0 * (k * u / (1.0 - k * u)) * S + 1e-300;
+  const double lo = S - E, hi = S + E;
+  const float m_lo = (float)(lo / (double)n), m_hi = (float)(hi / (double)n);
+  if (m_lo == m_hi) {
+    sc->mean = m_lo;
+    sc->certified = 1;
+    return;
+  }
+  double sum = 0.0;
+  for (int i = 0; i < n; i++) sum += (double)a[i];
+  sc->mean = (float)(sum / (double)n);
+  sc->certified = 0;
+}
+
+// field.rs:262-300 the per-atom adjustment, and field.rs:312-320 whether any
+// atom leaves the baseline (the field is then not neutral).
+__global__ __launch_bounds__(256) void k_aq_adj(int count, int mode, const float* __restrict__ act,
+                                                const float* __restrict__ ero, AqScratch* __restrict__ sc,
+                                                float* __restrict__ adj) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const bool uniform = mode == kAqUniform || mode == kAqFineUniform || mode == kAqEdgeRefine;
+  const bool fine = mode >= kAqFineMasking;
+  const float k = (uniform ? -1.0f : 1.0f) * 0.25f;
+  float a;
+  if (mode == kAqEdgeRefine) {
+    const float d = act[i] - ero[i] - 6.0f;
+    a = k * (d > 0.0f ? d : 0.0f);
+  } else {
+    const float v = k * ((fine ? ero[i] : act[i]) - sc->mean);
+    a = v < -1.0f ? -1.0f : (v > 1.0f ? 1.0f : v);
+  }
+  adj[i] = a;
+  const bool moved = fine ? aq_mul_of(16u, a) != 16u : roundf(a * 2.0f) != 0.0f;
+  if (moved) atomicOr(&sc->active, 1);
+}
+
+// The HfMul of every varblock of the lists, at its top-left block.
+template <int N>
+__global__ __launch_bounds__(256) void k_aq_block_muls(Geom g, const ProbeParams* __restrict__ pp,
+                                                       const uint32_t* __restrict__ list, int count,
+                                                       uint32_t* __restrict__ muls) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int bx = (int)(list[i] & 0xffffu), by = (int)(list[i] >> 16);
+  muls[(size_t)by * g.bw + bx] = pp->aq_adj ? aq_footprint_mul(pp, g.bw, g.bh, bx, by, N / 8) : pp->aq_baseline;
+}
+
 }  // namespace
 
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
@@ -695,7 +817,7 @@ void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const Pr
   if (c32 > 0)
     k_cover_cost<32><<<div_up(c32, 2), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost32, err);
   dim3 grid(div_up((g.bw + 3) / 4, 256), (g.bh + 3) / 4);
-  k_cover_decide<<<grid, 256, 0, s>>>(g, cost8, cost16, cost32, bmap);
+  k_cover_decide<<<grid, 256, 0, s>>>(g, pp, cost8, cost16, cost32, bmap);
 }
 
 }  // namespace jpxlg
@@ -726,6 +848,22 @@ void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, cons
   const unsigned blocks = std::max(1u, std::min(1024u, (lf_count + 2047) / 2048));
   k_cfl_lf_price<<<blocks, 256, 0, s>>>(lf_count, pp, lf_s, scratch);
   k_cfl_lf_select<<<1, 64, 0, s>>>(scratch, cflp);
+}
+
+void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3], float* act, float* ero,
+                     AqScratch* scratch, float* adj) {
+  const int count = g.bw * g.bh;
+  k_aq_activity<<<div_up(count, 256), 256, 0, s>>>(g, xyb[0], xyb[1], xyb[2], act);
+  if (mode >= kAqFineMasking) k_aq_erode<<<div_up(count, 256), 256, 0, s>>>(g.bw, g.bh, act, ero);
+  if (mode != kAqEdgeRefine) k_aq_mean<<<1, 1024, 0, s>>>(mode >= kAqFineMasking ? ero : act, count, scratch);
+  k_aq_adj<<<div_up(count, 256), 256, 0, s>>>(count, mode, act, ero, scratch, adj);
+}
+
+void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
+                          const int counts[3], uint32_t* muls) {
+  if (counts[0]) k_aq_block_muls<8><<<div_up(counts[0], 256), 256, 0, s>>>(g, pp, list[0], counts[0], muls);
+  if (counts[1]) k_aq_block_muls<16><<<div_up(counts[1], 256), 256, 0, s>>>(g, pp, list[1], counts[1], muls);
+  if (counts[2]) k_aq_block_muls<32><<<div_up(counts[2], 256), 256, 0, s>>>(g, pp, list[2], counts[2], muls);
 }
 
 }  // namespace jpxlg

@@ -302,7 +302,11 @@ void build_probe_params(const jpxlg_quantizer& q, int sharpness, ProbeParams* p)
   for (int c = 0; c < 3; c++) p->mdc[c] = denom > 0.0 ? (float)(65536.0 * (double)(w[c] / 128.0f) / denom) : 0.0f;
   p->mul = mul;
   const float lut[8] = {0.0f, 1.0f / 7.0f, 2.0f / 7.0f, 3.0f / 7.0f, 4.0f / 7.0f, 5.0f / 7.0f, 6.0f / 7.0f, 1.0f};
-  p->sigma = mul * 0.46f * lut[std::min(std::max(sharpness, 0), 7)];
+  p->sig_lut = lut[std::min(std::max(sharpness, 0), 7)];
+  p->sigma = mul * 0.46f * p->sig_lut;
+  p->gs = q.global_scale;
+  p->aq_baseline = q.hf_mul;
+  for (int c = 0; c < 3; c++) p->qm[c] = qm[c];
   p->zero_token_bits = q.zero_token_bits;
   p->truncate = q.truncate ? 1 : 0;
 }

@@ -467,3 +467,101 @@ def test_gaborish_precondition_bit_exact(ctx, wh):
                           orc.forward_frame(want, bmap).view(np.uint32))
     img.close()
 
+
+
+AQ_MODES = [1, 2, 3, 5]  # masking, uniform, fine masking, edge refine
+
+
+@pytest.mark.parametrize("mode", AQ_MODES)
+def test_aq_field_matches_the_oracle(ctx, mode):
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    w, h = 203, 141
+    rgb = _rgb(w, h, 31)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    active = img.set_aq(mode)
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    adj, oactive, _ = orc.aq_field(xyb, mode)
+    assert active == oactive
+    if active:
+        got = img.download(_lib.BUF_AQ_FIELD)
+        assert np.array_equal(got.view(np.uint32), adj.view(np.uint32))
+
+
+def test_aq_flat_frame_is_inactive(ctx):
+    import jpxl_amd
+
+    img = jpxl_amd.Image(ctx, np.full((64, 64, 3), 100, np.uint8), 8)
+    assert not img.set_aq(1)
+
+
+@pytest.mark.parametrize("mode", AQ_MODES)
+@pytest.mark.parametrize("cover", [0, 1])
+@pytest.mark.parametrize("gs,truncate", [(32768, 0), (3001, 1)])
+def test_aq_plan_and_render_match_the_oracle(ctx, mode, cover, gs, truncate):
+    """The fixed-quantizer plan with an AQ field (lib.rs:600-760): the wire
+    quantizer, cover, CfL, per-varblock HfMul, quantized planes and the
+    render with the per-varblock multiplier and EPF sigma."""
+    import jpxl_amd
+    from jpxl_amd import Quantizer, Restoration, _lib
+
+    w, h = 203, 141
+    rgb = _rgb(w, h, 31)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    active = img.set_aq(mode)
+    assert active
+    q = Quantizer(gs, 1, 16, 2, 2, truncate, 1.0, 1.0)
+    img.plan_structure(q, cover, True)
+    img.quantize(q)
+    plan = img.plan_download()
+    muls, wire, ro = img.plan_aq()
+    xyb, lin, _ = orc.prepare_frame(rgb, 8)
+    adj, _, fine = orc.aq_field(xyb, mode)
+    gs2, qlf2, base, oro = orc.aq_setup(gs, 16, 1, fine)
+    assert (wire.global_scale, wire.quant_lf, wire.hf_mul, ro) == (gs2, qlf2, base, oro)
+    qp = orc.with_aq(orc.QuantParams(gs2, base, qlf2, 2, 2, truncate, 1.0, 1.0), adj, oro, fine)
+    bmap = orc.cover(xyb, qp) if cover == 1 else orc.fixed_dct8_map(w, h)
+    assert np.array_equal(plan.block_map, bmap)
+    omuls = orc.aq_block_muls(w, h, bmap, qp)
+    assert np.array_equal(muls, omuls)
+    assert len(np.unique(omuls[bmap > 0])) > 1  # the field moves some varblocks off the baseline
+    coeff = orc.forward_frame(xyb, bmap)
+    cx, cb, lff = orc.estimate_cfl(w, h, bmap, coeff, qp)
+    assert np.array_equal(plan.cfl_x, cx) and np.array_equal(plan.cfl_b, cb) and plan.lf_factors == lff
+    oq, olf = orc.quantize_frame(w, h, bmap, coeff, qp, cx, cb, lff)
+    assert np.array_equal(plan.lf, olf)
+    assert np.array_equal(plan.quant, oq)
+    score, _, _ = img.render_score(Restoration(1, 0, 7))
+    rp = orc.with_aq(orc.RenderParams(gs2, base, qlf2, 2, 2, 1, 0, 7, 8), adj, oro, fine)
+    olin, oxyb = orc.render(w, h, bmap, oq, olf, rp, cx, cb, lff)
+    assert np.array_equal(img.download(_lib.BUF_RENDER_XYB).view(np.uint32), oxyb.view(np.uint32))
+    assert np.array_equal(img.download(_lib.BUF_LINEAR).view(np.uint32), olin.view(np.uint32))
+    oscore, _, _ = orc.ssimulacra2(lin, olin)
+    assert abs(score - oscore) < SCORE_TOL
+
+
+def test_aq_refine_only_fallback(ctx):
+    # an odd global_scale of 1 cannot be halved: AqSetup keeps the request's
+    # quantizer and only refines (lib.rs:1899-1906)
+    import jpxl_amd
+    from jpxl_amd import Quantizer
+
+    w, h = 96, 80
+    rgb = _rgb(w, h, 33)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    assert img.set_aq(1)
+    q = Quantizer(1, 1, 16, 2, 2, 0, 1.0, 1.0)
+    img.plan_structure(q, 0, False)
+    img.quantize(q)
+    muls, wire, ro = img.plan_aq()
+    assert ro and (wire.global_scale, wire.hf_mul, wire.quant_lf) == (1, 1, 16)
+    assert muls[muls > 0].min() >= 1 and muls.max() <= 2
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    adj, _, _ = orc.aq_field(xyb, 1)
+    qp = orc.with_aq(orc.QuantParams(1, 1, 16, 2, 2, 0, 1.0, 1.0), adj, True, False)
+    bmap = orc.fixed_dct8_map(w, h)
+    assert np.array_equal(muls, orc.aq_block_muls(w, h, bmap, qp))
+    oq, olf = orc.quantize_frame(w, h, bmap, orc.forward_frame(xyb, bmap), qp)
+    plan = img.plan_download()
+    assert np.array_equal(plan.quant, oq) and np.array_equal(plan.lf, olf)
