@@ -224,6 +224,21 @@ int32_t jpxlg_batch_probe(uint32_t n, jpxlg_img* const* imgs, const jpxlg_quanti
 int32_t jpxlg_batch_encode_quality(uint32_t n, jpxlg_img* const* imgs, const jpxlg_quality_request* reqs,
                                    jpxlg_quality_result* out);
 
+/* ---- captured probe chains (HIP graphs) -------------------------------- */
+/* The quantize -> reconstruct -> restoration -> SSIMULACRA2 chain of n images
+ * (one context, structures set, one DCT form) captured once as a HIP graph
+ * with restoration r; each jpxlg_graph_probe replays it with one quantizer
+ * per image and returns the scores (synchronous).  A graph is valid while
+ * its images keep the structure they had at capture (a changed structure,
+ * DCT form, AQ field or Gaborish precondition makes the probe fail with
+ * JPXLG_E_INVALID); destroy it before its images.  No counterpart in the
+ * reference, whose evaluator runs each probe eagerly
+ * (PerceptualEvaluator::evaluate, quality.rs:87-157). */
+typedef struct jpxlg_graph jpxlg_graph;
+int32_t jpxlg_graph_create(uint32_t n, jpxlg_img* const* imgs, const jpxlg_restoration* r, jpxlg_graph** out);
+int32_t jpxlg_graph_probe(jpxlg_graph* graph, const jpxlg_quantizer* qs, double* scores);
+void jpxlg_graph_destroy(jpxlg_graph* graph);
+
 /* ---- serving: continuous batching of quality requests ------------------- */
 /* A server owns the navigators of the requests submitted to it (images of
  * its context, resident until released).  Each step() runs one round: the

@@ -35,6 +35,10 @@ __all__ = [
     "JpxlgError",
     "score_pair",
     "quality_quantizer",
+    "batch_probe",
+    "batch_encode_quality",
+    "Graph",
+    "Server",
 ]
 
 
@@ -341,6 +345,40 @@ def batch_probe(images, quantizers, restoration: Restoration | None = None) -> n
     ctx.check(ctx.lib.jpxlg_batch_probe(len(images), arr, qs, ctypes.byref(r),
                                         scores.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
     return scores
+
+
+class Graph:
+    """The probe chain of a fixed set of images captured as a HIP graph;
+    ``probe(quantizers)`` replays it (one quantizer per image) and returns the
+    scores.  Valid while the images keep their structure."""
+
+    def __init__(self, images, restoration: Restoration | None = None):
+        self.ctx, arr = _image_array(images)
+        self.images = list(images)
+        self.handle = ctypes.c_void_p()
+        r = restoration or Restoration(1, 0, 7)
+        self.ctx.check(self.ctx.lib.jpxlg_graph_create(len(self.images), arr, ctypes.byref(r),
+                                                       ctypes.byref(self.handle)))
+
+    def probe(self, quantizers) -> np.ndarray:
+        if len(quantizers) != len(self.images):
+            raise ValueError("one quantizer per image")
+        qs = (Quantizer * len(self.images))(*quantizers)
+        scores = np.zeros(len(self.images), np.float64)
+        self.ctx.check(self.ctx.lib.jpxlg_graph_probe(self.handle, qs,
+                                                      scores.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return scores
+
+    def close(self):
+        if self.handle:
+            self.ctx.lib.jpxlg_graph_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def batch_encode_quality(images, targets, effort: str = "balanced", max_probes: int = 0, mode: int = 0):

@@ -135,6 +135,9 @@ SIGNATURES = {
     "jpxlg_gaborish": (_i32, [_vp, _u32, _u32, _vp, _vp]),
     "jpxlg_set_aq": (_i32, [_vp, _i32, ctypes.POINTER(ctypes.c_int32)]),
     "jpxlg_plan_aq": (_i32, [_vp, _vp, ctypes.POINTER(Quantizer), ctypes.POINTER(ctypes.c_int32)]),
+    "jpxlg_graph_create": (_i32, [ctypes.c_uint32, _vp, ctypes.POINTER(Restoration), ctypes.POINTER(_vp)]),
+    "jpxlg_graph_probe": (_i32, [_vp, ctypes.POINTER(Quantizer), _dp]),
+    "jpxlg_graph_destroy": (None, [_vp]),
     "jpxlg_profile_enable": (_i32, [_vp, _i32]),
     "jpxlg_profile_only": (_i32, [_vp, ctypes.c_char_p]),
     "jpxlg_profile_read": (_i32, [_vp, _vp, _dp, ctypes.POINTER(ctypes.c_uint64), _i32]),

@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <map>
@@ -271,6 +272,9 @@ struct jpxlg_img {
   int aq_mode = kAqOff;
   bool aq_active = false;
   bool preconditioned = false;
+  // Bumped whenever the varblock lists, the DCT form or the source planes
+  // change: a captured chain is valid only for the generation it saw.
+  uint64_t generation = 0;
   // host mirrors of the structure
   std::vector<uint8_t> bmap;
   std::vector<int8_t> cfl_x, cfl_b;
@@ -593,6 +597,7 @@ static int upload_structure(jpxlg_img* img) {
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipStreamSynchronize(st));
   img->has_structure = true;
+  img->generation++;
   img->has_quant = false;
   img->has_render = false;
   return JPXLG_OK;
@@ -649,6 +654,7 @@ extern "C" int32_t jpxlg_precondition_gaborish(jpxlg_img* img) {
   img->has_quant = false;
   img->has_render = false;
   img->preconditioned = true;
+  img->generation++;
   return JPXLG_OK;
 }
 
@@ -712,6 +718,7 @@ extern "C" int32_t jpxlg_set_aq(jpxlg_img* img, int32_t mode, int32_t* active) {
   }
   if (active) *active = img->aq_active ? 1 : 0;
   // the cover and the CfL estimate depend on the field
+  img->generation++;
   img->has_structure = false;
   img->has_quant = false;
   img->has_render = false;
@@ -749,6 +756,7 @@ extern "C" int32_t jpxlg_set_dct_form(jpxlg_img* img, int32_t form) {
   if (form != JPXLG_DCT_EXACT && form != JPXLG_DCT_FP32 && form != JPXLG_DCT_BF16X3)
     return fail(ctx, JPXLG_E_UNSUPPORTED, "unknown DCT form");
   if (form == img->dct_form) return JPXLG_OK;
+  img->generation++;
   img->dct_form = form;
   img->has_quant = false;
   img->has_render = false;
@@ -968,34 +976,34 @@ static int take_slot(jpxlg_ctx* ctx, size_t bytes, jpxlg_ctx::Slot** out) {
 }
 
 struct ChainRun {
-  jpxlg_ctx::Slot* slot = nullptr;
+  jpxlg_ctx::Slot* slot = nullptr;  // null for a graph's own block (already complete)
+  const char* host = nullptr;
   ChainLayout L;
 };
 
-// Enqueues quantize (with each image's quantizer in qs, when qs is non-null)
-// and/or reconstruction + restoration + SSIMULACRA2 (when r is non-null) for
-// n images of one context, with one descriptor upload; the sums and error
-// words come back with read_chain.
-static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
-                         const jpxlg_restoration* r, ChainRun* run) {
-  const ChainLayout L = chain_layout(n);
-  jpxlg_ctx::Slot* sl = nullptr;
-  int rc = take_slot(ctx, L.total, &sl);
-  if (rc) return rc;
-  char* H = sl->host;
-  memset(H, 0, L.total);
+// Workgroup totals per stage of a filled chain.
+struct ChainTotals {
+  int q[3] = {0, 0, 0}, r[3] = {0, 0, 0}, g = 0, e = 0, c[kDescScales] = {0, 0, 0, 0, 0, 0}, p = 0;
+};
+
+static void epf_steps_of(const jpxlg_restoration* r, int* nsteps, int steps[3]) {
+  *nsteps = 0;
+  steps[0] = steps[1] = steps[2] = 0;
+  if (!r) return;
+  if (r->epf_iters == 1) { steps[0] = 1; *nsteps = 1; }
+  else if (r->epf_iters == 2) { steps[0] = 1; steps[1] = 2; *nsteps = 2; }
+  else if (r->epf_iters >= 3) { steps[0] = 0; steps[1] = 1; steps[2] = 2; *nsteps = 3; }
+}
+
+// Writes the descriptors, probe parameters and workgroup prefixes of a chain
+// over n images into H, the host staging of the device block dev (layout L;
+// H zeroed by the caller).
+static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
+                      const jpxlg_restoration* r, char* H, char* dev, const ChainLayout& L, ChainTotals* T) {
   ProbeDesc* hd = (ProbeDesc*)(H + L.desc);
   ProbeParams* hp = (ProbeParams*)(H + L.pp);
   int* ho = (int*)(H + L.off);
-  char* dev = sl->dev;
-  int nsteps = 0, steps[3] = {0, 0, 0};
-  if (r) {
-    if (r->epf_iters == 1) { steps[0] = 1; nsteps = 1; }
-    else if (r->epf_iters == 2) { steps[0] = 1; steps[1] = 2; nsteps = 2; }
-    else if (r->epf_iters >= 3) { steps[0] = 0; steps[1] = 1; steps[2] = 2; nsteps = 3; }
-  }
-  const int gab = r && r->gaborish ? 1 : 0;
-  int tot_q[3] = {0, 0, 0}, tot_r[3] = {0, 0, 0}, tot_g = 0, tot_e = 0, tot_c[kDescScales] = {0}, tot_p = 0;
+  ChainTotals t;
   for (int k = 0; k < n; k++) {
     jpxlg_img* img = imgs[k];
     const Geom& g = img->g;
@@ -1040,84 +1048,118 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
     P.partials = img->d_partials;
     P.sums = (double*)(dev + L.sums) + (size_t)k * 18 * kDescScales;
     DepthSet dd;
-    rc = get_depth(ctx, img->bits, &dd);
+    const int rc = get_depth(ctx, img->bits, &dd);
     if (rc) return rc;
     P.thr = dd.thr;
     P.lut = dd.lut;
     P.levels = dd.levels;
     // workgroup prefixes
     for (int i = 0; i < 3; i++) {
-      ho[L.q_off[i] + k] = tot_q[i];
-      tot_q[i] += img->list_count[i] ? quant_groups(kSides[i], img->list_count[i]) : 0;
-      ho[L.r_off[i] + k] = tot_r[i];
-      tot_r[i] += img->list_count[i] ? render_groups(kSides[i], img->list_count[i]) : 0;
+      ho[L.q_off[i] + k] = t.q[i];
+      t.q[i] += img->list_count[i] ? quant_groups(kSides[i], img->list_count[i]) : 0;
+      ho[L.r_off[i] + k] = t.r[i];
+      t.r[i] += img->list_count[i] ? render_groups(kSides[i], img->list_count[i]) : 0;
     }
-    ho[L.g_off + k] = tot_g;
-    tot_g += gaborish_blocks(g.w, g.h);
-    ho[L.e_off + k] = tot_e;
-    tot_e += epf_tiles(g.w, g.h);
+    ho[L.g_off + k] = t.g;
+    t.g += gaborish_blocks(g.w, g.h);
+    ho[L.e_off + k] = t.e;
+    t.e += epf_tiles(g.w, g.h);
     for (int s = 0; s < kDescScales; s++) {
-      ho[L.c_off[s] + k] = tot_c[s];
-      if (s < img->nscales) tot_c[s] += colour_blocks(img->sw[s], img->sh[s]);
+      ho[L.c_off[s] + k] = t.c[s];
+      if (s < img->nscales) t.c[s] += colour_blocks(img->sw[s], img->sh[s]);
     }
     for (int s = 0; s < kDescScales; s++) {
-      ho[L.p_off + k * kDescScales + s] = tot_p;
-      if (s < img->nscales) tot_p += 3 * pool_units(img->sw[s], img->sh[s]);
+      ho[L.p_off + k * kDescScales + s] = t.p;
+      if (s < img->nscales) t.p += 3 * pool_units(img->sw[s], img->sh[s]);
     }
   }
   for (int i = 0; i < 3; i++) {
-    ho[L.q_off[i] + n] = tot_q[i];
-    ho[L.r_off[i] + n] = tot_r[i];
+    ho[L.q_off[i] + n] = t.q[i];
+    ho[L.r_off[i] + n] = t.r[i];
   }
-  ho[L.g_off + n] = tot_g;
-  ho[L.e_off + n] = tot_e;
-  for (int s = 0; s < kDescScales; s++) ho[L.c_off[s] + n] = tot_c[s];
-  ho[L.p_off + kDescScales * n] = tot_p;
+  ho[L.g_off + n] = t.g;
+  ho[L.e_off + n] = t.e;
+  for (int s = 0; s < kDescScales; s++) ho[L.c_off[s] + n] = t.c[s];
+  ho[L.p_off + kDescScales * n] = t.p;
+  *T = t;
+  return JPXLG_OK;
+}
+
+// Issues a filled chain on st: one upload of the descriptor block, the
+// stage launches (quantize when `quantize`, reconstruction + restoration +
+// SSIMULACRA2 when r), and the download of the error words (and sums).
+// *src = 1 when the restored planes end in the descriptors' aux planes.
+static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, const jpxlg_restoration* r, int form,
+                       char* H, char* dev, const ChainLayout& L, const ChainTotals& T, int* src_out) {
+  int nsteps, steps[3];
+  epf_steps_of(r, &nsteps, steps);
+  const int gab = r && r->gaborish ? 1 : 0;
   HIP_TRY(ctx, hipMemcpyAsync(dev, H, L.off + 4 * (size_t)L.noffs, hipMemcpyHostToDevice, st));
   const ProbeDesc* D = (const ProbeDesc*)(dev + L.desc);
   const int* O = (const int*)(dev + L.off);
-  if (qs) {
+  if (quantize) {
     static const char* kQNames[3] = {"quantize_8", "quantize_16", "quantize_32"};
     for (int i = 0; i < 3; i++) {
-      if (!tot_q[i]) continue;
+      if (!T.q[i]) continue;
       Prof p(ctx, kQNames[i], st);
-      launch_quantize(st, kSides[i], D, O + L.q_off[i], n, tot_q[i], ctx->d_tab);
+      launch_quantize(st, kSides[i], D, O + L.q_off[i], n, T.q[i], ctx->d_tab);
     }
   }
+  int src = 0;
   if (r) {
     static const char* kRNames[3] = {"render_8", "render_16", "render_32"};
-    const int form = imgs[0]->dct_form;
     for (int i = 0; i < 3; i++) {
-      if (!tot_r[i]) continue;
+      if (!T.r[i]) continue;
       Prof p(ctx, kRNames[i], st);
-      launch_render(st, kSides[i], form, D, O + L.r_off[i], n, tot_r[i], ctx->d_tab, ctx->d_mats);
+      launch_render(st, kSides[i], form, D, O + L.r_off[i], n, T.r[i], ctx->d_tab, ctx->d_mats);
     }
-    int src = 0;
     if (gab) {
-      { Prof p(ctx, "gaborish", st); launch_gaborish_b(st, D, O + L.g_off, n, tot_g, src); }
+      { Prof p(ctx, "gaborish", st); launch_gaborish_b(st, D, O + L.g_off, n, T.g, src); }
       src ^= 1;
     }
     static const char* kEpfNames[3] = {"epf_step0", "epf_step1", "epf_step2"};
     for (int i = 0; i < nsteps; i++) {
-      { Prof p(ctx, kEpfNames[steps[i]], st); launch_epf_b(st, steps[i], D, O + L.e_off, n, tot_e, src); }
+      { Prof p(ctx, kEpfNames[steps[i]], st); launch_epf_b(st, steps[i], D, O + L.e_off, n, T.e, src); }
       src ^= 1;
     }
     for (int s = 0; s < kDescScales; s++) {
-      if (!tot_c[s]) continue;
+      if (!T.c[s]) continue;
       Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour", st);
-      launch_colour_b(st, s, D, O + L.c_off[s], n, tot_c[s], src);
+      launch_colour_b(st, s, D, O + L.c_off[s], n, T.c[s], src);
     }
-    { Prof p(ctx, "blur_pool", st); launch_blur_pool_stream(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, tot_p); }
+    { Prof p(ctx, "blur_pool", st); launch_blur_pool_stream(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, T.p); }
     { Prof p(ctx, "pool_reduce", st); launch_reduce_b(st, D, n); }
-    // Keep the restored planes in rx (downloads read them there).
-    if (src)
-      for (int k = 0; k < n; k++)
-        for (int c = 0; c < 3; c++) std::swap(imgs[k]->rx[c], imgs[k]->aux[c]);
   }
   HIP_TRY(ctx, hipGetLastError());
-  // error words and sums come back to the slot's host staging
+  // error words and sums come back to the host staging
   HIP_TRY(ctx, hipMemcpyAsync(H + L.err, dev + L.err, L.off - L.err, hipMemcpyDeviceToHost, st));
   if (r) HIP_TRY(ctx, hipMemcpyAsync(H + L.sums, dev + L.sums, L.total - L.sums, hipMemcpyDeviceToHost, st));
+  *src_out = src;
+  return JPXLG_OK;
+}
+
+// Enqueues quantize (with each image's quantizer in qs, when qs is non-null)
+// and/or reconstruction + restoration + SSIMULACRA2 (when r is non-null) for
+// n images of one context, with one descriptor upload; the sums and error
+// words come back with read_chain.
+static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
+                         const jpxlg_restoration* r, ChainRun* run) {
+  const ChainLayout L = chain_layout(n);
+  jpxlg_ctx::Slot* sl = nullptr;
+  int rc = take_slot(ctx, L.total, &sl);
+  if (rc) return rc;
+  char* H = sl->host;
+  memset(H, 0, L.total);
+  ChainTotals T;
+  rc = fill_chain(ctx, imgs, n, qs, r, H, sl->dev, L, &T);
+  if (rc) return rc;
+  int src = 0;
+  rc = issue_chain(ctx, st, n, qs != nullptr, r, imgs[0]->dct_form, H, sl->dev, L, T, &src);
+  if (rc) return rc;
+  // Keep the restored planes in rx (downloads read them there).
+  if (src)
+    for (int k = 0; k < n; k++)
+      for (int c = 0; c < 3; c++) std::swap(imgs[k]->rx[c], imgs[k]->aux[c]);
   HIP_TRY(ctx, hipEventRecord(sl->done, st));
   sl->pending = true;
   for (int k = 0; k < n; k++) {
@@ -1128,6 +1170,7 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
     if (r) imgs[k]->last_sharpness = r->sharpness;
   }
   run->slot = sl;
+  run->host = H;
   run->L = L;
   return JPXLG_OK;
 }
@@ -1136,9 +1179,11 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
 // scored, its metric (scores[k], optional raw/terms of image 0).
 static int read_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const ChainRun& run, bool scored,
                       double* scores, double* raw0, double* terms0, uint32_t* nterms0) {
-  HIP_TRY(ctx, hipEventSynchronize(run.slot->done));
-  run.slot->pending = false;
-  const char* H = run.slot->host;
+  if (run.slot) {
+    HIP_TRY(ctx, hipEventSynchronize(run.slot->done));
+    run.slot->pending = false;
+  }
+  const char* H = run.host;
   const uint32_t* err = (const uint32_t*)(H + run.L.err);
   int bad = 0;
   for (int k = 0; k < n; k++) bad |= err[k] != 0;
@@ -1357,6 +1402,140 @@ extern "C" int32_t jpxlg_batch_probe(uint32_t n, jpxlg_img* const* imgs, const j
   rc = enqueue_chain(ctx, ctx->stream, imgs, (int)n, qs, r, &run);
   if (rc) return rc;
   return read_chain(ctx, imgs, (int)n, run, true, scores, nullptr, nullptr, nullptr);
+}
+
+// ----------------------------------------------------------------------------
+// Captured probe chains: the quantize -> reconstruct -> score chain of a
+// fixed set of images captured once as a HIP graph and replayed per probe
+// with new quantizers.  The graph owns its descriptor block (device + pinned
+// staging); a replay rewrites the probe parameters in the staging, whose
+// upload is the graph's first node, and the error words and metric sums come
+// back by its last nodes.
+// ----------------------------------------------------------------------------
+
+struct jpxlg_graph {
+  jpxlg_ctx* ctx = nullptr;
+  std::vector<jpxlg_img*> imgs;
+  std::vector<uint64_t> gens;
+  jpxlg_restoration r{};
+  ChainLayout L;
+  char* dev = nullptr;
+  char* host = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  // where each image's restored planes end after a replay
+  std::vector<std::array<float*, 3>> final_rx, final_aux;
+  ~jpxlg_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (dev) (void)hipFree(dev);
+    if (host) (void)hipHostFree(host);
+  }
+};
+
+extern "C" int32_t jpxlg_graph_create(uint32_t n, jpxlg_img* const* imgs, const jpxlg_restoration* r,
+                                      jpxlg_graph** out) {
+  if (!out) return JPXLG_E_INVALID;
+  *out = nullptr;
+  int rc = batch_check(n, imgs);
+  if (rc) return rc;
+  jpxlg_ctx* ctx = imgs[0]->ctx;
+  if (!r) return fail(ctx, JPXLG_E_INVALID, "null restoration");
+  if (r->epf_iters < 0 || r->epf_iters > 3) return fail(ctx, JPXLG_E_UNSUPPORTED, "epf_iters above 3");
+  for (uint32_t k = 0; k < n; k++) {
+    if (!imgs[k]->has_structure) return fail(ctx, JPXLG_E_INVALID, "an image has no plan structure yet");
+    if (imgs[k]->dct_form != imgs[0]->dct_form) return fail(ctx, JPXLG_E_UNSUPPORTED, "mixed DCT forms in one graph");
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  auto G = std::make_unique<jpxlg_graph>();
+  G->ctx = ctx;
+  G->imgs.assign(imgs, imgs + n);
+  for (uint32_t k = 0; k < n; k++) G->gens.push_back(imgs[k]->generation);
+  G->r = *r;
+  G->L = chain_layout((int)n);
+  if (hipMalloc(&G->dev, G->L.total) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "graph block allocation failed");
+  if (hipHostMalloc(&G->host, G->L.total, hipHostMallocDefault) != hipSuccess)
+    return fail(ctx, JPXLG_E_OOM, "graph staging allocation failed");
+  memset(G->host, 0, G->L.total);
+  // Descriptors now; the probe parameters are rewritten before each replay.
+  std::vector<jpxlg_quantizer> q0(n);
+  for (uint32_t k = 0; k < n; k++) q0[k] = jpxlg_quantizer{32768, 1, 16, 2, 2, 0, 1.0f, 1.0f};
+  ChainTotals T;
+  rc = fill_chain(ctx, imgs, (int)n, q0.data(), r, G->host, G->dev, G->L, &T);
+  if (rc) return rc;
+  // Every chain on the stream before the capture has finished (the stream
+  // must be idle for a clean capture), and timing events stay out of it.
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  const bool prof = ctx->prof;
+  ctx->prof = false;
+  int src = 0;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e == hipSuccess) {
+    rc = issue_chain(ctx, ctx->stream, (int)n, true, r, imgs[0]->dct_form, G->host, G->dev, G->L, T, &src);
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &G->graph);
+    if (!rc && e2 != hipSuccess) e = e2;
+  }
+  ctx->prof = prof;
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(ctx, JPXLG_E_DEVICE, std::string("graph capture: ") + hipGetErrorString(e));
+  HIP_TRY(ctx, hipGraphInstantiate(&G->exec, G->graph, nullptr, nullptr, 0));
+  const ProbeDesc* hd = (const ProbeDesc*)(G->host + G->L.desc);
+  for (uint32_t k = 0; k < n; k++) {
+    std::array<float*, 3> a, b;
+    for (int c = 0; c < 3; c++) {
+      a[c] = src ? hd[k].aux[c] : hd[k].rx[c];
+      b[c] = src ? hd[k].rx[c] : hd[k].aux[c];
+    }
+    G->final_rx.push_back(a);
+    G->final_aux.push_back(b);
+  }
+  *out = G.release();
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_graph_probe(jpxlg_graph* G, const jpxlg_quantizer* qs, double* scores) {
+  if (!G) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = G->ctx;
+  if (!qs || !scores) return fail(ctx, JPXLG_E_INVALID, "null argument");
+  const int n = (int)G->imgs.size();
+  for (int k = 0; k < n; k++) {
+    if (G->imgs[k]->generation != G->gens[k] || !G->imgs[k]->has_structure)
+      return fail(ctx, JPXLG_E_INVALID, "an image's structure changed since the graph was captured");
+    const int rc = check_quantizer(ctx, &qs[k]);
+    if (rc) return rc;
+  }
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // The previous replay has completed (this call is synchronous), so the
+  // staging can be rewritten.
+  ProbeParams* hp = (ProbeParams*)(G->host + G->L.pp);
+  for (int k = 0; k < n; k++) probe_params_for(G->imgs[k], qs[k], G->r.sharpness, &hp[k]);
+  memset(G->host + G->L.err, 0, 4 * (size_t)n);
+  {
+    Prof p(ctx, "graph_chain");
+    HIP_TRY(ctx, hipGraphLaunch(G->exec, ctx->stream));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < n; k++) {
+    jpxlg_img* img = G->imgs[k];
+    for (int c = 0; c < 3; c++) {
+      img->rx[c] = G->final_rx[k][c];
+      img->aux[c] = G->final_aux[k][c];
+    }
+    img->last_q = qs[k];
+    img->has_quant = true;
+    img->last_sharpness = G->r.sharpness;
+  }
+  ChainRun run;
+  run.host = G->host;
+  run.L = G->L;
+  return read_chain(ctx, G->imgs.data(), n, run, true, scores, nullptr, nullptr, nullptr);
+}
+
+extern "C" void jpxlg_graph_destroy(jpxlg_graph* G) {
+  if (!G) return;
+  (void)hipSetDevice(G->ctx->device);
+  (void)hipStreamSynchronize(G->ctx->stream);
+  delete G;
 }
 
 // ----------------------------------------------------------------------------

@@ -565,3 +565,52 @@ def test_aq_refine_only_fallback(ctx):
     oq, olf = orc.quantize_frame(w, h, bmap, orc.forward_frame(xyb, bmap), qp)
     plan = img.plan_download()
     assert np.array_equal(plan.quant, oq) and np.array_equal(plan.lf, olf)
+
+
+def test_graph_replays_match_the_eager_chain(ctx):
+    """A captured chain replayed several times, with eager chains and
+    synchronisations between the replays: every replay equals the eager
+    batch probe of the same quantizers (scores, quantized planes, render)."""
+    import jpxl_amd
+    from jpxl_amd import Quantizer, Restoration, _lib
+
+    sizes = [(203, 141), (96, 80), (130, 97)]
+    imgs = []
+    for i, (w, h) in enumerate(sizes):
+        img = jpxl_amd.Image(ctx, _rgb(w, h, 40 + i), 8)
+        img.plan_structure(_quantizer(gs=1500), _lib.COVER_HIERARCHICAL, True)
+        imgs.append(img)
+    r = Restoration(1, 0, 7)
+    g = jpxl_amd.Graph(imgs, r)
+    try:
+        for rep, gs in enumerate([900, 2500, 900, 6000]):
+            qs = [Quantizer(gs + 17 * k, 2, 4, 3, 3, 1, 4.0, 1.0) for k in range(len(imgs))]
+            got = g.probe(qs)
+            plans = [im.plan_download() for im in imgs]
+            lins = [im.download(_lib.BUF_LINEAR) for im in imgs]
+            # eager work between replays: the same probe through the batched chain
+            want = jpxl_amd.batch_probe(imgs, qs, r)
+            assert np.array_equal(got, want), (rep, got, want)
+            for im, p, lin in zip(imgs, plans, lins):
+                e = im.plan_download()
+                assert np.array_equal(p.quant, e.quant) and np.array_equal(p.lf, e.lf)
+                assert np.array_equal(lin.view(np.uint32), im.download(_lib.BUF_LINEAR).view(np.uint32))
+            ctx.lib.jpxlg_profile_enable(ctx.handle, 0)  # a host call between replays
+    finally:
+        g.close()
+
+
+def test_graph_rejects_a_changed_structure(ctx):
+    import jpxl_amd
+    from jpxl_amd import JpxlgError, Restoration, _lib
+
+    img = jpxl_amd.Image(ctx, _rgb(96, 80, 44), 8)
+    img.plan_structure(_quantizer(gs=1500), _lib.COVER_FIXED_DCT8, False)
+    g = jpxl_amd.Graph([img], Restoration(1, 0, 7))
+    try:
+        g.probe([_quantizer(gs=1500)])
+        img.plan_structure(_quantizer(gs=1500), _lib.COVER_HIERARCHICAL, False)
+        with pytest.raises(JpxlgError):
+            g.probe([_quantizer(gs=1500)])
+    finally:
+        g.close()
