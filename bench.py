@@ -430,6 +430,8 @@ def main() -> int:
     assert hl.hipMemcpy(dptr, rgb.ctypes.data, rgb.nbytes, 1) == 0  # host -> device
 
     ctx = jpxl_amd.Context(0)
+    # launch shapes for this size (untimed, like a library's benchmark mode)
+    tuned = ctx.autotune(WIDTH, HEIGHT)
 
     sizes = {}
 
@@ -513,6 +515,7 @@ def main() -> int:
             "chosen_global_scale": int(r0.global_scale),
             "chosen_hf_mul": int(r0.hf_mul),
             "varblocks": {str(k): v for k, v in sizes.items()},
+            "tuned_blur_shape": {"units_per_group": tuned[0], "xcd_swizzle": tuned[1]},
         },
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in rows},  # untimed breakdown pass

@@ -262,6 +262,18 @@ int32_t jpxlg_server_pending(jpxlg_server* srv);
 /* Runs outstanding requests to completion, then frees the server. */
 void jpxlg_server_destroy(jpxlg_server* srv);
 
+/* ---- launch-shape autotuning ------------------------------------------- */
+/* Times every candidate launch shape of the streaming SSIMULACRA2 pass (1, 2
+ * or 4 strip units per workgroup, with or without an XCD-aware workgroup
+ * order) on a synthetic image of width x height and keeps the fastest for
+ * chains of that many pixels (size classes by log2 of a chain's total
+ * pixels); the outputs (optional) report the choice.  No candidate changes a
+ * result (a mismatch fails with JPXLG_E_DEVICE).  Clears the per-kernel
+ * timing totals.  Untuned classes use 4 units, no swizzle.  No counterpart
+ * in the reference. */
+int32_t jpxlg_autotune(jpxlg_ctx* ctx, uint32_t width, uint32_t height, int32_t* units_per_group,
+                       int32_t* swizzle);
+
 /* ---- per-kernel timing ----------------------------------------------------- */
 /* When enabled, every launch the context issues is bracketed by HIP events on
  * its stream.  jpxlg_profile_read synchronises, folds the events into per-

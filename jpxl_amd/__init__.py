@@ -84,6 +84,13 @@ class Context:
         except Exception:
             pass
 
+    def autotune(self, width: int, height: int):
+        """Pick the fastest streaming-pass launch shape for chains of
+        width x height pixels; returns (units per workgroup, swizzle)."""
+        upw, sw = ctypes.c_int32(), ctypes.c_int32()
+        self.check(self.lib.jpxlg_autotune(self.handle, width, height, ctypes.byref(upw), ctypes.byref(sw)))
+        return upw.value, sw.value
+
     # --- per-kernel timing ---------------------------------------------
     def profile(self, on: bool = True) -> None:
         self.check(self.lib.jpxlg_profile_enable(self.handle, int(on)))

@@ -138,6 +138,7 @@ SIGNATURES = {
     "jpxlg_graph_create": (_i32, [ctypes.c_uint32, _vp, ctypes.POINTER(Restoration), ctypes.POINTER(_vp)]),
     "jpxlg_graph_probe": (_i32, [_vp, ctypes.POINTER(Quantizer), _dp]),
     "jpxlg_graph_destroy": (None, [_vp]),
+    "jpxlg_autotune": (_i32, [_vp, ctypes.c_uint32, ctypes.c_uint32, _i32p, _i32p]),
     "jpxlg_profile_enable": (_i32, [_vp, _i32]),
     "jpxlg_profile_only": (_i32, [_vp, ctypes.c_char_p]),
     "jpxlg_profile_read": (_i32, [_vp, _vp, _dp, ctypes.POINTER(ctypes.c_uint64), _i32]),

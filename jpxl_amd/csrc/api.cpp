@@ -67,6 +67,14 @@ struct jpxlg_ctx {
   };
   Slot slots[8];
   int next_slot = 0;
+  // Launch shapes of the streaming SSIMULACRA2 pass per size class (class =
+  // log2 of a chain's total pixels - 16, clamped): units per workgroup and
+  // the XCD swizzle, as picked by jpxlg_autotune (default 4, off).  No
+  // choice changes a result.
+  struct Shape {
+    int upw = 4, swizzle = 0;
+  };
+  Shape tune[12];
 };
 
 namespace {
@@ -984,7 +992,13 @@ struct ChainRun {
 // Workgroup totals per stage of a filled chain.
 struct ChainTotals {
   int q[3] = {0, 0, 0}, r[3] = {0, 0, 0}, g = 0, e = 0, c[kDescScales] = {0, 0, 0, 0, 0, 0}, p = 0;
+  int upw = 4, swizzle = 0;  // streaming-pass launch shape
 };
+
+static int tune_class(double pixels) {
+  int c = (int)floor(log2(std::max(pixels, 1.0))) - 16;
+  return std::min(std::max(c, 0), 11);
+}
 
 static void epf_steps_of(const jpxlg_restoration* r, int* nsteps, int steps[3]) {
   *nsteps = 0;
@@ -1004,6 +1018,11 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
   ProbeParams* hp = (ProbeParams*)(H + L.pp);
   int* ho = (int*)(H + L.off);
   ChainTotals t;
+  double pixels = 0.0;
+  for (int k = 0; k < n; k++) pixels += (double)imgs[k]->g.w * imgs[k]->g.h;
+  const jpxlg_ctx::Shape shape = ctx->tune[tune_class(pixels)];
+  t.upw = shape.upw;
+  t.swizzle = shape.swizzle;
   for (int k = 0; k < n; k++) {
     jpxlg_img* img = imgs[k];
     const Geom& g = img->g;
@@ -1043,7 +1062,8 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
       }
     for (int s = 0; s < img->nscales; s++) {
       P.ptile_off[s] = pt;
-      pt += pool_units(img->sw[s], img->sh[s]);
+      P.punits[s] = pool_units(img->sw[s], img->sh[s]);
+      pt += P.punits[s];
     }
     P.partials = img->d_partials;
     P.sums = (double*)(dev + L.sums) + (size_t)k * 18 * kDescScales;
@@ -1070,7 +1090,7 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
     }
     for (int s = 0; s < kDescScales; s++) {
       ho[L.p_off + k * kDescScales + s] = t.p;
-      if (s < img->nscales) t.p += 3 * pool_units(img->sw[s], img->sh[s]);
+      if (s < img->nscales) t.p += 3 * P.punits[s];
     }
   }
   for (int i = 0; i < 3; i++) {
@@ -1127,7 +1147,10 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
       Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour", st);
       launch_colour_b(st, s, D, O + L.c_off[s], n, T.c[s], src);
     }
-    { Prof p(ctx, "blur_pool", st); launch_blur_pool_stream(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, T.p); }
+    {
+      Prof p(ctx, "blur_pool", st);
+      launch_blur_pool_stream(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, T.p, T.upw, T.swizzle);
+    }
     { Prof p(ctx, "pool_reduce", st); launch_reduce_b(st, D, n); }
   }
   HIP_TRY(ctx, hipGetLastError());
@@ -2064,4 +2087,88 @@ extern "C" int32_t jpxlg_profile_read(jpxlg_ctx* ctx, char* names, double* total
   }
   ctx->totals.clear();
   return n;
+}
+
+// ----------------------------------------------------------------------------
+// Autotuning of launch shapes.  The candidates of a tunable launch compute
+// the same sums (bit for bit), so the choice is speed only: each candidate
+// runs the probe chain of a synthetic image of the requested size, the
+// streaming pass is timed with events, and the fastest shape is kept for the
+// size class (a chain's total pixels).
+// ----------------------------------------------------------------------------
+
+extern "C" int32_t jpxlg_autotune(jpxlg_ctx* ctx, uint32_t width, uint32_t height, int32_t* units_per_group,
+                                  int32_t* swizzle) {
+  if (!ctx) return JPXLG_E_INVALID;
+  if (width < 8 || height < 8) return fail(ctx, JPXLG_E_TOO_SMALL, "autotune size below 8x8");
+  // Deterministic content with flat, smooth and textured regions.
+  std::vector<uint8_t> rgb((size_t)width * height * 3);
+  uint32_t st = 0x9E3779B9u;
+  for (uint32_t y = 0; y < height; y++)
+    for (uint32_t x = 0; x < width; x++) {
+      st = st * 1664525u + 1013904223u;
+      const bool tex = ((x / 96) + (y / 64)) % 3 == 0;
+      const int base = (int)(128 + 60 * sin(x * 0.01) * cos(y * 0.013));
+      for (int c = 0; c < 3; c++) {
+        int v = base + (tex ? (int)((st >> (8 * c)) & 63) - 32 : 0) + 20 * c;
+        rgb[((size_t)y * width + x) * 3 + c] = (uint8_t)std::min(std::max(v, 0), 255);
+      }
+    }
+  jpxlg_img* img = nullptr;
+  int rc = jpxlg_image_upload_srgb(ctx, width, height, 8, rgb.data(), &img);
+  if (rc) return rc;
+  std::unique_ptr<jpxlg_img, void (*)(jpxlg_img*)> hold(img, jpxlg_image_destroy);
+  const jpxlg_quantizer q{3000, 2, 4, 3, 3, 1, 4.0f, 1.0f};
+  rc = jpxlg_plan_structure(img, &q, JPXLG_COVER_FIXED_DCT8, 0);
+  if (rc) return rc;
+  rc = jpxlg_quantize(img, &q);
+  if (rc) return rc;
+  const jpxlg_restoration r{1, 0, 7};
+  const int cls = tune_class((double)width * height);
+  const jpxlg_ctx::Shape keep = ctx->tune[cls];
+  const bool prof = ctx->prof;
+  const std::string only = ctx->prof_only;
+  static const jpxlg_ctx::Shape kCand[6] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {2, 1}, {4, 1}};
+  double best_ms = 1e30, ref_score = 0.0;
+  jpxlg_ctx::Shape best = keep;
+  bool first = true;
+  // earlier timing records are folded away before the candidates run
+  rc = jpxlg_profile_read(ctx, nullptr, nullptr, nullptr, 0);
+  for (int a = 0; a < 6 && rc >= 0; a++) {
+    ctx->tune[cls] = kCand[a];
+    double score = 0.0;
+    ctx->prof = false;
+    for (int it = 0; it < 2 && rc >= 0; it++) rc = jpxlg_render_score(img, &r, &score, nullptr, nullptr, nullptr);
+    if (rc < 0) break;
+    if (first) ref_score = score;
+    if (score != ref_score) {
+      rc = fail(ctx, JPXLG_E_DEVICE, "a launch shape changed the metric sums");
+      break;
+    }
+    ctx->prof = true;
+    ctx->prof_only = "blur_pool";
+    for (int it = 0; it < 5 && rc >= 0; it++) rc = jpxlg_render_score(img, &r, &score, nullptr, nullptr, nullptr);
+    char names[48 * 4] = {0};
+    double ms[4] = {0, 0, 0, 0};
+    uint64_t launches[4] = {0, 0, 0, 0};
+    const int nk = rc < 0 ? 0 : jpxlg_profile_read(ctx, names, ms, launches, 4);
+    ctx->prof = false;
+    ctx->prof_only.clear();
+    if (nk <= 0) {
+      if (rc >= 0) rc = nk < 0 ? nk : fail(ctx, JPXLG_E_DEVICE, "no timing records");
+      break;
+    }
+    if (ms[0] < best_ms) {
+      best_ms = ms[0];
+      best = ctx->tune[cls];
+    }
+    first = false;
+  }
+  ctx->prof = prof;
+  ctx->prof_only = only;
+  ctx->tune[cls] = rc < 0 ? keep : best;
+  if (rc < 0) return rc;
+  if (units_per_group) *units_per_group = best.upw;
+  if (swizzle) *swizzle = best.swizzle;
+  return JPXLG_OK;
 }

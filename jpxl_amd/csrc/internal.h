@@ -173,6 +173,7 @@ struct ProbeDesc {
   float* cand_rgb[kDescScales][3];  // [0] unused here
   double* partials;                 // 18 f64 per pooling tile, scale after scale
   int ptile_off[kDescScales];       // first tile of each scale in `partials`
+  int punits[kDescScales];          // streaming units (partial rows) per scale and channel
   double* sums;                     // 18 f64 per scale
   const float* thr;                 // depth quantizer thresholds and levels
   const float* lut;
@@ -216,9 +217,6 @@ void launch_render(hipStream_t s, int side, int form, const ProbeDesc* D, const 
 void launch_gaborish_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_epf_b(hipStream_t s, int step, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
-// off over the flattened (desc, scale) index d * kDescScales + scale.
-void launch_blur_pool_b(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
-                        int nwg);
 void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd);
 // Workgroup counts of the batched stages for one image.
 int quant_groups(int side, int count);
@@ -274,13 +272,14 @@ void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, u
                           const int counts[3], uint32_t* muls);
 // Pooling tiles of one scale (one partial row of 18 sums per tile).
 int pool_tiles(int w, int h);
-// Tiles of the streaming candidate pass (64-column strips x row segments);
-// the batched chain's partial rows and reduction use these.
+// Tiles of the streaming candidate pass (64-column strips x 64-row
+// segments); the batched chain's partial rows and reduction use these.
 int pool_units(int w, int h);
 // The streaming candidate pass: `units` (tile, channel) pairs over the
-// flattened (desc, scale) prefix `off` (in units), four per workgroup.
+// flattened (desc, scale) prefix `off` (in units), upw (1, 2 or 4) per
+// workgroup, with or without the XCD swizzle of the workgroup order.
 void launch_blur_pool_stream(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
-                             int units);
+                             int units, int upw, int swizzle);
 constexpr int kMaxPoolScales = 6;
 // Where each scale's tiles start in the partials buffer (in tiles).
 struct PoolLayout {

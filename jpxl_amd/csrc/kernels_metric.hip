@@ -541,36 +541,36 @@ __device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, i
   }
 }
 
-// Batched streaming launch: four (tile, channel) units per workgroup, one
-// per wave; units enumerate (desc, scale) through off[] in units (tiles * 3).
+// Batched streaming launch: blockDim.x / 64 (tile, channel) units per
+// workgroup, one per wave; units enumerate (desc, scale) through off[] in
+// units (tiles * 3).  With `swizzle`, the workgroups the dispatcher deals
+// round-robin to the eight XCDs take contiguous runs of units, so the strips
+// that share halo rows and columns meet in one XCD's L2.  The launch shape
+// changes no sum.
 __global__ __launch_bounds__(256) void k_blur_pool_stream(const ConstTables* __restrict__ tab,
                                                           const ProbeDesc* __restrict__ D,
-                                                          const int* __restrict__ off, int nds, int total) {
+                                                          const int* __restrict__ off, int nds, int total,
+                                                          int swizzle) {
   __shared__ double rows[4][3][72];
-  const int wave = threadIdx.x >> 6;
-  const int unit = blockIdx.x * 4 + wave;
+  const int wave = threadIdx.x >> 6, upw = blockDim.x >> 6;
+  const int nb = gridDim.x;
+  int b = blockIdx.x;
+  if (swizzle && nb >= 64) {
+    // workgroup b runs on XCD b % 8: give that XCD the b / 8-th block of its
+    // contiguous share (the last nb % 8 workgroups keep their own index)
+    const int per = nb / 8;
+    if (b < per * 8) b = (b % 8) * per + b / 8;
+  }
+  const int unit = b * upw + wave;
   if (unit >= total) return;
   const int ds = find_desc(off, nds, unit);
   const int d = ds / kDescScales, sc = ds % kDescScales;
   const ProbeDesc& P = D[d];
   const int w = P.sw[sc], h = P.sh[sc], tx = (w + kPW - 1) / kPW;
-  const int tiles = pool_units_dev(w, h);
+  const int tiles = P.punits[sc];
   const int lu = unit - off[ds], c = lu / tiles, lt = lu % tiles;
   stream_unit(tab, w, h, (lt % tx) * kPW, (lt / tx) * kSH, P.ref_x[sc][c], P.ref_mu[sc][c], P.ref_s11[sc][c],
               P.cand_x[sc][c], rows[wave], P.partials + (size_t)(P.ptile_off[sc] + lt) * 18 + c * 6);
-}
-
-// Batched: workgroups enumerate (desc, scale, tile) through off[] over the
-// flattened index d * kDescScales + scale.
-__global__ __launch_bounds__(256) void k_blur_pool_b(const ConstTables* __restrict__ tab,
-                                                     const ProbeDesc* __restrict__ D, const int* __restrict__ off,
-                                                     int nds) {
-  const int ds = find_desc(off, nds, blockIdx.x);
-  const int d = ds / kDescScales, sc = ds % kDescScales, c = blockIdx.z;
-  const ProbeDesc& P = D[d];
-  const int lt = blockIdx.x - off[ds], w = P.sw[sc], h = P.sh[sc], tx = (w + kPW - 1) / kPW;
-  blur_pool_tile(tab, w, h, (lt % tx) * kPW, (lt / tx) * kPH, P.ref_x[sc][c], P.ref_mu[sc][c], P.ref_s11[sc][c],
-                 P.cand_x[sc][c], P.partials + (size_t)(P.ptile_off[sc] + lt) * 18 + c * 6);
 }
 
 // Fixed-order reduction of every scale's per-tile partial sums in one launch:
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const ProbeDesc* __restrict__ 
   const int s = rem / 18, term = rem % 18;
   const ProbeDesc& P = D[d];
   if (s >= P.nscales) return;
-  const int tiles = pool_units_dev(P.sw[s], P.sh[s]);
+  const int tiles = P.punits[s];
   const double* p = P.partials + (size_t)P.ptile_off[s] * 18;
   double acc = 0.0;
   for (int t = threadIdx.x; t < tiles; t += 256) acc += p[(size_t)t * 18 + term];
@@ -664,16 +664,11 @@ void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* part
   k_reduce_scales<<<18 * L.nscales, 256, 0, s>>>(L, partials, sums);
 }
 
-void launch_blur_pool_b(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
-                        int nwg) {
-  if (nwg <= 0) return;
-  k_blur_pool_b<<<dim3(nwg, 1, 3), 256, 0, s>>>(tab, D, off, nds);
-}
-
 void launch_blur_pool_stream(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
-                             int units) {
+                             int units, int upw, int swizzle) {
   if (units <= 0) return;
-  k_blur_pool_stream<<<div_up(units, 4), 256, 0, s>>>(tab, D, off, nds, units);
+  upw = upw == 1 || upw == 2 ? upw : 4;
+  k_blur_pool_stream<<<div_up(units, upw), 64 * upw, 0, s>>>(tab, D, off, nds, units, swizzle);
 }
 
 void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd) {

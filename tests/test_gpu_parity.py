@@ -614,3 +614,21 @@ def test_graph_rejects_a_changed_structure(ctx):
             g.probe([_quantizer(gs=1500)])
     finally:
         g.close()
+
+
+def test_autotune_keeps_the_scores(ctx):
+    """Every launch shape the tuner may pick gives the same metric sums; the
+    chosen one is reported and used by later chains of that size."""
+    import jpxl_amd
+    from jpxl_amd import Restoration, _lib
+
+    w, h = 320, 200
+    img = jpxl_amd.Image(ctx, _rgb(w, h, 50), 8)
+    q = _quantizer(gs=1800)
+    img.plan_structure(q, _lib.COVER_HIERARCHICAL, True)
+    img.quantize(q)
+    before = img.render_score(Restoration(1, 0, 7))
+    upw, sw = ctx.autotune(w, h)
+    assert upw in (1, 2, 4) and sw in (0, 1)
+    after = img.render_score(Restoration(1, 0, 7))
+    assert before[0] == after[0] and np.array_equal(before[2], after[2])
