@@ -503,13 +503,15 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   // Source XYB + linear RGB (scale 0 of the metric reference).
   float** lin0 = img->cand_rgb[0];
   { Prof p(ctx, "xyb_fwd"); launch_xyb_fwd(st, g, (int)bits, img->d_rgb, lut, img->xyb, lin0, img->d_gray); }
-  // The reference pyramid (reference.rs:176-283): positive XYB, mu1, s11.
+  // The reference pyramid (reference.rs:176-283): positive XYB of every
+  // scale in one launch, then mu1 and s11.
+  {
+    Prof p(ctx, "ref_colour");
+    launch_colour_pyr_ref(st, img->nscales, img->sw, img->sh, lin0, img->ref_x);
+  }
   for (int s = 0; s < img->nscales; s++) {
-    float** down = s + 1 < img->nscales ? img->cand_rgb[s + 1] : nullptr;
-    { Prof p(ctx, "ref_colour"); launch_colour_metric(st, 1, img->sw[s], img->sh[s], img->cand_rgb[s], nullptr, nullptr, 0,
-                         img->ref_x[s], down); }
-    { Prof p(ctx, "ref_blur_moments"); launch_blur_moments(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s], img->ref_mu[s],
-                        img->ref_s11[s]); }
+    Prof p(ctx, "ref_blur_moments");
+    launch_blur_moments(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s], img->ref_mu[s], img->ref_s11[s]);
   }
   HIP_TRY(ctx, hipGetLastError());
   HIP_TRY(ctx, hipMemcpyAsync(&img->gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
@@ -1084,10 +1086,8 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
     t.g += gaborish_blocks(g.w, g.h);
     ho[L.e_off + k] = t.e;
     t.e += epf_tiles(g.w, g.h);
-    for (int s = 0; s < kDescScales; s++) {
-      ho[L.c_off[s] + k] = t.c[s];
-      if (s < img->nscales) t.c[s] += colour_blocks(img->sw[s], img->sh[s]);
-    }
+    ho[L.c_off[0] + k] = t.c[0];  // the colour pyramid's 64x64 tiles
+    t.c[0] += pyramid_tiles(g.w, g.h);
     for (int s = 0; s < kDescScales; s++) {
       ho[L.p_off + k * kDescScales + s] = t.p;
       if (s < img->nscales) t.p += 3 * P.punits[s];
@@ -1099,7 +1099,7 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
   }
   ho[L.g_off + n] = t.g;
   ho[L.e_off + n] = t.e;
-  for (int s = 0; s < kDescScales; s++) ho[L.c_off[s] + n] = t.c[s];
+  ho[L.c_off[0] + n] = t.c[0];
   ho[L.p_off + kDescScales * n] = t.p;
   *T = t;
   return JPXLG_OK;
@@ -1118,20 +1118,16 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
   const ProbeDesc* D = (const ProbeDesc*)(dev + L.desc);
   const int* O = (const int*)(dev + L.off);
   if (quantize) {
-    static const char* kQNames[3] = {"quantize_8", "quantize_16", "quantize_32"};
-    for (int i = 0; i < 3; i++) {
-      if (!T.q[i]) continue;
-      Prof p(ctx, kQNames[i], st);
-      launch_quantize(st, kSides[i], D, O + L.q_off[i], n, T.q[i], ctx->d_tab);
-    }
+    Prof p(ctx, "quantize", st);
+    const int* offs[3] = {O + L.q_off[0], O + L.q_off[1], O + L.q_off[2]};
+    launch_quantize_all(st, D, offs, n, T.q, ctx->d_tab);
   }
   int src = 0;
   if (r) {
-    static const char* kRNames[3] = {"render_8", "render_16", "render_32"};
-    for (int i = 0; i < 3; i++) {
-      if (!T.r[i]) continue;
-      Prof p(ctx, kRNames[i], st);
-      launch_render(st, kSides[i], form, D, O + L.r_off[i], n, T.r[i], ctx->d_tab, ctx->d_mats);
+    {
+      Prof p(ctx, "render", st);
+      const int* offs[3] = {O + L.r_off[0], O + L.r_off[1], O + L.r_off[2]};
+      launch_render_all(st, form, D, offs, n, T.r, ctx->d_tab, ctx->d_mats);
     }
     if (gab) {
       { Prof p(ctx, "gaborish", st); launch_gaborish_b(st, D, O + L.g_off, n, T.g, src); }
@@ -1142,10 +1138,9 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
       { Prof p(ctx, kEpfNames[steps[i]], st); launch_epf_b(st, steps[i], D, O + L.e_off, n, T.e, src); }
       src ^= 1;
     }
-    for (int s = 0; s < kDescScales; s++) {
-      if (!T.c[s]) continue;
-      Prof p(ctx, s == 0 ? "xyb_inv_depth_metric0" : "metric_colour", st);
-      launch_colour_b(st, s, D, O + L.c_off[s], n, T.c[s], src);
+    {
+      Prof p(ctx, "colour_pyramid", st);
+      launch_colour_pyr_b(st, D, O + L.c_off[0], n, T.c[0], src);
     }
     {
       Prof p(ctx, "blur_pool", st);

@@ -210,13 +210,25 @@ void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const Co
 // The per-probe chain over a batch of ProbeDescs (one image = a batch of
 // one).  `off` (device) holds the workgroup prefix of each desc, `nwg` the
 // total; src selects the restoration ping-pong input (0: rx, 1: aux).
-void launch_quantize(hipStream_t s, int side, const ProbeDesc* D, const int* off, int nd, int nwg,
-                     const ConstTables* tab);
-void launch_render(hipStream_t s, int side, int form, const ProbeDesc* D, const int* off, int nd, int nwg,
-                   const ConstTables* tab, const DctMats* mats);
+// Quantization of every varblock size in one launch (DCT32 workgroups first):
+// per-size prefixes off[0..2] (sides 8, 16, 32), nwg[0..2] workgroups.
+void launch_quantize_all(hipStream_t s, const ProbeDesc* D, const int* const off[3], int nd, const int nwg[3],
+                         const ConstTables* tab);
+// Reconstruction of every varblock size in one launch (DCT32 workgroups
+// first): per-size prefixes off[0..2] (sides 8, 16, 32), nwg[0..2] workgroups.
+void launch_render_all(hipStream_t s, int form, const ProbeDesc* D, const int* const off[3], int nd,
+                       const int nwg[3], const ConstTables* tab, const DctMats* mats);
 void launch_gaborish_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_epf_b(hipStream_t s, int step, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
+// The colour pass of every scale in one launch: one workgroup per 64x64 tile
+// of scale 0 (pyramid_tiles per image, prefix `off`).
+constexpr int kMaxPyr = 6;
+int pyramid_tiles(int w, int h);
+void launch_colour_pyr_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
+// The reference pyramid's positive XYB of one image from its linear RGB.
+void launch_colour_pyr_ref(hipStream_t s, int ns, const int* sw, const int* sh, float* const in[3],
+                           float* const (*out)[3]);
 void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd);
 // Workgroup counts of the batched stages for one image.
 int quant_groups(int side, int count);

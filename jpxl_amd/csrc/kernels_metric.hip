@@ -180,6 +180,122 @@ __global__ __launch_bounds__(256) void k_colour_b(const ProbeDesc* __restrict__ 
                     (lb % xb) * 256 + threadIdx.x, lb / xb);
 }
 
+// The colour pass of every scale in one workgroup: a 64x64 tile of scale 0
+// and the tiles it covers at the coarser scales (32x32, 16x16, ... 2x2).
+// Scale 0 converts its samples (XYB^-1 + depth for MODE 0, the linear input
+// for MODE 1); every scale writes positive XYB and box-filters its 2x2 quads
+// into the next scale, whose linear RGB stays in LDS (a and b alternate).
+// The per-sample arithmetic, clamps and order are colour_quad's.
+template <int MODE>
+__device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const float* const* in,
+                                    float* const (*out)[3], const float* thr, const float* lut, int levels,
+                                    int tx0, int ty0, float (*a)[1024], float (*b)[256]) {
+  const float root = MODE == 0 ? cbrt_core(-0.0037930733f) : 0.0f;
+  {
+    const int w = sw[0], h = sh[0], w2 = (w + 1) / 2, h2 = (h + 1) / 2;
+    for (int q = threadIdx.x; q < 32 * 32; q += blockDim.x) {
+      const int qx = q % 32, qy = q / 32, ox = tx0 / 2 + qx, oy = ty0 / 2 + qy;
+      if (ox >= w2 || oy >= h2) continue;
+      float lin[2][2][3];
+#pragma unroll
+      for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+        for (int dx = 0; dx < 2; dx++) {
+          const int x = min(2 * ox + dx, w - 1), y = min(2 * oy + dy, h - 1);
+          const size_t i = (size_t)y * w + x;
+          float v[3];
+          if constexpr (MODE == 0) {
+            float o[3];
+            opsin_inverse(in[0][i], in[1][i], in[2][i], root, o);
+#pragma unroll
+            for (int c = 0; c < 3; c++) v[c] = depth_value(o[c], thr, lut, levels);
+          } else {
+            v[0] = in[0][i]; v[1] = in[1][i]; v[2] = in[2][i];
+          }
+          lin[dy][dx][0] = v[0]; lin[dy][dx][1] = v[1]; lin[dy][dx][2] = v[2];
+          if (2 * ox + dx < w && 2 * oy + dy < h) {
+            float p0, p1, p2;
+            positive_xyb(v[0], v[1], v[2], &p0, &p1, &p2);
+            out[0][0][i] = p0; out[0][1][i] = p1; out[0][2][i] = p2;
+          }
+        }
+      if (ns > 1)
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+          a[c][qy * 32 + qx] = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
+    }
+  }
+  for (int s = 1; s < ns; s++) {
+    __syncthreads();
+    const int T = 64 >> s, w = sw[s], h = sh[s], X = tx0 >> s, Y = ty0 >> s;
+    float (*cur)[1024] = (s & 1) ? a : nullptr;
+    float (*curb)[256] = (s & 1) ? nullptr : b;
+    const bool down = s + 1 < ns;
+    if (X < w && Y < h) {
+      const int w2 = (w + 1) / 2, h2 = (h + 1) / 2, hq = T / 2;
+      for (int q = threadIdx.x; q < hq * hq; q += blockDim.x) {
+        const int qx = q % hq, qy = q / hq, ox = X / 2 + qx, oy = Y / 2 + qy;
+        if (ox >= w2 || oy >= h2) continue;
+        float lin[2][2][3];
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++)
+#pragma unroll
+          for (int dx = 0; dx < 2; dx++) {
+            const int x = min(2 * ox + dx, w - 1), y = min(2 * oy + dy, h - 1);
+            const int l = (y - Y) * T + (x - X);
+#pragma unroll
+            for (int c = 0; c < 3; c++) lin[dy][dx][c] = cur ? cur[c][l] : curb[c][l];
+            if (2 * ox + dx < w && 2 * oy + dy < h) {
+              float p0, p1, p2;
+              positive_xyb(lin[dy][dx][0], lin[dy][dx][1], lin[dy][dx][2], &p0, &p1, &p2);
+              const size_t i = (size_t)y * w + x;
+              out[s][0][i] = p0; out[s][1][i] = p1; out[s][2][i] = p2;
+            }
+          }
+        if (down) {
+          // the next scale's tile is hq x hq, in the other buffer
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const float v = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
+            if (cur) b[c][qy * hq + qx] = v;
+            else a[c][qy * hq + qx] = v;
+          }
+        }
+      }
+    }
+  }
+}
+
+// Batched colour pyramid: workgroups enumerate (desc, 64x64 tile of scale 0)
+// through off[]; the candidate's scale 0 is its restored planes (src 0) or
+// their ping-pong (src 1).
+__global__ __launch_bounds__(256) void k_colour_pyr_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
+                                                      int nd, int src) {
+  __shared__ float s_thr[256], s_lut[256];
+  __shared__ float a[3][1024], b[3][256];
+  const int d = find_desc(off, nd, blockIdx.x);
+  const ProbeDesc& P = D[d];
+  const float* thr = P.thr;
+  const float* lut = P.lut;
+  stage_depth(P.thr, P.lut, P.levels, s_thr, s_lut, &thr, &lut);
+  const int lb = blockIdx.x - off[d], ntx = (P.sw[0] + 63) / 64;
+  colour_pyramid_tile<0>(P.nscales, P.sw, P.sh, src ? P.aux : P.rx, P.cand_x, thr, lut, P.levels, (lb % ntx) * 64,
+                         (lb / ntx) * 64, a, b);
+}
+
+// The reference's pyramid of one image from its linear RGB.
+struct PyrPlanes {
+  int ns, sw[kMaxPyr], sh[kMaxPyr];
+  const float* in[3];
+  float* out[kMaxPyr][3];
+};
+__global__ __launch_bounds__(256) void k_colour_pyr_ref(PyrPlanes A) {
+  __shared__ float a[3][1024], b[3][256];
+  const int ntx = (A.sw[0] + 63) / 64;
+  colour_pyramid_tile<1>(A.ns, A.sw, A.sh, A.in, A.out, nullptr, nullptr, 0, (blockIdx.x % ntx) * 64,
+                         (blockIdx.x / ntx) * 64, a, b);
+}
+
 __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __restrict__ x,
                                                          const float* __restrict__ y,
                                                          const float* __restrict__ b,
@@ -630,6 +746,26 @@ void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* of
   if (nwg <= 0) return;
   if (scale == 0) k_colour_b<0><<<nwg, 256, 0, s>>>(D, off, nd, 0, src);
   else k_colour_b<1><<<nwg, 256, 0, s>>>(D, off, nd, scale, src);
+}
+
+int pyramid_tiles(int w, int h) { return ((w + 63) / 64) * ((h + 63) / 64); }
+
+void launch_colour_pyr_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
+  if (nwg <= 0) return;
+  k_colour_pyr_b<<<nwg, 256, 0, s>>>(D, off, nd, src);
+}
+
+void launch_colour_pyr_ref(hipStream_t s, int ns, const int* sw, const int* sh, float* const in[3],
+                           float* const (*out)[3]) {
+  PyrPlanes A{};
+  A.ns = ns;
+  for (int i = 0; i < ns; i++) {
+    A.sw[i] = sw[i];
+    A.sh[i] = sh[i];
+    for (int c = 0; c < 3; c++) A.out[i][c] = out[i][c];
+  }
+  for (int c = 0; c < 3; c++) A.in[c] = in[c];
+  k_colour_pyr_ref<<<pyramid_tiles(sw[0], sh[0]), 256, 0, s>>>(A);
 }
 
 void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], const float* thresholds,

@@ -386,7 +386,7 @@ __device__ void truncate_wave(int32_t* q, const float* targets, float* recon_y,
 // per 256-thread workgroup so its LDS footprint (~29 KB) leaves room for
 // five workgroups per CU; the smaller sizes keep 64/N varblocks on 192.
 constexpr int quant_vb(int n) { return n == 32 ? 1 : 64 / n; }
-constexpr int quant_threads(int n) { return n == 32 ? 256 : kThreads; }
+constexpr int quant_threads(int) { return 256; }
 
 // The same truncation spread over a group of `nw` waves (DCT32, where one
 // varblock per workgroup would otherwise leave waves idle): lane L of the
@@ -462,11 +462,31 @@ __device__ void truncate_group(int32_t* q, const float* targets, float* recon_y,
   }
 }
 
+// LDS of one quantize workgroup of side N.
 template <int N>
-__global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* __restrict__ D,
-                                                               const int* __restrict__ off, int nd,
-                                                               const ConstTables* __restrict__ tab) {
-  const int d = find_desc(off, nd, blockIdx.x);
+struct QuantSmem {
+  static constexpr int VB = quant_vb(N), NN = N * N;
+  float co[3][VB][NN];
+  int32_t qs[3][VB][NN];
+  float dyh[VB][NN];
+  float dylf[VB][16];
+  float lfs[3][VB][16], lfo[3][VB][16];
+  float khf[2][VB];
+  float vsc[VB][3];  // each varblock's step scales (its own HfMul under AQ)
+  int vbx[VB], vby[VB];
+  int xs[2][8];
+};
+union QuantSmemAll {
+  QuantSmem<8> s8;
+  QuantSmem<16> s16;
+  QuantSmem<32> s32;
+};
+
+// One quantize workgroup (logical index `block` of the size's prefix `off`).
+template <int N>
+__device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, const int* __restrict__ off, int nd,
+                                              int block, const ConstTables* __restrict__ tab, QuantSmem<N>& sm) {
+  const int d = find_desc(off, nd, block);
   const ProbeDesc& P = D[d];
   const Geom g = P.g;
   const uint32_t* __restrict__ list = P.list[size_index(N)];
@@ -487,15 +507,17 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
   uint32_t* __restrict__ err = P.err;
   constexpr int VB = quant_vb(N), T = quant_threads(N), NN = N * N, n = N / 8, si = size_index(N),
                 cb = kCellBase[si];
-  __shared__ float co[3][VB][NN];
-  __shared__ int32_t qs[3][VB][NN];
-  __shared__ float dyh[VB][NN];
-  __shared__ float dylf[VB][16];
-  __shared__ float lfs[3][VB][16], lfo[3][VB][16];
-  __shared__ float khf[2][VB];
-  __shared__ float vsc[VB][3];  // each varblock's step scales (its own HfMul under AQ)
-  __shared__ int vbx[VB], vby[VB];
-  const int base = (blockIdx.x - off[d]) * VB;
+  float (*co)[VB][NN] = sm.co;
+  int32_t (*qs)[VB][NN] = sm.qs;
+  float (*dyh)[NN] = sm.dyh;
+  float (*dylf)[16] = sm.dylf;
+  float (*lfs)[VB][16] = sm.lfs;
+  float (*lfo)[VB][16] = sm.lfo;
+  float (*khf)[VB] = sm.khf;
+  float (*vsc)[3] = sm.vsc;
+  int* vbx = sm.vbx;
+  int* vby = sm.vby;
+  const int base = (block - off[d]) * VB;
   const int nvb = min(VB, count - base);
   int bad = 0;
   if (threadIdx.x < VB) {
@@ -573,7 +595,7 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
   }
   __syncthreads();
   const int wave = threadIdx.x >> 6;
-  __shared__ int xs[2][8];
+  int (*xs)[8] = sm.xs;
   if (pp->truncate) {
     if constexpr (N == 32) {
       // one varblock: all T/64 waves walk its luma block together
@@ -636,14 +658,50 @@ __global__ __launch_bounds__(quant_threads(N)) void k_quantize(const ProbeDesc* 
 }
 
 // ----------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_quantize_all(const ProbeDesc* __restrict__ D, const int* __restrict__ off8,
+                                                      const int* __restrict__ off16, const int* __restrict__ off32,
+                                                      int nd, int n32, int n16, const ConstTables* __restrict__ tab) {
+  __shared__ QuantSmemAll sm;
+  int b = blockIdx.x;  // DCT32 workgroups first (the longest), then DCT16, then DCT8
+  if (b < n32) {
+    quantize_body<32>(D, off32, nd, b, tab, sm.s32);
+    return;
+  }
+  b -= n32;
+  if (b < n16) {
+    quantize_body<16>(D, off16, nd, b, tab, sm.s16);
+    return;
+  }
+  quantize_body<8>(D, off8, nd, b - n16, tab, sm.s8);
+}
+
+// ----------------------------------------------------------------------------
 // Reconstruction (jpxl-plan-render/src/lib.rs:853-1032, :1294-1359)
 // ----------------------------------------------------------------------------
 
+// LDS of one render workgroup of side N.
+template <int N>
+struct RenderSmem {
+  static constexpr int VB = 64 / N, P = N + 1;
+  float lds[VB * 3 * N * P];
+  float llf[VB][3][16];
+  int vbx[VB], vby[VB];
+  float khf[2][VB];
+  float vsc[VB][4];  // step scales and sigma (the varblock's own HfMul under AQ)
+};
+union RenderSmemAll {
+  RenderSmem<8> s8;
+  RenderSmem<16> s16;
+  RenderSmem<32> s32;
+};
+
+// One render workgroup (logical index `block` of the size's prefix `off`).
 template <int N, int FORM>
-__global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
-                                                     int nd, const ConstTables* __restrict__ tab,
-                                                     const DctMats* __restrict__ mats) {
-  const int d = find_desc(off, nd, blockIdx.x);
+__device__ __forceinline__ void render_body(const ProbeDesc* __restrict__ D, const int* __restrict__ off, int nd,
+                                            int block, const ConstTables* __restrict__ tab,
+                                            const DctMats* __restrict__ mats, RenderSmem<N>& sm) {
+  const int d = find_desc(off, nd, block);
   const ProbeDesc& PD = D[d];
   const Geom g = PD.g;
   const uint32_t* __restrict__ list = PD.list[size_index(N)];
@@ -663,12 +721,13 @@ __global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict
   float* __restrict__ B = PD.rx[2];
   float* __restrict__ sigma = PD.sigma;
   constexpr int VB = 64 / N, P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
-  __shared__ float lds[VB * 3 * N * P];
-  __shared__ float llf[VB][3][16];
-  __shared__ int vbx[VB], vby[VB];
-  __shared__ float khf[2][VB];
-  __shared__ float vsc[VB][4];  // step scales and sigma (the varblock's own HfMul under AQ)
-  const int base = (blockIdx.x - off[d]) * VB;
+  float* lds = sm.lds;
+  float (*llf)[3][16] = sm.llf;
+  int* vbx = sm.vbx;
+  int* vby = sm.vby;
+  float (*khf)[VB] = sm.khf;
+  float (*vsc)[4] = sm.vsc;
+  const int base = (block - off[d]) * VB;
   const int nvb = min(VB, count - base);
   if (threadIdx.x < VB) {
     int bx = 0, by = 0;
@@ -787,6 +846,26 @@ __global__ __launch_bounds__(kThreads) void k_render(const ProbeDesc* __restrict
       if (sx < g.bw && sy < g.bh) sigma[(size_t)sy * g.bw + sx] = vsc[jj][3];
     }
   }
+}
+
+template <int FORM>
+__global__ __launch_bounds__(kThreads) void k_render_all(const ProbeDesc* __restrict__ D,
+                                                         const int* __restrict__ off8, const int* __restrict__ off16,
+                                                         const int* __restrict__ off32, int nd, int n32, int n16,
+                                                         const ConstTables* __restrict__ tab,
+                                                         const DctMats* __restrict__ mats) {
+  __shared__ RenderSmemAll sm;
+  int b = blockIdx.x;  // DCT32 workgroups first (the longest), then DCT16, then DCT8
+  if (b < n32) {
+    render_body<32, FORM>(D, off32, nd, b, tab, mats, sm.s32);
+    return;
+  }
+  b -= n32;
+  if (b < n16) {
+    render_body<16, FORM>(D, off16, nd, b, tab, mats, sm.s16);
+    return;
+  }
+  render_body<8, FORM>(D, off8, nd, b - n16, tab, mats, sm.s8);
 }
 
 __global__ void k_fill(float* p, float v, size_t n) {
@@ -1057,32 +1136,23 @@ int quant_groups(int side, int count) {
 }
 int render_groups(int side, int count) { return (int)div_up(count, 64 / side); }
 
-void launch_quantize(hipStream_t s, int side, const ProbeDesc* D, const int* off, int nd, int nwg,
-                     const ConstTables* tab) {
-  if (nwg <= 0) return;
-  switch (side) {
-    case 8: k_quantize<8><<<nwg, quant_threads(8), 0, s>>>(D, off, nd, tab); break;
-    case 16: k_quantize<16><<<nwg, quant_threads(16), 0, s>>>(D, off, nd, tab); break;
-    default: k_quantize<32><<<nwg, quant_threads(32), 0, s>>>(D, off, nd, tab); break;
-  }
+void launch_quantize_all(hipStream_t s, const ProbeDesc* D, const int* const off[3], int nd, const int nwg[3],
+                         const ConstTables* tab) {
+  const int total = nwg[0] + nwg[1] + nwg[2];
+  if (total <= 0) return;
+  k_quantize_all<<<total, 256, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab);
 }
 
-template <int N>
-static void render_size(hipStream_t s, int form, const ProbeDesc* D, const int* off, int nd, int nwg,
-                        const ConstTables* tab, const DctMats* mats) {
-  if (form == kDctFp32) k_render<N, kDctFp32><<<nwg, kThreads, 0, s>>>(D, off, nd, tab, mats);
-  else if (form == kDctBf16x3) k_render<N, kDctBf16x3><<<nwg, kThreads, 0, s>>>(D, off, nd, tab, mats);
-  else k_render<N, kDctExact><<<nwg, kThreads, 0, s>>>(D, off, nd, tab, mats);
-}
-
-void launch_render(hipStream_t s, int side, int form, const ProbeDesc* D, const int* off, int nd, int nwg,
-                   const ConstTables* tab, const DctMats* mats) {
-  if (nwg <= 0) return;
-  switch (side) {
-    case 8: render_size<8>(s, form, D, off, nd, nwg, tab, mats); break;
-    case 16: render_size<16>(s, form, D, off, nd, nwg, tab, mats); break;
-    default: render_size<32>(s, form, D, off, nd, nwg, tab, mats); break;
-  }
+void launch_render_all(hipStream_t s, int form, const ProbeDesc* D, const int* const off[3], int nd,
+                       const int nwg[3], const ConstTables* tab, const DctMats* mats) {
+  const int total = nwg[0] + nwg[1] + nwg[2];
+  if (total <= 0) return;
+  if (form == kDctFp32)
+    k_render_all<kDctFp32><<<total, kThreads, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab, mats);
+  else if (form == kDctBf16x3)
+    k_render_all<kDctBf16x3><<<total, kThreads, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab, mats);
+  else
+    k_render_all<kDctExact><<<total, kThreads, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab, mats);
 }
 
 void launch_fill(hipStream_t s, float* p, float v, size_t n) {
