@@ -221,7 +221,7 @@ void launch_render_all(hipStream_t s, int form, const ProbeDesc* D, const int* c
 void launch_gaborish_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_epf_b(hipStream_t s, int step, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
-// The colour pass of every scale in one launch: one workgroup per 64x64 tile
+// The colour pass of every scale in one launch: one workgroup per 32x32 tile
 // of scale 0 (pyramid_tiles per image, prefix `off`).
 constexpr int kMaxPyr = 6;
 int pyramid_tiles(int w, int h);

@@ -180,21 +180,24 @@ __global__ __launch_bounds__(256) void k_colour_b(const ProbeDesc* __restrict__ 
                     (lb % xb) * 256 + threadIdx.x, lb / xb);
 }
 
-// The colour pass of every scale in one workgroup: a 64x64 tile of scale 0
-// and the tiles it covers at the coarser scales (32x32, 16x16, ... 2x2).
-// Scale 0 converts its samples (XYB^-1 + depth for MODE 0, the linear input
-// for MODE 1); every scale writes positive XYB and box-filters its 2x2 quads
-// into the next scale, whose linear RGB stays in LDS (a and b alternate).
-// The per-sample arithmetic, clamps and order are colour_quad's.
+// The colour pass of every scale in one workgroup: a 32x32 tile of scale 0
+// and the tiles it covers at the coarser scales (16x16, 8x8, ... 1x1; six
+// scales at most).  Scale 0 converts its samples by 2x2 quads (XYB^-1 +
+// depth for MODE 0, the linear input for MODE 1), writes their positive XYB
+// and box-filters each quad into scale 1; every coarser scale, held in LDS
+// (a and b alternate), writes its pixels' positive XYB and box-filters its
+// quads into the next.  The per-sample arithmetic, clamps and order are
+// colour_quad's.
+constexpr int kPyrTile = 32;
 template <int MODE>
 __device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const float* const* in,
                                     float* const (*out)[3], const float* thr, const float* lut, int levels,
-                                    int tx0, int ty0, float (*a)[1024], float (*b)[256]) {
+                                    int tx0, int ty0, float (*a)[256], float (*b)[64]) {
   const float root = MODE == 0 ? cbrt_core(-0.0037930733f) : 0.0f;
   {
     const int w = sw[0], h = sh[0], w2 = (w + 1) / 2, h2 = (h + 1) / 2;
-    for (int q = threadIdx.x; q < 32 * 32; q += blockDim.x) {
-      const int qx = q % 32, qy = q / 32, ox = tx0 / 2 + qx, oy = ty0 / 2 + qy;
+    for (int q = threadIdx.x; q < 16 * 16; q += blockDim.x) {
+      const int qx = q % 16, qy = q / 16, ox = tx0 / 2 + qx, oy = ty0 / 2 + qy;
       if (ox >= w2 || oy >= h2) continue;
       float lin[2][2][3];
 #pragma unroll
@@ -222,46 +225,36 @@ __device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const 
       if (ns > 1)
 #pragma unroll
         for (int c = 0; c < 3; c++)
-          a[c][qy * 32 + qx] = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
+          a[c][qy * 16 + qx] = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
     }
   }
   for (int s = 1; s < ns; s++) {
     __syncthreads();
-    const int T = 64 >> s, w = sw[s], h = sh[s], X = tx0 >> s, Y = ty0 >> s;
-    float (*cur)[1024] = (s & 1) ? a : nullptr;
-    float (*curb)[256] = (s & 1) ? nullptr : b;
-    const bool down = s + 1 < ns;
-    if (X < w && Y < h) {
-      const int w2 = (w + 1) / 2, h2 = (h + 1) / 2, hq = T / 2;
-      for (int q = threadIdx.x; q < hq * hq; q += blockDim.x) {
-        const int qx = q % hq, qy = q / hq, ox = X / 2 + qx, oy = Y / 2 + qy;
-        if (ox >= w2 || oy >= h2) continue;
-        float lin[2][2][3];
+    const int T = kPyrTile >> s, w = sw[s], h = sh[s], X = tx0 >> s, Y = ty0 >> s;
+    const float* cur[3] = {(s & 1) ? a[0] : b[0], (s & 1) ? a[1] : b[1], (s & 1) ? a[2] : b[2]};
+    float* nxt[3] = {(s & 1) ? b[0] : a[0], (s & 1) ? b[1] : a[1], (s & 1) ? b[2] : a[2]};
+    if (X >= w || Y >= h) continue;
+    // this scale's pixels
+    for (int p = threadIdx.x; p < T * T; p += blockDim.x) {
+      const int x = X + p % T, y = Y + p / T;
+      if (x >= w || y >= h) continue;
+      float p0, p1, p2;
+      positive_xyb(cur[0][p], cur[1][p], cur[2][p], &p0, &p1, &p2);
+      const size_t i = (size_t)y * w + x;
+      out[s][0][i] = p0; out[s][1][i] = p1; out[s][2][i] = p2;
+    }
+    if (s + 1 >= ns) continue;
+    // the next scale's tile (T/2 x T/2) from this scale's quads
+    const int w2 = (w + 1) / 2, h2 = (h + 1) / 2, hq = T / 2;
+    for (int q = threadIdx.x; q < hq * hq; q += blockDim.x) {
+      const int qx = q % hq, qy = q / hq, ox = X / 2 + qx, oy = Y / 2 + qy;
+      if (ox >= w2 || oy >= h2) continue;
+      const int x0 = 2 * ox - X, y0 = 2 * oy - Y;
+      const int x1 = min(2 * ox + 1, w - 1) - X, y1 = min(2 * oy + 1, h - 1) - Y;
 #pragma unroll
-        for (int dy = 0; dy < 2; dy++)
-#pragma unroll
-          for (int dx = 0; dx < 2; dx++) {
-            const int x = min(2 * ox + dx, w - 1), y = min(2 * oy + dy, h - 1);
-            const int l = (y - Y) * T + (x - X);
-#pragma unroll
-            for (int c = 0; c < 3; c++) lin[dy][dx][c] = cur ? cur[c][l] : curb[c][l];
-            if (2 * ox + dx < w && 2 * oy + dy < h) {
-              float p0, p1, p2;
-              positive_xyb(lin[dy][dx][0], lin[dy][dx][1], lin[dy][dx][2], &p0, &p1, &p2);
-              const size_t i = (size_t)y * w + x;
-              out[s][0][i] = p0; out[s][1][i] = p1; out[s][2][i] = p2;
-            }
-          }
-        if (down) {
-          // the next scale's tile is hq x hq, in the other buffer
-#pragma unroll
-          for (int c = 0; c < 3; c++) {
-            const float v = (lin[0][0][c] + lin[0][1][c] + lin[1][0][c] + lin[1][1][c]) * 0.25f;
-            if (cur) b[c][qy * hq + qx] = v;
-            else a[c][qy * hq + qx] = v;
-          }
-        }
-      }
+      for (int c = 0; c < 3; c++)
+        nxt[c][qy * hq + qx] = (cur[c][y0 * T + x0] + cur[c][y0 * T + x1] + cur[c][y1 * T + x0] + cur[c][y1 * T + x1]) *
+                               0.25f;
     }
   }
 }
@@ -272,15 +265,15 @@ __device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const 
 __global__ __launch_bounds__(256) void k_colour_pyr_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
                                                       int nd, int src) {
   __shared__ float s_thr[256], s_lut[256];
-  __shared__ float a[3][1024], b[3][256];
+  __shared__ float a[3][256], b[3][64];
   const int d = find_desc(off, nd, blockIdx.x);
   const ProbeDesc& P = D[d];
   const float* thr = P.thr;
   const float* lut = P.lut;
   stage_depth(P.thr, P.lut, P.levels, s_thr, s_lut, &thr, &lut);
-  const int lb = blockIdx.x - off[d], ntx = (P.sw[0] + 63) / 64;
-  colour_pyramid_tile<0>(P.nscales, P.sw, P.sh, src ? P.aux : P.rx, P.cand_x, thr, lut, P.levels, (lb % ntx) * 64,
-                         (lb / ntx) * 64, a, b);
+  const int lb = blockIdx.x - off[d], ntx = (P.sw[0] + kPyrTile - 1) / kPyrTile;
+  colour_pyramid_tile<0>(P.nscales, P.sw, P.sh, src ? P.aux : P.rx, P.cand_x, thr, lut, P.levels,
+                         (lb % ntx) * kPyrTile, (lb / ntx) * kPyrTile, a, b);
 }
 
 // The reference's pyramid of one image from its linear RGB.
@@ -290,10 +283,10 @@ struct PyrPlanes {
   float* out[kMaxPyr][3];
 };
 __global__ __launch_bounds__(256) void k_colour_pyr_ref(PyrPlanes A) {
-  __shared__ float a[3][1024], b[3][256];
-  const int ntx = (A.sw[0] + 63) / 64;
-  colour_pyramid_tile<1>(A.ns, A.sw, A.sh, A.in, A.out, nullptr, nullptr, 0, (blockIdx.x % ntx) * 64,
-                         (blockIdx.x / ntx) * 64, a, b);
+  __shared__ float a[3][256], b[3][64];
+  const int ntx = (A.sw[0] + kPyrTile - 1) / kPyrTile;
+  colour_pyramid_tile<1>(A.ns, A.sw, A.sh, A.in, A.out, nullptr, nullptr, 0, (blockIdx.x % ntx) * kPyrTile,
+                         (blockIdx.x / ntx) * kPyrTile, a, b);
 }
 
 __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __restrict__ x,
@@ -748,7 +741,7 @@ void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* of
   else k_colour_b<1><<<nwg, 256, 0, s>>>(D, off, nd, scale, src);
 }
 
-int pyramid_tiles(int w, int h) { return ((w + 63) / 64) * ((h + 63) / 64); }
+int pyramid_tiles(int w, int h) { return ((w + kPyrTile - 1) / kPyrTile) * ((h + kPyrTile - 1) / kPyrTile); }
 
 void launch_colour_pyr_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
   if (nwg <= 0) return;
