@@ -271,7 +271,11 @@ struct jpxlg_img {
   float4* d_lf_s = nullptr;
   float4* d_hf_s = nullptr;
   uint16_t* d_hf_cell = nullptr;
-  uint4* d_cfl_items[3] = {nullptr, nullptr, nullptr};
+  StructInfo* d_si = nullptr;        // counts of the structure built on the device
+  uint32_t* d_lf_off = nullptr;      // per block: the varblock's first CfL LF sample
+  uint32_t* d_hf_local = nullptr;    // per block: the varblock's first HF sample inside its tile
+  uint32_t* d_tile_count = nullptr;  // HF samples per CfL tile
+  bool gray_known = false;           // img->gray read back from the device
   uint32_t* d_tile_base = nullptr;
   // Adaptive quantization: the per-atom field (bw x bh), its scratch, and
   // whether it is active (a mode other than off whose field is not neutral).
@@ -459,7 +463,10 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_lf_s = (float4*)A.take(bpx * 16);
   img->d_hf_s = (float4*)A.take(cpx * 16);
   img->d_hf_cell = (uint16_t*)A.take(cpx * 2);
-  for (int i = 0; i < 3; i++) img->d_cfl_items[i] = (uint4*)A.take(bpx * 16);
+  img->d_si = (StructInfo*)A.take(sizeof(StructInfo));
+  img->d_lf_off = (uint32_t*)A.take(bpx * 4);
+  img->d_hf_local = (uint32_t*)A.take(bpx * 4);
+  img->d_tile_count = (uint32_t*)A.take((size_t)tiles_cfl * 4);
   img->d_tile_base = (uint32_t*)A.take((size_t)(tiles_cfl + 1) * 4);
   img->d_bmap = (uint8_t*)A.take(bpx);
   img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
@@ -477,7 +484,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
-                img->d_cfl_items[2] && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
+                img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
                 img->aq_adj && img->d_aq;
   for (int c = 0; c < 3; c++) {
@@ -509,13 +516,13 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     Prof p(ctx, "ref_colour");
     launch_colour_pyr_ref(st, img->nscales, img->sw, img->sh, lin0, img->ref_x);
   }
-  for (int s = 0; s < img->nscales; s++) {
+  {
     Prof p(ctx, "ref_blur_moments");
-    launch_blur_moments(st, img->sw[s], img->sh[s], ctx->d_tab, img->ref_x[s], img->ref_mu[s], img->ref_s11[s]);
+    launch_blur_moments_all(st, ctx->d_tab, img->nscales, img->sw, img->sh, img->ref_x, img->ref_mu, img->ref_s11);
   }
   HIP_TRY(ctx, hipGetLastError());
-  HIP_TRY(ctx, hipMemcpyAsync(&img->gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
+  // The grayscale flag stays on the device (the CfL kernels read it); the
+  // host reads it back only when asked (jpxlg_image_info).
   *out = img.release();
   return JPXLG_OK;
 }
@@ -542,7 +549,16 @@ extern "C" int32_t jpxlg_image_info(const jpxlg_img* img, uint32_t* bx, uint32_t
   if (!img) return JPXLG_E_INVALID;
   if (bx) *bx = (uint32_t)img->g.bw;
   if (by) *by = (uint32_t)img->g.bh;
-  if (gray) *gray = img->gray;
+  if (gray) {
+    if (!img->gray_known) {
+      jpxlg_img* m = const_cast<jpxlg_img*>(img);
+      HIP_TRY(m->ctx, hipSetDevice(m->ctx->device));
+      HIP_TRY(m->ctx, hipMemcpyAsync(&m->gray, m->d_gray, 4, hipMemcpyDeviceToHost, m->ctx->stream));
+      HIP_TRY(m->ctx, hipStreamSynchronize(m->ctx->stream));
+      m->gray_known = true;
+    }
+    *gray = img->gray;
+  }
   if (scales) *scales = (uint32_t)img->nscales;
   return JPXLG_OK;
 }
@@ -552,6 +568,7 @@ extern "C" int32_t jpxlg_image_info(const jpxlg_img* img, uint32_t* bx, uint32_t
 // ----------------------------------------------------------------------------
 
 static int check_quantizer(jpxlg_ctx* ctx, const jpxlg_quantizer* q);
+static void probe_params_for(const jpxlg_img* img, const jpxlg_quantizer& q, int sharpness, ProbeParams* p);
 
 static int validate_map(const Geom& g, const uint8_t* bmap, std::string* why) {
   std::vector<uint8_t> covered((size_t)g.bw * g.bh, 0);
@@ -577,39 +594,73 @@ static int validate_map(const Geom& g, const uint8_t* bmap, std::string* why) {
   return 1;
 }
 
-static int upload_structure(jpxlg_img* img) {
+// The block map in d_bmap (and, without a CfL estimate, the factors already
+// on the device) becomes the image's structure: the work lists and CfL
+// sample slots are built on the device, the forward DCT of every varblock
+// follows, and with q the CfL estimate; one synchronisation at the end
+// brings back the counts and the host mirrors (block map, factors).
+static int finish_structure(jpxlg_img* img, const jpxlg_quantizer* q) {
   jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
-  std::vector<uint32_t> lists[3];
-  for (int by = 0; by < g.bh; by++)
-    for (int bx = 0; bx < g.bw; bx++) {
-      int side = img->bmap[by * g.bw + bx];
-      if (side) lists[size_index(side)].push_back(((uint32_t)by << 16) | (uint32_t)bx);
-    }
   hipStream_t st = ctx->stream;
-  for (int i = 0; i < 3; i++) {
-    img->list_count[i] = (int)lists[i].size();
-    if (!lists[i].empty())
-      HIP_TRY(ctx, hipMemcpyAsync(img->d_list[i], lists[i].data(), lists[i].size() * 4,
-                                  hipMemcpyHostToDevice, st));
+  const size_t bpx = (size_t)g.bw * g.bh, tiles = (size_t)g.tw * g.th;
+  {
+    Prof p(ctx, "structure_lists");
+    launch_structure_lists(st, g, img->d_bmap, img->d_list, img->d_lf_off, img->d_hf_local, img->d_tile_count,
+                           img->d_tile_base, img->d_si);
   }
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_bmap, img->bmap.data(), img->bmap.size(), hipMemcpyHostToDevice, st));
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_x, img->cfl_x.data(), img->cfl_x.size(), hipMemcpyHostToDevice, st));
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_b, img->cfl_b.data(), img->cfl_b.size(), hipMemcpyHostToDevice, st));
-  CflParams cp{img->lf_factors[0], img->lf_factors[1]};
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_cflp, &cp, sizeof(cp), hipMemcpyHostToDevice, st));
   for (int i = 0; i < 3; i++) {
-    if (!img->list_count[i]) continue;
     Prof p(ctx, kDctNames[i]);
-    launch_dct_fwd(st, kSides[i], img->dct_form, g, img->d_list[i], img->list_count[i], ctx->d_tab, ctx->d_mats,
+    launch_dct_fwd(st, kSides[i], img->dct_form, g, img->d_list[i], 0, img->d_si, ctx->d_tab, ctx->d_mats,
                    img->xyb, img->coeff);
   }
+  if (q) {
+    ProbeParams pp;
+    probe_params_for(img, *q, img->last_sharpness, &pp);
+    HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+    {
+      Prof p(ctx, "cfl_collect");
+      launch_cfl_collect(st, g, ctx->d_tab, img->d_pp, img->d_list, img->d_si, img->d_lf_off, img->d_hf_local,
+                         img->d_tile_base, img->coeff, img->d_lf_s, img->d_hf_s, img->d_hf_cell, img->d_err);
+    }
+    // The frame-LF regression (one long sequential sum) runs on the side
+    // stream beside the per-tile regressions.
+    HIP_TRY(ctx, hipEventRecord(ctx->fork, st));
+    HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
+    {
+      Prof p(ctx, "cfl_lf", ctx->side);
+      launch_cfl_lf(ctx->side, img->d_pp, img->d_si, img->d_lf_s, img->d_lfc, img->d_cflp, img->d_gray);
+    }
+    HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->side));
+    {
+      Prof p(ctx, "cfl_tiles");
+      launch_cfl_tiles(st, g, ctx->d_tab, img->d_pp, img->d_tile_base, img->d_hf_s, img->d_hf_cell, img->d_cfl_x,
+                       img->d_cfl_b, img->d_gray);
+    }
+    HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join, 0));
+  }
   HIP_TRY(ctx, hipGetLastError());
+  StructInfo si;
+  CflParams cp;
+  uint32_t err = 0;
+  img->bmap.resize(bpx);
+  img->cfl_x.resize(tiles);
+  img->cfl_b.resize(tiles);
+  HIP_TRY(ctx, hipMemcpyAsync(&si, img->d_si, sizeof(si), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->bmap.data(), img->d_bmap, bpx, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_x.data(), img->d_cfl_x, tiles, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_b.data(), img->d_cfl_b, tiles, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(&cp, img->d_cflp, sizeof(cp), hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
-  img->has_structure = true;
+  for (int i = 0; i < 3; i++) img->list_count[i] = si.count[i];
+  img->lf_factors[0] = cp.x_factor_lf;
+  img->lf_factors[1] = cp.b_factor_lf;
+  img->has_structure = !err;
   img->generation++;
   img->has_quant = false;
   img->has_render = false;
+  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
   return JPXLG_OK;
 }
 
@@ -630,7 +681,17 @@ extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, cons
   if (img->lf_factors[0] < -128 || img->lf_factors[0] > 127 || img->lf_factors[1] < -128 ||
       img->lf_factors[1] > 127)
     return fail(img->ctx, JPXLG_E_RANGE, "LF CfL factor outside u8 after bias");
-  return upload_structure(img);
+  jpxlg_ctx* ctx = img->ctx;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_bmap, img->bmap.data(), img->bmap.size(), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_x, img->cfl_x.data(), img->cfl_x.size(), hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_b, img->cfl_b.data(), img->cfl_b.size(), hipMemcpyHostToDevice, st));
+  CflParams cp{img->lf_factors[0], img->lf_factors[1]};
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_cflp, &cp, sizeof(cp), hipMemcpyHostToDevice, st));
+  const uint32_t zero = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
+  return finish_structure(img, nullptr);
 }
 
 // prepare_gaborish_frame (jpxl-encode-policy/src/lib.rs:2256-2268): the
@@ -776,7 +837,7 @@ extern "C" int32_t jpxlg_set_dct_form(jpxlg_img* img, int32_t form) {
   for (int i = 0; i < 3; i++) {
     if (!img->list_count[i]) continue;
     Prof p(ctx, kDctNames[i]);
-    launch_dct_fwd(st, kSides[i], img->dct_form, img->g, img->d_list[i], img->list_count[i], ctx->d_tab,
+    launch_dct_fwd(st, kSides[i], img->dct_form, img->g, img->d_list[i], img->list_count[i], nullptr, ctx->d_tab,
                    ctx->d_mats, img->xyb, img->coeff);
   }
   HIP_TRY(ctx, hipGetLastError());
@@ -786,95 +847,6 @@ extern "C" int32_t jpxlg_set_dct_form(jpxlg_img* img, int32_t form) {
 // The chroma-from-luma estimate of build_pixel_plan (lib.rs:3437-3763) on
 // the structure just uploaded: the host lays out the sample slots in the
 // reference's order, the device gathers, regresses and prices.
-static int estimate_cfl(jpxlg_img* img, const jpxlg_quantizer* q) {
-  jpxlg_ctx* ctx = img->ctx;
-  const Geom& g = img->g;
-  hipStream_t st = ctx->stream;
-  std::vector<uint4> items[3];
-  const int tiles = g.tw * g.th;
-  std::vector<uint32_t> tile_count(tiles, 0), tile_base(tiles + 1, 0);
-  // Pass 1: per-tile HF sample counts.
-  for (int by = 0; by < g.bh; by++)
-    for (int bx = 0; bx < g.bw; bx++) {
-      int side = img->bmap[by * g.bw + bx];
-      if (!side) continue;
-      int n = side / 8;
-      tile_count[(by / 8) * g.tw + bx / 8] += (uint32_t)(side * side - n * n);
-    }
-  for (int t = 0; t < tiles; t++) tile_base[t + 1] = tile_base[t] + tile_count[t];
-  std::vector<uint32_t> tile_fill(tiles, 0);
-  uint32_t lf_off = 0;
-  // Pass 2: LF groups of 256 blocks, tile rows, varblocks in raster order.
-  for (int gy = 0; gy < (g.bh + 255) / 256; gy++)
-    for (int gx = 0; gx < (g.bw + 255) / 256; gx++) {
-      const int gbx0 = gx * 256, gby0 = gy * 256;
-      const int gbx1 = std::min(gbx0 + 256, g.bw), gby1 = std::min(gby0 + 256, g.bh);
-      for (int by = gby0; by < gby1; by++)
-        for (int bx = gbx0; bx < gbx1; bx++) {
-          int side = img->bmap[by * g.bw + bx];
-          if (!side) continue;
-          int n = side / 8, t = (by / 8) * g.tw + bx / 8;
-          uint4 it;
-          it.x = ((uint32_t)by << 16) | (uint32_t)bx;
-          it.y = lf_off;
-          it.z = tile_base[t] + tile_fill[t];
-          it.w = 0;
-          items[size_index(side)].push_back(it);
-          lf_off += (uint32_t)(n * n);
-          tile_fill[t] += (uint32_t)(side * side - n * n);
-        }
-    }
-  ProbeParams pp;
-  probe_params_for(img, *q, img->last_sharpness, &pp);
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
-  const uint32_t zero = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
-  int counts[3];
-  const uint4* ditems[3];
-  for (int i = 0; i < 3; i++) {
-    counts[i] = (int)items[i].size();
-    ditems[i] = img->d_cfl_items[i];
-    if (counts[i])
-      HIP_TRY(ctx, hipMemcpyAsync(img->d_cfl_items[i], items[i].data(), items[i].size() * sizeof(uint4),
-                                  hipMemcpyHostToDevice, st));
-  }
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_tile_base, tile_base.data(), tile_base.size() * 4, hipMemcpyHostToDevice, st));
-  {
-    Prof p(ctx, "cfl_collect");
-    launch_cfl_collect(st, g, ctx->d_tab, img->d_pp, ditems, counts, img->coeff, img->d_lf_s, img->d_hf_s,
-                       img->d_hf_cell, img->d_err);
-  }
-  // The frame-LF regression (one long sequential sum) runs on the side
-  // stream beside the per-tile regressions.
-  HIP_TRY(ctx, hipEventRecord(ctx->fork, st));
-  HIP_TRY(ctx, hipStreamWaitEvent(ctx->side, ctx->fork, 0));
-  {
-    Prof p(ctx, "cfl_lf", ctx->side);
-    launch_cfl_lf(ctx->side, img->d_pp, lf_off, img->d_lf_s, img->d_lfc, img->d_cflp);
-  }
-  HIP_TRY(ctx, hipEventRecord(ctx->join, ctx->side));
-  {
-    Prof p(ctx, "cfl_tiles");
-    launch_cfl_tiles(st, g, ctx->d_tab, img->d_pp, img->d_tile_base, img->d_hf_s, img->d_hf_cell, img->d_cfl_x,
-                     img->d_cfl_b);
-  }
-  HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join, 0));
-  HIP_TRY(ctx, hipGetLastError());
-  CflParams cp;
-  uint32_t err = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_x.data(), img->d_cfl_x, tiles, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_b.data(), img->d_cfl_b, tiles, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(&cp, img->d_cflp, sizeof(cp), hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
-  img->lf_factors[0] = cp.x_factor_lf;
-  img->lf_factors[1] = cp.b_factor_lf;
-  img->has_quant = false;
-  img->has_render = false;
-  return JPXLG_OK;
-}
-
 extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q, int32_t cover_mode,
                                         int32_t enable_cfl) {
   if (!img || !q) return JPXLG_E_INVALID;
@@ -885,30 +857,26 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
   int rc = check_quantizer(ctx, q);
   if (rc) return rc;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  (void)enable_cfl;
-  std::vector<uint8_t> bmap((size_t)g.bw * g.bh, 8);
+  hipStream_t st = ctx->stream;
+  const size_t bpx = (size_t)g.bw * g.bh, tiles = (size_t)g.tw * g.th;
+  const uint32_t zero = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
   if (cover_mode == JPXLG_COVER_HIERARCHICAL) {
-    hipStream_t st = ctx->stream;
     ProbeParams pp;
     probe_params_for(img, *q, img->last_sharpness, &pp);
     HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
-    const uint32_t zero = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
-    {
-      Prof p(ctx, "cover");
-      launch_cover(st, g, ctx->d_tab, img->d_pp, img->xyb, img->d_cost[0], img->d_cost[1],
-                   img->d_cost[2], img->d_bmap, img->d_err);
-    }
-    HIP_TRY(ctx, hipGetLastError());
-    uint32_t err = 0;
-    HIP_TRY(ctx, hipMemcpyAsync(bmap.data(), img->d_bmap, bmap.size(), hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, st));
-    HIP_TRY(ctx, hipStreamSynchronize(st));
-    if (err) return fail(ctx, JPXLG_E_RANGE, "a coefficient outside the quantizer's working range");
+    Prof p(ctx, "cover");
+    launch_cover(st, g, ctx->d_tab, img->d_pp, img->xyb, img->d_cost[0], img->d_cost[1], img->d_cost[2],
+                 img->d_bmap, img->d_err);
+  } else {
+    HIP_TRY(ctx, hipMemsetAsync(img->d_bmap, 8, bpx, st));  // every block its own DCT8
   }
-  rc = jpxlg_set_structure(img, bmap.data(), nullptr, nullptr, nullptr);
-  if (rc || !enable_cfl || img->gray) return rc;
-  return estimate_cfl(img, q);
+  if (!enable_cfl) {
+    HIP_TRY(ctx, hipMemsetAsync(img->d_cfl_x, 0, tiles, st));
+    HIP_TRY(ctx, hipMemsetAsync(img->d_cfl_b, 0, tiles, st));
+    HIP_TRY(ctx, hipMemsetAsync(img->d_cflp, 0, sizeof(CflParams), st));
+  }
+  return finish_structure(img, enable_cfl ? q : nullptr);
 }
 
 // ----------------------------------------------------------------------------

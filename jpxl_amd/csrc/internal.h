@@ -126,6 +126,13 @@ struct AqScratch {
   int32_t active;   // some atom leaves the baseline (the field is not neutral)
 };
 
+// Counts of a structure built on the device (device memory): varblocks per
+// size and the CfL LF / HF sample totals.
+struct StructInfo {
+  int count[3];
+  uint32_t lf_total, hf_total;
+};
+
 // Chroma-from-luma structure of a plan (device memory).
 struct CflParams {
   int32_t x_factor_lf, b_factor_lf;  // unbiased (-128..127)
@@ -203,8 +210,11 @@ void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, con
                     float* xyb[3], float* lin[3], int* gray);
 // `form` is a DctForm: the reference's butterflies (bit-exact) or one of the
 // matrix-core contractions of dct_matrix.h.
+// Forward DCT of a size's varblocks: count from the host, or (si non-null)
+// from the device structure over an upper-bound grid.
 void launch_dct_fwd(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
-                    const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]);
+                    const StructInfo* si, const ConstTables* tab, const DctMats* mats, float* const xyb[3],
+                    float* coeff[3]);
 void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const ConstTables* tab,
                        const float* in, float* out);
 // The per-probe chain over a batch of ProbeDescs (one image = a batch of
@@ -255,26 +265,35 @@ void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], con
                             const float* depth_lut, int levels, float* out[3]);
 void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
                          float* mu1[3], float* s11[3]);
+// mu1 and s11 of every scale and channel of a reference pyramid, one launch.
+void launch_blur_moments_all(hipStream_t s, const ConstTables* tab, int ns, const int* sw, const int* sh,
+                             float* const (*x)[3], float* const (*mu)[3], float* const (*s11)[3]);
 // Hierarchical cover (kernels_plan.hip): candidate costs for every square
 // (f64, indexed by the candidate's top-left block) and the per-region
 // quadtree decision written into the block map.
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
                   float* const xyb[3], double* cost8, double* cost16, double* cost32, uint8_t* bmap,
                   uint32_t* err);
-// Chroma-from-luma samples: per-size varblock items (x = (by << 16) | bx,
-// y = LF sample slot, z = HF sample slot) gather the LF and HF samples in
-// the reference's accumulation order.
+// The structure's work lists (in CfL order), counts, and each varblock's CfL
+// sample slots (LF offset per block; HF offset inside its tile, per block;
+// tile bases), from the device block map.
+void launch_structure_lists(hipStream_t s, const Geom& g, const uint8_t* bmap, uint32_t* const list[3],
+                            uint32_t* lf_off, uint32_t* hf_local, uint32_t* tile_count, uint32_t* tile_base,
+                            StructInfo* si);
+// Chroma-from-luma samples of every listed varblock, in the reference's
+// accumulation order (one workgroup per list entry up to each size's bound).
 void launch_cfl_collect(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
-                        const uint4* const items[3], const int counts[3], float* const coeff[3],
-                        float4* lf_s, float4* hf_s, uint16_t* hf_cell, uint32_t* err);
+                        uint32_t* const list[3], const StructInfo* si, const uint32_t* lf_off,
+                        const uint32_t* hf_local, const uint32_t* tile_base, float* const coeff[3], float4* lf_s,
+                        float4* hf_s, uint16_t* hf_cell, uint32_t* err);
 // The per-64x64-tile HF factors from each tile's sample range.
 void launch_cfl_tiles(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
                       const uint32_t* tile_base, const float4* hf_s, const uint16_t* hf_cell,
-                      int8_t* cfl_x, int8_t* cfl_b);
+                      int8_t* cfl_x, int8_t* cfl_b, const int* gray);
 // The frame-LF factors, on their own stream: sequential regression, then a
 // grid-wide pricing of the candidates, then the selection.
-void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, const float4* lf_s,
-                   CflLfScratch* scratch, CflParams* cflp);
+void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, const StructInfo* si, const float4* lf_s,
+                   CflLfScratch* scratch, CflParams* cflp, const int* gray);
 // The AQ field of `mode` from the source XYB planes into adj (bw x bh);
 // act/ero are bw x bh scratch planes; scratch->active must be zero before.
 void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3], float* act, float* ero,

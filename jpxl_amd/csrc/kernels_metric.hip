@@ -323,13 +323,11 @@ __device__ __forceinline__ float fir9(const float* v, int stride, const double* 
   return (float)acc;
 }
 
-__global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstTables* __restrict__ tab,
-                                                      const float* __restrict__ x,
-                                                      float* __restrict__ mu,
-                                                      float* __restrict__ s11) {
+__device__ __forceinline__ void blur_moments_tile(int w, int h, const ConstTables* __restrict__ tab,
+                                                  const float* __restrict__ x, float* __restrict__ mu,
+                                                  float* __restrict__ s11, int x0, int y0) {
   __shared__ float tin[kLH][kLW];
   __shared__ float hb[2][kLH][kTW];
-  const int x0 = blockIdx.x * kTW, y0 = blockIdx.y * kTH;
   double hf[5];
 #pragma unroll
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
@@ -358,6 +356,27 @@ __global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstT
     mu[(size_t)gy * w + gx] = fir9(&hb[0][ly + kHalo][lx], kTW, hf);
     s11[(size_t)gy * w + gx] = fir9(&hb[1][ly + kHalo][lx], kTW, hf);
   }
+}
+
+__global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstTables* __restrict__ tab,
+                                                      const float* __restrict__ x, float* __restrict__ mu,
+                                                      float* __restrict__ s11) {
+  blur_moments_tile(w, h, tab, x, mu, s11, blockIdx.x * kTW, blockIdx.y * kTH);
+}
+
+// Every scale and channel of a reference pyramid in one launch: workgroup
+// (tile of the flattened per-scale tile ranges, channel blockIdx.y).
+struct MomentPlanes {
+  int ns, sw[kMaxPyr], sh[kMaxPyr], off[kMaxPyr + 1];
+  const float* x[kMaxPyr][3];
+  float* mu[kMaxPyr][3];
+  float* s11[kMaxPyr][3];
+};
+__global__ __launch_bounds__(256) void k_blur_moments_all(const ConstTables* __restrict__ tab, MomentPlanes A) {
+  int s = 0;
+  while (s + 1 < A.ns && (int)blockIdx.x >= A.off[s + 1]) s++;
+  const int lt = blockIdx.x - A.off[s], tx = (A.sw[s] + kTW - 1) / kTW, c = blockIdx.y;
+  blur_moments_tile(A.sw[s], A.sh[s], tab, A.x[s][c], A.mu[s][c], A.s11[s][c], (lt % tx) * kTW, (lt / tx) * kTH);
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -766,6 +785,26 @@ void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], con
   int n = w * h;
   k_linear_at_depth<<<div_up(n, 256), 256, 0, s>>>(n, in[0], in[1], in[2], thresholds, depth_lut,
                                                    levels, out[0], out[1], out[2]);
+}
+
+void launch_blur_moments_all(hipStream_t s, const ConstTables* tab, int ns, const int* sw, const int* sh,
+                             float* const (*x)[3], float* const (*mu)[3], float* const (*s11)[3]) {
+  MomentPlanes A{};
+  A.ns = ns;
+  int total = 0;
+  for (int i = 0; i < ns; i++) {
+    A.sw[i] = sw[i];
+    A.sh[i] = sh[i];
+    A.off[i] = total;
+    total += (int)(div_up(sw[i], kTW) * div_up(sh[i], kTH));
+    for (int c = 0; c < 3; c++) {
+      A.x[i][c] = x[i][c];
+      A.mu[i][c] = mu[i][c];
+      A.s11[i][c] = s11[i][c];
+    }
+  }
+  A.off[ns] = total;
+  k_blur_moments_all<<<dim3(total, 3), 256, 0, s>>>(tab, A);
 }
 
 void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],

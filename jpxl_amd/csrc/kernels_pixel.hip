@@ -197,7 +197,8 @@ __device__ __forceinline__ void unpack(uint32_t o, int* bx, int* by) {
 
 template <int N, int FORM>
 __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __restrict__ list,
-                                                      int count, const ConstTables* __restrict__ tab,
+                                                      int count_host, const StructInfo* __restrict__ si,
+                                                      const ConstTables* __restrict__ tab,
                                                       const DctMats* __restrict__ mats,
                                                       const float* __restrict__ X,
                                                       const float* __restrict__ Y,
@@ -208,7 +209,11 @@ __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __
   constexpr int VB = 64 / N, P = N + 1, NN = N * N;
   __shared__ float lds[VB * 3 * N * P];
   __shared__ int vbx[VB], vby[VB];
+  // the count from the device structure when given (the grid is then an
+  // upper bound and workgroups past the count exit)
+  const int count = si ? si->count[size_index(N)] : count_host;
   const int base = blockIdx.x * VB;
+  if (base >= count) return;
   if (threadIdx.x < VB) {
     int bx = 0, by = 0;
     if (base + (int)threadIdx.x < count) unpack(list[base + threadIdx.x], &bx, &by);
@@ -1096,27 +1101,31 @@ void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, con
 }
 
 template <int N, int FORM>
-static void dct_fwd_form(hipStream_t s, const Geom& g, const uint32_t* list, int count, const ConstTables* tab,
-                         const DctMats* mats, float* const xyb[3], float* coeff[3]) {
-  k_dct_fwd<N, FORM><<<div_up(count, 64 / N), kThreads, 0, s>>>(g, list, count, tab, mats, xyb[0], xyb[1], xyb[2],
-                                                                 coeff[0], coeff[1], coeff[2]);
+static void dct_fwd_form(hipStream_t s, const Geom& g, const uint32_t* list, int count, const StructInfo* si,
+                         const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]) {
+  const int n = N / 8, bound = si ? (g.bw / n) * (g.bh / n) : count;
+  if (bound <= 0) return;
+  k_dct_fwd<N, FORM><<<div_up(bound, 64 / N), kThreads, 0, s>>>(g, list, count, si, tab, mats, xyb[0], xyb[1],
+                                                                 xyb[2], coeff[0], coeff[1], coeff[2]);
 }
 
 template <int N>
 static void dct_fwd_size(hipStream_t s, int form, const Geom& g, const uint32_t* list, int count,
-                         const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]) {
-  if (form == kDctFp32) dct_fwd_form<N, kDctFp32>(s, g, list, count, tab, mats, xyb, coeff);
-  else if (form == kDctBf16x3) dct_fwd_form<N, kDctBf16x3>(s, g, list, count, tab, mats, xyb, coeff);
-  else dct_fwd_form<N, kDctExact>(s, g, list, count, tab, mats, xyb, coeff);
+                         const StructInfo* si, const ConstTables* tab, const DctMats* mats, float* const xyb[3],
+                         float* coeff[3]) {
+  if (form == kDctFp32) dct_fwd_form<N, kDctFp32>(s, g, list, count, si, tab, mats, xyb, coeff);
+  else if (form == kDctBf16x3) dct_fwd_form<N, kDctBf16x3>(s, g, list, count, si, tab, mats, xyb, coeff);
+  else dct_fwd_form<N, kDctExact>(s, g, list, count, si, tab, mats, xyb, coeff);
 }
 
 void launch_dct_fwd(hipStream_t s, int side, int form, const Geom& g, const uint32_t* list, int count,
-                    const ConstTables* tab, const DctMats* mats, float* const xyb[3], float* coeff[3]) {
-  if (count <= 0) return;
+                    const StructInfo* si, const ConstTables* tab, const DctMats* mats, float* const xyb[3],
+                    float* coeff[3]) {
+  if (!si && count <= 0) return;
   switch (side) {
-    case 8: dct_fwd_size<8>(s, form, g, list, count, tab, mats, xyb, coeff); break;
-    case 16: dct_fwd_size<16>(s, form, g, list, count, tab, mats, xyb, coeff); break;
-    default: dct_fwd_size<32>(s, form, g, list, count, tab, mats, xyb, coeff); break;
+    case 8: dct_fwd_size<8>(s, form, g, list, count, si, tab, mats, xyb, coeff); break;
+    case 16: dct_fwd_size<16>(s, form, g, list, count, si, tab, mats, xyb, coeff); break;
+    default: dct_fwd_size<32>(s, form, g, list, count, si, tab, mats, xyb, coeff); break;
   }
 }
 

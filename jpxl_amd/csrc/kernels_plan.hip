@@ -291,23 +291,28 @@ __device__ int factor_candidates(int seed, int* out) {
 }
 
 template <int N>
-__global__ __launch_bounds__(64) void k_cfl_collect(Geom g, const uint4* __restrict__ items, int count,
+__global__ __launch_bounds__(64) void k_cfl_collect(Geom g, const uint32_t* __restrict__ list,
+                                                    const StructInfo* __restrict__ si,
+                                                    const uint32_t* __restrict__ lf_off_b,
+                                                    const uint32_t* __restrict__ hf_local,
+                                                    const uint32_t* __restrict__ tile_base,
                                                     const ConstTables* __restrict__ tab,
                                                     const ProbeParams* __restrict__ pp,
                                                     const float* __restrict__ CX, const float* __restrict__ CY,
                                                     const float* __restrict__ CB, float4* __restrict__ lf_s,
                                                     float4* __restrict__ hf_s, uint16_t* __restrict__ hf_cell,
                                                     uint32_t* __restrict__ err) {
-  constexpr int NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  constexpr int NN = N * N, n = N / 8, six = size_index(N), cb = kCellBase[six];
   const int i = blockIdx.x;
-  if (i >= count) return;
-  const uint4 it = items[i];
-  const int bx = (int)(it.x & 0xffffu), by = (int)(it.x >> 16);
-  const uint32_t lf_off = it.y, hf_off = it.z;
+  if (i >= si->count[six]) return;
+  const uint32_t e = list[i];
+  const int bx = (int)(e & 0xffffu), by = (int)(e >> 16);
+  const size_t blk = (size_t)by * g.bw + bx;
+  const uint32_t lf_off = lf_off_b[blk], hf_off = tile_base[(by / 8) * g.tw + bx / 8] + hf_local[blk];
   int bad = 0;
   const DctTables& t = tab->dct;
   if (threadIdx.x == 0) {
-    const float* sf = tab->scale_f[si];
+    const float* sf = tab->scale_f[six];
     float lf[3][n * n];
     const float* cp[3] = {CX, CY, CB};
     for (int c = 0; c < 3; c++) {
@@ -435,10 +440,14 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
                          const ConstTables* __restrict__ tab, const ProbeParams* __restrict__ pp,
                          const float4* __restrict__ hf_s, const uint16_t* __restrict__ hf_cell,
                          int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
-                         double (*stage)[kCflStage]) {
+                         double (*stage)[kCflStage], const int* __restrict__ gray) {
   __shared__ int cands[2][4];
   __shared__ int ncand[2];
   __shared__ unsigned long long cost[2][4];
+  if (*gray) {  // lib.rs:708: no CfL for a grayscale frame
+    if (threadIdx.x == 0) cfl_x[tile] = cfl_b[tile] = 0;
+    return;
+  }
   const uint32_t b0 = tile_base[tile], b1 = tile_base[tile + 1];
   const CflSums a = cfl_sequential_sums(hf_s, b0, b1, stage);
   if (threadIdx.x == 0) {
@@ -520,8 +529,9 @@ __device__ __forceinline__ DD dd_add(DD a, DD b) {
 
 __device__ __forceinline__ DD dd_of(double v) { return DD{v, 0.0}; }
 
-__global__ __launch_bounds__(256) void k_cfl_lf_fast_partial(uint32_t count, const float4* __restrict__ lf_s,
+__global__ __launch_bounds__(256) void k_cfl_lf_fast_partial(const StructInfo* __restrict__ si, const float4* __restrict__ lf_s,
                                                              CflLfScratch* __restrict__ sc) {
+  const uint32_t count = si->lf_total;
   __shared__ DD red[256][7];
   DD acc[7];
 #pragma unroll
@@ -584,7 +594,8 @@ __device__ int certified_seed(double syy, double syc, double scc, double ayc, fl
   return seed;
 }
 
-__global__ void k_cfl_lf_fast_final(uint32_t count, int groups, CflLfScratch* __restrict__ sc) {
+__global__ void k_cfl_lf_fast_final(const StructInfo* __restrict__ si, int groups, CflLfScratch* __restrict__ sc) {
+  const uint32_t count = si->lf_total;
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   DD tot[7];
   for (int k = 0; k < 7; k++) tot[k] = DD{0.0, 0.0};
@@ -609,8 +620,9 @@ __global__ void k_cfl_lf_fast_final(uint32_t count, int groups, CflLfScratch* __
   sc->certified = ok;
 }
 
-__global__ __launch_bounds__(kCflThreads) void k_cfl_lf_seed(uint32_t count, const float4* __restrict__ lf_s,
+__global__ __launch_bounds__(kCflThreads) void k_cfl_lf_seed(const StructInfo* __restrict__ si, const float4* __restrict__ lf_s,
                                                              CflLfScratch* __restrict__ sc) {
+  const uint32_t count = si->lf_total;
   __shared__ double stage[5][kCflStage];
   if (sc->certified) return;  // the fast path's seeds stand
   const CflSums a = cfl_sequential_sums(lf_s, 0, count, stage);
@@ -622,9 +634,10 @@ __global__ __launch_bounds__(kCflThreads) void k_cfl_lf_seed(uint32_t count, con
   }
 }
 
-__global__ __launch_bounds__(256) void k_cfl_lf_price(uint32_t count, const ProbeParams* __restrict__ pp,
+__global__ __launch_bounds__(256) void k_cfl_lf_price(const StructInfo* __restrict__ si, const ProbeParams* __restrict__ pp,
                                                       const float4* __restrict__ lf_s,
                                                       CflLfScratch* __restrict__ sc) {
+  const uint32_t count = si->lf_total;
   __shared__ unsigned long long cost[2][4];
   __shared__ int cands[2][4], ncand[2];
   if (threadIdx.x < 8) {
@@ -654,8 +667,13 @@ __global__ __launch_bounds__(256) void k_cfl_lf_price(uint32_t count, const Prob
     atomicAdd(&sc->cost[threadIdx.x / 4][threadIdx.x % 4], cost[threadIdx.x / 4][threadIdx.x % 4]);
 }
 
-__global__ void k_cfl_lf_select(const CflLfScratch* __restrict__ sc, CflParams* __restrict__ out) {
+__global__ void k_cfl_lf_select(const CflLfScratch* __restrict__ sc, CflParams* __restrict__ out,
+                                const int* __restrict__ gray) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (*gray) {
+    out->x_factor_lf = out->b_factor_lf = 0;
+    return;
+  }
   unsigned long long nx = 0, nb = 0;
   for (int j = 0; j < sc->ncand[0]; j++) if (sc->cands[0][j] == 0) nx = sc->cost[0][j];
   for (int j = 0; j < sc->ncand[1]; j++) if (sc->cands[1][j] == 0) nb = sc->cost[1][j];
@@ -683,9 +701,11 @@ __global__ __launch_bounds__(kCflThreads) void k_cfl_tiles(const uint32_t* __res
                                                            const ProbeParams* __restrict__ pp,
                                                            const float4* __restrict__ hf_s,
                                                            const uint16_t* __restrict__ hf_cell,
-                                                           int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b) {
+                                                           int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
+                                                           const int* __restrict__ gray) {
   __shared__ double stage[5][kCflStage];
-  if ((int)blockIdx.x < tiles) cfl_tile((int)blockIdx.x, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage);
+  if ((int)blockIdx.x < tiles)
+    cfl_tile((int)blockIdx.x, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage, gray);
 }
 
 // ----------------------------------------------------------------------------
@@ -805,6 +825,133 @@ __global__ __launch_bounds__(256) void k_aq_block_muls(Geom g, const ProbeParams
   muls[(size_t)by * g.bw + bx] = pp->aq_adj ? aq_footprint_mul(pp, g.bw, g.bh, bx, by, N / 8) : pp->aq_baseline;
 }
 
+// ----------------------------------------------------------------------------
+// The structure's work lists and CfL sample slots, built on the device from
+// the block map.  Varblocks are visited in the reference's CfL order: LF
+// groups of 256x256 blocks in raster order, blocks in raster order inside a
+// group (lib.rs:3437-3600).  Each size's list takes its varblocks in that
+// order; a varblock's LF samples start at the running sum of n*n; its HF
+// samples start at its tile's base plus the running sum of side^2 - n^2 over
+// the tile's earlier varblocks (tiles are 8x8 blocks, inside one group).
+// ----------------------------------------------------------------------------
+
+// Exclusive scan of four counters over a 1024-thread workgroup.
+__device__ __forceinline__ void scan4_1024(uint32_t v[4], uint32_t tot[4]) {
+  __shared__ uint32_t part[16][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    uint32_t x = v[k];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
+    }
+    inc[k] = x;
+  }
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 4; k++) part[wave][k] = inc[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    uint32_t run = 0;
+    for (int w = 0; w < 16; w++) {
+      const uint32_t t = part[w][threadIdx.x];
+      part[w][threadIdx.x] = run;
+      run += t;
+    }
+    tot[threadIdx.x] = run;  // tot is shared: read after the next barrier
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = part[wave][k] + inc[k] - v[k];
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(1024) void k_struct_lists(Geom g, const uint8_t* __restrict__ bmap,
+                                                       uint32_t* __restrict__ l8, uint32_t* __restrict__ l16,
+                                                       uint32_t* __restrict__ l32, uint32_t* __restrict__ lf_off,
+                                                       StructInfo* __restrict__ si) {
+  __shared__ uint32_t tot[4];
+  uint32_t carry[4] = {0, 0, 0, 0};
+  uint32_t* lists[3] = {l8, l16, l32};
+  const int gw = (g.bw + 255) / 256, gh = (g.bh + 255) / 256;
+  for (int gy = 0; gy < gh; gy++)
+    for (int gx = 0; gx < gw; gx++) {
+      const int bx0 = gx * 256, by0 = gy * 256;
+      const int wid = min(256, g.bw - bx0), hgt = min(256, g.bh - by0), nb = wid * hgt;
+      const int run = (nb + 1023) / 1024, j0 = min(nb, (int)threadIdx.x * run), j1 = min(nb, j0 + run);
+      uint32_t v[4] = {0, 0, 0, 0};  // DCT8, DCT16, DCT32 counts, LF samples
+      for (int j = j0; j < j1; j++) {
+        const int side = bmap[(size_t)(by0 + j / wid) * g.bw + bx0 + j % wid];
+        if (!side) continue;
+        v[size_index(side)]++;
+        v[3] += (uint32_t)(side / 8) * (side / 8);
+      }
+      scan4_1024(v, tot);
+      for (int j = j0; j < j1; j++) {
+        const int by = by0 + j / wid, bx = bx0 + j % wid;
+        const int side = bmap[(size_t)by * g.bw + bx];
+        if (!side) continue;
+        const int k = size_index(side);
+        lists[k][carry[k] + v[k]++] = ((uint32_t)by << 16) | (uint32_t)bx;
+        lf_off[(size_t)by * g.bw + bx] = carry[3] + v[3];
+        v[3] += (uint32_t)(side / 8) * (side / 8);
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) carry[k] += tot[k];
+      __syncthreads();
+    }
+  if (threadIdx.x == 0) {
+    si->count[0] = (int)carry[0];
+    si->count[1] = (int)carry[1];
+    si->count[2] = (int)carry[2];
+    si->lf_total = carry[3];
+  }
+}
+
+// One 64-thread workgroup per 8x8-block tile: each block's HF sample offset
+// inside its tile (raster order) and the tile's total.
+__global__ __launch_bounds__(64) void k_struct_tiles(Geom g, const uint8_t* __restrict__ bmap,
+                                                     uint32_t* __restrict__ hf_local,
+                                                     uint32_t* __restrict__ tile_count) {
+  const int t = blockIdx.x, tx = t % g.tw, ty = t / g.tw;
+  const int bx = tx * 8 + (threadIdx.x & 7), by = ty * 8 + (threadIdx.x >> 3);
+  const bool in = bx < g.bw && by < g.bh;
+  const int side = in ? bmap[(size_t)by * g.bw + bx] : 0;
+  const uint32_t cells = side ? (uint32_t)(side * side - (side / 8) * (side / 8)) : 0u;
+  uint32_t x = cells;
+  const int lane = threadIdx.x;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (in) hf_local[(size_t)by * g.bw + bx] = x - cells;
+  if (lane == 63) tile_count[t] = x;
+}
+
+// tile_base = exclusive prefix of the tile totals (tiles + 1 entries).
+__global__ __launch_bounds__(1024) void k_struct_tile_base(int tiles, const uint32_t* __restrict__ tile_count,
+                                                           uint32_t* __restrict__ tile_base,
+                                                           StructInfo* __restrict__ si) {
+  __shared__ uint32_t tot[4];
+  const int run = (tiles + 1023) / 1024, j0 = min(tiles, (int)threadIdx.x * run), j1 = min(tiles, j0 + run);
+  uint32_t v[4] = {0, 0, 0, 0};
+  for (int j = j0; j < j1; j++) v[0] += tile_count[j];
+  scan4_1024(v, tot);
+  uint32_t acc = v[0];
+  for (int j = j0; j < j1; j++) {
+    tile_base[j] = acc;
+    acc += tile_count[j];
+  }
+  if (threadIdx.x == 0) {
+    tile_base[tiles] = tot[0];
+    si->hf_total = tot[0];
+  }
+}
+
 }  // namespace
 
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
@@ -825,29 +972,40 @@ void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const Pr
 namespace jpxlg {
 
 void launch_cfl_collect(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
-                        const uint4* const items[3], const int counts[3], float* const coeff[3],
-                        float4* lf_s, float4* hf_s, uint16_t* hf_cell, uint32_t* err) {
-  if (counts[0]) k_cfl_collect<8><<<counts[0], 64, 0, s>>>(g, items[0], counts[0], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
-  if (counts[1]) k_cfl_collect<16><<<counts[1], 64, 0, s>>>(g, items[1], counts[1], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
-  if (counts[2]) k_cfl_collect<32><<<counts[2], 64, 0, s>>>(g, items[2], counts[2], tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+                        uint32_t* const list[3], const StructInfo* si, const uint32_t* lf_off,
+                        const uint32_t* hf_local, const uint32_t* tile_base, float* const coeff[3], float4* lf_s,
+                        float4* hf_s, uint16_t* hf_cell, uint32_t* err) {
+  // upper bounds of each size's varblock count; workgroups past the count exit
+  const int ub[3] = {g.bw * g.bh, (g.bw / 2) * (g.bh / 2), (g.bw / 4) * (g.bh / 4)};
+  if (ub[0]) k_cfl_collect<8><<<ub[0], 64, 0, s>>>(g, list[0], si, lf_off, hf_local, tile_base, tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+  if (ub[1]) k_cfl_collect<16><<<ub[1], 64, 0, s>>>(g, list[1], si, lf_off, hf_local, tile_base, tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+  if (ub[2]) k_cfl_collect<32><<<ub[2], 64, 0, s>>>(g, list[2], si, lf_off, hf_local, tile_base, tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
+}
+
+void launch_structure_lists(hipStream_t s, const Geom& g, const uint8_t* bmap, uint32_t* const list[3],
+                            uint32_t* lf_off, uint32_t* hf_local, uint32_t* tile_count, uint32_t* tile_base,
+                            StructInfo* si) {
+  k_struct_lists<<<1, 1024, 0, s>>>(g, bmap, list[0], list[1], list[2], lf_off, si);
+  const int tiles = g.tw * g.th;
+  k_struct_tiles<<<tiles, 64, 0, s>>>(g, bmap, hf_local, tile_count);
+  k_struct_tile_base<<<1, 1024, 0, s>>>(tiles, tile_count, tile_base, si);
 }
 
 void launch_cfl_tiles(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
                       const uint32_t* tile_base, const float4* hf_s, const uint16_t* hf_cell,
-                      int8_t* cfl_x, int8_t* cfl_b) {
+                      int8_t* cfl_x, int8_t* cfl_b, const int* gray) {
   const int tiles = g.tw * g.th;
-  k_cfl_tiles<<<tiles, kCflThreads, 0, s>>>(tile_base, tiles, tab, pp, hf_s, hf_cell, cfl_x, cfl_b);
+  k_cfl_tiles<<<tiles, kCflThreads, 0, s>>>(tile_base, tiles, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, gray);
 }
 
-void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, uint32_t lf_count, const float4* lf_s,
-                   CflLfScratch* scratch, CflParams* cflp) {
-  const int groups = (int)std::max(1u, std::min((unsigned)kCflLfFastGroups, (lf_count + 1023) / 1024));
-  k_cfl_lf_fast_partial<<<groups, 256, 0, s>>>(lf_count, lf_s, scratch);
-  k_cfl_lf_fast_final<<<1, 64, 0, s>>>(lf_count, groups, scratch);
-  k_cfl_lf_seed<<<1, kCflThreads, 0, s>>>(lf_count, lf_s, scratch);
-  const unsigned blocks = std::max(1u, std::min(1024u, (lf_count + 2047) / 2048));
-  k_cfl_lf_price<<<blocks, 256, 0, s>>>(lf_count, pp, lf_s, scratch);
-  k_cfl_lf_select<<<1, 64, 0, s>>>(scratch, cflp);
+void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, const StructInfo* si, const float4* lf_s,
+                   CflLfScratch* scratch, CflParams* cflp, const int* gray) {
+  // fixed grids; every kernel reads the sample count from si
+  k_cfl_lf_fast_partial<<<kCflLfFastGroups, 256, 0, s>>>(si, lf_s, scratch);
+  k_cfl_lf_fast_final<<<1, 64, 0, s>>>(si, kCflLfFastGroups, scratch);
+  k_cfl_lf_seed<<<1, kCflThreads, 0, s>>>(si, lf_s, scratch);
+  k_cfl_lf_price<<<256, 256, 0, s>>>(si, pp, lf_s, scratch);
+  k_cfl_lf_select<<<1, 64, 0, s>>>(scratch, cflp, gray);
 }
 
 void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3], float* act, float* ero,
