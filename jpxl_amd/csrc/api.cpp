@@ -67,6 +67,12 @@ struct jpxlg_ctx {
   };
   Slot slots[8];
   int next_slot = 0;
+  // Pinned staging: a ring for small uploads (probe parameters of the
+  // structure passes) and a block for structure readbacks.
+  char* up_host = nullptr;
+  size_t up_cap = 0, up_off = 0;
+  char* rb_host = nullptr;
+  size_t rb_cap = 0;
   // Launch shapes of the streaming SSIMULACRA2 pass per size class (class =
   // log2 of a chain's total pixels - 16, clamped): units per workgroup and
   // the XCD swizzle, as picked by jpxlg_autotune (default 4, off).  No
@@ -228,6 +234,7 @@ size_t image_bytes(int w, int h) {
   b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
   b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
   b += bw * bh * 4 + 256;             // AQ field + scratch
+  b += 64 + 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + bw * bh + 256;  // structure readback block
   b += 256 * 256;  // alignment of every carved buffer
   return b;
 }
@@ -275,6 +282,7 @@ struct jpxlg_img {
   uint32_t* d_lf_off = nullptr;      // per block: the varblock's first CfL LF sample
   uint32_t* d_hf_local = nullptr;    // per block: the varblock's first HF sample inside its tile
   uint32_t* d_tile_count = nullptr;  // HF samples per CfL tile
+  char* d_readback = nullptr;        // the structure's gathered readback block
   bool gray_known = false;           // img->gray read back from the device
   uint32_t* d_tile_base = nullptr;
   // Adaptive quantization: the per-atom field (bw x bh), its scratch, and
@@ -334,6 +342,8 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   srgb8_lut(lut8);
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_lut8, sizeof(lut8)));
   HIP_TRY(ctx.get(), hipMemcpy(ctx->d_lut8, lut8, sizeof(lut8), hipMemcpyHostToDevice));
+  ctx->up_cap = (size_t)256 << 10;
+  HIP_TRY(ctx.get(), hipHostMalloc((void**)&ctx->up_host, ctx->up_cap, hipHostMallocDefault));
   *out = ctx.release();
   return JPXLG_OK;
 }
@@ -356,6 +366,8 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
     if (sl.host) (void)hipHostFree(sl.host);
     if (sl.done) (void)hipEventDestroy(sl.done);
   }
+  if (ctx->up_host) (void)hipHostFree(ctx->up_host);
+  if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
@@ -467,6 +479,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_lf_off = (uint32_t*)A.take(bpx * 4);
   img->d_hf_local = (uint32_t*)A.take(bpx * 4);
   img->d_tile_count = (uint32_t*)A.take((size_t)tiles_cfl * 4);
+  img->d_readback = (char*)A.take(structure_readback_bytes(tiles_cfl, (int)bpx));
   img->d_tile_base = (uint32_t*)A.take((size_t)(tiles_cfl + 1) * 4);
   img->d_bmap = (uint8_t*)A.take(bpx);
   img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
@@ -484,7 +497,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
-                img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
+                img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
                 img->aq_adj && img->d_aq;
   for (int c = 0; c < 3; c++) {
@@ -498,8 +511,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   hipStream_t st = ctx->stream;
   HIP_TRY(ctx, hipMemcpyAsync(img->d_rgb, rgb, px * 3 * sample_bytes,
                               device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
-  const int one = 1;
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_gray, &one, 4, hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemsetD32Async((hipDeviceptr_t)img->d_gray, 1, 1, st));
   const float* lut = ctx->d_lut8;
   if (bits != 8) {
     float* l16 = nullptr;
@@ -567,6 +579,24 @@ extern "C" int32_t jpxlg_image_info(const jpxlg_img* img, uint32_t* bx, uint32_t
 // Structure
 // ----------------------------------------------------------------------------
 
+// Small host-to-device upload on st through the context's pinned ring (an
+// asynchronous copy, unlike one from pageable memory).  The ring wraps after
+// the stream has drained.
+static int upload_small(jpxlg_ctx* ctx, hipStream_t st, void* dev, const void* src, size_t n) {
+  const size_t need = (n + 255) & ~(size_t)255;
+  if (need > ctx->up_cap) return fail(ctx, JPXLG_E_INVALID, "upload larger than the staging ring");
+  if (ctx->up_off + need > ctx->up_cap) {
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->side));
+    ctx->up_off = 0;
+  }
+  char* h = ctx->up_host + ctx->up_off;
+  memcpy(h, src, n);
+  ctx->up_off += need;
+  HIP_TRY(ctx, hipMemcpyAsync(dev, h, n, hipMemcpyHostToDevice, st));
+  return JPXLG_OK;
+}
+
 static int check_quantizer(jpxlg_ctx* ctx, const jpxlg_quantizer* q);
 static void probe_params_for(const jpxlg_img* img, const jpxlg_quantizer& q, int sharpness, ProbeParams* p);
 
@@ -617,7 +647,8 @@ static int finish_structure(jpxlg_img* img, const jpxlg_quantizer* q) {
   if (q) {
     ProbeParams pp;
     probe_params_for(img, *q, img->last_sharpness, &pp);
-    HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+    int rc = upload_small(ctx, st, img->d_pp, &pp, sizeof(pp));
+    if (rc) return rc;
     {
       Prof p(ctx, "cfl_collect");
       launch_cfl_collect(st, g, ctx->d_tab, img->d_pp, img->d_list, img->d_si, img->d_lf_off, img->d_hf_local,
@@ -639,20 +670,30 @@ static int finish_structure(jpxlg_img* img, const jpxlg_quantizer* q) {
     }
     HIP_TRY(ctx, hipStreamWaitEvent(st, ctx->join, 0));
   }
+  launch_gather_structure(st, img->d_si, img->d_cflp, img->d_err, img->d_cfl_x, img->d_cfl_b, img->d_bmap,
+                          (int)tiles, (int)bpx, img->d_readback);
   HIP_TRY(ctx, hipGetLastError());
+  const size_t rb = structure_readback_bytes((int)tiles, (int)bpx);
+  if (ctx->rb_cap < rb) {
+    if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
+    ctx->rb_host = nullptr;
+    ctx->rb_cap = 0;
+    if (hipHostMalloc((void**)&ctx->rb_host, rb, hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, JPXLG_E_OOM, "readback staging allocation failed");
+    ctx->rb_cap = rb;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->rb_host, img->d_readback, rb, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  const char* R = ctx->rb_host;
   StructInfo si;
   CflParams cp;
   uint32_t err = 0;
-  img->bmap.resize(bpx);
-  img->cfl_x.resize(tiles);
-  img->cfl_b.resize(tiles);
-  HIP_TRY(ctx, hipMemcpyAsync(&si, img->d_si, sizeof(si), hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(img->bmap.data(), img->d_bmap, bpx, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_x.data(), img->d_cfl_x, tiles, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(img->cfl_b.data(), img->d_cfl_b, tiles, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(&cp, img->d_cflp, sizeof(cp), hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(&err, img->d_err, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
+  memcpy(&si, R, sizeof(si));
+  memcpy(&cp, R + 32, sizeof(cp));
+  memcpy(&err, R + 48, 4);
+  img->cfl_x.assign((const int8_t*)R + 64, (const int8_t*)R + 64 + tiles);
+  img->cfl_b.assign((const int8_t*)R + 64 + tiles, (const int8_t*)R + 64 + 2 * tiles);
+  img->bmap.assign((const uint8_t*)R + 64 + 2 * tiles, (const uint8_t*)R + 64 + 2 * tiles + bpx);
   for (int i = 0; i < 3; i++) img->list_count[i] = si.count[i];
   img->lf_factors[0] = cp.x_factor_lf;
   img->lf_factors[1] = cp.b_factor_lf;
@@ -859,12 +900,12 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   hipStream_t st = ctx->stream;
   const size_t bpx = (size_t)g.bw * g.bh, tiles = (size_t)g.tw * g.th;
-  const uint32_t zero = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(img->d_err, &zero, 4, hipMemcpyHostToDevice, st));
+  HIP_TRY(ctx, hipMemsetAsync(img->d_err, 0, 4, st));
   if (cover_mode == JPXLG_COVER_HIERARCHICAL) {
     ProbeParams pp;
     probe_params_for(img, *q, img->last_sharpness, &pp);
-    HIP_TRY(ctx, hipMemcpyAsync(img->d_pp, &pp, sizeof(pp), hipMemcpyHostToDevice, st));
+    rc = upload_small(ctx, st, img->d_pp, &pp, sizeof(pp));
+    if (rc) return rc;
     Prof p(ctx, "cover");
     launch_cover(st, g, ctx->d_tab, img->d_pp, img->xyb, img->d_cost[0], img->d_cost[1], img->d_cost[2],
                  img->d_bmap, img->d_err);

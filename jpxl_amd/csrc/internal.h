@@ -280,6 +280,13 @@ void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const Pr
 void launch_structure_lists(hipStream_t s, const Geom& g, const uint8_t* bmap, uint32_t* const list[3],
                             uint32_t* lf_off, uint32_t* hf_local, uint32_t* tile_count, uint32_t* tile_base,
                             StructInfo* si);
+// One contiguous readback block of a structure: StructInfo at 0, CflParams at
+// 32, the error word at 48, cfl_x and cfl_b (tiles bytes each) from 64, then
+// the block map.
+size_t structure_readback_bytes(int tiles, int bpx);
+void launch_gather_structure(hipStream_t s, const StructInfo* si, const CflParams* cp, const uint32_t* err,
+                             const int8_t* cx, const int8_t* cb, const uint8_t* bmap, int tiles, int bpx,
+                             char* out);
 // Chroma-from-luma samples of every listed varblock, in the reference's
 // accumulation order (one workgroup per list entry up to each size's bound).
 void launch_cfl_collect(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,

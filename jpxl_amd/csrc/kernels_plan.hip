@@ -433,6 +433,60 @@ __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0
   return CflSums{total[0], total[1], total[2], total[3], total[4]};
 }
 
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double err = (a.hi - (s - bb)) + (b.hi - bb);
+  const double e = err + (a.lo + b.lo);
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
+}
+
+__device__ __forceinline__ DD dd_of(double v) { return DD{v, 0.0}; }
+
+// best_factor_dev's seed from the exact sums, or INT_MIN when some sum within
+// `gam` (relative to its absolute sum) of the exact one could change it.
+__device__ int certified_seed(double syy, double syc, double scc, double ayc, float base, double gam) {
+  if (syy <= 0.0) return 0;  // non-negative terms: zero exactly when every term is zero
+  const double u = 1.1102230246251565e-16;
+  const double dyy = gam * syy, dyc = gam * ayc, dcc = gam * scc;
+  if (dyy >= 0.5 * syy) return INT_MIN;
+  const double k = syc / syy;
+  const double dk = (dyc + fabs(k) * dyy) / (syy - dyy) + 8.0 * u * fabs(k) + 1e-300;
+  const double star = (k - (double)base) * 84.0;
+  const double dstar = 84.0 * dk + 8.0 * u * (fabs(star) + 84.0 * fabs((double)base)) + 1e-300;
+  const double fr = fabs(star) - floor(fabs(star));
+  if (fabs(fr - 0.5) <= 2.0 * dstar) return INT_MIN;
+  const int seed = best_factor_dev(syy, syc, scc, base);
+  // the argmin among the candidates must hold under the perturbation
+  const double r = round(star);
+  const long long centre = r >= 9.2e18 ? LLONG_MAX : (r <= -9.2e18 ? LLONG_MIN : (long long)r);
+  int cand[6];
+  for (int i = 0; i < 6; i++) {
+    if (i < 5) {
+      long long c = centre + (i - 2);
+      cand[i] = (int)(c < -128 ? -128 : (c > 127 ? 127 : c));
+    } else {
+      cand[i] = 0;
+    }
+  }
+  const double kb = (double)cfl_k(base, seed);
+  const double eb = scc - 2.0 * kb * syc + kb * kb * syy;
+  const double bb = dcc + 2.0 * fabs(kb) * dyc + kb * kb * dyy + 12.0 * u * (scc + 2.0 * fabs(kb) * fabs(syc) + kb * kb * syy);
+  for (int i = 0; i < 6; i++) {
+    if (cand[i] == seed) continue;
+    const double kj = (double)cfl_k(base, cand[i]);
+    const double ej = scc - 2.0 * kj * syc + kj * kj * syy;
+    const double bj = dcc + 2.0 * fabs(kj) * dyc + kj * kj * dyy + 12.0 * u * (scc + 2.0 * fabs(kj) * fabs(syc) + kj * kj * syy);
+    if (!(ej - eb > 2.0 * (bj + bb))) return INT_MIN;
+  }
+  return seed;
+}
+
 // One 64x64 tile's HF factors: sequential regression seeds, then the
 // residual prices of the candidate factors against the DCT8 baseline
 // quantizer (lib.rs:3899-4014).
@@ -449,13 +503,60 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
     return;
   }
   const uint32_t b0 = tile_base[tile], b1 = tile_base[tile + 1];
-  const CflSums a = cfl_sequential_sums(hf_s, b0, b1, stage);
+  // Certified fast path: the exact sums (double-double, any order) decide the
+  // seeds when every sum within the recursive-summation bound of them gives
+  // the same seeds as the sequential f64 sums would; otherwise the sequential
+  // sums run.
+  __shared__ int seeds[3];
+  {
+    DD acc[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) acc[k] = DD{0.0, 0.0};
+    for (uint32_t i = b0 + threadIdx.x; i < b1; i += blockDim.x) {
+      const float4 v = hf_s[i];
+      const double y = (double)v.y, x = (double)v.x, b = (double)v.z;
+      const double t[7] = {y * y, y * x, x * x, y * b, b * b, fabs(y * x), fabs(y * b)};  // exact products
+#pragma unroll
+      for (int k = 0; k < 7; k++) acc[k] = dd_add(acc[k], dd_of(t[k]));
+    }
+    DD (*red)[7] = reinterpret_cast<DD (*)[7]>(&stage[0][0]);  // 320 x 7 DD fit the staging
+#pragma unroll
+    for (int k = 0; k < 7; k++) red[threadIdx.x][k] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < (unsigned)(kCflThreads - 256))
+#pragma unroll
+      for (int k = 0; k < 7; k++) red[threadIdx.x][k] = dd_add(red[threadIdx.x][k], red[threadIdx.x + 256][k]);
+    __syncthreads();
+    for (int stride = 128; stride > 0; stride >>= 1) {
+      if ((int)threadIdx.x < stride)
+#pragma unroll
+        for (int k = 0; k < 7; k++) red[threadIdx.x][k] = dd_add(red[threadIdx.x][k], red[threadIdx.x + stride][k]);
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double S[7];
+      for (int k = 0; k < 7; k++) S[k] = red[0][k].hi + red[0][k].lo;
+      const double n = (double)(b1 - b0) + 4.0, u = 1.1102230246251565e-16;
+      const double gam = 2.0 * (n * u / (1.0 - n * u)) + 1e-28;
+      const int sx = b1 > b0 ? certified_seed(S[0], S[1], S[2], S[5], 0.0f, gam) : 0;
+      const int sb = b1 > b0 ? certified_seed(S[0], S[3], S[4], S[6], 1.0f, gam) : 0;
+      seeds[0] = sx;
+      seeds[1] = sb;
+      seeds[2] = sx != INT_MIN && sb != INT_MIN;
+    }
+    __syncthreads();
+  }
+  if (!seeds[2]) {
+    const CflSums a = cfl_sequential_sums(hf_s, b0, b1, stage);
+    if (threadIdx.x == 0) {
+      // both channels share s_yy (same sample sequence)
+      seeds[0] = b1 > b0 ? best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f) : 0;
+      seeds[1] = b1 > b0 ? best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f) : 0;
+    }
+  }
   if (threadIdx.x == 0) {
-    // both channels share s_yy (same sample sequence)
-    const int seed_x = b1 > b0 ? best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f) : 0;
-    const int seed_b = b1 > b0 ? best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f) : 0;
-    ncand[0] = seed_x == 0 ? 0 : factor_candidates(seed_x, cands[0]);
-    ncand[1] = seed_b == 0 ? 0 : factor_candidates(seed_b, cands[1]);
+    ncand[0] = seeds[0] == 0 ? 0 : factor_candidates(seeds[0], cands[0]);
+    ncand[1] = seeds[1] == 0 ? 0 : factor_candidates(seeds[1], cands[1]);
     for (int ch = 0; ch < 2; ch++)
       for (int i = 0; i < 4; i++) cost[ch][i] = 0;
   }
@@ -514,20 +615,6 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
 // when the seeds best_factor derives are provably the same for every sum
 // within that bound of the exact ones, they are the sequential path's seeds
 // and the long chain is skipped.  Otherwise the chain runs.
-struct DD {
-  double hi, lo;
-};
-
-__device__ __forceinline__ DD dd_add(DD a, DD b) {
-  const double s = a.hi + b.hi;
-  const double bb = s - a.hi;
-  const double err = (a.hi - (s - bb)) + (b.hi - bb);
-  const double e = err + (a.lo + b.lo);
-  const double hi = s + e;
-  return DD{hi, e - (hi - s)};
-}
-
-__device__ __forceinline__ DD dd_of(double v) { return DD{v, 0.0}; }
 
 __global__ __launch_bounds__(256) void k_cfl_lf_fast_partial(const StructInfo* __restrict__ si, const float4* __restrict__ lf_s,
                                                              CflLfScratch* __restrict__ sc) {
@@ -555,52 +642,25 @@ __global__ __launch_bounds__(256) void k_cfl_lf_fast_partial(const StructInfo* _
   if (threadIdx.x < 7) sc->part[blockIdx.x][threadIdx.x] = make_double2(red[0][threadIdx.x].hi, red[0][threadIdx.x].lo);
 }
 
-// best_factor_dev's seed from the exact sums, or INT_MIN when some sum within
-// `gam` (relative to its absolute sum) of the exact one could change it.
-__device__ int certified_seed(double syy, double syc, double scc, double ayc, float base, double gam) {
-  if (syy <= 0.0) return 0;  // non-negative terms: zero exactly when every term is zero
-  const double u = 1.1102230246251565e-16;
-  const double dyy = gam * syy, dyc = gam * ayc, dcc = gam * scc;
-  if (dyy >= 0.5 * syy) return INT_MIN;
-  const double k = syc / syy;
-  const double dk = (dyc + fabs(k) * dyy) / (syy - dyy) + 8.0 * u * fabs(k) + 1e-300;
-  const double star = (k - (double)base) * 84.0;
-  const double dstar = 84.0 * dk + 8.0 * u * (fabs(star) + 84.0 * fabs((double)base)) + 1e-300;
-  const double fr = fabs(star) - floor(fabs(star));
-  if (fabs(fr - 0.5) <= 2.0 * dstar) return INT_MIN;
-  const int seed = best_factor_dev(syy, syc, scc, base);
-  // the argmin among the candidates must hold under the perturbation
-  const double r = round(star);
-  const long long centre = r >= 9.2e18 ? LLONG_MAX : (r <= -9.2e18 ? LLONG_MIN : (long long)r);
-  int cand[6];
-  for (int i = 0; i < 6; i++) {
-    if (i < 5) {
-      long long c = centre + (i - 2);
-      cand[i] = (int)(c < -128 ? -128 : (c > 127 ? 127 : c));
-    } else {
-      cand[i] = 0;
-    }
-  }
-  const double kb = (double)cfl_k(base, seed);
-  const double eb = scc - 2.0 * kb * syc + kb * kb * syy;
-  const double bb = dcc + 2.0 * fabs(kb) * dyc + kb * kb * dyy + 12.0 * u * (scc + 2.0 * fabs(kb) * fabs(syc) + kb * kb * syy);
-  for (int i = 0; i < 6; i++) {
-    if (cand[i] == seed) continue;
-    const double kj = (double)cfl_k(base, cand[i]);
-    const double ej = scc - 2.0 * kj * syc + kj * kj * syy;
-    const double bj = dcc + 2.0 * fabs(kj) * dyc + kj * kj * dyy + 12.0 * u * (scc + 2.0 * fabs(kj) * fabs(syc) + kj * kj * syy);
-    if (!(ej - eb > 2.0 * (bj + bb))) return INT_MIN;
-  }
-  return seed;
-}
 
-__global__ void k_cfl_lf_fast_final(const StructInfo* __restrict__ si, int groups, CflLfScratch* __restrict__ sc) {
+__global__ __launch_bounds__(256) void k_cfl_lf_fast_final(const StructInfo* __restrict__ si, int groups,
+                                                            CflLfScratch* __restrict__ sc) {
   const uint32_t count = si->lf_total;
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  // the per-group double-double partials, reduced by a tree (any order: the
+  // bound below covers the double-double's own error)
+  __shared__ DD red[256][7];
+  for (int k = 0; k < 7; k++)
+    red[threadIdx.x][k] = (int)threadIdx.x < groups ? DD{sc->part[threadIdx.x][k].x, sc->part[threadIdx.x][k].y}
+                                                    : DD{0.0, 0.0};
+  __syncthreads();
+  for (int stride = 128; stride > 0; stride >>= 1) {
+    if ((int)threadIdx.x < stride)
+      for (int k = 0; k < 7; k++) red[threadIdx.x][k] = dd_add(red[threadIdx.x][k], red[threadIdx.x + stride][k]);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
   DD tot[7];
-  for (int k = 0; k < 7; k++) tot[k] = DD{0.0, 0.0};
-  for (int g = 0; g < groups; g++)
-    for (int k = 0; k < 7; k++) tot[k] = dd_add(tot[k], DD{sc->part[g][k].x, sc->part[g][k].y});
+  for (int k = 0; k < 7; k++) tot[k] = red[0][k];
   double S[7];
   for (int k = 0; k < 7; k++) S[k] = tot[k].hi + tot[k].lo;
   // recursive-summation bound for n terms, with room for the double-double
@@ -952,6 +1012,29 @@ __global__ __launch_bounds__(1024) void k_struct_tile_base(int tiles, const uint
   }
 }
 
+// The structure's readback in one contiguous block: counts, LF factors, the
+// error word, the tile factors and the block map (layout of
+// structure_readback_bytes).
+__global__ __launch_bounds__(256) void k_gather_structure(const StructInfo* __restrict__ si,
+                                                          const CflParams* __restrict__ cp,
+                                                          const uint32_t* __restrict__ err,
+                                                          const int8_t* __restrict__ cx,
+                                                          const int8_t* __restrict__ cb,
+                                                          const uint8_t* __restrict__ bmap, int tiles, int bpx,
+                                                          char* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, step = gridDim.x * blockDim.x;
+  if (i == 0) {
+    *(StructInfo*)out = *si;
+    *(CflParams*)(out + 32) = *cp;
+    *(uint32_t*)(out + 48) = *err;
+  }
+  for (int k = i; k < tiles; k += step) {
+    out[64 + k] = (char)cx[k];
+    out[64 + tiles + k] = (char)cb[k];
+  }
+  for (int k = i; k < bpx; k += step) out[64 + 2 * tiles + k] = (char)bmap[k];
+}
+
 }  // namespace
 
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
@@ -1002,7 +1085,7 @@ void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, const StructInfo* si, c
                    CflLfScratch* scratch, CflParams* cflp, const int* gray) {
   // fixed grids; every kernel reads the sample count from si
   k_cfl_lf_fast_partial<<<kCflLfFastGroups, 256, 0, s>>>(si, lf_s, scratch);
-  k_cfl_lf_fast_final<<<1, 64, 0, s>>>(si, kCflLfFastGroups, scratch);
+  k_cfl_lf_fast_final<<<1, 256, 0, s>>>(si, kCflLfFastGroups, scratch);
   k_cfl_lf_seed<<<1, kCflThreads, 0, s>>>(si, lf_s, scratch);
   k_cfl_lf_price<<<256, 256, 0, s>>>(si, pp, lf_s, scratch);
   k_cfl_lf_select<<<1, 64, 0, s>>>(scratch, cflp, gray);
@@ -1022,6 +1105,15 @@ void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, u
   if (counts[0]) k_aq_block_muls<8><<<div_up(counts[0], 256), 256, 0, s>>>(g, pp, list[0], counts[0], muls);
   if (counts[1]) k_aq_block_muls<16><<<div_up(counts[1], 256), 256, 0, s>>>(g, pp, list[1], counts[1], muls);
   if (counts[2]) k_aq_block_muls<32><<<div_up(counts[2], 256), 256, 0, s>>>(g, pp, list[2], counts[2], muls);
+}
+
+size_t structure_readback_bytes(int tiles, int bpx) { return 64 + 2 * (size_t)tiles + (size_t)bpx; }
+
+void launch_gather_structure(hipStream_t s, const StructInfo* si, const CflParams* cp, const uint32_t* err,
+                             const int8_t* cx, const int8_t* cb, const uint8_t* bmap, int tiles, int bpx,
+                             char* out) {
+  const int n = std::max(tiles, bpx);
+  k_gather_structure<<<std::min((int)div_up(n, 256), 1024), 256, 0, s>>>(si, cp, err, cx, cb, bmap, tiles, bpx, out);
 }
 
 }  // namespace jpxlg
