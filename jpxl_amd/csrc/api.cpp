@@ -43,6 +43,9 @@ struct jpxlg_ctx {
   hipEvent_t fork = nullptr, join = nullptr;
   bool prof = false;
   std::string prof_only;  // when non-empty, only launches of this name are timed
+  // Self-check of the certified block-map search: every region through the
+  // exact pass (JPXLG_COVER_ALL_EXACT=1 at context creation).
+  int cover_all_exact = 0;
   std::vector<ProfRec> recs;
   std::vector<hipEvent_t> pool;
   std::map<std::string, std::pair<double, uint64_t>> totals;
@@ -223,7 +226,8 @@ size_t image_bytes(int w, int h) {
   b += 3 * cpx * 4 * 2;              // coefficients f32 + quantized i32
   b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
   b += 3 * bw * bh * 4;              // work lists
-  b += 3 * bw * bh * 8;              // cover candidate costs (f64)
+  b += 6 * bw * bh * 8;              // cover candidate costs (f64) and their bounds
+  b += (((bw + 3) / 4) * ((bh + 3) / 4) + 1) * 4;  // cover's exact-pass queue
   b += cpx * 18 + bw * bh * (16 + 3 * 16) + 2 * ((bw + 7) / 8 + 1) * ((bh + 7) / 8 + 1) * 4;  // CfL samples
   b += 6 * px * 4;                   // render planes + ping-pong
   b += 3 * pyr * 4;                  // candidate XYB' pyramid
@@ -275,6 +279,8 @@ struct jpxlg_img {
   CflLfScratch* d_lfc = nullptr;
   double* d_sums = nullptr;
   double* d_cost[3] = {nullptr, nullptr, nullptr};
+  double* d_rad[3] = {nullptr, nullptr, nullptr};  // bounds of the certified cover costs
+  uint32_t* d_cover_queue = nullptr;               // regions the exact cover pass redoes
   float4* d_lf_s = nullptr;
   float4* d_hf_s = nullptr;
   uint16_t* d_hf_cell = nullptr;
@@ -322,6 +328,10 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n <= device || device < 0) return JPXLG_E_DEVICE;
   std::unique_ptr<jpxlg_ctx> ctx(new jpxlg_ctx());
+  {
+    const char* e = getenv("JPXLG_COVER_ALL_EXACT");
+    ctx->cover_all_exact = e && e[0] == '1';
+  }
   ctx->device = device;
   HIP_TRY(ctx.get(), hipSetDevice(device));
   HIP_TRY(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -472,6 +482,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   }
   img->sigma = (float*)A.take(bpx * 4);
   for (int i = 0; i < 3; i++) img->d_cost[i] = (double*)A.take(bpx * 8);
+  for (int i = 0; i < 3; i++) img->d_rad[i] = (double*)A.take(bpx * 8);
+  img->d_cover_queue = (uint32_t*)A.take((size_t)(((g.bw + 3) / 4) * ((g.bh + 3) / 4) + 1) * 4);
   img->d_lf_s = (float4*)A.take(bpx * 16);
   img->d_hf_s = (float4*)A.take(cpx * 16);
   img->d_hf_cell = (uint16_t*)A.take(cpx * 2);
@@ -496,7 +508,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     ptiles += (size_t)std::max(pool_tiles(img->sw[s], img->sh[s]), pool_units(img->sw[s], img->sh[s]));
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
-  bool carved = img->d_cost[2] && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
+  bool carved = img->d_cost[2] && img->d_rad[2] && img->d_cover_queue && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
                 img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
                 img->aq_adj && img->d_aq;
@@ -907,8 +919,10 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
     rc = upload_small(ctx, st, img->d_pp, &pp, sizeof(pp));
     if (rc) return rc;
     Prof p(ctx, "cover");
+    HIP_TRY(ctx, hipMemsetAsync(img->d_cover_queue, 0, 4, st));
     launch_cover(st, g, ctx->d_tab, img->d_pp, img->xyb, img->d_cost[0], img->d_cost[1], img->d_cost[2],
-                 img->d_bmap, img->d_err);
+                 img->d_rad[0], img->d_rad[1], img->d_rad[2], img->d_cover_queue, img->d_bmap, img->d_err,
+                 ctx->cover_all_exact);
   } else {
     HIP_TRY(ctx, hipMemsetAsync(img->d_bmap, 8, bpx, st));  // every block its own DCT8
   }

@@ -271,9 +271,13 @@ void launch_blur_moments_all(hipStream_t s, const ConstTables* tab, int ns, cons
 // Hierarchical cover (kernels_plan.hip): candidate costs for every square
 // (f64, indexed by the candidate's top-left block) and the per-region
 // quadtree decision written into the block map.
+// Costs are double-double sums with a bound (rad*); regions the bounds
+// cannot settle are queued (queue[0] = count, zeroed by the caller) and
+// recomputed with the reference's sequential sums (every region when
+// all_exact, the self-check mode).
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
-                  float* const xyb[3], double* cost8, double* cost16, double* cost32, uint8_t* bmap,
-                  uint32_t* err);
+                  float* const xyb[3], double* cost8, double* cost16, double* cost32, double* rad8,
+                  double* rad16, double* rad32, uint32_t* queue, uint8_t* bmap, uint32_t* err, int all_exact);
 // The structure's work lists (in CfL order), counts, and each varblock's CfL
 // sample slots (LF offset per block; HF offset inside its tile, per block;
 // tile bases), from the device block map.

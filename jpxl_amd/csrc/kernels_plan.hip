@@ -67,29 +67,42 @@ __device__ __forceinline__ void fwd_line_pass1(float* tile, int col, const DctTa
   for (int r = 0; r < N; r++) tile[r * P + col] = v[r] * f;
 }
 
-template <int N>
-__global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTables* __restrict__ tab,
-                                                         const ProbeParams* __restrict__ pp,
-                                                         const float* __restrict__ X,
-                                                         const float* __restrict__ Y,
-                                                         const float* __restrict__ B,
-                                                         double* __restrict__ cost,
-                                                         uint32_t* __restrict__ err) {
+struct DD {
+  double hi, lo;
+};
+
+__device__ __forceinline__ DD dd_add(DD a, DD b) {
+  const double s = a.hi + b.hi;
+  const double bb = s - a.hi;
+  const double err = (a.hi - (s - bb)) + (b.hi - bb);
+  const double e = err + (a.lo + b.lo);
+  const double hi = s + e;
+  return DD{hi, e - (hi - s)};
+}
+
+__device__ __forceinline__ DD dd_of(double v) { return DD{v, 0.0}; }
+
+// The cost J of up to VB square candidates of side N whose top-left blocks
+// are vbx/vby (LDS, set by the caller; valid[j] = 0 for an unused slot).
+// EXACT: the f64 distortion sum is sequential in the reference's order (Y,
+// X, B, cells in raster order; one lane per candidate).  Otherwise the sum is
+// formed exactly as a double-double in any order and rad[] receives a bound
+// on |sequential - exact| (the terms are non-negative, so gamma_(n) * S),
+// for decisions that hold for every sum within it.
+template <int N, bool EXACT>
+__device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, const ProbeParams* __restrict__ pp,
+                                 const float* __restrict__ X, const float* __restrict__ Y,
+                                 const float* __restrict__ B, const int* vbx, const int* vby, const int* valid,
+                                 double* __restrict__ cost, double* __restrict__ rad, uint32_t* __restrict__ err) {
   constexpr int VB = 64 / N, P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  constexpr int G = kThreads / VB;  // threads per candidate for the double-double sums
   __shared__ float tile[VB * 3 * N * P];
   __shared__ float dyh[VB][NN];
   __shared__ double terms[VB][NN];
   __shared__ uint32_t bits[VB];
   __shared__ double sse[VB];
-  __shared__ int vbx[VB], vby[VB], valid[VB];
   __shared__ float vsc[VB][3];  // the candidate's step scales at its footprint's HfMul
-  const int ncx = g.bw / n, ncy = g.bh / n, count = ncx * ncy;
-  const int base = blockIdx.x * VB;
   if (threadIdx.x < VB) {
-    int i = base + threadIdx.x;
-    valid[threadIdx.x] = i < count;
-    vbx[threadIdx.x] = i < count ? (i % ncx) * n : 0;
-    vby[threadIdx.x] = i < count ? (i / ncx) * n : 0;
     bits[threadIdx.x] = 0;
     sse[threadIdx.x] = 0.0;
     varblock_scales(pp, g.bw, g.bh, vbx[threadIdx.x], vby[threadIdx.x], n, vsc[threadIdx.x], nullptr);
@@ -122,6 +135,7 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
   __syncthreads();
   int bad = 0;
   const double tsd = (double)(N * N) * 1.0;
+  DD part{0.0, 0.0};  // this thread's share of its candidate's distortion (double-double mode)
   for (int ci = 0; ci < 3; ci++) {
     const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);  // Y, then X, then B
     const float bias = kBias[c];
@@ -147,66 +161,133 @@ __global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTabl
       if (rb) atomicAdd(&bits[j], rb);
     }
     __syncthreads();
-    // Sequential f64 sum in cell order, one thread per candidate.
-    // The LLF cells hold +0.0, which leaves the (non-negative) sum unchanged.
-    if (threadIdx.x < VB) sse[threadIdx.x] = sequential_sum(terms[threadIdx.x], NN, sse[threadIdx.x]);
+    if constexpr (EXACT) {
+      // Sequential f64 sum in cell order, one thread per candidate.  The LLF
+      // cells hold +0.0, which leaves the (non-negative) sum unchanged.
+      if (threadIdx.x < VB) sse[threadIdx.x] = sequential_sum(terms[threadIdx.x], NN, sse[threadIdx.x]);
+    } else {
+      const int j = threadIdx.x / G, r = threadIdx.x % G;
+      for (int k = r; k < NN; k += G) part = dd_add(part, dd_of(terms[j][k]));
+    }
     __syncthreads();
   }
-  if (threadIdx.x < VB && valid[threadIdx.x]) {
-    const int j = threadIdx.x;
-    cost[(size_t)vby[j] * g.bw + vbx[j]] = (double)bits[j] * rate_scale(N) + sse[j];
+  if constexpr (EXACT) {
+    if (threadIdx.x < VB && valid[threadIdx.x]) {
+      const int j = threadIdx.x;
+      cost[(size_t)vby[j] * g.bw + vbx[j]] = (double)bits[j] * rate_scale(N) + sse[j];
+    }
+  } else {
+    DD* red = reinterpret_cast<DD*>(&terms[0][0]);  // kThreads double-doubles fit in terms
+    red[threadIdx.x] = part;
+    __syncthreads();
+    if (threadIdx.x % G == 0) {
+      const int j = threadIdx.x / G;
+      DD tot = red[threadIdx.x];
+      for (int r = 1; r < G; r++) tot = dd_add(tot, red[threadIdx.x + r]);
+      if (valid[j]) {
+        const double S = tot.hi + tot.lo;
+        const double nterms = 3.0 * NN + 4.0, u = 1.1102230246251565e-16;
+        const double gam = 2.0 * (nterms * u / (1.0 - nterms * u));
+        const double c = (double)bits[j] * rate_scale(N) + S;
+        const size_t o = (size_t)vby[j] * g.bw + vbx[j];
+        cost[o] = c;
+        // the sequential sum's distance from S, plus the roundings of S and c
+        rad[o] = gam * S + 4.0 * u * fabs(c) + 1e-300;
+      }
+    }
   }
   if (bad) atomicOr(err, kErrQuantRange);
 }
 
-// The quadtree of tile_region_with over one 32x32 region (4x4 blocks).
-__global__ __launch_bounds__(256) void k_cover_decide(Geom g, const ProbeParams* __restrict__ pp,
-                                                      const double* __restrict__ c8,
-                                                      const double* __restrict__ c16,
-                                                      const double* __restrict__ c32,
-                                                      uint8_t* __restrict__ bmap) {
-  const int rx = blockIdx.x * blockDim.x + threadIdx.x, ry = blockIdx.y;
-  const int nrx = (g.bw + 3) / 4;
-  if (rx >= nrx) return;
-  const int bx0 = 4 * rx, by0 = 4 * ry;
+template <int N>
+__global__ __launch_bounds__(kThreads) void k_cover_cost(Geom g, const ConstTables* __restrict__ tab,
+                                                         const ProbeParams* __restrict__ pp,
+                                                         const float* __restrict__ X,
+                                                         const float* __restrict__ Y,
+                                                         const float* __restrict__ B,
+                                                         double* __restrict__ cost, double* __restrict__ rad,
+                                                         uint32_t* __restrict__ err) {
+  constexpr int VB = 64 / N, n = N / 8;
+  __shared__ int vbx[VB], vby[VB], valid[VB];
+  const int ncx = g.bw / n, ncy = g.bh / n, count = ncx * ncy;
+  const int base = blockIdx.x * VB;
+  if (threadIdx.x < VB) {
+    int i = base + threadIdx.x;
+    valid[threadIdx.x] = i < count;
+    vbx[threadIdx.x] = i < count ? (i % ncx) * n : 0;
+    vby[threadIdx.x] = i < count ? (i / ncx) * n : 0;
+  }
+  __syncthreads();
+  cover_cost_group<N, false>(g, tab, pp, X, Y, B, vbx, vby, valid, cost, rad, err);
+}
+
+// An interval [mid - rad, mid + rad] that holds the value the reference's
+// f64 arithmetic computes.
+struct Ival {
+  double mid, rad;
+};
+__device__ __forceinline__ Ival iv_add(Ival a, Ival b) {
+  const double m = a.mid + b.mid;
+  const double r = a.rad + b.rad;
+  return Ival{m, r + 2.3e-16 * (fabs(m) + r)};
+}
+__device__ __forceinline__ Ival iv_const(double v) { return Ival{v, 0.0}; }
+
+// The quadtree of tile_region_with over one 32x32 region (4x4 blocks) at
+// (bx0, by0).  With rad arrays (certified costs) it returns false, writing
+// nothing, when some comparison could go either way within the costs'
+// bounds; with rad null (exact costs) it always decides.
+__device__ bool decide_region(Geom g, const ProbeParams* __restrict__ pp, const double* __restrict__ c8,
+                              const double* __restrict__ c16, const double* __restrict__ c32,
+                              const double* __restrict__ r8, const double* __restrict__ r16,
+                              const double* __restrict__ r32, uint8_t* __restrict__ bmap, int bx0, int by0) {
   const int bw = g.bw, bh = g.bh;
+  const bool cert = r8 != nullptr;
   uint8_t dec[4][4];
 #pragma unroll
   for (int y = 0; y < 4; y++)
 #pragma unroll
     for (int x = 0; x < 4; x++) dec[y][x] = 0;
-  double split4 = 0.0;
+  Ival split4 = iv_const(0.0);
   bool q16[2][2] = {{false, false}, {false, false}};
   const int qo[4][2] = {{0, 0}, {2, 0}, {0, 2}, {2, 2}};
   for (int qi = 0; qi < 4; qi++) {
     const int qx = bx0 + qo[qi][0], qy = by0 + qo[qi][1];
     if (qx >= bw || qy >= bh) continue;  // contributes 0
-    double split2 = 0.0;
+    Ival split2 = iv_const(0.0);
     const int so[4][2] = {{0, 0}, {1, 0}, {0, 1}, {1, 1}};
-    for (int si = 0; si < 4; si++) {
-      const int sx = qx + so[si][0], sy = qy + so[si][1];
+    for (int s4 = 0; s4 < 4; s4++) {
+      const int sx = qx + so[s4][0], sy = qy + so[s4][1];
       if (sx >= bw || sy >= bh) continue;
-      split2 += (c8[(size_t)sy * bw + sx] + fixed_bits(8)) + aq_signal_bits(pp, bw, bh, sx, sy, 1);
+      const size_t o = (size_t)sy * bw + sx;
+      const Ival c = Ival{c8[o], cert ? r8[o] : 0.0};
+      split2 = iv_add(split2, iv_add(iv_add(c, iv_const(fixed_bits(8))), iv_const(aq_signal_bits(pp, bw, bh, sx, sy, 1))));
     }
-    double best2 = split2;
+    Ival best2 = split2;
     if (qx + 2 <= bw && qy + 2 <= bh) {
       const double fixed = fixed_bits(16) + aq_signal_bits(pp, bw, bh, qx, qy, 2);
-      if (!(fixed >= split2)) {
-        double single = c16[(size_t)qy * bw + qx] + fixed;
-        if (single < split2) {
+      if (cert && fabs(fixed - split2.mid) <= split2.rad) return false;
+      if (!(fixed >= split2.mid)) {
+        const size_t o = (size_t)qy * bw + qx;
+        const Ival single = iv_add(Ival{c16[o], cert ? r16[o] : 0.0}, iv_const(fixed));
+        if (cert && fabs(single.mid - split2.mid) <= single.rad + split2.rad) return false;
+        if (single.mid < split2.mid) {
           best2 = single;
           q16[qo[qi][1] / 2][qo[qi][0] / 2] = true;
         }
       }
     }
-    split4 += best2;
+    split4 = iv_add(split4, best2);
   }
   bool use32 = false;
   if (bx0 + 4 <= bw && by0 + 4 <= bh) {
     const double fixed = fixed_bits(32) + aq_signal_bits(pp, bw, bh, bx0, by0, 4);
-    if (!(fixed >= split4)) {
-      double single = c32[(size_t)by0 * bw + bx0] + fixed;
-      if (single < split4) use32 = true;
+    if (cert && fabs(fixed - split4.mid) <= split4.rad) return false;
+    if (!(fixed >= split4.mid)) {
+      const size_t o = (size_t)by0 * bw + bx0;
+      const Ival single = iv_add(Ival{c32[o], cert ? r32[o] : 0.0}, iv_const(fixed));
+      if (cert && fabs(single.mid - split4.mid) <= single.rad + split4.rad) return false;
+      if (single.mid < split4.mid) use32 = true;
     }
   }
   if (use32) {
@@ -227,6 +308,80 @@ __global__ __launch_bounds__(256) void k_cover_decide(Geom g, const ProbeParams*
   for (int y = 0; y < 4; y++)
     for (int x = 0; x < 4; x++)
       if (bx0 + x < bw && by0 + y < bh) bmap[(size_t)(by0 + y) * bw + bx0 + x] = dec[y][x];
+  return true;
+}
+
+// One thread per 32x32 region; regions whose decision the certified costs
+// cannot settle are queued for the exact pass.
+__global__ __launch_bounds__(256) void k_cover_decide(Geom g, const ProbeParams* __restrict__ pp,
+                                                      const double* __restrict__ c8,
+                                                      const double* __restrict__ c16,
+                                                      const double* __restrict__ c32,
+                                                      const double* __restrict__ r8,
+                                                      const double* __restrict__ r16,
+                                                      const double* __restrict__ r32,
+                                                      uint8_t* __restrict__ bmap, uint32_t* __restrict__ queue,
+                                                      int all_exact) {
+  const int rx = blockIdx.x * blockDim.x + threadIdx.x, ry = blockIdx.y;
+  const int nrx = (g.bw + 3) / 4;
+  if (rx >= nrx) return;
+  if (all_exact || !decide_region(g, pp, c8, c16, c32, r8, r16, r32, bmap, 4 * rx, 4 * ry)) {
+    const uint32_t slot = atomicAdd(&queue[0], 1u);
+    queue[1 + slot] = ((uint32_t)ry << 16) | (uint32_t)rx;
+  }
+}
+
+// The exact pass over the queued regions: every candidate cost of the
+// region with the reference's sequential sums, then its decision.  A fixed
+// grid walks the queue (usually empty).
+__global__ __launch_bounds__(kThreads) void k_cover_exact(Geom g, const ConstTables* __restrict__ tab,
+                                                          const ProbeParams* __restrict__ pp,
+                                                          const float* __restrict__ X, const float* __restrict__ Y,
+                                                          const float* __restrict__ B, double* __restrict__ c8,
+                                                          double* __restrict__ c16, double* __restrict__ c32,
+                                                          uint8_t* __restrict__ bmap,
+                                                          const uint32_t* __restrict__ queue,
+                                                          uint32_t* __restrict__ err) {
+  __shared__ int vbx[8], vby[8], valid[8];
+  const uint32_t nq = queue[0];
+  for (uint32_t qi = blockIdx.x; qi < nq; qi += gridDim.x) {
+    const uint32_t e = queue[1 + qi];
+    const int bx0 = 4 * (int)(e & 0xffffu), by0 = 4 * (int)(e >> 16);
+    // DCT8 candidates: two groups of eight blocks
+    for (int half = 0; half < 2; half++) {
+      __syncthreads();
+      if (threadIdx.x < 8) {
+        const int bx = bx0 + threadIdx.x % 4, by = by0 + 2 * half + threadIdx.x / 4;
+        valid[threadIdx.x] = bx < g.bw && by < g.bh;
+        vbx[threadIdx.x] = min(bx, g.bw - 1);
+        vby[threadIdx.x] = min(by, g.bh - 1);
+      }
+      __syncthreads();
+      cover_cost_group<8, true>(g, tab, pp, X, Y, B, vbx, vby, valid, c8, nullptr, err);
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+      const int bx = bx0 + 2 * (threadIdx.x % 2), by = by0 + 2 * (threadIdx.x / 2);
+      valid[threadIdx.x] = bx + 2 <= g.bw && by + 2 <= g.bh;
+      vbx[threadIdx.x] = valid[threadIdx.x] ? bx : 0;
+      vby[threadIdx.x] = valid[threadIdx.x] ? by : 0;
+    }
+    __syncthreads();
+    cover_cost_group<16, true>(g, tab, pp, X, Y, B, vbx, vby, valid, c16, nullptr, err);
+    __syncthreads();
+    if (threadIdx.x < 2) {
+      valid[threadIdx.x] = threadIdx.x == 0 && bx0 + 4 <= g.bw && by0 + 4 <= g.bh;
+      vbx[threadIdx.x] = valid[threadIdx.x] ? bx0 : 0;
+      vby[threadIdx.x] = valid[threadIdx.x] ? by0 : 0;
+    }
+    __syncthreads();
+    cover_cost_group<32, true>(g, tab, pp, X, Y, B, vbx, vby, valid, c32, nullptr, err);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      __threadfence_block();
+      decide_region(g, pp, c8, c16, c32, nullptr, nullptr, nullptr, bmap, bx0, by0);
+    }
+  }
 }
 
 // ----------------------------------------------------------------------------
@@ -433,20 +588,6 @@ __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0
   return CflSums{total[0], total[1], total[2], total[3], total[4]};
 }
 
-struct DD {
-  double hi, lo;
-};
-
-__device__ __forceinline__ DD dd_add(DD a, DD b) {
-  const double s = a.hi + b.hi;
-  const double bb = s - a.hi;
-  const double err = (a.hi - (s - bb)) + (b.hi - bb);
-  const double e = err + (a.lo + b.lo);
-  const double hi = s + e;
-  return DD{hi, e - (hi - s)};
-}
-
-__device__ __forceinline__ DD dd_of(double v) { return DD{v, 0.0}; }
 
 // best_factor_dev's seed from the exact sums, or INT_MIN when some sum within
 // `gam` (relative to its absolute sum) of the exact one could change it.
@@ -1038,16 +1179,17 @@ __global__ __launch_bounds__(256) void k_gather_structure(const StructInfo* __re
 }  // namespace
 
 void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const ProbeParams* pp,
-                  float* const xyb[3], double* cost8, double* cost16, double* cost32, uint8_t* bmap,
-                  uint32_t* err) {
+                  float* const xyb[3], double* cost8, double* cost16, double* cost32, double* rad8,
+                  double* rad16, double* rad32, uint32_t* queue, uint8_t* bmap, uint32_t* err, int all_exact) {
   int c8 = g.bw * g.bh, c16 = (g.bw / 2) * (g.bh / 2), c32 = (g.bw / 4) * (g.bh / 4);
-  k_cover_cost<8><<<div_up(c8, 8), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost8, err);
+  k_cover_cost<8><<<div_up(c8, 8), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost8, rad8, err);
   if (c16 > 0)
-    k_cover_cost<16><<<div_up(c16, 4), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost16, err);
+    k_cover_cost<16><<<div_up(c16, 4), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost16, rad16, err);
   if (c32 > 0)
-    k_cover_cost<32><<<div_up(c32, 2), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost32, err);
+    k_cover_cost<32><<<div_up(c32, 2), kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost32, rad32, err);
   dim3 grid(div_up((g.bw + 3) / 4, 256), (g.bh + 3) / 4);
-  k_cover_decide<<<grid, 256, 0, s>>>(g, pp, cost8, cost16, cost32, bmap);
+  k_cover_decide<<<grid, 256, 0, s>>>(g, pp, cost8, cost16, cost32, rad8, rad16, rad32, bmap, queue, all_exact);
+  k_cover_exact<<<64, kThreads, 0, s>>>(g, tab, pp, xyb[0], xyb[1], xyb[2], cost8, cost16, cost32, bmap, queue, err);
 }
 
 }  // namespace jpxlg

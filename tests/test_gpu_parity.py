@@ -200,6 +200,43 @@ def test_cover_block_map_matches_the_oracle(ctx, wh, gs):
     assert set(np.unique(want)) >= {0, 8}  # the synthetic image merges some blocks
 
 
+def _flat_and_busy(w, h, seed):
+    # left half one grey level (every candidate ties on rate alone), right
+    # half synthetic detail
+    rgb = _rgb(w, h, seed)
+    rgb[:, : w // 2] = 128
+    return rgb
+
+
+@pytest.mark.parametrize("kind", ["synthetic", "flat_and_busy"])
+@pytest.mark.parametrize("wh,gs", [((512, 384), 1500), ((257, 131), 700)])
+def test_certified_cover_equals_the_exact_pass(ctx, wh, gs, kind, monkeypatch):
+    # The certified double-double costs decide most regions; the exact pass
+    # (the reference's sequential sums) redoes the rest.  Forcing every
+    # region through the exact pass must give the same block map.
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    w, h = wh
+    rgb = _flat_and_busy(w, h, 5) if kind == "flat_and_busy" else _rgb(w, h, 5)
+    q = _quantizer(gs=gs)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    img.plan_structure(q, _lib.COVER_HIERARCHICAL, False)
+    fast = img.plan_download_map()
+    monkeypatch.setenv("JPXLG_COVER_ALL_EXACT", "1")
+    ex_ctx = jpxl_amd.Context(0)
+    try:
+        img2 = jpxl_amd.Image(ex_ctx, rgb, 8)
+        img2.plan_structure(q, _lib.COVER_HIERARCHICAL, False)
+        exact = img2.plan_download_map()
+    finally:
+        ex_ctx.close()
+    assert np.array_equal(fast, exact)
+    if w <= 260:
+        xyb, _, _ = orc.prepare_frame(rgb, 8)
+        assert np.array_equal(fast, orc.cover(xyb, _qp(q)))
+
+
 def _tinted(w, h, seed):
     g = _rgb(w, h, seed)[..., 1].astype(np.float32)
     return np.stack([g * 0.6, g, np.clip(g * 0.3 + 60, 0, 255)], -1).astype(np.uint8)
