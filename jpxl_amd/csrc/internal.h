@@ -73,13 +73,24 @@ __host__ __device__ inline uint32_t aq_mul_of(uint32_t base, float adj) {
 // f32 mean of the adjustments (rows, then columns; zero outside the grid),
 // snapped to half octaves (legacy) or clamped to one octave (fine), capped
 // at zero under refine-only.
+// One sample of a per-block field in device memory (global loads on the
+// device: the pointer comes out of a parameter block, where the compiler
+// cannot see its address space).
+__host__ __device__ inline float field_at(const float* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(1))) float*)((const __attribute__((address_space(1))) char*)p + 4u * i);
+#else
+  return p[i];
+#endif
+}
+
 __host__ __device__ inline uint32_t aq_footprint_mul(const ProbeParams* pp, int bw, int bh, int bx, int by, int n) {
   const float* adj = pp->aq_adj;
   float sum = 0.0f;
   for (int dy = 0; dy < n; dy++)
     for (int dx = 0; dx < n; dx++) {
       const int x = bx + dx, y = by + dy;
-      sum += (x < bw && y < bh) ? adj[(size_t)y * bw + x] : 0.0f;
+      sum += (x < bw && y < bh) ? field_at(adj, (uint32_t)(y * bw + x)) : 0.0f;
     }
   float a = sum / (float)(n * n);
   if (pp->aq_fine) a = a < -1.0f ? -1.0f : (a > 1.0f ? 1.0f : a);
@@ -186,6 +197,33 @@ struct ProbeDesc {
   const float* lut;
   int levels;
 };
+
+// The descriptors of a batched launch are written by the host before the
+// launch and never by a kernel.  Read through the constant address space,
+// every pointer they hold is known to point to global memory, so the
+// compiler emits global loads and stores for the planes rather than flat
+// ones (which would also count against the LDS counter: every LDS wait
+// would wait for them as well).
+using DescSpace = const __attribute__((address_space(4))) ProbeDesc;
+__device__ __forceinline__ DescSpace& desc_at(const ProbeDesc* D, int d) { return ((DescSpace*)D)[d]; }
+__device__ __forceinline__ Geom geom_of(const __attribute__((address_space(4))) Geom& s) {
+  Geom g;
+  g.w = s.w, g.h = s.h, g.bw = s.bw, g.bh = s.bh, g.cstride = s.cstride, g.crows = s.crows, g.tw = s.tw, g.th = s.th;
+  return g;
+}
+
+// Loads through a pointer the compiler cannot place, stated to be global
+// (the 32-bit element index keeps a wave-uniform base in scalar registers).
+template <class T>
+__device__ __forceinline__ T ldg32(const T* p, uint32_t i) {
+  return *(const __attribute__((address_space(1))) T*)((const __attribute__((address_space(1))) char*)p +
+                                                         i * (uint32_t)sizeof(T));
+}
+
+template <class T>
+__device__ __forceinline__ void stg32(T* p, uint32_t i, T v) {
+  *(__attribute__((address_space(1))) T*)((__attribute__((address_space(1))) char*)p + i * (uint32_t)sizeof(T)) = v;
+}
 
 // Workgroup prefix sums of one batched launch: the workgroups of desc d are
 // [off[d], off[d+1]).  The desc of a workgroup is found by binary search.

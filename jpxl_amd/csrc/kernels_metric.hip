@@ -16,11 +16,20 @@ namespace {
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// Depth tables of up to 256 levels are staged in LDS (TAB_LDS), larger ones
+// are read from global memory; the accessor tells the compiler which.
+template <bool TAB_LDS>
+__device__ __forceinline__ float tab_at(const float* p, int i) {
+  if constexpr (TAB_LDS) return ((const __attribute__((address_space(3))) float*)p)[i];
+  else return ldg32(p, (uint32_t)i);
+}
+
 // jpxl-plan-render/src/lib.rs:341-347 level_of and :421-454 linear_at_depth:
 // the level's linear value for one sample, `lut[#thresholds <= x]`.
+template <bool TAB_LDS>
 __device__ __forceinline__ float depth_value(float x, const float* thr, const float* lut,
                                              int levels) {
-  if (!isfinite(x)) return lut[0];
+  if (!isfinite(x)) return tab_at<TAB_LDS>(lut, 0);
   // A first guess from the device's own sRGB encode (within a level of the
   // truth), then corrected against the exact thresholds, so the result is
   // the count of thresholds <= x whatever the guess.
@@ -32,17 +41,17 @@ __device__ __forceinline__ float depth_value(float x, const float* thr, const fl
   const int g = gl <= 0.0f ? 0 : (gl >= max ? levels - 1 : (int)gl);
   // The two thresholds either side of the guess, loaded together; the walks
   // run only when the guess is off.
-  const float tl = g > 0 ? thr[g - 1] : -INFINITY;
-  const float th = g < levels - 1 ? thr[g] : INFINITY;
+  const float tl = g > 0 ? tab_at<TAB_LDS>(thr, g - 1) : -INFINITY;
+  const float th = g < levels - 1 ? tab_at<TAB_LDS>(thr, g) : INFINITY;
   int lo = g;
   if (tl > x) {
     lo = g - 1;
-    while (lo > 0 && thr[lo - 1] > x) lo--;
+    while (lo > 0 && tab_at<TAB_LDS>(thr, lo - 1) > x) lo--;
   } else if (th <= x) {
     lo = g + 1;
-    while (lo < levels - 1 && thr[lo] <= x) lo++;
+    while (lo < levels - 1 && tab_at<TAB_LDS>(thr, lo) <= x) lo++;
   }
-  return lut[lo];
+  return tab_at<TAB_LDS>(lut, lo);
 }
 
 // jpxl-perceptual/src/color.rs:66-81 linear_srgb_to_positive_xyb.
@@ -86,7 +95,7 @@ __device__ __forceinline__ void opsin_inverse(float x, float y, float b, float r
 
 // One thread per 2x2 quad: converts up to four samples of this scale and
 // writes the next scale's box-filtered linear RGB (pyramid.rs:19-44).
-template <int MODE>
+template <int MODE, bool TAB_LDS>
 __device__ __forceinline__ void colour_quad(int w, int h, const float* __restrict__ in0,
                                             const float* __restrict__ in1, const float* __restrict__ in2,
                                             const float* thr, const float* lut, int levels, float* __restrict__ p0,
@@ -107,7 +116,7 @@ __device__ __forceinline__ void colour_quad(int w, int h, const float* __restric
         float o[3];
         opsin_inverse(in0[i], in1[i], in2[i], root, o);
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = depth_value(o[c], thr, lut, levels);
+        for (int c = 0; c < 3; c++) v[c] = depth_value<TAB_LDS>(o[c], thr, lut, levels);
       } else {
         v[0] = in0[i]; v[1] = in1[i]; v[2] = in2[i];
       }
@@ -154,8 +163,12 @@ __global__ __launch_bounds__(256) void k_colour_metric(
   const float* thr = thr_g;
   const float* lut = lut_g;
   if constexpr (MODE == 0) stage_depth(thr_g, lut_g, levels, s_thr, s_lut, &thr, &lut);
-  colour_quad<MODE>(w, h, in0, in1, in2, thr, lut, levels, p0, p1, p2, d0, d1, d2,
-                    blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+  if (MODE == 0 && levels <= 256)
+    colour_quad<MODE, true>(w, h, in0, in1, in2, thr, lut, levels, p0, p1, p2, d0, d1, d2,
+                            blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
+  else
+    colour_quad<MODE, false>(w, h, in0, in1, in2, thr, lut, levels, p0, p1, p2, d0, d1, d2,
+                             blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // Batched colour pass of scale s: workgroup (quad row, 256-quad block) of
@@ -167,17 +180,24 @@ __global__ __launch_bounds__(256) void k_colour_b(const ProbeDesc* __restrict__ 
                                                   int nd, int s, int src) {
   __shared__ float s_thr[256], s_lut[256];
   const int d = find_desc(off, nd, blockIdx.x);
-  const ProbeDesc& P = D[d];
+  DescSpace& P = desc_at(D, d);
   const int w = P.sw[s], h = P.sh[s];
   const float* thr = P.thr;
   const float* lut = P.lut;
   if constexpr (MODE == 0) stage_depth(P.thr, P.lut, P.levels, s_thr, s_lut, &thr, &lut);
   const int lb = blockIdx.x - off[d], xb = ((w + 1) / 2 + 255) / 256;
-  const float* const* in = MODE == 0 ? (src ? P.aux : P.rx) : P.cand_rgb[s];
-  float* const* dn = s + 1 < P.nscales ? P.cand_rgb[s + 1] : nullptr;
-  colour_quad<MODE>(w, h, in[0], in[1], in[2], thr, lut, P.levels, P.cand_x[s][0], P.cand_x[s][1], P.cand_x[s][2],
-                    dn ? dn[0] : nullptr, dn ? dn[1] : nullptr, dn ? dn[2] : nullptr,
-                    (lb % xb) * 256 + threadIdx.x, lb / xb);
+  const float* in[3];
+  float* dn[3] = {nullptr, nullptr, nullptr};
+  for (int c = 0; c < 3; c++) {
+    in[c] = MODE == 0 ? (src ? P.aux[c] : P.rx[c]) : P.cand_rgb[s][c];
+    if (s + 1 < P.nscales) dn[c] = P.cand_rgb[s + 1][c];
+  }
+  if (MODE == 0 && P.levels <= 256)
+    colour_quad<MODE, true>(w, h, in[0], in[1], in[2], thr, lut, P.levels, P.cand_x[s][0], P.cand_x[s][1],
+                            P.cand_x[s][2], dn[0], dn[1], dn[2], (lb % xb) * 256 + threadIdx.x, lb / xb);
+  else
+    colour_quad<MODE, false>(w, h, in[0], in[1], in[2], thr, lut, P.levels, P.cand_x[s][0], P.cand_x[s][1],
+                             P.cand_x[s][2], dn[0], dn[1], dn[2], (lb % xb) * 256 + threadIdx.x, lb / xb);
 }
 
 // The colour pass of every scale in one workgroup: a 32x32 tile of scale 0
@@ -189,9 +209,12 @@ __global__ __launch_bounds__(256) void k_colour_b(const ProbeDesc* __restrict__ 
 // quads into the next.  The per-sample arithmetic, clamps and order are
 // colour_quad's.
 constexpr int kPyrTile = 32;
-template <int MODE>
-__device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const float* const* in,
-                                    float* const (*out)[3], const float* thr, const float* lut, int levels,
+// Sizes and output planes are read where they lie (a descriptor in constant
+// memory or the kernel's parameter block): IntP and OutP are their pointer
+// types.
+template <int MODE, bool TAB_LDS, class IntP, class OutP>
+__device__ void colour_pyramid_tile(int ns, IntP sw, IntP sh, const float* const* in, OutP out,
+                                    const float* thr, const float* lut, int levels,
                                     int tx0, int ty0, float (*a)[256], float (*b)[64]) {
   const float root = MODE == 0 ? cbrt_core(-0.0037930733f) : 0.0f;
   {
@@ -211,7 +234,7 @@ __device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const 
             float o[3];
             opsin_inverse(in[0][i], in[1][i], in[2][i], root, o);
 #pragma unroll
-            for (int c = 0; c < 3; c++) v[c] = depth_value(o[c], thr, lut, levels);
+            for (int c = 0; c < 3; c++) v[c] = depth_value<TAB_LDS>(o[c], thr, lut, levels);
           } else {
             v[0] = in[0][i]; v[1] = in[1][i]; v[2] = in[2][i];
           }
@@ -240,8 +263,10 @@ __device__ void colour_pyramid_tile(int ns, const int* sw, const int* sh, const 
       if (x >= w || y >= h) continue;
       float p0, p1, p2;
       positive_xyb(cur[0][p], cur[1][p], cur[2][p], &p0, &p1, &p2);
-      const size_t i = (size_t)y * w + x;
-      out[s][0][i] = p0; out[s][1][i] = p1; out[s][2][i] = p2;
+      const uint32_t i = (uint32_t)(y * w + x);
+      stg32(out[s][0], i, p0);
+      stg32(out[s][1], i, p1);
+      stg32(out[s][2], i, p2);
     }
     if (s + 1 >= ns) continue;
     // the next scale's tile (T/2 x T/2) from this scale's quads
@@ -267,13 +292,19 @@ __global__ __launch_bounds__(256) void k_colour_pyr_b(const ProbeDesc* __restric
   __shared__ float s_thr[256], s_lut[256];
   __shared__ float a[3][256], b[3][64];
   const int d = find_desc(off, nd, blockIdx.x);
-  const ProbeDesc& P = D[d];
+  DescSpace& P = desc_at(D, d);
   const float* thr = P.thr;
   const float* lut = P.lut;
   stage_depth(P.thr, P.lut, P.levels, s_thr, s_lut, &thr, &lut);
   const int lb = blockIdx.x - off[d], ntx = (P.sw[0] + kPyrTile - 1) / kPyrTile;
-  colour_pyramid_tile<0>(P.nscales, P.sw, P.sh, src ? P.aux : P.rx, P.cand_x, thr, lut, P.levels,
-                         (lb % ntx) * kPyrTile, (lb / ntx) * kPyrTile, a, b);
+  const float* in[3] = {src ? P.aux[0] : P.rx[0], src ? P.aux[1] : P.rx[1], src ? P.aux[2] : P.rx[2]};
+  const int ns = min(P.nscales, kMaxPyr);
+  if (P.levels <= 256)
+    colour_pyramid_tile<0, true>(ns, P.sw, P.sh, in, P.cand_x, thr, lut, P.levels, (lb % ntx) * kPyrTile,
+                                 (lb / ntx) * kPyrTile, a, b);
+  else
+    colour_pyramid_tile<0, false>(ns, P.sw, P.sh, in, P.cand_x, thr, lut, P.levels, (lb % ntx) * kPyrTile,
+                                  (lb / ntx) * kPyrTile, a, b);
 }
 
 // The reference's pyramid of one image from its linear RGB.
@@ -285,7 +316,7 @@ struct PyrPlanes {
 __global__ __launch_bounds__(256) void k_colour_pyr_ref(PyrPlanes A) {
   __shared__ float a[3][256], b[3][64];
   const int ntx = (A.sw[0] + kPyrTile - 1) / kPyrTile;
-  colour_pyramid_tile<1>(A.ns, A.sw, A.sh, A.in, A.out, nullptr, nullptr, 0, (blockIdx.x % ntx) * kPyrTile,
+  colour_pyramid_tile<1, false>(A.ns, A.sw, A.sh, A.in, A.out, nullptr, nullptr, 0, (blockIdx.x % ntx) * kPyrTile,
                          (blockIdx.x / ntx) * kPyrTile, a, b);
 }
 
@@ -301,9 +332,9 @@ __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __r
   const float root = cbrt_core(-0.0037930733f);
   float o[3];
   opsin_inverse(x[i], y[i], b[i], root, o);
-  r[i] = depth_value(o[0], thr, lut, levels);
-  g[i] = depth_value(o[1], thr, lut, levels);
-  bl[i] = depth_value(o[2], thr, lut, levels);
+  r[i] = depth_value<false>(o[0], thr, lut, levels);
+  g[i] = depth_value<false>(o[1], thr, lut, levels);
+  bl[i] = depth_value<false>(o[2], thr, lut, levels);
 }
 
 // ----------------------------------------------------------------------------
@@ -542,36 +573,57 @@ __global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict
 // ring index is a register.
 // ---------------------------------------------------------------------------
 struct StreamRing {
-  float m[9][3];       // horizontally filtered moments (mu2, s22, s12) per ring row
-  float c1[9], c2[9];  // centre samples x1, x2 of each ring row
+  float m[9][3];  // horizontally filtered moments (mu2, s22, s12) per ring row
 };
 
+// The global samples one step of the walk reads: input row yi (a: column
+// gx-4; b: the right halo on lanes 0..7) and, at column gx of output row
+// yo = yi - 4, the reference's x1, mu1, s11 and the candidate's x2.
+struct StreamIn {
+  float a1, a2, b1, b2, mu1, s11, i1, i2;
+};
+
+__device__ __forceinline__ StreamIn stream_load(int t, int y0, int y1, int w, int h, int gx, int lane,
+                                                const float* __restrict__ X1, const float* __restrict__ X2,
+                                                const float* __restrict__ MU1,
+                                                const float* __restrict__ S11) {
+  const int yi = y0 - 4 + t, yo = yi - 4;
+  const int xa = gx - 4, xb = gx + 60;
+  // every address clamped into the plane, so the eight loads issue together
+  // without branches; samples outside are then zeroed
+  const uint32_t ri = (uint32_t)(min(max(yi, 0), h - 1) * w), ro = (uint32_t)(min(max(yo, 0), h - 1) * w);
+  const uint32_t ca = min(max(xa, 0), w - 1), cbx = min(xb, w - 1), cg = min(gx, w - 1);
+  StreamIn v;
+  v.a1 = ldg32(X1, ri + ca);
+  v.a2 = ldg32(X2, ri + ca);
+  v.b1 = ldg32(X1, ri + cbx);
+  v.b2 = ldg32(X2, ri + cbx);
+  v.mu1 = ldg32(MU1, ro + cg);
+  v.s11 = ldg32(S11, ro + cg);
+  v.i1 = ldg32(X1, ro + cg);
+  v.i2 = ldg32(X2, ro + cg);
+  return v;
+}
+
 template <int K>
-__device__ __forceinline__ void stream_row(int t, int y0, int y1, int w, int h, int gx, int lane,
-                                           const float* __restrict__ X1, const float* __restrict__ X2,
-                                           const float* __restrict__ MU1, const float* __restrict__ S11,
+__device__ __forceinline__ void stream_row(int t, int y1, int y0, int w, int h, int gx, int lane, const StreamIn& in,
                                            const double* hf, double (*row)[72], StreamRing& R,
                                            double* acc) {
   const int yi = y0 - 4 + t, yo = yi - 4;
   const bool out = t >= 8 && yo < y1 && gx < w;
-  // input samples of row yi (a: column gx-4, b: the right halo on lanes
-  // 0..7) and the pooling inputs of output row yo
+  // the loads read clamped addresses; samples outside the plane count as 0
   const bool rin = yi >= 0 && yi < h;
   const int xa = gx - 4, xb = gx + 60;
   const bool ina = rin && xa >= 0 && xa < w;
-  const bool inb = lane < 8 && rin && xb < w;
-  const float a1 = ina ? X1[(size_t)yi * w + xa] : 0.0f;
-  const float a2 = ina ? X2[(size_t)yi * w + xa] : 0.0f;
-  const float b1 = inb ? X1[(size_t)yi * w + xb] : 0.0f;
-  const float b2 = inb ? X2[(size_t)yi * w + xb] : 0.0f;
-  const float mu1 = out ? MU1[(size_t)yo * w + gx] : 0.0f;
-  const float s11 = out ? S11[(size_t)yo * w + gx] : 0.0f;
+  const bool inb = rin && xb < w;
+  const float a1 = ina ? in.a1 : 0.0f, a2 = ina ? in.a2 : 0.0f;
   // widened samples into the wave's LDS row: x2, x2^2, x1*x2 (the squared and
   // product inputs rounded to f32 first, blur.rs:97-113)
   row[0][lane] = (double)a2;
   row[1][lane] = (double)(a2 * a2);
   row[2][lane] = (double)(a1 * a2);
   if (lane < 8) {
+    const float b1 = inb ? in.b1 : 0.0f, b2 = inb ? in.b2 : 0.0f;
     row[0][64 + lane] = (double)b2;
     row[1][64 + lane] = (double)(b2 * b2);
     row[2][64 + lane] = (double)(b1 * b2);
@@ -580,14 +632,6 @@ __device__ __forceinline__ void stream_row(int t, int y0, int y1, int w, int h, 
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  // centre samples of this input row at column gx: lane+4's a*, or for the
-  // last four lanes the right halo's b* (all lanes take part in both shuffles)
-  {
-    const float sa1 = __shfl(a1, (lane + 4) & 63, 64), sa2 = __shfl(a2, (lane + 4) & 63, 64);
-    const float sb1 = __shfl(b1, (lane - 60) & 63, 64), sb2 = __shfl(b2, (lane - 60) & 63, 64);
-    R.c1[K] = lane >= 60 ? sb1 : sa1;
-    R.c2[K] = lane >= 60 ? sb2 : sa2;
-  }
 #pragma unroll
   for (int m = 0; m < 3; m++) {
     const double* v = row[m] + lane + 4;  // centre of this lane's window
@@ -613,7 +657,7 @@ __device__ __forceinline__ void stream_row(int t, int y0, int y1, int w, int h, 
     for (int k = 1; k <= 4; k++) a = fma(hf[k], d[4 - k] + d[4 + k], a);
     vm[m] = (float)a;
   }
-  const float i1 = R.c1[(K + 5) % 9], i2 = R.c2[(K + 5) % 9];
+  const float mu1 = in.mu1, s11 = in.s11, i1 = in.i1, i2 = in.i2;
   const float mu2 = vm[0], s22 = vm[1], s12 = vm[2];
   const float C2 = 0.0009f;
   const float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
@@ -635,7 +679,8 @@ __device__ __forceinline__ void stream_row(int t, int y0, int y1, int w, int h, 
 }
 
 // One wave: strip x0 .. x0+63 of channel planes, output rows y0 .. y1-1; the
-// 6 sums of this (strip, segment, channel) go to out6 (lane 0).
+// 6 sums of this (strip, segment, channel) go to out6 (lane 0).  The global
+// samples of step t+1 are requested before step t is computed.
 __device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, int x0, int y0,
                             const float* __restrict__ X1, const float* __restrict__ MU1,
                             const float* __restrict__ S11, const float* __restrict__ X2,
@@ -648,16 +693,21 @@ __device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, i
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
   StreamRing R;
 #pragma unroll
-  for (int k = 0; k < 9; k++) {
-    R.m[k][0] = R.m[k][1] = R.m[k][2] = 0.0f;
-    R.c1[k] = R.c2[k] = 0.0f;
-  }
+  for (int k = 0; k < 9; k++) R.m[k][0] = R.m[k][1] = R.m[k][2] = 0.0f;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const int rows = (y1 - y0) + 8;  // input rows y0-4 .. y1+3 (rows >= 9)
+  // three load buffers: step K computes from buf[K % 3] and requests step
+  // K + 2 into buf[(K + 2) % 3] (the unroll of 9 keeps every index constant)
+  StreamIn buf[3];
+  buf[0] = stream_load(0, y0, y1, w, h, gx, lane, X1, X2, MU1, S11);
+  buf[1] = stream_load(1, y0, y1, w, h, gx, lane, X1, X2, MU1, S11);
   for (int t = 0; t < rows; t += 9) {
 // step t + K writes ring slot K (compile-time over the unroll of 9)
-#define JPXLG_STEP(K) \
-    if (t + K < rows) stream_row<K>(t + K, y0, y1, w, h, gx, lane, X1, X2, MU1, S11, hf, row, R, acc);
+#define JPXLG_STEP(K)                                                                  \
+    if (t + K < rows) {                                                                \
+      buf[(K + 2) % 3] = stream_load(t + K + 2, y0, y1, w, h, gx, lane, X1, X2, MU1, S11); \
+      stream_row<K>(t + K, y1, y0, w, h, gx, lane, buf[K % 3], hf, row, R, acc);                \
+    }
     JPXLG_STEP(0) JPXLG_STEP(1) JPXLG_STEP(2) JPXLG_STEP(3) JPXLG_STEP(4)
     JPXLG_STEP(5) JPXLG_STEP(6) JPXLG_STEP(7) JPXLG_STEP(8)
 #undef JPXLG_STEP
@@ -693,7 +743,7 @@ __global__ __launch_bounds__(256) void k_blur_pool_stream(const ConstTables* __r
   if (unit >= total) return;
   const int ds = find_desc(off, nds, unit);
   const int d = ds / kDescScales, sc = ds % kDescScales;
-  const ProbeDesc& P = D[d];
+  DescSpace& P = desc_at(D, d);
   const int w = P.sw[sc], h = P.sh[sc], tx = (w + kPW - 1) / kPW;
   const int tiles = P.punits[sc];
   const int lu = unit - off[ds], c = lu / tiles, lt = lu % tiles;
@@ -722,7 +772,7 @@ __global__ __launch_bounds__(256) void k_reduce_b(const ProbeDesc* __restrict__ 
   __shared__ double red[4];
   const int d = blockIdx.x / (kDescScales * 18), rem = blockIdx.x % (kDescScales * 18);
   const int s = rem / 18, term = rem % 18;
-  const ProbeDesc& P = D[d];
+  DescSpace& P = desc_at(D, d);
   if (s >= P.nscales) return;
   const int tiles = P.punits[s];
   const double* p = P.partials + (size_t)P.ptile_off[s] * 18;

@@ -238,7 +238,7 @@ __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __
     float* dsts[3] = {CX, CY, CB};
     matrix_transform_tiles<N, FORM>(lds, VB * 3, L, [&](int t, int k, int r, float v) {
       const int j = t / 3;
-      if (base + j < count) dsts[t % 3][(size_t)(8 * vby[j] + k) * g.cstride + 8 * vbx[j] + r] = v;
+      if (base + j < count) stg32(dsts[t % 3], (uint32_t)((8 * vby[j] + k) * g.cstride + 8 * vbx[j] + r), v);
     });
     return;
   }
@@ -492,8 +492,8 @@ template <int N>
 __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, const int* __restrict__ off, int nd,
                                               int block, const ConstTables* __restrict__ tab, QuantSmem<N>& sm) {
   const int d = find_desc(off, nd, block);
-  const ProbeDesc& P = D[d];
-  const Geom g = P.g;
+  DescSpace& P = desc_at(D, d);
+  const Geom g = geom_of(P.g);
   const uint32_t* __restrict__ list = P.list[size_index(N)];
   const int count = P.count[size_index(N)];
   const ProbeParams* __restrict__ pp = P.pp;
@@ -707,8 +707,8 @@ __device__ __forceinline__ void render_body(const ProbeDesc* __restrict__ D, con
                                             int block, const ConstTables* __restrict__ tab,
                                             const DctMats* __restrict__ mats, RenderSmem<N>& sm) {
   const int d = find_desc(off, nd, block);
-  const ProbeDesc& PD = D[d];
-  const Geom g = PD.g;
+  DescSpace& PD = desc_at(D, d);
+  const Geom g = geom_of(PD.g);
   const uint32_t* __restrict__ list = PD.list[size_index(N)];
   const int count = PD.count[size_index(N)];
   const ProbeParams* __restrict__ pp = PD.pp;
@@ -822,7 +822,7 @@ __device__ __forceinline__ void render_body(const ProbeDesc* __restrict__ D, con
     matrix_transform_tiles<N, FORM>(lds, VB * 3, L, [&](int tt, int y, int x, float v) {
       const int jj = tt / 3;
       const int fx = 8 * vbx[jj] + x, fy = 8 * vby[jj] + y;
-      if (jj < nvb && fx < g.w && fy < g.h) dsts[tt % 3][(size_t)fy * g.w + fx] = v;
+      if (jj < nvb && fx < g.w && fy < g.h) stg32(dsts[tt % 3], (uint32_t)(fy * g.w + fx), v);
     });
   } else {
   const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
@@ -964,7 +964,7 @@ __global__ __launch_bounds__(256) void k_gaborish_jacobi(int w, int h, const flo
 __global__ __launch_bounds__(256) void k_gaborish_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
                                                     int nd, int src) {
   const int d = find_desc(off, nd, blockIdx.x);
-  const ProbeDesc& P = D[d];
+  DescSpace& P = desc_at(D, d);
   const int lb = blockIdx.x - off[d], xb = (P.g.w + 255) / 256;
   const int c = blockIdx.z;
   gaborish_px(P.g.w, P.g.h, src ? P.aux[c] : P.rx[c], src ? P.rx[c] : P.aux[c], (lb % xb) * 256 + threadIdx.x,
@@ -1073,10 +1073,10 @@ template <int STEP>
 __global__ __launch_bounds__(256) void k_epf_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
                                                int nd, int src) {
   const int d = find_desc(off, nd, blockIdx.x);
-  const ProbeDesc& P = D[d];
+  DescSpace& P = desc_at(D, d);
   const int lb = blockIdx.x - off[d], tx = (P.g.w + kEpfTW - 1) / kEpfTW;
-  float* const* in = src ? P.aux : P.rx;
-  float* const* out = src ? P.rx : P.aux;
+  float* in[3] = {src ? P.aux[0] : P.rx[0], src ? P.aux[1] : P.rx[1], src ? P.aux[2] : P.rx[2]};
+  float* out[3] = {src ? P.rx[0] : P.aux[0], src ? P.rx[1] : P.aux[1], src ? P.rx[2] : P.aux[2]};
   epf_tile<STEP>(P.g.w, P.g.h, in[0], in[1], in[2], P.sigma, out[0], out[1], out[2], (lb % tx) * kEpfTW,
                  (lb / tx) * kEpfTH);
 }

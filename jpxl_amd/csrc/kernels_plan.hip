@@ -237,7 +237,7 @@ __device__ __forceinline__ Ival iv_const(double v) { return Ival{v, 0.0}; }
 // (bx0, by0).  With rad arrays (certified costs) it returns false, writing
 // nothing, when some comparison could go either way within the costs'
 // bounds; with rad null (exact costs) it always decides.
-__device__ bool decide_region(Geom g, const ProbeParams* __restrict__ pp, const double* __restrict__ c8,
+__device__ __forceinline__ bool decide_region(Geom g, const ProbeParams* __restrict__ pp, const double* __restrict__ c8,
                               const double* __restrict__ c16, const double* __restrict__ c32,
                               const double* __restrict__ r8, const double* __restrict__ r16,
                               const double* __restrict__ r32, uint8_t* __restrict__ bmap, int bx0, int by0) {
