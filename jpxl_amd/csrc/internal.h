@@ -34,6 +34,8 @@ struct ConstTables {
   float matrix[3][kCellTotal];    // dequantization matrices per channel, sizes 8/16/32 concatenated
   float donor[kCellTotal];        // quantizer-donor frequency weights
   uint16_t order[kCellTotal];     // natural coefficient order (position -> cell)
+  float matrix_pos[3][kCellTotal];  // matrix and donor weights indexed by natural-order
+  float donor_pos[kCellTotal];      // position (matrix[c][order[p]], donor[order[p]])
   float scale_f[3][4];            // ScaleF(c, n) for n = 1, 2, 4 (index by size), c < n
   double fir[5];                  // metric blur taps h[0..4] (symmetric), f64
 };

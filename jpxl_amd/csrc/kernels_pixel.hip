@@ -467,6 +467,70 @@ __device__ void truncate_group(int32_t* q, const float* targets, float* recon_y,
   }
 }
 
+// truncate_group with the lane's tables loaded beforehand: cell[r], don[r]
+// and mpos[r] are the natural order, donor weight and matrix entry of the
+// lane's positions [L*R, L*R + R) (L = wi*64 + lane), so the walk waits on
+// no global load.  Same decisions as truncate_group.
+template <int N, int R>
+__device__ void truncate_pre(int32_t* q, const float* targets, float* recon_y, const int* cell, const float* don,
+                             const float* mpos, float sc, float bias, double lambda, float ztb, int nw, int wi,
+                             int* xs) {
+  constexpr int NN = N * N, nb = (N / 8) * (N / 8);
+  const int lane = threadIdx.x & 63;
+  const int pos0 = (wi * 64 + lane) * R;
+  int32_t v[R];
+  int lastnz = -1;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int p = pos0 + r;
+    v[r] = p >= nb ? q[cell[r]] : 0;
+    if (v[r] != 0) lastnz = p;
+  }
+  int incl = lastnz;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int o = __shfl_up(incl, d, 64);
+    if (lane >= d) incl = max(incl, o);
+  }
+  if (lane == 63) xs[wi] = incl;
+  __syncthreads();
+  int before = -1;  // last nonzero in the group's earlier waves
+  for (int k = 0; k < wi; k++) before = max(before, xs[k]);
+  int prev = __shfl_up(incl, 1, 64);
+  if (lane == 0) prev = -1;
+  prev = max(prev, before);
+  int fail = -1;
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int p = pos0 + r;
+    if (v[r] == 0) continue;
+    const float weight = (float)(lambda * (double)NN * (double)don[r]);
+    const int exposed = prev >= 0 ? p - prev - 1 : p - nb;
+    const float own = (float)(33 - __clz((unsigned)abs(v[r])));
+    const float saved = own + (float)exposed * ztb;
+    const float target = targets[cell[r]];
+    const float step = sc * mpos[r];
+    const float recon = bias_adjust(v[r], bias) * step;
+    const float kept = recon - target;
+    const float added = fmaf(target, target, -(kept * kept));
+    if (saved <= weight * added) fail = p;
+    prev = p;
+  }
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) fail = max(fail, __shfl_xor(fail, d, 64));
+  if (lane == 0) xs[nw + wi] = fail;
+  __syncthreads();
+  for (int k = 0; k < nw; k++) fail = max(fail, xs[nw + k]);
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int p = pos0 + r;
+    if (v[r] != 0 && p > fail) {
+      q[cell[r]] = 0;
+      if (recon_y) recon_y[cell[r]] = bias_adjust(0, bias) * (sc * mpos[r]);
+    }
+  }
+}
+
 // LDS of one quantize workgroup of side N.
 template <int N>
 struct QuantSmem {
@@ -536,6 +600,35 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
     varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], nullptr);
   }
   __syncthreads();
+  // DCT32 (one varblock per workgroup): the tables this thread needs later
+  // are requested together with its coefficients -- the matrix at its cells
+  // k = tid + 256 i and, for the truncation walks, the order, donor and
+  // matrix entries of its positions (Y: 4 per lane over the four waves;
+  // X on waves 0-1 and B on waves 2-3: 8 per lane).
+  constexpr int kR = N == 32 ? 1 : 0;
+  float mr[3][4 * kR + 1];
+  int cellY[4 * kR + 1], cellXB[8 * kR + 1];
+  float donY[4 * kR + 1], matY[4 * kR + 1], donXB[8 * kR + 1], matXB[8 * kR + 1];
+  if constexpr (N == 32) {
+    const int tid = threadIdx.x, w4 = tid >> 6, cxb = w4 < 2 ? 0 : 2;
+    const int py = tid * 4, pxb = ((w4 & 1) * 64 + (tid & 63)) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int c = 0; c < 3; c++) mr[c][i] = tab->matrix[c][cb + tid + 256 * i];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      cellY[r] = tab->order[cb + py + r];
+      donY[r] = tab->donor_pos[cb + py + r];
+      matY[r] = tab->matrix_pos[1][cb + py + r];
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      cellXB[r] = tab->order[cb + pxb + r];
+      donXB[r] = tab->donor_pos[cb + pxb + r];
+      matXB[r] = tab->matrix_pos[cxb][cb + pxb + r];
+    }
+  }
   const float* cin[3] = {CX, CY, CB};
   for (int e = threadIdx.x; e < 3 * VB * NN; e += T) {
     int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
@@ -587,7 +680,18 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       dylf[j][i] = mdc[1] * (float)v / 1.0f;
     }
   }
-  {
+  if constexpr (N == 32) {
+    const float bias = kQuantBias[1];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int k = threadIdx.x + 256 * i;
+      const float step = vsc[0][1] * mr[1][i];
+      int32_t v = 0;
+      if ((k % N) >= n || (k / N) >= n) v = hf_choose(co[1][0][k], step, bias, &bad);
+      qs[1][0][k] = v;
+      dyh[0][k] = bias_adjust(v, bias) * step;
+    }
+  } else {
     const float bias = kQuantBias[1];
     for (int e = threadIdx.x; e < VB * NN; e += T) {
       int j = e / NN, k = e % NN;
@@ -604,8 +708,8 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
   if (pp->truncate) {
     if constexpr (N == 32) {
       // one varblock: all T/64 waves walk its luma block together
-      truncate_group<N>(qs[1][0], co[1][0], dyh[0], tab->order + cb, tab->matrix[1] + cb, tab->donor + cb,
-                        vsc[0][1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits, T / 64, wave, xs[0]);
+      truncate_pre<N, 4>(qs[1][0], co[1][0], dyh[0], cellY, donY, matY, vsc[0][1], kQuantBias[1], pp->lambda[1],
+                         pp->zero_token_bits, T / 64, wave, xs[0]);
     } else {
       for (int j = wave; j < nvb; j += T / 64)
         truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
@@ -626,6 +730,20 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
   }
   __syncthreads();
   // --- X and B HF: targets against the reconstructed dY, then choose ---
+  if constexpr (N == 32) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int ci = i / 4, c = ci == 0 ? 0 : 2, k = threadIdx.x + 256 * (i % 4);
+      int32_t v = 0;
+      if ((k % N) >= n || (k / N) >= n) {
+        const float target = co[c][0][k] - khf[ci][0] * dyh[0][k];
+        co[c][0][k] = target;
+        const float step = vsc[0][c] * mr[c][i % 4];
+        v = hf_choose(target, step, kQuantBias[c], &bad);
+      }
+      qs[c][0][k] = v;
+    }
+  } else
   for (int e = threadIdx.x; e < 2 * VB * NN; e += T) {
     int ci = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN, c = ci == 0 ? 0 : 2;
     int32_t v = 0;
@@ -642,8 +760,8 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
     if constexpr (N == 32) {
       // X on the first half of the waves, B on the second half
       const int half = (T / 64) / 2, ci = wave / half, c = ci == 0 ? 0 : 2;
-      truncate_group<N>(qs[c][0], co[c][0], nullptr, tab->order + cb, tab->matrix[c] + cb, tab->donor + cb,
-                        vsc[0][c], kQuantBias[c], pp->lambda[c], pp->zero_token_bits, half, wave % half, xs[ci]);
+      truncate_pre<N, 8>(qs[c][0], co[c][0], nullptr, cellXB, donXB, matXB, vsc[0][c], kQuantBias[c],
+                         pp->lambda[c], pp->zero_token_bits, half, wave % half, xs[ci]);
     } else {
       for (int w = wave; w < 2 * VB; w += T / 64) {
         const int ci = w / VB, j = w % VB, c = ci == 0 ? 0 : 2;

@@ -216,6 +216,12 @@ void build_const_tables(ConstTables* t) {
     for (int c = 0; c < 3; c++) dequant_matrix(side, c, t->matrix[c] + kCellBase[si]);
     donor_weights(side, t->donor + kCellBase[si]);
     natural_order(side, t->order + kCellBase[si]);
+    const int cb = kCellBase[si];
+    for (int pos = 0; pos < side * side; pos++) {
+      const int cell = t->order[cb + pos];
+      for (int c = 0; c < 3; c++) t->matrix_pos[c][cb + pos] = t->matrix[c][cb + cell];
+      t->donor_pos[cb + pos] = t->donor[cb + cell];
+    }
     int n = side / 8;
     for (int c = 0; c < n; c++) t->scale_f[si][c] = scale_f(c, n);
   }
