@@ -589,28 +589,26 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
   const int base = (block - off[d]) * VB;
   const int nvb = min(VB, count - base);
   int bad = 0;
-  if (threadIdx.x < VB) {
-    int bx = 0, by = 0;
-    if (threadIdx.x < nvb) unpack(list[base + threadIdx.x], &bx, &by);
-    vbx[threadIdx.x] = bx;
-    vby[threadIdx.x] = by;
-    int tile = (by / 8) * g.tw + bx / 8;
-    khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
-    khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
-    varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], nullptr);
-  }
-  __syncthreads();
-  // DCT32 (one varblock per workgroup): the tables this thread needs later
-  // are requested together with its coefficients -- the matrix at its cells
-  // k = tid + 256 i and, for the truncation walks, the order, donor and
-  // matrix entries of its positions (Y: 4 per lane over the four waves;
-  // X on waves 0-1 and B on waves 2-3: 8 per lane).
+  // DCT32 (one varblock per workgroup, known to every thread): its
+  // coefficients and the tables this thread needs later are requested before
+  // anything waits -- the matrix at its cells k = tid + 256 i and, for the
+  // truncation walks, the order, donor and matrix entries of its positions
+  // (Y: 4 per lane over the four waves; X on waves 0-1 and B on waves 2-3:
+  // 8 per lane).
   constexpr int kR = N == 32 ? 1 : 0;
-  float mr[3][4 * kR + 1];
+  float mr[3][4 * kR + 1], cr[12 * kR + 1];
   int cellY[4 * kR + 1], cellXB[8 * kR + 1];
   float donY[4 * kR + 1], matY[4 * kR + 1], donXB[8 * kR + 1], matXB[8 * kR + 1];
+  const float* cin[3] = {CX, CY, CB};
   if constexpr (N == 32) {
     const int tid = threadIdx.x, w4 = tid >> 6, cxb = w4 < 2 ? 0 : 2;
+    int bx = 0, by = 0;
+    if (nvb > 0) unpack(list[base], &bx, &by);
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+      const int k = tid + 256 * (i % 4);
+      cr[i] = nvb > 0 ? ldg32(cin[i / 4], (uint32_t)((8 * by + k / N) * g.cstride + 8 * bx + k % N)) : 0.0f;
+    }
     const int py = tid * 4, pxb = ((w4 & 1) * 64 + (tid & 63)) * 8;
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -629,12 +627,27 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       matXB[r] = tab->matrix_pos[cxb][cb + pxb + r];
     }
   }
-  const float* cin[3] = {CX, CY, CB};
-  for (int e = threadIdx.x; e < 3 * VB * NN; e += T) {
-    int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
-    float v = 0.0f;
-    if (j < nvb) v = cin[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N];
-    co[c][j][k] = v;
+  if (threadIdx.x < VB) {
+    int bx = 0, by = 0;
+    if (threadIdx.x < nvb) unpack(list[base + threadIdx.x], &bx, &by);
+    vbx[threadIdx.x] = bx;
+    vby[threadIdx.x] = by;
+    int tile = (by / 8) * g.tw + bx / 8;
+    khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
+    khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
+    varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], nullptr);
+  }
+  if constexpr (N == 32) {
+#pragma unroll
+    for (int i = 0; i < 12; i++) co[i / 4][0][threadIdx.x + 256 * (i % 4)] = cr[i];
+  } else {
+    __syncthreads();
+    for (int e = threadIdx.x; e < 3 * VB * NN; e += T) {
+      int c = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN;
+      float v = 0.0f;
+      if (j < nvb) v = cin[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N];
+      co[c][j][k] = v;
+    }
   }
   __syncthreads();
   const DctTables& t = tab->dct;
@@ -852,6 +865,36 @@ __device__ __forceinline__ void render_body(const ProbeDesc* __restrict__ D, con
   float (*vsc)[4] = sm.vsc;
   const int base = (block - off[d]) * VB;
   const int nvb = min(VB, count - base);
+  // Every thread reads the workgroup's varblock positions (uniform loads) and
+  // requests all its quantized coefficients and matrix entries at once,
+  // before the per-varblock scales are computed.
+  constexpr int IT = (VB * NN + kThreads - 1) / kThreads;
+  int32_t qv[3][IT];
+  float mv[3][IT];
+  {
+    int px[VB], py[VB];
+#pragma unroll
+    for (int j = 0; j < VB; j++) {
+      px[j] = py[j] = 0;
+      if (j < nvb) unpack(list[base + j], &px[j], &py[j]);
+    }
+    const int32_t* qin[3] = {QX, QY, QB};
+#pragma unroll
+    for (int i = 0; i < IT; i++) {
+      const int e = threadIdx.x + kThreads * i, j = e / NN, k = e % NN;
+      int bx = 0, by = 0;
+#pragma unroll
+      for (int jj = 0; jj < VB; jj++)
+        if (jj == j) bx = px[jj], by = py[jj];
+      const bool live = e < VB * NN && j < nvb;
+      const uint32_t o = (uint32_t)((8 * by + k / N) * g.cstride + 8 * bx + k % N);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        qv[c][i] = live ? ldg32(qin[c], o) : 0;
+        mv[c][i] = live ? tab->matrix[c][cb + k] : 0.0f;
+      }
+    }
+  }
   if (threadIdx.x < VB) {
     int bx = 0, by = 0;
     if (threadIdx.x < nvb) unpack(list[base + threadIdx.x], &bx, &by);
@@ -864,14 +907,16 @@ __device__ __forceinline__ void render_body(const ProbeDesc* __restrict__ D, con
   }
   __syncthreads();
   // I.5.3 dequantization of all three channels, then I.6 HF CfL.
-  for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int e = threadIdx.x + kThreads * i;
+    if (e >= VB * NN) break;
     int j = e / NN, k = e % NN;
     float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
     if (j < nvb) {
-      size_t o = (size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N;
-      d0 = bias_adjust(QX[o], kQuantBias[0]) * vsc[j][0] * tab->matrix[0][cb + k];
-      d1 = bias_adjust(QY[o], kQuantBias[1]) * vsc[j][1] * tab->matrix[1][cb + k];
-      d2 = bias_adjust(QB[o], kQuantBias[2]) * vsc[j][2] * tab->matrix[2][cb + k];
+      d0 = bias_adjust(qv[0][i], kQuantBias[0]) * vsc[j][0] * mv[0][i];
+      d1 = bias_adjust(qv[1][i], kQuantBias[1]) * vsc[j][1] * mv[1][i];
+      d2 = bias_adjust(qv[2][i], kQuantBias[2]) * vsc[j][2] * mv[2][i];
     }
     int r = k / N, col = k % N;
     lds[((j * 3 + 0) * N + r) * P + col] = d0 + khf[0][j] * d1;
