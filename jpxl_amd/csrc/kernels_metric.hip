@@ -591,17 +591,19 @@ __device__ __forceinline__ StreamIn stream_load(int t, int y0, int y1, int w, in
   const int xa = gx - 4, xb = gx + 60;
   // every address clamped into the plane, so the eight loads issue together
   // without branches; samples outside are then zeroed
-  const uint32_t ri = (uint32_t)(min(max(yi, 0), h - 1) * w), ro = (uint32_t)(min(max(yo, 0), h - 1) * w);
+  // (row offsets are wave-uniform: the row pointers stay in scalar registers
+  // and each lane adds its fixed column)
+  const size_t ri = (size_t)min(max(yi, 0), h - 1) * w, ro = (size_t)min(max(yo, 0), h - 1) * w;
   const uint32_t ca = min(max(xa, 0), w - 1), cbx = min(xb, w - 1), cg = min(gx, w - 1);
   StreamIn v;
-  v.a1 = ldg32(X1, ri + ca);
-  v.a2 = ldg32(X2, ri + ca);
-  v.b1 = ldg32(X1, ri + cbx);
-  v.b2 = ldg32(X2, ri + cbx);
-  v.mu1 = ldg32(MU1, ro + cg);
-  v.s11 = ldg32(S11, ro + cg);
-  v.i1 = ldg32(X1, ro + cg);
-  v.i2 = ldg32(X2, ro + cg);
+  v.a1 = ldg32(X1 + ri, ca);
+  v.a2 = ldg32(X2 + ri, ca);
+  v.b1 = ldg32(X1 + ri, cbx);
+  v.b2 = ldg32(X2 + ri, cbx);
+  v.mu1 = ldg32(MU1 + ro, cg);
+  v.s11 = ldg32(S11 + ro, cg);
+  v.i1 = ldg32(X1 + ro, cg);
+  v.i2 = ldg32(X2 + ro, cg);
   return v;
 }
 
@@ -730,7 +732,9 @@ __global__ __launch_bounds__(256) void k_blur_pool_stream(const ConstTables* __r
                                                           const int* __restrict__ off, int nds, int total,
                                                           int swizzle) {
   __shared__ double rows[4][3][72];
-  const int wave = threadIdx.x >> 6, upw = blockDim.x >> 6;
+  // the wave index is wave-uniform: say so, so that everything derived from
+  // it (unit, plane pointers, row offsets) lives in scalar registers
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), upw = blockDim.x >> 6;
   const int nb = gridDim.x;
   int b = blockIdx.x;
   if (swizzle && nb >= 64) {

@@ -1210,10 +1210,20 @@ __device__ __forceinline__ void epf_tile(int w, int h, const float* __restrict__
   constexpr int LW = kEpfTW + 2 * kEpfR, LH = kEpfTH + 2 * kEpfR;
   __shared__ float tile[3][LH][LW];
   const float* in[3] = {in0, in1, in2};
-  for (int e = threadIdx.x; e < 3 * LH * LW; e += 256) {
-    int c = e / (LH * LW), rem = e % (LH * LW), ly = rem / LW, lx = rem % LW;
-    int sy = mirror1d(y0 + ly - kEpfR, h), sx = mirror1d(x0 + lx - kEpfR, w);
-    tile[c][ly][lx] = in[c][(size_t)sy * w + sx];
+  // all of this thread's halo-tile samples are requested before any is stored
+  constexpr int kLoads = (3 * LH * LW + 255) / 256;
+  float v[kLoads];
+#pragma unroll
+  for (int i = 0; i < kLoads; i++) {
+    const int e = threadIdx.x + 256 * i;
+    const int c = e / (LH * LW), rem = e % (LH * LW), ly = rem / LW, lx = rem % LW;
+    const int sy = mirror1d(y0 + ly - kEpfR, h), sx = mirror1d(x0 + lx - kEpfR, w);
+    v[i] = e < 3 * LH * LW ? ldg32(in[c < 3 ? c : 2], (uint32_t)(sy * w + sx)) : 0.0f;
+  }
+#pragma unroll
+  for (int i = 0; i < kLoads; i++) {
+    const int e = threadIdx.x + 256 * i;
+    if (e < 3 * LH * LW) (&tile[0][0][0])[e] = v[i];
   }
   __syncthreads();
   const int tx = threadIdx.x % kEpfTW;
