@@ -573,7 +573,7 @@ __global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict
 // ring index is a register.
 // ---------------------------------------------------------------------------
 struct StreamRing {
-  float m[9][3];  // horizontally filtered moments (mu2, s22, s12) per ring row
+  double m[9][3];  // horizontally filtered moments (mu2, s22, s12) per ring row, f32 values
 };
 
 // The global samples one step of the walk reads: input row yi (a: column
@@ -640,7 +640,7 @@ __device__ __forceinline__ void stream_row(int t, int y1, int y0, int w, int h, 
     double acc_h = hf[0] * v[0];
 #pragma unroll
     for (int k = 1; k <= 4; k++) acc_h = fma(hf[k], v[-k] + v[k], acc_h);
-    R.m[K][m] = (float)acc_h;
+    R.m[K][m] = (double)(float)acc_h;
   }
   // every lane's reads of this row before the next row overwrites it
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -653,7 +653,7 @@ __device__ __forceinline__ void stream_row(int t, int y1, int y0, int w, int h, 
   for (int m = 0; m < 3; m++) {
     double d[9];
 #pragma unroll
-    for (int k = 0; k < 9; k++) d[k] = (double)R.m[(K + 1 + k) % 9][m];
+    for (int k = 0; k < 9; k++) d[k] = R.m[(K + 1 + k) % 9][m];
     double a = hf[0] * d[4];
 #pragma unroll
     for (int k = 1; k <= 4; k++) a = fma(hf[k], d[4 - k] + d[4 + k], a);
@@ -695,7 +695,7 @@ __device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, i
   for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
   StreamRing R;
 #pragma unroll
-  for (int k = 0; k < 9; k++) R.m[k][0] = R.m[k][1] = R.m[k][2] = 0.0f;
+  for (int k = 0; k < 9; k++) R.m[k][0] = R.m[k][1] = R.m[k][2] = 0.0;
   double acc[6] = {0, 0, 0, 0, 0, 0};
   const int rows = (y1 - y0) + 8;  // input rows y0-4 .. y1+3 (rows >= 9)
   // three load buffers: step K computes from buf[K % 3] and requests step
