@@ -56,13 +56,31 @@ __device__ __forceinline__ float cbrt_core(float value) {
 // a few f64 ulps of an f32 rounding boundary (about one sample in 2^27),
 // and then by one f32 ulp.  It spends one f64 division where the
 // reference's recursion spends four.
+__device__ __forceinline__ float cbrt_metric_halley(double a, float y) {
+  const double yd = (double)y;
+  const double y3 = yd * yd * yd;
+  return (float)(yd * ((y3 + 2.0 * a) / (2.0 * y3 + a)));
+}
+
+// The same root without the division in all but ~1e-5 of the samples: one
+// Newton step from the library root with an f32 reciprocal puts t within
+// ~2^-43 (relative) of the root when the library root was within a few f32
+// ulps (checked through the size of the correction).  Rounding t to f32 is
+// then certain -- the root, the reference's f64 root and the Halley root all
+// round alike -- unless t lies within 2^11 f64 units of an f32 rounding
+// boundary (the low 29 mantissa bits near 2^28); those samples take the
+// Halley step, so the result equals the Halley form's everywhere.
 __device__ __forceinline__ float cbrt_metric(float value) {
   if (value == 0.0f || !isfinite(value)) return value;
-  const double a = fabs((double)value);
-  const double y = (double)cbrtf(fabsf(value));
-  const double y3 = y * y * y;
-  const double r = y * ((y3 + 2.0 * a) / (2.0 * y3 + a));
-  const float root = (float)r;
+  const float av = fabsf(value);
+  const float y = cbrtf(av);
+  const double a = (double)av, yd = (double)y, y2 = yd * yd;
+  const double d = (y2 * yd - a) * (double)__builtin_amdgcn_rcpf(3.0f * (float)y2);
+  const double t = yd - d;
+  const int32_t ext = (int32_t)((uint32_t)__double2loint(t) & 0x1fffffffu) - (1 << 28);
+  float root;
+  if (fabs(d) <= 0x1p-19 * t && (ext >= 2048 || ext <= -2048)) root = (float)t;
+  else root = cbrt_metric_halley(a, y);
   return value < 0.0f ? -root : root;
 }
 

@@ -16,6 +16,13 @@ namespace {
 
 inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
 
+// x^(1/2.4) for the depth quantizer's first guess: the hardware log2 and
+// exp2 (a few f32 ulps; the library pow costs ~160 instructions).  The guess
+// only picks where the exact threshold search starts.
+__device__ __forceinline__ float guess_pow(float x) {
+  return __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 2.4f));
+}
+
 // Depth tables of up to 256 levels are staged in LDS (TAB_LDS), larger ones
 // are read from global memory; the accessor tells the compiler which.
 template <bool TAB_LDS>
@@ -36,7 +43,7 @@ __device__ __forceinline__ float depth_value(float x, const float* thr, const fl
   const float max = (float)(levels - 1);
   float e;
   if (x <= 0.0031308f) e = 12.92f * x;
-  else e = 1.055f * __powf(x, 1.0f / 2.4f) - 0.055f;
+  else e = 1.055f * guess_pow(x) - 0.055f;
   float gl = rintf(e * max);
   const int g = gl <= 0.0f ? 0 : (gl >= max ? levels - 1 : (int)gl);
   // The two thresholds either side of the guess, loaded together; the walks
@@ -52,6 +59,52 @@ __device__ __forceinline__ float depth_value(float x, const float* thr, const fl
     while (lo < levels - 1 && tab_at<TAB_LDS>(thr, lo) <= x) lo++;
   }
   return tab_at<TAB_LDS>(lut, lo);
+}
+
+// depth_value over M samples at once: every guess first, then all their
+// threshold pairs (clamped indices, no branches), the rare corrective walks,
+// then all the level values -- two rounds of table latency for the M samples
+// instead of two per sample.  Same results as depth_value.
+template <bool TAB_LDS, int M>
+__device__ __forceinline__ void depth_values(const float (&x)[M], float (&out)[M], const float* thr,
+                                             const float* lut, int levels) {
+  const float lmax = (float)(levels - 1);
+  const int top = max(levels - 2, 0);  // last threshold index
+  int g[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    float e;
+    if (x[i] <= 0.0031308f) e = 12.92f * x[i];
+    else e = 1.055f * guess_pow(x[i]) - 0.055f;
+    const float gl = rintf(e * lmax);
+    g[i] = !(gl > 0.0f) ? 0 : (gl >= lmax ? levels - 1 : (int)gl);  // NaN -> 0 (unused)
+  }
+  float tl[M], th[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    tl[i] = levels > 1 ? tab_at<TAB_LDS>(thr, min(max(g[i] - 1, 0), top)) : 0.0f;
+    th[i] = levels > 1 ? tab_at<TAB_LDS>(thr, min(g[i], top)) : 0.0f;
+  }
+  int lo[M];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    const float xi = x[i];
+    const float l = g[i] > 0 ? tl[i] : -INFINITY;
+    const float hgh = g[i] < levels - 1 ? th[i] : INFINITY;
+    int v = g[i];
+    if (!isfinite(xi)) {
+      v = 0;
+    } else if (l > xi) {
+      v = g[i] - 1;
+      while (v > 0 && tab_at<TAB_LDS>(thr, v - 1) > xi) v--;
+    } else if (hgh <= xi) {
+      v = g[i] + 1;
+      while (v < levels - 1 && tab_at<TAB_LDS>(thr, v) <= xi) v++;
+    }
+    lo[i] = v;
+  }
+#pragma unroll
+  for (int i = 0; i < M; i++) out[i] = tab_at<TAB_LDS>(lut, lo[i]);
 }
 
 // jpxl-perceptual/src/color.rs:66-81 linear_srgb_to_positive_xyb.
@@ -105,6 +158,17 @@ __device__ __forceinline__ void colour_quad(int w, int h, const float* __restric
   if (ox >= w2 || oy >= h2) return;
   const float root = MODE == 0 ? cbrt_core(-0.0037930733f) : 0.0f;
   float lin[2][2][3];
+  float dv[12];
+  if constexpr (MODE == 0) {
+    float o[12];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int x = min(2 * ox + (q & 1), w - 1), y = min(2 * oy + (q >> 1), h - 1);
+      const size_t i = (size_t)y * w + x;
+      opsin_inverse(in0[i], in1[i], in2[i], root, o + 3 * q);
+    }
+    depth_values<TAB_LDS, 12>(o, dv, thr, lut, levels);
+  }
 #pragma unroll
   for (int dy = 0; dy < 2; dy++)
 #pragma unroll
@@ -113,10 +177,8 @@ __device__ __forceinline__ void colour_quad(int w, int h, const float* __restric
       const size_t i = (size_t)y * w + x;
       float v[3];
       if constexpr (MODE == 0) {
-        float o[3];
-        opsin_inverse(in0[i], in1[i], in2[i], root, o);
 #pragma unroll
-        for (int c = 0; c < 3; c++) v[c] = depth_value<TAB_LDS>(o[c], thr, lut, levels);
+        for (int c = 0; c < 3; c++) v[c] = dv[3 * (2 * dy + dx) + c];
       } else {
         v[0] = in0[i]; v[1] = in1[i]; v[2] = in2[i];
       }
@@ -223,6 +285,17 @@ __device__ void colour_pyramid_tile(int ns, IntP sw, IntP sh, const float* const
       const int qx = q % 16, qy = q / 16, ox = tx0 / 2 + qx, oy = ty0 / 2 + qy;
       if (ox >= w2 || oy >= h2) continue;
       float lin[2][2][3];
+      float dv[12];
+      if constexpr (MODE == 0) {
+        float o[12];
+#pragma unroll
+        for (int q4 = 0; q4 < 4; q4++) {
+          const int x = min(2 * ox + (q4 & 1), w - 1), y = min(2 * oy + (q4 >> 1), h - 1);
+          const size_t i = (size_t)y * w + x;
+          opsin_inverse(in[0][i], in[1][i], in[2][i], root, o + 3 * q4);
+        }
+        depth_values<TAB_LDS, 12>(o, dv, thr, lut, levels);
+      }
 #pragma unroll
       for (int dy = 0; dy < 2; dy++)
 #pragma unroll
@@ -231,10 +304,8 @@ __device__ void colour_pyramid_tile(int ns, IntP sw, IntP sh, const float* const
           const size_t i = (size_t)y * w + x;
           float v[3];
           if constexpr (MODE == 0) {
-            float o[3];
-            opsin_inverse(in[0][i], in[1][i], in[2][i], root, o);
 #pragma unroll
-            for (int c = 0; c < 3; c++) v[c] = depth_value<TAB_LDS>(o[c], thr, lut, levels);
+            for (int c = 0; c < 3; c++) v[c] = dv[3 * (2 * dy + dx) + c];
           } else {
             v[0] = in[0][i]; v[1] = in[1][i]; v[2] = in[2][i];
           }

@@ -311,6 +311,28 @@ def test_gaborish_bit_exact(ctx):
     assert np.array_equal(ctx.gaborish(planes).view(np.uint32), orc.gaborish(planes).view(np.uint32))
 
 
+@pytest.mark.parametrize("bits", [8, 16])
+def test_metric_positive_xyb_matches_the_oracle(ctx, bits):
+    # The metric's positive XYB of the reference image at scale 0 (its cube
+    # root is certified against f32 rounding boundaries on the device) against
+    # the oracle's correctly rounded root, sample by sample.
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    w, h = 96, 64
+    rgb = _rgb(w, h, 29, bits)
+    img = jpxl_amd.Image(ctx, rgb, bits)
+    got = img.download(_lib.BUF_REF_XYB0)
+    _, lin, _ = orc.prepare_frame(rgb, bits)
+    want = np.zeros_like(got)
+    for y in range(h):
+        for x in range(w):
+            want[:, y, x] = orc.positive_xyb(lin[:, y, x])
+    same = np.mean(got == want)
+    ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert same >= 0.9999 and ulps.max() <= 1
+
+
 def test_score_pair_identity_and_noise(ctx):
     rng = np.random.default_rng(2)
     ref = rng.uniform(0, 1, (3, 120, 100)).astype(np.float32)
