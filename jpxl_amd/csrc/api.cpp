@@ -1587,6 +1587,10 @@ extern "C" void jpxlg_graph_destroy(jpxlg_graph* G) {
 // can be submitted between steps and join the next round.
 // ----------------------------------------------------------------------------
 
+// New requests whose structure a server step builds before starting an idle
+// lane that has no request ready.
+constexpr int kServerBootstrap = 8;
+
 struct ServerReq {
   jpxlg_img* img = nullptr;
   jpxlg_quality_request q{};
@@ -1770,28 +1774,41 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     std::lock_guard<std::mutex> lk(srv->m);
     for (ServerReq* R : finishing) R->done = true;
   }
-  // Start a round in each idle lane: structures of new requests first (on the
-  // context's stream), then one chain per (DCT form, effort) group.
+  // A new request's structure (block map, lists, CfL) is built on the
+  // context's stream; it returns false when that failed (the request then
+  // ends with the error).
+  auto build = [&](ServerReq* R) -> bool {
+    const jpxlg_quantizer q = R->pol.quantizer_at(R->rung);
+    const int prc = jpxlg_plan_structure(R->img, &q, R->pol.cover_mode, 1);
+    if (prc) {
+      std::lock_guard<std::mutex> lk(srv->m);
+      R->rc = prc;
+      R->pending = false;
+      srv->waiting--;
+      return false;
+    }
+    R->anchored = true;
+    return true;
+  };
+  // Start a round in each idle lane with the requests whose structure is
+  // built; a lane with none builds a few first.  The other new requests'
+  // structures are built below, while the rounds run.
+  bool any_failed = false;
   for (int li = 0; li < 2; li++) {
     ServerLane& l = srv->lane[li];
     if (l.busy || pending[li].empty()) continue;
     std::vector<ServerReq*> ready;
-    for (ServerReq* R : pending[li]) {
-      const jpxlg_quantizer q = R->pol.quantizer_at(R->rung);
-      if (!R->anchored) {
-        const int prc = jpxlg_plan_structure(R->img, &q, R->pol.cover_mode, 1);
-        if (prc) {
-          std::lock_guard<std::mutex> lk(srv->m);
-          R->rc = prc;
-          R->pending = false;
-          srv->waiting--;
-          continue;
-        }
-        R->anchored = true;
+    for (ServerReq* R : pending[li])
+      if (R->anchored) ready.push_back(R);
+    if (ready.empty()) {
+      int built = 0;
+      for (ServerReq* R : pending[li]) {
+        if (built >= kServerBootstrap) break;
+        if (build(R)) ready.push_back(R);
+        else any_failed = true;
+        built++;
       }
-      ready.push_back(R);
     }
-    srv->cv_done.notify_all();
     if (ready.empty()) continue;
     // one chain per lane round: group by (form, effort) and take the largest
     // group now; the rest wait for the lane's next round
@@ -1827,6 +1844,18 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     l.reqs = best->second;
     l.busy = true;
   }
+  // Structures of the remaining new requests, while the rounds run on the
+  // lanes' streams (the GPU overlaps them); stop once a round has finished.
+  for (int li = 0; li < 2; li++)
+    for (ServerReq* R : pending[li]) {
+      bool lane_done = false;
+      for (auto& l : srv->lane)
+        if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) lane_done = true;
+      if (lane_done && (srv->lane[0].busy || srv->lane[1].busy)) break;
+      if (R->anchored || !R->pending || R->in_lane) continue;
+      if (!build(R)) any_failed = true;
+    }
+  if (any_failed) srv->cv_done.notify_all();
   // Wait for the first lane round to finish (short host polls), then hand
   // back every finished round.
   if (srv->lane[0].busy || srv->lane[1].busy) {
