@@ -97,8 +97,12 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
   constexpr int VB = 64 / N, P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
   constexpr int G = kThreads / VB;  // threads per candidate for the double-double sums
   __shared__ float tile[VB * 3 * N * P];
-  __shared__ float dyh[VB][NN];
-  __shared__ double terms[VB][NN];
+  // the exact form keeps every term for its sequential sums; the
+  // double-double form needs only the reduction of the lanes' partial sums
+  constexpr int TN = EXACT ? NN : 1;
+  __shared__ float dyh[VB][TN];
+  __shared__ double terms[VB][TN];
+  __shared__ DD red[EXACT ? 1 : kThreads];
   __shared__ uint32_t bits[VB];
   __shared__ double sse[VB];
   __shared__ float vsc[VB][3];  // the candidate's step scales at its footprint's HfMul
@@ -109,11 +113,22 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
   }
   __syncthreads();
   const float* planes[3] = {X, Y, B};
-  for (int e = threadIdx.x; e < VB * 3 * NN; e += kThreads) {
-    int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
-    int dy = k / N, dx = k % N;
-    int x = min(8 * vbx[j] + dx, g.w - 1), y = min(8 * vby[j] + dy, g.h - 1);
-    tile[((j * 3 + c) * N + dy) * P + dx] = planes[c][(size_t)y * g.w + x];
+  // every sample of this thread is requested before any is stored
+  constexpr int GL = VB * 3 * NN / kThreads;
+  static_assert(GL * kThreads == VB * 3 * NN, "gather split");
+  float gv[GL];
+#pragma unroll
+  for (int i = 0; i < GL; i++) {
+    const int e = threadIdx.x + kThreads * i;
+    const int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
+    const int x = min(8 * vbx[j] + k % N, g.w - 1), y = min(8 * vby[j] + k / N, g.h - 1);
+    gv[i] = ldg32(planes[c], (uint32_t)(y * g.w + x));
+  }
+#pragma unroll
+  for (int i = 0; i < GL; i++) {
+    const int e = threadIdx.x + kThreads * i;
+    const int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
+    tile[((j * 3 + c) * N + k / N) * P + k % N] = gv[i];
   }
   __syncthreads();
   const DctTables& t = tab->dct;
@@ -135,64 +150,98 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
   __syncthreads();
   int bad = 0;
   const double tsd = (double)(N * N) * 1.0;
-  DD part{0.0, 0.0};  // this thread's share of its candidate's distortion (double-double mode)
-  for (int ci = 0; ci < 3; ci++) {
-    const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);  // Y, then X, then B
-    const float bias = kBias[c];
-    const double lw = pp->lambda[c] * tsd;
-    for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
-      int j = e / NN, k = e % NN;
-      if ((k % N) < n && (k / N) < n) {
-        terms[j][k] = 0.0;
-        continue;
+  if constexpr (EXACT) {
+    for (int ci = 0; ci < 3; ci++) {
+      const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);  // Y, then X, then B
+      const float bias = kBias[c];
+      const double lw = pp->lambda[c] * tsd;
+      for (int e = threadIdx.x; e < VB * NN; e += kThreads) {
+        int j = e / NN, k = e % NN;
+        if ((k % N) < n && (k / N) < n) {
+          terms[j][k] = 0.0;
+          continue;
+        }
+        float coef = tile[((j * 3 + c) * N + k / N) * P + k % N];
+        float target;
+        if (c == 1) target = coef;
+        else if (c == 0) target = coef - 0.0f * dyh[j][k];
+        else target = coef - 1.0f * dyh[j][k];
+        float step = vsc[j][c] * tab->matrix[c][cb + k];
+        int32_t q = choose_nearest(target, step, bias, &bad);
+        float recon = bias_adjust(q, bias) * step;
+        if (c == 1) dyh[j][k] = recon;
+        double d = (double)(recon - target);
+        terms[j][k] = lw * (double)tab->donor[cb + k] * (d * d);
+        uint32_t rb = residual_bits(q);
+        if (rb) atomicAdd(&bits[j], rb);
       }
-      float coef = tile[((j * 3 + c) * N + k / N) * P + k % N];
-      float target;
-      if (c == 1) target = coef;
-      else if (c == 0) target = coef - 0.0f * dyh[j][k];
-      else target = coef - 1.0f * dyh[j][k];
-      float step = vsc[j][c] * tab->matrix[c][cb + k];
-      int32_t q = choose_nearest(target, step, bias, &bad);
-      float recon = bias_adjust(q, bias) * step;
-      if (c == 1) dyh[j][k] = recon;
-      double d = (double)(recon - target);
-      terms[j][k] = lw * (double)tab->donor[cb + k] * (d * d);
-      uint32_t rb = residual_bits(q);
-      if (rb) atomicAdd(&bits[j], rb);
-    }
-    __syncthreads();
-    if constexpr (EXACT) {
+      __syncthreads();
       // Sequential f64 sum in cell order, one thread per candidate.  The LLF
       // cells hold +0.0, which leaves the (non-negative) sum unchanged.
       if (threadIdx.x < VB) sse[threadIdx.x] = sequential_sum(terms[threadIdx.x], NN, sse[threadIdx.x]);
-    } else {
-      const int j = threadIdx.x / G, r = threadIdx.x % G;
-      for (int k = r; k < NN; k += G) part = dd_add(part, dd_of(terms[j][k]));
+      __syncthreads();
     }
-    __syncthreads();
-  }
-  if constexpr (EXACT) {
     if (threadIdx.x < VB && valid[threadIdx.x]) {
       const int j = threadIdx.x;
       cost[(size_t)vby[j] * g.bw + vbx[j]] = (double)bits[j] * rate_scale(N) + sse[j];
     }
   } else {
-    DD* red = reinterpret_cast<DD*>(&terms[0][0]);  // kThreads double-doubles fit in terms
+    // Lane r of candidate j's G lanes takes its cells k = r + m G through
+    // all three channels (Y first: X and B subtract its reconstruction), and
+    // adds their terms to its own double-double: the exact sum does not
+    // depend on the order.  The tables of its cells are requested first.
+    constexpr int MK = (NN + G - 1) / G;
+    const int j = threadIdx.x / G, r = threadIdx.x % G;
+    float mt[3][MK], dn[MK];
+#pragma unroll
+    for (int m = 0; m < MK; m++) {
+      const int k = min(r + m * G, NN - 1);
+#pragma unroll
+      for (int c = 0; c < 3; c++) mt[c][m] = tab->matrix[c][cb + k];
+      dn[m] = tab->donor[cb + k];
+    }
+    const double lw[3] = {pp->lambda[0] * tsd, pp->lambda[1] * tsd, pp->lambda[2] * tsd};
+    DD part{0.0, 0.0};
+    uint32_t nbits = 0;
+#pragma unroll
+    for (int m = 0; m < MK; m++) {
+      const int k = r + m * G;
+      if (k >= NN || ((k % N) < n && (k / N) < n)) continue;
+      float dy = 0.0f;
+#pragma unroll
+      for (int ci = 0; ci < 3; ci++) {
+        const int c = ci == 0 ? 1 : (ci == 1 ? 0 : 2);
+        const float bias = kBias[c];
+        const float coef = tile[((j * 3 + c) * N + k / N) * P + k % N];
+        float target;
+        if (c == 1) target = coef;
+        else if (c == 0) target = coef - 0.0f * dy;
+        else target = coef - 1.0f * dy;
+        const float step = vsc[j][c] * mt[c][m];
+        const int32_t q = choose_nearest(target, step, bias, &bad);
+        const float recon = bias_adjust(q, bias) * step;
+        if (c == 1) dy = recon;
+        const double d = (double)(recon - target);
+        part = dd_add(part, dd_of(lw[c] * (double)dn[m] * (d * d)));
+        nbits += residual_bits(q);
+      }
+    }
+    if (nbits) atomicAdd(&bits[j], nbits);
     red[threadIdx.x] = part;
     __syncthreads();
     if (threadIdx.x % G == 0) {
-      const int j = threadIdx.x / G;
       DD tot = red[threadIdx.x];
-      for (int r = 1; r < G; r++) tot = dd_add(tot, red[threadIdx.x + r]);
+      for (int rr = 1; rr < G; rr++) tot = dd_add(tot, red[threadIdx.x + rr]);
       if (valid[j]) {
         const double S = tot.hi + tot.lo;
         const double nterms = 3.0 * NN + 4.0, u = 1.1102230246251565e-16;
         const double gam = 2.0 * (nterms * u / (1.0 - nterms * u));
-        const double c = (double)bits[j] * rate_scale(N) + S;
+        const double cst = (double)bits[j] * rate_scale(N) + S;
         const size_t o = (size_t)vby[j] * g.bw + vbx[j];
-        cost[o] = c;
-        // the sequential sum's distance from S, plus the roundings of S and c
-        rad[o] = gam * S + 4.0 * u * fabs(c) + 1e-300;
+        cost[o] = cst;
+        // the sequential sum's distance from S, plus the roundings of S and
+        // of the cost
+        rad[o] = gam * S + 4.0 * u * fabs(cst) + 1e-300;
       }
     }
   }
