@@ -1141,11 +1141,37 @@ __global__ __launch_bounds__(256) void k_gaborish_b(const ProbeDesc* __restrict_
 // 64x16 output tile per 256-thread workgroup (each lane four rows), halo 3.
 constexpr int kEpfTW = 64, kEpfTH = 16, kEpfR = 3, kEpfRows = kEpfTH / 4;
 
+constexpr int kEpfTapX[13] = {0, -1, 1, 0, 0, 1, 1, -1, -1, -2, 2, 0, 0};
+constexpr int kEpfTapY[13] = {0, 0, 0, -1, 1, -1, 1, 1, -1, 0, 0, 2, -2};
+
+// The channel-weighted distance of tap (kx, ky) at tile position (cx, cy):
+// the sum over the three channels and (steps 0 and 1) the five-sample cross
+// of |centre - tap| * scale, accumulated in that order.
 template <int STEP, int LH, int LW>
-__device__ __forceinline__ void epf_pixel(int w, int h, int x, int y, int tx, int ty,
-                                          const float (&tile)[3][LH][LW], const float* __restrict__ sigma,
-                                          float* __restrict__ o0, float* __restrict__ o1,
-                                          float* __restrict__ o2) {
+__device__ __forceinline__ float epf_sad(const float (&tile)[3][LH][LW], int cx, int cy, int kx, int ky) {
+  const float cscale[3] = {40.0f, 5.0f, 3.5f};
+  float d = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if constexpr (STEP == 2) {
+      d = fmaf(fabsf(tile[c][cy][cx] - tile[c][cy + ky][cx + kx]), cscale[c], d);
+    } else {
+#pragma unroll
+      for (int jx = 0; jx < 5; jx++) {
+        const int ix = kEpfTapX[jx], iy = kEpfTapY[jx];
+        d = fmaf(fabsf(tile[c][cy + iy][cx + ix] - tile[c][cy + ky + iy][cx + kx + ix]), cscale[c], d);
+      }
+    }
+  }
+  return d;
+}
+
+// One output pixel from its taps' distances dist[tp].
+template <int STEP, int LH, int LW>
+__device__ __forceinline__ void epf_pixel_d(int w, int h, int x, int y, int tx, int ty,
+                                            const float (&tile)[3][LH][LW], const float* __restrict__ sigma,
+                                            float* __restrict__ o0, float* __restrict__ o1,
+                                            float* __restrict__ o2, const float* dist) {
   if (x >= w || y >= h) return;
   const int bw = (w + 7) / 8;
   const size_t idx = (size_t)y * w + x;
@@ -1164,27 +1190,11 @@ __device__ __forceinline__ void epf_pixel(int w, int h, int x, int y, int tx, in
   const bool at_border = ry == 0 || ry == 7 || rx == 0 || rx == 7;
   const float pm = at_border ? 2.0f / 3.0f : 1.0f;
   constexpr int kTaps = STEP == 0 ? 13 : 5;
-  const int tapx[13] = {0, -1, 1, 0, 0, 1, 1, -1, -1, -2, 2, 0, 0};
-  const int tapy[13] = {0, 0, 0, -1, 1, -1, 1, 1, -1, 0, 0, 2, -2};
-  const float cscale[3] = {40.0f, 5.0f, 3.5f};
   float sum_w = 0.0f, acc0 = 0.0f, acc1 = 0.0f, acc2 = 0.0f;
 #pragma unroll
   for (int tp = 0; tp < kTaps; tp++) {
-    const int kx = tapx[tp], ky = tapy[tp];
-    float d = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 3; c++) {
-      if constexpr (STEP == 2) {
-        d = fmaf(fabsf(tile[c][cy][cx] - tile[c][cy + ky][cx + kx]), cscale[c], d);
-      } else {
-#pragma unroll
-        for (int jx = 0; jx < 5; jx++) {
-          const int ix = tapx[jx], iy = tapy[jx];
-          d = fmaf(fabsf(tile[c][cy + iy][cx + ix] - tile[c][cy + ky + iy][cx + kx + ix]), cscale[c], d);
-        }
-      }
-    }
-    const float sd = pm * d;
+    const int kx = kEpfTapX[tp], ky = kEpfTapY[tp];
+    const float sd = pm * dist[tp];
     float wgt;
     if (sd == 0.0f) {
       wgt = 1.0f;
@@ -1200,6 +1210,44 @@ __device__ __forceinline__ void epf_pixel(int w, int h, int x, int y, int tx, in
   out[0][idx] = acc0 / sum_w;
   out[1][idx] = acc1 / sum_w;
   out[2][idx] = acc2 / sum_w;
+}
+
+template <int STEP, int LH, int LW>
+__device__ __forceinline__ void epf_pixel(int w, int h, int x, int y, int tx, int ty,
+                                          const float (&tile)[3][LH][LW], const float* __restrict__ sigma,
+                                          float* __restrict__ o0, float* __restrict__ o1,
+                                          float* __restrict__ o2) {
+  constexpr int kTaps = STEP == 0 ? 13 : 5;
+  float dist[kTaps];
+#pragma unroll
+  for (int tp = 0; tp < kTaps; tp++)
+    dist[tp] = epf_sad<STEP>(tile, tx + kEpfR, ty + kEpfR, kEpfTapX[tp], kEpfTapY[tp]);
+  epf_pixel_d<STEP>(w, h, x, y, tx, ty, tile, sigma, o0, o1, o2, dist);
+}
+
+// Step 1 over a lane's column of kEpfRows pixels, sharing distances: the
+// centre tap's is 0 (the samples are finite), and the upward tap of a row is
+// the downward tap of the row above (the same pairs in the same order, and
+// |a - b| = |b - a|), so each row computes three of its five.
+template <int LH, int LW>
+__device__ __forceinline__ void epf_column_step1(int w, int h, int x, int y, int tx, int ty,
+                                                 const float (&tile)[3][LH][LW], const float* __restrict__ sigma,
+                                                 float* __restrict__ o0, float* __restrict__ o1,
+                                                 float* __restrict__ o2) {
+  // taps: 0 centre, 1 (-1,0), 2 (1,0), 3 (0,-1), 4 (0,1)
+  float up = epf_sad<1>(tile, tx + kEpfR, ty + kEpfR, 0, -1);
+#pragma unroll
+  for (int r = 0; r < kEpfRows; r++) {
+    const int cx = tx + kEpfR, cy = ty + r + kEpfR;
+    float dist[5];
+    dist[0] = 0.0f;
+    dist[1] = epf_sad<1>(tile, cx, cy, -1, 0);
+    dist[2] = epf_sad<1>(tile, cx, cy, 1, 0);
+    dist[3] = up;
+    dist[4] = epf_sad<1>(tile, cx, cy, 0, 1);
+    up = dist[4];
+    epf_pixel_d<1>(w, h, x, y + r, tx, ty + r, tile, sigma, o0, o1, o2, dist);
+  }
 }
 
 template <int STEP>
@@ -1227,8 +1275,13 @@ __device__ __forceinline__ void epf_tile(int w, int h, const float* __restrict__
   }
   __syncthreads();
   const int tx = threadIdx.x % kEpfTW;
-  for (int ty = (threadIdx.x / kEpfTW) * kEpfRows; ty < (int)(threadIdx.x / kEpfTW + 1) * kEpfRows; ty++)
-    epf_pixel<STEP>(w, h, x0 + tx, y0 + ty, tx, ty, tile, sigma, o0, o1, o2);
+  if constexpr (STEP == 1) {
+    const int ty = (threadIdx.x / kEpfTW) * kEpfRows;
+    epf_column_step1(w, h, x0 + tx, y0 + ty, tx, ty, tile, sigma, o0, o1, o2);
+  } else {
+    for (int ty = (threadIdx.x / kEpfTW) * kEpfRows; ty < (int)(threadIdx.x / kEpfTW + 1) * kEpfRows; ty++)
+      epf_pixel<STEP>(w, h, x0 + tx, y0 + ty, tx, ty, tile, sigma, o0, o1, o2);
+  }
 }
 
 template <int STEP>
