@@ -226,6 +226,7 @@ size_t image_bytes(int w, int h) {
   b += 3 * cpx * 4 * 2;              // coefficients f32 + quantized i32
   b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
   b += 3 * bw * bh * 4;              // work lists
+  b += 2 * ((bw + 255) / 256) * ((bh + 255) / 256) * 256 * 16;  // list-building row counts and bases
   b += 6 * bw * bh * 8;              // cover candidate costs (f64) and their bounds
   b += (((bw + 3) / 4) * ((bh + 3) / 4) + 1) * 4;  // cover's exact-pass queue
   b += cpx * 18 + bw * bh * (16 + 3 * 16) + 2 * ((bw + 7) / 8 + 1) * ((bh + 7) / 8 + 1) * 4;  // CfL samples
@@ -291,6 +292,7 @@ struct jpxlg_img {
   char* d_readback = nullptr;        // the structure's gathered readback block
   bool gray_known = false;           // img->gray read back from the device
   uint32_t* d_tile_base = nullptr;
+  uint4* d_rows = nullptr;           // list-building scratch: per-row counts and bases
   // Adaptive quantization: the per-atom field (bw x bh), its scratch, and
   // whether it is active (a mode other than off whose field is not neutral).
   float* aq_adj = nullptr;
@@ -493,6 +495,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_tile_count = (uint32_t*)A.take((size_t)tiles_cfl * 4);
   img->d_readback = (char*)A.take(structure_readback_bytes(tiles_cfl, (int)bpx));
   img->d_tile_base = (uint32_t*)A.take((size_t)(tiles_cfl + 1) * 4);
+  img->d_rows = (uint4*)A.take((size_t)structure_rows(g) * 2 * sizeof(uint4));
   img->d_bmap = (uint8_t*)A.take(bpx);
   img->d_cfl_x = (int8_t*)A.take(tiles_cfl);
   img->d_cfl_b = (int8_t*)A.take(tiles_cfl);
@@ -509,7 +512,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_partials = (double*)A.take(ptiles * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_rad[2] && img->d_cover_queue && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
-                img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
+                img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rows && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
                 img->aq_adj && img->d_aq;
   for (int c = 0; c < 3; c++) {
@@ -649,7 +652,7 @@ static int finish_structure(jpxlg_img* img, const jpxlg_quantizer* q) {
   {
     Prof p(ctx, "structure_lists");
     launch_structure_lists(st, g, img->d_bmap, img->d_list, img->d_lf_off, img->d_hf_local, img->d_tile_count,
-                           img->d_tile_base, img->d_si);
+                           img->d_tile_base, img->d_si, img->d_rows);
   }
   for (int i = 0; i < 3; i++) {
     Prof p(ctx, kDctNames[i]);

@@ -321,9 +321,11 @@ void launch_cover(hipStream_t s, const Geom& g, const ConstTables* tab, const Pr
 // The structure's work lists (in CfL order), counts, and each varblock's CfL
 // sample slots (LF offset per block; HF offset inside its tile, per block;
 // tile bases), from the device block map.
+// rows: 2 * structure_rows(g) uint4 of scratch (per-row counts and bases).
+int structure_rows(const Geom& g);
 void launch_structure_lists(hipStream_t s, const Geom& g, const uint8_t* bmap, uint32_t* const list[3],
                             uint32_t* lf_off, uint32_t* hf_local, uint32_t* tile_count, uint32_t* tile_base,
-                            StructInfo* si);
+                            StructInfo* si, uint4* rows);
 // One contiguous readback block of a structure: StructInfo at 0, CflParams at
 // 32, the error word at 48, cfl_x and cfl_b (tiles bytes each) from 64, then
 // the block map.

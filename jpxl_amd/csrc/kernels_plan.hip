@@ -1119,46 +1119,112 @@ __device__ __forceinline__ void scan4_1024(uint32_t v[4], uint32_t tot[4]) {
   __syncthreads();
 }
 
-__global__ __launch_bounds__(1024) void k_struct_lists(Geom g, const uint8_t* __restrict__ bmap,
-                                                       uint32_t* __restrict__ l8, uint32_t* __restrict__ l16,
-                                                       uint32_t* __restrict__ l32, uint32_t* __restrict__ lf_off,
-                                                       StructInfo* __restrict__ si) {
-  __shared__ uint32_t tot[4];
-  uint32_t carry[4] = {0, 0, 0, 0};
-  uint32_t* lists[3] = {l8, l16, l32};
-  const int gw = (g.bw + 255) / 256, gh = (g.bh + 255) / 256;
-  for (int gy = 0; gy < gh; gy++)
-    for (int gx = 0; gx < gw; gx++) {
-      const int bx0 = gx * 256, by0 = gy * 256;
-      const int wid = min(256, g.bw - bx0), hgt = min(256, g.bh - by0), nb = wid * hgt;
-      const int run = (nb + 1023) / 1024, j0 = min(nb, (int)threadIdx.x * run), j1 = min(nb, j0 + run);
-      uint32_t v[4] = {0, 0, 0, 0};  // DCT8, DCT16, DCT32 counts, LF samples
-      for (int j = j0; j < j1; j++) {
-        const int side = bmap[(size_t)(by0 + j / wid) * g.bw + bx0 + j % wid];
-        if (!side) continue;
-        v[size_index(side)]++;
-        v[3] += (uint32_t)(side / 8) * (side / 8);
-      }
-      scan4_1024(v, tot);
-      for (int j = j0; j < j1; j++) {
-        const int by = by0 + j / wid, bx = bx0 + j % wid;
-        const int side = bmap[(size_t)by * g.bw + bx];
-        if (!side) continue;
-        const int k = size_index(side);
-        lists[k][carry[k] + v[k]++] = ((uint32_t)by << 16) | (uint32_t)bx;
-        lf_off[(size_t)by * g.bw + bx] = carry[3] + v[3];
-        v[3] += (uint32_t)(side / 8) * (side / 8);
-      }
+// The varblock lists over the whole chip: a chunk is one block row of one
+// 256x256-block LF group (chunk id = group * 256 + row, groups in raster
+// order; ids past a partial group's rows are empty), so the reference's
+// list order (groups, then rows, then columns) is chunk order, then column.
+// k_rows_count counts each chunk's varblocks per size and LF samples,
+// k_rows_scan turns the counts into each chunk's first list slots, and
+// k_rows_write places every varblock with a workgroup scan of its row.
+__device__ __forceinline__ void scan4_256(uint32_t v[4], uint32_t tot[4]) {
+  __shared__ uint32_t part[4][4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint32_t inc[4];
 #pragma unroll
-      for (int k = 0; k < 4; k++) carry[k] += tot[k];
-      __syncthreads();
+  for (int k = 0; k < 4; k++) {
+    uint32_t x = v[k];
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(x, d, 64);
+      if (lane >= d) x += y;
     }
-  if (threadIdx.x == 0) {
-    si->count[0] = (int)carry[0];
-    si->count[1] = (int)carry[1];
-    si->count[2] = (int)carry[2];
-    si->lf_total = carry[3];
+    inc[k] = x;
   }
+  if (lane == 63)
+#pragma unroll
+    for (int k = 0; k < 4; k++) part[wave][k] = inc[k];
+  __syncthreads();
+  if (threadIdx.x < 4) {
+    uint32_t run = 0;
+    for (int w = 0; w < 4; w++) {
+      const uint32_t t = part[w][threadIdx.x];
+      part[w][threadIdx.x] = run;
+      run += t;
+    }
+    tot[threadIdx.x] = run;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 4; k++) v[k] = part[wave][k] + inc[k] - v[k];  // exclusive
+  __syncthreads();
+}
+
+// This thread's block of chunk blockIdx.x: side 0 when outside the frame.
+__device__ __forceinline__ int chunk_block(const Geom& g, const uint8_t* __restrict__ bmap, int* bx, int* by) {
+  const int gw = (g.bw + 255) / 256;
+  const int grp = blockIdx.x / 256, dy = blockIdx.x % 256;
+  *by = (grp / gw) * 256 + dy;
+  *bx = (grp % gw) * 256 + (int)threadIdx.x;
+  if (*by >= g.bh || *bx >= g.bw) return 0;
+  return bmap[(size_t)*by * g.bw + *bx];
+}
+
+__global__ __launch_bounds__(256) void k_rows_count(Geom g, const uint8_t* __restrict__ bmap, uint4* __restrict__ cnt) {
+  __shared__ uint32_t tot[4];
+  int bx, by;
+  const int side = chunk_block(g, bmap, &bx, &by);
+  uint32_t v[4] = {0, 0, 0, 0};
+  if (side) {
+    v[size_index(side)] = 1;
+    v[3] = (uint32_t)(side / 8) * (side / 8);
+  }
+  scan4_256(v, tot);
+  if (threadIdx.x == 0) cnt[blockIdx.x] = make_uint4(tot[0], tot[1], tot[2], tot[3]);
+}
+
+__global__ __launch_bounds__(1024) void k_rows_scan(int nch, const uint4* __restrict__ cnt, uint4* __restrict__ base,
+                                                    StructInfo* __restrict__ si) {
+  __shared__ uint32_t tot[4];
+  const int run = (nch + 1023) / 1024, j0 = min(nch, (int)threadIdx.x * run), j1 = min(nch, j0 + run);
+  uint32_t v[4] = {0, 0, 0, 0};
+  for (int j = j0; j < j1; j++) {
+    const uint4 c = cnt[j];
+    v[0] += c.x, v[1] += c.y, v[2] += c.z, v[3] += c.w;
+  }
+  scan4_1024(v, tot);
+  for (int j = j0; j < j1; j++) {
+    base[j] = make_uint4(v[0], v[1], v[2], v[3]);
+    const uint4 c = cnt[j];
+    v[0] += c.x, v[1] += c.y, v[2] += c.z, v[3] += c.w;
+  }
+  if (threadIdx.x == 0) {
+    si->count[0] = (int)tot[0];
+    si->count[1] = (int)tot[1];
+    si->count[2] = (int)tot[2];
+    si->lf_total = tot[3];
+  }
+}
+
+__global__ __launch_bounds__(256) void k_rows_write(Geom g, const uint8_t* __restrict__ bmap,
+                                                    const uint4* __restrict__ base, uint32_t* __restrict__ l8,
+                                                    uint32_t* __restrict__ l16, uint32_t* __restrict__ l32,
+                                                    uint32_t* __restrict__ lf_off) {
+  __shared__ uint32_t tot[4];
+  int bx, by;
+  const int side = chunk_block(g, bmap, &bx, &by);
+  uint32_t v[4] = {0, 0, 0, 0};
+  if (side) {
+    v[size_index(side)] = 1;
+    v[3] = (uint32_t)(side / 8) * (side / 8);
+  }
+  scan4_256(v, tot);
+  if (!side) return;
+  const uint4 b = base[blockIdx.x];
+  const int k = size_index(side);
+  uint32_t* list = k == 0 ? l8 : (k == 1 ? l16 : l32);
+  const uint32_t slot = (k == 0 ? b.x : (k == 1 ? b.y : b.z)) + v[k];
+  list[slot] = ((uint32_t)by << 16) | (uint32_t)bx;
+  lf_off[(size_t)by * g.bw + bx] = b.w + v[3];
 }
 
 // One 64-thread workgroup per 8x8-block tile: each block's HF sample offset
@@ -1256,10 +1322,15 @@ void launch_cfl_collect(hipStream_t s, const Geom& g, const ConstTables* tab, co
   if (ub[2]) k_cfl_collect<32><<<ub[2], 64, 0, s>>>(g, list[2], si, lf_off, hf_local, tile_base, tab, pp, coeff[0], coeff[1], coeff[2], lf_s, hf_s, hf_cell, err);
 }
 
+int structure_rows(const Geom& g) { return ((g.bw + 255) / 256) * ((g.bh + 255) / 256) * 256; }
+
 void launch_structure_lists(hipStream_t s, const Geom& g, const uint8_t* bmap, uint32_t* const list[3],
                             uint32_t* lf_off, uint32_t* hf_local, uint32_t* tile_count, uint32_t* tile_base,
-                            StructInfo* si) {
-  k_struct_lists<<<1, 1024, 0, s>>>(g, bmap, list[0], list[1], list[2], lf_off, si);
+                            StructInfo* si, uint4* rows) {
+  const int nch = structure_rows(g);
+  k_rows_count<<<nch, 256, 0, s>>>(g, bmap, rows);
+  k_rows_scan<<<1, 1024, 0, s>>>(nch, rows, rows + nch, si);
+  k_rows_write<<<nch, 256, 0, s>>>(g, bmap, rows + nch, list[0], list[1], list[2], lf_off);
   const int tiles = g.tw * g.th;
   k_struct_tiles<<<tiles, 64, 0, s>>>(g, bmap, hf_local, tile_count);
   k_struct_tile_base<<<1, 1024, 0, s>>>(tiles, tile_count, tile_base, si);
