@@ -537,7 +537,6 @@ struct QuantSmem {
   static constexpr int VB = quant_vb(N), NN = N * N;
   float co[3][VB][NN];
   int32_t qs[3][VB][NN];
-  float dyh[VB][NN];
   float dylf[VB][16];
   float lfs[3][VB][16], lfo[3][VB][16];
   float khf[2][VB];
@@ -578,7 +577,6 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
                 cb = kCellBase[si];
   float (*co)[VB][NN] = sm.co;
   int32_t (*qs)[VB][NN] = sm.qs;
-  float (*dyh)[NN] = sm.dyh;
   float (*dylf)[16] = sm.dylf;
   float (*lfs)[VB][16] = sm.lfs;
   float (*lfo)[VB][16] = sm.lfo;
@@ -592,16 +590,17 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
   // DCT32 (one varblock per workgroup, known to every thread): its
   // coefficients and the tables this thread needs later are requested before
   // anything waits -- the matrix at its cells k = tid + 256 i and, for the
-  // truncation walks, the order, donor and matrix entries of its positions
-  // (Y: 4 per lane over the four waves; X on waves 0-1 and B on waves 2-3:
-  // 8 per lane).
+  // luma truncation walk, the order, donor and matrix entries of its 4
+  // positions (the X/B walk's 8 per lane, X on waves 0-1 and B on waves 2-3,
+  // are requested before the X/B choices, which keeps the register count
+  // within six waves per SIMD).
   constexpr int kR = N == 32 ? 1 : 0;
   float mr[3][4 * kR + 1], cr[12 * kR + 1];
   int cellY[4 * kR + 1], cellXB[8 * kR + 1];
   float donY[4 * kR + 1], matY[4 * kR + 1], donXB[8 * kR + 1], matXB[8 * kR + 1];
   const float* cin[3] = {CX, CY, CB};
   if constexpr (N == 32) {
-    const int tid = threadIdx.x, w4 = tid >> 6, cxb = w4 < 2 ? 0 : 2;
+    const int tid = threadIdx.x;
     int bx = 0, by = 0;
     if (nvb > 0) unpack(list[base], &bx, &by);
 #pragma unroll
@@ -609,7 +608,7 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       const int k = tid + 256 * (i % 4);
       cr[i] = nvb > 0 ? ldg32(cin[i / 4], (uint32_t)((8 * by + k / N) * g.cstride + 8 * bx + k % N)) : 0.0f;
     }
-    const int py = tid * 4, pxb = ((w4 & 1) * 64 + (tid & 63)) * 8;
+    const int py = tid * 4;
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -619,12 +618,6 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       cellY[r] = tab->order[cb + py + r];
       donY[r] = tab->donor_pos[cb + py + r];
       matY[r] = tab->matrix_pos[1][cb + py + r];
-    }
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-      cellXB[r] = tab->order[cb + pxb + r];
-      donXB[r] = tab->donor_pos[cb + pxb + r];
-      matXB[r] = tab->matrix_pos[cxb][cb + pxb + r];
     }
   }
   if (threadIdx.x < VB) {
@@ -702,7 +695,6 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       int32_t v = 0;
       if ((k % N) >= n || (k / N) >= n) v = hf_choose(co[1][0][k], step, bias, &bad);
       qs[1][0][k] = v;
-      dyh[0][k] = bias_adjust(v, bias) * step;
     }
   } else {
     const float bias = kQuantBias[1];
@@ -712,7 +704,6 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       int32_t v = 0;
       if ((k % N) >= n || (k / N) >= n) v = hf_choose(co[1][j][k], step, bias, &bad);
       qs[1][j][k] = v;
-      dyh[j][k] = bias_adjust(v, bias) * step;
     }
   }
   __syncthreads();
@@ -721,12 +712,22 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
   if (pp->truncate) {
     if constexpr (N == 32) {
       // one varblock: all T/64 waves walk its luma block together
-      truncate_pre<N, 4>(qs[1][0], co[1][0], dyh[0], cellY, donY, matY, vsc[0][1], kQuantBias[1], pp->lambda[1],
+      truncate_pre<N, 4>(qs[1][0], co[1][0], nullptr, cellY, donY, matY, vsc[0][1], kQuantBias[1], pp->lambda[1],
                          pp->zero_token_bits, T / 64, wave, xs[0]);
     } else {
       for (int j = wave; j < nvb; j += T / 64)
-        truncate_wave<N>(qs[1][j], co[1][j], dyh[j], tab->order + cb, tab->matrix[1] + cb,
+        truncate_wave<N>(qs[1][j], co[1][j], nullptr, tab->order + cb, tab->matrix[1] + cb,
                          tab->donor + cb, vsc[j][1], kQuantBias[1], pp->lambda[1], pp->zero_token_bits);
+    }
+  }
+  if constexpr (N == 32) {
+    // the X/B truncation tables, requested before the X/B choices
+    const int w4 = threadIdx.x >> 6, cxb = w4 < 2 ? 0 : 2, pxb = ((w4 & 1) * 64 + (threadIdx.x & 63)) * 8;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      cellXB[r] = tab->order[cb + pxb + r];
+      donXB[r] = tab->donor_pos[cb + pxb + r];
+      matXB[r] = tab->matrix_pos[cxb][cb + pxb + r];
     }
   }
   // --- X and B LF against the reconstructed Y LF ---
@@ -749,7 +750,11 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       const int ci = i / 4, c = ci == 0 ? 0 : 2, k = threadIdx.x + 256 * (i % 4);
       int32_t v = 0;
       if ((k % N) >= n || (k / N) >= n) {
-        const float target = co[c][0][k] - khf[ci][0] * dyh[0][k];
+        // the luma reconstruction after its truncation, from qs (a truncated
+        // coefficient reconstructs to bias_adjust(0) * step = 0, as stored
+        // before)
+        const float dyk = bias_adjust(qs[1][0][k], kQuantBias[1]) * (vsc[0][1] * mr[1][i % 4]);
+        const float target = co[c][0][k] - khf[ci][0] * dyk;
         co[c][0][k] = target;
         const float step = vsc[0][c] * mr[c][i % 4];
         v = hf_choose(target, step, kQuantBias[c], &bad);
@@ -761,7 +766,8 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
     int ci = e / (VB * NN), rem = e % (VB * NN), j = rem / NN, k = rem % NN, c = ci == 0 ? 0 : 2;
     int32_t v = 0;
     if ((k % N) >= n || (k / N) >= n) {
-      float target = co[c][j][k] - khf[ci][j] * dyh[j][k];
+      const float dyk = bias_adjust(qs[1][j][k], kQuantBias[1]) * (vsc[j][1] * tab->matrix[1][cb + k]);
+      float target = co[c][j][k] - khf[ci][j] * dyk;
       co[c][j][k] = target;
       float step = vsc[j][c] * tab->matrix[c][cb + k];
       v = hf_choose(target, step, kQuantBias[c], &bad);
