@@ -268,6 +268,33 @@ def test_cfl_estimate_matches_the_oracle(ctx, tinted, gs):
     assert np.array_equal(plan.quant, oq) and np.array_equal(plan.lf, olf)
 
 
+@pytest.mark.parametrize("wh", [(2100, 72), (96, 2060), (2056, 2056)])
+def test_lists_across_lf_groups_match_the_oracle(ctx, wh):
+    # Frames wider or taller than one 256x256-block LF group: the varblock
+    # lists, CfL sample order and LF offsets run group after group, row after
+    # row; structure, CfL and quantized planes against the oracle.
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    w, h = wh
+    rgb = _tinted(w, h, 41)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    q = _quantizer(gs=900)
+    img.plan_structure(q, _lib.COVER_HIERARCHICAL, True)
+    img.quantize(q)
+    plan = img.plan_download()
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    bmap = plan.block_map
+    if w * h <= 300_000:
+        assert np.array_equal(bmap, orc.cover(xyb, _qp(q)))
+    coeff = orc.forward_frame(xyb, bmap)
+    cx, cb, lff = orc.estimate_cfl(w, h, bmap, coeff, _qp(q))
+    assert np.array_equal(plan.cfl_x, cx) and np.array_equal(plan.cfl_b, cb)
+    assert plan.lf_factors == lff
+    oq, olf = orc.quantize_frame(w, h, bmap, coeff, _qp(q), cx, cb, lff)
+    assert np.array_equal(plan.quant, oq) and np.array_equal(plan.lf, olf)
+
+
 @pytest.mark.parametrize("epf,gab", [(0, 0), (1, 0), (2, 0), (3, 0), (1, 1), (3, 1)])
 @pytest.mark.parametrize("mixed", [False, True])
 def test_render_bit_exact(ctx, epf, gab, mixed):
