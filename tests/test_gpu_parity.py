@@ -718,3 +718,63 @@ def test_autotune_keeps_the_scores(ctx):
     assert upw in (1, 2, 4) and sw in (0, 1)
     after = img.render_score(Restoration(1, 0, 7))
     assert before[0] == after[0] and np.array_equal(before[2], after[2])
+
+
+@pytest.mark.parametrize("case", range(12))
+def test_random_configurations_end_to_end(ctx, case):
+    # Seeded sweep over frame sizes, bit depths, quantizers, cover modes, CfL,
+    # AQ and restoration settings: structure, CfL, quantized planes and the
+    # restored render bit-exact against the oracle, the score within the
+    # metric tolerance.
+    import jpxl_amd
+    from jpxl_amd import Quantizer, Restoration, _lib
+
+    rng = np.random.default_rng(1000 + case)
+    w, h = int(rng.integers(9, 420)), int(rng.integers(9, 300))
+    bits = (8, 8, 12, 16)[case % 4]
+    gs = int(rng.choice([300, 900, 2000, 6000, 20000]))
+    cover = int(rng.choice([_lib.COVER_FIXED_DCT8, _lib.COVER_HIERARCHICAL]))
+    use_cfl = case % 3 != 0
+    truncate = int(rng.integers(0, 2))
+    aq = (orc.AQ_MASKING, orc.AQ_FINE_MASKING, orc.AQ_EDGE_REFINE)[case % 3] if bits == 8 else orc.AQ_OFF
+    epf, gab = int(rng.integers(0, 4)), int(rng.integers(0, 2))
+    print(f"case {case}: {w}x{h} bits {bits} gs {gs} cover {cover} cfl {use_cfl} trunc {truncate} aq {aq} "
+          f"epf {epf} gab {gab}", flush=True)
+    rgb = _rgb(w, h, 500 + case, bits)
+    img = jpxl_amd.Image(ctx, rgb, bits)
+    xyb, lin, _ = orc.prepare_frame(rgb, bits)
+    q = Quantizer(gs, 1, 4, 3, 3, truncate, 4.0, 1.0)
+    qp = _qp(q)
+    rp = orc.RenderParams(gs, 1, 4, 3, 3, epf, gab, 7, bits)
+    if aq != orc.AQ_OFF and img.set_aq(aq):
+        q = Quantizer(gs, 1, 16, 2, 2, truncate, 1.0, 1.0)
+        img.plan_structure(q, cover, use_cfl)
+        img.quantize(q)
+        _, wire, ro = img.plan_aq()
+        adj, _, fine = orc.aq_field(xyb, aq)
+        gs2, qlf2, base, oro = orc.aq_setup(gs, 16, 1, fine)
+        assert (wire.global_scale, wire.quant_lf, wire.hf_mul, ro) == (gs2, qlf2, base, oro)
+        qp = orc.with_aq(orc.QuantParams(gs2, base, qlf2, 2, 2, truncate, 1.0, 1.0), adj, oro, fine)
+        rp = orc.with_aq(orc.RenderParams(gs2, base, qlf2, 2, 2, epf, gab, 7, bits), adj, oro, fine)
+    else:
+        img.plan_structure(q, cover, use_cfl)
+    img.quantize(q)
+    plan = img.plan_download()
+    bmap = orc.cover(xyb, qp) if cover == _lib.COVER_HIERARCHICAL else orc.fixed_dct8_map(w, h)
+    assert np.array_equal(plan.block_map, bmap)
+    coeff = orc.forward_frame(xyb, bmap)
+    if use_cfl:
+        cx, cb, lff = orc.estimate_cfl(w, h, bmap, coeff, qp)
+    else:
+        cx = cb = np.zeros_like(plan.cfl_x)
+        lff = (0, 0)
+    assert np.array_equal(plan.cfl_x, cx) and np.array_equal(plan.cfl_b, cb) and plan.lf_factors == lff
+    oq, olf = orc.quantize_frame(w, h, bmap, coeff, qp, cx, cb, lff)
+    assert np.array_equal(plan.lf, olf) and np.array_equal(plan.quant, oq)
+    score, _, _ = img.render_score(Restoration(epf, gab, 7))
+    olin, oxyb = orc.render(w, h, bmap, oq, olf, rp, cx, cb, lff)
+    assert np.array_equal(img.download(_lib.BUF_RENDER_XYB).view(np.uint32), oxyb.view(np.uint32))
+    assert np.array_equal(img.download(_lib.BUF_LINEAR).view(np.uint32), olin.view(np.uint32))
+    oscore, _, _ = orc.ssimulacra2(lin, olin)
+    assert abs(score - oscore) < SCORE_TOL
+    img.close()
