@@ -1143,14 +1143,16 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
   HIP_TRY(ctx, hipMemcpyAsync(dev, H, L.off + 4 * (size_t)L.noffs, hipMemcpyHostToDevice, st));
   const ProbeDesc* D = (const ProbeDesc*)(dev + L.desc);
   const int* O = (const int*)(dev + L.off);
+  // quantize and the exact-form reconstruction in one launch when both run
+  const bool fused = quantize && r && form == kDctExact;
   if (quantize) {
-    Prof p(ctx, "quantize", st);
+    Prof p(ctx, fused ? "quantize_render" : "quantize", st);
     const int* offs[3] = {O + L.q_off[0], O + L.q_off[1], O + L.q_off[2]};
-    launch_quantize_all(st, D, offs, n, T.q, ctx->d_tab);
+    launch_quantize_all(st, D, offs, n, T.q, ctx->d_tab, fused);
   }
   int src = 0;
   if (r) {
-    {
+    if (!fused) {
       Prof p(ctx, "render", st);
       const int* offs[3] = {O + L.r_off[0], O + L.r_off[1], O + L.r_off[2]};
       launch_render_all(st, form, D, offs, n, T.r, ctx->d_tab, ctx->d_mats);

@@ -261,9 +261,11 @@ void launch_dct_blocks(hipStream_t s, int side, int inverse, int count, const Co
 // one).  `off` (device) holds the workgroup prefix of each desc, `nwg` the
 // total; src selects the restoration ping-pong input (0: rx, 1: aux).
 // Quantization of every varblock size in one launch (DCT32 workgroups first):
-// per-size prefixes off[0..2] (sides 8, 16, 32), nwg[0..2] workgroups.
+// per-size prefixes off[0..2] (sides 8, 16, 32), nwg[0..2] workgroups.  With
+// render, each workgroup also reconstructs its varblocks in the exact DCT
+// form (render planes and EPF sigma, as launch_render_all).
 void launch_quantize_all(hipStream_t s, const ProbeDesc* D, const int* const off[3], int nd, const int nwg[3],
-                         const ConstTables* tab);
+                         const ConstTables* tab, bool render);
 // Reconstruction of every varblock size in one launch (DCT32 workgroups
 // first): per-size prefixes off[0..2] (sides 8, 16, 32), nwg[0..2] workgroups.
 void launch_render_all(hipStream_t s, int form, const ProbeDesc* D, const int* const off[3], int nd,

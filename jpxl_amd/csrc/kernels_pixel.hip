@@ -534,13 +534,17 @@ __device__ void truncate_pre(int32_t* q, const float* targets, float* recon_y, c
 // LDS of one quantize workgroup of side N.
 template <int N>
 struct QuantSmem {
-  static constexpr int VB = quant_vb(N), NN = N * N;
-  float co[3][VB][NN];
+  static constexpr int VB = quant_vb(N), NN = N * N, n2 = (N / 8) * (N / 8);
+  union {
+    float co[3][VB][NN];
+    float tile[VB * 3 * N * (N + 1)];  // the fused reconstruction's IDCT tiles, after the walks
+  };
   int32_t qs[3][VB][NN];
-  float dylf[VB][16];
-  float lfs[3][VB][16], lfo[3][VB][16];
+  float dylf[VB][n2];
+  float lfs[3][VB][n2], lfo[3][VB][n2];
+  int32_t lfq[3][VB][n2];  // the LF levels, for the fused reconstruction
   float khf[2][VB];
-  float vsc[VB][3];  // each varblock's step scales (its own HfMul under AQ)
+  float vsc[VB][4];  // each varblock's step scales (its own HfMul under AQ) and EPF sigma
   int vbx[VB], vby[VB];
   int xs[2][8];
 };
@@ -551,7 +555,112 @@ union QuantSmemAll {
 };
 
 // One quantize workgroup (logical index `block` of the size's prefix `off`).
-template <int N>
+// The reconstruction of a quantize workgroup's varblocks from the levels it
+// has just chosen (still in LDS): the stages of render_body -- I.5.3
+// dequantization with the HF CfL, the LLF cells from the LF levels, the
+// I.9 inverse DCT and the EPF sigma -- with the same operations, so the
+// planes equal the render kernel's bit for bit.  T threads, VB varblocks.
+template <int N, int T>
+__device__ __forceinline__ void render_from_levels(const Geom& g, const ConstTables* __restrict__ tab,
+                                                   const ProbeParams* __restrict__ pp,
+                                                   const CflParams* __restrict__ cflp, QuantSmem<N>& sm, int nvb,
+                                                   float* __restrict__ X, float* __restrict__ Y,
+                                                   float* __restrict__ B, float* __restrict__ sigma) {
+  constexpr int VB = quant_vb(N), P = N + 1, NN = N * N, n = N / 8, si = size_index(N), cb = kCellBase[si];
+  float* tile = sm.tile;
+  for (int e = threadIdx.x; e < VB * NN; e += T) {
+    const int j = e / NN, k = e % NN;
+    float d0 = 0.0f, d1 = 0.0f, d2 = 0.0f;
+    if (j < nvb) {
+      d0 = bias_adjust(sm.qs[0][j][k], kQuantBias[0]) * sm.vsc[j][0] * tab->matrix[0][cb + k];
+      d1 = bias_adjust(sm.qs[1][j][k], kQuantBias[1]) * sm.vsc[j][1] * tab->matrix[1][cb + k];
+      d2 = bias_adjust(sm.qs[2][j][k], kQuantBias[2]) * sm.vsc[j][2] * tab->matrix[2][cb + k];
+    }
+    const int r = k / N, col = k % N;
+    tile[((j * 3 + 0) * N + r) * P + col] = d0 + sm.khf[0][j] * d1;
+    tile[((j * 3 + 1) * N + r) * P + col] = d1;
+    tile[((j * 3 + 2) * N + r) * P + col] = d2 + sm.khf[1][j] * d1;
+  }
+  __syncthreads();
+  const DctTables& t = tab->dct;
+  {
+    const float kxl = 0.0f + (float)cflp->x_factor_lf / 84.0f;
+    const float kbl = 1.0f + (float)cflp->b_factor_lf / 84.0f;
+    const float* sf = tab->scale_f[si];
+    if constexpr (n == 1) {
+      if (threadIdx.x < 3 * VB) {
+        const int j = threadIdx.x / 3, c = threadIdx.x % 3;
+        if (j < nvb) {
+          const float vy = pp->mdc[1] * (float)sm.lfq[1][j][0] / 1.0f;
+          float v;
+          if (c == 1) v = vy;
+          else if (c == 0) v = pp->mdc[0] * (float)sm.lfq[0][j][0] / 1.0f + kxl * vy;
+          else v = pp->mdc[2] * (float)sm.lfq[2][j][0] / 1.0f + kbl * vy;
+          tile[(j * 3 + c) * N * P] = v;
+        }
+      }
+    } else {
+      const float f = fwd_factor<n>();
+      const int j = threadIdx.x / (3 * n), c = (threadIdx.x / n) % 3, line = threadIdx.x % n;
+      const bool act = threadIdx.x < 3 * VB * n && j < nvb;
+      float (*llf)[n * n] = sm.lfs[0];  // the LF passes' scratch is free again
+      if (act) {
+        float v[n];
+#pragma unroll
+        for (int r = 0; r < n; r++) {
+          const int i = r * n + line;
+          const float vy = pp->mdc[1] * (float)sm.lfq[1][j][i] / 1.0f;
+          if (c == 1) v[r] = vy;
+          else if (c == 0) v[r] = pp->mdc[0] * (float)sm.lfq[0][j][i] / 1.0f + kxl * vy;
+          else v[r] = pp->mdc[2] * (float)sm.lfq[2][j][i] / 1.0f + kbl * vy;
+        }
+        dct_ii<n>(v, t);
+#pragma unroll
+        for (int r = 0; r < n; r++) llf[j * 3 + c][r * n + line] = v[r] * f;
+      }
+      __syncthreads();
+      if (act) {
+        float v[n];
+#pragma unroll
+        for (int k = 0; k < n; k++) v[k] = llf[j * 3 + c][line * n + k];
+        dct_ii<n>(v, t);
+        float* tl = tile + (j * 3 + c) * N * P;
+#pragma unroll
+        for (int k = 0; k < n; k++) tl[k * P + line] = v[k] * f * sf[k] * sf[line];
+      }
+    }
+  }
+  __syncthreads();
+  {
+    const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
+    const bool act = threadIdx.x < VB * 3 * N;
+    float* tl = tile + (act ? (j * 3 + c) * N * P : 0);
+    if (act) inv_pass1<N>(tl, line, t);
+    __syncthreads();
+    if (act && j < nvb) {
+      float out[N];
+      inv_pass2<N>(tl, line, out, t);
+      float* dst = c == 0 ? X : (c == 1 ? Y : B);
+      const int fx = 8 * sm.vbx[j] + line;
+      if (fx < g.w) {
+#pragma unroll
+        for (int r = 0; r < N; r++) {
+          const int fy = 8 * sm.vby[j] + r;
+          if (fy < g.h) dst[(size_t)fy * g.w + fx] = out[r];
+        }
+      }
+    }
+  }
+  if (threadIdx.x < VB * n * n) {
+    const int jj = threadIdx.x / (n * n), b = threadIdx.x % (n * n);
+    if (jj < nvb) {
+      const int sx = sm.vbx[jj] + b % n, sy = sm.vby[jj] + b / n;
+      if (sx < g.bw && sy < g.bh) sigma[(size_t)sy * g.bw + sx] = sm.vsc[jj][3];
+    }
+  }
+}
+
+template <int N, bool RENDER>
 __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, const int* __restrict__ off, int nd,
                                               int block, const ConstTables* __restrict__ tab, QuantSmem<N>& sm) {
   const int d = find_desc(off, nd, block);
@@ -577,11 +686,12 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
                 cb = kCellBase[si];
   float (*co)[VB][NN] = sm.co;
   int32_t (*qs)[VB][NN] = sm.qs;
-  float (*dylf)[16] = sm.dylf;
-  float (*lfs)[VB][16] = sm.lfs;
-  float (*lfo)[VB][16] = sm.lfo;
+  constexpr int n2 = n * n;
+  float (*dylf)[n2] = sm.dylf;
+  float (*lfs)[VB][n2] = sm.lfs;
+  float (*lfo)[VB][n2] = sm.lfo;
   float (*khf)[VB] = sm.khf;
-  float (*vsc)[3] = sm.vsc;
+  float (*vsc)[4] = sm.vsc;
   int* vbx = sm.vbx;
   int* vby = sm.vby;
   const int base = (block - off[d]) * VB;
@@ -628,7 +738,7 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
     int tile = (by / 8) * g.tw + bx / 8;
     khf[0][threadIdx.x] = 0.0f + (float)cfl_x[tile] / 84.0f;
     khf[1][threadIdx.x] = 1.0f + (float)cfl_b[tile] / 84.0f;
-    varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], nullptr);
+    varblock_scales(pp, g.bw, g.bh, bx, by, n, vsc[threadIdx.x], RENDER ? &vsc[threadIdx.x][3] : nullptr);
   }
   if constexpr (N == 32) {
 #pragma unroll
@@ -683,6 +793,7 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
     if (j < nvb) {
       int32_t v = clamp_round(lfo[1][j][i] * 1.0f / mdc[1], &bad);
       LY[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = v;
+      if constexpr (RENDER) sm.lfq[1][j][i] = v;
       dylf[j][i] = mdc[1] * (float)v / 1.0f;
     }
   }
@@ -738,7 +849,9 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
       if (j < nvb) {
         const float target = lfo[c][j][i] - klf[ci] * dylf[j][i];
         int32_t* L = ci == 0 ? LX : LB;
-        L[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = clamp_round(target * 1.0f / mdc[c], &bad);
+        const int32_t v = clamp_round(target * 1.0f / mdc[c], &bad);
+        L[(size_t)(vby[j] + i / n) * g.bw + vbx[j] + i % n] = v;
+        if constexpr (RENDER) sm.lfq[c][j][i] = v;
       }
     }
   }
@@ -797,25 +910,30 @@ __device__ __forceinline__ void quantize_body(const ProbeDesc* __restrict__ D, c
     if (j < nvb) qout[c][(size_t)(8 * vby[j] + k / N) * g.cstride + 8 * vbx[j] + k % N] = qs[c][j][k];
   }
   if (bad) atomicOr(err, kErrQuantRange);
+  if constexpr (RENDER) render_from_levels<N, T>(g, tab, pp, cflp, sm, nvb, P.rx[0], P.rx[1], P.rx[2], P.sigma);
 }
 
 // ----------------------------------------------------------------------------
 
+// RENDER: the reconstruction of the exact DCT form follows in the same
+// workgroup, from the levels in LDS (the render kernel's work for these
+// varblocks, without reading the levels back from memory).
+template <bool RENDER>
 __global__ __launch_bounds__(256) void k_quantize_all(const ProbeDesc* __restrict__ D, const int* __restrict__ off8,
                                                       const int* __restrict__ off16, const int* __restrict__ off32,
                                                       int nd, int n32, int n16, const ConstTables* __restrict__ tab) {
   __shared__ QuantSmemAll sm;
   int b = blockIdx.x;  // DCT32 workgroups first (the longest), then DCT16, then DCT8
   if (b < n32) {
-    quantize_body<32>(D, off32, nd, b, tab, sm.s32);
+    quantize_body<32, RENDER>(D, off32, nd, b, tab, sm.s32);
     return;
   }
   b -= n32;
   if (b < n16) {
-    quantize_body<16>(D, off16, nd, b, tab, sm.s16);
+    quantize_body<16, RENDER>(D, off16, nd, b, tab, sm.s16);
     return;
   }
-  quantize_body<8>(D, off8, nd, b - n16, tab, sm.s8);
+  quantize_body<8, RENDER>(D, off8, nd, b - n16, tab, sm.s8);
 }
 
 // ----------------------------------------------------------------------------
@@ -1378,10 +1496,13 @@ int quant_groups(int side, int count) {
 int render_groups(int side, int count) { return (int)div_up(count, 64 / side); }
 
 void launch_quantize_all(hipStream_t s, const ProbeDesc* D, const int* const off[3], int nd, const int nwg[3],
-                         const ConstTables* tab) {
+                         const ConstTables* tab, bool render) {
   const int total = nwg[0] + nwg[1] + nwg[2];
   if (total <= 0) return;
-  k_quantize_all<<<total, 256, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab);
+  if (render)
+    k_quantize_all<true><<<total, 256, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab);
+  else
+    k_quantize_all<false><<<total, 256, 0, s>>>(D, off[0], off[1], off[2], nd, nwg[2], nwg[1], tab);
 }
 
 void launch_render_all(hipStream_t s, int form, const ProbeDesc* D, const int* const off[3], int nd,

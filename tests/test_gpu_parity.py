@@ -778,3 +778,40 @@ def test_random_configurations_end_to_end(ctx, case):
     oscore, _, _ = orc.ssimulacra2(lin, olin)
     assert abs(score - oscore) < SCORE_TOL
     img.close()
+
+
+@pytest.mark.parametrize("aq", [False, True])
+@pytest.mark.parametrize("epf", [0, 1, 3])
+def test_fused_quantize_render_matches_the_separate_stages(ctx, aq, epf):
+    # A probe chain quantizes and reconstructs the exact DCT form in one
+    # launch (the reconstruction works from the levels in LDS); its planes,
+    # levels and score must equal the separate quantize and render stages,
+    # and the render must equal the oracle's.
+    import jpxl_amd
+    from jpxl_amd import Quantizer, Restoration, _lib
+
+    w, h = 237, 165
+    rgb = _rgb(w, h, 77)
+    q = Quantizer(1200, 1, 16 if aq else 4, 2 if aq else 3, 2 if aq else 3, 1, 1.0 if aq else 4.0, 1.0)
+    r = Restoration(epf, 0, 7)
+    imgs = []
+    for _ in range(2):
+        img = jpxl_amd.Image(ctx, rgb, 8)
+        if aq:
+            assert img.set_aq(orc.AQ_MASKING)
+        img.plan_structure(q, _lib.COVER_HIERARCHICAL, True)
+        imgs.append(img)
+    fused_score = jpxl_amd.batch_probe([imgs[0]], [q], r)[0]
+    imgs[1].quantize(q)
+    sep_score, _, _ = imgs[1].render_score(r)
+    assert fused_score == sep_score
+    for buf in (_lib.BUF_RENDER_XYB, _lib.BUF_LINEAR, _lib.BUF_SIGMA):
+        assert np.array_equal(imgs[0].download(buf).view(np.uint32), imgs[1].download(buf).view(np.uint32))
+    p0, p1 = imgs[0].plan_download(), imgs[1].plan_download()
+    assert np.array_equal(p0.quant, p1.quant) and np.array_equal(p0.lf, p1.lf)
+    if not aq:
+        rp = orc.RenderParams(q.global_scale, q.hf_mul, q.quant_lf, q.x_qm_scale, q.b_qm_scale, epf, 0, 7, 8)
+        olin, oxyb = orc.render(w, h, p1.block_map, p1.quant, p1.lf, rp, p1.cfl_x, p1.cfl_b, p1.lf_factors)
+        assert np.array_equal(imgs[0].download(_lib.BUF_RENDER_XYB).view(np.uint32), oxyb.view(np.uint32))
+    for img in imgs:
+        img.close()
