@@ -1135,6 +1135,12 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
 // stage launches (quantize when `quantize`, reconstruction + restoration +
 // SSIMULACRA2 when r), and the download of the error words (and sums).
 // *src = 1 when the restored planes end in the descriptors' aux planes.
+// Whether the image's device levels are those of quantizer q (its last
+// quantization, on the current structure).
+static bool holds_levels_of(const jpxlg_img* img, const jpxlg_quantizer& q) {
+  return img->has_quant && memcmp(&img->last_q, &q, sizeof(q)) == 0;
+}
+
 static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, const jpxlg_restoration* r, int form,
                        char* H, char* dev, const ChainLayout& L, const ChainTotals& T, int* src_out) {
   int nsteps, steps[3];
@@ -1397,10 +1403,13 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
   int rc = req->mode == 1 ? bisect_controller(req->target, req->max_probes, probe, &oc)
                           : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc);
   if (rc) return rc;
-  // Leave the chosen plan quantized on the device for the hand-off.
+  // Leave the chosen plan quantized on the device for the hand-off (the
+  // last probe's levels already are, when it probed the chosen rung).
   jpxlg_quantizer qc = pol.quantizer_at(oc.rung);
-  rc = jpxlg_quantize(img, &qc);
-  if (rc) return rc;
+  if (!holds_levels_of(img, qc)) {
+    rc = jpxlg_quantize(img, &qc);
+    if (rc) return rc;
+  }
   out->score = oc.score;
   out->rung = oc.rung;
   out->global_scale = qc.global_scale;
@@ -1754,8 +1763,10 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     memset(&o, 0, sizeof(o));
     if (R->nav_rc) continue;
     const jpxlg_quantizer qc = R->pol.quantizer_at(R->oc.rung);
-    fimgs.push_back(R->img);
-    fqs.push_back(qc);
+    if (!holds_levels_of(R->img, qc)) {
+      fimgs.push_back(R->img);
+      fqs.push_back(qc);
+    }
     o.score = R->oc.score;
     o.rung = R->oc.rung;
     o.global_scale = qc.global_scale;
