@@ -589,15 +589,20 @@ __global__ __launch_bounds__(64) void k_cfl_collect(Geom g, const uint32_t* __re
 // (kCflThreads-wide) workgroup must call it.
 constexpr int kCflThreads = 320;
 constexpr int kCflStage = 1024;
+// The per-tile kernel's staging: 5 x 1000 doubles (the fallback's chunks and
+// the fast path's 320 x 7 double-double tree fit), so that four 320-thread
+// workgroups share a CU's LDS.
+constexpr int kCflTileStage = 1000;
 
 struct CflSums {
   double yy, ycx, ccx, ycb, ccb;
 };
 
+template <int kStage>
 __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0, uint32_t b1,
-                                       double (*prod)[kCflStage]) {
+                                       double (*prod)[kStage]) {
   __shared__ double total[5];
-  constexpr int kPer = (kCflStage + kCflThreads - 1) / kCflThreads;  // samples per thread per chunk
+  constexpr int kPer = (kStage + kCflThreads - 1) / kCflThreads;  // samples per thread per chunk
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   double acc = 0.0;
   // Each thread prefetches its share of the next chunk into registers while
@@ -606,10 +611,10 @@ __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0
 #pragma unroll
   for (int k = 0; k < kPer; k++) {
     const uint32_t i = b0 + threadIdx.x + k * kCflThreads;
-    pre[k] = i < b1 && threadIdx.x + k * kCflThreads < kCflStage ? s[i] : make_float4(0, 0, 0, 0);
+    pre[k] = i < b1 && threadIdx.x + k * kCflThreads < kStage ? s[i] : make_float4(0, 0, 0, 0);
   }
-  for (uint32_t c0 = b0; c0 < b1; c0 += kCflStage) {
-    const uint32_t m = min((uint32_t)kCflStage, b1 - c0);
+  for (uint32_t c0 = b0; c0 < b1; c0 += kStage) {
+    const uint32_t m = min((uint32_t)kStage, b1 - c0);
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       const uint32_t i = threadIdx.x + k * kCflThreads;
@@ -623,11 +628,11 @@ __device__ CflSums cfl_sequential_sums(const float4* __restrict__ s, uint32_t b0
       }
     }
     __syncthreads();
-    const uint32_t n0 = c0 + kCflStage;
+    const uint32_t n0 = c0 + kStage;
 #pragma unroll
     for (int k = 0; k < kPer; k++) {
       const uint32_t i = n0 + threadIdx.x + k * kCflThreads;
-      if (n0 < b1 && i < b1 && threadIdx.x + k * kCflThreads < kCflStage) pre[k] = s[i];
+      if (n0 < b1 && i < b1 && threadIdx.x + k * kCflThreads < kStage) pre[k] = s[i];
     }
     if (lane == 0 && wave < 5) acc = sequential_sum(prod[wave], (int)m, acc);
     __syncthreads();
@@ -684,7 +689,7 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
                          const ConstTables* __restrict__ tab, const ProbeParams* __restrict__ pp,
                          const float4* __restrict__ hf_s, const uint16_t* __restrict__ hf_cell,
                          int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
-                         double (*stage)[kCflStage], const int* __restrict__ gray) {
+                         double (*stage)[kCflTileStage], const int* __restrict__ gray) {
   __shared__ int cands[2][4];
   __shared__ int ncand[2];
   __shared__ unsigned long long cost[2][4];
@@ -709,6 +714,7 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
 #pragma unroll
       for (int k = 0; k < 7; k++) acc[k] = dd_add(acc[k], dd_of(t[k]));
     }
+    static_assert(5 * kCflTileStage * sizeof(double) >= kCflThreads * 7 * sizeof(DD), "tree fits the staging");
     DD (*red)[7] = reinterpret_cast<DD (*)[7]>(&stage[0][0]);  // 320 x 7 DD fit the staging
 #pragma unroll
     for (int k = 0; k < 7; k++) red[threadIdx.x][k] = acc[k];
@@ -737,7 +743,7 @@ __device__ void cfl_tile(int tile, const uint32_t* __restrict__ tile_base,
     __syncthreads();
   }
   if (!seeds[2]) {
-    const CflSums a = cfl_sequential_sums(hf_s, b0, b1, stage);
+    const CflSums a = cfl_sequential_sums<kCflTileStage>(hf_s, b0, b1, stage);
     if (threadIdx.x == 0) {
       // both channels share s_yy (same sample sequence)
       seeds[0] = b1 > b0 ? best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f) : 0;
@@ -875,7 +881,7 @@ __global__ __launch_bounds__(kCflThreads) void k_cfl_lf_seed(const StructInfo* _
   const uint32_t count = si->lf_total;
   __shared__ double stage[5][kCflStage];
   if (sc->certified) return;  // the fast path's seeds stand
-  const CflSums a = cfl_sequential_sums(lf_s, 0, count, stage);
+  const CflSums a = cfl_sequential_sums<kCflStage>(lf_s, 0, count, stage);
   if (threadIdx.x == 0) {
     sc->ncand[0] = factor_candidates(best_factor_dev(a.yy, a.ycx, a.ccx, 0.0f), sc->cands[0]);
     sc->ncand[1] = factor_candidates(best_factor_dev(a.yy, a.ycb, a.ccb, 1.0f), sc->cands[1]);
@@ -946,14 +952,14 @@ __global__ void k_cfl_lf_select(const CflLfScratch* __restrict__ sc, CflParams* 
   out->b_factor_lf = bb;
 }
 
-__global__ __launch_bounds__(kCflThreads) void k_cfl_tiles(const uint32_t* __restrict__ tile_base, int tiles,
+__global__ __launch_bounds__(kCflThreads) __attribute__((amdgpu_waves_per_eu(5))) void k_cfl_tiles(const uint32_t* __restrict__ tile_base, int tiles,
                                                            const ConstTables* __restrict__ tab,
                                                            const ProbeParams* __restrict__ pp,
                                                            const float4* __restrict__ hf_s,
                                                            const uint16_t* __restrict__ hf_cell,
                                                            int8_t* __restrict__ cfl_x, int8_t* __restrict__ cfl_b,
                                                            const int* __restrict__ gray) {
-  __shared__ double stage[5][kCflStage];
+  __shared__ double stage[5][kCflTileStage];
   if ((int)blockIdx.x < tiles)
     cfl_tile((int)blockIdx.x, tile_base, tab, pp, hf_s, hf_cell, cfl_x, cfl_b, stage, gray);
 }
