@@ -34,6 +34,11 @@ struct ProfRec {
   hipEvent_t a, b;
 };
 
+// Released image arenas kept for reuse per context: up to half of the device
+// memory (set at context creation; at least 16 GiB), so a batch or a served
+// stream that cycles many images does not allocate and free them again.
+constexpr size_t kSpareCapMin = (size_t)16 << 30;
+
 struct jpxlg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -59,6 +64,7 @@ struct jpxlg_ctx {
   // of a similar size (size -> device pointer); freed with the context.
   std::multimap<size_t, void*> spare;
   size_t spare_bytes = 0;
+  size_t spare_cap = kSpareCapMin;
   // Ring of upload slots for the probe chain's descriptors (device buffer,
   // host staging, completion event); a slot is reused only after its event.
   struct Slot {
@@ -88,8 +94,6 @@ struct jpxlg_ctx {
 
 namespace {
 
-// At most this much device memory is kept in released arenas per context.
-constexpr size_t kSpareCap = (size_t)16 << 30;
 
 // Arena sizes are rounded up to classes of 2^k x {1, 1.25, 1.5, 1.75} so a
 // released arena fits the next image of a similar size (at most 25% slack).
@@ -116,13 +120,13 @@ void* take_spare(jpxlg_ctx* ctx, size_t bytes, size_t* got) {
 
 void release_arena(jpxlg_ctx* ctx, void* base, size_t bytes) {
   if (!base) return;
-  while (ctx->spare_bytes + bytes > kSpareCap && !ctx->spare.empty()) {
+  while (ctx->spare_bytes + bytes > ctx->spare_cap && !ctx->spare.empty()) {
     auto it = ctx->spare.begin();
     (void)hipFree(it->second);
     ctx->spare_bytes -= it->first;
     ctx->spare.erase(it);
   }
-  if (bytes > kSpareCap) {
+  if (bytes > ctx->spare_cap) {
     (void)hipFree(base);
     return;
   }
@@ -336,6 +340,10 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   }
   ctx->device = device;
   HIP_TRY(ctx.get(), hipSetDevice(device));
+  {
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) == hipSuccess) ctx->spare_cap = std::max(kSpareCapMin, total_b / 2);
+  }
   HIP_TRY(ctx.get(), hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   HIP_TRY(ctx.get(), hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking));
   HIP_TRY(ctx.get(), hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
@@ -1619,6 +1627,7 @@ struct ServerReq {
   bool finished = false;  // navigator returned
   bool done = false;      // result available
   int nav_rc = 0;
+  std::condition_variable cv;  // its navigator waits here for its probe's result
   ControllerOutcome oc;
   jpxlg_quality_result res{};
 };
@@ -1636,7 +1645,7 @@ struct ServerLane {
 struct jpxlg_server {
   jpxlg_ctx* ctx = nullptr;
   std::mutex m;
-  std::condition_variable cv_req, cv_done;
+  std::condition_variable cv_req;
   int active = 0, waiting = 0;
   uint64_t next_id = 1;
   std::map<uint64_t, std::unique_ptr<ServerReq>> reqs;
@@ -1690,7 +1699,7 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
       R->pending = true;
       srv->waiting++;
       srv->cv_req.notify_all();
-      srv->cv_done.wait(lk, [R] { return !R->pending; });
+      R->cv.wait(lk, [R] { return !R->pending; });
       *score = R->score;
       *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
       return R->rc;
@@ -1726,9 +1735,10 @@ static int server_collect(jpxlg_server* srv, ServerLane& l) {
       srv->waiting--;
     }
   }
+  // wake exactly this round's navigators
+  for (ServerReq* R : l.reqs) R->cv.notify_one();
   l.reqs.clear();
   l.busy = false;
-  srv->cv_done.notify_all();
   return rc;
 }
 
@@ -1801,6 +1811,7 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
       R->rc = prc;
       R->pending = false;
       srv->waiting--;
+      R->cv.notify_one();
       return false;
     }
     R->anchored = true;
@@ -1809,7 +1820,6 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
   // Start a round in each idle lane with the requests whose structure is
   // built; a lane with none builds a few first.  The other new requests'
   // structures are built below, while the rounds run.
-  bool any_failed = false;
   for (int li = 0; li < 2; li++) {
     ServerLane& l = srv->lane[li];
     if (l.busy || pending[li].empty()) continue;
@@ -1821,7 +1831,6 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
       for (ServerReq* R : pending[li]) {
         if (built >= kServerBootstrap) break;
         if (build(R)) ready.push_back(R);
-        else any_failed = true;
         built++;
       }
     }
@@ -1848,8 +1857,8 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
         R->rc = brc;
         R->pending = false;
         srv->waiting--;
+        R->cv.notify_one();
       }
-      srv->cv_done.notify_all();
       status = brc;
       continue;
     }
@@ -1869,9 +1878,8 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
         if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) lane_done = true;
       if (lane_done && (srv->lane[0].busy || srv->lane[1].busy)) break;
       if (R->anchored || !R->pending || R->in_lane) continue;
-      if (!build(R)) any_failed = true;
+      build(R);  // a failure ends that request (its navigator is woken)
     }
-  if (any_failed) srv->cv_done.notify_all();
   // Wait for the first lane round to finish (short host polls), then hand
   // back every finished round.
   if (srv->lane[0].busy || srv->lane[1].busy) {
