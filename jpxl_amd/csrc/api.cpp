@@ -66,16 +66,21 @@ struct jpxlg_ctx {
   size_t spare_bytes = 0;
   size_t spare_cap = kSpareCapMin;
   // Ring of upload slots for the probe chain's descriptors (device buffer,
-  // host staging, completion event); a slot is reused only after its event.
+  // host staging, completion event).  A slot is claimed from the chain's
+  // enqueue until its results are read back (a server lane's round can stay
+  // claimed across many other chains); the ring hands out only unclaimed
+  // slots, waits for a slot's last use before reusing it, and grows when
+  // every slot is claimed.  Slots are never moved (chains hold pointers).
   struct Slot {
     char* dev = nullptr;
     char* host = nullptr;  // pinned staging of the same size
     size_t cap = 0;
     hipEvent_t done = nullptr;
     bool pending = false;
+    bool claimed = false;
   };
-  Slot slots[8];
-  int next_slot = 0;
+  std::vector<std::unique_ptr<Slot>> slots;
+  size_t next_slot = 0;
   // Pinned staging: a ring for small uploads (probe parameters of the
   // structure passes) and a block for structure readbacks.
   char* up_host = nullptr;
@@ -132,6 +137,23 @@ void release_arena(jpxlg_ctx* ctx, void* base, size_t bytes) {
   }
   ctx->spare.emplace(bytes, base);
   ctx->spare_bytes += bytes;
+}
+
+// hipMalloc that, when the device is out of memory, frees the context's
+// spare arenas (largest first) and tries again before giving up.  A failed
+// attempt's error is cleared so later launch checks do not report it.
+bool device_alloc(jpxlg_ctx* ctx, void** p, size_t bytes) {
+  *p = nullptr;
+  for (;;) {
+    if (hipMalloc(p, bytes) == hipSuccess) return true;
+    (void)hipGetLastError();
+    *p = nullptr;
+    if (ctx->spare.empty()) return false;
+    auto it = std::prev(ctx->spare.end());
+    (void)hipFree(it->second);
+    ctx->spare_bytes -= it->first;
+    ctx->spare.erase(it);
+  }
 }
 
 }  // namespace
@@ -362,6 +384,7 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   srgb8_lut(lut8);
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_lut8, sizeof(lut8)));
   HIP_TRY(ctx.get(), hipMemcpy(ctx->d_lut8, lut8, sizeof(lut8), hipMemcpyHostToDevice));
+  for (int i = 0; i < 8; i++) ctx->slots.emplace_back(new jpxlg_ctx::Slot());
   ctx->up_cap = (size_t)256 << 10;
   HIP_TRY(ctx.get(), hipHostMalloc((void**)&ctx->up_host, ctx->up_cap, hipHostMallocDefault));
   *out = ctx.release();
@@ -382,9 +405,9 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   }
   for (auto& kv : ctx->spare) (void)hipFree(kv.second);
   for (auto& sl : ctx->slots) {
-    if (sl.dev) (void)hipFree(sl.dev);
-    if (sl.host) (void)hipHostFree(sl.host);
-    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl->dev) (void)hipFree(sl->dev);
+    if (sl->host) (void)hipHostFree(sl->host);
+    if (sl->done) (void)hipEventDestroy(sl->done);
   }
   if (ctx->up_host) (void)hipHostFree(ctx->up_host);
   if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
@@ -463,7 +486,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   scale_dims(g.w, g.h, &img->nscales, img->sw, img->sh);
   size_t bytes = arena_class(image_bytes(g.w, g.h));
   void* base = take_spare(ctx, bytes, &bytes);
-  if (!base && hipMalloc(&base, bytes) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
+  if (!base && !device_alloc(ctx, &base, bytes)) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
   img->arena.base = (char*)base;
   img->arena.size = bytes;
   Arena& A = img->arena;
@@ -997,10 +1020,25 @@ ChainLayout chain_layout(int n) {
 
 }  // namespace
 
-// Takes the next upload slot with room for `bytes` (waiting for its last use).
+// Takes the next unclaimed upload slot with room for `bytes` (waiting for
+// its last use) and claims it; read_chain releases it.  With every slot
+// claimed the ring grows by one.
 static int take_slot(jpxlg_ctx* ctx, size_t bytes, jpxlg_ctx::Slot** out) {
-  jpxlg_ctx::Slot& sl = ctx->slots[ctx->next_slot];
-  ctx->next_slot = (ctx->next_slot + 1) % 8;
+  const size_t n = ctx->slots.size();
+  jpxlg_ctx::Slot* pick = nullptr;
+  for (size_t i = 0; i < n && !pick; i++) {
+    jpxlg_ctx::Slot* c = ctx->slots[(ctx->next_slot + i) % n].get();
+    if (!c->claimed) {
+      pick = c;
+      ctx->next_slot = (ctx->next_slot + i + 1) % n;
+    }
+  }
+  if (!pick) {
+    ctx->slots.emplace_back(new jpxlg_ctx::Slot());
+    pick = ctx->slots.back().get();
+    ctx->next_slot = 0;
+  }
+  jpxlg_ctx::Slot& sl = *pick;
   if (sl.pending) HIP_TRY(ctx, hipEventSynchronize(sl.done));
   sl.pending = false;
   if (!sl.done) HIP_TRY(ctx, hipEventCreateWithFlags(&sl.done, hipEventDisableTiming));
@@ -1010,11 +1048,14 @@ static int take_slot(jpxlg_ctx* ctx, size_t bytes, jpxlg_ctx::Slot** out) {
     sl.dev = sl.host = nullptr;
     sl.cap = 0;
     const size_t cap = std::max(bytes, (size_t)64 << 10);
-    if (hipMalloc(&sl.dev, cap) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "chain slot allocation failed");
-    if (hipHostMalloc(&sl.host, cap, hipHostMallocDefault) != hipSuccess)
+    if (!device_alloc(ctx, (void**)&sl.dev, cap)) return fail(ctx, JPXLG_E_OOM, "chain slot allocation failed");
+    if (hipHostMalloc(&sl.host, cap, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
       return fail(ctx, JPXLG_E_OOM, "chain slot staging allocation failed");
+    }
     sl.cap = cap;
   }
+  sl.claimed = true;
   *out = &sl;
   return JPXLG_OK;
 }
@@ -1212,10 +1253,14 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
   memset(H, 0, L.total);
   ChainTotals T;
   rc = fill_chain(ctx, imgs, n, qs, r, H, sl->dev, L, &T);
-  if (rc) return rc;
   int src = 0;
-  rc = issue_chain(ctx, st, n, qs != nullptr, r, imgs[0]->dct_form, H, sl->dev, L, T, &src);
-  if (rc) return rc;
+  if (!rc) rc = issue_chain(ctx, st, n, qs != nullptr, r, imgs[0]->dct_form, H, sl->dev, L, T, &src);
+  if (rc) {
+    // Nothing reads this chain back: wait for what was issued, free the slot.
+    (void)hipStreamSynchronize(st);
+    sl->claimed = false;
+    return rc;
+  }
   // Keep the restored planes in rx (downloads read them there).
   if (src)
     for (int k = 0; k < n; k++)
@@ -1238,15 +1283,25 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
 // Waits for a chain and reads back each image's error word and, when it
 // scored, its metric (scores[k], optional raw/terms of image 0).
 static int read_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const ChainRun& run, bool scored,
-                      double* scores, double* raw0, double* terms0, uint32_t* nterms0) {
+                      double* scores, double* raw0, double* terms0, uint32_t* nterms0, int* rcs = nullptr) {
+  if (rcs)
+    for (int k = 0; k < n; k++) rcs[k] = JPXLG_E_DEVICE;
   if (run.slot) {
     HIP_TRY(ctx, hipEventSynchronize(run.slot->done));
     run.slot->pending = false;
   }
+  // The slot is free for the next chain once this function has read it.
+  struct Unclaim {
+    jpxlg_ctx::Slot* s;
+    ~Unclaim() { if (s) s->claimed = false; }
+  } unclaim{run.slot};
   const char* H = run.host;
   const uint32_t* err = (const uint32_t*)(H + run.L.err);
   int bad = 0;
-  for (int k = 0; k < n; k++) bad |= err[k] != 0;
+  for (int k = 0; k < n; k++) {
+    bad |= err[k] != 0;
+    if (rcs) rcs[k] = err[k] ? JPXLG_E_RANGE : JPXLG_OK;
+  }
   if (scored) {
     for (int k = 0; k < n; k++) {
       jpxlg_img* img = imgs[k];
@@ -1516,7 +1571,7 @@ extern "C" int32_t jpxlg_graph_create(uint32_t n, jpxlg_img* const* imgs, const 
   for (uint32_t k = 0; k < n; k++) G->gens.push_back(imgs[k]->generation);
   G->r = *r;
   G->L = chain_layout((int)n);
-  if (hipMalloc(&G->dev, G->L.total) != hipSuccess) return fail(ctx, JPXLG_E_OOM, "graph block allocation failed");
+  if (!device_alloc(ctx, (void**)&G->dev, G->L.total)) return fail(ctx, JPXLG_E_OOM, "graph block allocation failed");
   if (hipHostMalloc(&G->host, G->L.total, hipHostMallocDefault) != hipSuccess)
     return fail(ctx, JPXLG_E_OOM, "graph staging allocation failed");
   memset(G->host, 0, G->L.total);
@@ -1719,17 +1774,23 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
 }
 
 // Hands a lane's finished round back to its requests' navigators.
+// Each request gets its own image's status (a coefficient out of range in
+// one image fails that request only); the return value is a device failure
+// of the round, if any.
 static int server_collect(jpxlg_server* srv, ServerLane& l) {
   std::vector<jpxlg_img*> imgs;
   for (ServerReq* R : l.reqs) imgs.push_back(R->img);
   std::vector<double> sc(imgs.size(), 0.0);
-  const int rc = read_chain(srv->ctx, imgs.data(), (int)imgs.size(), l.run, true, sc.data(), nullptr, nullptr, nullptr);
+  std::vector<int> rcs(imgs.size(), JPXLG_E_DEVICE);
+  int rc = read_chain(srv->ctx, imgs.data(), (int)imgs.size(), l.run, true, sc.data(), nullptr, nullptr, nullptr,
+                      rcs.data());
+  if (rc == JPXLG_E_RANGE) rc = JPXLG_OK;
   {
     std::lock_guard<std::mutex> lk(srv->m);
     for (size_t t = 0; t < l.reqs.size(); t++) {
       ServerReq* R = l.reqs[t];
       R->score = sc[t];
-      R->rc = rc;
+      R->rc = rcs[t];
       R->pending = false;
       R->in_lane = false;
       srv->waiting--;
@@ -1791,10 +1852,21 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     }
   }
   if (!fimgs.empty()) {
+    // A request whose plan could not be left quantized fails with that
+    // image's status (poll reports it); the others complete.
+    std::vector<int> qrcs(fimgs.size(), JPXLG_E_DEVICE);
     ChainRun run;
     int qrc = enqueue_chain(ctx, ctx->stream, fimgs.data(), (int)fimgs.size(), fqs.data(), nullptr, &run);
-    if (!qrc) qrc = read_chain(ctx, fimgs.data(), (int)fimgs.size(), run, false, nullptr, nullptr, nullptr, nullptr);
-    if (qrc) status = qrc;
+    if (qrc) std::fill(qrcs.begin(), qrcs.end(), qrc);
+    else qrc = read_chain(ctx, fimgs.data(), (int)fimgs.size(), run, false, nullptr, nullptr, nullptr, nullptr,
+                          qrcs.data());
+    for (ServerReq* R : finishing)
+      for (size_t t = 0; t < fimgs.size(); t++)
+        if (fimgs[t] == R->img && qrcs[t]) {
+          R->nav_rc = qrcs[t];
+          memset(&R->res, 0, sizeof(R->res));
+        }
+    if (qrc && qrc != JPXLG_E_RANGE) status = qrc;
   }
   {
     std::lock_guard<std::mutex> lk(srv->m);
@@ -1875,7 +1947,7 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     for (ServerReq* R : pending[li]) {
       bool lane_done = false;
       for (auto& l : srv->lane)
-        if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) lane_done = true;
+        if (l.busy && hipEventQuery(l.run.slot->done) != hipErrorNotReady) lane_done = true;
       if (lane_done && (srv->lane[0].busy || srv->lane[1].busy)) break;
       if (R->anchored || !R->pending || R->in_lane) continue;
       build(R);  // a failure ends that request (its navigator is woken)
@@ -1886,12 +1958,12 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     for (;;) {
       bool any = false;
       for (auto& l : srv->lane)
-        if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) any = true;
+        if (l.busy && hipEventQuery(l.run.slot->done) != hipErrorNotReady) any = true;
       if (any) break;
       std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
     for (auto& l : srv->lane)
-      if (l.busy && hipEventQuery(l.run.slot->done) == hipSuccess) {
+      if (l.busy && hipEventQuery(l.run.slot->done) != hipErrorNotReady) {
         const int rc = server_collect(srv, l);
         if (rc && !status) status = rc;
       }
@@ -1934,11 +2006,47 @@ extern "C" int32_t jpxlg_server_pending(jpxlg_server* srv) {
   return n;
 }
 
+// Ends every navigator that is waiting for a probe with `rc` (used when the
+// device stops making progress, so no thread is left waiting).
+static void server_abort_waiting(jpxlg_server* srv, int rc) {
+  std::lock_guard<std::mutex> lk(srv->m);
+  for (auto& kv : srv->reqs) {
+    ServerReq* R = kv.second.get();
+    if (R->pending && !R->in_lane) {
+      R->rc = rc;
+      R->pending = false;
+      srv->waiting--;
+      R->cv.notify_one();
+    }
+  }
+}
+
+// Steps until nothing is pending.  Every step either completes requests or
+// advances their navigators (a failed round hands each request its error,
+// which ends its search); after kMaxFailedSteps failing steps in a row with
+// no request completed, the waiting navigators are ended with the error.
+static int server_drain(jpxlg_server* srv) {
+  constexpr int kMaxFailedSteps = 64;
+  int status = JPXLG_OK, failed = 0;
+  while (jpxlg_server_pending(srv) > 0) {
+    const int st = jpxlg_server_step(srv);
+    if (st >= 0) {
+      failed = 0;
+      continue;
+    }
+    if (!status) status = st;
+    if (++failed >= kMaxFailedSteps) {
+      server_abort_waiting(srv, st);
+      failed = 0;
+    }
+  }
+  return status;
+}
+
 extern "C" void jpxlg_server_destroy(jpxlg_server* srv) {
   if (!srv) return;
   // Drive every outstanding request to completion before the threads go.
-  while (jpxlg_server_pending(srv) > 0)
-    if (jpxlg_server_step(srv) < 0) break;
+  (void)server_drain(srv);
   for (auto& kv : srv->reqs)
     if (kv.second->th.joinable()) kv.second->th.join();
   delete srv;
@@ -1959,13 +2067,8 @@ extern "C" int32_t jpxlg_batch_encode_quality(uint32_t n, jpxlg_img* const* imgs
     if (rc) break;
   }
   int status = rc;
-  while (jpxlg_server_pending(&srv) > 0) {
-    const int st = jpxlg_server_step(&srv);
-    if (st < 0) {
-      if (!status) status = st;
-      // keep stepping so every navigator thread ends
-    }
-  }
+  const int drc = server_drain(&srv);
+  if (!status) status = drc;
   for (uint32_t k = 0; k < n && !rc; k++) {
     const int p = jpxlg_server_poll(&srv, tickets[k], &out[k]);
     if (p < 0 && !status) status = p;

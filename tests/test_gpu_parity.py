@@ -529,6 +529,53 @@ def test_server_results_are_the_same_alone_and_in_the_mix(ctx):
         im.close()
 
 
+def test_server_large_lane_round_outlives_many_small_rounds(ctx):
+    """An image above 4 MP runs in the large-image lane; while its round is on
+    the device, many small-lane rounds and hand-off quantizations take upload
+    slots.  The large round's slot must stay its own until it is read back, so
+    every result equals the request served alone."""
+    import jpxl_amd
+
+    big = (2080, 2064)  # 4.29 MP: the large lane
+    smalls = [(96 + 8 * k, 80 + 4 * k) for k in range(12)]
+    sizes = [big] + smalls
+    targets = [85.0] + [70.0, 85.0, 90.0] * 4
+    efforts = ["balanced"] + ["fast", "balanced"] * 6
+    rgbs = [_rgb(w, h, 200 + k) for k, (w, h) in enumerate(sizes)]
+    alone = []
+    for k, x in enumerate(rgbs):
+        im = jpxl_amd.Image(ctx, x, 8)
+        r = im.encode_quality(targets[k], efforts[k])
+        alone.append((r.rung, r.probes, r.score, im.plan_download().quant))
+        im.close()
+    srv = jpxl_amd.Server(ctx)
+    imgs = [jpxl_amd.Image(ctx, x, 8) for x in rgbs]
+    tickets = {0: srv.submit(imgs[0], targets[0], efforts[0])}
+    nxt, done, steps = 1, {}, 0
+    while len(done) < len(rgbs):
+        srv.step()
+        steps += 1
+        for k, t in tickets.items():
+            if k not in done:
+                res = srv.poll(t)
+                if res is not None:
+                    done[k] = res
+        # two small requests join per step so the small lane keeps cycling
+        for _ in range(2):
+            if nxt < len(rgbs):
+                tickets[nxt] = srv.submit(imgs[nxt], targets[nxt], efforts[nxt])
+                nxt += 1
+    assert steps >= 9
+    for k in range(len(rgbs)):
+        r = done[k]
+        assert (r.rung, r.probes, r.score) == alone[k][:3], k
+        assert np.array_equal(imgs[k].plan_download().quant, alone[k][3]), k
+        srv.release(tickets[k])
+    srv.close()
+    for im in imgs:
+        im.close()
+
+
 @pytest.mark.parametrize("wh", [(61, 37), (256, 130), (9, 11)])
 def test_gaborish_precondition_bit_exact(ctx, wh):
     """prepare_gaborish_frame: the source XYB replaced by its 12-step Jacobi
