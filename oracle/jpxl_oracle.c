@@ -503,18 +503,38 @@ static const double P32[3][8] = {
     {3803.53173721215041536, -3.060733579805728, -2.0413270132490346, -2.0235650159727417,
      -0.5495389509954993, -0.4, -0.4, -0.3}};
 
+/* dequant.rs:400-402 Mult. */
 static double mult_(double v) { return v > 0.0 ? 1.0 + v : 1.0 / (1.0 - v); }
+EXPORT double orc_dequant_mult(double v) { return mult_(v); }
 
-/* dequant.rs:418-440 interpolate. */
+/* dequant.rs:418-440 interpolate; NaN where the reference returns an error
+ * (no bands, or a position scaling outside them). */
 static double interpolate(double pos, double max, const double *bands, int nb) {
     int last = nb - 1;
+    if (last < 0) return NAN;
     if (last == 0) return bands[0];
     double scaled_pos = pos * (double)last / max;
     double si = floor(scaled_pos);
+    if (!(si >= 0.0 && si < (double)last)) return NAN;
     int idx = (int)si;
     double frac = scaled_pos - si;
     double a = bands[idx], b = bands[idx + 1];
     return a * pow(b / a, frac);
+}
+EXPORT double orc_dequant_interpolate(double pos, double max, const double *bands, int nb) {
+    return interpolate(pos, max, bands, nb);
+}
+
+/* dequant.rs:449-468 bands_from: bands[0] = params[0], bands[i] =
+ * bands[i-1] * Mult(params[i]).  Returns -1 where the reference errors. */
+EXPORT int orc_dequant_bands(const double *params, int np, double *bands) {
+    if (np < 1 || !isfinite(params[0]) || params[0] <= 0.0) return -1;
+    bands[0] = params[0];
+    for (int i = 1; i < np; i++) {
+        bands[i] = bands[i - 1] * mult_(params[i]);
+        if (!isfinite(bands[i]) || bands[i] <= 0.0) return -1;
+    }
+    return 0;
 }
 
 /* dequant.rs:466-510 dct_quant_weights + dequant.rs:321-338 reciprocal. */
@@ -525,8 +545,7 @@ EXPORT void orc_dequant_matrix(int side, int channel, float *out) {
     else if (side == 16) { params = P16[channel]; np = 7; }
     else { params = P32[channel]; np = 8; }
     double bands[8];
-    bands[0] = params[0];
-    for (int i = 1; i < np; i++) bands[i] = bands[i - 1] * mult_(params[i]);
+    orc_dequant_bands(params, np, bands);
     double max = M_SQRT2 + 1e-6;
     double span = (double)(side - 1);
     for (int y = 0; y < side; y++)
@@ -814,6 +833,17 @@ static void truncate_trailing(const hf_quantizer *q, int c, int32_t *quant, int 
         if (recon_y) recon_y[cell] = hf_recon(q, 0, c, cell);
         last = previous;
     }
+}
+
+/* One varblock's trailing truncation (quantize.rs:468-545) under the
+ * quantizer of side `side` and params p: quant (cell order, side*side) is
+ * updated in place; targets are the pre-quantization coefficients. */
+EXPORT void orc_truncate_block(int side, int c, const orc_quant_params *p, int32_t *quant, const float *targets) {
+    hf_quantizer *q = calloc(1, sizeof(hf_quantizer));
+    build_hf_quantizer(q, side, p);
+    int n = side / 8;
+    truncate_trailing(q, c, quant, n * n, p->zero_token_bits, targets, NULL);
+    free(q);
 }
 
 /* quantize.rs:150-213 LfQuantizer. */
@@ -1542,6 +1572,33 @@ static void pool_maps(size_t n, const float *i1, const float *m1, const float *v
     }
 }
 
+EXPORT void orc_pool_maps(size_t n, const float *i1, const float *m1, const float *v11,
+                          const float *i2, const float *m2, const float *v22, const float *v12,
+                          double *sums) {
+    pool_maps(n, i1, m1, v11, i2, m2, v22, v12, sums);
+}
+
+/* pool.rs:60-68 ChannelTerms::from_sums: [1-norm, 4-norm] per map, written
+ * in (norm, map) order: ssim, artifact, detail lost. */
+EXPORT void orc_from_sums(const double *sums, size_t pixels, double *out) {
+    double inv = 1.0 / (double)(pixels ? pixels : 1);
+    for (int m = 0; m < 3; m++) {
+        out[m] = inv * sums[2 * m];
+        out[3 + m] = sqrt(sqrt(inv * sums[2 * m + 1]));
+    }
+}
+
+/* ssimulacra2.rs:509-528 weighted_error: the weights are consumed in
+ * (component, scale, norm, map) order over the ns scales that exist. */
+EXPORT double orc_weighted_error(const double *terms, int ns) {
+    double err = 0.0;
+    int wi = 0;
+    for (int c = 0; c < 3; c++)
+        for (int sc = 0; sc < ns; sc++)
+            for (int k = 0; k < 6; k++) err += WEIGHTS[wi++] * fabs(terms[c * ns * 6 + sc * 6 + k]);
+    return err;
+}
+
 /* ssimulacra2.rs:79-135 Ssimulacra2::score with reference.rs:90-200
  * PrecomputedReference::new (Moments retention).  terms[c*ns*6 + s*6 + n*3 +
  * map] holds the pooled terms; returns the score, writes scale count. */
@@ -1593,32 +1650,19 @@ EXPORT double orc_ssimulacra2(int w, int h, const float *ref[3], const float *ca
                 pool_maps(bn, x1[c] + o, m1 + o, v11 + o, x2[c] + o, m2 + o, v22 + o, v12 + o, band);
                 for (int k = 0; k < 6; k++) sums[k] += band[k];
             }
-            double inv = 1.0 / (double)(n ? n : 1);
-            /* pool.rs:60-68 from_sums: [1-norm, 4-norm] per map, stored in
-             * (norm, map) order: ssim, artifact, lost. */
-            double norms[2][3];
-            for (int m = 0; m < 3; m++) {
-                norms[0][m] = inv * sums[2 * m];
-                norms[1][m] = sqrt(sqrt(inv * sums[2 * m + 1]));
-            }
-            for (int nn = 0; nn < 2; nn++)
-                for (int m = 0; m < 3; m++) all[s][c][nn * 3 + m] = norms[nn][m];
+            orc_from_sums(sums, n, all[s][c]);
         }
         free(m1); free(v11); free(m2); free(v22); free(v12);
         for (int c = 0; c < 3; c++) { free(x1[c]); free(x2[c]); }
     }
     int ns = s;
     for (int c = 0; c < 3; c++) { free(rr[c]); free(cr[c]); }
-    /* ssimulacra2.rs:509-528 weighted_error. */
-    double err = 0.0;
-    int wi = 0;
+    double tl[108];
     for (int c = 0; c < 3; c++)
         for (int sc = 0; sc < ns; sc++)
-            for (int k = 0; k < 6; k++) {
-                double v = all[sc][c][k];
-                if (terms) terms[c * ns * 6 + sc * 6 + k] = v;
-                err += WEIGHTS[wi++] * fabs(v);
-            }
+            for (int k = 0; k < 6; k++) tl[c * ns * 6 + sc * 6 + k] = all[sc][c][k];
+    if (terms) memcpy(terms, tl, sizeof(double) * 18 * ns);
+    double err = orc_weighted_error(tl, ns);
     if (nscales) *nscales = ns;
     if (raw_error) *raw_error = err;
     return orc_remap(err);
@@ -1786,7 +1830,10 @@ static int best_factor(const cfl_acc *a, float base, int lo, int hi) {
     for (int i = 0; i < 6; i++) {
         int cand;
         if (i < 5) {
-            int64_t c = centre + (i - 2);
+            int64_t d = i - 2, c;
+            if (d > 0 && centre > INT64_MAX - d) c = INT64_MAX;      /* saturating_add */
+            else if (d < 0 && centre < INT64_MIN - d) c = INT64_MIN;
+            else c = centre + d;
             if (c < lo) c = lo;
             if (c > hi) c = hi;
             cand = (int)c;
@@ -1796,6 +1843,14 @@ static int best_factor(const cfl_acc *a, float base, int lo, int hi) {
         if (!have || e < best_e || (e == best_e && abs(cand) < abs(best))) { have = 1; best = cand; best_e = e; }
     }
     return best;
+}
+
+/* quantize.rs:1628-1634 + 1656-1684: the accumulator over n (y, c) pairs
+ * and best_factor with colour factor 84. */
+EXPORT int orc_cfl_best_factor(int n, const float *y, const float *c, float base, int lo, int hi) {
+    cfl_acc a = {0.0, 0.0, 0.0};
+    for (int i = 0; i < n; i++) acc_add(&a, y[i], c[i]);
+    return best_factor(&a, base, lo, hi);
 }
 
 static uint64_t factor_bits(int f) { return f == 0 ? 0 : (uint64_t)(32 - __builtin_clz((uint32_t)abs(f))) + 2; }
