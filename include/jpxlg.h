@@ -80,9 +80,14 @@ typedef struct {
   int32_t sharpness;   /* uniform EPF sharpness 0..7 */
 } jpxlg_restoration;
 
-/* Cover modes (request.rs CoverMode). */
+/* Cover modes (request.rs CoverMode).  HIERARCHICAL prices candidates as the
+ * quality and target-rate requests do (the calibrated rate model and the
+ * quantizer-donor cell weights, request.rs:912-925); HIERARCHICAL_LEGACY as
+ * the fixed-quantizer request does (EncodeRequest::defaults: the legacy rate
+ * proxy and flat cell weights, request.rs:855-880). */
 #define JPXLG_COVER_FIXED_DCT8 0
 #define JPXLG_COVER_HIERARCHICAL 1
+#define JPXLG_COVER_HIERARCHICAL_LEGACY 2
 
 /* ---- context ------------------------------------------------------------ */
 int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out);
@@ -195,6 +200,15 @@ typedef struct {
   int32_t mode;         /* 0 = JPXL navigator, 1 = bisection over the ladder */
 } jpxlg_quality_request;
 
+/* Why the search stopped where it did (QualityStatus, quality.rs:440-470). */
+#define JPXLG_QUALITY_MET 0                     /* met inside the accept band */
+#define JPXLG_QUALITY_MET_ADJACENT_RUNGS 1      /* met; the next coarser rung was verified not to */
+#define JPXLG_QUALITY_MET_WORK_CAP 2            /* met; the budget ran out before tightening */
+#define JPXLG_QUALITY_SATURATED_TOP 3           /* even the finest rung misses the target */
+#define JPXLG_QUALITY_SATURATED_FLOOR 4         /* even the coarsest rung meets the target */
+#define JPXLG_QUALITY_UNDER_TARGET_WORK_CAP 5   /* nothing met inside the budget */
+#define JPXLG_QUALITY_RESCUED_FRESH_STRUCTURE 6 /* the chosen plan was rebuilt with its own structure */
+
 typedef struct {
   double score;              /* achieved score of the chosen plan */
   uint32_t global_scale, hf_mul, quant_lf;
@@ -204,10 +218,19 @@ typedef struct {
   double probe_scores[16];
   uint32_t probe_rungs[16];
   double probe_ms[16];
+  int32_t status;            /* JPXLG_QUALITY_* */
+  uint32_t predicted_rung;   /* the first probe's rung */
+  int32_t structural_builds; /* cover + CfL builds (the anchor and any rebuilt finalist) */
+  int32_t structure_rebuilt; /* the chosen plan's structure was rebuilt at its own rung */
+  uint8_t probe_fresh[16];   /* the probe built its own structure */
 } jpxlg_quality_result;
 
 int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
                              jpxlg_quality_result* out);
+/* The quantizer a quality search of `effort` probes at ladder rung `rung`
+ * (QuantizerChoice::at, jpxl-encode-policy/src/quantizer_ladder.rs:202-260,
+ * over EncodeRequest::for_quality, request.rs:955-972). */
+int32_t jpxlg_rung_quantizer(uint32_t rung, int32_t effort, jpxlg_quantizer* out);
 
 /* ---- batches: many images of one context, one launch per stage ---------- */
 /* One probe of n images: each quantized with its own quantizer, all

@@ -183,6 +183,18 @@ def quality_quantizer(rung_fields: tuple[int, int], effort: str = "balanced", qu
     return q
 
 
+def rung_quantizer(rung: int, effort: str = "balanced") -> Quantizer:
+    """The quantizer a quality search of `effort` probes at ladder rung `rung`
+    (QuantizerChoice::at, quantizer_ladder.rs:202-260)."""
+    q = Quantizer()
+    lib = _lib.load()
+    rc = lib.jpxlg_rung_quantizer(rung, _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED,
+                                  ctypes.byref(q))
+    if rc:
+        raise JpxlgError(rc, "rung outside the ladder")
+    return q
+
+
 @dataclass
 class Plan:
     """The quantized plan handed to the CPU entropy coder (QuantizedFrameIr)."""
@@ -512,10 +524,11 @@ class Encoder:
             else:
                 # EncodeRequest::for_fixed_quantizer over the defaults
                 # (request.rs:855-880, :936-943): quant_lf 16, hf_mul 1, AQ
-                # masking, hierarchical cover, CfL on, the nearest rule.
+                # masking, hierarchical cover priced with the legacy rate
+                # proxy and flat cell weights, CfL on, the nearest rule.
                 q = Quantizer(self.global_scale or 32768, 1, 16, 2, 2, 0, 1.0, 1.0)
                 img.set_aq(_lib.AQ_MASKING)
-                img.plan_structure(q, _lib.COVER_HIERARCHICAL, True)
+                img.plan_structure(q, _lib.COVER_HIERARCHICAL_LEGACY, True)
                 img.quantize(q)
                 _, wire, _ = img.plan_aq()
                 report = EncodeReport(float("nan"), float("nan"), True, wire.global_scale, wire.hf_mul,

@@ -33,6 +33,9 @@ BUF_AQ_FIELD = 6
 
 COVER_FIXED_DCT8 = 0
 COVER_HIERARCHICAL = 1
+COVER_HIERARCHICAL_LEGACY = 2
+QUALITY_STATUS = ["met", "met_adjacent_rungs", "met_work_cap", "saturated_top", "saturated_floor",
+                  "under_target_work_cap", "rescued_fresh_structure"]
 AQ_OFF, AQ_MASKING, AQ_UNIFORM, AQ_FINE_MASKING, AQ_FINE_UNIFORM, AQ_EDGE_REFINE = range(6)
 
 EFFORT_FAST = 0
@@ -89,6 +92,11 @@ class QualityResult(ctypes.Structure):
         ("probe_scores", ctypes.c_double * 16),
         ("probe_rungs", ctypes.c_uint32 * 16),
         ("probe_ms", ctypes.c_double * 16),
+        ("status", ctypes.c_int32),
+        ("predicted_rung", ctypes.c_uint32),
+        ("structural_builds", ctypes.c_int32),
+        ("structure_rebuilt", ctypes.c_int32),
+        ("probe_fresh", ctypes.c_uint8 * 16),
     ]
 
 
@@ -118,6 +126,7 @@ SIGNATURES = {
     "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),
     "jpxlg_encode_quality": (_i32, [_vp, ctypes.POINTER(QualityRequest), ctypes.POINTER(QualityResult)]),
+    "jpxlg_rung_quantizer": (_i32, [_u32, _i32, ctypes.POINTER(Quantizer)]),
     "jpxlg_batch_probe": (_i32, [ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(Quantizer),
                                  ctypes.POINTER(Restoration), ctypes.POINTER(ctypes.c_double)]),
     "jpxlg_batch_encode_quality": (_i32, [ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(QualityRequest),

@@ -326,6 +326,10 @@ struct jpxlg_img {
   int aq_mode = kAqOff;
   bool aq_active = false;
   bool preconditioned = false;
+  // Frame features of the first-guess predictor (quality_features.rs:173-213),
+  // computed on first use from the source planes.
+  bool features_known = false;
+  float luma_q50 = 0.0f, flat_fraction = 0.0f;
   // Bumped whenever the varblock lists, the DCT form or the source planes
   // change: a captured chain is valid only for the generation it saw.
   uint64_t generation = 0;
@@ -939,7 +943,8 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
   if (!img || !q) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
-  if (cover_mode != JPXLG_COVER_FIXED_DCT8 && cover_mode != JPXLG_COVER_HIERARCHICAL)
+  if (cover_mode != JPXLG_COVER_FIXED_DCT8 && cover_mode != JPXLG_COVER_HIERARCHICAL &&
+      cover_mode != JPXLG_COVER_HIERARCHICAL_LEGACY)
     return fail(ctx, JPXLG_E_UNSUPPORTED, "unknown cover mode");
   int rc = check_quantizer(ctx, q);
   if (rc) return rc;
@@ -947,9 +952,10 @@ extern "C" int32_t jpxlg_plan_structure(jpxlg_img* img, const jpxlg_quantizer* q
   hipStream_t st = ctx->stream;
   const size_t bpx = (size_t)g.bw * g.bh, tiles = (size_t)g.tw * g.th;
   HIP_TRY(ctx, hipMemsetAsync(img->d_err, 0, 4, st));
-  if (cover_mode == JPXLG_COVER_HIERARCHICAL) {
+  if (cover_mode != JPXLG_COVER_FIXED_DCT8) {
     ProbeParams pp;
     probe_params_for(img, *q, img->last_sharpness, &pp);
+    set_cover_objective(&pp, cover_mode == JPXLG_COVER_HIERARCHICAL_LEGACY);
     rc = upload_small(ctx, st, img->d_pp, &pp, sizeof(pp));
     if (rc) return rc;
     Prof p(ctx, "cover");
@@ -1439,52 +1445,162 @@ extern "C" int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* hos
 // Quality controller
 // ----------------------------------------------------------------------------
 
+// The frame features of the first-guess predictor: every atom's Y variance
+// on the device, the median and the flat fraction on the host.
+static int image_features(jpxlg_img* img) {
+  if (img->features_known) return JPXLG_OK;
+  jpxlg_ctx* ctx = img->ctx;
+  const Geom& g = img->g;
+  const size_t bpx = (size_t)g.bw * g.bh;
+  hipStream_t st = ctx->stream;
+  float* d_var = (float*)img->d_cost[0];  // cover scratch, free between structure builds
+  launch_atlas_luma_var(st, g, img->xyb[1], d_var);
+  HIP_TRY(ctx, hipGetLastError());
+  if (ctx->rb_cap < bpx * 4) {
+    if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
+    ctx->rb_host = nullptr;
+    ctx->rb_cap = 0;
+    if (hipHostMalloc((void**)&ctx->rb_host, bpx * 4, hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, JPXLG_E_OOM, "readback staging allocation failed");
+    ctx->rb_cap = bpx * 4;
+  }
+  HIP_TRY(ctx, hipMemcpyAsync(ctx->rb_host, d_var, bpx * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  std::vector<float> var((const float*)ctx->rb_host, (const float*)ctx->rb_host + bpx);
+  source_features(std::move(var), &img->luma_q50, &img->flat_fraction);
+  img->features_known = true;
+  return JPXLG_OK;
+}
+
+// The legacy predictor's first rung for `target` (quality.rs:2452-2460).
+static int predicted_rung_for(jpxlg_img* img, double target, uint32_t* rung) {
+  const int rc = image_features(img);
+  if (rc) return rc;
+  *rung = rung_for_scale(predicted_effective_scale(img->luma_q50, img->flat_fraction, target));
+  return JPXLG_OK;
+}
+
+// A structure kept on the host (the anchor of a search), to be set again
+// after a finalist was rebuilt with its own.
+struct SavedStructure {
+  std::vector<uint8_t> bmap;
+  std::vector<int8_t> cx, cb;
+  int32_t lf[2] = {0, 0};
+  void save(const jpxlg_img* img) {
+    bmap = img->bmap;
+    cx = img->cfl_x;
+    cb = img->cfl_b;
+    lf[0] = img->lf_factors[0];
+    lf[1] = img->lf_factors[1];
+  }
+  int restore(jpxlg_img* img) const {
+    return jpxlg_set_structure(img, bmap.data(), cx.data(), cb.data(), lf);
+  }
+};
+
+// Which structure the image holds during a search, and the builds that put
+// it there: a fresh build at a rung (the first is the anchor) or the
+// anchor set again.
+struct SearchStructure {
+  SavedStructure anchor;
+  bool anchored = false;
+  bool holds_anchor = false;
+  uint32_t fresh_rung = 0;  // the rung of the build the image holds, when not the anchor
+  int need(jpxlg_img* img, const QualityPolicy& pol, uint32_t rung, bool fresh) {
+    if (fresh) {
+      if (anchored && !holds_anchor && fresh_rung == rung) return JPXLG_OK;
+      const jpxlg_quantizer q = pol.quantizer_at(rung);
+      const int rc = jpxlg_plan_structure(img, &q, pol.cover_mode, pol.enable_cfl);
+      if (rc) return rc;
+      if (!anchored) {
+        anchor.save(img);
+        anchored = true;
+        holds_anchor = true;
+      } else {
+        holds_anchor = false;
+        fresh_rung = rung;
+      }
+      return JPXLG_OK;
+    }
+    if (holds_anchor) return JPXLG_OK;
+    const int rc = anchor.restore(img);
+    if (rc) return rc;
+    holds_anchor = true;
+    return JPXLG_OK;
+  }
+  // Whether the probe (rung, fresh) can run on the structure held now.
+  bool ready(uint32_t rung, bool fresh) const {
+    if (!anchored) return false;
+    return fresh ? (!holds_anchor && fresh_rung == rung) : holds_anchor;
+  }
+};
+
+static void fill_quality_result(const ControllerOutcome& oc, const QualityPolicy& pol, uint32_t predicted,
+                                jpxlg_quality_result* o) {
+  memset(o, 0, sizeof(*o));
+  const jpxlg_quantizer qc = pol.quantizer_at(oc.rung);
+  o->score = oc.score;
+  o->rung = oc.rung;
+  o->global_scale = qc.global_scale;
+  o->hf_mul = qc.hf_mul;
+  o->quant_lf = qc.quant_lf;
+  o->met = oc.met;
+  o->probes = (int32_t)oc.probes.size();
+  for (size_t i = 0; i < oc.probes.size() && i < 16; i++) {
+    o->probe_scores[i] = oc.probes[i].score;
+    o->probe_rungs[i] = oc.probes[i].rung;
+    o->probe_ms[i] = oc.probes[i].ms;
+    o->probe_fresh[i] = oc.probes[i].fresh ? 1 : 0;
+  }
+  o->status = oc.status;
+  o->predicted_rung = predicted;
+  o->structural_builds = oc.structural_builds;
+  o->structure_rebuilt = oc.chosen_fresh ? 1 : 0;
+}
+
 extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
                                         jpxlg_quality_result* out) {
   if (!img || !req || !out) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = img->ctx;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   memset(out, 0, sizeof(*out));
-  QualityPolicy pol = quality_policy(req->effort);
-  bool anchored = false;
-  ProbeFn probe = [&](uint32_t rung, double* score, double* ms) -> int {
+  const QualityPolicy pol = quality_policy(req->effort);
+  uint32_t predicted = 0;
+  int rc = predicted_rung_for(img, req->target, &predicted);
+  if (rc) return rc;
+  SearchStructure ss;
+  ProbeFn probe = [&](uint32_t rung, bool fresh, double* score, double* ms) -> int {
     auto t0 = std::chrono::steady_clock::now();
-    jpxlg_quantizer q = pol.quantizer_at(rung);
-    // The first probe builds the structure (cover + CfL) at its own
-    // quantizer; later probes reuse it (quality.rs:1153-1190).
-    if (!anchored) {
-      int rc = jpxlg_plan_structure(img, &q, pol.cover_mode, 1);
-      if (rc) return rc;
-      anchored = true;
-    }
+    // Navigator::probe (quality.rs:1153-1190): a fresh probe builds the cover
+    // and CfL at its own quantizer, the others reuse the anchor's.
+    int prc = ss.need(img, pol, rung, fresh);
+    if (prc) return prc;
+    const jpxlg_quantizer q = pol.quantizer_at(rung);
     jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
-    int rc = jpxlg_probe(img, &q, &r, score);
+    prc = jpxlg_probe(img, &q, &r, score);
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    return rc;
+    return prc;
   };
   ControllerOutcome oc;
-  int rc = req->mode == 1 ? bisect_controller(req->target, req->max_probes, probe, &oc)
-                          : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc);
+  rc = req->mode == 1 ? bisect_controller(req->target, req->max_probes, probe, &oc)
+                      : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc, predicted);
   if (rc) return rc;
-  // Leave the chosen plan quantized on the device for the hand-off (the
-  // last probe's levels already are, when it probed the chosen rung).
-  jpxlg_quantizer qc = pol.quantizer_at(oc.rung);
+  // Leave the chosen plan quantized on the device for the hand-off, on the
+  // structure it was scored with (the last probe's levels usually are).
+  rc = ss.need(img, pol, oc.rung, oc.chosen_fresh);
+  if (rc) return rc;
+  const jpxlg_quantizer qc = pol.quantizer_at(oc.rung);
   if (!holds_levels_of(img, qc)) {
     rc = jpxlg_quantize(img, &qc);
     if (rc) return rc;
   }
-  out->score = oc.score;
-  out->rung = oc.rung;
-  out->global_scale = qc.global_scale;
-  out->hf_mul = qc.hf_mul;
-  out->quant_lf = qc.quant_lf;
-  out->met = oc.met;
-  out->probes = (int32_t)oc.probes.size();
-  for (size_t i = 0; i < oc.probes.size() && i < 16; i++) {
-    out->probe_scores[i] = oc.probes[i].score;
-    out->probe_rungs[i] = oc.probes[i].rung;
-    out->probe_ms[i] = oc.probes[i].ms;
-  }
+  fill_quality_result(oc, pol, req->mode == 1 ? (oc.probes.empty() ? 0 : oc.probes[0].rung) : predicted, out);
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_rung_quantizer(uint32_t rung, int32_t effort, jpxlg_quantizer* out) {
+  if (!out || rung > Rung::top()) return JPXLG_E_INVALID;
+  *out = quality_policy(effort).quantizer_at(rung);
   return JPXLG_OK;
 }
 
@@ -1673,9 +1789,11 @@ struct ServerReq {
   jpxlg_quality_request q{};
   QualityPolicy pol{};
   std::thread th;
-  bool anchored = false;
+  SearchStructure ss;     // the structure its image holds (anchor or a rebuilt finalist's)
+  uint32_t predicted = 0; // the legacy predictor's first rung
   bool pending = false;
   uint32_t rung = 0;
+  bool fresh = false;     // the pending probe builds its own structure
   double score = 0.0;
   int rc = 0;
   bool in_lane = false;   // its probe is in a lane's running round
@@ -1742,15 +1860,21 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
   R->img = img;
   R->q = *q;
   R->pol = quality_policy(q->effort);
+  if (q->mode != 1) {
+    HIP_TRY(srv->ctx, hipSetDevice(srv->ctx->device));
+    const int prc = predicted_rung_for(img, q->target, &R->predicted);
+    if (prc) return prc;
+  }
   std::lock_guard<std::mutex> lk(srv->m);
   const uint64_t id = srv->next_id++;
   srv->reqs.emplace(id, std::move(r));
   srv->active++;
   R->th = std::thread([srv, R]() {
-    ProbeFn probe = [srv, R](uint32_t rung, double* score, double* ms) -> int {
+    ProbeFn probe = [srv, R](uint32_t rung, bool fresh, double* score, double* ms) -> int {
       auto t0 = std::chrono::steady_clock::now();
       std::unique_lock<std::mutex> lk(srv->m);
       R->rung = rung;
+      R->fresh = fresh;
       R->pending = true;
       srv->waiting++;
       srv->cv_req.notify_all();
@@ -1761,7 +1885,8 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
     };
     ControllerOutcome oc;
     const int rc = R->q.mode == 1 ? bisect_controller(R->q.target, R->q.max_probes, probe, &oc)
-                                  : navigate_controller(R->q.target, R->q.effort, R->q.max_probes, probe, &oc);
+                                  : navigate_controller(R->q.target, R->q.effort, R->q.max_probes, probe, &oc,
+                                                        R->predicted);
     std::lock_guard<std::mutex> lk(srv->m);
     R->oc = oc;
     R->nav_rc = rc;
@@ -1830,26 +1955,21 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
   std::vector<jpxlg_img*> fimgs;
   std::vector<jpxlg_quantizer> fqs;
   for (ServerReq* R : finishing) {
-    jpxlg_quality_result& o = R->res;
-    memset(&o, 0, sizeof(o));
+    memset(&R->res, 0, sizeof(R->res));
     if (R->nav_rc) continue;
+    // the chosen plan's own structure (the anchor, or its rebuilt one)
+    const int src = R->ss.need(R->img, R->pol, R->oc.rung, R->oc.chosen_fresh);
+    if (src) {
+      R->nav_rc = src;
+      continue;
+    }
     const jpxlg_quantizer qc = R->pol.quantizer_at(R->oc.rung);
     if (!holds_levels_of(R->img, qc)) {
       fimgs.push_back(R->img);
       fqs.push_back(qc);
     }
-    o.score = R->oc.score;
-    o.rung = R->oc.rung;
-    o.global_scale = qc.global_scale;
-    o.hf_mul = qc.hf_mul;
-    o.quant_lf = qc.quant_lf;
-    o.met = R->oc.met;
-    o.probes = (int32_t)R->oc.probes.size();
-    for (size_t i = 0; i < R->oc.probes.size() && i < 16; i++) {
-      o.probe_scores[i] = R->oc.probes[i].score;
-      o.probe_rungs[i] = R->oc.probes[i].rung;
-      o.probe_ms[i] = R->oc.probes[i].ms;
-    }
+    const uint32_t pred = R->q.mode == 1 ? (R->oc.probes.empty() ? 0 : R->oc.probes[0].rung) : R->predicted;
+    fill_quality_result(R->oc, R->pol, pred, &R->res);
   }
   if (!fimgs.empty()) {
     // A request whose plan could not be left quantized fails with that
@@ -1872,12 +1992,12 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
     std::lock_guard<std::mutex> lk(srv->m);
     for (ServerReq* R : finishing) R->done = true;
   }
-  // A new request's structure (block map, lists, CfL) is built on the
+  // The structure a request's pending probe needs (a new request's anchor,
+  // a finalist rebuilt at its rung, or the anchor set again) is built on the
   // context's stream; it returns false when that failed (the request then
   // ends with the error).
   auto build = [&](ServerReq* R) -> bool {
-    const jpxlg_quantizer q = R->pol.quantizer_at(R->rung);
-    const int prc = jpxlg_plan_structure(R->img, &q, R->pol.cover_mode, 1);
+    const int prc = R->ss.need(R->img, R->pol, R->rung, R->fresh);
     if (prc) {
       std::lock_guard<std::mutex> lk(srv->m);
       R->rc = prc;
@@ -1886,31 +2006,31 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
       R->cv.notify_one();
       return false;
     }
-    R->anchored = true;
     return true;
   };
+  auto ready = [](const ServerReq* R) { return R->ss.ready(R->rung, R->fresh); };
   // Start a round in each idle lane with the requests whose structure is
   // built; a lane with none builds a few first.  The other new requests'
   // structures are built below, while the rounds run.
   for (int li = 0; li < 2; li++) {
     ServerLane& l = srv->lane[li];
     if (l.busy || pending[li].empty()) continue;
-    std::vector<ServerReq*> ready;
+    std::vector<ServerReq*> ready_reqs;
     for (ServerReq* R : pending[li])
-      if (R->anchored) ready.push_back(R);
-    if (ready.empty()) {
+      if (ready(R)) ready_reqs.push_back(R);
+    if (ready_reqs.empty()) {
       int built = 0;
       for (ServerReq* R : pending[li]) {
         if (built >= kServerBootstrap) break;
-        if (build(R)) ready.push_back(R);
+        if (build(R)) ready_reqs.push_back(R);
         built++;
       }
     }
-    if (ready.empty()) continue;
+    if (ready_reqs.empty()) continue;
     // one chain per lane round: group by (form, effort) and take the largest
     // group now; the rest wait for the lane's next round
     std::map<std::pair<int, int>, std::vector<ServerReq*>> groups;
-    for (ServerReq* R : ready) groups[{R->img->dct_form, R->q.effort}].push_back(R);
+    for (ServerReq* R : ready_reqs) groups[{R->img->dct_form, R->q.effort}].push_back(R);
     auto best = groups.begin();
     for (auto it = groups.begin(); it != groups.end(); ++it)
       if (it->second.size() > best->second.size()) best = it;
@@ -1949,7 +2069,7 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
       for (auto& l : srv->lane)
         if (l.busy && hipEventQuery(l.run.slot->done) != hipErrorNotReady) lane_done = true;
       if (lane_done && (srv->lane[0].busy || srv->lane[1].busy)) break;
-      if (R->anchored || !R->pending || R->in_lane) continue;
+      if (!R->pending || R->in_lane || ready(R)) continue;
       build(R);  // a failure ends that request (its navigator is woken)
     }
   // Wait for the first lane round to finish (short host polls), then hand

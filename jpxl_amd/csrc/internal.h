@@ -59,6 +59,12 @@ struct ProbeParams {
   int32_t aq_refine_only, aq_fine;
   float qm[3];           // per-channel qm multipliers (Y: 1)
   float sig_lut;         // the EPF sharpness LUT entry of sigma
+  // The cover objective (CoverRateModel, request.rs:585-636; cell weights,
+  // request.rs:650-690): residual-bit scale and fixed bits per DCT8/16/32,
+  // and flat (1) or quantizer-donor (0) cell weights.
+  double cover_rate[3];
+  double cover_fixed[3];
+  int32_t cover_flat;
 };
 
 // field.rs:383-395 mul_of: baseline * 2^-adj rounded to the nearest integer,
@@ -354,6 +360,7 @@ void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, const StructInfo* si, c
 void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3], float* act, float* ero,
                      AqScratch* scratch, float* adj);
 // Each listed varblock's HfMul at its top-left block.
+void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var);
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
                           const int counts[3], uint32_t* muls);
 // Pooling tiles of one scale (one partial row of 18 sums per tile).

@@ -26,9 +26,15 @@ inline unsigned div_up(size_t a, size_t b) { return (unsigned)((a + b - 1) / b);
 __device__ __constant__ float kBias[3] = {1.0f - 0.054650073f, 1.0f - 0.0700545f,
                                           1.0f - 0.049935103f};
 
-// CoverRateModel::Calibrated (request.rs:585-636).
-__host__ __device__ constexpr double rate_scale(int side) { return side == 16 ? 1.63 : (side == 32 ? 1.88 : 1.56); }
-__host__ __device__ constexpr double fixed_bits(int side) { return side == 16 ? 8.4 + 4.0 : (side == 32 ? 6.5 + 4.0 : 5.5); }
+// The cover's rate model (CoverRateModel, request.rs:585-636) and cell
+// weights, from the probe parameters: calibrated + quantizer-donor for the
+// quality and target-rate requests, legacy + flat for fixed quantizers.
+__device__ __forceinline__ double rate_scale(const ProbeParams* pp, int side) {
+  return pp->cover_rate[side == 8 ? 0 : (side == 16 ? 1 : 2)];
+}
+__device__ __forceinline__ double fixed_bits(const ProbeParams* pp, int side) {
+  return pp->cover_fixed[side == 8 ? 0 : (side == 16 ? 1 : 2)];
+}
 
 __device__ __forceinline__ int32_t choose_nearest(float target, float step, float bias, int* bad) {
   float thr = 0.5f * bias * step;
@@ -171,7 +177,7 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
         float recon = bias_adjust(q, bias) * step;
         if (c == 1) dyh[j][k] = recon;
         double d = (double)(recon - target);
-        terms[j][k] = lw * (double)tab->donor[cb + k] * (d * d);
+        terms[j][k] = lw * (double)(pp->cover_flat ? 1.0f : tab->donor[cb + k]) * (d * d);
         uint32_t rb = residual_bits(q);
         if (rb) atomicAdd(&bits[j], rb);
       }
@@ -183,7 +189,7 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
     }
     if (threadIdx.x < VB && valid[threadIdx.x]) {
       const int j = threadIdx.x;
-      cost[(size_t)vby[j] * g.bw + vbx[j]] = (double)bits[j] * rate_scale(N) + sse[j];
+      cost[(size_t)vby[j] * g.bw + vbx[j]] = (double)bits[j] * rate_scale(pp, N) + sse[j];
     }
   } else {
     // Lane r of candidate j's G lanes takes its cells k = r + m G through
@@ -198,7 +204,7 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
       const int k = min(r + m * G, NN - 1);
 #pragma unroll
       for (int c = 0; c < 3; c++) mt[c][m] = tab->matrix[c][cb + k];
-      dn[m] = tab->donor[cb + k];
+      dn[m] = pp->cover_flat ? 1.0f : tab->donor[cb + k];
     }
     const double lw[3] = {pp->lambda[0] * tsd, pp->lambda[1] * tsd, pp->lambda[2] * tsd};
     DD part{0.0, 0.0};
@@ -236,7 +242,7 @@ __device__ void cover_cost_group(Geom g, const ConstTables* __restrict__ tab, co
         const double S = tot.hi + tot.lo;
         const double nterms = 3.0 * NN + 4.0, u = 1.1102230246251565e-16;
         const double gam = 2.0 * (nterms * u / (1.0 - nterms * u));
-        const double cst = (double)bits[j] * rate_scale(N) + S;
+        const double cst = (double)bits[j] * rate_scale(pp, N) + S;
         const size_t o = (size_t)vby[j] * g.bw + vbx[j];
         cost[o] = cst;
         // the sequential sum's distance from S, plus the roundings of S and
@@ -310,11 +316,11 @@ __device__ __forceinline__ bool decide_region(Geom g, const ProbeParams* __restr
       if (sx >= bw || sy >= bh) continue;
       const size_t o = (size_t)sy * bw + sx;
       const Ival c = Ival{c8[o], cert ? r8[o] : 0.0};
-      split2 = iv_add(split2, iv_add(iv_add(c, iv_const(fixed_bits(8))), iv_const(aq_signal_bits(pp, bw, bh, sx, sy, 1))));
+      split2 = iv_add(split2, iv_add(iv_add(c, iv_const(fixed_bits(pp, 8))), iv_const(aq_signal_bits(pp, bw, bh, sx, sy, 1))));
     }
     Ival best2 = split2;
     if (qx + 2 <= bw && qy + 2 <= bh) {
-      const double fixed = fixed_bits(16) + aq_signal_bits(pp, bw, bh, qx, qy, 2);
+      const double fixed = fixed_bits(pp, 16) + aq_signal_bits(pp, bw, bh, qx, qy, 2);
       if (cert && fabs(fixed - split2.mid) <= split2.rad) return false;
       if (!(fixed >= split2.mid)) {
         const size_t o = (size_t)qy * bw + qx;
@@ -330,7 +336,7 @@ __device__ __forceinline__ bool decide_region(Geom g, const ProbeParams* __restr
   }
   bool use32 = false;
   if (bx0 + 4 <= bw && by0 + 4 <= bh) {
-    const double fixed = fixed_bits(32) + aq_signal_bits(pp, bw, bh, bx0, by0, 4);
+    const double fixed = fixed_bits(pp, 32) + aq_signal_bits(pp, bw, bh, bx0, by0, 4);
     if (cert && fabs(fixed - split4.mid) <= split4.rad) return false;
     if (!(fixed >= split4.mid)) {
       const size_t o = (size_t)by0 * bw + bx0;
@@ -1005,6 +1011,30 @@ __global__ __launch_bounds__(256) void k_aq_activity(Geom g, const float* __rest
   act[atom] = luma + 0.35f * chroma;
 }
 
+// analysis.rs:111-158 for the Y plane alone: each atom's population
+// variance over the samples it holds (f64 sums in raster order, floored at
+// zero, narrowed once), the input of the frame features
+// (quality_features.rs:173-213).
+__global__ __launch_bounds__(256) void k_atlas_luma_var(Geom g, const float* __restrict__ Y,
+                                                        float* __restrict__ var) {
+  const int atom = blockIdx.x * blockDim.x + threadIdx.x;
+  if (atom >= g.bw * g.bh) return;
+  const int ax = atom % g.bw, ay = atom / g.bw;
+  const int x0 = ax * 8, y0 = ay * 8, x1 = min(x0 + 8, g.w), y1 = min(y0 + 8, g.h);
+  const double count = (double)(x1 - x0) * (double)(y1 - y0);
+  double sum = 0.0, sum_sq = 0.0;
+  for (int y = y0; y < y1; y++)
+    for (int x = x0; x < x1; x++) {
+      const double v = (double)Y[(size_t)y * g.w + x];
+      sum += v;
+      sum_sq += v * v;
+    }
+  const double mean = sum / count;
+  double variance = sum_sq / count - mean * mean;
+  if (!(variance > 0.0)) variance = 0.0;
+  var[atom] = (float)variance;
+}
+
 // field.rs:200-219: the minimum over the clamped 3x3 neighbourhood.
 __global__ __launch_bounds__(256) void k_aq_erode(int bw, int bh, const float* __restrict__ a,
                                                   float* __restrict__ out) {
@@ -1366,6 +1396,11 @@ void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3]
   if (mode >= kAqFineMasking) k_aq_erode<<<div_up(count, 256), 256, 0, s>>>(g.bw, g.bh, act, ero);
   if (mode != kAqEdgeRefine) k_aq_mean<<<1, 1024, 0, s>>>(mode >= kAqFineMasking ? ero : act, count, scratch);
   k_aq_adj<<<div_up(count, 256), 256, 0, s>>>(count, mode, act, ero, scratch, adj);
+}
+
+void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var) {
+  const int count = g.bw * g.bh;
+  k_atlas_luma_var<<<div_up(count, 256), 256, 0, s>>>(g, Y, var);
 }
 
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],

@@ -315,6 +315,19 @@ void build_probe_params(const jpxlg_quantizer& q, int sharpness, ProbeParams* p)
   for (int c = 0; c < 3; c++) p->qm[c] = qm[c];
   p->zero_token_bits = q.zero_token_bits;
   p->truncate = q.truncate ? 1 : 0;
+  set_cover_objective(p, false);
+}
+
+void set_cover_objective(ProbeParams* p, bool legacy) {
+  // CoverRateModel::Calibrated (the quality and target-rate requests) or
+  // ::Legacy (EncodeRequest::defaults); request.rs:585-636.
+  const double cal_rate[3] = {1.56, 1.63, 1.88}, cal_fixed[3] = {5.5, 8.4 + 4.0, 6.5 + 4.0};
+  const double leg_rate[3] = {1.0, 1.0, 1.0}, leg_fixed[3] = {2.0, 34.0, 34.0};
+  for (int i = 0; i < 3; i++) {
+    p->cover_rate[i] = legacy ? leg_rate[i] : cal_rate[i];
+    p->cover_fixed[i] = legacy ? leg_fixed[i] : cal_fixed[i];
+  }
+  p->cover_flat = legacy ? 1 : 0;
 }
 
 }  // namespace jpxlg

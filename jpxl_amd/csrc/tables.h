@@ -13,6 +13,10 @@ void srgb8_lut(float* lut256);
 void srgb16_lut(int bits, float* lut65536);
 int depth_tables(int bits, std::vector<float>* thresholds, std::vector<float>* lut);
 void build_probe_params(const jpxlg_quantizer& q, int sharpness, ProbeParams* p);
+// The cover objective of a structure build: legacy = the fixed-quantizer
+// request's (legacy rate proxy, flat cell weights), else the calibrated rate
+// model with quantizer-donor weights.
+void set_cover_objective(ProbeParams* p, bool legacy);
 float qm_multiplier(uint32_t qm_scale);
 float hf_multiplier(uint32_t global_scale, uint32_t hf_mul);
 

@@ -707,6 +707,7 @@ typedef struct {
     float zero_token_bits;
     const float *aq_adj; /* per-atom AQ adjustment (bw x bh), NULL = no field */
     int aq_refine_only, aq_fine;
+    int cover_legacy;    /* the cover prices with CoverRateModel::Legacy and flat cell weights */
 } orc_quant_params;
 
 typedef struct {
@@ -1697,9 +1698,16 @@ static uint64_t residual_bits(int32_t q) {
     return (uint64_t)(32 - __builtin_clz((uint32_t)abs(q))) + 1;
 }
 
-/* request.rs:585-636 CoverRateModel::Calibrated. */
-static double cover_rate_scale(int side) { return side == 16 ? 1.63 : (side == 32 ? 1.88 : 1.56); }
-static double cover_fixed_bits(int side) { return side == 16 ? 8.4 + 4.0 : (side == 32 ? 6.5 + 4.0 : 5.5); }
+/* request.rs:585-636 CoverRateModel::Calibrated (the quality and
+ * target-rate requests) and ::Legacy (EncodeRequest::defaults). */
+static double cover_rate_scale(const orc_quant_params *p, int side) {
+    if (p->cover_legacy) return 1.0;
+    return side == 16 ? 1.63 : (side == 32 ? 1.88 : 1.56);
+}
+static double cover_fixed_bits(const orc_quant_params *p, int side) {
+    if (p->cover_legacy) return side == 8 ? 2.0 : 34.0;
+    return side == 16 ? 8.4 + 4.0 : (side == 32 ? 6.5 + 4.0 : 5.5);
+}
 
 typedef struct {
     int w, h, bw, bh;
@@ -1745,7 +1753,7 @@ static double block_cost(cover_ctx *cc, int side, int bx, int by, uint32_t mul) 
             }
         }
     }
-    return (double)bits * cover_rate_scale(side) + sse;
+    return (double)bits * cover_rate_scale(cc->p, side) + sse;
 }
 
 /* lib.rs:5115-5223 tile_region_with. */
@@ -1753,7 +1761,7 @@ static double tile_region(cover_ctx *cc, int bx, int by, int size, uint8_t *bmap
     if (bx >= cc->bw || by >= cc->bh) return 0.0;
     if (size == 1) {
         uint32_t mul = aq_vb_mul(cc->p, cc->bw, cc->bh, bx, by, 1);
-        double cost = block_cost(cc, 8, bx, by, mul) + cover_fixed_bits(8) + aq_signal_bits(cc->p, mul);
+        double cost = block_cost(cc, 8, bx, by, mul) + cover_fixed_bits(cc->p, 8) + aq_signal_bits(cc->p, mul);
         if (write) bmap[by * cc->bw + bx] = 8;
         return cost;
     }
@@ -1767,7 +1775,7 @@ static double tile_region(cover_ctx *cc, int bx, int by, int size, uint8_t *bmap
     double best = split;
     if (fits) {
         uint32_t mul = aq_vb_mul(cc->p, cc->bw, cc->bh, bx, by, size);
-        double fixed = cover_fixed_bits(side) + aq_signal_bits(cc->p, mul);
+        double fixed = cover_fixed_bits(cc->p, side) + aq_signal_bits(cc->p, mul);
         if (!(fixed >= split)) {
             double c = block_cost(cc, side, bx, by, mul) + fixed;
             if (c < split) { single = 1; best = c; }
@@ -1794,7 +1802,13 @@ EXPORT void orc_cover(int w, int h, const float *xyb[3], const orc_quant_params 
     for (int c = 0; c < 3; c++) cc.xyb[c] = xyb[c];
     cc.hc = (hq_cache){NULL, 0, 0, p};
     cc.p = p;
-    for (int i = 0; i < 3; i++) orc_donor_weights(8 << i, cc.donor[i]);
+    /* request.rs:650-690: quantizer-donor cell weights, or flat (every
+     * cell 1.0) under the fixed-quantizer request. */
+    for (int i = 0; i < 3; i++) {
+        orc_donor_weights(8 << i, cc.donor[i]);
+        if (p->cover_legacy)
+            for (int k = 0; k < (64 << (2 * i)); k++) cc.donor[i][k] = 1.0f;
+    }
     for (int c = 0; c < 3; c++) cc.lambda[c] = lambda_for(p, c);
     memset(bmap, 0, (size_t)cc.bw * cc.bh);
     for (int sby = 0; sby < cc.bh; sby += 4)

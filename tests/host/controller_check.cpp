@@ -1,0 +1,105 @@
+// Host-only test shim over the library's controller and metric pooling
+// (jpxl_amd/csrc/controller.cpp, metric_host.cpp), compiled with g++ by the
+// CPU tests: the navigator runs against closed-form score curves, the way
+// the reference's controller tests drive it with their CurveEvaluator
+// (jpxl-encode-policy/src/quality.rs:2752-2797).  Test infrastructure only.
+#include <math.h>
+#include <string.h>
+
+#include "../../jpxl_amd/csrc/controller.h"
+#include "../../jpxl_amd/csrc/metric_host.h"
+
+using namespace jpxlg;
+
+extern "C" {
+
+struct cc_outcome {
+  uint32_t rung;
+  double score;
+  int32_t met, status, chosen_fresh;
+  uint32_t anchor_rung;
+  int32_t pixel_probes, structural_builds, priced, nprobes;
+  uint32_t probe_rungs[32];
+  double probe_scores[32];
+  int32_t probe_fresh[32];
+};
+
+double cc_met_band(double t, double r) { return met_band(t, r); }
+double cc_aim(double t, double r) { return aim_score_for(t, r); }
+uint32_t cc_log_loss_crossing(uint32_t lo, double ls, uint32_t hi, double hs, double aim, int* ok) {
+  return log_loss_crossing(lo, ls, hi, hs, aim, ok);
+}
+uint32_t cc_geometric_step(uint32_t from, int finer) { return geometric_step(from, finer); }
+uint32_t cc_top() { return Rung::top(); }
+uint64_t cc_effective_scale(uint32_t rung) { return Rung::effective_scale(rung); }
+uint32_t cc_rung_for_scale(double s) { return rung_for_scale(s); }
+double cc_predicted_scale(float q50, float flat, double target) {
+  return predicted_effective_scale(q50, flat, target);
+}
+void cc_features(const float* v, int n, float* q50, float* flat) {
+  source_features(std::vector<float>(v, v + n), q50, flat);
+}
+int cc_policy(int effort, int32_t* out6, double* reserve) {
+  const QualityPolicy p = quality_policy(effort);
+  out6[0] = p.pixel_probes;
+  out6[1] = p.exact_prices;
+  out6[2] = p.structural_builds;
+  out6[3] = p.enable_cfl;
+  out6[4] = p.cover_mode;
+  out6[5] = p.truncate;
+  *reserve = p.reserve;
+  return 0;
+}
+
+// Navigates a curve loss = a * (scale / 1000)^-b in the rung's effective
+// scale; a fresh (rebuilt) structure away from the anchor scores `fresh_gain`
+// points higher.  effort < 0: the explicit budget; mode 1: bisection.
+int cc_navigate_curve(double target, int effort, int pixel, int exact, int builds, double reserve,
+                      uint32_t predicted, double a, double b, double fresh_gain, int mode, cc_outcome* out) {
+  memset(out, 0, sizeof(*out));
+  uint32_t anchor = 0;
+  bool anchored = false;
+  ProbeFn probe = [&](uint32_t rung, bool fresh, double* score, double* ms) -> int {
+    const double scale = (double)Rung::effective_scale(rung);
+    double s = 100.0 - a * pow(scale / 1000.0, -b);
+    if (fresh && anchored && rung != anchor) s += fresh_gain;
+    if (fresh && !anchored) {
+      anchored = true;
+      anchor = rung;
+    }
+    *score = s;
+    *ms = 0.0;
+    return 0;
+  };
+  ControllerOutcome oc;
+  int rc;
+  if (mode == 1) rc = bisect_controller(target, pixel, probe, &oc);
+  else if (effort >= 0) rc = navigate_controller(target, effort, pixel, probe, &oc, predicted);
+  else rc = navigate_with_budget(target, NavBudget{pixel, exact, builds, reserve}, probe, &oc, predicted);
+  if (rc) return rc;
+  out->rung = oc.rung;
+  out->score = oc.score;
+  out->met = oc.met;
+  out->status = oc.status;
+  out->chosen_fresh = oc.chosen_fresh;
+  out->anchor_rung = oc.anchor_rung;
+  out->pixel_probes = oc.pixel_probes;
+  out->structural_builds = oc.structural_builds;
+  out->priced = oc.priced;
+  out->nprobes = (int32_t)oc.probes.size();
+  for (size_t i = 0; i < oc.probes.size() && i < 32; i++) {
+    out->probe_rungs[i] = oc.probes[i].rung;
+    out->probe_scores[i] = oc.probes[i].score;
+    out->probe_fresh[i] = oc.probes[i].fresh;
+  }
+  return 0;
+}
+
+double cc_pool_terms(const double* sums, int ns, const int* sw, const int* sh, double* terms, double* raw) {
+  MetricResult m = pool_terms(sums, ns, sw, sh);
+  memcpy(terms, m.terms, sizeof(double) * 18 * ns);
+  *raw = m.raw_error;
+  return m.score;
+}
+
+}  // extern "C"

@@ -28,6 +28,7 @@ class QuantParams(ctypes.Structure):
         ("aq_adj", ctypes.c_void_p),
         ("aq_refine_only", ctypes.c_int),
         ("aq_fine", ctypes.c_int),
+        ("cover_legacy", ctypes.c_int),
     ]
 
 

@@ -408,3 +408,21 @@ def test_cover_equals_the_brute_force_objective(wh, seed, gs):
     want = rr.cover_brute_force(xyb, gs, 1, 3, 3, 4.0, _dct_fwd)
     assert np.array_equal(got, want)
     assert (got == 16).any() or (got == 32).any() or gs < 1000  # merges happen
+
+
+@pytest.mark.parametrize("wh,seed,gs", [((64, 64), 6, 2000), ((61, 45), 7, 5000)])
+def test_legacy_cover_equals_the_brute_force_objective(wh, seed, gs):
+    """The fixed-quantizer request's cover objective (legacy rate proxy 1.0
+    per bit with 2 / 34 / 34 fixed bits, flat cell weights, lambda scale 1)."""
+    from jpxl_amd.synth import synthetic_rgb
+
+    w, h = wh
+    rgb = synthetic_rgb(w, h, seed, 8)
+    xyb, _, _ = orc.prepare_frame(rgb, 8)
+    qp = orc.QuantParams(gs, 1, 16, 2, 2, 0, 1.0, 1.0)
+    qp.cover_legacy = 1
+    got = orc.cover(xyb, qp)
+    want = rr.cover_brute_force(xyb, gs, 1, 2, 2, 1.0, _dct_fwd, calibrated=False, donor=False)
+    assert np.array_equal(got, want)
+    qp.cover_legacy = 0
+    assert not np.array_equal(got, orc.cover(xyb, qp)) or gs > 4000  # the objectives differ
