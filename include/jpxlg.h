@@ -313,6 +313,18 @@ int32_t jpxlg_profile_read(jpxlg_ctx* ctx, char* names, double* total_ms, uint64
 /* Planes are planar f32 linear sRGB, 3 x W x H, host memory. */
 int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t height, const float* reference,
                          const float* candidate, double* score, double* terms, uint32_t* nterms);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): jpxlg_score_pair with the two images in DEVICE memory
+ * (planar f32 linear sRGB, 3 x W x H, read only) and every intermediate
+ * buffer in one file-scope device array of JPXLG_NOALLOC_SCRATCH_BYTES per
+ * GPU, so the call allocates nothing.  A pair whose pyramids need more
+ * (about 1.3 MP) fails with JPXLG_E_UNSUPPORTED.  Every call on one GPU, from
+ * any context, uses the same array: calls that overlap on one GPU race.
+ * Synchronous on the context's stream. */
+#define JPXLG_NOALLOC_SCRATCH_BYTES ((size_t)96 << 20)
+int32_t jpxlg_score_pair_noalloc(jpxlg_ctx* ctx, uint32_t width, uint32_t height, const float* d_reference,
+                                 const float* d_candidate, double* score, double* terms, uint32_t* nterms);
 /* count N x N blocks (N = 8, 16, 32), row-major; inverse = 0 forward DCT_2D. */
 int32_t jpxlg_dct_blocks(jpxlg_ctx* ctx, int32_t side, int32_t inverse, uint32_t count,
                          const float* in, float* out);

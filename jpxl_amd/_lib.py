@@ -139,6 +139,7 @@ SIGNATURES = {
     "jpxlg_server_pending": (_i32, [_vp]),
     "jpxlg_server_destroy": (None, [_vp]),
     "jpxlg_score_pair": (_i32, [_vp, _u32, _u32, _vp, _vp, _dp, _dp, _u32p]),
+    "jpxlg_score_pair_noalloc": (_i32, [_vp, _u32, _u32, _vp, _vp, _dp, _dp, _u32p]),
     "jpxlg_dct_blocks": (_i32, [_vp, _i32, _i32, _u32, _vp, _vp]),
     "jpxlg_epf_step": (_i32, [_vp, _i32, _u32, _u32, _vp, _vp, _vp]),
     "jpxlg_gaborish": (_i32, [_vp, _u32, _u32, _vp, _vp]),

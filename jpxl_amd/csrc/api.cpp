@@ -23,6 +23,13 @@
 
 using namespace jpxlg;
 
+// The scratch of jpxlg_score_pair_noalloc: a file-scope device array, one
+// per GPU, held until the process ends.  Host code reaches it only through
+// the device address hipGetSymbolAddress gives at context creation
+// (jpxlg_ctx::static_scratch); its name is a host-side stand-in.  Every
+// call on one GPU shares it.
+static __device__ __align__(256) char scratch[JPXLG_NOALLOC_SCRATCH_BYTES];
+
 struct DepthSet {
   float* thr = nullptr;
   float* lut = nullptr;
@@ -87,6 +94,10 @@ struct jpxlg_ctx {
   size_t up_cap = 0, up_off = 0;
   char* rb_host = nullptr;
   size_t rb_cap = 0;
+  // This device's address of the no-allocation scratch array (from
+  // hipGetSymbolAddress at creation) and pinned space for its sums.
+  char* static_scratch = nullptr;
+  double* static_sums = nullptr;
   // Launch shapes of the streaming SSIMULACRA2 pass per size class (class =
   // log2 of a chain's total pixels - 16, clamped): units per workgroup and
   // the XCD swizzle, as picked by jpxlg_autotune (default 4, off).  No
@@ -389,6 +400,15 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_lut8, sizeof(lut8)));
   HIP_TRY(ctx.get(), hipMemcpy(ctx->d_lut8, lut8, sizeof(lut8), hipMemcpyHostToDevice));
   for (int i = 0; i < 8; i++) ctx->slots.emplace_back(new jpxlg_ctx::Slot());
+  {
+    // The device address of the no-allocation scratch array on this device
+    // (the array's name is a host-side stand-in, not a device address).
+    void* p = nullptr;
+    HIP_TRY(ctx.get(), hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    ctx->static_scratch = (char*)p;
+    HIP_TRY(ctx.get(), hipHostMalloc((void**)&ctx->static_sums, sizeof(double) * 18 * kMaxScales,
+                                     hipHostMallocDefault));
+  }
   ctx->up_cap = (size_t)256 << 10;
   HIP_TRY(ctx.get(), hipHostMalloc((void**)&ctx->up_host, ctx->up_cap, hipHostMallocDefault));
   *out = ctx.release();
@@ -415,6 +435,7 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   }
   if (ctx->up_host) (void)hipHostFree(ctx->up_host);
   if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
+  if (ctx->static_sums) (void)hipHostFree(ctx->static_sums);
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
   if (ctx->side) (void)hipStreamSynchronize(ctx->side);
@@ -2267,45 +2288,68 @@ extern "C" int32_t jpxlg_gaborish(jpxlg_ctx* ctx, uint32_t width, uint32_t heigh
   return JPXLG_OK;
 }
 
-extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t height,
-                                    const float* reference, const float* candidate, double* score,
-                                    double* terms, uint32_t* nterms) {
-  if (!ctx || !reference || !candidate) return JPXLG_E_INVALID;
-  if (width < 8 || height < 8) return fail(ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
-  HIP_TRY(ctx, hipSetDevice(ctx->device));
-  const int w = (int)width, h = (int)height;
+// Device bytes jpxlg_score_pair's intermediate buffers take for a w x h
+// pair: every scale's linear RGB (scale 0 only when the inputs are copied
+// in), positive XYB, mu1, s11 and the candidate's positive XYB, then the
+// pooling partials and the per-scale sums; 256-byte aligned carve-outs.
+static size_t score_pair_bytes(int w, int h, bool device_inputs) {
   int n, sw[kMaxScales], sh[kMaxScales];
   scale_dims(w, h, &n, sw, sh);
-  size_t pyr = 0;
-  for (int s = 0; s < n; s++) pyr += (size_t)sw[s] * sh[s];
-  size_t ptiles = 0;
-  for (int s = 0; s < n; s++) ptiles += (size_t)pool_tiles(sw[s], sh[s]);
-  DevBuf<char> mem;
-  size_t bytes = 18 * pyr * 4 + ptiles * 18 * 8 + kMaxScales * 18 * 8 + 256 * 256;
-  HIP_TRY(ctx, hipMalloc(&mem.p, bytes));
-  Arena A{mem.p, bytes, 0};
-  float *rrgb[kMaxScales][3], *crgb[kMaxScales][3], *rx[kMaxScales][3], *mu[kMaxScales][3],
-      *s11[kMaxScales][3], *cx[kMaxScales][3];
+  size_t b = 0;
+  auto take = [&b](size_t bytes) { b = ((b + 255) & ~(size_t)255) + bytes; };
   for (int s = 0; s < n; s++)
     for (int c = 0; c < 3; c++) {
-      size_t sp = (size_t)sw[s] * sh[s] * 4;
-      rrgb[s][c] = (float*)A.take(sp); crgb[s][c] = (float*)A.take(sp);
+      const size_t sp = (size_t)sw[s] * sh[s] * 4;
+      if (s > 0 || !device_inputs) { take(sp); take(sp); }
+      take(sp); take(sp); take(sp); take(sp);
+    }
+  size_t ptiles = 0;
+  for (int s = 0; s < n; s++) ptiles += (size_t)pool_tiles(sw[s], sh[s]);
+  take(ptiles * 18 * 8);
+  take(kMaxScales * 18 * 8);
+  return b + 256;
+}
+
+// ssimulacra2.rs:79-135 on a w x h pair whose buffers are carved out of
+// [base, base + bytes) (256-aligned): the inputs are copied from the host,
+// or read in place when device_inputs.  The per-scale sums come back into
+// host_sums (pinned or pageable) on the context stream, synchronously.
+static int score_pair_run(jpxlg_ctx* ctx, int w, int h, const float* reference, const float* candidate,
+                          bool device_inputs, char* base, size_t bytes, double* host_sums, double* score,
+                          double* terms, uint32_t* nterms) {
+  int n, sw[kMaxScales], sh[kMaxScales];
+  scale_dims(w, h, &n, sw, sh);
+  size_t ptiles = 0;
+  for (int s = 0; s < n; s++) ptiles += (size_t)pool_tiles(sw[s], sh[s]);
+  Arena A{base, bytes, 0};
+  const size_t px = (size_t)w * h;
+  float *rrgb[kMaxScales][3], *crgb[kMaxScales][3], *rx[kMaxScales][3], *mu[kMaxScales][3],
+      *s11[kMaxScales][3], *cx[kMaxScales][3];
+  bool carved = true;
+  for (int s = 0; s < n; s++)
+    for (int c = 0; c < 3; c++) {
+      const size_t sp = (size_t)sw[s] * sh[s] * 4;
+      if (s == 0 && device_inputs) {
+        // read only: the colour stage writes positive XYB and the next scale
+        rrgb[0][c] = const_cast<float*>(reference) + c * px;
+        crgb[0][c] = const_cast<float*>(candidate) + c * px;
+      } else {
+        rrgb[s][c] = (float*)A.take(sp);
+        crgb[s][c] = (float*)A.take(sp);
+      }
       rx[s][c] = (float*)A.take(sp); mu[s][c] = (float*)A.take(sp);
       s11[s][c] = (float*)A.take(sp); cx[s][c] = (float*)A.take(sp);
+      carved = carved && rrgb[s][c] && crgb[s][c] && rx[s][c] && mu[s][c] && s11[s][c] && cx[s][c];
     }
   double* partials = (double*)A.take(ptiles * 18 * 8);
   double* sums_d = (double*)A.take(kMaxScales * 18 * 8);
-  bool carved = partials && sums_d;
-  for (int s = 0; s < n; s++)
-    for (int c = 0; c < 3; c++)
-      carved = carved && rrgb[s][c] && crgb[s][c] && rx[s][c] && mu[s][c] && s11[s][c] && cx[s][c];
-  if (!carved) return fail(ctx, JPXLG_E_OOM, "arena sizing");
-  const size_t px = (size_t)w * h;
+  if (!carved || !partials || !sums_d) return fail(ctx, JPXLG_E_OOM, "arena sizing");
   hipStream_t st = ctx->stream;
-  for (int c = 0; c < 3; c++) {
-    HIP_TRY(ctx, hipMemcpyAsync(rrgb[0][c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
-    HIP_TRY(ctx, hipMemcpyAsync(crgb[0][c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
-  }
+  if (!device_inputs)
+    for (int c = 0; c < 3; c++) {
+      HIP_TRY(ctx, hipMemcpyAsync(rrgb[0][c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
+      HIP_TRY(ctx, hipMemcpyAsync(crgb[0][c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
+    }
   PoolScales P{};
   P.L.nscales = n;
   for (int s = 0; s < n; s++) {
@@ -2326,14 +2370,48 @@ extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t hei
   launch_blur_pool(st, ctx->d_tab, P, partials);
   launch_reduce_scales(st, P.L, partials, sums_d);
   HIP_TRY(ctx, hipGetLastError());
-  double sums[kMaxScales * 18];
-  HIP_TRY(ctx, hipMemcpyAsync(sums, sums_d, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(host_sums, sums_d, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
-  MetricResult m = pool_terms(sums, n, sw, sh);
+  MetricResult m = pool_terms(host_sums, n, sw, sh);
   if (score) *score = m.score;
   if (terms) memcpy(terms, m.terms, sizeof(double) * 18 * n);
   if (nterms) *nterms = (uint32_t)(18 * n);
   return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_score_pair(jpxlg_ctx* ctx, uint32_t width, uint32_t height,
+                                    const float* reference, const float* candidate, double* score,
+                                    double* terms, uint32_t* nterms) {
+  if (!ctx || !reference || !candidate) return JPXLG_E_INVALID;
+  if (width < 8 || height < 8) return fail(ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
+  if (width > 65535 * 8 || height > 65535 * 8) return fail(ctx, JPXLG_E_UNSUPPORTED, "frame too large");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const size_t bytes = score_pair_bytes((int)width, (int)height, false);
+  DevBuf<char> mem;
+  if (!device_alloc(ctx, (void**)&mem.p, bytes)) return fail(ctx, JPXLG_E_OOM, "device allocation failed");
+  double sums[kMaxScales * 18];
+  return score_pair_run(ctx, (int)width, (int)height, reference, candidate, false, mem.p, bytes, sums, score,
+                        terms, nterms);
+}
+
+// The no-allocation score: every intermediate buffer lives in the
+// file-scope array `scratch` (see its definition), through this context's
+// device address of it.
+extern "C" int32_t jpxlg_score_pair_noalloc(jpxlg_ctx* ctx, uint32_t width, uint32_t height,
+                                            const float* d_reference, const float* d_candidate, double* score,
+                                            double* terms, uint32_t* nterms) {
+  if (!ctx || !d_reference || !d_candidate) return JPXLG_E_INVALID;
+  if (width < 8 || height < 8) return fail(ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
+  if (width > 65535 * 8 || height > 65535 * 8) return fail(ctx, JPXLG_E_UNSUPPORTED, "frame too large");
+  if (!ctx->static_scratch) return fail(ctx, JPXLG_E_DEVICE, "the static scratch array has no device address");
+  const size_t bytes = score_pair_bytes((int)width, (int)height, true);
+  if (bytes > JPXLG_NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, JPXLG_E_UNSUPPORTED,
+                "image too large for the no-allocation score: its pyramids need " + std::to_string(bytes) +
+                    " bytes of the " + std::to_string((size_t)JPXLG_NOALLOC_SCRATCH_BYTES) + "-byte scratch array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  return score_pair_run(ctx, (int)width, (int)height, d_reference, d_candidate, true, ctx->static_scratch,
+                        JPXLG_NOALLOC_SCRATCH_BYTES, ctx->static_sums, score, terms, nterms);
 }
 
 // ----------------------------------------------------------------------------
