@@ -181,6 +181,43 @@ int32_t jpxlg_render_score(jpxlg_img* img, const jpxlg_restoration* r, double* s
 int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* lf, uint8_t* block_map,
                             int8_t* cfl_x, int8_t* cfl_b, int32_t* lf_factors);
 
+/* The QuantizedFrameIr hand-off (jpxl-encode/src/vardct/plan.rs:519-636),
+ * the layout attach_entropy consumes (jpxl-encode-policy/src/lib.rs:1012):
+ * coeff_arena (192 * blocks_x * blocks_y int32) holds every varblock's X, Y
+ * and B coefficient arrays back to back, each in the array's own raster
+ * order (VarblockCoefficients; the arena of VarblockCoefficients::from_arena
+ * with starts start, start + n, start + 2n for n = side * side);
+ * lf_groups (3 * blocks_x * blocks_y int32) holds each LF group's three
+ * LfQuantPlanes planes in raster order, groups in raster order (a group is
+ * 256 x 256 blocks, clipped at the frame edge); varblocks (one per varblock,
+ * at most blocks_x * blocks_y) lists them in the IR's order: LF groups in
+ * raster order, inside a group by the block-raster order of their top-left
+ * blocks (the BlockInfo order of QuantizedLfGroup::coefficients).  Any
+ * output pointer may be NULL. */
+typedef struct {
+  uint32_t lf_group;  /* raster index of the LF group */
+  uint32_t bx, by;    /* top-left 8x8 block in the frame */
+  uint32_t side;      /* 8, 16 or 32 */
+  uint32_t hf_mul;    /* the varblock's HfMul */
+  uint32_t reserved;
+  uint64_t start;     /* its X array's offset in coeff_arena */
+} jpxlg_varblock;
+int32_t jpxlg_plan_download_ir(jpxlg_img* img, int32_t* coeff_arena, int32_t* lf_groups, jpxlg_varblock* varblocks,
+                               uint32_t* n_varblocks);
+
+/* PerceptualEvaluator::evaluate(&ValidatedPixelPlan) (quality.rs:87-157):
+ * a plan built elsewhere (the CPU planner) replaces the image's structure and
+ * levels, and the next jpxlg_render_score renders and scores exactly it.
+ * wire: the quantizer the frame signals (global_scale, quant_lf, qm scales;
+ * hf_mul is every varblock's HfMul when hf_mul is NULL); block_map, cfl_x,
+ * cfl_b, lf_factors as jpxlg_set_structure; hf_mul (optional, blocks_x *
+ * blocks_y) each varblock's HfMul at its top-left block; quant and lf as
+ * jpxlg_plan_download returns them (footprint layout).  A later quantize or
+ * structure call replaces the plan. */
+int32_t jpxlg_set_plan(jpxlg_img* img, const jpxlg_quantizer* wire, const uint8_t* block_map, const int8_t* cfl_x,
+                       const int8_t* cfl_b, const int32_t* lf_factors, const uint32_t* hf_mul, const int32_t* quant,
+                       const int32_t* lf);
+
 #define JPXLG_BUF_XYB 0          /* 3 x W x H f32: source XYB */
 #define JPXLG_BUF_COEFF 1        /* 3 x 8bw x 8bh f32: forward DCT (footprint) */
 #define JPXLG_BUF_RENDER_XYB 2   /* 3 x W x H f32: reconstructed XYB after restoration */

@@ -315,6 +315,33 @@ class Image:
                                                         _ptr(cb), _ptr(lff)))
         return Plan(q, lf, bm, cx, cb, (int(lff[0]), int(lff[1])))
 
+    def plan_download_ir(self):
+        """The QuantizedFrameIr hand-off: (coefficient arena, per-group LF
+        planes, varblock table as a list of dicts in the IR's order)."""
+        arena = np.empty(192 * self.bw * self.bh, np.int32)
+        lfg = np.empty(3 * self.bw * self.bh, np.int32)
+        table = (_lib.Varblock * (self.bw * self.bh))()
+        n = ctypes.c_uint32()
+        self.ctx.check(self.ctx.lib.jpxlg_plan_download_ir(self.handle, _ptr(arena), _ptr(lfg), table,
+                                                           ctypes.byref(n)))
+        vbs = [dict(lf_group=v.lf_group, bx=v.bx, by=v.by, side=v.side, hf_mul=v.hf_mul, start=v.start)
+               for v in table[: n.value]]
+        return arena, lfg, vbs
+
+    def set_plan(self, wire: Quantizer, plan: "Plan", hf_mul: np.ndarray | None = None) -> None:
+        """PerceptualEvaluator::evaluate's input: a plan built elsewhere becomes
+        this image's structure and levels (the next render_score renders it)."""
+        bm = np.ascontiguousarray(plan.block_map, np.uint8)
+        cx = np.ascontiguousarray(plan.cfl_x, np.int8)
+        cb = np.ascontiguousarray(plan.cfl_b, np.int8)
+        lff = np.array(plan.lf_factors, np.int32)
+        q = np.ascontiguousarray(plan.quant, np.int32)
+        lf = np.ascontiguousarray(plan.lf, np.int32)
+        muls = None if hf_mul is None else np.ascontiguousarray(hf_mul, np.uint32)
+        self.ctx.check(self.ctx.lib.jpxlg_set_plan(self.handle, ctypes.byref(wire), _ptr(bm), _ptr(cx), _ptr(cb),
+                                                   _ptr(lff), _ptr(muls) if muls is not None else None, _ptr(q),
+                                                   _ptr(lf)))
+
     def plan_download_map(self) -> np.ndarray:
         """The current block map (valid right after plan_structure)."""
         bm = np.empty((self.bh, self.bw), np.uint8)

@@ -78,6 +78,20 @@ class QualityRequest(ctypes.Structure):
     ]
 
 
+class Varblock(ctypes.Structure):
+    """jpxlg_varblock: one entry of the QuantizedFrameIr varblock order."""
+
+    _fields_ = [
+        ("lf_group", ctypes.c_uint32),
+        ("bx", ctypes.c_uint32),
+        ("by", ctypes.c_uint32),
+        ("side", ctypes.c_uint32),
+        ("hf_mul", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("start", ctypes.c_uint64),
+    ]
+
+
 class QualityResult(ctypes.Structure):
     """jpxlg_quality_result."""
 
@@ -125,6 +139,8 @@ SIGNATURES = {
     "jpxlg_render_score": (_i32, [_vp, ctypes.POINTER(Restoration), _dp, _dp, _dp, _u32p]),
     "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),
+    "jpxlg_plan_download_ir": (_i32, [_vp, _vp, _vp, _vp, _u32p]),
+    "jpxlg_set_plan": (_i32, [_vp, ctypes.POINTER(Quantizer), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_encode_quality": (_i32, [_vp, ctypes.POINTER(QualityRequest), ctypes.POINTER(QualityResult)]),
     "jpxlg_rung_quantizer": (_i32, [_u32, _i32, ctypes.POINTER(Quantizer)]),
     "jpxlg_batch_probe": (_i32, [ctypes.c_uint32, ctypes.POINTER(_vp), ctypes.POINTER(Quantizer),

@@ -276,6 +276,7 @@ size_t image_bytes(int w, int h) {
   b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
   b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
   b += bw * bh * 4 + 256;             // AQ field + scratch
+  b += bw * bh * 4;                   // a caller-supplied plan's per-varblock HfMul
   b += 64 + 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + bw * bh + 256;  // structure readback block
   b += 256 * 256;  // alignment of every carved buffer
   return b;
@@ -334,6 +335,11 @@ struct jpxlg_img {
   // whether it is active (a mode other than off whose field is not neutral).
   float* aq_adj = nullptr;
   AqScratch* d_aq = nullptr;
+  // A caller-supplied plan (jpxlg_set_plan): each varblock's HfMul at its
+  // top-left block, on the device and the host, used while explicit_muls.
+  uint32_t* d_vb_mul = nullptr;
+  std::vector<uint32_t> vb_mul;
+  bool explicit_muls = false;
   int aq_mode = kAqOff;
   bool aq_active = false;
   bool preconditioned = false;
@@ -562,6 +568,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->d_gray = (int*)A.take(4);
   img->aq_adj = (float*)A.take(bpx * 4);
   img->d_aq = (AqScratch*)A.take(sizeof(AqScratch));
+  img->d_vb_mul = (uint32_t*)A.take(bpx * 4);
   size_t ptiles = 0;
   for (int s = 0; s < img->nscales; s++)
     ptiles += (size_t)std::max(pool_tiles(img->sw[s], img->sh[s]), pool_units(img->sw[s], img->sh[s]));
@@ -570,7 +577,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   bool carved = img->d_cost[2] && img->d_rad[2] && img->d_cover_queue && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
                 img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rows && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
-                img->aq_adj && img->d_aq;
+                img->aq_adj && img->d_aq && img->d_vb_mul;
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
              img->d_list[c] && img->rx[c] && img->aux[c];
@@ -704,6 +711,7 @@ static int finish_structure(jpxlg_img* img, const jpxlg_quantizer* q) {
   jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
   hipStream_t st = ctx->stream;
+  img->explicit_muls = false;  // a new structure drops a supplied plan
   const size_t bpx = (size_t)g.bw * g.bh, tiles = (size_t)g.tw * g.th;
   {
     Prof p(ctx, "structure_lists");
@@ -806,6 +814,45 @@ extern "C" int32_t jpxlg_set_structure(jpxlg_img* img, const uint8_t* bmap, cons
   return finish_structure(img, nullptr);
 }
 
+// PerceptualEvaluator::evaluate(&ValidatedPixelPlan) (quality.rs:87-157):
+// a plan built elsewhere becomes the image's structure and levels, so the
+// next render renders exactly it.
+extern "C" int32_t jpxlg_set_plan(jpxlg_img* img, const jpxlg_quantizer* wire, const uint8_t* bmap,
+                                  const int8_t* cfl_x, const int8_t* cfl_b, const int32_t* lf_factors,
+                                  const uint32_t* hf_mul, const int32_t* quant, const int32_t* lf) {
+  if (!img || !wire || !bmap || !quant || !lf) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  int rc = check_quantizer(ctx, wire);
+  if (rc) return rc;
+  const Geom& g = img->g;
+  const size_t bpx = (size_t)g.bw * g.bh, cpx = (size_t)g.cstride * g.crows;
+  for (size_t i = 0; i < 3 * cpx; i++)
+    if (quant[i] > kMaxQuant || quant[i] < -kMaxQuant)
+      return fail(ctx, JPXLG_E_RANGE, "a quantized coefficient outside the quantizer's working range");
+  std::vector<uint32_t> muls(bpx, 0u);
+  for (size_t i = 0; i < bpx; i++)
+    if (bmap[i]) {
+      muls[i] = hf_mul ? hf_mul[i] : wire->hf_mul;
+      if (muls[i] < 1 || muls[i] > 0x7fffffffu) return fail(ctx, JPXLG_E_RANGE, "a varblock HfMul outside 1..2^31-1");
+    }
+  rc = jpxlg_set_structure(img, bmap, cfl_x, cfl_b, lf_factors);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  HIP_TRY(ctx, hipMemcpyAsync(img->d_vb_mul, muls.data(), bpx * 4, hipMemcpyHostToDevice, st));
+  for (int c = 0; c < 3; c++) {
+    HIP_TRY(ctx, hipMemcpyAsync(img->quant[c], quant + c * cpx, cpx * 4, hipMemcpyHostToDevice, st));
+    HIP_TRY(ctx, hipMemcpyAsync(img->lf[c], lf + c * bpx, bpx * 4, hipMemcpyHostToDevice, st));
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  img->vb_mul = std::move(muls);
+  img->explicit_muls = true;
+  img->last_q = *wire;
+  img->has_quant = true;
+  img->has_render = false;
+  img->generation++;
+  return JPXLG_OK;
+}
+
 // prepare_gaborish_frame (jpxl-encode-policy/src/lib.rs:2256-2268): the
 // source XYB planes replaced by their 12-step Jacobi Gaborish inverse, so a
 // plan rendered with Gaborish restores them.  Any structure is dropped.
@@ -865,6 +912,12 @@ static int aq_wire(const jpxlg_quantizer& q, bool fine, jpxlg_quantizer* w) {
 // The probe parameters of quantizer q on this image: the wire quantizer and
 // the per-varblock HfMul field when the image's AQ field is active.
 static void probe_params_for(const jpxlg_img* img, const jpxlg_quantizer& q, int sharpness, ProbeParams* p) {
+  if (img->explicit_muls) {
+    // a caller-supplied plan: its wire quantizer and per-varblock HfMul
+    build_probe_params(q, sharpness, p);
+    p->vb_mul = img->d_vb_mul;
+    return;
+  }
   const bool fine = img->aq_mode >= kAqFineMasking;
   jpxlg_quantizer w = q;
   const int refine_only = img->aq_active ? aq_wire(q, fine, &w) : 0;
@@ -912,6 +965,12 @@ extern "C" int32_t jpxlg_plan_aq(jpxlg_img* img, uint32_t* hf_mul, jpxlg_quantiz
   if (!img) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = img->ctx;
   if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  if (img->explicit_muls) {
+    if (wire) *wire = img->last_q;
+    if (refine_only) *refine_only = 0;
+    if (hf_mul) memcpy(hf_mul, img->vb_mul.data(), img->vb_mul.size() * 4);
+    return JPXLG_OK;
+  }
   const bool fine = img->aq_mode >= kAqFineMasking;
   jpxlg_quantizer w = img->last_q;
   const int ro = img->aq_active ? aq_wire(img->last_q, fine, &w) : 0;
@@ -1273,6 +1332,12 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
 static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs, int n, const jpxlg_quantizer* qs,
                          const jpxlg_restoration* r, ChainRun* run) {
   const ChainLayout L = chain_layout(n);
+  if (qs)
+    for (int k = 0; k < n; k++)
+      if (imgs[k]->explicit_muls) {  // quantizing replaces a supplied plan
+        imgs[k]->explicit_muls = false;
+        imgs[k]->generation++;
+      }
   jpxlg_ctx::Slot* sl = nullptr;
   int rc = take_slot(ctx, L.total, &sl);
   if (rc) return rc;
@@ -1408,6 +1473,65 @@ extern "C" int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* 
   if (cfl_x) memcpy(cfl_x, img->cfl_x.data(), img->cfl_x.size());
   if (cfl_b) memcpy(cfl_b, img->cfl_b.data(), img->cfl_b.size());
   if (lf_factors) { lf_factors[0] = img->lf_factors[0]; lf_factors[1] = img->lf_factors[1]; }
+  return JPXLG_OK;
+}
+
+// The QuantizedFrameIr hand-off (jpxl-encode/src/vardct/plan.rs:519-636):
+// the quantized plan regathered on the device from the footprint layout
+// into the IR's order (LF groups in raster order; inside a group, varblocks
+// in block-raster order of their top-left blocks; per varblock its X, Y and
+// B coefficient arrays, each in the array's own raster order), and each
+// group's LF planes.
+extern "C" int32_t jpxlg_plan_download_ir(jpxlg_img* img, int32_t* coeff_arena, int32_t* lf_groups,
+                                          jpxlg_varblock* varblocks, uint32_t* n_varblocks) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_structure) return fail(ctx, JPXLG_E_INVALID, "no plan structure yet");
+  if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  const Geom& g = img->g;
+  const size_t bpx = (size_t)g.bw * g.bh;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (coeff_arena || lf_groups) {
+    // the CfL sample buffers are free outside a structure build
+    int32_t* d_arena = (int32_t*)img->d_hf_s;
+    int32_t* d_lfg = (int32_t*)img->d_lf_s;
+    launch_gather_ir(st, g, img->d_list, img->list_count, img->d_lf_off, img->quant, img->lf, d_arena, d_lfg);
+    HIP_TRY(ctx, hipGetLastError());
+    if (coeff_arena)
+      HIP_TRY(ctx, hipMemcpyAsync(coeff_arena, d_arena, 192 * bpx * 4, hipMemcpyDeviceToHost, st));
+    if (lf_groups) HIP_TRY(ctx, hipMemcpyAsync(lf_groups, d_lfg, 3 * bpx * 4, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+  }
+  if (varblocks || n_varblocks) {
+    std::vector<uint32_t> muls(bpx, 0u);
+    const int rc = jpxlg_plan_aq(img, muls.data(), nullptr, nullptr);
+    if (rc) return rc;
+    const int gw = (g.bw + 255) / 256, gh = (g.bh + 255) / 256;
+    uint64_t off = 0;
+    uint32_t n = 0;
+    for (int gy = 0; gy < gh; gy++)
+      for (int gx = 0; gx < gw; gx++)
+        for (int by = 256 * gy; by < std::min(g.bh, 256 * gy + 256); by++)
+          for (int bx = 256 * gx; bx < std::min(g.bw, 256 * gx + 256); bx++) {
+            const size_t i = (size_t)by * g.bw + bx;
+            const int side = img->bmap[i];
+            if (!side) continue;
+            if (varblocks) {
+              jpxlg_varblock& v = varblocks[n];
+              v.lf_group = (uint32_t)(gy * gw + gx);
+              v.bx = (uint32_t)bx;
+              v.by = (uint32_t)by;
+              v.side = (uint32_t)side;
+              v.hf_mul = muls[i];
+              v.reserved = 0;
+              v.start = 192 * off;
+            }
+            off += (uint64_t)(side / 8) * (side / 8);
+            n++;
+          }
+    if (n_varblocks) *n_varblocks = n;
+  }
   return JPXLG_OK;
 }
 

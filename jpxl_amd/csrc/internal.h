@@ -65,6 +65,9 @@ struct ProbeParams {
   double cover_rate[3];
   double cover_fixed[3];
   int32_t cover_flat;
+  // A plan supplied by the caller: each varblock's HfMul at its top-left
+  // block (bw x bh, u32), used instead of the AQ field when non-null.
+  const uint32_t* vb_mul;
 };
 
 // field.rs:383-395 mul_of: baseline * 2^-adj rounded to the nearest integer,
@@ -92,7 +95,16 @@ __host__ __device__ inline float field_at(const float* p, uint32_t i) {
 #endif
 }
 
+__host__ __device__ inline uint32_t u32_at(const uint32_t* p, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return *(const __attribute__((address_space(1))) uint32_t*)((const __attribute__((address_space(1))) char*)p + 4u * i);
+#else
+  return p[i];
+#endif
+}
+
 __host__ __device__ inline uint32_t aq_footprint_mul(const ProbeParams* pp, int bw, int bh, int bx, int by, int n) {
+  if (pp->vb_mul) return u32_at(pp->vb_mul, (uint32_t)(by * bw + bx));
   const float* adj = pp->aq_adj;
   float sum = 0.0f;
   for (int dy = 0; dy < n; dy++)
@@ -111,7 +123,7 @@ __host__ __device__ inline uint32_t aq_footprint_mul(const ProbeParams* pp, int 
 // side 8n at block (bx, by) (reconstruct.rs:27-36, restoration.rs:338-349).
 __device__ inline void varblock_scales(const ProbeParams* pp, int bw, int bh, int bx, int by, int n, float* sc,
                                        float* sigma) {
-  if (!pp->aq_adj) {
+  if (!pp->aq_adj && !pp->vb_mul) {
     sc[0] = pp->sc[0];
     sc[1] = pp->sc[1];
     sc[2] = pp->sc[2];
@@ -361,6 +373,11 @@ void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3]
                      AqScratch* scratch, float* adj);
 // Each listed varblock's HfMul at its top-left block.
 void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var);
+// The QuantizedFrameIr arena (every varblock's X, Y, B coefficients in LF
+// group then block-raster order) and the per-group LF planes.
+void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3],
+                      const uint32_t* lf_off, int32_t* const quant[3], int32_t* const lf[3], int32_t* arena,
+                      int32_t* lf_out);
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
                           const int counts[3], uint32_t* muls);
 // Pooling tiles of one scale (one partial row of 18 sums per tile).

@@ -1108,7 +1108,8 @@ __global__ __launch_bounds__(256) void k_aq_block_muls(Geom g, const ProbeParams
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
   const int bx = (int)(list[i] & 0xffffu), by = (int)(list[i] >> 16);
-  muls[(size_t)by * g.bw + bx] = pp->aq_adj ? aq_footprint_mul(pp, g.bw, g.bh, bx, by, N / 8) : pp->aq_baseline;
+  muls[(size_t)by * g.bw + bx] =
+      (pp->aq_adj || pp->vb_mul) ? aq_footprint_mul(pp, g.bw, g.bh, bx, by, N / 8) : pp->aq_baseline;
 }
 
 // ----------------------------------------------------------------------------
@@ -1396,6 +1397,60 @@ void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3]
   if (mode >= kAqFineMasking) k_aq_erode<<<div_up(count, 256), 256, 0, s>>>(g.bw, g.bh, act, ero);
   if (mode != kAqEdgeRefine) k_aq_mean<<<1, 1024, 0, s>>>(mode >= kAqFineMasking ? ero : act, count, scratch);
   k_aq_adj<<<div_up(count, 256), 256, 0, s>>>(count, mode, act, ero, scratch, adj);
+}
+
+// The QuantizedFrameIr hand-off (jpxl-encode/src/vardct/plan.rs:519-636):
+// each varblock's three channels, each in its coefficient array's own
+// raster order (cell k of a side-s square at footprint row 8 by + k / s,
+// column 8 bx + k % s), stored contiguously at 3 * 64 * lf_off[origin] in
+// the frame's arena (lf_off is the varblock's position in LF-group, then
+// block-raster order, counted in 8x8 blocks); one workgroup per varblock.
+__global__ __launch_bounds__(256) void k_gather_ir(Geom g, const uint32_t* __restrict__ list, int count, int side,
+                                                   const uint32_t* __restrict__ lf_off,
+                                                   const int32_t* __restrict__ QX, const int32_t* __restrict__ QY,
+                                                   const int32_t* __restrict__ QB, int32_t* __restrict__ arena) {
+  const int v = blockIdx.x;
+  if (v >= count) return;
+  const uint32_t e = list[v];
+  const int bx = (int)(e & 0xffffu), by = (int)(e >> 16);
+  const size_t base = (size_t)192 * lf_off[(size_t)by * g.bw + bx];
+  const int nn = side * side;
+  const int32_t* q[3] = {QX, QY, QB};
+  for (int i = threadIdx.x; i < 3 * nn; i += blockDim.x) {
+    const int c = i / nn, k = i % nn;
+    arena[base + i] = q[c][(size_t)(8 * by + k / side) * g.cstride + 8 * bx + k % side];
+  }
+}
+
+// Per LF group (raster order), the group's three LF planes in raster order:
+// LfQuantPlanes of every QuantizedLfGroup, concatenated.  A group of the
+// frame's block grid is 256 x 256 blocks, clipped at the right and bottom.
+__global__ __launch_bounds__(256) void k_gather_lf_groups(Geom g, const int32_t* __restrict__ LX,
+                                                          const int32_t* __restrict__ LY,
+                                                          const int32_t* __restrict__ LB,
+                                                          int32_t* __restrict__ out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)g.bw * g.bh) return;
+  const int bx = (int)(i % g.bw), by = (int)(i / g.bw);
+  const int gx = bx / 256, gy = by / 256;
+  const int gw = min(256, g.bw - 256 * gx), gh = min(256, g.bh - 256 * gy);
+  const size_t off = 3 * ((size_t)256 * gy * g.bw + (size_t)256 * gx * gh);
+  const size_t in_g = (size_t)(by % 256) * gw + bx % 256, plane = (size_t)gw * gh;
+  out[off + in_g] = LX[i];
+  out[off + plane + in_g] = LY[i];
+  out[off + 2 * plane + in_g] = LB[i];
+}
+
+void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3],
+                      const uint32_t* lf_off, int32_t* const quant[3], int32_t* const lf[3], int32_t* arena,
+                      int32_t* lf_out) {
+  const int sides[3] = {8, 16, 32};
+  for (int i = 0; i < 3; i++)
+    if (counts[i])
+      k_gather_ir<<<counts[i], 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0], quant[1], quant[2],
+                                            arena);
+  const size_t bpx = (size_t)g.bw * g.bh;
+  k_gather_lf_groups<<<div_up(bpx, 256), 256, 0, s>>>(g, lf[0], lf[1], lf[2], lf_out);
 }
 
 void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var) {
