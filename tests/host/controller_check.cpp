@@ -95,6 +95,56 @@ int cc_navigate_curve(double target, int effort, int pixel, int exact, int build
   return 0;
 }
 
+// The navigator over a caller's probe (the CPU restatement's whole encode):
+// probe(rung, fresh, &score) -> status.
+typedef int (*cc_probe_fn)(uint32_t rung, int fresh, double* score);
+int cc_navigate_callback(double target, int effort, int max_probes, uint32_t predicted, int mode, cc_probe_fn fn,
+                         cc_outcome* out) {
+  memset(out, 0, sizeof(*out));
+  ProbeFn probe = [&](uint32_t rung, bool fresh, double* score, double* ms) -> int {
+    *ms = 0.0;
+    return fn(rung, fresh ? 1 : 0, score);
+  };
+  ControllerOutcome oc;
+  const int rc = mode == 1 ? bisect_controller(target, max_probes, probe, &oc)
+                           : navigate_controller(target, effort, max_probes, probe, &oc, predicted);
+  if (rc) return rc;
+  out->rung = oc.rung;
+  out->score = oc.score;
+  out->met = oc.met;
+  out->status = oc.status;
+  out->chosen_fresh = oc.chosen_fresh;
+  out->anchor_rung = oc.anchor_rung;
+  out->pixel_probes = oc.pixel_probes;
+  out->structural_builds = oc.structural_builds;
+  out->priced = oc.priced;
+  out->nprobes = (int32_t)oc.probes.size();
+  for (size_t i = 0; i < oc.probes.size() && i < 32; i++) {
+    out->probe_rungs[i] = oc.probes[i].rung;
+    out->probe_scores[i] = oc.probes[i].score;
+    out->probe_fresh[i] = oc.probes[i].fresh;
+  }
+  return 0;
+}
+
+// The quantizer and planning tier of a quality probe at `rung`.
+int cc_rung_policy(uint32_t rung, int effort, uint32_t* fields6, int32_t* tiers3, float* ls_ztb) {
+  const QualityPolicy p = quality_policy(effort);
+  const jpxlg_quantizer q = p.quantizer_at(rung);
+  fields6[0] = q.global_scale;
+  fields6[1] = q.hf_mul;
+  fields6[2] = q.quant_lf;
+  fields6[3] = q.x_qm_scale;
+  fields6[4] = q.b_qm_scale;
+  fields6[5] = (uint32_t)q.truncate;
+  tiers3[0] = p.cover_mode;
+  tiers3[1] = p.enable_cfl;
+  tiers3[2] = p.epf_iters * 16 + p.sharpness;
+  ls_ztb[0] = q.lambda_scale;
+  ls_ztb[1] = q.zero_token_bits;
+  return 0;
+}
+
 double cc_pool_terms(const double* sums, int ns, const int* sw, const int* sh, double* terms, double* raw) {
   MetricResult m = pool_terms(sums, ns, sw, sh);
   memcpy(terms, m.terms, sizeof(double) * 18 * ns);

@@ -273,3 +273,17 @@ def test_library_weights_sit_at_their_terms(cc):
                     val = 16.0 if norm == 0 else 2.0
                     assert terms[idx] == val
                     assert raw.value == wts[idx] * val
+
+
+def test_cpu_encode_runs_the_whole_search():
+    """The CPU restatement of the whole encode (the bench's CPU baseline): the
+    navigator over oracle probes meets the target on a small image."""
+    import cpu_encode
+    from jpxl_amd.synth import synthetic_rgb
+
+    rgb = synthetic_rgb(96, 80, 3, 8)
+    out, plan = cpu_encode.encode(rgb, 8, 85.0, BALANCED)
+    assert out.met and out.score >= 85.0 and 1 <= out.nprobes <= 6
+    assert out.probe_fresh[0] == 1
+    quant, lf, bmap = plan[:3]
+    assert quant.shape == (3, 80, 96) and lf.shape == (3, 10, 12) and bmap.shape == (10, 12)
