@@ -74,12 +74,14 @@ def algorithmic_bytes(name: str, w: int, h: int, sizes: dict, bits: int = 16) ->
         return (6.0 if bits > 8 else 3.0) * px + 24.0 * px
     if name == "colour_pyramid":                        # restored XYB in; positive XYB of every scale out
         return 12.0 * px + 12.0 * pyr
-    if name == "blur_pool":                             # every scale: candidate XYB' + reference XYB', mu1, s11
-        return 48.0 * pyr
+    if name == "blur_h":                                # every scale and channel: x1, x2 in; three row blurs + x2 transposed out
+        return 72.0 * pyr
+    if name == "blur_pool":                             # every scale and channel: three row blurs, x1, x2, mu1, s11 in (transposed)
+        return 84.0 * pyr
     if name == "ref_colour":                            # linear RGB in; positive XYB of every scale out
         return 12.0 * px + 12.0 * pyr
-    if name == "ref_blur_moments":                      # XYB' in; mu1, s11 out, every scale
-        return 36.0 * pyr
+    if name == "ref_blur":                              # both passes: x1 in, two row blurs + x1 transposed out; row blurs in, mu1, s11 out
+        return 96.0 * pyr
     if name == "cover":                                 # XYB read per candidate size; f64 cost + bound out
         cand = blocks + blocks / 4.0 + blocks / 16.0
         return 3 * 12.0 * px + 16.0 * cand + blocks
@@ -92,9 +94,9 @@ def algorithmic_bytes(name: str, w: int, h: int, sizes: dict, bits: int = 16) ->
 
 # Device kernels behind each bench label (the rocprofv3 kernel names).
 KERNEL_SYMBOLS = {
-    "blur_pool": "k_blur_pool_stream", "quantize_render": "k_quantize_all<true>",
+    "blur_pool": "k_blur_v<1>", "blur_h": "k_blur_h<1>", "quantize_render": "k_quantize_all<true>",
     "quantize": "k_quantize_all<false>", "colour_pyramid": "k_colour_pyr_b", "epf_step1": "k_epf_b<1>",
-    "xyb_fwd": "k_xyb_fwd<false>", "ref_blur_moments": "k_blur_moments_all", "ref_colour": "k_colour_pyr_ref",
+    "xyb_fwd": "k_xyb_fwd<false>", "ref_colour": "k_colour_pyr_ref",
     "cfl_tiles": "k_cfl_tiles", "render": "k_render_all<0>",
 }
 TRAFFIC_SUMMARY = os.path.join(ROOT, "profiles", "encode_q85_2048_rgb16_hbm_traffic.md")

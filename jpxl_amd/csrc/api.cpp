@@ -251,15 +251,12 @@ size_t image_bytes(int w, int h) {
   const size_t cpx = 64 * bw * bh;
   int n, sw[kMaxScales], sh[kMaxScales];
   scale_dims(w, h, &n, sw, sh);
-  size_t pyr = 0, pyr1 = 0;
-  for (int s = 0; s < n; s++) {
-    pyr += (size_t)sw[s] * sh[s];
-    if (s > 0) pyr1 += (size_t)sw[s] * sh[s];
-  }
+  size_t pyr = 0;
+  for (int s = 0; s < n; s++) pyr += (size_t)sw[s] * sh[s];
   size_t b = 0;
   b += 6 * px;                       // rgb (up to 16-bit)
   b += 3 * px * 4;                   // source XYB
-  b += 9 * pyr * 4;                  // reference pyramid: XYB', mu1, s11
+  b += 12 * pyr * 4;                 // reference pyramid: XYB' (and transposed), mu1, s11
   b += 3 * cpx * 4 * 2;              // coefficients f32 + quantized i32
   b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
   b += 3 * bw * bh * 4;              // work lists
@@ -268,17 +265,16 @@ size_t image_bytes(int w, int h) {
   b += (((bw + 3) / 4) * ((bh + 3) / 4) + 1) * 4;  // cover's exact-pass queue
   b += cpx * 18 + bw * bh * (16 + 3 * 16) + 2 * ((bw + 7) / 8 + 1) * ((bh + 7) / 8 + 1) * 4;  // CfL samples
   b += 6 * px * 4;                   // render planes + ping-pong
-  b += 3 * pyr * 4;                  // candidate XYB' pyramid
-  b += 3 * pyr1 * 4;                 // linear RGB pyramid (scales >= 1)
-  b += 3 * px * 4;                   // linear RGB at scale 0 (reference build)
-  size_t tiles = 0;
-  for (int s = 0; s < n; s++) tiles += (size_t)std::max(pool_tiles(sw[s], sh[s]), pool_units(sw[s], sh[s]));
-  b += tiles * 18 * 8 + kMaxScales * 18 * 8 + 4096;
+  b += 15 * pyr * 4;                 // candidate XYB' pyramid (and transposed), three horizontal blurs
+  size_t units = 0;
+  for (int s = 0; s < n; s++) units += (size_t)pool_units(sw[s], sh[s]);
+  b += units * 18 * 8 + kMaxScales * 18 * 8 + 4096;
+  b += 3 * kMaxScales * sizeof(LineSet) + 2 * (3 * kMaxScales + 1) * 4;  // the reference's line sets
   b += 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + 4096;
   b += bw * bh * 4 + 256;             // AQ field + scratch
   b += bw * bh * 4;                   // a caller-supplied plan's per-varblock HfMul
   b += 64 + 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + bw * bh + 256;  // structure readback block
-  b += 256 * 256;  // alignment of every carved buffer
+  b += 512 * 256;  // alignment of every carved buffer
   return b;
 }
 
@@ -295,7 +291,8 @@ struct jpxlg_img {
   void* d_rgb = nullptr;
   float* xyb[3];
   float* ref_x[kMaxScales][3];
-  float* ref_mu[kMaxScales][3];
+  float* ref_xT[kMaxScales][3];   // the metric reference planes transposed (column x at x * h)
+  float* ref_mu[kMaxScales][3];   // mu1 and s11 of every scale, transposed
   float* ref_s11[kMaxScales][3];
   float* coeff[3];
   int32_t* quant[3];
@@ -312,8 +309,10 @@ struct jpxlg_img {
   int* d_gray = nullptr;
   float* rx[3];
   float* aux[3];
-  float* cand_x[kMaxScales][3];
-  float* cand_rgb[kMaxScales][3];  // [0] is the reference's linear RGB scratch
+  float* cand_x[kMaxScales][3];   // [0] also the reference's linear RGB while the image is built
+  float* cand_xT[kMaxScales][3];  // the candidate planes transposed
+  float* cand_h[kMaxScales][3][3];  // the horizontal blurs of the moments, transposed
+  LineSet* d_ref_sets = nullptr;  // the reference's blur line sets, then their unit prefixes (device)
   double* d_partials = nullptr;
   CflLfScratch* d_lfc = nullptr;
   double* d_sums = nullptr;
@@ -492,6 +491,8 @@ static int get_depth(jpxlg_ctx* ctx, int bits, DepthSet* out) {
 // Images
 // ----------------------------------------------------------------------------
 
+static int upload_small(jpxlg_ctx* ctx, hipStream_t st, void* dev, const void* src, size_t n);
+
 static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_t bits,
                         const void* rgb, bool device_src, jpxlg_img** out) {
   if (!ctx || !out || !rgb) return fail(ctx, JPXLG_E_INVALID, "null argument");
@@ -538,12 +539,15 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     size_t sp = (size_t)img->sw[s] * img->sh[s];
     for (int c = 0; c < 3; c++) {
       img->ref_x[s][c] = (float*)A.take(sp * 4);
+      img->ref_xT[s][c] = (float*)A.take(sp * 4);
       img->ref_mu[s][c] = (float*)A.take(sp * 4);
       img->ref_s11[s][c] = (float*)A.take(sp * 4);
       img->cand_x[s][c] = (float*)A.take(sp * 4);
-      img->cand_rgb[s][c] = (float*)A.take(sp * 4);
+      img->cand_xT[s][c] = (float*)A.take(sp * 4);
+      for (int m = 0; m < 3; m++) img->cand_h[s][c][m] = (float*)A.take(sp * 4);
     }
   }
+  img->d_ref_sets = (LineSet*)A.take(3 * kMaxScales * sizeof(LineSet) + 2 * (3 * kMaxScales + 1) * 4);
   img->sigma = (float*)A.take(bpx * 4);
   for (int i = 0; i < 3; i++) img->d_cost[i] = (double*)A.take(bpx * 8);
   for (int i = 0; i < 3; i++) img->d_rad[i] = (double*)A.take(bpx * 8);
@@ -569,21 +573,21 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->aq_adj = (float*)A.take(bpx * 4);
   img->d_aq = (AqScratch*)A.take(sizeof(AqScratch));
   img->d_vb_mul = (uint32_t*)A.take(bpx * 4);
-  size_t ptiles = 0;
-  for (int s = 0; s < img->nscales; s++)
-    ptiles += (size_t)std::max(pool_tiles(img->sw[s], img->sh[s]), pool_units(img->sw[s], img->sh[s]));
-  img->d_partials = (double*)A.take(ptiles * 18 * 8);
+  size_t punits = 0;
+  for (int s = 0; s < img->nscales; s++) punits += (size_t)pool_units(img->sw[s], img->sh[s]);
+  img->d_partials = (double*)A.take(punits * 18 * 8);
   img->d_sums = (double*)A.take(kMaxScales * 18 * 8);
   bool carved = img->d_cost[2] && img->d_rad[2] && img->d_cover_queue && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
                 img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rows && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
-                img->aq_adj && img->d_aq && img->d_vb_mul;
+                img->aq_adj && img->d_aq && img->d_vb_mul && img->d_ref_sets;
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
              img->d_list[c] && img->rx[c] && img->aux[c];
     for (int s = 0; s < img->nscales; s++)
-      carved = carved && img->ref_x[s][c] && img->ref_mu[s][c] && img->ref_s11[s][c] &&
-               img->cand_x[s][c] && img->cand_rgb[s][c];
+      carved = carved && img->ref_x[s][c] && img->ref_xT[s][c] && img->ref_mu[s][c] && img->ref_s11[s][c] &&
+               img->cand_x[s][c] && img->cand_xT[s][c] && img->cand_h[s][c][0] && img->cand_h[s][c][1] &&
+               img->cand_h[s][c][2];
   }
   if (!carved) return fail(ctx, JPXLG_E_OOM, "arena sizing");
   hipStream_t st = ctx->stream;
@@ -598,7 +602,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     lut = l16;
   }
   // Source XYB + linear RGB (scale 0 of the metric reference).
-  float** lin0 = img->cand_rgb[0];
+  float** lin0 = img->cand_x[0];
   { Prof p(ctx, "xyb_fwd"); launch_xyb_fwd(st, g, (int)bits, img->d_rgb, lut, img->xyb, lin0, img->d_gray); }
   // The reference pyramid (reference.rs:176-283): positive XYB of every
   // scale in one launch, then mu1 and s11.
@@ -607,8 +611,45 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     launch_colour_pyr_ref(st, img->nscales, img->sw, img->sh, lin0, img->ref_x);
   }
   {
-    Prof p(ctx, "ref_blur_moments");
-    launch_blur_moments_all(st, ctx->d_tab, img->nscales, img->sw, img->sh, img->ref_x, img->ref_mu, img->ref_s11);
+    // mu1 and s11 of every scale and channel: the horizontal pass leaves the
+    // moments' row blurs (in the candidate's blur planes, free until the
+    // first probe) and the transposed reference; the vertical pass finishes.
+    LineSet sets[3 * kMaxScales];
+    int off_b[3 * kMaxScales + 1], off_c[3 * kMaxScales + 1];
+    int ns = 0, nb = 0, ncol = 0;
+    for (int s = 0; s < img->nscales; s++)
+      for (int c = 0; c < 3; c++, ns++) {
+        LineSet& L = sets[ns];
+        L = LineSet{};
+        L.w = img->sw[s];
+        L.h = img->sh[s];
+        L.x1 = img->ref_x[s][c];
+        L.hT[0] = img->cand_h[s][c][0];
+        L.hT[1] = img->cand_h[s][c][1];
+        L.xT = img->ref_xT[s][c];
+        L.out[0] = img->ref_mu[s][c];
+        L.out[1] = img->ref_s11[s][c];
+        off_b[ns] = nb;
+        off_c[ns] = ncol;
+        nb += blur_bands(L.w, L.h);
+        ncol += pool_units(L.w, L.h);
+      }
+    off_b[ns] = nb;
+    off_c[ns] = ncol;
+    // one upload: the sets, then the band prefixes, then the column prefixes
+    static_assert(sizeof(LineSet) % 8 == 0, "line sets stay 8-byte aligned");
+    char blob[sizeof(sets) + sizeof(off_b) + sizeof(off_c)];
+    const size_t sb = sizeof(LineSet) * ns, ob = 4 * (size_t)(ns + 1);
+    memcpy(blob, sets, sb);
+    memcpy(blob + sb, off_b, ob);
+    memcpy(blob + sb + ob, off_c, ob);
+    int rc = upload_small(ctx, st, img->d_ref_sets, blob, sb + 2 * ob);
+    if (rc) return rc;
+    const int* d_ob = (const int*)((char*)img->d_ref_sets + sb);
+    const int* d_oc = (const int*)((char*)img->d_ref_sets + sb + ob);
+    Prof p(ctx, "ref_blur");
+    launch_blur_h(st, 0, ctx->d_tab, img->d_ref_sets, d_ob, ns, nb);
+    launch_blur_v(st, 0, ctx->d_tab, img->d_ref_sets, d_oc, ns, ncol, 4, 0);
   }
   HIP_TRY(ctx, hipGetLastError());
   // The grayscale flag stays on the device (the CfL kernels read it); the
@@ -1075,10 +1116,11 @@ namespace {
 // Per-slot layout: descs, probe params, error words, workgroup offsets, and
 // the 18-per-scale metric sums of every image.
 struct ChainLayout {
-  size_t desc = 0, pp = 0, err = 0, off = 0, sums = 0, total = 0;
+  size_t desc = 0, pp = 0, sets = 0, err = 0, off = 0, sums = 0, total = 0;
   int n = 0;
-  // offset arrays (each n+1 ints, except pool: 6n+1)
-  int q_off[3], r_off[3], g_off, e_off, c_off[kDescScales], p_off;
+  // offset arrays (each n+1 ints, except the blur passes': 18n+1, one entry
+  // per line set)
+  int q_off[3], r_off[3], g_off, e_off, c_off[kDescScales], b_off, v_off;
   int noffs = 0;
 };
 
@@ -1089,6 +1131,7 @@ ChainLayout chain_layout(int n) {
   size_t o = 0;
   L.desc = o; o = al(o + sizeof(ProbeDesc) * n);
   L.pp = o; o = al(o + sizeof(ProbeParams) * n);
+  L.sets = o; o = al(o + sizeof(LineSet) * 3 * kDescScales * n);
   L.err = o; o = al(o + 4 * n);
   int k = 0;
   for (int i = 0; i < 3; i++) { L.q_off[i] = k; k += n + 1; }
@@ -1096,7 +1139,8 @@ ChainLayout chain_layout(int n) {
   L.g_off = k; k += n + 1;
   L.e_off = k; k += n + 1;
   for (int i = 0; i < kDescScales; i++) { L.c_off[i] = k; k += n + 1; }
-  L.p_off = k; k += kDescScales * n + 1;
+  L.b_off = k; k += 3 * kDescScales * n + 1;
+  L.v_off = k; k += 3 * kDescScales * n + 1;
   L.noffs = k;
   L.off = o; o = al(o + 4 * (size_t)k);
   L.sums = o; o = al(o + sizeof(double) * 18 * kDescScales * n);
@@ -1154,8 +1198,9 @@ struct ChainRun {
 
 // Workgroup totals per stage of a filled chain.
 struct ChainTotals {
-  int q[3] = {0, 0, 0}, r[3] = {0, 0, 0}, g = 0, e = 0, c[kDescScales] = {0, 0, 0, 0, 0, 0}, p = 0;
-  int upw = 4, swizzle = 0;  // streaming-pass launch shape
+  int q[3] = {0, 0, 0}, r[3] = {0, 0, 0}, g = 0, e = 0, c[kDescScales] = {0, 0, 0, 0, 0, 0};
+  int nsets = 0, bands = 0, cols = 0;  // blur line sets, horizontal bands, vertical columns
+  int upw = 4, swizzle = 0;            // vertical-pass launch shape
 };
 
 static int tune_class(double pixels) {
@@ -1179,6 +1224,7 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
                       const jpxlg_restoration* r, char* H, char* dev, const ChainLayout& L, ChainTotals* T) {
   ProbeDesc* hd = (ProbeDesc*)(H + L.desc);
   ProbeParams* hp = (ProbeParams*)(H + L.pp);
+  LineSet* hs = (LineSet*)(H + L.sets);
   int* ho = (int*)(H + L.off);
   ChainTotals t;
   double pixels = 0.0;
@@ -1216,19 +1262,34 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
     P.err = (uint32_t*)(dev + L.err) + k;
     int pt = 0;
     for (int s = 0; s < img->nscales; s++)
-      for (int c = 0; c < 3; c++) {
-        P.ref_x[s][c] = img->ref_x[s][c];
-        P.ref_mu[s][c] = img->ref_mu[s][c];
-        P.ref_s11[s][c] = img->ref_s11[s][c];
-        P.cand_x[s][c] = img->cand_x[s][c];
-        P.cand_rgb[s][c] = img->cand_rgb[s][c];
-      }
+      for (int c = 0; c < 3; c++) P.cand_x[s][c] = img->cand_x[s][c];
     for (int s = 0; s < img->nscales; s++) {
       P.ptile_off[s] = pt;
       P.punits[s] = pool_units(img->sw[s], img->sh[s]);
       pt += P.punits[s];
     }
     P.partials = img->d_partials;
+    // the blur passes' line sets: (scale, channel) of this image
+    for (int s = 0; s < img->nscales; s++)
+      for (int c = 0; c < 3; c++) {
+        LineSet& S = hs[t.nsets];
+        S.w = img->sw[s];
+        S.h = img->sh[s];
+        S.x1 = img->ref_x[s][c];
+        S.x2 = img->cand_x[s][c];
+        for (int m = 0; m < 3; m++) S.hT[m] = img->cand_h[s][c][m];
+        S.xT = img->cand_xT[s][c];
+        S.x1T = img->ref_xT[s][c];
+        S.x2T = img->cand_xT[s][c];
+        S.mu1T = img->ref_mu[s][c];
+        S.s11T = img->ref_s11[s][c];
+        S.partials = img->d_partials + (size_t)P.ptile_off[s] * 18 + c * 6;
+        ho[L.b_off + t.nsets] = t.bands;
+        ho[L.v_off + t.nsets] = t.cols;
+        t.bands += blur_bands(S.w, S.h);
+        t.cols += pool_units(S.w, S.h);
+        t.nsets++;
+      }
     P.sums = (double*)(dev + L.sums) + (size_t)k * 18 * kDescScales;
     DepthSet dd;
     const int rc = get_depth(ctx, img->bits, &dd);
@@ -1249,10 +1310,6 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
     t.e += epf_tiles(g.w, g.h);
     ho[L.c_off[0] + k] = t.c[0];  // the colour pyramid's 64x64 tiles
     t.c[0] += pyramid_tiles(g.w, g.h);
-    for (int s = 0; s < kDescScales; s++) {
-      ho[L.p_off + k * kDescScales + s] = t.p;
-      if (s < img->nscales) t.p += 3 * P.punits[s];
-    }
   }
   for (int i = 0; i < 3; i++) {
     ho[L.q_off[i] + n] = t.q[i];
@@ -1261,7 +1318,8 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
   ho[L.g_off + n] = t.g;
   ho[L.e_off + n] = t.e;
   ho[L.c_off[0] + n] = t.c[0];
-  ho[L.p_off + kDescScales * n] = t.p;
+  ho[L.b_off + t.nsets] = t.bands;
+  ho[L.v_off + t.nsets] = t.cols;
   *T = t;
   return JPXLG_OK;
 }
@@ -1311,9 +1369,14 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
       Prof p(ctx, "colour_pyramid", st);
       launch_colour_pyr_b(st, D, O + L.c_off[0], n, T.c[0], src);
     }
+    const LineSet* S = (const LineSet*)(dev + L.sets);
+    {
+      Prof p(ctx, "blur_h", st);
+      launch_blur_h(st, 1, ctx->d_tab, S, O + L.b_off, T.nsets, T.bands);
+    }
     {
       Prof p(ctx, "blur_pool", st);
-      launch_blur_pool_stream(st, ctx->d_tab, D, O + L.p_off, kDescScales * n, T.p, T.upw, T.swizzle);
+      launch_blur_v(st, 1, ctx->d_tab, S, O + L.v_off, T.nsets, T.cols, T.upw, T.swizzle);
     }
     { Prof p(ctx, "pool_reduce", st); launch_reduce_b(st, D, n); }
   }
@@ -2412,26 +2475,67 @@ extern "C" int32_t jpxlg_gaborish(jpxlg_ctx* ctx, uint32_t width, uint32_t heigh
   return JPXLG_OK;
 }
 
-// Device bytes jpxlg_score_pair's intermediate buffers take for a w x h
-// pair: every scale's linear RGB (scale 0 only when the inputs are copied
-// in), positive XYB, mu1, s11 and the candidate's positive XYB, then the
-// pooling partials and the per-scale sums; 256-byte aligned carve-outs.
-static size_t score_pair_bytes(int w, int h, bool device_inputs) {
+// jpxlg_score_pair's buffers for a w x h pair, carved in this order from
+// one 256-aligned region: the scale-0 linear RGB of both images (unless the
+// inputs are read in place), two ping-pong sets of the next scales' linear
+// RGB, this scale's positive XYB of both images, one channel's blur planes
+// (both transposed XYB planes, three horizontal blurs, mu1 and s11; reused
+// channel after channel and scale after scale), the pooling partials and
+// sums, and the line-set table.
+struct PairBuffers {
+  float *in_r[3], *in_c[3];         // scale 0 linear RGB
+  float *pp_r[2][3], *pp_c[2][3];   // scales >= 1 linear RGB (ping-pong)
+  float *rx[3], *cx[3];             // positive XYB of the current scale
+  float *x1T, *x2T, *h[3], *mu, *s11;
+  double *partials, *sums;
+  char* table;
+};
+static constexpr int kPairSets = 2 * 3 * kMaxScales;
+static constexpr size_t kPairTable = kPairSets * (sizeof(LineSet) + 4 * sizeof(int));
+
+static bool carve_pair(Arena& A, int w, int h, bool device_inputs, const float* reference, const float* candidate,
+                       PairBuffers* B) {
   int n, sw[kMaxScales], sh[kMaxScales];
   scale_dims(w, h, &n, sw, sh);
-  size_t b = 0;
-  auto take = [&b](size_t bytes) { b = ((b + 255) & ~(size_t)255) + bytes; };
-  for (int s = 0; s < n; s++)
-    for (int c = 0; c < 3; c++) {
-      const size_t sp = (size_t)sw[s] * sh[s] * 4;
-      if (s > 0 || !device_inputs) { take(sp); take(sp); }
-      take(sp); take(sp); take(sp); take(sp);
+  const size_t px = (size_t)w * h, p1 = n > 1 ? (size_t)sw[1] * sh[1] : 0;
+  bool ok = true;
+  for (int c = 0; c < 3; c++) {
+    if (device_inputs) {
+      B->in_r[c] = const_cast<float*>(reference) + c * px;
+      B->in_c[c] = const_cast<float*>(candidate) + c * px;
+    } else {
+      B->in_r[c] = (float*)A.take(px * 4);
+      B->in_c[c] = (float*)A.take(px * 4);
     }
-  size_t ptiles = 0;
-  for (int s = 0; s < n; s++) ptiles += (size_t)pool_tiles(sw[s], sh[s]);
-  take(ptiles * 18 * 8);
-  take(kMaxScales * 18 * 8);
-  return b + 256;
+    for (int k = 0; k < 2; k++) {
+      B->pp_r[k][c] = p1 ? (float*)A.take(p1 * 4) : nullptr;
+      B->pp_c[k][c] = p1 ? (float*)A.take(p1 * 4) : nullptr;
+      ok = ok && (!p1 || (B->pp_r[k][c] && B->pp_c[k][c]));
+    }
+    B->rx[c] = (float*)A.take(px * 4);
+    B->cx[c] = (float*)A.take(px * 4);
+    ok = ok && B->in_r[c] && B->in_c[c] && B->rx[c] && B->cx[c];
+  }
+  B->x1T = (float*)A.take(px * 4);
+  B->x2T = (float*)A.take(px * 4);
+  for (int m = 0; m < 3; m++) ok = ok && (B->h[m] = (float*)A.take(px * 4)) != nullptr;
+  B->mu = (float*)A.take(px * 4);
+  B->s11 = (float*)A.take(px * 4);
+  size_t units = 0;
+  for (int s = 0; s < n; s++) units += (size_t)pool_units(sw[s], sh[s]);
+  B->partials = (double*)A.take(units * 18 * 8);
+  B->sums = (double*)A.take(kMaxScales * 18 * 8);
+  B->table = (char*)A.take(kPairTable);
+  return ok && B->x1T && B->x2T && B->mu && B->s11 && B->partials && B->sums && B->table;
+}
+
+// Device bytes jpxlg_score_pair's buffers take for a w x h pair.
+static size_t score_pair_bytes(int w, int h, bool device_inputs) {
+  Arena A{nullptr, ~(size_t)0 >> 1, 0};
+  PairBuffers B;
+  A.base = (char*)(uintptr_t)256;  // sizing only: nothing is written
+  carve_pair(A, w, h, device_inputs, nullptr, nullptr, &B);
+  return A.used + 256;
 }
 
 // ssimulacra2.rs:79-135 on a w x h pair whose buffers are carved out of
@@ -2443,58 +2547,78 @@ static int score_pair_run(jpxlg_ctx* ctx, int w, int h, const float* reference, 
                           double* terms, uint32_t* nterms) {
   int n, sw[kMaxScales], sh[kMaxScales];
   scale_dims(w, h, &n, sw, sh);
-  size_t ptiles = 0;
-  for (int s = 0; s < n; s++) ptiles += (size_t)pool_tiles(sw[s], sh[s]);
   Arena A{base, bytes, 0};
+  PairBuffers B;
+  if (!carve_pair(A, w, h, device_inputs, reference, candidate, &B)) return fail(ctx, JPXLG_E_OOM, "arena sizing");
   const size_t px = (size_t)w * h;
-  float *rrgb[kMaxScales][3], *crgb[kMaxScales][3], *rx[kMaxScales][3], *mu[kMaxScales][3],
-      *s11[kMaxScales][3], *cx[kMaxScales][3];
-  bool carved = true;
-  for (int s = 0; s < n; s++)
-    for (int c = 0; c < 3; c++) {
-      const size_t sp = (size_t)sw[s] * sh[s] * 4;
-      if (s == 0 && device_inputs) {
-        // read only: the colour stage writes positive XYB and the next scale
-        rrgb[0][c] = const_cast<float*>(reference) + c * px;
-        crgb[0][c] = const_cast<float*>(candidate) + c * px;
-      } else {
-        rrgb[s][c] = (float*)A.take(sp);
-        crgb[s][c] = (float*)A.take(sp);
-      }
-      rx[s][c] = (float*)A.take(sp); mu[s][c] = (float*)A.take(sp);
-      s11[s][c] = (float*)A.take(sp); cx[s][c] = (float*)A.take(sp);
-      carved = carved && rrgb[s][c] && crgb[s][c] && rx[s][c] && mu[s][c] && s11[s][c] && cx[s][c];
-    }
-  double* partials = (double*)A.take(ptiles * 18 * 8);
-  double* sums_d = (double*)A.take(kMaxScales * 18 * 8);
-  if (!carved || !partials || !sums_d) return fail(ctx, JPXLG_E_OOM, "arena sizing");
   hipStream_t st = ctx->stream;
   if (!device_inputs)
     for (int c = 0; c < 3; c++) {
-      HIP_TRY(ctx, hipMemcpyAsync(rrgb[0][c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
-      HIP_TRY(ctx, hipMemcpyAsync(crgb[0][c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
+      HIP_TRY(ctx, hipMemcpyAsync(B.in_r[c], reference + c * px, px * 4, hipMemcpyHostToDevice, st));
+      HIP_TRY(ctx, hipMemcpyAsync(B.in_c[c], candidate + c * px, px * 4, hipMemcpyHostToDevice, st));
     }
-  PoolScales P{};
-  P.L.nscales = n;
+  // line sets: per (scale, channel) the reference's, then the candidate's
+  LineSet sets[kPairSets];
+  int offs[kPairSets][4];  // band prefix {0, bands}, column prefix {0, columns}
+  PoolLayout PL{};
+  PL.nscales = n;
+  int pu = 0, ns = 0;
   for (int s = 0; s < n; s++) {
-    float** rd = s + 1 < n ? rrgb[s + 1] : nullptr;
-    float** cd = s + 1 < n ? crgb[s + 1] : nullptr;
-    launch_colour_metric(st, 1, sw[s], sh[s], rrgb[s], nullptr, nullptr, 0, rx[s], rd);
-    launch_colour_metric(st, 1, sw[s], sh[s], crgb[s], nullptr, nullptr, 0, cx[s], cd);
-    launch_blur_moments(st, sw[s], sh[s], ctx->d_tab, rx[s], mu[s], s11[s]);
-    P.w[s] = sw[s];
-    P.h[s] = sh[s];
+    PL.off[s] = pu;
+    PL.tiles[s] = pool_units(sw[s], sh[s]);
     for (int c = 0; c < 3; c++) {
-      P.x1[s][c] = rx[s][c];
-      P.mu1[s][c] = mu[s][c];
-      P.s11[s][c] = s11[s][c];
-      P.x2[s][c] = cx[s][c];
+      LineSet R{}, C{};
+      R.w = C.w = sw[s];
+      R.h = C.h = sh[s];
+      R.x1 = C.x1 = B.rx[c];
+      R.hT[0] = B.h[0];
+      R.hT[1] = B.h[1];
+      R.xT = B.x1T;
+      R.out[0] = B.mu;
+      R.out[1] = B.s11;
+      C.x2 = B.cx[c];
+      for (int m = 0; m < 3; m++) C.hT[m] = B.h[m];
+      C.xT = B.x2T;
+      C.x1T = B.x1T;
+      C.x2T = B.x2T;
+      C.mu1T = B.mu;
+      C.s11T = B.s11;
+      C.partials = B.partials + (size_t)pu * 18 + c * 6;
+      for (const LineSet* L : {&R, &C}) {
+        sets[ns] = *L;
+        offs[ns][0] = 0;
+        offs[ns][1] = blur_bands(L->w, L->h);
+        offs[ns][2] = 0;
+        offs[ns][3] = pool_units(L->w, L->h);
+        ns++;
+      }
+    }
+    pu += PL.tiles[s];
+  }
+  char blob[kPairTable];
+  memcpy(blob, sets, sizeof(LineSet) * ns);
+  memcpy(blob + sizeof(LineSet) * kPairSets, offs, sizeof(int) * 4 * ns);
+  int rc = upload_small(ctx, st, B.table, blob, kPairTable);
+  if (rc) return rc;
+  const LineSet* dsets = (const LineSet*)B.table;
+  const int* doffs = (const int*)(B.table + sizeof(LineSet) * kPairSets);
+  for (int s = 0, j = 0; s < n; s++) {
+    float* const* rin = s == 0 ? B.in_r : B.pp_r[(s - 1) & 1];
+    float* const* cin = s == 0 ? B.in_c : B.pp_c[(s - 1) & 1];
+    float** rd = s + 1 < n ? B.pp_r[s & 1] : nullptr;
+    float** cd = s + 1 < n ? B.pp_c[s & 1] : nullptr;
+    launch_colour_metric(st, 1, sw[s], sh[s], rin, nullptr, nullptr, 0, B.rx, rd);
+    launch_colour_metric(st, 1, sw[s], sh[s], cin, nullptr, nullptr, 0, B.cx, cd);
+    for (int c = 0; c < 3; c++, j += 2) {
+      launch_blur_h(st, 0, ctx->d_tab, dsets + j, doffs + 4 * j, 1, offs[j][1]);
+      launch_blur_v(st, 0, ctx->d_tab, dsets + j, doffs + 4 * j + 2, 1, offs[j][3], 4, 0);
+      launch_blur_h(st, 1, ctx->d_tab, dsets + j + 1, doffs + 4 * (j + 1), 1, offs[j + 1][1]);
+      launch_blur_v(st, 1, ctx->d_tab, dsets + j + 1, doffs + 4 * (j + 1) + 2, 1, offs[j + 1][3], 4, 0);
     }
   }
-  launch_blur_pool(st, ctx->d_tab, P, partials);
-  launch_reduce_scales(st, P.L, partials, sums_d);
+  launch_reduce_scales(st, PL, B.partials, B.sums);
   HIP_TRY(ctx, hipGetLastError());
-  HIP_TRY(ctx, hipMemcpyAsync(host_sums, sums_d, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(host_sums, B.sums, sizeof(double) * 18 * n, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   MetricResult m = pool_terms(host_sums, n, sw, sh);
   if (score) *score = m.score;

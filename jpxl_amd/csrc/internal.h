@@ -204,14 +204,10 @@ struct ProbeDesc {
   const CflParams* cflp;
   const ProbeParams* pp;
   uint32_t* err;
-  const float* ref_x[kDescScales][3];
-  const float* ref_mu[kDescScales][3];
-  const float* ref_s11[kDescScales][3];
-  float* cand_x[kDescScales][3];
-  float* cand_rgb[kDescScales][3];  // [0] unused here
-  double* partials;                 // 18 f64 per pooling tile, scale after scale
-  int ptile_off[kDescScales];       // first tile of each scale in `partials`
-  int punits[kDescScales];          // streaming units (partial rows) per scale and channel
+  float* cand_x[kDescScales][3];    // the candidate's positive XYB pyramid
+  double* partials;                 // 18 f64 per pooling unit (column), scale after scale
+  int ptile_off[kDescScales];       // first unit of each scale in `partials`
+  int punits[kDescScales];          // pooling units (columns) per scale
   double* sums;                     // 18 f64 per scale
   const float* thr;                 // depth quantizer thresholds and levels
   const float* lut;
@@ -290,7 +286,6 @@ void launch_render_all(hipStream_t s, int form, const ProbeDesc* D, const int* c
                        const int nwg[3], const ConstTables* tab, const DctMats* mats);
 void launch_gaborish_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 void launch_epf_b(hipStream_t s, int step, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
-void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src);
 // The colour pass of every scale in one launch: one workgroup per 32x32 tile
 // of scale 0 (pyramid_tiles per image, prefix `off`).
 constexpr int kMaxPyr = 6;
@@ -305,7 +300,6 @@ int quant_groups(int side, int count);
 int render_groups(int side, int count);
 int epf_tiles(int w, int h);
 int gaborish_blocks(int w, int h);
-int colour_blocks(int w, int h);
 void launch_fill(hipStream_t s, float* p, float v, size_t n);
 void launch_gaborish(hipStream_t s, int w, int h, float* const in[3], float* out[3]);
 // One Jacobi step of the encoder's Gaborish inverse: out = x + (f - Kx)/centre.
@@ -323,11 +317,6 @@ void launch_colour_metric(hipStream_t s, int mode, int w, int h, float* const in
 // XYB -> linear RGB at the source depth, one plane set (test download).
 void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], const float* thresholds,
                             const float* depth_lut, int levels, float* out[3]);
-void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
-                         float* mu1[3], float* s11[3]);
-// mu1 and s11 of every scale and channel of a reference pyramid, one launch.
-void launch_blur_moments_all(hipStream_t s, const ConstTables* tab, int ns, const int* sw, const int* sh,
-                             float* const (*x)[3], float* const (*mu)[3], float* const (*s11)[3]);
 // Hierarchical cover (kernels_plan.hip): candidate costs for every square
 // (f64, indexed by the candidate's top-left block) and the per-region
 // quadtree decision written into the block map.
@@ -380,37 +369,44 @@ void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], con
                       int32_t* lf_out);
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
                           const int counts[3], uint32_t* muls);
-// Pooling tiles of one scale (one partial row of 18 sums per tile).
-int pool_tiles(int w, int h);
-// Tiles of the streaming candidate pass (64-column strips x 64-row
-// segments); the batched chain's partial rows and reduction use these.
+// One scale and channel of the metric's blur passes: the planes each pass
+// reads and writes.  Transposed planes hold column x at x * h.
+struct LineSet {
+  int w, h;
+  const float* x1;     // horizontal pass: the reference plane (row-major)
+  const float* x2;     // horizontal pass (candidate): the candidate plane (row-major)
+  float* hT[3];        // horizontal outputs per moment (transposed)
+  float* xT;           // horizontal pass: its first moment's source, transposed
+  const float* x1T;    // vertical pass (candidate): reference and candidate planes
+  const float* x2T;    //   and the reference's mu1 and s11, transposed
+  const float* mu1T;
+  const float* s11T;
+  float* out[2];       // vertical pass (reference): mu1 and s11 (transposed)
+  double* partials;    // vertical pass (candidate): six sums per wave at unit * 18
+};
+// Workgroups of the horizontal pass over a w x h plane (bands of rows).
+int blur_bands(int w, int h);
+// Waves of the vertical pass over a w x h plane (groups of columns); each
+// candidate wave leaves one partial row of six sums per channel.
 int pool_units(int w, int h);
-// The streaming candidate pass: `units` (tile, channel) pairs over the
-// flattened (desc, scale) prefix `off` (in units), upw (1, 2 or 4) per
-// workgroup, with or without the XCD swizzle of the workgroup order.
-void launch_blur_pool_stream(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
-                             int units, int upw, int swizzle);
+// The blur passes over `nsets` line sets (device array) with unit prefixes
+// `off` (nsets + 1 ints, device): the horizontal pass in bands (one
+// workgroup each, blur_bands per set), the vertical pass in waves
+// (pool_units per set, upw 1, 2 or 4 per workgroup, with or without the XCD
+// swizzle of the workgroup order).
+// cand 0: the reference's moments (x1, x1^2), 1: the candidate's (x2, x2^2,
+// x1 x2) with the SSIM and edge pooling.
+void launch_blur_h(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
+                   int bands);
+void launch_blur_v(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
+                   int columns, int upw, int swizzle);
 constexpr int kMaxPoolScales = 6;
-// Where each scale's tiles start in the partials buffer (in tiles).
+// Where each scale's units start in the partials buffer.
 struct PoolLayout {
   int nscales;
   int off[kMaxPoolScales];
   int tiles[kMaxPoolScales];
 };
-// Every scale of one candidate: sizes and the reference (x1, mu1, s11) and
-// candidate (x2) positive-XYB planes per scale and channel.
-struct PoolScales {
-  PoolLayout L;
-  int w[kMaxPoolScales], h[kMaxPoolScales], tx[kMaxPoolScales];
-  const float* x1[kMaxPoolScales][3];
-  const float* mu1[kMaxPoolScales][3];
-  const float* s11[kMaxPoolScales][3];
-  const float* x2[kMaxPoolScales][3];
-};
-// The candidate blurs, SSIM/edge maps and per-tile f64 partial sums of every
-// scale in one launch (18 partial sums per tile; tile offsets and counts are
-// filled into P.L).
-void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, double* partials);
 // Sums every scale's partials into 18 f64 sums per scale, in a fixed order.
 void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums);
 

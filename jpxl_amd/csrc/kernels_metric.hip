@@ -1,13 +1,9 @@
 // gfx950 kernels for the colour stage and SSIMULACRA2: XYB^-1 + depth
-// quantizer, positive XYB, the 2:1 pyramid, the sigma-1.5 Gaussian blur and
-// the fused SSIM / artifact / detail-lost pooling with f64 partial sums.
-//
-// The blur is the 9-tap FIR that the reference's recursive truncated-cosine
-// filter equals in exact arithmetic (jpxl-perceptual/src/blur.rs:1-51): the
-// three poles sit on the unit circle at 2cos(w_k) with w_k = k*pi/10, k odd,
-// so the response to an impulse vanishes beyond |t| = 4.  The taps are the
-// recursion's own impulse response, computed on the host in f64.  Zero padding
-// at the plane border is kept, exactly as the metric defines it.
+// quantizer, positive XYB, the 2:1 pyramid, the metric's blur (the
+// reference's recursion, jpxl-perceptual/src/blur.rs, evaluated exactly in a
+// parallel form; see the blur section) and the SSIM / artifact / detail-lost
+// pooling with f64 partial sums.  Zero padding at the plane border is kept,
+// exactly as the metric defines it.
 #include "internal.h"
 
 namespace jpxlg {
@@ -233,35 +229,6 @@ __global__ __launch_bounds__(256) void k_colour_metric(
                              blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
-// Batched colour pass of scale s: workgroup (quad row, 256-quad block) of
-// each desc.  Scale 0 reads the reconstructed XYB (render planes, src 0, or
-// their ping-pong, src 1) through the depth quantizer; later scales read the
-// previous scale's downsampled linear RGB.
-template <int MODE>
-__global__ __launch_bounds__(256) void k_colour_b(const ProbeDesc* __restrict__ D, const int* __restrict__ off,
-                                                  int nd, int s, int src) {
-  __shared__ float s_thr[256], s_lut[256];
-  const int d = find_desc(off, nd, blockIdx.x);
-  DescSpace& P = desc_at(D, d);
-  const int w = P.sw[s], h = P.sh[s];
-  const float* thr = P.thr;
-  const float* lut = P.lut;
-  if constexpr (MODE == 0) stage_depth(P.thr, P.lut, P.levels, s_thr, s_lut, &thr, &lut);
-  const int lb = blockIdx.x - off[d], xb = ((w + 1) / 2 + 255) / 256;
-  const float* in[3];
-  float* dn[3] = {nullptr, nullptr, nullptr};
-  for (int c = 0; c < 3; c++) {
-    in[c] = MODE == 0 ? (src ? P.aux[c] : P.rx[c]) : P.cand_rgb[s][c];
-    if (s + 1 < P.nscales) dn[c] = P.cand_rgb[s + 1][c];
-  }
-  if (MODE == 0 && P.levels <= 256)
-    colour_quad<MODE, true>(w, h, in[0], in[1], in[2], thr, lut, P.levels, P.cand_x[s][0], P.cand_x[s][1],
-                            P.cand_x[s][2], dn[0], dn[1], dn[2], (lb % xb) * 256 + threadIdx.x, lb / xb);
-  else
-    colour_quad<MODE, false>(w, h, in[0], in[1], in[2], thr, lut, P.levels, P.cand_x[s][0], P.cand_x[s][1],
-                             P.cand_x[s][2], dn[0], dn[1], dn[2], (lb % xb) * 256 + threadIdx.x, lb / xb);
-}
-
 // The colour pass of every scale in one workgroup: a 32x32 tile of scale 0
 // and the tiles it covers at the coarser scales (16x16, 8x8, ... 1x1; six
 // scales at most).  Scale 0 converts its samples by 2x2 quads (XYB^-1 +
@@ -409,76 +376,305 @@ __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __r
 }
 
 // ----------------------------------------------------------------------------
-// Blur (FIR form of blur.rs) and pooling (pool.rs:97-278)
+// Blur (blur.rs:97-174, evaluated in a parallel form) and pooling
+// (pool.rs:97-278)
+// ----------------------------------------------------------------------------
+// The reference blurs a line u[0..L) with a recursion fed, at step p, the f32
+// pair sum s_p = f32(u[p-10] + u[p]) (zero outside the line) through three
+// f64 resonators, and emits output n = p - 4 rounded to f32.  The
+// resonators' summed response g to one sample is anti-periodic
+// (g[j+10] = -g[j]) and over one half-period it is the symmetric 9-tap blur
+// (g[4+t] = g[4-t] = fir[|t|]; g[9] ~ 1e-16), so the recursion's value before
+// its rounding is
+//     Y[p] = Psi[p] - Y[p-10],   Psi[p] = sum_{|t|<=4} fir[|t|] s[p-4+t],
+// Psi in f64 from the f32 pair sums, whose rounding is the reference's own.
+// A line is cut into segments of ten positions, one per lane: a lane holds
+// the ten classes p mod 10 of its segment, so each stride-10 chain runs
+// across consecutive lanes, and with sigma = (-1)^lane the chain is
+// Y = sigma * (prefix sum over lanes of sigma * Psi).  A line of up to 636
+// outputs takes one group of lanes (a power of two; several lines per wave).
+// A longer one is walked in blocks of 64 lanes; a block's first two lanes
+// repeat the previous block's last two segments (the pair sums and 9-tap
+// window of its third lane) and its chain continues from the previous
+// block's last lane.  This f64 evaluation and the reference's f64 resonators
+// differ by a few f64 ulps: about one output in 10^6 rounds to the other f32
+// neighbour.
+//
+// The horizontal pass walks the rows of a plane (16 waves per workgroup)
+// and writes its outputs transposed through an LDS tile, so that the
+// vertical pass walks contiguous lines again (column x at x * h).  The
+// reference's vertical pass stores mu1 and s11 (transposed); the
+// candidate's forms the SSIM and edge maps and one six-sum partial per wave.
 // ----------------------------------------------------------------------------
 
-constexpr int kTW = 64, kTH = 16, kHalo = 4;
-constexpr int kLW = kTW + 2 * kHalo, kLH = kTH + 2 * kHalo;
+constexpr int kSeg = 10;         // positions per lane
+constexpr int kBlockStep = 620;  // positions a continued block adds (62 lanes)
+constexpr int kHWaves = 16;      // horizontal pass: waves (1024 threads) per workgroup
+constexpr int kTile = 10752;     // horizontal pass: LDS tile (R rows x (10 G + 1), R = 1024 / G)
 
-// Accumulated in f64 with one rounding to f32 per pass, as the reference's
-// f64-state recursion rounds each pass's output once.
-__device__ __forceinline__ float fir9(const float* v, int stride, const double* hf) {
-  // v points at the centre sample.
-  double acc = hf[0] * (double)v[0];
-#pragma unroll
-  for (int t = 1; t <= 4; t++) acc = fma(hf[t], (double)v[-t * stride] + (double)v[t * stride], acc);
-  return (float)acc;
-}
+using SetSpace = const __attribute__((address_space(4))) LineSet;
 
-__device__ __forceinline__ void blur_moments_tile(int w, int h, const ConstTables* __restrict__ tab,
-                                                  const float* __restrict__ x, float* __restrict__ mu,
-                                                  float* __restrict__ s11, int x0, int y0) {
-  __shared__ float tin[kLH][kLW];
-  __shared__ float hb[2][kLH][kTW];
-  double hf[5];
-#pragma unroll
-  for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
-  for (int e = threadIdx.x; e < kLH * kLW; e += 256) {
-    int ly = e / kLW, lx = e % kLW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
-    tin[ly][lx] = (gx >= 0 && gx < w && gy >= 0 && gy < h) ? x[(size_t)gy * w + gx] : 0.0f;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kLH * kTW; e += 256) {
-    int ly = e / kTW, lx = e % kTW;
-    const float* c = &tin[ly][lx + kHalo];
-    // the squared input is rounded to f32 first (blur.rs:97-113)
-    double a = hf[0] * (double)c[0], b = hf[0] * (double)(c[0] * c[0]);
-#pragma unroll
-    for (int t = 1; t <= 4; t++) {
-      a = fma(hf[t], (double)c[-t] + (double)c[t], a);
-      b = fma(hf[t], (double)(c[-t] * c[-t]) + (double)(c[t] * c[t]), b);
-    }
-    hb[0][ly][lx] = (float)a;
-    hb[1][ly][lx] = (float)b;
-  }
-  __syncthreads();
-  for (int e = threadIdx.x; e < kTH * kTW; e += 256) {
-    int ly = e / kTW, lx = e % kTW, gx = x0 + lx, gy = y0 + ly;
-    if (gx >= w || gy >= h) continue;
-    mu[(size_t)gy * w + gx] = fir9(&hb[0][ly + kHalo][lx], kTW, hf);
-    s11[(size_t)gy * w + gx] = fir9(&hb[1][ly + kHalo][lx], kTW, hf);
-  }
-}
-
-__global__ __launch_bounds__(256) void k_blur_moments(int w, int h, const ConstTables* __restrict__ tab,
-                                                      const float* __restrict__ x, float* __restrict__ mu,
-                                                      float* __restrict__ s11) {
-  blur_moments_tile(w, h, tab, x, mu, s11, blockIdx.x * kTW, blockIdx.y * kTH);
-}
-
-// Every scale and channel of a reference pyramid in one launch: workgroup
-// (tile of the flattened per-scale tile ranges, channel blockIdx.y).
-struct MomentPlanes {
-  int ns, sw[kMaxPyr], sh[kMaxPyr], off[kMaxPyr + 1];
-  const float* x[kMaxPyr][3];
-  float* mu[kMaxPyr][3];
-  float* s11[kMaxPyr][3];
+struct LineShape {
+  int G;       // lanes per line: a power of two <= 64
+  int lpw;     // lines per wave (64 / G)
+  int blocks;  // blocks per line (more than one only with G == 64)
 };
-__global__ __launch_bounds__(256) void k_blur_moments_all(const ConstTables* __restrict__ tab, MomentPlanes A) {
-  int s = 0;
-  while (s + 1 < A.ns && (int)blockIdx.x >= A.off[s + 1]) s++;
-  const int lt = blockIdx.x - A.off[s], tx = (A.sw[s] + kTW - 1) / kTW, c = blockIdx.y;
-  blur_moments_tile(A.sw[s], A.sh[s], tab, A.x[s][c], A.mu[s][c], A.s11[s][c], (lt % tx) * kTW, (lt / tx) * kTH);
+
+__host__ __device__ inline LineShape line_shape(int L) {
+  const int need = (L + 4 + kSeg - 1) / kSeg;  // segments of positions 0 .. L + 3
+  LineShape s;
+  if (need <= 64) {
+    int G = 2;
+    while (G < need) G <<= 1;
+    s.G = G;
+    s.lpw = 64 / G;
+    s.blocks = 1;
+  } else {
+    s.G = 64;
+    s.lpw = 1;
+    s.blocks = 1 + (L + 4 - 640 + kBlockStep - 1) / kBlockStep;
+  }
+  return s;
+}
+
+struct LineTaps {
+  double fir[5];
+};
+
+__device__ __forceinline__ LineTaps line_taps(const ConstTables* __restrict__ tab) {
+  LineTaps T;
+#pragma unroll
+  for (int t = 0; t < 5; t++) T.fir[t] = tab->fir[t];
+  return T;
+}
+
+// Lane i receives lane i-1's v; lane 0 receives 0.
+__device__ __forceinline__ float from_prev_lane(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v), __float_as_int(v), 0x138, 0xf, 0xf, true));
+}
+
+// A DPP move of an f64 value.  Row shifts (BC): lanes without a source
+// receive 0.  Row broadcasts: rows outside ROWS receive 0.
+template <int CTRL, int ROWS, bool BC>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const int rlo = __builtin_amdgcn_update_dpp(BC ? lo : 0, lo, CTRL, ROWS, 0xf, BC);
+  const int rhi = __builtin_amdgcn_update_dpp(BC ? hi : 0, hi, CTRL, ROWS, 0xf, BC);
+  return __hiloint2double(rhi, rlo);
+}
+
+template <int CTRL, int ROWS, bool BC>
+__device__ __forceinline__ void prefix_step(double (&v)[kSeg]) {
+  double t[kSeg];
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) t[j] = dpp_f64<CTRL, ROWS, BC>(v[j]);
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) v[j] += t[j];
+}
+
+template <int CTRL, int D>
+__device__ __forceinline__ void prefix_step_masked(double (&v)[kSeg], int gl) {
+  double t[kSeg];
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) t[j] = dpp_f64<CTRL, 0xf, true>(v[j]);
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) v[j] += gl >= D ? t[j] : 0.0;
+}
+
+// Inclusive prefix sums of the ten values over aligned groups of G lanes.
+// G >= 16: row shifts by 1, 2, 4, 8 within each row of 16 lanes, then row
+// 0's last lane into row 1 and row 2's into row 3, then lane 31 into rows 2
+// and 3.  G < 16: row shifts masked at the group's start.
+template <int G>
+__device__ __forceinline__ void group_prefix(double (&v)[kSeg], int gl) {
+  if constexpr (G >= 16) {
+    prefix_step<0x111, 0xf, true>(v);
+    prefix_step<0x112, 0xf, true>(v);
+    prefix_step<0x114, 0xf, true>(v);
+    prefix_step<0x118, 0xf, true>(v);
+    if constexpr (G >= 32) prefix_step<0x142, 0xa, false>(v);
+    if constexpr (G == 64) prefix_step<0x143, 0xc, false>(v);
+  } else {
+    prefix_step_masked<0x111, 1>(v, gl);
+    if constexpr (G > 2) prefix_step_masked<0x112, 2>(v, gl);
+    if constexpr (G > 4) prefix_step_masked<0x114, 4>(v, gl);
+  }
+}
+
+// Lane `src`'s v, in every lane.
+__device__ __forceinline__ double lane_value(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+
+// Barrier of the workgroup that orders its LDS accesses only (global stores
+// stay in flight).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+// The M moment values of an input pair (the reference's x1, x1^2 or the
+// candidate's x2, x2^2, x1 x2; products rounded to f32 first, blur.rs:97-113).
+template <int M>
+__device__ __forceinline__ float moment_value(int m, float a, float b) {
+  if constexpr (M == 2) return m == 0 ? a : a * a;
+  else return m == 0 ? b : (m == 1 ? b * b : a * b);
+}
+
+// One block of one line and moment: from the lane's samples u[j] at
+// positions P + j (P = block start + 10 gl) the recursion's values Y[j] at the
+// same positions, before their f32 rounding.  gl: lane within the line's group
+// of G lanes; the first `warm` lanes only seed their neighbours; C[j]: Y at
+// P0 - 10 + j for the first lane P0 past them (zero at a line's start).
+template <int G, bool MULTI>
+__device__ __forceinline__ void seg_block(const float (&u)[kSeg], int gl, int warm, const double (&C)[kSeg],
+                                          const LineTaps& T, double (&Y)[kSeg]) {
+  // a group start receives zeros (the line starts there); with G == 64 the
+  // only group start is lane 0, which the shift already fills with 0
+  const bool zero_in = G < 64 && gl == 0;
+  float s[kSeg];
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) {
+    const float up = from_prev_lane(u[j]);  // u[P + j - 10]
+    s[j] = (zero_in ? 0.0f : up) + u[j];
+  }
+  // sd[i + 9]: s at P + i for i in [-8, 9] (the previous lane's s[2..9] below P)
+  double sd[2 * kSeg - 1];
+  sd[0] = 0.0;
+#pragma unroll
+  for (int j = 2; j < kSeg; j++) {
+    const float sp = from_prev_lane(s[j]);
+    sd[j - 1] = (double)(zero_in ? 0.0f : sp);
+  }
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) sd[9 + j] = (double)s[j];
+  // sigma = (-1)^gl as a sign bit on the high word
+  const int sign = (gl & 1) << 31;
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) {
+    const double* c = &sd[9 + j - 4];
+    double psi = T.fir[0] * c[0];
+#pragma unroll
+    for (int t = 1; t <= 4; t++) psi = fma(T.fir[t], c[-t] + c[t], psi);
+    Y[j] = __hiloint2double(__double2hiint(psi) ^ sign, __double2loint(psi));
+  }
+  if constexpr (MULTI) {
+    // continued blocks: the leading lanes only seed their neighbours, and the
+    // chain starts from the carry at the first lane past them
+#pragma unroll
+    for (int j = 0; j < kSeg; j++) Y[j] = gl < warm ? 0.0 : (gl == warm ? Y[j] - C[j] : Y[j]);
+  }
+  group_prefix<G>(Y, gl);
+#pragma unroll
+  for (int j = 0; j < kSeg; j++) Y[j] = __hiloint2double(__double2hiint(Y[j]) ^ sign, __double2loint(Y[j]));
+}
+
+// Horizontal pass: workgroup (set, band of R = 1024 / G rows), each wave
+// 64 / G rows.  The outputs (and the first moment's source plane) go out
+// transposed.
+template <int CAND, int G, bool MULTI>
+__device__ __forceinline__ void blur_h_band(SetSpace& S, int band, const LineTaps& T, float* tile) {
+  constexpr int M = CAND ? 3 : 2;
+  constexpr int LPW = 64 / G, R = 1024 / G, STRIDE = kSeg * G + 1;
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int gl = lane & (G - 1), grp = lane / G;
+  const int w = S.w, h = S.h;
+  const int blocks = MULTI ? line_shape(w).blocks : 1;
+  const int y0 = band * R;
+  const int rl = wv * LPW + grp;  // the group's row in the band
+  // rows past the plane repeat its last row; their outputs are not stored
+  const int y = min(y0 + rl, h - 1);
+  const float* r1 = S.x1 + (size_t)y * w;
+  const float* r2 = CAND ? S.x2 + (size_t)y * w : r1;
+  double C[M][kSeg];
+#pragma unroll
+  for (int m = 0; m < M; m++)
+#pragma unroll
+    for (int j = 0; j < kSeg; j++) C[m][j] = 0.0;
+  // the lane's samples of block b: positions 620 b + 10 gl + j
+  auto fetch = [&](int b, float (&a)[kSeg], float (&bv)[kSeg]) {
+#pragma unroll
+    for (int j = 0; j < kSeg; j++) {
+      const int p = kBlockStep * b + kSeg * gl + j;
+      const bool in = p < w;
+      const uint32_t pc = min(p, w - 1);
+      a[j] = in ? ldg32(r1, pc) : 0.0f;
+      bv[j] = in ? ldg32(r2, pc) : 0.0f;
+    }
+  };
+  float a[kSeg], bv[kSeg];
+  fetch(0, a, bv);
+  for (int b = 0; b < blocks; b++) {
+    const int pb = kBlockStep * b, warm = b ? 2 : 0;
+    float na[kSeg], nbv[kSeg];
+    if (MULTI && b + 1 < blocks) fetch(b + 1, na, nbv);
+#pragma unroll
+    for (int m = 0; m <= M; m++) {
+      float out[kSeg];
+      if (m < M) {
+        float u[kSeg];
+#pragma unroll
+        for (int j = 0; j < kSeg; j++) u[j] = moment_value<M>(m, a[j], bv[j]);
+        double Y[kSeg];
+        seg_block<G, MULTI>(u, gl, warm, C[m], T, Y);
+#pragma unroll
+        for (int j = 0; j < kSeg; j++) out[j] = (float)Y[j];
+        if constexpr (MULTI)
+#pragma unroll
+          for (int j = 0; j < kSeg; j++) C[m][j] = lane_value(Y[j], 63);
+      } else {
+#pragma unroll
+        for (int j = 0; j < kSeg; j++) out[j] = CAND ? bv[j] : a[j];  // the source plane
+      }
+#pragma unroll
+      for (int j = 0; j < kSeg; j++) tile[rl * STRIDE + kSeg * gl + j] = out[j];
+      lds_barrier();
+      // tile column col holds position pb + col: output n = pb + col - 4, or
+      // the source sample n = pb + col; R consecutive rows per column
+      float* dst = m < M ? S.hT[m] : S.xT;
+      const int nb = m < M ? pb - 4 : pb;
+#pragma unroll
+      for (int it = 0; it < kSeg; it++) {
+        const int e = it * 1024 + (int)threadIdx.x;
+        const int col = e / R, row = e % R;
+        const int n = nb + col, yy = y0 + row;
+        if (col >= kSeg * warm && n >= 0 && n < w && yy < h)
+          stg32(dst, (uint32_t)n * (uint32_t)h + (uint32_t)yy, tile[row * STRIDE + col]);
+      }
+      lds_barrier();
+    }
+    if (MULTI && b + 1 < blocks)
+#pragma unroll
+      for (int j = 0; j < kSeg; j++) {
+        a[j] = na[j];
+        bv[j] = nbv[j];
+      }
+  }
+}
+
+template <int CAND>
+__global__ __launch_bounds__(1024) void k_blur_h(const ConstTables* __restrict__ tab, const LineSet* __restrict__ sets,
+                                                 const int* __restrict__ off, int nsets) {
+  __shared__ float tile[kTile];
+  const int k = find_desc(off, nsets, blockIdx.x);
+  SetSpace& S = ((SetSpace*)sets)[k];
+  const int band = blockIdx.x - off[k];
+  const LineTaps T = line_taps(tab);
+  const LineShape sh = line_shape(S.w);
+  switch (sh.G) {
+    case 64:
+      if (sh.blocks > 1) blur_h_band<CAND, 64, true>(S, band, T, tile);
+      else blur_h_band<CAND, 64, false>(S, band, T, tile);
+      break;
+    case 32: blur_h_band<CAND, 32, false>(S, band, T, tile); break;
+    case 16: blur_h_band<CAND, 16, false>(S, band, T, tile); break;
+    case 8: blur_h_band<CAND, 8, false>(S, band, T, tile); break;
+    case 4: blur_h_band<CAND, 4, false>(S, band, T, tile); break;
+    default: blur_h_band<CAND, 2, false>(S, band, T, tile); break;
+  }
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -487,343 +683,122 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// The candidate pass of one scale and channel over a 64x24 tile: the three
-// candidate blurs (mu2 = G*x2, s22 = G*x2^2, s12 = G*x1x2), the SSIM and
-// edge maps, and the tile's f64 partial sums (pool.rs:97-278).  Each lane
-// slides its 9-tap window over a run of outputs held in registers, so every
-// input sample is widened to f64 once per run instead of once per tap; the
-// taps, their pairing and the one f32 rounding per pass are those of fir9.
-constexpr int kPW = 64, kPH = 24;
-constexpr int kPIW = kPW + 2 * kHalo, kPIH = kPH + 2 * kHalo;  // 72 x 32
-constexpr int kHRun = 8;                                      // horizontal outputs per lane
-__host__ __device__ constexpr int pool_tiles_dev(int w, int h) { return ((w + kPW - 1) / kPW) * ((h + kPH - 1) / kPH); }
-// Streaming pass: 64-column strips cut into segments of kSH output rows.
-constexpr int kSH = 64;
-__host__ __device__ constexpr int pool_units_dev(int w, int h) { return ((w + kPW - 1) / kPW) * ((h + kSH - 1) / kSH); }
-constexpr int kVRun = 6;                                      // vertical outputs per lane
-
-template <int R>
-__device__ __forceinline__ void fir_run(const double (&d)[R + 8], const double* hf, double (&o)[R]) {
-#pragma unroll
-  for (int i = 0; i < R; i++) {
-    double acc = hf[0] * d[i + 4];
-#pragma unroll
-    for (int t = 1; t <= 4; t++) acc = fma(hf[t], d[i + 4 - t] + d[i + 4 + t], acc);
-    o[i] = acc;
-  }
-}
-
-// One 64x24 tile of channel c at one scale; writes the tile's 6 partial
-// sums of that channel to out6[0..5].
-__device__ __forceinline__ void blur_pool_tile(const ConstTables* __restrict__ tab, int w, int h, int x0, int y0,
-                                               const float* __restrict__ X1, const float* __restrict__ MU1,
-                                               const float* __restrict__ S11, const float* __restrict__ X2,
-                                               double* __restrict__ out6) {
-  __shared__ float t1[kPIH][kPIW];
-  __shared__ float t2[kPIH][kPIW];
-  __shared__ float hb[kPIH][kPW];
-  __shared__ double red[4][6];
-  double hf[5];
-#pragma unroll
-  for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
-  // The pooling inputs do not depend on the blurs: issue their loads first.
-  const int plx = threadIdx.x & 63, ply0 = (threadIdx.x >> 6) * kVRun;
-  float pmu[kVRun], ps11[kVRun];
-#pragma unroll
-  for (int i = 0; i < kVRun; i++) {
-    const int gx = x0 + plx, gy = y0 + ply0 + i;
-    const bool in = gx < w && gy < h;
-    pmu[i] = in ? MU1[(size_t)gy * w + gx] : 0.0f;
-    ps11[i] = in ? S11[(size_t)gy * w + gx] : 0.0f;
-  }
-  for (int e = threadIdx.x; e < kPIH * kPIW; e += 256) {
-    const int ly = e / kPIW, lx = e % kPIW, gx = x0 + lx - kHalo, gy = y0 + ly - kHalo;
-    const bool in = gx >= 0 && gx < w && gy >= 0 && gy < h;
-    const size_t o = (size_t)gy * w + gx;
-    t1[ly][lx] = in ? X1[o] : 0.0f;
-    t2[ly][lx] = in ? X2[o] : 0.0f;
-  }
-  __syncthreads();
-  // The three moments one after another through one LDS plane (a smaller
-  // footprint keeps more workgroups resident): horizontal 32 rows x 8 runs
-  // of 8 outputs, then vertical 64 columns x 4 runs of 6 rows (wave k takes
-  // rows 6k..6k+5).
-  const int hly = threadIdx.x >> 3, hlx0 = (threadIdx.x & 7) * kHRun;
-  const int lx = threadIdx.x & 63, ly0 = (threadIdx.x >> 6) * kVRun;
-  const int gx = x0 + lx;
-  float vm[3][kVRun];
-#pragma unroll
-  for (int m = 0; m < 3; m++) {
-    {
-      double d[kHRun + 8], o[kHRun];
-#pragma unroll
-      for (int i = 0; i < kHRun + 8; i++) {
-        const float a = t1[hly][hlx0 + i], b = t2[hly][hlx0 + i];
-        // the squared and product inputs are rounded to f32 first (blur.rs:97-113)
-        d[i] = m == 0 ? (double)b : (m == 1 ? (double)(b * b) : (double)(a * b));
-      }
-      fir_run<kHRun>(d, hf, o);
-#pragma unroll
-      for (int i = 0; i < kHRun; i++) hb[hly][hlx0 + i] = (float)o[i];
-    }
-    __syncthreads();
-    {
-      double d[kVRun + 8], o[kVRun];
-#pragma unroll
-      for (int i = 0; i < kVRun + 8; i++) d[i] = (double)hb[ly0 + i][lx];
-      fir_run<kVRun>(d, hf, o);
-#pragma unroll
-      for (int i = 0; i < kVRun; i++) vm[m][i] = (float)o[i];
-    }
-    __syncthreads();
-  }
-  float* mu2 = vm[0];
-  float* s22 = vm[1];
-  float* s12 = vm[2];
-  const float C2 = 0.0009f;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-#pragma unroll
-  for (int i = 0; i < kVRun; i++) {
-    const int gy = y0 + ly0 + i;
-    if (gx >= w || gy >= h) continue;
-    const float i1 = t1[ly0 + i + kHalo][lx + kHalo], i2 = t2[ly0 + i + kHalo][lx + kHalo];
-    const float mu1 = pmu[i], s11 = ps11[i];
-    const float mu11 = mu1 * mu1, mu22 = mu2[i] * mu2[i], mu12 = mu1 * mu2[i];
-    const float md = mu1 - mu2[i];
-    const float num_m = 1.0f - md * md;
-    const float num_s = 2.0f * (s12[i] - mu12) + C2;
-    const float den_s = (s11 - mu11) + (s22[i] - mu22) + C2;
-    double d = 1.0 - (double)((num_m * num_s) / den_s);
-    d = d > 0.0 ? d : 0.0;
-    acc[0] += d;
-    acc[1] += d * d * d * d;
-    double d1 = (1.0 + (double)fabsf(i2 - mu2[i])) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
-    double art = d1 > 0.0 ? d1 : 0.0;
-    double lost = -d1 > 0.0 ? -d1 : 0.0;
-    acc[2] += art;
-    acc[3] += art * art * art * art;
-    acc[4] += lost;
-    acc[5] += lost * lost * lost * lost;
-  }
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-#pragma unroll
-  for (int k = 0; k < 6; k++) {
-    double v = wave_sum(acc[k]);
-    if (lane == 0) red[wave][k] = v;
-  }
-  __syncthreads();
-  if (threadIdx.x < 6) {
-    double v = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
-    out6[threadIdx.x] = v;
-  }
-}
-
-__global__ __launch_bounds__(256) void k_blur_pool(const ConstTables* __restrict__ tab, PoolScales P,
-                                                   double* __restrict__ partials) {
-  const int c = blockIdx.z;
-  const int tile = blockIdx.x;
-  int sc = 0;
-#pragma unroll
-  for (int i = 1; i < kMaxPoolScales; i++)
-    if (i < P.L.nscales && tile >= P.L.off[i]) sc = i;
-  const int lt = tile - P.L.off[sc];
-  blur_pool_tile(tab, P.w[sc], P.h[sc], (lt % P.tx[sc]) * kPW, (lt / P.tx[sc]) * kPH, P.x1[sc][c], P.mu1[sc][c],
-                 P.s11[sc][c], P.x2[sc][c], partials + (size_t)tile * 18 + c * 6);
-}
-
-// ---------------------------------------------------------------------------
-// Streaming form of the candidate pass: one wave walks a 64-column strip of
-// one channel down a segment of kSH output rows.  Each input row is loaded
-// once (plus a 4-column halo either side), widened to f64 once per sample
-// into the wave's LDS row, filtered horizontally (one output column per
-// lane) and pushed into a 9-row ring held in the lane's registers; the
-// vertical filter of the row 4 behind runs from that ring, so the vertical
-// halo costs 8 rows per segment instead of a recomputed band per tile.  The
-// taps, pairing and roundings are those of blur_pool_tile, so the sums are
-// the same per pixel.  The row loop is unrolled by the ring size so every
-// ring index is a register.
-// ---------------------------------------------------------------------------
-struct StreamRing {
-  double m[9][3];  // horizontally filtered moments (mu2, s22, s12) per ring row, f32 values
-};
-
-// The global samples one step of the walk reads: input row yi (a: column
-// gx-4; b: the right halo on lanes 0..7) and, at column gx of output row
-// yo = yi - 4, the reference's x1, mu1, s11 and the candidate's x2.
-struct StreamIn {
-  float a1, a2, b1, b2, mu1, s11, i1, i2;
-};
-
-__device__ __forceinline__ StreamIn stream_load(int t, int y0, int y1, int w, int h, int gx, int lane,
-                                                const float* __restrict__ X1, const float* __restrict__ X2,
-                                                const float* __restrict__ MU1,
-                                                const float* __restrict__ S11) {
-  const int yi = y0 - 4 + t, yo = yi - 4;
-  const int xa = gx - 4, xb = gx + 60;
-  // every address clamped into the plane, so the eight loads issue together
-  // without branches; samples outside are then zeroed
-  // (row offsets are wave-uniform: the row pointers stay in scalar registers
-  // and each lane adds its fixed column)
-  const size_t ri = (size_t)min(max(yi, 0), h - 1) * w, ro = (size_t)min(max(yo, 0), h - 1) * w;
-  const uint32_t ca = min(max(xa, 0), w - 1), cbx = min(xb, w - 1), cg = min(gx, w - 1);
-  StreamIn v;
-  v.a1 = ldg32(X1 + ri, ca);
-  v.a2 = ldg32(X2 + ri, ca);
-  v.b1 = ldg32(X1 + ri, cbx);
-  v.b2 = ldg32(X2 + ri, cbx);
-  v.mu1 = ldg32(MU1 + ro, cg);
-  v.s11 = ldg32(S11 + ro, cg);
-  v.i1 = ldg32(X1 + ro, cg);
-  v.i2 = ldg32(X2 + ro, cg);
-  return v;
-}
-
-template <int K>
-__device__ __forceinline__ void stream_row(int t, int y1, int y0, int w, int h, int gx, int lane, const StreamIn& in,
-                                           const double* hf, double (*row)[72], StreamRing& R,
-                                           double* acc) {
-  const int yi = y0 - 4 + t, yo = yi - 4;
-  const bool out = t >= 8 && yo < y1 && gx < w;
-  // the loads read clamped addresses; samples outside the plane count as 0
-  const bool rin = yi >= 0 && yi < h;
-  const int xa = gx - 4, xb = gx + 60;
-  const bool ina = rin && xa >= 0 && xa < w;
-  const bool inb = rin && xb < w;
-  const float a1 = ina ? in.a1 : 0.0f, a2 = ina ? in.a2 : 0.0f;
-  // widened samples into the wave's LDS row: x2, x2^2, x1*x2 (the squared and
-  // product inputs rounded to f32 first, blur.rs:97-113)
-  row[0][lane] = (double)a2;
-  row[1][lane] = (double)(a2 * a2);
-  row[2][lane] = (double)(a1 * a2);
-  if (lane < 8) {
-    const float b1 = inb ? in.b1 : 0.0f, b2 = inb ? in.b2 : 0.0f;
-    row[0][64 + lane] = (double)b2;
-    row[1][64 + lane] = (double)(b2 * b2);
-    row[2][64 + lane] = (double)(b1 * b2);
-  }
-  // the row's writes by every lane before any lane reads its neighbours
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-#pragma unroll
-  for (int m = 0; m < 3; m++) {
-    const double* v = row[m] + lane + 4;  // centre of this lane's window
-    double acc_h = hf[0] * v[0];
-#pragma unroll
-    for (int k = 1; k <= 4; k++) acc_h = fma(hf[k], v[-k] + v[k], acc_h);
-    R.m[K][m] = (double)(float)acc_h;
-  }
-  // every lane's reads of this row before the next row overwrites it
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (!out) return;
-  // vertical filter over ring rows yo-4 .. yo+4: slots (K+1) .. K (mod 9)
-  float vm[3];
-#pragma unroll
-  for (int m = 0; m < 3; m++) {
-    double d[9];
-#pragma unroll
-    for (int k = 0; k < 9; k++) d[k] = R.m[(K + 1 + k) % 9][m];
-    double a = hf[0] * d[4];
-#pragma unroll
-    for (int k = 1; k <= 4; k++) a = fma(hf[k], d[4 - k] + d[4 + k], a);
-    vm[m] = (float)a;
-  }
-  const float mu1 = in.mu1, s11 = in.s11, i1 = in.i1, i2 = in.i2;
-  const float mu2 = vm[0], s22 = vm[1], s12 = vm[2];
+// The SSIM and edge terms of one pixel (pool.rs:97-278) into acc.
+__device__ __forceinline__ void pool_pixel(float mu1, float s11, float i1, float i2, float mu2, float s22, float s12,
+                                           double (&acc)[6]) {
   const float C2 = 0.0009f;
   const float mu11 = mu1 * mu1, mu22 = mu2 * mu2, mu12 = mu1 * mu2;
   const float md = mu1 - mu2;
   const float num_m = 1.0f - md * md;
   const float num_s = 2.0f * (s12 - mu12) + C2;
   const float den_s = (s11 - mu11) + (s22 - mu22) + C2;
-  double dd = 1.0 - (double)((num_m * num_s) / den_s);
-  dd = dd > 0.0 ? dd : 0.0;
-  acc[0] += dd;
-  acc[1] += dd * dd * dd * dd;
-  double d1 = (1.0 + (double)fabsf(i2 - mu2)) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
-  double art = d1 > 0.0 ? d1 : 0.0;
-  double lost = -d1 > 0.0 ? -d1 : 0.0;
+  double d = 1.0 - (double)((num_m * num_s) / den_s);
+  d = d > 0.0 ? d : 0.0;
+  acc[0] += d;
+  acc[1] += d * d * d * d;
+  const double d1 = (1.0 + (double)fabsf(i2 - mu2)) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
+  const double art = d1 > 0.0 ? d1 : 0.0;
+  const double lost = -d1 > 0.0 ? -d1 : 0.0;
   acc[2] += art;
   acc[3] += art * art * art * art;
   acc[4] += lost;
   acc[5] += lost * lost * lost * lost;
 }
 
-// One wave: strip x0 .. x0+63 of channel planes, output rows y0 .. y1-1; the
-// 6 sums of this (strip, segment, channel) go to out6 (lane 0).  The global
-// samples of step t+1 are requested before step t is computed.
-__device__ void stream_unit(const ConstTables* __restrict__ tab, int w, int h, int x0, int y0,
-                            const float* __restrict__ X1, const float* __restrict__ MU1,
-                            const float* __restrict__ S11, const float* __restrict__ X2,
-                            double (*row)[72], double* __restrict__ out6) {
+// Vertical pass: one wave per (set, 64 / G columns), blockDim.x / 64 waves
+// per workgroup.  With `swizzle`, the workgroups the dispatcher deals
+// round-robin to the eight XCDs take contiguous runs of units.  The launch
+// shape changes no output.
+template <int CAND, int G, bool MULTI>
+__device__ __forceinline__ void blur_v_unit(SetSpace& S, int ul, const LineTaps& T) {
+  constexpr int M = CAND ? 3 : 2;
+  constexpr int LPW = 64 / G;
   const int lane = threadIdx.x & 63;
-  const int gx = x0 + lane;
-  const int y1 = min(y0 + kSH, h);
-  double hf[5];
+  const int gl = lane & (G - 1), grp = lane / G;
+  const int w = S.w, h = S.h;
+  const int blocks = MULTI ? line_shape(h).blocks : 1;
+  const int xr = ul * LPW + grp;
+  const bool line_ok = xr < w;
+  const size_t base = (size_t)min(xr, w - 1) * h;
+  double C[M][kSeg];
 #pragma unroll
-  for (int t = 0; t < 5; t++) hf[t] = tab->fir[t];
-  StreamRing R;
+  for (int m = 0; m < M; m++)
 #pragma unroll
-  for (int k = 0; k < 9; k++) R.m[k][0] = R.m[k][1] = R.m[k][2] = 0.0;
+    for (int j = 0; j < kSeg; j++) C[m][j] = 0.0;
   double acc[6] = {0, 0, 0, 0, 0, 0};
-  const int rows = (y1 - y0) + 8;  // input rows y0-4 .. y1+3 (rows >= 9)
-  // three load buffers: step K computes from buf[K % 3] and requests step
-  // K + 2 into buf[(K + 2) % 3] (the unroll of 9 keeps every index constant)
-  StreamIn buf[3];
-  buf[0] = stream_load(0, y0, y1, w, h, gx, lane, X1, X2, MU1, S11);
-  buf[1] = stream_load(1, y0, y1, w, h, gx, lane, X1, X2, MU1, S11);
-  for (int t = 0; t < rows; t += 9) {
-// step t + K writes ring slot K (compile-time over the unroll of 9)
-#define JPXLG_STEP(K)                                                                  \
-    if (t + K < rows) {                                                                \
-      buf[(K + 2) % 3] = stream_load(t + K + 2, y0, y1, w, h, gx, lane, X1, X2, MU1, S11); \
-      stream_row<K>(t + K, y1, y0, w, h, gx, lane, buf[K % 3], hf, row, R, acc);                \
-    }
-    JPXLG_STEP(0) JPXLG_STEP(1) JPXLG_STEP(2) JPXLG_STEP(3) JPXLG_STEP(4)
-    JPXLG_STEP(5) JPXLG_STEP(6) JPXLG_STEP(7) JPXLG_STEP(8)
-#undef JPXLG_STEP
-  }
+  for (int b = 0; b < blocks; b++) {
+    const int pb = kBlockStep * b, warm = b ? 2 : 0;
+    float y[M][kSeg];
 #pragma unroll
-  for (int k = 0; k < 6; k++) {
-    const double v = wave_sum(acc[k]);
-    if (lane == 0) out6[k] = v;
+    for (int m = 0; m < M; m++) {
+      const float* in = S.hT[m] + base;
+      float u[kSeg];
+#pragma unroll
+      for (int j = 0; j < kSeg; j++) {
+        const int p = pb + kSeg * gl + j;
+        u[j] = p < h ? ldg32(in, (uint32_t)min(p, h - 1)) : 0.0f;
+      }
+      double Y[kSeg];
+      seg_block<G, MULTI>(u, gl, warm, C[m], T, Y);
+#pragma unroll
+      for (int j = 0; j < kSeg; j++) y[m][j] = (float)Y[j];
+      if constexpr (MULTI)
+#pragma unroll
+        for (int j = 0; j < kSeg; j++) C[m][j] = lane_value(Y[j], 63);
+    }
+    // outputs n = P + j - 4 of the lane's own positions
+    const bool own = line_ok && gl >= warm;
+#pragma unroll
+    for (int j = 0; j < kSeg; j++) {
+      const int n = pb + kSeg * gl + j - 4;
+      if (!own || n < 0 || n >= h) continue;
+      if constexpr (CAND != 0) {
+        const uint32_t o = (uint32_t)n;
+        pool_pixel(ldg32(S.mu1T + base, o), ldg32(S.s11T + base, o), ldg32(S.x1T + base, o),
+                   ldg32(S.x2T + base, o), y[0][j], y[1][j], y[2][j], acc);
+      } else {
+        stg32(S.out[0] + base, (uint32_t)n, y[0][j]);
+        stg32(S.out[1] + base, (uint32_t)n, y[1][j]);
+      }
+    }
+  }
+  if constexpr (CAND != 0) {
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      const double v = wave_sum(acc[t]);
+      if (lane == 0) S.partials[(size_t)ul * 18 + t] = v;
+    }
   }
 }
 
-// Batched streaming launch: blockDim.x / 64 (tile, channel) units per
-// workgroup, one per wave; units enumerate (desc, scale) through off[] in
-// units (tiles * 3).  With `swizzle`, the workgroups the dispatcher deals
-// round-robin to the eight XCDs take contiguous runs of units, so the strips
-// that share halo rows and columns meet in one XCD's L2.  The launch shape
-// changes no sum.
-__global__ __launch_bounds__(256) void k_blur_pool_stream(const ConstTables* __restrict__ tab,
-                                                          const ProbeDesc* __restrict__ D,
-                                                          const int* __restrict__ off, int nds, int total,
-                                                          int swizzle) {
-  __shared__ double rows[4][3][72];
-  // the wave index is wave-uniform: say so, so that everything derived from
-  // it (unit, plane pointers, row offsets) lives in scalar registers
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), upw = blockDim.x >> 6;
+template <int CAND>
+__global__ __launch_bounds__(256) void k_blur_v(const ConstTables* __restrict__ tab, const LineSet* __restrict__ sets,
+                                                const int* __restrict__ off, int nsets, int total, int swizzle) {
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), upw = blockDim.x >> 6;
   const int nb = gridDim.x;
-  int b = blockIdx.x;
+  int bi = blockIdx.x;
   if (swizzle && nb >= 64) {
-    // workgroup b runs on XCD b % 8: give that XCD the b / 8-th block of its
-    // contiguous share (the last nb % 8 workgroups keep their own index)
     const int per = nb / 8;
-    if (b < per * 8) b = (b % 8) * per + b / 8;
+    if (bi < per * 8) bi = (bi % 8) * per + bi / 8;
   }
-  const int unit = b * upw + wave;
+  const int unit = bi * upw + wv;
   if (unit >= total) return;
-  const int ds = find_desc(off, nds, unit);
-  const int d = ds / kDescScales, sc = ds % kDescScales;
-  DescSpace& P = desc_at(D, d);
-  const int w = P.sw[sc], h = P.sh[sc], tx = (w + kPW - 1) / kPW;
-  const int tiles = P.punits[sc];
-  const int lu = unit - off[ds], c = lu / tiles, lt = lu % tiles;
-  stream_unit(tab, w, h, (lt % tx) * kPW, (lt / tx) * kSH, P.ref_x[sc][c], P.ref_mu[sc][c], P.ref_s11[sc][c],
-              P.cand_x[sc][c], rows[wave], P.partials + (size_t)(P.ptile_off[sc] + lt) * 18 + c * 6);
+  const int k = find_desc(off, nsets, unit);
+  SetSpace& S = ((SetSpace*)sets)[k];
+  const int ul = unit - off[k];
+  const LineTaps T = line_taps(tab);
+  const LineShape sh = line_shape(S.h);
+  switch (sh.G) {
+    case 64:
+      if (sh.blocks > 1) blur_v_unit<CAND, 64, true>(S, ul, T);
+      else blur_v_unit<CAND, 64, false>(S, ul, T);
+      break;
+    case 32: blur_v_unit<CAND, 32, false>(S, ul, T); break;
+    case 16: blur_v_unit<CAND, 16, false>(S, ul, T); break;
+    case 8: blur_v_unit<CAND, 8, false>(S, ul, T); break;
+    case 4: blur_v_unit<CAND, 4, false>(S, ul, T); break;
+    default: blur_v_unit<CAND, 2, false>(S, ul, T); break;
+  }
 }
 
 // Fixed-order reduction of every scale's per-tile partial sums in one launch:
@@ -877,14 +852,6 @@ void launch_colour_metric(hipStream_t s, int mode, int w, int h, float* const in
                                             pxyb[1], pxyb[2], d0, d1, d2);
 }
 
-int colour_blocks(int w, int h) { return (int)(div_up((w + 1) / 2, 256) * ((h + 1) / 2)); }
-
-void launch_colour_b(hipStream_t s, int scale, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
-  if (nwg <= 0) return;
-  if (scale == 0) k_colour_b<0><<<nwg, 256, 0, s>>>(D, off, nd, 0, src);
-  else k_colour_b<1><<<nwg, 256, 0, s>>>(D, off, nd, scale, src);
-}
-
 int pyramid_tiles(int w, int h) { return ((w + kPyrTile - 1) / kPyrTile) * ((h + kPyrTile - 1) / kPyrTile); }
 
 void launch_colour_pyr_b(hipStream_t s, const ProbeDesc* D, const int* off, int nd, int nwg, int src) {
@@ -912,56 +879,26 @@ void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], con
                                                    levels, out[0], out[1], out[2]);
 }
 
-void launch_blur_moments_all(hipStream_t s, const ConstTables* tab, int ns, const int* sw, const int* sh,
-                             float* const (*x)[3], float* const (*mu)[3], float* const (*s11)[3]) {
-  MomentPlanes A{};
-  A.ns = ns;
-  int total = 0;
-  for (int i = 0; i < ns; i++) {
-    A.sw[i] = sw[i];
-    A.sh[i] = sh[i];
-    A.off[i] = total;
-    total += (int)(div_up(sw[i], kTW) * div_up(sh[i], kTH));
-    for (int c = 0; c < 3; c++) {
-      A.x[i][c] = x[i][c];
-      A.mu[i][c] = mu[i][c];
-      A.s11[i][c] = s11[i][c];
-    }
-  }
-  A.off[ns] = total;
-  k_blur_moments_all<<<dim3(total, 3), 256, 0, s>>>(tab, A);
+int blur_bands(int w, int h) { return (int)div_up(h, kHWaves * line_shape(w).lpw); }
+int pool_units(int w, int h) { return (int)div_up(w, line_shape(h).lpw); }
+
+void launch_blur_h(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
+                   int bands) {
+  if (bands <= 0) return;
+  if (cand) k_blur_h<1><<<bands, 64 * kHWaves, 0, s>>>(tab, sets, off, nsets);
+  else k_blur_h<0><<<bands, 64 * kHWaves, 0, s>>>(tab, sets, off, nsets);
 }
 
-void launch_blur_moments(hipStream_t s, int w, int h, const ConstTables* tab, float* const x1[3],
-                         float* mu1[3], float* s11[3]) {
-  dim3 grid(div_up(w, kTW), div_up(h, kTH));
-  for (int c = 0; c < 3; c++) k_blur_moments<<<grid, 256, 0, s>>>(w, h, tab, x1[c], mu1[c], s11[c]);
-}
-
-int pool_tiles(int w, int h) { return pool_tiles_dev(w, h); }
-int pool_units(int w, int h) { return pool_units_dev(w, h); }
-
-void launch_blur_pool(hipStream_t s, const ConstTables* tab, PoolScales& P, double* partials) {
-  int total = 0;
-  for (int i = 0; i < P.L.nscales; i++) {
-    P.tx[i] = (int)div_up(P.w[i], kPW);
-    P.L.off[i] = total;
-    P.L.tiles[i] = pool_tiles(P.w[i], P.h[i]);
-    total += P.L.tiles[i];
-  }
-  dim3 grid(total, 1, 3);
-  k_blur_pool<<<grid, 256, 0, s>>>(tab, P, partials);
+void launch_blur_v(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
+                   int columns, int upw, int swizzle) {
+  if (columns <= 0) return;
+  upw = upw == 1 || upw == 2 ? upw : 4;
+  if (cand) k_blur_v<1><<<div_up(columns, upw), 64 * upw, 0, s>>>(tab, sets, off, nsets, columns, swizzle);
+  else k_blur_v<0><<<div_up(columns, upw), 64 * upw, 0, s>>>(tab, sets, off, nsets, columns, swizzle);
 }
 
 void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums) {
   k_reduce_scales<<<18 * L.nscales, 256, 0, s>>>(L, partials, sums);
-}
-
-void launch_blur_pool_stream(hipStream_t s, const ConstTables* tab, const ProbeDesc* D, const int* off, int nds,
-                             int units, int upw, int swizzle) {
-  if (units <= 0) return;
-  upw = upw == 1 || upw == 2 ? upw : 4;
-  k_blur_pool_stream<<<div_up(units, upw), 64 * upw, 0, s>>>(tab, D, off, nds, units, swizzle);
 }
 
 void launch_reduce_b(hipStream_t s, const ProbeDesc* D, int nd) {

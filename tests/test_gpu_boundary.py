@@ -10,7 +10,7 @@ import oracle_binding as orc
 
 pytestmark = pytest.mark.gpu
 
-SCORE_TOL = 0.02
+SCORE_TOL = 2e-4
 
 
 @pytest.fixture(scope="module")

@@ -2,9 +2,15 @@
 
 Integer and index outputs (quantized coefficients, LF planes) must be
 bit-identical; float planes of the pixel path are compared bit for bit as well
-(the device restates the reference's operation order exactly); SSIMULACRA2
-scores must agree within 0.02 points (the GPU blur is the FIR form of the
-reference's f64 recursion).
+(the device restates the reference's operation order exactly).  The GPU
+blur evaluates the reference's recursion (its f32 pair sums and their
+rounding included) exactly before the one f32 rounding of each output; the
+reference's f64 resonators round a few ulps of f64 away from that value, which
+turns about one output in 10^6 to the other f32 neighbour (the vertical pass
+then carries that input change down its column).  SSIMULACRA2 scores of the
+same planes therefore agree within PAIR_TOL; scores of rendered candidates
+within SCORE_TOL, which also covers the rare one-ulp differences of the
+device's positive-XYB cube root.
 """
 import numpy as np
 import pytest
@@ -13,15 +19,16 @@ import oracle_binding as orc
 
 pytestmark = pytest.mark.gpu
 
-SCORE_TOL = 0.02
+SCORE_TOL = 2e-4
+PAIR_TOL = 1e-7
 
 
-def _terms_close(terms, oterms):
-    # The FIR blur differs from the f64 recursion by ~1e-7 per sample; the
-    # 4-norms of near-zero artifact/detail-lost maps amplify that, so terms
-    # are compared with an absolute bound (the score bound is the contract).
+def _terms_close(terms, oterms, tol=1e-6):
+    # Terms of rendered candidates carry the cube root's rare one-ulp
+    # differences (amplified by the 4-norms of near-zero maps), hence an
+    # absolute bound.
     assert len(terms) == len(oterms)
-    assert np.abs(terms - oterms).max() < 2e-5
+    assert np.abs(terms - oterms).max() < tol
 
 
 @pytest.fixture(scope="module")
@@ -430,8 +437,8 @@ def test_score_pair_identity_and_noise(ctx):
         cand = np.clip(ref + rng.uniform(-amp, amp, ref.shape).astype(np.float32), 0, 1)
         s, terms = ctx.score_pair(ref, cand)
         os_, oterms, _ = orc.ssimulacra2(ref, cand)
-        assert abs(s - os_) < SCORE_TOL
-        _terms_close(terms, oterms)
+        assert abs(s - os_) < PAIR_TOL
+        _terms_close(terms, oterms, 1e-9)
 
 
 @pytest.mark.parametrize("wh", [(8, 8), (9, 17), (256, 192)])
@@ -442,7 +449,50 @@ def test_score_pair_scale_counts(ctx, wh):
     s, terms = ctx.score_pair(ref, cand)
     os_, oterms, _ = orc.ssimulacra2(ref, cand)
     assert len(terms) == len(oterms)
-    assert abs(s - os_) < SCORE_TOL
+    assert abs(s - os_) < PAIR_TOL
+
+
+@pytest.mark.parametrize("wh", [(1024, 768), (1531, 1027)])
+def test_score_pair_long_lines_match_the_recursion(ctx, wh):
+    # Long lines: the reference's recursion carries the rounding of every
+    # f32 pair sum from the start of the line (an exact 9-tap blur drifts
+    # from it by ~0.1 point at these sizes); the device carries the same.
+    # Terms: an absolute bound for the rare outputs the reference's own f64
+    # rounding sends to the other f32 neighbour.
+    w, h = wh
+    rng = np.random.default_rng(w)
+    yy, xx = np.mgrid[0:h, 0:w]
+    base = 0.5 + 0.3 * np.sin(xx / 37.0)[None] * np.cos(yy / 23.0)[None]
+    ref = np.clip(base + rng.uniform(-0.05, 0.05, (3, h, w)), 0, 1).astype(np.float32)
+    cand = np.clip(ref + rng.uniform(-0.02, 0.02, ref.shape).astype(np.float32), 0, 1)
+    s, terms = ctx.score_pair(ref, cand)
+    os_, oterms, _ = orc.ssimulacra2(ref, cand)
+    assert abs(s - os_) < PAIR_TOL
+    _terms_close(terms, oterms, 1e-7)
+
+
+@pytest.mark.parametrize("case", [(1024, 768, 16, 5, 59199), (2048, 2048, 16, 2, 66525)])
+def test_render_score_at_full_size(ctx, case):
+    # The probe chain's score of a rendered candidate at the encode sizes
+    # (2048 x 2048 RGB16 is the throughput workload) against the oracle's
+    # SSIMULACRA2 of the same linear planes.
+    import jpxl_amd
+    from jpxl_amd import Quantizer, Restoration, _lib
+
+    w, h, bits, seed, gs = case
+    rgb = _rgb(w, h, seed, bits)
+    img = jpxl_amd.Image(ctx, rgb, bits)
+    bmap = orc.fixed_dct8_map(w, h)
+    z = np.zeros(((bmap.shape[0] + 7) // 8, (bmap.shape[1] + 7) // 8), np.int8)
+    img.set_structure(bmap, z, z, (0, 0))
+    img.quantize(Quantizer(gs, 1, 4, 3, 3, 1, 4.0, 1.0))
+    score, _, terms = img.render_score(Restoration(1, 1, 7))
+    got_lin = img.download(_lib.BUF_LINEAR)
+    _, lin, _ = orc.prepare_frame(rgb, bits)
+    oscore, oterms, _ = orc.ssimulacra2(lin, got_lin)
+    assert abs(score - oscore) < SCORE_TOL
+    _terms_close(terms, oterms)
+    img.close()
 
 
 def test_encode_quality_meets_target(ctx):

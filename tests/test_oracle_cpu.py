@@ -351,6 +351,66 @@ def test_blur_equals_the_nine_tap_truncated_cosine_fir():
     assert np.abs(got - want).max() < 2e-6
 
 
+def _exact_split_lines(u, carry=True):
+    """The device's form of the blur recursion over the rows of u (f32):
+    y = f32(F + E) with F the 9-tap blur in f64 and E the alternating
+    stride-10 prefix of Phi, Phi the resonators' 10-tap response to the
+    rounding errors of the f32 pair sums (exact by TwoSum)."""
+    mul_in = [0.05529523572608661, -0.05883668702694998, 0.012955819110517063]
+    mul_prev = [1.902113032590307, 1.1755705045849463, 1.2246467991473532e-16]
+    g = []
+    p1, p2 = [0.0] * 3, [0.0] * 3
+    for r in range(10):
+        o = [(1.0 if r == 0 else 0.0) * mul_in[k] + mul_prev[k] * p1[k] - p2[k] for k in range(3)]
+        p2, p1 = p1, o
+        g.append(o[0] + o[1] + o[2])
+    g32 = np.array(g, np.float32)
+    n, L = u.shape
+    pad = np.zeros((n, L + 24), np.float32)
+    pad[:, 12:12 + L] = u
+    at = lambda i: pad[:, 12 + i]
+    eps = np.zeros((n, L + 4), np.float32)
+    for p in range(L + 4):
+        a, b = at(p - 10), at(p)
+        t = a + b
+        bv = t - a
+        eps[:, p] = -((a - (t - bv)) + (b - bv))
+    E = np.zeros((n, L))
+    out = np.zeros((n, L), np.float32)
+    for m in range(L):
+        phi = np.zeros(n, np.float32)
+        for r in range(9, -1, -1):
+            if m + 4 - r >= 0:
+                phi = (g32[r] * eps[:, m + 4 - r] + phi).astype(np.float32)
+        E[:, m] = phi.astype(np.float64) - (E[:, m - 10] if m >= 10 else 0.0)
+        F = g[4] * at(m).astype(np.float64)
+        for t in range(1, 5):
+            F = F + g[4 + t] * (at(m - t).astype(np.float64) + at(m + t).astype(np.float64))
+        out[:, m] = (F + (E[:, m] if carry else 0.0)).astype(np.float32)
+    return out
+
+
+def test_blur_exact_split_equals_the_recursion():
+    # The form the device evaluates (an exact blur plus the carried pair
+    # roundings) reproduces the recursion on long lines, where the exact
+    # 9-tap blur alone drifts from it.  It rounds the recursion's exact value;
+    # the reference's f64 resonators land on the other f32 neighbour for
+    # about one output in 10^6 (none on this plane).
+    rng = np.random.default_rng(12)
+    h, w = 24, 700
+    yy, xx = np.mgrid[0:h, 0:w]
+    a = (0.45 + 0.2 * np.sin(xx / 17.0) * np.cos(yy / 5.0) + rng.random((h, w)) * 0.05).astype(np.float32)
+    for plane in (a, (a * a).astype(np.float32)):
+        want = orc.blur(plane)
+        hor = _exact_split_lines(plane)
+        got = _exact_split_lines(np.ascontiguousarray(hor.T)).T
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    # the carried roundings matter: the blur without them differs
+    hor = _exact_split_lines(a, carry=False)
+    got = _exact_split_lines(np.ascontiguousarray(hor.T), carry=False).T
+    assert not np.array_equal(got, orc.blur(a))
+
+
 def test_blur_constant_interior_and_dark_border():
     out = orc.blur(np.ones((48, 64), np.float32))
     assert abs(out[24, 32] - 1.0) < 2e-3
