@@ -323,14 +323,14 @@ int32_t jpxlg_server_pending(jpxlg_server* srv);
 void jpxlg_server_destroy(jpxlg_server* srv);
 
 /* ---- launch-shape autotuning ------------------------------------------- */
-/* Times every candidate launch shape of the streaming SSIMULACRA2 pass (1, 2
- * or 4 strip units per workgroup, with or without an XCD-aware workgroup
- * order) on a synthetic image of width x height and keeps the fastest for
- * chains of that many pixels (size classes by log2 of a chain's total
- * pixels); the outputs (optional) report the choice.  No candidate changes a
- * result (a mismatch fails with JPXLG_E_DEVICE).  Clears the per-kernel
- * timing totals.  Untuned classes use 4 units, no swizzle.  No counterpart
- * in the reference. */
+/* Times the pooling SSIMULACRA2 pass (the vertical blur, four waves per
+ * workgroup) with and without an XCD-aware workgroup order on a synthetic
+ * image of width x height and keeps the faster for chains of that many
+ * pixels (size classes by log2 of a chain's total pixels); the outputs
+ * (optional) report the choice (units_per_group: the waves per workgroup,
+ * 4).  No candidate changes a result (a mismatch fails with
+ * JPXLG_E_DEVICE).  Clears the per-kernel timing totals.  Untuned classes
+ * use no swizzle.  No counterpart in the reference. */
 int32_t jpxlg_autotune(jpxlg_ctx* ctx, uint32_t width, uint32_t height, int32_t* units_per_group,
                        int32_t* swizzle);
 

@@ -100,8 +100,8 @@ struct jpxlg_ctx {
   double* static_sums = nullptr;
   // Launch shapes of the streaming SSIMULACRA2 pass per size class (class =
   // log2 of a chain's total pixels - 16, clamped): units per workgroup and
-  // the XCD swizzle, as picked by jpxlg_autotune (default 4, off).  No
-  // choice changes a result.
+  // the XCD swizzle, as picked by jpxlg_autotune (waves per workgroup fixed
+  // at 4; swizzle default off).  No choice changes a result.
   struct Shape {
     int upw = 4, swizzle = 0;
   };
@@ -251,12 +251,15 @@ size_t image_bytes(int w, int h) {
   const size_t cpx = 64 * bw * bh;
   int n, sw[kMaxScales], sh[kMaxScales];
   scale_dims(w, h, &n, sw, sh);
-  size_t pyr = 0;
-  for (int s = 0; s < n; s++) pyr += (size_t)sw[s] * sh[s];
+  size_t pyr = 0, pyrT = 0;
+  for (int s = 0; s < n; s++) {
+    pyr += (size_t)sw[s] * sh[s];
+    pyrT += transposed_floats(sw[s], sh[s]);
+  }
   size_t b = 0;
   b += 6 * px;                       // rgb (up to 16-bit)
   b += 3 * px * 4;                   // source XYB
-  b += 12 * pyr * 4;                 // reference pyramid: XYB' (and transposed), mu1, s11
+  b += 3 * pyr * 4 + 9 * pyrT * 4;  // reference pyramid: XYB', and transposed: XYB', mu1, s11
   b += 3 * cpx * 4 * 2;              // coefficients f32 + quantized i32
   b += 3 * bw * bh * 4 + bw * bh * 4 + bw * bh;  // LF planes, sigma, block map
   b += 3 * bw * bh * 4;              // work lists
@@ -265,7 +268,7 @@ size_t image_bytes(int w, int h) {
   b += (((bw + 3) / 4) * ((bh + 3) / 4) + 1) * 4;  // cover's exact-pass queue
   b += cpx * 18 + bw * bh * (16 + 3 * 16) + 2 * ((bw + 7) / 8 + 1) * ((bh + 7) / 8 + 1) * 4;  // CfL samples
   b += 6 * px * 4;                   // render planes + ping-pong
-  b += 15 * pyr * 4;                 // candidate XYB' pyramid (and transposed), three horizontal blurs
+  b += 3 * pyr * 4 + 12 * pyrT * 4; // candidate XYB' pyramid, and transposed: XYB', three horizontal blurs
   size_t units = 0;
   for (int s = 0; s < n; s++) units += (size_t)pool_units(sw[s], sh[s]);
   b += units * 18 * 8 + kMaxScales * 18 * 8 + 4096;
@@ -538,13 +541,14 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   for (int s = 0; s < img->nscales; s++) {
     size_t sp = (size_t)img->sw[s] * img->sh[s];
     for (int c = 0; c < 3; c++) {
+      const size_t tp = transposed_floats(img->sw[s], img->sh[s]);
       img->ref_x[s][c] = (float*)A.take(sp * 4);
-      img->ref_xT[s][c] = (float*)A.take(sp * 4);
-      img->ref_mu[s][c] = (float*)A.take(sp * 4);
-      img->ref_s11[s][c] = (float*)A.take(sp * 4);
+      img->ref_xT[s][c] = (float*)A.take(tp * 4);
+      img->ref_mu[s][c] = (float*)A.take(tp * 4);
+      img->ref_s11[s][c] = (float*)A.take(tp * 4);
       img->cand_x[s][c] = (float*)A.take(sp * 4);
-      img->cand_xT[s][c] = (float*)A.take(sp * 4);
-      for (int m = 0; m < 3; m++) img->cand_h[s][c][m] = (float*)A.take(sp * 4);
+      img->cand_xT[s][c] = (float*)A.take(tp * 4);
+      for (int m = 0; m < 3; m++) img->cand_h[s][c][m] = (float*)A.take(tp * 4);
     }
   }
   img->d_ref_sets = (LineSet*)A.take(3 * kMaxScales * sizeof(LineSet) + 2 * (3 * kMaxScales + 1) * 4);
@@ -623,6 +627,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
         L = LineSet{};
         L.w = img->sw[s];
         L.h = img->sh[s];
+        L.pitch = line_pitch(L.h);
         L.x1 = img->ref_x[s][c];
         L.hT[0] = img->cand_h[s][c][0];
         L.hT[1] = img->cand_h[s][c][1];
@@ -632,7 +637,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
         off_b[ns] = nb;
         off_c[ns] = ncol;
         nb += blur_bands(L.w, L.h);
-        ncol += pool_units(L.w, L.h);
+        ncol += blur_columns(L.w, L.h);
       }
     off_b[ns] = nb;
     off_c[ns] = ncol;
@@ -649,7 +654,7 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
     const int* d_oc = (const int*)((char*)img->d_ref_sets + sb + ob);
     Prof p(ctx, "ref_blur");
     launch_blur_h(st, 0, ctx->d_tab, img->d_ref_sets, d_ob, ns, nb);
-    launch_blur_v(st, 0, ctx->d_tab, img->d_ref_sets, d_oc, ns, ncol, 4, 0);
+    launch_blur_v(st, 0, ctx->d_tab, img->d_ref_sets, d_oc, ns, ncol, 0);
   }
   HIP_TRY(ctx, hipGetLastError());
   // The grayscale flag stays on the device (the CfL kernels read it); the
@@ -1275,6 +1280,7 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
         LineSet& S = hs[t.nsets];
         S.w = img->sw[s];
         S.h = img->sh[s];
+        S.pitch = line_pitch(S.h);
         S.x1 = img->ref_x[s][c];
         S.x2 = img->cand_x[s][c];
         for (int m = 0; m < 3; m++) S.hT[m] = img->cand_h[s][c][m];
@@ -1287,7 +1293,7 @@ static int fill_chain(jpxlg_ctx* ctx, jpxlg_img* const* imgs, int n, const jpxlg
         ho[L.b_off + t.nsets] = t.bands;
         ho[L.v_off + t.nsets] = t.cols;
         t.bands += blur_bands(S.w, S.h);
-        t.cols += pool_units(S.w, S.h);
+        t.cols += blur_columns(S.w, S.h);
         t.nsets++;
       }
     P.sums = (double*)(dev + L.sums) + (size_t)k * 18 * kDescScales;
@@ -1376,7 +1382,7 @@ static int issue_chain(jpxlg_ctx* ctx, hipStream_t st, int n, bool quantize, con
     }
     {
       Prof p(ctx, "blur_pool", st);
-      launch_blur_v(st, 1, ctx->d_tab, S, O + L.v_off, T.nsets, T.cols, T.upw, T.swizzle);
+      launch_blur_v(st, 1, ctx->d_tab, S, O + L.v_off, T.nsets, T.cols, T.swizzle);
     }
     { Prof p(ctx, "pool_reduce", st); launch_reduce_b(st, D, n); }
   }
@@ -2516,11 +2522,12 @@ static bool carve_pair(Arena& A, int w, int h, bool device_inputs, const float* 
     B->cx[c] = (float*)A.take(px * 4);
     ok = ok && B->in_r[c] && B->in_c[c] && B->rx[c] && B->cx[c];
   }
-  B->x1T = (float*)A.take(px * 4);
-  B->x2T = (float*)A.take(px * 4);
-  for (int m = 0; m < 3; m++) ok = ok && (B->h[m] = (float*)A.take(px * 4)) != nullptr;
-  B->mu = (float*)A.take(px * 4);
-  B->s11 = (float*)A.take(px * 4);
+  const size_t tp = transposed_floats(w, h);  // the largest scale's transposed plane
+  B->x1T = (float*)A.take(tp * 4);
+  B->x2T = (float*)A.take(tp * 4);
+  for (int m = 0; m < 3; m++) ok = ok && (B->h[m] = (float*)A.take(tp * 4)) != nullptr;
+  B->mu = (float*)A.take(tp * 4);
+  B->s11 = (float*)A.take(tp * 4);
   size_t units = 0;
   for (int s = 0; s < n; s++) units += (size_t)pool_units(sw[s], sh[s]);
   B->partials = (double*)A.take(units * 18 * 8);
@@ -2570,6 +2577,7 @@ static int score_pair_run(jpxlg_ctx* ctx, int w, int h, const float* reference, 
       LineSet R{}, C{};
       R.w = C.w = sw[s];
       R.h = C.h = sh[s];
+      R.pitch = C.pitch = line_pitch(sh[s]);
       R.x1 = C.x1 = B.rx[c];
       R.hT[0] = B.h[0];
       R.hT[1] = B.h[1];
@@ -2589,7 +2597,7 @@ static int score_pair_run(jpxlg_ctx* ctx, int w, int h, const float* reference, 
         offs[ns][0] = 0;
         offs[ns][1] = blur_bands(L->w, L->h);
         offs[ns][2] = 0;
-        offs[ns][3] = pool_units(L->w, L->h);
+        offs[ns][3] = blur_columns(L->w, L->h);
         ns++;
       }
     }
@@ -2611,9 +2619,9 @@ static int score_pair_run(jpxlg_ctx* ctx, int w, int h, const float* reference, 
     launch_colour_metric(st, 1, sw[s], sh[s], cin, nullptr, nullptr, 0, B.cx, cd);
     for (int c = 0; c < 3; c++, j += 2) {
       launch_blur_h(st, 0, ctx->d_tab, dsets + j, doffs + 4 * j, 1, offs[j][1]);
-      launch_blur_v(st, 0, ctx->d_tab, dsets + j, doffs + 4 * j + 2, 1, offs[j][3], 4, 0);
+      launch_blur_v(st, 0, ctx->d_tab, dsets + j, doffs + 4 * j + 2, 1, offs[j][3], 0);
       launch_blur_h(st, 1, ctx->d_tab, dsets + j + 1, doffs + 4 * (j + 1), 1, offs[j + 1][1]);
-      launch_blur_v(st, 1, ctx->d_tab, dsets + j + 1, doffs + 4 * (j + 1) + 2, 1, offs[j + 1][3], 4, 0);
+      launch_blur_v(st, 1, ctx->d_tab, dsets + j + 1, doffs + 4 * (j + 1) + 2, 1, offs[j + 1][3], 0);
     }
   }
   launch_reduce_scales(st, PL, B.partials, B.sums);
@@ -2748,13 +2756,14 @@ extern "C" int32_t jpxlg_autotune(jpxlg_ctx* ctx, uint32_t width, uint32_t heigh
   const jpxlg_ctx::Shape keep = ctx->tune[cls];
   const bool prof = ctx->prof;
   const std::string only = ctx->prof_only;
-  static const jpxlg_ctx::Shape kCand[6] = {{1, 0}, {2, 0}, {4, 0}, {1, 1}, {2, 1}, {4, 1}};
+  // the vertical pass's workgroups have four waves; the order is tuned
+  static const jpxlg_ctx::Shape kCand[2] = {{4, 0}, {4, 1}};
   double best_ms = 1e30, ref_score = 0.0;
   jpxlg_ctx::Shape best = keep;
   bool first = true;
   // earlier timing records are folded away before the candidates run
   rc = jpxlg_profile_read(ctx, nullptr, nullptr, nullptr, 0);
-  for (int a = 0; a < 6 && rc >= 0; a++) {
+  for (int a = 0; a < 2 && rc >= 0; a++) {
     ctx->tune[cls] = kCand[a];
     double score = 0.0;
     ctx->prof = false;

@@ -370,9 +370,10 @@ void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], con
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
                           const int counts[3], uint32_t* muls);
 // One scale and channel of the metric's blur passes: the planes each pass
-// reads and writes.  Transposed planes hold column x at x * h.
+// reads and writes.  Transposed planes hold column x at x * pitch (pitch =
+// h rounded up to even, so that pairs of samples load as one 8-byte word).
 struct LineSet {
-  int w, h;
+  int w, h, pitch;
   const float* x1;     // horizontal pass: the reference plane (row-major)
   const float* x2;     // horizontal pass (candidate): the candidate plane (row-major)
   float* hT[3];        // horizontal outputs per moment (transposed)
@@ -386,20 +387,25 @@ struct LineSet {
 };
 // Workgroups of the horizontal pass over a w x h plane (bands of rows).
 int blur_bands(int w, int h);
-// Waves of the vertical pass over a w x h plane (groups of columns); each
-// candidate wave leaves one partial row of six sums per channel.
+// Line pitch of the transposed planes of a plane h samples high, and their
+// size in floats.
+inline int line_pitch(int h) { return (h + 1) & ~1; }
+inline size_t transposed_floats(int w, int h) { return (size_t)w * line_pitch(h); }
+// Workgroups of the vertical pass over a w x h plane (groups of columns, or
+// one long column each), and the partial rows of six sums per channel they
+// leave (one per wave).
+int blur_columns(int w, int h);
 int pool_units(int w, int h);
-// The blur passes over `nsets` line sets (device array) with unit prefixes
-// `off` (nsets + 1 ints, device): the horizontal pass in bands (one
-// workgroup each, blur_bands per set), the vertical pass in waves
-// (pool_units per set, upw 1, 2 or 4 per workgroup, with or without the XCD
-// swizzle of the workgroup order).
-// cand 0: the reference's moments (x1, x1^2), 1: the candidate's (x2, x2^2,
-// x1 x2) with the SSIM and edge pooling.
+// The blur passes over `nsets` line sets (device array) with workgroup
+// prefixes `off` (nsets + 1 ints, device): the horizontal pass in bands
+// (blur_bands per set), the vertical pass in column groups (blur_columns per
+// set, with or without the XCD swizzle of the workgroup order).  cand 0: the
+// reference's moments (x1, x1^2), 1: the candidate's (x2, x2^2, x1 x2) with
+// the SSIM and edge pooling.
 void launch_blur_h(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
                    int bands);
 void launch_blur_v(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
-                   int columns, int upw, int swizzle);
+                   int groups, int swizzle);
 constexpr int kMaxPoolScales = 6;
 // Where each scale's units start in the partials buffer.
 struct PoolLayout {

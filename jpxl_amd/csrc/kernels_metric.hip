@@ -410,7 +410,7 @@ __global__ __launch_bounds__(256) void k_linear_at_depth(int n, const float* __r
 constexpr int kSeg = 10;         // positions per lane
 constexpr int kBlockStep = 620;  // positions a continued block adds (62 lanes)
 constexpr int kHWaves = 16;      // horizontal pass: waves (1024 threads) per workgroup
-constexpr int kTile = 10752;     // horizontal pass: LDS tile (R rows x (10 G + 1), R = 1024 / G)
+constexpr int kTile = 10752;     // horizontal pass: LDS tile of R rows x (10 G + 1), R = 1024 / G
 
 using SetSpace = const __attribute__((address_space(4))) LineSet;
 
@@ -463,22 +463,30 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __hiloint2double(rhi, rlo);
 }
 
+// One step for the ten values, five at a time (the moves of five values
+// cover each other's DPP wait states).
 template <int CTRL, int ROWS, bool BC>
 __device__ __forceinline__ void prefix_step(double (&v)[kSeg]) {
-  double t[kSeg];
 #pragma unroll
-  for (int j = 0; j < kSeg; j++) t[j] = dpp_f64<CTRL, ROWS, BC>(v[j]);
+  for (int h = 0; h < kSeg; h += 5) {
+    double t[5];
 #pragma unroll
-  for (int j = 0; j < kSeg; j++) v[j] += t[j];
+    for (int j = 0; j < 5; j++) t[j] = dpp_f64<CTRL, ROWS, BC>(v[h + j]);
+#pragma unroll
+    for (int j = 0; j < 5; j++) v[h + j] += t[j];
+  }
 }
 
 template <int CTRL, int D>
 __device__ __forceinline__ void prefix_step_masked(double (&v)[kSeg], int gl) {
-  double t[kSeg];
 #pragma unroll
-  for (int j = 0; j < kSeg; j++) t[j] = dpp_f64<CTRL, 0xf, true>(v[j]);
+  for (int h = 0; h < kSeg; h += 5) {
+    double t[5];
 #pragma unroll
-  for (int j = 0; j < kSeg; j++) v[j] += gl >= D ? t[j] : 0.0;
+    for (int j = 0; j < 5; j++) t[j] = dpp_f64<CTRL, 0xf, true>(v[h + j]);
+#pragma unroll
+    for (int j = 0; j < 5; j++) v[h + j] += gl >= D ? t[j] : 0.0;
+  }
 }
 
 // Inclusive prefix sums of the ten values over aligned groups of G lanes.
@@ -526,12 +534,15 @@ __device__ __forceinline__ float moment_value(int m, float a, float b) {
 
 // One block of one line and moment: from the lane's samples u[j] at
 // positions P + j (P = block start + 10 gl) the recursion's values Y[j] at the
-// same positions, before their f32 rounding.  gl: lane within the line's group
-// of G lanes; the first `warm` lanes only seed their neighbours; C[j]: Y at
-// P0 - 10 + j for the first lane P0 past them (zero at a line's start).
+// same positions, before their f32 rounding, for a chain that starts at zero
+// at lane `warm` (the first lanes of a continued block only seed their
+// neighbours).  gl: lane within the line's group of G lanes.  A continued
+// block's true values are Y[j] - (-1)^gl C[j], C[j] being the previous
+// block's last-lane value of class j (a linear chain: its start only adds
+// that alternating term).
 template <int G, bool MULTI>
-__device__ __forceinline__ void seg_block(const float (&u)[kSeg], int gl, int warm, const double (&C)[kSeg],
-                                          const LineTaps& T, double (&Y)[kSeg]) {
+__device__ __forceinline__ void seg_block(const float (&u)[kSeg], int gl, int warm, const LineTaps& T,
+                                          double (&Y)[kSeg]) {
   // a group start receives zeros (the line starts there); with G == 64 the
   // only group start is lane 0, which the shift already fills with 0
   const bool zero_in = G < 64 && gl == 0;
@@ -562,26 +573,30 @@ __device__ __forceinline__ void seg_block(const float (&u)[kSeg], int gl, int wa
     Y[j] = __hiloint2double(__double2hiint(psi) ^ sign, __double2loint(psi));
   }
   if constexpr (MULTI) {
-    // continued blocks: the leading lanes only seed their neighbours, and the
-    // chain starts from the carry at the first lane past them
 #pragma unroll
-    for (int j = 0; j < kSeg; j++) Y[j] = gl < warm ? 0.0 : (gl == warm ? Y[j] - C[j] : Y[j]);
+    for (int j = 0; j < kSeg; j++) Y[j] = gl < warm ? 0.0 : Y[j];
   }
   group_prefix<G>(Y, gl);
 #pragma unroll
   for (int j = 0; j < kSeg; j++) Y[j] = __hiloint2double(__double2hiint(Y[j]) ^ sign, __double2loint(Y[j]));
 }
 
+// Y - (-1)^gl c, the continued chain's values.
+__device__ __forceinline__ double continue_chain(double y, double c, int gl) {
+  return (gl & 1) ? y + c : y - c;
+}
+
 // Horizontal pass: workgroup (set, band of R = 1024 / G rows), each wave
-// 64 / G rows.  The outputs (and the first moment's source plane) go out
-// transposed.
+// 64 / G rows, a long row's blocks one after another (the carry is the
+// previous block's last lane, read from the wave's LDS slot).  The outputs
+// (and the first moment's source plane) go out transposed.
 template <int CAND, int G, bool MULTI>
-__device__ __forceinline__ void blur_h_band(SetSpace& S, int band, const LineTaps& T, float* tile) {
+__device__ __forceinline__ void blur_h_band(SetSpace& S, int band, const LineTaps& T, float* tile, double* carry) {
   constexpr int M = CAND ? 3 : 2;
-  constexpr int LPW = 64 / G, R = 1024 / G, STRIDE = kSeg * G + 1;
+  constexpr int LPW = 64 / G, R = kHWaves * LPW, STRIDE = kSeg * G + 1, NT = 64 * kHWaves;
   const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
   const int gl = lane & (G - 1), grp = lane / G;
-  const int w = S.w, h = S.h;
+  const int w = S.w, h = S.h, pitch = S.pitch;
   const int blocks = MULTI ? line_shape(w).blocks : 1;
   const int y0 = band * R;
   const int rl = wv * LPW + grp;  // the group's row in the band
@@ -589,11 +604,9 @@ __device__ __forceinline__ void blur_h_band(SetSpace& S, int band, const LineTap
   const int y = min(y0 + rl, h - 1);
   const float* r1 = S.x1 + (size_t)y * w;
   const float* r2 = CAND ? S.x2 + (size_t)y * w : r1;
-  double C[M][kSeg];
-#pragma unroll
-  for (int m = 0; m < M; m++)
-#pragma unroll
-    for (int j = 0; j < kSeg; j++) C[m][j] = 0.0;
+  // this wave's carries per moment (MULTI: one row per wave)
+  double* C = carry + wv * 3 * kSeg;
+  if (MULTI && lane < M * kSeg) C[lane] = 0.0;
   // the lane's samples of block b: positions 620 b + 10 gl + j
   auto fetch = [&](int b, float (&a)[kSeg], float (&bv)[kSeg]) {
 #pragma unroll
@@ -609,56 +622,82 @@ __device__ __forceinline__ void blur_h_band(SetSpace& S, int band, const LineTap
   fetch(0, a, bv);
   for (int b = 0; b < blocks; b++) {
     const int pb = kBlockStep * b, warm = b ? 2 : 0;
-    float na[kSeg], nbv[kSeg];
-    if (MULTI && b + 1 < blocks) fetch(b + 1, na, nbv);
-#pragma unroll
-    for (int m = 0; m <= M; m++) {
+    // stage plane m (a moment's blur, or with m == M the source plane) in
+    // tile half q
+    auto stage = [&](auto mc, int q) {
+      constexpr int m = decltype(mc)::value;
       float out[kSeg];
-      if (m < M) {
+      if constexpr (m < M) {
         float u[kSeg];
 #pragma unroll
         for (int j = 0; j < kSeg; j++) u[j] = moment_value<M>(m, a[j], bv[j]);
         double Y[kSeg];
-        seg_block<G, MULTI>(u, gl, warm, C[m], T, Y);
+        seg_block<G, MULTI>(u, gl, warm, T, Y);
+        if constexpr (MULTI) {
+#pragma unroll
+          for (int j = 0; j < kSeg; j++) Y[j] = continue_chain(Y[j], C[m * kSeg + j], gl);
+          // the next block continues from this block's last lane
+          if (gl == 63)
+#pragma unroll
+            for (int j = 0; j < kSeg; j++) C[m * kSeg + j] = Y[j];
+        }
 #pragma unroll
         for (int j = 0; j < kSeg; j++) out[j] = (float)Y[j];
-        if constexpr (MULTI)
-#pragma unroll
-          for (int j = 0; j < kSeg; j++) C[m][j] = lane_value(Y[j], 63);
       } else {
 #pragma unroll
-        for (int j = 0; j < kSeg; j++) out[j] = CAND ? bv[j] : a[j];  // the source plane
+        for (int j = 0; j < kSeg; j++) out[j] = CAND ? bv[j] : a[j];
       }
+      float* t = tile + q * kTile;
 #pragma unroll
-      for (int j = 0; j < kSeg; j++) tile[rl * STRIDE + kSeg * gl + j] = out[j];
-      lds_barrier();
-      // tile column col holds position pb + col: output n = pb + col - 4, or
-      // the source sample n = pb + col; R consecutive rows per column
+      for (int j = 0; j < kSeg; j++) t[rl * STRIDE + kSeg * gl + j] = out[j];
+    };
+    // write plane m from tile half q: tile column col holds position
+    // pb + col, i.e. output n = pb + col - 4 (or source sample n = pb + col);
+    // R consecutive rows per column
+    auto flush = [&](int m, int q) {
       float* dst = m < M ? S.hT[m] : S.xT;
+      const float* t = tile + q * kTile;
       const int nb = m < M ? pb - 4 : pb;
 #pragma unroll
       for (int it = 0; it < kSeg; it++) {
-        const int e = it * 1024 + (int)threadIdx.x;
+        const int e = it * NT + (int)threadIdx.x;
         const int col = e / R, row = e % R;
         const int n = nb + col, yy = y0 + row;
         if (col >= kSeg * warm && n >= 0 && n < w && yy < h)
-          stg32(dst, (uint32_t)n * (uint32_t)h + (uint32_t)yy, tile[row * STRIDE + col]);
+          stg32(dst, (uint32_t)n * (uint32_t)pitch + (uint32_t)yy, t[row * STRIDE + col]);
       }
+    };
+    // one plane per round: the moments' blurs, then the source plane (the
+    // next block's samples are requested before the last round goes out)
+    stage(std::integral_constant<int, 0>{}, 0);
+    lds_barrier();
+    flush(0, 0);
+    lds_barrier();
+    stage(std::integral_constant<int, 1>{}, 0);
+    lds_barrier();
+    flush(1, 0);
+    lds_barrier();
+    stage(std::integral_constant<int, 2>{}, 0);
+    lds_barrier();
+    if constexpr (M == 2)
+      if (MULTI && b + 1 < blocks) fetch(b + 1, a, bv);
+    flush(2, 0);
+    lds_barrier();
+    if constexpr (M == 3) {
+      stage(std::integral_constant<int, 3>{}, 0);
+      lds_barrier();
+      if (MULTI && b + 1 < blocks) fetch(b + 1, a, bv);
+      flush(3, 0);
       lds_barrier();
     }
-    if (MULTI && b + 1 < blocks)
-#pragma unroll
-      for (int j = 0; j < kSeg; j++) {
-        a[j] = na[j];
-        bv[j] = nbv[j];
-      }
   }
 }
 
 template <int CAND>
-__global__ __launch_bounds__(1024) void k_blur_h(const ConstTables* __restrict__ tab, const LineSet* __restrict__ sets,
+__global__ __launch_bounds__(64 * kHWaves) void k_blur_h(const ConstTables* __restrict__ tab, const LineSet* __restrict__ sets,
                                                  const int* __restrict__ off, int nsets) {
   __shared__ float tile[kTile];
+  __shared__ double carry[kHWaves * 3 * kSeg];
   const int k = find_desc(off, nsets, blockIdx.x);
   SetSpace& S = ((SetSpace*)sets)[k];
   const int band = blockIdx.x - off[k];
@@ -666,14 +705,14 @@ __global__ __launch_bounds__(1024) void k_blur_h(const ConstTables* __restrict__
   const LineShape sh = line_shape(S.w);
   switch (sh.G) {
     case 64:
-      if (sh.blocks > 1) blur_h_band<CAND, 64, true>(S, band, T, tile);
-      else blur_h_band<CAND, 64, false>(S, band, T, tile);
+      if (sh.blocks > 1) blur_h_band<CAND, 64, true>(S, band, T, tile, carry);
+      else blur_h_band<CAND, 64, false>(S, band, T, tile, carry);
       break;
-    case 32: blur_h_band<CAND, 32, false>(S, band, T, tile); break;
-    case 16: blur_h_band<CAND, 16, false>(S, band, T, tile); break;
-    case 8: blur_h_band<CAND, 8, false>(S, band, T, tile); break;
-    case 4: blur_h_band<CAND, 4, false>(S, band, T, tile); break;
-    default: blur_h_band<CAND, 2, false>(S, band, T, tile); break;
+    case 32: blur_h_band<CAND, 32, false>(S, band, T, tile, carry); break;
+    case 16: blur_h_band<CAND, 16, false>(S, band, T, tile, carry); break;
+    case 8: blur_h_band<CAND, 8, false>(S, band, T, tile, carry); break;
+    case 4: blur_h_band<CAND, 4, false>(S, band, T, tile, carry); break;
+    default: blur_h_band<CAND, 2, false>(S, band, T, tile, carry); break;
   }
 }
 
@@ -683,7 +722,19 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// The SSIM and edge terms of one pixel (pool.rs:97-278) into acc.
+// The SSIM and edge terms of one pixel (pool.rs:97-278) into acc.  The
+// SSIM ratio is the reference's f32 division; the edge ratio (an f64
+// division in the reference) is a reciprocal refined by two Newton steps and
+// a corrected quotient, within an ulp of the quotient (it only enters f64
+// sums).
+__device__ __forceinline__ double edge_ratio(double a, double b) {
+  double r = __builtin_amdgcn_rcp(b);
+  r = fma(r, fma(-b, r, 1.0), r);
+  r = fma(r, fma(-b, r, 1.0), r);
+  const double q = a * r;
+  return fma(r, fma(-b, q, a), q);
+}
+
 __device__ __forceinline__ void pool_pixel(float mu1, float s11, float i1, float i2, float mu2, float s22, float s12,
                                            double (&acc)[6]) {
   const float C2 = 0.0009f;
@@ -695,109 +746,200 @@ __device__ __forceinline__ void pool_pixel(float mu1, float s11, float i1, float
   double d = 1.0 - (double)((num_m * num_s) / den_s);
   d = d > 0.0 ? d : 0.0;
   acc[0] += d;
-  acc[1] += d * d * d * d;
-  const double d1 = (1.0 + (double)fabsf(i2 - mu2)) / (1.0 + (double)fabsf(i1 - mu1)) - 1.0;
+  const double d2 = d * d;
+  acc[1] += d2 * d2;
+  const double d1 = edge_ratio(1.0 + (double)fabsf(i2 - mu2), 1.0 + (double)fabsf(i1 - mu1)) - 1.0;
   const double art = d1 > 0.0 ? d1 : 0.0;
   const double lost = -d1 > 0.0 ? -d1 : 0.0;
+  const double a2 = art * art, l2 = lost * lost;
   acc[2] += art;
-  acc[3] += art * art * art * art;
+  acc[3] += a2 * a2;
   acc[4] += lost;
-  acc[5] += lost * lost * lost * lost;
+  acc[5] += l2 * l2;
 }
 
-// Vertical pass: one wave per (set, 64 / G columns), blockDim.x / 64 waves
-// per workgroup.  With `swizzle`, the workgroups the dispatcher deals
-// round-robin to the eight XCDs take contiguous runs of units.  The launch
-// shape changes no output.
-template <int CAND, int G, bool MULTI>
-__device__ __forceinline__ void blur_v_unit(SetSpace& S, int ul, const LineTaps& T) {
-  constexpr int M = CAND ? 3 : 2;
-  constexpr int LPW = 64 / G;
-  const int lane = threadIdx.x & 63;
-  const int gl = lane & (G - 1), grp = lane / G;
-  const int w = S.w, h = S.h;
-  const int blocks = MULTI ? line_shape(h).blocks : 1;
-  const int xr = ul * LPW + grp;
-  const bool line_ok = xr < w;
-  const size_t base = (size_t)min(xr, w - 1) * h;
-  double C[M][kSeg];
+// Pairs of samples of a transposed line: v[j] at position p0 + j (p0 even),
+// zero before 0 and at and past h; the line's allocation ends at its even
+// pitch.
+__device__ __forceinline__ void load_line10(const float* __restrict__ line, int p0, int h, float (&v)[kSeg]) {
 #pragma unroll
-  for (int m = 0; m < M; m++)
+  for (int k = 0; k < kSeg; k += 2) {
+    const int p = p0 + k;
+    uint64_t t = 0;
+    if (p >= 0 && p < h) t = ldg32((const uint64_t*)line, (uint32_t)(p >> 1));
+    v[k] = __uint_as_float((uint32_t)t);
+    v[k + 1] = p + 1 < h ? __uint_as_float((uint32_t)(t >> 32)) : 0.0f;
+  }
+}
+
+// Vertical pass: workgroups of four waves.  A short line (one block) takes
+// part of a wave (four waves: 4 * 64 / G lines); a long line takes a whole
+// workgroup, its blocks four at a time, one per wave: each block's chain
+// runs from zero, its last lane's values go to LDS, and after a barrier each
+// block continues from the sum of the earlier blocks' last values (the
+// running carry of the rounds before plus this round's earlier blocks); the
+// chains' f64 values wait in registers for their carries.  Every wave leaves one
+// partial row of six sums (zero when it pools nothing).  With `swizzle`, the
+// workgroups the dispatcher deals round-robin to the eight XCDs take
+// contiguous runs.  The launch shape changes no output.
+constexpr int kVWaves = 4;
+
+struct VShared {
+  float ys[kVWaves][3 * kSeg * 64];   // rounded outputs per wave: [moment][j][lane]
+  double ends[kVWaves][3 * kSeg];     // each round's blocks' last-lane values
+  double carry[3 * kSeg];             // the running carry entering the round
+};
+
+// Outputs of one block held in ys (moment m, class j at ys[(m * 10 + j) * 64 +
+// lane]) for positions n0 + j: the candidate's SSIM and edge sums, or the
+// reference's mu1 and s11.
+template <int CAND>
+__device__ __forceinline__ void v_outputs(SetSpace& S, size_t base, int h, int n0, bool own, const float* ys,
+                                          int lane, double (&acc)[6]) {
+  if constexpr (CAND != 0) {
+    // the reference's mu1, s11, x1 and the candidate's x2, a pair of
+    // positions at a time
+    const float* planes[4] = {S.mu1T + base, S.s11T + base, S.x1T + base, S.x2T + base};
 #pragma unroll
-    for (int j = 0; j < kSeg; j++) C[m][j] = 0.0;
-  double acc[6] = {0, 0, 0, 0, 0, 0};
-  for (int b = 0; b < blocks; b++) {
-    const int pb = kBlockStep * b, warm = b ? 2 : 0;
-    float y[M][kSeg];
+    for (int k = 0; k < kSeg; k += 2) {
+      const int n = n0 + k;
+      if (!own || n < 0 || n >= h) continue;
+      float q[4][2];
 #pragma unroll
-    for (int m = 0; m < M; m++) {
-      const float* in = S.hT[m] + base;
-      float u[kSeg];
-#pragma unroll
-      for (int j = 0; j < kSeg; j++) {
-        const int p = pb + kSeg * gl + j;
-        u[j] = p < h ? ldg32(in, (uint32_t)min(p, h - 1)) : 0.0f;
+      for (int i = 0; i < 4; i++) {
+        const uint64_t t = ldg32((const uint64_t*)planes[i], (uint32_t)(n >> 1));
+        q[i][0] = __uint_as_float((uint32_t)t);
+        q[i][1] = __uint_as_float((uint32_t)(t >> 32));
       }
-      double Y[kSeg];
-      seg_block<G, MULTI>(u, gl, warm, C[m], T, Y);
-#pragma unroll
-      for (int j = 0; j < kSeg; j++) y[m][j] = (float)Y[j];
-      if constexpr (MULTI)
-#pragma unroll
-        for (int j = 0; j < kSeg; j++) C[m][j] = lane_value(Y[j], 63);
+      const float* y0 = ys + k * 64 + lane;
+      pool_pixel(q[0][0], q[1][0], q[2][0], q[3][0], y0[0], y0[kSeg * 64], y0[2 * kSeg * 64], acc);
+      if (n + 1 < h)
+        pool_pixel(q[0][1], q[1][1], q[2][1], q[3][1], y0[64], y0[kSeg * 64 + 64], y0[2 * kSeg * 64 + 64], acc);
     }
-    // outputs n = P + j - 4 of the lane's own positions
-    const bool own = line_ok && gl >= warm;
+  } else {
 #pragma unroll
     for (int j = 0; j < kSeg; j++) {
-      const int n = pb + kSeg * gl + j - 4;
+      const int n = n0 + j;
       if (!own || n < 0 || n >= h) continue;
-      if constexpr (CAND != 0) {
-        const uint32_t o = (uint32_t)n;
-        pool_pixel(ldg32(S.mu1T + base, o), ldg32(S.s11T + base, o), ldg32(S.x1T + base, o),
-                   ldg32(S.x2T + base, o), y[0][j], y[1][j], y[2][j], acc);
-      } else {
-        stg32(S.out[0] + base, (uint32_t)n, y[0][j]);
-        stg32(S.out[1] + base, (uint32_t)n, y[1][j]);
-      }
+      stg32(S.out[0] + base, (uint32_t)n, ys[j * 64 + lane]);
+      stg32(S.out[1] + base, (uint32_t)n, ys[(kSeg + j) * 64 + lane]);
     }
   }
-  if constexpr (CAND != 0) {
+}
+
+// Short lines: wave wv of the workgroup's lw-th group of 4 * 64 / G lines.
+template <int CAND, int G>
+__device__ __forceinline__ void blur_v_short(SetSpace& S, int lw, const LineTaps& T, VShared& sh, double (&acc)[6]) {
+  constexpr int M = CAND ? 3 : 2;
+  constexpr int LPW = 64 / G;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int gl = lane & (G - 1), grp = lane / G;
+  const int w = S.w, h = S.h;
+  const int xr = (lw * kVWaves + wv) * LPW + grp;
+  const bool line_ok = xr < w;
+  const size_t base = (size_t)min(xr, w - 1) * S.pitch;
+  float* ys = sh.ys[wv];
+  float u[M][kSeg];
 #pragma unroll
-    for (int t = 0; t < 6; t++) {
-      const double v = wave_sum(acc[t]);
-      if (lane == 0) S.partials[(size_t)ul * 18 + t] = v;
+  for (int m = 0; m < M; m++) load_line10(S.hT[m] + base, kSeg * gl, h, u[m]);
+#pragma unroll
+  for (int m = 0; m < M; m++) {
+    double Y[kSeg];
+    seg_block<G, false>(u[m], gl, 0, T, Y);
+#pragma unroll
+    for (int j = 0; j < kSeg; j++) ys[(m * kSeg + j) * 64 + lane] = (float)Y[j];
+  }
+  v_outputs<CAND>(S, base, h, kSeg * gl - 4, line_ok, ys, lane, acc);
+}
+
+// A long line x = lw: blocks in rounds of four, one per wave.
+template <int CAND>
+__device__ __forceinline__ void blur_v_long(SetSpace& S, int lw, const LineTaps& T, VShared& sh, double (&acc)[6]) {
+  constexpr int M = CAND ? 3 : 2;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int h = S.h;
+  const int blocks = line_shape(h).blocks;
+  const size_t base = (size_t)lw * S.pitch;
+  float* ys = sh.ys[wv];
+  if (threadIdx.x < M * kSeg) sh.carry[threadIdx.x] = 0.0;
+  for (int r = 0; r * kVWaves < blocks; r++) {
+    const int b = r * kVWaves + wv;
+    const bool active = b < blocks;  // wave-uniform
+    const int pb = kBlockStep * b, warm = b ? 2 : 0;
+    double Y[M][kSeg];
+    if (active) {
+      float u[M][kSeg];
+#pragma unroll
+      for (int m = 0; m < M; m++) load_line10(S.hT[m] + base, pb + kSeg * lane, h, u[m]);
+#pragma unroll
+      for (int m = 0; m < M; m++) {
+        seg_block<64, true>(u[m], lane, warm, T, Y[m]);
+        if (lane == 63)
+#pragma unroll
+          for (int j = 0; j < kSeg; j++) sh.ends[wv][m * kSeg + j] = Y[m][j];
+      }
     }
+    lds_barrier();
+    if (active) {
+      // this block's carry: the running carry plus this round's earlier ends
+#pragma unroll
+      for (int m = 0; m < M; m++)
+#pragma unroll
+        for (int j = 0; j < kSeg; j++) {
+          double c = sh.carry[m * kSeg + j];
+          for (int k = 0; k < wv; k++) c += sh.ends[k][m * kSeg + j];
+          ys[(m * kSeg + j) * 64 + lane] = (float)continue_chain(Y[m][j], c, lane);
+        }
+      v_outputs<CAND>(S, base, h, pb + kSeg * lane - 4, lane >= warm, ys, lane, acc);
+    }
+    lds_barrier();
+    // the carry entering the next round (the blocks' ends in order)
+    if (threadIdx.x < M * kSeg) {
+      double c = sh.carry[threadIdx.x];
+      for (int k = 0; k < kVWaves && r * kVWaves + k < blocks; k++) c += sh.ends[k][threadIdx.x];
+      sh.carry[threadIdx.x] = c;
+    }
+    lds_barrier();
   }
 }
 
 template <int CAND>
-__global__ __launch_bounds__(256) void k_blur_v(const ConstTables* __restrict__ tab, const LineSet* __restrict__ sets,
-                                                const int* __restrict__ off, int nsets, int total, int swizzle) {
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), upw = blockDim.x >> 6;
+__global__ __launch_bounds__(64 * kVWaves) void k_blur_v(const ConstTables* __restrict__ tab,
+                                                         const LineSet* __restrict__ sets,
+                                                         const int* __restrict__ off, int nsets, int total,
+                                                         int swizzle) {
+  __shared__ VShared sh;
   const int nb = gridDim.x;
   int bi = blockIdx.x;
   if (swizzle && nb >= 64) {
     const int per = nb / 8;
     if (bi < per * 8) bi = (bi % 8) * per + bi / 8;
   }
-  const int unit = bi * upw + wv;
-  if (unit >= total) return;
-  const int k = find_desc(off, nsets, unit);
+  if (bi >= total) return;
+  const int k = find_desc(off, nsets, bi);
   SetSpace& S = ((SetSpace*)sets)[k];
-  const int ul = unit - off[k];
+  const int lw = bi - off[k];
   const LineTaps T = line_taps(tab);
-  const LineShape sh = line_shape(S.h);
-  switch (sh.G) {
+  const LineShape shp = line_shape(S.h);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  switch (shp.G) {
     case 64:
-      if (sh.blocks > 1) blur_v_unit<CAND, 64, true>(S, ul, T);
-      else blur_v_unit<CAND, 64, false>(S, ul, T);
+      if (shp.blocks > 1) blur_v_long<CAND>(S, lw, T, sh, acc);
+      else blur_v_short<CAND, 64>(S, lw, T, sh, acc);
       break;
-    case 32: blur_v_unit<CAND, 32, false>(S, ul, T); break;
-    case 16: blur_v_unit<CAND, 16, false>(S, ul, T); break;
-    case 8: blur_v_unit<CAND, 8, false>(S, ul, T); break;
-    case 4: blur_v_unit<CAND, 4, false>(S, ul, T); break;
-    default: blur_v_unit<CAND, 2, false>(S, ul, T); break;
+    case 32: blur_v_short<CAND, 32>(S, lw, T, sh, acc); break;
+    case 16: blur_v_short<CAND, 16>(S, lw, T, sh, acc); break;
+    case 8: blur_v_short<CAND, 8>(S, lw, T, sh, acc); break;
+    case 4: blur_v_short<CAND, 4>(S, lw, T, sh, acc); break;
+    default: blur_v_short<CAND, 2>(S, lw, T, sh, acc); break;
+  }
+  if constexpr (CAND != 0) {
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int t = 0; t < 6; t++) {
+      const double v = wave_sum(acc[t]);
+      if (lane == 0) S.partials[(size_t)(lw * kVWaves + wv) * 18 + t] = v;
+    }
   }
 }
 
@@ -880,7 +1022,12 @@ void launch_linear_at_depth(hipStream_t s, int w, int h, float* const in[3], con
 }
 
 int blur_bands(int w, int h) { return (int)div_up(h, kHWaves * line_shape(w).lpw); }
-int pool_units(int w, int h) { return (int)div_up(w, line_shape(h).lpw); }
+// Workgroups of the vertical pass over a w x h plane.
+int blur_columns(int w, int h) {
+  const LineShape s = line_shape(h);
+  return s.blocks > 1 ? w : (int)div_up(w, kVWaves * s.lpw);
+}
+int pool_units(int w, int h) { return kVWaves * blur_columns(w, h); }
 
 void launch_blur_h(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
                    int bands) {
@@ -890,11 +1037,10 @@ void launch_blur_h(hipStream_t s, int cand, const ConstTables* tab, const LineSe
 }
 
 void launch_blur_v(hipStream_t s, int cand, const ConstTables* tab, const LineSet* sets, const int* off, int nsets,
-                   int columns, int upw, int swizzle) {
-  if (columns <= 0) return;
-  upw = upw == 1 || upw == 2 ? upw : 4;
-  if (cand) k_blur_v<1><<<div_up(columns, upw), 64 * upw, 0, s>>>(tab, sets, off, nsets, columns, swizzle);
-  else k_blur_v<0><<<div_up(columns, upw), 64 * upw, 0, s>>>(tab, sets, off, nsets, columns, swizzle);
+                   int groups, int swizzle) {
+  if (groups <= 0) return;
+  if (cand) k_blur_v<1><<<groups, 64 * kVWaves, 0, s>>>(tab, sets, off, nsets, groups, swizzle);
+  else k_blur_v<0><<<groups, 64 * kVWaves, 0, s>>>(tab, sets, off, nsets, groups, swizzle);
 }
 
 void launch_reduce_scales(hipStream_t s, const PoolLayout& L, const double* partials, double* sums) {
