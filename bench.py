@@ -367,22 +367,21 @@ def serving_sizes(count: int, seed: int):
 def served_form(args) -> int:
     """The serving form: Poisson arrivals of quality requests (targets uniform
     over {70, 85, 90}, effort fast/balanced 50/50) at each swept rate, served
-    with continuous batching: arrived requests are admitted while their
-    projected HBM footprint fits, join the running rounds at the next step,
-    and leave with their plan downloaded.  Latency = completion - arrival,
-    including the upload of the request's pixels.  One JSON line."""
+    with continuous batching: arrived requests go to the server as host
+    samples, which admits them in arrival order while their projected HBM
+    footprint fits its budget; admitted requests join the running rounds at
+    the next step and leave with their plan downloaded.  Latency =
+    completion - arrival, including the upload of the request's pixels.  One
+    JSON line."""
     import multiprocessing as mp
 
     import jpxl_amd
-    from jpxl_amd import _lib
 
     pool_n = args.pool_images
     sizes = serving_sizes(pool_n, 5)
     with mp.get_context("fork").Pool(min(16, os.cpu_count() or 1)) as pool:
         rgbs = pool.map(_gen, [(w, h, 2000 + k) for k, (w, h) in enumerate(sizes)], chunksize=1)
     ctx = jpxl_amd.Context(0)
-    lib = ctx.lib
-    budget = 288e9 * 0.9 - sum(x.nbytes for x in rgbs)
     rates = [float(r) for r in args.rates.split(",")]
     out = {"metric": "served VarDCT encode latency and MP/s", "unit": "ms",
            "config": {"workload": f"Poisson arrivals, {args.requests} requests per rate, resolutions lognormal "
@@ -402,45 +401,39 @@ def served_form(args) -> int:
         targets = rng.choice([70.0, 85.0, 90.0], n)
         efforts = np.where(rng.uniform(size=n) < 0.5, "fast", "balanced")
         srv = jpxl_amd.Server(ctx)
-        live = {}            # ticket -> (request index, image, footprint)
+        live = {}            # ticket -> request index
         lat = np.zeros(n)
         mps = np.zeros(n)
         met = 0
-        used = 0.0
         nxt = 0
+        peak_queue = 0
         t0 = time.perf_counter()
         done = 0
         while done < n:
             now = time.perf_counter() - t0
-            # admit arrived requests in order while their footprint fits
+            # arrived requests go to the server, which admits them in order
+            # while their projected footprint fits its budget
             while nxt < n and arrive[nxt] <= now:
-                w, h = sizes[pick[nxt]]
-                fp = float(lib.jpxlg_image_footprint(w, h))
-                if used + fp > budget and live:
-                    break
-                img = jpxl_amd.Image(ctx, rgbs[pick[nxt]], 8)
-                t = srv.submit(img, float(targets[nxt]), str(efforts[nxt]))
-                live[t] = (nxt, img, fp)
-                used += fp
+                t = srv.submit_srgb(rgbs[pick[nxt]], float(targets[nxt]), str(efforts[nxt]), 8)
+                live[t] = nxt
                 nxt += 1
             if not live:
                 if nxt < n:
                     time.sleep(max(0.0, arrive[nxt] - (time.perf_counter() - t0)))
                 continue
             srv.step()
+            peak_queue = max(peak_queue, srv.admission()[2])
             for t in list(live):
                 res = srv.poll(t)
                 if res is None:
                     continue
-                k, img, fp = live.pop(t)
-                img.plan_download()  # the hand-off to the CPU entropy coder
+                k = live.pop(t)
+                srv.plan(t)  # the hand-off to the CPU entropy coder
                 lat[k] = (time.perf_counter() - t0) - arrive[k]
                 w, h = sizes[pick[k]]
                 mps[k] = w * h / 1e6
                 met += int(res.met)
                 srv.release(t)
-                img.close()
-                used -= fp
                 done += 1
         t_end = time.perf_counter() - t0
         srv.close()
@@ -452,6 +445,7 @@ def served_form(args) -> int:
             "mps": round(float(mps.sum()) / t_end, 2),
             "served_rate": round(n / t_end, 1),
             "met": int(met),
+            "peak_queued": int(peak_queue),
         }
     out["value"] = out["rates"][str(int(rates[0]))]["p99_ms"]
     print(json.dumps(out))

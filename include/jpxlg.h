@@ -309,17 +309,40 @@ void jpxlg_graph_destroy(jpxlg_graph* graph);
  * encoder serves one request per call, jpxl/src/lib.rs:507-794). */
 typedef struct jpxlg_server jpxlg_server;
 int32_t jpxlg_server_create(jpxlg_ctx* ctx, jpxlg_server** out);
+/* A request on an image the caller created (and keeps). */
 int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quality_request* req,
                             uint64_t* ticket);
-/* >= 0: requests completed by this round; < 0: JPXLG_E_*. */
+/* A request as interleaved host samples (the jpxlg_image_upload_srgb
+ * layout; rgb must stay valid until the request completes).  The server
+ * admits such requests in arrival order while the projected device
+ * footprint (jpxlg_image_footprint) of the admitted, unreleased ones fits
+ * its budget (one is always admitted when none holds memory); admitting
+ * creates the request's image, releasing it frees it.  Until admitted a
+ * request is pending and polls 0. */
+int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, uint32_t height, uint32_t bits,
+                                 const void* rgb, const jpxlg_quality_request* req, uint64_t* ticket);
+/* The admission budget in bytes (default: 90% of the device's memory). */
+int32_t jpxlg_server_set_budget(jpxlg_server* srv, uint64_t bytes);
+/* The admitted requests' projected footprint, the budget, and the requests
+ * waiting for admission (each pointer optional). */
+int32_t jpxlg_server_admission(jpxlg_server* srv, uint64_t* used, uint64_t* budget, uint32_t* queued);
+/* The image of a request (an admitted one, for a request given as samples),
+ * e.g. to download its chosen plan before releasing it. */
+int32_t jpxlg_server_image(jpxlg_server* srv, uint64_t ticket, jpxlg_img** out);
+/* >= 0: requests completed by this round; < 0: JPXLG_E_*.  Admission runs
+ * first. */
 int32_t jpxlg_server_step(jpxlg_server* srv);
-/* 1: done (result written), 0: running, < 0: the request failed / bad ticket. */
+/* 1: done (result written), 0: queued or running, < 0: the request failed /
+ * bad ticket. */
 int32_t jpxlg_server_poll(jpxlg_server* srv, uint64_t ticket, jpxlg_quality_result* out);
-/* Forgets a completed request (its image stays the caller's). */
+/* Forgets a completed request; its image stays the caller's, or, for a
+ * request given as samples, is freed (returning its footprint to the
+ * budget). */
 int32_t jpxlg_server_release(jpxlg_server* srv, uint64_t ticket);
-/* Requests not yet completed. */
+/* Requests not yet completed (queued or running). */
 int32_t jpxlg_server_pending(jpxlg_server* srv);
-/* Runs outstanding requests to completion, then frees the server. */
+/* Ends queued requests unrun (JPXLG_E_INVALID), runs the admitted ones to
+ * completion, then frees the server and the images it created. */
 void jpxlg_server_destroy(jpxlg_server* srv);
 
 /* ---- launch-shape autotuning ------------------------------------------- */

@@ -241,9 +241,25 @@ class Image:
         self.bw, self.bh, self.grayscale, self.scales = bx.value, by.value, bool(gray.value), scales.value
         self.tw, self.th = (self.bw + 7) // 8, (self.bh + 7) // 8
 
+    @classmethod
+    def _borrowed(cls, ctx: Context, handle) -> "Image":
+        """A view of an image another owner keeps (closing it frees nothing)."""
+        self = cls.__new__(cls)
+        self.handle = None
+        self.ctx = ctx
+        self._owner = False
+        bx, by, gray, scales = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_int32(), ctypes.c_uint32()
+        ctx.check(ctx.lib.jpxlg_image_info(handle, ctypes.byref(bx), ctypes.byref(by), ctypes.byref(gray),
+                                           ctypes.byref(scales)))
+        self.handle = handle
+        self.bw, self.bh, self.grayscale, self.scales = bx.value, by.value, bool(gray.value), scales.value
+        self.tw, self.th = (self.bw + 7) // 8, (self.bh + 7) // 8
+        self.width, self.height, self.bits = 8 * self.bw, 8 * self.bh, None
+        return self
+
     def close(self):
         if self.handle:
-            if self.ctx.handle:
+            if self.ctx.handle and getattr(self, "_owner", True):
                 self.ctx.lib.jpxlg_image_destroy(self.handle)
             self.handle = None
 
@@ -447,6 +463,7 @@ class Server:
 
     def __init__(self, ctx: Context):
         self.ctx = ctx
+        self._held = {}  # ticket -> host samples of a request given as samples
         h = ctypes.c_void_p()
         ctx.check(ctx.lib.jpxlg_server_create(ctx.handle, ctypes.byref(h)))
         self.handle = h
@@ -458,6 +475,41 @@ class Server:
         t = ctypes.c_uint64()
         self.ctx.check(self.ctx.lib.jpxlg_server_submit(self.handle, image.handle, ctypes.byref(req), ctypes.byref(t)))
         return t.value
+
+    def submit_srgb(self, rgb: np.ndarray, target: float, effort: str = "balanced", bits: int | None = None,
+                    max_probes: int = 0, mode: int = 0) -> int:
+        """A request as host samples: the server admits it (creating its image)
+        when its projected footprint fits the budget; the samples are held
+        until the request is released."""
+        rgb = np.ascontiguousarray(rgb)
+        height, width, three = rgb.shape
+        assert three == 3
+        if bits is None:
+            bits = 8 if rgb.dtype == np.uint8 else 16
+        rgb = rgb.astype(np.uint8 if bits == 8 else np.uint16, copy=False)
+        req = _lib.QualityRequest(float(target), _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED,
+                                  max_probes, mode)
+        t = ctypes.c_uint64()
+        self.ctx.check(self.ctx.lib.jpxlg_server_submit_srgb(self.handle, width, height, bits, _ptr(rgb),
+                                                             ctypes.byref(req), ctypes.byref(t)))
+        self._held[t.value] = rgb
+        return t.value
+
+    def set_budget(self, nbytes: int) -> None:
+        self.ctx.check(self.ctx.lib.jpxlg_server_set_budget(self.handle, int(nbytes)))
+
+    def admission(self) -> tuple[int, int, int]:
+        """(projected footprint of the admitted requests, budget, queued requests)."""
+        used, budget, queued = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+        self.ctx.check(self.ctx.lib.jpxlg_server_admission(self.handle, ctypes.byref(used), ctypes.byref(budget),
+                                                           ctypes.byref(queued)))
+        return used.value, budget.value, queued.value
+
+    def plan(self, ticket: int) -> "Plan":
+        """The chosen plan of a completed request (its image's quantized planes)."""
+        h = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.jpxlg_server_image(self.handle, ticket, ctypes.byref(h)))
+        return Image._borrowed(self.ctx, h).plan_download()
 
     def step(self) -> int:
         rc = self.ctx.lib.jpxlg_server_step(self.handle)
@@ -475,6 +527,7 @@ class Server:
 
     def release(self, ticket: int) -> None:
         self.ctx.check(self.ctx.lib.jpxlg_server_release(self.handle, ticket))
+        self._held.pop(ticket, None)
 
     def pending(self) -> int:
         return self.ctx.lib.jpxlg_server_pending(self.handle)
@@ -483,6 +536,7 @@ class Server:
         if self.handle:
             self.ctx.lib.jpxlg_server_destroy(self.handle)
             self.handle = None
+        self._held.clear()
 
     def __del__(self):
         try:

@@ -8,6 +8,7 @@
 #include <array>
 #include <chrono>
 #include <condition_variable>
+#include <deque>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -2000,6 +2001,13 @@ constexpr int kServerBootstrap = 8;
 
 struct ServerReq {
   jpxlg_img* img = nullptr;
+  // Requests submitted as host samples: the server creates (and destroys)
+  // their image when it admits (and releases) them.
+  bool owned = false;
+  bool admitted = true;
+  const void* rgb = nullptr;
+  uint32_t w = 0, h = 0, bits = 0;
+  size_t footprint = 0;
   jpxlg_quality_request q{};
   QualityPolicy pol{};
   std::thread th;
@@ -2037,6 +2045,11 @@ struct jpxlg_server {
   uint64_t next_id = 1;
   std::map<uint64_t, std::unique_ptr<ServerReq>> reqs;
   ServerLane lane[2];
+  // Admission of requests submitted as host samples: in arrival order while
+  // the projected device footprint of the admitted, unreleased ones fits the
+  // budget (one request is always admitted when none holds memory).
+  std::deque<uint64_t> queue;
+  size_t budget = 0, used = 0;
   ~jpxlg_server() {
     for (auto& l : lane)
       if (l.stream) {
@@ -2053,6 +2066,9 @@ static int server_init(jpxlg_server* srv, jpxlg_ctx* ctx) {
   srv->ctx = ctx;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   for (auto& l : srv->lane) HIP_TRY(ctx, hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
+  size_t free_b = 0, total_b = 0;
+  HIP_TRY(ctx, hipMemGetInfo(&free_b, &total_b));
+  srv->budget = total_b / 10 * 9;
   return JPXLG_OK;
 }
 
@@ -2065,23 +2081,9 @@ extern "C" int32_t jpxlg_server_create(jpxlg_ctx* ctx, jpxlg_server** out) {
   return JPXLG_OK;
 }
 
-extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quality_request* q,
-                                       uint64_t* ticket) {
-  if (!srv || !img || !q || !ticket) return JPXLG_E_INVALID;
-  if (img->ctx != srv->ctx) return fail(srv->ctx, JPXLG_E_INVALID, "the image belongs to another context");
-  auto r = std::make_unique<ServerReq>();
-  ServerReq* R = r.get();
-  R->img = img;
-  R->q = *q;
-  R->pol = quality_policy(q->effort);
-  if (q->mode != 1) {
-    HIP_TRY(srv->ctx, hipSetDevice(srv->ctx->device));
-    const int prc = predicted_rung_for(img, q->target, &R->predicted);
-    if (prc) return prc;
-  }
-  std::lock_guard<std::mutex> lk(srv->m);
-  const uint64_t id = srv->next_id++;
-  srv->reqs.emplace(id, std::move(r));
+// Starts the navigator of a request whose image is on the device (its
+// first guess needs the image's features); called with the lock held.
+static void server_start(jpxlg_server* srv, ServerReq* R) {
   srv->active++;
   R->th = std::thread([srv, R]() {
     ProbeFn probe = [srv, R](uint32_t rung, bool fresh, double* score, double* ms) -> int {
@@ -2108,8 +2110,84 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
     srv->active--;
     srv->cv_req.notify_all();
   });
+}
+
+extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quality_request* q,
+                                       uint64_t* ticket) {
+  if (!srv || !img || !q || !ticket) return JPXLG_E_INVALID;
+  if (img->ctx != srv->ctx) return fail(srv->ctx, JPXLG_E_INVALID, "the image belongs to another context");
+  auto r = std::make_unique<ServerReq>();
+  ServerReq* R = r.get();
+  R->img = img;
+  R->q = *q;
+  R->pol = quality_policy(q->effort);
+  if (q->mode != 1) {
+    HIP_TRY(srv->ctx, hipSetDevice(srv->ctx->device));
+    const int prc = predicted_rung_for(img, q->target, &R->predicted);
+    if (prc) return prc;
+  }
+  std::lock_guard<std::mutex> lk(srv->m);
+  const uint64_t id = srv->next_id++;
+  srv->reqs.emplace(id, std::move(r));
+  server_start(srv, R);
   *ticket = id;
   return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, uint32_t height, uint32_t bits,
+                                            const void* rgb, const jpxlg_quality_request* q, uint64_t* ticket) {
+  if (!srv || !rgb || !q || !ticket) return JPXLG_E_INVALID;
+  if (width < 8 || height < 8) return fail(srv->ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
+  if (width > 65535 * 8 || height > 65535 * 8) return fail(srv->ctx, JPXLG_E_UNSUPPORTED, "frame too large");
+  if (bits < 8 || bits > 16) return fail(srv->ctx, JPXLG_E_UNSUPPORTED, "bit depth outside 8..16");
+  auto r = std::make_unique<ServerReq>();
+  ServerReq* R = r.get();
+  R->owned = true;
+  R->admitted = false;
+  R->rgb = rgb;
+  R->w = width;
+  R->h = height;
+  R->bits = bits;
+  R->footprint = (size_t)jpxlg_image_footprint(width, height);
+  R->q = *q;
+  R->pol = quality_policy(q->effort);
+  std::lock_guard<std::mutex> lk(srv->m);
+  const uint64_t id = srv->next_id++;
+  srv->reqs.emplace(id, std::move(r));
+  srv->queue.push_back(id);
+  *ticket = id;
+  return JPXLG_OK;
+}
+
+// Admits queued requests in arrival order while their projected footprint
+// fits: each gets its image (the upload, XYB and metric reference) and its
+// navigator.  A request whose image cannot be created ends with that status.
+static void server_admit(jpxlg_server* srv) {
+  for (;;) {
+    ServerReq* R = nullptr;
+    {
+      std::lock_guard<std::mutex> lk(srv->m);
+      if (srv->queue.empty()) return;
+      ServerReq* F = srv->reqs.at(srv->queue.front()).get();
+      if (srv->used > 0 && srv->used + F->footprint > srv->budget) return;
+      srv->queue.pop_front();
+      srv->used += F->footprint;
+      R = F;
+    }
+    jpxlg_img* img = nullptr;
+    int rc = image_create(srv->ctx, R->w, R->h, R->bits, R->rgb, false, &img);
+    if (!rc && R->q.mode != 1) rc = predicted_rung_for(img, R->q.target, &R->predicted);
+    std::lock_guard<std::mutex> lk(srv->m);
+    R->img = img;
+    R->admitted = true;
+    if (rc) {
+      memset(&R->res, 0, sizeof(R->res));
+      R->nav_rc = rc;
+      R->finished = R->done = true;
+      continue;
+    }
+    server_start(srv, R);
+  }
 }
 
 // Hands a lane's finished round back to its requests' navigators.
@@ -2150,12 +2228,14 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
   if (!srv) return JPXLG_E_INVALID;
   jpxlg_ctx* ctx = srv->ctx;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
+  server_admit(srv);
   std::vector<ServerReq*> pending[2], finishing;
   {
     std::unique_lock<std::mutex> lk(srv->m);
     srv->cv_req.wait(lk, [srv] { return srv->waiting == srv->active; });
     for (auto& kv : srv->reqs) {
       ServerReq* R = kv.second.get();
+      if (!R->admitted) continue;
       if (R->pending && !R->in_lane) {
         const uint64_t px = (uint64_t)R->img->g.w * R->img->g.h;
         pending[px > kLargeLanePixels ? 1 : 0].push_back(R);
@@ -2329,6 +2409,37 @@ extern "C" int32_t jpxlg_server_release(jpxlg_server* srv, uint64_t ticket) {
     srv->reqs.erase(it);
   }
   if (r->th.joinable()) r->th.join();
+  if (r->owned) {
+    if (r->img) jpxlg_image_destroy(r->img);
+    std::lock_guard<std::mutex> lk(srv->m);
+    srv->used -= r->footprint;
+  }
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_image(jpxlg_server* srv, uint64_t ticket, jpxlg_img** out) {
+  if (!srv || !out) return JPXLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->m);
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end()) return fail(srv->ctx, JPXLG_E_INVALID, "unknown ticket");
+  if (!it->second->img) return fail(srv->ctx, JPXLG_E_INVALID, "the request is not admitted yet");
+  *out = it->second->img;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_set_budget(jpxlg_server* srv, uint64_t bytes) {
+  if (!srv) return JPXLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->m);
+  srv->budget = (size_t)bytes;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_admission(jpxlg_server* srv, uint64_t* used, uint64_t* budget, uint32_t* queued) {
+  if (!srv) return JPXLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->m);
+  if (used) *used = srv->used;
+  if (budget) *budget = srv->budget;
+  if (queued) *queued = (uint32_t)srv->queue.size();
   return JPXLG_OK;
 }
 
@@ -2379,10 +2490,23 @@ static int server_drain(jpxlg_server* srv) {
 
 extern "C" void jpxlg_server_destroy(jpxlg_server* srv) {
   if (!srv) return;
-  // Drive every outstanding request to completion before the threads go.
+  // Requests never admitted end unrun; every admitted one is driven to
+  // completion before the threads go.
+  {
+    std::lock_guard<std::mutex> lk(srv->m);
+    for (uint64_t id : srv->queue) {
+      ServerReq* R = srv->reqs.at(id).get();
+      memset(&R->res, 0, sizeof(R->res));
+      R->nav_rc = JPXLG_E_INVALID;
+      R->admitted = R->finished = R->done = true;
+    }
+    srv->queue.clear();
+  }
   (void)server_drain(srv);
-  for (auto& kv : srv->reqs)
+  for (auto& kv : srv->reqs) {
     if (kv.second->th.joinable()) kv.second->th.join();
+    if (kv.second->owned && kv.second->img) jpxlg_image_destroy(kv.second->img);
+  }
   delete srv;
 }
 

@@ -206,3 +206,54 @@ def test_served_mixed_requests_equal_each_served_alone(ctx):
         assert (r.rung, r.probes, r.score, r.status) == alone[k], k
     for im in imgs:
         im.close()
+
+
+def test_server_admission_queues_within_budget(ctx):
+    """Requests given as host samples under a budget of about two requests'
+    projected footprint: the server queues the rest (queued > 0 seen), the
+    admitted footprint never exceeds the budget unless one request alone
+    holds memory, each released request returns its footprint, and every
+    result (and the chosen plan of every other request) equals the request
+    encoded alone."""
+    import jpxl_amd
+
+    rng = np.random.default_rng(31)
+    reqs = []
+    for k in range(8):
+        w, h = (int(v) for v in rng.integers(200, 900, 2))
+        reqs.append((_rgb(w, h, 300 + k), float(rng.choice([70, 85, 90])), str(rng.choice(["fast", "balanced"]))))
+    fps = [int(ctx.lib.jpxlg_image_footprint(x.shape[1], x.shape[0])) for x, _, _ in reqs]
+    alone = []
+    for x, t, e in reqs:
+        im = jpxl_amd.Image(ctx, x, 8)
+        r = im.encode_quality(t, e)
+        alone.append(((r.rung, r.probes, r.score, r.status), im.plan_download().quant))
+        im.close()
+    budget = int(2.2 * float(np.median(fps)))
+    srv = jpxl_amd.Server(ctx)
+    srv.set_budget(budget)
+    tickets = [srv.submit_srgb(x, t, e) for x, t, e in reqs]
+    used, b, queued = srv.admission()
+    assert (used, b, queued) == (0, budget, len(reqs))
+    done, saw_queue = {}, False
+    while len(done) < len(reqs):
+        srv.step()
+        used, _, queued = srv.admission()
+        saw_queue |= queued > 0
+        live = [k for k in range(len(reqs)) if k not in done]
+        # admission is in arrival order: the queued requests are the last ones
+        assert used == sum(fps[k] for k in live[:len(live) - queued])
+        assert used <= budget or len(live) - queued == 1
+        for k in live:
+            r = srv.poll(tickets[k])
+            if r is not None:
+                done[k] = r
+                if k % 2 == 0:
+                    assert np.array_equal(srv.plan(tickets[k]).quant, alone[k][1]), k
+                srv.release(tickets[k])
+    assert saw_queue
+    assert srv.admission()[:1] == (0,) and srv.admission()[2] == 0
+    srv.close()
+    for k in range(len(reqs)):
+        r = done[k]
+        assert (r.rung, r.probes, r.score, r.status) == alone[k][0], k
