@@ -264,6 +264,70 @@ typedef struct {
 
 int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
                              jpxlg_quality_result* out);
+/* ---- first-guess features and the crossing predictor ---------------------- */
+/* SourceFeatures (jpxl-encode-policy/src/quality_features.rs:21-47) with the
+ * PreanalysisFeatures it carries (preanalysis.rs:99-137): the atlas's
+ * per-atom variance quantiles, and the 32x32-cell structure statistics of
+ * the source's 8-bit samples (has_preanalysis = 0 without a whole cell). */
+typedef struct {
+  uint32_t width, height;
+  int32_t grayscale;
+  float luma_variance_q10, luma_variance_q50, luma_variance_q90;
+  float chroma_variance_q50, flat_fraction, edge_proxy;
+  int32_t has_preanalysis;
+  uint32_t cells;
+  float share[7];  /* Flat, Gradient, ChromaCritical, Noisy, Texture, DirectionalEdge, TextLike */
+  float log2var_mean, log2var_std, log2var_q10, log2var_q90;
+  float edge_mean, edge_q90, flat_mean, noise_mean, noise_q90;
+  float chroma_log2_mean, chroma_log2_q90;
+  float orient_entropy_mean, dir_dominance_mean, axis_aligned_mean, hetero_frac;
+} jpxlg_source_features;
+
+/* TransformFeatureSummary (quality_features.rs:93-121) of the frame's
+ * aligned DCT8x8 coefficients. */
+typedef struct {
+  uint64_t blocks;
+  double ln_ac_y_mean, ln_ac_y_q50, ln_ac_y_q90, ln_ac_y_q99;
+  double high_low_ratio, directional_asymmetry, chroma_ac_ratio;
+  double near_zero_frac_1e3, near_zero_frac_1e2, dc_variance_y;
+} jpxlg_transform_summary;
+
+/* QualityPredictionV2 (quality_prediction.rs:48-77).  ood_flags: bit i for
+ * feature i of the model's vector (the qpv2-stp/1 order of
+ * quality_prediction.rs:373-385), bit 62 tiny_frame, bit 63 case_spread. */
+#define JPXLG_OOD_TINY_FRAME (1ull << 62)
+#define JPXLG_OOD_CASE_SPREAD (1ull << 63)
+#define JPXLG_FALLBACK_NONE 0
+#define JPXLG_FALLBACK_OOD_FEATURE 1
+#define JPXLG_FALLBACK_WIDE_INTERVAL 2
+#define JPXLG_FALLBACK_SATURATION_RISK 3
+#define JPXLG_PREDICTOR_DISPATCH 0  /* quality_prediction::predict (case table, else knot model) */
+#define JPXLG_PREDICTOR_CASES 1     /* predict_cases */
+#define JPXLG_PREDICTOR_KNOTS 2     /* predict_v2 */
+typedef struct {
+  uint32_t median_rung, candidate_rung, interval_low, interval_high;
+  double local_loss_exponent, saturation_risk;
+  uint64_t ood_flags;
+  int32_t fallback;             /* JPXLG_FALLBACK_* */
+  int32_t large_frame_only_ood;
+  int32_t model;                /* JPXLG_PREDICTOR_CASES or _KNOTS: the table that answered */
+} jpxlg_crossing_prediction;
+
+/* The image's source features and DCT8 summary, as the quality search reads
+ * them (quality.rs:2449-2474; source_features quality_features.rs:173-213,
+ * preanalysis_srgb8/16 preanalysis.rs:542-574, quality_transform_summary
+ * jpxl-encode-policy/src/lib.rs:2769-2955).  Either pointer may be null. */
+int32_t jpxlg_image_quality_features(jpxlg_img* img, jpxlg_source_features* source,
+                                     jpxlg_transform_summary* transform);
+/* quality_prediction::predict / predict_cases / predict_v2
+ * (quality_prediction.rs:351-367, :453-598, :234-305) by `which`
+ * (JPXLG_PREDICTOR_*), reserve = the effort's loss-relative reserve.
+ * Returns 1 with *out filled, 0 when the chosen table gives no prediction
+ * (predict_cases on a frame without preanalysis), or a negative status.
+ * Host-only: needs no device. */
+int32_t jpxlg_predict_crossing(const jpxlg_source_features* source, const jpxlg_transform_summary* transform,
+                               double target, double reserve, int32_t which, jpxlg_crossing_prediction* out);
+
 /* The quantizer a quality search of `effort` probes at ladder rung `rung`
  * (QuantizerChoice::at, jpxl-encode-policy/src/quantizer_ladder.rs:202-260,
  * over EncodeRequest::for_quality, request.rs:955-972). */

@@ -320,6 +320,13 @@ class Image:
             terms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
         return score.value, raw.value, terms[: n.value]
 
+    def quality_features(self):
+        """(SourceFeatures, TransformSummary) the quality search's first
+        guess reads (ctypes structs)."""
+        sf, ts = _lib.SourceFeatures(), _lib.TransformSummary()
+        self.ctx.check(self.ctx.lib.jpxlg_image_quality_features(self.handle, ctypes.byref(sf), ctypes.byref(ts)))
+        return sf, ts
+
     def plan_download(self) -> Plan:
         q = np.empty((3, 8 * self.bh, 8 * self.bw), np.int32)
         lf = np.empty((3, self.bh, self.bw), np.int32)
@@ -383,6 +390,17 @@ class Image:
         res = _lib.QualityResult()
         self.ctx.check(self.ctx.lib.jpxlg_encode_quality(self.handle, ctypes.byref(req), ctypes.byref(res)))
         return res
+
+
+def predict_crossing(source, transform, target: float, reserve: float, which: int = 0):
+    """quality_prediction::predict (which 0), predict_cases (1) or
+    predict_v2 (2): a CrossingPrediction, or None.  Host-only."""
+    out = _lib.CrossingPrediction()
+    rc = _lib.load().jpxlg_predict_crossing(ctypes.byref(source), ctypes.byref(transform), float(target),
+                                            float(reserve), int(which), ctypes.byref(out))
+    if rc < 0:
+        raise JpxlgError(rc, "jpxlg_predict_crossing")
+    return out if rc == 1 else None
 
 
 def _image_array(images):

@@ -92,6 +92,44 @@ class Varblock(ctypes.Structure):
     ]
 
 
+class SourceFeatures(ctypes.Structure):
+    """jpxlg_source_features (SourceFeatures + PreanalysisFeatures)."""
+
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("grayscale", ctypes.c_int32)] + [
+        (n, ctypes.c_float) for n in ("luma_variance_q10", "luma_variance_q50", "luma_variance_q90",
+                                      "chroma_variance_q50", "flat_fraction", "edge_proxy")] + [
+        ("has_preanalysis", ctypes.c_int32), ("cells", ctypes.c_uint32), ("share", ctypes.c_float * 7)] + [
+        (n, ctypes.c_float) for n in ("log2var_mean", "log2var_std", "log2var_q10", "log2var_q90", "edge_mean",
+                                      "edge_q90", "flat_mean", "noise_mean", "noise_q90", "chroma_log2_mean",
+                                      "chroma_log2_q90", "orient_entropy_mean", "dir_dominance_mean",
+                                      "axis_aligned_mean", "hetero_frac")]
+
+
+class TransformSummary(ctypes.Structure):
+    """jpxlg_transform_summary (TransformFeatureSummary)."""
+
+    _fields_ = [("blocks", ctypes.c_uint64)] + [
+        (n, ctypes.c_double) for n in ("ln_ac_y_mean", "ln_ac_y_q50", "ln_ac_y_q90", "ln_ac_y_q99",
+                                       "high_low_ratio", "directional_asymmetry", "chroma_ac_ratio",
+                                       "near_zero_frac_1e3", "near_zero_frac_1e2", "dc_variance_y")]
+
+
+class CrossingPrediction(ctypes.Structure):
+    """jpxlg_crossing_prediction (QualityPredictionV2)."""
+
+    _fields_ = [("median_rung", ctypes.c_uint32), ("candidate_rung", ctypes.c_uint32),
+                ("interval_low", ctypes.c_uint32), ("interval_high", ctypes.c_uint32),
+                ("local_loss_exponent", ctypes.c_double), ("saturation_risk", ctypes.c_double),
+                ("ood_flags", ctypes.c_uint64), ("fallback", ctypes.c_int32),
+                ("large_frame_only_ood", ctypes.c_int32), ("model", ctypes.c_int32)]
+
+
+OOD_TINY_FRAME = 1 << 62
+OOD_CASE_SPREAD = 1 << 63
+FALLBACK_NONE, FALLBACK_OOD_FEATURE, FALLBACK_WIDE_INTERVAL, FALLBACK_SATURATION_RISK = 0, 1, 2, 3
+PREDICTOR_DISPATCH, PREDICTOR_CASES, PREDICTOR_KNOTS = 0, 1, 2
+
+
 class QualityResult(ctypes.Structure):
     """jpxlg_quality_result."""
 
@@ -155,6 +193,9 @@ SIGNATURES = {
     "jpxlg_server_admission": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _u32p]),
     "jpxlg_server_image": (_i32, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "jpxlg_server_step": (_i32, [_vp]),
+    "jpxlg_image_quality_features": (_i32, [_vp, ctypes.POINTER(SourceFeatures), ctypes.POINTER(TransformSummary)]),
+    "jpxlg_predict_crossing": (_i32, [ctypes.POINTER(SourceFeatures), ctypes.POINTER(TransformSummary), ctypes.c_double, ctypes.c_double,
+                                      _i32, ctypes.POINTER(CrossingPrediction)]),
     "jpxlg_server_poll": (_i32, [_vp, ctypes.c_uint64, ctypes.POINTER(QualityResult)]),
     "jpxlg_server_release": (_i32, [_vp, ctypes.c_uint64]),
     "jpxlg_server_pending": (_i32, [_vp]),

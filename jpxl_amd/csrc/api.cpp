@@ -278,6 +278,8 @@ size_t image_bytes(int w, int h) {
   b += bw * bh * 4 + 256;             // AQ field + scratch
   b += bw * bh * 4;                   // a caller-supplied plan's per-varblock HfMul
   b += 64 + 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + bw * bh + 256;  // structure readback block
+  b += (size_t)preanalysis_cells(w, h) * sizeof(CellStats);  // first-guess features: cell sums,
+  b += (bw * bh / (8 * 16) + 2) * sizeof(SummaryPartial) + kSummaryBins * 4;  // DCT8 summary partials
   b += 512 * 256;  // alignment of every carved buffer
   return b;
 }
@@ -346,10 +348,15 @@ struct jpxlg_img {
   int aq_mode = kAqOff;
   bool aq_active = false;
   bool preconditioned = false;
-  // Frame features of the first-guess predictor (quality_features.rs:173-213),
-  // computed on first use from the source planes.
+  // Frame features of the first-guess predictors (quality_features.rs,
+  // preanalysis.rs, the DCT8 summary of lib.rs:2769), computed on first use
+  // from the source planes and samples.
   bool features_known = false;
-  float luma_q50 = 0.0f, flat_fraction = 0.0f;
+  jpxlg_source_features sf{};
+  jpxlg_transform_summary ts{};
+  CellStats* d_cells = nullptr;
+  SummaryPartial* d_sum_parts = nullptr;
+  uint32_t* d_hist = nullptr;
   // Bumped whenever the varblock lists, the DCT form or the source planes
   // change: a captured chain is valid only for the generation it saw.
   uint64_t generation = 0;
@@ -578,6 +585,10 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   img->aq_adj = (float*)A.take(bpx * 4);
   img->d_aq = (AqScratch*)A.take(sizeof(AqScratch));
   img->d_vb_mul = (uint32_t*)A.take(bpx * 4);
+  const int ncells = preanalysis_cells(g.w, g.h);
+  img->d_cells = ncells ? (CellStats*)A.take((size_t)ncells * sizeof(CellStats)) : nullptr;
+  img->d_sum_parts = (SummaryPartial*)A.take((size_t)summary_groups(g) * sizeof(SummaryPartial));
+  img->d_hist = (uint32_t*)A.take(kSummaryBins * 4);
   size_t punits = 0;
   for (int s = 0; s < img->nscales; s++) punits += (size_t)pool_units(img->sw[s], img->sh[s]);
   img->d_partials = (double*)A.take(punits * 18 * 8);
@@ -585,7 +596,8 @@ static int image_create(jpxlg_ctx* ctx, uint32_t width, uint32_t height, uint32_
   bool carved = img->d_cost[2] && img->d_rad[2] && img->d_cover_queue && img->d_lf_s && img->d_hf_s && img->d_hf_cell &&
                 img->d_si && img->d_lf_off && img->d_hf_local && img->d_tile_count && img->d_readback && img->d_tile_base && img->d_rows && img->d_rgb && img->sigma && img->d_bmap && img->d_cfl_x && img->d_cfl_b &&
                 img->d_cflp && img->d_lfc && img->d_pp && img->d_err && img->d_gray && img->d_partials && img->d_sums &&
-                img->aq_adj && img->d_aq && img->d_vb_mul && img->d_ref_sets;
+                img->aq_adj && img->d_aq && img->d_vb_mul && img->d_ref_sets && img->d_sum_parts && img->d_hist &&
+                (img->d_cells || ncells == 0);
   for (int c = 0; c < 3; c++) {
     carved = carved && img->xyb[c] && img->coeff[c] && img->quant[c] && img->lf[c] &&
              img->d_list[c] && img->rx[c] && img->aux[c];
@@ -1662,36 +1674,84 @@ extern "C" int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* hos
 
 // The frame features of the first-guess predictor: every atom's Y variance
 // on the device, the median and the flat fraction on the host.
+// The first-guess features (quality.rs:2449-2474): per-atom variances of
+// the XYB planes (quality_features.rs:173-213), the 32x32 preanalysis cells
+// of the source samples (preanalysis.rs:277-484) and the DCT8 summary
+// (lib.rs:2769-2955) on the device; their order statistics and reductions on
+// the host.  Computed once per image (the source planes do not change).
 static int image_features(jpxlg_img* img) {
   if (img->features_known) return JPXLG_OK;
   jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
   const size_t bpx = (size_t)g.bw * g.bh;
+  const int ncells = preanalysis_cells(g.w, g.h), nparts = summary_groups(g);
   hipStream_t st = ctx->stream;
-  float* d_var = (float*)img->d_cost[0];  // cover scratch, free between structure builds
-  launch_atlas_luma_var(st, g, img->xyb[1], d_var);
+  // cover scratch, free between structure builds
+  float* d_luma = (float*)img->d_cost[0];
+  float* d_chroma = (float*)img->d_cost[1];
+  launch_atlas_var(st, g, img->xyb, d_luma, d_chroma);
+  launch_preanalysis_cells(st, g.w, g.h, img->bits, img->d_rgb, img->d_cells);
+  launch_dct8_summary(st, g, ctx->d_tab, img->xyb, img->d_sum_parts, img->d_hist);
   HIP_TRY(ctx, hipGetLastError());
-  if (ctx->rb_cap < bpx * 4) {
+  const size_t o_chroma = bpx * 4, o_cells = 2 * bpx * 4;
+  const size_t o_parts = o_cells + (size_t)ncells * sizeof(CellStats);
+  const size_t o_hist = o_parts + (size_t)nparts * sizeof(SummaryPartial);
+  const size_t need = o_hist + kSummaryBins * 4;
+  if (ctx->rb_cap < need) {
     if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
     ctx->rb_host = nullptr;
     ctx->rb_cap = 0;
-    if (hipHostMalloc((void**)&ctx->rb_host, bpx * 4, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc((void**)&ctx->rb_host, need, hipHostMallocDefault) != hipSuccess)
       return fail(ctx, JPXLG_E_OOM, "readback staging allocation failed");
-    ctx->rb_cap = bpx * 4;
+    ctx->rb_cap = need;
   }
-  HIP_TRY(ctx, hipMemcpyAsync(ctx->rb_host, d_var, bpx * 4, hipMemcpyDeviceToHost, st));
+  char* h = (char*)ctx->rb_host;
+  HIP_TRY(ctx, hipMemcpyAsync(h, d_luma, bpx * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(h + o_chroma, d_chroma, bpx * 4, hipMemcpyDeviceToHost, st));
+  if (ncells)
+    HIP_TRY(ctx, hipMemcpyAsync(h + o_cells, img->d_cells, (size_t)ncells * sizeof(CellStats),
+                                hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(h + o_parts, img->d_sum_parts, (size_t)nparts * sizeof(SummaryPartial),
+                              hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(h + o_hist, img->d_hist, kSummaryBins * 4, hipMemcpyDeviceToHost, st));
+  if (!img->gray_known) HIP_TRY(ctx, hipMemcpyAsync(&img->gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
-  std::vector<float> var((const float*)ctx->rb_host, (const float*)ctx->rb_host + bpx);
-  source_features(std::move(var), &img->luma_q50, &img->flat_fraction);
+  img->gray_known = true;
+  jpxlg_source_features& f = img->sf;
+  f = jpxlg_source_features{};
+  f.width = (uint32_t)g.w;
+  f.height = (uint32_t)g.h;
+  f.grayscale = img->gray ? 1 : 0;
+  source_features_from(std::vector<float>((const float*)h, (const float*)h + bpx),
+                       std::vector<float>((const float*)(h + o_chroma), (const float*)(h + o_chroma) + bpx), &f);
+  preanalysis_from_cells((const CellStats*)(h + o_cells), ncells, &f);
+  transform_summary_from((const SummaryPartial*)(h + o_parts), nparts, (const uint32_t*)(h + o_hist), &img->ts);
   img->features_known = true;
   return JPXLG_OK;
 }
 
-// The legacy predictor's first rung for `target` (quality.rs:2452-2460).
-static int predicted_rung_for(jpxlg_img* img, double target, uint32_t* rung) {
+// The first probe and the navigator's prior exponent (quality.rs:2449-2530):
+// the crossing predictor's rung where it is trusted, else the legacy
+// table's (quality.rs:2452-2460).
+static int predicted_start(jpxlg_img* img, double target, int32_t effort, uint32_t* rung, double* prior_beta) {
   const int rc = image_features(img);
   if (rc) return rc;
-  *rung = rung_for_scale(predicted_effective_scale(img->luma_q50, img->flat_fraction, target));
+  const uint32_t legacy =
+      rung_for_scale(predicted_effective_scale(img->sf.luma_variance_q50, img->sf.flat_fraction, target));
+  const QualityStart q = quality_start(img->sf, img->ts, target, quality_policy(effort).reserve, legacy);
+  *rung = q.rung;
+  *prior_beta = q.prior_beta;
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_image_quality_features(jpxlg_img* img, jpxlg_source_features* source,
+                                                jpxlg_transform_summary* transform) {
+  if (!img) return JPXLG_E_INVALID;
+  HIP_TRY(img->ctx, hipSetDevice(img->ctx->device));
+  const int rc = image_features(img);
+  if (rc) return rc;
+  if (source) *source = img->sf;
+  if (transform) *transform = img->ts;
   return JPXLG_OK;
 }
 
@@ -1781,7 +1841,8 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
   memset(out, 0, sizeof(*out));
   const QualityPolicy pol = quality_policy(req->effort);
   uint32_t predicted = 0;
-  int rc = predicted_rung_for(img, req->target, &predicted);
+  double prior_beta = NAN;
+  int rc = predicted_start(img, req->target, req->effort, &predicted, &prior_beta);
   if (rc) return rc;
   SearchStructure ss;
   ProbeFn probe = [&](uint32_t rung, bool fresh, double* score, double* ms) -> int {
@@ -1798,7 +1859,8 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
   };
   ControllerOutcome oc;
   rc = req->mode == 1 ? bisect_controller(req->target, req->max_probes, probe, &oc)
-                      : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc, predicted);
+                      : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc, predicted,
+                                            prior_beta);
   if (rc) return rc;
   // Leave the chosen plan quantized on the device for the hand-off, on the
   // structure it was scored with (the last probe's levels usually are).
@@ -2012,7 +2074,8 @@ struct ServerReq {
   QualityPolicy pol{};
   std::thread th;
   SearchStructure ss;     // the structure its image holds (anchor or a rebuilt finalist's)
-  uint32_t predicted = 0; // the legacy predictor's first rung
+  uint32_t predicted = 0; // the first probe's rung
+  double prior_beta = NAN; // the crossing predictor's loss exponent, when trusted
   bool pending = false;
   uint32_t rung = 0;
   bool fresh = false;     // the pending probe builds its own structure
@@ -2102,7 +2165,7 @@ static void server_start(jpxlg_server* srv, ServerReq* R) {
     ControllerOutcome oc;
     const int rc = R->q.mode == 1 ? bisect_controller(R->q.target, R->q.max_probes, probe, &oc)
                                   : navigate_controller(R->q.target, R->q.effort, R->q.max_probes, probe, &oc,
-                                                        R->predicted);
+                                                        R->predicted, R->prior_beta);
     std::lock_guard<std::mutex> lk(srv->m);
     R->oc = oc;
     R->nav_rc = rc;
@@ -2123,7 +2186,7 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
   R->pol = quality_policy(q->effort);
   if (q->mode != 1) {
     HIP_TRY(srv->ctx, hipSetDevice(srv->ctx->device));
-    const int prc = predicted_rung_for(img, q->target, &R->predicted);
+    const int prc = predicted_start(img, q->target, q->effort, &R->predicted, &R->prior_beta);
     if (prc) return prc;
   }
   std::lock_guard<std::mutex> lk(srv->m);
@@ -2176,7 +2239,7 @@ static void server_admit(jpxlg_server* srv) {
     }
     jpxlg_img* img = nullptr;
     int rc = image_create(srv->ctx, R->w, R->h, R->bits, R->rgb, false, &img);
-    if (!rc && R->q.mode != 1) rc = predicted_rung_for(img, R->q.target, &R->predicted);
+    if (!rc && R->q.mode != 1) rc = predicted_start(img, R->q.target, R->q.effort, &R->predicted, &R->prior_beta);
     std::lock_guard<std::mutex> lk(srv->m);
     R->img = img;
     R->admitted = true;

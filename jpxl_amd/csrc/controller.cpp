@@ -59,10 +59,11 @@ struct Point {
   double score;
 };
 
-// quality.rs:933-988 extrapolated_step_from (no model prior: the prior
-// exponent stands in while only one point exists).
-bool extrapolated_step_from(std::vector<Point> pts, double threshold, bool finer, uint32_t attempt,
-                            uint32_t* out) {
+// quality.rs:933-988 extrapolated_step_from: the crossing predictor's loss
+// exponent (prior_beta, NaN when none) or else PRIOR_LOSS_EXPONENT stands in
+// while no measured slope fits.
+bool extrapolated_step_from(std::vector<Point> pts, double threshold, double prior_beta, bool finer,
+                            uint32_t attempt, uint32_t* out) {
   if (pts.empty()) return false;
   std::stable_sort(pts.begin(), pts.end(), [](const Point& a, const Point& b) { return a.rung < b.rung; });
   Point from;
@@ -76,7 +77,7 @@ bool extrapolated_step_from(std::vector<Point> pts, double threshold, bool finer
     if (pts.size() >= 2) { other = pts[1]; has_other = true; }
   }
   if ((finer && from.rung == Rung::top()) || (!finer && from.rung == 0)) return false;
-  double alpha = kPriorLossExponent;
+  double alpha = std::isnan(prior_beta) ? kPriorLossExponent : prior_beta;
   if (has_other) {
     double x0 = ln_scale(other.rung), y0 = neg_ln_loss(other.score);
     double x1 = ln_scale(from.rung), y1 = neg_ln_loss(from.score);
@@ -357,7 +358,7 @@ void source_features(std::vector<float> luma, float* luma_q50, float* flat_fract
 }
 
 int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& probe, ControllerOutcome* out,
-                         uint32_t predicted_rung) {
+                         uint32_t predicted_rung, double prior_beta) {
   const double threshold = target;  // DEFAULT_SCORE_GUARD = 0
   const double band = met_band(threshold, budget.reserve);
   std::vector<ProbeResult>& probes = out->probes;
@@ -448,7 +449,7 @@ int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& 
     while (!bracket(&lo, &hi) && budget_left()) {
       const bool finer = !probes.back().feasible;
       uint32_t next;
-      if (!extrapolated_step_from(points(), threshold, finer, attempt, &next)) break;
+      if (!extrapolated_step_from(points(), threshold, prior_beta, finer, attempt, &next)) break;
       if (already(next)) break;
       if ((rc = run(next, false, nullptr))) return rc;
       attempt++;
@@ -471,7 +472,7 @@ int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& 
     bool any_feasible = false;
     for (auto& p : probes) any_feasible |= p.feasible;
     uint32_t next;
-    if (!any_feasible && extrapolated_step_from(points(), threshold, true, 3, &next) && !already(next))
+    if (!any_feasible && extrapolated_step_from(points(), threshold, prior_beta, true, 3, &next) && !already(next))
       if ((rc = run(next, false, nullptr))) return rc;
   }
 
@@ -551,10 +552,10 @@ int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& 
 }
 
 int navigate_controller(double target, int32_t effort, int max_probes, const ProbeFn& probe,
-                        ControllerOutcome* out, uint32_t predicted_rung) {
+                        ControllerOutcome* out, uint32_t predicted_rung, double prior_beta) {
   const QualityPolicy pol = quality_policy(effort);
   NavBudget b{max_probes > 0 ? max_probes : pol.pixel_probes, pol.exact_prices, pol.structural_builds, pol.reserve};
-  return navigate_with_budget(target, b, probe, out, predicted_rung);
+  return navigate_with_budget(target, b, probe, out, predicted_rung, prior_beta);
 }
 
 int bisect_controller(double target, int max_probes, const ProbeFn& probe, ControllerOutcome* out) {

@@ -109,10 +109,12 @@ int bisect_controller(double target, int max_probes, const ProbeFn& probe, Contr
 // needs the entropy coder, which this library leaves to the caller: among
 // the feasible finalists the coarsest rung is chosen (the smallest stream at
 // equal structure).
+// prior_beta: the crossing predictor's local loss exponent, which replaces
+// PRIOR_LOSS_EXPONENT in the slope-extrapolated steps (NaN: none).
 int navigate_controller(double target, int32_t effort, int max_probes, const ProbeFn& probe,
-                        ControllerOutcome* out, uint32_t predicted_rung);
+                        ControllerOutcome* out, uint32_t predicted_rung, double prior_beta = NAN);
 int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& probe, ControllerOutcome* out,
-                         uint32_t predicted_rung);
+                         uint32_t predicted_rung, double prior_beta = NAN);
 
 // The legacy initial-quantizer predictor (quality.rs:1021-1079 over the
 // table of quality_predictor.rs): the effective scale the calibration

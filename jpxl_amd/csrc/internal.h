@@ -8,6 +8,7 @@
 
 #include "dct_matrix.h"
 #include "device_math.h"
+#include "predictor.h"
 
 namespace jpxlg {
 
@@ -362,6 +363,17 @@ void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3]
                      AqScratch* scratch, float* adj);
 // Each listed varblock's HfMul at its top-left block.
 void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var);
+// Per-atom Y variance, and X + B variance (the source features' inputs).
+void launch_atlas_var(hipStream_t s, const Geom& g, float* const xyb[3], float* luma, float* chroma);
+// Whole 32x32 preanalysis cells of a w x h frame, and their integer sums
+// from the source samples (8-bit, or 9..16-bit reduced to their top 8 bits).
+int preanalysis_cells(int w, int h);
+void launch_preanalysis_cells(hipStream_t s, int w, int h, int bits, const void* rgb, CellStats* out);
+// The quality predictor's DCT8 summary: summary_groups(g) partials and the
+// kSummaryBins-bin histogram (cleared first).
+int summary_groups(const Geom& g);
+void launch_dct8_summary(hipStream_t s, const Geom& g, const ConstTables* tab, float* const xyb[3],
+                         SummaryPartial* parts, uint32_t* hist);
 // The QuantizedFrameIr arena (every varblock's X, Y, B coefficients in LF
 // group then block-raster order) and the per-group LF planes.
 void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3],

@@ -256,6 +256,129 @@ __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __
   }
 }
 
+// The DCT8 summary of the quality predictor (quality_transform_summary,
+// jpxl-encode-policy/src/lib.rs:2769-2874): every aligned 8x8 block of the
+// XYB planes (edge blocks replicate the last column and row, as the cover's
+// candidates do) through the exact DCT8, then per block, in cell order, its
+// Y AC energy, low/high/row/column band energies, near-zero counts, DC and
+// X+B AC energy.  A workgroup runs kSummaryRounds rounds of 8 consecutive
+// raster blocks and keeps their running sums in block order; the histogram
+// of ln(Y AC energy) bins is an integer count.
+constexpr int kSummaryRounds = 16;
+
+__global__ __launch_bounds__(kThreads) void k_dct8_summary(Geom g, const ConstTables* __restrict__ tab,
+                                                           const float* __restrict__ X,
+                                                           const float* __restrict__ Y,
+                                                           const float* __restrict__ B,
+                                                           SummaryPartial* __restrict__ parts,
+                                                           uint32_t* __restrict__ hist) {
+  constexpr int N = 8, VB = 8, P = N + 1, NN = 64;
+  __shared__ float lds[VB * 3 * N * P];
+  __shared__ float coef[VB][3][NN];
+  __shared__ double bsum[VB][7];
+  __shared__ uint32_t bcnt[VB][2];
+  const float* planes[3] = {X, Y, B};
+  const int nblocks = g.bw * g.bh;
+  const DctTables& t = tab->dct;
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint32_t n3 = 0, n2 = 0, nb = 0;
+  for (int round = 0; round < kSummaryRounds; round++) {
+    const int base = ((int)blockIdx.x * kSummaryRounds + round) * VB;
+    if (base >= nblocks) break;
+    for (int e = threadIdx.x; e < VB * 3 * NN; e += kThreads) {
+      const int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
+      const int dy = k / N, dx = k % N;
+      float v = 0.0f;
+      if (base + j < nblocks) {
+        const int bx = (base + j) % g.bw, by = (base + j) / g.bw;
+        const int x = min(8 * bx + dx, g.w - 1), y = min(8 * by + dy, g.h - 1);
+        v = planes[c][(size_t)y * g.w + x];
+      }
+      lds[((j * 3 + c) * N + dy) * P + dx] = v;
+    }
+    __syncthreads();
+    const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
+    float* tile = lds + (j * 3 + c) * N * P;
+    fwd_pass1<N>(tile, line, t);
+    __syncthreads();
+    float out[N];
+    fwd_pass2<N>(tile, line, out, t);
+#pragma unroll
+    for (int k = 0; k < N; k++) coef[j][c][k * N + line] = out[k];
+    __syncthreads();
+    if (threadIdx.x < 2 * VB) {
+      const int jb = threadIdx.x >> 1;
+      if (base + jb < nblocks) {
+        if ((threadIdx.x & 1) == 0) {
+          const float* cy = coef[jb][1];
+          double ey = 0.0, lo = 0.0, hi = 0.0, rw = 0.0, cl = 0.0;
+          uint32_t a3 = 0, a2 = 0;
+          for (int cell = 1; cell < NN; cell++) {
+            const int row = cell >> 3, col = cell & 7;
+            const double v = (double)cy[cell];
+            const double en = v * v;
+            ey += en;
+            if (row + col <= 2) lo += en;
+            if (max(row, col) >= 4) hi += en;
+            if (row > col) rw += en;
+            else if (col > row) cl += en;
+            const double m = fabs(v);
+            a3 += m < 1e-3 ? 1u : 0u;
+            a2 += m < 1e-2 ? 1u : 0u;
+          }
+          bsum[jb][0] = ey;
+          bsum[jb][1] = lo;
+          bsum[jb][2] = hi;
+          bsum[jb][3] = rw;
+          bsum[jb][4] = cl;
+          bsum[jb][6] = (double)cy[0];
+          bcnt[jb][0] = a3;
+          bcnt[jb][1] = a2;
+          const double bin = fmin(fmax((log(ey + 1e-30) - (-46.0)) / 0.125, 0.0), (double)(kSummaryBins - 1));
+          atomicAdd(&hist[(int)bin], 1u);
+        } else {
+          double xb = 0.0;
+          for (int cell = 1; cell < NN; cell++) xb += (double)coef[jb][0][cell] * (double)coef[jb][0][cell];
+          for (int cell = 1; cell < NN; cell++) xb += (double)coef[jb][2][cell] * (double)coef[jb][2][cell];
+          bsum[jb][5] = xb;
+        }
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      for (int jb = 0; jb < VB && base + jb < nblocks; jb++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += bsum[jb][q];
+        acc[6] += bsum[jb][6];
+        acc[7] += bsum[jb][6] * bsum[jb][6];
+        n3 += bcnt[jb][0];
+        n2 += bcnt[jb][1];
+        nb++;
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    SummaryPartial& o = parts[blockIdx.x];
+    o.total_y = acc[0];
+    o.total_low = acc[1];
+    o.total_high = acc[2];
+    o.total_row = acc[3];
+    o.total_col = acc[4];
+    o.total_xb = acc[5];
+    o.dc_sum = acc[6];
+    o.dc_sumsq = acc[7];
+    o.near_1e3 = n3;
+    o.near_1e2 = n2;
+    o.blocks = nb;
+    o.ac_cells = nb * 63u;
+  }
+}
+
+__global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
 // Forward or inverse DCT of `count` contiguous N x N blocks, through the same
 // device passes the pipeline uses (the C-ABI's transform entry point).
 template <int N>
@@ -1448,6 +1571,14 @@ void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, con
   else
     k_xyb_fwd<false><<<div_up(n, 256), 256, 0, s>>>(n, rgb, lut, xyb[0], xyb[1], xyb[2], lin[0],
                                                     lin[1], lin[2], gray);
+}
+
+int summary_groups(const Geom& g) { return (int)div_up((size_t)g.bw * g.bh, 8 * kSummaryRounds); }
+
+void launch_dct8_summary(hipStream_t s, const Geom& g, const ConstTables* tab, float* const xyb[3],
+                         SummaryPartial* parts, uint32_t* hist) {
+  k_fill_u32<<<div_up(kSummaryBins, 256), 256, 0, s>>>(hist, kSummaryBins, 0u);
+  k_dct8_summary<<<summary_groups(g), kThreads, 0, s>>>(g, tab, xyb[0], xyb[1], xyb[2], parts, hist);
 }
 
 template <int N, int FORM>

@@ -1015,24 +1015,109 @@ __global__ __launch_bounds__(256) void k_aq_activity(Geom g, const float* __rest
 // variance over the samples it holds (f64 sums in raster order, floored at
 // zero, narrowed once), the input of the frame features
 // (quality_features.rs:173-213).
-__global__ __launch_bounds__(256) void k_atlas_luma_var(Geom g, const float* __restrict__ Y,
-                                                        float* __restrict__ var) {
-  const int atom = blockIdx.x * blockDim.x + threadIdx.x;
-  if (atom >= g.bw * g.bh) return;
-  const int ax = atom % g.bw, ay = atom / g.bw;
-  const int x0 = ax * 8, y0 = ay * 8, x1 = min(x0 + 8, g.w), y1 = min(y0 + 8, g.h);
+__device__ __forceinline__ double atom_variance(const Geom& g, const float* __restrict__ P, int x0, int y0, int x1,
+                                                int y1) {
   const double count = (double)(x1 - x0) * (double)(y1 - y0);
   double sum = 0.0, sum_sq = 0.0;
   for (int y = y0; y < y1; y++)
     for (int x = x0; x < x1; x++) {
-      const double v = (double)Y[(size_t)y * g.w + x];
+      const double v = (double)P[(size_t)y * g.w + x];
       sum += v;
       sum_sq += v * v;
     }
   const double mean = sum / count;
   double variance = sum_sq / count - mean * mean;
   if (!(variance > 0.0)) variance = 0.0;
-  var[atom] = (float)variance;
+  return variance;
+}
+
+// luma: the Y variance; chroma (when given): X variance + B variance, each
+// narrowed to f32 first (quality_features.rs:183-187).
+__global__ __launch_bounds__(256) void k_atlas_var(Geom g, const float* __restrict__ X, const float* __restrict__ Y,
+                                                   const float* __restrict__ B, float* __restrict__ luma,
+                                                   float* __restrict__ chroma) {
+  const int atom = blockIdx.x * blockDim.x + threadIdx.x;
+  if (atom >= g.bw * g.bh) return;
+  const int ax = atom % g.bw, ay = atom / g.bw;
+  const int x0 = ax * 8, y0 = ay * 8, x1 = min(x0 + 8, g.w), y1 = min(y0 + 8, g.h);
+  luma[atom] = (float)atom_variance(g, Y, x0, y0, x1, y1);
+  if (chroma) {
+    const float vx = (float)atom_variance(g, X, x0, y0, x1, y1);
+    const float vb = (float)atom_variance(g, B, x0, y0, x1, y1);
+    chroma[atom] = vx + vb;
+  }
+}
+
+// preanalysis_features (preanalysis.rs:277-484): one workgroup per whole
+// 32x32 cell.  The samples' top eight bits give integer BT.601 planes; the
+// 3x3 Sobel reads luma clamped to the frame; the cell's integer sums are
+// exact in any order, so waves reduce them freely.
+template <bool U8>
+__global__ __launch_bounds__(256) void k_preanalysis_cells(int w, int h, int shift, const void* __restrict__ rgb,
+                                                           int cells_x, CellStats* __restrict__ out) {
+  __shared__ int lum[34][35];
+  __shared__ int red[4][14];
+  const int x0 = (int)(blockIdx.x % cells_x) * 32, y0 = (int)(blockIdx.x / cells_x) * 32;
+  auto sample = [&](int x, int y, int ch) -> int {
+    const size_t i = ((size_t)y * w + x) * 3 + ch;
+    if constexpr (U8) return (int)((const uint8_t*)rgb)[i];
+    return min((int)(((const uint16_t*)rgb)[i] >> shift), 255);
+  };
+  for (int e = threadIdx.x; e < 34 * 34; e += 256) {
+    const int ly = e / 34, lx = e % 34;
+    const int x = min(max(x0 - 1 + lx, 0), w - 1), y = min(max(y0 - 1 + ly, 0), h - 1);
+    lum[ly][lx] = (77 * sample(x, y, 0) + 150 * sample(x, y, 1) + 29 * sample(x, y, 2) + 128) >> 8;
+  }
+  __syncthreads();
+  // sum, sum_sq, edge, flat, noise_sum, weak, dir[4], cb_sum, cb_sq, cr_sum, cr_sq
+  int a[14] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int p = (int)threadIdx.x + 256 * k, px = p & 31, py = p >> 5;
+    const int lx = px + 1, ly = py + 1;
+    const int c = lum[ly][lx];
+    const int tl = lum[ly - 1][lx - 1], tc = lum[ly - 1][lx], tr = lum[ly - 1][lx + 1];
+    const int ml = lum[ly][lx - 1], mr = lum[ly][lx + 1];
+    const int bl = lum[ly + 1][lx - 1], bc = lum[ly + 1][lx], br = lum[ly + 1][lx + 1];
+    a[0] += c;
+    a[1] += c * c;
+    const int gx = (tr + 2 * mr + br) - (tl + 2 * ml + bl);
+    const int gy = (bl + 2 * bc + br) - (tl + 2 * tc + tr);
+    const int ax = abs(gx), ay = abs(gy), grad = ax + ay;
+    if (grad > 48) {
+      a[2]++;
+      const int bin = ax >= 2 * ay ? 0 : (ay >= 2 * ax ? 1 : ((gx > 0) == (gy > 0) ? 2 : 3));
+      a[6 + bin]++;
+    }
+    if (grad < 12) a[3]++;
+    if (grad < 24) {
+      const int box = (tl + tc + tr + ml + c + mr + bl + bc + br) / 9;
+      a[4] += abs(c - box);
+      a[5]++;
+    }
+    const int x = x0 + px, y = y0 + py;
+    const int r = sample(x, y, 0), gg = sample(x, y, 1), b = sample(x, y, 2);
+    const int cb = min(max(((-43 * r - 85 * gg + 128 * b + 128) >> 8) + 128, 0), 255);
+    const int cr = min(max(((128 * r - 107 * gg - 21 * b + 128) >> 8) + 128, 0), 255);
+    a[10] += cb;
+    a[11] += cb * cb;
+    a[12] += cr;
+    a[13] += cr * cr;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for (int i = 0; i < 14; i++) {
+    int v = a[i];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) red[wv][i] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < 14) {
+    const int i = threadIdx.x;
+    const int v = red[0][i] + red[1][i] + red[2][i] + red[3][i];
+    int* o = (int*)&out[blockIdx.x];
+    o[i] = v;
+  }
 }
 
 // field.rs:200-219: the minimum over the clamped 3x3 neighbourhood.
@@ -1455,7 +1540,23 @@ void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], con
 
 void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var) {
   const int count = g.bw * g.bh;
-  k_atlas_luma_var<<<div_up(count, 256), 256, 0, s>>>(g, Y, var);
+  k_atlas_var<<<div_up(count, 256), 256, 0, s>>>(g, Y, Y, Y, var, nullptr);
+}
+
+void launch_atlas_var(hipStream_t s, const Geom& g, float* const xyb[3], float* luma, float* chroma) {
+  const int count = g.bw * g.bh;
+  k_atlas_var<<<div_up(count, 256), 256, 0, s>>>(g, xyb[0], xyb[1], xyb[2], luma, chroma);
+}
+
+int preanalysis_cells(int w, int h) { return (w / 32) * (h / 32); }
+
+void launch_preanalysis_cells(hipStream_t s, int w, int h, int bits, const void* rgb, CellStats* out) {
+  const int n = preanalysis_cells(w, h);
+  if (n == 0) return;
+  if (bits == 8)
+    k_preanalysis_cells<true><<<n, 256, 0, s>>>(w, h, 0, rgb, w / 32, out);
+  else
+    k_preanalysis_cells<false><<<n, 256, 0, s>>>(w, h, bits - 8, rgb, w / 32, out);
 }
 
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],

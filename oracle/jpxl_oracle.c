@@ -2017,3 +2017,265 @@ EXPORT void orc_estimate_cfl(int w, int h, const uint8_t *bmap, const float *coe
     free(tx); free(tb); free(lfx.s); free(lfb.s);
     hq_free(&hc);
 }
+
+/* ---- first-guess features of the quality search --------------------------- */
+
+/* preanalysis_features (jpxl-encode-policy/src/preanalysis.rs:277-540) over
+ * the 8-bit samples preanalysis_srgb8 / preanalysis_srgb16 hand it
+ * (:542-574: 9..16-bit samples reduced to their top eight bits).  Returns the
+ * number of whole 32x32 cells (0: None, nothing written); out[22] = share[7],
+ * log2var mean/std/q10/q90, edge mean/q90, flat mean, noise mean/q90, chroma
+ * log2 mean/q90, orientation entropy, direction dominance, axis-aligned
+ * share and heterogeneous fraction, as the struct's f32 fields. */
+static int cmp_f64(const void *a, const void *b) {
+    double x = *(const double *)a, y = *(const double *)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+static double pa_quantile(const double *s, int n, double q) { /* preanalysis.rs:243-268 */
+    if (n == 0) return 0.0;
+    double pos = q * (double)(n - 1);
+    int lo = (int)floor(pos);
+    double frac = pos - (double)lo;
+    double a = lo < n ? s[lo] : s[n - 1];
+    double b = lo + 1 < n ? s[lo + 1] : a;
+    return a + (b - a) * frac;
+}
+
+EXPORT uint32_t orc_preanalysis(int w, int h, int bits, const void *rgb, float *out) {
+    int cells_x = w / 32, cells_y = h / 32;
+    if (cells_x == 0 || cells_y == 0) return 0;
+    size_t px = (size_t)w * h;
+    uint8_t *lum = malloc(px), *cbp = malloc(px), *crp = malloc(px);
+    int shift = bits - 8;
+    for (size_t i = 0; i < px; i++) {
+        int s[3];
+        for (int c = 0; c < 3; c++) {
+            if (bits == 8) s[c] = ((const uint8_t *)rgb)[3 * i + c];
+            else {
+                int v = ((const uint16_t *)rgb)[3 * i + c] >> shift;
+                s[c] = v < 255 ? v : 255;
+            }
+        }
+        int r = s[0], g = s[1], b = s[2];
+        lum[i] = (uint8_t)((77 * r + 150 * g + 29 * b + 128) >> 8);
+        int cb = ((-43 * r - 85 * g + 128 * b + 128) >> 8) + 128;
+        int cr = ((128 * r - 107 * g - 21 * b + 128) >> 8) + 128;
+        cbp[i] = (uint8_t)(cb < 0 ? 0 : (cb > 255 ? 255 : cb));
+        crp[i] = (uint8_t)(cr < 0 ? 0 : (cr > 255 ? 255 : cr));
+    }
+#define LUM(xx, yy) ((int)lum[(size_t)((yy) < 0 ? 0 : ((yy) >= h ? h - 1 : (yy))) * w + \
+                               ((xx) < 0 ? 0 : ((xx) >= w ? w - 1 : (xx)))])
+    int n = cells_x * cells_y;
+    int *cls = malloc(sizeof(int) * n), *edge_q8 = malloc(sizeof(int) * n), *flat_q8 = malloc(sizeof(int) * n);
+    int *noise = malloc(sizeof(int) * n), *has_edges = malloc(sizeof(int) * n);
+    int *ent = malloc(sizeof(int) * n), *dom = malloc(sizeof(int) * n), *axis = malloc(sizeof(int) * n);
+    uint32_t *var = malloc(sizeof(uint32_t) * n), *chroma = malloc(sizeof(uint32_t) * n);
+    const int64_t N = 1024;
+    for (int cy = 0; cy < cells_y; cy++)
+        for (int cx = 0; cx < cells_x; cx++) {
+            int k = cy * cells_x + cx, x0 = cx * 32, y0 = cy * 32;
+            int64_t sum = 0, sum_sq = 0, edge_count = 0, flat_count = 0, noise_sum = 0, weak = 0;
+            int64_t dir[4] = {0, 0, 0, 0}, cb_sum = 0, cb_sq = 0, cr_sum = 0, cr_sq = 0;
+            for (int y = y0; y < y0 + 32; y++)
+                for (int x = x0; x < x0 + 32; x++) {
+                    int c = LUM(x, y);
+                    sum += c;
+                    sum_sq += c * c;
+                    int tl = LUM(x - 1, y - 1), tc = LUM(x, y - 1), tr = LUM(x + 1, y - 1);
+                    int ml = LUM(x - 1, y), mr = LUM(x + 1, y);
+                    int bl = LUM(x - 1, y + 1), bc = LUM(x, y + 1), br = LUM(x + 1, y + 1);
+                    int gx = (tr + 2 * mr + br) - (tl + 2 * ml + bl);
+                    int gy = (bl + 2 * bc + br) - (tl + 2 * tc + tr);
+                    int grad = abs(gx) + abs(gy);
+                    if (grad > 48) {
+                        edge_count++;
+                        int ax = abs(gx), ay = abs(gy);
+                        int bin = ax >= 2 * ay ? 0 : (ay >= 2 * ax ? 1 : ((gx > 0) == (gy > 0) ? 2 : 3));
+                        dir[bin]++;
+                    }
+                    if (grad < 12) flat_count++;
+                    if (grad < 24) {
+                        int box_mean = (tl + tc + tr + ml + c + mr + bl + bc + br) / 9;
+                        noise_sum += abs(c - box_mean);
+                        weak++;
+                    }
+                    size_t i = (size_t)y * w + x;
+                    int64_t vb = cbp[i], vr = crp[i];
+                    cb_sum += vb; cb_sq += vb * vb; cr_sum += vr; cr_sq += vr * vr;
+                }
+            int64_t mean = sum / N;
+            int64_t v = sum_sq / N - mean * mean;
+            var[k] = (uint32_t)(v > 0 ? v : 0);
+            int64_t cbm = cb_sum / N, crm = cr_sum / N;
+            int64_t vcb = cb_sq / N - cbm * cbm, vcr = cr_sq / N - crm * crm;
+            int64_t ch = (vcb > 0 ? vcb : 0) + (vcr > 0 ? vcr : 0);
+            chroma[k] = (uint32_t)(ch < 65535 ? ch : 65535);
+            int64_t edge_total = dir[0] + dir[1] + dir[2] + dir[3];
+            int64_t den = edge_total > 1 ? edge_total : 1;
+            int64_t dmax = dir[0];
+            for (int b = 1; b < 4; b++) dmax = dir[b] > dmax ? dir[b] : dmax;
+            edge_q8[k] = (int)((edge_count * 256) / N);
+            flat_q8[k] = (int)((flat_count * 256) / N);
+            noise[k] = weak > 0 ? (int)(noise_sum / weak) : 0;
+            has_edges[k] = edge_total > 0;
+            /* entropy4_q8 (preanalysis.rs:192-212) */
+            int e = 0;
+            if (edge_total > 0) {
+                double t = (double)edge_total, hh = 0.0;
+                for (int b = 0; b < 4; b++)
+                    if (dir[b] > 0) {
+                        double p = (double)dir[b] / t;
+                        hh -= p * log2(p);
+                    }
+                double q = round(hh / 2.0 * 256.0);
+                e = (int)(q < 0 ? 0 : (q > 256 ? 256 : q));
+            }
+            ent[k] = e;
+            dom[k] = (int)((dmax * 256) / den);
+            axis[k] = (int)(((dir[0] + dir[1]) * 256) / den);
+            /* classify (preanalysis.rs:214-241) */
+            int c;
+            if (edge_q8[k] >= 64 && ent[k] < 110 && axis[k] >= 160 && noise[k] < 7) c = 6;
+            else if (edge_q8[k] >= 38 && dom[k] >= 128) c = 5;
+            else if (noise[k] >= 7 && var[k] < 64) c = 3;
+            else if (var[k] >= 64 && ent[k] >= 150) c = 4;
+            else if (var[k] < 12 && edge_q8[k] < 13) c = 0;
+            else if (edge_q8[k] < 26) c = 1;
+            else if (chroma[k] >= 96) c = 2;
+            else c = 4;
+            cls[k] = c;
+        }
+#undef LUM
+    double count = (double)n;
+    float share[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int k = 0; k < n; k++) share[cls[k]] += 1.0f;
+    for (int i = 0; i < 7; i++) share[i] /= (float)count;
+    double *lv = malloc(sizeof(double) * n), *ed = malloc(sizeof(double) * n);
+    double *no = malloc(sizeof(double) * n), *chv = malloc(sizeof(double) * n);
+    double lv_sum = 0.0, ed_sum = 0.0, fl_sum = 0.0, no_sum = 0.0, ch_sum = 0.0;
+    for (int k = 0; k < n; k++) {
+        lv[k] = log2(1.0 + (double)var[k]);
+        lv_sum += lv[k];
+    }
+    double lv_mean = lv_sum / count, ss = 0.0, het = 0.0;
+    for (int k = 0; k < n; k++) ss += (lv[k] - lv_mean) * (lv[k] - lv_mean);
+    for (int k = 0; k < n; k++) het += fabs(lv[k] - lv_mean) > 2.0 ? 1.0 : 0.0;
+    for (int k = 0; k < n; k++) { ed[k] = (double)edge_q8[k] / 256.0; ed_sum += ed[k]; }
+    for (int k = 0; k < n; k++) fl_sum += (double)flat_q8[k] / 256.0;
+    for (int k = 0; k < n; k++) { no[k] = (double)noise[k]; no_sum += no[k]; }
+    for (int k = 0; k < n; k++) { chv[k] = log2(1.0 + (double)chroma[k]); ch_sum += chv[k]; }
+    qsort(lv, n, sizeof(double), cmp_f64);
+    qsort(ed, n, sizeof(double), cmp_f64);
+    qsort(no, n, sizeof(double), cmp_f64);
+    qsort(chv, n, sizeof(double), cmp_f64);
+    double e_sum = 0.0, d_sum = 0.0, a_sum = 0.0;
+    int edged = 0;
+    for (int k = 0; k < n; k++) if (has_edges[k]) { e_sum += (double)ent[k] / 256.0; edged++; }
+    for (int k = 0; k < n; k++) if (has_edges[k]) d_sum += (double)dom[k] / 256.0;
+    for (int k = 0; k < n; k++) if (has_edges[k]) a_sum += (double)axis[k] / 256.0;
+    for (int i = 0; i < 7; i++) out[i] = share[i];
+    out[7] = (float)lv_mean;
+    out[8] = (float)sqrt(ss / count);
+    out[9] = (float)pa_quantile(lv, n, 0.1);
+    out[10] = (float)pa_quantile(lv, n, 0.9);
+    out[11] = (float)(ed_sum / count);
+    out[12] = (float)pa_quantile(ed, n, 0.9);
+    out[13] = (float)(fl_sum / count);
+    out[14] = (float)(no_sum / count);
+    out[15] = (float)pa_quantile(no, n, 0.9);
+    out[16] = (float)(ch_sum / count);
+    out[17] = (float)pa_quantile(chv, n, 0.9);
+    out[18] = edged ? (float)(e_sum / (double)edged) : 0.0f;
+    out[19] = edged ? (float)(d_sum / (double)edged) : 0.0f;
+    out[20] = edged ? (float)(a_sum / (double)edged) : 0.0f;
+    out[21] = (float)(het / count);
+    free(lum); free(cbp); free(crp); free(cls); free(edge_q8); free(flat_q8); free(noise); free(has_edges);
+    free(ent); free(dom); free(axis); free(var); free(chroma); free(lv); free(ed); free(no); free(chv);
+    return (uint32_t)n;
+}
+
+/* quality_transform_summary (jpxl-encode-policy/src/lib.rs:2769-2955): the
+ * aligned DCT8x8 of every block, LF groups (2048-sample squares) in raster
+ * order and blocks in raster order within each, with the reference's
+ * running sums.  out[11] = blocks, ln_ac_y_mean, ln_ac_y_q50, _q90, _q99,
+ * high_low_ratio, directional_asymmetry, chroma_ac_ratio, near_zero_frac_1e3,
+ * near_zero_frac_1e2, dc_variance_y; hist[512] = the ln-energy histogram. */
+EXPORT void orc_transform_summary(int w, int h, const float *xyb[3], double *out, uint32_t *hist) {
+    const double EPS = 1e-30, HIST_LO = -46.0, HIST_WIDTH = 0.125;
+    const int BINS = 512, LFG = 2048;
+    for (int b = 0; b < BINS; b++) hist[b] = 0;
+    uint64_t blocks = 0, ac_cells = 0, near3 = 0, near2 = 0;
+    double total_y = 0, total_low = 0, total_high = 0, total_row = 0, total_col = 0, total_xb = 0;
+    double dc_sum = 0, dc_sumsq = 0;
+    float samples[64], co[3][64];
+    int gx_n = (w + LFG - 1) / LFG, gy_n = (h + LFG - 1) / LFG;
+    for (int gy = 0; gy < gy_n; gy++)
+        for (int gx = 0; gx < gx_n; gx++) {
+            /* one group's partial, combined in group order */
+            uint64_t p_blocks = 0, p_ac = 0, p3 = 0, p2 = 0;
+            double py_ = 0, plo = 0, phi = 0, prow = 0, pcol = 0, pxb = 0, pdc = 0, pdq = 0;
+            int rx0 = gx * LFG, ry0 = gy * LFG;
+            int rw = (w - rx0 < LFG ? w - rx0 : LFG), rh = (h - ry0 < LFG ? h - ry0 : LFG);
+            int gbw = (rw + 7) / 8, gbh = (rh + 7) / 8;
+            for (int by = 0; by < gbh; by++)
+                for (int bx = 0; bx < gbw; bx++) {
+                    int px = rx0 + 8 * bx, py = ry0 + 8 * by;
+                    for (int c = 0; c < 3; c++) {
+                        gather_square(xyb[c], w, h, px, py, 8, samples);
+                        orc_dct2d_fwd(8, 8, samples, co[c]);
+                    }
+                    double block_y = 0.0;
+                    for (int cell = 0; cell < 64; cell++) {
+                        double v = (double)co[1][cell];
+                        if (cell == 0) { pdc += v; pdq += v * v; continue; }
+                        int row = cell / 8, col = cell % 8;
+                        double energy = v * v;
+                        block_y += energy;
+                        if (row + col <= 2) plo += energy;
+                        if ((row > col ? row : col) >= 4) phi += energy;
+                        if (row > col) prow += energy;
+                        else if (col > row) pcol += energy;
+                        double m = fabs(v);
+                        if (m < 1e-3) p3++;
+                        if (m < 1e-2) p2++;
+                        p_ac++;
+                    }
+                    for (int c = 0; c < 3; c += 2)
+                        for (int cell = 1; cell < 64; cell++) pxb += (double)co[c][cell] * (double)co[c][cell];
+                    py_ += block_y;
+                    p_blocks++;
+                    double bin = (log(block_y + EPS) - HIST_LO) / HIST_WIDTH;
+                    bin = bin < 0.0 ? 0.0 : (bin > (double)(BINS - 1) ? (double)(BINS - 1) : bin);
+                    hist[(int)bin]++;
+                }
+            blocks += p_blocks; ac_cells += p_ac; total_y += py_; total_low += plo; total_high += phi;
+            total_row += prow; total_col += pcol; total_xb += pxb; near3 += p3; near2 += p2;
+            dc_sum += pdc; dc_sumsq += pdq;
+        }
+    double q_out[3];
+    const double qs[3] = {0.5, 0.9, 0.99};
+    for (int qi = 0; qi < 3; qi++) {
+        double wf = qs[qi] * (double)blocks, cap = (double)(blocks ? blocks - 1 : 0);
+        uint64_t want = (uint64_t)(wf < cap ? wf : cap), seen = 0;
+        q_out[qi] = HIST_LO + ((double)BINS - 0.5) * HIST_WIDTH;
+        for (int b = 0; b < BINS; b++) {
+            seen += hist[b];
+            if (seen > want) { q_out[qi] = HIST_LO + ((double)b + 0.5) * HIST_WIDTH; break; }
+        }
+    }
+    double nb = blocks > 1 ? (double)blocks : 1.0;
+    double acn = ac_cells > 1 ? (double)ac_cells : 1.0;
+    out[0] = (double)blocks;
+    out[1] = log(total_y / nb + EPS);
+    out[2] = q_out[0];
+    out[3] = q_out[1];
+    out[4] = q_out[2];
+    out[5] = total_high / (total_low + EPS);
+    out[6] = fabs(total_row - total_col) / (total_row + total_col + EPS);
+    out[7] = total_xb / (total_y + EPS);
+    out[8] = (double)near3 / acn;
+    out[9] = (double)near2 / acn;
+    double mean = dc_sum / nb, dv = dc_sumsq / nb - mean * mean;
+    out[10] = dv > 0.0 ? dv : 0.0;
+}

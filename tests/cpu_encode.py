@@ -1,8 +1,9 @@
 """The whole quality encode on the CPU, as test infrastructure: the library's
 navigator (jpxl_amd/csrc/controller.cpp compiled host-only, the same code the
 GPU path runs) driving the oracle's probes (cover + CfL on fresh probes,
-quantize, render, SSIMULACRA2), with the legacy predictor's start from the
-oracle's atlas.  It is the CPU restatement of search_frame_perceptual
+quantize, render, SSIMULACRA2), started where the library's crossing
+predictor (or the legacy table) puts the first probe for the oracle's
+features.  It is the CPU restatement of search_frame_perceptual
 (jpxl-encode-policy/src/quality.rs:2405-2740) used as the bench's CPU
 baseline and as the GPU navigator's end-to-end check; never shipped."""
 from __future__ import annotations
@@ -17,9 +18,12 @@ import oracle_binding as orc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 SO = os.path.join(ROOT, "tests", "host", "libcontroller_check.so")
+CSRC = os.path.join(ROOT, "jpxl_amd", "csrc")
 SRCS = [os.path.join(ROOT, "tests", "host", "controller_check.cpp"),
-        os.path.join(ROOT, "jpxl_amd", "csrc", "controller.cpp"),
-        os.path.join(ROOT, "jpxl_amd", "csrc", "metric_host.cpp")]
+        os.path.join(CSRC, "controller.cpp"),
+        os.path.join(CSRC, "metric_host.cpp"),
+        os.path.join(CSRC, "predictor.cpp"),
+        os.path.join(CSRC, "predictor_tables.S")]
 
 PROBE_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_double))
 
@@ -35,9 +39,10 @@ class Outcome(ctypes.Structure):
 
 def build(force: bool = False) -> str:
     """Compiles the host-only controller shim (g++, a few seconds)."""
-    if force or not os.path.exists(SO) or any(os.path.getmtime(s) > os.path.getmtime(SO) for s in SRCS):
-        subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off", "-o", SO] + SRCS,
-                       check=True)
+    deps = SRCS + [os.path.join(CSRC, "predictor_tables.bin")]
+    if force or not os.path.exists(SO) or any(os.path.getmtime(s) > os.path.getmtime(SO) for s in deps):
+        subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off", f"-Wa,-I{CSRC}",
+                        "-o", SO] + SRCS, check=True)
     return SO
 
 
@@ -56,8 +61,12 @@ def lib():
         cc.cc_features.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.POINTER(ctypes.c_float),
                                    ctypes.POINTER(ctypes.c_float)]
         cc.cc_navigate_callback.restype = ctypes.c_int
-        cc.cc_navigate_callback.argtypes = [d, ctypes.c_int, ctypes.c_int, u32, ctypes.c_int, PROBE_FN,
+        cc.cc_navigate_callback.argtypes = [d, ctypes.c_int, ctypes.c_int, u32, d, ctypes.c_int, PROBE_FN,
                                             ctypes.POINTER(Outcome)]
+        from jpxl_amd import _lib as jl
+
+        cc.cc_quality_start.argtypes = [ctypes.POINTER(jl.SourceFeatures), ctypes.POINTER(jl.TransformSummary), d,
+                                        ctypes.c_int, u32, ctypes.POINTER(u32), ctypes.POINTER(d)]
         cc.cc_rung_policy.argtypes = [u32, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
         _LIB = cc
     return _LIB
@@ -70,6 +79,38 @@ def predicted_rung(xyb: np.ndarray, target: float) -> int:
     q50, flat = ctypes.c_float(), ctypes.c_float()
     cc.cc_features(var.ctypes.data, var.size, ctypes.byref(q50), ctypes.byref(flat))
     return int(cc.cc_rung_for_scale(cc.cc_predicted_scale(q50.value, flat.value, target)))
+
+
+def oracle_features(rgb: np.ndarray, bits: int, xyb: np.ndarray, gray: bool):
+    """The first-guess features from the oracle (quality.rs:2449-2474), as
+    the library's ctypes structs."""
+    from jpxl_amd import _lib as jl
+
+    sf = jl.SourceFeatures()
+    for k, v in orc.source_features(xyb, gray).items():
+        setattr(sf, k, v)
+    pa = orc.preanalysis(rgb, bits)
+    if pa is not None:
+        sf.has_preanalysis = 1
+        sf.cells = pa["cells"]
+        for i in range(7):
+            sf.share[i] = float(pa["share"][i])
+        for k in orc.PREANALYSIS_FIELDS[1:]:
+            setattr(sf, k, float(pa[k]))
+    ts = jl.TransformSummary()
+    t, _ = orc.transform_summary(xyb)
+    for k, v in t.items():
+        setattr(ts, k, v)
+    return sf, ts
+
+
+def quality_start(rgb: np.ndarray, bits: int, xyb: np.ndarray, gray: bool, target: float, effort: int):
+    """(first rung, prior exponent or NaN) from the oracle's features."""
+    sf, ts = oracle_features(rgb, bits, xyb, gray)
+    r, b = ctypes.c_uint32(), ctypes.c_double()
+    lib().cc_quality_start(ctypes.byref(sf), ctypes.byref(ts), target, effort, predicted_rung(xyb, target),
+                           ctypes.byref(r), ctypes.byref(b))
+    return int(r.value), float(b.value)
 
 
 def rung_policy(rung: int, effort: int):
@@ -89,7 +130,7 @@ def encode(rgb: np.ndarray, bits: int, target: float, effort: int = 1, mode: int
     (quant, lf, bmap, cfl_x, cfl_b, lf_factors))."""
     h, w, _ = rgb.shape
     xyb, lin, gray = orc.prepare_frame(rgb, bits)
-    pred = predicted_rung(xyb, target) if mode == 0 else 0
+    pred, prior = quality_start(rgb, bits, xyb, gray, target, effort) if mode == 0 else (0, float("nan"))
     state = {}
 
     def structure(rung):
@@ -128,7 +169,7 @@ def encode(rgb: np.ndarray, bits: int, target: float, effort: int = 1, mode: int
 
     out = Outcome()
     cb_fn = PROBE_FN(probe)
-    rc = lib().cc_navigate_callback(target, effort, max_probes, pred, mode, cb_fn, ctypes.byref(out))
+    rc = lib().cc_navigate_callback(target, effort, max_probes, pred, prior, mode, cb_fn, ctypes.byref(out))
     if rc:
         raise RuntimeError(f"CPU navigator failed ({rc})")
     plan, _, _, _ = plan_at(out.rung, bool(out.chosen_fresh))

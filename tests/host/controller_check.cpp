@@ -8,6 +8,7 @@
 
 #include "../../jpxl_amd/csrc/controller.h"
 #include "../../jpxl_amd/csrc/metric_host.h"
+#include "../../jpxl_amd/csrc/predictor.h"
 
 using namespace jpxlg;
 
@@ -98,8 +99,8 @@ int cc_navigate_curve(double target, int effort, int pixel, int exact, int build
 // The navigator over a caller's probe (the CPU restatement's whole encode):
 // probe(rung, fresh, &score) -> status.
 typedef int (*cc_probe_fn)(uint32_t rung, int fresh, double* score);
-int cc_navigate_callback(double target, int effort, int max_probes, uint32_t predicted, int mode, cc_probe_fn fn,
-                         cc_outcome* out) {
+int cc_navigate_callback(double target, int effort, int max_probes, uint32_t predicted, double prior_beta, int mode,
+                         cc_probe_fn fn, cc_outcome* out) {
   memset(out, 0, sizeof(*out));
   ProbeFn probe = [&](uint32_t rung, bool fresh, double* score, double* ms) -> int {
     *ms = 0.0;
@@ -107,7 +108,7 @@ int cc_navigate_callback(double target, int effort, int max_probes, uint32_t pre
   };
   ControllerOutcome oc;
   const int rc = mode == 1 ? bisect_controller(target, max_probes, probe, &oc)
-                           : navigate_controller(target, effort, max_probes, probe, &oc, predicted);
+                           : navigate_controller(target, effort, max_probes, probe, &oc, predicted, prior_beta);
   if (rc) return rc;
   out->rung = oc.rung;
   out->score = oc.score;
@@ -150,6 +151,16 @@ double cc_pool_terms(const double* sums, int ns, const int* sw, const int* sh, d
   memcpy(terms, m.terms, sizeof(double) * 18 * ns);
   *raw = m.raw_error;
   return m.score;
+}
+
+// The first probe and prior exponent of a quality search from a frame's
+// features (the library's routing of the crossing predictor).
+int cc_quality_start(const jpxlg_source_features* f, const jpxlg_transform_summary* t, double target, int effort,
+                     uint32_t legacy, uint32_t* rung, double* prior_beta) {
+  const QualityStart q = quality_start(*f, *t, target, quality_policy(effort).reserve, legacy);
+  *rung = q.rung;
+  *prior_beta = q.prior_beta;
+  return 0;
 }
 
 }  // extern "C"

@@ -76,6 +76,7 @@ def lib():
         o.orc_aq_footprint_mul.argtypes = [ctypes.c_void_p] + [ctypes.c_int] * 6 + [ctypes.c_uint32, ctypes.c_int,
                                                                                    ctypes.c_int]
         o.orc_aq_setup.restype = ctypes.c_int
+        o.orc_preanalysis.restype = ctypes.c_uint32
         _o = o
     return _o
 
@@ -396,3 +397,63 @@ def aq_block_muls(w: int, h: int, bmap: np.ndarray, q: QuantParams) -> np.ndarra
     out = np.zeros(bm.shape, np.uint32)
     lib().orc_aq_block_muls(w, h, P(bm), ctypes.byref(q), P(out))
     return out
+
+
+# ---- first-guess features of the quality search ------------------------------
+
+PREANALYSIS_FIELDS = ["share", "log2var_mean", "log2var_std", "log2var_q10", "log2var_q90", "edge_mean",
+                      "edge_q90", "flat_mean", "noise_mean", "noise_q90", "chroma_log2_mean", "chroma_log2_q90",
+                      "orient_entropy_mean", "dir_dominance_mean", "axis_aligned_mean", "hetero_frac"]
+
+TRANSFORM_FIELDS = ["blocks", "ln_ac_y_mean", "ln_ac_y_q50", "ln_ac_y_q90", "ln_ac_y_q99", "high_low_ratio",
+                    "directional_asymmetry", "chroma_ac_ratio", "near_zero_frac_1e3", "near_zero_frac_1e2",
+                    "dc_variance_y"]
+
+
+def preanalysis(rgb: np.ndarray, bits: int):
+    """preanalysis_srgb8/16 -> None (no whole 32x32 cell) or a dict of the
+    PreanalysisFeatures fields (share as a (7,) f32 array), with 'cells'."""
+    h, w, _ = rgb.shape
+    rgb = np.ascontiguousarray(rgb, np.uint8 if bits == 8 else np.uint16)
+    out = np.zeros(22, np.float32)
+    n = lib().orc_preanalysis(w, h, bits, P(rgb), P(out))
+    if n == 0:
+        return None
+    d = {"cells": int(n), "share": out[:7].copy()}
+    for k, name in enumerate(PREANALYSIS_FIELDS[1:]):
+        d[name] = np.float32(out[7 + k])
+    return d
+
+
+def transform_summary(xyb: np.ndarray):
+    """quality_transform_summary -> (dict of the TransformFeatureSummary
+    fields, the (512,) ln-energy histogram)."""
+    _, h, w = xyb.shape
+    xyb = np.ascontiguousarray(xyb, np.float32)
+    out = np.zeros(11, np.float64)
+    hist = np.zeros(512, np.uint32)
+    lib().orc_transform_summary(w, h, planes_ptr([xyb[0], xyb[1], xyb[2]]), P(out), P(hist))
+    d = {name: float(out[k]) for k, name in enumerate(TRANSFORM_FIELDS)}
+    d["blocks"] = int(out[0])
+    return d, hist
+
+
+def floor_quantile(v: np.ndarray, q: float) -> np.float32:
+    """quality_features.rs:141-152: the element at floor((n-1) q)."""
+    if v.size == 0:
+        return np.float32(0.0)
+    s = np.sort(v.astype(np.float32))
+    return s[int(np.floor((s.size - 1) * float(np.float32(q))))]
+
+
+def source_features(xyb: np.ndarray, grayscale: bool):
+    """source_features (quality_features.rs:173-213) from the atlas."""
+    var = atlas_variance(xyb)
+    luma = var[1].reshape(-1)
+    chroma = (var[0] + var[2]).reshape(-1).astype(np.float32)
+    q10, q50, q90 = (floor_quantile(luma, q) for q in (0.1, 0.5, 0.9))
+    flat = np.float32(np.count_nonzero(luma < np.float32(2e-5))) / np.float32(luma.size)
+    return {"width": xyb.shape[2], "height": xyb.shape[1], "grayscale": bool(grayscale),
+            "luma_variance_q10": q10, "luma_variance_q50": q50, "luma_variance_q90": q90,
+            "chroma_variance_q50": floor_quantile(chroma, 0.5), "flat_fraction": np.float32(flat),
+            "edge_proxy": np.float32(q90 - q50)}

@@ -6,19 +6,12 @@ quality.rs:1021-1079) against the reference's own table, and the metric's
 108 weights as the library's host code applies them."""
 import ctypes
 import math
-import os
-import subprocess
 
 import numpy as np
 import pytest
 
 import reference_restated as rr
 
-ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SO = os.path.join(ROOT, "_build", "libcontroller_check.so")
-SRCS = [os.path.join(ROOT, "tests", "host", "controller_check.cpp"),
-        os.path.join(ROOT, "jpxl_amd", "csrc", "controller.cpp"),
-        os.path.join(ROOT, "jpxl_amd", "csrc", "metric_host.cpp")]
 
 MET, ADJ, CAP, SAT_TOP, SAT_FLOOR, UNDER, RESCUED = range(7)
 FAST, BALANCED = 0, 1
@@ -35,10 +28,9 @@ class Outcome(ctypes.Structure):
 
 @pytest.fixture(scope="module")
 def cc():
-    os.makedirs(os.path.dirname(SO), exist_ok=True)
-    subprocess.run(["g++", "-O2", "-fPIC", "-shared", "-std=c++17", "-ffp-contract=off", "-o", SO] + SRCS,
-                   check=True)
-    lib = ctypes.CDLL(SO)
+    import cpu_encode
+
+    lib = ctypes.CDLL(cpu_encode.build())  # the host-only build of the navigator
     d, u32, i32 = ctypes.c_double, ctypes.c_uint32, ctypes.c_int
     lib.cc_met_band.restype = d
     lib.cc_met_band.argtypes = [d, d]
