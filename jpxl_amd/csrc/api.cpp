@@ -65,6 +65,9 @@ struct jpxlg_ctx {
   ConstTables* d_tab = nullptr;
   DctMats* d_mats = nullptr;
   float* d_lut8 = nullptr;
+  // Order-statistic scratch of the first-guess features (one image at a
+  // time on the context's stream).
+  SelectScratch* d_select = nullptr;
   std::map<int, float*> d_lut16;
   std::map<int, DepthSet> depth;
   std::string last_error;
@@ -279,7 +282,7 @@ size_t image_bytes(int w, int h) {
   b += bw * bh * 4;                   // a caller-supplied plan's per-varblock HfMul
   b += 64 + 2 * ((bw + 7) / 8) * ((bh + 7) / 8) + bw * bh + 256;  // structure readback block
   b += (size_t)preanalysis_cells(w, h) * sizeof(CellStats);  // first-guess features: cell sums,
-  b += (bw * bh / (8 * 16) + 2) * sizeof(SummaryPartial) + kSummaryBins * 4;  // DCT8 summary partials
+  b += (bw * bh / 64 + 2) * sizeof(SummaryPartial) + kSummaryBins * 4;  // DCT8 summary partials
   b += 512 * 256;  // alignment of every carved buffer
   return b;
 }
@@ -415,6 +418,7 @@ extern "C" int32_t jpxlg_ctx_create(int32_t device, jpxlg_ctx** out) {
   srgb8_lut(lut8);
   HIP_TRY(ctx.get(), hipMalloc(&ctx->d_lut8, sizeof(lut8)));
   HIP_TRY(ctx.get(), hipMemcpy(ctx->d_lut8, lut8, sizeof(lut8), hipMemcpyHostToDevice));
+  HIP_TRY(ctx.get(), hipMalloc(&ctx->d_select, sizeof(SelectScratch)));
   for (int i = 0; i < 8; i++) ctx->slots.emplace_back(new jpxlg_ctx::Slot());
   {
     // The device address of the no-allocation scratch array on this device
@@ -438,6 +442,7 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   (void)hipFree(ctx->d_tab);
   (void)hipFree(ctx->d_mats);
   (void)hipFree(ctx->d_lut8);
+  (void)hipFree(ctx->d_select);
   for (auto& kv : ctx->d_lut16) (void)hipFree(kv.second);
   for (auto& kv : ctx->depth) {
     (void)hipFree(kv.second.thr);
@@ -1683,17 +1688,29 @@ static int image_features(jpxlg_img* img) {
   if (img->features_known) return JPXLG_OK;
   jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
-  const size_t bpx = (size_t)g.bw * g.bh;
+  const int n = g.bw * g.bh;
   const int ncells = preanalysis_cells(g.w, g.h), nparts = summary_groups(g);
   hipStream_t st = ctx->stream;
   // cover scratch, free between structure builds
   float* d_luma = (float*)img->d_cost[0];
   float* d_chroma = (float*)img->d_cost[1];
-  launch_atlas_var(st, g, img->xyb, d_luma, d_chroma);
-  launch_preanalysis_cells(st, g.w, g.h, img->bits, img->d_rgb, img->d_cells);
-  launch_dct8_summary(st, g, ctx->d_tab, img->xyb, img->d_sum_parts, img->d_hist);
+  {
+    Prof p(ctx, "features");
+    launch_block_features(st, g, ctx->d_tab, img->xyb, d_luma, d_chroma, img->d_sum_parts, img->d_hist);
+    // quality_features.rs:141-152: ranks floor((n - 1) q) for the luma
+    // q10/q50/q90 and the chroma q50
+    SelectRanks r{};
+    const float qs[kSelectQueries] = {0.1f, 0.5f, 0.9f, 0.5f};
+    for (int q = 0; q < kSelectQueries; q++) {
+      r.rank[q] = (uint32_t)std::min<double>(floor((double)(n - 1) * (double)qs[q]), (double)(n - 1));
+      r.set[q] = q < 3 ? 0 : 1;
+    }
+    launch_select(st, d_luma, d_chroma, n, kFlatVariance, r, ctx->d_select);
+    launch_preanalysis_cells(st, g.w, g.h, img->bits, img->d_rgb, img->d_cells);
+  }
   HIP_TRY(ctx, hipGetLastError());
-  const size_t o_chroma = bpx * 4, o_cells = 2 * bpx * 4;
+  const size_t sel_off = offsetof(SelectScratch, flat), sel_bytes = sizeof(SelectScratch) - sel_off;
+  const size_t o_cells = 256;
   const size_t o_parts = o_cells + (size_t)ncells * sizeof(CellStats);
   const size_t o_hist = o_parts + (size_t)nparts * sizeof(SummaryPartial);
   const size_t need = o_hist + kSummaryBins * 4;
@@ -1706,8 +1723,7 @@ static int image_features(jpxlg_img* img) {
     ctx->rb_cap = need;
   }
   char* h = (char*)ctx->rb_host;
-  HIP_TRY(ctx, hipMemcpyAsync(h, d_luma, bpx * 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(h + o_chroma, d_chroma, bpx * 4, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(h, (const char*)ctx->d_select + sel_off, sel_bytes, hipMemcpyDeviceToHost, st));
   if (ncells)
     HIP_TRY(ctx, hipMemcpyAsync(h + o_cells, img->d_cells, (size_t)ncells * sizeof(CellStats),
                                 hipMemcpyDeviceToHost, st));
@@ -1717,13 +1733,14 @@ static int image_features(jpxlg_img* img) {
   if (!img->gray_known) HIP_TRY(ctx, hipMemcpyAsync(&img->gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
   HIP_TRY(ctx, hipStreamSynchronize(st));
   img->gray_known = true;
+  SelectScratch sel;  // only its tail (flat count onward) was read back
+  memcpy((char*)&sel + sel_off, h, sel_bytes);
   jpxlg_source_features& f = img->sf;
   f = jpxlg_source_features{};
   f.width = (uint32_t)g.w;
   f.height = (uint32_t)g.h;
   f.grayscale = img->gray ? 1 : 0;
-  source_features_from(std::vector<float>((const float*)h, (const float*)h + bpx),
-                       std::vector<float>((const float*)(h + o_chroma), (const float*)(h + o_chroma) + bpx), &f);
+  source_features_set(sel.value[0], sel.value[1], sel.value[2], sel.value[3], sel.flat, (uint32_t)n, &f);
   preanalysis_from_cells((const CellStats*)(h + o_cells), ncells, &f);
   transform_summary_from((const SummaryPartial*)(h + o_parts), nparts, (const uint32_t*)(h + o_hist), &img->ts);
   img->features_known = true;

@@ -362,18 +362,35 @@ void launch_cfl_lf(hipStream_t s, const ProbeParams* pp, const StructInfo* si, c
 void launch_aq_field(hipStream_t s, const Geom& g, int mode, float* const xyb[3], float* act, float* ero,
                      AqScratch* scratch, float* adj);
 // Each listed varblock's HfMul at its top-left block.
-void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var);
-// Per-atom Y variance, and X + B variance (the source features' inputs).
-void launch_atlas_var(hipStream_t s, const Geom& g, float* const xyb[3], float* luma, float* chroma);
+// Exact order statistics of the per-atom variances: kSelectQueries ranks,
+// each of set 0 (luma) or 1 (chroma), and the count of luma below a bound.
+constexpr int kSelectQueries = 4;
+struct SelectRanks {
+  uint32_t rank[kSelectQueries];
+  int set[kSelectQueries];
+};
+struct SelectScratch {
+  uint32_t h1[2][2048];                // digit 1 (bits 31-21) of luma, chroma
+  uint32_t h2[kSelectQueries][2048];   // digit 2 (bits 20-10) per query
+  uint32_t h3[kSelectQueries][1024];   // digit 3 (bits 9-0) per query
+  uint32_t flat;
+  uint32_t pad_;
+  uint32_t prefix[kSelectQueries];
+  uint32_t rank[kSelectQueries];
+  float value[kSelectQueries];
+};
+void launch_select(hipStream_t s, const float* luma, const float* chroma, int n, float flat_below,
+                   const SelectRanks& r, SelectScratch* sc);
 // Whole 32x32 preanalysis cells of a w x h frame, and their integer sums
 // from the source samples (8-bit, or 9..16-bit reduced to their top 8 bits).
 int preanalysis_cells(int w, int h);
 void launch_preanalysis_cells(hipStream_t s, int w, int h, int bits, const void* rgb, CellStats* out);
-// The quality predictor's DCT8 summary: summary_groups(g) partials and the
+// The per-block first-guess features in one pass: each atom's Y variance
+// and X + B variance, the DCT8 summary's summary_groups(g) partials and its
 // kSummaryBins-bin histogram (cleared first).
 int summary_groups(const Geom& g);
-void launch_dct8_summary(hipStream_t s, const Geom& g, const ConstTables* tab, float* const xyb[3],
-                         SummaryPartial* parts, uint32_t* hist);
+void launch_block_features(hipStream_t s, const Geom& g, const ConstTables* tab, float* const xyb[3],
+                           float* luma_var, float* chroma_var, SummaryPartial* parts, uint32_t* hist);
 // The QuantizedFrameIr arena (every varblock's X, Y, B coefficients in LF
 // group then block-raster order) and the per-group LF planes.
 void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3],

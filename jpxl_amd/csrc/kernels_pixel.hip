@@ -256,122 +256,182 @@ __global__ __launch_bounds__(kThreads) void k_dct_fwd(Geom g, const uint32_t* __
   }
 }
 
-// The DCT8 summary of the quality predictor (quality_transform_summary,
-// jpxl-encode-policy/src/lib.rs:2769-2874): every aligned 8x8 block of the
-// XYB planes (edge blocks replicate the last column and row, as the cover's
-// candidates do) through the exact DCT8, then per block, in cell order, its
-// Y AC energy, low/high/row/column band energies, near-zero counts, DC and
-// X+B AC energy.  A workgroup runs kSummaryRounds rounds of 8 consecutive
-// raster blocks and keeps their running sums in block order; the histogram
-// of ln(Y AC energy) bins is an integer count.
-constexpr int kSummaryRounds = 16;
+// The per-block first-guess features of the quality search, one pass over
+// the XYB planes: each aligned 8x8 block's sample variance per channel (the
+// analysis atlas's atoms, over the samples the frame has, in raster order:
+// quality_features.rs:173-213, analysis.rs:97-157) and its DCT8 summary terms
+// (quality_transform_summary, jpxl-encode-policy/src/lib.rs:2769-2874: the
+// exact DCT8 of the block, edge blocks replicating the last column and row
+// as the cover's candidates do; per block, in cell order, the Y AC energy,
+// low/high/row/column band energies, near-zero counts, DC and X+B AC
+// energy).  A workgroup stages kSummaryBlocks consecutive raster blocks'
+// samples in LDS with row-contiguous loads, reduces each block on a lane of
+// its own, transforms eight blocks at a time in place, and sums its blocks in
+// block order; the histogram of ln(Y AC energy) bins is an integer count,
+// kept in LDS per workgroup.
+constexpr int kSummaryBlocks = 64;
 
 __global__ __launch_bounds__(kThreads) void k_dct8_summary(Geom g, const ConstTables* __restrict__ tab,
                                                            const float* __restrict__ X,
                                                            const float* __restrict__ Y,
                                                            const float* __restrict__ B,
+                                                           float* __restrict__ luma_var,
+                                                           float* __restrict__ chroma_var,
                                                            SummaryPartial* __restrict__ parts,
                                                            uint32_t* __restrict__ hist) {
-  constexpr int N = 8, VB = 8, P = N + 1, NN = 64;
-  __shared__ float lds[VB * 3 * N * P];
-  __shared__ float coef[VB][3][NN];
-  __shared__ double bsum[VB][7];
-  __shared__ uint32_t bcnt[VB][2];
+  constexpr int N = 8, VB = 8, P = N + 1, NN = 64, CP = kSummaryBlocks + 1;
+  constexpr int kStrip = kSummaryBlocks * N;  // samples per staged row
+  __shared__ float tile[VB * 3 * N * P];
+  __shared__ float smp[3][NN][CP];  // channel, cell, block: samples, then coefficients
+  __shared__ double bsum[9][kSummaryBlocks];  // y, low, high, row, col, X, dc, dc^2, B
+  __shared__ uint32_t bcnt[2][kSummaryBlocks];
+  __shared__ uint32_t lhist[kSummaryBins];
+  __shared__ float vxb[2][kSummaryBlocks];
+  for (int i = threadIdx.x; i < kSummaryBins; i += kThreads) lhist[i] = 0u;
   const float* planes[3] = {X, Y, B};
   const int nblocks = g.bw * g.bh;
+  const int base = (int)blockIdx.x * kSummaryBlocks;
+  const int nb = min(kSummaryBlocks, nblocks - base);
+  // stage: consecutive lanes read consecutive samples of a row strip; each
+  // lane requests a batch of samples before storing any
+  static_assert((3 * N * kStrip) % (kThreads * 16) == 0, "whole staging batches");
+  for (int e0 = 0; e0 < 3 * N * kStrip; e0 += kThreads * 16) {
+    float v[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = e0 + u * kThreads + (int)threadIdx.x;
+      const int c = e / (N * kStrip), rem = e % (N * kStrip), dy = rem / kStrip, xx = rem % kStrip;
+      const int jb = xx / N, dx = xx % N;
+      const int blk = min(base + jb, nblocks - 1);
+      const int bx = blk % g.bw, by = blk / g.bw;
+      const int x = min(8 * bx + dx, g.w - 1), y = min(8 * by + dy, g.h - 1);
+      v[u] = planes[c][(size_t)y * g.w + x];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+      const int e = e0 + u * kThreads + (int)threadIdx.x;
+      const int c = e / (N * kStrip), rem = e % (N * kStrip), dy = rem / kStrip, xx = rem % kStrip;
+      smp[c][dy * N + xx % N][xx / N] = v[u];
+    }
+  }
+  __syncthreads();
+  // atom variances: a lane per (channel, block)
+  {
+    const int c = threadIdx.x >> 6, jb = threadIdx.x & 63;
+    if (jb < nb) {
+      const int bx = (base + jb) % g.bw, by = (base + jb) / g.bw;
+      const int vw = min(N, g.w - 8 * bx), vh = min(N, g.h - 8 * by);
+      double sum = 0.0, sum_sq = 0.0;
+      for (int dy = 0; dy < vh; dy++)
+        for (int dx = 0; dx < vw; dx++) {
+          const double v = (double)smp[c][dy * N + dx][jb];
+          sum += v;
+          sum_sq += v * v;
+        }
+      const double count = (double)vw * (double)vh;
+      const double mean = sum / count;
+      double variance = sum_sq / count - mean * mean;
+      if (!(variance > 0.0)) variance = 0.0;
+      if (c == 1) luma_var[base + jb] = (float)variance;
+      else vxb[c >> 1][jb] = (float)variance;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < nb) chroma_var[base + threadIdx.x] = vxb[0][threadIdx.x] + vxb[1][threadIdx.x];
   const DctTables& t = tab->dct;
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  uint32_t n3 = 0, n2 = 0, nb = 0;
-  for (int round = 0; round < kSummaryRounds; round++) {
-    const int base = ((int)blockIdx.x * kSummaryRounds + round) * VB;
-    if (base >= nblocks) break;
+  for (int sub = 0; sub * VB < nb; sub++) {
     for (int e = threadIdx.x; e < VB * 3 * NN; e += kThreads) {
       const int j = e / (3 * NN), rem = e % (3 * NN), c = rem / NN, k = rem % NN;
-      const int dy = k / N, dx = k % N;
-      float v = 0.0f;
-      if (base + j < nblocks) {
-        const int bx = (base + j) % g.bw, by = (base + j) / g.bw;
-        const int x = min(8 * bx + dx, g.w - 1), y = min(8 * by + dy, g.h - 1);
-        v = planes[c][(size_t)y * g.w + x];
-      }
-      lds[((j * 3 + c) * N + dy) * P + dx] = v;
+      tile[((j * 3 + c) * N + k / N) * P + k % N] = smp[c][k][sub * VB + j];
     }
     __syncthreads();
     const int j = threadIdx.x / (3 * N), c = (threadIdx.x / N) % 3, line = threadIdx.x % N;
-    float* tile = lds + (j * 3 + c) * N * P;
-    fwd_pass1<N>(tile, line, t);
+    float* tl = tile + (j * 3 + c) * N * P;
+    fwd_pass1<N>(tl, line, t);
     __syncthreads();
     float out[N];
-    fwd_pass2<N>(tile, line, out, t);
+    fwd_pass2<N>(tl, line, out, t);
 #pragma unroll
-    for (int k = 0; k < N; k++) coef[j][c][k * N + line] = out[k];
+    for (int k = 0; k < N; k++) smp[c][k * N + line][sub * VB + j] = out[k];
     __syncthreads();
-    if (threadIdx.x < 2 * VB) {
-      const int jb = threadIdx.x >> 1;
-      if (base + jb < nblocks) {
-        if ((threadIdx.x & 1) == 0) {
-          const float* cy = coef[jb][1];
-          double ey = 0.0, lo = 0.0, hi = 0.0, rw = 0.0, cl = 0.0;
-          uint32_t a3 = 0, a2 = 0;
-          for (int cell = 1; cell < NN; cell++) {
-            const int row = cell >> 3, col = cell & 7;
-            const double v = (double)cy[cell];
-            const double en = v * v;
-            ey += en;
-            if (row + col <= 2) lo += en;
-            if (max(row, col) >= 4) hi += en;
-            if (row > col) rw += en;
-            else if (col > row) cl += en;
-            const double m = fabs(v);
-            a3 += m < 1e-3 ? 1u : 0u;
-            a2 += m < 1e-2 ? 1u : 0u;
-          }
-          bsum[jb][0] = ey;
-          bsum[jb][1] = lo;
-          bsum[jb][2] = hi;
-          bsum[jb][3] = rw;
-          bsum[jb][4] = cl;
-          bsum[jb][6] = (double)cy[0];
-          bcnt[jb][0] = a3;
-          bcnt[jb][1] = a2;
-          const double bin = fmin(fmax((log(ey + 1e-30) - (-46.0)) / 0.125, 0.0), (double)(kSummaryBins - 1));
-          atomicAdd(&hist[(int)bin], 1u);
-        } else {
-          double xb = 0.0;
-          for (int cell = 1; cell < NN; cell++) xb += (double)coef[jb][0][cell] * (double)coef[jb][0][cell];
-          for (int cell = 1; cell < NN; cell++) xb += (double)coef[jb][2][cell] * (double)coef[jb][2][cell];
-          bsum[jb][5] = xb;
+  }
+  // Per block, three waves in parallel (a lane per block): wave 0 the Y AC
+  // energy in cell order, the near-zero counts and the histogram bin; wave 1
+  // the low/high bands and X; wave 2 the row/column sides and B.
+  {
+    const int wv = threadIdx.x >> 6, jb = threadIdx.x & 63;
+    if (jb < nb) {
+      if (wv == 0) {
+        double ey = 0.0;
+        uint32_t a3 = 0, a2 = 0;
+        for (int cell = 1; cell < NN; cell++) {
+          const double v = (double)smp[1][cell][jb];
+          ey += v * v;
+          const double m = fabs(v);
+          a3 += m < 1e-3 ? 1u : 0u;
+          a2 += m < 1e-2 ? 1u : 0u;
         }
+        bsum[0][jb] = ey;
+        const double dc = (double)smp[1][0][jb];
+        bsum[6][jb] = dc;
+        bsum[7][jb] = dc * dc;
+        bcnt[0][jb] = a3;
+        bcnt[1][jb] = a2;
+        const double bin = fmin(fmax((log(ey + 1e-30) - (-46.0)) / 0.125, 0.0), (double)(kSummaryBins - 1));
+        atomicAdd(&lhist[(int)bin], 1u);
+      } else if (wv == 1) {
+        double lo = 0.0, hi = 0.0, xx = 0.0;
+        for (int cell = 1; cell < NN; cell++) {
+          const int row = cell >> 3, col = cell & 7;
+          const double v = (double)smp[1][cell][jb];
+          const double en = v * v;
+          if (row + col <= 2) lo += en;
+          if (max(row, col) >= 4) hi += en;
+          const double x = (double)smp[0][cell][jb];
+          xx += x * x;
+        }
+        bsum[1][jb] = lo;
+        bsum[2][jb] = hi;
+        bsum[5][jb] = xx;
+      } else {
+        double rw = 0.0, cl = 0.0, bb = 0.0;
+        for (int cell = 1; cell < NN; cell++) {
+          const int row = cell >> 3, col = cell & 7;
+          const double v = (double)smp[1][cell][jb];
+          const double en = v * v;
+          if (row > col) rw += en;
+          else if (col > row) cl += en;
+          const double b = (double)smp[2][cell][jb];
+          bb += b * b;
+        }
+        bsum[3][jb] = rw;
+        bsum[4][jb] = cl;
+        bsum[8][jb] = bb;
       }
     }
-    __syncthreads();
-    if (threadIdx.x == 0)
-      for (int jb = 0; jb < VB && base + jb < nblocks; jb++) {
-#pragma unroll
-        for (int q = 0; q < 6; q++) acc[q] += bsum[jb][q];
-        acc[6] += bsum[jb][6];
-        acc[7] += bsum[jb][6] * bsum[jb][6];
-        n3 += bcnt[jb][0];
-        n2 += bcnt[jb][1];
-        nb++;
-      }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) {
+  __syncthreads();
+  // the workgroup's sums in block order, one quantity per thread
+  const int q = threadIdx.x;
+  if (q < 8) {
+    double acc = 0.0;
+    for (int jb = 0; jb < nb; jb++) acc += q == 5 ? bsum[5][jb] + bsum[8][jb] : bsum[q][jb];
+    double* o = &parts[blockIdx.x].total_y;
+    o[q] = acc;
+  } else if (q == 8) {
+    uint32_t n3 = 0, n2 = 0;
+    for (int jb = 0; jb < nb; jb++) {
+      n3 += bcnt[0][jb];
+      n2 += bcnt[1][jb];
+    }
     SummaryPartial& o = parts[blockIdx.x];
-    o.total_y = acc[0];
-    o.total_low = acc[1];
-    o.total_high = acc[2];
-    o.total_row = acc[3];
-    o.total_col = acc[4];
-    o.total_xb = acc[5];
-    o.dc_sum = acc[6];
-    o.dc_sumsq = acc[7];
     o.near_1e3 = n3;
     o.near_1e2 = n2;
-    o.blocks = nb;
-    o.ac_cells = nb * 63u;
+    o.blocks = (uint32_t)nb;
+    o.ac_cells = (uint32_t)nb * 63u;
   }
+  for (int i = threadIdx.x; i < kSummaryBins; i += kThreads)
+    if (lhist[i]) atomicAdd(&hist[i], lhist[i]);
 }
 
 __global__ void k_fill_u32(uint32_t* p, int n, uint32_t v) {
@@ -1573,12 +1633,13 @@ void launch_xyb_fwd(hipStream_t s, const Geom& g, int bits, const void* rgb, con
                                                     lin[1], lin[2], gray);
 }
 
-int summary_groups(const Geom& g) { return (int)div_up((size_t)g.bw * g.bh, 8 * kSummaryRounds); }
+int summary_groups(const Geom& g) { return (int)div_up((size_t)g.bw * g.bh, kSummaryBlocks); }
 
-void launch_dct8_summary(hipStream_t s, const Geom& g, const ConstTables* tab, float* const xyb[3],
-                         SummaryPartial* parts, uint32_t* hist) {
+void launch_block_features(hipStream_t s, const Geom& g, const ConstTables* tab, float* const xyb[3],
+                           float* luma_var, float* chroma_var, SummaryPartial* parts, uint32_t* hist) {
   k_fill_u32<<<div_up(kSummaryBins, 256), 256, 0, s>>>(hist, kSummaryBins, 0u);
-  k_dct8_summary<<<summary_groups(g), kThreads, 0, s>>>(g, tab, xyb[0], xyb[1], xyb[2], parts, hist);
+  k_dct8_summary<<<summary_groups(g), kThreads, 0, s>>>(g, tab, xyb[0], xyb[1], xyb[2], luma_var, chroma_var,
+                                                         parts, hist);
 }
 
 template <int N, int FORM>

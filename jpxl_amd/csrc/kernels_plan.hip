@@ -12,6 +12,7 @@
 // strictly cheaper than its split.
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 
 #include "internal.h"
 
@@ -1011,40 +1012,97 @@ __global__ __launch_bounds__(256) void k_aq_activity(Geom g, const float* __rest
   act[atom] = luma + 0.35f * chroma;
 }
 
-// analysis.rs:111-158 for the Y plane alone: each atom's population
-// variance over the samples it holds (f64 sums in raster order, floored at
-// zero, narrowed once), the input of the frame features
-// (quality_features.rs:173-213).
-__device__ __forceinline__ double atom_variance(const Geom& g, const float* __restrict__ P, int x0, int y0, int x1,
-                                                int y1) {
-  const double count = (double)(x1 - x0) * (double)(y1 - y0);
-  double sum = 0.0, sum_sq = 0.0;
-  for (int y = y0; y < y1; y++)
-    for (int x = x0; x < x1; x++) {
-      const double v = (double)P[(size_t)y * g.w + x];
-      sum += v;
-      sum_sq += v * v;
-    }
-  const double mean = sum / count;
-  double variance = sum_sq / count - mean * mean;
-  if (!(variance > 0.0)) variance = 0.0;
-  return variance;
+// Exact order statistics of the per-atom variances (non-negative f32, so
+// their bit patterns sort as the values do) by radix selection over three
+// digits (bits 31-21, 20-10, 9-0): each pass counts the digit of the
+// elements that share the prefix found so far, in LDS per workgroup (hot
+// bins stay on-chip) and then once per non-empty bin in global memory; the
+// select step finds the digit holding each query's rank.  Integer counts:
+// the same answer in any order.
+constexpr int kSelectBlocks = 128;
+
+__global__ void k_zero_u32(uint32_t* p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
-// luma: the Y variance; chroma (when given): X variance + B variance, each
-// narrowed to f32 first (quality_features.rs:183-187).
-__global__ __launch_bounds__(256) void k_atlas_var(Geom g, const float* __restrict__ X, const float* __restrict__ Y,
-                                                   const float* __restrict__ B, float* __restrict__ luma,
-                                                   float* __restrict__ chroma) {
-  const int atom = blockIdx.x * blockDim.x + threadIdx.x;
-  if (atom >= g.bw * g.bh) return;
-  const int ax = atom % g.bw, ay = atom / g.bw;
-  const int x0 = ax * 8, y0 = ay * 8, x1 = min(x0 + 8, g.w), y1 = min(y0 + 8, g.h);
-  luma[atom] = (float)atom_variance(g, Y, x0, y0, x1, y1);
-  if (chroma) {
-    const float vx = (float)atom_variance(g, X, x0, y0, x1, y1);
-    const float vb = (float)atom_variance(g, B, x0, y0, x1, y1);
-    chroma[atom] = vx + vb;
+template <int PASS>
+__global__ __launch_bounds__(256) void k_select_hist(const float* __restrict__ luma, const float* __restrict__ chroma,
+                                                     int n, float flat_below, SelectRanks r,
+                                                     SelectScratch* __restrict__ s) {
+  constexpr int kShift = PASS == 0 ? 21 : (PASS == 1 ? 10 : 0);
+  constexpr int kBins = PASS == 2 ? 1024 : 2048;
+  constexpr int kSets = PASS == 0 ? 2 : kSelectQueries;
+  __shared__ uint32_t hist[kSets][kBins];
+  __shared__ uint32_t flat;
+  for (int i = threadIdx.x; i < kSets * kBins; i += 256) (&hist[0][0])[i] = 0u;
+  if (threadIdx.x == 0) flat = 0u;
+  uint32_t prefix[kSelectQueries];
+  if constexpr (PASS > 0)
+    for (int q = 0; q < kSelectQueries; q++) prefix[q] = s->prefix[q];
+  __syncthreads();
+  uint32_t my_flat = 0;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += kSelectBlocks * 256) {
+    const uint32_t ul = __float_as_uint(luma[i]), uc = __float_as_uint(chroma[i]);
+    if constexpr (PASS == 0) {
+      atomicAdd(&hist[0][ul >> kShift], 1u);
+      atomicAdd(&hist[1][uc >> kShift], 1u);
+      my_flat += luma[i] < flat_below ? 1u : 0u;
+    } else {
+#pragma unroll
+      for (int q = 0; q < kSelectQueries; q++) {
+        const uint32_t u = r.set[q] == 0 ? ul : uc;
+        if ((u >> (kShift + (PASS == 1 ? 11 : 10))) == prefix[q]) atomicAdd(&hist[q][(u >> kShift) & (kBins - 1)], 1u);
+      }
+    }
+  }
+  if constexpr (PASS == 0)
+    if (my_flat) atomicAdd(&flat, my_flat);
+  __syncthreads();
+  uint32_t* out = PASS == 0 ? &s->h1[0][0] : (PASS == 1 ? &s->h2[0][0] : &s->h3[0][0]);
+  for (int i = threadIdx.x; i < kSets * kBins; i += 256) {
+    const uint32_t c = (&hist[0][0])[i];
+    if (c) atomicAdd(&out[i], c);
+  }
+  if (PASS == 0 && threadIdx.x == 0 && flat) atomicAdd(&s->flat, flat);
+}
+
+// One workgroup per query: the digit whose cumulative count first exceeds
+// the query's remaining rank; the prefix gains the digit, the rank loses the
+// counts below it.  On the last pass the prefix is the value.
+template <int PASS>
+__global__ __launch_bounds__(256) void k_select_digit(SelectRanks r, SelectScratch* __restrict__ s) {
+  constexpr int kBins = PASS == 2 ? 1024 : 2048, kPer = kBins / 256;
+  constexpr int kDigitBits = PASS == 2 ? 10 : 11;
+  __shared__ uint32_t part[256];
+  const int q = blockIdx.x, t = threadIdx.x;
+  const uint32_t* hist = PASS == 0 ? s->h1[r.set[q]] : (PASS == 1 ? s->h2[q] : s->h3[q]);
+  const uint32_t rank = PASS == 0 ? r.rank[q] : s->rank[q];
+  uint32_t sum = 0;
+  for (int b = 0; b < kPer; b++) sum += hist[t * kPer + b];
+  part[t] = sum;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    const uint32_t v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  const uint32_t before = part[t] - sum;
+  if (rank >= before && rank < part[t]) {
+    uint32_t cum = before;
+    for (int b = 0; b < kPer; b++) {
+      const uint32_t c = hist[t * kPer + b];
+      if (rank < cum + c) {
+        const uint32_t digit = (uint32_t)(t * kPer + b);
+        const uint32_t prefix = (PASS == 0 ? 0u : s->prefix[q] << kDigitBits) | digit;
+        s->prefix[q] = prefix;
+        s->rank[q] = rank - cum;
+        if (PASS == 2) s->value[q] = __uint_as_float(prefix);
+        break;
+      }
+      cum += c;
+    }
   }
 }
 
@@ -1063,10 +1121,27 @@ __global__ __launch_bounds__(256) void k_preanalysis_cells(int w, int h, int shi
     if constexpr (U8) return (int)((const uint8_t*)rgb)[i];
     return min((int)(((const uint16_t*)rgb)[i] >> shift), 255);
   };
-  for (int e = threadIdx.x; e < 34 * 34; e += 256) {
+  // the tile's samples (34 x 34 = 1156 < 5 x 256) and the lane's four
+  // interior pixels' are requested before any is used
+  int tr[5][3], px3[4][3];
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const int e = min((int)threadIdx.x + 256 * k, 34 * 34 - 1);
     const int ly = e / 34, lx = e % 34;
     const int x = min(max(x0 - 1 + lx, 0), w - 1), y = min(max(y0 - 1 + ly, 0), h - 1);
-    lum[ly][lx] = (77 * sample(x, y, 0) + 150 * sample(x, y, 1) + 29 * sample(x, y, 2) + 128) >> 8;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) tr[k][ch] = sample(x, y, ch);
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const int p = (int)threadIdx.x + 256 * k;
+#pragma unroll
+    for (int ch = 0; ch < 3; ch++) px3[k][ch] = sample(x0 + (p & 31), y0 + (p >> 5), ch);
+  }
+#pragma unroll
+  for (int k = 0; k < 5; k++) {
+    const int e = (int)threadIdx.x + 256 * k;
+    if (e < 34 * 34) lum[e / 34][e % 34] = (77 * tr[k][0] + 150 * tr[k][1] + 29 * tr[k][2] + 128) >> 8;
   }
   __syncthreads();
   // sum, sum_sq, edge, flat, noise_sum, weak, dir[4], cb_sum, cb_sq, cr_sum, cr_sq
@@ -1095,8 +1170,7 @@ __global__ __launch_bounds__(256) void k_preanalysis_cells(int w, int h, int shi
       a[4] += abs(c - box);
       a[5]++;
     }
-    const int x = x0 + px, y = y0 + py;
-    const int r = sample(x, y, 0), gg = sample(x, y, 1), b = sample(x, y, 2);
+    const int r = px3[k][0], gg = px3[k][1], b = px3[k][2];
     const int cb = min(max(((-43 * r - 85 * gg + 128 * b + 128) >> 8) + 128, 0), 255);
     const int cr = min(max(((128 * r - 107 * gg - 21 * b + 128) >> 8) + 128, 0), 255);
     a[10] += cb;
@@ -1538,14 +1612,17 @@ void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], con
   k_gather_lf_groups<<<div_up(bpx, 256), 256, 0, s>>>(g, lf[0], lf[1], lf[2], lf_out);
 }
 
-void launch_atlas_luma_var(hipStream_t s, const Geom& g, const float* Y, float* var) {
-  const int count = g.bw * g.bh;
-  k_atlas_var<<<div_up(count, 256), 256, 0, s>>>(g, Y, Y, Y, var, nullptr);
-}
 
-void launch_atlas_var(hipStream_t s, const Geom& g, float* const xyb[3], float* luma, float* chroma) {
-  const int count = g.bw * g.bh;
-  k_atlas_var<<<div_up(count, 256), 256, 0, s>>>(g, xyb[0], xyb[1], xyb[2], luma, chroma);
+void launch_select(hipStream_t s, const float* luma, const float* chroma, int n, float flat_below,
+                   const SelectRanks& r, SelectScratch* sc) {
+  const int words = (int)(offsetof(SelectScratch, prefix) / 4);
+  k_zero_u32<<<div_up(words, 256), 256, 0, s>>>((uint32_t*)sc, words);
+  k_select_hist<0><<<kSelectBlocks, 256, 0, s>>>(luma, chroma, n, flat_below, r, sc);
+  k_select_digit<0><<<kSelectQueries, 256, 0, s>>>(r, sc);
+  k_select_hist<1><<<kSelectBlocks, 256, 0, s>>>(luma, chroma, n, flat_below, r, sc);
+  k_select_digit<1><<<kSelectQueries, 256, 0, s>>>(r, sc);
+  k_select_hist<2><<<kSelectBlocks, 256, 0, s>>>(luma, chroma, n, flat_below, r, sc);
+  k_select_digit<2><<<kSelectQueries, 256, 0, s>>>(r, sc);
 }
 
 int preanalysis_cells(int w, int h) { return (w / 32) * (h / 32); }

@@ -25,20 +25,6 @@ namespace {
 // Source features (quality_features.rs)
 // ---------------------------------------------------------------------------
 
-constexpr float kFlatVariance = 2e-5f;  // quality_features.rs:19
-
-// quantile (quality_features.rs:141-152): the element at floor((n-1) q) of
-// the ascending order.
-float floor_quantile(std::vector<float>& v, float q) {
-  if (v.empty()) return 0.0f;
-  const size_t last = v.size() - 1;
-  const double qq = (double)std::min(std::max(q, 0.0f), 1.0f);
-  size_t index = (size_t)floor((double)last * qq);
-  index = std::min(index, last);
-  std::nth_element(v.begin(), v.begin() + index, v.end(), [](float a, float b) { return a < b; });
-  return v[index];
-}
-
 // ---------------------------------------------------------------------------
 // Preanalysis (preanalysis.rs)
 // ---------------------------------------------------------------------------
@@ -87,22 +73,32 @@ int classify(const Cell& c) {
   return kTexture;
 }
 
-// quantile (preanalysis.rs:243-268): linear interpolation between order
-// statistics (numpy's default) over an ascending vector.
-double lerp_quantile(const std::vector<double>& s, double q) {
-  if (s.empty()) return 0.0;
-  const double pos = q * (double)(s.size() - 1);
-  const size_t lo = (size_t)floor(pos);
-  const double frac = pos - (double)lo;
-  const double a = lo < s.size() ? s[lo] : s.back();
-  const double b = lo + 1 < s.size() ? s[lo + 1] : a;
-  return a + (b - a) * frac;
+// log2(1 + v) for the integer cell statistics (variance and chroma, both
+// below 65536), as the reference evaluates it.
+const double* log2_1p_table() {
+  static const std::vector<double> t = [] {
+    std::vector<double> v(65536);
+    for (int i = 0; i < 65536; i++) v[i] = log2(1.0 + (double)i);
+    return v;
+  }();
+  return t.data();
 }
 
-double sequential_mean(const std::vector<double>& v, double count) {
-  double s = 0.0;
-  for (double x : v) s += x;
-  return s / count;
+// quantile (preanalysis.rs:243-268), linear interpolation between the order
+// statistics at floor(q (n-1)) and the next, over values f(key) of integer
+// keys that f maps monotonically (so the keys' order is the values').
+template <typename F>
+double lerp_quantile_keys(std::vector<uint32_t> keys, double q, F f) {
+  if (keys.empty()) return 0.0;
+  const size_t n = keys.size();
+  const double pos = q * (double)(n - 1);
+  const size_t lo = (size_t)floor(pos);
+  const double frac = pos - (double)lo;
+  std::nth_element(keys.begin(), keys.begin() + lo, keys.end());
+  const double a = f(keys[lo]);
+  double b = a;
+  if (lo + 1 < n) b = f(*std::min_element(keys.begin() + lo + 1, keys.end()));
+  return a + (b - a) * frac;
 }
 
 // ---------------------------------------------------------------------------
@@ -407,15 +403,14 @@ bool predict_cases(const jpxlg_source_features& f, const jpxlg_transform_summary
 
 }  // namespace
 
-void source_features_from(std::vector<float> luma, std::vector<float> chroma, jpxlg_source_features* f) {
-  size_t flat = 0;
-  for (float v : luma) flat += v < kFlatVariance ? 1 : 0;
-  f->flat_fraction = luma.empty() ? 0.0f : (float)flat / (float)luma.size();
-  f->luma_variance_q10 = floor_quantile(luma, 0.1f);
-  f->luma_variance_q50 = floor_quantile(luma, 0.5f);
-  f->luma_variance_q90 = floor_quantile(luma, 0.9f);
-  f->chroma_variance_q50 = floor_quantile(chroma, 0.5f);
-  f->edge_proxy = f->luma_variance_q90 - f->luma_variance_q50;
+void source_features_set(float q10, float q50, float q90, float chroma_q50, uint32_t flat, uint32_t n,
+                         jpxlg_source_features* f) {
+  f->flat_fraction = n ? (float)flat / (float)n : 0.0f;
+  f->luma_variance_q10 = q10;
+  f->luma_variance_q50 = q50;
+  f->luma_variance_q90 = q90;
+  f->chroma_variance_q50 = chroma_q50;
+  f->edge_proxy = q90 - q50;
 }
 
 void preanalysis_from_cells(const CellStats* cs, int n, jpxlg_source_features* f) {
@@ -449,28 +444,31 @@ void preanalysis_from_cells(const CellStats* cs, int n, jpxlg_source_features* f
   float share[7] = {0, 0, 0, 0, 0, 0, 0};
   for (const Cell& c : cells) share[c.cls] += 1.0f;
   for (float& s : share) s /= (float)count;
-  std::vector<double> lv(n), edge(n), noise(n), chroma(n);
+  const double* L = log2_1p_table();
+  std::vector<uint32_t> var_k(n), edge_k(n), noise_k(n), chroma_k(n);
+  double lv_sum = 0.0, edge_sum = 0.0, noise_sum = 0.0, chroma_sum = 0.0, flat_sum = 0.0;
   for (int k = 0; k < n; k++) {
-    lv[k] = log2(1.0 + (double)cells[k].variance);
-    edge[k] = (double)cells[k].edge_q8 / 256.0;
-    noise[k] = (double)cells[k].noise;
-    chroma[k] = log2(1.0 + (double)cells[k].chroma);
+    const Cell& c = cells[k];
+    var_k[k] = c.variance;
+    edge_k[k] = (uint32_t)c.edge_q8;
+    noise_k[k] = (uint32_t)c.noise;
+    chroma_k[k] = c.chroma;
+    lv_sum += L[c.variance];
   }
-  const double lv_mean = sequential_mean(lv, count);
+  const double lv_mean = lv_sum / count;
   double ss = 0.0;
-  for (double v : lv) ss += (v - lv_mean) * (v - lv_mean);
-  const double lv_std = sqrt(ss / count);
   int64_t hetero = 0;
-  for (double v : lv) hetero += fabs(v - lv_mean) > kHeteroBits ? 1 : 0;
-  const double edge_mean = sequential_mean(edge, count);
-  double flat_sum = 0.0;
+  for (int k = 0; k < n; k++) {
+    const double v = L[cells[k].variance];
+    ss += (v - lv_mean) * (v - lv_mean);
+  }
+  for (int k = 0; k < n; k++) hetero += fabs(L[cells[k].variance] - lv_mean) > kHeteroBits ? 1 : 0;
+  const double lv_std = sqrt(ss / count);
+  for (int k = 0; k < n; k++) edge_sum += (double)cells[k].edge_q8 / 256.0;
   for (const Cell& c : cells) flat_sum += (double)c.flat_q8 / 256.0;
-  const double noise_mean = sequential_mean(noise, count);
-  const double chroma_mean = sequential_mean(chroma, count);
-  std::sort(lv.begin(), lv.end());
-  std::sort(edge.begin(), edge.end());
-  std::sort(noise.begin(), noise.end());
-  std::sort(chroma.begin(), chroma.end());
+  for (int k = 0; k < n; k++) noise_sum += (double)cells[k].noise;
+  for (int k = 0; k < n; k++) chroma_sum += L[cells[k].chroma];
+  const double edge_mean = edge_sum / count, noise_mean = noise_sum / count, chroma_mean = chroma_sum / count;
   double ent = 0.0, dom = 0.0, axis = 0.0;
   int64_t edged = 0;
   for (const Cell& c : cells)
@@ -482,18 +480,21 @@ void preanalysis_from_cells(const CellStats* cs, int n, jpxlg_source_features* f
     if (c.has_edges) dom += (double)c.dominance_q8 / 256.0;
   for (const Cell& c : cells)
     if (c.has_edges) axis += (double)c.axis_q8 / 256.0;
+  auto lv_of = [L](uint32_t v) { return L[v]; };
+  auto edge_of = [](uint32_t v) { return (double)v / 256.0; };
+  auto noise_of = [](uint32_t v) { return (double)v; };
   for (int i = 0; i < 7; i++) f->share[i] = share[i];
   f->log2var_mean = (float)lv_mean;
   f->log2var_std = (float)lv_std;
-  f->log2var_q10 = (float)lerp_quantile(lv, 0.1);
-  f->log2var_q90 = (float)lerp_quantile(lv, 0.9);
+  f->log2var_q10 = (float)lerp_quantile_keys(var_k, 0.1, lv_of);
+  f->log2var_q90 = (float)lerp_quantile_keys(var_k, 0.9, lv_of);
   f->edge_mean = (float)edge_mean;
-  f->edge_q90 = (float)lerp_quantile(edge, 0.9);
+  f->edge_q90 = (float)lerp_quantile_keys(edge_k, 0.9, edge_of);
   f->flat_mean = (float)(flat_sum / count);
   f->noise_mean = (float)noise_mean;
-  f->noise_q90 = (float)lerp_quantile(noise, 0.9);
+  f->noise_q90 = (float)lerp_quantile_keys(noise_k, 0.9, noise_of);
   f->chroma_log2_mean = (float)chroma_mean;
-  f->chroma_log2_q90 = (float)lerp_quantile(chroma, 0.9);
+  f->chroma_log2_q90 = (float)lerp_quantile_keys(chroma_k, 0.9, lv_of);
   f->orient_entropy_mean = edged ? (float)(ent / (double)edged) : 0.0f;
   f->dir_dominance_mean = edged ? (float)(dom / (double)edged) : 0.0f;
   f->axis_aligned_mean = edged ? (float)(axis / (double)edged) : 0.0f;

@@ -32,9 +32,14 @@ struct SummaryPartial {
 static_assert(sizeof(SummaryPartial) == 80, "device record layout");
 constexpr int kSummaryBins = 512;  // lib.rs:2776-2779 HIST_BINS
 
-// SourceFeatures without the preanalysis, from per-atom variances
-// (quality_features.rs:173-213).
-void source_features_from(std::vector<float> luma, std::vector<float> chroma, jpxlg_source_features* f);
+// quality_features.rs:19 FLAT_VARIANCE.
+constexpr float kFlatVariance = 2e-5f;
+
+// SourceFeatures without the preanalysis (quality_features.rs:173-213) from
+// the per-atom variances' order statistics (floor-indexed q10/q50/q90 of
+// luma, q50 of chroma) and the count of atoms below FLAT_VARIANCE among n.
+void source_features_set(float q10, float q50, float q90, float chroma_q50, uint32_t flat, uint32_t n,
+                         jpxlg_source_features* f);
 
 // PreanalysisFeatures of the cell records (preanalysis.rs:486-540).
 void preanalysis_from_cells(const CellStats* cells, int n, jpxlg_source_features* f);
