@@ -438,15 +438,19 @@ def served_form(args) -> int:
         t_end = time.perf_counter() - t0
         srv.close()
         two = np.array([abs(mps[k] - 2.0) <= 0.5 for k in range(n)])
+        two85 = two & (targets == 85.0)
         out["rates"][str(int(rate))] = {
             "p50_ms": round(1000 * float(np.percentile(lat, 50)), 2),
             "p99_ms": round(1000 * float(np.percentile(lat, 99)), 2),
             "p99_ms_2mp": round(1000 * float(np.percentile(lat[two], 99)), 2) if two.any() else None,
+            "p99_ms_2mp_q85": round(1000 * float(np.percentile(lat[two85], 99)), 2) if two85.any() else None,
+            "n_2mp_q85": int(two85.sum()),
             "mps": round(float(mps.sum()) / t_end, 2),
             "served_rate": round(n / t_end, 1),
             "met": int(met),
             "peak_queued": int(peak_queue),
         }
+        print(f"rate {rate:g}: {out['rates'][str(int(rate))]}", file=sys.stderr, flush=True)
     out["value"] = out["rates"][str(int(rates[0]))]["p99_ms"]
     print(json.dumps(out))
     ctx.close()

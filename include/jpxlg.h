@@ -382,10 +382,13 @@ int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quali
  * footprint (jpxlg_image_footprint) of the admitted, unreleased ones fits
  * its budget (one is always admitted when none holds memory); admitting
  * creates the request's image, releasing it frees it.  Until admitted a
- * request is pending and polls 0. */
+ * request is pending and polls 0.  When the device cannot hold an image the
+ * budget allowed while others are admitted, the request goes back to the
+ * head of the queue and the budget drops to the admitted footprint. */
 int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, uint32_t height, uint32_t bits,
                                  const void* rgb, const jpxlg_quality_request* req, uint64_t* ticket);
-/* The admission budget in bytes (default: 90% of the device's memory). */
+/* The admission budget in bytes (default: 90% of the device's free memory
+ * when the server is created). */
 int32_t jpxlg_server_set_budget(jpxlg_server* srv, uint64_t bytes);
 /* The admitted requests' projected footprint, the budget, and the requests
  * waiting for admission (each pointer optional). */

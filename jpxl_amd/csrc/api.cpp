@@ -1684,12 +1684,41 @@ extern "C" int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* hos
 // of the source samples (preanalysis.rs:277-484) and the DCT8 summary
 // (lib.rs:2769-2955) on the device; their order statistics and reductions on
 // the host.  Computed once per image (the source planes do not change).
-static int image_features(jpxlg_img* img) {
-  if (img->features_known) return JPXLG_OK;
+// Where an image's feature results land in its share of the pinned
+// staging: the order-statistic tail, the grayscale flag, the cell sums, the
+// summary partials and the histogram.
+struct FeatureStaging {
+  size_t sel_off, sel_bytes, o_gray, o_cells, o_parts, o_hist, bytes;
+  explicit FeatureStaging(const Geom& g) {
+    sel_off = offsetof(SelectScratch, flat);
+    sel_bytes = sizeof(SelectScratch) - sel_off;
+    o_gray = 192;
+    o_cells = 256;
+    o_parts = o_cells + (size_t)preanalysis_cells(g.w, g.h) * sizeof(CellStats);
+    o_hist = o_parts + (size_t)summary_groups(g) * sizeof(SummaryPartial);
+    bytes = (o_hist + kSummaryBins * 4 + 255) & ~(size_t)255;
+  }
+};
+
+static int readback_staging(jpxlg_ctx* ctx, size_t need) {
+  if (ctx->rb_cap >= need) return JPXLG_OK;
+  if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
+  ctx->rb_host = nullptr;
+  ctx->rb_cap = 0;
+  if (hipHostMalloc((void**)&ctx->rb_host, need, hipHostMallocDefault) != hipSuccess)
+    return fail(ctx, JPXLG_E_OOM, "readback staging allocation failed");
+  ctx->rb_cap = need;
+  return JPXLG_OK;
+}
+
+// Launches an image's feature kernels on the context's stream and the copies
+// of their results into h (FeatureStaging(g).bytes of pinned memory).
+static int features_enqueue(jpxlg_img* img, char* h) {
   jpxlg_ctx* ctx = img->ctx;
   const Geom& g = img->g;
   const int n = g.bw * g.bh;
   const int ncells = preanalysis_cells(g.w, g.h), nparts = summary_groups(g);
+  const FeatureStaging L(g);
   hipStream_t st = ctx->stream;
   // cover scratch, free between structure builds
   float* d_luma = (float*)img->d_cost[0];
@@ -1709,55 +1738,93 @@ static int image_features(jpxlg_img* img) {
     launch_preanalysis_cells(st, g.w, g.h, img->bits, img->d_rgb, img->d_cells);
   }
   HIP_TRY(ctx, hipGetLastError());
-  const size_t sel_off = offsetof(SelectScratch, flat), sel_bytes = sizeof(SelectScratch) - sel_off;
-  const size_t o_cells = 256;
-  const size_t o_parts = o_cells + (size_t)ncells * sizeof(CellStats);
-  const size_t o_hist = o_parts + (size_t)nparts * sizeof(SummaryPartial);
-  const size_t need = o_hist + kSummaryBins * 4;
-  if (ctx->rb_cap < need) {
-    if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
-    ctx->rb_host = nullptr;
-    ctx->rb_cap = 0;
-    if (hipHostMalloc((void**)&ctx->rb_host, need, hipHostMallocDefault) != hipSuccess)
-      return fail(ctx, JPXLG_E_OOM, "readback staging allocation failed");
-    ctx->rb_cap = need;
-  }
-  char* h = (char*)ctx->rb_host;
-  HIP_TRY(ctx, hipMemcpyAsync(h, (const char*)ctx->d_select + sel_off, sel_bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(h, (const char*)ctx->d_select + L.sel_off, L.sel_bytes, hipMemcpyDeviceToHost, st));
+  HIP_TRY(ctx, hipMemcpyAsync(h + L.o_gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
   if (ncells)
-    HIP_TRY(ctx, hipMemcpyAsync(h + o_cells, img->d_cells, (size_t)ncells * sizeof(CellStats),
+    HIP_TRY(ctx, hipMemcpyAsync(h + L.o_cells, img->d_cells, (size_t)ncells * sizeof(CellStats),
                                 hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(h + o_parts, img->d_sum_parts, (size_t)nparts * sizeof(SummaryPartial),
+  HIP_TRY(ctx, hipMemcpyAsync(h + L.o_parts, img->d_sum_parts, (size_t)nparts * sizeof(SummaryPartial),
                               hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipMemcpyAsync(h + o_hist, img->d_hist, kSummaryBins * 4, hipMemcpyDeviceToHost, st));
-  if (!img->gray_known) HIP_TRY(ctx, hipMemcpyAsync(&img->gray, img->d_gray, 4, hipMemcpyDeviceToHost, st));
-  HIP_TRY(ctx, hipStreamSynchronize(st));
-  img->gray_known = true;
+  HIP_TRY(ctx, hipMemcpyAsync(h + L.o_hist, img->d_hist, kSummaryBins * 4, hipMemcpyDeviceToHost, st));
+  return JPXLG_OK;
+}
+
+// The host half, once the copies of features_enqueue have landed.
+static void features_finish(jpxlg_img* img, const char* h) {
+  const Geom& g = img->g;
+  const FeatureStaging L(g);
+  if (!img->gray_known) {
+    memcpy(&img->gray, h + L.o_gray, 4);
+    img->gray_known = true;
+  }
   SelectScratch sel;  // only its tail (flat count onward) was read back
-  memcpy((char*)&sel + sel_off, h, sel_bytes);
+  memcpy((char*)&sel + L.sel_off, h, L.sel_bytes);
   jpxlg_source_features& f = img->sf;
   f = jpxlg_source_features{};
   f.width = (uint32_t)g.w;
   f.height = (uint32_t)g.h;
   f.grayscale = img->gray ? 1 : 0;
-  source_features_set(sel.value[0], sel.value[1], sel.value[2], sel.value[3], sel.flat, (uint32_t)n, &f);
-  preanalysis_from_cells((const CellStats*)(h + o_cells), ncells, &f);
-  transform_summary_from((const SummaryPartial*)(h + o_parts), nparts, (const uint32_t*)(h + o_hist), &img->ts);
+  source_features_set(sel.value[0], sel.value[1], sel.value[2], sel.value[3], sel.flat, (uint32_t)(g.bw * g.bh),
+                      &f);
+  preanalysis_from_cells((const CellStats*)(h + L.o_cells), preanalysis_cells(g.w, g.h), &f);
+  transform_summary_from((const SummaryPartial*)(h + L.o_parts), summary_groups(g),
+                         (const uint32_t*)(h + L.o_hist), &img->ts);
   img->features_known = true;
+}
+
+// The first-guess features (quality.rs:2449-2474) of several images of one
+// context: per-atom variances of the XYB planes (quality_features.rs:173-213),
+// the 32x32 preanalysis cells of the source samples (preanalysis.rs:277-484)
+// and the DCT8 summary (lib.rs:2769-2955) on the device, one synchronisation
+// for all; their reductions on the host.  Computed once per image (the
+// source planes do not change).
+static int images_features(jpxlg_img* const* imgs, int n) {
+  constexpr size_t kChunkBytes = (size_t)16 << 20;  // pinned staging per synchronisation
+  if (n == 0) return JPXLG_OK;
+  jpxlg_ctx* ctx = imgs[0]->ctx;
+  for (int k0 = 0; k0 < n;) {
+    // a run of images whose staging fits the chunk (at least one)
+    std::vector<size_t> off(1, 0);
+    int k1 = k0;
+    while (k1 < n) {
+      const size_t b = imgs[k1]->features_known ? 0 : FeatureStaging(imgs[k1]->g).bytes;
+      if (k1 > k0 && off.back() + b > kChunkBytes) break;
+      off.push_back(off.back() + b);
+      k1++;
+    }
+    if (off.back() > 0) {
+      int rc = readback_staging(ctx, off.back());
+      if (rc) return rc;
+      char* h = (char*)ctx->rb_host;
+      for (int k = k0; k < k1; k++)
+        if (!imgs[k]->features_known && (rc = features_enqueue(imgs[k], h + off[k - k0]))) return rc;
+      HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+      for (int k = k0; k < k1; k++)
+        if (!imgs[k]->features_known) features_finish(imgs[k], h + off[k - k0]);
+    }
+    k0 = k1;
+  }
   return JPXLG_OK;
 }
 
-// The first probe and the navigator's prior exponent (quality.rs:2449-2530):
-// the crossing predictor's rung where it is trusted, else the legacy
-// table's (quality.rs:2452-2460).
-static int predicted_start(jpxlg_img* img, double target, int32_t effort, uint32_t* rung, double* prior_beta) {
-  const int rc = image_features(img);
-  if (rc) return rc;
+static int image_features(jpxlg_img* img) { return images_features(&img, 1); }
+
+// The first probe and the navigator's prior exponent (quality.rs:2449-2530)
+// from an image's features: the crossing predictor's rung where it is
+// trusted, else the legacy table's (quality.rs:2452-2460).
+static void start_from_features(const jpxlg_img* img, double target, int32_t effort, uint32_t* rung,
+                                double* prior_beta) {
   const uint32_t legacy =
       rung_for_scale(predicted_effective_scale(img->sf.luma_variance_q50, img->sf.flat_fraction, target));
   const QualityStart q = quality_start(img->sf, img->ts, target, quality_policy(effort).reserve, legacy);
   *rung = q.rung;
   *prior_beta = q.prior_beta;
+}
+
+static int predicted_start(jpxlg_img* img, double target, int32_t effort, uint32_t* rung, double* prior_beta) {
+  const int rc = image_features(img);
+  if (rc) return rc;
+  start_from_features(img, target, effort, rung, prior_beta);
   return JPXLG_OK;
 }
 
@@ -2130,6 +2197,9 @@ struct jpxlg_server {
   // budget (one request is always admitted when none holds memory).
   std::deque<uint64_t> queue;
   size_t budget = 0, used = 0;
+  // Requests whose image exists and whose navigator has not started yet
+  // (a step starts them together, after one batched feature pass).
+  std::vector<uint64_t> unstarted;
   ~jpxlg_server() {
     for (auto& l : lane)
       if (l.stream) {
@@ -2148,7 +2218,7 @@ static int server_init(jpxlg_server* srv, jpxlg_ctx* ctx) {
   for (auto& l : srv->lane) HIP_TRY(ctx, hipStreamCreateWithFlags(&l.stream, hipStreamNonBlocking));
   size_t free_b = 0, total_b = 0;
   HIP_TRY(ctx, hipMemGetInfo(&free_b, &total_b));
-  srv->budget = total_b / 10 * 9;
+  srv->budget = free_b / 10 * 9;
   return JPXLG_OK;
 }
 
@@ -2201,15 +2271,10 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
   R->img = img;
   R->q = *q;
   R->pol = quality_policy(q->effort);
-  if (q->mode != 1) {
-    HIP_TRY(srv->ctx, hipSetDevice(srv->ctx->device));
-    const int prc = predicted_start(img, q->target, q->effort, &R->predicted, &R->prior_beta);
-    if (prc) return prc;
-  }
   std::lock_guard<std::mutex> lk(srv->m);
   const uint64_t id = srv->next_id++;
   srv->reqs.emplace(id, std::move(r));
-  server_start(srv, R);
+  srv->unstarted.push_back(id);
   *ticket = id;
   return JPXLG_OK;
 }
@@ -2240,29 +2305,62 @@ extern "C" int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, u
 }
 
 // Admits queued requests in arrival order while their projected footprint
-// fits: each gets its image (the upload, XYB and metric reference) and its
-// navigator.  A request whose image cannot be created ends with that status.
+// fits (each gets its image: the upload, XYB and metric reference), then
+// starts every admitted request's navigator, the first guesses of all of
+// them from one batched feature pass.  A request whose image cannot be
+// created ends with that status.
 static void server_admit(jpxlg_server* srv) {
+  std::vector<ServerReq*> batch;
   for (;;) {
     ServerReq* R = nullptr;
     {
       std::lock_guard<std::mutex> lk(srv->m);
-      if (srv->queue.empty()) return;
-      ServerReq* F = srv->reqs.at(srv->queue.front()).get();
-      if (srv->used > 0 && srv->used + F->footprint > srv->budget) return;
+      if (srv->queue.empty()) break;
+      const uint64_t id = srv->queue.front();
+      ServerReq* F = srv->reqs.at(id).get();
+      if (srv->used > 0 && srv->used + F->footprint > srv->budget) break;
       srv->queue.pop_front();
       srv->used += F->footprint;
+      srv->unstarted.push_back(id);
       R = F;
     }
     jpxlg_img* img = nullptr;
-    int rc = image_create(srv->ctx, R->w, R->h, R->bits, R->rgb, false, &img);
-    if (!rc && R->q.mode != 1) rc = predicted_start(img, R->q.target, R->q.effort, &R->predicted, &R->prior_beta);
+    const int rc = image_create(srv->ctx, R->w, R->h, R->bits, R->rgb, false, &img);
     std::lock_guard<std::mutex> lk(srv->m);
+    if (rc == JPXLG_E_OOM && srv->used > R->footprint) {
+      // The device holds less than the budget assumed: while other admitted
+      // requests hold memory, the request goes back to the head of the
+      // queue and the budget drops to what is admitted now (releases make
+      // room); alone, it fails.
+      srv->used -= R->footprint;
+      srv->budget = std::min(srv->budget, srv->used);
+      const uint64_t id = srv->unstarted.back();
+      srv->unstarted.pop_back();
+      srv->queue.push_front(id);
+      break;
+    }
     R->img = img;
     R->admitted = true;
-    if (rc) {
+    if (rc) R->nav_rc = rc;
+  }
+  {
+    std::lock_guard<std::mutex> lk(srv->m);
+    for (uint64_t id : srv->unstarted) batch.push_back(srv->reqs.at(id).get());
+    srv->unstarted.clear();
+  }
+  if (batch.empty()) return;
+  std::vector<jpxlg_img*> imgs;
+  for (ServerReq* R : batch)
+    if (!R->nav_rc && R->q.mode != 1) imgs.push_back(R->img);
+  const int frc = images_features(imgs.data(), (int)imgs.size());
+  std::lock_guard<std::mutex> lk(srv->m);
+  for (ServerReq* R : batch) {
+    if (!R->nav_rc && R->q.mode != 1) {
+      if (frc) R->nav_rc = frc;
+      else start_from_features(R->img, R->q.target, R->q.effort, &R->predicted, &R->prior_beta);
+    }
+    if (R->nav_rc) {
       memset(&R->res, 0, sizeof(R->res));
-      R->nav_rc = rc;
       R->finished = R->done = true;
       continue;
     }
