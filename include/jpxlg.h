@@ -387,6 +387,10 @@ int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const jpxlg_quali
  * head of the queue and the budget drops to the admitted footprint. */
 int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, uint32_t height, uint32_t bits,
                                  const void* rgb, const jpxlg_quality_request* req, uint64_t* ticket);
+/* The tickets of requests completed since the last call, in completion
+ * order (up to cap; *n = how many were written).  A serving loop polls and
+ * releases only these instead of every ticket it holds. */
+int32_t jpxlg_server_completed(jpxlg_server* srv, uint64_t* tickets, uint32_t cap, uint32_t* n);
 /* The admission budget in bytes (default: 90% of the device's free memory
  * when the server is created). */
 int32_t jpxlg_server_set_budget(jpxlg_server* srv, uint64_t bytes);

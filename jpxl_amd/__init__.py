@@ -513,6 +513,13 @@ class Server:
         self._held[t.value] = rgb
         return t.value
 
+    def completed(self, cap: int = 4096) -> list[int]:
+        """Tickets of the requests completed since the last call."""
+        buf = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_uint32()
+        self.ctx.check(self.ctx.lib.jpxlg_server_completed(self.handle, buf, cap, ctypes.byref(n)))
+        return list(buf[: n.value])
+
     def set_budget(self, nbytes: int) -> None:
         self.ctx.check(self.ctx.lib.jpxlg_server_set_budget(self.handle, int(nbytes)))
 

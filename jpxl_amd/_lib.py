@@ -190,6 +190,7 @@ SIGNATURES = {
     "jpxlg_server_submit_srgb": (_i32, [_vp, _u32, _u32, _u32, _vp, ctypes.POINTER(QualityRequest),
                                         ctypes.POINTER(ctypes.c_uint64)]),
     "jpxlg_server_set_budget": (_i32, [_vp, ctypes.c_uint64]),
+    "jpxlg_server_completed": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64), _u32, _u32p]),
     "jpxlg_server_admission": (_i32, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _u32p]),
     "jpxlg_server_image": (_i32, [_vp, ctypes.c_uint64, ctypes.POINTER(_vp)]),
     "jpxlg_server_step": (_i32, [_vp]),

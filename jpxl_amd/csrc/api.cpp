@@ -2146,6 +2146,7 @@ extern "C" void jpxlg_graph_destroy(jpxlg_graph* G) {
 constexpr int kServerBootstrap = 8;
 
 struct ServerReq {
+  uint64_t id = 0;  // its ticket
   jpxlg_img* img = nullptr;
   // Requests submitted as host samples: the server creates (and destroys)
   // their image when it admits (and releases) them.
@@ -2200,6 +2201,8 @@ struct jpxlg_server {
   // Requests whose image exists and whose navigator has not started yet
   // (a step starts them together, after one batched feature pass).
   std::vector<uint64_t> unstarted;
+  // Requests completed since the caller last asked (jpxlg_server_completed).
+  std::deque<uint64_t> completed;
   ~jpxlg_server() {
     for (auto& l : lane)
       if (l.stream) {
@@ -2273,6 +2276,7 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
   R->pol = quality_policy(q->effort);
   std::lock_guard<std::mutex> lk(srv->m);
   const uint64_t id = srv->next_id++;
+  R->id = id;
   srv->reqs.emplace(id, std::move(r));
   srv->unstarted.push_back(id);
   *ticket = id;
@@ -2298,6 +2302,7 @@ extern "C" int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, u
   R->pol = quality_policy(q->effort);
   std::lock_guard<std::mutex> lk(srv->m);
   const uint64_t id = srv->next_id++;
+  R->id = id;
   srv->reqs.emplace(id, std::move(r));
   srv->queue.push_back(id);
   *ticket = id;
@@ -2362,6 +2367,7 @@ static void server_admit(jpxlg_server* srv) {
     if (R->nav_rc) {
       memset(&R->res, 0, sizeof(R->res));
       R->finished = R->done = true;
+      srv->completed.push_back(R->id);
       continue;
     }
     server_start(srv, R);
@@ -2462,7 +2468,10 @@ extern "C" int32_t jpxlg_server_step(jpxlg_server* srv) {
   }
   {
     std::lock_guard<std::mutex> lk(srv->m);
-    for (ServerReq* R : finishing) R->done = true;
+    for (ServerReq* R : finishing) {
+      R->done = true;
+      srv->completed.push_back(R->id);
+    }
   }
   // The structure a request's pending probe needs (a new request's anchor,
   // a finalist rebuilt at its rung, or the anchor set again) is built on the
@@ -2618,6 +2627,18 @@ extern "C" int32_t jpxlg_server_admission(jpxlg_server* srv, uint64_t* used, uin
   if (used) *used = srv->used;
   if (budget) *budget = srv->budget;
   if (queued) *queued = (uint32_t)srv->queue.size();
+  return JPXLG_OK;
+}
+
+extern "C" int32_t jpxlg_server_completed(jpxlg_server* srv, uint64_t* tickets, uint32_t cap, uint32_t* n) {
+  if (!srv || !n || (cap && !tickets)) return JPXLG_E_INVALID;
+  std::lock_guard<std::mutex> lk(srv->m);
+  uint32_t k = 0;
+  while (k < cap && !srv->completed.empty()) {
+    tickets[k++] = srv->completed.front();
+    srv->completed.pop_front();
+  }
+  *n = k;
   return JPXLG_OK;
 }
 

@@ -212,9 +212,10 @@ def test_server_admission_queues_within_budget(ctx):
     """Requests given as host samples under a budget of about two requests'
     projected footprint: the server queues the rest (queued > 0 seen), the
     admitted footprint never exceeds the budget unless one request alone
-    holds memory, each released request returns its footprint, and every
-    result (and the chosen plan of every other request) equals the request
-    encoded alone."""
+    holds memory, each released request returns its footprint, the
+    completed-ticket list names exactly the requests that poll done, and
+    every result (and the chosen plan of every other request) equals the
+    request encoded alone."""
     import jpxl_amd
 
     rng = np.random.default_rng(31)
@@ -244,14 +245,16 @@ def test_server_admission_queues_within_budget(ctx):
         # admission is in arrival order: the queued requests are the last ones
         assert used == sum(fps[k] for k in live[:len(live) - queued])
         assert used <= budget or len(live) - queued == 1
+        finished = set(srv.completed())
         for k in live:
             r = srv.poll(tickets[k])
+            assert (r is not None) == (tickets[k] in finished), k
             if r is not None:
                 done[k] = r
                 if k % 2 == 0:
                     assert np.array_equal(srv.plan(tickets[k]).quant, alone[k][1]), k
                 srv.release(tickets[k])
-    assert saw_queue
+    assert saw_queue and srv.completed() == []
     assert srv.admission()[:1] == (0,) and srv.admission()[2] == 0
     srv.close()
     for k in range(len(reqs)):
