@@ -800,7 +800,7 @@ __device__ __forceinline__ void v_outputs(SetSpace& S, size_t base, int h, int n
     // the reference's mu1, s11, x1 and the candidate's x2, a pair of
     // positions at a time
     const float* planes[4] = {S.mu1T + base, S.s11T + base, S.x1T + base, S.x2T + base};
-#pragma unroll
+#pragma unroll 3
     for (int k = 0; k < kSeg; k += 2) {
       const int n = n0 + k;
       if (!own || n < 0 || n >= h) continue;
@@ -866,32 +866,32 @@ __device__ __forceinline__ void blur_v_long(SetSpace& S, int lw, const LineTaps&
     const int b = r * kVWaves + wv;
     const bool active = b < blocks;  // wave-uniform
     const int pb = kBlockStep * b, warm = b ? 2 : 0;
-    double Y[M][kSeg];
-    if (active) {
-      float u[M][kSeg];
+    float u[M][kSeg];
+    if (active)
 #pragma unroll
       for (int m = 0; m < M; m++) load_line10(S.hT[m] + base, pb + kSeg * lane, h, u[m]);
+    // one moment at a time, so only one moment's chain values wait (in
+    // registers) for their carry across the barrier
 #pragma unroll
-      for (int m = 0; m < M; m++) {
-        seg_block<64, true>(u[m], lane, warm, T, Y[m]);
+    for (int m = 0; m < M; m++) {
+      double Y[kSeg];
+      if (active) {
+        seg_block<64, true>(u[m], lane, warm, T, Y);
         if (lane == 63)
 #pragma unroll
-          for (int j = 0; j < kSeg; j++) sh.ends[wv][m * kSeg + j] = Y[m][j];
+          for (int j = 0; j < kSeg; j++) sh.ends[wv][m * kSeg + j] = Y[j];
       }
-    }
-    lds_barrier();
-    if (active) {
-      // this block's carry: the running carry plus this round's earlier ends
-#pragma unroll
-      for (int m = 0; m < M; m++)
+      lds_barrier();
+      if (active)
+        // this block's carry: the running carry plus this round's earlier ends
 #pragma unroll
         for (int j = 0; j < kSeg; j++) {
           double c = sh.carry[m * kSeg + j];
           for (int k = 0; k < wv; k++) c += sh.ends[k][m * kSeg + j];
-          ys[(m * kSeg + j) * 64 + lane] = (float)continue_chain(Y[m][j], c, lane);
+          ys[(m * kSeg + j) * 64 + lane] = (float)continue_chain(Y[j], c, lane);
         }
-      v_outputs<CAND>(S, base, h, pb + kSeg * lane - 4, lane >= warm, ys, lane, acc);
     }
+    if (active) v_outputs<CAND>(S, base, h, pb + kSeg * lane - 4, lane >= warm, ys, lane, acc);
     lds_barrier();
     // the carry entering the next round (the blocks' ends in order)
     if (threadIdx.x < M * kSeg) {
@@ -904,7 +904,7 @@ __device__ __forceinline__ void blur_v_long(SetSpace& S, int lw, const LineTaps&
 }
 
 template <int CAND>
-__global__ __launch_bounds__(64 * kVWaves) void k_blur_v(const ConstTables* __restrict__ tab,
+__global__ __launch_bounds__(64 * kVWaves) __attribute__((amdgpu_waves_per_eu(5))) void k_blur_v(const ConstTables* __restrict__ tab,
                                                          const LineSet* __restrict__ sets,
                                                          const int* __restrict__ off, int nsets, int total,
                                                          int swizzle) {
