@@ -3,8 +3,13 @@
 A step is one whole quality encode of the 2048x2048 RGB16 synthetic image
 (seed 2) whose samples are already resident in HBM: XYB + metric reference
 pyramid, plan structure + forward DCT, the navigator's quantize -> render ->
-SSIMULACRA2 probes, and the chosen plan quantized on the device for the CPU
-entropy coder.  Prints one JSON line (the driver's contract).
+SSIMULACRA2 probes, the chosen plan regathered into the QuantizedFrameIr and
+read back into pinned host memory, and the CPU codestream stage (entropy
+coding and bitstream writing) producing the JPEG XL bytes.  Steps are
+pipelined the way an encoder serving a stream of images runs: the host
+writes image i's codestream while the device searches image i + 1; the timed
+region ends when the last codestream is written.  Prints one JSON line (the
+driver's contract).
 """
 from __future__ import annotations
 
@@ -124,14 +129,36 @@ def measured_traffic(label: str) -> float | None:
 
 
 def _cpu_encode_worker(args):
-    """One whole quality encode on the CPU restatement (a worker process)."""
+    """One whole quality encode on the CPU restatement (a worker process):
+    the navigator over the oracle's probes, then the chosen plan through the
+    same host codestream stage the GPU path uses (one worker thread)."""
     rgb, bits, target, effort = args
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import codestream_binding
     import cpu_encode
+    from jpxl_amd import _lib
 
     t0 = time.perf_counter()
-    out, _ = cpu_encode.encode(rgb, bits, target, effort)
-    return time.perf_counter() - t0, int(out.rung), int(out.nprobes), float(out.score)
+    out, plan = cpu_encode.encode(rgb, bits, target, effort)
+    quant, lf, bmap, cx, cb, lff = plan
+    qp, _, _, epf, sharp = cpu_encode.rung_policy(int(out.rung), effort)
+    arena, lfg, vbs = codestream_binding.ir_from_footprint(quant, lf, bmap, np.full(bmap.shape, qp.hf_mul, np.uint32))
+    h, w, _ = rgb.shape
+    f = _lib.FrameDesc(width=w, height=h, bits_per_sample=bits, global_scale=qp.global_scale, quant_lf=qp.quant_lf,
+                       x_qm_scale=qp.x_qm, b_qm_scale=qp.b_qm, x_factor_lf=lff[0], b_factor_lf=lff[1], gaborish=0,
+                       epf_iters=epf, sharpness=sharp)
+    stream, _ = codestream_binding.emit(f, arena, lfg, vbs, cx, cb, threads=1)
+    return time.perf_counter() - t0, int(out.rung), int(out.nprobes), float(out.score), len(stream)
+
+
+def progress(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_cores() -> int:
+    """Host cores this process may use, at most 16 (one GPU's share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    return max(1, min(16, n))
 
 
 def cpu_model() -> str:
@@ -155,23 +182,26 @@ def cpu_baseline(rgb: np.ndarray, effort: int, gpu_rung: int) -> dict:
     import cpu_encode
 
     cpu_encode.build()
-    n = max(1, min(16, len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count() or 1))
+    n = host_cores()
     t0 = time.perf_counter()
+    # forked workers run host code only (the oracle and the codestream stage,
+    # whose worker pool a forked child rebuilds for itself)
     with mp.get_context("fork").Pool(n) as pool:
         res = pool.map(_cpu_encode_worker, [(rgb, BITS, TARGET, effort)] * n, chunksize=1)
     wall = time.perf_counter() - t0
     h, w, _ = rgb.shape
     mp_total = n * w * h / 1e6
-    rung, probes, score = res[0][1], res[0][2], res[0][3]
+    rung, probes, score, nbytes = res[0][1], res[0][2], res[0][3], res[0][4]
     return {
         "value": round(mp_total / wall, 4),
         "unit": "MP/s",
         "cores": n,
         "kind": "port",
         "sample": f"CPU restatement (not the reference binary): {n} concurrent whole quality encodes of the "
-                  f"{w}x{h} RGB{BITS} image (q{TARGET:g}, navigator over oracle probes), one per core, "
-                  f"{wall:.1f} s wall; {probes} probes each, chosen rung {rung} (GPU: {gpu_rung}), "
-                  f"score {score:.4f}; host {cpu_model()}, nproc {os.cpu_count()}",
+                  f"{w}x{h} RGB{BITS} image (q{TARGET:g}, navigator over oracle probes, then the chosen "
+                  f"plan through the host codestream stage), one per core, {wall:.1f} s wall; {probes} probes "
+                  f"each, chosen rung {rung} (GPU: {gpu_rung}), score {score:.4f}, {nbytes} bytes; host "
+                  f"{cpu_model()}, nproc {os.cpu_count()}",
     }
 
 
@@ -575,27 +605,95 @@ def main() -> int:
     tuned = ctx.autotune(WIDTH, HEIGHT)
 
     sizes = {}
+    lib = ctx.lib
+    bw, bh = (WIDTH + 7) // 8, (HEIGHT + 7) // 8
+    tw, th = (bw + 7) // 8, (bh + 7) // 8
+    emit_threads = max(1, host_cores() - 1)  # one core drives the device
 
-    def step():
+    class Slot:
+        """One image's hand-off to the codestream stage: the IR in pinned
+        host memory, the CfL tiles, the frame fields and the output."""
+
+        def __init__(self):
+            self.bufs = []
+            for nbytes in (192 * bw * bh * 4, 3 * bw * bh * 4):
+                p = ctypes.c_void_p()
+                assert hl.hipHostMalloc(ctypes.byref(p), nbytes, 0) == 0
+                self.bufs.append(p)
+            self.table = (jpxl_amd._lib.Varblock * (bw * bh))()
+            self.n = ctypes.c_uint32()
+            self.cx = np.zeros((th, tw), np.int8)
+            self.cb = np.zeros((th, tw), np.int8)
+            self.frame = jpxl_amd._lib.FrameDesc()
+            self.out = np.empty(64 << 20, np.uint8)
+            self.out_len = ctypes.c_size_t()
+            self.sizes = jpxl_amd._lib.EmitSizes()
+            self.pending = None
+
+        def free(self):
+            for p in self.bufs:
+                hl.hipHostFree(p)
+
+    hl.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+    hl.hipHostFree.argtypes = [ctypes.c_void_p]
+    slots = [Slot(), Slot()]
+
+    def search(slot):
+        """The device part of a step: the quality search and the hand-off."""
         img = jpxl_amd.Image(ctx, None, BITS, device_ptr=dptr.value, shape=(HEIGHT, WIDTH))
         res = img.encode_quality(TARGET, args.effort)
         if not sizes:
             bm = img.plan_download_map()
             for side in (8, 16, 32):
                 sizes[side] = int((bm == side).sum())
+        h = img.handle
+        ctx.check(lib.jpxlg_plan_download_ir(h, slot.bufs[0], slot.bufs[1], slot.table, ctypes.byref(slot.n)))
+        ctx.check(lib.jpxlg_plan_download(h, None, None, None, slot.cx.ctypes.data, slot.cb.ctypes.data, None))
+        ctx.check(lib.jpxlg_plan_frame_desc(h, None, ctypes.byref(slot.frame)))
         img.close()
         return res
 
-    for _ in range(args.warmup):
-        step()
+    def emit(slot):
+        """The host part of a step: the codestream stage."""
+        rc = lib.jpxlg_emit_codestream(ctypes.byref(slot.frame), slot.bufs[0], slot.bufs[1], slot.table,
+                                       slot.n.value, slot.cx.ctypes.data, slot.cb.ctypes.data, emit_threads,
+                                       slot.out.ctypes.data, slot.out.size, ctypes.byref(slot.out_len),
+                                       ctypes.byref(slot.sizes))
+        if rc != 0:
+            raise RuntimeError(f"jpxlg_emit_codestream: {rc}")
+        return slot.out_len.value
+
+    from concurrent.futures import ThreadPoolExecutor
+
+    host = ThreadPoolExecutor(1)
+
+    def run(k):
+        """k pipelined steps: search i on the device while the host emits i - 1."""
+        results, nbytes = [], []
+        for i in range(k):
+            slot = slots[i % 2]
+            if slot.pending is not None:
+                nbytes.append(slot.pending.result())
+                slot.pending = None
+            results.append(search(slot))
+            slot.pending = host.submit(emit, slot)
+        for slot in slots:
+            if slot.pending is not None:
+                nbytes.append(slot.pending.result())
+                slot.pending = None
+        return results, nbytes
+
+    run(args.warmup)
+    progress(f"warmup: {args.warmup} pipelined steps done")
     # Per-kernel breakdown: one untimed pass with every launch bracketed by
     # events (the events themselves cost ~15% of the step, so this pass is
     # kept out of the timed region).
     ctx.profile(True)
     ctx.profile_read()  # clear
     for _ in range(args.steps):
-        step()
+        search(slots[0])
     breakdown = ctx.profile_read()
+    progress("per-kernel breakdown pass done")
     rows = sorted(breakdown.items(), key=lambda kv: -kv[1][0])
     # Dominant kernel by total time: the one whose roofline is reported.
     dominant = next((name for name, _ in rows if algorithmic_bytes(name, WIDTH, HEIGHT, sizes) is not None),
@@ -606,12 +704,31 @@ def main() -> int:
     ctx.profile_read()  # clear
     hl.hipDeviceSynchronize()
     t0 = time.perf_counter()
-    results = [step() for _ in range(args.steps)]
+    results, nbytes = run(args.steps)
     hl.hipDeviceSynchronize()
     elapsed = time.perf_counter() - t0
     prof = ctx.profile_read()
     ctx.profile(False)
     ctx.profile_only(None)
+    progress(f"timed region: {args.steps} steps in {elapsed:.3f} s")
+
+    # Secondary keys (never the value): one step unpipelined (its latency),
+    # its device part alone and its host part alone.
+    lat, dev, emit_ms = [], [], []
+    for _ in range(3):
+        a = time.perf_counter()
+        search(slots[0])
+        hl.hipDeviceSynchronize()
+        b = time.perf_counter()
+        emit(slots[0])
+        c = time.perf_counter()
+        lat.append(c - a)
+        dev.append(b - a)
+        emit_ms.append(c - b)
+    emit_sizes = slots[0].sizes
+    for slot in slots:
+        slot.free()
+    host.shutdown()
 
     ms_per_step = 1000.0 * elapsed / args.steps
     mp = WIDTH * HEIGHT / 1e6
@@ -621,6 +738,7 @@ def main() -> int:
     # through the QuantizedFrameIr hand-off into pinned host buffers, i.e.
     # with both PCIe transfers.
     pcie = pcie_inclusive_rate(ctx, hl, rgb, args)
+    progress("PCIe-inclusive rate done")
 
     roofline = None
     if dominant in prof:
@@ -656,7 +774,8 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": f"{WIDTH}x{HEIGHT} RGB{BITS} synthetic (seed {SEED}), quality {TARGET:g}, "
-                        f"{args.effort} effort, one image resident in HBM",
+                        f"{args.effort} effort, samples resident in HBM, through the JPEG XL codestream "
+                        f"(host entropy coding pipelined with the next image's search)",
             "probes": int(r0.probes),
             "achieved_score": round(r0.score, 4),
             "chosen_global_scale": int(r0.global_scale),
@@ -667,11 +786,22 @@ def main() -> int:
             "varblocks": {str(k): v for k, v in sizes.items()},
             "tuned_blur_shape": {"units_per_group": tuned[0], "xcd_swizzle": tuned[1]},
             "pcie_inclusive_mps": pcie,
+            "codestream_bytes": int(nbytes[-1]),
+            "bits_per_pixel": round(8.0 * nbytes[-1] / (WIDTH * HEIGHT), 4),
+            "codestream_sections": {"lf_global": int(emit_sizes.lf_global), "lf_groups": int(emit_sizes.lf_groups),
+                                    "hf_global": int(emit_sizes.hf_global),
+                                    "pass_groups": int(emit_sizes.pass_groups)},
+            "emit_threads": emit_threads,
+            "step_latency_ms": round(1000 * min(lat), 3),
+            "device_search_ms": round(1000 * min(dev), 3),
+            "host_emit_ms": round(1000 * min(emit_ms), 3),
+            "search_only_mps": round(mp / min(dev), 2),
         },
         "roofline": roofline,
         "kernels_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in rows},  # untimed breakdown pass
     }
     if not args.no_cpu_baseline:
+        progress("CPU baseline: starting the concurrent CPU encodes")
         out["cpu_baseline"] = cpu_baseline(rgb, effort_code, int(r0.rung))
     print(json.dumps(out))
     ctx.close()

@@ -5,9 +5,10 @@
  * The library keeps each image's XYB planes, forward-DCT coefficients and
  * SSIMULACRA2 reference pyramid resident in HBM, and runs the
  * quantize -> reconstruct -> score chain of the quality search on the GPU.
- * Entropy coding and bitstream writing stay with the caller (CPU); the plan
- * handed back (quantized coefficients, LF planes, block map, CfL factors)
- * is the input of the reference's `attach_entropy`.
+ * Entropy coding and bitstream writing run on the host: the plan handed back
+ * (quantized coefficients, LF planes, block map, CfL factors) is the input
+ * of the reference's `attach_entropy`, and jpxlg_emit_codestream /
+ * jpxlg_encode_emit write the JPEG XL codestream from it on the CPU.
  *
  * Conventions: plain C types, caller-owned host buffers, int32 status
  * (0 = OK, negative = JPXLG_E_*).  One context = one device + one HIP stream;
@@ -30,6 +31,9 @@
  *                              jpxl-perceptual/src/ssimulacra2.rs:79)
  *   jpxlg_plan_download       the QuantizedFrameIr hand-off to attach_entropy
  *                             (jpxl-encode/src/vardct/plan.rs:614-636)
+ *   jpxlg_emit_codestream     attach_entropy + emit_codestream_with_executor
+ *                             (jpxl-encode-policy/src/lib.rs:1012,
+ *                              jpxl-encode/src/vardct/write.rs:350)
  *   jpxlg_encode_quality      search_frame_perceptual
  *                             (jpxl-encode-policy/src/quality.rs:2405)
  *   jpxlg_score_pair          jpxl_perceptual::score_pair
@@ -204,6 +208,59 @@ typedef struct {
 } jpxlg_varblock;
 int32_t jpxlg_plan_download_ir(jpxlg_img* img, int32_t* coeff_arena, int32_t* lf_groups, jpxlg_varblock* varblocks,
                                uint32_t* n_varblocks);
+
+/* ---- codestream emission (CPU) -------------------------------------------- */
+/* What the frame signals besides the plan: the fields of FrameDecision,
+ * QuantizerDecision, LfCorrelationDecision and RestorationDecision
+ * (jpxl-encode/src/vardct/plan.rs:409-470) that the headers and LfGlobal
+ * write (jpxl-encode/src/vardct/headers.rs:104-233, write.rs:643-682). */
+typedef struct {
+  uint32_t width, height;           /* frame size in pixels */
+  uint32_t bits_per_sample;         /* ImageMetadata bit depth, 1..16 */
+  uint32_t global_scale, quant_lf;  /* I.2.1 Quantizer */
+  uint32_t x_qm_scale, b_qm_scale;  /* frame header, 0..7 */
+  int32_t x_factor_lf, b_factor_lf; /* LfChannelCorrelation factors minus 128 (both 0: the default) */
+  uint32_t gaborish;                /* restoration filter: Gaborish on (1) or off (0) */
+  uint32_t epf_iters;               /* 0..3 */
+  uint32_t sharpness;               /* every block's EPF sharpness, 0..7 */
+} jpxlg_frame_desc;
+
+/* CodestreamSizing (write.rs:289-320): total bytes and where they went. */
+typedef struct {
+  uint64_t total, image_headers, frame_header_bits, toc_bits;
+  uint64_t lf_global, lf_groups, hf_global, pass_groups; /* section bytes by kind (one section: all in pass_groups) */
+} jpxlg_emit_sizes;
+
+/* attach_entropy (jpxl-encode-policy/src/lib.rs:1012) +
+ * emit_codestream_with_executor (jpxl-encode/src/vardct/write.rs:350): the
+ * JPEG XL codestream of a quantized plan given in the jpxlg_plan_download_ir
+ * layout (coeff_arena, lf_groups, varblocks) with the frame's CfL tiles
+ * (cfl_x, cfl_b as jpxlg_plan_download; NULL = all zero).  Runs on the host
+ * with `threads` workers (<= 0: every core).  out NULL prices the stream
+ * (price_codestream_with, write.rs:302): *out_len and sizes are filled.  A
+ * capacity below the stream's size returns JPXLG_E_TOO_SMALL with *out_len
+ * the size needed.  The varblocks must tile the block grid exactly. */
+int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* frame, const int32_t* coeff_arena, const int32_t* lf_groups,
+                              const jpxlg_varblock* varblocks, uint32_t n_varblocks, const int8_t* cfl_x,
+                              const int8_t* cfl_b, int32_t threads, uint8_t* out, size_t capacity, size_t* out_len,
+                              jpxlg_emit_sizes* sizes);
+
+/* The frame fields of an image's last quantized plan for
+ * jpxlg_emit_codestream: its wire quantizer (jpxlg_plan_aq), LF factors,
+ * source depth and `r` (NULL: the restoration of the image's last render).
+ * With jpxlg_plan_download_ir and the CfL tiles of jpxlg_plan_download this
+ * is everything the codestream stage reads, so the emit can run on another
+ * host thread while the device works on the next image. */
+int32_t jpxlg_plan_frame_desc(jpxlg_img* img, const jpxlg_restoration* r, jpxlg_frame_desc* frame);
+
+/* The same over an image's last quantized plan (attach_entropy + emit after
+ * search_frame_perceptual, jpxl-encode-policy/src/lib.rs:5501-5600): the IR
+ * is read back into the context's pinned staging, the frame signals the
+ * plan's wire quantizer, CfL tiles and LF factors, and `r` is the
+ * restoration the plan was rendered with (NULL: that of the image's last
+ * render, e.g. the quality search's). */
+int32_t jpxlg_encode_emit(jpxlg_img* img, const jpxlg_restoration* r, int32_t threads, uint8_t* out,
+                          size_t capacity, size_t* out_len, jpxlg_emit_sizes* sizes);
 
 /* PerceptualEvaluator::evaluate(&ValidatedPixelPlan) (quality.rs:87-157):
  * a plan built elsewhere (the CPU planner) replaces the image's structure and

@@ -338,6 +338,20 @@ class Image:
                                                         _ptr(cb), _ptr(lff)))
         return Plan(q, lf, bm, cx, cb, (int(lff[0]), int(lff[1])))
 
+    def encode_emit(self, restoration: "Restoration | None" = None, threads: int = 0):
+        """The JPEG XL codestream of the last quantized plan (attach_entropy +
+        emit on the host): (bytes, EmitSizes).  ``restoration`` defaults to
+        that of the image's last render."""
+        lib = self.ctx.lib
+        n = ctypes.c_size_t()
+        sizes = _lib.EmitSizes()
+        rp = ctypes.byref(restoration) if restoration is not None else None
+        self.ctx.check(lib.jpxlg_encode_emit(self.handle, rp, threads, None, 0, ctypes.byref(n), ctypes.byref(sizes)))
+        out = np.empty(n.value, np.uint8)
+        self.ctx.check(lib.jpxlg_encode_emit(self.handle, rp, threads, _ptr(out), out.size, ctypes.byref(n),
+                                             ctypes.byref(sizes)))
+        return out.tobytes(), sizes
+
     def plan_download_ir(self):
         """The QuantizedFrameIr hand-off: (coefficient arena, per-group LF
         planes, varblock table as a list of dicts in the IR's order)."""
@@ -581,14 +595,17 @@ class EncodeReport:
     hf_mul: int
     quant_lf: int
     probes: list = field(default_factory=list)
+    bytes: int = 0
 
 
 class Encoder:
     """Builder mirroring jpxl::Encoder for the quality / fixed-quantizer paths.
 
-    ``encode_rgb8_reported`` returns ``(Plan, EncodeReport)``: the quantized
-    plan is what the reference hands to its CPU entropy coder; the entropy
-    coder itself is not part of this library.
+    ``encode_rgb8`` returns the JPEG XL codestream (``bytes``) and
+    ``encode_rgb8_reported`` ``(bytes, EncodeReport)``, as the facade's
+    ``Result<Vec<u8>>`` / ``Result<(Vec<u8>, EncodeReport)>``; ``plan_rgb8``
+    returns the quantized plan itself (what the reference hands to its
+    entropy coder).
     """
 
     def __init__(self, ctx: Context | None = None):
@@ -596,6 +613,7 @@ class Encoder:
         self.quality: float | None = None
         self.global_scale: int | None = None
         self.effort = "balanced"
+        self.threads = 0
 
     def with_quality(self, quality: float) -> "Encoder":
         if not (0.0 <= quality <= 100.0):
@@ -617,10 +635,11 @@ class Encoder:
         self.effort = effort
         return self
 
-    def with_threads(self, _threads: int) -> "Encoder":
-        return self  # one GPU stream; kept for call-surface parity
+    def with_threads(self, threads: int) -> "Encoder":
+        self.threads = int(threads)  # host workers of the codestream stage (0: every core)
+        return self
 
-    def _encode(self, rgb: np.ndarray, bits: int):
+    def _encode(self, rgb: np.ndarray, bits: int, emit: bool = True):
         img = Image(self.ctx, rgb, bits)
         try:
             if self.quality is not None:
@@ -639,6 +658,12 @@ class Encoder:
                 _, wire, _ = img.plan_aq()
                 report = EncodeReport(float("nan"), float("nan"), True, wire.global_scale, wire.hf_mul,
                                       wire.quant_lf)
+                # the fixed-quantizer request's restoration (request.rs:905)
+                img.render_score(Restoration(1, 0, 7))
+            if emit:
+                stream, sizes = img.encode_emit(None, self.threads)
+                report.bytes = sizes.total
+                return stream, report
             return img.plan_download(), report
         finally:
             img.close()
@@ -649,6 +674,11 @@ class Encoder:
 
     def encode_rgb16_reported(self, width: int, height: int, rgb: np.ndarray, bits: int = 16):
         return self._encode(np.asarray(rgb, np.uint16).reshape(height, width, 3), bits)
+
+    def plan_rgb8(self, width: int, height: int, rgb):
+        """(Plan, EncodeReport): the quantized plan without the codestream."""
+        a = np.frombuffer(bytes(rgb), np.uint8) if not isinstance(rgb, np.ndarray) else rgb
+        return self._encode(a.reshape(height, width, 3).astype(np.uint8), 8, emit=False)
 
     def encode_rgb8(self, width: int, height: int, rgb):
         return self.encode_rgb8_reported(width, height, rgb)[0]

@@ -67,6 +67,22 @@ class Restoration(ctypes.Structure):
     ]
 
 
+class FrameDesc(ctypes.Structure):
+    """jpxlg_frame_desc: what the codestream signals besides the plan."""
+
+    _fields_ = [(n, ctypes.c_uint32) for n in ("width", "height", "bits_per_sample", "global_scale", "quant_lf",
+                                                 "x_qm_scale", "b_qm_scale")] + [
+        ("x_factor_lf", ctypes.c_int32), ("b_factor_lf", ctypes.c_int32),
+        ("gaborish", ctypes.c_uint32), ("epf_iters", ctypes.c_uint32), ("sharpness", ctypes.c_uint32)]
+
+
+class EmitSizes(ctypes.Structure):
+    """jpxlg_emit_sizes (CodestreamSizing)."""
+
+    _fields_ = [(n, ctypes.c_uint64) for n in ("total", "image_headers", "frame_header_bits", "toc_bits",
+                                                 "lf_global", "lf_groups", "hf_global", "pass_groups")]
+
+
 class QualityRequest(ctypes.Structure):
     """jpxlg_quality_request."""
 
@@ -178,6 +194,11 @@ SIGNATURES = {
     "jpxlg_plan_download": (_i32, [_vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_image_download": (_i32, [_vp, _i32, _vp, ctypes.c_size_t]),
     "jpxlg_plan_download_ir": (_i32, [_vp, _vp, _vp, _vp, _u32p]),
+    "jpxlg_emit_codestream": (_i32, [ctypes.POINTER(FrameDesc), _vp, _vp, _vp, _u32, _vp, _vp, _i32, _vp,
+                                     ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(EmitSizes)]),
+    "jpxlg_plan_frame_desc": (_i32, [_vp, ctypes.POINTER(Restoration), ctypes.POINTER(FrameDesc)]),
+    "jpxlg_encode_emit": (_i32, [_vp, ctypes.POINTER(Restoration), _i32, _vp, ctypes.c_size_t,
+                                 ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(EmitSizes)]),
     "jpxlg_set_plan": (_i32, [_vp, ctypes.POINTER(Quantizer), _vp, _vp, _vp, _vp, _vp, _vp, _vp]),
     "jpxlg_encode_quality": (_i32, [_vp, ctypes.POINTER(QualityRequest), ctypes.POINTER(QualityResult)]),
     "jpxlg_rung_quantizer": (_i32, [_u32, _i32, ctypes.POINTER(Quantizer)]),

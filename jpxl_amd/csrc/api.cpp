@@ -98,6 +98,10 @@ struct jpxlg_ctx {
   size_t up_cap = 0, up_off = 0;
   char* rb_host = nullptr;
   size_t rb_cap = 0;
+  // Pinned staging of the QuantizedFrameIr hand-off read back for the
+  // codestream stage (arena, LF planes, varblock table), grown on demand.
+  char* ir_host = nullptr;
+  size_t ir_cap = 0;
   // This device's address of the no-allocation scratch array (from
   // hipGetSymbolAddress at creation) and pinned space for its sums.
   char* static_scratch = nullptr;
@@ -372,6 +376,7 @@ struct jpxlg_img {
   bool has_render = false;
   jpxlg_quantizer last_q{};
   int last_sharpness = 7;
+  jpxlg_restoration last_r{1, 0, 7};  // the restoration of the last render (the frame header signals it)
   int dct_form = kDctExact;  // DctForm of the structure's forward DCT and the render's IDCT
   // The arena goes back to the context (or is freed) when the image dies.
   ~jpxlg_img() {
@@ -456,6 +461,7 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   }
   if (ctx->up_host) (void)hipHostFree(ctx->up_host);
   if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
+  if (ctx->ir_host) (void)hipHostFree(ctx->ir_host);
   if (ctx->static_sums) (void)hipHostFree(ctx->static_sums);
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
@@ -1451,7 +1457,10 @@ static int enqueue_chain(jpxlg_ctx* ctx, hipStream_t st, jpxlg_img* const* imgs,
       imgs[k]->last_q = qs[k];
       imgs[k]->has_quant = true;
     }
-    if (r) imgs[k]->last_sharpness = r->sharpness;
+    if (r) {
+      imgs[k]->last_sharpness = r->sharpness;
+      imgs[k]->last_r = *r;
+    }
   }
   run->slot = sl;
   run->host = H;
@@ -1560,6 +1569,74 @@ extern "C" int32_t jpxlg_plan_download(jpxlg_img* img, int32_t* quant, int32_t* 
   if (cfl_x) memcpy(cfl_x, img->cfl_x.data(), img->cfl_x.size());
   if (cfl_b) memcpy(cfl_b, img->cfl_b.data(), img->cfl_b.size());
   if (lf_factors) { lf_factors[0] = img->lf_factors[0]; lf_factors[1] = img->lf_factors[1]; }
+  return JPXLG_OK;
+}
+
+// What the frame signals besides the QuantizedFrameIr: the wire quantizer
+// (jpxlg_plan_aq), the structure's LF factors, the source depth and the
+// restoration the plan was rendered with.
+extern "C" int32_t jpxlg_plan_frame_desc(jpxlg_img* img, const jpxlg_restoration* r, jpxlg_frame_desc* f) {
+  if (!img || !f) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!r) r = &img->last_r;
+  if (r->epf_iters < 0 || r->epf_iters > 3 || r->gaborish < 0 || r->gaborish > 1 || r->sharpness < 0 ||
+      r->sharpness > 7)
+    return fail(ctx, JPXLG_E_RANGE, "restoration outside its ranges");
+  jpxlg_quantizer wire;
+  const int32_t rc = jpxlg_plan_aq(img, nullptr, &wire, nullptr);
+  if (rc) return rc;
+  f->width = (uint32_t)img->g.w;
+  f->height = (uint32_t)img->g.h;
+  f->bits_per_sample = (uint32_t)img->bits;
+  f->global_scale = wire.global_scale;
+  f->quant_lf = wire.quant_lf;
+  f->x_qm_scale = wire.x_qm_scale;
+  f->b_qm_scale = wire.b_qm_scale;
+  f->x_factor_lf = img->lf_factors[0];
+  f->b_factor_lf = img->lf_factors[1];
+  f->gaborish = (uint32_t)r->gaborish;
+  f->epf_iters = (uint32_t)r->epf_iters;
+  f->sharpness = (uint32_t)r->sharpness;
+  return JPXLG_OK;
+}
+
+// attach_entropy (jpxl-encode-policy/src/lib.rs:1012) and the emit
+// (jpxl-encode/src/vardct/write.rs:350) over the image's last quantized plan:
+// the IR regathered on the device is read back into the context's pinned
+// staging, and the CPU codestream stage writes the frame with the quantizer
+// the frame signals (jpxlg_plan_aq's wire), the structure's CfL and LF
+// factors, and the restoration the plan was rendered with.
+extern "C" int32_t jpxlg_encode_emit(jpxlg_img* img, const jpxlg_restoration* r, int32_t threads, uint8_t* out,
+                                     size_t capacity, size_t* out_len, jpxlg_emit_sizes* sizes) {
+  if (!img) return JPXLG_E_INVALID;
+  if (!r) r = &img->last_r;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_structure || !img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  const Geom& g = img->g;
+  const size_t bpx = (size_t)g.bw * g.bh;
+  const size_t a_bytes = 192 * bpx * 4, l_bytes = 3 * bpx * 4, v_bytes = bpx * sizeof(jpxlg_varblock);
+  const size_t need = a_bytes + l_bytes + v_bytes;
+  if (need > ctx->ir_cap) {
+    if (ctx->ir_host) (void)hipHostFree(ctx->ir_host);
+    ctx->ir_host = nullptr;
+    ctx->ir_cap = 0;
+    if (hipHostMalloc((void**)&ctx->ir_host, need, hipHostMallocDefault) != hipSuccess)
+      return fail(ctx, JPXLG_E_OOM, "pinned staging for the IR hand-off");
+    ctx->ir_cap = need;
+  }
+  int32_t* arena = (int32_t*)ctx->ir_host;
+  int32_t* lfg = (int32_t*)(ctx->ir_host + a_bytes);
+  jpxlg_varblock* vbs = (jpxlg_varblock*)(ctx->ir_host + a_bytes + l_bytes);
+  uint32_t n = 0;
+  int32_t rc = jpxlg_plan_download_ir(img, arena, lfg, vbs, &n);
+  if (rc) return rc;
+  jpxlg_frame_desc f;
+  rc = jpxlg_plan_frame_desc(img, r, &f);
+  if (rc) return rc;
+  rc = jpxlg_emit_codestream(&f, arena, lfg, vbs, n, img->cfl_x.data(), img->cfl_b.data(), threads, out, capacity,
+                             out_len, sizes);
+  if (rc == JPXLG_E_TOO_SMALL) return fail(ctx, rc, "output buffer smaller than the codestream");
+  if (rc) return fail(ctx, rc, "codestream stage refused the plan");
   return JPXLG_OK;
 }
 
@@ -2119,6 +2196,7 @@ extern "C" int32_t jpxlg_graph_probe(jpxlg_graph* G, const jpxlg_quantizer* qs, 
     img->last_q = qs[k];
     img->has_quant = true;
     img->last_sharpness = G->r.sharpness;
+    img->last_r = G->r;
   }
   ChainRun run;
   run.host = G->host;

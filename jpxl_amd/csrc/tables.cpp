@@ -178,6 +178,8 @@ float powi_f(float a, int b) {
 
 }  // namespace
 
+void natural_order_table(int side, uint16_t* order) { natural_order(side, order); }
+
 float qm_multiplier(uint32_t qm_scale) { return powi_f(0.8f, (int)qm_scale - 2); }
 
 float hf_multiplier(uint32_t gs, uint32_t hf_mul) {  // reconstruct.rs:27-36

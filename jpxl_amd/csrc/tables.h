@@ -17,6 +17,8 @@ void build_probe_params(const jpxlg_quantizer& q, int sharpness, ProbeParams* p)
 // request's (legacy rate proxy, flat cell weights), else the calibrated rate
 // model with quantizer-donor weights.
 void set_cover_objective(ProbeParams* p, bool legacy);
+// I.3.2 natural coefficient order of a square DCT (position -> cell).
+void natural_order_table(int side, uint16_t* order);
 float qm_multiplier(uint32_t qm_scale);
 float hf_multiplier(uint32_t global_scale, uint32_t hf_mul);
 

@@ -1,0 +1,880 @@
+// Test infrastructure: a JPEG XL codestream reader used to check the
+// library's emitted streams.  It restates the reading rules of the
+// reference's decoder (jpxl-decode, jpxl-entropy, jpxl-bitstream):
+// headers (jpxl-decode/src/headers/*.rs), the frame header and TOC
+// (frame/header.rs, frame/toc.rs), the entropy streams (jpxl-entropy/src/
+// decoder.rs, dist.rs, prefix.rs, hybrid.rs, ans.rs, lz77.rs), modular
+// sub-bitstreams (modular/mod.rs, tree.rs, predictor.rs) and the VarDCT
+// sections (vardct/block_ctx.rs, hf_coeff.rs, hf_meta.rs, lf.rs), and returns
+// the quantized plan it read: HF coefficients in the footprint layout, the LF
+// planes, the block map, HfMul, the CfL tiles and the sharpness.  It never
+// renders.  Features the library does not emit and the checks do not need
+// (patches, splines, noise, custom dequantization matrices, transforms of
+// modular images, progressive passes, ICC profiles) are refused with a
+// message naming the clause.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct Fail : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+[[noreturn]] void fail(const std::string& m) { throw Fail(m); }
+
+// ---- bits (B.2) ------------------------------------------------------------
+struct Reader {
+  const uint8_t* d;
+  size_t n;
+  uint64_t pos = 0;  // bits
+  uint32_t bits(uint32_t k) {
+    if (k == 0) return 0;
+    if (pos + k > (uint64_t)n * 8) fail("read past the end of the stream");
+    uint64_t v = 0;
+    const size_t byte = pos >> 3;
+    const uint32_t sh = pos & 7;
+    for (uint32_t i = 0; i < 8 && byte + i < n; i++) v |= (uint64_t)d[byte + i] << (8 * i);
+    pos += k;
+    return (uint32_t)((v >> sh) & ((k == 32) ? 0xFFFFFFFFull : ((1ull << k) - 1)));
+  }
+  bool b() { return bits(1) != 0; }
+  void pad() {
+    const uint32_t r = pos & 7;
+    if (r) {
+      if (bits(8 - r) != 0) fail("B.2.7: ZeroPadToByte bits are not zero");
+    }
+  }
+};
+
+struct Dist {
+  bool val;
+  uint32_t bits, off;
+};
+constexpr Dist V(uint32_t v) { return {true, 0, v}; }
+constexpr Dist B(uint32_t b, uint32_t o = 0) { return {false, b, o}; }
+uint32_t u32(Reader& r, const Dist (&d)[4]) {
+  const Dist& x = d[r.bits(2)];
+  return x.val ? x.off : x.off + r.bits(x.bits);
+}
+uint64_t u64(Reader& r) {
+  const uint32_t s = r.bits(2);
+  if (s == 0) return 0;
+  if (s == 1) return 1 + r.bits(4);
+  if (s == 2) return 17 + r.bits(8);
+  uint64_t v = r.bits(12);
+  uint32_t shift = 12;
+  while (r.b()) {
+    if (shift == 60) {
+      v |= (uint64_t)r.bits(4) << 60;
+      break;
+    }
+    v |= (uint64_t)r.bits(8) << shift;
+    shift += 8;
+  }
+  return v;
+}
+float f16(Reader& r) {
+  const uint32_t h = r.bits(16);
+  const uint32_t sign = h >> 15, e = (h >> 10) & 31, m = h & 1023;
+  if (e == 31) fail("B.2.4: F16 is not finite");
+  float v = e == 0 ? std::ldexp((float)m, -24) : std::ldexp((float)(m + 1024), (int)e - 25);
+  return sign ? -v : v;
+}
+uint32_t bit_width(uint32_t n) { return n ? 32 - __builtin_clz(n) : 0; }
+uint32_t ceil_log2(uint64_t n) {
+  uint32_t b = 0;
+  while ((1ull << b) < n) b++;
+  return b;
+}
+int32_t unpack_signed(uint32_t u) { return (u & 1) ? -(int32_t)(u >> 1) - 1 : (int32_t)(u >> 1); }
+
+// ---- entropy (C.2, C.3) -----------------------------------------------------
+struct Hybrid {
+  uint32_t se = 0, msb = 0, lsb = 0;
+  void read(Reader& r, uint32_t log_alpha) {
+    se = r.bits(bit_width(log_alpha));
+    if (se > log_alpha) fail("C.2.3: split_exponent exceeds log_alphabet_size");
+    if (se == log_alpha) { msb = lsb = 0; return; }
+    msb = r.bits(bit_width(se));
+    if (msb > se) fail("C.2.3: msb_in_token exceeds split_exponent");
+    lsb = r.bits(bit_width(se - msb));
+    if (lsb > se - msb) fail("C.2.3: lsb_in_token too large");
+  }
+  uint32_t value(Reader& r, uint32_t tok) const {
+    const uint32_t split = 1u << se;
+    if (tok < split) return tok;
+    const uint32_t in = msb + lsb;
+    const uint32_t nb = se - in + ((tok - split) >> in);
+    if (nb >= 32) fail("C.3.3: extra-bit count too large");
+    const uint32_t low = tok & ((1u << lsb) - 1);
+    uint32_t mid = (tok >> lsb) & ((1u << msb) - 1);
+    mid |= 1u << msb;
+    const uint64_t v = ((((uint64_t)mid << nb) | r.bits(nb)) << lsb) | low;
+    if (v >> 32) fail("C.3.3: value exceeds 32 bits");
+    return (uint32_t)v;
+  }
+};
+
+struct Prefix {
+  int single = -1;
+  uint32_t counts[16] = {0};
+  std::vector<uint32_t> sorted;
+  void from_lengths(const std::vector<uint8_t>& len) {
+    int present = 0, last = 0;
+    memset(counts, 0, sizeof counts);
+    for (size_t s = 0; s < len.size(); s++)
+      if (len[s]) {
+        if (len[s] > 15) fail("RFC 7932 3.2: code length over 15");
+        counts[len[s]]++;
+        present++;
+        last = (int)s;
+      }
+    if (present == 0) fail("RFC 7932 3.2: no symbol has a code");
+    if (present == 1) { single = last; return; }
+    uint64_t used = 0;
+    for (int l = 1; l <= 15; l++) used += (uint64_t)counts[l] << (15 - l);
+    if (used != (1u << 15)) fail("RFC 7932 3.2: prefix code incomplete or over-subscribed");
+    sorted.clear();
+    for (int l = 1; l <= 15; l++)
+      for (size_t s = 0; s < len.size(); s++)
+        if (len[s] == l) sorted.push_back((uint32_t)s);
+  }
+  uint32_t decode(Reader& r) const {
+    if (single >= 0) return (uint32_t)single;
+    uint32_t code = 0, first = 0, index = 0;
+    for (int l = 1; l <= 15; l++) {
+      code |= r.bits(1);
+      if (code < first + counts[l]) return sorted[index + code - first];
+      index += counts[l];
+      first = (first + counts[l]) << 1;
+      code <<= 1;
+    }
+    fail("RFC 7932 3.2: no code matched");
+  }
+};
+
+const int kClcOrder[18] = {1, 2, 3, 4, 0, 5, 17, 6, 16, 7, 8, 9, 10, 11, 12, 13, 14, 15};
+
+Prefix read_prefix(Reader& r, uint32_t alphabet) {
+  Prefix p;
+  if (alphabet == 1) { p.single = 0; return p; }
+  const uint32_t hskip = r.bits(2);
+  if (hskip == 1) {
+    const uint32_t abits = bit_width(alphabet - 1);
+    const uint32_t nsym = r.bits(2) + 1;
+    uint32_t sym[4];
+    for (uint32_t i = 0; i < nsym; i++) {
+      sym[i] = r.bits(abits);
+      if (sym[i] >= alphabet) fail("RFC 7932 3.4: symbol outside the alphabet");
+      for (uint32_t j = 0; j < i; j++)
+        if (sym[j] == sym[i]) fail("RFC 7932 3.4: duplicate symbol");
+    }
+    if (nsym == 1) { p.single = (int)sym[0]; return p; }
+    static const uint8_t L2[] = {1, 1}, L3[] = {1, 2, 2}, L4a[] = {2, 2, 2, 2}, L4b[] = {1, 2, 3, 3};
+    const uint8_t* lens = nsym == 2 ? L2 : nsym == 3 ? L3 : (r.b() ? L4b : L4a);
+    std::vector<uint8_t> len(alphabet, 0);
+    for (uint32_t i = 0; i < nsym; i++) len[sym[i]] = lens[i];
+    p.from_lengths(len);
+    return p;
+  }
+  Prefix fixed;
+  fixed.from_lengths({2, 4, 3, 2, 2, 4});
+  std::vector<uint8_t> clc(18, 0);
+  int space = 32, nonzero = 0;
+  for (int k = (int)hskip; k < 18; k++) {
+    const uint32_t l = fixed.decode(r);
+    clc[kClcOrder[k]] = (uint8_t)l;
+    if (l) {
+      nonzero++;
+      space -= 32 >> l;
+      if (space <= 0) break;
+    }
+  }
+  if (nonzero != 1 && space != 0) fail("RFC 7932 3.5: code-length code incomplete");
+  Prefix cl;
+  if (nonzero == 1) {
+    for (int s = 0; s < 18; s++)
+      if (clc[s]) cl.single = s;
+  } else {
+    cl.from_lengths(clc);
+  }
+  std::vector<uint8_t> len(alphabet, 0);
+  int64_t sp = 32768;
+  uint32_t idx = 0, repeat = 0;
+  uint8_t prev_nz = 8, repeat_len = 0;
+  while (idx < alphabet && sp > 0) {
+    const uint32_t s = cl.decode(r);
+    if (s < 16) {
+      repeat = 0;
+      len[idx++] = (uint8_t)s;
+      if (s) {
+        prev_nz = (uint8_t)s;
+        sp -= 32768 >> s;
+      }
+      continue;
+    }
+    const uint32_t eb = s == 16 ? 2 : 3;
+    const uint8_t nl = s == 16 ? prev_nz : 0;
+    if (repeat_len != nl) {
+      repeat = 0;
+      repeat_len = nl;
+    }
+    const uint32_t before = repeat;
+    if (repeat > 0) repeat = (repeat - 2) << eb;
+    repeat += 3 + r.bits(eb);
+    const uint32_t cnt = repeat - before;
+    if (idx + cnt > alphabet) fail("RFC 7932 3.5: repeat past the alphabet");
+    for (uint32_t i = 0; i < cnt; i++) len[idx++] = nl;
+    if (nl) sp -= (int64_t)cnt * (32768 >> nl);
+  }
+  if (sp != 0) fail("RFC 7932 3.5: prefix code incomplete");
+  p.from_lengths(len);
+  return p;
+}
+
+struct SymbolDecoder;
+std::vector<uint8_t> read_cluster_map(Reader& r, uint32_t num_dist, int depth);
+
+struct SymbolDecoder {
+  std::vector<uint8_t> map;
+  uint32_t num_clusters = 1;
+  std::vector<Hybrid> cfg;
+  std::vector<Prefix> codes;
+  void open(Reader& r, uint32_t num_dist, int depth = 0) {
+    if (depth > 4) fail("C.2.1: nested too deep");
+    if (r.b()) fail("C.2.1: LZ77 streams are not read by this checker");
+    map = read_cluster_map(r, num_dist, depth);
+    num_clusters = 0;
+    for (uint8_t c : map) num_clusters = std::max<uint32_t>(num_clusters, c + 1u);
+    if (!r.b()) fail("C.2.1: ANS streams are not read by this checker");
+    cfg.resize(num_clusters);
+    for (auto& c : cfg) c.read(r, 15);
+    std::vector<uint32_t> count(num_clusters);
+    for (auto& c : count) {
+      if (r.b()) {
+        const uint32_t nb = r.bits(4);
+        c = 1 + (1u << nb) + r.bits(nb);
+        if (c > 32768) fail("C.2.1: alphabet too large");
+      } else {
+        c = 1;
+      }
+    }
+    codes.clear();
+    for (uint32_t k = 0; k < num_clusters; k++) codes.push_back(read_prefix(r, count[k]));
+  }
+  uint32_t read(Reader& r, uint32_t ctx) {
+    if (ctx >= map.size()) fail("C.2.2: context outside the stream");
+    const uint32_t c = map[ctx];
+    return cfg[c].value(r, codes[c].decode(r));
+  }
+};
+
+std::vector<uint8_t> read_cluster_map(Reader& r, uint32_t num_dist, int depth) {
+  std::vector<uint8_t> m(num_dist, 0);
+  if (num_dist == 1) return m;
+  if (r.b()) {
+    const uint32_t nb = r.bits(2);
+    for (auto& v : m) v = (uint8_t)r.bits(nb);
+  } else {
+    const bool mtf = r.b();
+    SymbolDecoder nested;
+    nested.open(r, 1, depth + 1);
+    for (auto& v : m) {
+      const uint32_t x = nested.read(r, 0);
+      if (x > 255) fail("C.2.2: cluster index over 255");
+      v = (uint8_t)x;
+    }
+    if (mtf) {
+      uint8_t t[256];
+      for (int i = 0; i < 256; i++) t[i] = (uint8_t)i;
+      for (auto& v : m) {
+        const uint8_t idx = v, val = t[idx];
+        v = val;
+        for (int j = idx; j > 0; j--) t[j] = t[j - 1];
+        t[0] = val;
+      }
+    }
+  }
+  uint32_t nc = 0;
+  for (uint8_t c : m) nc = std::max<uint32_t>(nc, c + 1u);
+  std::vector<bool> seen(nc, false);
+  for (uint8_t c : m) seen[c] = true;
+  for (bool s : seen)
+    if (!s) fail("C.2.2: cluster map is not dense");
+  return m;
+}
+
+// ---- modular (Annex H) -------------------------------------------------------
+struct Channel {
+  uint32_t w = 0, h = 0;
+  std::vector<int32_t> v;
+  int32_t at(uint32_t x, uint32_t y) const { return v[(size_t)y * w + x]; }
+};
+
+struct Node {
+  bool leaf;
+  uint32_t prop;
+  int32_t value;
+  uint32_t left, right;
+  uint32_t ctx, pred;
+  int32_t offset;
+  int64_t mul;
+};
+
+std::vector<Node> read_tree(Reader& r, SymbolDecoder& sd, uint32_t* num_leaves) {
+  std::vector<Node> t;
+  uint32_t left = 1, ctx = 0;
+  while (left > 0) {
+    left--;
+    if (t.size() >= (1u << 20)) fail("H.4.2: tree too large");
+    Node n{};
+    const uint32_t p = sd.read(r, 1);
+    if (p > 0) {
+      n.leaf = false;
+      n.prop = p - 1;
+      n.value = unpack_signed(sd.read(r, 0));
+      n.left = (uint32_t)t.size() + left + 1;
+      n.right = n.left + 1;
+      left += 2;
+    } else {
+      n.leaf = true;
+      n.ctx = ctx++;
+      n.pred = sd.read(r, 2);
+      if (n.pred > 13) fail("H.4.2: predictor out of range");
+      n.offset = unpack_signed(sd.read(r, 3));
+      const uint32_t ml = sd.read(r, 4);
+      const uint32_t mb = sd.read(r, 5);
+      if (ml > 30 || mb > (1u << (31 - ml)) - 2) fail("H.4.2: multiplier out of range");
+      n.mul = (int64_t)(mb + 1) << ml;
+    }
+    t.push_back(n);
+  }
+  *num_leaves = ctx;
+  return t;
+}
+
+struct Nb {
+  int64_t w, n, nw, ne, nn, nee, ww;
+};
+Nb gather(const Channel& c, uint32_t x, uint32_t y) {
+  Nb b;
+  auto at = [&](uint32_t X, uint32_t Y) { return (int64_t)c.at(X, Y); };
+  b.w = x > 0 ? at(x - 1, y) : (y > 0 ? at(x, y - 1) : 0);
+  b.n = y > 0 ? at(x, y - 1) : b.w;
+  b.nw = (x > 0 && y > 0) ? at(x - 1, y - 1) : b.w;
+  b.ne = (x + 1 < c.w && y > 0) ? at(x + 1, y - 1) : b.n;
+  b.nn = y > 1 ? at(x, y - 2) : b.n;
+  b.nee = (x + 2 < c.w && y > 0) ? at(x + 2, y - 1) : b.ne;
+  b.ww = x > 1 ? at(x - 2, y) : b.w;
+  return b;
+}
+int64_t gradient(int64_t w, int64_t n, int64_t nw) {
+  const int64_t g = w + n - nw;
+  return std::min(std::max(g, std::min(w, n)), std::max(w, n));
+}
+int32_t narrow(int64_t v) { return (int32_t)std::min<int64_t>(std::max<int64_t>(v, INT32_MIN), INT32_MAX); }
+
+void read_modular(Reader& r, std::vector<Channel>& ch, uint32_t stream_id) {
+  if (ch.empty()) return;
+  if (r.b()) fail("H.2: global trees are not read by this checker");
+  if (!r.b()) fail("H.5.1: custom weighted-predictor parameters are not read by this checker");
+  static const Dist kNbT[4] = {V(0), V(1), B(4, 2), B(8, 18)};
+  if (u32(r, kNbT) != 0) fail("H.6: modular transforms are not read by this checker");
+  SymbolDecoder ts;
+  ts.open(r, 6);
+  uint32_t leaves = 0;
+  std::vector<Node> tree = read_tree(r, ts, &leaves);
+  SymbolDecoder ds;
+  ds.open(r, leaves);
+  for (size_t i = 0; i < ch.size(); i++) {
+    Channel& c = ch[i];
+    if (!c.w || !c.h) continue;
+    // previous channels of identical shape (H.4.1)
+    std::vector<size_t> prevs;
+    for (size_t j = i; j-- > 0;)
+      if (ch[j].w == c.w && ch[j].h == c.h) prevs.push_back(j);
+    std::vector<int32_t> props(16 + 4 * prevs.size());
+    for (uint32_t y = 0; y < c.h; y++)
+      for (uint32_t x = 0; x < c.w; x++) {
+        const Nb nb = gather(c, x, y);
+        int64_t wge = nb.w;
+        if (x > 0) {
+          const Nb l = gather(c, x - 1, y);
+          wge = nb.w - (l.w + l.n - l.nw);
+        }
+        const int64_t st[16] = {(int64_t)i, stream_id, y, x, nb.n < 0 ? -nb.n : nb.n, nb.w < 0 ? -nb.w : nb.w, nb.n, nb.w, wge,
+                                nb.w + nb.n - nb.nw, nb.w - nb.nw, nb.nw - nb.n, nb.n - nb.ne, nb.n - nb.nn,
+                                nb.w - nb.ww, 0};
+        for (int k = 0; k < 16; k++) props[k] = narrow(st[k]);
+        size_t k = 16;
+        for (size_t j : prevs) {
+          const Channel& o = ch[j];
+          const int64_t rc = o.at(x, y);
+          const int64_t rw = x > 0 ? o.at(x - 1, y) : 0;
+          const int64_t rn = y > 0 ? o.at(x, y - 1) : rw;
+          const int64_t rnw = (x > 0 && y > 0) ? o.at(x - 1, y - 1) : rw;
+          const int64_t rg = gradient(rw, rn, rnw);
+          const int64_t d = rc - rg;
+          for (int64_t v : {rc < 0 ? -rc : rc, rc, d < 0 ? -d : d, d}) props[k++] = narrow(v);
+        }
+        uint32_t at = 0;
+        while (!tree[at].leaf) {
+          const Node& n = tree[at];
+          if (n.prop == 15) fail("H.4.1: property 15 needs the weighted predictor, not read by this checker");
+          const int32_t p = n.prop < props.size() ? props[n.prop] : 0;
+          at = p > n.value ? n.left : n.right;
+          if (at >= tree.size()) fail("H.4.1: tree walk out of range");
+        }
+        const Node& lf = tree[at];
+        int64_t pred;
+        switch (lf.pred) {
+          case 0: pred = 0; break;
+          case 1: pred = nb.w; break;
+          case 2: pred = nb.n; break;
+          case 3: pred = (nb.w + nb.n) / 2; break;
+          case 4: {
+            const int64_t a = nb.n - nb.nw, b2 = nb.w - nb.nw;
+            pred = (a < 0 ? -a : a) < (b2 < 0 ? -b2 : b2) ? nb.w : nb.n;
+            break;
+          }
+          case 5: pred = gradient(nb.w, nb.n, nb.nw); break;
+          case 6: fail("H.5: the weighted predictor is not read by this checker");
+          case 7: pred = nb.ne; break;
+          case 8: pred = nb.nw; break;
+          case 9: pred = nb.ww; break;
+          case 10: pred = (nb.w + nb.nw) / 2; break;
+          case 11: pred = (nb.n + nb.nw) / 2; break;
+          case 12: pred = (nb.n + nb.ne) / 2; break;
+          default: pred = (6 * nb.n - 2 * nb.nn + 7 * nb.w + nb.ww + nb.nee + 3 * nb.ne + 8) / 16; break;
+        }
+        const int64_t diff = (int64_t)unpack_signed(ds.read(r, lf.ctx)) * lf.mul + lf.offset;
+        c.v[(size_t)y * c.w + x] = narrow(diff + pred);
+      }
+  }
+}
+
+// ---- VarDCT tables -------------------------------------------------------------
+const uint8_t kDefaultBlockCtx[39] = {0, 1, 2, 2, 3, 3, 4, 5, 6, 6, 6, 6, 6, 7, 8, 9, 9, 10, 11, 12, 13, 14,
+                                      14, 14, 14, 14, 7, 8, 9, 9, 10, 11, 12, 13, 14, 14, 14, 14, 14};
+const uint32_t kFreq[64] = {0,  0,  1,  2,  3,  4,  5,  6,  7,  8,  9,  10, 11, 12, 13, 14,
+                            15, 15, 16, 16, 17, 17, 18, 18, 19, 19, 20, 20, 21, 21, 22, 22,
+                            23, 23, 23, 23, 24, 24, 24, 24, 25, 25, 25, 25, 26, 26, 26, 26,
+                            27, 27, 27, 27, 28, 28, 28, 28, 29, 29, 29, 29, 30, 30, 30, 30};
+const uint32_t kNzc[64] = {0,   0,   31,  62,  62,  93,  93,  93,  93,  123, 123, 123, 123,
+                           152, 152, 152, 152, 152, 152, 152, 152, 180, 180, 180, 180, 180,
+                           180, 180, 180, 180, 180, 180, 180, 206, 206, 206, 206, 206, 206,
+                           206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206,
+                           206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206, 206};
+
+// I.3.2 natural order of a square DCT of side s (varblock.rs:665-745):
+// the LLF cells first, then the anti-diagonals, alternating direction.
+std::vector<uint32_t> natural_order(uint32_t s) {
+  const uint32_t c = s / 8;
+  std::vector<uint32_t> o;
+  for (uint32_t y = 0; y < c; y++)
+    for (uint32_t x = 0; x < c; x++) o.push_back(y * s + x);
+  for (uint32_t d = 0; d < 2 * s - 1; d++) {
+    std::vector<uint32_t> diag;
+    for (uint32_t y = 0; y < s; y++) {
+      if (d < y || d - y >= s) continue;
+      const uint32_t x = d - y;
+      if (x < c && y < c) continue;
+      diag.push_back(y * s + x);
+    }
+    // along the diagonal by x - y, reversed on odd diagonals
+    std::vector<std::pair<int, uint32_t>> k;
+    for (uint32_t cell : diag) {
+      const int x = (int)(cell % s), y = (int)(cell / s);
+      k.push_back({(d % 2) ? -(x - y) : (x - y), cell});
+    }
+    std::sort(k.begin(), k.end());
+    for (auto& e : k) o.push_back(e.second);
+  }
+  return o;
+}
+
+}  // namespace
+
+extern "C" {
+
+typedef struct {
+  uint32_t width, height, bits, xyb;
+  uint32_t encoding, frame_type, x_qm_scale, b_qm_scale, gaborish, epf_iters, num_passes, is_last;
+  uint64_t flags;
+  uint32_t global_scale, quant_lf;
+  int32_t x_factor_lf, b_factor_lf;
+  uint32_t lf_corr_default, num_sections, num_groups, num_lf_groups;
+  uint64_t header_bytes, toc_sum;
+  uint32_t stage;  // 1 headers, 2 frame header, 3 TOC, 4 LfGlobal, 5 LF groups, 6 HfGlobal, 7 pass groups
+} cs_info;
+
+// Reads `data`; fills info as far as it gets.  The plan outputs (any may be
+// NULL): quant 3 x (8 bh) x (8 bw) footprint layout, HF cells only (LLF
+// cells 0); lf 3 x bh x bw; bmap bh x bw (8/16/32 at a varblock's top-left,
+// 0 covered, 100 + DctSelect for other transforms); hf_mul bh x bw at the
+// top-left; cfl_x/cfl_b ceil(bh/8) x ceil(bw/8); sharp bh x bw.
+int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, int32_t* lf, uint8_t* bmap,
+              uint32_t* hf_mul, int8_t* cfl_x, int8_t* cfl_b, uint8_t* sharp, char* err, int errlen) {
+  memset(info, 0, sizeof(*info));
+  try {
+    Reader r{data, len};
+    // Headers: signature, SizeHeader, ImageMetadata (headers/size.rs, metadata.rs)
+    if (r.bits(16) != 0x0AFF) fail("A.1: not a naked codestream (FF 0A)");
+    static const Dist kDim[4] = {B(9, 1), B(13, 1), B(18, 1), B(30, 1)};
+    uint32_t h, w;
+    const bool small = r.b();
+    if (small) h = (r.bits(5) + 1) * 8;
+    else h = u32(r, kDim);
+    const uint32_t ratio = r.bits(3);
+    static const uint32_t kRn[8] = {1, 1, 12, 4, 3, 16, 5, 2}, kRd[8] = {1, 1, 10, 3, 2, 9, 4, 1};
+    if (ratio) w = (uint32_t)((uint64_t)h * kRn[ratio] / kRd[ratio]);
+    else w = small ? (r.bits(5) + 1) * 8 : u32(r, kDim);
+    info->width = w;
+    info->height = h;
+    info->bits = 8;
+    info->xyb = 1;
+    const bool all_default = r.b();
+    if (!all_default) {
+      const bool extra = r.b();
+      if (extra) {
+        r.bits(3);  // orientation - 1
+        if (r.b()) fail("A.3: intrinsic size is not read by this checker");
+        if (r.b()) fail("A.3: previews are not read by this checker");
+        if (r.b()) fail("A.3: animation is not read by this checker");
+      }
+      if (r.b()) fail("A.3: float samples are not read by this checker");
+      static const Dist kBps[4] = {V(8), V(10), V(12), B(6, 1)};
+      info->bits = u32(r, kBps);
+      r.b();  // modular_16_bit_buffer_sufficient
+      static const Dist kNe[4] = {V(0), V(1), B(4, 2), B(12, 1)};
+      if (u32(r, kNe) != 0) fail("A.3: extra channels are not read by this checker");
+      info->xyb = r.b();
+      if (!r.b()) fail("A.4: a non-default colour encoding is not read by this checker");
+      if (extra) {
+        if (!r.b()) fail("A.3: custom tone mapping is not read by this checker");
+      }
+      if (u64(r) != 0) fail("A.3: metadata extensions are not read by this checker");
+    }
+    if (!r.b()) fail("A.3: custom transform data (default_m false) is not read by this checker");
+    r.pad();
+    info->stage = 1;
+    // Frame header (frame/header.rs)
+    if (r.b()) fail("F.2: an all-default frame header is not what the checks expect");
+    info->frame_type = r.bits(2);
+    info->encoding = r.bits(1);
+    info->flags = u64(r);
+    if (info->frame_type != 0 || info->encoding != 0) fail("F.2: only regular VarDCT frames are read");
+    if (info->flags & ~0x80ull) fail("F.2: frame flags other than skip-adaptive-LF-smoothing are not read");
+    if (!info->xyb) fail("F.2: non-XYB VarDCT frames are not read");
+    static const Dist kUp[4] = {V(1), V(2), V(4), V(8)};
+    if (u32(r, kUp) != 1) fail("F.2: upsampling is not read");
+    info->x_qm_scale = r.bits(3);
+    info->b_qm_scale = r.bits(3);
+    static const Dist kNp[4] = {V(1), V(2), V(3), B(3, 4)};
+    info->num_passes = u32(r, kNp);
+    if (info->num_passes != 1) fail("F.2: progressive passes are not read");
+    if (r.b()) fail("F.2: cropped frames are not read");
+    static const Dist kBm[4] = {V(0), V(1), V(2), B(2, 3)};
+    if (u32(r, kBm) != 0) fail("F.2: blending is not read");
+    info->is_last = r.b();
+    if (!info->is_last) r.bits(2);  // save_as_reference
+    static const Dist kNl[4] = {V(0), B(4), B(5, 16), B(10, 48)};
+    const uint32_t name_len = u32(r, kNl);
+    for (uint32_t i = 0; i < name_len; i++) r.bits(8);
+    // RestorationFilter
+    if (r.b()) {
+      info->gaborish = 1;
+      info->epf_iters = 1;
+    } else {
+      info->gaborish = r.b();
+      if (info->gaborish && r.b()) fail("J.1: custom Gaborish weights are not read");
+      info->epf_iters = r.bits(2);
+      if (info->epf_iters) {
+        if (r.b()) fail("J.1: custom EPF sharpness is not read");
+        if (r.b()) fail("J.1: custom EPF weights are not read");
+        if (r.b()) fail("J.1: custom EPF sigma is not read");
+      }
+      if (u64(r) != 0) fail("J.1: restoration extensions are not read");
+    }
+    if (u64(r) != 0) fail("F.2: frame extensions are not read");
+    info->stage = 2;
+    // TOC (frame/toc.rs)
+    const uint32_t bw = (w + 7) / 8, bh = (h + 7) / 8;
+    const uint32_t gx = (w + 255) / 256, gy = (h + 255) / 256, lgx = (w + 2047) / 2048, lgy = (h + 2047) / 2048;
+    const uint32_t ng = gx * gy, nlf = lgx * lgy;
+    info->num_groups = ng;
+    info->num_lf_groups = nlf;
+    const bool single = ng == 1;
+    const uint32_t nsec = single ? 1 : 2 + nlf + ng;
+    info->num_sections = nsec;
+    if (r.b()) fail("F.3.2: permuted TOCs are not read");
+    r.pad();
+    static const Dist kToc[4] = {B(10), B(14, 1024), B(22, 17408), B(30, 4211712)};
+    std::vector<uint64_t> sec(nsec);
+    for (auto& s : sec) {
+      s = u32(r, kToc);
+      info->toc_sum += s;
+    }
+    r.pad();
+    info->header_bytes = r.pos / 8;
+    if (info->header_bytes + info->toc_sum != len)
+      fail("F.3: the TOC's section lengths do not add up to the stream's length");
+    info->stage = 3;
+    std::vector<uint64_t> start(nsec);
+    uint64_t at = info->header_bytes;
+    for (uint32_t i = 0; i < nsec; i++) {
+      start[i] = at;
+      at += sec[i];
+    }
+    auto section = [&](uint32_t i) {
+      Reader s{data + start[i], (size_t)sec[i]};
+      return s;
+    };
+    Reader one = section(0);
+    // LfGlobal (G.1)
+    {
+      Reader& s = one;
+      if (!s.b()) fail("G.1.2: custom LF dequantization is not read");
+      static const Dist kGs[4] = {B(11, 1), B(11, 2049), B(12, 4097), B(16, 8193)};
+      static const Dist kQl[4] = {V(16), B(5, 1), B(8, 1), B(16, 1)};
+      info->global_scale = u32(s, kGs);
+      info->quant_lf = u32(s, kQl);
+      if (!s.b()) fail("I.2.2: custom HF block contexts are not read");
+      info->lf_corr_default = s.b();
+      if (!info->lf_corr_default) {
+        static const Dist kCf[4] = {V(84), V(256), B(8, 2), B(16, 258)};
+        if (u32(s, kCf) != 84) fail("I.2.3: colour factor other than 84 is not read");
+        if (f16(s) != 0.0f || f16(s) != 1.0f) fail("I.2.3: base correlations other than 0 and 1 are not read");
+        info->x_factor_lf = (int32_t)s.bits(8) - 128;
+        info->b_factor_lf = (int32_t)s.bits(8) - 128;
+      }
+      if (s.b()) fail("G.1.3: global trees are not read");
+      if (!single) s.pad();
+    }
+    info->stage = 4;
+    const uint32_t tx = (bw + 7) / 8;
+    std::vector<uint8_t> bm((size_t)bw * bh, 255);
+    std::vector<uint32_t> mul((size_t)bw * bh, 0);
+    // LF groups (G.2)
+    for (uint32_t id = 0; id < nlf; id++) {
+      Reader sep = single ? Reader{nullptr, 0} : section(1 + id);
+      Reader& s = single ? one : sep;
+      const uint32_t lx = id % lgx, ly = id / lgx, b0x = lx * 256, b0y = ly * 256;
+      const uint32_t gw = std::min(256u, bw - b0x), gh = std::min(256u, bh - b0y);
+      if (s.bits(2) != 0) fail("G.2.2: extra LF precision is not read");
+      std::vector<Channel> c(3);
+      for (auto& x : c) {
+        x.w = gw;
+        x.h = gh;
+        x.v.assign((size_t)gw * gh, 0);
+      }
+      read_modular(s, c, 1 + id);
+      if (lf) {
+        const int plane_of[3] = {1, 0, 2};
+        for (int i = 0; i < 3; i++)
+          for (uint32_t y = 0; y < gh; y++)
+            for (uint32_t x = 0; x < gw; x++)
+              lf[(size_t)plane_of[i] * bw * bh + (size_t)(b0y + y) * bw + b0x + x] = c[i].at(x, y);
+      }
+      const uint32_t nb = s.bits(ceil_log2(std::max<uint64_t>((uint64_t)gw * gh, 1))) + 1;
+      const uint32_t tw = (gw + 7) / 8, th = (gh + 7) / 8;
+      std::vector<Channel> m(4);
+      m[0].w = m[1].w = tw;
+      m[0].h = m[1].h = th;
+      m[2].w = nb;
+      m[2].h = 2;
+      m[3].w = gw;
+      m[3].h = gh;
+      for (auto& x : m) x.v.assign((size_t)x.w * x.h, 0);
+      read_modular(s, m, 0);
+      for (uint32_t y = 0; y < th; y++)
+        for (uint32_t x = 0; x < tw; x++) {
+          const int32_t vx = m[0].at(x, y), vb = m[1].at(x, y);
+          if (vx < -128 || vx > 127 || vb < -128 || vb > 127) fail("G.2.4: CfL factor outside int8");
+          const size_t t = (size_t)(ly * 32 + y) * tx + lx * 32 + x;
+          if (cfl_x) cfl_x[t] = (int8_t)vx;
+          if (cfl_b) cfl_b[t] = (int8_t)vb;
+        }
+      // place the varblocks: each goes to the first uncovered block in raster order
+      uint32_t cur = 0;
+      for (uint32_t i = 0; i < nb; i++) {
+        const int32_t sel = m[2].at(i, 0), q = m[2].at(i, 1);
+        if (sel < 0 || sel > 26) fail("G.2.4: DctSelect out of range");
+        if (q < 0) fail("G.2.4: negative HfMul");
+        uint32_t bs = sel == 0 ? 1 : sel == 4 ? 2 : sel == 5 ? 4 : 0;
+        if (!bs) fail("I.2: transforms other than DCT8x8/16x16/32x32 are not read by this checker");
+        while (cur < (uint32_t)gw * gh && bm[(size_t)(b0y + cur / gw) * bw + b0x + cur % gw] != 255) cur++;
+        if (cur >= (uint32_t)gw * gh) fail("G.2.4: more varblocks than blocks");
+        const uint32_t X = cur % gw, Y = cur / gw;
+        if (X % bs || Y % bs || X + bs > gw || Y + bs > gh) fail("G.2.4: varblock misplaced");
+        for (uint32_t j = 0; j < bs; j++)
+          for (uint32_t k = 0; k < bs; k++) {
+            uint8_t& cell = bm[(size_t)(b0y + Y + j) * bw + b0x + X + k];
+            if (cell != 255) fail("G.2.4: varblocks overlap");
+            cell = 0;
+          }
+        bm[(size_t)(b0y + Y) * bw + b0x + X] = (uint8_t)(8 * bs);
+        mul[(size_t)(b0y + Y) * bw + b0x + X] = (uint32_t)q + 1;
+      }
+      for (uint32_t y = 0; y < gh; y++)
+        for (uint32_t x = 0; x < gw; x++) {
+          if (bm[(size_t)(b0y + y) * bw + b0x + x] == 255) fail("G.2.4: blocks left uncovered");
+          const int32_t sv = m[3].at(x, y);
+          if (sv < 0 || sv > 7) fail("G.2.4: sharpness out of range");
+          if (sharp) sharp[(size_t)(b0y + y) * bw + b0x + x] = (uint8_t)sv;
+        }
+      if (!single) s.pad();
+    }
+    if (bmap) memcpy(bmap, bm.data(), bm.size());
+    if (hf_mul) memcpy(hf_mul, mul.data(), mul.size() * 4);
+    info->stage = 5;
+    // HfGlobal (G.3)
+    const uint32_t nctx = 495 * 15;
+    SymbolDecoder hf;
+    uint32_t presets;
+    {
+      Reader sep = single ? Reader{nullptr, 0} : section(1 + nlf);
+      Reader& s = single ? one : sep;
+      if (!s.b()) fail("I.2.4: custom dequantization matrices are not read");
+      presets = s.bits(ceil_log2(ng)) + 1;
+      static const Dist kUo[4] = {V(0x5F), V(0x13), V(0), B(13)};
+      if (u32(s, kUo) != 0) fail("I.3.1: custom coefficient orders are not read");
+      hf.open(s, nctx * presets);
+      if (!single) s.pad();
+    }
+    info->stage = 6;
+    // Pass groups (G.4, I.4)
+    const std::vector<uint32_t> o8 = natural_order(8), o16 = natural_order(16), o32 = natural_order(32);
+    const size_t cs = (size_t)8 * bw, cpl = cs * 8 * bh;
+    if (quant) memset(quant, 0, 3 * cpl * 4);
+    for (uint32_t gid = 0; gid < ng; gid++) {
+      Reader sep = single ? Reader{nullptr, 0} : section(2 + nlf + gid);
+      Reader& s = single ? one : sep;
+      const uint32_t hfp = s.bits(ceil_log2(presets));
+      if (hfp >= presets) fail("I.4: hfp out of range");
+      const uint32_t off = nctx * hfp;
+      const uint32_t g0x = (gid % gx) * 32, g0y = (gid / gx) * 32;
+      const uint32_t gw = std::min(32u, bw - g0x), gh = std::min(32u, bh - g0y);
+      std::vector<uint32_t> grid[3];
+      for (auto& g : grid) g.assign((size_t)gw * gh, 0);
+      for (uint32_t y = 0; y < gh; y++)
+        for (uint32_t x = 0; x < gw; x++) {
+          const uint8_t side = bm[(size_t)(g0y + y) * bw + g0x + x];
+          if (!side) continue;
+          const uint32_t nbk = side / 8, nblocks = nbk * nbk, size = (uint32_t)side * side;
+          const uint32_t order_id = side == 8 ? 0 : side == 16 ? 2 : 3;
+          const std::vector<uint32_t>& order = side == 8 ? o8 : side == 16 ? o16 : o32;
+          for (int c : {1, 0, 2}) {
+            const uint32_t bctx = kDefaultBlockCtx[(c < 2 ? c ^ 1 : 2) * 13 + order_id];
+            std::vector<uint32_t>& g = grid[c];
+            uint32_t pred;
+            if (x == 0 && y == 0) pred = 32;
+            else if (x == 0) pred = g[(y - 1) * gw + x];
+            else if (y == 0) pred = g[y * gw + x - 1];
+            else pred = (g[(y - 1) * gw + x] + g[y * gw + x - 1] + 1) >> 1;
+            pred = std::min(pred, 64u);
+            const uint32_t nz = hf.read(s, bctx + 15 * (pred < 8 ? pred : 4 + pred / 2) + off);
+            if (nz > size - nblocks) fail("I.4: more non-zeros than HF positions");
+            const uint32_t per = (nz + nblocks - 1) / nblocks;
+            for (uint32_t j = 0; j < nbk; j++)
+              for (uint32_t i = 0; i < nbk; i++) g[(y + j) * gw + x + i] = per;
+            if (!nz) continue;
+            uint32_t rem = nz, prev = nz <= size / 16 ? 1 : 0;
+            for (uint32_t k = nblocks; k < size; k++) {
+              const uint32_t ctx = (kNzc[(rem + nblocks - 1) / nblocks] + kFreq[k / nblocks]) * 2 + prev +
+                                   bctx * 458 + 37 * 15 + off;
+              const uint32_t u = hf.read(s, ctx);
+              const int32_t v = unpack_signed(u);
+              if (quant) {
+                const uint32_t cell = order[k];
+                quant[(size_t)c * cpl + (size_t)(8 * (g0y + y) + cell / side) * cs + 8 * (g0x + x) + cell % side] = v;
+              }
+              prev = u != 0;
+              if (u) {
+                if (--rem == 0) break;
+              }
+            }
+            if (rem) fail("I.4: stream ended the varblock with non-zeros undelivered");
+          }
+        }
+      if (!single) {
+        // G.4.2 carries no modular group data here; the rest is padding
+        s.pad();
+        if (s.pos != (uint64_t)s.n * 8) fail("G.4: bytes left over in a pass group");
+      }
+    }
+    if (single) {
+      one.pad();
+      if (one.pos != (uint64_t)one.n * 8) fail("F.3: bytes left over in the single section");
+    }
+    info->stage = 7;
+    return 0;
+  } catch (const Fail& e) {
+    if (err && errlen > 0) snprintf(err, errlen, "%s", e.what());
+    return -1;
+  }
+}
+
+}  // extern "C"
+
+// Entry points for the known-answer checks of the reading rules.
+extern "C" {
+
+// Decodes `count` symbols with the canonical prefix code of `lengths`.
+int cs_prefix_decode(const uint8_t* lengths, int n, const uint8_t* data, size_t len, uint32_t* syms, int count,
+                     uint64_t* bits_read) {
+  try {
+    Prefix p;
+    p.from_lengths(std::vector<uint8_t>(lengths, lengths + n));
+    Reader r{data, len};
+    for (int i = 0; i < count; i++) syms[i] = p.decode(r);
+    *bits_read = r.pos;
+    return 0;
+  } catch (const Fail&) {
+    return -1;
+  }
+}
+
+// C.2.2 cluster map of num_dist contexts.
+int cs_cluster_map(const uint8_t* data, size_t len, uint32_t num_dist, uint8_t* out, uint64_t* bits_read) {
+  try {
+    Reader r{data, len};
+    std::vector<uint8_t> m = read_cluster_map(r, num_dist, 0);
+    memcpy(out, m.data(), m.size());
+    *bits_read = r.pos;
+    return 0;
+  } catch (const Fail&) {
+    return -1;
+  }
+}
+
+// C.3.3 hybrid integer of `token` under (se, msb, lsb), extra bits from data.
+int cs_hybrid(uint32_t se, uint32_t msb, uint32_t lsb, uint32_t token, const uint8_t* data, size_t len,
+              uint32_t* value, uint64_t* bits_read) {
+  try {
+    Hybrid h;
+    h.se = se;
+    h.msb = msb;
+    h.lsb = lsb;
+    Reader r{data, len};
+    *value = h.value(r, token);
+    *bits_read = r.pos;
+    return 0;
+  } catch (const Fail&) {
+    return -1;
+  }
+}
+
+// I.3.2 natural order of a square DCT of side s.
+void cs_natural_order(uint32_t side, uint32_t* out) {
+  const std::vector<uint32_t> o = natural_order(side);
+  memcpy(out, o.data(), o.size() * 4);
+}
+
+}  // extern "C"
