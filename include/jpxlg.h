@@ -291,7 +291,9 @@ typedef struct {
   double target;        /* minimum SSIMULACRA2 score */
   int32_t effort;       /* JPXLG_EFFORT_* */
   int32_t max_probes;   /* 0 = the effort's budget */
-  int32_t mode;         /* 0 = JPXL navigator, 1 = bisection over the ladder */
+  int32_t mode;         /* 0 = JPXL navigator, 1 = bisection over the ladder, 2 = the navigator with
+                           its finalists priced exactly by the codestream stage (the reference's
+                           default build: the smallest feasible stream wins, quality.rs:1886-2010) */
 } jpxlg_quality_request;
 
 /* Why the search stopped where it did (QualityStatus, quality.rs:440-470). */
@@ -317,6 +319,8 @@ typedef struct {
   int32_t structural_builds; /* cover + CfL builds (the anchor and any rebuilt finalist) */
   int32_t structure_rebuilt; /* the chosen plan's structure was rebuilt at its own rung */
   uint8_t probe_fresh[16];   /* the probe built its own structure */
+  int32_t exact_prices;      /* finalists priced by the codestream stage (mode 2) */
+  uint64_t bytes;            /* the chosen finalist's codestream size when priced, else 0 */
 } jpxlg_quality_result;
 
 int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,

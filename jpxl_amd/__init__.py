@@ -643,7 +643,8 @@ class Encoder:
         img = Image(self.ctx, rgb, bits)
         try:
             if self.quality is not None:
-                res = img.encode_quality(self.quality, self.effort)
+                # the reference's default build: finalists priced exactly
+                res = img.encode_quality(self.quality, self.effort, mode=_lib.MODE_PRICED)
                 report = EncodeReport(self.quality, res.score, bool(res.met), res.global_scale, res.hf_mul,
                                       res.quant_lf, [(res.probe_rungs[i], res.probe_scores[i]) for i in range(res.probes)])
             else:

@@ -40,6 +40,9 @@ AQ_OFF, AQ_MASKING, AQ_UNIFORM, AQ_FINE_MASKING, AQ_FINE_UNIFORM, AQ_EDGE_REFINE
 
 EFFORT_FAST = 0
 EFFORT_BALANCED = 1
+MODE_NAVIGATOR = 0
+MODE_BISECTION = 1
+MODE_PRICED = 2
 
 
 class Quantizer(ctypes.Structure):
@@ -165,6 +168,8 @@ class QualityResult(ctypes.Structure):
         ("structural_builds", ctypes.c_int32),
         ("structure_rebuilt", ctypes.c_int32),
         ("probe_fresh", ctypes.c_uint8 * 16),
+        ("exact_prices", ctypes.c_int32),
+        ("bytes", ctypes.c_uint64),
     ]
 
 

@@ -1992,6 +1992,8 @@ static void fill_quality_result(const ControllerOutcome& oc, const QualityPolicy
   o->predicted_rung = predicted;
   o->structural_builds = oc.structural_builds;
   o->structure_rebuilt = oc.chosen_fresh ? 1 : 0;
+  o->exact_prices = oc.exact_prices;
+  o->bytes = oc.bytes;
 }
 
 extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
@@ -2018,10 +2020,27 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
     *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     return prc;
   };
+  if (req->mode < 0 || req->mode > 2) return fail(ctx, JPXLG_E_INVALID, "unknown quality search mode");
+  // The exact price of a finalist (quality.rs:1886-1995): its plan quantized
+  // on its structure and written by the codestream stage (sizes only).
+  PriceFn price = [&](uint32_t rung, bool fresh, uint64_t* bytes) -> int {
+    int prc = ss.need(img, pol, rung, fresh);
+    if (prc) return prc;
+    const jpxlg_quantizer q = pol.quantizer_at(rung);
+    if (!holds_levels_of(img, q)) {
+      prc = jpxlg_quantize(img, &q);
+      if (prc) return prc;
+    }
+    jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
+    size_t n = 0;
+    prc = jpxlg_encode_emit(img, &r, 0, nullptr, 0, &n, nullptr);
+    *bytes = n;
+    return prc;
+  };
   ControllerOutcome oc;
   rc = req->mode == 1 ? bisect_controller(req->target, req->max_probes, probe, &oc)
                       : navigate_controller(req->target, req->effort, req->max_probes, probe, &oc, predicted,
-                                            prior_beta);
+                                            prior_beta, req->mode == 2 ? &price : nullptr);
   if (rc) return rc;
   // Leave the chosen plan quantized on the device for the hand-off, on the
   // structure it was scored with (the last probe's levels usually are).
@@ -2347,6 +2366,8 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
                                        uint64_t* ticket) {
   if (!srv || !img || !q || !ticket) return JPXLG_E_INVALID;
   if (img->ctx != srv->ctx) return fail(srv->ctx, JPXLG_E_INVALID, "the image belongs to another context");
+  if (q->mode < 0 || q->mode > 1)
+    return fail(srv->ctx, JPXLG_E_UNSUPPORTED, "served and batched searches are navigator or bisection only");
   auto r = std::make_unique<ServerReq>();
   ServerReq* R = r.get();
   R->img = img;
@@ -2364,6 +2385,8 @@ extern "C" int32_t jpxlg_server_submit(jpxlg_server* srv, jpxlg_img* img, const 
 extern "C" int32_t jpxlg_server_submit_srgb(jpxlg_server* srv, uint32_t width, uint32_t height, uint32_t bits,
                                             const void* rgb, const jpxlg_quality_request* q, uint64_t* ticket) {
   if (!srv || !rgb || !q || !ticket) return JPXLG_E_INVALID;
+  if (q->mode < 0 || q->mode > 1)
+    return fail(srv->ctx, JPXLG_E_UNSUPPORTED, "served and batched searches are navigator or bisection only");
   if (width < 8 || height < 8) return fail(srv->ctx, JPXLG_E_TOO_SMALL, "image is smaller than the metric's 8x8 floor");
   if (width > 65535 * 8 || height > 65535 * 8) return fail(srv->ctx, JPXLG_E_UNSUPPORTED, "frame too large");
   if (bits < 8 || bits > 16) return fail(srv->ctx, JPXLG_E_UNSUPPORTED, "bit depth outside 8..16");

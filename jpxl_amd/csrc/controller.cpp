@@ -358,7 +358,7 @@ void source_features(std::vector<float> luma, float* luma_q50, float* flat_fract
 }
 
 int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& probe, ControllerOutcome* out,
-                         uint32_t predicted_rung, double prior_beta) {
+                         uint32_t predicted_rung, double prior_beta, const PriceFn* price) {
   const double threshold = target;  // DEFAULT_SCORE_GUARD = 0
   const double band = met_band(threshold, budget.reserve);
   std::vector<ProbeResult>& probes = out->probes;
@@ -518,11 +518,29 @@ int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& 
     out->priced++;
     finalists.push_back({p.rung, p.score, p.feasible, false});
   }
-  // Selection: the feasible finalist with the coarsest rung (the reference
-  // keeps the smallest exact stream); with none feasible, the finest.
+  // Selection (quality.rs:1996-2010): the smallest exact feasible stream
+  // (the first of equal sizes) when there is a choice to price; else the
+  // feasible finalist with the coarsest rung; with none feasible, the finest.
   const Finalist* pick = nullptr;
-  for (auto& f : finalists)
-    if (f.feasible && (!pick || f.rung < pick->rung)) pick = &f;
+  int nfeasible = 0;
+  for (auto& f : finalists) nfeasible += f.feasible;
+  if (price && nfeasible >= 2) {
+    uint64_t best = UINT64_MAX;
+    for (auto& f : finalists) {
+      if (!f.feasible) continue;
+      uint64_t bytes = 0;
+      if ((rc = (*price)(f.rung, f.fresh, &bytes))) return rc;
+      out->exact_prices++;
+      if (bytes < best) {
+        best = bytes;
+        pick = &f;
+      }
+    }
+    out->bytes = best;
+  }
+  if (!pick)
+    for (auto& f : finalists)
+      if (f.feasible && (!pick || f.rung < pick->rung)) pick = &f;
   if (!pick)
     for (auto& f : finalists)
       if (!pick || f.rung >= pick->rung) pick = &f;
@@ -552,10 +570,10 @@ int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& 
 }
 
 int navigate_controller(double target, int32_t effort, int max_probes, const ProbeFn& probe,
-                        ControllerOutcome* out, uint32_t predicted_rung, double prior_beta) {
+                        ControllerOutcome* out, uint32_t predicted_rung, double prior_beta, const PriceFn* price) {
   const QualityPolicy pol = quality_policy(effort);
   NavBudget b{max_probes > 0 ? max_probes : pol.pixel_probes, pol.exact_prices, pol.structural_builds, pol.reserve};
-  return navigate_with_budget(target, b, probe, out, predicted_rung, prior_beta);
+  return navigate_with_budget(target, b, probe, out, predicted_rung, prior_beta, price);
 }
 
 int bisect_controller(double target, int max_probes, const ProbeFn& probe, ControllerOutcome* out) {

@@ -85,6 +85,8 @@ struct ControllerOutcome {
   int pixel_probes = 0;
   int structural_builds = 0;
   int priced = 0;
+  int exact_prices = 0;     // finalists priced by the price function
+  uint64_t bytes = 0;       // the chosen finalist's exact size, when priced
   std::vector<ProbeResult> probes;
 };
 
@@ -92,6 +94,9 @@ struct ControllerOutcome {
 // at this rung's quantizer (the first probe's build is the anchor); otherwise
 // reuse the anchor's structure.
 using ProbeFn = std::function<int(uint32_t, bool, double*, double*)>;
+// price(rung, fresh, &bytes) -> status: the exact codestream size of the plan
+// at this rung on the anchor's structure (fresh: on its own).
+using PriceFn = std::function<int(uint32_t, bool, uint64_t*)>;
 
 // Bisection over the ladder for the coarsest rung meeting `target` (every
 // probe on the first probe's structure).
@@ -105,16 +110,18 @@ int bisect_controller(double target, int max_probes, const ProbeFn& probe, Contr
 // log-loss crossing aims at target + reserve while the coarsest feasible
 // probe lies outside the reserve-coupled accept band, one rescue probe when
 // nothing met.  Finalists are the two coarsest feasible probes; one far from
-// the anchor is rebuilt with its own structure and re-scored.  Exact pricing
-// needs the entropy coder, which this library leaves to the caller: among
-// the feasible finalists the coarsest rung is chosen (the smallest stream at
-// equal structure).
+// the anchor is rebuilt with its own structure and re-scored.  With a price
+// function the feasible finalists are priced exactly and the smallest stream
+// is chosen (the first of equal sizes), as the reference's default build
+// does; a single feasible finalist is chosen without pricing (its size is
+// the emitted stream's).  Without one the coarsest feasible rung is chosen.
 // prior_beta: the crossing predictor's local loss exponent, which replaces
 // PRIOR_LOSS_EXPONENT in the slope-extrapolated steps (NaN: none).
 int navigate_controller(double target, int32_t effort, int max_probes, const ProbeFn& probe,
-                        ControllerOutcome* out, uint32_t predicted_rung, double prior_beta = NAN);
+                        ControllerOutcome* out, uint32_t predicted_rung, double prior_beta = NAN,
+                        const PriceFn* price = nullptr);
 int navigate_with_budget(double target, const NavBudget& budget, const ProbeFn& probe, ControllerOutcome* out,
-                         uint32_t predicted_rung, double prior_beta = NAN);
+                         uint32_t predicted_rung, double prior_beta = NAN, const PriceFn* price = nullptr);
 
 // The legacy initial-quantizer predictor (quality.rs:1021-1079 over the
 // table of quality_predictor.rs): the effective scale the calibration

@@ -733,33 +733,43 @@ struct PreparedChannel {
   uint32_t w = 0, h = 0;
   int pred = 5;
   int32_t offset = 0;
-  std::vector<uint32_t> res;
+  std::vector<uint64_t> tape;         // its residuals' tokens (context added when written)
+  std::vector<uint32_t> hist;         // their counts (kAlphabet)
 };
 
+constexpr Hybrid kModularHybrid{4, 1, 0};
+
 void prepare_channel(const Channel& ch, PreparedChannel* pc) {
-  const Hybrid cfg{4, 1, 0};
   pc->w = ch.w;
   pc->h = ch.h;
   pc->pred = 5;
   pc->offset = 0;
-  pc->res.clear();
+  pc->tape.clear();
+  pc->hist.assign(kAlphabet, 0);
   if (!ch.w || !ch.h) return;
+  const size_t n = (size_t)ch.w * ch.h;
+  std::vector<uint32_t> res;
   if (std::all_of(ch.v.begin(), ch.v.end(), [&](int32_t x) { return x == ch.v[0]; })) {
     pc->pred = 0;
     pc->offset = ch.v[0];
-    pc->res.assign((size_t)ch.w * ch.h, 0u);
-    return;
+    res.assign(n, 0u);
+  } else {
+    const uint32_t step = (uint32_t)std::max<size_t>(1, n / 16384);
+    double best = 1e300;
+    for (int p : {5, 0, 1, 2}) {
+      channel_residuals(ch, p, 0, step, &res);
+      const double cost = residual_cost(res, kModularHybrid);
+      if (cost < best) { best = cost; pc->pred = p; }
+    }
+    channel_residuals(ch, pc->pred, 0, 1, &res);
   }
-  const size_t n = (size_t)ch.w * ch.h;
-  const uint32_t step = (uint32_t)std::max<size_t>(1, n / 16384);
-  double best = 1e300;
-  std::vector<uint32_t> r;
-  for (int p : {5, 0, 1, 2}) {
-    channel_residuals(ch, p, 0, step, &r);
-    const double cost = residual_cost(r, cfg);
-    if (cost < best) { best = cost; pc->pred = p; }
+  pc->tape.resize(n);
+  for (size_t k = 0; k < n; k++) {
+    uint32_t tok, nb, bits;
+    hybrid_token(kModularHybrid, res[k], &tok, &nb, &bits);
+    pc->hist[tok]++;
+    pc->tape[k] = tape_entry(0, tok, nb, bits);
   }
-  channel_residuals(ch, pc->pred, 0, 1, &pc->res);
 }
 
 // Writes a modular sub-bitstream of prepared channels (H.2): local tree,
@@ -792,14 +802,17 @@ void write_modular_prepared(BitWriter& w, const std::vector<PreparedChannel>& pc
     leaf(0);
   }
   write_stream(w, tree, 6, Hybrid{4, 0, 0}, 6);
-  // leaf context of channel i is C-1-i (leaves are numbered in decode order)
-  std::vector<std::pair<uint32_t, uint32_t>> data;
-  size_t total = 0;
-  for (int i = 0; i < C; i++) total += pcs[i].res.size();
-  data.reserve(total);
+  // the data: leaf context of channel i is C-1-i (leaves are numbered in decode order)
+  std::vector<uint32_t> hist((size_t)C * kAlphabet);
   for (int i = 0; i < C; i++)
-    for (uint32_t v : pcs[i].res) data.push_back({(uint32_t)(C - 1 - i), v});
-  write_stream(w, data, C, Hybrid{4, 1, 0}, C);
+    std::copy(pcs[i].hist.begin(), pcs[i].hist.end(), hist.begin() + (size_t)(C - 1 - i) * kAlphabet);
+  EntropyCode ec;
+  ec.build(hist, C, kModularHybrid, C);
+  ec.write_header(w);
+  for (int i = 0; i < C; i++) {
+    const uint64_t ctx = (uint64_t)(C - 1 - i) << 45;
+    for (uint64_t e : pcs[i].tape) ec.put(w, e | ctx);
+  }
 }
 
 void write_modular(BitWriter& w, const std::vector<Channel>& chans) {

@@ -163,4 +163,39 @@ int cc_quality_start(const jpxlg_source_features* f, const jpxlg_transform_summa
   return 0;
 }
 
+
+// The navigator with exact finalist pricing (quality.rs:1886-2010) over a
+// caller's probe and price: price(rung, fresh, &bytes) -> status.
+typedef int (*cc_price_fn)(uint32_t rung, int fresh, uint64_t* bytes);
+int cc_navigate_priced(double target, int effort, int max_probes, uint32_t predicted, cc_probe_fn fn,
+                       cc_price_fn pfn, cc_outcome* out, int32_t* exact_prices, uint64_t* bytes) {
+  memset(out, 0, sizeof(*out));
+  ProbeFn probe = [&](uint32_t rung, bool fresh, double* score, double* ms) -> int {
+    *ms = 0.0;
+    return fn(rung, fresh ? 1 : 0, score);
+  };
+  PriceFn price = [&](uint32_t rung, bool fresh, uint64_t* b) -> int { return pfn(rung, fresh ? 1 : 0, b); };
+  ControllerOutcome oc;
+  const int rc = navigate_controller(target, effort, max_probes, probe, &oc, predicted, NAN, pfn ? &price : nullptr);
+  if (rc) return rc;
+  out->rung = oc.rung;
+  out->score = oc.score;
+  out->met = oc.met;
+  out->status = oc.status;
+  out->chosen_fresh = oc.chosen_fresh;
+  out->anchor_rung = oc.anchor_rung;
+  out->pixel_probes = oc.pixel_probes;
+  out->structural_builds = oc.structural_builds;
+  out->priced = oc.priced;
+  out->nprobes = (int32_t)oc.probes.size();
+  for (size_t i = 0; i < oc.probes.size() && i < 32; i++) {
+    out->probe_rungs[i] = oc.probes[i].rung;
+    out->probe_scores[i] = oc.probes[i].score;
+    out->probe_fresh[i] = oc.probes[i].fresh;
+  }
+  *exact_prices = oc.exact_prices;
+  *bytes = oc.bytes;
+  return 0;
+}
+
 }  // extern "C"

@@ -279,3 +279,57 @@ def test_cpu_encode_runs_the_whole_search():
     assert out.probe_fresh[0] == 1
     quant, lf, bmap = plan[:3]
     assert quant.shape == (3, 80, 96) and lf.shape == (3, 10, 12) and bmap.shape == (10, 12)
+
+
+PROBE_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_double))
+PRICE_CB = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_uint32, ctypes.c_int, ctypes.POINTER(ctypes.c_uint64))
+
+
+def _navigate_priced(cc, target, bytes_of, effort=BALANCED, predicted=None, a=60.0, b=0.8):
+    cc.cc_navigate_priced.restype = ctypes.c_int
+    cc.cc_navigate_priced.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, PROBE_CB,
+                                      PRICE_CB, ctypes.POINTER(Outcome), ctypes.POINTER(ctypes.c_int32),
+                                      ctypes.POINTER(ctypes.c_uint64)]
+    priced = []
+
+    def probe(rung, fresh, out):
+        out[0] = 100.0 - a * (cc.cc_effective_scale(rung) / 1000.0) ** (-b)
+        return 0
+
+    def price(rung, fresh, out):
+        priced.append(rung)
+        out[0] = bytes_of(rung)
+        return 0
+
+    pcb, qcb = PROBE_CB(probe), PRICE_CB(price) if bytes_of else ctypes.cast(None, PRICE_CB)
+    o, n, nb = Outcome(), ctypes.c_int32(), ctypes.c_uint64()
+    pred = predicted if predicted is not None else cc.cc_rung_for_scale(8000.0)
+    assert cc.cc_navigate_priced(target, effort, 0, pred, pcb, qcb, ctypes.byref(o), ctypes.byref(n),
+                                 ctypes.byref(nb)) == 0
+    return o, n.value, nb.value, priced
+
+
+def test_priced_finalists_keep_the_smallest_stream(cc):
+    """quality.rs:1886-2010: the two coarsest feasible probes are priced
+    exactly and the smallest feasible stream wins (the first of equal sizes);
+    a single feasible finalist is taken without pricing."""
+    seen_two = 0
+    for target in np.linspace(60.0, 92.0, 33):
+        for pocket in (False, True):
+            def bytes_of(rung, pocket=pocket):  # size rising with the rung, with a pocket at odd rungs
+                return 1000 + 10 * rung - (25 if pocket and rung % 2 else 0)
+
+            o, n, nb, priced = _navigate_priced(cc, float(target), bytes_of)
+            feas = sorted(o.probe_rungs[i] for i in range(o.nprobes) if o.probe_scores[i] >= target)
+            fin = sorted(set(feas))[:2]
+            if len(fin) < 2:
+                assert n == 0 and not priced
+                continue
+            seen_two += 1
+            assert n == 2 and sorted(priced) == fin
+            sizes = [bytes_of(r) for r in fin]
+            assert o.rung == fin[sizes.index(min(sizes))] and nb == min(sizes)
+            # without a price the coarsest feasible rung is chosen
+            o2, n2, _, _ = _navigate_priced(cc, float(target), None)
+            assert n2 == 0 and o2.rung == fin[0]
+    assert seen_two > 0

@@ -152,3 +152,31 @@ def test_facade_returns_a_codestream(ctx):
     assert (d.info.width, d.info.height, d.info.global_scale) == (w, h, report.global_scale)
     fixed = jpxl_amd.Encoder(ctx).with_global_scale(32768).encode_rgb8(w, h, rgb)
     assert cs.decode(fixed).info.stage == 7
+
+
+def test_priced_search_chooses_the_smallest_stream(ctx):
+    """Mode 2 (the reference's default build, quality.rs:1886-2010): with two
+    feasible finalists both are priced by the codestream stage and the chosen
+    plan's stream has the priced size; with one, the choice is mode 0's."""
+    import jpxl_amd
+    from jpxl_amd import _lib
+
+    priced = 0
+    for k, (w, h, t) in enumerate([(320, 240, 85.0), (256, 256, 90.0), (400, 300, 70.0), (512, 384, 80.0),
+                                   (300, 200, 88.0), (384, 384, 75.0)]):
+        rgb = _rgb(w, h, 20 + k)
+        img = jpxl_amd.Image(ctx, rgb, 8)
+        res = img.encode_quality(t, "balanced", mode=_lib.MODE_PRICED)
+        stream, sizes = img.encode_emit()
+        assert cs.decode(stream).info.stage == 7
+        plain = jpxl_amd.Image(ctx, rgb, 8)
+        r0 = plain.encode_quality(t, "balanced")
+        if res.exact_prices:
+            priced += 1
+            assert res.exact_prices == 2 and res.bytes == len(stream)
+            s0, _ = plain.encode_emit()
+            assert len(stream) <= len(s0)  # never a larger stream than the coarsest finalist's
+        else:
+            assert res.bytes == 0 and res.rung == r0.rung
+        plain.close()
+        img.close()
