@@ -34,6 +34,8 @@
  *   jpxlg_emit_codestream     attach_entropy + emit_codestream_with_executor
  *                             (jpxl-encode-policy/src/lib.rs:1012,
  *                              jpxl-encode/src/vardct/write.rs:350)
+ *   jpxlg_encode_bytes        rate::search_frame (byte target, exhaustive ladder)
+ *                             (jpxl-encode-policy/src/rate.rs:1143, :387-921)
  *   jpxlg_encode_quality      search_frame_perceptual
  *                             (jpxl-encode-policy/src/quality.rs:2405)
  *   jpxlg_score_pair          jpxl_perceptual::score_pair
@@ -325,6 +327,33 @@ typedef struct {
 
 int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_request* req,
                              jpxlg_quality_result* out);
+/* ---- byte-rate controller ------------------------------------------------- */
+/* search_frame over a byte target (jpxl-encode-policy/src/rate.rs:1143) as
+ * the exhaustive ladder search does it (rate.rs:2016-2140 over search_ladder,
+ * :387-921): a cold bracket doubling the effective scale from the request's
+ * rung (global scale 32768), aimed bisection, a fill above the best; every
+ * candidate rung is planned at its own quantizer on the device (cover, CfL,
+ * quantize under EncodeRequest::for_target, request.rs:883-910) and priced
+ * exactly by the codestream stage.  The largest stream at or under the
+ * target wins and stays quantized on the image for jpxlg_encode_emit. */
+typedef struct {
+  uint64_t target_bytes;
+  int32_t effort;       /* JPXLG_EFFORT_*: the request's chroma policy */
+  uint32_t max_prices;  /* 0: RateSearchBudget::default (40), of which the ladder spends 24 */
+} jpxlg_rate_request;
+
+typedef struct {
+  uint32_t rung, global_scale, hf_mul, quant_lf;
+  uint64_t bytes;          /* the chosen plan's codestream size */
+  int32_t saturated;       /* the finest rung fits the target */
+  int32_t prices;          /* candidates priced */
+  uint32_t trace_rungs[64];
+  uint64_t trace_bytes[64];
+  uint64_t floor_bytes;    /* JPXLG_E_RANGE for a target below the coarsest rung: that rung's size */
+} jpxlg_rate_result;
+
+int32_t jpxlg_encode_bytes(jpxlg_img* img, const jpxlg_rate_request* req, jpxlg_rate_result* out);
+
 /* ---- first-guess features and the crossing predictor ---------------------- */
 /* SourceFeatures (jpxl-encode-policy/src/quality_features.rs:21-47) with the
  * PreanalysisFeatures it carries (preanalysis.rs:99-137): the atlas's

@@ -16,6 +16,7 @@ Every call runs on the GPU through ``libjpxlg.so``; there is no CPU fallback.
 from __future__ import annotations
 
 import ctypes
+import math
 import weakref
 from dataclasses import dataclass, field
 
@@ -398,12 +399,25 @@ class Image:
         self.ctx.check(self.ctx.lib.jpxlg_image_download(self.handle, which, _ptr(out), out.nbytes))
         return out
 
+    def encode_bytes(self, target_bytes: int, effort: str = "balanced", max_prices: int = 0):
+        """rate::search_frame over a byte target: the RateResult; the chosen
+        plan stays quantized (jpxlg_encode_emit writes it)."""
+        return _encode_bytes(self, target_bytes, effort, max_prices)
+
     def encode_quality(self, target: float, effort: str = "balanced", max_probes: int = 0, mode: int = 0):
         req = _lib.QualityRequest(target, _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED,
                                   max_probes, mode)
         res = _lib.QualityResult()
         self.ctx.check(self.ctx.lib.jpxlg_encode_quality(self.handle, ctypes.byref(req), ctypes.byref(res)))
         return res
+
+
+def _encode_bytes(img, target_bytes: int, effort: str = "balanced", max_prices: int = 0):
+    req = _lib.RateRequest(int(target_bytes), _lib.EFFORT_FAST if effort == "fast" else _lib.EFFORT_BALANCED,
+                           max_prices)
+    res = _lib.RateResult()
+    img.ctx.check(img.ctx.lib.jpxlg_encode_bytes(img.handle, ctypes.byref(req), ctypes.byref(res)))
+    return res
 
 
 def predict_crossing(source, transform, target: float, reserve: float, which: int = 0):
@@ -614,11 +628,28 @@ class Encoder:
         self.global_scale: int | None = None
         self.effort = "balanced"
         self.threads = 0
+        self.target_bytes: int | None = None
+        self.target_bpp: float | None = None
+
+    def with_target_bytes(self, target_bytes: int) -> "Encoder":
+        """Lossy VarDCT to an exact byte ceiling (jpxl/src/lib.rs:546)."""
+        if int(target_bytes) <= 0:
+            raise ValueError("target byte count must be greater than zero")
+        self.target_bytes, self.target_bpp, self.quality, self.global_scale = int(target_bytes), None, None, None
+        return self
+
+    def with_target_bpp(self, bits_per_pixel: float) -> "Encoder":
+        """Lossy VarDCT to a bits-per-pixel ceiling (jpxl/src/lib.rs:534)."""
+        if not (bits_per_pixel > 0.0 and bits_per_pixel < float("inf")):
+            raise ValueError("target bits per pixel must be finite and greater than zero")
+        self.target_bpp, self.target_bytes, self.quality, self.global_scale = float(bits_per_pixel), None, None, None
+        return self
 
     def with_quality(self, quality: float) -> "Encoder":
         if not (0.0 <= quality <= 100.0):
             raise ValueError("quality must be within 0..=100")
         self.quality, self.global_scale = float(quality), None
+        self.target_bytes = self.target_bpp = None
         return self
 
     with_ssimulacra2_score = with_quality
@@ -627,6 +658,7 @@ class Encoder:
         if not (1 <= global_scale <= 73728):
             raise ValueError("global_scale outside 1..=73728")
         self.global_scale, self.quality = int(global_scale), None
+        self.target_bytes = self.target_bpp = None
         return self
 
     def with_effort(self, effort: str) -> "Encoder":
@@ -642,7 +674,14 @@ class Encoder:
     def _encode(self, rgb: np.ndarray, bits: int, emit: bool = True):
         img = Image(self.ctx, rgb, bits)
         try:
-            if self.quality is not None:
+            if self.target_bytes is not None or self.target_bpp is not None:
+                h, w = rgb.shape[:2]
+                # RateTarget::bytes_for (request.rs:46-70)
+                target = self.target_bytes if self.target_bytes is not None else \
+                    int(math.floor(self.target_bpp * w * h / 8.0))
+                res = img.encode_bytes(target, self.effort)
+                report = EncodeReport(float("nan"), float("nan"), True, res.global_scale, res.hf_mul, res.quant_lf)
+            elif self.quality is not None:
                 # the reference's default build: finalists priced exactly
                 res = img.encode_quality(self.quality, self.effort, mode=_lib.MODE_PRICED)
                 report = EncodeReport(self.quality, res.score, bool(res.met), res.global_scale, res.hf_mul,

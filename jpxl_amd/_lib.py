@@ -149,6 +149,21 @@ FALLBACK_NONE, FALLBACK_OOD_FEATURE, FALLBACK_WIDE_INTERVAL, FALLBACK_SATURATION
 PREDICTOR_DISPATCH, PREDICTOR_CASES, PREDICTOR_KNOTS = 0, 1, 2
 
 
+class RateRequest(ctypes.Structure):
+    """jpxlg_rate_request."""
+
+    _fields_ = [("target_bytes", ctypes.c_uint64), ("effort", ctypes.c_int32), ("max_prices", ctypes.c_uint32)]
+
+
+class RateResult(ctypes.Structure):
+    """jpxlg_rate_result."""
+
+    _fields_ = [("rung", ctypes.c_uint32), ("global_scale", ctypes.c_uint32), ("hf_mul", ctypes.c_uint32),
+                ("quant_lf", ctypes.c_uint32), ("bytes", ctypes.c_uint64), ("saturated", ctypes.c_int32),
+                ("prices", ctypes.c_int32), ("trace_rungs", ctypes.c_uint32 * 64),
+                ("trace_bytes", ctypes.c_uint64 * 64), ("floor_bytes", ctypes.c_uint64)]
+
+
 class QualityResult(ctypes.Structure):
     """jpxlg_quality_result."""
 
@@ -201,6 +216,7 @@ SIGNATURES = {
     "jpxlg_plan_download_ir": (_i32, [_vp, _vp, _vp, _vp, _u32p]),
     "jpxlg_emit_codestream": (_i32, [ctypes.POINTER(FrameDesc), _vp, _vp, _vp, _u32, _vp, _vp, _i32, _vp,
                                      ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(EmitSizes)]),
+    "jpxlg_encode_bytes": (_i32, [_vp, ctypes.POINTER(RateRequest), ctypes.POINTER(RateResult)]),
     "jpxlg_plan_frame_desc": (_i32, [_vp, ctypes.POINTER(Restoration), ctypes.POINTER(FrameDesc)]),
     "jpxlg_encode_emit": (_i32, [_vp, ctypes.POINTER(Restoration), _i32, _vp, ctypes.c_size_t,
                                  ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(EmitSizes)]),

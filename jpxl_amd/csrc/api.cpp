@@ -2055,6 +2055,73 @@ extern "C" int32_t jpxlg_encode_quality(jpxlg_img* img, const jpxlg_quality_requ
   return JPXLG_OK;
 }
 
+// The byte-rate search (rate.rs:2016-2140, search_ladder :387-921): each
+// candidate planned fresh at its rung and priced by the codestream stage.
+extern "C" int32_t jpxlg_encode_bytes(jpxlg_img* img, const jpxlg_rate_request* req, jpxlg_rate_result* out) {
+  if (!img || !req || !out) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  memset(out, 0, sizeof(*out));
+  if (req->target_bytes == 0) return fail(ctx, JPXLG_E_RANGE, "a zero byte target");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  // EncodeRequest::for_target (request.rs:883-910): quant_lf 4, AQ off, EPF 1
+  // at uniform sharpness 7, trailing truncation at lambda x4, the calibrated
+  // cover with quantizer-donor weights and CfL; chroma per the effort.
+  QualityPolicy pol = quality_policy(req->effort);
+  pol.cover_mode = JPXLG_COVER_HIERARCHICAL;
+  pol.enable_cfl = 1;
+  int rc = jpxlg_set_aq(img, JPXLG_AQ_OFF, nullptr);
+  if (rc) return rc;
+  const jpxlg_restoration r{pol.epf_iters, 0, pol.sharpness};
+  // RateSearchBudget::default: 40 prices, 4 fill probes; the exhaustive
+  // search spends max_prices - full_refinement_reserve on the ladder
+  // (rate.rs:146-160, :2050-2056).
+  const uint32_t max_prices = req->max_prices ? req->max_prices : 40;
+  const uint32_t reserve = max_prices <= 6 ? 1 : std::min(std::max(std::min(16u, max_prices / 2), 8u), max_prices - 1);
+  const uint32_t fast_cap = std::max(1u, max_prices - reserve);
+  auto plan_at = [&](uint32_t rung) -> int {
+    const jpxlg_quantizer q = pol.quantizer_at(rung);
+    int prc = jpxlg_plan_structure(img, &q, pol.cover_mode, pol.enable_cfl);
+    if (prc) return prc;
+    return jpxlg_quantize(img, &q);
+  };
+  uint32_t last_planned = UINT32_MAX;
+  RatePriceFn price = [&](uint32_t rung, uint64_t* bytes) -> int {
+    int prc = plan_at(rung);
+    if (prc) return prc;
+    last_planned = rung;
+    size_t n = 0;
+    prc = jpxlg_encode_emit(img, &r, 0, nullptr, 0, &n, nullptr);
+    *bytes = n;
+    return prc;
+  };
+  RateOutcome oc;
+  rc = rate_search_ladder(Rung::for_global_scale(32768), req->target_bytes, 8, 0.01, fast_cap, 4, price, &oc);
+  out->prices = (int32_t)oc.trace.size();
+  for (size_t i = 0; i < oc.trace.size() && i < 64; i++) {
+    out->trace_rungs[i] = oc.trace[i].rung;
+    out->trace_bytes[i] = oc.trace[i].bytes;
+  }
+  if (rc == kRateUnreachable) {
+    out->floor_bytes = oc.floor_bytes;
+    return fail(ctx, JPXLG_E_RANGE, "the byte target is below the coarsest rung's size");
+  }
+  if (rc == kRateBudgetExhausted) return fail(ctx, JPXLG_E_RANGE, "no rung fits the target within the price budget");
+  if (rc) return rc;
+  if (last_planned != oc.rung) {
+    rc = plan_at(oc.rung);
+    if (rc) return rc;
+  }
+  img->last_r = r;
+  const jpxlg_quantizer qc = pol.quantizer_at(oc.rung);
+  out->rung = oc.rung;
+  out->global_scale = qc.global_scale;
+  out->hf_mul = qc.hf_mul;
+  out->quant_lf = qc.quant_lf;
+  out->bytes = oc.bytes;
+  out->saturated = oc.saturated ? 1 : 0;
+  return JPXLG_OK;
+}
+
 extern "C" int32_t jpxlg_rung_quantizer(uint32_t rung, int32_t effort, jpxlg_quantizer* out) {
   if (!out || rung > Rung::top()) return JPXLG_E_INVALID;
   *out = quality_policy(effort).quantizer_at(rung);

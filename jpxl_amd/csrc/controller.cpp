@@ -5,6 +5,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <cmath>
 
 namespace jpxlg {
 
@@ -605,6 +606,186 @@ int bisect_controller(double target, int max_probes, const ProbeFn& probe, Contr
   out->score = best->score;
   out->met = best->feasible ? 1 : 0;
   out->status = best->feasible ? kStatusMet : kStatusUnderTargetWorkCap;
+  return 0;
+}
+
+
+// ----------------------------------------------------------------------------
+// The exact rate loop's ladder search (jpxl-encode-policy/src/rate.rs:387-921):
+// a cold bracket doubling the effective scale, aimed bisection alternating
+// with midpoints, and a fill of the rungs just above the best feasible one.
+// ----------------------------------------------------------------------------
+namespace {
+
+// RateTolerance::bytes_for (request.rs:205-240): max(bytes, floor(fraction x target)).
+uint64_t rate_slack(uint64_t target, uint64_t bytes, double fraction) {
+  const double scaled = std::floor(std::max(fraction, 0.0) * (double)target);
+  return std::max<uint64_t>(bytes, std::isfinite(scaled) ? (uint64_t)scaled : 0);
+}
+
+double ln_scale_of(uint32_t rung) { return std::log((double)Rung::effective_scale(rung)); }
+
+uint32_t rung_for_scale_f(double scale) {  // navigation.rs:65-70
+  if (!std::isfinite(scale)) return 0;
+  return Rung::for_effective_scale((uint64_t)std::min(std::max(std::round(scale), 1.0), 1.8e19));
+}
+
+}  // namespace
+
+uint32_t rate_cold_step(uint32_t current, bool up) {  // rate.rs:490-502
+  const uint64_t scale = Rung::effective_scale(current);
+  const uint64_t aimed = std::max<uint64_t>(up ? scale * 2 : scale / 2, 1);
+  const uint32_t aim = Rung::for_effective_scale(aimed);
+  // force_progress(.., AtLeastOneRung) (navigation.rs:162-170)
+  if (up) return std::max(aim, std::min(current + 1, Rung::top()));
+  return std::min(aim, current == 0 ? 0u : current - 1);
+}
+
+bool rate_interpolated_rung(uint32_t lo_rung, uint64_t lo_bytes, uint32_t hi_rung, uint64_t hi_bytes,
+                            uint64_t target, uint32_t* out) {  // rate.rs:522-557
+  if (hi_rung <= lo_rung + 1 || lo_bytes == 0 || target == 0) return false;
+  if (hi_bytes <= lo_bytes) return false;
+  const double lo_x = ln_scale_of(lo_rung), hi_x = ln_scale_of(hi_rung);
+  const double lo_y = std::log((double)lo_bytes), hi_y = std::log((double)hi_bytes), t_y = std::log((double)target);
+  const double span = hi_y - lo_y;  // bounded_fraction_crossing (navigation.rs:131-138)
+  if (!std::isfinite(span) || span <= 0.0) return false;
+  const double fraction = std::min(std::max((t_y - lo_y) / span, 0.0), 1.0);
+  const double guess_x = lo_x + fraction * (hi_x - lo_x);
+  if (!std::isfinite(guess_x)) return false;
+  const double guess_scale = std::exp(guess_x);
+  if (!std::isfinite(guess_scale) || guess_scale < 0.0) return false;
+  const uint32_t guess = rung_for_scale_f(guess_scale);
+  const uint32_t low = lo_rung + 1;
+  if (hi_rung == 0) return false;
+  const uint32_t high = hi_rung - 1;
+  if (low > high) return false;
+  *out = std::min(std::max(guess, low), high);
+  return true;
+}
+
+int rate_search_ladder(uint32_t start, uint64_t target, uint64_t tol_bytes, double tol_fraction, uint32_t max_prices,
+                       uint32_t fill_probes, const RatePriceFn& price, RateOutcome* out) {
+  out->trace.clear();
+  out->saturated = false;
+  out->floor_bytes = 0;
+  max_prices = std::max(max_prices, 1u);
+  const uint32_t top = Rung::top();
+  start = std::min(start, top);
+  const uint64_t slack = rate_slack(target, tol_bytes, tol_fraction);
+  bool have_best = false;
+  uint32_t best_rung = 0;
+  uint64_t best_bytes = 0;
+  auto affordable = [&]() { return out->trace.size() < max_prices; };
+  // Search::eval (rate.rs:660-680): a rung priced once; the largest feasible
+  // size is the best (the finer rung among equal sizes).
+  auto eval = [&](uint32_t rung, int phase, uint64_t* bytes) -> int {
+    for (auto& s : out->trace)
+      if (s.rung == rung) {
+        *bytes = s.bytes;
+        return 0;
+      }
+    uint64_t b = 0;
+    const int rc = price(rung, &b);
+    if (rc) return rc;
+    const bool feasible = b <= target;
+    out->trace.push_back({rung, b, feasible, phase});
+    if (feasible && (!have_best || b > best_bytes || (b == best_bytes && rung > best_rung))) {
+      have_best = true;
+      best_rung = rung;
+      best_bytes = b;
+    }
+    *bytes = b;
+    return 0;
+  };
+  int rc;
+  bool have_lo = false, have_hi = false, floor_hit = false;
+  uint32_t lo = 0, hi = 0;
+  uint64_t lo_b = 0, hi_b = 0, first = 0;
+  if ((rc = eval(start, kRateBracket, &first))) return rc;
+  if (first <= target) {
+    have_lo = true, lo = start, lo_b = first;
+    uint32_t current = start;
+    while (current < top && affordable()) {
+      const uint32_t next = rate_cold_step(current, true);
+      if (next <= current) break;
+      uint64_t b;
+      if ((rc = eval(next, kRateBracket, &b))) return rc;
+      if (b > target) {
+        have_hi = true, hi = next, hi_b = b;
+        break;
+      }
+      lo = next, lo_b = b;
+      current = next;
+    }
+  } else {
+    have_hi = true, hi = start, hi_b = first;
+    uint32_t current = start;
+    for (;;) {
+      if (current == 0) {
+        floor_hit = true;
+        out->floor_bytes = hi_b;
+        break;
+      }
+      if (!affordable()) break;
+      const uint32_t next = rate_cold_step(current, false);
+      uint64_t b;
+      if ((rc = eval(next, kRateBracket, &b))) return rc;
+      if (b <= target) {
+        have_lo = true, lo = next, lo_b = b;
+        break;
+      }
+      hi = next, hi_b = b;
+      current = next;
+    }
+  }
+  if (!have_lo) return floor_hit ? kRateUnreachable : kRateBudgetExhausted;
+  // Aimed bisection (rate.rs:760-800): an aimed probe may raise lo but never
+  // becomes hi (a spike above the crossing must not fence off finer rungs).
+  bool take_aimed = true, aiming_pays = true;
+  while (have_hi) {
+    if (hi <= lo + 1 || !affordable()) break;
+    const uint64_t bb = have_best ? best_bytes : 0;
+    if (hi_b - std::min(hi_b, bb) <= slack) break;
+    if (target - std::min(target, bb) <= slack) break;
+    const uint32_t mid = lo + (hi - lo) / 2;
+    uint32_t probe = mid;
+    bool aimed = false;
+    if (take_aimed && aiming_pays) {
+      uint32_t guess;
+      if (rate_interpolated_rung(lo, lo_b, hi, hi_b, target, &guess) && guess != mid) {
+        probe = guess;
+        aimed = true;
+      }
+    }
+    take_aimed = !take_aimed;
+    uint64_t b;
+    if ((rc = eval(probe, kRateBisect, &b))) return rc;
+    if (b <= target) {
+      lo = probe, lo_b = b;
+    } else if (aimed) {
+      aiming_pays = false;
+    } else {
+      hi = probe, hi_b = b;
+    }
+  }
+  // Fill (rate.rs:802-820): the rungs just above the best, while undershoot remains.
+  if (have_best && target - best_bytes > slack) {
+    const uint32_t base = best_rung;
+    for (uint32_t off = 1; off <= fill_probes; off++) {
+      if (!affordable()) break;
+      const uint32_t cand = std::min(base + off, top);
+      if (cand <= base) break;
+      bool seen = false;
+      for (auto& s : out->trace) seen |= s.rung == cand;
+      if (seen) continue;
+      uint64_t b;
+      if ((rc = eval(cand, kRateFill, &b))) return rc;
+    }
+  }
+  if (!have_best) return kRateBudgetExhausted;
+  out->rung = best_rung;
+  out->bytes = best_bytes;
+  for (auto& s : out->trace) out->saturated |= s.rung == top && s.feasible;
   return 0;
 }
 

@@ -142,4 +142,34 @@ uint32_t geometric_step(uint32_t from, int finer);
 double aim_score_for(double threshold, double reserve);
 double met_band(double threshold, double reserve);
 
+// The exact rate loop's ladder search (search_ladder, rate.rs:387-921) with a
+// cold bracket: price(rung, &bytes) -> status.  Returns 0 with the largest
+// feasible size found (the finer rung of equal sizes), kRateUnreachable
+// when even the coarsest rung is over the target (floor_bytes = its size),
+// kRateBudgetExhausted when no feasible rung was priced within max_prices,
+// or the price function's status.
+enum { kRateBracket = 0, kRateBisect = 1, kRateFill = 2 };
+constexpr int kRateUnreachable = 1;
+constexpr int kRateBudgetExhausted = 2;
+struct RateStep {
+  uint32_t rung;
+  uint64_t bytes;
+  bool feasible;
+  int phase;
+};
+struct RateOutcome {
+  uint32_t rung = 0;
+  uint64_t bytes = 0;
+  bool saturated = false;  // the ladder's top was priced feasible
+  uint64_t floor_bytes = 0;
+  std::vector<RateStep> trace;
+};
+using RatePriceFn = std::function<int(uint32_t, uint64_t*)>;
+// tol_bytes, tol_fraction: RateTolerance (the default is 8 bytes, 1 %).
+int rate_search_ladder(uint32_t start, uint64_t target, uint64_t tol_bytes, double tol_fraction, uint32_t max_prices,
+                       uint32_t fill_probes, const RatePriceFn& price, RateOutcome* out);
+uint32_t rate_cold_step(uint32_t current, bool up);
+bool rate_interpolated_rung(uint32_t lo_rung, uint64_t lo_bytes, uint32_t hi_rung, uint64_t hi_bytes,
+                            uint64_t target, uint32_t* out);
+
 }  // namespace jpxlg

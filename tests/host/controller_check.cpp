@@ -198,4 +198,37 @@ int cc_navigate_priced(double target, int effort, int max_probes, uint32_t predi
   return 0;
 }
 
+// The rate loop's ladder search over a caller's price: price(rung, &bytes).
+typedef int (*cc_rate_price_fn)(uint32_t rung, uint64_t* bytes);
+int cc_rate_search(uint32_t start, uint64_t target, uint64_t tol_bytes, double tol_fraction, uint32_t max_prices,
+                   uint32_t fill_probes, cc_rate_price_fn fn, uint32_t* rung, uint64_t* bytes, int32_t* saturated,
+                   uint64_t* floor_bytes, uint32_t* trace_rungs, uint64_t* trace_bytes, int32_t* ntrace) {
+  RateOutcome oc;
+  RatePriceFn price = [&](uint32_t r, uint64_t* b) -> int { return fn(r, b); };
+  const int rc = rate_search_ladder(start, target, tol_bytes, tol_fraction, max_prices, fill_probes, price, &oc);
+  *rung = oc.rung;
+  *bytes = oc.bytes;
+  *saturated = oc.saturated ? 1 : 0;
+  *floor_bytes = oc.floor_bytes;
+  *ntrace = (int32_t)oc.trace.size();
+  for (size_t i = 0; i < oc.trace.size() && i < 256; i++) {
+    trace_rungs[i] = oc.trace[i].rung;
+    trace_bytes[i] = oc.trace[i].bytes;
+  }
+  return rc;
+}
+
+uint32_t cc_rate_cold_step(uint32_t rung, int up) { return rate_cold_step(rung, up != 0); }
+
+int cc_rate_interpolated(uint32_t lo, uint64_t lo_b, uint32_t hi, uint64_t hi_b, uint64_t target, uint32_t* out) {
+  return rate_interpolated_rung(lo, lo_b, hi, hi_b, target, out) ? 1 : 0;
+}
+
+// Every rung's (global_scale, hf_mul) (quantizer_ladder.rs QuantizerChoice::at).
+uint32_t cc_ladder_fields(uint32_t* gs, uint32_t* mul) {
+  const uint32_t n = Rung::ladder_len();
+  for (uint32_t r = 0; r < n; r++) Rung::fields(r, &gs[r], &mul[r]);
+  return n;
+}
+
 }  // extern "C"

@@ -180,3 +180,45 @@ def test_priced_search_chooses_the_smallest_stream(ctx):
             assert res.bytes == 0 and res.rung == r0.rung
         plain.close()
         img.close()
+
+
+def test_byte_target_search_lands_under_the_target(ctx):
+    """The byte-rate search (rate.rs:2016-2140 over search_ladder): every
+    candidate priced exactly; the chosen plan's stream is the largest priced
+    size at or under the target and is what jpxlg_encode_emit writes."""
+    import jpxl_amd
+
+    w, h = 480, 320
+    rgb = _rgb(w, h, 11)
+    img = jpxl_amd.Image(ctx, rgb, 8)
+    for target in (6000, 20000, 45000):
+        res = img.encode_bytes(target)
+        stream, _ = img.encode_emit()
+        trace = [(res.trace_rungs[i], res.trace_bytes[i]) for i in range(min(res.prices, 64))]
+        assert 0 < len(stream) <= target and len(stream) == res.bytes
+        assert res.bytes == max(b for _, b in trace if b <= target)
+        assert res.prices <= 24
+        d = cs.decode(stream)
+        assert (d.info.global_scale, d.info.quant_lf) == (res.global_scale, res.quant_lf)
+    img.close()
+
+
+def test_byte_target_below_the_floor_is_refused(ctx):
+    import jpxl_amd
+    from jpxl_amd import JpxlgError
+
+    img = jpxl_amd.Image(ctx, _rgb(200, 150, 12), 8)
+    with pytest.raises(JpxlgError):
+        img.encode_bytes(40)
+    img.close()
+
+
+def test_facade_target_bytes_and_bpp(ctx):
+    import jpxl_amd
+
+    w, h = 320, 240
+    rgb = _rgb(w, h, 13)
+    s1 = jpxl_amd.Encoder(ctx).with_target_bytes(12000).encode_rgb8(w, h, rgb)
+    assert len(s1) <= 12000 and cs.decode(s1).info.stage == 7
+    s2 = jpxl_amd.Encoder(ctx).with_target_bpp(1.0).encode_rgb8(w, h, rgb)
+    assert len(s2) <= w * h // 8 and cs.decode(s2).info.stage == 7
