@@ -30,7 +30,10 @@ class Info(ctypes.Structure):
         ("flags", ctypes.c_uint64), ("global_scale", ctypes.c_uint32), ("quant_lf", ctypes.c_uint32),
         ("x_factor_lf", ctypes.c_int32), ("b_factor_lf", ctypes.c_int32), ("lf_corr_default", ctypes.c_uint32),
         ("num_sections", ctypes.c_uint32), ("num_groups", ctypes.c_uint32), ("num_lf_groups", ctypes.c_uint32),
-        ("header_bytes", ctypes.c_uint64), ("toc_sum", ctypes.c_uint64), ("stage", ctypes.c_uint32)]
+        ("header_bytes", ctypes.c_uint64), ("toc_sum", ctypes.c_uint64), ("stage", ctypes.c_uint32),
+        ("colour_space", ctypes.c_uint32), ("colour_factor", ctypes.c_uint32), ("base_corr_x", ctypes.c_float),
+        ("base_corr_b", ctypes.c_float), ("global_tree", ctypes.c_uint32), ("used_orders", ctypes.c_uint32),
+        ("extra_precision", ctypes.c_uint32), ("dct_select_histogram", ctypes.c_uint32 * 27)]
 
 
 _CHECK = None
