@@ -496,16 +496,25 @@ void cluster_contexts(const std::vector<uint32_t>& hist, int num_ctx, int max_cl
       cost_table(cent.back().data(), L.back().data());
     };
     add_seed(first);
+    // Seeds are drawn from the heaviest contexts only (their coding-cost
+    // gaps are the ones kept up to date); every used context is assigned.
+    std::vector<int> cand(used);
+    if (cand.size() > 384) {
+      std::nth_element(cand.begin(), cand.begin() + 384, cand.end(),
+                       [&](int a, int b) { return tot[a] != tot[b] ? tot[a] > tot[b] : a < b; });
+      cand.resize(384);
+      std::sort(cand.begin(), cand.end());
+    }
     std::vector<double> gap(num_ctx, 0);
-    for (int c : used) gap[c] = sp[c].cost(L[0].data()) - self[c];
+    for (int c : cand) gap[c] = sp[c].cost(L[0].data()) - self[c];
     while ((int)cent.size() < max_clusters) {
       int best = -1;
-      for (int c : used)
+      for (int c : cand)
         if (best < 0 || gap[c] > gap[best]) best = c;
       if (gap[best] < min_gain) break;
       add_seed(best);
       const double* Lj = L.back().data();
-      for (int c : used) gap[c] = std::min(gap[c], sp[c].cost(Lj) - self[c]);
+      for (int c : cand) gap[c] = std::min(gap[c], sp[c].cost(Lj) - self[c]);
     }
     const int nu = (int)used.size(), chunks = pool && nu > 256 ? 32 : 1;
     for (int round = 0; round < 2; round++) {
@@ -1060,11 +1069,21 @@ void write_hf_metadata(BitWriter& w, const Frame& fr, uint32_t id) {
   write_modular(w, ch);
 }
 
-// Per-worker HF token counts: tokens below kHfTok in a dense table, the rare
-// larger ones (coefficients of magnitude 512 and above) listed apart.
-constexpr int kHfTok = 40;
+// Per-worker HF token counts in a dense table sized for the common tokens:
+// the non-zero-count contexts (the first kNonZeroBuckets * kNbBlockCtx) keep
+// kNzTok tokens each, the coefficient contexts kCoefTok; rarer, larger
+// tokens are listed apart.  Small tables keep each worker's counts in its
+// own cache and the merge short.
+constexpr int kNzContexts = kNonZeroBuckets * kNbBlockCtx;
+constexpr int kNzTok = 48, kCoefTok = 24;
+constexpr size_t kDenseCounts = (size_t)kNzContexts * kNzTok + (size_t)(kHfContexts - kNzContexts) * kCoefTok;
+inline size_t dense_slot(uint32_t ctx) {
+  return ctx < (uint32_t)kNzContexts ? (size_t)ctx * kNzTok
+                                     : (size_t)kNzContexts * kNzTok + (size_t)(ctx - kNzContexts) * kCoefTok;
+}
+inline uint32_t dense_width(uint32_t ctx) { return ctx < (uint32_t)kNzContexts ? kNzTok : kCoefTok; }
 struct HfCounts {
-  std::vector<uint32_t> dense;   // kHfContexts x kHfTok
+  std::vector<uint32_t> dense;   // kDenseCounts
   std::vector<uint32_t> escape;  // ctx << 8 | token
 };
 
@@ -1096,7 +1115,7 @@ void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
     } else {
       hybrid_token(Hybrid{4, 2, 0}, v, &tok, &nb, &bits);
     }
-    if (tok < (uint32_t)kHfTok) dense[(size_t)ctx * kHfTok + tok]++;
+    if (tok < dense_width(ctx)) dense[dense_slot(ctx) + tok]++;
     else hc->escape.push_back(ctx << 8 | tok);
     *out++ = tape_entry(ctx, tok, nb, bits);
   };
@@ -1231,7 +1250,7 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
   std::vector<BitWriter> hf_meta(nlf);
   std::vector<std::vector<PreparedChannel>> lf_prep(nlf, std::vector<PreparedChannel>(3));
   pool->run(nw, [&](int i, int) {
-    counts[i].dense.assign((size_t)kHfContexts * kHfTok, 0);
+    counts[i].dense.assign(kDenseCounts, 0);
     counts[i].escape.clear();
   });
   pool->run((int)(4 * nlf + ng), [&](int t, int worker) {
@@ -1250,8 +1269,9 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
       uint32_t* d = &hist[(size_t)ctx * kAlphabet];
       std::fill(d, d + kAlphabet, 0u);
       for (int k = 0; k < nw; k++) {
-        const uint32_t* c = &counts[k].dense[(size_t)ctx * kHfTok];
-        for (int t = 0; t < kHfTok; t++) d[t] += c[t];
+        const uint32_t* c = &counts[k].dense[dense_slot(ctx)];
+        const uint32_t wdt = dense_width(ctx);
+        for (uint32_t t = 0; t < wdt; t++) d[t] += c[t];
       }
     }
   });
