@@ -33,7 +33,18 @@ class Info(ctypes.Structure):
         ("header_bytes", ctypes.c_uint64), ("toc_sum", ctypes.c_uint64), ("stage", ctypes.c_uint32),
         ("colour_space", ctypes.c_uint32), ("colour_factor", ctypes.c_uint32), ("base_corr_x", ctypes.c_float),
         ("base_corr_b", ctypes.c_float), ("global_tree", ctypes.c_uint32), ("used_orders", ctypes.c_uint32),
-        ("extra_precision", ctypes.c_uint32), ("dct_select_histogram", ctypes.c_uint32 * 27)]
+        ("extra_precision", ctypes.c_uint32), ("dct_select_histogram", ctypes.c_uint32 * 27),
+        ("m_lf", ctypes.c_float * 3), ("intensity_target", ctypes.c_float)]
+
+
+class Varblock(ctypes.Structure):
+    _fields_ = [("dct_select", ctypes.c_uint32), ("bx", ctypes.c_uint32), ("by", ctypes.c_uint32),
+                ("hf_mul", ctypes.c_uint32), ("start", ctypes.c_uint64)]
+
+
+class Full(ctypes.Structure):
+    _fields_ = [("coeffs", ctypes.c_void_p), ("varblocks", ctypes.POINTER(Varblock)), ("capacity", ctypes.c_uint32),
+                ("count", ctypes.c_uint32), ("extra_precision", ctypes.c_void_p)]
 
 
 _CHECK = None
@@ -53,6 +64,9 @@ def checker():
         lib.cs_hybrid.argtypes = [ctypes.c_uint32] * 4 + [ctypes.c_char_p, ctypes.c_size_t,
                                                            ctypes.POINTER(ctypes.c_uint32), u64p]
         lib.cs_natural_order.argtypes = [ctypes.c_uint32, ctypes.c_void_p]
+        lib.cs_decode_full.restype = ctypes.c_int
+        lib.cs_decode_full.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(Info)] + [
+            ctypes.c_void_p] * 5 + [ctypes.POINTER(Full), ctypes.c_char_p, ctypes.c_int]
         _CHECK = lib
     return _CHECK
 
@@ -91,6 +105,46 @@ def decode(stream: bytes) -> Decoded:
     if rc != 0:
         raise ValueError(f"stage {info.stage}: {err.value.decode()}")
     return Decoded(info, quant, lf, bmap, mul, cx, cb, sh)
+
+
+class FullFrame:
+    """A frame's varblocks with their full quantized coefficients."""
+
+    def __init__(self, info, lf, bmap, cfl_x, cfl_b, sharp, varblocks, coeffs, extra_precision):
+        self.info, self.lf, self.block_map = info, lf, bmap
+        self.cfl_x, self.cfl_b, self.sharpness = cfl_x, cfl_b, sharp
+        self.varblocks, self.coeffs, self.extra_precision = varblocks, coeffs, extra_precision
+
+
+def decode_full(stream: bytes) -> FullFrame:
+    """Reads a codestream keeping every varblock's coefficient arrays (any
+    DctSelect type): varblocks is a list of (dct_select, bx, by, hf_mul,
+    start); coeffs[start + c * n : start + (c + 1) * n] is channel c's
+    landscape coefficient array (X, Y, B) of n = 64 x blocks cells."""
+    lib = checker()
+    info = Info()
+    err = ctypes.create_string_buffer(256)
+    lib.cs_decode(stream, len(stream), ctypes.byref(info), None, None, None, None, None, None, None, err, 256)
+    w, h = info.width, info.height
+    if not w:
+        raise ValueError(err.value.decode())
+    bw, bh = (w + 7) // 8, (h + 7) // 8
+    tw, th = (bw + 7) // 8, (bh + 7) // 8
+    lf = np.zeros((3, bh, bw), np.int32)
+    bmap = np.zeros((bh, bw), np.uint8)
+    cx = np.zeros((th, tw), np.int8)
+    cb = np.zeros((th, tw), np.int8)
+    sh = np.zeros((bh, bw), np.uint8)
+    coeffs = np.zeros(3 * 64 * bw * bh, np.int32)
+    table = (Varblock * (bw * bh))()
+    ep = np.zeros(max(1, ((w + 2047) // 2048) * ((h + 2047) // 2048)), np.uint8)
+    full = Full(coeffs.ctypes.data, table, bw * bh, 0, ep.ctypes.data)
+    rc = lib.cs_decode_full(stream, len(stream), ctypes.byref(info), _p(lf), _p(bmap), _p(cx), _p(cb), _p(sh),
+                            ctypes.byref(full), err, 256)
+    if rc != 0:
+        raise ValueError(f"stage {info.stage}: {err.value.decode()}")
+    vbs = [(t.dct_select, t.bx, t.by, t.hf_mul, t.start) for t in table[:full.count]]
+    return FullFrame(info, lf, bmap, cx, cb, sh, vbs, coeffs, ep)
 
 
 def prefix_decode(lengths, data: bytes, count: int):

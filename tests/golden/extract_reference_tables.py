@@ -55,6 +55,17 @@ def main(root: str) -> None:
         "16": _rows(_block(dq, r"pub const DCT16X16_PARAMS: \[\[f64; 7\]; 3\]")),
         "32": _rows(_block(dq, r"pub const DCT32X32_PARAMS: \[\[f64; 8\]; 3\]")),
     }
+    # Table I.6's parameter sets of the other transform types (the decoder's
+    # side: rectangles, DCT2x2 and the 64-point DCTs of indices 11 and 12)
+    out["dequant_other_params"] = {
+        "file": "jpxl-core/src/dequant.rs",
+        "16x8": _rows(_block(dq, r"pub const DCT16X8_PARAMS: \[\[f64; 7\]; 3\]")),
+        "32x8": _rows(_block(dq, r"pub const DCT32X8_PARAMS: \[\[f64; 8\]; 3\]")),
+        "16x32": _rows(_block(dq, r"pub const DCT16X32_PARAMS: \[\[f64; 8\]; 3\]")),
+        "dct2": _rows(_block(dq, r"pub const DCT2_PARAMS: \[\[f64; 6\]; 3\]")),
+        "large_bases": _rows(_block(dq, r"pub const LARGE_DCT_BASES: \[\[f64; 3\]; 6\]")),
+        "seq": [_numbers(_block(dq, r"pub const SEQ_%s: \[f64; 7\]" % k)) for k in "ABC"],
+    }
 
     s2 = rd("jpxl-perceptual/src/ssimulacra2.rs")
     weights = _numbers(_block(s2, r"const WEIGHTS: \[f64; 108\]"))

@@ -949,13 +949,14 @@ uint32_t read_colour_encoding(Reader& r) {
   return space;
 }
 
-// A.3 ToneMapping (headers/colour.rs:299-320).
-void read_tone_mapping(Reader& r) {
-  if (r.b()) return;
-  f16(r);
+// A.3 ToneMapping (headers/colour.rs:299-320): returns intensity_target.
+float read_tone_mapping(Reader& r) {
+  if (r.b()) return 255.0f;
+  const float target = f16(r);
   f16(r);
   r.b();
   f16(r);
+  return target;
 }
 
 // B.2.6 Enum(): U32(0, 1, 2 + u(4), 18 + u(6)).
@@ -997,16 +998,42 @@ typedef struct {
   float base_corr_x, base_corr_b;
   uint32_t global_tree, used_orders, extra_precision;
   uint32_t dct_select_histogram[27];
+  float m_lf[3];            // G.1.2 LfChannelDequantization (m_x_lf, m_y_lf, m_b_lf)
+  float intensity_target;   // A.3 tone mapping
 } cs_info;
+
+// Every varblock of a frame with its full quantized coefficients: for
+// varblock i, DctSelect, top-left block, HfMul and the arena offset of its X,
+// Y, B coefficient arrays (each in the landscape layout of I.3.2: 8 min(N, M)
+// rows by 8 max(N, M) columns, row-major, LLF cells 0), in the order of the
+// LF groups and, within one, the raster order of the top-left blocks.
+typedef struct {
+  uint32_t dct_select, bx, by, hf_mul;
+  uint64_t start;
+} cs_varblock;
+
+typedef struct {
+  int32_t* coeffs;       // arena, capacity 3 x 64 x blocks
+  cs_varblock* varblocks;
+  uint32_t capacity;     // entries of varblocks
+  uint32_t count;        // filled
+  uint8_t* extra_precision;  // per LF group
+} cs_full;
 
 // Reads `data`; fills info as far as it gets.  The plan outputs (any may be
 // NULL): quant 3 x (8 bh) x (8 bw) footprint layout, HF cells only (LLF
 // cells 0); lf 3 x bh x bw; bmap bh x bw (8/16/32 at a varblock's top-left,
 // 0 covered, 100 + DctSelect for other transforms); hf_mul bh x bw at the
 // top-left; cfl_x/cfl_b ceil(bh/8) x ceil(bw/8); sharp bh x bw.
-int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, int32_t* lf, uint8_t* bmap,
-              uint32_t* hf_mul, int8_t* cfl_x, int8_t* cfl_b, uint8_t* sharp, char* err, int errlen) {
+static int decode_impl(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, int32_t* lf, uint8_t* bmap,
+                       uint32_t* hf_mul, int8_t* cfl_x, int8_t* cfl_b, uint8_t* sharp, cs_full* full, char* err,
+                       int errlen) {
   memset(info, 0, sizeof(*info));
+  info->m_lf[0] = 1.0f / 32.0f;
+  info->m_lf[1] = 1.0f / 4.0f;
+  info->m_lf[2] = 1.0f / 2.0f;
+  info->intensity_target = 255.0f;
+  if (full) full->count = 0;
   try {
     Reader r{data, len};
     // Headers: signature, SizeHeader, ImageMetadata (headers/size.rs, metadata.rs)
@@ -1041,7 +1068,7 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
       if (u32(r, kNe) != 0) fail("A.3: extra channels are not read by this checker");
       info->xyb = r.b();
       info->colour_space = read_colour_encoding(r);
-      if (extra) read_tone_mapping(r);
+      if (extra) info->intensity_target = read_tone_mapping(r);
       skip_extensions(r);
     }
     if (!r.b()) fail("A.3: custom transform data (default_m false) is not read by this checker");
@@ -1131,9 +1158,9 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
     {
       Reader& s = one;
       if (!s.b()) {  // G.1.2 LfChannelDequantization
-        f16(s);
-        f16(s);
-        f16(s);
+        info->m_lf[0] = f16(s);
+        info->m_lf[1] = f16(s);
+        info->m_lf[2] = f16(s);
       }
       static const Dist kGs[4] = {B(11, 1), B(11, 2049), B(12, 4097), B(16, 8193)};
       static const Dist kQl[4] = {V(16), B(5, 1), B(8, 1), B(16, 1)};
@@ -1194,6 +1221,8 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
     std::vector<uint8_t> sel((size_t)bw * bh, 255);
     std::vector<uint32_t> mul((size_t)bw * bh, 0);
     std::vector<int32_t> lfq((size_t)3 * bw * bh, 0);
+    std::vector<uint64_t> vstart((size_t)bw * bh, 0);
+    uint64_t arena_at = 0;
     // LF groups (G.2)
     for (uint32_t id = 0; id < nlf; id++) {
       Reader sep = single ? Reader{nullptr, 0} : section(1 + id);
@@ -1201,6 +1230,7 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
       const uint32_t lx = id % lgx, ly = id / lgx, b0x = lx * 256, b0y = ly * 256;
       const uint32_t gw = std::min(256u, bw - b0x), gh = std::min(256u, bh - b0y);
       info->extra_precision = s.bits(2);
+      if (full && full->extra_precision) full->extra_precision[id] = (uint8_t)info->extra_precision;
       std::vector<Channel> c(3);
       for (auto& x : c) {
         x.w = gw;
@@ -1256,6 +1286,17 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
         sel[at] = (uint8_t)ds;
         mul[at] = (uint32_t)q + 1;
         info->dct_select_histogram[ds]++;
+        if (full) {
+          if (full->count >= full->capacity) fail("cs_full: varblock table too small");
+          cs_varblock& v = full->varblocks[full->count++];
+          v.dct_select = (uint32_t)ds;
+          v.bx = b0x + X;
+          v.by = b0y + Y;
+          v.hf_mul = (uint32_t)q + 1;
+          v.start = arena_at;
+          vstart[at] = arena_at;
+          arena_at += (uint64_t)3 * 64 * rows * cols;
+        }
       }
       for (uint32_t y = 0; y < gh; y++)
         for (uint32_t x = 0; x < gw; x++) {
@@ -1327,6 +1368,7 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
     // Pass groups (G.4, I.4)
     const size_t cs = (size_t)8 * bw, cpl = cs * 8 * bh;
     if (quant) memset(quant, 0, 3 * cpl * 4);
+    if (full && full->coeffs) memset(full->coeffs, 0, arena_at * 4);
     for (uint32_t gid = 0; gid < ng; gid++) {
       Reader sep = single ? Reader{nullptr, 0} : section(2 + nlf + gid);
       Reader& s = single ? one : sep;
@@ -1391,6 +1433,7 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
                 const uint32_t cell = order[k];
                 quant[(size_t)c * cpl + (size_t)(8 * (g0y + y) + cell / bwd) * cs + 8 * (g0x + x) + cell % bwd] = v;
               }
+              if (full && full->coeffs) full->coeffs[vstart[at] + (uint64_t)c * size + order[k]] = v;
               prev = u != 0;
               if (u) {
                 if (--rem == 0) break;
@@ -1416,6 +1459,18 @@ int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, in
     if (err && errlen > 0) snprintf(err, errlen, "%s", e.what());
     return -1;
   }
+}
+
+int cs_decode(const uint8_t* data, size_t len, cs_info* info, int32_t* quant, int32_t* lf, uint8_t* bmap,
+              uint32_t* hf_mul, int8_t* cfl_x, int8_t* cfl_b, uint8_t* sharp, char* err, int errlen) {
+  return decode_impl(data, len, info, quant, lf, bmap, hf_mul, cfl_x, cfl_b, sharp, nullptr, err, errlen);
+}
+
+// cs_decode plus every varblock's full coefficient arrays (all DctSelect
+// types), for reconstructing a frame's samples.
+int cs_decode_full(const uint8_t* data, size_t len, cs_info* info, int32_t* lf, uint8_t* bmap, int8_t* cfl_x,
+                   int8_t* cfl_b, uint8_t* sharp, cs_full* full, char* err, int errlen) {
+  return decode_impl(data, len, info, nullptr, lf, bmap, nullptr, cfl_x, cfl_b, sharp, full, err, errlen);
 }
 
 }  // extern "C"
