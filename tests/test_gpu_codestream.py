@@ -222,3 +222,34 @@ def test_facade_target_bytes_and_bpp(ctx):
     assert len(s1) <= 12000 and cs.decode(s1).info.stage == 7
     s2 = jpxl_amd.Encoder(ctx).with_target_bpp(1.0).encode_rgb8(w, h, rgb)
     assert len(s2) <= w * h // 8 and cs.decode(s2).info.stage == 7
+
+
+@pytest.mark.parametrize("aq,r", [(True, (2, 1, 5)), (False, (3, 1, 2)), (True, (0, 0, 7))])
+def test_device_render_is_the_decoded_stream(ctx, aq, r):
+    """The device's rendered XYB planes (jpxlg_render_score, read back as
+    JPXLG_BUF_RENDER_XYB) equal the planes the fixture-graded reconstruction
+    (tests/vardct_render.py, within 4.6e-5 of the reference's djxl decodes)
+    draws from the stream jpxlg_encode_emit writes for the same plan:
+    per-varblock HfMul, CfL, Gaborish and EPF included."""
+    import jpxl_amd
+    import vardct_render as vr
+    from jpxl_amd import Quantizer, Restoration, _lib
+
+    w, h = 480, 272
+    img = jpxl_amd.Image(ctx, _rgb(w, h, 31), 8)
+    if aq:
+        assert img.set_aq(_lib.AQ_MASKING)
+    q = Quantizer(12000, 2, 16, 3, 2, 1, 1.0, 0.0)
+    img.plan_structure(q, _lib.COVER_HIERARCHICAL, True)
+    img.quantize(q)
+    rest = Restoration(*r)
+    img.render_score(rest)
+    device = img.download(_lib.BUF_RENDER_XYB)
+    stream, _ = img.encode_emit(rest)
+    full = cs.decode_full(stream)
+    if aq:
+        assert len({v[3] for v in full.varblocks}) > 1
+    ours = vr.render_xyb(full)
+    err = np.abs(ours - device).max()
+    assert err < 5e-6, err
+    img.close()
