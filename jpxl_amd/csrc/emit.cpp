@@ -557,29 +557,47 @@ struct EntropyCode {
   std::vector<PrefixCode> codes;
   std::vector<uint32_t> alphabet;
 
-  // hist: num_ctx x kAlphabet token counts.
-  void build(const std::vector<uint32_t>& hist, int nctx, Hybrid c, int max_clusters, Pool* pool = nullptr) {
+  // hist: num_ctx x kAlphabet token counts; or, with ids, one row per
+  // context of ids (ascending; the contexts with tokens), the other contexts
+  // of the nctx having none.
+  void build(const std::vector<uint32_t>& hist, int nctx, Hybrid c, int max_clusters, Pool* pool = nullptr,
+             const std::vector<uint32_t>* ids = nullptr) {
     num_ctx = nctx;
     cfg = c;
+    const int rows = ids ? (int)ids->size() : nctx;
+    std::vector<uint8_t> row_map;
     if (nctx == 1) {
-      map.assign(1, 0);
+      row_map.assign(rows, 0);
       num_clusters = 1;
     } else {
-      cluster_contexts(hist, nctx, max_clusters, 64.0, &map, &num_clusters, pool);
+      cluster_contexts(hist, rows, max_clusters, 64.0, &row_map, &num_clusters, pool);
+    }
+    if (ids) {
+      // contexts without tokens join the cluster of the context before them
+      map.assign(nctx, 0);
+      uint8_t prev = 0;
+      size_t u = 0;
+      for (int ctx = 0; ctx < nctx; ctx++) {
+        if (u < ids->size() && (*ids)[u] == (uint32_t)ctx) prev = row_map[u++];
+        map[ctx] = prev;
+      }
+    } else {
+      map = row_map;
     }
     std::vector<uint32_t> ch((size_t)num_clusters * kAlphabet, 0);
-    for (int ctx = 0; ctx < nctx; ctx++) {
-      const uint32_t* h = &hist[(size_t)ctx * kAlphabet];
-      uint32_t* d = &ch[(size_t)map[ctx] * kAlphabet];
+    for (int r = 0; r < rows; r++) {
+      const uint32_t* h = &hist[(size_t)r * kAlphabet];
+      uint32_t* d = &ch[(size_t)row_map[r] * kAlphabet];
       int last = kAlphabet;
-      if (nctx > 1024)  // most of a large stream's contexts are sparse: stop at the last count
+      if (rows > 1024)  // most of a large stream's contexts are sparse: stop at the last count
         while (last > 0 && !h[last - 1]) last--;
       for (int s = 0; s < last; s++) d[s] += h[s];
     }
     codes.assign(num_clusters, {});
     alphabet.assign(num_clusters, 1);
     single_symbol_.assign(num_clusters, -1);
-    for (int k = 0; k < num_clusters; k++) {
+    packed_.assign((size_t)num_clusters * kAlphabet, 0u);
+    auto code_of = [&](int k, int) {
       const uint32_t* h = &ch[(size_t)k * kAlphabet];
       int last = 0;
       for (int s = 0; s < kAlphabet; s++)
@@ -597,10 +615,20 @@ struct EntropyCode {
       } else {
         canonical_codes(pc.len.data(), alphabet[k], pc.rev.data());
       }
-    }
+      for (uint32_t s = 0; s < alphabet[k]; s++)
+        packed_[(size_t)k * kAlphabet + s] = pc.len[s] | (uint32_t)pc.rev[s] << 5;
+    };
+    if (pool && num_clusters > 8) pool->run(num_clusters, code_of);
+    else
+      for (int k = 0; k < num_clusters; k++) code_of(k, 0);
   }
 
   void write_header(BitWriter& w) const;
+
+  uint32_t code_len(uint32_t ctx, uint32_t tok) const {
+    const PrefixCode& pc = codes[map[ctx]];
+    return tok < pc.len.size() ? pc.len[tok] : 0;
+  }
 
   inline void put(BitWriter& w, uint64_t e) const {
     const PrefixCode& pc = codes[map[tape_ctx(e)]];
@@ -610,7 +638,26 @@ struct EntropyCode {
     if (nb) w.put(nb, tape_bits(e));
   }
 
+  // A tape's codes and extra bits, in order (the hot loop of the pass
+  // groups): one table load per token, one write when both fit 32 bits.
+  void put_tape(BitWriter& w, const uint64_t* e, size_t n, uint64_t ctx_bits = 0) const {
+    const uint8_t* m = map.data();
+    const uint32_t* pk = packed_.data();
+    for (size_t i = 0; i < n; i++) {
+      const uint64_t x = e[i] | ctx_bits;
+      const uint32_t c = pk[(size_t)m[tape_ctx(x)] * kAlphabet + tape_tok(x)];
+      const uint32_t len = c & 31, nb = tape_nb(x);
+      if (len + nb <= 32) {
+        w.put(len + nb, (uint64_t)(c >> 5) | ((uint64_t)tape_bits(x) << len));
+      } else {
+        w.put(len, c >> 5);
+        w.put(nb, tape_bits(x));
+      }
+    }
+  }
+
   std::vector<int> single_symbol_;
+  std::vector<uint32_t> packed_;  // per cluster and token: code length | bit-reversed code << 5
 };
 
 void write_cluster_map(BitWriter& w, const std::vector<uint8_t>& map, int num_clusters);
@@ -710,28 +757,25 @@ inline int64_t predict(int pred, const int32_t* row, const int32_t* up, uint32_t
   }
 }
 
-// Packed residuals of rows y0, y0 + step, ... under predictor pred and offset.
-void channel_residuals(const Channel& c, int pred, int32_t offset, uint32_t step, std::vector<uint32_t>* out) {
-  out->clear();
-  out->reserve((size_t)c.w * ((c.h + step - 1) / step));
+// Calls f on the packed residual of every sample of rows 0, step, 2 step, ...
+// under predictor P (a template argument, so the switch folds away).
+template <int P, class F>
+void for_residuals(const Channel& c, uint32_t step, F&& f) {
   for (uint32_t y = 0; y < c.h; y += step) {
     const int32_t* row = &c.v[(size_t)y * c.w];
     const int32_t* up = y ? row - c.w : row;
-    for (uint32_t x = 0; x < c.w; x++)
-      out->push_back(pack_signed((int32_t)(row[x] - predict(pred, row, up, x, y) - offset)));
+    for (uint32_t x = 0; x < c.w; x++) f(pack_signed((int32_t)(row[x] - predict(P, row, up, x, y))));
   }
 }
 
-double residual_cost(const std::vector<uint32_t>& r, Hybrid cfg) {
-  uint32_t h[kAlphabet] = {0};
-  double extra = 0;
-  for (uint32_t v : r) {
-    uint32_t tok, nb, bits;
-    hybrid_token(cfg, v, &tok, &nb, &bits);
-    h[tok]++;
-    extra += nb;
+template <class F>
+void for_residuals(int pred, const Channel& c, uint32_t step, F&& f) {
+  switch (pred) {
+    case 0: for_residuals<0>(c, step, f); break;
+    case 1: for_residuals<1>(c, step, f); break;
+    case 2: for_residuals<2>(c, step, f); break;
+    default: for_residuals<5>(c, step, f); break;
   }
-  return entropy_bits(h, kAlphabet) + extra;
 }
 
 // A channel ready for a modular stream: its leaf's predictor and offset and
@@ -742,7 +786,8 @@ struct PreparedChannel {
   uint32_t w = 0, h = 0;
   int pred = 5;
   int32_t offset = 0;
-  std::vector<uint64_t> tape;         // its residuals' tokens (context added when written)
+  bool constant = false;              // every residual is token 0 with no extra bits
+  std::vector<uint64_t> tape;         // its residuals' tokens (context added when written); empty if constant
   std::vector<uint32_t> hist;         // their counts (kAlphabet)
 };
 
@@ -757,28 +802,38 @@ void prepare_channel(const Channel& ch, PreparedChannel* pc) {
   pc->hist.assign(kAlphabet, 0);
   if (!ch.w || !ch.h) return;
   const size_t n = (size_t)ch.w * ch.h;
-  std::vector<uint32_t> res;
   if (std::all_of(ch.v.begin(), ch.v.end(), [&](int32_t x) { return x == ch.v[0]; })) {
     pc->pred = 0;
     pc->offset = ch.v[0];
-    res.assign(n, 0u);
-  } else {
-    const uint32_t step = (uint32_t)std::max<size_t>(1, n / 16384);
-    double best = 1e300;
-    for (int p : {5, 0, 1, 2}) {
-      channel_residuals(ch, p, 0, step, &res);
-      const double cost = residual_cost(res, kModularHybrid);
-      if (cost < best) { best = cost; pc->pred = p; }
-    }
-    channel_residuals(ch, pc->pred, 0, 1, &res);
+    pc->constant = true;
+    pc->hist[0] = (uint32_t)n;
+    return;
+  }
+  // price each predictor on about 4096 samples (whole rows)
+  const uint32_t rows = std::max<uint32_t>(1, (4096 + ch.w - 1) / ch.w);
+  const uint32_t step = std::max<uint32_t>(1, ch.h / rows);
+  double best = 1e300;
+  for (int p : {5, 0, 1, 2}) {
+    uint32_t h[kAlphabet] = {0};
+    double extra = 0;
+    for_residuals(p, ch, step, [&](uint32_t r) {
+      uint32_t tok, nb, bits;
+      hybrid_token(kModularHybrid, r, &tok, &nb, &bits);
+      h[tok]++;
+      extra += nb;
+    });
+    const double cost = entropy_bits(h, kAlphabet) + extra;
+    if (cost < best) { best = cost; pc->pred = p; }
   }
   pc->tape.resize(n);
-  for (size_t k = 0; k < n; k++) {
+  uint64_t* t = pc->tape.data();
+  uint32_t* hist = pc->hist.data();
+  for_residuals(pc->pred, ch, 1, [&](uint32_t r) {
     uint32_t tok, nb, bits;
-    hybrid_token(kModularHybrid, res[k], &tok, &nb, &bits);
-    pc->hist[tok]++;
-    pc->tape[k] = tape_entry(0, tok, nb, bits);
-  }
+    hybrid_token(kModularHybrid, r, &tok, &nb, &bits);
+    hist[tok]++;
+    *t++ = tape_entry(0, tok, nb, bits);
+  });
 }
 
 // Writes a modular sub-bitstream of prepared channels (H.2): local tree,
@@ -819,8 +874,16 @@ void write_modular_prepared(BitWriter& w, const std::vector<PreparedChannel>& pc
   ec.build(hist, C, kModularHybrid, C);
   ec.write_header(w);
   for (int i = 0; i < C; i++) {
-    const uint64_t ctx = (uint64_t)(C - 1 - i) << 45;
-    for (uint64_t e : pcs[i].tape) ec.put(w, e | ctx);
+    const uint32_t c = (uint32_t)(C - 1 - i);
+    const uint64_t ctx = (uint64_t)c << 45;
+    if (pcs[i].constant) {
+      // token 0 without extra bits, w x h times: nothing to write when its
+      // code is the zero-length one of a single-symbol cluster
+      if (ec.code_len(c, 0))
+        for (size_t k = 0; k < (size_t)pcs[i].w * pcs[i].h; k++) ec.put(w, tape_entry(c, 0, 0, 0));
+      continue;
+    }
+    ec.put_tape(w, pcs[i].tape.data(), pcs[i].tape.size(), ctx);
   }
 }
 
@@ -1082,10 +1145,28 @@ inline size_t dense_slot(uint32_t ctx) {
                                      : (size_t)kNzContexts * kNzTok + (size_t)(ctx - kNzContexts) * kCoefTok;
 }
 inline uint32_t dense_width(uint32_t ctx) { return ctx < (uint32_t)kNzContexts ? kNzTok : kCoefTok; }
+constexpr int kTouchedWords = (kHfContexts + 63) / 64;
 struct HfCounts {
-  std::vector<uint32_t> dense;   // kDenseCounts
-  std::vector<uint32_t> escape;  // ctx << 8 | token
+  std::vector<uint32_t> dense;    // kDenseCounts; all zero between calls
+  std::vector<uint64_t> touched;  // kTouchedWords: the contexts counted; all zero between calls
+  std::vector<uint32_t> escape;   // ctx << 8 | token
 };
+
+// The pool workers' count tables, kept across calls (and zero between them:
+// the merge clears every slot it reads).  Guarded by g_pool_mu.
+std::vector<HfCounts>& worker_counts(int nw) {
+  static std::vector<HfCounts> counts;
+  static pid_t owner = 0;
+  if (owner != getpid()) counts.clear();
+  owner = getpid();
+  if ((int)counts.size() < nw) counts.resize(nw);
+  for (auto& c : counts) {
+    if (c.dense.size() != kDenseCounts) c.dense.assign(kDenseCounts, 0);
+    if (c.touched.size() != (size_t)kTouchedWords) c.touched.assign(kTouchedWords, 0);
+    c.escape.clear();
+  }
+  return counts;
+}
 
 // A pass group's token tape (uninitialised storage sized by a bound).
 struct Tape {
@@ -1107,7 +1188,9 @@ void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
   uint32_t nzgrid[3][32 * 32];
   const NaturalOrders& nat = natural_orders();
   uint32_t* dense = hc->dense.data();
+  uint64_t* touched = hc->touched.data();
   auto emit = [&](uint32_t ctx, uint32_t v) {
+    touched[ctx >> 6] |= 1ull << (ctx & 63);
     uint32_t tok, nb, bits;
     if (v < 16) {
       tok = v;
@@ -1246,13 +1329,9 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
   // Phase 1: LF-group parts (independent of the HF tables) and the pass
   // groups' tokens with per-worker counts, all at once.
   std::vector<Tape> tapes(ng);
-  std::vector<HfCounts> counts(nw);
+  std::vector<HfCounts>& counts = worker_counts(nw);
   std::vector<BitWriter> hf_meta(nlf);
   std::vector<std::vector<PreparedChannel>> lf_prep(nlf, std::vector<PreparedChannel>(3));
-  pool->run(nw, [&](int i, int) {
-    counts[i].dense.assign(kDenseCounts, 0);
-    counts[i].escape.clear();
-  });
   pool->run((int)(4 * nlf + ng), [&](int t, int worker) {
     if (t < (int)(4 * nlf)) {
       if (t % 4 == 3) write_hf_metadata(hf_meta[t / 4], fr, t / 4);
@@ -1261,24 +1340,42 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
     }
     tokenize_group(fr, t - 4 * nlf, &tapes[t - 4 * nlf], &counts[worker]);
   });
-  // Phase 2: the frame's HF histograms, clusters and codes.
-  std::vector<uint32_t> hist((size_t)kHfContexts * kAlphabet);
-  const int chunks = 32;
+  // Phase 2: the frame's HF histograms (over the contexts with tokens),
+  // clusters and codes; the workers' tables are cleared as they are read.
+  uint64_t any[kTouchedWords] = {0};
+  for (int k = 0; k < nw; k++)
+    for (int i = 0; i < kTouchedWords; i++) any[i] |= counts[k].touched[i];
+  std::vector<uint32_t> used;
+  std::vector<int32_t> row_of(kHfContexts, -1);
+  for (int i = 0; i < kTouchedWords; i++)
+    for (uint64_t b = any[i]; b; b &= b - 1) {
+      const uint32_t ctx = (uint32_t)(i * 64 + __builtin_ctzll(b));
+      row_of[ctx] = (int32_t)used.size();
+      used.push_back(ctx);
+    }
+  std::vector<uint32_t> hist(used.size() * kAlphabet, 0u);
+  const int chunks = 32, nu = (int)used.size();
   pool->run(chunks, [&](int ci, int) {
-    for (int ctx = kHfContexts * ci / chunks; ctx < kHfContexts * (ci + 1) / chunks; ctx++) {
-      uint32_t* d = &hist[(size_t)ctx * kAlphabet];
-      std::fill(d, d + kAlphabet, 0u);
+    for (int u = nu * ci / chunks; u < nu * (ci + 1) / chunks; u++) {
+      const uint32_t ctx = used[u];
+      uint32_t* d = &hist[(size_t)u * kAlphabet];
+      const size_t slot = dense_slot(ctx);
+      const uint32_t wdt = dense_width(ctx);
       for (int k = 0; k < nw; k++) {
-        const uint32_t* c = &counts[k].dense[dense_slot(ctx)];
-        const uint32_t wdt = dense_width(ctx);
+        if (!(counts[k].touched[ctx >> 6] >> (ctx & 63) & 1)) continue;
+        uint32_t* c = &counts[k].dense[slot];
         for (uint32_t t = 0; t < wdt; t++) d[t] += c[t];
+        std::fill(c, c + wdt, 0u);
       }
     }
   });
-  for (auto& c : counts)
-    for (uint32_t e : c.escape) hist[(size_t)(e >> 8) * kAlphabet + (e & 0xFF)]++;
+  for (auto& c : counts) {
+    for (uint32_t e : c.escape) hist[(size_t)row_of[e >> 8] * kAlphabet + (e & 0xFF)]++;
+    c.escape.clear();
+    std::fill(c.touched.begin(), c.touched.end(), 0ull);
+  }
   EntropyCode hf;
-  hf.build(hist, kHfContexts, Hybrid{4, 2, 0}, 64, pool);
+  hf.build(hist, kHfContexts, Hybrid{4, 2, 0}, 64, pool, &used);
 
   // Phase 3: sections in TOC order: LfGlobal, LF groups, HfGlobal, pass groups.
   const bool single = ng == 1;
@@ -1293,7 +1390,7 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
   auto pass_group = [&](BitWriter& w, uint32_t gi) {
     // hfp takes ceil(log2(1)) = 0 bits; no modular group data follows
     const Tape& tp = tapes[gi];
-    for (size_t k = 0; k < tp.n; k++) hf.put(w, tp.e[k]);
+    hf.put_tape(w, tp.e.get(), tp.n);
   };
   if (single) {
     BitWriter& w = sec[0];
