@@ -3,8 +3,9 @@
 A step is one whole quality encode of the 2048x2048 RGB16 synthetic image
 (seed 2) whose samples are already resident in HBM: XYB + metric reference
 pyramid, plan structure + forward DCT, the navigator's quantize -> render ->
-SSIMULACRA2 probes, the chosen plan regathered into the QuantizedFrameIr and
-read back into pinned host memory, and the CPU codestream stage (entropy
+SSIMULACRA2 probes, the chosen plan's token-ready runs (the QuantizedFrameIr
+coefficients trimmed on the device to what the entropy coder reads) read
+back into pinned host memory, and the CPU codestream stage (entropy
 coding and bitstream writing) producing the JPEG XL bytes.  Steps are
 pipelined the way an encoder serving a stream of images runs: the host
 writes image i's codestream while the device searches image i + 1; the timed
@@ -611,16 +612,19 @@ def main() -> int:
     emit_threads = max(1, host_cores() - 1)  # one core drives the device
 
     class Slot:
-        """One image's hand-off to the codestream stage: the IR in pinned
-        host memory, the CfL tiles, the frame fields and the output."""
+        """One image's hand-off to the codestream stage: the token-ready runs
+        and LF planes in pinned host memory, the CfL tiles, the frame fields
+        and the output."""
 
         def __init__(self):
             self.bufs = []
-            for nbytes in (192 * bw * bh * 4, 3 * bw * bh * 4):
+            for nbytes in (189 * bw * bh * 4, 3 * bw * bh * 4):
                 p = ctypes.c_void_p()
                 assert hl.hipHostMalloc(ctypes.byref(p), nbytes, 0) == 0
                 self.bufs.append(p)
             self.table = (jpxl_amd._lib.Varblock * (bw * bh))()
+            self.runs = (jpxl_amd._lib.Run * (3 * bw * bh))()
+            self.arena_len = ctypes.c_size_t()
             self.n = ctypes.c_uint32()
             self.cx = np.zeros((th, tw), np.int8)
             self.cb = np.zeros((th, tw), np.int8)
@@ -647,7 +651,8 @@ def main() -> int:
             for side in (8, 16, 32):
                 sizes[side] = int((bm == side).sum())
         h = img.handle
-        ctx.check(lib.jpxlg_plan_download_ir(h, slot.bufs[0], slot.bufs[1], slot.table, ctypes.byref(slot.n)))
+        ctx.check(lib.jpxlg_plan_download_runs(h, slot.bufs[0], 189 * bw * bh, ctypes.byref(slot.arena_len), slot.runs,
+                                               slot.bufs[1], slot.table, ctypes.byref(slot.n)))
         ctx.check(lib.jpxlg_plan_download(h, None, None, None, slot.cx.ctypes.data, slot.cb.ctypes.data, None))
         ctx.check(lib.jpxlg_plan_frame_desc(h, None, ctypes.byref(slot.frame)))
         img.close()
@@ -655,10 +660,10 @@ def main() -> int:
 
     def emit(slot):
         """The host part of a step: the codestream stage."""
-        rc = lib.jpxlg_emit_codestream(ctypes.byref(slot.frame), slot.bufs[0], slot.bufs[1], slot.table,
-                                       slot.n.value, slot.cx.ctypes.data, slot.cb.ctypes.data, emit_threads,
-                                       slot.out.ctypes.data, slot.out.size, ctypes.byref(slot.out_len),
-                                       ctypes.byref(slot.sizes))
+        rc = lib.jpxlg_emit_codestream_runs(ctypes.byref(slot.frame), slot.bufs[0], slot.arena_len.value, slot.runs,
+                                            slot.bufs[1], slot.table, slot.n.value, slot.cx.ctypes.data,
+                                            slot.cb.ctypes.data, emit_threads, slot.out.ctypes.data, slot.out.size,
+                                            ctypes.byref(slot.out_len), ctypes.byref(slot.sizes))
         if rc != 0:
             raise RuntimeError(f"jpxlg_emit_codestream: {rc}")
         return slot.out_len.value

@@ -247,6 +247,43 @@ int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* frame, const int32_t* coef
                               const int8_t* cfl_b, int32_t threads, uint8_t* out, size_t capacity, size_t* out_len,
                               jpxlg_emit_sizes* sizes);
 
+/* The plan's HF coefficients as token-ready runs, the hand-off that the
+ * codestream stage reads fastest.  No reference counterpart: the same
+ * QuantizedFrameIr content (jpxl-encode/src/vardct/plan.rs:519-636)
+ * trimmed to what attach_entropy codes (hf_coeff.rs:480-575 writes, per
+ * varblock and channel, the non-zero count and then the coefficients in
+ * natural order up to the last non-zero one).  runs[3 i + c] belongs to
+ * varblock i of the jpxlg_plan_download_ir order and channel c (0 X, 1 Y,
+ * 2 B): its HF coefficients at natural-order positions nblocks ..
+ * nblocks + count - 1 (nblocks = (side / 8)^2, the LLF cells carried by the
+ * LF planes) are run_arena[start .. start + count - 1]; the last of them is
+ * non-zero and `nonzeros` of them are (count 0 when the channel has no
+ * non-zero HF coefficient). */
+typedef struct {
+  uint64_t start;
+  uint32_t count, nonzeros;
+} jpxlg_run;
+
+/* Reads the image's last quantized plan back as runs: run_arena
+ * (capacity entries; *arena_len receives the entries used, at most 189 per
+ * 8x8 block of the frame), runs (3 per varblock), lf_groups and varblocks
+ * as jpxlg_plan_download_ir (any output may be NULL except that run_arena
+ * and runs go together).  The device computes the runs; only they cross
+ * PCIe.  JPXLG_E_TOO_SMALL when the arena does not fit capacity. */
+int32_t jpxlg_plan_download_runs(jpxlg_img* img, int32_t* run_arena, size_t capacity, size_t* arena_len,
+                                 jpxlg_run* runs, int32_t* lf_groups, jpxlg_varblock* varblocks,
+                                 uint32_t* n_varblocks);
+
+/* jpxlg_emit_codestream over runs: the same stream, byte for byte, as over
+ * the coefficient arena the runs were taken from.  Each varblock's `start`
+ * field is ignored; runs that break their definition (a count past the
+ * array, a run outside arena_len, a zero last coefficient, a non-zero count
+ * that disagrees) are refused with JPXLG_E_INVALID. */
+int32_t jpxlg_emit_codestream_runs(const jpxlg_frame_desc* frame, const int32_t* run_arena, size_t arena_len,
+                                   const jpxlg_run* runs, const int32_t* lf_groups, const jpxlg_varblock* varblocks,
+                                   uint32_t n_varblocks, const int8_t* cfl_x, const int8_t* cfl_b, int32_t threads,
+                                   uint8_t* out, size_t capacity, size_t* out_len, jpxlg_emit_sizes* sizes);
+
 /* The frame fields of an image's last quantized plan for
  * jpxlg_emit_codestream: its wire quantizer (jpxlg_plan_aq), LF factors,
  * source depth and `r` (NULL: the restoration of the image's last render).

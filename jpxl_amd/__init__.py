@@ -366,6 +366,22 @@ class Image:
                for v in table[: n.value]]
         return arena, lfg, vbs
 
+    def plan_download_runs(self):
+        """The token-ready hand-off: (run arena, runs (3 per varblock, ctypes
+        array of Run), per-group LF planes, varblock table as in
+        plan_download_ir)."""
+        bpx = self.bw * self.bh
+        arena = np.empty(189 * bpx, np.int32)
+        runs = (_lib.Run * (3 * bpx))()
+        lfg = np.empty(3 * bpx, np.int32)
+        table = (_lib.Varblock * bpx)()
+        n, used = ctypes.c_uint32(), ctypes.c_size_t()
+        self.ctx.check(self.ctx.lib.jpxlg_plan_download_runs(self.handle, _ptr(arena), arena.size, ctypes.byref(used),
+                                                             runs, _ptr(lfg), table, ctypes.byref(n)))
+        vbs = [dict(lf_group=v.lf_group, bx=v.bx, by=v.by, side=v.side, hf_mul=v.hf_mul, start=v.start)
+               for v in table[: n.value]]
+        return arena[: used.value].copy(), runs[: 3 * n.value], lfg, vbs
+
     def set_plan(self, wire: Quantizer, plan: "Plan", hf_mul: np.ndarray | None = None) -> None:
         """PerceptualEvaluator::evaluate's input: a plan built elsewhere becomes
         this image's structure and levels (the next render_score renders it)."""

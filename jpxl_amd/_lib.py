@@ -97,6 +97,12 @@ class QualityRequest(ctypes.Structure):
     ]
 
 
+class Run(ctypes.Structure):
+    """jpxlg_run: one varblock channel's token-ready run."""
+
+    _fields_ = [("start", ctypes.c_uint64), ("count", ctypes.c_uint32), ("nonzeros", ctypes.c_uint32)]
+
+
 class Varblock(ctypes.Structure):
     """jpxlg_varblock: one entry of the QuantizedFrameIr varblock order."""
 
@@ -216,6 +222,11 @@ SIGNATURES = {
     "jpxlg_plan_download_ir": (_i32, [_vp, _vp, _vp, _vp, _u32p]),
     "jpxlg_emit_codestream": (_i32, [ctypes.POINTER(FrameDesc), _vp, _vp, _vp, _u32, _vp, _vp, _i32, _vp,
                                      ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(EmitSizes)]),
+    "jpxlg_plan_download_runs": (_i32, [_vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t), _vp, _vp, _vp,
+                                        _u32p]),
+    "jpxlg_emit_codestream_runs": (_i32, [ctypes.POINTER(FrameDesc), _vp, ctypes.c_size_t, _vp, _vp, _vp, _u32, _vp,
+                                          _vp, _i32, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                          ctypes.POINTER(EmitSizes)]),
     "jpxlg_encode_bytes": (_i32, [_vp, ctypes.POINTER(RateRequest), ctypes.POINTER(RateResult)]),
     "jpxlg_plan_frame_desc": (_i32, [_vp, ctypes.POINTER(Restoration), ctypes.POINTER(FrameDesc)]),
     "jpxlg_encode_emit": (_i32, [_vp, ctypes.POINTER(Restoration), _i32, _vp, ctypes.c_size_t,

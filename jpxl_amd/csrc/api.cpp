@@ -102,6 +102,8 @@ struct jpxlg_ctx {
   // codestream stage (arena, LF planes, varblock table), grown on demand.
   char* ir_host = nullptr;
   size_t ir_cap = 0;
+  char* runs_host = nullptr;  // pinned staging of the runs' count and start tables
+  size_t runs_cap = 0;
   // This device's address of the no-allocation scratch array (from
   // hipGetSymbolAddress at creation) and pinned space for its sums.
   char* static_scratch = nullptr;
@@ -462,6 +464,7 @@ extern "C" void jpxlg_ctx_destroy(jpxlg_ctx* ctx) {
   if (ctx->up_host) (void)hipHostFree(ctx->up_host);
   if (ctx->rb_host) (void)hipHostFree(ctx->rb_host);
   if (ctx->ir_host) (void)hipHostFree(ctx->ir_host);
+  if (ctx->runs_host) (void)hipHostFree(ctx->runs_host);
   if (ctx->static_sums) (void)hipHostFree(ctx->static_sums);
   for (auto& r : ctx->recs) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
   for (auto e : ctx->pool) (void)hipEventDestroy(e);
@@ -1614,8 +1617,9 @@ extern "C" int32_t jpxlg_encode_emit(jpxlg_img* img, const jpxlg_restoration* r,
   if (!img->has_structure || !img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
   const Geom& g = img->g;
   const size_t bpx = (size_t)g.bw * g.bh;
-  const size_t a_bytes = 192 * bpx * 4, l_bytes = 3 * bpx * 4, v_bytes = bpx * sizeof(jpxlg_varblock);
-  const size_t need = a_bytes + l_bytes + v_bytes;
+  const size_t a_bytes = 189 * bpx * 4, l_bytes = 3 * bpx * 4, v_bytes = bpx * sizeof(jpxlg_varblock);
+  const size_t r_bytes = 3 * bpx * sizeof(jpxlg_run);
+  const size_t need = a_bytes + l_bytes + v_bytes + r_bytes;
   if (need > ctx->ir_cap) {
     if (ctx->ir_host) (void)hipHostFree(ctx->ir_host);
     ctx->ir_host = nullptr;
@@ -1627,18 +1631,23 @@ extern "C" int32_t jpxlg_encode_emit(jpxlg_img* img, const jpxlg_restoration* r,
   int32_t* arena = (int32_t*)ctx->ir_host;
   int32_t* lfg = (int32_t*)(ctx->ir_host + a_bytes);
   jpxlg_varblock* vbs = (jpxlg_varblock*)(ctx->ir_host + a_bytes + l_bytes);
+  jpxlg_run* runs = (jpxlg_run*)(ctx->ir_host + a_bytes + l_bytes + v_bytes);
   uint32_t n = 0;
-  int32_t rc = jpxlg_plan_download_ir(img, arena, lfg, vbs, &n);
+  size_t arena_len = 0;
+  int32_t rc = jpxlg_plan_download_runs(img, arena, 189 * bpx, &arena_len, runs, lfg, vbs, &n);
   if (rc) return rc;
   jpxlg_frame_desc f;
   rc = jpxlg_plan_frame_desc(img, r, &f);
   if (rc) return rc;
-  rc = jpxlg_emit_codestream(&f, arena, lfg, vbs, n, img->cfl_x.data(), img->cfl_b.data(), threads, out, capacity,
-                             out_len, sizes);
+  rc = jpxlg_emit_codestream_runs(&f, arena, arena_len, runs, lfg, vbs, n, img->cfl_x.data(), img->cfl_b.data(),
+                                  threads, out, capacity, out_len, sizes);
   if (rc == JPXLG_E_TOO_SMALL) return fail(ctx, rc, "output buffer smaller than the codestream");
   if (rc) return fail(ctx, rc, "codestream stage refused the plan");
   return JPXLG_OK;
 }
+
+static int32_t ir_table(jpxlg_img* img, jpxlg_varblock* varblocks, uint32_t* n_varblocks,
+                        std::vector<uint32_t>* offsets);
 
 // The QuantizedFrameIr hand-off (jpxl-encode/src/vardct/plan.rs:519-636):
 // the quantized plan regathered on the device from the footprint layout
@@ -1667,10 +1676,23 @@ extern "C" int32_t jpxlg_plan_download_ir(jpxlg_img* img, int32_t* coeff_arena, 
     if (lf_groups) HIP_TRY(ctx, hipMemcpyAsync(lf_groups, d_lfg, 3 * bpx * 4, hipMemcpyDeviceToHost, st));
     HIP_TRY(ctx, hipStreamSynchronize(st));
   }
-  if (varblocks || n_varblocks) {
+  return ir_table(img, varblocks, n_varblocks, nullptr);
+}
+
+// The IR's varblock table (LF groups in raster order, inside a group the
+// block-raster order of the top-left blocks) and, when offsets is given,
+// each varblock's offset in 8x8 blocks.
+static int32_t ir_table(jpxlg_img* img, jpxlg_varblock* varblocks, uint32_t* n_varblocks,
+                        std::vector<uint32_t>* offsets) {
+  const Geom& g = img->g;
+  const size_t bpx = (size_t)g.bw * g.bh;
+  if (varblocks || n_varblocks || offsets) {
     std::vector<uint32_t> muls(bpx, 0u);
-    const int rc = jpxlg_plan_aq(img, muls.data(), nullptr, nullptr);
-    if (rc) return rc;
+    if (varblocks) {
+      const int rc = jpxlg_plan_aq(img, muls.data(), nullptr, nullptr);
+      if (rc) return rc;
+    }
+    if (offsets) offsets->clear();
     const int gw = (g.bw + 255) / 256, gh = (g.bh + 255) / 256;
     uint64_t off = 0;
     uint32_t n = 0;
@@ -1691,12 +1713,75 @@ extern "C" int32_t jpxlg_plan_download_ir(jpxlg_img* img, int32_t* coeff_arena, 
               v.reserved = 0;
               v.start = 192 * off;
             }
+            if (offsets) offsets->push_back((uint32_t)off);
             off += (uint64_t)(side / 8) * (side / 8);
             n++;
           }
     if (n_varblocks) *n_varblocks = n;
   }
   return JPXLG_OK;
+}
+
+// Token-ready runs (no reference counterpart; the QuantizedFrameIr content
+// trimmed to what attach_entropy codes): computed on the device from the
+// footprint planes, so only the runs cross PCIe.
+extern "C" int32_t jpxlg_plan_download_runs(jpxlg_img* img, int32_t* run_arena, size_t capacity, size_t* arena_len,
+                                            jpxlg_run* runs, int32_t* lf_groups, jpxlg_varblock* varblocks,
+                                            uint32_t* n_varblocks) {
+  if (!img) return JPXLG_E_INVALID;
+  jpxlg_ctx* ctx = img->ctx;
+  if (!img->has_structure) return fail(ctx, JPXLG_E_INVALID, "no plan structure yet");
+  if (!img->has_quant) return fail(ctx, JPXLG_E_INVALID, "nothing quantized yet");
+  if ((run_arena == nullptr) != (runs == nullptr)) return fail(ctx, JPXLG_E_INVALID, "run_arena and runs go together");
+  const Geom& g = img->g;
+  const size_t bpx = (size_t)g.bw * g.bh;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t st = ctx->stream;
+  if (lf_groups) {
+    int32_t* d_lfg = (int32_t*)img->d_lf_s;
+    launch_gather_ir(st, g, img->d_list, img->list_count, img->d_lf_off, img->quant, img->lf, nullptr, d_lfg);
+    HIP_TRY(ctx, hipGetLastError());
+    HIP_TRY(ctx, hipMemcpyAsync(lf_groups, d_lfg, 3 * bpx * 4, hipMemcpyDeviceToHost, st));
+  }
+  if (runs) {
+    // the CfL sample buffer (256 words per block) holds the count table
+    // (3 per block), the start table (3 per block + the total) and the runs
+    // (at most 189 per block)
+    uint32_t* d_len = (uint32_t*)img->d_hf_s;
+    uint32_t* d_start = d_len + 3 * bpx;
+    int32_t* d_arena = (int32_t*)(d_len + 7 * bpx);
+    launch_runs(st, g, img->d_list, img->list_count, img->d_lf_off, img->quant, ctx->d_tab, d_len, d_start, d_arena);
+    HIP_TRY(ctx, hipGetLastError());
+    const size_t t_bytes = (6 * bpx + 1) * 4;
+    if (t_bytes > ctx->runs_cap) {
+      if (ctx->runs_host) (void)hipHostFree(ctx->runs_host);
+      ctx->runs_host = nullptr;
+      ctx->runs_cap = 0;
+      if (hipHostMalloc((void**)&ctx->runs_host, t_bytes, hipHostMallocDefault) != hipSuccess)
+        return fail(ctx, JPXLG_E_OOM, "pinned staging for the runs' tables");
+      ctx->runs_cap = t_bytes;
+    }
+    uint32_t* h_len = (uint32_t*)ctx->runs_host;
+    uint32_t* h_start = h_len + 3 * bpx;
+    HIP_TRY(ctx, hipMemcpyAsync(h_len, d_len, t_bytes, hipMemcpyDeviceToHost, st));
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    const size_t total = h_start[3 * bpx];
+    if (arena_len) *arena_len = total;
+    if (total > capacity) return fail(ctx, JPXLG_E_TOO_SMALL, "run arena larger than its buffer");
+    if (total) HIP_TRY(ctx, hipMemcpyAsync(run_arena, d_arena, total * 4, hipMemcpyDeviceToHost, st));
+    std::vector<uint32_t> offsets;
+    const int32_t rc = ir_table(img, varblocks, n_varblocks, &offsets);
+    if (rc) return rc;
+    for (size_t i = 0; i < offsets.size(); i++)
+      for (int c = 0; c < 3; c++) {
+        const size_t at = 3 * (size_t)offsets[i] + c;
+        runs[3 * i + c] = {h_start[at], h_len[at] & 0xffffu, h_len[at] >> 16};
+      }
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    return JPXLG_OK;
+  }
+  HIP_TRY(ctx, hipStreamSynchronize(st));
+  return ir_table(img, varblocks, n_varblocks, nullptr);
 }
 
 extern "C" int32_t jpxlg_image_download(jpxlg_img* img, int32_t which, void* host, size_t bytes) {

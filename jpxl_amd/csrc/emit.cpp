@@ -1048,6 +1048,8 @@ struct Frame {
   uint32_t nvb;
   const int8_t* cfl_x;
   const int8_t* cfl_b;
+  const jpxlg_run* runs = nullptr;  // with runs, arena holds the run arena of runs_len entries
+  size_t runs_len = 0;
   std::vector<std::vector<uint32_t>> group_vbs;  // per pass group, IR order
   std::vector<std::vector<uint32_t>> lf_group_vbs;
   std::vector<size_t> lf_offset;  // start of each LF group's planes in lf
@@ -1175,7 +1177,9 @@ struct Tape {
 };
 
 // I.4 tokens of one pass group, appended to its tape; counted into hc.
-void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
+// Returns false when a run breaks its definition (the tokens are then void).
+bool tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
+  bool ok = true;
   const Geometry& g = fr.g;
   const uint32_t g0x = (gid % g.gx) * 32, g0y = (gid / g.gx) * 32;
   size_t bound = 0;
@@ -1211,7 +1215,6 @@ void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
     const uint16_t* inv = nat.inv(side);
     const uint32_t x = v.bx - g0x, y = v.by - g0y;
     for (int c : {1, 0, 2}) {
-      const int32_t* a = fr.arena + v.start + (uint64_t)c * size;
       const uint32_t block_ctx = kDefaultBlockCtx[(c < 2 ? c ^ 1 : 2) * 13 + order_id];
       uint32_t* grid = nzgrid[c];
       uint32_t pred;
@@ -1219,14 +1222,26 @@ void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
       else if (x == 0) pred = grid[(y - 1) * 32 + x];
       else if (y == 0) pred = grid[y * 32 + x - 1];
       else pred = (grid[(y - 1) * 32 + x] + grid[y * 32 + x - 1] + 1) >> 1;
-      // non-zeros among the HF positions and the last one's position, over
-      // the array in raster order (LLF cells are the positions below nblocks)
+      // non-zeros among the HF positions and the last one's position
       uint32_t nz = 0, last = 0;
-      for (uint32_t cell = 0; cell < size; cell++) {
-        const uint32_t k = inv[cell];
-        const uint32_t m = (uint32_t)(a[cell] != 0) & (uint32_t)(k >= nblocks);
-        nz += m;
-        last = std::max(last, m * k);
+      const int32_t* a;     // coefficient of natural-order position k: a[order[k]], or run[k - nblocks]
+      const int32_t* run = nullptr;
+      if (fr.runs) {
+        const jpxlg_run& r = fr.runs[3 * (size_t)idx + c];
+        nz = r.nonzeros;
+        last = r.count ? nblocks + r.count - 1 : 0;
+        run = fr.arena + r.start;
+        a = nullptr;
+        if (r.count && run[r.count - 1] == 0) ok = false;
+      } else {
+        // over the array in raster order (LLF cells are the positions below nblocks)
+        a = fr.arena + v.start + (uint64_t)c * size;
+        for (uint32_t cell = 0; cell < size; cell++) {
+          const uint32_t k = inv[cell];
+          const uint32_t m = (uint32_t)(a[cell] != 0) & (uint32_t)(k >= nblocks);
+          nz += m;
+          last = std::max(last, m * k);
+        }
       }
       const uint32_t p = std::min(pred, 64u);
       emit(block_ctx + kNbBlockCtx * (p < 8 ? p : 4 + p / 2), nz);
@@ -1236,6 +1251,22 @@ void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
       if (!nz) continue;
       const uint32_t base = block_ctx * kZeroDensityCtx + kNonZeroBuckets * kNbBlockCtx;
       uint32_t prev = nz <= size / 16 ? 1 : 0, rem = nz;
+      if (run) {
+        for (uint32_t k = nblocks; k <= last; k++) {
+          const int32_t q = run[k - nblocks];
+          if (q != 0 && rem == 0) {  // more non-zeros than the run declares
+            ok = false;
+            break;
+          }
+          const uint32_t ctx =
+              base + (kCoeffNumNonzeroCtx[(rem + nblocks - 1) >> lb] + kCoeffFreqCtx[k >> lb]) * 2 + prev;
+          emit(ctx, pack_signed(q));
+          prev = q != 0;
+          rem -= prev;
+        }
+        if (rem) ok = false;
+        continue;
+      }
       for (uint32_t k = nblocks; k <= last; k++) {
         const int32_t q = a[order[k]];
         const uint32_t ctx = base + (kCoeffNumNonzeroCtx[(rem + nblocks - 1) >> lb] + kCoeffFreqCtx[k >> lb]) * 2 + prev;
@@ -1246,6 +1277,7 @@ void tokenize_group(const Frame& fr, uint32_t gid, Tape* tape, HfCounts* hc) {
     }
   }
   tape->n = (size_t)(out - tape->e.get());
+  return ok;
 }
 
 int32_t validate(const jpxlg_frame_desc* f, const jpxlg_varblock* vb, uint32_t n, Frame* fr) {
@@ -1269,7 +1301,16 @@ int32_t validate(const jpxlg_frame_desc* f, const jpxlg_varblock* vb, uint32_t n
     const uint32_t s = v.side / 8;
     if (v.bx % s || v.by % s) return JPXLG_E_INVALID;
     if (v.hf_mul < 1 || v.hf_mul > (1u << 20)) return JPXLG_E_RANGE;
-    if (v.start + 3ull * v.side * v.side > arena_len) return JPXLG_E_INVALID;
+    if (fr->runs) {
+      const uint32_t hf = v.side * v.side - s * s;
+      for (int c = 0; c < 3; c++) {
+        const jpxlg_run& r = fr->runs[3 * (size_t)i + c];
+        if (r.count > hf || r.nonzeros > r.count || (r.count == 0) != (r.nonzeros == 0)) return JPXLG_E_INVALID;
+        if (r.start > fr->runs_len || r.count > fr->runs_len - r.start) return JPXLG_E_INVALID;
+      }
+    } else if (v.start + 3ull * v.side * v.side > arena_len) {
+      return JPXLG_E_INVALID;
+    }
     const uint32_t lgid = (v.by / 256) * g.lgx + v.bx / 256;
     if (v.lf_group != lgid) return JPXLG_E_INVALID;
     for (uint32_t j = 0; j < s; j++)
@@ -1303,22 +1344,12 @@ int32_t validate(const jpxlg_frame_desc* f, const jpxlg_varblock* vb, uint32_t n
 
 }  // namespace
 
-extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_t* coeff_arena,
-                                         const int32_t* lf_groups, const jpxlg_varblock* varblocks,
-                                         uint32_t n_varblocks, const int8_t* cfl_x, const int8_t* cfl_b,
-                                         int32_t threads, uint8_t* out, size_t capacity, size_t* out_len,
-                                         jpxlg_emit_sizes* sizes) {
-  if (!f) return JPXLG_E_INVALID;
-  Frame fr;
-  fr.f = f;
-  fr.g = geometry_of(f->width, f->height);
-  fr.arena = coeff_arena;
-  fr.lf = lf_groups;
-  fr.vb = varblocks;
-  fr.nvb = n_varblocks;
-  fr.cfl_x = cfl_x;
-  fr.cfl_b = cfl_b;
-  if (int32_t rc = validate(f, varblocks, n_varblocks, &fr)) return rc;
+namespace {
+
+int32_t emit_frame(Frame& fr, int32_t threads, uint8_t* out, size_t capacity, size_t* out_len,
+                   jpxlg_emit_sizes* sizes) {
+  const jpxlg_frame_desc* f = fr.f;
+  if (int32_t rc = validate(f, fr.vb, fr.nvb, &fr)) return rc;
   const Geometry& g = fr.g;
   const uint32_t ng = g.num_groups(), nlf = g.num_lf_groups();
   std::lock_guard<std::mutex> lk(g_pool_mu);
@@ -1332,13 +1363,14 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
   std::vector<HfCounts>& counts = worker_counts(nw);
   std::vector<BitWriter> hf_meta(nlf);
   std::vector<std::vector<PreparedChannel>> lf_prep(nlf, std::vector<PreparedChannel>(3));
+  std::atomic<bool> runs_ok{true};
   pool->run((int)(4 * nlf + ng), [&](int t, int worker) {
     if (t < (int)(4 * nlf)) {
       if (t % 4 == 3) write_hf_metadata(hf_meta[t / 4], fr, t / 4);
       else prepare_lf_plane(fr, t / 4, t % 4, &lf_prep[t / 4][t % 4]);
       return;
     }
-    tokenize_group(fr, t - 4 * nlf, &tapes[t - 4 * nlf], &counts[worker]);
+    if (!tokenize_group(fr, t - 4 * nlf, &tapes[t - 4 * nlf], &counts[worker])) runs_ok = false;
   });
   // Phase 2: the frame's HF histograms (over the contexts with tokens),
   // clusters and codes; the workers' tables are cleared as they are read.
@@ -1374,6 +1406,7 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
     c.escape.clear();
     std::fill(c.touched.begin(), c.touched.end(), 0ull);
   }
+  if (!runs_ok) return JPXLG_E_INVALID;  // the workers' tables are clear again
   EntropyCode hf;
   hf.build(hist, kHfContexts, Hybrid{4, 2, 0}, 64, pool, &used);
 
@@ -1460,4 +1493,46 @@ extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_
     at += s.size();
   }
   return JPXLG_OK;
+}
+
+}  // namespace
+
+extern "C" int32_t jpxlg_emit_codestream(const jpxlg_frame_desc* f, const int32_t* coeff_arena,
+                                         const int32_t* lf_groups, const jpxlg_varblock* varblocks,
+                                         uint32_t n_varblocks, const int8_t* cfl_x, const int8_t* cfl_b,
+                                         int32_t threads, uint8_t* out, size_t capacity, size_t* out_len,
+                                         jpxlg_emit_sizes* sizes) {
+  if (!f) return JPXLG_E_INVALID;
+  Frame fr;
+  fr.f = f;
+  fr.g = geometry_of(f->width, f->height);
+  fr.arena = coeff_arena;
+  fr.lf = lf_groups;
+  fr.vb = varblocks;
+  fr.nvb = n_varblocks;
+  fr.cfl_x = cfl_x;
+  fr.cfl_b = cfl_b;
+  return emit_frame(fr, threads, out, capacity, out_len, sizes);
+}
+
+extern "C" int32_t jpxlg_emit_codestream_runs(const jpxlg_frame_desc* f, const int32_t* run_arena, size_t arena_len,
+                                              const jpxlg_run* runs, const int32_t* lf_groups,
+                                              const jpxlg_varblock* varblocks, uint32_t n_varblocks,
+                                              const int8_t* cfl_x, const int8_t* cfl_b, int32_t threads,
+                                              uint8_t* out, size_t capacity, size_t* out_len,
+                                              jpxlg_emit_sizes* sizes) {
+  if (!f || !runs || (!run_arena && arena_len)) return JPXLG_E_INVALID;
+  static const int32_t kNone = 0;
+  Frame fr;
+  fr.f = f;
+  fr.g = geometry_of(f->width, f->height);
+  fr.arena = run_arena ? run_arena : &kNone;
+  fr.runs = runs;
+  fr.runs_len = arena_len;
+  fr.lf = lf_groups;
+  fr.vb = varblocks;
+  fr.nvb = n_varblocks;
+  fr.cfl_x = cfl_x;
+  fr.cfl_b = cfl_b;
+  return emit_frame(fr, threads, out, capacity, out_len, sizes);
 }

@@ -396,6 +396,12 @@ void launch_block_features(hipStream_t s, const Geom& g, const ConstTables* tab,
 void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3],
                       const uint32_t* lf_off, int32_t* const quant[3], int32_t* const lf[3], int32_t* arena,
                       int32_t* lf_out);
+// Token-ready runs of the quantized plan (count, prefix sum, write): len_nz
+// and start are indexed 3 * IR block offset + channel (start has one more
+// entry, the total); arena receives the runs.
+void launch_runs(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3], const uint32_t* lf_off,
+                 int32_t* const quant[3], const ConstTables* tab, uint32_t* len_nz, uint32_t* start,
+                 int32_t* arena);
 void launch_aq_block_muls(hipStream_t s, const Geom& g, const ProbeParams* pp, uint32_t* const list[3],
                           const int counts[3], uint32_t* muls);
 // One scale and channel of the metric's blur passes: the planes each pass

@@ -1604,12 +1604,120 @@ void launch_gather_ir(hipStream_t s, const Geom& g, uint32_t* const list[3], con
                       const uint32_t* lf_off, int32_t* const quant[3], int32_t* const lf[3], int32_t* arena,
                       int32_t* lf_out) {
   const int sides[3] = {8, 16, 32};
+  if (arena)
+    for (int i = 0; i < 3; i++)
+      if (counts[i])
+        k_gather_ir<<<counts[i], 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0], quant[1], quant[2],
+                                              arena);
+  const size_t bpx = (size_t)g.bw * g.bh;
+  if (lf_out) k_gather_lf_groups<<<div_up(bpx, 256), 256, 0, s>>>(g, lf[0], lf[1], lf[2], lf_out);
+}
+
+// Token-ready runs of the quantized plan.  One wavefront per (varblock,
+// channel), four per workgroup; the varblock's entries live at its block
+// offset in the IR (lf_off), so the tables are indexed 3 * offset + channel
+// and entries of blocks that are not a varblock's top-left stay zero.
+//
+// Count: the HF non-zeros (natural-order positions nblocks .. size - 1) and
+// the run length up to the last of them, packed count | nonzeros << 16.
+__global__ __launch_bounds__(256) void k_runs_count(Geom g, const uint32_t* __restrict__ list, int count, int side,
+                                                    const uint32_t* __restrict__ lf_off,
+                                                    const int32_t* __restrict__ QX, const int32_t* __restrict__ QY,
+                                                    const int32_t* __restrict__ QB,
+                                                    const uint16_t* __restrict__ order,
+                                                    uint32_t* __restrict__ len_nz) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
+  const int v = wave / 3, c = wave % 3;
+  if (v >= count) return;  // the whole wavefront leaves together
+  const uint32_t e = list[v];
+  const int bx = (int)(e & 0xffffu), by = (int)(e >> 16);
+  const int nb1 = side / 8, nblocks = nb1 * nb1, size = side * side;
+  const int32_t* q = c == 0 ? QX : (c == 1 ? QY : QB);
+  const size_t base = (size_t)(8 * by) * g.cstride + 8 * bx;
+  uint32_t nz = 0, last = 0;  // last: natural position + 1 of this lane's last non-zero
+  for (int k = nblocks + lane; k < size; k += 64) {
+    const int cell = order[k];
+    if (q[base + (size_t)(cell / side) * g.cstride + cell % side] != 0) {
+      nz++;
+      last = (uint32_t)k + 1;
+    }
+  }
+  for (int m = 32; m >= 1; m >>= 1) {
+    nz += __shfl_xor(nz, m, 64);
+    last = max(last, (uint32_t)__shfl_xor(last, m, 64));
+  }
+  if (lane == 0) {
+    const uint32_t off = lf_off[(size_t)by * g.bw + bx];
+    len_nz[3 * (size_t)off + c] = (last ? last - (uint32_t)nblocks : 0u) | nz << 16;
+  }
+}
+
+// Exclusive prefix sum of the run lengths (one workgroup): start[i] for the
+// n entries, start[n] the total.
+__global__ __launch_bounds__(1024) void k_runs_scan(const uint32_t* __restrict__ len_nz, int n,
+                                                    uint32_t* __restrict__ start) {
+  __shared__ uint32_t part[1024];
+  const int t = (int)threadIdx.x;
+  const int per = (n + 1023) / 1024, b = min(n, t * per), e = min(n, b + per);
+  uint32_t sum = 0;
+  for (int i = b; i < e; i++) sum += len_nz[i] & 0xffffu;
+  part[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 1024; d <<= 1) {
+    const uint32_t add = t >= d ? part[t - d] : 0u;
+    __syncthreads();
+    part[t] += add;
+    __syncthreads();
+  }
+  uint32_t run = t ? part[t - 1] : 0u;
+  for (int i = b; i < e; i++) {
+    start[i] = run;
+    run += len_nz[i] & 0xffffu;
+  }
+  if (t == 1023) start[n] = part[1023];
+}
+
+// Write: each run's coefficients in natural order at its start.
+__global__ __launch_bounds__(256) void k_runs_write(Geom g, const uint32_t* __restrict__ list, int count, int side,
+                                                    const uint32_t* __restrict__ lf_off,
+                                                    const int32_t* __restrict__ QX, const int32_t* __restrict__ QY,
+                                                    const int32_t* __restrict__ QB,
+                                                    const uint16_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ len_nz,
+                                                    const uint32_t* __restrict__ start, int32_t* __restrict__ arena) {
+  const int wave = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), lane = (int)(threadIdx.x & 63);
+  const int v = wave / 3, c = wave % 3;
+  if (v >= count) return;
+  const uint32_t e = list[v];
+  const int bx = (int)(e & 0xffffu), by = (int)(e >> 16);
+  const int nb1 = side / 8, nblocks = nb1 * nb1;
+  const int32_t* q = c == 0 ? QX : (c == 1 ? QY : QB);
+  const size_t base = (size_t)(8 * by) * g.cstride + 8 * bx;
+  const size_t at = 3 * (size_t)lf_off[(size_t)by * g.bw + bx] + c;
+  const uint32_t n = len_nz[at] & 0xffffu;
+  int32_t* out = arena + start[at];
+  for (uint32_t j = (uint32_t)lane; j < n; j += 64) {
+    const int cell = order[nblocks + (int)j];
+    out[j] = q[base + (size_t)(cell / side) * g.cstride + cell % side];
+  }
+}
+
+void launch_runs(hipStream_t s, const Geom& g, uint32_t* const list[3], const int counts[3], const uint32_t* lf_off,
+                 int32_t* const quant[3], const ConstTables* tab, uint32_t* len_nz, uint32_t* start,
+                 int32_t* arena) {
+  const int sides[3] = {8, 16, 32}, order_at[3] = {0, 64, 64 + 256};
+  const int n = 3 * g.bw * g.bh;
+  k_zero_u32<<<div_up(n, 256), 256, 0, s>>>(len_nz, n);
   for (int i = 0; i < 3; i++)
     if (counts[i])
-      k_gather_ir<<<counts[i], 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0], quant[1], quant[2],
-                                            arena);
-  const size_t bpx = (size_t)g.bw * g.bh;
-  k_gather_lf_groups<<<div_up(bpx, 256), 256, 0, s>>>(g, lf[0], lf[1], lf[2], lf_out);
+      k_runs_count<<<div_up(3 * counts[i], 4), 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0],
+                                                             quant[1], quant[2], tab->order + order_at[i], len_nz);
+  k_runs_scan<<<1, 1024, 0, s>>>(len_nz, n, start);
+  for (int i = 0; i < 3; i++)
+    if (counts[i])
+      k_runs_write<<<div_up(3 * counts[i], 4), 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0],
+                                                             quant[1], quant[2], tab->order + order_at[i], len_nz,
+                                                             start, arena);
 }
 
 

@@ -251,3 +251,59 @@ def llf_zeroed(quant, bmap):
                 c = s // 8
                 q[:, 8 * by:8 * by + c, 8 * bx:8 * bx + c] = 0
     return q
+
+
+def runs_from_ir(arena, vbs):
+    """The token-ready runs of an IR (restated from their definition): per
+    varblock and channel, the HF coefficients in natural order up to the last
+    non-zero one.  -> (run arena int32, list of (start, count, nonzeros))."""
+    orders = {s: natural_order(s) for s in (8, 16, 32)}
+    out, runs, at = [], [], 0
+    for v in vbs:
+        s = v["side"]
+        nb = (s // 8) ** 2
+        for c in range(3):
+            a = arena[v["start"] + c * s * s: v["start"] + (c + 1) * s * s]
+            nat = a[orders[s]][nb:]
+            nzp = np.nonzero(nat)[0]
+            n = int(nzp[-1]) + 1 if nzp.size else 0
+            out.append(nat[:n])
+            runs.append((at, n, int(nzp.size)))
+            at += n
+    arena_out = np.concatenate(out).astype(np.int32) if out else np.zeros(0, np.int32)
+    return arena_out, runs
+
+
+def run_table(runs):
+    from jpxl_amd import _lib
+
+    t = (_lib.Run * max(1, len(runs)))()
+    for i, r in enumerate(runs):
+        st, n, nz = (r.start, r.count, r.nonzeros) if hasattr(r, "start") else r
+        t[i].start, t[i].count, t[i].nonzeros = st, n, nz
+    return t
+
+
+def emit_runs(frame, run_arena, runs, lfg, vbs, cfl_x=None, cfl_b=None, threads: int = 4):
+    """jpxlg_emit_codestream_runs through the C ABI: (bytes, Sizes); raises
+    RuntimeError(rc) on refusal."""
+    from jpxl_amd import _lib
+
+    lib = _lib.load()
+    table = varblock_table(vbs)
+    rt = run_table(list(runs))
+    ra = np.ascontiguousarray(run_arena, np.int32)
+    cx = np.ascontiguousarray(cfl_x, np.int8) if cfl_x is not None else None
+    cb = np.ascontiguousarray(cfl_b, np.int8) if cfl_b is not None else None
+    n = ctypes.c_size_t()
+    sizes = _lib.EmitSizes()
+    args = (ctypes.byref(frame), _p(ra) if ra.size else None, ra.size, rt, _p(lfg), table, len(vbs), _p(cx), _p(cb),
+            threads)
+    rc = lib.jpxlg_emit_codestream_runs(*args, None, 0, ctypes.byref(n), ctypes.byref(sizes))
+    if rc != 0:
+        raise RuntimeError(rc)
+    out = np.empty(n.value, np.uint8)
+    rc = lib.jpxlg_emit_codestream_runs(*args, _p(out), out.size, ctypes.byref(n), ctypes.byref(sizes))
+    if rc != 0:
+        raise RuntimeError(rc)
+    return out.tobytes(), sizes

@@ -238,3 +238,55 @@ def test_bad_plans_are_refused():
     assert rc(bad) == _lib.E_RANGE
     assert rc(vbs, _frame(w, h, sharp=8)) == _lib.E_RANGE
     assert rc(vbs, _frame(w, h, lff=(128, 0))) == _lib.E_RANGE
+
+
+@pytest.mark.parametrize("w,h,mixed", [(8, 8, False), (203, 141, True), (600, 520, True)])
+def test_runs_give_the_same_stream(w, h, mixed):
+    """jpxlg_emit_codestream_runs over the runs of an IR writes the bytes
+    jpxlg_emit_codestream writes over the IR itself."""
+    bmap, q, lf = _plan(w, h, 9, mixed)
+    rng = np.random.default_rng(2)
+    mul = rng.integers(1, 5, bmap.shape).astype(np.uint32)
+    arena, lfg, vbs = cs.ir_from_footprint(q, lf, bmap, mul)
+    f = _frame(w, h, gab=1, epf=2)
+    ref, _ = cs.emit(f, arena, lfg, vbs)
+    ra, runs = cs.runs_from_ir(arena, vbs)
+    assert ra.size < arena.size // 2  # the runs are the coefficients up to the last non-zero ones
+    for threads in (1, 4):
+        got, _ = cs.emit_runs(f, ra, runs, lfg, vbs, threads=threads)
+        assert got == ref
+
+
+def test_runs_that_break_their_definition_are_refused():
+    from jpxl_amd import _lib
+
+    w, h = 203, 141
+    bmap, q, lf = _plan(w, h, 9, True)
+    arena, lfg, vbs = cs.ir_from_footprint(q, lf, bmap)
+    f = _frame(w, h)
+    ra, runs = cs.runs_from_ir(arena, vbs)
+    i = next(k for k, r in enumerate(runs) if r[1] >= 3 and r[2] >= 2)
+    st, n, nz = runs[i]
+
+    def refused(ra2, runs2):
+        with pytest.raises(RuntimeError) as e:
+            cs.emit_runs(f, ra2, runs2, lfg, vbs)
+        assert e.value.args[0] == _lib.E_INVALID
+
+    bad = list(runs)
+    bad[i] = (st, n, nz + 1)  # more non-zeros declared than present
+    refused(ra, bad)
+    bad[i] = (st, n, nz - 1)  # fewer
+    refused(ra, bad)
+    r2 = ra.copy()
+    r2[st + n - 1] = 0  # the run no longer ends on a non-zero
+    refused(r2, runs)
+    bad = list(runs)
+    bad[i] = (ra.size - 1, n, nz)  # outside the arena
+    refused(ra, bad)
+    bad = list(runs)
+    side = vbs[i // 3]["side"]
+    bad[i] = (st, side * side, nz)  # longer than the HF positions
+    refused(ra, bad)
+    # and the library is unharmed: the good runs still give the IR's stream
+    assert cs.emit_runs(f, ra, runs, lfg, vbs)[0] == cs.emit(f, arena, lfg, vbs)[0]

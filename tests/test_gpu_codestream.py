@@ -8,6 +8,8 @@ HfMul, the CfL tiles and LF factors, and the frame fields (wire quantizer,
 restoration, bit depth).  On the quality path the plan read back is also
 rendered by the oracle and scored: the stream decodes to the pixels the
 search scored."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -252,4 +254,31 @@ def test_device_render_is_the_decoded_stream(ctx, aq, r):
     ours = vr.render_xyb(full)
     err = np.abs(ours - device).max()
     assert err < 5e-6, err
+    img.close()
+
+
+@pytest.mark.parametrize("w,h,bits", [(512, 512, 8), (2100, 300, 8), (2048, 2048, 16)])
+def test_device_runs_are_the_runs_of_the_ir(ctx, w, h, bits):
+    """jpxlg_plan_download_runs: the device's runs equal those restated from
+    jpxlg_plan_download_ir's arena, and both hand-offs give one stream."""
+    import jpxl_amd
+
+    img = jpxl_amd.Image(ctx, _rgb(w, h, 40, bits), bits)
+    img.encode_quality(85.0, "balanced")
+    arena, lfg, vbs = img.plan_download_ir()
+    ra, runs, lfg2, vbs2 = img.plan_download_runs()
+    assert vbs2 == vbs and np.array_equal(lfg2, lfg)
+    want_arena, want = cs.runs_from_ir(arena, vbs)
+    assert [(r.start, r.count, r.nonzeros) for r in runs] == want
+    assert np.array_equal(ra, want_arena)
+    from jpxl_amd import _lib
+
+    frame = _lib.FrameDesc()
+    ctx.check(ctx.lib.jpxlg_plan_frame_desc(img.handle, None, ctypes.byref(frame)))
+    plan = img.plan_download()
+    a, _ = cs.emit(frame, arena, lfg, vbs, plan.cfl_x, plan.cfl_b)
+    b, _ = cs.emit_runs(frame, ra, runs, lfg, vbs, plan.cfl_x, plan.cfl_b)
+    assert a == b
+    c, _ = img.encode_emit()  # the library's own path goes through the runs
+    assert c == a
     img.close()
