@@ -1745,11 +1745,12 @@ extern "C" int32_t jpxlg_plan_download_runs(jpxlg_img* img, int32_t* run_arena, 
   }
   if (runs) {
     // the CfL sample buffer (256 words per block) holds the count table
-    // (3 per block), the start table (3 per block + the total) and the runs
-    // (at most 189 per block)
+    // (3 per block), the start table (3 per block + the total + the scan's
+    // workgroup totals, one per 1024 entries) and the runs (at most 189 per
+    // block)
     uint32_t* d_len = (uint32_t*)img->d_hf_s;
     uint32_t* d_start = d_len + 3 * bpx;
-    int32_t* d_arena = (int32_t*)(d_len + 7 * bpx);
+    int32_t* d_arena = (int32_t*)(d_len + 8 * bpx);
     launch_runs(st, g, img->d_list, img->list_count, img->d_lf_off, img->quant, ctx->d_tab, d_len, d_start, d_arena);
     HIP_TRY(ctx, hipGetLastError());
     const size_t t_bytes = (6 * bpx + 1) * 4;

@@ -1652,29 +1652,67 @@ __global__ __launch_bounds__(256) void k_runs_count(Geom g, const uint32_t* __re
   }
 }
 
-// Exclusive prefix sum of the run lengths (one workgroup): start[i] for the
-// n entries, start[n] the total.
-__global__ __launch_bounds__(1024) void k_runs_scan(const uint32_t* __restrict__ len_nz, int n,
-                                                    uint32_t* __restrict__ start) {
-  __shared__ uint32_t part[1024];
+// Exclusive prefix sum of the run lengths in three passes: each workgroup
+// scans 1024 entries (4 per thread, in order) and leaves its total in part;
+// one workgroup scans the totals (start[n] receives the sum); every
+// workgroup then adds its offset.
+constexpr int kScanTile = 1024;
+
+__global__ __launch_bounds__(256) void k_runs_scan_local(const uint32_t* __restrict__ len_nz, int n,
+                                                         uint32_t* __restrict__ start, uint32_t* __restrict__ part) {
+  __shared__ uint32_t sh[256];
+  const int t = (int)threadIdx.x, base = (int)blockIdx.x * kScanTile + 4 * t;
+  uint32_t v[4], sum = 0;
+  for (int i = 0; i < 4; i++) {
+    v[i] = base + i < n ? (len_nz[base + i] & 0xffffu) : 0u;
+    sum += v[i];
+  }
+  sh[t] = sum;
+  __syncthreads();
+  for (int d = 1; d < 256; d <<= 1) {
+    const uint32_t add = t >= d ? sh[t - d] : 0u;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  uint32_t run = t ? sh[t - 1] : 0u;
+  for (int i = 0; i < 4; i++) {
+    if (base + i < n) start[base + i] = run;
+    run += v[i];
+  }
+  if (t == 255) part[blockIdx.x] = sh[255];
+}
+
+__global__ __launch_bounds__(1024) void k_runs_scan_parts(uint32_t* __restrict__ part, int nb,
+                                                          uint32_t* __restrict__ total) {
+  __shared__ uint32_t sh[1024];
   const int t = (int)threadIdx.x;
-  const int per = (n + 1023) / 1024, b = min(n, t * per), e = min(n, b + per);
+  const int per = (nb + 1023) / 1024, b = min(nb, t * per), e = min(nb, b + per);
   uint32_t sum = 0;
-  for (int i = b; i < e; i++) sum += len_nz[i] & 0xffffu;
-  part[t] = sum;
+  for (int i = b; i < e; i++) sum += part[i];
+  sh[t] = sum;
   __syncthreads();
   for (int d = 1; d < 1024; d <<= 1) {
-    const uint32_t add = t >= d ? part[t - d] : 0u;
+    const uint32_t add = t >= d ? sh[t - d] : 0u;
     __syncthreads();
-    part[t] += add;
+    sh[t] += add;
     __syncthreads();
   }
-  uint32_t run = t ? part[t - 1] : 0u;
+  uint32_t run = t ? sh[t - 1] : 0u;
   for (int i = b; i < e; i++) {
-    start[i] = run;
-    run += len_nz[i] & 0xffffu;
+    const uint32_t x = part[i];
+    part[i] = run;
+    run += x;
   }
-  if (t == 1023) start[n] = part[1023];
+  if (t == 1023) *total = sh[1023];
+}
+
+__global__ __launch_bounds__(256) void k_runs_scan_add(uint32_t* __restrict__ start, int n,
+                                                       const uint32_t* __restrict__ part) {
+  const int base = (int)blockIdx.x * kScanTile + 4 * (int)threadIdx.x;
+  const uint32_t off = part[blockIdx.x];
+  for (int i = 0; i < 4; i++)
+    if (base + i < n) start[base + i] += off;
 }
 
 // Write: each run's coefficients in natural order at its start.
@@ -1712,7 +1750,11 @@ void launch_runs(hipStream_t s, const Geom& g, uint32_t* const list[3], const in
     if (counts[i])
       k_runs_count<<<div_up(3 * counts[i], 4), 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0],
                                                              quant[1], quant[2], tab->order + order_at[i], len_nz);
-  k_runs_scan<<<1, 1024, 0, s>>>(len_nz, n, start);
+  const int nb = div_up(n, kScanTile);
+  uint32_t* part = start + n + 1;  // nb words after the total
+  k_runs_scan_local<<<nb, 256, 0, s>>>(len_nz, n, start, part);
+  k_runs_scan_parts<<<1, 1024, 0, s>>>(part, nb, start + n);
+  k_runs_scan_add<<<nb, 256, 0, s>>>(start, n, part);
   for (int i = 0; i < 3; i++)
     if (counts[i])
       k_runs_write<<<div_up(3 * counts[i], 4), 256, 0, s>>>(g, list[i], counts[i], sides[i], lf_off, quant[0],
