@@ -562,10 +562,12 @@ def pcie_inclusive_rate(ctx, hl, rgb: np.ndarray, args, reps: int = 3) -> float:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--effort", default="balanced")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--emit-threads", type=int, default=0,
+                    help="host workers of the codestream stage (0: the host's cores minus one)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain copy of known size (for HBM counter calibration)")
     ap.add_argument("--form", default="single", choices=["single", "precision", "batched", "served"],
@@ -609,7 +611,8 @@ def main() -> int:
     lib = ctx.lib
     bw, bh = (WIDTH + 7) // 8, (HEIGHT + 7) // 8
     tw, th = (bw + 7) // 8, (bh + 7) // 8
-    emit_threads = max(1, host_cores() - 1)  # one core drives the device
+    # one core drives the device
+    emit_threads = args.emit_threads if args.emit_threads > 0 else max(1, host_cores() - 1)
 
     class Slot:
         """One image's hand-off to the codestream stage: the token-ready runs
